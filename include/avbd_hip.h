@@ -1,0 +1,239 @@
+/*
+ * avbd_hip.h - C ABI of the MI355X AVBD rigid-body step.
+ *
+ * One step of the augmented-vertex-block-descent solver for box bodies and ball/angular
+ * joints, run on one AMD Instinct MI355X (gfx950).  The ABI is plain C: POD structs of
+ * host pointers and sizes, integer status codes, no exceptions, no torch types.
+ *
+ * Index conventions (they decide results, so they are part of the contract):
+ *  - Body index i is the reference solver's list position: the reference prepends every
+ *    new body to an intrusive list, so index 0 is the body created LAST
+ *    (reference src/avbd/rigid.cpp:28-30, solver_broadphase.cpp:87-92).
+ *  - Joint index j is likewise the force-list position (reference src/avbd/force.cpp:19-21).
+ *  - Manifold index m is the position in the solver's live list (creation order,
+ *    reference src/avbd/manifold.hpp:92, solver_broadphase.cpp:153-184).
+ *
+ * Ownership: every pointer passed in belongs to the caller and is only read/written for
+ * the duration of the call.  A world or batch handle owns its device memory.  A handle is
+ * used by one host thread at a time; distinct handles may be used from distinct threads.
+ */
+#ifndef AVBD_HIP_H
+#define AVBD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t avbd_status;
+#define AVBD_OK 0
+#define AVBD_ERR_INVALID (-1)      /* bad argument (null pointer, negative count, ...)   */
+#define AVBD_ERR_UNSUPPORTED (-2)  /* parameter or feature outside this build's path     */
+#define AVBD_ERR_CAPACITY (-3)     /* a device buffer would overflow its capacity        */
+#define AVBD_ERR_DEVICE (-4)       /* a HIP runtime call failed                          */
+#define AVBD_ERR_NO_DEVICE (-5)    /* no usable gfx950 device                            */
+
+/* Solver parameters.  Field meaning and defaults follow the reference's public Solver
+ * fields (reference src/avbd/solver.h:54-144) and constants (src/avbd/types.hpp:23-45).
+ * avbd_params_default() fills the reference defaults.  penalty_max is the reference's
+ * compile-time PENALTY_MAX (1e10) made a parameter. */
+typedef struct avbd_params {
+    float dt;                        /* 1/60 */
+    float gravity;                   /* -10, along -Z; copied into bodies that pass no gravity */
+    int32_t iterations;              /* 8 */
+    float alpha;                     /* 0.99 */
+    float beta_lin;                  /* 1e4 */
+    float beta_ang;                  /* 100 */
+    float gamma;                     /* 0.999 */
+    float newton_ratio;              /* 1 */
+    float stiffness_decay;           /* 1 */
+    int32_t adaptive_beta;           /* 0 */
+    float adaptive_beta_error_floor; /* 1e-3 */
+    float convergence_threshold;     /* 0 (only 0 is supported) */
+    float jacobi_relaxation;         /* 0.7 */
+    int32_t contact_only_overflow;   /* 1 */
+    float stick_drift_heal;          /* 0.002 */
+    float max_penetration;           /* 0.02 */
+    float jacobian_rebuild_distance; /* 0 (only 0 is supported) */
+    float speculative_margin;        /* 0.01 */
+    int32_t position_projection;     /* 1 */
+    int32_t projection_iterations;   /* 4 */
+    float projection_beta;           /* 1 */
+    float projection_slop;           /* 0.008 */
+    float projection_max_correction; /* 0 */
+    int32_t contact_grace_ticks;     /* 0 (only 0 is supported) */
+    int32_t threads;                 /* ignored on the GPU */
+    float sleep_linear_threshold;    /* 0.1 */
+    float sleep_angular_threshold;   /* 0.139626 */
+    int32_t sleep_frames;            /* 0 (only 0 is supported) */
+    float penalty_max;               /* 1e10 */
+} avbd_params;
+
+/* Body arrays, all of length `count`, list order.  Vectors are packed xyz (3 floats),
+ * quaternions xyzw (4 floats).  On upload the constant fields and the dynamic state are
+ * read; on download only non-null pointers are written. */
+typedef struct avbd_bodies {
+    int32_t count;
+    float *size;             /* 3n full box extents                                  */
+    float *mass;             /* n, 0 = static                                        */
+    float *moment;           /* 3n principal moments of inertia                      */
+    float *friction;         /* n                                                    */
+    float *radius;           /* n bounding radius |size/2|                           */
+    float *gravity;          /* n signed gravity along -Z (null: params.gravity)     */
+    uint32_t *layer;         /* n collision layer (null: 1)                          */
+    uint32_t *mask;          /* n collision mask (null: 1)                           */
+    uint8_t *lock_linear;    /* n axis-lock bits 0..2 (null: 0)                      */
+    uint8_t *lock_angular;   /* n (null: 0)                                          */
+    float *position;         /* 3n                                                   */
+    float *rotation;         /* 4n                                                   */
+    float *velocity;         /* 3n                                                   */
+    float *angular_velocity; /* 3n                                                   */
+    float *prev_velocity;    /* 3n previous step's linear velocity (null: velocity)  */
+} avbd_bodies;
+
+/* Joint arrays (reference `Joint`, src/avbd/forces.hpp:58-87), list order.  body_a = -1
+ * anchors body_b to the world point r_a.  A ball joint is stiffness_lin = +inf,
+ * stiffness_ang = 0.  The dual state fields are read on upload and written on download. */
+typedef struct avbd_joints {
+    int32_t count;
+    int32_t *body_a;
+    int32_t *body_b;
+    float *r_a;           /* 3n local anchor on A (world point if body_a < 0) */
+    float *r_b;           /* 3n local anchor on B                             */
+    float *stiffness_lin; /* n */
+    float *stiffness_ang; /* n */
+    float *fracture;      /* n (+inf: unbreakable)                            */
+    float *torque_arm;    /* n |size_a + size_b|^2                            */
+    float *c0_lin;        /* 3n */
+    float *c0_ang;        /* 3n */
+    float *penalty_lin;   /* 3n */
+    float *penalty_ang;   /* 3n */
+    float *lambda_lin;    /* 3n */
+    float *lambda_ang;    /* 3n */
+    uint8_t *broken;      /* n */
+} avbd_joints;
+
+#define AVBD_MAX_CONTACTS 8
+
+/* Contact manifolds in live order (reference ManifoldStore, src/avbd/manifold.hpp:68-131).
+ * Per-contact arrays have AVBD_MAX_CONTACTS entries per manifold.  basis is row-major with
+ * row 0 the contact normal (pointing from B to A) and rows 1-2 the tangents. */
+typedef struct avbd_manifolds {
+    int32_t count;
+    int32_t *body_a;
+    int32_t *body_b;
+    float *basis;        /* 9 per manifold          */
+    float *friction;     /* 1 per manifold          */
+    int32_t *num_contacts;
+    int32_t *feature;    /* 8 per manifold          */
+    float *r_a;          /* 24 per manifold (8 x 3) */
+    float *r_b;          /* 24 per manifold         */
+    float *c0;           /* 24 per manifold         */
+    float *penalty;      /* 24 per manifold         */
+    float *lambda;       /* 24 per manifold         */
+    uint8_t *stick;      /* 8 per manifold          */
+    float *beta_scale;   /* 8 per manifold          */
+} avbd_manifolds;
+
+/* Per-step counters and phase times of the last step (milliseconds, from HIP events). */
+typedef struct avbd_step_stats {
+    int32_t bodies;
+    int32_t joints;
+    int32_t manifolds;
+    int32_t contacts;
+    int32_t candidate_pairs;
+    int32_t new_manifolds;
+    int32_t colours;
+    int32_t overflow_bodies;
+    int32_t brake_levels;
+    float ms_broad;
+    float ms_narrow;
+    float ms_solve;
+    float ms_finish;
+    float ms_total;
+} avbd_step_stats;
+
+/* ---------------------------------------------------------------- helpers ---- */
+
+/* Fill `p` with the reference defaults. */
+void avbd_params_default(avbd_params *p);
+
+/* Box mass properties exactly as the reference body constructor computes them
+ * (src/avbd/rigid.cpp:59-70): mass = x*y*z*density, moment_x = (y^2+z^2)/12*mass, ...,
+ * radius = |size/2|. */
+void avbd_box_properties(const float size[3], float density, float *mass, float moment[3],
+                         float *radius);
+
+/* FNV-1a digest over the raw bytes of position(12), rotation(16), velocity(12) and
+ * angular velocity(12) of every body in list order (reference test/core_test.cpp:133-149). */
+uint64_t avbd_state_digest(const avbd_bodies *bodies);
+
+/* Message of the last error on the calling thread ("" if none). */
+const char *avbd_last_error(void);
+
+/* Number of visible gfx950 devices (0 when none; never fails). */
+int32_t avbd_device_count(void);
+
+/* ------------------------------------------------- single world (plain path) ---- */
+
+typedef struct avbd_world avbd_world;
+
+/* Create a world with room for the given counts on the current HIP device.  Replaces
+ * constructing a reference `avbd::Solver` (src/avbd/solver.h:150). */
+avbd_status avbd_world_create(const avbd_params *params, int32_t max_bodies,
+                              int32_t max_joints, int32_t max_manifolds, avbd_world **out);
+avbd_status avbd_world_destroy(avbd_world *world);
+/* Replaces writing the public Solver fields (src/avbd/solver.h:54-144). */
+avbd_status avbd_world_set_params(avbd_world *world, const avbd_params *params);
+/* Load bodies and joints (in list order) and clear every manifold, as creating the
+ * reference's `Rigid`/`Joint` objects in a fresh Solver does (types.hpp:114-117,
+ * forces.hpp:68). */
+avbd_status avbd_world_upload(avbd_world *world, const avbd_bodies *bodies,
+                              const avbd_joints *joints);
+/* One step: replaces `Solver::step()` (src/avbd/solver.h:234, solver_step.cpp:44-75). */
+avbd_status avbd_world_step(avbd_world *world);
+/* Replaces `Solver::stepInterleaved(substeps, rounds)` (solver.h:241, solver_step.cpp:109-160);
+ * params.dt is the substep dt. */
+avbd_status avbd_world_step_interleaved(avbd_world *world, int32_t substeps, int32_t rounds);
+/* Copy the state back: replaces reading the `Rigid`/`Joint` fields.  Null members skip. */
+avbd_status avbd_world_download(avbd_world *world, avbd_bodies *bodies, avbd_joints *joints);
+/* Manifold accessors: replace Solver::manifoldCount/manifoldContact/... (solver.h:170-231).
+ * get: `out` arrays must hold `out->count` manifolds on entry; count is set on return. */
+avbd_status avbd_world_get_manifolds(avbd_world *world, avbd_manifolds *out);
+avbd_status avbd_world_set_manifolds(avbd_world *world, const avbd_manifolds *in);
+avbd_status avbd_world_stats(avbd_world *world, avbd_step_stats *out);
+
+/* Broad-phase probe for parity tests: Morton codes of the current swept-AABB centres
+ * (reference bvh_broadphase.cpp:19-36, 69-85) with the (code, index) sort order, and the
+ * sorted unique candidate pairs (min<<32 | max) the step would emit.  `codes`/`order`
+ * hold count bodies; `pairs` holds *pair_count on entry, the number found on return. */
+avbd_status avbd_world_probe_broadphase(avbd_world *world, uint64_t *codes, int32_t *order,
+                                        uint64_t *pairs, int32_t *pair_count);
+
+/* ----------------------------------------------------- batched worlds ---- */
+
+typedef struct avbd_batch avbd_batch;
+
+/* A batch steps many independent worlds with one captured graph per step.  Capacities
+ * are totals over all worlds. */
+avbd_status avbd_batch_create(const avbd_params *params, int32_t max_worlds,
+                              int32_t max_bodies, int32_t max_joints, int32_t max_manifolds,
+                              avbd_batch **out);
+avbd_status avbd_batch_destroy(avbd_batch *batch);
+/* Add a world; its id is written to *world_id.  Takes effect at the next step. */
+avbd_status avbd_batch_add_world(avbd_batch *batch, const avbd_bodies *bodies,
+                                 const avbd_joints *joints, int32_t *world_id);
+avbd_status avbd_batch_remove_world(avbd_batch *batch, int32_t world_id);
+/* One step of every world in the batch. */
+avbd_status avbd_batch_step(avbd_batch *batch);
+avbd_status avbd_batch_download_world(avbd_batch *batch, int32_t world_id,
+                                      avbd_bodies *bodies, avbd_joints *joints);
+avbd_status avbd_batch_world_count(avbd_batch *batch, int32_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AVBD_HIP_H */

@@ -1,0 +1,1654 @@
+// ORACLE - test infrastructure only.  A serial CPU restatement of the reference's
+// `avbd::Solver::step()` / `stepInterleaved()` for box bodies and `Joint`s, used by the
+// tests, the smoke check and the bench's CPU baseline as the checker of the HIP path.
+// Nothing in the product may include, link or call this file.
+//
+// It restates (file:line in the reference tree):
+//   bodies            src/avbd/types.hpp:75-139, rigid.cpp:22-108
+//   broad phase       src/avbd/solver_broadphase.cpp:30-186 (pair set of bvh_broadphase.cpp:225-290)
+//   Morton codes      src/avbd/bvh_broadphase.cpp:19-36, 61-96
+//   narrow phase      src/avbd/collide.cpp:71-490, 1001-1120 (box-box only)
+//   manifold store    src/avbd/manifold.cpp:20-88, 123-166, 168-671
+//   joints            src/avbd/joint.cpp:28-230
+//   step sequencing   src/avbd/solver_step.cpp:44-316, solver_solve.cpp:31-424
+//   brake/projection  src/avbd/solver_projection.cpp:33-281
+//   6x6 solves        src/avbd/maths.h:154-343, types.hpp:147-201
+// in the reference's floating-point evaluation order, serially (the reference's results do
+// not depend on its thread count: test/core_test.cpp:612-681).
+//
+// Out of scope (rejected with AVBD_ERR_UNSUPPORTED): sleeping (sleep_frames > 0), contact
+// grace ticks, jacobian arm caching, the convergence early exit, non-box shapes, springs
+// and 6-DOF joints.
+#include "avbd_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "orc_math.hpp"
+
+using namespace orc;
+
+namespace {
+
+constexpr float PENALTY_MIN = 1.0f;
+constexpr float COLLISION_MARGIN = 0.001f;
+constexpr float STICK_THRESH = 0.00001f;
+constexpr float CONTACT_RELEASE_GAP = 2.0e-3f;
+constexpr float ADAPTIVE_BETA_MAX = 64.0f;
+constexpr int MAXC = 8;
+constexpr int kMaxColours = 8;
+constexpr uint8_t kOverflow = kMaxColours;
+
+struct Body {
+    f3 pos; quat q;
+    f3 initLin; quat initAng;
+    f3 inertLin; quat inertAng;
+    f3 vel, angVel, prevVel;
+    f3 size; float mass; f3 moment; float friction, radius, gravity;
+    uint32_t layer, mask; uint8_t lockLin, lockAng;
+    bool inUpdate = false; int updateIndex = -1; bool contactOnly = true;
+};
+
+struct Joint {
+    int a, b; f3 rA, rB, C0Lin, C0Ang, penLin, penAng, lamLin, lamAng;
+    float stiffLin, stiffAng, fracture, torqueArm; bool broken;
+};
+
+struct Contact { int32_t key; f3 rA, rB; };
+
+struct Block {
+    m3 lhsLin, lhsAng, lhsCross; f3 rhsLin, rhsAng;
+};
+
+// ---------------------------------------------------------------- 6x6 solves ----
+// maths.h:156-198 (unpivoted LDL^T, sanity |x| < 10).
+bool ldlt6(const float a[6][6], const float b[6], float x[6])
+{
+    float l[6][6] = {}; float d[6] = {};
+    for (int j = 0; j < 6; j++) {
+        float dj = a[j][j];
+        for (int s = 0; s < j; s++) dj -= l[j][s] * l[j][s] * d[s];
+        if (!(dj > 1.0e-12f)) return false;
+        d[j] = dj; l[j][j] = 1.0f;
+        for (int i = j + 1; i < 6; i++) {
+            float lij = a[i][j];
+            for (int s = 0; s < j; s++) lij -= l[i][s] * l[j][s] * d[s];
+            l[i][j] = lij / d[j];
+        }
+    }
+    float z[6] = {};
+    for (int i = 0; i < 6; i++) {
+        float v = b[i];
+        for (int s = 0; s < i; s++) v -= l[i][s] * z[s];
+        z[i] = v;
+    }
+    for (int i = 0; i < 6; i++) x[i] = z[i] / d[i];
+    for (int i = 5; i >= 0; i--) {
+        float v = x[i];
+        for (int s = i + 1; s < 6; s++) v -= l[s][i] * x[s];
+        x[i] = v;
+    }
+    float worst = 0.0f;
+    for (int i = 0; i < 6; i++) worst = smax(worst, std::fabs(x[i]));
+    return std::isfinite(worst) && worst < 1.0e1f;
+}
+
+// maths.h:202-307 (Eigen-order pivoted LDL^T).
+bool ldlt6Pivoted(const float ain[6][6], const float b[6], float x[6])
+{
+    float a[6][6]; std::memcpy(a, ain, sizeof(a));
+    float d[6] = {}; int t[6] = {0, 1, 2, 3, 4, 5};
+    bool zeroPivot = false;
+    for (int k = 0; k < 6; k++) {
+        int j = k; float best = std::fabs(a[k][k]);
+        for (int i = k + 1; i < 6; i++) {
+            const float c = std::fabs(a[i][i]);
+            if (c > best) { best = c; j = i; }
+        }
+        t[k] = j;
+        if (j != k) {
+            for (int c = 0; c < 6; c++) std::swap(a[k][c], a[j][c]);
+            for (int r = 0; r < 6; r++) std::swap(a[r][k], a[r][j]);
+        }
+        if (k > 0) {
+            float temp[6] = {}, prod[6] = {};
+            for (int i = 0; i < k; i++) temp[i] = d[i] * a[k][i];
+            for (int i = 0; i < k; i++) prod[i] = a[k][i] * temp[i];
+            float dt;
+            switch (k) {
+            case 1: dt = prod[0]; break;
+            case 2: dt = prod[0] + prod[1]; break;
+            case 3: dt = (prod[0] + prod[1]) + prod[2]; break;
+            case 4: dt = (prod[0] + prod[2]) + (prod[1] + prod[3]); break;
+            default: dt = ((prod[0] + prod[2]) + (prod[1] + prod[3])) + prod[4]; break;
+            }
+            a[k][k] -= dt;
+            for (int i = 0; i < k; i++) {
+                const float ti = temp[i];
+                for (int r = k + 1; r < 6; r++) a[r][k] -= a[r][i] * ti;
+            }
+        }
+        const float pivot = a[k][k];
+        const bool valid = std::fabs(pivot) > 0.0f;
+        if (!valid) {
+            for (int r = k + 1; r < 6; r++)
+                if (a[r][k] != 0.0f) return false;
+            zeroPivot = true;
+        } else if (zeroPivot) {
+            return false;
+        }
+        if (valid)
+            for (int r = k + 1; r < 6; r++) a[r][k] /= pivot;
+        d[k] = pivot;
+    }
+    for (int i = 0; i < 6; i++) x[i] = b[i];
+    for (int k = 0; k < 6; k++) std::swap(x[k], x[t[k]]);
+    for (int jj = 0; jj < 5; jj++) {
+        const float v = x[jj];
+        if (v != 0.0f)
+            for (int i = jj + 1; i < 6; i++) x[i] -= a[i][jj] * v;
+    }
+    const float tol = std::numeric_limits<float>::min();
+    for (int i = 0; i < 6; i++) x[i] = std::fabs(d[i]) > tol ? x[i] / d[i] : 0.0f;
+    for (int jj = 5; jj >= 1; jj--) {
+        const float v = x[jj];
+        if (v != 0.0f)
+            for (int i = jj - 1; i >= 0; i--) x[i] -= a[jj][i] * v;
+    }
+    for (int k = 5; k >= 0; k--) std::swap(x[k], x[t[k]]);
+    float worst = 0.0f;
+    for (int i = 0; i < 6; i++) worst = smax(worst, std::fabs(x[i]));
+    return std::isfinite(worst) && worst < 1.0e1f;
+}
+
+// types.hpp:176-197 + maths.h:315-343: LDL^T, else pivoted, else zero.
+void solveDelta(const Block &blk, f3 &dLin, f3 &dAng)
+{
+    float a[6][6] = {};
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            a[r][c] = blk.lhsLin[c][r];
+            a[r + 3][c + 3] = blk.lhsAng[c][r];
+            a[r + 3][c] = blk.lhsCross[c][r];
+            a[r][c + 3] = blk.lhsCross[r][c];
+        }
+    const float b[6] = {-blk.rhsLin.x, -blk.rhsLin.y, -blk.rhsLin.z,
+                        -blk.rhsAng.x, -blk.rhsAng.y, -blk.rhsAng.z};
+    float x[6] = {};
+    if (ldlt6(a, b, x) || ldlt6Pivoted(a, b, x)) {
+        dLin = f3{x[0], x[1], x[2]};
+        dAng = f3{x[3], x[4], x[5]};
+        return;
+    }
+    dLin = f3{0, 0, 0};
+    dAng = f3{0, 0, 0};
+}
+
+// types.hpp:157-172
+Block makeBlock(const Body &b, float dtSq)
+{
+    Block k;
+    const f3 mo = f3{b.mass, b.mass, b.mass} / dtSq;
+    k.lhsLin = diagonal(mo.x, mo.y, mo.z);
+    k.lhsCross = zero3();
+    k.rhsLin = cmul(mo, b.pos - b.inertLin);
+    const m3 rot = qmat(b.q);
+    const m3 iw = mul(rot, mul(diagonal(b.moment.x, b.moment.y, b.moment.z), transpose(rot)));
+    k.lhsAng = iw / dtSq;
+    k.rhsAng = mul(k.lhsAng, qsub(b.q, b.inertAng));
+    return k;
+}
+
+// ---------------------------------------------------------------- narrow phase ----
+// collide.cpp box-box path.
+struct OBB { f3 center; quat rot; f3 half; f3 axis[3]; };
+struct SatAxis { int type; int ia, ib; float sep; f3 n; bool valid; };
+struct FaceFrame { int axisIndex; f3 normal, center, u, v; float eu, ev; };
+enum { AXIS_FACE_A = 0, AXIS_FACE_B = 1, AXIS_EDGE = 2 };
+constexpr float SAT_EPS = 1.0e-6f;
+constexpr float PLANE_EPS = 1.0e-5f;
+constexpr float MERGE_DIST_SQ = 1.0e-6f;
+constexpr int MAX_POLY = 16;
+
+OBB makeOBB(f3 center, quat rot, f3 half)
+{
+    OBB o{center, rot, half, {}};
+    o.axis[0] = qrot(rot, f3{1, 0, 0});
+    o.axis[1] = qrot(rot, f3{0, 1, 0});
+    o.axis[2] = qrot(rot, f3{0, 0, 1});
+    return o;
+}
+
+f3 supportPoint(const OBB &b, f3 dir)
+{
+    const float sx = dot(dir, b.axis[0]) >= 0.0f ? 1.0f : -1.0f;
+    const float sy = dot(dir, b.axis[1]) >= 0.0f ? 1.0f : -1.0f;
+    const float sz = dot(dir, b.axis[2]) >= 0.0f ? 1.0f : -1.0f;
+    return ((b.center + b.axis[0] * (b.half.x * sx)) + b.axis[1] * (b.half.y * sy)) + b.axis[2] * (b.half.z * sz);
+}
+
+void faceAxes(const OBB &b, int i, f3 &u, f3 &v, float &eu, float &ev)
+{
+    if (i == 0) { u = b.axis[1]; v = b.axis[2]; eu = b.half.y; ev = b.half.z; }
+    else if (i == 1) { u = b.axis[0]; v = b.axis[2]; eu = b.half.x; ev = b.half.z; }
+    else { u = b.axis[0]; v = b.axis[1]; eu = b.half.x; ev = b.half.y; }
+}
+
+int clipPoly(const f3 *in, int n, f3 pn, float off, f3 *out)
+{
+    if (n <= 0) return 0;
+    int m = 0;
+    f3 a = in[n - 1];
+    float da = dot(pn, a) - off;
+    for (int i = 0; i < n; i++) {
+        const f3 b = in[i];
+        const float db = dot(pn, b) - off;
+        const bool ai = da <= PLANE_EPS, bi = db <= PLANE_EPS;
+        if (ai != bi) {
+            float t = 0.0f;
+            const float den = da - db;
+            if (std::fabs(den) > SAT_EPS) t = sclamp(da / den, 0.0f, 1.0f);
+            if (m < MAX_POLY) out[m++] = a + (b - a) * t;
+        }
+        if (bi && m < MAX_POLY) out[m++] = b;
+        a = b; da = db;
+    }
+    return m;
+}
+
+struct ShapeIn { f3 center; quat rot; };
+
+bool addContact(const ShapeIn &A, const ShapeIn &B, Contact *cs, int &cnt, f3 *mids, f3 xA, f3 xB, int key)
+{
+    const f3 mid = (xA + xB) * 0.5f;
+    for (int i = 0; i < cnt; i++)
+        if (length2(mid - mids[i]) < MERGE_DIST_SQ) return false;
+    if (cnt >= MAXC) return false;
+    Contact &c = cs[cnt];
+    c.key = key;
+    c.rA = qrot(qconj(A.rot), xA - A.center);
+    c.rB = qrot(qconj(B.rot), xB - B.center);
+    mids[cnt] = mid;
+    ++cnt;
+    return true;
+}
+
+int faceManifold(const ShapeIn &sA, const ShapeIn &sB, const OBB &bA, const OBB &bB, bool refIsA, int refAxis,
+                 f3 nAB, float margin, Contact *cs)
+{
+    const OBB &ref = refIsA ? bA : bB;
+    const OBB &inc = refIsA ? bB : bA;
+    const f3 refOut = refIsA ? nAB : -nAB;
+    FaceFrame fr;
+    {
+        const float sign = dot(refOut, ref.axis[refAxis]) >= 0.0f ? 1.0f : -1.0f;
+        fr.axisIndex = refAxis;
+        fr.normal = ref.axis[refAxis] * sign;
+        fr.center = ref.center + fr.normal * ref.half[refAxis];
+        faceAxes(ref, refAxis, fr.u, fr.v, fr.eu, fr.ev);
+    }
+    int incAxis = 0;
+    {
+        float best = -FLT_MAX;
+        for (int i = 0; i < 3; i++) {
+            const float d = std::fabs(dot(inc.axis[i], fr.normal));
+            if (d > best) { best = d; incAxis = i; }
+        }
+    }
+    f3 c0[MAX_POLY], c1[MAX_POLY];
+    {
+        const float sign = dot(inc.axis[incAxis], fr.normal) > 0.0f ? -1.0f : 1.0f;
+        const f3 fn = inc.axis[incAxis] * sign;
+        const f3 fc = inc.center + fn * inc.half[incAxis];
+        f3 u, v; float eu, ev;
+        faceAxes(inc, incAxis, u, v, eu, ev);
+        c0[0] = (fc + u * eu) + v * ev;
+        c0[1] = (fc - u * eu) + v * ev;
+        c0[2] = (fc - u * eu) - v * ev;
+        c0[3] = (fc + u * eu) - v * ev;
+    }
+    int n = 4;
+    f3 p;
+    p = fr.u;  n = clipPoly(c0, n, p, dot(p, fr.center) + fr.eu, c1); if (!n) return 0;
+    p = -fr.u; n = clipPoly(c1, n, p, dot(p, fr.center) + fr.eu, c0); if (!n) return 0;
+    p = fr.v;  n = clipPoly(c0, n, p, dot(p, fr.center) + fr.ev, c1); if (!n) return 0;
+    p = -fr.v; n = clipPoly(c1, n, p, dot(p, fr.center) + fr.ev, c0); if (!n) return 0;
+
+    int cnt = 0;
+    f3 mids[MAXC];
+    int prefix = (refIsA ? AXIS_FACE_A : AXIS_FACE_B) << 24;
+    prefix |= (refAxis & 0xFF) << 16;
+    prefix |= (incAxis & 0xFF) << 8;
+    const float accept = smax(margin, PLANE_EPS);
+    for (int i = 0; i < n && cnt < MAXC; i++) {
+        const f3 pi = c0[i];
+        const float dist = dot(pi - fr.center, fr.normal);
+        if (dist > accept) continue;
+        const f3 pr = pi - fr.normal * dist;
+        addContact(sA, sB, cs, cnt, mids, refIsA ? pr : pi, refIsA ? pi : pr, prefix | (i & 0xFF));
+    }
+    if (!cnt) {
+        const f3 xA = supportPoint(bA, nAB);
+        f3 xB = supportPoint(bB, -nAB);
+        xB = xA - nAB * dot(xA - xB, nAB);
+        addContact(sA, sB, cs, cnt, mids, xA, xB, prefix);
+    }
+    return cnt;
+}
+
+void supportEdge(const OBB &b, int ai, f3 dir, f3 &ea, f3 &eb)
+{
+    const int a1 = (ai + 1) % 3, a2 = (ai + 2) % 3;
+    const float s1 = dot(dir, b.axis[a1]) >= 0.0f ? 1.0f : -1.0f;
+    const float s2 = dot(dir, b.axis[a2]) >= 0.0f ? 1.0f : -1.0f;
+    const f3 ec = (b.center + b.axis[a1] * (b.half[a1] * s1)) + b.axis[a2] * (b.half[a2] * s2);
+    ea = ec - b.axis[ai] * b.half[ai];
+    eb = ec + b.axis[ai] * b.half[ai];
+}
+
+void closestSeg(f3 p0, f3 p1, f3 q0, f3 q1, f3 &c0, f3 &c1)
+{
+    const f3 d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
+    const float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+    float s = 0.0f, t = 0.0f;
+    if (a <= SAT_EPS && e <= SAT_EPS) { c0 = p0; c1 = q0; return; }
+    if (a <= SAT_EPS) {
+        t = sclamp(f / e, 0.0f, 1.0f);
+    } else {
+        const float c = dot(d1, r);
+        if (e <= SAT_EPS) {
+            s = sclamp(-c / a, 0.0f, 1.0f);
+        } else {
+            const float b = dot(d1, d2);
+            const float den = a * e - b * b;
+            if (std::fabs(den) > SAT_EPS) s = sclamp((b * f - c * e) / den, 0.0f, 1.0f);
+            t = (b * s + f) / e;
+            if (t < 0.0f) { t = 0.0f; s = sclamp(-c / a, 0.0f, 1.0f); }
+            else if (t > 1.0f) { t = 1.0f; s = sclamp((b - c) / a, 0.0f, 1.0f); }
+        }
+    }
+    c0 = p0 + d1 * s;
+    c1 = q0 + d2 * t;
+}
+
+int edgeContact(const ShapeIn &sA, const ShapeIn &sB, const OBB &bA, const OBB &bB, int axA, int axB, f3 nAB,
+                Contact *cs)
+{
+    f3 a0, a1, b0, b1;
+    supportEdge(bA, axA, nAB, a0, a1);
+    supportEdge(bB, axB, -nAB, b0, b1);
+    f3 xA, xB;
+    closestSeg(a0, a1, b0, b1, xA, xB);
+    int cnt = 0;
+    f3 mids[MAXC];
+    const int key = (AXIS_EDGE << 24) | ((axA & 0xFF) << 8) | (axB & 0xFF);
+    addContact(sA, sB, cs, cnt, mids, xA, xB, key);
+    if (!cnt) {
+        xA = supportPoint(bA, nAB);
+        xB = supportPoint(bB, -nAB);
+        addContact(sA, sB, cs, cnt, mids, xA, xB, key);
+    }
+    return cnt;
+}
+
+} // namespace
+
+// collide.cpp:1034-1097 (box-box SAT + clipping).  Returns the contact count; `basis` is
+// written only when the boxes are within the margin on every axis.
+extern "C" int oracle_collide_boxes(const float posA[3], const float rotA[4], const float sizeA[3],
+                                    const float posB[3], const float rotB[4], const float sizeB[3],
+                                    float margin, float basis[9], int32_t keys[8], float rA[24],
+                                    float rB[24])
+{
+    const ShapeIn sA{{posA[0], posA[1], posA[2]}, {rotA[0], rotA[1], rotA[2], rotA[3]}};
+    const ShapeIn sB{{posB[0], posB[1], posB[2]}, {rotB[0], rotB[1], rotB[2], rotB[3]}};
+    const f3 hA = f3{sizeA[0], sizeA[1], sizeA[2]} * 0.5f;
+    const f3 hB = f3{sizeB[0], sizeB[1], sizeB[2]} * 0.5f;
+    const OBB bA = makeOBB(sA.center, sA.rot, hA);
+    const OBB bB = makeOBB(sB.center, sB.rot, hB);
+    const f3 delta = bB.center - bA.center;
+    float M[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) M[i][j] = dot(bA.axis[i], bB.axis[j]);
+    const f3 dA{dot(delta, bA.axis[0]), dot(delta, bA.axis[1]), dot(delta, bA.axis[2])};
+    const f3 dB{dot(delta, bB.axis[0]), dot(delta, bB.axis[1]), dot(delta, bB.axis[2])};
+    SatAxis bf{}; bf.sep = -FLT_MAX; bf.valid = false;
+    SatAxis be{}; be.sep = -FLT_MAX; be.valid = false;
+    for (int i = 0; i < 3; i++) {
+        const float sep = std::fabs(dA[i]) -
+                (((bA.half[i] + bB.half.x * std::fabs(M[i][0])) + bB.half.y * std::fabs(M[i][1])) +
+                        bB.half.z * std::fabs(M[i][2]));
+        if (sep > margin) return 0;
+        if (!bf.valid || sep > bf.sep) {
+            bf.valid = true; bf.type = AXIS_FACE_A; bf.ia = i; bf.ib = -1; bf.sep = sep;
+            bf.n = bA.axis[i] * (dA[i] < 0.0f ? -1.0f : 1.0f);
+        }
+    }
+    for (int j = 0; j < 3; j++) {
+        const float sep = std::fabs(dB[j]) -
+                (((bB.half[j] + bA.half.x * std::fabs(M[0][j])) + bA.half.y * std::fabs(M[1][j])) +
+                        bA.half.z * std::fabs(M[2][j]));
+        if (sep > margin) return 0;
+        if (!bf.valid || sep > bf.sep) {
+            bf.valid = true; bf.type = AXIS_FACE_B; bf.ia = -1; bf.ib = j; bf.sep = sep;
+            bf.n = bB.axis[j] * (dB[j] < 0.0f ? -1.0f : 1.0f);
+        }
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            const f3 axis = cross(bA.axis[i], bB.axis[j]);
+            const float lenSq = length2(axis);
+            if (lenSq < SAT_EPS) continue;
+            const float invLen = 1.0f / std::sqrt(lenSq);
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            const float ra = (bA.half[i1] * std::fabs(M[i2][j]) + bA.half[i2] * std::fabs(M[i1][j])) * invLen;
+            const float rb = (bB.half[j1] * std::fabs(M[i][j2]) + bB.half[j2] * std::fabs(M[i][j1])) * invLen;
+            const float sd = dot(delta, axis) * invLen;
+            const float sep = std::fabs(sd) - (ra + rb);
+            if (sep > margin) return 0;
+            if (!be.valid || sep > be.sep) {
+                be.valid = true; be.type = AXIS_EDGE; be.ia = i; be.ib = j; be.sep = sep;
+                be.n = axis * (sd < 0.0f ? -invLen : invLen);
+            }
+        }
+    if (!bf.valid) return 0;
+    SatAxis best = bf;
+    if (be.valid && 0.95f * be.sep > bf.sep + 0.01f) best = be;
+    const m3 B = orthonormal(-best.n);
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) basis[r * 3 + c] = B[c][r];
+    Contact cs[MAXC] = {};
+    int n;
+    if (best.type == AXIS_EDGE) n = edgeContact(sA, sB, bA, bB, best.ia, best.ib, best.n, cs);
+    else if (best.type == AXIS_FACE_A) n = faceManifold(sA, sB, bA, bB, true, best.ia, best.n, margin, cs);
+    else n = faceManifold(sA, sB, bA, bB, false, best.ib, best.n, margin, cs);
+    for (int k = 0; k < n; k++) {
+        keys[k] = cs[k].key;
+        rA[3 * k] = cs[k].rA.x; rA[3 * k + 1] = cs[k].rA.y; rA[3 * k + 2] = cs[k].rA.z;
+        rB[3 * k] = cs[k].rB.x; rB[3 * k + 1] = cs[k].rB.y; rB[3 * k + 2] = cs[k].rB.z;
+    }
+    return n;
+}
+
+namespace {
+
+// ---------------------------------------------------------------- the world ----
+struct World {
+    avbd_params P;
+    float collisionDt;
+    int roundsThisStep = 8;
+    std::vector<Body> bodies;
+    std::vector<Joint> joints;
+    // Manifold store: stable slots, live walk order, LIFO free list (manifold.cpp:20-88).
+    std::vector<int> mA, mB;
+    std::vector<m3> basis;
+    std::vector<int> numC;
+    std::vector<float> mFric;
+    std::vector<Contact> cont;
+    std::vector<f3> C0, pen, lam;
+    std::vector<uint8_t> stick;
+    std::vector<float> betaScale;
+    std::vector<uint32_t> live, freeList;
+    // Adjacency CSR by body index: joints (list order) then manifolds (live order).
+    // An entry >= 0 is a manifold slot, < 0 is joint -(j+1).
+    std::vector<uint32_t> adjStart;
+    std::vector<int> adj;
+    std::vector<int> updateOrder;
+    std::vector<uint8_t> colour;
+    std::vector<int> colourBegins, colourOrder, overflowOrder;
+    int colourCount = 0;
+    std::vector<f3> dLin, dAng;
+    // statistics of the last step
+    int lastPairs = 0, lastNew = 0, lastLevels = 0;
+
+    float pmax() const { return P.penalty_max; }
+
+    uint32_t acquire(int a, int b)
+    {
+        uint32_t s;
+        if (!freeList.empty()) { s = freeList.back(); freeList.pop_back(); }
+        else {
+            s = static_cast<uint32_t>(mA.size());
+            mA.push_back(0); mB.push_back(0); basis.push_back(zero3()); numC.push_back(0); mFric.push_back(0);
+            cont.resize(cont.size() + MAXC); C0.resize(C0.size() + MAXC); pen.resize(pen.size() + MAXC);
+            lam.resize(lam.size() + MAXC); stick.resize(stick.size() + MAXC); betaScale.resize(betaScale.size() + MAXC);
+        }
+        mA[s] = a; mB[s] = b; basis[s] = zero3(); numC[s] = 0; mFric[s] = 0.0f;
+        for (int k = 0; k < MAXC; k++) {
+            const int i = s * MAXC + k;
+            cont[i] = Contact{0, {0, 0, 0}, {0, 0, 0}};
+            C0[i] = pen[i] = lam[i] = f3{0, 0, 0};
+            stick[i] = 0; betaScale[i] = 1.0f;
+        }
+        live.push_back(s);
+        return s;
+    }
+
+    void buildAdjacency()
+    {
+        const int n = static_cast<int>(bodies.size());
+        adjStart.assign(n + 1, 0);
+        for (const Joint &j : joints) {
+            if (j.a >= 0) adjStart[j.a + 1]++;
+            adjStart[j.b + 1]++;
+        }
+        for (uint32_t s : live) { adjStart[mA[s] + 1]++; adjStart[mB[s] + 1]++; }
+        for (int i = 0; i < n; i++) adjStart[i + 1] += adjStart[i];
+        adj.assign(adjStart[n], 0);
+        std::vector<uint32_t> cur(adjStart.begin(), adjStart.begin() + n);
+        for (size_t k = 0; k < joints.size(); k++) {
+            const int ref = -static_cast<int>(k) - 1;
+            if (joints[k].a >= 0) adj[cur[joints[k].a]++] = ref;
+            adj[cur[joints[k].b]++] = ref;
+        }
+        for (uint32_t s : live) {
+            adj[cur[mA[s]]++] = static_cast<int>(s);
+            adj[cur[mB[s]]++] = static_cast<int>(s);
+        }
+    }
+
+    int otherOf(int ref, int body) const
+    {
+        int a, b;
+        if (ref >= 0) { a = mA[ref]; b = mB[ref]; }
+        else { const Joint &j = joints[-ref - 1]; a = j.a; b = j.b; }
+        if (a == body) return b;
+        if (b == body) return a;
+        return -1;
+    }
+
+    // ------------------------------------------------------------ broad phase ----
+    void sweptAabbs(std::vector<f3> &cen, std::vector<f3> &half) const
+    {
+        const int n = static_cast<int>(bodies.size());
+        cen.resize(n); half.resize(n);
+        for (int i = 0; i < n; i++) {
+            const Body &b = bodies[i];
+            const f3 h = b.size * 0.5f;
+            const m3 R = qmat(b.q);
+            const f3 tight{
+                (std::fabs(R[0][0]) * h.x + std::fabs(R[1][0]) * h.y) + std::fabs(R[2][0]) * h.z,
+                (std::fabs(R[0][1]) * h.x + std::fabs(R[1][1]) * h.y) + std::fabs(R[2][1]) * h.z,
+                (std::fabs(R[0][2]) * h.x + std::fabs(R[1][2]) * h.y) + std::fabs(R[2][2]) * h.z};
+            f3 disp = b.vel * collisionDt;
+            if (b.mass > 0) disp += f3{0, 0, b.gravity * collisionDt * collisionDt};
+            const float rotSweep = b.radius * length(b.angVel) * collisionDt;
+            const float grow = rotSweep + P.speculative_margin;
+            cen[i] = b.pos + disp * 0.5f;
+            half[i] = (tight + fabs3(disp) * 0.5f) + f3{grow, grow, grow};
+        }
+    }
+
+    static bool overlap(f3 ac, f3 ah, f3 bc, f3 bh)
+    {
+        const f3 aMin = ac - ah, aMax = ac + ah, bMin = bc - bh, bMax = bc + bh;
+        return !(bMin.x > aMax.x || bMin.y > aMax.y || bMin.z > aMax.z || aMin.x > bMax.x || aMin.y > bMax.y ||
+                 aMin.z > bMax.z);
+    }
+
+    // The candidate set the reference's LBVH self-query emits: every i < j whose swept
+    // AABBs overlap on closed intervals (bvh_broadphase.cpp:231-290), sorted and unique.
+    // Found here by a sweep over x, which yields the same set.
+    void candidatePairs(const std::vector<f3> &cen, const std::vector<f3> &half, std::vector<uint64_t> &out) const
+    {
+        const int n = static_cast<int>(cen.size());
+        out.clear();
+        std::vector<int> ord(n);
+        std::vector<float> lo(n);
+        for (int i = 0; i < n; i++) { ord[i] = i; lo[i] = cen[i].x - half[i].x; }
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return lo[a] < lo[b]; });
+        for (int p = 0; p < n; p++) {
+            const int i = ord[p];
+            const float hi = cen[i].x + half[i].x;
+            for (int q = p + 1; q < n; q++) {
+                const int j = ord[q];
+                if (lo[j] > hi) break;
+                if (overlap(cen[i], half[i], cen[j], half[j])) {
+                    const uint64_t a = static_cast<uint32_t>(std::min(i, j));
+                    const uint64_t b = static_cast<uint32_t>(std::max(i, j));
+                    out.push_back((a << 32) | b);
+                }
+            }
+        }
+        std::sort(out.begin(), out.end());
+        out.erase(std::unique(out.begin(), out.end()), out.end());
+    }
+
+    bool constrained(int a, int b) const
+    {
+        for (uint32_t k = adjStart[a]; k < adjStart[a + 1]; k++)
+            if (otherOf(adj[k], a) == b) return true;
+        return false;
+    }
+
+    void broadPhase()
+    {
+        buildAdjacency();
+        const int n = static_cast<int>(bodies.size());
+        lastPairs = 0; lastNew = 0;
+        if (n < 2) return;
+        std::vector<f3> cen, half;
+        sweptAabbs(cen, half);
+        std::vector<uint64_t> pairs;
+        candidatePairs(cen, half, pairs);
+        lastPairs = static_cast<int>(pairs.size());
+        std::vector<uint8_t> valid(pairs.size());
+        for (size_t k = 0; k < pairs.size(); k++) {
+            const int i = static_cast<int>(pairs[k] >> 32), j = static_cast<int>(pairs[k] & 0xFFFFFFFFu);
+            const Body &A = bodies[i], &B = bodies[j];
+            const bool ov = overlap(cen[i], half[i], cen[j], half[j]);
+            const bool layers = ((A.layer & B.mask) != 0) || ((B.layer & A.mask) != 0);
+            const bool bothStatic = A.mass <= 0.0f && B.mass <= 0.0f;
+            valid[k] = ov && layers && !constrained(i, j) && !bothStatic;
+        }
+        for (size_t k = 0; k < pairs.size(); k++) {
+            if (!valid[k]) continue;
+            const int i = static_cast<int>(pairs[k] >> 32), j = static_cast<int>(pairs[k] & 0xFFFFFFFFu);
+            bool exists = false;
+            for (uint32_t e = adjStart[i]; e < adjStart[i + 1]; e++)
+                if (adj[e] >= 0 && otherOf(adj[e], i) == j) { exists = true; break; }
+            if (exists) continue;
+            acquire(i, j);
+            lastNew++;
+        }
+    }
+
+    // ------------------------------------------------------------ manifolds ----
+    f3 constraintError(f3 c0, f3 bracket, const m3 &B, float oma) const
+    {
+        const bool spec = P.speculative_margin > 0.0f;
+        const bool separated = spec && c0.x > COLLISION_MARGIN + CONTACT_RELEASE_GAP;
+        return c0 * (separated ? 1.0f : oma) + mul(B, bracket);
+    }
+
+    struct Clamp { float bounds, scale; };
+    static Clamp clampForce(f3 &F, float mu)
+    {
+        F[0] = smin(F[0], 0.0f);
+        const float bounds = std::fabs(F[0]) * mu;
+        const float fs = length2d(F[1], F[2]);
+        if (fs > bounds && fs > 0) {
+            F[1] *= bounds / fs;
+            F[2] *= bounds / fs;
+        }
+        return {bounds, fs};
+    }
+
+    struct Hess { float s00, s01, s02, s11, s12, s22; };
+    static Hess basisHessian(const m3 &B, f3 k)
+    {
+        const f3 b0 = row(B, 0), b1 = row(B, 1), b2 = row(B, 2);
+        const float b0x2 = b0.x * b0.x, b0y2 = b0.y * b0.y, b0z2 = b0.z * b0.z;
+        const float b0xy = b0.x * b0.y, b0xz = b0.x * b0.z, b0yz = b0.y * b0.z;
+        const float b1x2 = b1.x * b1.x, b1y2 = b1.y * b1.y, b1z2 = b1.z * b1.z;
+        const float b1xy = b1.x * b1.y, b1xz = b1.x * b1.z, b1yz = b1.y * b1.z;
+        const float b2x2 = b2.x * b2.x, b2y2 = b2.y * b2.y, b2z2 = b2.z * b2.z;
+        const float b2xy = b2.x * b2.y, b2xz = b2.x * b2.z, b2yz = b2.y * b2.z;
+        return {k.x * b0x2 + k.y * b1x2 + k.z * b2x2, k.x * b0xy + k.y * b1xy + k.z * b2xy,
+                k.x * b0xz + k.y * b1xz + k.z * b2xz, k.x * b0y2 + k.y * b1y2 + k.z * b2y2,
+                k.x * b0yz + k.y * b1yz + k.z * b2yz, k.x * b0z2 + k.y * b1z2 + k.z * b2z2};
+    }
+
+    float marginFor(const Body &a, const Body &b) const
+    {
+        float m = P.speculative_margin;
+        if (m > 0.0f) {
+            const float h = collisionDt;
+            const float gA = a.mass > 0.0f ? a.gravity : 0.0f;
+            const float gB = b.mass > 0.0f ? b.gravity : 0.0f;
+            const f3 disp = (a.vel - b.vel) * h + f3{0, 0, (gA - gB) * h * h};
+            m += length(disp) + (a.radius * length(a.angVel) + b.radius * length(b.angVel)) * h;
+        }
+        return m;
+    }
+
+    // manifold.cpp:237-435 (grace ticks off).
+    bool initialize(uint32_t s)
+    {
+        const Body &a = bodies[mA[s]], &b = bodies[mB[s]];
+        const int first = static_cast<int>(s) * MAXC;
+        mFric[s] = std::sqrt(a.friction * b.friction);
+        Contact nc[MAXC] = {};
+        float bas[9];
+        int32_t keys[8]; float ra[24], rb[24];
+        const float pa[3] = {a.pos.x, a.pos.y, a.pos.z}, qa[4] = {a.q.x, a.q.y, a.q.z, a.q.w},
+                    sa[3] = {a.size.x, a.size.y, a.size.z};
+        const float pb[3] = {b.pos.x, b.pos.y, b.pos.z}, qb[4] = {b.q.x, b.q.y, b.q.z, b.q.w},
+                    sb[3] = {b.size.x, b.size.y, b.size.z};
+        const int nn = oracle_collide_boxes(pa, qa, sa, pb, qb, sb, marginFor(a, b), bas, keys, ra, rb);
+        for (int k = 0; k < nn; k++)
+            nc[k] = Contact{keys[k], {ra[3 * k], ra[3 * k + 1], ra[3 * k + 2]}, {rb[3 * k], rb[3 * k + 1], rb[3 * k + 2]}};
+        // The reference writes the basis only when the pair is within the margin on every
+        // axis; a pair with no contact is released before its basis is read again.
+        if (nn > 0)
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) basis[s][c][r] = bas[r * 3 + c];
+
+        f3 nPen[MAXC] = {}, nLam[MAXC] = {};
+        uint8_t nStick[MAXC] = {};
+        float nBeta[MAXC], prevErr[MAXC];
+        for (int i = 0; i < MAXC; i++) { nBeta[i] = 1.0f; prevErr[i] = -1.0f; }
+        for (int i = 0; i < nn; i++) {
+            for (int j = 0; j < numC[s]; j++) {
+                if (nc[i].key != cont[first + j].key) continue;
+                const f3 newRA = nc[i].rA, newRB = nc[i].rB;
+                nc[i] = cont[first + j];
+                prevErr[i] = smax(0.0f, COLLISION_MARGIN - C0[first + j].x);
+                nPen[i] = pen[first + j];
+                nLam[i] = lam[first + j];
+                nStick[i] = stick[first + j];
+                nBeta[i] = betaScale[first + j];
+                if (!stick[first + j]) {
+                    nc[i].rA = newRA; nc[i].rB = newRB;
+                } else {
+                    const f3 oldA = a.pos + qrot(a.q, cont[first + j].rA);
+                    const f3 oldB = b.pos + qrot(b.q, cont[first + j].rB);
+                    const f3 newA = a.pos + qrot(a.q, newRA);
+                    const f3 newB = b.pos + qrot(b.q, newRB);
+                    const float drift = smax(length(oldA - newA), length(oldB - newB));
+                    if (drift > P.stick_drift_heal) {
+                        nc[i].rA = newRA; nc[i].rB = newRB;
+                        nStick[i] = 0;
+                        nLam[i] = f3{nLam[i].x, 0.0f, 0.0f};
+                    }
+                }
+                break;
+            }
+        }
+        numC[s] = nn;
+        for (int i = 0; i < nn; i++) {
+            cont[first + i] = nc[i]; pen[first + i] = nPen[i]; lam[first + i] = nLam[i];
+            stick[first + i] = nStick[i]; betaScale[first + i] = nBeta[i];
+        }
+        for (int i = 0; i < numC[s]; i++) {
+            const f3 xA = a.pos + qrot(a.q, cont[first + i].rA);
+            const f3 xB = b.pos + qrot(b.q, cont[first + i].rB);
+            C0[first + i] = mul(basis[s], xA - xB) + f3{COLLISION_MARGIN, 0, 0};
+            if (C0[first + i].x < -P.max_penetration) C0[first + i].x = -P.max_penetration;
+            if (P.adaptive_beta && prevErr[i] >= 0.0f) {
+                const float err = smax(0.0f, COLLISION_MARGIN - C0[first + i].x);
+                const float floor = P.adaptive_beta_error_floor;
+                float sc = nBeta[i];
+                if (err <= floor) sc = smax(sc * 0.5f, 1.0f);
+                else if (prevErr[i] <= floor) sc = smin(sc * 2.0f, ADAPTIVE_BETA_MAX);
+                else if (err > 0.5f * prevErr[i]) sc = smin(sc * 2.0f, ADAPTIVE_BETA_MAX);
+                else if (err < 0.1f * prevErr[i]) sc = smax(sc * 0.5f, 1.0f);
+                betaScale[first + i] = sc;
+            } else {
+                betaScale[first + i] = 1.0f;
+            }
+            if (C0[first + i][0] > COLLISION_MARGIN + CONTACT_RELEASE_GAP) {
+                lam[first + i] = f3{0, 0, 0};
+                stick[first + i] = 0;
+            }
+            lam[first + i] = lam[first + i] * P.alpha * P.gamma;
+            pen[first + i] = lclamp3(pen[first + i] * P.gamma, f3{PENALTY_MIN, PENALTY_MIN, PENALTY_MIN},
+                                     f3{pmax(), pmax(), pmax()});
+        }
+        return numC[s] > 0;
+    }
+
+    bool frozen(uint32_t s) const
+    {
+        auto awake = [](const Body &b) { return b.mass > 0.0f; };
+        return !awake(bodies[mA[s]]) && !awake(bodies[mB[s]]);
+    }
+
+    // Contact forces of one manifold at the current poses (manifold.cpp:650-686): world
+    // arms and cone-clamped forces.  The reference caches these; recomputing gives the same
+    // floats because nothing they read changes between the cache fill and its use.
+    void contactForces(uint32_t s, f3 *F, f3 *rAw, f3 *rBw) const
+    {
+        const Body &a = bodies[mA[s]], &b = bodies[mB[s]];
+        const int first = static_cast<int>(s) * MAXC;
+        const f3 dqAL = a.pos - a.initLin, dqAA = qsub(a.q, a.initAng);
+        const f3 dqBL = b.pos - b.initLin, dqBA = qsub(b.q, b.initAng);
+        const m3 rotA = qmat(a.q), rotB = qmat(b.q);
+        for (int i = 0; i < numC[s]; i++) {
+            const f3 rA = mul(rotA, cont[first + i].rA);
+            const f3 rB = mul(rotB, cont[first + i].rB);
+            rAw[i] = rA; rBw[i] = rB;
+            const f3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
+            const f3 C = constraintError(C0[first + i], br, basis[s], 1 - P.alpha);
+            f3 f = cmul(pen[first + i], C) + lam[first + i];
+            clampForce(f, mFric[s]);
+            F[i] = f;
+        }
+    }
+
+    // manifold.cpp:688-755
+    void stampManifold(uint32_t s, int body, Block &blk) const
+    {
+        f3 F[MAXC], rAw[MAXC], rBw[MAXC];
+        contactForces(s, F, rAw, rBw);
+        const bool fromA = body == mA[s];
+        const float side = fromA ? 1.0f : -1.0f;
+        const int first = static_cast<int>(s) * MAXC;
+        const m3 &fr = basis[s];
+        const f3 b0 = row(fr, 0), b1 = row(fr, 1), b2 = row(fr, 2);
+        m3 accLin = zero3(), accAng = zero3(), accCross = zero3();
+        f3 accRL{0, 0, 0}, accRA{0, 0, 0};
+        for (int i = 0; i < numC[s]; i++) {
+            const f3 f = F[i];
+            const f3 r = fromA ? rAw[i] : rBw[i];
+            const Hess h = basisHessian(fr, pen[first + i]);
+            const m3 S{{f3{h.s00, h.s01, h.s02}, f3{h.s01, h.s11, h.s12}, f3{h.s02, h.s12, h.s22}}};
+            const m3 RS{{cross(r, S[0]), cross(r, S[1]), cross(r, S[2])}};
+            const float rx2 = r.x * r.x, ry2 = r.y * r.y, rz2 = r.z * r.z;
+            const float rxy = r.x * r.y, rxz = r.x * r.z, ryz = r.y * r.z;
+            const float h00 = h.s11 * rz2 - 2.0f * h.s12 * ryz + h.s22 * ry2;
+            const float h11 = h.s22 * rx2 - 2.0f * h.s02 * rxz + h.s00 * rz2;
+            const float h22 = h.s00 * ry2 - 2.0f * h.s01 * rxy + h.s11 * rx2;
+            const float h01 = h.s12 * rxz + h.s02 * ryz - h.s22 * rxy - h.s01 * rz2;
+            const float h02 = h.s12 * rxy + h.s01 * ryz - h.s11 * rxz - h.s02 * ry2;
+            const float h12 = h.s02 * rxy + h.s01 * rxz - h.s00 * ryz - h.s12 * rx2;
+            const f3 fw = (b0 * f.x + b1 * f.y) + b2 * f.z;
+            const f3 fwAng = cross(r, fw);
+            accLin += S;
+            accAng += m3{{f3{h00, h01, h02}, f3{h01, h11, h12}, f3{h02, h12, h22}}};
+            accCross += RS;
+            accRL += fw;
+            accRA += fwAng;
+        }
+        blk.lhsLin += accLin;
+        blk.lhsAng += accAng;
+        blk.lhsCross += accCross;
+        blk.rhsLin += accRL * side;
+        blk.rhsAng += accRA * side;
+    }
+
+    // manifold.cpp:768-856 (arm caching off)
+    void updateDual(uint32_t s)
+    {
+        const Body &a = bodies[mA[s]], &b = bodies[mB[s]];
+        const int first = static_cast<int>(s) * MAXC;
+        const m3 &fr = basis[s];
+        const float mu = mFric[s];
+        const f3 dqAL = a.pos - a.initLin, dqAA = qsub(a.q, a.initAng);
+        const f3 dqBL = b.pos - b.initLin, dqBA = qsub(b.q, b.initAng);
+        const m3 rotA = qmat(a.q), rotB = qmat(b.q);
+        for (int i = 0; i < numC[s]; i++) {
+            const f3 rA = mul(rotA, cont[first + i].rA);
+            const f3 rB = mul(rotB, cont[first + i].rB);
+            const f3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
+            const f3 C = constraintError(C0[first + i], br, fr, 1 - P.alpha);
+            f3 F = cmul(pen[first + i], C) + lam[first + i];
+            const Clamp cone = clampForce(F, mu);
+            lam[first + i] = F;
+            const float beta = P.beta_lin * betaScale[first + i];
+            if (F[0] < 0) pen[first + i][0] = smin(pen[first + i][0] + beta * std::fabs(C[0]), pmax());
+            if (cone.scale <= cone.bounds) {
+                pen[first + i][1] = smin(pen[first + i][1] + beta * std::fabs(C[1]), pmax());
+                pen[first + i][2] = smin(pen[first + i][2] + beta * std::fabs(C[2]), pmax());
+                stick[first + i] = length2d(C[1], C[2]) < STICK_THRESH;
+            }
+        }
+    }
+
+    void decayPenalty(uint32_t s, float decay)
+    {
+        const int first = static_cast<int>(s) * MAXC;
+        for (int i = 0; i < numC[s]; i++)
+            pen[first + i] = lclamp3(pen[first + i] * decay, f3{PENALTY_MIN, PENALTY_MIN, PENALTY_MIN},
+                                     f3{pmax(), pmax(), pmax()});
+    }
+
+    void rebase(uint32_t s)
+    {
+        const Body &a = bodies[mA[s]], &b = bodies[mB[s]];
+        const int first = static_cast<int>(s) * MAXC;
+        for (int i = 0; i < numC[s]; i++) {
+            const f3 xA = a.pos + qrot(a.q, cont[first + i].rA);
+            const f3 xB = b.pos + qrot(b.q, cont[first + i].rB);
+            C0[first + i] = mul(basis[s], xA - xB) + f3{COLLISION_MARGIN, 0, 0};
+            if (C0[first + i].x < -P.max_penetration) C0[first + i].x = -P.max_penetration;
+        }
+    }
+
+    // ------------------------------------------------------------ joints ----
+    f3 jointLin(const Joint &j) const
+    {
+        const f3 pa = j.a >= 0 ? bodies[j.a].pos + qrot(bodies[j.a].q, j.rA) : j.rA;
+        const Body &B = bodies[j.b];
+        return pa - (B.pos + qrot(B.q, j.rB));
+    }
+    f3 jointAng(const Joint &j) const
+    {
+        const quat qa = j.a >= 0 ? bodies[j.a].q : quat{0, 0, 0, 1};
+        return qsub(qa, bodies[j.b].q) * j.torqueArm;
+    }
+    static f3 warmPenalty(f3 p, float gamma, float stiff, float pmaxv)
+    {
+        return {smin(sclamp(p.x * gamma, PENALTY_MIN, pmaxv), stiff),
+                smin(sclamp(p.y * gamma, PENALTY_MIN, pmaxv), stiff),
+                smin(sclamp(p.z * gamma, PENALTY_MIN, pmaxv), stiff)};
+    }
+    bool jointInit(Joint &j)
+    {
+        j.C0Lin = jointLin(j);
+        j.C0Ang = jointAng(j);
+        j.lamLin = j.lamLin * P.alpha * P.gamma;
+        j.lamAng = j.lamAng * P.alpha * P.gamma;
+        j.penLin = warmPenalty(j.penLin, P.gamma, j.stiffLin, pmax());
+        j.penAng = warmPenalty(j.penAng, P.gamma, j.stiffAng, pmax());
+        return !j.broken;
+    }
+    static m3 gsBall(int k, f3 v)
+    {
+        m3 m = diagonal(-v[k], -v[k], -v[k]);
+        m[k][0] += v[0]; m[k][1] += v[1]; m[k][2] += v[2];
+        return m;
+    }
+    // joint.cpp:80-160
+    void jointPrimal(const Joint &j, int body, Block &blk) const
+    {
+        const bool isA = body == j.a;
+        if (length2(j.penLin) > 0) {
+            const m3 K = diagonal(j.penLin.x, j.penLin.y, j.penLin.z);
+            f3 C = jointLin(j);
+            if (std::isinf(j.stiffLin)) C -= j.C0Lin * P.alpha;
+            const f3 F = mul(K, C) + j.lamLin;
+            const m3 jLin = isA ? identity3() : -identity3();
+            m3 jAng;
+            if (isA) {
+                const f3 r = -qrot(bodies[j.a].q, j.rA);
+                jAng = m3{{f3{0, r.z, -r.y}, f3{-r.z, 0, r.x}, f3{r.y, -r.x, 0}}};
+            } else {
+                const f3 r = qrot(bodies[j.b].q, j.rB);
+                jAng = m3{{f3{0, r.z, -r.y}, f3{-r.z, 0, r.x}, f3{r.y, -r.x, 0}}};
+            }
+            const m3 jLinT = transpose(jLin), jAngT = transpose(jAng);
+            const m3 jAngTk = mul(jAngT, K);
+            blk.lhsLin += mul(jLinT, K, jLin);
+            blk.lhsAng += mul(jAngTk, jAng);
+            blk.lhsCross += mul(jAngTk, jLin);
+            const f3 r = isA ? qrot(bodies[j.a].q, j.rA) : -qrot(bodies[j.b].q, j.rB);
+            const m3 H = (gsBall(0, r) * F[0] + gsBall(1, r) * F[1]) + gsBall(2, r) * F[2];
+            blk.lhsAng += diagonal(H[0][0], H[1][1], H[2][2]);
+            blk.rhsLin += mul(jLinT, F);
+            blk.rhsAng += mul(jAngT, F);
+        }
+        if (length2(j.penAng) > 0) {
+            const m3 K = diagonal(j.penAng.x, j.penAng.y, j.penAng.z);
+            f3 C = jointAng(j);
+            if (std::isinf(j.stiffAng)) C -= j.C0Ang * P.alpha;
+            const f3 F = mul(K, C) + j.lamAng;
+            const m3 jA = (isA ? identity3() : m3{{f3{0, 0, 0}, f3{0, -1, 0}, f3{0, 0, -1}}}) * j.torqueArm;
+            blk.lhsAng += mul(transpose(jA), K, jA);
+            blk.rhsAng += mul(transpose(jA), F);
+        }
+    }
+    // joint.cpp:171-230
+    void jointDual(Joint &j) const
+    {
+        if (length2(j.penLin) > 0) {
+            const m3 K = diagonal(j.penLin.x, j.penLin.y, j.penLin.z);
+            f3 C = jointLin(j);
+            if (std::isinf(j.stiffLin)) {
+                C -= j.C0Lin * P.alpha;
+                j.lamLin = mul(K, C) + j.lamLin;
+            }
+            j.penLin = lmin3(j.penLin + fabs3(C) * P.beta_lin,
+                             lmin3(f3{j.stiffLin, j.stiffLin, j.stiffLin}, f3{pmax(), pmax(), pmax()}));
+        }
+        if (length2(j.penAng) > 0) {
+            const m3 K = diagonal(j.penAng.x, j.penAng.y, j.penAng.z);
+            f3 C = jointAng(j);
+            if (std::isinf(j.stiffAng)) {
+                C -= j.C0Ang * P.alpha;
+                j.lamAng = mul(K, C) + j.lamAng;
+            }
+            j.penAng = lmin3(j.penAng + fabs3(C) * P.beta_ang,
+                             lmin3(f3{j.stiffAng, j.stiffAng, j.stiffAng}, f3{pmax(), pmax(), pmax()}));
+        }
+        if (length2(j.lamAng) > j.fracture * j.fracture) {
+            j.penLin = j.penAng = j.lamLin = j.lamAng = f3{0, 0, 0};
+            j.broken = true;
+        }
+    }
+
+    // ------------------------------------------------------------ step phases ----
+    void warmstartForces()
+    {
+        std::vector<uint8_t> act(joints.size());
+        for (size_t k = 0; k < joints.size(); k++) act[k] = jointInit(joints[k]);
+        size_t w = 0;
+        for (size_t k = 0; k < joints.size(); k++)
+            if (act[k]) joints[w++] = joints[k];
+        joints.resize(w);
+        std::vector<uint8_t> mact(live.size());
+        for (size_t i = 0; i < live.size(); i++) mact[i] = frozen(live[i]) || initialize(live[i]);
+        size_t lw = 0;
+        for (size_t i = 0; i < live.size(); i++) {
+            if (!mact[i]) { freeList.push_back(live[i]); continue; }
+            live[lw++] = live[i];
+        }
+        live.resize(lw);
+        buildAdjacency();
+    }
+
+    void colourGraph()
+    {
+        const int n = static_cast<int>(updateOrder.size());
+        colour.assign(n, kOverflow);
+        colourOrder.clear();
+        overflowOrder.clear();
+        for (int i = 0; i < n; i++) {
+            const int body = updateOrder[i];
+            uint8_t used[kMaxColours] = {};
+            for (uint32_t e = adjStart[body]; e < adjStart[body + 1]; e++) {
+                const int o = otherOf(adj[e], body);
+                if (o < 0 || bodies[o].updateIndex < 0 || bodies[o].updateIndex >= i) continue;
+                const uint8_t c = colour[bodies[o].updateIndex];
+                if (c < kMaxColours) used[c] = 1;
+            }
+            uint8_t c = kOverflow;
+            for (int k = 0; k < kMaxColours; k++)
+                if (!used[k]) { c = static_cast<uint8_t>(k); break; }
+            colour[i] = c;
+        }
+        colourBegins.assign(kMaxColours + 1, 0);
+        for (int i = 0; i < n; i++) {
+            if (P.contact_only_overflow && bodies[updateOrder[i]].contactOnly) {
+                colour[i] = kOverflow;
+                overflowOrder.push_back(i);
+            }
+            if (colour[i] != kOverflow) colourBegins[colour[i] + 1]++;
+        }
+        for (int c = 0; c < kMaxColours; c++) colourBegins[c + 1] += colourBegins[c];
+        colourOrder.resize(n - overflowOrder.size());
+        int cursor[kMaxColours];
+        for (int c = 0; c < kMaxColours; c++) cursor[c] = colourBegins[c];
+        for (int i = 0; i < n; i++)
+            if (colour[i] < kMaxColours) colourOrder[cursor[colour[i]]++] = i;
+        colourCount = 0;
+        for (int c = 0; c < kMaxColours; c++)
+            if (colourBegins[c + 1] > colourBegins[c]) colourCount = c + 1;
+    }
+
+    void selectUpdateBodies()
+    {
+        updateOrder.clear();
+        for (size_t i = 0; i < bodies.size(); i++) {
+            Body &b = bodies[i];
+            b.inUpdate = false; b.updateIndex = -1; b.contactOnly = true;
+            if (b.mass > 0) {
+                b.updateIndex = static_cast<int>(updateOrder.size());
+                updateOrder.push_back(static_cast<int>(i));
+                b.inUpdate = true;
+            }
+        }
+        dLin.assign(updateOrder.size(), f3{0, 0, 0});
+        dAng.assign(updateOrder.size(), f3{0, 0, 0});
+        for (const Joint &j : joints) {
+            if (j.a >= 0) bodies[j.a].contactOnly = false;
+            bodies[j.b].contactOnly = false;
+        }
+        colourGraph();
+    }
+
+    static void maskLocked(uint8_t locks, f3 &v)
+    {
+        if (locks)
+            for (int k = 0; k < 3; k++)
+                if (locks & (1 << k)) v[k] = 0.0f;
+    }
+
+    void warmstartBody(Body &b)
+    {
+        const float dt = P.dt;
+        maskLocked(b.lockLin, b.vel);
+        maskLocked(b.lockAng, b.angVel);
+        const float g = (b.lockLin & 0x4) ? 0.0f : b.gravity;
+        b.inertLin = b.pos + b.vel * dt;
+        if (b.mass > 0) b.inertLin += f3{0, 0, g} * (dt * dt);
+        b.inertAng = qadd(b.q, b.angVel * dt);
+        const f3 accel = (b.vel - b.prevVel) / dt;
+        const float accelExt = accel.z * std::copysign(1.0f, g);
+        float w = sclamp(accelExt / std::fabs(g), 0.0f, 1.0f);
+        if (!std::isfinite(w)) w = 0.0f;
+        b.initLin = b.pos;
+        b.initAng = b.q;
+        if (b.mass > 0) {
+            b.pos = (b.pos + b.vel * dt) + f3{0, 0, g} * (w * dt * dt);
+            b.q = qadd(b.q, b.angVel * dt);
+        }
+    }
+
+    void finishBody(Body &b)
+    {
+        const float dt = P.dt;
+        b.prevVel = b.vel;
+        if (b.mass > 0) {
+            b.vel = (b.pos - b.initLin) / dt;
+            b.angVel = qsub(b.q, b.initAng) / dt;
+        }
+        maskLocked(b.lockLin, b.vel);
+        maskLocked(b.lockAng, b.angVel);
+    }
+
+    Block assemble(int body, bool fromCache) const
+    {
+        (void)fromCache;
+        const Body &b = bodies[body];
+        Block blk = makeBlock(b, P.dt * P.dt);
+        for (uint32_t e = adjStart[body]; e < adjStart[body + 1]; e++) {
+            const int ref = adj[e];
+            if (ref >= 0) stampManifold(static_cast<uint32_t>(ref), body, blk);
+            else jointPrimal(joints[-ref - 1], body, blk);
+        }
+        return blk;
+    }
+
+    void iterate(int target)
+    {
+        const int newtonRounds = std::clamp(
+                static_cast<int>(std::ceil(P.newton_ratio * static_cast<float>(target))), 1, target);
+        for (int it = 0; it < target; it++) {
+            const bool newton = it < newtonRounds;
+            for (int c = 0; newton && c < colourCount; c++) {
+                for (int k = colourBegins[c]; k < colourBegins[c + 1]; k++) {
+                    const int body = updateOrder[colourOrder[k]];
+                    const Block blk = assemble(body, false);
+                    f3 dl, da;
+                    solveDelta(blk, dl, da);
+                    bodies[body].pos = bodies[body].pos + dl;
+                    bodies[body].q = qadd(bodies[body].q, da);
+                }
+            }
+            const int ov = newton ? static_cast<int>(overflowOrder.size()) : 0;
+            if (ov > 0) {
+                for (int k = 0; k < ov; k++) {
+                    const int i = overflowOrder[k];
+                    const Block blk = assemble(updateOrder[i], true);
+                    solveDelta(blk, dLin[i], dAng[i]);
+                }
+                for (int k = 0; k < ov; k++) {
+                    const int i = overflowOrder[k];
+                    Body &b = bodies[updateOrder[i]];
+                    b.pos = b.pos + dLin[i] * P.jacobi_relaxation;
+                    b.q = qadd(b.q, dAng[i] * P.jacobi_relaxation);
+                }
+            }
+            const float decay = newton ? 1.0f : P.stiffness_decay;
+            for (uint32_t s : live) {
+                if (frozen(s)) continue;
+                if (decay < 1.0f) decayPenalty(s, decay);
+                updateDual(s);
+            }
+            for (Joint &j : joints) jointDual(j);
+        }
+    }
+
+    // solver_projection.cpp:33-53
+    static float angCompliance(const Body &b, f3 t)
+    {
+        float sum = 0.0f;
+        for (int k = 0; k < 3; k++)
+            if (b.moment[k] > 0.0f) sum += t[k] * t[k] / b.moment[k];
+        return sum;
+    }
+    static f3 angResponse(const Body &b, f3 t)
+    {
+        f3 o{0, 0, 0};
+        for (int k = 0; k < 3; k++)
+            if (b.moment[k] > 0.0f) o[k] = t[k] / b.moment[k];
+        return qrot(b.q, o);
+    }
+
+    // solver_projection.cpp:67-281
+    void projectContacts(bool positions, float brakeWindow)
+    {
+        if (live.empty()) return;
+        const float window = brakeWindow > 0.0f ? brakeWindow : collisionDt;
+        std::vector<uint32_t> active;
+        for (uint32_t s : live) {
+            if (numC[s] == 0) continue;
+            const Body &A = bodies[mA[s]], &B = bodies[mB[s]];
+            if (!(A.mass > 0.0f && A.inUpdate) && !(B.mass > 0.0f && B.inUpdate)) continue;
+            active.push_back(s);
+        }
+        if (active.empty()) return;
+        if (P.speculative_margin > 0.0f) {
+            for (uint32_t s : active) {
+                Body &A = bodies[mA[s]], &B = bodies[mB[s]];
+                const int first = static_cast<int>(s) * MAXC;
+                const bool moveA = A.mass > 0.0f && A.inUpdate, moveB = B.mass > 0.0f && B.inUpdate;
+                const float imA = moveA ? 1.0f / A.mass : 0.0f, imB = moveB ? 1.0f / B.mass : 0.0f;
+                const f3 n = row(basis[s], 0);
+                f3 pRA{0, 0, 0}, pRB{0, 0, 0};
+                float verr = 0.0f;
+                int braked = 0;
+                for (int k = 0; k < numC[s]; k++) {
+                    const f3 rA = qrot(A.q, cont[first + k].rA);
+                    const f3 rB = qrot(B.q, cont[first + k].rB);
+                    const float gap = dot(n, (A.pos + rA) - (B.pos + rB));
+                    if (!(gap > 0.0f)) continue;
+                    const f3 vA = A.vel + cross(A.angVel, rA), vB = B.vel + cross(B.angVel, rB);
+                    const float vn = dot(vA - vB, n);
+                    const float target = -gap / window;
+                    if (vn < target) { pRA += rA; pRB += rB; verr += target - vn; braked++; }
+                }
+                if (braked == 0) continue;
+                const float invN = 1.0f / braked;
+                const f3 tA = moveA ? qrot(qconj(A.q), cross(pRA * invN, n)) : f3{0, 0, 0};
+                const f3 tB = moveB ? qrot(qconj(B.q), cross(pRB * invN, n)) : f3{0, 0, 0};
+                const float K = imA + imB + (moveA ? angCompliance(A, tA) : 0.0f) + (moveB ? angCompliance(B, tB) : 0.0f);
+                const float j = verr * invN / K;
+                if (moveA) {
+                    f3 dv = n * (imA * j), dw = angResponse(A, tA) * j;
+                    maskLocked(A.lockLin, dv); maskLocked(A.lockAng, dw);
+                    A.vel += dv; A.angVel += dw;
+                }
+                if (moveB) {
+                    f3 dv = n * (imB * j), dw = angResponse(B, tB) * j;
+                    maskLocked(B.lockLin, dv); maskLocked(B.lockAng, dw);
+                    B.vel -= dv; B.angVel -= dw;
+                }
+            }
+        }
+        if (!positions || P.projection_iterations <= 0) return;
+        const size_t nu = updateOrder.size();
+        std::vector<f3> pl(nu), pa(nu);
+        std::vector<int> cnt(nu);
+        const float tol = smax(P.projection_slop * 0.01f, 1.0e-6f);
+        for (int iter = 0; iter < P.projection_iterations; iter++) {
+            std::fill(cnt.begin(), cnt.end(), 0);
+            std::fill(pl.begin(), pl.end(), f3{0, 0, 0});
+            std::fill(pa.begin(), pa.end(), f3{0, 0, 0});
+            float maxDepth = 0.0f;
+            for (uint32_t s : active) {
+                const Body &A = bodies[mA[s]], &B = bodies[mB[s]];
+                const int first = static_cast<int>(s) * MAXC;
+                const bool moveA = A.mass > 0.0f && A.inUpdate, moveB = B.mass > 0.0f && B.inUpdate;
+                const float imA = moveA ? 1.0f / A.mass : 0.0f, imB = moveB ? 1.0f / B.mass : 0.0f;
+                const f3 n = row(basis[s], 0);
+                f3 pRA{0, 0, 0}, pRB{0, 0, 0};
+                float depthSum = 0.0f;
+                int corr = 0;
+                const m3 rotA = qmat(A.q), rotB = qmat(B.q);
+                for (int k = 0; k < numC[s]; k++) {
+                    const f3 rA = mul(rotA, cont[first + k].rA), rB = mul(rotB, cont[first + k].rB);
+                    const float raw = -dot(n, (A.pos + rA) - (B.pos + rB));
+                    const float depth = raw - P.projection_slop;
+                    if (depth > 0.0f) { pRA += rA; pRB += rB; depthSum += depth; corr++; }
+                }
+                if (corr == 0) continue;
+                const float invN = 1.0f / corr;
+                maxDepth = smax(maxDepth, depthSum * invN);
+                const f3 tA = moveA ? qrot(qconj(A.q), cross(pRA * invN, n)) : f3{0, 0, 0};
+                const f3 tB = moveB ? qrot(qconj(B.q), cross(pRB * invN, n)) : f3{0, 0, 0};
+                const float K = imA + imB + (moveA ? angCompliance(A, tA) : 0.0f) + (moveB ? angCompliance(B, tB) : 0.0f);
+                const float dp = P.projection_beta * depthSum * invN / K;
+                if (moveA) {
+                    const int i = A.updateIndex;
+                    cnt[i]++; pl[i] += n * (imA * dp); pa[i] += angResponse(A, tA) * dp;
+                }
+                if (moveB) {
+                    const int i = B.updateIndex;
+                    cnt[i]++; pl[i] -= n * (imB * dp); pa[i] -= angResponse(B, tB) * dp;
+                }
+            }
+            for (size_t i = 0; i < nu; i++) {
+                Body &b = bodies[updateOrder[i]];
+                const float inv = 1.0f / std::max(cnt[i], 1);
+                f3 dl = pl[i] * inv, da = pa[i] * inv;
+                maskLocked(b.lockLin, dl); maskLocked(b.lockAng, da);
+                const float cap = P.projection_max_correction;
+                if (cap > 0.0f) {
+                    const float moved = length(dl);
+                    if (moved > cap) dl = dl * (cap / moved);
+                    const float turned = length(da);
+                    if (turned > cap) da = da * (cap / turned);
+                }
+                b.pos = b.pos + dl;
+                b.q = qadd(b.q, da);
+            }
+            if (maxDepth <= tol) break;
+        }
+    }
+
+    void rebaseAll()
+    {
+        for (uint32_t s : live) rebase(s);
+    }
+
+    void runSubstep(int target)
+    {
+        for (Body &b : bodies) warmstartBody(b);
+        iterate(target);
+        for (Body &b : bodies) finishBody(b);
+    }
+
+    void step()
+    {
+        collisionDt = P.dt;
+        broadPhase();
+        warmstartForces();
+        selectUpdateBodies();
+        projectContacts(false, 0.0f);
+        const int target = P.iterations;
+        roundsThisStep = std::max(1, target);
+        runSubstep(target);
+        projectContacts(P.position_projection != 0, 0.0f);
+        rebaseAll();
+    }
+
+    void stepInterleaved(int substeps, int roundsPer)
+    {
+        if (substeps <= 1) { step(); return; }
+        collisionDt = P.dt * substeps;
+        broadPhase();
+        warmstartForces();
+        selectUpdateBodies();
+        const int rounds = roundsPer > 0 ? std::max(1, roundsPer)
+                                         : std::max(2, (P.iterations + substeps - 1) / substeps);
+        roundsThisStep = rounds;
+        for (int sub = 0; sub < substeps; ++sub) {
+            if (sub > 0) rebaseAll();
+            projectContacts(false, P.dt * (substeps - sub));
+            runSubstep(rounds);
+        }
+        projectContacts(P.position_projection != 0, collisionDt);
+        rebaseAll();
+    }
+};
+
+int checkParams(const avbd_params *p)
+{
+    if (!p) return AVBD_ERR_INVALID;
+    if (p->sleep_frames > 0 || p->contact_grace_ticks > 0 || p->jacobian_rebuild_distance > 0.0f ||
+        p->convergence_threshold > 0.0f)
+        return AVBD_ERR_UNSUPPORTED;
+    return AVBD_OK;
+}
+
+f3 load3(const float *p, int i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+void store3(float *p, int i, f3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
+
+} // namespace
+
+struct oracle_world {
+    World w;
+};
+
+extern "C" {
+
+oracle_world *oracle_create(const avbd_params *params)
+{
+    if (checkParams(params) != AVBD_OK) return nullptr;
+    auto *o = new oracle_world();
+    o->w.P = *params;
+    o->w.collisionDt = params->dt;
+    return o;
+}
+
+void oracle_destroy(oracle_world *o) { delete o; }
+
+int oracle_set_params(oracle_world *o, const avbd_params *params)
+{
+    const int st = checkParams(params);
+    if (st != AVBD_OK) return st;
+    o->w.P = *params;
+    return AVBD_OK;
+}
+
+int oracle_upload(oracle_world *o, const avbd_bodies *b, const avbd_joints *j)
+{
+    if (!o || !b || b->count < 0) return AVBD_ERR_INVALID;
+    World &w = o->w;
+    w.bodies.assign(b->count, Body{});
+    for (int i = 0; i < b->count; i++) {
+        Body &x = w.bodies[i];
+        x.size = load3(b->size, i);
+        x.mass = b->mass[i];
+        x.moment = load3(b->moment, i);
+        x.friction = b->friction[i];
+        x.radius = b->radius[i];
+        x.gravity = b->gravity ? b->gravity[i] : w.P.gravity;
+        x.layer = b->layer ? b->layer[i] : 1u;
+        x.mask = b->mask ? b->mask[i] : 1u;
+        x.lockLin = b->lock_linear ? b->lock_linear[i] : 0;
+        x.lockAng = b->lock_angular ? b->lock_angular[i] : 0;
+        x.pos = load3(b->position, i);
+        x.q = quat{b->rotation[4 * i], b->rotation[4 * i + 1], b->rotation[4 * i + 2], b->rotation[4 * i + 3]};
+        x.vel = load3(b->velocity, i);
+        x.angVel = load3(b->angular_velocity, i);
+        x.prevVel = b->prev_velocity ? load3(b->prev_velocity, i) : x.vel;
+        x.initLin = x.inertLin = x.pos;
+        x.initAng = x.inertAng = x.q;
+    }
+    w.joints.clear();
+    if (j && j->count > 0) {
+        w.joints.resize(j->count);
+        for (int k = 0; k < j->count; k++) {
+            Joint &x = w.joints[k];
+            x.a = j->body_a[k]; x.b = j->body_b[k];
+            x.rA = load3(j->r_a, k); x.rB = load3(j->r_b, k);
+            x.stiffLin = j->stiffness_lin[k]; x.stiffAng = j->stiffness_ang[k];
+            x.fracture = j->fracture[k]; x.torqueArm = j->torque_arm[k];
+            x.C0Lin = j->c0_lin ? load3(j->c0_lin, k) : f3{0, 0, 0};
+            x.C0Ang = j->c0_ang ? load3(j->c0_ang, k) : f3{0, 0, 0};
+            x.penLin = j->penalty_lin ? load3(j->penalty_lin, k) : f3{0, 0, 0};
+            x.penAng = j->penalty_ang ? load3(j->penalty_ang, k) : f3{0, 0, 0};
+            x.lamLin = j->lambda_lin ? load3(j->lambda_lin, k) : f3{0, 0, 0};
+            x.lamAng = j->lambda_ang ? load3(j->lambda_ang, k) : f3{0, 0, 0};
+            x.broken = j->broken ? j->broken[k] != 0 : false;
+            if (x.b < 0 || x.b >= b->count || x.a >= b->count) return AVBD_ERR_INVALID;
+        }
+    }
+    w.mA.clear(); w.mB.clear(); w.basis.clear(); w.numC.clear(); w.mFric.clear(); w.cont.clear();
+    w.C0.clear(); w.pen.clear(); w.lam.clear(); w.stick.clear(); w.betaScale.clear();
+    w.live.clear(); w.freeList.clear();
+    return AVBD_OK;
+}
+
+int oracle_step(oracle_world *o)
+{
+    o->w.step();
+    return AVBD_OK;
+}
+
+int oracle_step_interleaved(oracle_world *o, int substeps, int rounds)
+{
+    o->w.stepInterleaved(substeps, rounds);
+    return AVBD_OK;
+}
+
+int oracle_download(oracle_world *o, avbd_bodies *b, avbd_joints *j)
+{
+    const World &w = o->w;
+    if (b) {
+        b->count = static_cast<int32_t>(w.bodies.size());
+        for (size_t i = 0; i < w.bodies.size(); i++) {
+            const Body &x = w.bodies[i];
+            const int k = static_cast<int>(i);
+            if (b->position) store3(b->position, k, x.pos);
+            if (b->rotation) {
+                b->rotation[4 * k] = x.q.x; b->rotation[4 * k + 1] = x.q.y;
+                b->rotation[4 * k + 2] = x.q.z; b->rotation[4 * k + 3] = x.q.w;
+            }
+            if (b->velocity) store3(b->velocity, k, x.vel);
+            if (b->angular_velocity) store3(b->angular_velocity, k, x.angVel);
+            if (b->prev_velocity) store3(b->prev_velocity, k, x.prevVel);
+        }
+    }
+    if (j) {
+        j->count = static_cast<int32_t>(w.joints.size());
+        for (size_t k = 0; k < w.joints.size(); k++) {
+            const Joint &x = w.joints[k];
+            const int i = static_cast<int>(k);
+            if (j->body_a) j->body_a[i] = x.a;
+            if (j->body_b) j->body_b[i] = x.b;
+            if (j->c0_lin) store3(j->c0_lin, i, x.C0Lin);
+            if (j->c0_ang) store3(j->c0_ang, i, x.C0Ang);
+            if (j->penalty_lin) store3(j->penalty_lin, i, x.penLin);
+            if (j->penalty_ang) store3(j->penalty_ang, i, x.penAng);
+            if (j->lambda_lin) store3(j->lambda_lin, i, x.lamLin);
+            if (j->lambda_ang) store3(j->lambda_ang, i, x.lamAng);
+            if (j->broken) j->broken[i] = x.broken ? 1 : 0;
+        }
+    }
+    return AVBD_OK;
+}
+
+int oracle_manifold_count(oracle_world *o) { return static_cast<int>(o->w.live.size()); }
+
+int oracle_get_manifolds(oracle_world *o, avbd_manifolds *m)
+{
+    const World &w = o->w;
+    const int n = static_cast<int>(w.live.size());
+    if (m->count < n) return AVBD_ERR_CAPACITY;
+    m->count = n;
+    for (int i = 0; i < n; i++) {
+        const uint32_t s = w.live[i];
+        if (m->body_a) m->body_a[i] = w.mA[s];
+        if (m->body_b) m->body_b[i] = w.mB[s];
+        if (m->basis)
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) m->basis[9 * i + 3 * r + c] = w.basis[s][c][r];
+        if (m->friction) m->friction[i] = w.mFric[s];
+        if (m->num_contacts) m->num_contacts[i] = w.numC[s];
+        for (int k = 0; k < MAXC; k++) {
+            const int src = static_cast<int>(s) * MAXC + k, dst = i * MAXC + k;
+            if (m->feature) m->feature[dst] = w.cont[src].key;
+            if (m->r_a) store3(m->r_a, dst, w.cont[src].rA);
+            if (m->r_b) store3(m->r_b, dst, w.cont[src].rB);
+            if (m->c0) store3(m->c0, dst, w.C0[src]);
+            if (m->penalty) store3(m->penalty, dst, w.pen[src]);
+            if (m->lambda) store3(m->lambda, dst, w.lam[src]);
+            if (m->stick) m->stick[dst] = w.stick[src];
+            if (m->beta_scale) m->beta_scale[dst] = w.betaScale[src];
+        }
+    }
+    return AVBD_OK;
+}
+
+int oracle_set_manifolds(oracle_world *o, const avbd_manifolds *m)
+{
+    World &w = o->w;
+    w.mA.clear(); w.mB.clear(); w.basis.clear(); w.numC.clear(); w.mFric.clear(); w.cont.clear();
+    w.C0.clear(); w.pen.clear(); w.lam.clear(); w.stick.clear(); w.betaScale.clear();
+    w.live.clear(); w.freeList.clear();
+    for (int i = 0; i < m->count; i++) {
+        const uint32_t s = w.acquire(m->body_a[i], m->body_b[i]);
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) w.basis[s][c][r] = m->basis[9 * i + 3 * r + c];
+        w.mFric[s] = m->friction[i];
+        w.numC[s] = m->num_contacts[i];
+        for (int k = 0; k < MAXC; k++) {
+            const int dst = static_cast<int>(s) * MAXC + k, src = i * MAXC + k;
+            w.cont[dst] = Contact{m->feature[src], load3(m->r_a, src), load3(m->r_b, src)};
+            w.C0[dst] = load3(m->c0, src);
+            w.pen[dst] = load3(m->penalty, src);
+            w.lam[dst] = load3(m->lambda, src);
+            w.stick[dst] = m->stick[src];
+            w.betaScale[dst] = m->beta_scale ? m->beta_scale[src] : 1.0f;
+        }
+    }
+    return AVBD_OK;
+}
+
+uint64_t oracle_digest(oracle_world *o)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](const void *p, size_t n) {
+        const unsigned char *c = static_cast<const unsigned char *>(p);
+        for (size_t i = 0; i < n; i++) { h ^= c[i]; h *= 1099511628211ull; }
+    };
+    for (const Body &b : o->w.bodies) {
+        mix(&b.pos, 12); mix(&b.q, 16); mix(&b.vel, 12); mix(&b.angVel, 12);
+    }
+    return h;
+}
+
+void oracle_stats(oracle_world *o, int32_t out[6])
+{
+    const World &w = o->w;
+    int contacts = 0;
+    for (uint32_t s : w.live) contacts += w.numC[s];
+    out[0] = static_cast<int32_t>(w.live.size());
+    out[1] = contacts;
+    out[2] = w.lastPairs;
+    out[3] = w.lastNew;
+    out[4] = w.colourCount;
+    out[5] = static_cast<int32_t>(w.overflowOrder.size());
+}
+
+// Swept AABBs of the current state (solver_broadphase.cpp:101-122).
+void oracle_swept_aabbs(oracle_world *o, float *centers, float *halves)
+{
+    std::vector<f3> c, h;
+    o->w.collisionDt = o->w.P.dt;
+    o->w.sweptAabbs(c, h);
+    for (size_t i = 0; i < c.size(); i++) {
+        store3(centers, static_cast<int>(i), c[i]);
+        store3(halves, static_cast<int>(i), h[i]);
+    }
+}
+
+// Candidate pairs (sorted unique) of the current state, before the filter.
+int oracle_candidate_pairs(oracle_world *o, uint64_t *pairs, int capacity)
+{
+    std::vector<f3> c, h;
+    std::vector<uint64_t> p;
+    o->w.collisionDt = o->w.P.dt;
+    o->w.sweptAabbs(c, h);
+    o->w.candidatePairs(c, h, p);
+    const int n = static_cast<int>(p.size());
+    for (int i = 0; i < n && i < capacity; i++) pairs[i] = p[i];
+    return n;
+}
+
+// bvh_broadphase.cpp:19-36, 69-96: Morton codes over the centres (every axis offset by
+// lo.x, as the reference does) and the (code, index) order.
+void oracle_morton(const float *centers, int n, uint64_t *codes, int32_t *order)
+{
+    auto expand = [](uint32_t v) {
+        uint64_t x = v & 0x1fffffu;
+        x = (x | (x << 32)) & 0x1f00000000ffffull;
+        x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+        x = (x | (x << 8)) & 0x100f00f00f00f00full;
+        x = (x | (x << 4)) & 0x10cfcfcfcfcfcfcfull;
+        x = (x | (x << 2)) & 0x1249249249249249ull;
+        return x;
+    };
+    f3 lo{FLT_MAX, FLT_MAX, FLT_MAX}, hi{-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int i = 0; i < n; i++) {
+        const f3 c = load3(centers, i);
+        lo = f3{smin(lo.x, c.x), smin(lo.y, c.y), smin(lo.z, c.z)};
+        hi = f3{smax(hi.x, c.x), smax(hi.y, c.y), smax(hi.z, c.z)};
+    }
+    const f3 ext{hi.x - lo.x, hi.y - lo.y, hi.z - lo.z};
+    const float scale = 2097151.0f / std::max({ext.x, ext.y, ext.z, 1.0e-6f});
+    auto q = [&](float v) { return static_cast<uint32_t>(sclamp((v - lo.x) * scale, 0.0f, 2097151.0f)); };
+    for (int i = 0; i < n; i++) {
+        const f3 c = load3(centers, i);
+        codes[i] = expand(q(c.x)) | (expand(q(c.y)) << 1) | (expand(q(c.z)) << 2);
+    }
+    std::vector<int> ord(n);
+    for (int i = 0; i < n; i++) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](int a, int b) { return codes[a] != codes[b] ? codes[a] < codes[b] : a < b; });
+    for (int i = 0; i < n; i++) order[i] = ord[i];
+}
+
+// ldlt6_solve / pivoted / solve chain on one row-major 6x6 system (maths.h:154-343).
+int oracle_solve6(const float *a36, const float *b6, float *x6)
+{
+    float a[6][6];
+    for (int r = 0; r < 6; r++)
+        for (int c = 0; c < 6; c++) a[r][c] = a36[6 * r + c];
+    float x[6] = {};
+    int path = 0;
+    if (ldlt6(a, b6, x)) path = 1;
+    else if (ldlt6Pivoted(a, b6, x)) path = 2;
+    else for (int i = 0; i < 6; i++) x[i] = 0.0f;
+    for (int i = 0; i < 6; i++) x6[i] = x[i];
+    return path;
+}
+
+} // extern "C"

@@ -1,0 +1,36 @@
+// ORACLE - test infrastructure only.  C interface of the serial CPU restatement of the
+// reference AVBD step (avbd_oracle.cpp).  Used by tests/, the smoke check and the bench's
+// CPU baseline as the checker of the HIP path; never by the product.
+#pragma once
+#include "../include/avbd_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_world oracle_world;
+
+oracle_world *oracle_create(const avbd_params *params); /* null on unsupported params */
+void oracle_destroy(oracle_world *o);
+int oracle_set_params(oracle_world *o, const avbd_params *params);
+int oracle_upload(oracle_world *o, const avbd_bodies *b, const avbd_joints *j);
+int oracle_step(oracle_world *o);
+int oracle_step_interleaved(oracle_world *o, int substeps, int rounds);
+int oracle_download(oracle_world *o, avbd_bodies *b, avbd_joints *j);
+int oracle_manifold_count(oracle_world *o);
+int oracle_get_manifolds(oracle_world *o, avbd_manifolds *m);
+int oracle_set_manifolds(oracle_world *o, const avbd_manifolds *m);
+uint64_t oracle_digest(oracle_world *o);
+/* out: manifolds, contacts, candidate pairs, new manifolds, colours, overflow bodies */
+void oracle_stats(oracle_world *o, int32_t out[6]);
+void oracle_swept_aabbs(oracle_world *o, float *centers, float *halves);
+int oracle_candidate_pairs(oracle_world *o, uint64_t *pairs, int capacity);
+void oracle_morton(const float *centers, int n, uint64_t *codes, int32_t *order);
+int oracle_solve6(const float *a36, const float *b6, float *x6);
+int oracle_collide_boxes(const float posA[3], const float rotA[4], const float sizeA[3],
+                         const float posB[3], const float rotB[4], const float sizeB[3],
+                         float margin, float basis[9], int32_t keys[8], float rA[24], float rB[24]);
+
+#ifdef __cplusplus
+}
+#endif

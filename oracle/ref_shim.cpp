@@ -1,0 +1,58 @@
+// ORACLE - test infrastructure only.  C entry points over the reference's own sources,
+// compiled from where they lie in the reference tree (see oracle/Makefile): the box-box
+// narrow phase of src/avbd/collide.cpp and the header-only 6x6 solves of
+// src/avbd/maths.h.  They pin the restatement in avbd_oracle.cpp directly.
+#include "avbd/solver.h"
+
+extern "C" int ref_collide_boxes(const float posA[3], const float rotA[4], const float sizeA[3],
+                                 const float posB[3], const float rotB[4], const float sizeB[3],
+                                 float margin, float basis[9], int32_t keys[8], float rA[24],
+                                 float rB[24])
+{
+    using namespace avbd;
+    auto shape = [](const float *p, const float *q, const float *s) {
+        Shape sh;
+        sh.type = ShapeType::Box;
+        sh.center = float3{p[0], p[1], p[2]};
+        sh.rotation = quat{q[0], q[1], q[2], q[3]};
+        sh.half = float3{s[0], s[1], s[2]} * 0.5f;
+        sh.radius = s[0];
+        sh.halfHeight = s[2] * 0.5f;
+        sh.axis = qrot(sh.rotation, float3{0, 0, 1});
+        return sh;
+    };
+    std::array<Manifold::Contact, 8> cs{};
+    float3x3 b{};
+    const int n = collideShapes(shape(posA, rotA, sizeA), shape(posB, rotB, sizeB),
+                                std::span<Manifold::Contact>(cs), b, margin);
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) basis[3 * r + c] = b[c][r];
+    for (int k = 0; k < n; k++) {
+        keys[k] = cs[k].feature.key;
+        for (int c = 0; c < 3; c++) {
+            rA[3 * k + c] = cs[k].rA[c];
+            rB[3 * k + c] = cs[k].rB[c];
+        }
+    }
+    return n;
+}
+
+extern "C" int ref_solve6(const float *a36, const float *b6, float *x6)
+{
+    std::array<std::array<float, 6>, 6> a{};
+    std::array<float, 6> b{}, x{};
+    for (int r = 0; r < 6; r++) {
+        b[r] = b6[r];
+        for (int c = 0; c < 6; c++) a[r][c] = a36[6 * r + c];
+    }
+    int path = 0;
+    if (avbd::ldlt6_solve(a, b, x)) {
+        path = 1;
+    } else {
+        x = {};
+        if (avbd::ldlt6_pivoted_solve(a, b, x)) path = 2;
+        else x = {};
+    }
+    for (int i = 0; i < 6; i++) x6[i] = x[i];
+    return path;
+}
