@@ -1,0 +1,745 @@
+// Per-body, per-manifold and per-joint device routines of the AVBD step.
+// Each routine cites the reference function whose results it reproduces; the evaluation
+// order of every expression follows it (see dmath.hpp for the conventions).
+#pragma once
+
+#include "dmath.hpp"
+#include "state.hpp"
+
+namespace avbd_dev {
+
+constexpr float PENALTY_MIN = 1.0f;
+constexpr float COLLISION_MARGIN = 0.001f;
+constexpr float STICK_THRESH = 0.00001f;
+constexpr float CONTACT_RELEASE_GAP = 2.0e-3f;
+constexpr float ADAPTIVE_BETA_MAX = 64.0f;
+constexpr int MAXC = 8;
+constexpr float FLT_MAXV = 3.402823466e+38f;
+
+struct Block {
+    m3 lhsLin, lhsAng, lhsCross;
+    v3 rhsLin, rhsAng;
+};
+
+// ------------------------------------------------------------- body loads ----
+AD v3 bPos(const State &S, int b) { return ld3(S.pos[b]); }
+AD qt bRot(const State &S, int b) { return ldq(S.rot[b]); }
+AD float bMass(const State &S, int b) { return S.massI[b].x; }
+AD v3 bMoment(const State &S, int b) { const float4 m = S.massI[b]; return {m.y, m.z, m.w}; }
+
+// types.hpp:157-172: M/dt^2 block with world inertia R diag(I) R^T / dt^2.
+AD void makeBlock(const State &S, int b, Block &k)
+{
+    const float dtSq = S.P.dtSq;
+    const float4 mi = S.massI[b];
+    const v3 mo = v3{mi.x, mi.x, mi.x} / dtSq;
+    k.lhsLin = diag(mo.x, mo.y, mo.z);
+    k.lhsCross = zero3();
+    k.rhsLin = cmul(mo, bPos(S, b) - ld3(S.inertLin[b]));
+    const qt q = bRot(S, b);
+    const m3 rot = qmat(q);
+    const m3 iw = mul(rot, mul(diag(mi.y, mi.z, mi.w), transpose(rot)));
+    k.lhsAng = mdiv(iw, dtSq);
+    k.rhsAng = mul(k.lhsAng, qsub(q, ldq(S.inertAng[b])));
+}
+
+// maths.h:156-198: unpivoted LDL^T, accepted when every pivot > 1e-12 and max|x| < 10
+// (the max drops NaN entries exactly as std::max does).  Fully unrolled: a, l, d and x
+// live in VGPRs.
+AD bool ldlt6(const float (&a)[6][6], const float (&b)[6], float (&x)[6])
+{
+    float l[6][6];
+    float d[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+        for (int j = 0; j < 6; j++) l[i][j] = 0.0f;
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        float dj = a[j][j];
+#pragma unroll
+        for (int s = 0; s < j; s++) dj -= l[j][s] * l[j][s] * d[s];
+        ok = ok && (dj > 1.0e-12f);
+        d[j] = dj;
+        l[j][j] = 1.0f;
+#pragma unroll
+        for (int i = j + 1; i < 6; i++) {
+            float lij = a[i][j];
+#pragma unroll
+            for (int s = 0; s < j; s++) lij -= l[i][s] * l[j][s] * d[s];
+            l[i][j] = lij / d[j];
+        }
+    }
+    if (!ok) return false;
+    float z[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        float v = b[i];
+#pragma unroll
+        for (int s = 0; s < i; s++) v -= l[i][s] * z[s];
+        z[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) x[i] = z[i] / d[i];
+#pragma unroll
+    for (int i = 5; i >= 0; i--) {
+        float v = x[i];
+#pragma unroll
+        for (int s = i + 1; s < 6; s++) v -= l[s][i] * x[s];
+        x[i] = v;
+    }
+    float worst = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 6; i++) worst = smax(worst, fabsf(x[i]));
+    return __builtin_isfinite(worst) && worst < 1.0e1f;
+}
+
+// maths.h:202-307: Eigen-order pivoted LDL^T, the fallback for indefinite blocks.
+__device__ __noinline__ bool ldlt6Pivoted(const float (&ain)[6][6], const float (&b)[6], float (&x)[6])
+{
+    float a[6][6];
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) a[i][j] = ain[i][j];
+    float d[6] = {0, 0, 0, 0, 0, 0};
+    int t[6] = {0, 1, 2, 3, 4, 5};
+    bool zeroPivot = false;
+    for (int k = 0; k < 6; k++) {
+        int j = k;
+        float best = fabsf(a[k][k]);
+        for (int i = k + 1; i < 6; i++) {
+            const float c = fabsf(a[i][i]);
+            if (c > best) { best = c; j = i; }
+        }
+        t[k] = j;
+        if (j != k) {
+            for (int c = 0; c < 6; c++) { const float tmp = a[k][c]; a[k][c] = a[j][c]; a[j][c] = tmp; }
+            for (int r = 0; r < 6; r++) { const float tmp = a[r][k]; a[r][k] = a[r][j]; a[r][j] = tmp; }
+        }
+        if (k > 0) {
+            float temp[6], prod[6];
+            for (int i = 0; i < k; i++) temp[i] = d[i] * a[k][i];
+            for (int i = 0; i < k; i++) prod[i] = a[k][i] * temp[i];
+            float dt;
+            if (k == 1) dt = prod[0];
+            else if (k == 2) dt = prod[0] + prod[1];
+            else if (k == 3) dt = (prod[0] + prod[1]) + prod[2];
+            else if (k == 4) dt = (prod[0] + prod[2]) + (prod[1] + prod[3]);
+            else dt = ((prod[0] + prod[2]) + (prod[1] + prod[3])) + prod[4];
+            a[k][k] -= dt;
+            for (int i = 0; i < k; i++) {
+                const float ti = temp[i];
+                for (int r = k + 1; r < 6; r++) a[r][k] -= a[r][i] * ti;
+            }
+        }
+        const float pivot = a[k][k];
+        const bool valid = fabsf(pivot) > 0.0f;
+        if (!valid) {
+            for (int r = k + 1; r < 6; r++)
+                if (a[r][k] != 0.0f) return false;
+            zeroPivot = true;
+        } else if (zeroPivot) {
+            return false;
+        }
+        if (valid)
+            for (int r = k + 1; r < 6; r++) a[r][k] /= pivot;
+        d[k] = pivot;
+    }
+    for (int i = 0; i < 6; i++) x[i] = b[i];
+    for (int k = 0; k < 6; k++) { const float tmp = x[k]; x[k] = x[t[k]]; x[t[k]] = tmp; }
+    for (int jj = 0; jj < 5; jj++) {
+        const float v = x[jj];
+        if (v != 0.0f)
+            for (int i = jj + 1; i < 6; i++) x[i] -= a[i][jj] * v;
+    }
+    const float tol = 1.17549435e-38f;
+    for (int i = 0; i < 6; i++) x[i] = fabsf(d[i]) > tol ? x[i] / d[i] : 0.0f;
+    for (int jj = 5; jj >= 1; jj--) {
+        const float v = x[jj];
+        if (v != 0.0f)
+            for (int i = jj - 1; i >= 0; i--) x[i] -= a[jj][i] * v;
+    }
+    for (int k = 5; k >= 0; k--) { const float tmp = x[k]; x[k] = x[t[k]]; x[t[k]] = tmp; }
+    float worst = 0.0f;
+    for (int i = 0; i < 6; i++) worst = smax(worst, fabsf(x[i]));
+    return __builtin_isfinite(worst) && worst < 1.0e1f;
+}
+
+// types.hpp:176-197 with maths.h:315-343: LDL^T, else pivoted LDL^T, else a zero update.
+AD void solveDelta(const Block &k, v3 &dLin, v3 &dAng)
+{
+    float a[6][6];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            a[r][c] = k.lhsLin.c[c][r];
+            a[r + 3][c + 3] = k.lhsAng.c[c][r];
+            a[r + 3][c] = k.lhsCross.c[c][r];
+            a[r][c + 3] = k.lhsCross.c[r][c];
+        }
+    const float b[6] = {-k.rhsLin.x, -k.rhsLin.y, -k.rhsLin.z, -k.rhsAng.x, -k.rhsAng.y, -k.rhsAng.z};
+    float x[6];
+    if (ldlt6(a, b, x) || ldlt6Pivoted(a, b, x)) {
+        dLin = v3{x[0], x[1], x[2]};
+        dAng = v3{x[3], x[4], x[5]};
+        return;
+    }
+    dLin = v3{0, 0, 0};
+    dAng = v3{0, 0, 0};
+}
+
+// ----------------------------------------------------------- manifolds ----
+AD m3 loadBasis(const State &S, int slot)
+{
+    const v3 r0 = ld3(S.mBasis[3 * slot]), r1 = ld3(S.mBasis[3 * slot + 1]), r2 = ld3(S.mBasis[3 * slot + 2]);
+    return transpose(m3{{r0, r1, r2}});
+}
+
+// manifold.cpp:383-388
+AD v3 constraintError(const State &S, v3 c0, v3 bracket, const m3 &B)
+{
+    const bool spec = S.P.specMargin > 0.0f;
+    const bool separated = spec && c0.x > COLLISION_MARGIN + CONTACT_RELEASE_GAP;
+    return c0 * (separated ? 1.0f : S.P.oneMinusAlpha) + mul(B, bracket);
+}
+
+// manifold.cpp:407-419: push-only normal, tangential clamped to the friction cone.
+// Returns the pre-clamp tangential magnitude; `bounds` receives mu*|Fn|.
+AD float clampForce(v3 &F, float mu, float &bounds)
+{
+    F.x = smin(F.x, 0.0f);
+    bounds = fabsf(F.x) * mu;
+    const float fs = length2d(F.y, F.z);
+    if (fs > bounds && fs > 0) {
+        F.y *= bounds / fs;
+        F.z *= bounds / fs;
+    }
+    return fs;
+}
+
+struct Hess { float s00, s01, s02, s11, s12, s22; };
+// manifold.cpp:362-379: S = B^T K B.
+AD Hess basisHessian(const m3 &B, v3 k)
+{
+    const v3 b0 = row(B, 0), b1 = row(B, 1), b2 = row(B, 2);
+    const float b0x2 = b0.x * b0.x, b0y2 = b0.y * b0.y, b0z2 = b0.z * b0.z;
+    const float b0xy = b0.x * b0.y, b0xz = b0.x * b0.z, b0yz = b0.y * b0.z;
+    const float b1x2 = b1.x * b1.x, b1y2 = b1.y * b1.y, b1z2 = b1.z * b1.z;
+    const float b1xy = b1.x * b1.y, b1xz = b1.x * b1.z, b1yz = b1.y * b1.z;
+    const float b2x2 = b2.x * b2.x, b2y2 = b2.y * b2.y, b2z2 = b2.z * b2.z;
+    const float b2xy = b2.x * b2.y, b2xz = b2.x * b2.z, b2yz = b2.y * b2.z;
+    return {k.x * b0x2 + k.y * b1x2 + k.z * b2x2, k.x * b0xy + k.y * b1xy + k.z * b2xy,
+            k.x * b0xz + k.y * b1xz + k.z * b2xz, k.x * b0y2 + k.y * b1y2 + k.z * b2y2,
+            k.x * b0yz + k.y * b1yz + k.z * b2yz, k.x * b0z2 + k.y * b1z2 + k.z * b2z2};
+}
+
+// One manifold's contribution to one body's block (manifold.cpp:650-686 force + :688-755
+// stamp, fused).  Forces are recomputed at the current poses; the reference's cache holds
+// the same floats because nothing it reads moves between fill and use.
+AD void stampManifold(const State &S, int slot, int body, Block &blk)
+{
+    const int a = S.mA[slot], b = S.mB[slot];
+    const m3 B = loadBasis(S, slot);
+    const float mu = S.mFric[slot];
+    const int n = S.mNum[slot];
+    const qt qa = bRot(S, a), qb = bRot(S, b);
+    const v3 dqAL = bPos(S, a) - ld3(S.initLin[a]), dqAA = qsub(qa, ldq(S.initAng[a]));
+    const v3 dqBL = bPos(S, b) - ld3(S.initLin[b]), dqBA = qsub(qb, ldq(S.initAng[b]));
+    const m3 rotA = qmat(qa), rotB = qmat(qb);
+    const bool fromA = body == a;
+    const float side = fromA ? 1.0f : -1.0f;
+    const v3 b0 = row(B, 0), b1 = row(B, 1), b2 = row(B, 2);
+    m3 accLin = zero3(), accAng = zero3(), accCross = zero3();
+    v3 accRL{0, 0, 0}, accRA{0, 0, 0};
+    for (int i = 0; i < n; i++) {
+        const int c = slot * MAXC + i;
+        const v3 rA = mul(rotA, ld3(S.cRA[c]));
+        const v3 rB = mul(rotB, ld3(S.cRB[c]));
+        const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
+        const v3 C = constraintError(S, ld3(S.cC0[c]), br, B);
+        const v3 pen = ld3(S.cPen[c]);
+        v3 f = cmul(pen, C) + ld3(S.cLam[c]);
+        float bounds;
+        clampForce(f, mu, bounds);
+        const v3 r = fromA ? rA : rB;
+        const Hess h = basisHessian(B, pen);
+        const m3 Sm{{v3{h.s00, h.s01, h.s02}, v3{h.s01, h.s11, h.s12}, v3{h.s02, h.s12, h.s22}}};
+        const m3 RS{{cross(r, Sm.c[0]), cross(r, Sm.c[1]), cross(r, Sm.c[2])}};
+        const float rx2 = r.x * r.x, ry2 = r.y * r.y, rz2 = r.z * r.z;
+        const float rxy = r.x * r.y, rxz = r.x * r.z, ryz = r.y * r.z;
+        const float h00 = h.s11 * rz2 - 2.0f * h.s12 * ryz + h.s22 * ry2;
+        const float h11 = h.s22 * rx2 - 2.0f * h.s02 * rxz + h.s00 * rz2;
+        const float h22 = h.s00 * ry2 - 2.0f * h.s01 * rxy + h.s11 * rx2;
+        const float h01 = h.s12 * rxz + h.s02 * ryz - h.s22 * rxy - h.s01 * rz2;
+        const float h02 = h.s12 * rxy + h.s01 * ryz - h.s11 * rxz - h.s02 * ry2;
+        const float h12 = h.s02 * rxy + h.s01 * rxz - h.s00 * ryz - h.s12 * rx2;
+        const v3 fw = (b0 * f.x + b1 * f.y) + b2 * f.z;
+        const v3 fwAng = cross(r, fw);
+        accLin = madd(accLin, Sm);
+        accAng = madd(accAng, m3{{v3{h00, h01, h02}, v3{h01, h11, h12}, v3{h02, h12, h22}}});
+        accCross = madd(accCross, RS);
+        accRL += fw;
+        accRA += fwAng;
+    }
+    blk.lhsLin = madd(blk.lhsLin, accLin);
+    blk.lhsAng = madd(blk.lhsAng, accAng);
+    blk.lhsCross = madd(blk.lhsCross, accCross);
+    blk.rhsLin += accRL * side;
+    blk.rhsAng += accRA * side;
+}
+
+// manifold.cpp:768-856 (arm caching off): lambda <- clamped force, penalty ramps.
+AD void manifoldDual(const State &S, int slot, float decay)
+{
+    const int a = S.mA[slot], b = S.mB[slot];
+    const m3 B = loadBasis(S, slot);
+    const float mu = S.mFric[slot];
+    const int n = S.mNum[slot];
+    const qt qa = bRot(S, a), qb = bRot(S, b);
+    const v3 dqAL = bPos(S, a) - ld3(S.initLin[a]), dqAA = qsub(qa, ldq(S.initAng[a]));
+    const v3 dqBL = bPos(S, b) - ld3(S.initLin[b]), dqBA = qsub(qb, ldq(S.initAng[b]));
+    const m3 rotA = qmat(qa), rotB = qmat(qb);
+    const float pmax = S.P.penaltyMax;
+    for (int i = 0; i < n; i++) {
+        const int c = slot * MAXC + i;
+        v3 pen = ld3(S.cPen[c]);
+        if (decay < 1.0f) pen = lclamp3(pen * decay, PENALTY_MIN, pmax);
+        const float4 ra4 = S.cRA[c];
+        const v3 rA = mul(rotA, ld3(ra4));
+        const v3 rB = mul(rotB, ld3(S.cRB[c]));
+        const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
+        const v3 C = constraintError(S, ld3(S.cC0[c]), br, B);
+        v3 F = cmul(pen, C) + ld3(S.cLam[c]);
+        float bounds;
+        const float scale = clampForce(F, mu, bounds);
+        S.cLam[c] = st3(F);
+        const float beta = S.P.betaLin * ra4.w;
+        if (F.x < 0) pen.x = smin(pen.x + beta * fabsf(C.x), pmax);
+        float4 c0 = S.cC0[c];
+        bool writeStick = false;
+        if (scale <= bounds) {
+            pen.y = smin(pen.y + beta * fabsf(C.y), pmax);
+            pen.z = smin(pen.z + beta * fabsf(C.z), pmax);
+            c0.w = length2d(C.y, C.z) < STICK_THRESH ? 1.0f : 0.0f;
+            writeStick = true;
+        }
+        S.cPen[c] = st3(pen);
+        if (writeStick) S.cC0[c] = c0;
+    }
+}
+
+// manifold.cpp:633-646
+AD void manifoldRebase(const State &S, int slot)
+{
+    const int a = S.mA[slot], b = S.mB[slot];
+    const m3 B = loadBasis(S, slot);
+    const int n = S.mNum[slot];
+    const v3 pa = bPos(S, a), pb = bPos(S, b);
+    const qt qa = bRot(S, a), qb = bRot(S, b);
+    for (int i = 0; i < n; i++) {
+        const int c = slot * MAXC + i;
+        const v3 xA = pa + qrot(qa, ld3(S.cRA[c]));
+        const v3 xB = pb + qrot(qb, ld3(S.cRB[c]));
+        v3 c0 = mul(B, xA - xB) + v3{COLLISION_MARGIN, 0, 0};
+        if (c0.x < -S.P.maxPen) c0.x = -S.P.maxPen;
+        const float4 old = S.cC0[c];
+        S.cC0[c] = st3(c0, old.w);
+    }
+}
+
+// ------------------------------------------------------------- joints ----
+AD v3 jointLin(const State &S, int j)
+{
+    const int a = S.jA[j], b = S.jB[j];
+    const v3 rA = ld3(S.jRA[j]);
+    const v3 pa = a >= 0 ? bPos(S, a) + qrot(bRot(S, a), rA) : rA;
+    return pa - (bPos(S, b) + qrot(bRot(S, b), ld3(S.jRB[j])));
+}
+AD v3 jointAng(const State &S, int j)
+{
+    const int a = S.jA[j];
+    const qt qa = a >= 0 ? bRot(S, a) : qt{0, 0, 0, 1};
+    return qsub(qa, bRot(S, S.jB[j])) * S.jArm[j];
+}
+AD m3 gsBall(int k, v3 v)
+{
+    m3 m = diag(-v[k], -v[k], -v[k]);
+    m.c[k][0] += v[0];
+    m.c[k][1] += v[1];
+    m.c[k][2] += v[2];
+    return m;
+}
+// joint.cpp:80-160
+AD void jointPrimal(const State &S, int j, int body, Block &blk)
+{
+    const bool isA = body == S.jA[j];
+    const v3 penL = ld3(S.jPenL[j]);
+    if (length2(penL) > 0) {
+        const m3 K = diag(penL.x, penL.y, penL.z);
+        v3 C = jointLin(S, j);
+        if (__builtin_isinf(S.jStiffL[j])) C -= ld3(S.jC0L[j]) * S.P.alpha;
+        const v3 F = mul(K, C) + ld3(S.jLamL[j]);
+        const m3 jLin = isA ? ident() : mneg(ident());
+        m3 jAng;
+        if (isA) {
+            const v3 r = -qrot(bRot(S, S.jA[j]), ld3(S.jRA[j]));
+            jAng = m3{{v3{0, r.z, -r.y}, v3{-r.z, 0, r.x}, v3{r.y, -r.x, 0}}};
+        } else {
+            const v3 r = qrot(bRot(S, S.jB[j]), ld3(S.jRB[j]));
+            jAng = m3{{v3{0, r.z, -r.y}, v3{-r.z, 0, r.x}, v3{r.y, -r.x, 0}}};
+        }
+        const m3 jLinT = transpose(jLin), jAngT = transpose(jAng);
+        const m3 jAngTk = mul(jAngT, K);
+        blk.lhsLin = madd(blk.lhsLin, mul(mul(jLinT, K), jLin));
+        blk.lhsAng = madd(blk.lhsAng, mul(jAngTk, jAng));
+        blk.lhsCross = madd(blk.lhsCross, mul(jAngTk, jLin));
+        const v3 r = isA ? qrot(bRot(S, S.jA[j]), ld3(S.jRA[j])) : -qrot(bRot(S, S.jB[j]), ld3(S.jRB[j]));
+        const m3 H = madd(madd(mscale(gsBall(0, r), F.x), mscale(gsBall(1, r), F.y)), mscale(gsBall(2, r), F.z));
+        blk.lhsAng = madd(blk.lhsAng, diag(H.c[0][0], H.c[1][1], H.c[2][2]));
+        blk.rhsLin += mul(jLinT, F);
+        blk.rhsAng += mul(jAngT, F);
+    }
+    const v3 penA = ld3(S.jPenA[j]);
+    if (length2(penA) > 0) {
+        const m3 K = diag(penA.x, penA.y, penA.z);
+        v3 C = jointAng(S, j);
+        if (__builtin_isinf(S.jStiffA[j])) C -= ld3(S.jC0A[j]) * S.P.alpha;
+        const v3 F = mul(K, C) + ld3(S.jLamA[j]);
+        const float arm = S.jArm[j];
+        const m3 jA = mscale(isA ? ident() : m3{{v3{0, 0, 0}, v3{0, -1, 0}, v3{0, 0, -1}}}, arm);
+        blk.lhsAng = madd(blk.lhsAng, mul(mul(transpose(jA), K), jA));
+        blk.rhsAng += mul(transpose(jA), F);
+    }
+}
+
+// joint.cpp:171-230
+AD void jointDual(const State &S, int j)
+{
+    const float pmax = S.P.penaltyMax;
+    v3 penL = ld3(S.jPenL[j]), penA = ld3(S.jPenA[j]);
+    v3 lamL = ld3(S.jLamL[j]), lamA = ld3(S.jLamA[j]);
+    if (length2(penL) > 0) {
+        const m3 K = diag(penL.x, penL.y, penL.z);
+        v3 C = jointLin(S, j);
+        const float st = S.jStiffL[j];
+        if (__builtin_isinf(st)) {
+            C -= ld3(S.jC0L[j]) * S.P.alpha;
+            lamL = mul(K, C) + lamL;
+        }
+        penL = lmin3(penL + fabs3(C) * S.P.betaLin, lmin3(v3{st, st, st}, v3{pmax, pmax, pmax}));
+    }
+    if (length2(penA) > 0) {
+        const m3 K = diag(penA.x, penA.y, penA.z);
+        v3 C = jointAng(S, j);
+        const float st = S.jStiffA[j];
+        if (__builtin_isinf(st)) {
+            C -= ld3(S.jC0A[j]) * S.P.alpha;
+            lamA = mul(K, C) + lamA;
+        }
+        penA = lmin3(penA + fabs3(C) * S.P.betaAng, lmin3(v3{st, st, st}, v3{pmax, pmax, pmax}));
+    }
+    const float fr = S.jFrac[j];
+    if (length2(lamA) > fr * fr) {
+        penL = penA = lamL = lamA = v3{0, 0, 0};
+        S.jBroken[j] = 1;
+    }
+    S.jPenL[j] = st3(penL);
+    S.jPenA[j] = st3(penA);
+    S.jLamL[j] = st3(lamL);
+    S.jLamA[j] = st3(lamA);
+}
+
+// Block of one body: inertia plus every incident constraint, in adjacency order (joints in
+// list order, then manifolds in live order; manifold.cpp:123-166, solver_step.cpp:215-226).
+AD void assemble(const State &S, int body, Block &blk)
+{
+    makeBlock(S, body, blk);
+    const int e0 = S.adjStart[body], e1 = S.adjStart[body + 1];
+    for (int e = e0; e < e1; e++) {
+        const int ref = S.adj[e];
+        if (ref >= 0) stampManifold(S, ref, body, blk);
+        else jointPrimal(S, -ref - 1, body, blk);
+    }
+}
+
+// ------------------------------------------------------- narrow phase ----
+// collide.cpp:71-490, 1034-1097: box-box SAT with face/edge selection, reference-face
+// clipping and feature keys.
+struct OBB { v3 c; qt q; v3 h; v3 ax[3]; };
+
+AD OBB makeOBB(v3 c, qt q, v3 h)
+{
+    OBB o;
+    o.c = c; o.q = q; o.h = h;
+    o.ax[0] = qrot(q, v3{1, 0, 0});
+    o.ax[1] = qrot(q, v3{0, 1, 0});
+    o.ax[2] = qrot(q, v3{0, 0, 1});
+    return o;
+}
+AD v3 supportPoint(const OBB &b, v3 dir)
+{
+    const float sx = dot(dir, b.ax[0]) >= 0.0f ? 1.0f : -1.0f;
+    const float sy = dot(dir, b.ax[1]) >= 0.0f ? 1.0f : -1.0f;
+    const float sz = dot(dir, b.ax[2]) >= 0.0f ? 1.0f : -1.0f;
+    return ((b.c + b.ax[0] * (b.h.x * sx)) + b.ax[1] * (b.h.y * sy)) + b.ax[2] * (b.h.z * sz);
+}
+AD void faceAxes(const OBB &b, int i, v3 &u, v3 &v, float &eu, float &ev)
+{
+    if (i == 0) { u = b.ax[1]; v = b.ax[2]; eu = b.h.y; ev = b.h.z; }
+    else if (i == 1) { u = b.ax[0]; v = b.ax[2]; eu = b.h.x; ev = b.h.z; }
+    else { u = b.ax[0]; v = b.ax[1]; eu = b.h.x; ev = b.h.y; }
+}
+constexpr int MAX_POLY = 16;
+AD int clipPoly(const v3 *in, int n, v3 pn, float off, v3 *out)
+{
+    if (n <= 0) return 0;
+    int m = 0;
+    v3 a = in[n - 1];
+    float da = dot(pn, a) - off;
+    for (int i = 0; i < n; i++) {
+        const v3 b = in[i];
+        const float db = dot(pn, b) - off;
+        const bool ai = da <= 1.0e-5f, bi = db <= 1.0e-5f;
+        if (ai != bi) {
+            float t = 0.0f;
+            const float den = da - db;
+            if (fabsf(den) > 1.0e-6f) t = sclamp(da / den, 0.0f, 1.0f);
+            if (m < MAX_POLY) out[m++] = a + (b - a) * t;
+        }
+        if (bi && m < MAX_POLY) out[m++] = b;
+        a = b;
+        da = db;
+    }
+    return m;
+}
+
+struct Contacts {
+    int n;
+    int key[MAXC];
+    v3 rA[MAXC], rB[MAXC];
+    v3 mid[MAXC];
+};
+
+AD void addContact(qt qA, v3 cA, qt qB, v3 cB, Contacts &cs, v3 xA, v3 xB, int key)
+{
+    const v3 mid = (xA + xB) * 0.5f;
+    for (int i = 0; i < cs.n; i++)
+        if (length2(mid - cs.mid[i]) < 1.0e-6f) return;
+    if (cs.n >= MAXC) return;
+    cs.key[cs.n] = key;
+    cs.rA[cs.n] = qrot(qconj(qA), xA - cA);
+    cs.rB[cs.n] = qrot(qconj(qB), xB - cB);
+    cs.mid[cs.n] = mid;
+    cs.n++;
+}
+
+AD void faceManifold(const OBB &bA, const OBB &bB, bool refIsA, int refAxis, v3 nAB, float margin, Contacts &cs)
+{
+    const OBB &ref = refIsA ? bA : bB;
+    const OBB &inc = refIsA ? bB : bA;
+    const v3 refOut = refIsA ? nAB : -nAB;
+    const float sgn = dot(refOut, ref.ax[refAxis]) >= 0.0f ? 1.0f : -1.0f;
+    const v3 fn = ref.ax[refAxis] * sgn;
+    const v3 fc = ref.c + fn * ref.h[refAxis];
+    v3 fu, fv;
+    float feu, fev;
+    faceAxes(ref, refAxis, fu, fv, feu, fev);
+    int incAxis = 0;
+    {
+        float best = -FLT_MAXV;
+        for (int i = 0; i < 3; i++) {
+            const float d = fabsf(dot(inc.ax[i], fn));
+            if (d > best) { best = d; incAxis = i; }
+        }
+    }
+    v3 c0[MAX_POLY], c1[MAX_POLY];
+    {
+        const float s2 = dot(inc.ax[incAxis], fn) > 0.0f ? -1.0f : 1.0f;
+        const v3 in = inc.ax[incAxis] * s2;
+        const v3 ic = inc.c + in * inc.h[incAxis];
+        v3 u, v;
+        float eu, ev;
+        faceAxes(inc, incAxis, u, v, eu, ev);
+        c0[0] = (ic + u * eu) + v * ev;
+        c0[1] = (ic - u * eu) + v * ev;
+        c0[2] = (ic - u * eu) - v * ev;
+        c0[3] = (ic + u * eu) - v * ev;
+    }
+    int n = 4;
+    v3 p;
+    p = fu;  n = clipPoly(c0, n, p, dot(p, fc) + feu, c1); if (!n) return;
+    p = -fu; n = clipPoly(c1, n, p, dot(p, fc) + feu, c0); if (!n) return;
+    p = fv;  n = clipPoly(c0, n, p, dot(p, fc) + fev, c1); if (!n) return;
+    p = -fv; n = clipPoly(c1, n, p, dot(p, fc) + fev, c0); if (!n) return;
+    int prefix = (refIsA ? 0 : 1) << 24;
+    prefix |= (refAxis & 0xFF) << 16;
+    prefix |= (incAxis & 0xFF) << 8;
+    const float accept = smax(margin, 1.0e-5f);
+    for (int i = 0; i < n && cs.n < MAXC; i++) {
+        const v3 pi = c0[i];
+        const float dist = dot(pi - fc, fn);
+        if (dist > accept) continue;
+        const v3 pr = pi - fn * dist;
+        addContact(bA.q, bA.c, bB.q, bB.c, cs, refIsA ? pr : pi, refIsA ? pi : pr, prefix | (i & 0xFF));
+    }
+    if (!cs.n) {
+        const v3 xA = supportPoint(bA, nAB);
+        v3 xB = supportPoint(bB, -nAB);
+        xB = xA - nAB * dot(xA - xB, nAB);
+        addContact(bA.q, bA.c, bB.q, bB.c, cs, xA, xB, prefix);
+    }
+}
+
+AD void supportEdge(const OBB &b, int ai, v3 dir, v3 &ea, v3 &eb)
+{
+    const int a1 = (ai + 1) % 3, a2 = (ai + 2) % 3;
+    const float s1 = dot(dir, b.ax[a1]) >= 0.0f ? 1.0f : -1.0f;
+    const float s2 = dot(dir, b.ax[a2]) >= 0.0f ? 1.0f : -1.0f;
+    const v3 ec = (b.c + b.ax[a1] * (b.h[a1] * s1)) + b.ax[a2] * (b.h[a2] * s2);
+    ea = ec - b.ax[ai] * b.h[ai];
+    eb = ec + b.ax[ai] * b.h[ai];
+}
+
+AD void closestSeg(v3 p0, v3 p1, v3 q0, v3 q1, v3 &c0, v3 &c1)
+{
+    const v3 d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
+    const float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+    float s = 0.0f, t = 0.0f;
+    if (a <= 1.0e-6f && e <= 1.0e-6f) { c0 = p0; c1 = q0; return; }
+    if (a <= 1.0e-6f) {
+        t = sclamp(f / e, 0.0f, 1.0f);
+    } else {
+        const float c = dot(d1, r);
+        if (e <= 1.0e-6f) {
+            s = sclamp(-c / a, 0.0f, 1.0f);
+        } else {
+            const float bb = dot(d1, d2);
+            const float den = a * e - bb * bb;
+            if (fabsf(den) > 1.0e-6f) s = sclamp((bb * f - c * e) / den, 0.0f, 1.0f);
+            t = (bb * s + f) / e;
+            if (t < 0.0f) { t = 0.0f; s = sclamp(-c / a, 0.0f, 1.0f); }
+            else if (t > 1.0f) { t = 1.0f; s = sclamp((bb - c) / a, 0.0f, 1.0f); }
+        }
+    }
+    c0 = p0 + d1 * s;
+    c1 = q0 + d2 * t;
+}
+
+// Returns the contact count (0 when separated beyond the margin); `basisOut` is written
+// exactly when the reference writes it.
+AD int collideBoxes(v3 pA, qt qA, v3 hA, v3 pB, qt qB, v3 hB, float margin, m3 &basisOut, bool &wroteBasis,
+                    Contacts &cs)
+{
+    cs.n = 0;
+    wroteBasis = false;
+    const OBB bA = makeOBB(pA, qA, hA);
+    const OBB bB = makeOBB(pB, qB, hB);
+    const v3 delta = bB.c - bA.c;
+    float M[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) M[i][j] = dot(bA.ax[i], bB.ax[j]);
+    const v3 dA{dot(delta, bA.ax[0]), dot(delta, bA.ax[1]), dot(delta, bA.ax[2])};
+    const v3 dB{dot(delta, bB.ax[0]), dot(delta, bB.ax[1]), dot(delta, bB.ax[2])};
+    // best face axis: type 0 = face of A, 1 = face of B; best edge separately.
+    bool fValid = false;
+    int fType = 0, fIdx = 0;
+    float fSep = -FLT_MAXV;
+    v3 fN{0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float sep = fabsf(dA[i]) -
+                (((bA.h[i] + bB.h.x * fabsf(M[i][0])) + bB.h.y * fabsf(M[i][1])) + bB.h.z * fabsf(M[i][2]));
+        if (sep > margin) return 0;
+        if (!fValid || sep > fSep) {
+            fValid = true; fType = 0; fIdx = i; fSep = sep;
+            fN = bA.ax[i] * (dA[i] < 0.0f ? -1.0f : 1.0f);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const float sep = fabsf(dB[j]) -
+                (((bB.h[j] + bA.h.x * fabsf(M[0][j])) + bA.h.y * fabsf(M[1][j])) + bA.h.z * fabsf(M[2][j]));
+        if (sep > margin) return 0;
+        if (!fValid || sep > fSep) {
+            fValid = true; fType = 1; fIdx = j; fSep = sep;
+            fN = bB.ax[j] * (dB[j] < 0.0f ? -1.0f : 1.0f);
+        }
+    }
+    bool eValid = false;
+    int eI = 0, eJ = 0;
+    float eSep = -FLT_MAXV;
+    v3 eN{0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const v3 axis = cross(bA.ax[i], bB.ax[j]);
+            const float lenSq = length2(axis);
+            if (lenSq < 1.0e-6f) continue;
+            const float invLen = 1.0f / sqrtf(lenSq);
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            const float ra = (bA.h[i1] * fabsf(M[i2][j]) + bA.h[i2] * fabsf(M[i1][j])) * invLen;
+            const float rb = (bB.h[j1] * fabsf(M[i][j2]) + bB.h[j2] * fabsf(M[i][j1])) * invLen;
+            const float sd = dot(delta, axis) * invLen;
+            const float sep = fabsf(sd) - (ra + rb);
+            if (sep > margin) return 0;
+            if (!eValid || sep > eSep) {
+                eValid = true; eI = i; eJ = j; eSep = sep;
+                eN = axis * (sd < 0.0f ? -invLen : invLen);
+            }
+        }
+    if (!fValid) return 0;
+    const bool useEdge = eValid && 0.95f * eSep > fSep + 0.01f;
+    const v3 nAB = useEdge ? eN : fN;
+    basisOut = orthonormal(-nAB);
+    wroteBasis = true;
+    if (useEdge) {
+        v3 a0, a1, b0, b1;
+        supportEdge(bA, eI, nAB, a0, a1);
+        supportEdge(bB, eJ, -nAB, b0, b1);
+        v3 xA, xB;
+        closestSeg(a0, a1, b0, b1, xA, xB);
+        const int key = (2 << 24) | ((eI & 0xFF) << 8) | (eJ & 0xFF);
+        addContact(qA, pA, qB, pB, cs, xA, xB, key);
+        if (!cs.n) {
+            xA = supportPoint(bA, nAB);
+            xB = supportPoint(bB, -nAB);
+            addContact(qA, pA, qB, pB, cs, xA, xB, key);
+        }
+        return cs.n;
+    }
+    faceManifold(bA, bB, fType == 0, fIdx, nAB, margin, cs);
+    return cs.n;
+}
+
+// ------------------------------------------------------ projection bits ----
+// solver_projection.cpp:33-53
+AD float angCompliance(v3 I, v3 t)
+{
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        if (I[k] > 0.0f) sum += t[k] * t[k] / I[k];
+    return sum;
+}
+AD v3 angResponse(v3 I, qt q, v3 t)
+{
+    v3 o{0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        if (I[k] > 0.0f) o[k] = t[k] / I[k];
+    return qrot(q, o);
+}
+AD void maskLocked(uint8_t locks, v3 &v)
+{
+    if (locks) {
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            if (locks & (1 << k)) v[k] = 0.0f;
+    }
+}
+
+} // namespace avbd_dev

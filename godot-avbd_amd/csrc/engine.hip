@@ -1,0 +1,1155 @@
+// Host side of the MI355X AVBD step: device memory, world layout, the per-step launch
+// sequence, HIP-graph capture for batches, and the C ABI declared in include/avbd_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/avbd_hip.h"
+#include "kernels.hpp"
+
+using namespace avbd_dev;
+
+namespace {
+
+thread_local std::string g_lastError;
+
+avbd_status fail(avbd_status code, const std::string &msg)
+{
+    g_lastError = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(AVBD_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+// A first-fit allocator of [0, capacity) ranges (world bodies, joints, manifold slots).
+struct RangeAlloc {
+    int capacity = 0;
+    std::map<int, int> freeRanges;  // start -> length
+    void init(int cap)
+    {
+        capacity = cap;
+        freeRanges.clear();
+        if (cap > 0) freeRanges[0] = cap;
+    }
+    int take(int n)
+    {
+        if (n == 0) return 0;
+        for (auto it = freeRanges.begin(); it != freeRanges.end(); ++it) {
+            if (it->second >= n) {
+                const int s = it->first, len = it->second;
+                freeRanges.erase(it);
+                if (len > n) freeRanges[s + n] = len - n;
+                return s;
+            }
+        }
+        return -1;
+    }
+    void give(int s, int n)
+    {
+        if (n <= 0) return;
+        auto it = freeRanges.emplace(s, n).first;
+        auto nx = std::next(it);
+        if (nx != freeRanges.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            freeRanges.erase(nx);
+        }
+        if (it != freeRanges.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                freeRanges.erase(it);
+            }
+        }
+    }
+};
+
+template <typename T> hipError_t dalloc(T *&p, size_t n)
+{
+    return hipMalloc(reinterpret_cast<void **>(&p), std::max<size_t>(n, 1) * sizeof(T));
+}
+
+void setParams(Params &P, const avbd_params &p, float collisionDt)
+{
+    P.dt = p.dt;
+    P.dtSq = p.dt * p.dt;
+    P.collisionDt = collisionDt;
+    P.alpha = p.alpha;
+    P.oneMinusAlpha = 1 - p.alpha;
+    P.betaLin = p.beta_lin;
+    P.betaAng = p.beta_ang;
+    P.gamma = p.gamma;
+    P.jacobiRelax = p.jacobi_relaxation;
+    P.stickDriftHeal = p.stick_drift_heal;
+    P.maxPen = p.max_penetration;
+    P.specMargin = p.speculative_margin;
+    P.projBeta = p.projection_beta;
+    P.projSlop = p.projection_slop;
+    P.projMaxCorr = p.projection_max_correction;
+    P.penaltyMax = p.penalty_max;
+    P.adaptiveFloor = p.adaptive_beta_error_floor;
+    P.stiffnessDecay = p.stiffness_decay;
+    P.adaptiveBeta = p.adaptive_beta;
+    P.contactOnlyOverflow = p.contact_only_overflow;
+    P.projIters = p.projection_iterations;
+}
+
+avbd_status checkParams(const avbd_params *p)
+{
+    if (!p) return fail(AVBD_ERR_INVALID, "params is null");
+    if (!(p->dt > 0.0f)) return fail(AVBD_ERR_INVALID, "dt must be positive");
+    if (p->iterations < 1) return fail(AVBD_ERR_INVALID, "iterations must be >= 1");
+    if (p->sleep_frames > 0) return fail(AVBD_ERR_UNSUPPORTED, "sleeping (sleep_frames > 0) is outside this path");
+    if (p->contact_grace_ticks > 0) return fail(AVBD_ERR_UNSUPPORTED, "contact grace ticks are outside this path");
+    if (p->jacobian_rebuild_distance > 0.0f)
+        return fail(AVBD_ERR_UNSUPPORTED, "jacobian arm caching is outside this path");
+    if (p->convergence_threshold > 0.0f)
+        return fail(AVBD_ERR_UNSUPPORTED, "the convergence early exit is outside this path");
+    return AVBD_OK;
+}
+
+struct WorldInfo {
+    bool active = false;
+    int bodyBase = 0, bodyCount = 0, jointBase = 0, jointCount = 0, slotBase = 0, slotCap = 0;
+    bool hasJoints = false;
+};
+
+} // namespace
+
+struct Engine {
+    avbd_params prm{};
+    int Wcap = 0, Ncap = 0, Jcap = 0, Scap = 0, Pcap = 0;
+    State S{};
+    hipStream_t stream = nullptr;
+    std::vector<WorldInfo> worlds;
+    RangeAlloc bodyAlloc, jointAlloc, slotAlloc;
+    // scratch
+    uint32_t *tileSums = nullptr;
+    int tileSumsCap = 0;
+    uint32_t *radixHist = nullptr;
+    int radixBlocks = 0;
+    int *slotToLive = nullptr;
+    int *errHost = nullptr;  // pinned
+    int *countsHost = nullptr;
+    bool csrValid = false;
+    int treeAge = 1 << 20;
+    int refitBudget = 16;
+    bool batchMode = false;
+    int colourRounds = 48;
+    hipGraphExec_t graphs[2] = {nullptr, nullptr};  // [rebuild, refit]
+    bool graphJoints = false;
+    bool graphsValid = false;
+    hipEvent_t ev[5] = {};
+    avbd_step_stats stats{};
+    std::vector<hipEvent_t> owned;
+
+    ~Engine() { release(); }
+
+    void release()
+    {
+        for (auto &g : graphs)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+        freeAll();
+        if (errHost) (void)hipHostFree(errHost);
+        if (countsHost) (void)hipHostFree(countsHost);
+        errHost = countsHost = nullptr;
+        for (auto &e : ev)
+            if (e) { (void)hipEventDestroy(e); e = nullptr; }
+        if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
+    }
+
+    std::vector<void *> allocs;
+    template <typename T> avbd_status A(T *&p, size_t n)
+    {
+        hipError_t e = dalloc(p, n);
+        if (e != hipSuccess) return fail(AVBD_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(p);
+        e = hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) return fail(AVBD_ERR_DEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
+        return AVBD_OK;
+    }
+    void fill(int *p, int v, int n)
+    {
+        if (n > 0) k_fill_i32<<<nblk(n), TPB, 0, stream>>>(p, v, n);
+    }
+    void freeAll()
+    {
+        for (void *p : allocs) (void)hipFree(p);
+        allocs.clear();
+    }
+
+    avbd_status create(const avbd_params &p, int worldsCap, int bodies, int joints, int slots, bool batch)
+    {
+        prm = p;
+        batchMode = batch;
+        Wcap = std::max(worldsCap, 1);
+        Ncap = std::max(bodies, 2);
+        Jcap = std::max(joints, 1);
+        Scap = std::max(slots, 8);
+        Pcap = 12 * Ncap + 4096;
+        worlds.assign(Wcap, WorldInfo{});
+        bodyAlloc.init(Ncap);
+        jointAlloc.init(Jcap);
+        slotAlloc.init(Scap);
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            return fail(AVBD_ERR_NO_DEVICE, "no HIP device visible");
+        HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&errHost), sizeof(int) * Wcap, hipHostMallocDefault));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&countsHost), sizeof(int) * 64, hipHostMallocDefault));
+        avbd_status st;
+#define AL(ptr, n) if ((st = A(ptr, n)) != AVBD_OK) return st
+        const size_t N = Ncap, J = Jcap, M = Scap, C = static_cast<size_t>(Scap) * MAXC, W = Wcap, P = Pcap;
+        S.numWorlds = Wcap; S.numBodies = Ncap; S.slotCap = Scap; S.jointCap = Jcap; S.pairCap = Pcap;
+        AL(S.pos, N); AL(S.rot, N); AL(S.initLin, N); AL(S.initAng, N); AL(S.inertLin, N); AL(S.inertAng, N);
+        AL(S.vel, N); AL(S.angVel, N); AL(S.prevVel, N); AL(S.sizeR, N); AL(S.massI, N);
+        AL(S.fric, N); AL(S.grav, N); AL(S.layer, N); AL(S.mask, N); AL(S.lockLin, N); AL(S.lockAng, N);
+        AL(S.bodyWorld, N); AL(S.updateIndex, N); AL(S.colour, N); AL(S.contactOnly, N);
+        AL(S.dLin, N); AL(S.dAng, N); AL(S.aabbC, N); AL(S.aabbH, N);
+        AL(S.adjCount, N + 1); AL(S.adjStart, N + 1); AL(S.adj, 2 * (M + J)); AL(S.adjTmp, 2 * (M + J));
+        AL(S.jA, J); AL(S.jB, J); AL(S.jWorld, J); AL(S.jRA, J); AL(S.jRB, J); AL(S.jC0L, J); AL(S.jC0A, J);
+        AL(S.jPenL, J); AL(S.jPenA, J); AL(S.jLamL, J); AL(S.jLamA, J); AL(S.jStiffL, J); AL(S.jStiffA, J);
+        AL(S.jFrac, J); AL(S.jArm, J); AL(S.jBroken, J); AL(S.jDead, J);
+        AL(S.mA, M); AL(S.mB, M); AL(S.mBasis, 3 * M); AL(S.mFric, M); AL(S.mNum, M);
+        AL(S.cRA, C); AL(S.cRB, C); AL(S.cC0, C); AL(S.cPen, C); AL(S.cLam, C);
+        AL(S.slotWorld, M); AL(S.live, M); AL(S.liveTmp, M); AL(S.freeList, M);
+        AL(S.keep, M + 1); AL(S.keepScan, M + 1); AL(S.relScan, M + 1);
+        AL(S.succA, M); AL(S.succB, M); AL(S.indeg, M); AL(S.levelOrder, M);
+        AL(S.brakeLevels, W); AL(S.levelStart, M); AL(S.wLevelEnd, W);
+        AL(S.projLA, M); AL(S.projAA, M); AL(S.projLB, M); AL(S.projAB, M); AL(S.projOn, M);
+        AL(S.wBodyBase, W); AL(S.wBodyCount, W); AL(S.wJointBase, W); AL(S.wJointCount, W);
+        AL(S.wSlotBase, W); AL(S.wSlotCap, W); AL(S.wLive, W); AL(S.wFree, W); AL(S.wTop, W); AL(S.wNew, W);
+        AL(S.wHasJoints, W); AL(S.wUpdateCount, W); AL(S.wErr, W); AL(S.wProjDone, W); AL(S.wMaxDepth, W);
+        AL(S.wColourLeft, W);
+        AL(S.mortonCode, N); AL(S.sortKey, N); AL(S.sortKeyAlt, N); AL(S.sortVal, N); AL(S.sortValAlt, N);
+        AL(S.nodeLeft, N); AL(S.nodeRight, N); AL(S.nodeParent, 2 * N); AL(S.nodeFirst, N); AL(S.nodeLast, N);
+        AL(S.nodeFlag, N); AL(S.nodeMin, 2 * N); AL(S.nodeMax, 2 * N); AL(S.leafBody, N); AL(S.bodyLeaf, N);
+        AL(S.pairCount, N + 1); AL(S.pairOff, N + 1); AL(S.pairI, P); AL(S.pairJ, P);
+        AL(S.pairValid, P + 1); AL(S.pairValidScan, P + 1);
+        AL(S.wBoundsLo, W); AL(S.wBoundsHi, W);
+        AL(slotToLive, M);
+        radixBlocks = (Ncap + RADIX_TILE - 1) / RADIX_TILE;
+        AL(radixHist, static_cast<size_t>(256) * radixBlocks + 1);
+        const int biggest = std::max({Ncap + 1, Pcap + 1, Scap + 1, 256 * radixBlocks + 1, 2 * (Scap + Jcap)});
+        tileSumsCap = (biggest + SCAN_TILE - 1) / SCAN_TILE + 1;
+        AL(tileSums, tileSumsCap);
+#undef AL
+        // all bodies / slots / joints start outside any world
+        k_fill_i32<<<nblk(Ncap), TPB, 0, stream>>>(S.bodyWorld, -1, Ncap);
+        k_fill_i32<<<nblk(Ncap), TPB, 0, stream>>>(S.updateIndex, -1, Ncap);
+        k_fill_i32<<<nblk(Scap), TPB, 0, stream>>>(S.slotWorld, -1, Scap);
+        k_fill_i32<<<nblk(Jcap), TPB, 0, stream>>>(S.jWorld, -1, Jcap);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
+        setParams(S.P, prm, prm.dt);
+        return AVBD_OK;
+    }
+
+    // ---- primitives -------------------------------------------------------------
+    void scan(const uint32_t *in, uint32_t *out, int n)
+    {
+        const int nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+        k_scan_tiles<<<std::max(nt, 1), TPB, 0, stream>>>(in, out, tileSums, n);
+        k_scan_sums<<<1, TPB, 0, stream>>>(tileSums, std::max(nt, 1), out, n);
+        k_scan_add<<<std::max(nt, 1), TPB, 0, stream>>>(out, tileSums, n);
+    }
+
+    void radixSort()
+    {
+        const int n = Ncap, nb = radixBlocks;
+        unsigned long long *kin = S.sortKey, *kout = S.sortKeyAlt;
+        int *vin = S.sortVal, *vout = S.sortValAlt;
+        for (int shift = 0; shift < 64; shift += 8) {
+            k_radix_hist<<<nb, TPB, 0, stream>>>(kin, n, shift, radixHist, nb);
+            scan(radixHist, radixHist, 256 * nb);
+            k_radix_scatter<<<nb, TPB, 0, stream>>>(kin, vin, kout, vout, n, shift, radixHist, nb);
+            std::swap(kin, kout);
+            std::swap(vin, vout);
+        }
+        // eight passes: the result is back in sortKey / sortVal
+    }
+
+    void buildAdjacency()
+    {
+        const int n = Ncap;
+        const int t = std::max(Scap, Jcap);
+        k_fill_i32<<<nblk(n + 1), TPB, 0, stream>>>(S.adjCount, 0, n + 1);
+        k_adj_count<<<nblk(t), TPB, 0, stream>>>(S);
+        scan(reinterpret_cast<uint32_t *>(S.adjCount), reinterpret_cast<uint32_t *>(S.adjStart), n);
+        k_fill_i32<<<nblk(n + 1), TPB, 0, stream>>>(S.adjCount, 0, n + 1);
+        k_adj_fill<<<nblk(t), TPB, 0, stream>>>(S);
+        k_adj_sort<<<nblk(n), TPB, 0, stream>>>(S);
+    }
+
+    bool anyJoints() const
+    {
+        if (!prm.contact_only_overflow) return true;
+        for (const auto &w : worlds)
+            if (w.active && w.hasJoints) return true;
+        return false;
+    }
+
+    // ---- the step -------------------------------------------------------------------
+    // Enqueue one reference step (substeps <= 1) or interleaved tick on `stream`.
+    void enqueueStep(int substeps, int roundsPer, bool rebuildTree, bool withColours, bool timed)
+    {
+        const bool inter = substeps > 1;
+        const float collisionDt = inter ? prm.dt * static_cast<float>(substeps) : prm.dt;
+        setParams(S.P, prm, collisionDt);
+        const int nb = nblk(Ncap);
+        if (timed) (void)hipEventRecord(ev[0], stream);
+        if (!csrValid) buildAdjacency();
+        // ---- broad phase (solver_broadphase.cpp:85-186)
+        k_swept_aabb<<<nb, TPB, 0, stream>>>(S);
+        if (rebuildTree) {
+            k_world_bounds<<<Wcap, TPB, 0, stream>>>(S);
+            k_morton<<<nb, TPB, 0, stream>>>(S, batchMode ? 0 : 1);
+            radixSort();
+            k_karras<<<nblk(Ncap - 1), TPB, 0, stream>>>(S, Ncap);
+        } else {
+            k_fill_i32<<<nblk(Ncap), TPB, 0, stream>>>(S.nodeFlag, 0, Ncap);
+        }
+        k_refit<<<nb, TPB, 0, stream>>>(S, Ncap);
+        k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap, 0);
+        scan(S.pairCount, S.pairOff, Ncap);
+        k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap, 1);
+        k_pair_filter<<<nblk(Pcap), TPB, 0, stream>>>(S);
+        scan(S.pairValid, S.pairValidScan, Pcap);
+        k_alloc_new<<<nblk(Pcap), TPB, 0, stream>>>(S);
+        k_alloc_counts<<<nblk(Wcap), TPB, 0, stream>>>(S);
+        if (timed) (void)hipEventRecord(ev[1], stream);
+        // ---- warm start forces / narrow phase (solver_solve.cpp:31-105)
+        k_joint_init<<<nblk(Jcap), TPB, 0, stream>>>(S);
+        k_manifold_init<<<nblk(Scap), TPB, 0, stream>>>(S);
+        k_release_flags<<<nblk(Scap), TPB, 0, stream>>>(S);
+        scan(S.keep, S.keepScan, Scap);
+        scan(S.relScan, S.relScan, Scap);
+        k_compact<<<nblk(Scap), TPB, 0, stream>>>(S);
+        k_compact_counts<<<nblk(Wcap), TPB, 0, stream>>>(S);
+        k_copy_live<<<nblk(Scap), TPB, 0, stream>>>(S);
+        buildAdjacency();
+        // ---- update set and colouring (solver_solve.cpp:263-424)
+        k_colour_prepare<<<nblk(std::max(Ncap, Wcap)), TPB, 0, stream>>>(S);
+        k_colour_joints<<<nblk(Jcap), TPB, 0, stream>>>(S);
+        k_colour_init<<<nb, TPB, 0, stream>>>(S);
+        if (withColours) {
+            for (int r = 0; r + 1 < colourRounds; r++) k_colour_round<<<nb, TPB, 0, stream>>>(S);
+            k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wColourLeft, 0, Wcap);
+            k_colour_round<<<nb, TPB, 0, stream>>>(S);
+            k_colour_finish<<<nblk(Wcap, 64), 64, 0, stream>>>(S);
+        }
+        k_colour_override<<<nb, TPB, 0, stream>>>(S);
+        // ---- brake schedule and first brake (solver_projection.cpp:96-167)
+        k_slot_to_live<<<nblk(Scap), TPB, 0, stream>>>(S, slotToLive);
+        k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wUpdateCount, 0, Wcap);
+        k_brake_links<<<nblk(Scap), TPB, 0, stream>>>(S, slotToLive);
+        const bool brake = prm.speculative_margin > 0.0f;
+        k_brake_levels<<<Wcap, BRAKE_TPB, 0, stream>>>(S, collisionDt, brake ? 1 : 0);
+        if (timed) (void)hipEventRecord(ev[2], stream);
+        // ---- substeps (solver_step.cpp:79-105, 142-150)
+        const int rounds = !inter ? std::max(1, prm.iterations)
+                                  : (roundsPer > 0 ? std::max(1, roundsPer)
+                                                   : std::max(2, (prm.iterations + substeps - 1) / substeps));
+        const int newtonRounds =
+                std::clamp(static_cast<int>(std::ceil(prm.newton_ratio * static_cast<float>(rounds))), 1, rounds);
+        const int subs = inter ? substeps : 1;
+        for (int sub = 0; sub < subs; sub++) {
+            if (sub > 0) {
+                k_rebase<<<nblk(Scap), TPB, 0, stream>>>(S);
+                if (brake) k_brake_replay<<<Wcap, BRAKE_TPB, 0, stream>>>(S, prm.dt * static_cast<float>(subs - sub));
+            }
+            k_warmstart<<<nb, TPB, 0, stream>>>(S);
+            for (int it = 0; it < rounds; it++) {
+                const bool newton = it < newtonRounds;
+                if (newton) {
+                    if (withColours)
+                        for (int c = 0; c < 8; c++) k_colour_pass<<<nb, TPB, 0, stream>>>(S, c);
+                    k_overflow_solve<<<nb, TPB, 0, stream>>>(S);
+                    k_overflow_apply<<<nb, TPB, 0, stream>>>(S);
+                }
+                k_dual<<<nblk(std::max(Scap, Jcap)), TPB, 0, stream>>>(S, newton ? 1.0f : prm.stiffness_decay);
+            }
+            k_finish<<<nb, TPB, 0, stream>>>(S);
+        }
+        if (timed) (void)hipEventRecord(ev[3], stream);
+        // ---- final brake, position projection, rebase (solver_step.cpp:69-70)
+        if (brake) k_brake_replay<<<Wcap, BRAKE_TPB, 0, stream>>>(S, collisionDt);
+        if (prm.position_projection && prm.projection_iterations > 0) {
+            k_proj_reset<<<nblk(Wcap), TPB, 0, stream>>>(S);
+            for (int pass = 0; pass < prm.projection_iterations; pass++) {
+                k_proj_manifold<<<nblk(Scap), TPB, 0, stream>>>(S);
+                k_proj_body<<<nb, TPB, 0, stream>>>(S, slotToLive);
+                k_proj_flags<<<nblk(Wcap), TPB, 0, stream>>>(S);
+            }
+        }
+        k_rebase<<<nblk(Scap), TPB, 0, stream>>>(S);
+        if (timed) (void)hipEventRecord(ev[4], stream);
+        csrValid = true;
+    }
+
+    avbd_status checkErrors()
+    {
+        HIPCHK(hipMemcpyAsync(errHost, S.wErr, sizeof(int) * Wcap, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int w = 0; w < Wcap; w++) {
+            if (errHost[w]) {
+                const int e = errHost[w];
+                (void)hipMemsetAsync(S.wErr, 0, sizeof(int) * Wcap, stream);
+                (void)hipStreamSynchronize(stream);
+                if (e & 1) return fail(AVBD_ERR_CAPACITY, "manifold slot capacity exceeded in world " + std::to_string(w));
+                if (e & 4) return fail(AVBD_ERR_CAPACITY, "broad-phase traversal stack exceeded in world " + std::to_string(w));
+                return fail(AVBD_ERR_CAPACITY, "capacity exceeded in world " + std::to_string(w));
+            }
+        }
+        // pair capacity
+        uint32_t total = 0;
+        HIPCHK(hipMemcpy(&total, S.pairOff + Ncap, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (total > static_cast<uint32_t>(Pcap)) return fail(AVBD_ERR_CAPACITY, "candidate pair capacity exceeded");
+        return AVBD_OK;
+    }
+
+    avbd_status stepPlain(int substeps, int rounds)
+    {
+        const bool rebuild = treeAge >= refitBudget;
+        treeAge = rebuild ? 0 : treeAge + 1;
+        enqueueStep(substeps, rounds, rebuild, anyJoints(), true);
+        HIPCHK(hipGetLastError());
+        avbd_status st = checkErrors();
+        if (st != AVBD_OK) return st;
+        float t01 = 0, t12 = 0, t23 = 0, t34 = 0;
+        (void)hipEventElapsedTime(&t01, ev[0], ev[1]);
+        (void)hipEventElapsedTime(&t12, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&t23, ev[2], ev[3]);
+        (void)hipEventElapsedTime(&t34, ev[3], ev[4]);
+        stats.ms_broad = t01;
+        stats.ms_narrow = t12;
+        stats.ms_solve = t23;
+        stats.ms_finish = t34;
+        stats.ms_total = t01 + t12 + t23 + t34;
+        return AVBD_OK;
+    }
+
+    avbd_status captureGraphs()
+    {
+        for (auto &g : graphs)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+        const bool joints = anyJoints();
+        for (int k = 0; k < 2; k++) {
+            hipGraph_t graph;
+            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            const bool savedCsr = csrValid;
+            csrValid = true;
+            enqueueStep(1, 0, k == 0, joints, false);
+            csrValid = savedCsr;
+            HIPCHK(hipStreamEndCapture(stream, &graph));
+            HIPCHK(hipGraphInstantiate(&graphs[k], graph, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(graph);
+        }
+        graphJoints = joints;
+        graphsValid = true;
+        return AVBD_OK;
+    }
+
+    avbd_status stepBatch()
+    {
+        if (!csrValid) {
+            buildAdjacency();
+            csrValid = true;
+        }
+        if (!graphsValid || graphJoints != anyJoints()) {
+            avbd_status st = captureGraphs();
+            if (st != AVBD_OK) return st;
+        }
+        const bool rebuild = treeAge >= refitBudget;
+        treeAge = rebuild ? 0 : treeAge + 1;
+        HIPCHK(hipGraphLaunch(graphs[rebuild ? 0 : 1], stream));
+        return checkErrors();
+    }
+
+    // ---- upload / download --------------------------------------------------------
+    avbd_status uploadWorld(int w, const avbd_bodies *b, const avbd_joints *j, int slotCap)
+    {
+        const int n = b->count;
+        const int nj = j ? j->count : 0;
+        WorldInfo &wi = worlds[w];
+        wi.bodyCount = n;
+        wi.jointCount = nj;
+        wi.slotCap = slotCap;
+        wi.hasJoints = nj > 0;
+        std::vector<float4> pos(n), rot(n), vel(n), ang(n), prev(n), sizeR(n), massI(n);
+        std::vector<float> fric(n), grav(n);
+        std::vector<uint32_t> layer(n), mask(n);
+        std::vector<uint8_t> ll(n), la(n);
+        std::vector<int> bw(n, w), ui(n, -1);
+        int movable = 0;
+        for (int i = 0; i < n; i++) {
+            pos[i] = make_float4(b->position[3 * i], b->position[3 * i + 1], b->position[3 * i + 2], 0);
+            rot[i] = make_float4(b->rotation[4 * i], b->rotation[4 * i + 1], b->rotation[4 * i + 2], b->rotation[4 * i + 3]);
+            vel[i] = make_float4(b->velocity[3 * i], b->velocity[3 * i + 1], b->velocity[3 * i + 2], 0);
+            ang[i] = make_float4(b->angular_velocity[3 * i], b->angular_velocity[3 * i + 1], b->angular_velocity[3 * i + 2], 0);
+            prev[i] = b->prev_velocity ? make_float4(b->prev_velocity[3 * i], b->prev_velocity[3 * i + 1], b->prev_velocity[3 * i + 2], 0)
+                                       : vel[i];
+            sizeR[i] = make_float4(b->size[3 * i], b->size[3 * i + 1], b->size[3 * i + 2], b->radius[i]);
+            massI[i] = make_float4(b->mass[i], b->moment[3 * i], b->moment[3 * i + 1], b->moment[3 * i + 2]);
+            fric[i] = b->friction[i];
+            grav[i] = b->gravity ? b->gravity[i] : prm.gravity;
+            layer[i] = b->layer ? b->layer[i] : 1u;
+            mask[i] = b->mask ? b->mask[i] : 1u;
+            ll[i] = b->lock_linear ? b->lock_linear[i] : 0;
+            la[i] = b->lock_angular ? b->lock_angular[i] : 0;
+            if (b->mass[i] > 0.0f) ui[i] = movable++;
+        }
+        const int o = wi.bodyBase;
+        auto put = [&](auto *dst, const auto &src) {
+            return hipMemcpyAsync(dst + o, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice, stream);
+        };
+        HIPCHK(put(S.pos, pos)); HIPCHK(put(S.rot, rot)); HIPCHK(put(S.vel, vel)); HIPCHK(put(S.angVel, ang));
+        HIPCHK(put(S.prevVel, prev)); HIPCHK(put(S.initLin, pos)); HIPCHK(put(S.initAng, rot));
+        HIPCHK(put(S.inertLin, pos)); HIPCHK(put(S.inertAng, rot));
+        HIPCHK(put(S.sizeR, sizeR)); HIPCHK(put(S.massI, massI)); HIPCHK(put(S.fric, fric)); HIPCHK(put(S.grav, grav));
+        HIPCHK(put(S.layer, layer)); HIPCHK(put(S.mask, mask)); HIPCHK(put(S.lockLin, ll)); HIPCHK(put(S.lockAng, la));
+        HIPCHK(put(S.bodyWorld, bw)); HIPCHK(put(S.updateIndex, ui));
+        if (nj > 0) {
+            std::vector<int> ja(nj), jb(nj), jw(nj, w);
+            std::vector<float4> ra(nj), rb(nj), c0l(nj), c0a(nj), pl(nj), pa(nj), ll2(nj), la2(nj);
+            std::vector<float> sl(nj), sa(nj), fr(nj), arm(nj);
+            std::vector<uint8_t> br(nj), dead(nj, 0);
+            auto f4 = [](const float *p, int k) { return p ? make_float4(p[3 * k], p[3 * k + 1], p[3 * k + 2], 0) : make_float4(0, 0, 0, 0); };
+            for (int k = 0; k < nj; k++) {
+                if (j->body_b[k] < 0 || j->body_b[k] >= n || j->body_a[k] >= n)
+                    return fail(AVBD_ERR_INVALID, "joint body index out of range");
+                ja[k] = j->body_a[k] >= 0 ? j->body_a[k] + o : -1;
+                jb[k] = j->body_b[k] + o;
+                ra[k] = f4(j->r_a, k); rb[k] = f4(j->r_b, k);
+                c0l[k] = f4(j->c0_lin, k); c0a[k] = f4(j->c0_ang, k);
+                pl[k] = f4(j->penalty_lin, k); pa[k] = f4(j->penalty_ang, k);
+                ll2[k] = f4(j->lambda_lin, k); la2[k] = f4(j->lambda_ang, k);
+                sl[k] = j->stiffness_lin[k]; sa[k] = j->stiffness_ang[k];
+                fr[k] = j->fracture[k]; arm[k] = j->torque_arm[k];
+                br[k] = j->broken ? j->broken[k] : 0;
+            }
+            const int jo = wi.jointBase;
+            auto putj = [&](auto *dst, const auto &src) {
+                return hipMemcpyAsync(dst + jo, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice, stream);
+            };
+            HIPCHK(putj(S.jA, ja)); HIPCHK(putj(S.jB, jb)); HIPCHK(putj(S.jWorld, jw)); HIPCHK(putj(S.jRA, ra));
+            HIPCHK(putj(S.jRB, rb)); HIPCHK(putj(S.jC0L, c0l)); HIPCHK(putj(S.jC0A, c0a)); HIPCHK(putj(S.jPenL, pl));
+            HIPCHK(putj(S.jPenA, pa)); HIPCHK(putj(S.jLamL, ll2)); HIPCHK(putj(S.jLamA, la2));
+            HIPCHK(putj(S.jStiffL, sl)); HIPCHK(putj(S.jStiffA, sa)); HIPCHK(putj(S.jFrac, fr)); HIPCHK(putj(S.jArm, arm));
+            HIPCHK(putj(S.jBroken, br)); HIPCHK(putj(S.jDead, dead));
+        }
+        // slot range ownership and empty manifold store
+        std::vector<int> sw(slotCap, w);
+        HIPCHK(hipMemcpyAsync(S.slotWorld + wi.slotBase, sw.data(), sizeof(int) * slotCap, hipMemcpyHostToDevice, stream));
+        const int tab[6] = {wi.bodyBase, n, wi.jointBase, nj, wi.slotBase, slotCap};
+        int *dst[6] = {S.wBodyBase, S.wBodyCount, S.wJointBase, S.wJointCount, S.wSlotBase, S.wSlotCap};
+        for (int k = 0; k < 6; k++) HIPCHK(hipMemcpyAsync(dst[k] + w, &tab[k], sizeof(int), hipMemcpyHostToDevice, stream));
+        const int zero = 0;
+        int *zs[5] = {S.wLive, S.wFree, S.wTop, S.wNew, S.wErr};
+        for (int k = 0; k < 5; k++) HIPCHK(hipMemcpyAsync(zs[k] + w, &zero, sizeof(int), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        wi.active = true;
+        csrValid = false;
+        treeAge = 1 << 20;
+        return AVBD_OK;
+    }
+
+    avbd_status clearWorld(int w)
+    {
+        WorldInfo &wi = worlds[w];
+        if (!wi.active) return fail(AVBD_ERR_INVALID, "world is not active");
+        fill(S.bodyWorld + wi.bodyBase, -1, wi.bodyCount);
+        fill(S.updateIndex + wi.bodyBase, -1, wi.bodyCount);
+        fill(S.jWorld + wi.jointBase, -1, wi.jointCount);
+        fill(S.slotWorld + wi.slotBase, -1, wi.slotCap);
+        const int zero = 0;
+        int *zs[8] = {S.wBodyCount, S.wJointCount, S.wSlotCap, S.wLive, S.wFree, S.wTop, S.wNew, S.wErr};
+        for (int k = 0; k < 8; k++) HIPCHK(hipMemcpyAsync(zs[k] + w, &zero, sizeof(int), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        bodyAlloc.give(wi.bodyBase, wi.bodyCount);
+        jointAlloc.give(wi.jointBase, wi.jointCount);
+        slotAlloc.give(wi.slotBase, wi.slotCap);
+        wi = WorldInfo{};
+        csrValid = false;
+        treeAge = 1 << 20;
+        return AVBD_OK;
+    }
+
+    avbd_status downloadWorld(int w, avbd_bodies *b, avbd_joints *j)
+    {
+        const WorldInfo &wi = worlds[w];
+        if (!wi.active) return fail(AVBD_ERR_INVALID, "world is not active");
+        const int n = wi.bodyCount, o = wi.bodyBase;
+        HIPCHK(hipStreamSynchronize(stream));
+        if (b) {
+            if (b->count < n) return fail(AVBD_ERR_CAPACITY, "body arrays are too small");
+            b->count = n;
+            std::vector<float4> tmp(n);
+            auto get = [&](const float4 *src, float *dst, int comps) -> hipError_t {
+                if (!dst) return hipSuccess;
+                hipError_t e = hipMemcpy(tmp.data(), src + o, sizeof(float4) * n, hipMemcpyDeviceToHost);
+                if (e != hipSuccess) return e;
+                for (int i = 0; i < n; i++) {
+                    const float v[4] = {tmp[i].x, tmp[i].y, tmp[i].z, tmp[i].w};
+                    for (int c = 0; c < comps; c++) dst[comps * i + c] = v[c];
+                }
+                return hipSuccess;
+            };
+            HIPCHK(get(S.pos, b->position, 3));
+            HIPCHK(get(S.rot, b->rotation, 4));
+            HIPCHK(get(S.vel, b->velocity, 3));
+            HIPCHK(get(S.angVel, b->angular_velocity, 3));
+            HIPCHK(get(S.prevVel, b->prev_velocity, 3));
+        }
+        if (j) {
+            const int nj = wi.jointCount, jo = wi.jointBase;
+            std::vector<uint8_t> dead(std::max(nj, 1)), broken(std::max(nj, 1));
+            std::vector<int> ja(std::max(nj, 1)), jb(std::max(nj, 1));
+            std::vector<float4> c0l(nj), c0a(nj), pl(nj), pa(nj), ll(nj), la(nj);
+            if (nj > 0) {
+                HIPCHK(hipMemcpy(dead.data(), S.jDead + jo, nj, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(broken.data(), S.jBroken + jo, nj, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(ja.data(), S.jA + jo, sizeof(int) * nj, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(jb.data(), S.jB + jo, sizeof(int) * nj, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(c0l.data(), S.jC0L + jo, sizeof(float4) * nj, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(c0a.data(), S.jC0A + jo, sizeof(float4) * nj, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(pl.data(), S.jPenL + jo, sizeof(float4) * nj, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(pa.data(), S.jPenA + jo, sizeof(float4) * nj, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(ll.data(), S.jLamL + jo, sizeof(float4) * nj, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(la.data(), S.jLamA + jo, sizeof(float4) * nj, hipMemcpyDeviceToHost));
+            }
+            // a dead joint was reaped by the reference: compact the list as it does
+            int k = 0;
+            auto put3 = [](float *dst, int i, float4 v) { if (dst) { dst[3 * i] = v.x; dst[3 * i + 1] = v.y; dst[3 * i + 2] = v.z; } };
+            for (int q = 0; q < nj; q++) {
+                if (dead[q]) continue;
+                if (k >= j->count) return fail(AVBD_ERR_CAPACITY, "joint arrays are too small");
+                if (j->body_a) j->body_a[k] = ja[q] >= 0 ? ja[q] - o : -1;
+                if (j->body_b) j->body_b[k] = jb[q] - o;
+                put3(j->c0_lin, k, c0l[q]); put3(j->c0_ang, k, c0a[q]);
+                put3(j->penalty_lin, k, pl[q]); put3(j->penalty_ang, k, pa[q]);
+                put3(j->lambda_lin, k, ll[q]); put3(j->lambda_ang, k, la[q]);
+                if (j->broken) j->broken[k] = broken[q];
+                k++;
+            }
+            j->count = k;
+        }
+        return AVBD_OK;
+    }
+
+    avbd_status getManifolds(int w, avbd_manifolds *m)
+    {
+        const WorldInfo &wi = worlds[w];
+        HIPCHK(hipStreamSynchronize(stream));
+        int nLive = 0;
+        HIPCHK(hipMemcpy(&nLive, S.wLive + w, sizeof(int), hipMemcpyDeviceToHost));
+        if (m->count < nLive) {
+            m->count = nLive;
+            return fail(AVBD_ERR_CAPACITY, "manifold arrays are too small");
+        }
+        m->count = nLive;
+        if (nLive == 0) return AVBD_OK;
+        std::vector<int> live(nLive);
+        HIPCHK(hipMemcpy(live.data(), S.live + wi.slotBase, sizeof(int) * nLive, hipMemcpyDeviceToHost));
+        const int cap = wi.slotCap, sb = wi.slotBase;
+        std::vector<int> mA(cap), mB(cap), num(cap);
+        std::vector<float> fr(cap);
+        std::vector<float4> basis(3 * cap), cra(cap * MAXC), crb(cap * MAXC), cc0(cap * MAXC), cp(cap * MAXC), cl(cap * MAXC);
+        HIPCHK(hipMemcpy(mA.data(), S.mA + sb, sizeof(int) * cap, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(mB.data(), S.mB + sb, sizeof(int) * cap, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(num.data(), S.mNum + sb, sizeof(int) * cap, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(fr.data(), S.mFric + sb, sizeof(float) * cap, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(basis.data(), S.mBasis + 3 * sb, sizeof(float4) * 3 * cap, hipMemcpyDeviceToHost));
+        const size_t cb = static_cast<size_t>(sb) * MAXC, cn = static_cast<size_t>(cap) * MAXC;
+        HIPCHK(hipMemcpy(cra.data(), S.cRA + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(crb.data(), S.cRB + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(cc0.data(), S.cC0 + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(cp.data(), S.cPen + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(cl.data(), S.cLam + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
+        const int o = wi.bodyBase;
+        auto put3 = [](float *dst, int i, float4 v) { if (dst) { dst[3 * i] = v.x; dst[3 * i + 1] = v.y; dst[3 * i + 2] = v.z; } };
+        for (int i = 0; i < nLive; i++) {
+            const int s = live[i] - sb;
+            if (m->body_a) m->body_a[i] = mA[s] - o;
+            if (m->body_b) m->body_b[i] = mB[s] - o;
+            if (m->basis)
+                for (int r = 0; r < 3; r++) {
+                    const float4 v = basis[3 * s + r];
+                    m->basis[9 * i + 3 * r] = v.x; m->basis[9 * i + 3 * r + 1] = v.y; m->basis[9 * i + 3 * r + 2] = v.z;
+                }
+            if (m->friction) m->friction[i] = fr[s];
+            if (m->num_contacts) m->num_contacts[i] = num[s];
+            for (int k = 0; k < MAXC; k++) {
+                const int src = s * MAXC + k, dst = i * MAXC + k;
+                const bool used = k < num[s];
+                const float4 z = make_float4(0, 0, 0, 0);
+                int key = 0;
+                std::memcpy(&key, &crb[src].w, sizeof(int));
+                if (m->feature) m->feature[dst] = used ? key : 0;
+                put3(m->r_a, dst, used ? cra[src] : z);
+                put3(m->r_b, dst, used ? crb[src] : z);
+                put3(m->c0, dst, used ? cc0[src] : z);
+                put3(m->penalty, dst, used ? cp[src] : z);
+                put3(m->lambda, dst, used ? cl[src] : z);
+                if (m->stick) m->stick[dst] = used && cc0[src].w != 0.0f ? 1 : 0;
+                if (m->beta_scale) m->beta_scale[dst] = used ? cra[src].w : 1.0f;
+            }
+        }
+        return AVBD_OK;
+    }
+
+    avbd_status setManifolds(int w, const avbd_manifolds *m)
+    {
+        const WorldInfo &wi = worlds[w];
+        const int n = m->count;
+        if (n > wi.slotCap) return fail(AVBD_ERR_CAPACITY, "more manifolds than the world's slot capacity");
+        const int sb = wi.slotBase, o = wi.bodyBase;
+        std::vector<int> live(n), mA(n), mB(n), num(n);
+        std::vector<float> fr(n);
+        std::vector<float4> basis(3 * n), cra(n * MAXC), crb(n * MAXC), cc0(n * MAXC), cp(n * MAXC), cl(n * MAXC);
+        for (int i = 0; i < n; i++) {
+            live[i] = sb + i;
+            if (m->body_a[i] < 0 || m->body_a[i] >= wi.bodyCount || m->body_b[i] < 0 || m->body_b[i] >= wi.bodyCount)
+                return fail(AVBD_ERR_INVALID, "manifold body index out of range");
+            mA[i] = m->body_a[i] + o;
+            mB[i] = m->body_b[i] + o;
+            num[i] = m->num_contacts[i];
+            fr[i] = m->friction[i];
+            for (int r = 0; r < 3; r++)
+                basis[3 * i + r] = make_float4(m->basis[9 * i + 3 * r], m->basis[9 * i + 3 * r + 1], m->basis[9 * i + 3 * r + 2], 0);
+            for (int k = 0; k < MAXC; k++) {
+                const int q = i * MAXC + k;
+                float keyf;
+                std::memcpy(&keyf, &m->feature[q], sizeof(float));
+                cra[q] = make_float4(m->r_a[3 * q], m->r_a[3 * q + 1], m->r_a[3 * q + 2], m->beta_scale ? m->beta_scale[q] : 1.0f);
+                crb[q] = make_float4(m->r_b[3 * q], m->r_b[3 * q + 1], m->r_b[3 * q + 2], keyf);
+                cc0[q] = make_float4(m->c0[3 * q], m->c0[3 * q + 1], m->c0[3 * q + 2], m->stick[q] ? 1.0f : 0.0f);
+                cp[q] = make_float4(m->penalty[3 * q], m->penalty[3 * q + 1], m->penalty[3 * q + 2], 0);
+                cl[q] = make_float4(m->lambda[3 * q], m->lambda[3 * q + 1], m->lambda[3 * q + 2], 0);
+            }
+        }
+        if (n > 0) {
+            HIPCHK(hipMemcpy(S.live + sb, live.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S.mA + sb, mA.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S.mB + sb, mB.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S.mNum + sb, num.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S.mFric + sb, fr.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S.mBasis + 3 * sb, basis.data(), sizeof(float4) * 3 * n, hipMemcpyHostToDevice));
+            const size_t cb = static_cast<size_t>(sb) * MAXC, cn = static_cast<size_t>(n) * MAXC;
+            HIPCHK(hipMemcpy(S.cRA + cb, cra.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S.cRB + cb, crb.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S.cC0 + cb, cc0.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S.cPen + cb, cp.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S.cLam + cb, cl.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
+        }
+        const int vals[3] = {n, 0, n};
+        int *dst[3] = {S.wLive, S.wFree, S.wTop};
+        for (int k = 0; k < 3; k++) HIPCHK(hipMemcpy(dst[k] + w, &vals[k], sizeof(int), hipMemcpyHostToDevice));
+        csrValid = false;
+        return AVBD_OK;
+    }
+
+    avbd_status probeBroadphase(uint64_t *codes, int32_t *order, uint64_t *pairs, int32_t *pairCount)
+    {
+        const WorldInfo &wi = worlds[0];
+        setParams(S.P, prm, prm.dt);
+        if (!csrValid) buildAdjacency();
+        csrValid = true;
+        const int nb = nblk(Ncap);
+        k_swept_aabb<<<nb, TPB, 0, stream>>>(S);
+        k_world_bounds<<<Wcap, TPB, 0, stream>>>(S);
+        k_morton<<<nb, TPB, 0, stream>>>(S, 1);
+        radixSort();
+        k_karras<<<nblk(Ncap - 1), TPB, 0, stream>>>(S, Ncap);
+        k_refit<<<nb, TPB, 0, stream>>>(S, Ncap);
+        k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap, 0);
+        scan(S.pairCount, S.pairOff, Ncap);
+        k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap, 1);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
+        treeAge = 1 << 20;
+        const int n = wi.bodyCount;
+        if (codes) {
+            std::vector<unsigned long long> c(n);
+            HIPCHK(hipMemcpy(c.data(), S.mortonCode, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost));
+            for (int i = 0; i < n; i++) codes[i] = c[i];
+        }
+        if (order) {
+            // the world occupies positions [0, n) in sorted order (bodies outside it sort later)
+            std::vector<int> v(n);
+            HIPCHK(hipMemcpy(v.data(), S.sortVal, sizeof(int) * n, hipMemcpyDeviceToHost));
+            for (int i = 0; i < n; i++) order[i] = v[i];
+        }
+        uint32_t total = 0;
+        HIPCHK(hipMemcpy(&total, S.pairOff + Ncap, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (pairCount) {
+            const int cap = *pairCount;
+            *pairCount = static_cast<int32_t>(total);
+            if (pairs && total > 0) {
+                if (static_cast<int>(total) > cap) return fail(AVBD_ERR_CAPACITY, "pair array too small");
+                if (total > static_cast<uint32_t>(Pcap)) return fail(AVBD_ERR_CAPACITY, "candidate pair capacity exceeded");
+                std::vector<int> pi(total), pj(total);
+                HIPCHK(hipMemcpy(pi.data(), S.pairI, sizeof(int) * total, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(pj.data(), S.pairJ, sizeof(int) * total, hipMemcpyDeviceToHost));
+                for (uint32_t k = 0; k < total; k++)
+                    pairs[k] = (static_cast<uint64_t>(static_cast<uint32_t>(pi[k])) << 32) | static_cast<uint32_t>(pj[k]);
+            }
+        }
+        return AVBD_OK;
+    }
+
+    avbd_status fillStats(int w, avbd_step_stats *out)
+    {
+        HIPCHK(hipStreamSynchronize(stream));
+        avbd_step_stats s = stats;
+        const WorldInfo &wi = worlds[w];
+        s.bodies = wi.bodyCount;
+        s.joints = wi.jointCount;
+        int v[3] = {0, 0, 0};
+        HIPCHK(hipMemcpy(&v[0], S.wLive + w, sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&v[1], S.wNew + w, sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&v[2], S.brakeLevels + w, sizeof(int), hipMemcpyDeviceToHost));
+        s.manifolds = v[0];
+        s.new_manifolds = v[1];
+        s.brake_levels = v[2];
+        if (v[0] > 0) {
+            std::vector<int> live(v[0]);
+            HIPCHK(hipMemcpy(live.data(), S.live + wi.slotBase, sizeof(int) * v[0], hipMemcpyDeviceToHost));
+            std::vector<int> num(wi.slotCap);
+            HIPCHK(hipMemcpy(num.data(), S.mNum + wi.slotBase, sizeof(int) * wi.slotCap, hipMemcpyDeviceToHost));
+            int c = 0;
+            for (int s2 : live) c += num[s2 - wi.slotBase];
+            s.contacts = c;
+        }
+        uint32_t a = 0, b = 0;
+        HIPCHK(hipMemcpy(&a, S.pairOff + wi.bodyBase, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&b, S.pairOff + wi.bodyBase + wi.bodyCount, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        s.candidate_pairs = static_cast<int32_t>(b - a);
+        std::vector<uint8_t> col(wi.bodyCount);
+        HIPCHK(hipMemcpy(col.data(), S.colour + wi.bodyBase, wi.bodyCount, hipMemcpyDeviceToHost));
+        std::vector<int> ui(wi.bodyCount);
+        HIPCHK(hipMemcpy(ui.data(), S.updateIndex + wi.bodyBase, sizeof(int) * wi.bodyCount, hipMemcpyDeviceToHost));
+        int maxc = -1, ov = 0;
+        for (int i = 0; i < wi.bodyCount; i++) {
+            if (ui[i] < 0) continue;
+            if (col[i] < 8) maxc = std::max<int>(maxc, col[i]);
+            else if (col[i] == 8) ov++;
+        }
+        s.colours = maxc + 1;
+        s.overflow_bodies = ov;
+        *out = s;
+        return AVBD_OK;
+    }
+};
+
+// ---------------------------------------------------------------------- C ABI ----
+struct avbd_world {
+    Engine eng;
+};
+struct avbd_batch {
+    Engine eng;
+};
+
+extern "C" {
+
+void avbd_params_default(avbd_params *p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->dt = 1.0f / 60.0f;
+    p->gravity = -10.0f;
+    p->iterations = 8;
+    p->alpha = 0.99f;
+    p->beta_lin = 10000.0f;
+    p->beta_ang = 100.0f;
+    p->gamma = 0.999f;
+    p->newton_ratio = 1.0f;
+    p->stiffness_decay = 1.0f;
+    p->adaptive_beta = 0;
+    p->adaptive_beta_error_floor = 1e-3f;
+    p->convergence_threshold = 0.0f;
+    p->jacobi_relaxation = 0.7f;
+    p->contact_only_overflow = 1;
+    p->stick_drift_heal = 0.002f;
+    p->max_penetration = 0.02f;
+    p->jacobian_rebuild_distance = 0.0f;
+    p->speculative_margin = 0.01f;
+    p->position_projection = 1;
+    p->projection_iterations = 4;
+    p->projection_beta = 1.0f;
+    p->projection_slop = 0.008f;
+    p->projection_max_correction = 0.0f;
+    p->contact_grace_ticks = 0;
+    p->threads = 0;
+    p->sleep_linear_threshold = 0.1f;
+    p->sleep_angular_threshold = 0.139626f;
+    p->sleep_frames = 0;
+    p->penalty_max = 1e10f;
+}
+
+void avbd_box_properties(const float size[3], float density, float *mass, float moment[3], float *radius)
+{
+    const float m = size[0] * size[1] * size[2] * density;
+    if (mass) *mass = m;
+    if (moment) {
+        moment[0] = (size[1] * size[1] + size[2] * size[2]) / 12.0f * m;
+        moment[1] = (size[0] * size[0] + size[2] * size[2]) / 12.0f * m;
+        moment[2] = (size[0] * size[0] + size[1] * size[1]) / 12.0f * m;
+    }
+    if (radius) {
+        const float hx = size[0] * 0.5f, hy = size[1] * 0.5f, hz = size[2] * 0.5f;
+        *radius = std::sqrt(((0.0f + hx * hx) + hy * hy) + hz * hz);
+    }
+}
+
+uint64_t avbd_state_digest(const avbd_bodies *b)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](const void *p, size_t n) {
+        const unsigned char *c = static_cast<const unsigned char *>(p);
+        for (size_t i = 0; i < n; i++) {
+            h ^= c[i];
+            h *= 1099511628211ull;
+        }
+    };
+    if (!b) return h;
+    for (int i = 0; i < b->count; i++) {
+        mix(b->position + 3 * i, 12);
+        mix(b->rotation + 4 * i, 16);
+        mix(b->velocity + 3 * i, 12);
+        mix(b->angular_velocity + 3 * i, 12);
+    }
+    return h;
+}
+
+const char *avbd_last_error(void) { return g_lastError.c_str(); }
+
+int32_t avbd_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+static avbd_status validBodies(const avbd_bodies *b)
+{
+    if (!b) return fail(AVBD_ERR_INVALID, "bodies is null");
+    if (b->count < 0) return fail(AVBD_ERR_INVALID, "negative body count");
+    if (b->count > 0 && (!b->size || !b->mass || !b->moment || !b->friction || !b->radius || !b->position ||
+                         !b->rotation || !b->velocity || !b->angular_velocity))
+        return fail(AVBD_ERR_INVALID, "a required body array is null");
+    return AVBD_OK;
+}
+static avbd_status validJoints(const avbd_joints *j)
+{
+    if (!j || j->count == 0) return AVBD_OK;
+    if (j->count < 0) return fail(AVBD_ERR_INVALID, "negative joint count");
+    if (!j->body_a || !j->body_b || !j->r_a || !j->r_b || !j->stiffness_lin || !j->stiffness_ang || !j->fracture ||
+        !j->torque_arm)
+        return fail(AVBD_ERR_INVALID, "a required joint array is null");
+    return AVBD_OK;
+}
+
+avbd_status avbd_world_create(const avbd_params *params, int32_t max_bodies, int32_t max_joints,
+                              int32_t max_manifolds, avbd_world **out)
+{
+    if (!out) return fail(AVBD_ERR_INVALID, "out is null");
+    *out = nullptr;
+    avbd_status st = checkParams(params);
+    if (st != AVBD_OK) return st;
+    if (max_bodies < 0 || max_joints < 0) return fail(AVBD_ERR_INVALID, "negative capacity");
+    auto *w = new avbd_world();
+    const int slots = max_manifolds > 0 ? max_manifolds : 8 * std::max(max_bodies, 1) + 64;
+    st = w->eng.create(*params, 1, max_bodies, max_joints, slots, false);
+    if (st != AVBD_OK) {
+        delete w;
+        return st;
+    }
+    *out = w;
+    return AVBD_OK;
+}
+
+avbd_status avbd_world_destroy(avbd_world *world)
+{
+    delete world;
+    return AVBD_OK;
+}
+
+avbd_status avbd_world_set_params(avbd_world *world, const avbd_params *params)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    avbd_status st = checkParams(params);
+    if (st != AVBD_OK) return st;
+    world->eng.prm = *params;
+    return AVBD_OK;
+}
+
+avbd_status avbd_world_upload(avbd_world *world, const avbd_bodies *bodies, const avbd_joints *joints)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    avbd_status st = validBodies(bodies);
+    if (st != AVBD_OK) return st;
+    if ((st = validJoints(joints)) != AVBD_OK) return st;
+    Engine &e = world->eng;
+    if (bodies->count > e.Ncap) return fail(AVBD_ERR_CAPACITY, "more bodies than the world's capacity");
+    if (joints && joints->count > e.Jcap) return fail(AVBD_ERR_CAPACITY, "more joints than the world's capacity");
+    if (e.worlds[0].active) {
+        st = e.clearWorld(0);
+        if (st != AVBD_OK) return st;
+    }
+    WorldInfo &wi = e.worlds[0];
+    wi.bodyBase = e.bodyAlloc.take(bodies->count);
+    wi.jointBase = e.jointAlloc.take(joints ? joints->count : 0);
+    wi.slotBase = e.slotAlloc.take(e.Scap);
+    return e.uploadWorld(0, bodies, joints, e.Scap);
+}
+
+avbd_status avbd_world_step(avbd_world *world)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    if (!world->eng.worlds[0].active) return fail(AVBD_ERR_INVALID, "nothing uploaded");
+    return world->eng.stepPlain(1, 0);
+}
+
+avbd_status avbd_world_step_interleaved(avbd_world *world, int32_t substeps, int32_t rounds)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    if (!world->eng.worlds[0].active) return fail(AVBD_ERR_INVALID, "nothing uploaded");
+    return world->eng.stepPlain(substeps <= 1 ? 1 : substeps, rounds);
+}
+
+avbd_status avbd_world_download(avbd_world *world, avbd_bodies *bodies, avbd_joints *joints)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    return world->eng.downloadWorld(0, bodies, joints);
+}
+
+avbd_status avbd_world_get_manifolds(avbd_world *world, avbd_manifolds *out)
+{
+    if (!world || !out) return fail(AVBD_ERR_INVALID, "null argument");
+    return world->eng.getManifolds(0, out);
+}
+
+avbd_status avbd_world_set_manifolds(avbd_world *world, const avbd_manifolds *in)
+{
+    if (!world || !in) return fail(AVBD_ERR_INVALID, "null argument");
+    if (in->count > 0 && (!in->body_a || !in->body_b || !in->basis || !in->friction || !in->num_contacts ||
+                          !in->feature || !in->r_a || !in->r_b || !in->c0 || !in->penalty || !in->lambda || !in->stick))
+        return fail(AVBD_ERR_INVALID, "a required manifold array is null");
+    return world->eng.setManifolds(0, in);
+}
+
+avbd_status avbd_world_stats(avbd_world *world, avbd_step_stats *out)
+{
+    if (!world || !out) return fail(AVBD_ERR_INVALID, "null argument");
+    return world->eng.fillStats(0, out);
+}
+
+avbd_status avbd_world_probe_broadphase(avbd_world *world, uint64_t *codes, int32_t *order, uint64_t *pairs,
+                                        int32_t *pair_count)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    if (!world->eng.worlds[0].active) return fail(AVBD_ERR_INVALID, "nothing uploaded");
+    return world->eng.probeBroadphase(codes, order, pairs, pair_count);
+}
+
+avbd_status avbd_batch_create(const avbd_params *params, int32_t max_worlds, int32_t max_bodies, int32_t max_joints,
+                              int32_t max_manifolds, avbd_batch **out)
+{
+    if (!out) return fail(AVBD_ERR_INVALID, "out is null");
+    *out = nullptr;
+    avbd_status st = checkParams(params);
+    if (st != AVBD_OK) return st;
+    if (max_worlds < 1 || max_bodies < 0 || max_joints < 0) return fail(AVBD_ERR_INVALID, "bad capacity");
+    auto *b = new avbd_batch();
+    const int slots = max_manifolds > 0 ? max_manifolds : 8 * std::max(max_bodies, 1) + 64 * max_worlds;
+    st = b->eng.create(*params, max_worlds, max_bodies, max_joints, slots, true);
+    if (st != AVBD_OK) {
+        delete b;
+        return st;
+    }
+    *out = b;
+    return AVBD_OK;
+}
+
+avbd_status avbd_batch_destroy(avbd_batch *batch)
+{
+    delete batch;
+    return AVBD_OK;
+}
+
+avbd_status avbd_batch_add_world(avbd_batch *batch, const avbd_bodies *bodies, const avbd_joints *joints,
+                                 int32_t *world_id)
+{
+    if (!batch || !world_id) return fail(AVBD_ERR_INVALID, "null argument");
+    avbd_status st = validBodies(bodies);
+    if (st != AVBD_OK) return st;
+    if ((st = validJoints(joints)) != AVBD_OK) return st;
+    Engine &e = batch->eng;
+    int w = -1;
+    for (int k = 0; k < e.Wcap; k++)
+        if (!e.worlds[k].active) { w = k; break; }
+    if (w < 0) return fail(AVBD_ERR_CAPACITY, "no free world slot");
+    const int nb = bodies->count, nj = joints ? joints->count : 0;
+    const int slots = std::max(8 * nb + 16, 32);
+    const int bb = e.bodyAlloc.take(nb);
+    if (bb < 0) return fail(AVBD_ERR_CAPACITY, "not enough body capacity");
+    const int jb = e.jointAlloc.take(nj);
+    if (jb < 0) { e.bodyAlloc.give(bb, nb); return fail(AVBD_ERR_CAPACITY, "not enough joint capacity"); }
+    const int sb = e.slotAlloc.take(slots);
+    if (sb < 0) {
+        e.bodyAlloc.give(bb, nb);
+        e.jointAlloc.give(jb, nj);
+        return fail(AVBD_ERR_CAPACITY, "not enough manifold capacity");
+    }
+    WorldInfo &wi = e.worlds[w];
+    wi.bodyBase = bb;
+    wi.jointBase = jb;
+    wi.slotBase = sb;
+    st = e.uploadWorld(w, bodies, joints, slots);
+    if (st != AVBD_OK) return st;
+    *world_id = w;
+    return AVBD_OK;
+}
+
+avbd_status avbd_batch_remove_world(avbd_batch *batch, int32_t world_id)
+{
+    if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
+    if (world_id < 0 || world_id >= batch->eng.Wcap) return fail(AVBD_ERR_INVALID, "bad world id");
+    return batch->eng.clearWorld(world_id);
+}
+
+avbd_status avbd_batch_step(avbd_batch *batch)
+{
+    if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
+    return batch->eng.stepBatch();
+}
+
+avbd_status avbd_batch_download_world(avbd_batch *batch, int32_t world_id, avbd_bodies *bodies, avbd_joints *joints)
+{
+    if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
+    if (world_id < 0 || world_id >= batch->eng.Wcap) return fail(AVBD_ERR_INVALID, "bad world id");
+    return batch->eng.downloadWorld(world_id, bodies, joints);
+}
+
+avbd_status avbd_batch_world_count(avbd_batch *batch, int32_t *count)
+{
+    if (!batch || !count) return fail(AVBD_ERR_INVALID, "null argument");
+    int c = 0;
+    for (const auto &w : batch->eng.worlds) c += w.active ? 1 : 0;
+    *count = c;
+    return AVBD_OK;
+}
+
+} // extern "C"

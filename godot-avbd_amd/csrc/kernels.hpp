@@ -1,0 +1,1229 @@
+// Kernels of the AVBD step.  One thread per body, per manifold slot, per joint or per
+// candidate pair, with per-world workgroups for the two passes the reference runs in
+// a data-dependent order (greedy colouring, velocity brake).  Grids are sized by the
+// batch capacities, so a launch sequence never depends on data and can be captured in a
+// HIP graph; per-world counts on the device say which threads have work.
+#pragma once
+
+#include "dsolver.hpp"
+
+namespace avbd_dev {
+
+constexpr int TPB = 256;
+__host__ __device__ inline int nblk(int n, int tpb = TPB) { return (n + tpb - 1) / tpb; }
+
+#define GID (static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x))
+
+// ------------------------------------------------------------------ utilities ----
+__global__ void k_fill_i32(int *p, int v, int n)
+{
+    const int i = GID;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, int n)
+{
+    const int i = GID;
+    if (i < n) p[i] = v;
+}
+
+// Exclusive scan of n u32 values, 1024 per block; out has n+1 entries (out[n] = total).
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = TPB * SCAN_ITEMS;
+
+__device__ __forceinline__ uint32_t blockExclusiveScan(uint32_t v, uint32_t *sm, uint32_t &total)
+{
+    const int tid = threadIdx.x;
+    sm[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < TPB; off <<= 1) {
+        const uint32_t x = tid >= off ? sm[tid - off] : 0u;
+        __syncthreads();
+        sm[tid] += x;
+        __syncthreads();
+    }
+    total = sm[TPB - 1];
+    const uint32_t incl = sm[tid];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void k_scan_tiles(const uint32_t *in, uint32_t *out, uint32_t *tileSums, int n)
+{
+    __shared__ uint32_t sm[TPB];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t run = blockExclusiveScan(s, sm, total);
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) tileSums[blockIdx.x] = total;
+}
+
+__global__ void k_scan_sums(uint32_t *tileSums, int nt, uint32_t *out, int n)
+{
+    __shared__ uint32_t sm[TPB];
+    uint32_t carry = 0;
+    for (int start = 0; start < nt; start += TPB) {
+        const int i = start + threadIdx.x;
+        const uint32_t v = i < nt ? tileSums[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = blockExclusiveScan(v, sm, total);
+        if (i < nt) tileSums[i] = ex + carry;
+        carry += total;
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ void k_scan_add(uint32_t *out, const uint32_t *tileSums, int n)
+{
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    const uint32_t add = tileSums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++)
+        if (base + k < n) out[base + k] += add;
+}
+
+// LSD radix sort pass over 8 key bits: per-tile digit histograms, then a stable scatter
+// that ranks equal digits within each wavefront with ballots.
+constexpr int RADIX_ITEMS = 8;
+constexpr int RADIX_TILE = TPB * RADIX_ITEMS;
+
+__global__ void k_radix_hist(const unsigned long long *keys, int n, int shift, uint32_t *hist, int nb)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int start = blockIdx.x * RADIX_TILE;
+    for (int r = 0; r < RADIX_ITEMS; r++) {
+        const int i = start + r * TPB + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void k_radix_scatter(const unsigned long long *kin, const int *vin, unsigned long long *kout, int *vout,
+                                int n, int shift, const uint32_t *offs, int nb)
+{
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t wcnt[TPB / 64][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    base[tid] = offs[tid * nb + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < TPB / 64; w++) wcnt[w][tid] = 0;
+    __syncthreads();
+    const int start = blockIdx.x * RADIX_TILE;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int r = 0; r < RADIX_ITEMS; r++) {
+        const int i = start + r * TPB + tid;
+        const bool valid = i < n;
+        const unsigned long long key = valid ? kin[i] : 0ull;
+        const int val = valid ? vin[i] : 0;
+        const uint32_t d = valid ? static_cast<uint32_t>((key >> shift) & 255u) : 0u;
+        unsigned long long m = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            const unsigned long long b = __ballot((d >> bit) & 1u);
+            m &= ((d >> bit) & 1u) ? b : ~b;
+        }
+        const uint32_t rank = __popcll(m & below);
+        if (valid && rank == 0) wcnt[wave][d] = __popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = base[d];
+            for (int w = 0; w < wave; w++) off += wcnt[w][d];
+            kout[off + rank] = key;
+            vout[off + rank] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < TPB / 64; w++) { add += wcnt[w][tid]; wcnt[w][tid] = 0; }
+        base[tid] += add;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ broad phase ----
+// solver_broadphase.cpp:101-122 + rigid.cpp:83-106: swept AABB of every body.  Bodies
+// outside any world get an empty box so they never overlap.
+__global__ void k_swept_aabb(State S)
+{
+    const int i = GID;
+    if (i >= S.numBodies) return;
+    if (S.bodyWorld[i] < 0) {
+        S.aabbC[i] = make_float4(0, 0, 0, 0);
+        S.aabbH[i] = make_float4(-1, -1, -1, 0);
+        return;
+    }
+    const float4 sr = S.sizeR[i];
+    const v3 h = v3{sr.x, sr.y, sr.z} * 0.5f;
+    const m3 R = qmat(bRot(S, i));
+    const v3 tight{(fabsf(R.c[0][0]) * h.x + fabsf(R.c[1][0]) * h.y) + fabsf(R.c[2][0]) * h.z,
+                   (fabsf(R.c[0][1]) * h.x + fabsf(R.c[1][1]) * h.y) + fabsf(R.c[2][1]) * h.z,
+                   (fabsf(R.c[0][2]) * h.x + fabsf(R.c[1][2]) * h.y) + fabsf(R.c[2][2]) * h.z};
+    const float cdt = S.P.collisionDt;
+    v3 disp = ld3(S.vel[i]) * cdt;
+    if (bMass(S, i) > 0) disp += v3{0, 0, S.grav[i] * cdt * cdt};
+    const float rotSweep = sr.w * length(ld3(S.angVel[i])) * cdt;
+    const float grow = rotSweep + S.P.specMargin;
+    S.aabbC[i] = st3(bPos(S, i) + disp * 0.5f);
+    S.aabbH[i] = st3((tight + fabs3(disp) * 0.5f) + v3{grow, grow, grow});
+}
+
+// Scene bounds of each world's AABB centres (bvh_broadphase.cpp:332-342); one block per
+// world.  min/max are order-independent, so the block reduction is exact.
+__global__ void k_world_bounds(State S)
+{
+    const int w = blockIdx.x;
+    __shared__ float lo[3][TPB], hi[3][TPB];
+    const int base = S.wBodyBase[w], cnt = S.wBodyCount[w];
+    float l0 = FLT_MAXV, l1 = FLT_MAXV, l2 = FLT_MAXV, h0 = -FLT_MAXV, h1 = -FLT_MAXV, h2 = -FLT_MAXV;
+    for (int k = threadIdx.x; k < cnt; k += TPB) {
+        const float4 c = S.aabbC[base + k];
+        l0 = smin(l0, c.x); l1 = smin(l1, c.y); l2 = smin(l2, c.z);
+        h0 = smax(h0, c.x); h1 = smax(h1, c.y); h2 = smax(h2, c.z);
+    }
+    lo[0][threadIdx.x] = l0; lo[1][threadIdx.x] = l1; lo[2][threadIdx.x] = l2;
+    hi[0][threadIdx.x] = h0; hi[1][threadIdx.x] = h1; hi[2][threadIdx.x] = h2;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                lo[a][threadIdx.x] = smin(lo[a][threadIdx.x], lo[a][threadIdx.x + s]);
+                hi[a][threadIdx.x] = smax(hi[a][threadIdx.x], hi[a][threadIdx.x + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        S.wBoundsLo[w] = make_float4(lo[0][0], lo[1][0], lo[2][0], 0);
+        S.wBoundsHi[w] = make_float4(hi[0][0], hi[1][0], hi[2][0], 0);
+    }
+}
+
+__device__ __forceinline__ unsigned long long expandBits21(uint32_t v)
+{
+    unsigned long long x = v & 0x1fffffu;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10cfcfcfcfcfcfcfull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+// bvh_broadphase.cpp:19-36, 343-348: 63-bit Morton code of the centre, every axis offset
+// by the scene's lo.x (the reference's quantisation, kept for bit-exact codes).  The sort
+// key is the full code for a single world; for a batch it is (segment << 43 | code >> 20)
+// so each world's bodies sort within their own index range.
+__global__ void k_morton(State S, int fullKey)
+{
+    const int i = GID;
+    if (i >= S.numBodies) return;
+    const int w = S.bodyWorld[i];
+    unsigned long long code = 0;
+    if (w >= 0) {
+        const float4 lo = S.wBoundsLo[w], hi = S.wBoundsHi[w];
+        const float ex = hi.x - lo.x, ey = hi.y - lo.y, ez = hi.z - lo.z;
+        float m = ex;
+        m = smax(m, ey);
+        m = smax(m, ez);
+        m = smax(m, 1.0e-6f);
+        const float scale = 2097151.0f / m;
+        const float4 c = S.aabbC[i];
+        const uint32_t qx = static_cast<uint32_t>(sclamp((c.x - lo.x) * scale, 0.0f, 2097151.0f));
+        const uint32_t qy = static_cast<uint32_t>(sclamp((c.y - lo.x) * scale, 0.0f, 2097151.0f));
+        const uint32_t qz = static_cast<uint32_t>(sclamp((c.z - lo.x) * scale, 0.0f, 2097151.0f));
+        code = expandBits21(qx) | (expandBits21(qy) << 1) | (expandBits21(qz) << 2);
+    }
+    S.mortonCode[i] = code;
+    const unsigned long long seg = w >= 0 ? static_cast<unsigned long long>(S.wBodyBase[w]) : static_cast<unsigned long long>(i);
+    // bodies outside every world sort after the world (single-world key) or into their
+    // own index position (segment key)
+    S.sortKey[i] = fullKey ? (w >= 0 ? code : ~0ull) : ((seg << 43) | (code >> 20));
+    S.sortVal[i] = i;
+}
+
+// Common-prefix length of sorted keys i and j, ties broken by position (Karras 2012).
+__device__ __forceinline__ int kdelta(const unsigned long long *k, int n, int i, int j)
+{
+    if (j < 0 || j >= n) return -1;
+    const unsigned long long a = k[i], b = k[j];
+    if (a == b) return 64 + __clz(static_cast<uint32_t>(i ^ j));
+    return __clzll(a ^ b);
+}
+
+// Karras radix-tree build: internal node i of n-1, children encoded as node ids
+// (internal [0, n-1), leaf p at n-1+p).
+__global__ void k_karras(State S, int n)
+{
+    const int i = GID;
+    if (i >= n - 1) return;
+    const unsigned long long *k = S.sortKey;
+    const int d = (kdelta(k, n, i, i + 1) - kdelta(k, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = kdelta(k, n, i, i - d);
+    int lmax = 2;
+    while (kdelta(k, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (kdelta(k, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = kdelta(k, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (kdelta(k, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + (d < 0 ? -1 : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const int left = (lo == gamma) ? (n - 1 + gamma) : gamma;
+    const int right = (hi == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
+    S.nodeLeft[i] = left;
+    S.nodeRight[i] = right;
+    S.nodeFirst[i] = lo;
+    S.nodeLast[i] = hi;
+    S.nodeParent[left] = i;
+    S.nodeParent[right] = i;
+    S.nodeFlag[i] = 0;
+}
+
+// Leaf boxes (centre -/+ half, the reference's bounds) and bottom-up refit: the second
+// child to arrive at a parent writes the union and continues.
+__global__ void k_refit(State S, int n)
+{
+    const int p = GID;
+    if (p >= n) return;
+    const int body = S.sortVal[p];
+    S.leafBody[p] = body;
+    S.bodyLeaf[body] = p;
+    const v3 c = ld3(S.aabbC[body]), h = ld3(S.aabbH[body]);
+    v3 mn = c - h, mx = c + h;
+    if (S.bodyWorld[body] < 0) {
+        mn = v3{FLT_MAXV, FLT_MAXV, FLT_MAXV};
+        mx = v3{-FLT_MAXV, -FLT_MAXV, -FLT_MAXV};
+    }
+    int node = n - 1 + p;
+    S.nodeMin[node] = st3(mn);
+    S.nodeMax[node] = st3(mx);
+    if (n == 1) return;
+    __threadfence();
+    int parent = S.nodeParent[node];
+    while (true) {
+        if (atomicAdd(&S.nodeFlag[parent], 1) == 0) return;
+        __threadfence();
+        const int l = S.nodeLeft[parent], r = S.nodeRight[parent];
+        const float4 a0 = S.nodeMin[l];
+        const float4 a1 = S.nodeMax[l];
+        const float4 b0 = S.nodeMin[r];
+        const float4 b1 = S.nodeMax[r];
+        S.nodeMin[parent] = make_float4(smin(a0.x, b0.x), smin(a0.y, b0.y), smin(a0.z, b0.z), 0);
+        S.nodeMax[parent] = make_float4(smax(a1.x, b1.x), smax(a1.y, b1.y), smax(a1.z, b1.z), 0);
+        __threadfence();
+        if (parent == 0) return;
+        parent = S.nodeParent[parent];
+    }
+}
+
+// Closed-interval AABB overlap exactly as the reference tests it
+// (solver_broadphase.cpp:35-40, bvh_broadphase.cpp:494-500).
+__device__ __forceinline__ bool aabbOverlap(v3 aMin, v3 aMax, v3 bMin, v3 bMax)
+{
+    return !(bMin.x > aMax.x || bMin.y > aMax.y || bMin.z > aMax.z || aMin.x > bMax.x || aMin.y > bMax.y ||
+             aMin.z > bMax.z);
+}
+
+constexpr int TRAV_STACK = 64;
+
+// Candidate pairs of body i: every j > i of the same world whose swept AABB overlaps.
+// Pass 0 counts, pass 1 writes the sorted j list at pairOff[i].
+__global__ void k_traverse(State S, int n, int pass)
+{
+    const int i = GID;
+    if (i >= S.numBodies) return;
+    const int w = S.bodyWorld[i];
+    if (w < 0 || n < 2) {
+        if (pass == 0) S.pairCount[i] = 0;
+        return;
+    }
+    const int wLo = S.wBodyBase[w], wHi = wLo + S.wBodyCount[w] - 1;
+    const v3 c = ld3(S.aabbC[i]), h = ld3(S.aabbH[i]);
+    const v3 qMin = c - h, qMax = c + h;
+    uint32_t out = pass ? S.pairOff[i] : 0u;
+    const uint32_t outEnd = pass ? S.pairOff[i + 1] : 0u;
+    const uint32_t outBegin = out;
+    uint32_t count = 0;
+    int stack[TRAV_STACK];
+    int sp = 0;
+    stack[sp++] = 0;
+    bool overflow = false;
+    while (sp > 0) {
+        const int node = stack[--sp];
+        if (node >= n - 1) {
+            const int p = node - (n - 1);
+            if (p < wLo || p > wHi) continue;
+            const int j = S.leafBody[p];
+            if (j <= i) continue;
+            const v3 cj = ld3(S.aabbC[j]), hj = ld3(S.aabbH[j]);
+            if (!aabbOverlap(qMin, qMax, cj - hj, cj + hj)) continue;
+            if (pass) {
+                if (out < outEnd) S.pairJ[out++] = j;
+            } else {
+                count++;
+            }
+            continue;
+        }
+        if (S.nodeLast[node] < wLo || S.nodeFirst[node] > wHi) continue;
+        const v3 nMin = ld3(S.nodeMin[node]), nMax = ld3(S.nodeMax[node]);
+        if (nMin.x > qMax.x || nMax.x < qMin.x || nMin.y > qMax.y || nMax.y < qMin.y || nMin.z > qMax.z ||
+            nMax.z < qMin.z)
+            continue;
+        if (sp + 2 > TRAV_STACK) { overflow = true; break; }
+        stack[sp++] = S.nodeRight[node];
+        stack[sp++] = S.nodeLeft[node];
+    }
+    if (overflow) atomicOr(S.wErr + w, 4);
+    if (pass == 0) {
+        S.pairCount[i] = count;
+        return;
+    }
+    // insertion sort of the segment by j, then the owner column
+    for (uint32_t a = outBegin + 1; a < out; a++) {
+        const int v = S.pairJ[a];
+        uint32_t b = a;
+        while (b > outBegin && S.pairJ[b - 1] > v) { S.pairJ[b] = S.pairJ[b - 1]; b--; }
+        S.pairJ[b] = v;
+    }
+    for (uint32_t a = outBegin; a < out; a++) S.pairI[a] = i;
+}
+
+// The body on the other side of an adjacency entry (solver_internal.hpp:364-384).
+__device__ __forceinline__ int otherOf(const State &S, int ref, int body)
+{
+    int a, b;
+    if (ref >= 0) { a = S.mA[ref]; b = S.mB[ref]; }
+    else { const int j = -ref - 1; a = S.jA[j]; b = S.jB[j]; }
+    if (a == body) return b;
+    if (b == body) return a;
+    return -1;
+}
+
+// solver_broadphase.cpp:30-65: bounds, layers, no force or manifold between the pair,
+// not both static.  `constrained` scans the shorter adjacency list (same answer).
+__global__ void k_pair_filter(State S)
+{
+    const int p = GID;
+    if (p >= S.pairCap) return;
+    const uint32_t total = S.pairOff[S.numBodies];
+    if (static_cast<uint32_t>(p) >= total) { S.pairValid[p] = 0; return; }
+    const int i = S.pairI[p], j = S.pairJ[p];
+    const v3 ci = ld3(S.aabbC[i]), hi = ld3(S.aabbH[i]), cj = ld3(S.aabbC[j]), hj = ld3(S.aabbH[j]);
+    const bool ov = aabbOverlap(ci - hi, ci + hi, cj - hj, cj + hj);
+    const bool layers = ((S.layer[i] & S.mask[j]) != 0) || ((S.layer[j] & S.mask[i]) != 0);
+    const bool bothStatic = bMass(S, i) <= 0.0f && bMass(S, j) <= 0.0f;
+    bool constrained = false;
+    if (ov && layers && !bothStatic) {
+        const int ni = S.adjStart[i + 1] - S.adjStart[i], nj = S.adjStart[j + 1] - S.adjStart[j];
+        const int a = ni <= nj ? i : j, b = ni <= nj ? j : i;
+        for (int e = S.adjStart[a]; e < S.adjStart[a + 1]; e++)
+            if (otherOf(S, S.adj[e], a) == b) { constrained = true; break; }
+    }
+    S.pairValid[p] = (ov && layers && !bothStatic && !constrained) ? 1u : 0u;
+}
+
+// solver_broadphase.cpp:153-184 + manifold.cpp:205-268: the r-th new pair of a world
+// takes the r-th slot popped from its free list (LIFO), else a fresh slot, and is
+// appended to the live list in sorted pair order.
+__global__ void k_alloc_new(State S)
+{
+    const int p = GID;
+    if (p >= S.pairCap) return;
+    if (!S.pairValid[p]) return;
+    const int i = S.pairI[p], j = S.pairJ[p];
+    const int w = S.bodyWorld[i];
+    const uint32_t start = S.pairOff[S.wBodyBase[w]];
+    const int r = static_cast<int>(S.pairValidScan[p] - S.pairValidScan[start]);
+    const int base = S.wSlotBase[w], cap = S.wSlotCap[w];
+    const int fr = S.wFree[w], lv = S.wLive[w], top = S.wTop[w];
+    int slot;
+    if (r < fr) {
+        slot = S.freeList[base + fr - 1 - r];
+    } else {
+        const int fresh = top + (r - fr);
+        if (fresh >= cap) { atomicOr(S.wErr + w, 1); return; }
+        slot = base + fresh;
+    }
+    if (lv + r >= cap) { atomicOr(S.wErr + w, 1); return; }
+    S.live[base + lv + r] = slot;
+    S.mA[slot] = i;
+    S.mB[slot] = j;
+    S.mNum[slot] = 0;
+    S.mFric[slot] = 0.0f;
+    S.mBasis[3 * slot] = make_float4(0, 0, 0, 0);
+    S.mBasis[3 * slot + 1] = make_float4(0, 0, 0, 0);
+    S.mBasis[3 * slot + 2] = make_float4(0, 0, 0, 0);
+}
+
+__global__ void k_alloc_counts(State S)
+{
+    const int w = GID;
+    if (w >= S.numWorlds) return;
+    const int cnt = S.wBodyCount[w];
+    int newCount = 0;
+    if (cnt > 0) {
+        const uint32_t a = S.pairOff[S.wBodyBase[w]], b = S.pairOff[S.wBodyBase[w] + cnt];
+        newCount = static_cast<int>(S.pairValidScan[b] - S.pairValidScan[a]);
+    }
+    const int fr = S.wFree[w];
+    const int used = newCount < fr ? newCount : fr;
+    int top = S.wTop[w] + (newCount - used);
+    int lv = S.wLive[w] + newCount;
+    if (top > S.wSlotCap[w]) top = S.wSlotCap[w];
+    if (lv > S.wSlotCap[w]) lv = S.wSlotCap[w];
+    S.wFree[w] = fr - used;
+    S.wTop[w] = top;
+    S.wLive[w] = lv;
+    S.wNew[w] = newCount;
+}
+
+// ------------------------------------------------------------ narrow phase ----
+// manifold.cpp:237-435 with collide.cpp:1100-1120: collide at the step-start pose,
+// carry hot state across matching feature keys, C0, release, warm-start decay.
+__global__ void k_manifold_init(State S)
+{
+    const int t = GID;
+    if (t >= S.slotCap) return;
+    const int w = S.slotWorld[t];
+    if (w < 0) { S.keep[t] = 0; return; }
+    const int base = S.wSlotBase[w];
+    if (t - base >= S.wLive[w]) { S.keep[t] = 0; return; }
+    const int slot = S.live[t];
+    const int a = S.mA[slot], b = S.mB[slot];
+    const float ma = bMass(S, a), mb = bMass(S, b);
+    if (!(ma > 0.0f) && !(mb > 0.0f)) { S.keep[t] = 1; return; }  // frozen: parked verbatim
+    const float fa = S.fric[a], fb = S.fric[b];
+    const float mu = sqrtf(fa * fb);
+    S.mFric[slot] = mu;
+    const v3 pa = bPos(S, a), pb = bPos(S, b);
+    const qt qa = bRot(S, a), qb = bRot(S, b);
+    const float4 sa = S.sizeR[a], sb = S.sizeR[b];
+    // speculative margin grown by the pair's predicted relative motion
+    float margin = S.P.specMargin;
+    if (margin > 0.0f) {
+        const float hz = S.P.collisionDt;
+        const float gA = ma > 0.0f ? S.grav[a] : 0.0f;
+        const float gB = mb > 0.0f ? S.grav[b] : 0.0f;
+        const v3 disp = (ld3(S.vel[a]) - ld3(S.vel[b])) * hz + v3{0, 0, (gA - gB) * hz * hz};
+        margin += length(disp) + (sa.w * length(ld3(S.angVel[a])) + sb.w * length(ld3(S.angVel[b]))) * hz;
+    }
+    m3 B;
+    bool wrote;
+    Contacts cs;
+    const int nn = collideBoxes(pa, qa, v3{sa.x, sa.y, sa.z} * 0.5f, pb, qb, v3{sb.x, sb.y, sb.z} * 0.5f, margin,
+                                B, wrote, cs);
+    if (wrote) {
+        S.mBasis[3 * slot] = st3(row(B, 0));
+        S.mBasis[3 * slot + 1] = st3(row(B, 1));
+        S.mBasis[3 * slot + 2] = st3(row(B, 2));
+    } else {
+        B = loadBasis(S, slot);
+    }
+    const int oldN = S.mNum[slot];
+    // old contact state
+    int oKey[MAXC];
+    v3 oRA[MAXC], oRB[MAXC], oPen[MAXC], oLam[MAXC];
+    float oBeta[MAXC], oC0x[MAXC];
+    uint8_t oStick[MAXC];
+    for (int j = 0; j < oldN; j++) {
+        const int c = slot * MAXC + j;
+        const float4 ra = S.cRA[c], rb = S.cRB[c], c0 = S.cC0[c];
+        oRA[j] = ld3(ra); oBeta[j] = ra.w;
+        oRB[j] = ld3(rb); oKey[j] = __float_as_int(rb.w);
+        oC0x[j] = c0.x; oStick[j] = c0.w != 0.0f ? 1 : 0;
+        oPen[j] = ld3(S.cPen[c]);
+        oLam[j] = ld3(S.cLam[c]);
+    }
+    const float pmax = S.P.penaltyMax;
+    for (int i = 0; i < nn; i++) {
+        v3 rA = cs.rA[i], rB = cs.rB[i];
+        v3 pen{0, 0, 0}, lam{0, 0, 0};
+        uint8_t stick = 0;
+        float beta = 1.0f, prevErr = -1.0f;
+        for (int j = 0; j < oldN; j++) {
+            if (cs.key[i] != oKey[j]) continue;
+            prevErr = smax(0.0f, COLLISION_MARGIN - oC0x[j]);
+            pen = oPen[j]; lam = oLam[j]; stick = oStick[j]; beta = oBeta[j];
+            if (oStick[j]) {
+                const v3 oldA = pa + qrot(qa, oRA[j]);
+                const v3 oldB = pb + qrot(qb, oRB[j]);
+                const v3 newA = pa + qrot(qa, rA);
+                const v3 newB = pb + qrot(qb, rB);
+                const float drift = smax(length(oldA - newA), length(oldB - newB));
+                if (drift > S.P.stickDriftHeal) {
+                    stick = 0;
+                    lam = v3{lam.x, 0.0f, 0.0f};
+                } else {
+                    rA = oRA[j]; rB = oRB[j];
+                }
+            }
+            break;
+        }
+        const v3 xA = pa + qrot(qa, rA);
+        const v3 xB = pb + qrot(qb, rB);
+        v3 c0 = mul(B, xA - xB) + v3{COLLISION_MARGIN, 0, 0};
+        if (c0.x < -S.P.maxPen) c0.x = -S.P.maxPen;
+        float scale = 1.0f;
+        if (S.P.adaptiveBeta && prevErr >= 0.0f) {
+            const float err = smax(0.0f, COLLISION_MARGIN - c0.x);
+            const float floor = S.P.adaptiveFloor;
+            scale = beta;
+            if (err <= floor) scale = smax(scale * 0.5f, 1.0f);
+            else if (prevErr <= floor) scale = smin(scale * 2.0f, ADAPTIVE_BETA_MAX);
+            else if (err > 0.5f * prevErr) scale = smin(scale * 2.0f, ADAPTIVE_BETA_MAX);
+            else if (err < 0.1f * prevErr) scale = smax(scale * 0.5f, 1.0f);
+        }
+        if (c0.x > COLLISION_MARGIN + CONTACT_RELEASE_GAP) {
+            lam = v3{0, 0, 0};
+            stick = 0;
+        }
+        lam = lam * S.P.alpha * S.P.gamma;
+        pen = lclamp3(pen * S.P.gamma, PENALTY_MIN, pmax);
+        const int c = slot * MAXC + i;
+        S.cRA[c] = st3(rA, scale);
+        S.cRB[c] = st3(rB, __int_as_float(cs.key[i]));
+        S.cC0[c] = st3(c0, stick ? 1.0f : 0.0f);
+        S.cPen[c] = st3(pen);
+        S.cLam[c] = st3(lam);
+    }
+    S.mNum[slot] = nn;
+    S.keep[t] = nn > 0 ? 1u : 0u;
+}
+
+// joint.cpp:62-78: C0 at the step-start pose, warm-start decay; a broken joint is reaped
+// (marked dead: it leaves the adjacency exactly as a deleted force leaves the list).
+__global__ void k_joint_init(State S)
+{
+    const int j = GID;
+    if (j >= S.jointCap) return;
+    if (S.jWorld[j] < 0 || S.jDead[j]) return;
+    if (S.jBroken[j]) { S.jDead[j] = 1; return; }
+    S.jC0L[j] = st3(jointLin(S, j));
+    S.jC0A[j] = st3(jointAng(S, j));
+    S.jLamL[j] = st3(ld3(S.jLamL[j]) * S.P.alpha * S.P.gamma);
+    S.jLamA[j] = st3(ld3(S.jLamA[j]) * S.P.alpha * S.P.gamma);
+    const float pmax = S.P.penaltyMax;
+    const v3 pl = ld3(S.jPenL[j]), pa = ld3(S.jPenA[j]);
+    const float sl = S.jStiffL[j], sa = S.jStiffA[j];
+    const float g = S.P.gamma;
+    S.jPenL[j] = st3(v3{smin(sclamp(pl.x * g, PENALTY_MIN, pmax), sl), smin(sclamp(pl.y * g, PENALTY_MIN, pmax), sl),
+                        smin(sclamp(pl.z * g, PENALTY_MIN, pmax), sl)});
+    S.jPenA[j] = st3(v3{smin(sclamp(pa.x * g, PENALTY_MIN, pmax), sa), smin(sclamp(pa.y * g, PENALTY_MIN, pmax), sa),
+                        smin(sclamp(pa.z * g, PENALTY_MIN, pmax), sa)});
+}
+
+// solver_solve.cpp:66-87: stable compaction of each world's live list, releasing the
+// slots of separated pairs to the free list.
+__global__ void k_release_flags(State S)
+{
+    const int t = GID;
+    if (t >= S.slotCap) return;
+    const int w = S.slotWorld[t];
+    S.relScan[t] = 0;
+    if (w < 0) return;
+    // relScan temporarily holds the "released" flag; scanned in place afterwards
+    S.relScan[t] = (t - S.wSlotBase[w] < S.wLive[w] && !S.keep[t]) ? 1u : 0u;
+}
+
+__global__ void k_compact(State S)
+{
+    const int t = GID;
+    if (t >= S.slotCap) return;
+    const int w = S.slotWorld[t];
+    if (w < 0) return;
+    const int base = S.wSlotBase[w];
+    if (t - base >= S.wLive[w]) return;
+    const int slot = S.live[t];
+    if (S.keep[t]) {
+        S.liveTmp[base + static_cast<int>(S.keepScan[t] - S.keepScan[base])] = slot;
+    } else {
+        S.freeList[base + S.wFree[w] + static_cast<int>(S.relScan[t] - S.relScan[base])] = slot;
+    }
+}
+
+__global__ void k_compact_counts(State S)
+{
+    const int w = GID;
+    if (w >= S.numWorlds) return;
+    const int base = S.wSlotBase[w], cap = S.wSlotCap[w];
+    if (cap <= 0) return;
+    S.wLive[w] = static_cast<int>(S.keepScan[base + cap] - S.keepScan[base]);
+    S.wFree[w] += static_cast<int>(S.relScan[base + cap] - S.relScan[base]);
+}
+
+__global__ void k_copy_live(State S)
+{
+    const int t = GID;
+    if (t >= S.slotCap) return;
+    const int w = S.slotWorld[t];
+    if (w < 0) return;
+    if (t - S.wSlotBase[w] < S.wLive[w]) S.live[t] = S.liveTmp[t];
+}
+
+// ------------------------------------------------------------ adjacency ----
+// manifold.cpp:123-166: CSR per body, joints in list order then manifolds in live order.
+// Counts and fills use atomics; each segment is then sorted by its sequence number.
+__global__ void k_adj_count(State S)
+{
+    const int t = GID;
+    if (t < S.jointCap) {
+        if (S.jWorld[t] >= 0 && !S.jDead[t]) {
+            if (S.jA[t] >= 0) atomicAdd(&S.adjCount[S.jA[t]], 1);
+            atomicAdd(&S.adjCount[S.jB[t]], 1);
+        }
+    }
+    if (t < S.slotCap) {
+        const int w = S.slotWorld[t];
+        if (w >= 0 && t - S.wSlotBase[w] < S.wLive[w]) {
+            const int slot = S.live[t];
+            atomicAdd(&S.adjCount[S.mA[slot]], 1);
+            atomicAdd(&S.adjCount[S.mB[slot]], 1);
+        }
+    }
+}
+
+__global__ void k_adj_fill(State S)
+{
+    const int t = GID;
+    if (t < S.jointCap) {
+        const int w = S.jWorld[t];
+        if (w >= 0 && !S.jDead[t]) {
+            const unsigned long long seq = static_cast<unsigned long long>(t - S.wJointBase[w]);
+            const unsigned long long e = (seq << 32) | static_cast<uint32_t>(-(t + 1));
+            if (S.jA[t] >= 0) {
+                const int a = S.jA[t];
+                S.adjTmp[S.adjStart[a] + atomicAdd(&S.adjCount[a], 1)] = e;
+            }
+            const int b = S.jB[t];
+            S.adjTmp[S.adjStart[b] + atomicAdd(&S.adjCount[b], 1)] = e;
+        }
+    }
+    if (t < S.slotCap) {
+        const int w = S.slotWorld[t];
+        if (w >= 0 && t - S.wSlotBase[w] < S.wLive[w]) {
+            const int slot = S.live[t];
+            const unsigned long long seq = 0x40000000ull + static_cast<unsigned long long>(t - S.wSlotBase[w]);
+            const unsigned long long e = (seq << 32) | static_cast<uint32_t>(slot);
+            const int a = S.mA[slot], b = S.mB[slot];
+            S.adjTmp[S.adjStart[a] + atomicAdd(&S.adjCount[a], 1)] = e;
+            S.adjTmp[S.adjStart[b] + atomicAdd(&S.adjCount[b], 1)] = e;
+        }
+    }
+}
+
+__global__ void k_adj_sort(State S)
+{
+    const int b = GID;
+    if (b >= S.numBodies) return;
+    const int e0 = S.adjStart[b], e1 = S.adjStart[b + 1];
+    const bool ordered = bMass(S, b) > 0.0f;  // only movable bodies sum in this order
+    if (ordered) {
+        for (int a = e0 + 1; a < e1; a++) {
+            const unsigned long long v = S.adjTmp[a];
+            int c = a;
+            while (c > e0 && S.adjTmp[c - 1] > v) { S.adjTmp[c] = S.adjTmp[c - 1]; c--; }
+            S.adjTmp[c] = v;
+        }
+    }
+    for (int a = e0; a < e1; a++) S.adj[a] = static_cast<int>(static_cast<uint32_t>(S.adjTmp[a] & 0xFFFFFFFFull));
+}
+
+// ------------------------------------------------------ update set, colours ----
+// solver_solve.cpp:296-358: contactOnly = no force touches the body; worlds with a
+// live joint (or with contact-only bodies kept out of the overflow) need colours.
+__global__ void k_colour_prepare(State S)
+{
+    const int i = GID;
+    if (i < S.numBodies) S.contactOnly[i] = 1;
+    if (i < S.numWorlds) S.wHasJoints[i] = S.P.contactOnlyOverflow ? 0 : 1;
+}
+__global__ void k_colour_joints(State S)
+{
+    const int j = GID;
+    if (j >= S.jointCap) return;
+    if (S.jWorld[j] < 0 || S.jDead[j]) return;
+    if (S.jA[j] >= 0) S.contactOnly[S.jA[j]] = 0;
+    S.contactOnly[S.jB[j]] = 0;
+    S.wHasJoints[S.jWorld[j]] = 1;
+}
+__global__ void k_colour_init(State S)
+{
+    const int i = GID;
+    if (i >= S.numBodies) return;
+    const int w = S.bodyWorld[i];
+    if (w < 0) return;
+    S.colour[i] = (S.wHasJoints[w] && S.updateIndex[i] >= 0) ? 0xFF : 8;
+}
+
+// Jones-Plassmann round with priority = lower updateIndex: a body takes the smallest
+// colour its lower-index neighbours do not use once all of them are coloured.  This is
+// exactly the reference's sequential greedy pass (solver_solve.cpp:365-397).
+__device__ __forceinline__ bool tryColour(const State &S, int i)
+{
+    const int ui = S.updateIndex[i];
+    uint32_t used = 0;
+    for (int e = S.adjStart[i]; e < S.adjStart[i + 1]; e++) {
+        const int o = otherOf(S, S.adj[e], i);
+        if (o < 0) continue;
+        const int uo = S.updateIndex[o];
+        if (uo < 0 || uo >= ui) continue;
+        const uint8_t c = __hip_atomic_load(&S.colour[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (c == 0xFF) return false;
+        if (c < 8) used |= 1u << c;
+    }
+    uint8_t c = 8;
+    for (int k = 0; k < 8; k++)
+        if (!(used & (1u << k))) { c = static_cast<uint8_t>(k); break; }
+    __hip_atomic_store(&S.colour[i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+__global__ void k_colour_round(State S)
+{
+    const int i = GID;
+    if (i >= S.numBodies) return;
+    if (S.colour[i] != 0xFF) return;
+    if (!tryColour(S, i)) atomicAdd(&S.wColourLeft[S.bodyWorld[i]], 1);
+}
+// Bodies left after the bounded rounds are finished in list order by one thread per world.
+__global__ void k_colour_finish(State S)
+{
+    const int w = GID;
+    if (w >= S.numWorlds) return;
+    if (!S.wHasJoints[w]) return;
+    const int base = S.wBodyBase[w], cnt = S.wBodyCount[w];
+    for (int k = 0; k < cnt; k++)
+        if (S.colour[base + k] == 0xFF) tryColour(S, base + k);
+}
+// solver_solve.cpp:399-418: contact-only bodies form the Jacobi bucket (colour 8).  A
+// body whose greedy colour overflowed but touches a joint lands in neither the colour runs
+// nor the bucket, so the reference never solves it; it is marked 9 here and skipped too.
+__global__ void k_colour_override(State S)
+{
+    const int i = GID;
+    if (i >= S.numBodies) return;
+    if (S.bodyWorld[i] < 0) return;
+    if (S.P.contactOnlyOverflow && S.contactOnly[i]) S.colour[i] = 8;
+    else if (S.colour[i] == 8) S.colour[i] = 9;
+}
+
+// ------------------------------------------------------------ velocity brake ----
+// solver_projection.cpp:96-167 walks the active manifolds in live order and changes body
+// velocities in place.  A manifold depends only on the earlier active manifolds that
+// share a movable body with it, so running it after exactly those (level by level) gives
+// the reference's result.  succ* link each manifold to the next active manifold of each
+// movable body (that body's adjacency is in live order).
+__device__ __forceinline__ bool bodyMoves(const State &S, int b) { return bMass(S, b) > 0.0f && S.updateIndex[b] >= 0; }
+
+__device__ __forceinline__ bool manifoldActive(const State &S, int slot)
+{
+    return S.mNum[slot] > 0 && (bodyMoves(S, S.mA[slot]) || bodyMoves(S, S.mB[slot]));
+}
+
+__global__ void k_slot_to_live(State S, int *slotToLive)
+{
+    const int t = GID;
+    if (t >= S.slotCap) return;
+    const int w = S.slotWorld[t];
+    if (w < 0) return;
+    if (t - S.wSlotBase[w] < S.wLive[w]) slotToLive[S.live[t]] = t;
+}
+
+__global__ void k_brake_links(State S, const int *slotToLive)
+{
+    const int t = GID;
+    if (t >= S.slotCap) return;
+    const int w = S.slotWorld[t];
+    if (w < 0) return;
+    if (t - S.wSlotBase[w] >= S.wLive[w]) return;
+    const int slot = S.live[t];
+    S.succA[t] = -1;
+    S.succB[t] = -1;
+    S.indeg[t] = 0;
+    if (!manifoldActive(S, slot)) { S.indeg[t] = -1; return; }
+    int deg = 0;
+    for (int side = 0; side < 2; side++) {
+        const int body = side == 0 ? S.mA[slot] : S.mB[slot];
+        if (!bodyMoves(S, body)) continue;
+        const int e0 = S.adjStart[body], e1 = S.adjStart[body + 1];
+        int me = e0;
+        while (me < e1 && S.adj[me] != slot) me++;
+        if (me == e1) continue;
+        for (int e = me - 1; e >= e0; e--) {
+            const int r = S.adj[e];
+            if (r < 0) break;
+            if (manifoldActive(S, r)) { deg++; break; }
+        }
+        for (int e = me + 1; e < e1; e++) {
+            const int r = S.adj[e];
+            if (r >= 0 && manifoldActive(S, r)) {
+                if (side == 0) S.succA[t] = slotToLive[r];
+                else S.succB[t] = slotToLive[r];
+                break;
+            }
+        }
+    }
+    S.indeg[t] = deg;
+    atomicAdd(&S.wUpdateCount[w], 1);  // active manifolds of the world
+}
+
+__device__ void brakeManifold(const State &S, int slot, float window)
+{
+    const int a = S.mA[slot], b = S.mB[slot];
+    const bool moveA = bodyMoves(S, a), moveB = bodyMoves(S, b);
+    const float ma = bMass(S, a), mb = bMass(S, b);
+    const float imA = moveA ? 1.0f / ma : 0.0f, imB = moveB ? 1.0f / mb : 0.0f;
+    const v3 n = ld3(S.mBasis[3 * slot]);
+    const v3 pa = bPos(S, a), pb = bPos(S, b);
+    const qt qa = bRot(S, a), qb = bRot(S, b);
+    const v3 va = ld3(S.vel[a]), wa = ld3(S.angVel[a]), vb = ld3(S.vel[b]), wb = ld3(S.angVel[b]);
+    v3 pRA{0, 0, 0}, pRB{0, 0, 0};
+    float verr = 0.0f;
+    int braked = 0;
+    const int nc = S.mNum[slot];
+    for (int k = 0; k < nc; k++) {
+        const int c = slot * MAXC + k;
+        const v3 rA = qrot(qa, ld3(S.cRA[c]));
+        const v3 rB = qrot(qb, ld3(S.cRB[c]));
+        const float gap = dot(n, (pa + rA) - (pb + rB));
+        if (!(gap > 0.0f)) continue;
+        const v3 vAp = va + cross(wa, rA), vBp = vb + cross(wb, rB);
+        const float vn = dot(vAp - vBp, n);
+        const float target = -gap / window;
+        if (vn < target) {
+            pRA += rA;
+            pRB += rB;
+            verr += target - vn;
+            braked++;
+        }
+    }
+    if (braked == 0) return;
+    const float invN = 1.0f / static_cast<float>(braked);
+    const v3 IA = bMoment(S, a), IB = bMoment(S, b);
+    const v3 tA = moveA ? qrot(qconj(qa), cross(pRA * invN, n)) : v3{0, 0, 0};
+    const v3 tB = moveB ? qrot(qconj(qb), cross(pRB * invN, n)) : v3{0, 0, 0};
+    const float K = imA + imB + (moveA ? angCompliance(IA, tA) : 0.0f) + (moveB ? angCompliance(IB, tB) : 0.0f);
+    const float j = verr * invN / K;
+    if (moveA) {
+        v3 dv = n * (imA * j), dw = angResponse(IA, qa, tA) * j;
+        maskLocked(S.lockLin[a], dv);
+        maskLocked(S.lockAng[a], dw);
+        S.vel[a] = st3(va + dv);
+        S.angVel[a] = st3(wa + dw);
+    }
+    if (moveB) {
+        v3 dv = n * (imB * j), dw = angResponse(IB, qb, tB) * j;
+        maskLocked(S.lockLin[b], dv);
+        maskLocked(S.lockAng[b], dw);
+        S.vel[b] = st3(vb - dv);
+        S.angVel[b] = st3(wb - dw);
+    }
+}
+
+// One workgroup per world: Kahn's algorithm by levels over the dependency links,
+// braking each level's manifolds in parallel; the level order is kept for the replay.
+// levelStart[base + l] is the first entry of level l; wLevelEnd[w] ends the last level.
+constexpr int BRAKE_TPB = 1024;
+__global__ void __launch_bounds__(BRAKE_TPB) k_brake_levels(State S, float window, int doBrake)
+{
+    const int w = blockIdx.x;
+    const int base = S.wSlotBase[w];
+    const int nLive = S.wLive[w];
+    __shared__ int tail, head, levels;
+    if (threadIdx.x == 0) { tail = 0; head = 0; levels = 0; }
+    __syncthreads();
+    if (nLive <= 0 || S.wSlotCap[w] <= 0) {
+        if (threadIdx.x == 0) { S.brakeLevels[w] = 0; S.wLevelEnd[w] = 0; }
+        return;
+    }
+    for (int k = threadIdx.x; k < nLive; k += BRAKE_TPB)
+        if (S.indeg[base + k] == 0) S.levelOrder[base + atomicAdd(&tail, 1)] = base + k;
+    __syncthreads();
+    while (true) {
+        const int h = head, e = tail;
+        if (h == e) break;
+        if (threadIdx.x == 0) S.levelStart[base + levels] = h;
+        for (int k = h + threadIdx.x; k < e; k += BRAKE_TPB) {
+            const int t = S.levelOrder[base + k];
+            if (doBrake) brakeManifold(S, S.live[t], window);
+            const int sa = S.succA[t], sb = S.succB[t];
+            if (sa >= 0 && atomicSub(&S.indeg[sa], 1) == 1) S.levelOrder[base + atomicAdd(&tail, 1)] = sa;
+            if (sb >= 0 && atomicSub(&S.indeg[sb], 1) == 1) S.levelOrder[base + atomicAdd(&tail, 1)] = sb;
+        }
+        __threadfence_block();
+        __syncthreads();
+        if (threadIdx.x == 0) { head = e; levels++; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        S.brakeLevels[w] = levels;
+        S.wLevelEnd[w] = tail;
+    }
+}
+
+__global__ void __launch_bounds__(BRAKE_TPB) k_brake_replay(State S, float window)
+{
+    const int w = blockIdx.x;
+    const int base = S.wSlotBase[w];
+    const int L = S.brakeLevels[w];
+    for (int l = 0; l < L; l++) {
+        const int h = S.levelStart[base + l];
+        const int e = l + 1 < L ? S.levelStart[base + l + 1] : S.wLevelEnd[w];
+        for (int k = h + threadIdx.x; k < e; k += BRAKE_TPB) brakeManifold(S, S.live[S.levelOrder[base + k]], window);
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ body phases ----
+// solver_solve.cpp:109-142
+__global__ void k_warmstart(State S)
+{
+    const int i = GID;
+    if (i >= S.numBodies || S.bodyWorld[i] < 0) return;
+    const float dt = S.P.dt;
+    v3 vel = ld3(S.vel[i]), ang = ld3(S.angVel[i]);
+    const uint8_t ll = S.lockLin[i];
+    maskLocked(ll, vel);
+    maskLocked(S.lockAng[i], ang);
+    S.vel[i] = st3(vel);
+    S.angVel[i] = st3(ang);
+    const float g = (ll & 0x4) ? 0.0f : S.grav[i];
+    const float m = bMass(S, i);
+    const v3 p = bPos(S, i);
+    const qt q = bRot(S, i);
+    v3 inl = p + vel * dt;
+    if (m > 0) inl += v3{0, 0, g} * (dt * dt);
+    S.inertLin[i] = st3(inl);
+    S.inertAng[i] = stq(qadd(q, ang * dt));
+    const v3 accel = (vel - ld3(S.prevVel[i])) / dt;
+    const float accelExt = accel.z * __builtin_copysignf(1.0f, g);
+    float wgt = sclamp(accelExt / fabsf(g), 0.0f, 1.0f);
+    if (!__builtin_isfinite(wgt)) wgt = 0.0f;
+    S.initLin[i] = st3(p);
+    S.initAng[i] = stq(q);
+    if (m > 0) {
+        S.pos[i] = st3((p + vel * dt) + v3{0, 0, g} * (wgt * dt * dt));
+        S.rot[i] = stq(qadd(q, ang * dt));
+    }
+}
+
+// solver_solve.cpp:146-176 (sleep off)
+__global__ void k_finish(State S)
+{
+    const int i = GID;
+    if (i >= S.numBodies || S.bodyWorld[i] < 0) return;
+    const float dt = S.P.dt;
+    v3 vel = ld3(S.vel[i]), ang = ld3(S.angVel[i]);
+    S.prevVel[i] = st3(vel);
+    if (bMass(S, i) > 0) {
+        vel = (bPos(S, i) - ld3(S.initLin[i])) / dt;
+        ang = qsub(bRot(S, i), ldq(S.initAng[i])) / dt;
+    }
+    maskLocked(S.lockLin[i], vel);
+    maskLocked(S.lockAng[i], ang);
+    S.vel[i] = st3(vel);
+    S.angVel[i] = st3(ang);
+}
+
+// solver_step.cpp:209-229: one colour of the Gauss-Seidel sweep.
+__global__ void k_colour_pass(State S, int colour)
+{
+    const int i = GID;
+    if (i >= S.numBodies || S.colour[i] != colour || S.bodyWorld[i] < 0) return;
+    if (S.updateIndex[i] < 0) return;
+    Block blk;
+    assemble(S, i, blk);
+    v3 dl, da;
+    solveDelta(blk, dl, da);
+    S.pos[i] = st3(bPos(S, i) + dl);
+    S.rot[i] = stq(qadd(bRot(S, i), da));
+}
+
+// solver_step.cpp:248-264: overflow bodies solve against one pose snapshot.
+__global__ void k_overflow_solve(State S)
+{
+    const int i = GID;
+    if (i >= S.numBodies || S.colour[i] != 8 || S.bodyWorld[i] < 0) return;
+    if (S.updateIndex[i] < 0) return;
+    Block blk;
+    assemble(S, i, blk);
+    v3 dl, da;
+    solveDelta(blk, dl, da);
+    S.dLin[i] = st3(dl);
+    S.dAng[i] = st3(da);
+}
+
+// solver_step.cpp:268-273
+__global__ void k_overflow_apply(State S)
+{
+    const int i = GID;
+    if (i >= S.numBodies || S.colour[i] != 8 || S.bodyWorld[i] < 0) return;
+    if (S.updateIndex[i] < 0) return;
+    const float r = S.P.jacobiRelax;
+    S.pos[i] = st3(bPos(S, i) + ld3(S.dLin[i]) * r);
+    S.rot[i] = stq(qadd(bRot(S, i), ld3(S.dAng[i]) * r));
+}
+
+// solver_step.cpp:277-296: manifolds (not frozen), then joints.
+__global__ void k_dual(State S, float decay)
+{
+    const int t = GID;
+    if (t < S.slotCap) {
+        const int w = S.slotWorld[t];
+        if (w >= 0 && t - S.wSlotBase[w] < S.wLive[w]) {
+            const int slot = S.live[t];
+            if (bMass(S, S.mA[slot]) > 0.0f || bMass(S, S.mB[slot]) > 0.0f) manifoldDual(S, slot, decay);
+        }
+    }
+    if (t < S.jointCap) {
+        if (S.jWorld[t] >= 0 && !S.jDead[t]) jointDual(S, t);
+    }
+}
+
+// ------------------------------------------------------------ projection ----
+__global__ void k_proj_reset(State S)
+{
+    const int w = GID;
+    if (w >= S.numWorlds) return;
+    S.wMaxDepth[w] = 0u;
+    S.wProjDone[w] = (S.wUpdateCount[w] > 0) ? 0 : 1;  // no active manifold: no pass
+}
+
+// solver_projection.cpp:642-709: per active manifold, patch depth beyond the slop and the
+// per-side corrections (applied by the body gather below).
+__global__ void k_proj_manifold(State S)
+{
+    const int t = GID;
+    if (t >= S.slotCap) return;
+    const int w = S.slotWorld[t];
+    if (w < 0) return;
+    if (t - S.wSlotBase[w] >= S.wLive[w]) return;
+    if (S.wProjDone[w]) return;
+    S.projOn[t] = 0;
+    const int slot = S.live[t];
+    if (!manifoldActive(S, slot)) return;
+    const int a = S.mA[slot], b = S.mB[slot];
+    const bool moveA = bodyMoves(S, a), moveB = bodyMoves(S, b);
+    const float imA = moveA ? 1.0f / bMass(S, a) : 0.0f, imB = moveB ? 1.0f / bMass(S, b) : 0.0f;
+    const v3 n = ld3(S.mBasis[3 * slot]);
+    const v3 pa = bPos(S, a), pb = bPos(S, b);
+    const qt qa = bRot(S, a), qb = bRot(S, b);
+    const m3 rotA = qmat(qa), rotB = qmat(qb);
+    v3 pRA{0, 0, 0}, pRB{0, 0, 0};
+    float depthSum = 0.0f;
+    int corr = 0;
+    const int nc = S.mNum[slot];
+    for (int k = 0; k < nc; k++) {
+        const int c = slot * MAXC + k;
+        const v3 rA = mul(rotA, ld3(S.cRA[c])), rB = mul(rotB, ld3(S.cRB[c]));
+        const float raw = -dot(n, (pa + rA) - (pb + rB));
+        const float depth = raw - S.P.projSlop;
+        if (depth > 0.0f) {
+            pRA += rA;
+            pRB += rB;
+            depthSum += depth;
+            corr++;
+        }
+    }
+    if (corr == 0) return;
+    const float invN = 1.0f / static_cast<float>(corr);
+    const float avg = depthSum * invN;
+    if (avg > 0.0f) atomicMax(&S.wMaxDepth[w], __float_as_uint(avg));
+    const v3 IA = bMoment(S, a), IB = bMoment(S, b);
+    const v3 tA = moveA ? qrot(qconj(qa), cross(pRA * invN, n)) : v3{0, 0, 0};
+    const v3 tB = moveB ? qrot(qconj(qb), cross(pRB * invN, n)) : v3{0, 0, 0};
+    const float K = imA + imB + (moveA ? angCompliance(IA, tA) : 0.0f) + (moveB ? angCompliance(IB, tB) : 0.0f);
+    const float dp = S.P.projBeta * depthSum * invN / K;
+    if (moveA) {
+        S.projLA[t] = st3(n * (imA * dp));
+        S.projAA[t] = st3(angResponse(IA, qa, tA) * dp);
+    }
+    if (moveB) {
+        S.projLB[t] = st3(n * (imB * dp));
+        S.projAB[t] = st3(angResponse(IB, qb, tB) * dp);
+    }
+    S.projOn[t] = 1;
+}
+
+// solver_projection.cpp:711-730: each movable body averages its corrections, summed in
+// live order, and moves; every body of a world with active manifolds is renormalised.
+__global__ void k_proj_body(State S, const int *slotToLive)
+{
+    const int i = GID;
+    if (i >= S.numBodies) return;
+    const int w = S.bodyWorld[i];
+    if (w < 0 || S.updateIndex[i] < 0) return;
+    if (S.wProjDone[w]) return;
+    v3 sl{0, 0, 0}, sa{0, 0, 0};
+    int cnt = 0;
+    for (int e = S.adjStart[i]; e < S.adjStart[i + 1]; e++) {
+        const int ref = S.adj[e];
+        if (ref < 0) continue;
+        const int t = slotToLive[ref];
+        if (!S.projOn[t]) continue;
+        cnt++;
+        if (S.mA[ref] == i) {
+            sl += ld3(S.projLA[t]);
+            sa += ld3(S.projAA[t]);
+        } else {
+            sl -= ld3(S.projLB[t]);
+            sa -= ld3(S.projAB[t]);
+        }
+    }
+    const float inv = 1.0f / static_cast<float>(cnt > 1 ? cnt : 1);
+    v3 dl = sl * inv, da = sa * inv;
+    maskLocked(S.lockLin[i], dl);
+    maskLocked(S.lockAng[i], da);
+    const float cap = S.P.projMaxCorr;
+    if (cap > 0.0f) {
+        const float moved = length(dl);
+        if (moved > cap) dl = dl * (cap / moved);
+        const float turned = length(da);
+        if (turned > cap) da = da * (cap / turned);
+    }
+    S.pos[i] = st3(bPos(S, i) + dl);
+    S.rot[i] = stq(qadd(bRot(S, i), da));
+}
+
+__global__ void k_proj_flags(State S)
+{
+    const int w = GID;
+    if (w >= S.numWorlds) return;
+    if (S.wProjDone[w]) return;
+    const float tol = smax(S.P.projSlop * 0.01f, 1.0e-6f);
+    if (__uint_as_float(S.wMaxDepth[w]) <= tol) S.wProjDone[w] = 1;
+    S.wMaxDepth[w] = 0u;
+}
+
+// solver_step.cpp:164-170
+__global__ void k_rebase(State S)
+{
+    const int t = GID;
+    if (t >= S.slotCap) return;
+    const int w = S.slotWorld[t];
+    if (w < 0) return;
+    if (t - S.wSlotBase[w] >= S.wLive[w]) return;
+    manifoldRebase(S, S.live[t]);
+}
+
+} // namespace avbd_dev

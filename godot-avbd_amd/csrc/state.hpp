@@ -1,0 +1,100 @@
+// Device-resident state of a batch of worlds, passed by value to every kernel.
+//
+// HBM layout (structure of arrays, 16-byte vectors so one thread issues one dwordx4 per
+// field):
+//   bodies    float4 pos/rot/init*/inertial*/vel/angVel/prevVel, float4 size+radius,
+//             float4 mass+moment; index = world body base + reference list position
+//   joints    float4 anchors, C0, penalty, lambda (lin and ang); index = world joint base
+//             + reference force-list position
+//   manifolds stable slots in per-world ranges; per slot: bodies, 3 float4 basis rows,
+//             friction, count; per contact (slot*8 + k): float4 rA|betaScale,
+//             rB|featureKey, C0|stick, penalty, lambda
+//   live      per-world slot lists in creation order (the walk order of every
+//             order-dependent pass)
+//   adjacency CSR per body: joints in list order, then manifolds in live order
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace avbd_dev {
+
+struct Params {
+    float dt, dtSq, collisionDt, alpha, oneMinusAlpha, betaLin, betaAng, gamma;
+    float jacobiRelax, stickDriftHeal, maxPen, specMargin;
+    float projBeta, projSlop, projMaxCorr, penaltyMax, adaptiveFloor, stiffnessDecay;
+    int adaptiveBeta, contactOnlyOverflow, projIters;
+};
+
+struct State {
+    Params P;
+    int numWorlds;
+    int numBodies;   // total over worlds (capacity in use)
+    int slotCap;     // total manifold slot capacity
+    int jointCap;
+    int pairCap;
+
+    // ---- bodies
+    float4 *pos, *rot, *initLin, *initAng, *inertLin, *inertAng, *vel, *angVel, *prevVel;
+    float4 *sizeR, *massI;
+    float *fric, *grav;
+    uint32_t *layer, *mask;
+    uint8_t *lockLin, *lockAng;
+    int *bodyWorld;
+    int *updateIndex;  // rank among movable bodies of the world (list order), -1 static
+    uint8_t *colour;   // 0..7, 8 = Jacobi overflow, 0xFF = not yet coloured
+    uint8_t *contactOnly;
+    float4 *dLin, *dAng;
+    float4 *aabbC, *aabbH;
+    int *adjCount, *adjStart, *adj;
+    unsigned long long *adjTmp;
+
+    // ---- joints
+    int *jA, *jB, *jWorld;
+    float4 *jRA, *jRB, *jC0L, *jC0A, *jPenL, *jPenA, *jLamL, *jLamA;
+    float *jStiffL, *jStiffA, *jFrac, *jArm;
+    uint8_t *jBroken, *jDead;
+
+    // ---- manifold slots
+    int *mA, *mB;
+    float4 *mBasis;
+    float *mFric;
+    int *mNum;
+    float4 *cRA, *cRB, *cC0, *cPen, *cLam;
+    int *slotWorld;     // static: world owning slot-range position t
+    int *live;          // per world range: slots in live order
+    int *liveTmp;
+    int *freeList;      // per world range
+    uint32_t *keep;     // per slot-range position flag / scan input
+    uint32_t *keepScan;
+    uint32_t *relScan;
+    // per-manifold brake schedule (indexed by slot-range position of the live entry)
+    int *succA, *succB, *indeg, *levelOrder;
+    int *brakeLevels;   // per world: number of levels
+    int *levelStart;    // per world slot range: first entry of each level
+    int *wLevelEnd;     // per world: end of the last level
+    float4 *projLA, *projAA, *projLB, *projAB; // per live position: per-side corrections
+    uint8_t *projOn;    // per live position: manifold corrected this pass
+
+    // ---- per-world tables
+    int *wBodyBase, *wBodyCount, *wJointBase, *wJointCount, *wSlotBase, *wSlotCap;
+    int *wLive, *wFree, *wTop, *wNew, *wHasJoints, *wUpdateCount;
+    int *wErr;
+    int *wProjDone;
+    unsigned int *wMaxDepth;   // float bits (non-negative) for atomicMax
+    int *wColourLeft;
+
+    // ---- broad phase
+    unsigned long long *mortonCode, *sortKey, *sortKeyAlt;
+    int *sortVal, *sortValAlt;
+    int *nodeLeft, *nodeRight, *nodeParent, *nodeFirst, *nodeLast, *nodeFlag;
+    float4 *nodeMin, *nodeMax;   // internal nodes [0, n-1), leaves [n-1, 2n-1)
+    int *leafBody;               // sorted position -> body
+    int *bodyLeaf;               // body -> sorted position
+    uint32_t *pairCount, *pairOff;
+    int *pairI, *pairJ;
+    uint32_t *pairValid, *pairValidScan;
+    float4 *wBoundsLo, *wBoundsHi;
+};
+
+} // namespace avbd_dev

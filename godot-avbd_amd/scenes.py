@@ -1,0 +1,215 @@
+"""Scene builders: body and joint arrays in the reference's list order.
+
+A scene is built in creation order (the order a reference user calls `new Rigid` /
+`new Joint`) and then reversed, because the reference prepends each new body and force to
+its intrusive list (src/avbd/rigid.cpp:28-30, force.cpp:19-21): list index 0 is the body
+created last.  Mass, inertia and bounding radius follow the reference's box constructor
+(src/avbd/rigid.cpp:59-70) in float32.
+
+Builders:
+  digest_stack        ground + 10 cubes (test/core_test.cpp:391-399)
+  digest_mixed        30-link Joint(1000, 250) chain + 40 cubes (test/core_test.cpp:612-631)
+  pyramid             C1: 28-row triangular stack of 406 unit cubes on a 100x1x100 ground
+  pile                C2: N boxes, U[0.25,1] half extents, log-normal density, random
+                      orientation, dropped from a jittered 3-D grid
+  chains              C3: chains of unit boxes linked by ball joints hanging from anchors
+  batch_piles         C4: many independent piles with log-uniform body counts
+"""
+import math
+
+import numpy as np
+
+F = np.float32
+
+
+class SceneBuilder:
+    def __init__(self, gravity=-10.0):
+        self.gravity = F(gravity)
+        self.b = []   # dicts in creation order
+        self.j = []
+
+    def box(self, size, density, friction, pos, rot=(0.0, 0.0, 0.0, 1.0), vel=(0.0, 0.0, 0.0)):
+        size = np.asarray(size, dtype=F)
+        d = F(density)
+        m = size[0] * size[1] * size[2] * d
+        moment = np.array([
+            (size[1] * size[1] + size[2] * size[2]) / F(12.0) * m,
+            (size[0] * size[0] + size[2] * size[2]) / F(12.0) * m,
+            (size[0] * size[0] + size[1] * size[1]) / F(12.0) * m], dtype=F)
+        h = size * F(0.5)
+        radius = np.sqrt(((F(0.0) + h[0] * h[0]) + h[1] * h[1]) + h[2] * h[2])
+        self.b.append(dict(size=size, mass=F(m), moment=moment, friction=F(friction), radius=F(radius),
+                           gravity=self.gravity, position=np.asarray(pos, dtype=F),
+                           rotation=np.asarray(rot, dtype=F), velocity=np.asarray(vel, dtype=F)))
+        return len(self.b) - 1
+
+    def joint(self, a, b, r_a, r_b, k_lin=math.inf, k_ang=0.0, fracture=math.inf):
+        sa = self.b[a]["size"] if a is not None and a >= 0 else np.zeros(3, F)
+        s = sa + self.b[b]["size"]
+        arm = ((F(0.0) + s[0] * s[0]) + s[1] * s[1]) + s[2] * s[2]
+        self.j.append(dict(a=-1 if a is None else a, b=b, r_a=np.asarray(r_a, F), r_b=np.asarray(r_b, F),
+                           k_lin=F(k_lin), k_ang=F(k_ang), fracture=F(fracture), arm=F(arm)))
+        return len(self.j) - 1
+
+    def build(self):
+        """(bodies, joints) arrays in list order."""
+        n = len(self.b)
+        order = list(range(n - 1, -1, -1))
+        pos_of = {c: n - 1 - c for c in range(n)}
+        bodies = {
+            "size": np.stack([self.b[c]["size"] for c in order]) if n else np.zeros((0, 3), F),
+            "mass": np.array([self.b[c]["mass"] for c in order], F),
+            "moment": np.stack([self.b[c]["moment"] for c in order]) if n else np.zeros((0, 3), F),
+            "friction": np.array([self.b[c]["friction"] for c in order], F),
+            "radius": np.array([self.b[c]["radius"] for c in order], F),
+            "gravity": np.array([self.b[c]["gravity"] for c in order], F),
+            "position": np.stack([self.b[c]["position"] for c in order]) if n else np.zeros((0, 3), F),
+            "rotation": np.stack([self.b[c]["rotation"] for c in order]) if n else np.zeros((0, 4), F),
+            "velocity": np.stack([self.b[c]["velocity"] for c in order]) if n else np.zeros((0, 3), F),
+            "angular_velocity": np.zeros((n, 3), F),
+        }
+        bodies["prev_velocity"] = bodies["velocity"].copy()
+        nj = len(self.j)
+        jorder = list(range(nj - 1, -1, -1))
+        joints = None
+        if nj:
+            joints = {
+                "body_a": np.array([pos_of[self.j[c]["a"]] if self.j[c]["a"] >= 0 else -1 for c in jorder], np.int32),
+                "body_b": np.array([pos_of[self.j[c]["b"]] for c in jorder], np.int32),
+                "r_a": np.stack([self.j[c]["r_a"] for c in jorder]),
+                "r_b": np.stack([self.j[c]["r_b"] for c in jorder]),
+                "stiffness_lin": np.array([self.j[c]["k_lin"] for c in jorder], F),
+                "stiffness_ang": np.array([self.j[c]["k_ang"] for c in jorder], F),
+                "fracture": np.array([self.j[c]["fracture"] for c in jorder], F),
+                "torque_arm": np.array([self.j[c]["arm"] for c in jorder], F),
+            }
+            for k in ("c0_lin", "c0_ang", "penalty_lin", "penalty_ang", "lambda_lin", "lambda_ang"):
+                joints[k] = np.zeros((nj, 3), F)
+            joints["broken"] = np.zeros(nj, np.uint8)
+        return bodies, joints
+
+
+def digest_stack():
+    s = SceneBuilder()
+    s.box((100, 100, 1), 0.0, 0.5, (0, 0, 0))
+    for i in range(10):
+        s.box((1, 1, 1), 1.0, 0.5, (0, 0, F(i) * F(1.5) + F(1.0)))
+    return s.build()
+
+
+def digest_mixed():
+    s = SceneBuilder()
+    s.box((100, 100, 1), 0.0, 0.5, (0, 0, 0))
+    prev = None
+    for i in range(30):
+        link = s.box((1, 1, 1), 1.0, 0.5, (0, 0, F(20.0) - F(i) * F(1.5)))
+        if prev is not None:
+            s.joint(prev, link, (0, 0, -0.5), (0, 0, 0.5), 1000.0, 250.0)
+        prev = link
+    for i in range(40):
+        s.box((1, 1, 1), 1.0, 0.5, (F(5.0) + F(i) * F(1.2), 0, F(3.0) + F(i % 5)))
+    return s.build()
+
+
+def hard_chain():
+    """test/core_test.cpp:271-300: 4 links on infinitely stiff ball joints under an anchor."""
+    s = SceneBuilder()
+    anchor = s.box((1, 1, 1), 0.0, 0.5, (0, 0, 10))
+    prev = anchor
+    for i in range(4):
+        link = s.box((1, 1, 1), 1.0, 0.5, (0, 0, F(9.0) - F(i)))
+        s.joint(prev, link, (0, 0, -0.5), (0, 0, 0.5), math.inf, 0.0)
+        prev = link
+    return s.build()
+
+
+def _shoemake(rng, n):
+    u = rng.random((n, 3))
+    q = np.stack([np.sqrt(1 - u[:, 0]) * np.sin(2 * np.pi * u[:, 1]),
+                  np.sqrt(1 - u[:, 0]) * np.cos(2 * np.pi * u[:, 1]),
+                  np.sqrt(u[:, 0]) * np.sin(2 * np.pi * u[:, 2]),
+                  np.sqrt(u[:, 0]) * np.cos(2 * np.pi * u[:, 2])], axis=1)
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    return q.astype(F)
+
+
+def pyramid(rows=28, seed=1):
+    """C1: `rows`-row triangular stack (406 unit cubes for 28 rows) on a 100x1x100 ground."""
+    rng = np.random.default_rng(seed)
+    s = SceneBuilder()
+    s.box((100, 100, 1), 0.0, 0.5, (0, 0, -0.5))
+    for r in range(rows):
+        for i in range(rows - r):
+            jx, jy = (rng.random(2) * 2.0 - 1.0) * 1e-3
+            x = (i - (rows - 1 - r) / 2.0) * 1.002 + jx
+            s.box((1, 1, 1), 1.0, 0.5, (x, jy, 0.5 + r))
+    return s.build()
+
+
+def pile(n=8192, seed=2, grid=None, pitch=4.0, ground=400.0):
+    """C2: n boxes with half extents U[0.25, 1], log-normal density (sigma 0.7, clipped to
+    2 sigma), uniform random orientation, on a jittered 3-D grid (lowest centre z = 2)."""
+    rng = np.random.default_rng(seed)
+    if grid is None:
+        nx = max(1, int(round((n / 2) ** (1.0 / 3.0))))
+        ny = nx
+        nz = int(math.ceil(n / (nx * ny)))
+        grid = (nx, ny, nz)
+    nx, ny, nz = grid
+    s = SceneBuilder()
+    s.box((ground, ground, 1), 0.0, 0.5, (0, 0, -0.5))
+    half = rng.uniform(0.25, 1.0, size=(n, 3))
+    dens = np.exp(np.clip(rng.normal(0.0, 0.7, size=n), -1.4, 1.4))
+    quats = _shoemake(rng, n)
+    jit = rng.uniform(-0.25, 0.25, size=(n, 3))
+    k = 0
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                if k >= n:
+                    break
+                p = ((x - (nx - 1) / 2.0) * pitch + jit[k, 0], (y - (ny - 1) / 2.0) * pitch + jit[k, 1],
+                     2.0 + z * pitch + jit[k, 2])
+                s.box(2.0 * half[k], dens[k], 0.5, p, rot=quats[k])
+                k += 1
+    return s.build()
+
+
+def chains(n_chains=4096, links=32, seed=3, grid=None, pitch_y=1.5, pitch_x=34.0, anchor_z=30.0):
+    """C3: chains of unit boxes along +x linked by ball joints (k_lin = inf, k_ang = 0),
+    each hanging from a static anchor, with a ground box below."""
+    rng = np.random.default_rng(seed)
+    if grid is None:
+        gx = max(1, int(round(math.sqrt(n_chains))))
+        gy = int(math.ceil(n_chains / gx))
+        grid = (gx, gy)
+    gx, gy = grid
+    s = SceneBuilder()
+    span_x = gx * pitch_x + 2 * links
+    span_y = gy * pitch_y + 10
+    s.box((span_x, span_y, 1), 0.0, 0.5, ((gx - 1) * pitch_x / 2.0, (gy - 1) * pitch_y / 2.0, -0.5))
+    c = 0
+    for iy in range(gy):
+        for ix in range(gx):
+            if c >= n_chains:
+                break
+            x0, y0 = ix * pitch_x, iy * pitch_y
+            prev = s.box((1, 1, 1), 0.0, 0.5, (x0, y0, anchor_z))
+            for k in range(links):
+                jx, jy, jz = rng.uniform(-1e-3, 1e-3, size=3)
+                link = s.box((1, 1, 1), 1.0, 0.5, (x0 + 1.0 + k + jx, y0 + jy, anchor_z + jz))
+                s.joint(prev, link, (0.5, 0, 0), (-0.5, 0, 0))
+                prev = link
+            c += 1
+    return s.build()
+
+
+def batch_sizes(worlds=4096, lo=8, hi=2048, seed=4):
+    rng = np.random.default_rng(seed)
+    return np.rint(np.exp(rng.uniform(math.log(lo), math.log(hi), size=worlds))).astype(int)
+
+
+def batch_piles(worlds=4096, lo=8, hi=2048, seed=4):
+    """C4: independent C2-style piles with log-uniform body counts; world w uses seed 4000+w."""
+    sizes = batch_sizes(worlds, lo, hi, seed)
+    return [pile(int(n), seed=4000 + w, pitch=4.0, ground=max(40.0, 8.0 * math.sqrt(n))) for w, n in enumerate(sizes)]

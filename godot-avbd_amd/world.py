@@ -1,0 +1,171 @@
+"""Python handles over the C ABI: a plain single world and a batch of worlds.
+
+These are thin: every call goes straight to libavbd_hip.so, which owns the device state.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+class World:
+    """One world, stepped by the plain single-world path (avbd_world_*)."""
+
+    def __init__(self, params, max_bodies, max_joints=0, max_manifolds=0):
+        self.lib = abi.load_library()
+        self.params = params
+        h = C.c_void_p()
+        abi.check(self.lib.avbd_world_create(C.byref(params), int(max_bodies), int(max_joints),
+                                             int(max_manifolds), C.byref(h)))
+        self.h = h
+        self.n = 0
+        self.nj = 0
+
+    def close(self):
+        if self.h:
+            self.lib.avbd_world_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params):
+        self.params = params
+        abi.check(self.lib.avbd_world_set_params(self.h, C.byref(params)))
+
+    def upload(self, bodies, joints=None):
+        bs = abi.bodies_struct(bodies)
+        js = abi.joints_struct(joints)
+        abi.check(self.lib.avbd_world_upload(self.h, C.byref(bs), C.byref(js)))
+        self.n = bs.count
+        self.nj = js.count
+
+    def step(self, n=1):
+        for _ in range(n):
+            abi.check(self.lib.avbd_world_step(self.h))
+
+    def step_interleaved(self, substeps, rounds):
+        abi.check(self.lib.avbd_world_step_interleaved(self.h, int(substeps), int(rounds)))
+
+    def download(self, joints=False):
+        out = abi.empty_arrays({k: abi.BODY_FIELDS[k] for k in
+                                ("position", "rotation", "velocity", "angular_velocity", "prev_velocity")}, self.n)
+        bs = abi.bodies_struct({**out, "mass": np.zeros(self.n, np.float32)})
+        bs.mass = None
+        bs.count = self.n
+        js = None
+        jout = None
+        if joints and self.nj:
+            jout = abi.empty_arrays({k: abi.JOINT_FIELDS[k] for k in
+                                     ("body_a", "body_b", "c0_lin", "c0_ang", "penalty_lin", "penalty_ang",
+                                      "lambda_lin", "lambda_ang", "broken")}, self.nj)
+            js = abi.joints_struct({**jout, "body_b": jout["body_b"]})
+            js.count = self.nj
+        abi.check(self.lib.avbd_world_download(self.h, C.byref(bs), C.byref(js) if js is not None else None))
+        if jout is not None:
+            k = js.count
+            jout = {key: v[:k] for key, v in jout.items()}
+        return (out, jout) if joints else out
+
+    def manifolds(self, capacity=None):
+        cap = capacity or max(64, 8 * self.n)
+        arr = abi.empty_arrays(abi.MANIFOLD_FIELDS, cap)
+        ms = abi.manifolds_struct(arr)
+        rc = self.lib.avbd_world_get_manifolds(self.h, C.byref(ms))
+        if rc == abi.AVBD_ERR_CAPACITY:
+            return self.manifolds(ms.count + 16)
+        abi.check(rc)
+        m = ms.count
+        return {k: v[:m] for k, v in arr.items()}
+
+    def set_manifolds(self, arrays):
+        ms = abi.manifolds_struct(arrays)
+        abi.check(self.lib.avbd_world_set_manifolds(self.h, C.byref(ms)))
+
+    def stats(self):
+        s = abi.StepStats()
+        abi.check(self.lib.avbd_world_stats(self.h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in abi.StepStats._fields_}
+
+    def probe_broadphase(self, pair_capacity=None):
+        n = self.n
+        codes = np.zeros(n, np.uint64)
+        order = np.zeros(n, np.int32)
+        cap = pair_capacity or 16 * n + 64
+        pairs = np.zeros(cap, np.uint64)
+        cnt = C.c_int32(cap)
+        abi.check(self.lib.avbd_world_probe_broadphase(self.h, codes.ctypes.data_as(abi.u64p),
+                                                       order.ctypes.data_as(abi.i32p),
+                                                       pairs.ctypes.data_as(abi.u64p), C.byref(cnt)))
+        return codes, order, pairs[:cnt.value]
+
+    def digest(self):
+        st = self.download()
+        return state_digest(st)
+
+
+class Batch:
+    """Many independent worlds stepped together (avbd_batch_*)."""
+
+    def __init__(self, params, max_worlds, max_bodies, max_joints=0, max_manifolds=0):
+        self.lib = abi.load_library()
+        h = C.c_void_p()
+        abi.check(self.lib.avbd_batch_create(C.byref(params), int(max_worlds), int(max_bodies),
+                                             int(max_joints), int(max_manifolds), C.byref(h)))
+        self.h = h
+        self.sizes = {}
+
+    def close(self):
+        if self.h:
+            self.lib.avbd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, bodies, joints=None):
+        bs = abi.bodies_struct(bodies)
+        js = abi.joints_struct(joints)
+        wid = C.c_int32(-1)
+        abi.check(self.lib.avbd_batch_add_world(self.h, C.byref(bs), C.byref(js), C.byref(wid)))
+        self.sizes[wid.value] = bs.count
+        return wid.value
+
+    def remove(self, wid):
+        abi.check(self.lib.avbd_batch_remove_world(self.h, int(wid)))
+        self.sizes.pop(wid, None)
+
+    def step(self):
+        abi.check(self.lib.avbd_batch_step(self.h))
+
+    def download(self, wid):
+        n = self.sizes[wid]
+        out = abi.empty_arrays({k: abi.BODY_FIELDS[k] for k in
+                                ("position", "rotation", "velocity", "angular_velocity", "prev_velocity")}, n)
+        bs = abi.bodies_struct({**out, "mass": np.zeros(n, np.float32)})
+        bs.mass = None
+        bs.count = n
+        abi.check(self.lib.avbd_batch_download_world(self.h, int(wid), C.byref(bs), None))
+        return out
+
+
+def state_digest(state):
+    """FNV-1a over position, rotation, velocity, angular velocity per body (list order),
+    as the reference's test digest (test/core_test.cpp:133-149)."""
+    n = len(state["position"])
+    buf = np.concatenate([state["position"].astype(np.float32).reshape(n, 3).view(np.uint8),
+                          state["rotation"].astype(np.float32).reshape(n, 4).view(np.uint8),
+                          state["velocity"].astype(np.float32).reshape(n, 3).view(np.uint8),
+                          state["angular_velocity"].astype(np.float32).reshape(n, 3).view(np.uint8)], axis=1)
+    h = 1469598103934665603
+    for byte in buf.reshape(-1).tolist():
+        h ^= byte
+        h = (h * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h

@@ -1515,6 +1515,19 @@ int oracle_get_manifolds(oracle_world *o, avbd_manifolds *m)
         if (m->num_contacts) m->num_contacts[i] = w.numC[s];
         for (int k = 0; k < MAXC; k++) {
             const int src = static_cast<int>(s) * MAXC + k, dst = i * MAXC + k;
+            if (k >= w.numC[s]) {
+                // slots beyond the count hold stale data the solver never reads
+                const f3 z{0, 0, 0};
+                if (m->feature) m->feature[dst] = 0;
+                if (m->r_a) store3(m->r_a, dst, z);
+                if (m->r_b) store3(m->r_b, dst, z);
+                if (m->c0) store3(m->c0, dst, z);
+                if (m->penalty) store3(m->penalty, dst, z);
+                if (m->lambda) store3(m->lambda, dst, z);
+                if (m->stick) m->stick[dst] = 0;
+                if (m->beta_scale) m->beta_scale[dst] = 1.0f;
+                continue;
+            }
             if (m->feature) m->feature[dst] = w.cont[src].key;
             if (m->r_a) store3(m->r_a, dst, w.cont[src].rA);
             if (m->r_b) store3(m->r_b, dst, w.cont[src].rB);
