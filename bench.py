@@ -1,0 +1,228 @@
+"""Benchmark of the MI355X AVBD step: body-steps/s at 10 iterations, batched vs plain.
+
+Workload (default): 4,096 independent worlds in one batch, body counts log-uniform in
+[8, 2048] (about 1.55 M bodies), each a seeded random box pile (half extents U[0.25, 1] m,
+log-normal density, random orientation, dropped from a jittered grid onto a ground box),
+mu 0.5, dt 1/60, 10 iterations, FP32.  One step = one captured-graph step of every world.
+
+Printed (one JSON line): `value` = batched body-steps/s (movable bodies x steps / seconds,
+inputs resident in HBM); p50/p99 step latency; the plain single-world path on the same
+worlds (each world stepped through avbd_world_step, one after another); the roofline of the
+dominant kernel; and the CPU baseline (the oracle, a serial restatement of the reference's
+CPU step, timed on this host on a bounded sample of the same worlds).
+
+`--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import importlib
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+pkg = importlib.import_module("godot-avbd_amd")
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes (compulsory FP32 SoA traffic, each field touched once per pass):
+#   per step      424 N_b + 256 N_m + 124 N_c
+#   per iteration 100 N_b + 320 N_m + 144 N_c + 408 N_j  (primal: 100 N_b + 160 N_m + 60 N_c,
+#                                                         dual:   160 N_m + 84 N_c + 408 N_j)
+def step_bytes(nb, nm, nc, nj, iters):
+    return 424 * nb + 256 * nm + 124 * nc + iters * (100 * nb + 320 * nm + 144 * nc + 408 * nj)
+
+
+def primal_bytes(nb_overflow, nm, nc):
+    return 100 * nb_overflow + 160 * nm + 60 * nc
+
+
+def dual_bytes(nm, nc, nj):
+    return 160 * nm + 84 * nc + 408 * nj
+
+
+def percentile(xs, q):
+    return float(np.percentile(np.asarray(xs), q))
+
+
+def sample_worlds(sizes, k, seed=0):
+    """k world ids stratified over the size distribution (sorted by size)."""
+    order = np.argsort(sizes, kind="stable")
+    idx = np.linspace(0, len(order) - 1, k).round().astype(int)
+    return sorted(set(int(order[i]) for i in idx))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=120)
+    ap.add_argument("--worlds", type=int, default=4096)
+    ap.add_argument("--min-bodies", type=int, default=8)
+    ap.add_argument("--max-bodies", type=int, default=2048)
+    ap.add_argument("--iterations", type=int, default=10)
+    ap.add_argument("--plain-sample", type=int, default=48)
+    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(worlds would shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    params = pkg.default_params(iterations=args.iterations)
+    t0 = time.time()
+    worlds = pkg.scenes.batch_piles(args.worlds, args.min_bodies, args.max_bodies)
+    sizes = np.array([len(b["mass"]) for b, _ in worlds])
+    total_bodies = int(sizes.sum())
+    log(f"built {len(worlds)} worlds, {total_bodies} bodies in {time.time() - t0:.1f}s")
+
+    batch = pkg.Batch(params, len(worlds), total_bodies, 0, 0)
+    ids = [batch.add(b, j) for b, j in worlds]
+    log(f"uploaded in {time.time() - t0:.1f}s")
+
+    for s in range(args.warmup):
+        batch.step()
+        if (s + 1) % 30 == 0:
+            log(f"warmup step {s + 1}/{args.warmup}")
+    tot = batch.totals()
+    movable = tot["movable_bodies"]
+
+    # ---- timed region: K captured-graph steps of the whole batch
+    lat = []
+    pkg.abi.load_library()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        batch.step()
+        lat.append((time.perf_counter() - a) * 1e3)
+    elapsed = time.perf_counter() - t_start
+    tot = batch.totals()
+    value = movable * args.steps / elapsed
+    ms_per_step = elapsed * 1e3 / args.steps
+    log(f"batched: {ms_per_step:.3f} ms/step, {value / 1e6:.1f} M body-steps/s, totals {tot}")
+
+    # ---- per-kernel durations (HIP events on the batch's stream) for the roofline
+    prof = batch.profile(args.profile_steps)
+    tot_p = batch.totals()
+    iters = args.iterations
+    dom = max(("primal", "dual", "narrow", "brake_levels", "traverse"),
+              key=lambda k: prof["avg_ms"][k] * prof["launches"][k])
+    nb, nm, nc, nj = tot_p["movable_bodies"], tot_p["manifolds"], tot_p["contacts"], tot_p["joints"]
+    kernel_bytes = {"primal": primal_bytes(nb, nm, nc), "dual": dual_bytes(nm, nc, nj)}
+    roof_kernel = dom if dom in kernel_bytes else max(kernel_bytes, key=lambda k: prof["avg_ms"][k])
+    kb = kernel_bytes[roof_kernel]
+    kms = prof["avg_ms"][roof_kernel]
+    achieved = kb / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{roof_kernel}_batch.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    sb = step_bytes(nb, nm, nc, nj, iters)
+    log(f"profile: {json.dumps(prof)}")
+
+    # ---- plain single-world path on the same worlds (stratified sample, one after another)
+    pick = sample_worlds(sizes, min(args.plain_sample, len(worlds)))
+    plain_t, plain_n = [], []
+    snaps = {}
+    for w in pick:
+        b, j = worlds[w]
+        st = batch.download(ids[w])
+        m = batch.manifolds(ids[w])
+        start = {**b, **st}
+        snaps[w] = (start, m)
+        pw = pkg.World(params, len(b["mass"]), 0)
+        pw.upload(start, j)
+        pw.set_manifolds(m)
+        pw.step()                   # first call pays one-time setup
+        a = time.perf_counter()
+        pw.step(args.plain_steps)
+        plain_t.append((time.perf_counter() - a) / args.plain_steps)
+        plain_n.append(int((b["mass"] > 0).sum()))
+        pw.close()
+    # per-world time model t(n) = c0 + c1 n, summed over every world of the batch
+    A = np.stack([np.ones(len(plain_n)), np.asarray(plain_n, float)], axis=1)
+    coef, *_ = np.linalg.lstsq(A, np.asarray(plain_t), rcond=None)
+    movable_w = np.array([int((b["mass"] > 0).sum()) for b, _ in worlds])
+    plain_step_s = float(np.sum(np.maximum(coef[0] + coef[1] * movable_w, min(plain_t))))
+    plain_value = float(movable_w.sum()) / plain_step_s
+    log(f"plain: sample {len(pick)} worlds, model {coef[0] * 1e3:.3f} ms + {coef[1] * 1e6:.3f} us/body; "
+        f"{plain_value / 1e6:.3f} M body-steps/s")
+
+    # ---- CPU baseline: the oracle (serial restatement of the reference step) on this host
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        done_bodies, spent = 0, 0.0
+        order = sorted(snaps, key=lambda w: -sizes[w])
+        used = []
+        for w in order:
+            start, m = snaps[w]
+            o = oracle.OracleWorld(params)
+            o.upload(start, worlds[w][1])
+            o.set_manifolds(m)
+            steps = 3
+            a = time.perf_counter()
+            o.step(steps)
+            spent += time.perf_counter() - a
+            done_bodies += int((start["mass"] > 0).sum()) * steps
+            used.append(w)
+            if spent >= args.cpu_seconds:
+                break
+        cpu = {"value": done_bodies / spent, "unit": "body-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{len(used)} of the same warmed-up worlds (largest first), 3 steps each, "
+                         f"{done_bodies} body-steps in {spent:.1f}s, 10 iterations, serial oracle"}
+        log(f"cpu baseline: {cpu['value'] / 1e3:.1f} k body-steps/s")
+
+    line = {
+        "metric": "body-steps/sec @10 iters (batched vs plain, same GPU); p50/p99 step latency",
+        "value": value,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"C4 batched: {len(worlds)} independent worlds, bodies log-uniform "
+                        f"[{args.min_bodies}, {args.max_bodies}], seeded random box piles, one captured graph per step",
+            "worlds": len(worlds), "bodies": total_bodies, "movable_bodies": movable,
+            "manifolds": tot["manifolds"], "contacts": tot["contacts"], "iterations": iters,
+            "dt": "1/60", "parallelism": "single GPU",
+        },
+        "p50_step_ms": percentile(lat, 50),
+        "p99_step_ms": percentile(lat, 99),
+        "plain": {"value": plain_value, "unit": "body-steps/s", "sample_worlds": len(pick),
+                  "model_ms": [coef[0] * 1e3, coef[1] * 1e3]},
+        "batched_over_plain": value / plain_value,
+        "roofline": {"bound": "hbm", "kernel": roof_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": kb, "avg_launch_ms": kms},
+        "step_roofline": {"algorithmic_bytes_per_step": sb,
+                          "achieved_GBs": sb / (ms_per_step * 1e-3) / 1e9,
+                          "frac": sb / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "kernel_ms": prof["avg_ms"],
+        "kernel_launches_per_step": {k: v / max(1, prof["steps"]) for k, v in prof["launches"].items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -96,6 +96,20 @@ class StepStats(C.Structure):
     ]
 
 
+K_CLASSES = ("primal", "dual", "narrow", "brake_levels", "brake_replay", "traverse", "colour_pass",
+             "proj_manifold", "proj_body", "adjacency", "sort", "body")
+
+
+class KernelProfile(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("avg_ms", C.c_float * 12), ("launches", C.c_int32 * 12),
+                ("step_ms", C.c_float)]
+
+    def as_dict(self):
+        return {"steps": self.steps, "step_ms": self.step_ms,
+                "avg_ms": dict(zip(K_CLASSES, list(self.avg_ms))),
+                "launches": dict(zip(K_CLASSES, list(self.launches)))}
+
+
 def _ptr(a, t):
     return a.ctypes.data_as(t) if a is not None else None
 
@@ -189,6 +203,8 @@ def load_library(path=LIB_PATH):
         "avbd_world_set_manifolds": (C.c_int32, [vp, C.POINTER(Manifolds)]),
         "avbd_world_stats": (C.c_int32, [vp, C.POINTER(StepStats)]),
         "avbd_world_probe_broadphase": (C.c_int32, [vp, u64p, i32p, u64p, i32p]),
+        "avbd_world_profile": (C.c_int32, [vp, C.c_int32, C.POINTER(KernelProfile)]),
+        "avbd_batch_profile": (C.c_int32, [vp, C.c_int32, C.POINTER(KernelProfile)]),
         "avbd_batch_create": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                           C.POINTER(vp)]),
         "avbd_batch_destroy": (C.c_int32, [vp]),
@@ -197,6 +213,10 @@ def load_library(path=LIB_PATH):
         "avbd_batch_step": (C.c_int32, [vp]),
         "avbd_batch_download_world": (C.c_int32, [vp, C.c_int32, C.POINTER(Bodies), C.POINTER(Joints)]),
         "avbd_batch_world_count": (C.c_int32, [vp, i32p]),
+        "avbd_batch_get_manifolds": (C.c_int32, [vp, C.c_int32, C.POINTER(Manifolds)]),
+        "avbd_batch_set_manifolds": (C.c_int32, [vp, C.c_int32, C.POINTER(Manifolds)]),
+        "avbd_batch_totals": (C.c_int32, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                          C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

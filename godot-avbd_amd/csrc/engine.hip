@@ -151,7 +151,47 @@ struct Engine {
     bool graphsValid = false;
     hipEvent_t ev[5] = {};
     avbd_step_stats stats{};
-    std::vector<hipEvent_t> owned;
+    // per-launch profiling (avbd_*_profile): event pairs around launches of each class
+    bool profiling = false;
+    std::vector<hipEvent_t> profEvents;
+    std::vector<std::pair<int, int>> profMarks;  // (class, first event index)
+    int profUsed = 0;
+    double profMs[AVBD_K_CLASSES] = {};
+    long profLaunches[AVBD_K_CLASSES] = {};
+    int profCls = -1;
+
+    void pbeg(int cls)
+    {
+        if (!profiling) return;
+        if (profUsed + 2 > static_cast<int>(profEvents.size())) {
+            for (int k = 0; k < 256; k++) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return;
+                profEvents.push_back(e);
+            }
+        }
+        (void)hipEventRecord(profEvents[profUsed], stream);
+        profMarks.emplace_back(cls, profUsed);
+        profUsed += 2;
+    }
+    void pend()
+    {
+        if (!profiling || profMarks.empty()) return;
+        (void)hipEventRecord(profEvents[profMarks.back().second + 1], stream);
+    }
+    void profCollect()
+    {
+        (void)hipStreamSynchronize(stream);
+        for (const auto &m : profMarks) {
+            float t = 0.0f;
+            if (hipEventElapsedTime(&t, profEvents[m.second], profEvents[m.second + 1]) == hipSuccess) {
+                profMs[m.first] += t;
+                profLaunches[m.first] += 1;
+            }
+        }
+        profMarks.clear();
+        profUsed = 0;
+    }
 
     ~Engine() { release(); }
 
@@ -165,6 +205,8 @@ struct Engine {
         errHost = countsHost = nullptr;
         for (auto &e : ev)
             if (e) { (void)hipEventDestroy(e); e = nullptr; }
+        for (auto &e : profEvents) (void)hipEventDestroy(e);
+        profEvents.clear();
         if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
     }
 
@@ -315,15 +357,21 @@ struct Engine {
         if (rebuildTree) {
             k_world_bounds<<<Wcap, TPB, 0, stream>>>(S);
             k_morton<<<nb, TPB, 0, stream>>>(S, batchMode ? 0 : 1);
+            pbeg(AVBD_K_SORT);
             radixSort();
+            pend();
             k_karras<<<nblk(Ncap - 1), TPB, 0, stream>>>(S, Ncap);
         } else {
             k_fill_i32<<<nblk(Ncap), TPB, 0, stream>>>(S.nodeFlag, 0, Ncap);
         }
         k_refit<<<nb, TPB, 0, stream>>>(S, Ncap);
+        pbeg(AVBD_K_TRAVERSE);
         k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap, 0);
+        pend();
         scan(S.pairCount, S.pairOff, Ncap);
+        pbeg(AVBD_K_TRAVERSE);
         k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap, 1);
+        pend();
         k_pair_filter<<<nblk(Pcap), TPB, 0, stream>>>(S);
         scan(S.pairValid, S.pairValidScan, Pcap);
         k_alloc_new<<<nblk(Pcap), TPB, 0, stream>>>(S);
@@ -331,14 +379,18 @@ struct Engine {
         if (timed) (void)hipEventRecord(ev[1], stream);
         // ---- warm start forces / narrow phase (solver_solve.cpp:31-105)
         k_joint_init<<<nblk(Jcap), TPB, 0, stream>>>(S);
+        pbeg(AVBD_K_NARROW);
         k_manifold_init<<<nblk(Scap), TPB, 0, stream>>>(S);
+        pend();
         k_release_flags<<<nblk(Scap), TPB, 0, stream>>>(S);
         scan(S.keep, S.keepScan, Scap);
         scan(S.relScan, S.relScan, Scap);
         k_compact<<<nblk(Scap), TPB, 0, stream>>>(S);
         k_compact_counts<<<nblk(Wcap), TPB, 0, stream>>>(S);
         k_copy_live<<<nblk(Scap), TPB, 0, stream>>>(S);
+        pbeg(AVBD_K_ADJACENCY);
         buildAdjacency();
+        pend();
         // ---- update set and colouring (solver_solve.cpp:263-424)
         k_colour_prepare<<<nblk(std::max(Ncap, Wcap)), TPB, 0, stream>>>(S);
         k_colour_joints<<<nblk(Jcap), TPB, 0, stream>>>(S);
@@ -355,7 +407,9 @@ struct Engine {
         k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wUpdateCount, 0, Wcap);
         k_brake_links<<<nblk(Scap), TPB, 0, stream>>>(S, slotToLive);
         const bool brake = prm.speculative_margin > 0.0f;
+        pbeg(AVBD_K_BRAKE_LEVELS);
         k_brake_levels<<<Wcap, BRAKE_TPB, 0, stream>>>(S, collisionDt, brake ? 1 : 0);
+        pend();
         if (timed) (void)hipEventRecord(ev[2], stream);
         // ---- substeps (solver_step.cpp:79-105, 142-150)
         const int rounds = !inter ? std::max(1, prm.iterations)
@@ -367,29 +421,55 @@ struct Engine {
         for (int sub = 0; sub < subs; sub++) {
             if (sub > 0) {
                 k_rebase<<<nblk(Scap), TPB, 0, stream>>>(S);
-                if (brake) k_brake_replay<<<Wcap, BRAKE_TPB, 0, stream>>>(S, prm.dt * static_cast<float>(subs - sub));
+                if (brake) {
+                    pbeg(AVBD_K_BRAKE_REPLAY);
+                    k_brake_replay<<<Wcap, BRAKE_TPB, 0, stream>>>(S, prm.dt * static_cast<float>(subs - sub));
+                    pend();
+                }
             }
+            pbeg(AVBD_K_BODY);
             k_warmstart<<<nb, TPB, 0, stream>>>(S);
+            pend();
             for (int it = 0; it < rounds; it++) {
                 const bool newton = it < newtonRounds;
                 if (newton) {
                     if (withColours)
-                        for (int c = 0; c < 8; c++) k_colour_pass<<<nb, TPB, 0, stream>>>(S, c);
+                        for (int c = 0; c < 8; c++) {
+                            pbeg(AVBD_K_COLOUR_PASS);
+                            k_colour_pass<<<nb, TPB, 0, stream>>>(S, c);
+                            pend();
+                        }
+                    pbeg(AVBD_K_PRIMAL);
                     k_overflow_solve<<<nb, TPB, 0, stream>>>(S);
+                    pend();
+                    pbeg(AVBD_K_BODY);
                     k_overflow_apply<<<nb, TPB, 0, stream>>>(S);
+                    pend();
                 }
+                pbeg(AVBD_K_DUAL);
                 k_dual<<<nblk(std::max(Scap, Jcap)), TPB, 0, stream>>>(S, newton ? 1.0f : prm.stiffness_decay);
+                pend();
             }
+            pbeg(AVBD_K_BODY);
             k_finish<<<nb, TPB, 0, stream>>>(S);
+            pend();
         }
         if (timed) (void)hipEventRecord(ev[3], stream);
         // ---- final brake, position projection, rebase (solver_step.cpp:69-70)
-        if (brake) k_brake_replay<<<Wcap, BRAKE_TPB, 0, stream>>>(S, collisionDt);
+        if (brake) {
+            pbeg(AVBD_K_BRAKE_REPLAY);
+            k_brake_replay<<<Wcap, BRAKE_TPB, 0, stream>>>(S, collisionDt);
+            pend();
+        }
         if (prm.position_projection && prm.projection_iterations > 0) {
             k_proj_reset<<<nblk(Wcap), TPB, 0, stream>>>(S);
             for (int pass = 0; pass < prm.projection_iterations; pass++) {
+                pbeg(AVBD_K_PROJ_MANIFOLD);
                 k_proj_manifold<<<nblk(Scap), TPB, 0, stream>>>(S);
+                pend();
+                pbeg(AVBD_K_PROJ_BODY);
                 k_proj_body<<<nb, TPB, 0, stream>>>(S, slotToLive);
+                pend();
                 k_proj_flags<<<nblk(Wcap), TPB, 0, stream>>>(S);
             }
         }
@@ -437,6 +517,37 @@ struct Engine {
         stats.ms_solve = t23;
         stats.ms_finish = t34;
         stats.ms_total = t01 + t12 + t23 + t34;
+        return AVBD_OK;
+    }
+
+    avbd_status profile(int steps, int substeps, int rounds, avbd_kernel_profile *out)
+    {
+        if (steps < 1) return fail(AVBD_ERR_INVALID, "steps must be >= 1");
+        for (int k = 0; k < AVBD_K_CLASSES; k++) { profMs[k] = 0; profLaunches[k] = 0; }
+        profiling = true;
+        double stepMs = 0.0;
+        for (int s2 = 0; s2 < steps; s2++) {
+            const bool rebuild = treeAge >= refitBudget;
+            treeAge = rebuild ? 0 : treeAge + 1;
+            (void)hipEventRecord(ev[0], stream);
+            enqueueStep(substeps, rounds, rebuild, anyJoints(), false);
+            (void)hipEventRecord(ev[4], stream);
+            profCollect();
+            float t = 0.0f;
+            (void)hipEventElapsedTime(&t, ev[0], ev[4]);
+            stepMs += t;
+            avbd_status st = checkErrors();
+            if (st != AVBD_OK) { profiling = false; return st; }
+        }
+        profiling = false;
+        HIPCHK(hipGetLastError());
+        std::memset(out, 0, sizeof(*out));
+        out->steps = steps;
+        out->step_ms = static_cast<float>(stepMs / steps);
+        for (int k = 0; k < AVBD_K_CLASSES; k++) {
+            out->launches[k] = static_cast<int32_t>(profLaunches[k]);
+            out->avg_ms[k] = profLaunches[k] ? static_cast<float>(profMs[k] / profLaunches[k]) : 0.0f;
+        }
         return AVBD_OK;
     }
 
@@ -809,6 +920,33 @@ struct Engine {
         return AVBD_OK;
     }
 
+    avbd_status totals(int64_t *movable, int64_t *manifolds, int64_t *contacts, int64_t *joints)
+    {
+        HIPCHK(hipStreamSynchronize(stream));
+        std::vector<int> bw(Ncap), ui(Ncap), live(Wcap), slotWorld(Scap), liveList(Scap), num(Scap), jw(Jcap);
+        std::vector<uint8_t> dead(Jcap);
+        HIPCHK(hipMemcpy(ui.data(), S.updateIndex, sizeof(int) * Ncap, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(bw.data(), S.bodyWorld, sizeof(int) * Ncap, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(live.data(), S.wLive, sizeof(int) * Wcap, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(liveList.data(), S.live, sizeof(int) * Scap, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(num.data(), S.mNum, sizeof(int) * Scap, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(jw.data(), S.jWorld, sizeof(int) * Jcap, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(dead.data(), S.jDead, Jcap, hipMemcpyDeviceToHost));
+        int64_t nb = 0, nm = 0, nc = 0, nj = 0;
+        for (int i = 0; i < Ncap; i++) nb += (bw[i] >= 0 && ui[i] >= 0) ? 1 : 0;
+        for (int w = 0; w < Wcap; w++) {
+            if (!worlds[w].active) continue;
+            nm += live[w];
+            for (int k = 0; k < live[w]; k++) nc += num[liveList[worlds[w].slotBase + k]];
+        }
+        for (int j = 0; j < Jcap; j++) nj += (jw[j] >= 0 && !dead[j]) ? 1 : 0;
+        if (movable) *movable = nb;
+        if (manifolds) *manifolds = nm;
+        if (contacts) *contacts = nc;
+        if (joints) *joints = nj;
+        return AVBD_OK;
+    }
+
     avbd_status fillStats(int w, avbd_step_stats *out)
     {
         HIPCHK(hipStreamSynchronize(stream));
@@ -1056,6 +1194,23 @@ avbd_status avbd_world_stats(avbd_world *world, avbd_step_stats *out)
     return world->eng.fillStats(0, out);
 }
 
+avbd_status avbd_world_profile(avbd_world *world, int32_t steps, avbd_kernel_profile *out)
+{
+    if (!world || !out) return fail(AVBD_ERR_INVALID, "null argument");
+    if (!world->eng.worlds[0].active) return fail(AVBD_ERR_INVALID, "nothing uploaded");
+    return world->eng.profile(steps, 1, 0, out);
+}
+
+avbd_status avbd_batch_profile(avbd_batch *batch, int32_t steps, avbd_kernel_profile *out)
+{
+    if (!batch || !out) return fail(AVBD_ERR_INVALID, "null argument");
+    if (!batch->eng.csrValid) {
+        batch->eng.buildAdjacency();
+        batch->eng.csrValid = true;
+    }
+    return batch->eng.profile(steps, 1, 0, out);
+}
+
 avbd_status avbd_world_probe_broadphase(avbd_world *world, uint64_t *codes, int32_t *order, uint64_t *pairs,
                                         int32_t *pair_count)
 {
@@ -1150,6 +1305,32 @@ avbd_status avbd_batch_world_count(avbd_batch *batch, int32_t *count)
     for (const auto &w : batch->eng.worlds) c += w.active ? 1 : 0;
     *count = c;
     return AVBD_OK;
+}
+
+avbd_status avbd_batch_get_manifolds(avbd_batch *batch, int32_t world_id, avbd_manifolds *out)
+{
+    if (!batch || !out) return fail(AVBD_ERR_INVALID, "null argument");
+    if (world_id < 0 || world_id >= batch->eng.Wcap || !batch->eng.worlds[world_id].active)
+        return fail(AVBD_ERR_INVALID, "bad world id");
+    return batch->eng.getManifolds(world_id, out);
+}
+
+avbd_status avbd_batch_set_manifolds(avbd_batch *batch, int32_t world_id, const avbd_manifolds *in)
+{
+    if (!batch || !in) return fail(AVBD_ERR_INVALID, "null argument");
+    if (world_id < 0 || world_id >= batch->eng.Wcap || !batch->eng.worlds[world_id].active)
+        return fail(AVBD_ERR_INVALID, "bad world id");
+    if (in->count > 0 && (!in->body_a || !in->body_b || !in->basis || !in->friction || !in->num_contacts ||
+                          !in->feature || !in->r_a || !in->r_b || !in->c0 || !in->penalty || !in->lambda || !in->stick))
+        return fail(AVBD_ERR_INVALID, "a required manifold array is null");
+    return batch->eng.setManifolds(world_id, in);
+}
+
+avbd_status avbd_batch_totals(avbd_batch *batch, int64_t *movable_bodies, int64_t *manifolds, int64_t *contacts,
+                              int64_t *joints)
+{
+    if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
+    return batch->eng.totals(movable_bodies, manifolds, contacts, joints);
 }
 
 } // extern "C"

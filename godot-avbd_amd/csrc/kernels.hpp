@@ -960,6 +960,9 @@ __global__ void __launch_bounds__(BRAKE_TPB) k_brake_levels(State S, float windo
     __syncthreads();
     while (true) {
         const int h = head, e = tail;
+        // every thread holds this level's bounds before any thread appends to the next
+        // level, so all threads see the same h == e and leave the loop together
+        __syncthreads();
         if (h == e) break;
         if (threadIdx.x == 0) S.levelStart[base + levels] = h;
         for (int k = h + threadIdx.x; k < e; k += BRAKE_TPB) {

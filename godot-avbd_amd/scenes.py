@@ -146,9 +146,28 @@ def pyramid(rows=28, seed=1):
     return s.build()
 
 
+def box_arrays(size, density, friction, pos, rot, gravity=-10.0):
+    """Vectorised form of SceneBuilder.box for many boxes (same float32 expressions)."""
+    size = np.asarray(size, F)
+    d = np.asarray(density, F)
+    m = size[:, 0] * size[:, 1] * size[:, 2] * d
+    moment = np.stack([(size[:, 1] * size[:, 1] + size[:, 2] * size[:, 2]) / F(12.0) * m,
+                       (size[:, 0] * size[:, 0] + size[:, 2] * size[:, 2]) / F(12.0) * m,
+                       (size[:, 0] * size[:, 0] + size[:, 1] * size[:, 1]) / F(12.0) * m], axis=1).astype(F)
+    h = size * F(0.5)
+    radius = np.sqrt(((F(0.0) + h[:, 0] * h[:, 0]) + h[:, 1] * h[:, 1]) + h[:, 2] * h[:, 2])
+    n = len(m)
+    return dict(size=size, mass=m.astype(F), moment=moment, friction=np.full(n, friction, F) if np.isscalar(friction)
+                else np.asarray(friction, F), radius=radius.astype(F), gravity=np.full(n, gravity, F),
+                position=np.asarray(pos, F), rotation=np.asarray(rot, F), velocity=np.zeros((n, 3), F),
+                angular_velocity=np.zeros((n, 3), F))
+
+
 def pile(n=8192, seed=2, grid=None, pitch=4.0, ground=400.0):
     """C2: n boxes with half extents U[0.25, 1], log-normal density (sigma 0.7, clipped to
-    2 sigma), uniform random orientation, on a jittered 3-D grid (lowest centre z = 2)."""
+    2 sigma), uniform random orientation, on a jittered 3-D grid (lowest centre z = 2), created
+    after a static ground box; body k of the grid is at x = k % nx, y = (k // nx) % ny,
+    z = k // (nx ny)."""
     rng = np.random.default_rng(seed)
     if grid is None:
         nx = max(1, int(round((n / 2) ** (1.0 / 3.0))))
@@ -156,23 +175,21 @@ def pile(n=8192, seed=2, grid=None, pitch=4.0, ground=400.0):
         nz = int(math.ceil(n / (nx * ny)))
         grid = (nx, ny, nz)
     nx, ny, nz = grid
-    s = SceneBuilder()
-    s.box((ground, ground, 1), 0.0, 0.5, (0, 0, -0.5))
     half = rng.uniform(0.25, 1.0, size=(n, 3))
     dens = np.exp(np.clip(rng.normal(0.0, 0.7, size=n), -1.4, 1.4))
     quats = _shoemake(rng, n)
     jit = rng.uniform(-0.25, 0.25, size=(n, 3))
-    k = 0
-    for z in range(nz):
-        for y in range(ny):
-            for x in range(nx):
-                if k >= n:
-                    break
-                p = ((x - (nx - 1) / 2.0) * pitch + jit[k, 0], (y - (ny - 1) / 2.0) * pitch + jit[k, 1],
-                     2.0 + z * pitch + jit[k, 2])
-                s.box(2.0 * half[k], dens[k], 0.5, p, rot=quats[k])
-                k += 1
-    return s.build()
+    k = np.arange(n)
+    x, y, z = k % nx, (k // nx) % ny, k // (nx * ny)
+    pos = np.stack([(x - (nx - 1) / 2.0) * pitch + jit[:, 0], (y - (ny - 1) / 2.0) * pitch + jit[:, 1],
+                    2.0 + z * pitch + jit[:, 2]], axis=1)
+    boxes = box_arrays((2.0 * half).astype(F), dens.astype(F), 0.5, pos.astype(F), quats)
+    gnd = box_arrays(np.array([[ground, ground, 1.0]], F), np.zeros(1, F), 0.5, np.array([[0, 0, -0.5]], F),
+                     np.array([[0, 0, 0, 1]], F))
+    # creation order: ground, then boxes; list order is the reverse
+    bodies = {key: np.concatenate([gnd[key], boxes[key]])[::-1].copy() for key in gnd}
+    bodies["prev_velocity"] = bodies["velocity"].copy()
+    return bodies, None
 
 
 def chains(n_chains=4096, links=32, seed=3, grid=None, pitch_y=1.5, pitch_x=34.0, anchor_z=30.0):

@@ -107,6 +107,11 @@ class World:
         st = self.download()
         return state_digest(st)
 
+    def profile(self, steps):
+        out = abi.KernelProfile()
+        abi.check(self.lib.avbd_world_profile(self.h, int(steps), C.byref(out)))
+        return out.as_dict()
+
 
 class Batch:
     """Many independent worlds stepped together (avbd_batch_*)."""
@@ -154,6 +159,31 @@ class Batch:
         bs.count = n
         abi.check(self.lib.avbd_batch_download_world(self.h, int(wid), C.byref(bs), None))
         return out
+
+
+    def manifolds(self, wid, capacity=None):
+        cap = capacity or max(64, 8 * self.sizes[wid])
+        arr = abi.empty_arrays(abi.MANIFOLD_FIELDS, cap)
+        ms = abi.manifolds_struct(arr)
+        rc = self.lib.avbd_batch_get_manifolds(self.h, int(wid), C.byref(ms))
+        if rc == abi.AVBD_ERR_CAPACITY:
+            return self.manifolds(wid, ms.count + 16)
+        abi.check(rc)
+        return {k: v[:ms.count] for k, v in arr.items()}
+
+    def set_manifolds(self, wid, arrays):
+        ms = abi.manifolds_struct(arrays)
+        abi.check(self.lib.avbd_batch_set_manifolds(self.h, int(wid), C.byref(ms)))
+
+    def profile(self, steps):
+        out = abi.KernelProfile()
+        abi.check(self.lib.avbd_batch_profile(self.h, int(steps), C.byref(out)))
+        return out.as_dict()
+
+    def totals(self):
+        v = [C.c_int64() for _ in range(4)]
+        abi.check(self.lib.avbd_batch_totals(self.h, *[C.byref(x) for x in v]))
+        return dict(zip(("movable_bodies", "manifolds", "contacts", "joints"), [x.value for x in v]))
 
 
 def state_digest(state):

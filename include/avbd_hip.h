@@ -155,6 +155,29 @@ typedef struct avbd_step_stats {
     float ms_total;
 } avbd_step_stats;
 
+/* Kernel classes timed by avbd_*_profile (average duration per launch, HIP events on the
+ * handle's stream around each launch). */
+#define AVBD_K_PRIMAL 0        /* Jacobi primal: per-body block assembly + 6x6 solve     */
+#define AVBD_K_DUAL 1          /* dual: per-manifold lambda / penalty update + joints      */
+#define AVBD_K_NARROW 2        /* narrow phase + warm start, per manifold                  */
+#define AVBD_K_BRAKE_LEVELS 3  /* level schedule + first velocity brake, per world         */
+#define AVBD_K_BRAKE_REPLAY 4  /* second velocity brake, per world                         */
+#define AVBD_K_TRAVERSE 5      /* LBVH traversal (count + emit)                            */
+#define AVBD_K_COLOUR_PASS 6   /* one colour of the Gauss-Seidel sweep                     */
+#define AVBD_K_PROJ_MANIFOLD 7 /* projection: per-manifold corrections                     */
+#define AVBD_K_PROJ_BODY 8     /* projection: per-body gather and apply                    */
+#define AVBD_K_ADJACENCY 9     /* CSR adjacency build                                      */
+#define AVBD_K_SORT 10         /* Morton radix sort                                        */
+#define AVBD_K_BODY 11         /* warm start + finish + Jacobi combine                     */
+#define AVBD_K_CLASSES 12
+
+typedef struct avbd_kernel_profile {
+    int32_t steps;
+    float avg_ms[AVBD_K_CLASSES];
+    int32_t launches[AVBD_K_CLASSES];
+    float step_ms;   /* average wall time of a profiled step (events, stream launches) */
+} avbd_kernel_profile;
+
 /* ---------------------------------------------------------------- helpers ---- */
 
 /* Fill `p` with the reference defaults. */
@@ -204,6 +227,8 @@ avbd_status avbd_world_download(avbd_world *world, avbd_bodies *bodies, avbd_joi
 avbd_status avbd_world_get_manifolds(avbd_world *world, avbd_manifolds *out);
 avbd_status avbd_world_set_manifolds(avbd_world *world, const avbd_manifolds *in);
 avbd_status avbd_world_stats(avbd_world *world, avbd_step_stats *out);
+/* Run `steps` steps with every launch bracketed by HIP events; fills per-class averages. */
+avbd_status avbd_world_profile(avbd_world *world, int32_t steps, avbd_kernel_profile *out);
 
 /* Broad-phase probe for parity tests: Morton codes of the current swept-AABB centres
  * (reference bvh_broadphase.cpp:19-36, 69-85) with the (code, index) sort order, and the
@@ -231,6 +256,16 @@ avbd_status avbd_batch_step(avbd_batch *batch);
 avbd_status avbd_batch_download_world(avbd_batch *batch, int32_t world_id,
                                       avbd_bodies *bodies, avbd_joints *joints);
 avbd_status avbd_batch_world_count(avbd_batch *batch, int32_t *count);
+/* Manifolds of one world of the batch, as avbd_world_get_manifolds. */
+avbd_status avbd_batch_get_manifolds(avbd_batch *batch, int32_t world_id, avbd_manifolds *out);
+/* Upload a world's manifold state (e.g. a snapshot), as avbd_world_set_manifolds. */
+avbd_status avbd_batch_set_manifolds(avbd_batch *batch, int32_t world_id, const avbd_manifolds *in);
+/* Run `steps` batch steps as stream launches with per-launch HIP events (same kernels as
+ * the captured graph); fills per-class average launch durations. */
+avbd_status avbd_batch_profile(avbd_batch *batch, int32_t steps, avbd_kernel_profile *out);
+/* Totals over the batch after the last step: bodies (movable), manifolds, contacts. */
+avbd_status avbd_batch_totals(avbd_batch *batch, int64_t *movable_bodies, int64_t *manifolds,
+                              int64_t *contacts, int64_t *joints);
 
 #ifdef __cplusplus
 }

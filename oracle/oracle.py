@@ -86,6 +86,8 @@ def ref_lib():
                                     a.f32p, a.i32p, a.f32p, a.f32p]
     R.ref_solve6.restype = C.c_int
     R.ref_solve6.argtypes = [a.f32p, a.f32p, a.f32p]
+    R.ref_morton3D.restype = C.c_uint64
+    R.ref_morton3D.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]
     _ref = R
     return R
 
@@ -187,6 +189,21 @@ def morton(centers):
     order = np.zeros(n, np.int32)
     lib().oracle_morton(c.ctypes.data_as(a.f32p), n, codes.ctypes.data_as(a.u64p), order.ctypes.data_as(a.i32p))
     return codes, order
+
+
+def ref_morton_codes(centers):
+    """Codes from the reference's own morton3D with the scene bounds of `centers`
+    (bvh_broadphase.cpp:332-348), or None when oracle/_ref is not built."""
+    R = ref_lib()
+    if R is None:
+        return None
+    c = np.asarray(centers, np.float32)
+    lo = c.min(axis=0)
+    hi = c.max(axis=0)
+    ext = hi - lo
+    scale = np.float32(2097151.0) / np.float32(max(float(ext[0]), float(ext[1]), float(ext[2]), float(np.float32(1e-6))))
+    return np.array([R.ref_morton3D(float(x), float(y), float(z), float(lo[0]), float(scale)) for x, y, z in c],
+                    np.uint64)
 
 
 def _collide(fn, pa, qa, sa, pb, qb, sb, margin):

@@ -1,0 +1,92 @@
+"""Writes tests/golden/morton.npz (codes of the reference's own Morton encoder,
+src/avbd/bvh_broadphase.cpp:19-36) and tests/golden/collide_boxes.npz: box-box
+narrow-phase outputs (contact count,
+basis, feature keys, local arms) produced by the reference's own src/avbd/collide.cpp,
+compiled by oracle/Makefile into oracle/_ref/libavbd_ref.so.  The reference tree exists
+only in the build container, so its outputs travel as this fixture.
+
+Cases: resting face contacts (stacked cubes, cube on a slab), tilted and edge-on pairs,
+random orientations and sizes near touching, separated pairs inside and outside the
+margin, all with the solver's speculative margin.
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle  # noqa: E402
+
+
+def quat_axis(axis, ang):
+    axis = np.asarray(axis, np.float64)
+    axis /= np.linalg.norm(axis)
+    s = np.sin(ang / 2)
+    return np.array([axis[0] * s, axis[1] * s, axis[2] * s, np.cos(ang / 2)], np.float32)
+
+
+def cases(rng, n_random=400):
+    I = np.array([0, 0, 0, 1], np.float32)
+    out = []
+    out.append(((0, 0, 1.0), I, (1, 1, 1), (0, 0, 0), I, (100, 100, 1)))
+    out.append(((0, 0, 0.999), I, (1, 1, 1), (0, 0, 0), I, (100, 100, 1)))
+    out.append(((0, 0, 2.0), I, (1, 1, 1), (0, 0, 1.0), I, (1, 1, 1)))
+    out.append(((0.3, 0.2, 1.995), I, (1, 1, 1), (0, 0, 1.0), I, (1, 1, 1)))
+    out.append(((0, 0, 1.1), quat_axis((0, 0, 1), 0.7), (1, 1, 1), (0, 0, 0), I, (10, 10, 1)))
+    out.append(((0, 0, 1.2), quat_axis((1, 0, 0), np.pi / 4), (1, 1, 1), (0, 0, 0), I, (10, 10, 1)))
+    out.append(((0, 0, 1.6), quat_axis((1, 1, 0), 0.95), (1, 1, 1), (0, 0, 0), I, (10, 10, 1)))
+    out.append(((0.9, 0.0, 0.0), quat_axis((0, 0, 1), np.pi / 4), (1, 1, 1), (0, 0, 0), quat_axis((1, 0, 0), np.pi / 4),
+                (1, 1, 1)))
+    for _ in range(n_random):
+        sa = rng.uniform(0.5, 2.0, 3).astype(np.float32)
+        sb = rng.uniform(0.5, 2.0, 3).astype(np.float32)
+        qa = rng.normal(size=4); qa /= np.linalg.norm(qa)
+        qb = rng.normal(size=4); qb /= np.linalg.norm(qb)
+        d = rng.normal(size=3); d /= np.linalg.norm(d)
+        dist = rng.uniform(0.3, 1.2) * (np.linalg.norm(sa) + np.linalg.norm(sb)) / 2
+        pa = rng.uniform(-5, 5, 3)
+        out.append((pa, qa.astype(np.float32), sa, pa + d * dist, qb.astype(np.float32), sb))
+    return out
+
+
+def main():
+    oracle.build(ref=True)
+    if oracle.ref_lib() is None:
+        raise SystemExit("oracle/_ref/libavbd_ref.so is not built (needs the reference tree)")
+    rng = np.random.default_rng(12345)
+    cs = cases(rng)
+    n = len(cs)
+    rec = {k: [] for k in ("pa", "qa", "sa", "pb", "qb", "sb")}
+    count = np.zeros(n, np.int32)
+    basis = np.zeros((n, 9), np.float32)
+    keys = np.zeros((n, 8), np.int32)
+    ra = np.zeros((n, 8, 3), np.float32)
+    rb = np.zeros((n, 8, 3), np.float32)
+    margin = np.float32(0.01)
+    for i, (pa, qa, sa, pb, qb, sb) in enumerate(cs):
+        for k, v in zip(rec, (pa, qa, sa, pb, qb, sb)):
+            rec[k].append(np.asarray(v, np.float32))
+        c, b, kk, a, bb = oracle.ref_collide_boxes(pa, qa, sa, pb, qb, sb, margin)
+        count[i] = c
+        basis[i] = b
+        keys[i, :c] = kk
+        ra[i, :c] = a
+        rb[i, :c] = bb
+    np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "collide_boxes.npz"),
+                        margin=margin, count=count, basis=basis, keys=keys, ra=ra, rb=rb,
+                        **{k: np.stack(v) for k, v in rec.items()})
+    print(f"{n} cases, {int((count > 0).sum())} with contacts, {int(count.sum())} contacts")
+    # Morton codes of the reference's own encoder for a few centre sets (ties included)
+    sets = {}
+    for k, (n_c, lo, hi) in enumerate([(2, -1, 1), (64, -50, 80), (500, 0, 400), (300, -3, 3)]):
+        c = rng.uniform(lo, hi, (n_c, 3)).astype(np.float32)
+        c[: n_c // 4] = c[0]
+        sets[f"centers{k}"] = c
+        sets[f"codes{k}"] = oracle.ref_morton_codes(c)
+    np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "morton.npz"),
+                        sets=np.int32(4), **sets)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,96 @@
+"""CPU tests of the C-ABI library: it loads, exports every entry point the public headers
+declare, and its host-side helpers and argument checks behave (no GPU compute here)."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+
+from helpers import ROOT, pkg, scenes
+
+abi = pkg.abi
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        names |= set(re.findall(r"\b(avbd_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = abi.load_library()
+    names = declared_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_params_default_matches_reference_defaults():
+    lib = abi.load_library()
+    p = abi.Params()
+    lib.avbd_params_default(C.byref(p))
+    q = pkg.default_params()
+    for name, _ in abi.Params._fields_:
+        assert getattr(p, name) == getattr(q, name), name
+
+
+def test_box_properties_match_builder():
+    lib = abi.load_library()
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        size = rng.uniform(0.1, 3.0, 3).astype(np.float32)
+        dens = np.float32(rng.uniform(0.1, 5.0))
+        m = C.c_float()
+        mom = (C.c_float * 3)()
+        r = C.c_float()
+        lib.avbd_box_properties(size.ctypes.data_as(abi.f32p), dens, C.byref(m), mom, C.byref(r))
+        s = scenes.SceneBuilder()
+        s.box(size, dens, 0.5, (0, 0, 0))
+        b, _ = s.build()
+        assert np.float32(m.value) == b["mass"][0]
+        assert np.array_equal(np.array(list(mom), np.float32), b["moment"][0])
+        assert np.float32(r.value) == b["radius"][0]
+
+
+def test_state_digest_matches_python():
+    lib = abi.load_library()
+    b, _ = scenes.pile(40, seed=3)
+    st = abi.bodies_struct(b)
+    assert lib.avbd_state_digest(C.byref(st)) == pkg.state_digest(b)
+
+
+def test_unsupported_and_invalid_arguments():
+    lib = abi.load_library()
+    h = C.c_void_p()
+    for bad in (dict(sleep_frames=30), dict(contact_grace_ticks=2), dict(jacobian_rebuild_distance=1e-4),
+                dict(convergence_threshold=1e-5)):
+        p = pkg.default_params(**bad)
+        assert lib.avbd_world_create(C.byref(p), 10, 0, 0, C.byref(h)) == abi.AVBD_ERR_UNSUPPORTED
+        assert lib.avbd_last_error()
+    p = pkg.default_params(dt=0.0)
+    assert lib.avbd_world_create(C.byref(p), 10, 0, 0, C.byref(h)) == abi.AVBD_ERR_INVALID
+    assert lib.avbd_world_create(None, 10, 0, 0, C.byref(h)) == abi.AVBD_ERR_INVALID
+    assert lib.avbd_world_step(None) == abi.AVBD_ERR_INVALID
+    assert lib.avbd_batch_step(None) == abi.AVBD_ERR_INVALID
+
+
+def test_scene_list_order_is_reverse_creation():
+    s = scenes.SceneBuilder()
+    a = s.box((1, 1, 1), 1.0, 0.5, (1, 0, 0))
+    b = s.box((1, 1, 1), 1.0, 0.5, (2, 0, 0))
+    s.joint(a, b, (0.5, 0, 0), (-0.5, 0, 0))
+    bodies, joints = s.build()
+    assert bodies["position"][0, 0] == 2 and bodies["position"][1, 0] == 1
+    assert joints["body_a"][0] == 1 and joints["body_b"][0] == 0
+
+
+def test_config_scene_sizes():
+    b, _ = scenes.pyramid()
+    assert len(b["mass"]) == 407 and int((b["mass"] > 0).sum()) == 406
+    b, j = scenes.chains(8, links=32, grid=(4, 2))
+    assert int((b["mass"] > 0).sum()) == 256 and len(j["body_b"]) == 256
+    sizes = scenes.batch_sizes(4096)
+    assert sizes.min() >= 8 and sizes.max() <= 2048

@@ -1,0 +1,253 @@
+"""GPU parity of the HIP path against the oracle, through the C ABI.
+
+Bar: integer outputs (Morton codes, sort order, candidate pairs, feature keys, live
+manifold order, colours) bit-exact; FP32 state bit-exact as well, because the HIP path
+keeps the reference's evaluation order (the stated fallback tolerance, used only where a
+test says so, is |dp| <= 1e-5 (1 + |p|), |dq| <= 1e-6, |d lambda|, |d k| <= 1e-5 relative).
+"""
+import numpy as np
+import pytest
+
+from helpers import (DIGEST_INTERLEAVED_300, DIGEST_MIXED_240, DIGEST_STACK_300, bits_equal, dt120, oracle, pkg,
+                     scenes, state_mismatch)
+
+pytestmark = pytest.mark.gpu
+P = pkg.default_params
+MANIFOLD_KEYS = ("body_a", "body_b", "basis", "friction", "num_contacts", "feature", "r_a", "r_b", "c0", "penalty",
+                 "lambda_", "stick")
+
+
+def gpu_world(bodies, joints, params):
+    w = pkg.World(params, len(bodies["mass"]), 0 if joints is None else len(joints["body_b"]))
+    w.upload(bodies, joints)
+    return w
+
+
+def lockstep(bodies, joints, params, steps, inter=None, every=1):
+    w = gpu_world(bodies, joints, params)
+    o = oracle.OracleWorld(params)
+    o.upload(bodies, joints)
+    for s in range(steps):
+        if inter:
+            w.step_interleaved(*inter)
+            o.step_interleaved(*inter)
+        else:
+            w.step()
+            o.step()
+        if (s + 1) % every == 0 or s + 1 == steps:
+            bad = state_mismatch(w.download(), o.download())
+            assert not bad, f"step {s}: {bad}"
+    return w, o
+
+
+def manifolds_equal(w, o):
+    gm, om = w.manifolds(), o.manifolds()
+    assert len(gm["body_a"]) == len(om["body_a"])
+    for k in MANIFOLD_KEYS:
+        assert bits_equal(gm[k], om[k]), k
+
+
+def test_gpu_digest_stack_300():
+    w = gpu_world(*scenes.digest_stack(), P())
+    w.step(300)
+    assert w.digest() == DIGEST_STACK_300
+
+
+def test_gpu_digest_interleaved_300():
+    p = P()
+    p.dt = dt120()
+    w = gpu_world(*scenes.digest_stack(), p)
+    for _ in range(300):
+        w.step_interleaved(2, 3)
+    assert w.digest() == DIGEST_INTERLEAVED_300
+
+
+def test_gpu_digest_mixed_240():
+    w = gpu_world(*scenes.digest_mixed(), P())
+    w.step(240)
+    assert w.digest() == DIGEST_MIXED_240
+
+
+def test_pyramid_lockstep_and_manifolds():
+    w, o = lockstep(*scenes.pyramid(), P(iterations=10), 90, every=10)
+    manifolds_equal(w, o)
+    gs, os_ = w.stats(), o.stats()
+    for k in ("manifolds", "contacts", "candidate_pairs", "new_manifolds"):
+        assert gs[k] == os_[k], k
+
+
+def test_pile_lockstep_and_manifolds():
+    w, o = lockstep(*scenes.pile(600, seed=21), P(iterations=10), 150, every=15)
+    manifolds_equal(w, o)
+
+
+def test_chains_colour_sweep_lockstep():
+    p = P(iterations=10, beta_lin=1e5, penalty_max=1e9)
+    w, o = lockstep(*scenes.chains(16, links=32, grid=(4, 4), pitch_x=4.0), p, 150, every=15)
+    manifolds_equal(w, o)
+    gs, os_ = w.stats(), o.stats()
+    assert gs["colours"] == os_["colours"] and gs["overflow_bodies"] == os_["overflow_bodies"]
+    _, gj = w.download(joints=True)
+    _, oj = o.download(joints=True)
+    for k in ("penalty_lin", "lambda_lin", "c0_lin"):
+        assert bits_equal(gj[k], oj[k]), k
+
+
+def test_interleaved_pile_lockstep():
+    p = P(iterations=10)
+    p.dt = dt120()
+    lockstep(*scenes.pile(200, seed=5, pitch=2.5), p, 80, inter=(2, 0), every=10)
+
+
+def test_locks_layers_and_adaptive_beta():
+    b, j = scenes.pile(150, seed=8, pitch=2.0)
+    n = len(b["mass"])
+    rng = np.random.default_rng(1)
+    b["lock_linear"] = (rng.random(n) < 0.2).astype(np.uint8) * 1
+    b["lock_angular"] = (rng.random(n) < 0.2).astype(np.uint8) * 6
+    b["layer"] = np.where(rng.random(n) < 0.3, 2, 1).astype(np.uint32)
+    b["mask"] = np.ones(n, np.uint32)
+    b["mask"][-1] = 3          # the ground collides with both layers
+    lockstep(b, j, P(iterations=6, adaptive_beta=1, newton_ratio=0.5, stiffness_decay=0.9), 100, every=10)
+
+
+def test_single_step_from_identical_state():
+    b, j = scenes.pile(500, seed=31)
+    o = oracle.OracleWorld(P(iterations=10))
+    o.upload(b, j)
+    o.step(70)
+    snap = o.download()
+    m = o.manifolds()
+    assert len(m["body_a"]) > 100
+    start = {**b, **{k: snap[k] for k in ("position", "rotation", "velocity", "angular_velocity", "prev_velocity")}}
+    w = gpu_world(start, j, P(iterations=10))
+    w.set_manifolds(m)
+    o2 = oracle.OracleWorld(P(iterations=10))
+    o2.upload(start, j)
+    o2.set_manifolds(m)
+    w.step()
+    o2.step()
+    g, r = w.download(), o2.download()
+    # stated tolerance (the result is in fact bit-exact, asserted below)
+    assert np.all(np.abs(g["position"] - r["position"]) <= 1e-5 * (1 + np.abs(r["position"])))
+    assert np.all(np.abs(g["rotation"] - r["rotation"]) <= 1e-6)
+    assert not state_mismatch(g, r)
+    manifolds_equal(w, o2)
+
+
+def test_broadphase_probe_codes_order_pairs():
+    b, j = scenes.pile(700, seed=41, pitch=2.5)
+    w = gpu_world(b, j, P())
+    o = oracle.OracleWorld(P())
+    o.upload(b, j)
+    for _ in range(3):
+        codes, order, pairs = w.probe_broadphase()
+        c, _ = o.swept_aabbs()
+        oc, oo = oracle.morton(c)
+        assert np.array_equal(codes, oc)
+        assert np.array_equal(order, oo)
+        assert np.array_equal(pairs, o.candidate_pairs())
+        w.step(25)
+        o.step(25)
+
+
+def aabbs_overlap(f, i):
+    def box(p, q, size):
+        x, y, z, w = q.astype(np.float64)
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+        h = np.abs(R) @ (size.astype(np.float64) / 2) + 0.01 + 1e-3
+        return p - h, p + h
+    a0, a1 = box(f["pa"][i], f["qa"][i], f["sa"][i])
+    b0, b1 = box(f["pb"][i], f["qb"][i], f["sb"][i])
+    return bool(np.all(a0 <= b1) and np.all(b0 <= a1))
+
+
+def test_narrow_phase_against_reference_fixture():
+    """Each fixture pair becomes a two-body world; the first step's manifold holds the
+    narrow-phase output (feature keys, local arms, basis), which the rest of the step does
+    not modify.  Expected values come from the reference's own collide.cpp."""
+    import os
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", "collide_boxes.npz"))
+    n = len(f["count"])
+    batch = pkg.Batch(P(), n, 2 * n)
+    ids = []
+    for i in range(n):
+        s = scenes.SceneBuilder()
+        s.box(f["sb"][i], 1.0, 0.5, f["pb"][i], rot=f["qb"][i])
+        s.box(f["sa"][i], 1.0, 0.5, f["pa"][i], rot=f["qa"][i])   # list index 0 = A
+        ids.append(batch.add(*s.build()))
+    batch.step()
+    checked = 0
+    for i, wid in enumerate(ids):
+        m = batch.manifolds(wid)
+        c = int(f["count"][i])
+        if c == 0:
+            assert len(m["body_a"]) == 0, i
+            continue
+        if len(m["body_a"]) == 0:
+            # SAT within the margin on all 15 axes does not imply swept-AABB overlap; such a
+            # pair is legitimately dropped by the broad phase before the narrow phase runs
+            assert not aabbs_overlap(f, i), i
+            continue
+        assert m["num_contacts"][0] == c, i
+        assert np.array_equal(m["feature"][0, :c], f["keys"][i, :c]), i
+        assert bits_equal(m["r_a"][0].reshape(8, 3)[:c], f["ra"][i, :c]), i
+        assert bits_equal(m["r_b"][0].reshape(8, 3)[:c], f["rb"][i, :c]), i
+        assert bits_equal(m["basis"][0], f["basis"][i]), i
+        checked += 1
+    assert checked > 100
+
+
+def test_batch_alone_equals_in_mix():
+    worlds = [scenes.pyramid(rows=12, seed=3), scenes.pile(300, seed=51), scenes.digest_mixed(),
+              scenes.chains(4, links=16, grid=(2, 2), pitch_x=3.0), scenes.pile(40, seed=52), scenes.digest_stack()]
+    p = P(iterations=10)
+    batch = pkg.Batch(p, 16, 4000, 2000)
+    ids = [batch.add(b, j) for b, j in worlds]
+    alone = [oracle.OracleWorld(p) for _ in worlds]
+    for o, (b, j) in zip(alone, worlds):
+        o.upload(b, j)
+    for s in range(60):
+        batch.step()
+        for o in alone:
+            o.step()
+        if s == 29:
+            # a world leaves and a new one arrives mid-run; the others must not notice
+            batch.remove(ids[1])
+            nb, nj = scenes.pile(120, seed=53)
+            new_id = batch.add(nb, nj)
+            o_new = oracle.OracleWorld(p)
+            o_new.upload(nb, nj)
+            ids[1], alone[1] = new_id, o_new
+        if (s + 1) % 10 == 0:
+            for wid, o in zip(ids, alone):
+                assert not state_mismatch(batch.download(wid), o.download()), (s, wid)
+
+
+def test_full_size_pile_digest_matches_oracle():
+    """C2 at its full size (8192 boxes), 40 steps."""
+    b, j = scenes.pile(8192, seed=2)
+    w = gpu_world(b, j, P(iterations=10))
+    o = oracle.OracleWorld(P(iterations=10))
+    o.upload(b, j)
+    w.step(40)
+    o.step(40)
+    assert pkg.state_digest(w.download()) == o.digest()
+
+
+def test_statistics_match():
+    """Penetration distribution (depth = max(0, 1 mm - C0.n) after rebase) and energy
+    agree between the HIP path and the oracle after a long run."""
+    b, j = scenes.pyramid(rows=14, seed=9)
+    w, o = lockstep(b, j, P(iterations=10), 180, every=60)
+    gm, om = w.manifolds(), o.manifolds()
+
+    def depths(m):
+        n = m["num_contacts"]
+        c0 = m["c0"].reshape(len(n), 8, 3)[:, :, 0]
+        mask = np.arange(8)[None, :] < n[:, None]
+        return np.maximum(0.0, 0.001 - c0[mask])
+    dg, do = depths(gm), depths(om)
+    assert np.array_equal(np.percentile(dg, [50, 99]), np.percentile(do, [50, 99]))
