@@ -197,6 +197,7 @@ def load_library(path=LIB_PATH):
         "avbd_world_set_params": (C.c_int32, [vp, C.POINTER(Params)]),
         "avbd_world_upload": (C.c_int32, [vp, C.POINTER(Bodies), C.POINTER(Joints)]),
         "avbd_world_step": (C.c_int32, [vp]),
+        "avbd_world_write_bodies": (C.c_int32, [vp, C.POINTER(Bodies)]),
         "avbd_world_step_interleaved": (C.c_int32, [vp, C.c_int32, C.c_int32]),
         "avbd_world_download": (C.c_int32, [vp, C.POINTER(Bodies), C.POINTER(Joints)]),
         "avbd_world_get_manifolds": (C.c_int32, [vp, C.POINTER(Manifolds)]),

@@ -11,3 +11,9 @@ HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
     -Xclang -target-feature -Xclang -packed-fp32-ops \
     -Wall -Wno-unused-function -Wno-unused-result \
     -o "$HERE/libavbd_hip.so" "$HERE/csrc/engine.hip"
+
+# C++ facade test program (links the library; runs only where a GPU is present)
+mkdir -p "$HERE/bin"
+g++ -std=c++17 -O2 -ffp-contract=off -I"$HERE/../include" -I/opt/rocm/include \
+    -o "$HERE/bin/facade_test" "$HERE/../tests/cpp/facade_test.cpp" \
+    -L"$HERE" -lavbd_hip -Wl,-rpath,'$ORIGIN/..'

@@ -676,6 +676,32 @@ struct Engine {
         return AVBD_OK;
     }
 
+    avbd_status writeBodies(int w, const avbd_bodies *b)
+    {
+        const WorldInfo &wi = worlds[w];
+        if (b->count != wi.bodyCount) return fail(AVBD_ERR_INVALID, "body count differs from the uploaded world");
+        const int n = wi.bodyCount, o = wi.bodyBase;
+        std::vector<float4> tmp(n);
+        auto put = [&](float4 *dst, const float *src, int comps) -> hipError_t {
+            if (!src) return hipSuccess;
+            hipError_t e = hipMemcpy(tmp.data(), dst + o, sizeof(float4) * n, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return e;
+            for (int i = 0; i < n; i++) {
+                float v[4] = {tmp[i].x, tmp[i].y, tmp[i].z, tmp[i].w};
+                for (int c = 0; c < comps; c++) v[c] = src[comps * i + c];
+                tmp[i] = make_float4(v[0], v[1], v[2], v[3]);
+            }
+            return hipMemcpy(dst + o, tmp.data(), sizeof(float4) * n, hipMemcpyHostToDevice);
+        };
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(put(S.pos, b->position, 3));
+        HIPCHK(put(S.rot, b->rotation, 4));
+        HIPCHK(put(S.vel, b->velocity, 3));
+        HIPCHK(put(S.angVel, b->angular_velocity, 3));
+        HIPCHK(put(S.prevVel, b->prev_velocity, 3));
+        return AVBD_OK;
+    }
+
     avbd_status clearWorld(int w)
     {
         WorldInfo &wi = worlds[w];
@@ -1151,6 +1177,13 @@ avbd_status avbd_world_upload(avbd_world *world, const avbd_bodies *bodies, cons
     wi.jointBase = e.jointAlloc.take(joints ? joints->count : 0);
     wi.slotBase = e.slotAlloc.take(e.Scap);
     return e.uploadWorld(0, bodies, joints, e.Scap);
+}
+
+avbd_status avbd_world_write_bodies(avbd_world *world, const avbd_bodies *bodies)
+{
+    if (!world || !bodies) return fail(AVBD_ERR_INVALID, "null argument");
+    if (!world->eng.worlds[0].active) return fail(AVBD_ERR_INVALID, "nothing uploaded");
+    return world->eng.writeBodies(0, bodies);
 }
 
 avbd_status avbd_world_step(avbd_world *world)

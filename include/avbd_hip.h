@@ -215,6 +215,11 @@ avbd_status avbd_world_set_params(avbd_world *world, const avbd_params *params);
  * forces.hpp:68). */
 avbd_status avbd_world_upload(avbd_world *world, const avbd_bodies *bodies,
                               const avbd_joints *joints);
+/* Overwrite the dynamic state (position, rotation, velocity, angular velocity, previous
+ * velocity; null arrays are skipped) of the uploaded bodies, keeping manifolds and joints:
+ * what a host does when it writes `Rigid` fields between steps.  count must equal the
+ * uploaded body count. */
+avbd_status avbd_world_write_bodies(avbd_world *world, const avbd_bodies *bodies);
 /* One step: replaces `Solver::step()` (src/avbd/solver.h:234, solver_step.cpp:44-75). */
 avbd_status avbd_world_step(avbd_world *world);
 /* Replaces `Solver::stepInterleaved(substeps, rounds)` (solver.h:241, solver_step.cpp:109-160);

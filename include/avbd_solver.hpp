@@ -1,0 +1,487 @@
+// avbd_solver.hpp - C++ facade with the reference solver's interface over the C ABI.
+//
+// A drop-in for the reference's in-process API (src/avbd/solver.h:50-374,
+// src/avbd/types.hpp:75-139, src/avbd/forces.hpp:19-87): bodies and joints are created with
+// `new Rigid(&solver, size, density, friction, position[, velocity])` and
+// `new Joint(&solver, a, b, rA, rB[, kLin, kAng, fracture])`, which prepend themselves to the
+// solver's intrusive lists exactly as the reference's do; the caller sets the same public
+// fields, calls step() / stepInterleaved(), and reads the same `Rigid` / `Joint` fields.
+// The state lives on the GPU: each step writes the bodies' dynamic fields to the device,
+// steps there (avbd_world_step), and reads them back.  A topology change (a body or joint
+// created or deleted) re-uploads the lists and carries every surviving manifold over, so
+// warm starting continues as in the reference.  The solver deletes a broken joint during the
+// step after it broke, as the reference does (solver_solve.cpp:39-51).
+//
+// Errors: the reference has no error path; here a failed device call leaves the state
+// unchanged and sets lastStatus / lastError().  Header-only; link libavbd_hip.so.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <unordered_map>
+#include <vector>
+
+#include "avbd_hip.h"
+
+namespace avbd_hip {
+
+struct float3 {
+    float x = 0, y = 0, z = 0;
+    float &operator[](int i) { return (&x)[i]; }
+    float operator[](int i) const { return (&x)[i]; }
+};
+struct quat {
+    float x = 0, y = 0, z = 0, w = 1;
+};
+
+struct Solver;
+
+// reference types.hpp:75-139 (box bodies)
+struct Rigid {
+    Solver *solver;
+    Rigid *next = nullptr;
+    float3 positionLin;
+    quat positionAng;
+    float3 velocityLin;
+    float3 velocityAng;
+    float3 prevVelocityLin;
+    float3 size;
+    float mass = 0;
+    float3 moment;
+    float friction = 0;
+    float radius = 0;
+    float gravity = 0;
+    uint32_t collisionLayer = 1;
+    uint32_t collisionMask = 1;
+    uint8_t axisLockLinear = 0;
+    uint8_t axisLockAngular = 0;
+
+    inline Rigid(Solver *p_solver, float3 p_size, float p_density, float p_friction, float3 p_position,
+                 float3 p_velocity = float3{0, 0, 0});
+    inline ~Rigid();
+    Rigid(const Rigid &) = delete;
+    Rigid &operator=(const Rigid &) = delete;
+};
+
+// reference forces.hpp:19-55
+struct Force {
+    Solver *solver;
+    Rigid *bodyA;
+    Rigid *bodyB;
+    Force *next = nullptr;
+    void *owner = nullptr;
+    inline Force(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bodyB);
+    inline virtual ~Force();
+    virtual bool isBroken() const { return false; }
+    Force(const Force &) = delete;
+    Force &operator=(const Force &) = delete;
+};
+
+// reference forces.hpp:58-87 (joint.cpp): linear ball socket + optional angular lock
+struct Joint : Force {
+    float3 rA, rB;
+    float3 C0Lin, C0Ang;
+    float3 penaltyLin, penaltyAng;
+    float3 lambdaLin, lambdaAng;
+    float stiffnessLin, stiffnessAng, fracture;
+    float torqueArm;
+    bool broken = false;
+
+    inline Joint(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bodyB, float3 p_rA, float3 p_rB,
+                 float p_stiffnessLin = std::numeric_limits<float>::infinity(), float p_stiffnessAng = 0.0f,
+                 float p_fracture = std::numeric_limits<float>::infinity());
+    // Break now, discarding dual state (joint.cpp:162-169).
+    void breakNow()
+    {
+        penaltyLin = penaltyAng = lambdaLin = lambdaAng = float3{};
+        broken = true;
+        dirty = true;
+    }
+    bool isBroken() const override { return broken; }
+    bool dirty = false;   // host-side edit to push before the next step
+};
+
+// reference solver.h:50-144
+struct Solver {
+    float dt = 1.0f / 60.0f;
+    float collisionDt = 1.0f / 60.0f;
+    float gravity = -10.0f;
+    int iterations = 8;
+    float alpha = 0.99f;
+    float betaLin = 10000.0f;
+    float betaAng = 100.0f;
+    float gamma = 0.999f;
+    int roundsThisStep = 8;
+    float newtonRatio = 1.0f;
+    float stiffnessDecay = 1.0f;
+    bool adaptiveBeta = false;
+    float adaptiveBetaErrorFloor = 1e-3f;
+    float convergenceThreshold = 0.0f;
+    float jacobiRelaxation = 0.7f;
+    bool contactOnlyOverflow = true;
+    float stickDriftHeal = 0.002f;
+    float maxPenetration = 0.02f;
+    float jacobianRebuildDistance = 0.0f;
+    float speculativeMargin = 0.01f;
+    bool positionProjection = true;
+    int projectionIterations = 4;
+    float projectionBeta = 1.0f;
+    float projectionSlop = 0.008f;
+    float projectionMaxCorrection = 0.0f;
+    int contactGraceTicks = 0;
+    int threads = 0;
+    float sleepLinearThreshold = 0.1f;
+    float sleepAngularThreshold = 0.139626f;
+    int sleepFrames = 0;
+    // The reference's compile-time PENALTY_MAX (types.hpp:26) as a field.
+    float penaltyMax = 1e10f;
+
+    Rigid *bodies = nullptr;
+    Force *forces = nullptr;
+
+    avbd_status lastStatus = AVBD_OK;
+    const char *lastError() const { return avbd_last_error(); }
+
+    Solver() = default;
+    ~Solver()
+    {
+        clear();
+        if (world) avbd_world_destroy(world);
+    }
+    Solver(const Solver &) = delete;
+    Solver &operator=(const Solver &) = delete;
+
+    // Delete every body and force (solver_step.cpp:29-40).
+    void clear()
+    {
+        while (forces) delete forces;
+        while (bodies) delete bodies;
+        topologyChanged = true;
+        uploaded.clear();
+        uploadedJoints.clear();
+    }
+
+    void step() { advance(1, 0); }
+    void stepInterleaved(int substeps, int roundsPerSubstep) { advance(substeps, roundsPerSubstep); }
+
+    // Manifold accessors (solver.h:170-231), read from the device.
+    int manifoldCount()
+    {
+        avbd_step_stats s{};
+        return world && avbd_world_stats(world, &s) == AVBD_OK ? s.manifolds : 0;
+    }
+    int manifoldContactPoints()
+    {
+        avbd_step_stats s{};
+        return world && avbd_world_stats(world, &s) == AVBD_OK ? s.contacts : 0;
+    }
+    avbd_step_stats lastStepStats()
+    {
+        avbd_step_stats s{};
+        if (world) avbd_world_stats(world, &s);
+        return s;
+    }
+
+    // --- internal ---------------------------------------------------------------
+    bool topologyChanged = true;
+
+  private:
+    avbd_world *world = nullptr;
+    int capBodies = 0, capJoints = 0;
+    std::vector<Rigid *> uploaded;
+    std::vector<Joint *> uploadedJoints;
+
+    avbd_params params() const
+    {
+        avbd_params p;
+        avbd_params_default(&p);
+        p.dt = dt;
+        p.gravity = gravity;
+        p.iterations = iterations;
+        p.alpha = alpha;
+        p.beta_lin = betaLin;
+        p.beta_ang = betaAng;
+        p.gamma = gamma;
+        p.newton_ratio = newtonRatio;
+        p.stiffness_decay = stiffnessDecay;
+        p.adaptive_beta = adaptiveBeta ? 1 : 0;
+        p.adaptive_beta_error_floor = adaptiveBetaErrorFloor;
+        p.convergence_threshold = convergenceThreshold;
+        p.jacobi_relaxation = jacobiRelaxation;
+        p.contact_only_overflow = contactOnlyOverflow ? 1 : 0;
+        p.stick_drift_heal = stickDriftHeal;
+        p.max_penetration = maxPenetration;
+        p.jacobian_rebuild_distance = jacobianRebuildDistance;
+        p.speculative_margin = speculativeMargin;
+        p.position_projection = positionProjection ? 1 : 0;
+        p.projection_iterations = projectionIterations;
+        p.projection_beta = projectionBeta;
+        p.projection_slop = projectionSlop;
+        p.projection_max_correction = projectionMaxCorrection;
+        p.contact_grace_ticks = contactGraceTicks;
+        p.threads = threads;
+        p.sleep_linear_threshold = sleepLinearThreshold;
+        p.sleep_angular_threshold = sleepAngularThreshold;
+        p.sleep_frames = sleepFrames;
+        p.penalty_max = penaltyMax;
+        return p;
+    }
+
+    struct Arrays {
+        std::vector<float> size, mass, moment, friction, radius, gravity, pos, rot, vel, ang, prev;
+        std::vector<uint32_t> layer, mask;
+        std::vector<uint8_t> ll, la;
+        avbd_bodies view(int n)
+        {
+            avbd_bodies b{};
+            b.count = n;
+            b.size = size.data(); b.mass = mass.data(); b.moment = moment.data(); b.friction = friction.data();
+            b.radius = radius.data(); b.gravity = gravity.data(); b.layer = layer.data(); b.mask = mask.data();
+            b.lock_linear = ll.data(); b.lock_angular = la.data(); b.position = pos.data(); b.rotation = rot.data();
+            b.velocity = vel.data(); b.angular_velocity = ang.data(); b.prev_velocity = prev.data();
+            return b;
+        }
+    };
+
+    void gather(const std::vector<Rigid *> &list, Arrays &a)
+    {
+        const size_t n = list.size();
+        a.size.resize(3 * n); a.mass.resize(n); a.moment.resize(3 * n); a.friction.resize(n); a.radius.resize(n);
+        a.gravity.resize(n); a.pos.resize(3 * n); a.rot.resize(4 * n); a.vel.resize(3 * n); a.ang.resize(3 * n);
+        a.prev.resize(3 * n); a.layer.resize(n); a.mask.resize(n); a.ll.resize(n); a.la.resize(n);
+        for (size_t i = 0; i < n; i++) {
+            const Rigid *r = list[i];
+            for (int c = 0; c < 3; c++) {
+                a.size[3 * i + c] = r->size[c];
+                a.moment[3 * i + c] = r->moment[c];
+                a.pos[3 * i + c] = r->positionLin[c];
+                a.vel[3 * i + c] = r->velocityLin[c];
+                a.ang[3 * i + c] = r->velocityAng[c];
+                a.prev[3 * i + c] = r->prevVelocityLin[c];
+            }
+            a.rot[4 * i] = r->positionAng.x; a.rot[4 * i + 1] = r->positionAng.y;
+            a.rot[4 * i + 2] = r->positionAng.z; a.rot[4 * i + 3] = r->positionAng.w;
+            a.mass[i] = r->mass; a.friction[i] = r->friction; a.radius[i] = r->radius; a.gravity[i] = r->gravity;
+            a.layer[i] = r->collisionLayer; a.mask[i] = r->collisionMask;
+            a.ll[i] = r->axisLockLinear; a.la[i] = r->axisLockAngular;
+        }
+    }
+
+    // Re-upload the lists, carrying the manifolds of surviving body pairs over.
+    bool rebuild(const std::vector<Rigid *> &list, const std::vector<Joint *> &jl)
+    {
+        const int n = static_cast<int>(list.size()), nj = static_cast<int>(jl.size());
+        std::vector<int32_t> ma, mb, num, feat;
+        std::vector<float> basis, fr, ra, rb, c0, pen, lam, beta;
+        std::vector<uint8_t> stick;
+        int keepCount = 0;
+        if (world && !uploaded.empty()) {
+            avbd_step_stats st{};
+            avbd_world_stats(world, &st);
+            const int m = st.manifolds > 0 ? st.manifolds : 0;
+            ma.resize(m); mb.resize(m); num.resize(m); feat.resize(8 * m); basis.resize(9 * m); fr.resize(m);
+            ra.resize(24 * m); rb.resize(24 * m); c0.resize(24 * m); pen.resize(24 * m); lam.resize(24 * m);
+            beta.resize(8 * m); stick.resize(8 * m);
+            avbd_manifolds mf{m, ma.data(), mb.data(), basis.data(), fr.data(), num.data(), feat.data(), ra.data(),
+                              rb.data(), c0.data(), pen.data(), lam.data(), stick.data(), beta.data()};
+            if (m > 0 && avbd_world_get_manifolds(world, &mf) != AVBD_OK) mf.count = 0;
+            std::unordered_map<const Rigid *, int> now;
+            for (int i = 0; i < n; i++) now[list[i]] = i;
+            for (int k = 0; k < mf.count; k++) {
+                auto ia = now.find(uploaded[ma[k]]), ib = now.find(uploaded[mb[k]]);
+                if (ia == now.end() || ib == now.end()) continue;
+                int a = ia->second, b = ib->second;
+                if (a > b) continue;   // reordered pair: the broad phase recreates it
+                ma[keepCount] = a; mb[keepCount] = b; num[keepCount] = num[k]; fr[keepCount] = fr[k];
+                std::memmove(&basis[9 * keepCount], &basis[9 * k], 9 * sizeof(float));
+                std::memmove(&feat[8 * keepCount], &feat[8 * k], 8 * sizeof(int32_t));
+                std::memmove(&stick[8 * keepCount], &stick[8 * k], 8);
+                std::memmove(&beta[8 * keepCount], &beta[8 * k], 8 * sizeof(float));
+                for (auto *v : {&ra, &rb, &c0, &pen, &lam})
+                    std::memmove(&(*v)[24 * keepCount], &(*v)[24 * k], 24 * sizeof(float));
+                keepCount++;
+            }
+        }
+        if (!world || n > capBodies || nj > capJoints) {
+            if (world) avbd_world_destroy(world);
+            world = nullptr;
+            capBodies = n * 2 + 64;
+            capJoints = nj * 2 + 16;
+            const avbd_params p = params();
+            lastStatus = avbd_world_create(&p, capBodies, capJoints, 0, &world);
+            if (lastStatus != AVBD_OK) return false;
+        }
+        Arrays a;
+        gather(list, a);
+        avbd_bodies bv = a.view(n);
+        std::vector<int32_t> ja(nj), jb(nj);
+        std::vector<float> jra(3 * nj), jrb(3 * nj), ksl(nj), ksa(nj), jfr(nj), arm(nj), c0l(3 * nj), c0a(3 * nj),
+                pl(3 * nj), pa(3 * nj), ll(3 * nj), la(3 * nj);
+        std::vector<uint8_t> br(nj);
+        std::unordered_map<const Rigid *, int> idx;
+        for (int i = 0; i < n; i++) idx[list[i]] = i;
+        for (int k = 0; k < nj; k++) {
+            const Joint *j = jl[k];
+            ja[k] = j->bodyA ? idx[j->bodyA] : -1;
+            jb[k] = idx[j->bodyB];
+            for (int c = 0; c < 3; c++) {
+                jra[3 * k + c] = j->rA[c]; jrb[3 * k + c] = j->rB[c];
+                c0l[3 * k + c] = j->C0Lin[c]; c0a[3 * k + c] = j->C0Ang[c];
+                pl[3 * k + c] = j->penaltyLin[c]; pa[3 * k + c] = j->penaltyAng[c];
+                ll[3 * k + c] = j->lambdaLin[c]; la[3 * k + c] = j->lambdaAng[c];
+            }
+            ksl[k] = j->stiffnessLin; ksa[k] = j->stiffnessAng; jfr[k] = j->fracture; arm[k] = j->torqueArm;
+            br[k] = j->broken ? 1 : 0;
+        }
+        avbd_joints jv{nj, ja.data(), jb.data(), jra.data(), jrb.data(), ksl.data(), ksa.data(), jfr.data(),
+                       arm.data(), c0l.data(), c0a.data(), pl.data(), pa.data(), ll.data(), la.data(), br.data()};
+        lastStatus = avbd_world_upload(world, &bv, &jv);
+        if (lastStatus != AVBD_OK) return false;
+        if (keepCount > 0) {
+            avbd_manifolds mf{keepCount, ma.data(), mb.data(), basis.data(), fr.data(), num.data(), feat.data(),
+                              ra.data(), rb.data(), c0.data(), pen.data(), lam.data(), stick.data(), beta.data()};
+            lastStatus = avbd_world_set_manifolds(world, &mf);
+            if (lastStatus != AVBD_OK) return false;
+        }
+        uploaded = list;
+        uploadedJoints = jl;
+        return true;
+    }
+
+    void advance(int substeps, int rounds)
+    {
+        std::vector<Rigid *> list;
+        for (Rigid *r = bodies; r; r = r->next) list.push_back(r);
+        std::vector<Joint *> jl;
+        for (Force *f = forces; f; f = f->next) {
+            Joint *j = static_cast<Joint *>(f);
+            if (j->dirty) { topologyChanged = true; j->dirty = false; }
+            jl.push_back(j);
+        }
+        const avbd_params p = params();
+        if (world) {
+            lastStatus = avbd_world_set_params(world, &p);
+            if (lastStatus != AVBD_OK) return;
+        }
+        if (topologyChanged || !world) {
+            if (!rebuild(list, jl)) return;
+            topologyChanged = false;
+        } else {
+            Arrays a;
+            gather(list, a);
+            avbd_bodies bv = a.view(static_cast<int>(list.size()));
+            lastStatus = avbd_world_write_bodies(world, &bv);
+            if (lastStatus != AVBD_OK) return;
+        }
+        // joints that were broken before this step are reaped by it (solver_solve.cpp:39-51)
+        std::vector<Joint *> reaped;
+        for (Joint *j : jl)
+            if (j->broken) reaped.push_back(j);
+        collisionDt = substeps > 1 ? dt * static_cast<float>(substeps) : dt;
+        lastStatus = substeps > 1 ? avbd_world_step_interleaved(world, substeps, rounds) : avbd_world_step(world);
+        if (lastStatus != AVBD_OK) return;
+        pull(list, jl, reaped);
+    }
+
+    void pull(const std::vector<Rigid *> &list, std::vector<Joint *> jl, const std::vector<Joint *> &reaped)
+    {
+        const int n = static_cast<int>(list.size());
+        std::vector<float> pos(3 * n), rot(4 * n), vel(3 * n), ang(3 * n), prev(3 * n);
+        avbd_bodies b{};
+        b.count = n;
+        b.position = pos.data(); b.rotation = rot.data(); b.velocity = vel.data();
+        b.angular_velocity = ang.data(); b.prev_velocity = prev.data();
+        const int nj = static_cast<int>(jl.size());
+        std::vector<int32_t> ja(nj), jb(nj);
+        std::vector<float> c0l(3 * nj), c0a(3 * nj), pl(3 * nj), pa(3 * nj), ll(3 * nj), la(3 * nj);
+        std::vector<uint8_t> br(nj);
+        avbd_joints j{nj, ja.data(), jb.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c0l.data(),
+                      c0a.data(), pl.data(), pa.data(), ll.data(), la.data(), br.data()};
+        lastStatus = avbd_world_download(world, &b, nj ? &j : nullptr);
+        if (lastStatus != AVBD_OK) return;
+        for (int i = 0; i < n; i++) {
+            Rigid *r = list[i];
+            for (int c = 0; c < 3; c++) {
+                r->positionLin[c] = pos[3 * i + c];
+                r->velocityLin[c] = vel[3 * i + c];
+                r->velocityAng[c] = ang[3 * i + c];
+                r->prevVelocityLin[c] = prev[3 * i + c];
+            }
+            r->positionAng = quat{rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]};
+        }
+        // the device compacted the reaped joints away; drop them from the host list too
+        for (Joint *dead : reaped) {
+            for (size_t k = 0; k < jl.size(); k++)
+                if (jl[k] == dead) { jl.erase(jl.begin() + static_cast<long>(k)); break; }
+            dead->dirty = false;
+            delete dead;
+        }
+        for (int k = 0; k < j.count && k < static_cast<int>(jl.size()); k++) {
+            Joint *x = jl[k];
+            for (int c = 0; c < 3; c++) {
+                x->C0Lin[c] = c0l[3 * k + c]; x->C0Ang[c] = c0a[3 * k + c];
+                x->penaltyLin[c] = pl[3 * k + c]; x->penaltyAng[c] = pa[3 * k + c];
+                x->lambdaLin[c] = ll[3 * k + c]; x->lambdaAng[c] = la[3 * k + c];
+            }
+            x->broken = br[k] != 0;
+        }
+        if (!reaped.empty()) {
+            // the device list changed with them; keep the upload mapping in step
+            uploadedJoints = jl;
+            topologyChanged = false;
+        }
+    }
+};
+
+inline Rigid::Rigid(Solver *p_solver, float3 p_size, float p_density, float p_friction, float3 p_position,
+                    float3 p_velocity)
+    : solver(p_solver), positionLin(p_position), velocityLin(p_velocity), prevVelocityLin(p_velocity),
+      size(p_size), friction(p_friction), gravity(p_solver->gravity)
+{
+    next = p_solver->bodies;
+    p_solver->bodies = this;
+    const float s[3] = {p_size.x, p_size.y, p_size.z};
+    float mom[3];
+    avbd_box_properties(s, p_density, &mass, mom, &radius);
+    moment = float3{mom[0], mom[1], mom[2]};
+    p_solver->topologyChanged = true;
+}
+
+inline Rigid::~Rigid()
+{
+    Rigid **p = &solver->bodies;
+    while (*p != this) p = &(*p)->next;
+    *p = next;
+    solver->topologyChanged = true;
+}
+
+inline Force::Force(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bodyB)
+    : solver(p_solver), bodyA(p_bodyA), bodyB(p_bodyB)
+{
+    next = p_solver->forces;
+    p_solver->forces = this;
+}
+
+inline Force::~Force()
+{
+    Force **p = &solver->forces;
+    while (*p != this) p = &(*p)->next;
+    *p = next;
+    solver->topologyChanged = true;
+}
+
+inline Joint::Joint(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bodyB, float3 p_rA, float3 p_rB, float p_stiffnessLin,
+                    float p_stiffnessAng, float p_fracture)
+    : Force(p_solver, p_bodyA, p_bodyB), rA(p_rA), rB(p_rB), stiffnessLin(p_stiffnessLin),
+      stiffnessAng(p_stiffnessAng), fracture(p_fracture)
+{
+    // joint.cpp:28-42: |sizeA + sizeB|^2 with linalg's ((0 + x^2) + y^2) + z^2
+    const float3 sa = p_bodyA ? p_bodyA->size : float3{0, 0, 0};
+    const float sx = sa.x + p_bodyB->size.x, sy = sa.y + p_bodyB->size.y, sz = sa.z + p_bodyB->size.z;
+    torqueArm = ((0.0f + sx * sx) + sy * sy) + sz * sz;
+    p_solver->topologyChanged = true;
+}
+
+} // namespace avbd_hip

@@ -1,0 +1,151 @@
+// Parity checks of the HIP step through the C++ facade (include/avbd_solver.hpp), written
+// the way the reference's own core tests build their scenes (test/core_test.cpp:391-414,
+// 612-631, 271-300): `new Rigid(&s, ...)`, `new Joint(&s, ...)`, `s.step()`.  The expected
+// digests are the reference's recorded ones (tools/baseline/digests.txt).
+#include <cinttypes>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "avbd_solver.hpp"
+
+using namespace avbd_hip;
+
+static int g_checks = 0, g_failures = 0;
+
+static void report(bool ok, const char *name, const char *detail)
+{
+    g_checks++;
+    if (!ok) g_failures++;
+    std::printf("%s %-34s %s\n", ok ? "  ok " : "FAIL ", name, detail);
+}
+
+// FNV-1a over pos/quat/vel/omega in list order (test/core_test.cpp:133-149).
+static uint64_t state_digest(const Solver &s)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](const void *p, size_t n) {
+        const unsigned char *c = static_cast<const unsigned char *>(p);
+        for (size_t i = 0; i < n; i++) { h ^= c[i]; h *= 1099511628211ull; }
+    };
+    for (Rigid *b = s.bodies; b != nullptr; b = b->next) {
+        mix(&b->positionLin, 12);
+        mix(&b->positionAng, 16);
+        mix(&b->velocityLin, 12);
+        mix(&b->velocityAng, 12);
+    }
+    return h;
+}
+
+static uint64_t stack_digest()
+{
+    Solver s;
+    new Rigid(&s, {100, 100, 1}, 0.0f, 0.5f, {0, 0, 0});
+    for (int i = 0; i < 10; i++) new Rigid(&s, {1, 1, 1}, 1.0f, 0.5f, {0, 0, i * 1.5f + 1.0f});
+    for (int i = 0; i < 300; i++) s.step();
+    return s.lastStatus == AVBD_OK ? state_digest(s) : 0;
+}
+
+static uint64_t interleaved_digest()
+{
+    Solver s;
+    s.dt = 1.0f / 120.0f;
+    new Rigid(&s, {100, 100, 1}, 0.0f, 0.5f, {0, 0, 0});
+    for (int i = 0; i < 10; i++) new Rigid(&s, {1, 1, 1}, 1.0f, 0.5f, {0, 0, i * 1.5f + 1.0f});
+    for (int i = 0; i < 300; i++) s.stepInterleaved(2, 3);
+    return s.lastStatus == AVBD_OK ? state_digest(s) : 0;
+}
+
+static void build_mixed(Solver &s)
+{
+    new Rigid(&s, {100, 100, 1}, 0.0f, 0.5f, {0, 0, 0});
+    Rigid *prev = nullptr;
+    for (int i = 0; i < 30; i++) {
+        Rigid *link = new Rigid(&s, {1, 1, 1}, 1.0f, 0.5f, {0, 0, 20.0f - i * 1.5f});
+        if (prev != nullptr) new Joint(&s, prev, link, {0, 0, -0.5f}, {0, 0, 0.5f}, 1000.0f, 250.0f);
+        prev = link;
+    }
+    for (int i = 0; i < 40; i++) new Rigid(&s, {1, 1, 1}, 1.0f, 0.5f, {5.0f + (float)i * 1.2f, 0, 3.0f + (float)(i % 5)});
+}
+
+static uint64_t mixed_digest()
+{
+    Solver s;
+    build_mixed(s);
+    for (int i = 0; i < 240; i++) s.step();
+    return s.lastStatus == AVBD_OK ? state_digest(s) : 0;
+}
+
+// A body created mid-run keeps every existing manifold warm (the reference prepends it and
+// leaves the manifold store alone).  Added far from a resting stack, it must leave the
+// stack's trajectory bit-identical to a run without it; a lost warm start (penalty and
+// lambda restarting from the floor) would make the stack sink and differ at once.
+static uint64_t digest_except(const Solver &s, const Rigid *skip)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](const void *p, size_t n) {
+        const unsigned char *c = static_cast<const unsigned char *>(p);
+        for (size_t i = 0; i < n; i++) { h ^= c[i]; h *= 1099511628211ull; }
+    };
+    for (Rigid *b = s.bodies; b != nullptr; b = b->next) {
+        if (b == skip) continue;
+        mix(&b->positionLin, 12);
+        mix(&b->positionAng, 16);
+        mix(&b->velocityLin, 12);
+        mix(&b->velocityAng, 12);
+    }
+    return h;
+}
+
+static void test_topology_change_keeps_warm_start()
+{
+    Solver a, b;
+    for (Solver *s : {&a, &b}) {
+        new Rigid(s, {100, 100, 1}, 0.0f, 0.5f, {0, 0, 0});
+        for (int i = 0; i < 5; i++) new Rigid(s, {1, 1, 1}, 1.0f, 0.5f, {0, 0, 1.0f + i * 1.01f});
+    }
+    for (int i = 0; i < 60; i++) { a.step(); b.step(); }
+    Rigid *late = new Rigid(&a, {1, 1, 1}, 1.0f, 0.5f, {6, 0, 3});
+    const int before = a.manifoldCount();
+    for (int i = 0; i < 60; i++) { a.step(); b.step(); }
+    const uint64_t da = digest_except(a, late), db = digest_except(b, nullptr);
+    char d[160];
+    std::snprintf(d, sizeof(d), "manifolds kept %d, stack digests %016" PRIx64 " / %016" PRIx64, before, da, db);
+    report(before == 5 && da == db && a.lastStatus == AVBD_OK && late->positionLin.z < 1.1f,
+           "topology change keeps warm start", d);
+}
+
+// A joint broken by the host is reaped by the next step, as in the reference.
+static void test_break_now_reaps()
+{
+    Solver s;
+    Rigid *anchor = new Rigid(&s, {1, 1, 1}, 0.0f, 0.5f, {0, 0, 10});
+    Rigid *link = new Rigid(&s, {1, 1, 1}, 1.0f, 0.5f, {0, 0, 9});
+    Joint *j = new Joint(&s, anchor, link, {0, 0, -0.5f}, {0, 0, 0.5f});
+    for (int i = 0; i < 20; i++) s.step();
+    const float zHeld = link->positionLin.z;
+    j->breakNow();
+    s.step();
+    int joints = 0;
+    for (Force *f = s.forces; f; f = f->next) joints++;
+    for (int i = 0; i < 30; i++) s.step();
+    char d[128];
+    std::snprintf(d, sizeof(d), "joints after reap %d, z held %.3f -> %.3f", joints, zHeld, link->positionLin.z);
+    report(joints == 0 && link->positionLin.z < zHeld - 0.5f && s.lastStatus == AVBD_OK, "breakNow reaps joint", d);
+}
+
+int main()
+{
+    const uint64_t want[3] = {0xfbac4f08cb1c977cull, 0xb03adeb37742850aull, 0xfb3ca7a3162035c2ull};
+    const uint64_t got[3] = {stack_digest(), interleaved_digest(), mixed_digest()};
+    const char *names[3] = {"digest stack_300", "digest interleaved_300", "digest mixed_240"};
+    for (int k = 0; k < 3; k++) {
+        char d[96];
+        std::snprintf(d, sizeof(d), "0x%016" PRIx64 " (reference 0x%016" PRIx64 ")", got[k], want[k]);
+        report(got[k] == want[k], names[k], d);
+    }
+    test_topology_change_keeps_warm_start();
+    test_break_now_reaps();
+    std::printf("%d checks, %d failures\n", g_checks, g_failures);
+    return g_failures == 0 ? 0 : 1;
+}

@@ -115,10 +115,10 @@ def test_single_step_from_identical_state():
     b, j = scenes.pile(500, seed=31)
     o = oracle.OracleWorld(P(iterations=10))
     o.upload(b, j)
-    o.step(70)
+    o.step(150)
     snap = o.download()
     m = o.manifolds()
-    assert len(m["body_a"]) > 100
+    assert len(m["body_a"]) > 150
     start = {**b, **{k: snap[k] for k in ("position", "rotation", "velocity", "angular_velocity", "prev_velocity")}}
     w = gpu_world(start, j, P(iterations=10))
     w.set_manifolds(m)
