@@ -60,7 +60,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=120)
+    ap.add_argument("--warmup", type=int, default=150)
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--min-bodies", type=int, default=8)
     ap.add_argument("--max-bodies", type=int, default=2048)
@@ -70,6 +70,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one (for tracing tools)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -89,6 +90,8 @@ def main():
 
     batch = pkg.Batch(params, len(worlds), total_bodies, 0, 0)
     ids = [batch.add(b, j) for b, j in worlds]
+    if args.no_graph:
+        batch.use_graph(False)
     log(f"uploaded in {time.time() - t0:.1f}s")
 
     for s in range(args.warmup):
@@ -165,25 +168,28 @@ def main():
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        done_bodies, spent = 0, 0.0
+        done_bodies, spent, rounds = 0, 0.0, 0
         order = sorted(snaps, key=lambda w: -sizes[w])
-        used = []
+        oworlds = {}
         for w in order:
             start, m = snaps[w]
             o = oracle.OracleWorld(params)
             o.upload(start, worlds[w][1])
             o.set_manifolds(m)
-            steps = 3
-            a = time.perf_counter()
-            o.step(steps)
-            spent += time.perf_counter() - a
-            done_bodies += int((start["mass"] > 0).sum()) * steps
-            used.append(w)
-            if spent >= args.cpu_seconds:
-                break
+            oworlds[w] = o
+        # step the sampled worlds round-robin, one step each, until the time budget is spent
+        while spent < args.cpu_seconds and rounds < 200:
+            for w in order:
+                a = time.perf_counter()
+                oworlds[w].step()
+                spent += time.perf_counter() - a
+                done_bodies += int((snaps[w][0]["mass"] > 0).sum())
+            rounds += 1
+        used = order
         cpu = {"value": done_bodies / spent, "unit": "body-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{len(used)} of the same warmed-up worlds (largest first), 3 steps each, "
-                         f"{done_bodies} body-steps in {spent:.1f}s, 10 iterations, serial oracle"}
+               "sample": f"the {len(used)} sampled worlds (states copied from the GPU after warmup), "
+                         f"{rounds} steps each: {done_bodies} body-steps in {spent:.1f}s, "
+                         f"{iters} iterations, serial oracle on 1 host core"}
         log(f"cpu baseline: {cpu['value'] / 1e3:.1f} k body-steps/s")
 
     line = {

@@ -212,6 +212,7 @@ def load_library(path=LIB_PATH):
         "avbd_batch_add_world": (C.c_int32, [vp, C.POINTER(Bodies), C.POINTER(Joints), i32p]),
         "avbd_batch_remove_world": (C.c_int32, [vp, C.c_int32]),
         "avbd_batch_step": (C.c_int32, [vp]),
+        "avbd_batch_use_graph": (C.c_int32, [vp, C.c_int32]),
         "avbd_batch_download_world": (C.c_int32, [vp, C.c_int32, C.POINTER(Bodies), C.POINTER(Joints)]),
         "avbd_batch_world_count": (C.c_int32, [vp, i32p]),
         "avbd_batch_get_manifolds": (C.c_int32, [vp, C.c_int32, C.POINTER(Manifolds)]),

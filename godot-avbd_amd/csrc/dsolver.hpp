@@ -165,6 +165,74 @@ __device__ __noinline__ bool ldlt6Pivoted(const float (&ain)[6][6], const float 
     return __builtin_isfinite(worst) && worst < 1.0e1f;
 }
 
+AD void packA(const Block &k, float (&a)[6][6], float (&b)[6])
+{
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            a[r][c] = k.lhsLin.c[c][r];
+            a[r + 3][c + 3] = k.lhsAng.c[c][r];
+            a[r + 3][c] = k.lhsCross.c[c][r];
+            a[r][c + 3] = k.lhsCross.c[r][c];
+        }
+    b[0] = -k.rhsLin.x; b[1] = -k.rhsLin.y; b[2] = -k.rhsLin.z;
+    b[3] = -k.rhsAng.x; b[4] = -k.rhsAng.y; b[5] = -k.rhsAng.z;
+}
+
+// First half of types.hpp:176-197: the unpivoted LDL^T only.  Returns false when the
+// reference would fall through to the pivoted solve (queued by the caller).
+AD bool solveDeltaFast(const Block &k, v3 &dLin, v3 &dAng)
+{
+    float a[6][6], b[6], x[6];
+    packA(k, a, b);
+    if (!ldlt6(a, b, x)) return false;
+    dLin = v3{x[0], x[1], x[2]};
+    dAng = v3{x[3], x[4], x[5]};
+    return true;
+}
+
+// The rest of the chain for a queued block: pivoted LDL^T, else a zero update.
+__device__ void solveDeltaSlow(const Block &k, v3 &dLin, v3 &dAng)
+{
+    float a[6][6], b[6], x[6];
+    packA(k, a, b);
+    if (ldlt6Pivoted(a, b, x)) {
+        dLin = v3{x[0], x[1], x[2]};
+        dAng = v3{x[3], x[4], x[5]};
+        return;
+    }
+    dLin = v3{0, 0, 0};
+    dAng = v3{0, 0, 0};
+}
+
+AD void storeBlock(float *p, const Block &k)
+{
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            p[3 * c + r] = k.lhsLin.c[c][r];
+            p[9 + 3 * c + r] = k.lhsAng.c[c][r];
+            p[18 + 3 * c + r] = k.lhsCross.c[c][r];
+        }
+    p[27] = k.rhsLin.x; p[28] = k.rhsLin.y; p[29] = k.rhsLin.z;
+    p[30] = k.rhsAng.x; p[31] = k.rhsAng.y; p[32] = k.rhsAng.z;
+}
+AD Block loadBlock(const float *p)
+{
+    Block k;
+    for (int c = 0; c < 3; c++)
+        for (int r = 0; r < 3; r++) {
+            k.lhsLin.c[c][r] = p[3 * c + r];
+            k.lhsAng.c[c][r] = p[9 + 3 * c + r];
+            k.lhsCross.c[c][r] = p[18 + 3 * c + r];
+        }
+    k.rhsLin = v3{p[27], p[28], p[29]};
+    k.rhsAng = v3{p[30], p[31], p[32]};
+    return k;
+}
+
 // types.hpp:176-197 with maths.h:315-343: LDL^T, else pivoted LDL^T, else a zero update.
 AD void solveDelta(const Block &k, v3 &dLin, v3 &dAng)
 {

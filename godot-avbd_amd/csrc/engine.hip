@@ -259,13 +259,14 @@ struct Engine {
         AL(S.fric, N); AL(S.grav, N); AL(S.layer, N); AL(S.mask, N); AL(S.lockLin, N); AL(S.lockAng, N);
         AL(S.bodyWorld, N); AL(S.updateIndex, N); AL(S.colour, N); AL(S.contactOnly, N);
         AL(S.dLin, N); AL(S.dAng, N); AL(S.aabbC, N); AL(S.aabbH, N);
+        AL(S.fbCount, 1); AL(S.fbBody, N); AL(S.fbBlock, 33 * N);
         AL(S.adjCount, N + 1); AL(S.adjStart, N + 1); AL(S.adj, 2 * (M + J)); AL(S.adjTmp, 2 * (M + J));
         AL(S.jA, J); AL(S.jB, J); AL(S.jWorld, J); AL(S.jRA, J); AL(S.jRB, J); AL(S.jC0L, J); AL(S.jC0A, J);
         AL(S.jPenL, J); AL(S.jPenA, J); AL(S.jLamL, J); AL(S.jLamA, J); AL(S.jStiffL, J); AL(S.jStiffA, J);
         AL(S.jFrac, J); AL(S.jArm, J); AL(S.jBroken, J); AL(S.jDead, J);
         AL(S.mA, M); AL(S.mB, M); AL(S.mBasis, 3 * M); AL(S.mFric, M); AL(S.mNum, M);
         AL(S.cRA, C); AL(S.cRB, C); AL(S.cC0, C); AL(S.cPen, C); AL(S.cLam, C);
-        AL(S.slotWorld, M); AL(S.live, M); AL(S.liveTmp, M); AL(S.freeList, M);
+        AL(S.slotWorld, M); AL(S.gLive, M); AL(S.wLiveOff, W + 1); AL(S.live, M); AL(S.liveTmp, M); AL(S.freeList, M);
         AL(S.keep, M + 1); AL(S.keepScan, M + 1); AL(S.relScan, M + 1);
         AL(S.succA, M); AL(S.succB, M); AL(S.indeg, M); AL(S.levelOrder, M);
         AL(S.brakeLevels, W); AL(S.levelStart, M); AL(S.wLevelEnd, W);
@@ -279,6 +280,7 @@ struct Engine {
         AL(S.nodeFlag, N); AL(S.nodeMin, 2 * N); AL(S.nodeMax, 2 * N); AL(S.leafBody, N); AL(S.bodyLeaf, N);
         AL(S.pairCount, N + 1); AL(S.pairOff, N + 1); AL(S.pairI, P); AL(S.pairJ, P);
         AL(S.pairValid, P + 1); AL(S.pairValidScan, P + 1);
+        AL(S.pairScratch, N * PAIR_BUF); AL(S.wRoot, W);
         AL(S.wBoundsLo, W); AL(S.wBoundsHi, W);
         AL(slotToLive, M);
         radixBlocks = (Ncap + RADIX_TILE - 1) / RADIX_TILE;
@@ -322,15 +324,25 @@ struct Engine {
         // eight passes: the result is back in sortKey / sortVal
     }
 
+    // Grid for kernels that walk the compact live list: its length is only known on the
+    // device, so they grid-stride over a fixed grid sized to keep every CU busy.
+    int liveGrid() const { return std::max(1, std::min(nblk(std::max(Scap, Jcap)), LIVE_BLOCKS)); }
+
+    // gLive = every world's live positions, world after world (wLiveOff = exclusive scan of wLive).
+    void buildLiveList()
+    {
+        scan(reinterpret_cast<const uint32_t *>(S.wLive), S.wLiveOff, Wcap);
+        k_glive_fill<<<Wcap, TPB, 0, stream>>>(S);
+    }
+
     void buildAdjacency()
     {
         const int n = Ncap;
-        const int t = std::max(Scap, Jcap);
         k_fill_i32<<<nblk(n + 1), TPB, 0, stream>>>(S.adjCount, 0, n + 1);
-        k_adj_count<<<nblk(t), TPB, 0, stream>>>(S);
+        k_adj_count<<<liveGrid(), TPB, 0, stream>>>(S);
         scan(reinterpret_cast<uint32_t *>(S.adjCount), reinterpret_cast<uint32_t *>(S.adjStart), n);
         k_fill_i32<<<nblk(n + 1), TPB, 0, stream>>>(S.adjCount, 0, n + 1);
-        k_adj_fill<<<nblk(t), TPB, 0, stream>>>(S);
+        k_adj_fill<<<liveGrid(), TPB, 0, stream>>>(S);
         k_adj_sort<<<nblk(n), TPB, 0, stream>>>(S);
     }
 
@@ -351,7 +363,10 @@ struct Engine {
         setParams(S.P, prm, collisionDt);
         const int nb = nblk(Ncap);
         if (timed) (void)hipEventRecord(ev[0], stream);
-        if (!csrValid) buildAdjacency();
+        if (!csrValid) {
+            buildLiveList();
+            buildAdjacency();
+        }
         // ---- broad phase (solver_broadphase.cpp:85-186)
         k_swept_aabb<<<nb, TPB, 0, stream>>>(S);
         if (rebuildTree) {
@@ -361,17 +376,20 @@ struct Engine {
             radixSort();
             pend();
             k_karras<<<nblk(Ncap - 1), TPB, 0, stream>>>(S, Ncap);
+            k_world_roots_init<<<nblk(Wcap), TPB, 0, stream>>>(S, Ncap);
+            k_world_roots<<<nblk(Ncap - 1), TPB, 0, stream>>>(S, Ncap);
+            k_world_roots_check<<<nblk(Wcap), TPB, 0, stream>>>(S);
         } else {
-            k_fill_i32<<<nblk(Ncap), TPB, 0, stream>>>(S.nodeFlag, 0, Ncap);
+            k_refit_clear<<<nblk(Ncap), TPB, 0, stream>>>(S, Ncap);
         }
-        k_refit<<<nb, TPB, 0, stream>>>(S, Ncap);
         pbeg(AVBD_K_TRAVERSE);
-        k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap, 0);
+        k_refit_world<<<Wcap, REFIT_TPB, 0, stream>>>(S, Ncap);
+        pend();
+        pbeg(AVBD_K_TRAVERSE);
+        k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap);
         pend();
         scan(S.pairCount, S.pairOff, Ncap);
-        pbeg(AVBD_K_TRAVERSE);
-        k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap, 1);
-        pend();
+        k_pair_emit<<<nb, TPB, 0, stream>>>(S, Ncap);
         k_pair_filter<<<nblk(Pcap), TPB, 0, stream>>>(S);
         scan(S.pairValid, S.pairValidScan, Pcap);
         k_alloc_new<<<nblk(Pcap), TPB, 0, stream>>>(S);
@@ -388,6 +406,7 @@ struct Engine {
         k_compact<<<nblk(Scap), TPB, 0, stream>>>(S);
         k_compact_counts<<<nblk(Wcap), TPB, 0, stream>>>(S);
         k_copy_live<<<nblk(Scap), TPB, 0, stream>>>(S);
+        buildLiveList();
         pbeg(AVBD_K_ADJACENCY);
         buildAdjacency();
         pend();
@@ -403,9 +422,9 @@ struct Engine {
         }
         k_colour_override<<<nb, TPB, 0, stream>>>(S);
         // ---- brake schedule and first brake (solver_projection.cpp:96-167)
-        k_slot_to_live<<<nblk(Scap), TPB, 0, stream>>>(S, slotToLive);
+        k_slot_to_live<<<liveGrid(), TPB, 0, stream>>>(S, slotToLive);
         k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wUpdateCount, 0, Wcap);
-        k_brake_links<<<nblk(Scap), TPB, 0, stream>>>(S, slotToLive);
+        k_brake_links<<<liveGrid(), TPB, 0, stream>>>(S, slotToLive);
         const bool brake = prm.speculative_margin > 0.0f;
         pbeg(AVBD_K_BRAKE_LEVELS);
         k_brake_levels<<<Wcap, BRAKE_TPB, 0, stream>>>(S, collisionDt, brake ? 1 : 0);
@@ -420,7 +439,7 @@ struct Engine {
         const int subs = inter ? substeps : 1;
         for (int sub = 0; sub < subs; sub++) {
             if (sub > 0) {
-                k_rebase<<<nblk(Scap), TPB, 0, stream>>>(S);
+                k_rebase<<<liveGrid(), TPB, 0, stream>>>(S);
                 if (brake) {
                     pbeg(AVBD_K_BRAKE_REPLAY);
                     k_brake_replay<<<Wcap, BRAKE_TPB, 0, stream>>>(S, prm.dt * static_cast<float>(subs - sub));
@@ -438,16 +457,18 @@ struct Engine {
                             pbeg(AVBD_K_COLOUR_PASS);
                             k_colour_pass<<<nb, TPB, 0, stream>>>(S, c);
                             pend();
+                            k_solve_fallback<<<1, TPB, 0, stream>>>(S, 1);
                         }
                     pbeg(AVBD_K_PRIMAL);
                     k_overflow_solve<<<nb, TPB, 0, stream>>>(S);
                     pend();
+                    k_solve_fallback<<<1, TPB, 0, stream>>>(S, 0);
                     pbeg(AVBD_K_BODY);
                     k_overflow_apply<<<nb, TPB, 0, stream>>>(S);
                     pend();
                 }
                 pbeg(AVBD_K_DUAL);
-                k_dual<<<nblk(std::max(Scap, Jcap)), TPB, 0, stream>>>(S, newton ? 1.0f : prm.stiffness_decay);
+                k_dual<<<liveGrid(), TPB, 0, stream>>>(S, newton ? 1.0f : prm.stiffness_decay);
                 pend();
             }
             pbeg(AVBD_K_BODY);
@@ -465,7 +486,7 @@ struct Engine {
             k_proj_reset<<<nblk(Wcap), TPB, 0, stream>>>(S);
             for (int pass = 0; pass < prm.projection_iterations; pass++) {
                 pbeg(AVBD_K_PROJ_MANIFOLD);
-                k_proj_manifold<<<nblk(Scap), TPB, 0, stream>>>(S);
+                k_proj_manifold<<<liveGrid(), TPB, 0, stream>>>(S);
                 pend();
                 pbeg(AVBD_K_PROJ_BODY);
                 k_proj_body<<<nb, TPB, 0, stream>>>(S, slotToLive);
@@ -473,7 +494,7 @@ struct Engine {
                 k_proj_flags<<<nblk(Wcap), TPB, 0, stream>>>(S);
             }
         }
-        k_rebase<<<nblk(Scap), TPB, 0, stream>>>(S);
+        k_rebase<<<liveGrid(), TPB, 0, stream>>>(S);
         if (timed) (void)hipEventRecord(ev[4], stream);
         csrValid = true;
     }
@@ -489,6 +510,7 @@ struct Engine {
                 (void)hipStreamSynchronize(stream);
                 if (e & 1) return fail(AVBD_ERR_CAPACITY, "manifold slot capacity exceeded in world " + std::to_string(w));
                 if (e & 4) return fail(AVBD_ERR_CAPACITY, "broad-phase traversal stack exceeded in world " + std::to_string(w));
+                if (e & 8) return fail(AVBD_ERR_DEVICE, "broad-phase tree has no subtree for world " + std::to_string(w));
                 return fail(AVBD_ERR_CAPACITY, "capacity exceeded in world " + std::to_string(w));
             }
         }
@@ -572,11 +594,21 @@ struct Engine {
         return AVBD_OK;
     }
 
+    bool useGraph = true;
+
     avbd_status stepBatch()
     {
         if (!csrValid) {
+            buildLiveList();
             buildAdjacency();
             csrValid = true;
+        }
+        if (!useGraph) {
+            const bool rebuild = treeAge >= refitBudget;
+            treeAge = rebuild ? 0 : treeAge + 1;
+            enqueueStep(1, 0, rebuild, anyJoints(), false);
+            HIPCHK(hipGetLastError());
+            return checkErrors();
         }
         if (!graphsValid || graphJoints != anyJoints()) {
             avbd_status st = captureGraphs();
@@ -909,10 +941,13 @@ struct Engine {
         k_morton<<<nb, TPB, 0, stream>>>(S, 1);
         radixSort();
         k_karras<<<nblk(Ncap - 1), TPB, 0, stream>>>(S, Ncap);
-        k_refit<<<nb, TPB, 0, stream>>>(S, Ncap);
-        k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap, 0);
+        k_world_roots_init<<<nblk(Wcap), TPB, 0, stream>>>(S, Ncap);
+        k_world_roots<<<nblk(Ncap - 1), TPB, 0, stream>>>(S, Ncap);
+        k_world_roots_check<<<nblk(Wcap), TPB, 0, stream>>>(S);
+        k_refit_world<<<Wcap, REFIT_TPB, 0, stream>>>(S, Ncap);
+        k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap);
         scan(S.pairCount, S.pairOff, Ncap);
-        k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap, 1);
+        k_pair_emit<<<nb, TPB, 0, stream>>>(S, Ncap);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream));
         treeAge = 1 << 20;
@@ -1316,6 +1351,13 @@ avbd_status avbd_batch_remove_world(avbd_batch *batch, int32_t world_id)
     if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
     if (world_id < 0 || world_id >= batch->eng.Wcap) return fail(AVBD_ERR_INVALID, "bad world id");
     return batch->eng.clearWorld(world_id);
+}
+
+avbd_status avbd_batch_use_graph(avbd_batch *batch, int32_t enable)
+{
+    if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
+    batch->eng.useGraph = enable != 0;
+    return AVBD_OK;
 }
 
 avbd_status avbd_batch_step(avbd_batch *batch)

@@ -11,6 +11,8 @@ namespace avbd_dev {
 
 constexpr int TPB = 256;
 __host__ __device__ inline int nblk(int n, int tpb = TPB) { return (n + tpb - 1) / tpb; }
+// Workgroups of a grid-stride launch over the live manifold list (8 per CU on 256 CUs).
+constexpr int LIVE_BLOCKS = 2048;
 
 #define GID (static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x))
 
@@ -299,41 +301,49 @@ __global__ void k_karras(State S, int n)
     S.nodeFlag[i] = 0;
 }
 
-// Leaf boxes (centre -/+ half, the reference's bounds) and bottom-up refit: the second
-// child to arrive at a parent writes the union and continues.
-__global__ void k_refit(State S, int n)
+// Leaf boxes (centre -/+ half, the reference's bounds) and bottom-up refit of each world's
+// subtree, one workgroup per world.  The second child to arrive at a parent writes the
+// union and climbs on; arrivals are counted with workgroup-scope atomics and ordered with
+// workgroup-scope fences (a device-scope fence per thread would write back the L2 of every
+// XCD).  Queries start at world roots, so the levels above them are never read.
+constexpr int REFIT_TPB = 1024;
+__global__ void __launch_bounds__(REFIT_TPB) k_refit_world(State S, int n)
 {
-    const int p = GID;
-    if (p >= n) return;
-    const int body = S.sortVal[p];
-    S.leafBody[p] = body;
-    S.bodyLeaf[body] = p;
-    const v3 c = ld3(S.aabbC[body]), h = ld3(S.aabbH[body]);
-    v3 mn = c - h, mx = c + h;
-    if (S.bodyWorld[body] < 0) {
-        mn = v3{FLT_MAXV, FLT_MAXV, FLT_MAXV};
-        mx = v3{-FLT_MAXV, -FLT_MAXV, -FLT_MAXV};
+    const int w = blockIdx.x;
+    const int lo = S.wBodyBase[w], cnt = S.wBodyCount[w];
+    if (cnt <= 0) return;
+    const int root = S.wRoot[w];
+    for (int p = lo + threadIdx.x; p < lo + cnt; p += REFIT_TPB) {
+        const int body = S.sortVal[p];
+        S.leafBody[p] = body;
+        S.bodyLeaf[body] = p;
+        const v3 c = ld3(S.aabbC[body]), h = ld3(S.aabbH[body]);
+        int node = n - 1 + p;
+        S.nodeMin[node] = st3(c - h);
+        S.nodeMax[node] = st3(c + h);
+        if (node == root) continue;
+        __threadfence_block();
+        int parent = S.nodeParent[node];
+        while (true) {
+            if (__hip_atomic_fetch_add(&S.nodeFlag[parent], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+                break;
+            __threadfence_block();
+            const int l = S.nodeLeft[parent], r = S.nodeRight[parent];
+            const float4 a0 = S.nodeMin[l], a1 = S.nodeMax[l], b0 = S.nodeMin[r], b1 = S.nodeMax[r];
+            S.nodeMin[parent] = make_float4(smin(a0.x, b0.x), smin(a0.y, b0.y), smin(a0.z, b0.z), 0);
+            S.nodeMax[parent] = make_float4(smax(a1.x, b1.x), smax(a1.y, b1.y), smax(a1.z, b1.z), 0);
+            __threadfence_block();
+            if (parent == root) break;
+            parent = S.nodeParent[parent];
+        }
     }
-    int node = n - 1 + p;
-    S.nodeMin[node] = st3(mn);
-    S.nodeMax[node] = st3(mx);
-    if (n == 1) return;
-    __threadfence();
-    int parent = S.nodeParent[node];
-    while (true) {
-        if (atomicAdd(&S.nodeFlag[parent], 1) == 0) return;
-        __threadfence();
-        const int l = S.nodeLeft[parent], r = S.nodeRight[parent];
-        const float4 a0 = S.nodeMin[l];
-        const float4 a1 = S.nodeMax[l];
-        const float4 b0 = S.nodeMin[r];
-        const float4 b1 = S.nodeMax[r];
-        S.nodeMin[parent] = make_float4(smin(a0.x, b0.x), smin(a0.y, b0.y), smin(a0.z, b0.z), 0);
-        S.nodeMax[parent] = make_float4(smax(a1.x, b1.x), smax(a1.y, b1.y), smax(a1.z, b1.z), 0);
-        __threadfence();
-        if (parent == 0) return;
-        parent = S.nodeParent[parent];
-    }
+}
+
+// Arrival counters of the internal nodes, cleared before each refit.
+__global__ void k_refit_clear(State S, int n)
+{
+    const int i = GID;
+    if (i < n - 1) S.nodeFlag[i] = 0;
 }
 
 // Closed-interval AABB overlap exactly as the reference tests it
@@ -345,29 +355,44 @@ __device__ __forceinline__ bool aabbOverlap(v3 aMin, v3 aMax, v3 bMin, v3 bMax)
 }
 
 constexpr int TRAV_STACK = 64;
+constexpr int PAIR_BUF = 16;
 
-// Candidate pairs of body i: every j > i of the same world whose swept AABB overlaps.
-// Pass 0 counts, pass 1 writes the sorted j list at pairOff[i].
-__global__ void k_traverse(State S, int n, int pass)
+// Every world's sort keys share a prefix that no other key has, so its leaves form exactly
+// one subtree of the Karras tree; queries start there (a one-body world is its leaf).
+__global__ void k_world_roots_init(State S, int n)
+{
+    const int w = GID;
+    if (w >= S.numWorlds) return;
+    const int cnt = S.wBodyCount[w];
+    S.wRoot[w] = cnt == 1 ? n - 1 + S.wBodyBase[w] : -1;
+}
+__global__ void k_world_roots_check(State S)
+{
+    const int w = GID;
+    if (w >= S.numWorlds) return;
+    if (S.wBodyCount[w] >= 1 && S.wRoot[w] < 0) atomicOr(S.wErr + w, 8);
+}
+__global__ void k_world_roots(State S, int n)
 {
     const int i = GID;
-    if (i >= S.numBodies) return;
-    const int w = S.bodyWorld[i];
-    if (w < 0 || n < 2) {
-        if (pass == 0) S.pairCount[i] = 0;
-        return;
-    }
-    const int wLo = S.wBodyBase[w], wHi = wLo + S.wBodyCount[w] - 1;
+    if (i >= n - 1) return;
+    const int first = S.nodeFirst[i], last = S.nodeLast[i];
+    const int w = S.bodyWorld[S.sortVal[first]];
+    if (w < 0) return;
+    if (first == S.wBodyBase[w] && last == S.wBodyBase[w] + S.wBodyCount[w] - 1) S.wRoot[w] = i;
+}
+
+// Walk body i's world subtree; `visit(j)` for every j > i whose swept AABB overlaps i's.
+template <typename F>
+__device__ __forceinline__ bool traverseWorld(const State &S, int n, int i, int w, F &&visit)
+{
     const v3 c = ld3(S.aabbC[i]), h = ld3(S.aabbH[i]);
     const v3 qMin = c - h, qMax = c + h;
-    uint32_t out = pass ? S.pairOff[i] : 0u;
-    const uint32_t outEnd = pass ? S.pairOff[i + 1] : 0u;
-    const uint32_t outBegin = out;
-    uint32_t count = 0;
     int stack[TRAV_STACK];
     int sp = 0;
-    stack[sp++] = 0;
-    bool overflow = false;
+    const int root = S.wRoot[w];
+    stack[sp++] = root >= 0 ? root : 0;
+    const int wLo = S.wBodyBase[w], wHi = wLo + S.wBodyCount[w] - 1;
     while (sp > 0) {
         const int node = stack[--sp];
         if (node >= n - 1) {
@@ -377,35 +402,74 @@ __global__ void k_traverse(State S, int n, int pass)
             if (j <= i) continue;
             const v3 cj = ld3(S.aabbC[j]), hj = ld3(S.aabbH[j]);
             if (!aabbOverlap(qMin, qMax, cj - hj, cj + hj)) continue;
-            if (pass) {
-                if (out < outEnd) S.pairJ[out++] = j;
-            } else {
-                count++;
-            }
+            visit(j);
             continue;
         }
-        if (S.nodeLast[node] < wLo || S.nodeFirst[node] > wHi) continue;
+        if (root < 0 && (S.nodeLast[node] < wLo || S.nodeFirst[node] > wHi)) continue;
         const v3 nMin = ld3(S.nodeMin[node]), nMax = ld3(S.nodeMax[node]);
         if (nMin.x > qMax.x || nMax.x < qMin.x || nMin.y > qMax.y || nMax.y < qMin.y || nMin.z > qMax.z ||
             nMax.z < qMin.z)
             continue;
-        if (sp + 2 > TRAV_STACK) { overflow = true; break; }
+        if (sp + 2 > TRAV_STACK) return false;
         stack[sp++] = S.nodeRight[node];
         stack[sp++] = S.nodeLeft[node];
     }
-    if (overflow) atomicOr(S.wErr + w, 4);
-    if (pass == 0) {
-        S.pairCount[i] = count;
+    return true;
+}
+
+// Candidate pairs of body i: every j > i of the same world whose swept AABB overlaps.
+// One traversal counts them and keeps the first PAIR_BUF; the emit pass copies those (or,
+// for a body with more, walks again) into pairJ at pairOff[i], sorted by j.
+__global__ void __launch_bounds__(TPB) k_traverse(State S, int n)
+{
+    const int i = GID;
+    if (i >= S.numBodies) return;
+    const int w = S.bodyWorld[i];
+    if (w < 0 || n < 2) { S.pairCount[i] = 0; return; }
+    uint32_t count = 0;
+    int *buf = S.pairScratch + static_cast<size_t>(i) * PAIR_BUF;
+    const bool ok = traverseWorld(S, n, i, w, [&](int j) {
+        if (count < PAIR_BUF) buf[count] = j;
+        count++;
+    });
+    if (!ok) atomicOr(S.wErr + w, 4);
+    S.pairCount[i] = count;
+}
+
+__global__ void __launch_bounds__(TPB) k_pair_emit(State S, int n)
+{
+    const int i = GID;
+    if (i >= S.numBodies) return;
+    const uint32_t begin = S.pairOff[i], end = S.pairOff[i + 1];
+    const uint32_t cnt = end - begin;
+    if (cnt == 0) return;
+    if (end > static_cast<uint32_t>(S.pairCap)) return;   // reported as a capacity error
+    if (cnt <= PAIR_BUF) {
+        const int *buf = S.pairScratch + static_cast<size_t>(i) * PAIR_BUF;
+        int v[PAIR_BUF];
+        for (uint32_t k = 0; k < cnt; k++) {
+            const int x = buf[k];
+            uint32_t b = k;
+            while (b > 0 && v[b - 1] > x) { v[b] = v[b - 1]; b--; }
+            v[b] = x;
+        }
+        for (uint32_t k = 0; k < cnt; k++) {
+            S.pairJ[begin + k] = v[k];
+            S.pairI[begin + k] = i;
+        }
         return;
     }
-    // insertion sort of the segment by j, then the owner column
-    for (uint32_t a = outBegin + 1; a < out; a++) {
+    uint32_t out = begin;
+    traverseWorld(S, n, i, S.bodyWorld[i], [&](int j) {
+        if (out < end) S.pairJ[out++] = j;
+    });
+    for (uint32_t a = begin + 1; a < out; a++) {
         const int v = S.pairJ[a];
         uint32_t b = a;
-        while (b > outBegin && S.pairJ[b - 1] > v) { S.pairJ[b] = S.pairJ[b - 1]; b--; }
+        while (b > begin && S.pairJ[b - 1] > v) { S.pairJ[b] = S.pairJ[b - 1]; b--; }
         S.pairJ[b] = v;
     }
-    for (uint32_t a = outBegin; a < out; a++) S.pairI[a] = i;
+    for (uint32_t a = begin; a < out; a++) S.pairI[a] = i;
 }
 
 // The body on the other side of an adjacency entry (solver_internal.hpp:364-384).
@@ -417,6 +481,17 @@ __device__ __forceinline__ int otherOf(const State &S, int ref, int body)
     if (a == body) return b;
     if (b == body) return a;
     return -1;
+}
+
+// Grid-stride walk over every world's live manifolds (the compact list gLive).
+#define FOR_LIVE(k) for (int k = GID, k##_n = static_cast<int>(S.wLiveOff[S.numWorlds]); k < k##_n; \
+                         k += static_cast<int>(gridDim.x * blockDim.x))
+
+__global__ void k_glive_fill(State S)
+{
+    const int w = blockIdx.x;
+    const int off = static_cast<int>(S.wLiveOff[w]), n = S.wLive[w], base = S.wSlotBase[w];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) S.gLive[off + i] = base + i;
 }
 
 // solver_broadphase.cpp:30-65: bounds, layers, no force or manifold between the pair,
@@ -500,7 +575,7 @@ __global__ void k_alloc_counts(State S)
 // ------------------------------------------------------------ narrow phase ----
 // manifold.cpp:237-435 with collide.cpp:1100-1120: collide at the step-start pose,
 // carry hot state across matching feature keys, C0, release, warm-start decay.
-__global__ void k_manifold_init(State S)
+__global__ void __launch_bounds__(TPB) k_manifold_init(State S)
 {
     const int t = GID;
     if (t >= S.slotCap) return;
@@ -685,27 +760,22 @@ __global__ void k_copy_live(State S)
 // Counts and fills use atomics; each segment is then sorted by its sequence number.
 __global__ void k_adj_count(State S)
 {
-    const int t = GID;
-    if (t < S.jointCap) {
+    for (int t = GID; t < S.jointCap; t += gridDim.x * blockDim.x) {
         if (S.jWorld[t] >= 0 && !S.jDead[t]) {
             if (S.jA[t] >= 0) atomicAdd(&S.adjCount[S.jA[t]], 1);
             atomicAdd(&S.adjCount[S.jB[t]], 1);
         }
     }
-    if (t < S.slotCap) {
-        const int w = S.slotWorld[t];
-        if (w >= 0 && t - S.wSlotBase[w] < S.wLive[w]) {
-            const int slot = S.live[t];
-            atomicAdd(&S.adjCount[S.mA[slot]], 1);
-            atomicAdd(&S.adjCount[S.mB[slot]], 1);
-        }
+    FOR_LIVE(k) {
+        const int slot = S.live[S.gLive[k]];
+        atomicAdd(&S.adjCount[S.mA[slot]], 1);
+        atomicAdd(&S.adjCount[S.mB[slot]], 1);
     }
 }
 
 __global__ void k_adj_fill(State S)
 {
-    const int t = GID;
-    if (t < S.jointCap) {
+    for (int t = GID; t < S.jointCap; t += gridDim.x * blockDim.x) {
         const int w = S.jWorld[t];
         if (w >= 0 && !S.jDead[t]) {
             const unsigned long long seq = static_cast<unsigned long long>(t - S.wJointBase[w]);
@@ -718,16 +788,14 @@ __global__ void k_adj_fill(State S)
             S.adjTmp[S.adjStart[b] + atomicAdd(&S.adjCount[b], 1)] = e;
         }
     }
-    if (t < S.slotCap) {
-        const int w = S.slotWorld[t];
-        if (w >= 0 && t - S.wSlotBase[w] < S.wLive[w]) {
-            const int slot = S.live[t];
-            const unsigned long long seq = 0x40000000ull + static_cast<unsigned long long>(t - S.wSlotBase[w]);
-            const unsigned long long e = (seq << 32) | static_cast<uint32_t>(slot);
-            const int a = S.mA[slot], b = S.mB[slot];
-            S.adjTmp[S.adjStart[a] + atomicAdd(&S.adjCount[a], 1)] = e;
-            S.adjTmp[S.adjStart[b] + atomicAdd(&S.adjCount[b], 1)] = e;
-        }
+    FOR_LIVE(k) {
+        const int t = S.gLive[k];
+        const int slot = S.live[t];
+        const unsigned long long seq = 0x40000000ull + static_cast<unsigned long long>(t - S.wSlotBase[S.slotWorld[t]]);
+        const unsigned long long e = (seq << 32) | static_cast<uint32_t>(slot);
+        const int a = S.mA[slot], b = S.mB[slot];
+        S.adjTmp[S.adjStart[a] + atomicAdd(&S.adjCount[a], 1)] = e;
+        S.adjTmp[S.adjStart[b] + atomicAdd(&S.adjCount[b], 1)] = e;
     }
 }
 
@@ -841,20 +909,20 @@ __device__ __forceinline__ bool manifoldActive(const State &S, int slot)
 
 __global__ void k_slot_to_live(State S, int *slotToLive)
 {
-    const int t = GID;
-    if (t >= S.slotCap) return;
-    const int w = S.slotWorld[t];
-    if (w < 0) return;
-    if (t - S.wSlotBase[w] < S.wLive[w]) slotToLive[S.live[t]] = t;
+    FOR_LIVE(k) {
+        const int t = S.gLive[k];
+        slotToLive[S.live[t]] = t;
+    }
 }
 
+__device__ void brakeLinks(const State &S, const int *slotToLive, int t);
 __global__ void k_brake_links(State S, const int *slotToLive)
 {
-    const int t = GID;
-    if (t >= S.slotCap) return;
+    FOR_LIVE(k) brakeLinks(S, slotToLive, S.gLive[k]);
+}
+__device__ void brakeLinks(const State &S, const int *slotToLive, int t)
+{
     const int w = S.slotWorld[t];
-    if (w < 0) return;
-    if (t - S.wSlotBase[w] >= S.wLive[w]) return;
     const int slot = S.live[t];
     S.succA[t] = -1;
     S.succB[t] = -1;
@@ -1048,8 +1116,15 @@ __global__ void k_finish(State S)
     S.angVel[i] = st3(ang);
 }
 
+__device__ __forceinline__ void queueFallback(const State &S, int body, const Block &blk)
+{
+    const int k = atomicAdd(S.fbCount, 1);
+    S.fbBody[k] = body;
+    storeBlock(S.fbBlock + 33 * k, blk);
+}
+
 // solver_step.cpp:209-229: one colour of the Gauss-Seidel sweep.
-__global__ void k_colour_pass(State S, int colour)
+__global__ void __launch_bounds__(TPB) k_colour_pass(State S, int colour)
 {
     const int i = GID;
     if (i >= S.numBodies || S.colour[i] != colour || S.bodyWorld[i] < 0) return;
@@ -1057,13 +1132,13 @@ __global__ void k_colour_pass(State S, int colour)
     Block blk;
     assemble(S, i, blk);
     v3 dl, da;
-    solveDelta(blk, dl, da);
+    if (!solveDeltaFast(blk, dl, da)) { queueFallback(S, i, blk); return; }
     S.pos[i] = st3(bPos(S, i) + dl);
     S.rot[i] = stq(qadd(bRot(S, i), da));
 }
 
 // solver_step.cpp:248-264: overflow bodies solve against one pose snapshot.
-__global__ void k_overflow_solve(State S)
+__global__ void __launch_bounds__(TPB) k_overflow_solve(State S)
 {
     const int i = GID;
     if (i >= S.numBodies || S.colour[i] != 8 || S.bodyWorld[i] < 0) return;
@@ -1071,9 +1146,31 @@ __global__ void k_overflow_solve(State S)
     Block blk;
     assemble(S, i, blk);
     v3 dl, da;
-    solveDelta(blk, dl, da);
+    if (!solveDeltaFast(blk, dl, da)) { queueFallback(S, i, blk); return; }
     S.dLin[i] = st3(dl);
     S.dAng[i] = st3(da);
+}
+
+// Pivoted solves of the queued blocks (maths.h:202-343), then the queue is emptied.  One
+// workgroup; `apply` = colour pass (move the body now) or Jacobi (store the delta).
+__global__ void k_solve_fallback(State S, int apply)
+{
+    const int n = *S.fbCount;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const int i = S.fbBody[k];
+        const Block blk = loadBlock(S.fbBlock + 33 * k);
+        v3 dl, da;
+        solveDeltaSlow(blk, dl, da);
+        if (apply) {
+            S.pos[i] = st3(bPos(S, i) + dl);
+            S.rot[i] = stq(qadd(bRot(S, i), da));
+        } else {
+            S.dLin[i] = st3(dl);
+            S.dAng[i] = st3(da);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *S.fbCount = 0;
 }
 
 // solver_step.cpp:268-273
@@ -1088,19 +1185,14 @@ __global__ void k_overflow_apply(State S)
 }
 
 // solver_step.cpp:277-296: manifolds (not frozen), then joints.
-__global__ void k_dual(State S, float decay)
+__global__ void __launch_bounds__(TPB) k_dual(State S, float decay)
 {
-    const int t = GID;
-    if (t < S.slotCap) {
-        const int w = S.slotWorld[t];
-        if (w >= 0 && t - S.wSlotBase[w] < S.wLive[w]) {
-            const int slot = S.live[t];
-            if (bMass(S, S.mA[slot]) > 0.0f || bMass(S, S.mB[slot]) > 0.0f) manifoldDual(S, slot, decay);
-        }
+    FOR_LIVE(k) {
+        const int slot = S.live[S.gLive[k]];
+        if (bMass(S, S.mA[slot]) > 0.0f || bMass(S, S.mB[slot]) > 0.0f) manifoldDual(S, slot, decay);
     }
-    if (t < S.jointCap) {
+    for (int t = GID; t < S.jointCap; t += gridDim.x * blockDim.x)
         if (S.jWorld[t] >= 0 && !S.jDead[t]) jointDual(S, t);
-    }
 }
 
 // ------------------------------------------------------------ projection ----
@@ -1114,13 +1206,14 @@ __global__ void k_proj_reset(State S)
 
 // solver_projection.cpp:642-709: per active manifold, patch depth beyond the slop and the
 // per-side corrections (applied by the body gather below).
-__global__ void k_proj_manifold(State S)
+__device__ void projManifold(const State &S, int t);
+__global__ void __launch_bounds__(TPB) k_proj_manifold(State S)
 {
-    const int t = GID;
-    if (t >= S.slotCap) return;
+    FOR_LIVE(k) projManifold(S, S.gLive[k]);
+}
+__device__ void projManifold(const State &S, int t)
+{
     const int w = S.slotWorld[t];
-    if (w < 0) return;
-    if (t - S.wSlotBase[w] >= S.wLive[w]) return;
     if (S.wProjDone[w]) return;
     S.projOn[t] = 0;
     const int slot = S.live[t];
@@ -1221,12 +1314,7 @@ __global__ void k_proj_flags(State S)
 // solver_step.cpp:164-170
 __global__ void k_rebase(State S)
 {
-    const int t = GID;
-    if (t >= S.slotCap) return;
-    const int w = S.slotWorld[t];
-    if (w < 0) return;
-    if (t - S.wSlotBase[w] >= S.wLive[w]) return;
-    manifoldRebase(S, S.live[t]);
+    FOR_LIVE(k) manifoldRebase(S, S.live[S.gLive[k]]);
 }
 
 } // namespace avbd_dev

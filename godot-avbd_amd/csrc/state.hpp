@@ -46,6 +46,11 @@ struct State {
     uint8_t *contactOnly;
     float4 *dLin, *dAng;
     float4 *aabbC, *aabbH;
+    // bodies whose block failed the unpivoted LDL^T this pass: (body, block) queued for the
+    // pivoted solve, which runs in its own kernel to keep the hot kernels call- and scratch-free
+    int *fbCount;
+    int *fbBody;
+    float *fbBlock;  // 33 floats per entry (lhsLin, lhsAng, lhsCross column-major, rhsLin, rhsAng)
     int *adjCount, *adjStart, *adj;
     unsigned long long *adjTmp;
 
@@ -62,6 +67,9 @@ struct State {
     int *mNum;
     float4 *cRA, *cRB, *cC0, *cPen, *cLam;
     int *slotWorld;     // static: world owning slot-range position t
+    int *gLive;         // all worlds' live positions t, world after world (compact)
+    uint32_t *wLiveOff; // per world: offset of its live entries in gLive; [numWorlds] = total
+    float4 *stamp;      // per slot, 14 float4: one Jacobi stamp per side (see k_stamp)
     int *live;          // per world range: slots in live order
     int *liveTmp;
     int *freeList;      // per world range
@@ -93,6 +101,8 @@ struct State {
     int *bodyLeaf;               // body -> sorted position
     uint32_t *pairCount, *pairOff;
     int *pairI, *pairJ;
+    int *pairScratch;            // PAIR_BUF candidates per body from the single traversal
+    int *wRoot;                  // per world: node whose leaf range is exactly the world
     uint32_t *pairValid, *pairValidScan;
     float4 *wBoundsLo, *wBoundsHi;
 };

@@ -150,6 +150,9 @@ class Batch:
     def step(self):
         abi.check(self.lib.avbd_batch_step(self.h))
 
+    def use_graph(self, enable):
+        abi.check(self.lib.avbd_batch_use_graph(self.h, 1 if enable else 0))
+
     def download(self, wid):
         n = self.sizes[wid]
         out = abi.empty_arrays({k: abi.BODY_FIELDS[k] for k in

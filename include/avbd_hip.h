@@ -256,6 +256,9 @@ avbd_status avbd_batch_destroy(avbd_batch *batch);
 avbd_status avbd_batch_add_world(avbd_batch *batch, const avbd_bodies *bodies,
                                  const avbd_joints *joints, int32_t *world_id);
 avbd_status avbd_batch_remove_world(avbd_batch *batch, int32_t world_id);
+/* Launch batch steps as one captured HIP graph (enable = 1, the default) or as the same
+ * kernels launched one by one on the batch's stream (enable = 0; for tracing tools). */
+avbd_status avbd_batch_use_graph(avbd_batch *batch, int32_t enable);
 /* One step of every world in the batch. */
 avbd_status avbd_batch_step(avbd_batch *batch);
 avbd_status avbd_batch_download_world(avbd_batch *batch, int32_t world_id,
