@@ -97,11 +97,11 @@ class StepStats(C.Structure):
 
 
 K_CLASSES = ("primal", "dual", "narrow", "brake_levels", "brake_replay", "traverse", "colour_pass",
-             "proj_manifold", "proj_body", "adjacency", "sort", "body")
+             "proj_manifold", "proj_body", "adjacency", "sort", "body", "stamp", "refit")
 
 
 class KernelProfile(C.Structure):
-    _fields_ = [("steps", C.c_int32), ("avg_ms", C.c_float * 12), ("launches", C.c_int32 * 12),
+    _fields_ = [("steps", C.c_int32), ("avg_ms", C.c_float * len(K_CLASSES)), ("launches", C.c_int32 * len(K_CLASSES)),
                 ("step_ms", C.c_float)]
 
     def as_dict(self):

@@ -302,6 +302,23 @@ AD Hess basisHessian(const m3 &B, v3 k)
             k.x * b0yz + k.y * b1yz + k.z * b2yz, k.x * b0z2 + k.y * b1z2 + k.z * b2z2};
 }
 
+// Angular and cross terms of one contact's stamp on the body with arm r (manifold.cpp:688-755).
+AD void stampSide(const Hess &h, const m3 &Sm, v3 r, v3 fw, m3 &accAng, m3 &accCross, v3 &accRA)
+{
+    const m3 RS{{cross(r, Sm.c[0]), cross(r, Sm.c[1]), cross(r, Sm.c[2])}};
+    const float rx2 = r.x * r.x, ry2 = r.y * r.y, rz2 = r.z * r.z;
+    const float rxy = r.x * r.y, rxz = r.x * r.z, ryz = r.y * r.z;
+    const float h00 = h.s11 * rz2 - 2.0f * h.s12 * ryz + h.s22 * ry2;
+    const float h11 = h.s22 * rx2 - 2.0f * h.s02 * rxz + h.s00 * rz2;
+    const float h22 = h.s00 * ry2 - 2.0f * h.s01 * rxy + h.s11 * rx2;
+    const float h01 = h.s12 * rxz + h.s02 * ryz - h.s22 * rxy - h.s01 * rz2;
+    const float h02 = h.s12 * rxy + h.s01 * ryz - h.s11 * rxz - h.s02 * ry2;
+    const float h12 = h.s02 * rxy + h.s01 * rxz - h.s00 * ryz - h.s12 * rx2;
+    accAng = madd(accAng, m3{{v3{h00, h01, h02}, v3{h01, h11, h12}, v3{h02, h12, h22}}});
+    accCross = madd(accCross, RS);
+    accRA += cross(r, fw);
+}
+
 // One manifold's contribution to one body's block (manifold.cpp:650-686 force + :688-755
 // stamp, fused).  Forces are recomputed at the current poses; the reference's cache holds
 // the same floats because nothing it reads moves between fill and use.
@@ -330,25 +347,12 @@ AD void stampManifold(const State &S, int slot, int body, Block &blk)
         v3 f = cmul(pen, C) + ld3(S.cLam[c]);
         float bounds;
         clampForce(f, mu, bounds);
-        const v3 r = fromA ? rA : rB;
         const Hess h = basisHessian(B, pen);
         const m3 Sm{{v3{h.s00, h.s01, h.s02}, v3{h.s01, h.s11, h.s12}, v3{h.s02, h.s12, h.s22}}};
-        const m3 RS{{cross(r, Sm.c[0]), cross(r, Sm.c[1]), cross(r, Sm.c[2])}};
-        const float rx2 = r.x * r.x, ry2 = r.y * r.y, rz2 = r.z * r.z;
-        const float rxy = r.x * r.y, rxz = r.x * r.z, ryz = r.y * r.z;
-        const float h00 = h.s11 * rz2 - 2.0f * h.s12 * ryz + h.s22 * ry2;
-        const float h11 = h.s22 * rx2 - 2.0f * h.s02 * rxz + h.s00 * rz2;
-        const float h22 = h.s00 * ry2 - 2.0f * h.s01 * rxy + h.s11 * rx2;
-        const float h01 = h.s12 * rxz + h.s02 * ryz - h.s22 * rxy - h.s01 * rz2;
-        const float h02 = h.s12 * rxy + h.s01 * ryz - h.s11 * rxz - h.s02 * ry2;
-        const float h12 = h.s02 * rxy + h.s01 * rxz - h.s00 * ryz - h.s12 * rx2;
         const v3 fw = (b0 * f.x + b1 * f.y) + b2 * f.z;
-        const v3 fwAng = cross(r, fw);
+        stampSide(h, Sm, fromA ? rA : rB, fw, accAng, accCross, accRA);
         accLin = madd(accLin, Sm);
-        accAng = madd(accAng, m3{{v3{h00, h01, h02}, v3{h01, h11, h12}, v3{h02, h12, h22}}});
-        accCross = madd(accCross, RS);
         accRL += fw;
-        accRA += fwAng;
     }
     blk.lhsLin = madd(blk.lhsLin, accLin);
     blk.lhsAng = madd(blk.lhsAng, accAng);
@@ -357,7 +361,96 @@ AD void stampManifold(const State &S, int slot, int body, Block &blk)
     blk.rhsAng += accRA * side;
 }
 
-// manifold.cpp:768-856 (arm caching off): lambda <- clamped force, penalty ramps.
+// ---- per-manifold stamps, gathered by the Jacobi solve ----
+// A manifold's summed stamp is the same for every body that gathers it except for the arm
+// terms, so it is built once per iteration for both bodies (7 float4 per side, STAMP_F4 per
+// slot) and each body's solve adds its side in adjacency order: the floats are those
+// stampManifold produces.
+constexpr int STAMP_F4 = 14;
+
+struct StampAcc {
+    m3 lin, angA, crossA, angB, crossB;
+    v3 rl, raA, raB;
+};
+AD void stampInit(StampAcc &s)
+{
+    s.lin = s.angA = s.crossA = s.angB = s.crossB = zero3();
+    s.rl = s.raA = s.raB = v3{0, 0, 0};
+}
+AD void stampContact(StampAcc &s, const m3 &B, v3 pen, v3 f, v3 rA, v3 rB)
+{
+    const Hess h = basisHessian(B, pen);
+    const m3 Sm{{v3{h.s00, h.s01, h.s02}, v3{h.s01, h.s11, h.s12}, v3{h.s02, h.s12, h.s22}}};
+    const v3 fw = (row(B, 0) * f.x + row(B, 1) * f.y) + row(B, 2) * f.z;
+    stampSide(h, Sm, rA, fw, s.angA, s.crossA, s.raA);
+    stampSide(h, Sm, rB, fw, s.angB, s.crossB, s.raB);
+    s.lin = madd(s.lin, Sm);
+    s.rl += fw;
+}
+AD void storeStampSide(float4 *p, const m3 &L, v3 rl, const m3 &A, const m3 &X, v3 ra)
+{
+    p[0] = make_float4(L.c[0][0], L.c[0][1], L.c[0][2], L.c[1][1]);
+    p[1] = make_float4(L.c[1][2], L.c[2][2], rl.x, rl.y);
+    p[2] = make_float4(rl.z, A.c[0][0], A.c[0][1], A.c[0][2]);
+    p[3] = make_float4(A.c[1][1], A.c[1][2], A.c[2][2], X.c[0][0]);
+    p[4] = make_float4(X.c[0][1], X.c[0][2], X.c[1][0], X.c[1][1]);
+    p[5] = make_float4(X.c[1][2], X.c[2][0], X.c[2][1], X.c[2][2]);
+    p[6] = make_float4(ra.x, ra.y, ra.z, 0.0f);
+}
+AD void storeStamps(const State &S, int slot, const StampAcc &s)
+{
+    float4 *p = S.stamp + static_cast<size_t>(STAMP_F4) * slot;
+    storeStampSide(p, s.lin, s.rl, s.angA, s.crossA, s.raA);
+    storeStampSide(p + 7, s.lin, s.rl * -1.0f, s.angB, s.crossB, s.raB * -1.0f);
+}
+// Symmetric 3x3 from its upper triangle (the sums are symmetric bit for bit).
+AD m3 sym3(float a00, float a01, float a02, float a11, float a12, float a22)
+{
+    return m3{{v3{a00, a01, a02}, v3{a01, a11, a12}, v3{a02, a12, a22}}};
+}
+AD void addStampSide(Block &blk, const float4 *p)
+{
+    const float4 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], p4 = p[4], p5 = p[5], p6 = p[6];
+    blk.lhsLin = madd(blk.lhsLin, sym3(p0.x, p0.y, p0.z, p0.w, p1.x, p1.y));
+    blk.lhsAng = madd(blk.lhsAng, sym3(p2.y, p2.z, p2.w, p3.x, p3.y, p3.z));
+    blk.lhsCross = madd(blk.lhsCross, m3{{v3{p3.w, p4.x, p4.y}, v3{p4.z, p4.w, p5.x}, v3{p5.y, p5.z, p5.w}}});
+    blk.rhsLin += v3{p1.z, p1.w, p2.x};
+    blk.rhsAng += v3{p6.x, p6.y, p6.z};
+}
+
+// Both sides' stamps of one manifold at the current poses.
+AD void manifoldStamps(const State &S, int slot)
+{
+    const int a = S.mA[slot], b = S.mB[slot];
+    const m3 B = loadBasis(S, slot);
+    const float mu = S.mFric[slot];
+    const int n = S.mNum[slot];
+    const qt qa = bRot(S, a), qb = bRot(S, b);
+    const v3 dqAL = bPos(S, a) - ld3(S.initLin[a]), dqAA = qsub(qa, ldq(S.initAng[a]));
+    const v3 dqBL = bPos(S, b) - ld3(S.initLin[b]), dqBA = qsub(qb, ldq(S.initAng[b]));
+    const m3 rotA = qmat(qa), rotB = qmat(qb);
+    StampAcc acc;
+    stampInit(acc);
+    for (int i = 0; i < n; i++) {
+        const int c = slot * MAXC + i;
+        const v3 rA = mul(rotA, ld3(S.cRA[c]));
+        const v3 rB = mul(rotB, ld3(S.cRB[c]));
+        const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
+        const v3 C = constraintError(S, ld3(S.cC0[c]), br, B);
+        const v3 pen = ld3(S.cPen[c]);
+        v3 f = cmul(pen, C) + ld3(S.cLam[c]);
+        float bounds;
+        clampForce(f, mu, bounds);
+        stampContact(acc, B, pen, f, rA, rB);
+    }
+    storeStamps(S, slot, acc);
+}
+
+// manifold.cpp:768-856 (arm caching off): lambda <- clamped force, penalty ramps.  With
+// STAMP, the stamps of the next primal iteration are built from the updated lambda and
+// penalty: the poses do not move between this dual and that primal, so its constraint
+// error is the one computed here.
+template <bool STAMP>
 AD void manifoldDual(const State &S, int slot, float decay)
 {
     const int a = S.mA[slot], b = S.mB[slot];
@@ -369,6 +462,8 @@ AD void manifoldDual(const State &S, int slot, float decay)
     const v3 dqBL = bPos(S, b) - ld3(S.initLin[b]), dqBA = qsub(qb, ldq(S.initAng[b]));
     const m3 rotA = qmat(qa), rotB = qmat(qb);
     const float pmax = S.P.penaltyMax;
+    StampAcc acc;
+    if (STAMP) stampInit(acc);
     for (int i = 0; i < n; i++) {
         const int c = slot * MAXC + i;
         v3 pen = ld3(S.cPen[c]);
@@ -394,7 +489,14 @@ AD void manifoldDual(const State &S, int slot, float decay)
         }
         S.cPen[c] = st3(pen);
         if (writeStick) S.cC0[c] = c0;
+        if (STAMP) {
+            v3 f = cmul(pen, C) + F;
+            float bd;
+            clampForce(f, mu, bd);
+            stampContact(acc, B, pen, f, rA, rB);
+        }
     }
+    if (STAMP) storeStamps(S, slot, acc);
 }
 
 // manifold.cpp:633-646
@@ -527,6 +629,18 @@ AD void assemble(const State &S, int body, Block &blk)
     for (int e = e0; e < e1; e++) {
         const int ref = S.adj[e];
         if (ref >= 0) stampManifold(S, ref, body, blk);
+        else jointPrimal(S, -ref - 1, body, blk);
+    }
+}
+
+// The same block from the per-manifold stamps (joints are stamped here, as in assemble).
+AD void assembleStamped(const State &S, int body, Block &blk)
+{
+    makeBlock(S, body, blk);
+    const int e0 = S.adjStart[body], e1 = S.adjStart[body + 1];
+    for (int e = e0; e < e1; e++) {
+        const int ref = S.adj[e];
+        if (ref >= 0) addStampSide(blk, S.stamp + static_cast<size_t>(STAMP_F4) * ref + (S.mA[ref] == body ? 0 : 7));
         else jointPrimal(S, -ref - 1, body, blk);
     }
 }

@@ -266,7 +266,7 @@ struct Engine {
         AL(S.jFrac, J); AL(S.jArm, J); AL(S.jBroken, J); AL(S.jDead, J);
         AL(S.mA, M); AL(S.mB, M); AL(S.mBasis, 3 * M); AL(S.mFric, M); AL(S.mNum, M);
         AL(S.cRA, C); AL(S.cRB, C); AL(S.cC0, C); AL(S.cPen, C); AL(S.cLam, C);
-        AL(S.slotWorld, M); AL(S.gLive, M); AL(S.wLiveOff, W + 1); AL(S.live, M); AL(S.liveTmp, M); AL(S.freeList, M);
+        AL(S.stamp, STAMP_F4 * M); AL(S.slotWorld, M); AL(S.gLive, M); AL(S.wLiveOff, W + 1); AL(S.live, M); AL(S.liveTmp, M); AL(S.freeList, M);
         AL(S.keep, M + 1); AL(S.keepScan, M + 1); AL(S.relScan, M + 1);
         AL(S.succA, M); AL(S.succB, M); AL(S.indeg, M); AL(S.levelOrder, M);
         AL(S.brakeLevels, W); AL(S.levelStart, M); AL(S.wLevelEnd, W);
@@ -277,7 +277,7 @@ struct Engine {
         AL(S.wColourLeft, W);
         AL(S.mortonCode, N); AL(S.sortKey, N); AL(S.sortKeyAlt, N); AL(S.sortVal, N); AL(S.sortValAlt, N);
         AL(S.nodeLeft, N); AL(S.nodeRight, N); AL(S.nodeParent, 2 * N); AL(S.nodeFirst, N); AL(S.nodeLast, N);
-        AL(S.nodeFlag, N); AL(S.nodeMin, 2 * N); AL(S.nodeMax, 2 * N); AL(S.leafBody, N); AL(S.bodyLeaf, N);
+        AL(S.nodeFlag, N); AL(S.nodeBox, 4 * N); AL(S.nodeRope, 2 * N); AL(S.leafBody, N); AL(S.bodyLeaf, N);
         AL(S.pairCount, N + 1); AL(S.pairOff, N + 1); AL(S.pairI, P); AL(S.pairJ, P);
         AL(S.pairValid, P + 1); AL(S.pairValidScan, P + 1);
         AL(S.pairScratch, N * PAIR_BUF); AL(S.wRoot, W);
@@ -379,10 +379,11 @@ struct Engine {
             k_world_roots_init<<<nblk(Wcap), TPB, 0, stream>>>(S, Ncap);
             k_world_roots<<<nblk(Ncap - 1), TPB, 0, stream>>>(S, Ncap);
             k_world_roots_check<<<nblk(Wcap), TPB, 0, stream>>>(S);
+            k_ropes<<<nblk(2 * Ncap - 1), TPB, 0, stream>>>(S, Ncap);
         } else {
             k_refit_clear<<<nblk(Ncap), TPB, 0, stream>>>(S, Ncap);
         }
-        pbeg(AVBD_K_TRAVERSE);
+        pbeg(AVBD_K_REFIT);
         k_refit_world<<<Wcap, REFIT_TPB, 0, stream>>>(S, Ncap);
         pend();
         pbeg(AVBD_K_TRAVERSE);
@@ -451,6 +452,10 @@ struct Engine {
             pend();
             for (int it = 0; it < rounds; it++) {
                 const bool newton = it < newtonRounds;
+                // Jacobi stamps come from the previous dual unless colour passes moved bodies
+                // since (or this is the first iteration of the substep).
+                const bool stampHere = withColours || it == 0;
+                const bool stampInDual = !withColours && it + 1 < newtonRounds;
                 if (newton) {
                     if (withColours)
                         for (int c = 0; c < 8; c++) {
@@ -459,6 +464,11 @@ struct Engine {
                             pend();
                             k_solve_fallback<<<1, TPB, 0, stream>>>(S, 1);
                         }
+                    if (stampHere) {
+                        pbeg(AVBD_K_STAMP);
+                        k_stamp<<<liveGrid(), TPB, 0, stream>>>(S);
+                        pend();
+                    }
                     pbeg(AVBD_K_PRIMAL);
                     k_overflow_solve<<<nb, TPB, 0, stream>>>(S);
                     pend();
@@ -468,7 +478,10 @@ struct Engine {
                     pend();
                 }
                 pbeg(AVBD_K_DUAL);
-                k_dual<<<liveGrid(), TPB, 0, stream>>>(S, newton ? 1.0f : prm.stiffness_decay);
+                if (stampInDual)
+                    k_dual<true><<<liveGrid(), TPB, 0, stream>>>(S, newton ? 1.0f : prm.stiffness_decay);
+                else
+                    k_dual<false><<<liveGrid(), TPB, 0, stream>>>(S, newton ? 1.0f : prm.stiffness_decay);
                 pend();
             }
             pbeg(AVBD_K_BODY);
@@ -509,7 +522,7 @@ struct Engine {
                 (void)hipMemsetAsync(S.wErr, 0, sizeof(int) * Wcap, stream);
                 (void)hipStreamSynchronize(stream);
                 if (e & 1) return fail(AVBD_ERR_CAPACITY, "manifold slot capacity exceeded in world " + std::to_string(w));
-                if (e & 4) return fail(AVBD_ERR_CAPACITY, "broad-phase traversal stack exceeded in world " + std::to_string(w));
+                if (e & 4) return fail(AVBD_ERR_DEVICE, "broad-phase tree walk did not terminate in world " + std::to_string(w));
                 if (e & 8) return fail(AVBD_ERR_DEVICE, "broad-phase tree has no subtree for world " + std::to_string(w));
                 return fail(AVBD_ERR_CAPACITY, "capacity exceeded in world " + std::to_string(w));
             }
@@ -944,6 +957,7 @@ struct Engine {
         k_world_roots_init<<<nblk(Wcap), TPB, 0, stream>>>(S, Ncap);
         k_world_roots<<<nblk(Ncap - 1), TPB, 0, stream>>>(S, Ncap);
         k_world_roots_check<<<nblk(Wcap), TPB, 0, stream>>>(S);
+        k_ropes<<<nblk(2 * Ncap - 1), TPB, 0, stream>>>(S, Ncap);
         k_refit_world<<<Wcap, REFIT_TPB, 0, stream>>>(S, Ncap);
         k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap);
         scan(S.pairCount, S.pairOff, Ncap);

@@ -319,8 +319,8 @@ __global__ void __launch_bounds__(REFIT_TPB) k_refit_world(State S, int n)
         S.bodyLeaf[body] = p;
         const v3 c = ld3(S.aabbC[body]), h = ld3(S.aabbH[body]);
         int node = n - 1 + p;
-        S.nodeMin[node] = st3(c - h);
-        S.nodeMax[node] = st3(c + h);
+        S.nodeBox[2 * node] = st3(c - h, __int_as_float(body));
+        S.nodeBox[2 * node + 1] = st3(c + h, __int_as_float(S.nodeRope[node]));
         if (node == root) continue;
         __threadfence_block();
         int parent = S.nodeParent[node];
@@ -329,9 +329,11 @@ __global__ void __launch_bounds__(REFIT_TPB) k_refit_world(State S, int n)
                 break;
             __threadfence_block();
             const int l = S.nodeLeft[parent], r = S.nodeRight[parent];
-            const float4 a0 = S.nodeMin[l], a1 = S.nodeMax[l], b0 = S.nodeMin[r], b1 = S.nodeMax[r];
-            S.nodeMin[parent] = make_float4(smin(a0.x, b0.x), smin(a0.y, b0.y), smin(a0.z, b0.z), 0);
-            S.nodeMax[parent] = make_float4(smax(a1.x, b1.x), smax(a1.y, b1.y), smax(a1.z, b1.z), 0);
+            const float4 a0 = S.nodeBox[2 * l], a1 = S.nodeBox[2 * l + 1];
+            const float4 b0 = S.nodeBox[2 * r], b1 = S.nodeBox[2 * r + 1];
+            S.nodeBox[2 * parent] = make_float4(smin(a0.x, b0.x), smin(a0.y, b0.y), smin(a0.z, b0.z), __int_as_float(l));
+            S.nodeBox[2 * parent + 1] = make_float4(smax(a1.x, b1.x), smax(a1.y, b1.y), smax(a1.z, b1.z),
+                                                    __int_as_float(S.nodeRope[parent]));
             __threadfence_block();
             if (parent == root) break;
             parent = S.nodeParent[parent];
@@ -354,7 +356,6 @@ __device__ __forceinline__ bool aabbOverlap(v3 aMin, v3 aMax, v3 bMin, v3 bMax)
              aMin.z > bMax.z);
 }
 
-constexpr int TRAV_STACK = 64;
 constexpr int PAIR_BUF = 16;
 
 // Every world's sort keys share a prefix that no other key has, so its leaves form exactly
@@ -382,37 +383,48 @@ __global__ void k_world_roots(State S, int n)
     if (first == S.wBodyBase[w] && last == S.wBodyBase[w] + S.wBodyCount[w] - 1) S.wRoot[w] = i;
 }
 
-// Walk body i's world subtree; `visit(j)` for every j > i whose swept AABB overlaps i's.
+// Next node after the subtree whose leaves end at position `last`, in left-first
+// depth-first order of the world's subtree (-1 when the world's last leaf is done).  That
+// node is the right child of the split at `last`: internal node last+1 when its range
+// starts there (an internal node always has its own index as one end of its range), else
+// leaf last+1.  Computed once per tree build for every node.
+__global__ void k_ropes(State S, int n)
+{
+    const int node = GID;
+    if (node >= 2 * n - 1) return;
+    const int last = node >= n - 1 ? node - (n - 1) : S.nodeLast[node];
+    const int w = S.bodyWorld[S.sortVal[last]];
+    int rope = -1;
+    if (w >= 0 && last < S.wBodyBase[w] + S.wBodyCount[w] - 1) {
+        const int q = last + 1;
+        rope = (q < n - 1 && S.nodeFirst[q] == q) ? q : n - 1 + q;
+    }
+    S.nodeRope[node] = rope;
+}
+
+// Walk body i's world subtree without a stack (each node carries its rope); `visit(j)`
+// for every j > i whose swept AABB overlaps i's, in left-first depth-first order.
+// A walk visits each node at most once; one that does not end within 2n steps (a corrupt
+// tree) stops and returns false.
 template <typename F>
 __device__ __forceinline__ bool traverseWorld(const State &S, int n, int i, int w, F &&visit)
 {
+    const int root = S.wRoot[w];
+    if (root < 0) return true;   // flagged by k_world_roots_check
     const v3 c = ld3(S.aabbC[i]), h = ld3(S.aabbH[i]);
     const v3 qMin = c - h, qMax = c + h;
-    int stack[TRAV_STACK];
-    int sp = 0;
-    const int root = S.wRoot[w];
-    stack[sp++] = root >= 0 ? root : 0;
-    const int wLo = S.wBodyBase[w], wHi = wLo + S.wBodyCount[w] - 1;
-    while (sp > 0) {
-        const int node = stack[--sp];
+    int node = root;
+    for (int guard = 2 * n; node >= 0; guard--) {
+        if (guard == 0) return false;
+        const float4 lo = S.nodeBox[2 * node], hi = S.nodeBox[2 * node + 1];
+        const bool ov = aabbOverlap(qMin, qMax, ld3(lo), ld3(hi));
+        const int down = __float_as_int(lo.w), rope = __float_as_int(hi.w);
         if (node >= n - 1) {
-            const int p = node - (n - 1);
-            if (p < wLo || p > wHi) continue;
-            const int j = S.leafBody[p];
-            if (j <= i) continue;
-            const v3 cj = ld3(S.aabbC[j]), hj = ld3(S.aabbH[j]);
-            if (!aabbOverlap(qMin, qMax, cj - hj, cj + hj)) continue;
-            visit(j);
-            continue;
+            if (ov && down > i) visit(down);
+            node = rope;
+        } else {
+            node = ov ? down : rope;
         }
-        if (root < 0 && (S.nodeLast[node] < wLo || S.nodeFirst[node] > wHi)) continue;
-        const v3 nMin = ld3(S.nodeMin[node]), nMax = ld3(S.nodeMax[node]);
-        if (nMin.x > qMax.x || nMax.x < qMin.x || nMin.y > qMax.y || nMax.y < qMin.y || nMin.z > qMax.z ||
-            nMax.z < qMin.z)
-            continue;
-        if (sp + 2 > TRAV_STACK) return false;
-        stack[sp++] = S.nodeRight[node];
-        stack[sp++] = S.nodeLeft[node];
     }
     return true;
 }
@@ -422,8 +434,10 @@ __device__ __forceinline__ bool traverseWorld(const State &S, int n, int i, int 
 // for a body with more, walks again) into pairJ at pairOff[i], sorted by j.
 __global__ void __launch_bounds__(TPB) k_traverse(State S, int n)
 {
-    const int i = GID;
-    if (i >= S.numBodies) return;
+    // queries in Morton order: neighbouring lanes walk neighbouring paths
+    const int p = GID;
+    if (p >= S.numBodies) return;
+    const int i = S.sortVal[p];
     const int w = S.bodyWorld[i];
     if (w < 0 || n < 2) { S.pairCount[i] = 0; return; }
     uint32_t count = 0;
@@ -1144,7 +1158,7 @@ __global__ void __launch_bounds__(TPB) k_overflow_solve(State S)
     if (i >= S.numBodies || S.colour[i] != 8 || S.bodyWorld[i] < 0) return;
     if (S.updateIndex[i] < 0) return;
     Block blk;
-    assemble(S, i, blk);
+    assembleStamped(S, i, blk);
     v3 dl, da;
     if (!solveDeltaFast(blk, dl, da)) { queueFallback(S, i, blk); return; }
     S.dLin[i] = st3(dl);
@@ -1184,12 +1198,23 @@ __global__ void k_overflow_apply(State S)
     S.rot[i] = stq(qadd(bRot(S, i), ld3(S.dAng[i]) * r));
 }
 
-// solver_step.cpp:277-296: manifolds (not frozen), then joints.
+// Stamps of every live manifold that touches a movable body, at the current poses.
+__global__ void __launch_bounds__(TPB) k_stamp(State S)
+{
+    FOR_LIVE(k) {
+        const int slot = S.live[S.gLive[k]];
+        if (bMass(S, S.mA[slot]) > 0.0f || bMass(S, S.mB[slot]) > 0.0f) manifoldStamps(S, slot);
+    }
+}
+
+// solver_step.cpp:277-296: manifolds (not frozen), then joints.  STAMP: also build the next
+// Jacobi iteration's stamps.
+template <bool STAMP>
 __global__ void __launch_bounds__(TPB) k_dual(State S, float decay)
 {
     FOR_LIVE(k) {
         const int slot = S.live[S.gLive[k]];
-        if (bMass(S, S.mA[slot]) > 0.0f || bMass(S, S.mB[slot]) > 0.0f) manifoldDual(S, slot, decay);
+        if (bMass(S, S.mA[slot]) > 0.0f || bMass(S, S.mB[slot]) > 0.0f) manifoldDual<STAMP>(S, slot, decay);
     }
     for (int t = GID; t < S.jointCap; t += gridDim.x * blockDim.x)
         if (S.jWorld[t] >= 0 && !S.jDead[t]) jointDual(S, t);

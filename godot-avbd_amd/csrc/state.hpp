@@ -96,7 +96,10 @@ struct State {
     unsigned long long *mortonCode, *sortKey, *sortKeyAlt;
     int *sortVal, *sortValAlt;
     int *nodeLeft, *nodeRight, *nodeParent, *nodeFirst, *nodeLast, *nodeFlag;
-    float4 *nodeMin, *nodeMax;   // internal nodes [0, n-1), leaves [n-1, 2n-1)
+    // Two float4 per node (internal nodes [0, n-1), leaves [n-1, 2n-1)):
+    // {min, left child id | leaf: body}, {max, rope = next node after this subtree, -1 = end}
+    float4 *nodeBox;
+    int *nodeRope;
     int *leafBody;               // sorted position -> body
     int *bodyLeaf;               // body -> sorted position
     uint32_t *pairCount, *pairOff;

@@ -169,7 +169,9 @@ typedef struct avbd_step_stats {
 #define AVBD_K_ADJACENCY 9     /* CSR adjacency build                                      */
 #define AVBD_K_SORT 10         /* Morton radix sort                                        */
 #define AVBD_K_BODY 11         /* warm start + finish + Jacobi combine                     */
-#define AVBD_K_CLASSES 12
+#define AVBD_K_STAMP 12        /* Jacobi stamps of every manifold at the current poses     */
+#define AVBD_K_REFIT 13        /* LBVH refit, per world                                    */
+#define AVBD_K_CLASSES 14
 
 typedef struct avbd_kernel_profile {
     int32_t steps;
