@@ -258,6 +258,11 @@ AD void solveDelta(const Block &k, v3 &dLin, v3 &dAng)
 }
 
 // ----------------------------------------------------------- manifolds ----
+AD m3 loadBasisK(const State &S, int k)
+{
+    const v3 r0 = ld3(S.mBasisK[3 * k]), r1 = ld3(S.mBasisK[3 * k + 1]), r2 = ld3(S.mBasisK[3 * k + 2]);
+    return transpose(m3{{r0, r1, r2}});
+}
 AD m3 loadBasis(const State &S, int slot)
 {
     const v3 r0 = ld3(S.mBasis[3 * slot]), r1 = ld3(S.mBasis[3 * slot + 1]), r2 = ld3(S.mBasis[3 * slot + 2]);
@@ -322,12 +327,13 @@ AD void stampSide(const Hess &h, const m3 &Sm, v3 r, v3 fw, m3 &accAng, m3 &accC
 // One manifold's contribution to one body's block (manifold.cpp:650-686 force + :688-755
 // stamp, fused).  Forces are recomputed at the current poses; the reference's cache holds
 // the same floats because nothing it reads moves between fill and use.
-AD void stampManifold(const State &S, int slot, int body, Block &blk)
+AD void stampManifold(const State &S, int k, int body, Block &blk)
 {
-    const int a = S.mA[slot], b = S.mB[slot];
-    const m3 B = loadBasis(S, slot);
-    const float mu = S.mFric[slot];
-    const int n = S.mNum[slot];
+    const int4 hd = S.mHdr[k];
+    const int a = hd.x, b = hd.y, n = hd.w;
+    const uint32_t cb = static_cast<uint32_t>(hd.z);
+    const m3 B = loadBasisK(S, k);
+    const float mu = S.mFricK[k];
     const qt qa = bRot(S, a), qb = bRot(S, b);
     const v3 dqAL = bPos(S, a) - ld3(S.initLin[a]), dqAA = qsub(qa, ldq(S.initAng[a]));
     const v3 dqBL = bPos(S, b) - ld3(S.initLin[b]), dqBA = qsub(qb, ldq(S.initAng[b]));
@@ -338,7 +344,7 @@ AD void stampManifold(const State &S, int slot, int body, Block &blk)
     m3 accLin = zero3(), accAng = zero3(), accCross = zero3();
     v3 accRL{0, 0, 0}, accRA{0, 0, 0};
     for (int i = 0; i < n; i++) {
-        const int c = slot * MAXC + i;
+        const uint32_t c = cb + i;
         const v3 rA = mul(rotA, ld3(S.cRA[c]));
         const v3 rB = mul(rotB, ld3(S.cRB[c]));
         const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
@@ -397,9 +403,9 @@ AD void storeStampSide(float4 *p, const m3 &L, v3 rl, const m3 &A, const m3 &X, 
     p[5] = make_float4(X.c[1][2], X.c[2][0], X.c[2][1], X.c[2][2]);
     p[6] = make_float4(ra.x, ra.y, ra.z, 0.0f);
 }
-AD void storeStamps(const State &S, int slot, const StampAcc &s)
+AD void storeStamps(const State &S, int k, const StampAcc &s)
 {
-    float4 *p = S.stamp + static_cast<size_t>(STAMP_F4) * slot;
+    float4 *p = S.stamp + static_cast<size_t>(STAMP_F4) * k;
     storeStampSide(p, s.lin, s.rl, s.angA, s.crossA, s.raA);
     storeStampSide(p + 7, s.lin, s.rl * -1.0f, s.angB, s.crossB, s.raB * -1.0f);
 }
@@ -419,12 +425,13 @@ AD void addStampSide(Block &blk, const float4 *p)
 }
 
 // Both sides' stamps of one manifold at the current poses.
-AD void manifoldStamps(const State &S, int slot)
+AD void manifoldStamps(const State &S, int k)
 {
-    const int a = S.mA[slot], b = S.mB[slot];
-    const m3 B = loadBasis(S, slot);
-    const float mu = S.mFric[slot];
-    const int n = S.mNum[slot];
+    const int4 hd = S.mHdr[k];
+    const int a = hd.x, b = hd.y, n = hd.w;
+    const uint32_t cb = static_cast<uint32_t>(hd.z);
+    const m3 B = loadBasisK(S, k);
+    const float mu = S.mFricK[k];
     const qt qa = bRot(S, a), qb = bRot(S, b);
     const v3 dqAL = bPos(S, a) - ld3(S.initLin[a]), dqAA = qsub(qa, ldq(S.initAng[a]));
     const v3 dqBL = bPos(S, b) - ld3(S.initLin[b]), dqBA = qsub(qb, ldq(S.initAng[b]));
@@ -432,7 +439,7 @@ AD void manifoldStamps(const State &S, int slot)
     StampAcc acc;
     stampInit(acc);
     for (int i = 0; i < n; i++) {
-        const int c = slot * MAXC + i;
+        const uint32_t c = cb + i;
         const v3 rA = mul(rotA, ld3(S.cRA[c]));
         const v3 rB = mul(rotB, ld3(S.cRB[c]));
         const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
@@ -443,7 +450,7 @@ AD void manifoldStamps(const State &S, int slot)
         clampForce(f, mu, bounds);
         stampContact(acc, B, pen, f, rA, rB);
     }
-    storeStamps(S, slot, acc);
+    storeStamps(S, k, acc);
 }
 
 // manifold.cpp:768-856 (arm caching off): lambda <- clamped force, penalty ramps.  With
@@ -451,12 +458,13 @@ AD void manifoldStamps(const State &S, int slot)
 // penalty: the poses do not move between this dual and that primal, so its constraint
 // error is the one computed here.
 template <bool STAMP>
-AD void manifoldDual(const State &S, int slot, float decay)
+AD void manifoldDual(const State &S, int k, float decay)
 {
-    const int a = S.mA[slot], b = S.mB[slot];
-    const m3 B = loadBasis(S, slot);
-    const float mu = S.mFric[slot];
-    const int n = S.mNum[slot];
+    const int4 hd = S.mHdr[k];
+    const int a = hd.x, b = hd.y, n = hd.w;
+    const uint32_t cb = static_cast<uint32_t>(hd.z);
+    const m3 B = loadBasisK(S, k);
+    const float mu = S.mFricK[k];
     const qt qa = bRot(S, a), qb = bRot(S, b);
     const v3 dqAL = bPos(S, a) - ld3(S.initLin[a]), dqAA = qsub(qa, ldq(S.initAng[a]));
     const v3 dqBL = bPos(S, b) - ld3(S.initLin[b]), dqBA = qsub(qb, ldq(S.initAng[b]));
@@ -465,7 +473,7 @@ AD void manifoldDual(const State &S, int slot, float decay)
     StampAcc acc;
     if (STAMP) stampInit(acc);
     for (int i = 0; i < n; i++) {
-        const int c = slot * MAXC + i;
+        const uint32_t c = cb + i;
         v3 pen = ld3(S.cPen[c]);
         if (decay < 1.0f) pen = lclamp3(pen * decay, PENALTY_MIN, pmax);
         const float4 ra4 = S.cRA[c];
@@ -496,19 +504,20 @@ AD void manifoldDual(const State &S, int slot, float decay)
             stampContact(acc, B, pen, f, rA, rB);
         }
     }
-    if (STAMP) storeStamps(S, slot, acc);
+    if (STAMP) storeStamps(S, k, acc);
 }
 
 // manifold.cpp:633-646
-AD void manifoldRebase(const State &S, int slot)
+AD void manifoldRebase(const State &S, int k)
 {
-    const int a = S.mA[slot], b = S.mB[slot];
-    const m3 B = loadBasis(S, slot);
-    const int n = S.mNum[slot];
+    const int4 hd = S.mHdr[k];
+    const int a = hd.x, b = hd.y, n = hd.w;
+    const uint32_t cb = static_cast<uint32_t>(hd.z);
+    const m3 B = loadBasisK(S, k);
     const v3 pa = bPos(S, a), pb = bPos(S, b);
     const qt qa = bRot(S, a), qb = bRot(S, b);
     for (int i = 0; i < n; i++) {
-        const int c = slot * MAXC + i;
+        const uint32_t c = cb + i;
         const v3 xA = pa + qrot(qa, ld3(S.cRA[c]));
         const v3 xB = pb + qrot(qb, ld3(S.cRB[c]));
         v3 c0 = mul(B, xA - xB) + v3{COLLISION_MARGIN, 0, 0};
@@ -640,7 +649,7 @@ AD void assembleStamped(const State &S, int body, Block &blk)
     const int e0 = S.adjStart[body], e1 = S.adjStart[body + 1];
     for (int e = e0; e < e1; e++) {
         const int ref = S.adj[e];
-        if (ref >= 0) addStampSide(blk, S.stamp + static_cast<size_t>(STAMP_F4) * ref + (S.mA[ref] == body ? 0 : 7));
+        if (ref >= 0) addStampSide(blk, S.stamp + static_cast<size_t>(STAMP_F4) * ref + (S.mHdr[ref].x == body ? 0 : 7));
         else jointPrimal(S, -ref - 1, body, blk);
     }
 }

@@ -138,7 +138,6 @@ struct Engine {
     int tileSumsCap = 0;
     uint32_t *radixHist = nullptr;
     int radixBlocks = 0;
-    int *slotToLive = nullptr;
     int *errHost = nullptr;  // pinned
     int *countsHost = nullptr;
     bool csrValid = false;
@@ -266,6 +265,8 @@ struct Engine {
         AL(S.jFrac, J); AL(S.jArm, J); AL(S.jBroken, J); AL(S.jDead, J);
         AL(S.mA, M); AL(S.mB, M); AL(S.mBasis, 3 * M); AL(S.mFric, M); AL(S.mNum, M);
         AL(S.cRA, C); AL(S.cRB, C); AL(S.cC0, C); AL(S.cPen, C); AL(S.cLam, C);
+        AL(S.sRA, C); AL(S.sRB, C); AL(S.sC0, C); AL(S.sPen, C); AL(S.sLam, C);
+        AL(S.cOff, M); AL(S.cCount, M + 1); AL(S.cScan, M + 1);
         AL(S.stamp, STAMP_F4 * M); AL(S.slotWorld, M); AL(S.gLive, M); AL(S.wLiveOff, W + 1); AL(S.live, M); AL(S.liveTmp, M); AL(S.freeList, M);
         AL(S.keep, M + 1); AL(S.keepScan, M + 1); AL(S.relScan, M + 1);
         AL(S.succA, M); AL(S.succB, M); AL(S.indeg, M); AL(S.levelOrder, M);
@@ -282,7 +283,7 @@ struct Engine {
         AL(S.pairValid, P + 1); AL(S.pairValidScan, P + 1);
         AL(S.pairScratch, N * PAIR_BUF); AL(S.wRoot, W);
         AL(S.wBoundsLo, W); AL(S.wBoundsHi, W);
-        AL(slotToLive, M);
+        AL(S.mHdr, M); AL(S.mBasisK, 3 * M); AL(S.mFricK, M);
         radixBlocks = (Ncap + RADIX_TILE - 1) / RADIX_TILE;
         AL(radixHist, static_cast<size_t>(256) * radixBlocks + 1);
         const int biggest = std::max({Ncap + 1, Pcap + 1, Scap + 1, 256 * radixBlocks + 1, 2 * (Scap + Jcap)});
@@ -293,6 +294,8 @@ struct Engine {
         k_fill_i32<<<nblk(Ncap), TPB, 0, stream>>>(S.bodyWorld, -1, Ncap);
         k_fill_i32<<<nblk(Ncap), TPB, 0, stream>>>(S.updateIndex, -1, Ncap);
         k_fill_i32<<<nblk(Scap), TPB, 0, stream>>>(S.slotWorld, -1, Scap);
+        k_fill_i32<<<nblk(Scap), TPB, 0, stream>>>(reinterpret_cast<int *>(S.cOff), 0, Scap);
+        k_fill_i32<<<nblk(Scap), TPB, 0, stream>>>(S.mNum, 0, Scap);
         k_fill_i32<<<nblk(Jcap), TPB, 0, stream>>>(S.jWorld, -1, Jcap);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream));
@@ -335,6 +338,21 @@ struct Engine {
         k_glive_fill<<<Wcap, TPB, 0, stream>>>(S);
     }
 
+    // Dense contact layout for the current live list (staging -> cRA...).
+    void packContacts()
+    {
+        k_contact_count<<<liveGrid(), TPB, 0, stream>>>(S);
+        scan(S.cCount, S.cScan, Scap);
+        k_contact_pack<<<liveGrid(), TPB, 0, stream>>>(S);
+    }
+
+    // Live list and dense manifold view after host edits (contacts stay where they are).
+    void refreshView()
+    {
+        buildLiveList();
+        k_manifold_view<<<liveGrid(), TPB, 0, stream>>>(S);
+    }
+
     void buildAdjacency()
     {
         const int n = Ncap;
@@ -364,7 +382,7 @@ struct Engine {
         const int nb = nblk(Ncap);
         if (timed) (void)hipEventRecord(ev[0], stream);
         if (!csrValid) {
-            buildLiveList();
+            refreshView();
             buildAdjacency();
         }
         // ---- broad phase (solver_broadphase.cpp:85-186)
@@ -408,6 +426,7 @@ struct Engine {
         k_compact_counts<<<nblk(Wcap), TPB, 0, stream>>>(S);
         k_copy_live<<<nblk(Scap), TPB, 0, stream>>>(S);
         buildLiveList();
+        packContacts();
         pbeg(AVBD_K_ADJACENCY);
         buildAdjacency();
         pend();
@@ -423,9 +442,8 @@ struct Engine {
         }
         k_colour_override<<<nb, TPB, 0, stream>>>(S);
         // ---- brake schedule and first brake (solver_projection.cpp:96-167)
-        k_slot_to_live<<<liveGrid(), TPB, 0, stream>>>(S, slotToLive);
         k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wUpdateCount, 0, Wcap);
-        k_brake_links<<<liveGrid(), TPB, 0, stream>>>(S, slotToLive);
+        k_brake_links<<<liveGrid(), TPB, 0, stream>>>(S);
         const bool brake = prm.speculative_margin > 0.0f;
         pbeg(AVBD_K_BRAKE_LEVELS);
         k_brake_levels<<<Wcap, BRAKE_TPB, 0, stream>>>(S, collisionDt, brake ? 1 : 0);
@@ -502,7 +520,7 @@ struct Engine {
                 k_proj_manifold<<<liveGrid(), TPB, 0, stream>>>(S);
                 pend();
                 pbeg(AVBD_K_PROJ_BODY);
-                k_proj_body<<<nb, TPB, 0, stream>>>(S, slotToLive);
+                k_proj_body<<<nb, TPB, 0, stream>>>(S);
                 pend();
                 k_proj_flags<<<nblk(Wcap), TPB, 0, stream>>>(S);
             }
@@ -612,7 +630,7 @@ struct Engine {
     avbd_status stepBatch()
     {
         if (!csrValid) {
-            buildLiveList();
+            refreshView();
             buildAdjacency();
             csrValid = true;
         }
@@ -842,6 +860,10 @@ struct Engine {
         }
         m->count = nLive;
         if (nLive == 0) return AVBD_OK;
+        buildLiveList();
+        k_contact_unpack<<<liveGrid(), TPB, 0, stream>>>(S);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
         std::vector<int> live(nLive);
         HIPCHK(hipMemcpy(live.data(), S.live + wi.slotBase, sizeof(int) * nLive, hipMemcpyDeviceToHost));
         const int cap = wi.slotCap, sb = wi.slotBase;
@@ -854,11 +876,11 @@ struct Engine {
         HIPCHK(hipMemcpy(fr.data(), S.mFric + sb, sizeof(float) * cap, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(basis.data(), S.mBasis + 3 * sb, sizeof(float4) * 3 * cap, hipMemcpyDeviceToHost));
         const size_t cb = static_cast<size_t>(sb) * MAXC, cn = static_cast<size_t>(cap) * MAXC;
-        HIPCHK(hipMemcpy(cra.data(), S.cRA + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(crb.data(), S.cRB + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(cc0.data(), S.cC0 + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(cp.data(), S.cPen + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(cl.data(), S.cLam + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(cra.data(), S.sRA + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(crb.data(), S.sRB + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(cc0.data(), S.sC0 + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(cp.data(), S.sPen + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(cl.data(), S.sLam + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
         const int o = wi.bodyBase;
         auto put3 = [](float *dst, int i, float4 v) { if (dst) { dst[3 * i] = v.x; dst[3 * i + 1] = v.y; dst[3 * i + 2] = v.z; } };
         for (int i = 0; i < nLive; i++) {
@@ -907,6 +929,7 @@ struct Engine {
             mA[i] = m->body_a[i] + o;
             mB[i] = m->body_b[i] + o;
             num[i] = m->num_contacts[i];
+            if (num[i] < 0 || num[i] > MAXC) return fail(AVBD_ERR_INVALID, "manifold contact count out of range");
             fr[i] = m->friction[i];
             for (int r = 0; r < 3; r++)
                 basis[3 * i + r] = make_float4(m->basis[9 * i + 3 * r], m->basis[9 * i + 3 * r + 1], m->basis[9 * i + 3 * r + 2], 0);
@@ -921,6 +944,13 @@ struct Engine {
                 cl[q] = make_float4(m->lambda[3 * q], m->lambda[3 * q + 1], m->lambda[3 * q + 2], 0);
             }
         }
+        // every other live manifold's contacts go to staging, this world's are written there,
+        // and the dense layout is rebuilt from staging
+        HIPCHK(hipStreamSynchronize(stream));
+        buildLiveList();
+        k_contact_unpack<<<liveGrid(), TPB, 0, stream>>>(S);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
         if (n > 0) {
             HIPCHK(hipMemcpy(S.live + sb, live.data(), sizeof(int) * n, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(S.mA + sb, mA.data(), sizeof(int) * n, hipMemcpyHostToDevice));
@@ -929,15 +959,19 @@ struct Engine {
             HIPCHK(hipMemcpy(S.mFric + sb, fr.data(), sizeof(float) * n, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(S.mBasis + 3 * sb, basis.data(), sizeof(float4) * 3 * n, hipMemcpyHostToDevice));
             const size_t cb = static_cast<size_t>(sb) * MAXC, cn = static_cast<size_t>(n) * MAXC;
-            HIPCHK(hipMemcpy(S.cRA + cb, cra.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(S.cRB + cb, crb.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(S.cC0 + cb, cc0.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(S.cPen + cb, cp.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(S.cLam + cb, cl.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S.sRA + cb, cra.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S.sRB + cb, crb.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S.sC0 + cb, cc0.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S.sPen + cb, cp.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S.sLam + cb, cl.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
         }
         const int vals[3] = {n, 0, n};
         int *dst[3] = {S.wLive, S.wFree, S.wTop};
         for (int k = 0; k < 3; k++) HIPCHK(hipMemcpy(dst[k] + w, &vals[k], sizeof(int), hipMemcpyHostToDevice));
+        buildLiveList();
+        packContacts();
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
         csrValid = false;
         return AVBD_OK;
     }
@@ -946,7 +980,10 @@ struct Engine {
     {
         const WorldInfo &wi = worlds[0];
         setParams(S.P, prm, prm.dt);
-        if (!csrValid) buildAdjacency();
+        if (!csrValid) {
+            refreshView();
+            buildAdjacency();
+        }
         csrValid = true;
         const int nb = nblk(Ncap);
         k_swept_aabb<<<nb, TPB, 0, stream>>>(S);

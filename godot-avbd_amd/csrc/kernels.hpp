@@ -490,7 +490,7 @@ __global__ void __launch_bounds__(TPB) k_pair_emit(State S, int n)
 __device__ __forceinline__ int otherOf(const State &S, int ref, int body)
 {
     int a, b;
-    if (ref >= 0) { a = S.mA[ref]; b = S.mB[ref]; }
+    if (ref >= 0) { const int4 h = S.mHdr[ref]; a = h.x; b = h.y; }
     else { const int j = -ref - 1; a = S.jA[j]; b = S.jB[j]; }
     if (a == body) return b;
     if (b == body) return a;
@@ -586,6 +586,76 @@ __global__ void k_alloc_counts(State S)
     S.wNew[w] = newCount;
 }
 
+// ------------------------------------------------------- contact storage ----
+// Copy a live manifold's dense contacts to its staging rows.
+__device__ __forceinline__ void stageContacts(const State &S, int slot)
+{
+    const int n = S.mNum[slot];
+    const uint32_t o = S.cOff[slot];
+    const size_t d = static_cast<size_t>(slot) * MAXC;
+    for (int i = 0; i < n; i++) {
+        S.sRA[d + i] = S.cRA[o + i];
+        S.sRB[d + i] = S.cRB[o + i];
+        S.sC0[d + i] = S.cC0[o + i];
+        S.sPen[d + i] = S.cPen[o + i];
+        S.sLam[d + i] = S.cLam[o + i];
+    }
+}
+
+// Dense per-step view of live manifold k (stored in `slot`, contacts from `o`).
+__device__ __forceinline__ void manifoldView(const State &S, int k, int slot, uint32_t o, int n)
+{
+    S.mHdr[k] = make_int4(S.mA[slot], S.mB[slot], static_cast<int>(o), n);
+    S.mBasisK[3 * k] = S.mBasis[3 * slot];
+    S.mBasisK[3 * k + 1] = S.mBasis[3 * slot + 1];
+    S.mBasisK[3 * k + 2] = S.mBasis[3 * slot + 2];
+    S.mFricK[k] = S.mFric[slot];
+}
+
+// The view alone, contacts left where they are (after host edits of the live lists).
+__global__ void k_manifold_view(State S)
+{
+    FOR_LIVE(k) {
+        const int slot = S.live[S.gLive[k]];
+        manifoldView(S, k, slot, S.cOff[slot], S.mNum[slot]);
+    }
+}
+
+// Contact counts in gLive order (zero past the end, so a scan over the whole capacity
+// gives each manifold's dense offset).
+__global__ void k_contact_count(State S)
+{
+    const int total = static_cast<int>(S.wLiveOff[S.numWorlds]);
+    for (int k = GID; k < S.slotCap; k += gridDim.x * blockDim.x)
+        S.cCount[k] = k < total ? static_cast<uint32_t>(S.mNum[S.live[S.gLive[k]]]) : 0u;
+}
+
+// Staging -> dense arrays at the scanned offsets.
+__global__ void k_contact_pack(State S)
+{
+    FOR_LIVE(k) {
+        const int slot = S.live[S.gLive[k]];
+        const uint32_t o = S.cScan[k];
+        S.cOff[slot] = o;
+        const int n = S.mNum[slot];
+        manifoldView(S, k, slot, o, n);
+        const size_t d = static_cast<size_t>(slot) * MAXC;
+        for (int i = 0; i < n; i++) {
+            S.cRA[o + i] = S.sRA[d + i];
+            S.cRB[o + i] = S.sRB[d + i];
+            S.cC0[o + i] = S.sC0[d + i];
+            S.cPen[o + i] = S.sPen[d + i];
+            S.cLam[o + i] = S.sLam[d + i];
+        }
+    }
+}
+
+// Dense -> staging for every live manifold (host reads and writes go through staging).
+__global__ void k_contact_unpack(State S)
+{
+    FOR_LIVE(k) stageContacts(S, S.live[S.gLive[k]]);
+}
+
 // ------------------------------------------------------------ narrow phase ----
 // manifold.cpp:237-435 with collide.cpp:1100-1120: collide at the step-start pose,
 // carry hot state across matching feature keys, C0, release, warm-start decay.
@@ -600,7 +670,11 @@ __global__ void __launch_bounds__(TPB) k_manifold_init(State S)
     const int slot = S.live[t];
     const int a = S.mA[slot], b = S.mB[slot];
     const float ma = bMass(S, a), mb = bMass(S, b);
-    if (!(ma > 0.0f) && !(mb > 0.0f)) { S.keep[t] = 1; return; }  // frozen: parked verbatim
+    if (!(ma > 0.0f) && !(mb > 0.0f)) {   // frozen: parked verbatim
+        stageContacts(S, slot);
+        S.keep[t] = 1;
+        return;
+    }
     const float fa = S.fric[a], fb = S.fric[b];
     const float mu = sqrtf(fa * fb);
     S.mFric[slot] = mu;
@@ -628,21 +702,9 @@ __global__ void __launch_bounds__(TPB) k_manifold_init(State S)
     } else {
         B = loadBasis(S, slot);
     }
+    // old contacts stay in the dense arrays; the new ones go to staging
     const int oldN = S.mNum[slot];
-    // old contact state
-    int oKey[MAXC];
-    v3 oRA[MAXC], oRB[MAXC], oPen[MAXC], oLam[MAXC];
-    float oBeta[MAXC], oC0x[MAXC];
-    uint8_t oStick[MAXC];
-    for (int j = 0; j < oldN; j++) {
-        const int c = slot * MAXC + j;
-        const float4 ra = S.cRA[c], rb = S.cRB[c], c0 = S.cC0[c];
-        oRA[j] = ld3(ra); oBeta[j] = ra.w;
-        oRB[j] = ld3(rb); oKey[j] = __float_as_int(rb.w);
-        oC0x[j] = c0.x; oStick[j] = c0.w != 0.0f ? 1 : 0;
-        oPen[j] = ld3(S.cPen[c]);
-        oLam[j] = ld3(S.cLam[c]);
-    }
+    const uint32_t ob = oldN > 0 ? S.cOff[slot] : 0u;
     const float pmax = S.P.penaltyMax;
     for (int i = 0; i < nn; i++) {
         v3 rA = cs.rA[i], rB = cs.rB[i];
@@ -650,12 +712,17 @@ __global__ void __launch_bounds__(TPB) k_manifold_init(State S)
         uint8_t stick = 0;
         float beta = 1.0f, prevErr = -1.0f;
         for (int j = 0; j < oldN; j++) {
-            if (cs.key[i] != oKey[j]) continue;
-            prevErr = smax(0.0f, COLLISION_MARGIN - oC0x[j]);
-            pen = oPen[j]; lam = oLam[j]; stick = oStick[j]; beta = oBeta[j];
-            if (oStick[j]) {
-                const v3 oldA = pa + qrot(qa, oRA[j]);
-                const v3 oldB = pb + qrot(qb, oRB[j]);
+            const float4 orb = S.cRB[ob + j];
+            if (cs.key[i] != __float_as_int(orb.w)) continue;
+            const float4 ora = S.cRA[ob + j], oc0 = S.cC0[ob + j];
+            prevErr = smax(0.0f, COLLISION_MARGIN - oc0.x);
+            pen = ld3(S.cPen[ob + j]);
+            lam = ld3(S.cLam[ob + j]);
+            stick = oc0.w != 0.0f ? 1 : 0;
+            beta = ora.w;
+            if (stick) {
+                const v3 oldA = pa + qrot(qa, ld3(ora));
+                const v3 oldB = pb + qrot(qb, ld3(orb));
                 const v3 newA = pa + qrot(qa, rA);
                 const v3 newB = pb + qrot(qb, rB);
                 const float drift = smax(length(oldA - newA), length(oldB - newB));
@@ -663,7 +730,7 @@ __global__ void __launch_bounds__(TPB) k_manifold_init(State S)
                     stick = 0;
                     lam = v3{lam.x, 0.0f, 0.0f};
                 } else {
-                    rA = oRA[j]; rB = oRB[j];
+                    rA = ld3(ora); rB = ld3(orb);
                 }
             }
             break;
@@ -688,12 +755,12 @@ __global__ void __launch_bounds__(TPB) k_manifold_init(State S)
         }
         lam = lam * S.P.alpha * S.P.gamma;
         pen = lclamp3(pen * S.P.gamma, PENALTY_MIN, pmax);
-        const int c = slot * MAXC + i;
-        S.cRA[c] = st3(rA, scale);
-        S.cRB[c] = st3(rB, __int_as_float(cs.key[i]));
-        S.cC0[c] = st3(c0, stick ? 1.0f : 0.0f);
-        S.cPen[c] = st3(pen);
-        S.cLam[c] = st3(lam);
+        const size_t c = static_cast<size_t>(slot) * MAXC + i;
+        S.sRA[c] = st3(rA, scale);
+        S.sRB[c] = st3(rB, __int_as_float(cs.key[i]));
+        S.sC0[c] = st3(c0, stick ? 1.0f : 0.0f);
+        S.sPen[c] = st3(pen);
+        S.sLam[c] = st3(lam);
     }
     S.mNum[slot] = nn;
     S.keep[t] = nn > 0 ? 1u : 0u;
@@ -781,9 +848,9 @@ __global__ void k_adj_count(State S)
         }
     }
     FOR_LIVE(k) {
-        const int slot = S.live[S.gLive[k]];
-        atomicAdd(&S.adjCount[S.mA[slot]], 1);
-        atomicAdd(&S.adjCount[S.mB[slot]], 1);
+        const int4 h = S.mHdr[k];
+        atomicAdd(&S.adjCount[h.x], 1);
+        atomicAdd(&S.adjCount[h.y], 1);
     }
 }
 
@@ -803,11 +870,11 @@ __global__ void k_adj_fill(State S)
         }
     }
     FOR_LIVE(k) {
-        const int t = S.gLive[k];
-        const int slot = S.live[t];
-        const unsigned long long seq = 0x40000000ull + static_cast<unsigned long long>(t - S.wSlotBase[S.slotWorld[t]]);
-        const unsigned long long e = (seq << 32) | static_cast<uint32_t>(slot);
-        const int a = S.mA[slot], b = S.mB[slot];
+        // k grows with live order inside a world, and a body's entries are all of one world
+        const unsigned long long seq = 0x40000000ull + static_cast<unsigned long long>(k);
+        const unsigned long long e = (seq << 32) | static_cast<uint32_t>(k);
+        const int4 h = S.mHdr[k];
+        const int a = h.x, b = h.y;
         S.adjTmp[S.adjStart[a] + atomicAdd(&S.adjCount[a], 1)] = e;
         S.adjTmp[S.adjStart[b] + atomicAdd(&S.adjCount[b], 1)] = e;
     }
@@ -916,39 +983,31 @@ __global__ void k_colour_override(State S)
 // movable body (that body's adjacency is in live order).
 __device__ __forceinline__ bool bodyMoves(const State &S, int b) { return bMass(S, b) > 0.0f && S.updateIndex[b] >= 0; }
 
-__device__ __forceinline__ bool manifoldActive(const State &S, int slot)
+__device__ __forceinline__ bool manifoldActive(const State &S, int k)
 {
-    return S.mNum[slot] > 0 && (bodyMoves(S, S.mA[slot]) || bodyMoves(S, S.mB[slot]));
+    const int4 h = S.mHdr[k];
+    return h.w > 0 && (bodyMoves(S, h.x) || bodyMoves(S, h.y));
 }
 
-__global__ void k_slot_to_live(State S, int *slotToLive)
+__device__ void brakeLinks(const State &S, int k);
+__global__ void k_brake_links(State S)
 {
-    FOR_LIVE(k) {
-        const int t = S.gLive[k];
-        slotToLive[S.live[t]] = t;
-    }
+    FOR_LIVE(k) brakeLinks(S, k);
 }
-
-__device__ void brakeLinks(const State &S, const int *slotToLive, int t);
-__global__ void k_brake_links(State S, const int *slotToLive)
+__device__ void brakeLinks(const State &S, int k)
 {
-    FOR_LIVE(k) brakeLinks(S, slotToLive, S.gLive[k]);
-}
-__device__ void brakeLinks(const State &S, const int *slotToLive, int t)
-{
-    const int w = S.slotWorld[t];
-    const int slot = S.live[t];
-    S.succA[t] = -1;
-    S.succB[t] = -1;
-    S.indeg[t] = 0;
-    if (!manifoldActive(S, slot)) { S.indeg[t] = -1; return; }
+    S.succA[k] = -1;
+    S.succB[k] = -1;
+    S.indeg[k] = 0;
+    if (!manifoldActive(S, k)) { S.indeg[k] = -1; return; }
+    const int4 h = S.mHdr[k];
     int deg = 0;
     for (int side = 0; side < 2; side++) {
-        const int body = side == 0 ? S.mA[slot] : S.mB[slot];
+        const int body = side == 0 ? h.x : h.y;
         if (!bodyMoves(S, body)) continue;
         const int e0 = S.adjStart[body], e1 = S.adjStart[body + 1];
         int me = e0;
-        while (me < e1 && S.adj[me] != slot) me++;
+        while (me < e1 && S.adj[me] != k) me++;
         if (me == e1) continue;
         for (int e = me - 1; e >= e0; e--) {
             const int r = S.adj[e];
@@ -958,32 +1017,33 @@ __device__ void brakeLinks(const State &S, const int *slotToLive, int t)
         for (int e = me + 1; e < e1; e++) {
             const int r = S.adj[e];
             if (r >= 0 && manifoldActive(S, r)) {
-                if (side == 0) S.succA[t] = slotToLive[r];
-                else S.succB[t] = slotToLive[r];
+                if (side == 0) S.succA[k] = r;
+                else S.succB[k] = r;
                 break;
             }
         }
     }
-    S.indeg[t] = deg;
-    atomicAdd(&S.wUpdateCount[w], 1);  // active manifolds of the world
+    S.indeg[k] = deg;
+    atomicAdd(&S.wUpdateCount[S.bodyWorld[h.x]], 1);  // active manifolds of the world
 }
 
-__device__ void brakeManifold(const State &S, int slot, float window)
+__device__ void brakeManifold(const State &S, int k, float window)
 {
-    const int a = S.mA[slot], b = S.mB[slot];
+    const int4 hd = S.mHdr[k];
+    const int a = hd.x, b = hd.y;
     const bool moveA = bodyMoves(S, a), moveB = bodyMoves(S, b);
     const float ma = bMass(S, a), mb = bMass(S, b);
     const float imA = moveA ? 1.0f / ma : 0.0f, imB = moveB ? 1.0f / mb : 0.0f;
-    const v3 n = ld3(S.mBasis[3 * slot]);
+    const v3 n = ld3(S.mBasisK[3 * k]);
     const v3 pa = bPos(S, a), pb = bPos(S, b);
     const qt qa = bRot(S, a), qb = bRot(S, b);
     const v3 va = ld3(S.vel[a]), wa = ld3(S.angVel[a]), vb = ld3(S.vel[b]), wb = ld3(S.angVel[b]);
     v3 pRA{0, 0, 0}, pRB{0, 0, 0};
     float verr = 0.0f;
     int braked = 0;
-    const int nc = S.mNum[slot];
-    for (int k = 0; k < nc; k++) {
-        const int c = slot * MAXC + k;
+    const int nc = hd.w;
+    for (int i = 0; i < nc; i++) {
+        const uint32_t c = static_cast<uint32_t>(hd.z) + i;
         const v3 rA = qrot(qa, ld3(S.cRA[c]));
         const v3 rB = qrot(qb, ld3(S.cRB[c]));
         const float gap = dot(n, (pa + rA) - (pb + rB));
@@ -1028,7 +1088,7 @@ constexpr int BRAKE_TPB = 1024;
 __global__ void __launch_bounds__(BRAKE_TPB) k_brake_levels(State S, float window, int doBrake)
 {
     const int w = blockIdx.x;
-    const int base = S.wSlotBase[w];
+    const int base = static_cast<int>(S.wLiveOff[w]);
     const int nLive = S.wLive[w];
     __shared__ int tail, head, levels;
     if (threadIdx.x == 0) { tail = 0; head = 0; levels = 0; }
@@ -1049,7 +1109,7 @@ __global__ void __launch_bounds__(BRAKE_TPB) k_brake_levels(State S, float windo
         if (threadIdx.x == 0) S.levelStart[base + levels] = h;
         for (int k = h + threadIdx.x; k < e; k += BRAKE_TPB) {
             const int t = S.levelOrder[base + k];
-            if (doBrake) brakeManifold(S, S.live[t], window);
+            if (doBrake) brakeManifold(S, t, window);
             const int sa = S.succA[t], sb = S.succB[t];
             if (sa >= 0 && atomicSub(&S.indeg[sa], 1) == 1) S.levelOrder[base + atomicAdd(&tail, 1)] = sa;
             if (sb >= 0 && atomicSub(&S.indeg[sb], 1) == 1) S.levelOrder[base + atomicAdd(&tail, 1)] = sb;
@@ -1068,12 +1128,12 @@ __global__ void __launch_bounds__(BRAKE_TPB) k_brake_levels(State S, float windo
 __global__ void __launch_bounds__(BRAKE_TPB) k_brake_replay(State S, float window)
 {
     const int w = blockIdx.x;
-    const int base = S.wSlotBase[w];
+    const int base = static_cast<int>(S.wLiveOff[w]);
     const int L = S.brakeLevels[w];
     for (int l = 0; l < L; l++) {
         const int h = S.levelStart[base + l];
         const int e = l + 1 < L ? S.levelStart[base + l + 1] : S.wLevelEnd[w];
-        for (int k = h + threadIdx.x; k < e; k += BRAKE_TPB) brakeManifold(S, S.live[S.levelOrder[base + k]], window);
+        for (int k = h + threadIdx.x; k < e; k += BRAKE_TPB) brakeManifold(S, S.levelOrder[base + k], window);
         __threadfence_block();
         __syncthreads();
     }
@@ -1202,8 +1262,8 @@ __global__ void k_overflow_apply(State S)
 __global__ void __launch_bounds__(TPB) k_stamp(State S)
 {
     FOR_LIVE(k) {
-        const int slot = S.live[S.gLive[k]];
-        if (bMass(S, S.mA[slot]) > 0.0f || bMass(S, S.mB[slot]) > 0.0f) manifoldStamps(S, slot);
+        const int4 h = S.mHdr[k];
+        if (bMass(S, h.x) > 0.0f || bMass(S, h.y) > 0.0f) manifoldStamps(S, k);
     }
 }
 
@@ -1213,8 +1273,8 @@ template <bool STAMP>
 __global__ void __launch_bounds__(TPB) k_dual(State S, float decay)
 {
     FOR_LIVE(k) {
-        const int slot = S.live[S.gLive[k]];
-        if (bMass(S, S.mA[slot]) > 0.0f || bMass(S, S.mB[slot]) > 0.0f) manifoldDual<STAMP>(S, slot, decay);
+        const int4 h = S.mHdr[k];
+        if (bMass(S, h.x) > 0.0f || bMass(S, h.y) > 0.0f) manifoldDual<STAMP>(S, k, decay);
     }
     for (int t = GID; t < S.jointCap; t += gridDim.x * blockDim.x)
         if (S.jWorld[t] >= 0 && !S.jDead[t]) jointDual(S, t);
@@ -1231,31 +1291,31 @@ __global__ void k_proj_reset(State S)
 
 // solver_projection.cpp:642-709: per active manifold, patch depth beyond the slop and the
 // per-side corrections (applied by the body gather below).
-__device__ void projManifold(const State &S, int t);
+__device__ void projManifold(const State &S, int k);
 __global__ void __launch_bounds__(TPB) k_proj_manifold(State S)
 {
-    FOR_LIVE(k) projManifold(S, S.gLive[k]);
+    FOR_LIVE(k) projManifold(S, k);
 }
-__device__ void projManifold(const State &S, int t)
+__device__ void projManifold(const State &S, int k)
 {
-    const int w = S.slotWorld[t];
+    const int4 hd = S.mHdr[k];
+    const int a = hd.x, b = hd.y;
+    const int w = S.bodyWorld[a];
     if (S.wProjDone[w]) return;
-    S.projOn[t] = 0;
-    const int slot = S.live[t];
-    if (!manifoldActive(S, slot)) return;
-    const int a = S.mA[slot], b = S.mB[slot];
+    S.projOn[k] = 0;
+    if (!(hd.w > 0 && (bodyMoves(S, a) || bodyMoves(S, b)))) return;
     const bool moveA = bodyMoves(S, a), moveB = bodyMoves(S, b);
     const float imA = moveA ? 1.0f / bMass(S, a) : 0.0f, imB = moveB ? 1.0f / bMass(S, b) : 0.0f;
-    const v3 n = ld3(S.mBasis[3 * slot]);
+    const v3 n = ld3(S.mBasisK[3 * k]);
     const v3 pa = bPos(S, a), pb = bPos(S, b);
     const qt qa = bRot(S, a), qb = bRot(S, b);
     const m3 rotA = qmat(qa), rotB = qmat(qb);
     v3 pRA{0, 0, 0}, pRB{0, 0, 0};
     float depthSum = 0.0f;
     int corr = 0;
-    const int nc = S.mNum[slot];
-    for (int k = 0; k < nc; k++) {
-        const int c = slot * MAXC + k;
+    const int nc = hd.w;
+    for (int i = 0; i < nc; i++) {
+        const uint32_t c = static_cast<uint32_t>(hd.z) + i;
         const v3 rA = mul(rotA, ld3(S.cRA[c])), rB = mul(rotB, ld3(S.cRB[c]));
         const float raw = -dot(n, (pa + rA) - (pb + rB));
         const float depth = raw - S.P.projSlop;
@@ -1276,19 +1336,19 @@ __device__ void projManifold(const State &S, int t)
     const float K = imA + imB + (moveA ? angCompliance(IA, tA) : 0.0f) + (moveB ? angCompliance(IB, tB) : 0.0f);
     const float dp = S.P.projBeta * depthSum * invN / K;
     if (moveA) {
-        S.projLA[t] = st3(n * (imA * dp));
-        S.projAA[t] = st3(angResponse(IA, qa, tA) * dp);
+        S.projLA[k] = st3(n * (imA * dp));
+        S.projAA[k] = st3(angResponse(IA, qa, tA) * dp);
     }
     if (moveB) {
-        S.projLB[t] = st3(n * (imB * dp));
-        S.projAB[t] = st3(angResponse(IB, qb, tB) * dp);
+        S.projLB[k] = st3(n * (imB * dp));
+        S.projAB[k] = st3(angResponse(IB, qb, tB) * dp);
     }
-    S.projOn[t] = 1;
+    S.projOn[k] = 1;
 }
 
 // solver_projection.cpp:711-730: each movable body averages its corrections, summed in
 // live order, and moves; every body of a world with active manifolds is renormalised.
-__global__ void k_proj_body(State S, const int *slotToLive)
+__global__ void k_proj_body(State S)
 {
     const int i = GID;
     if (i >= S.numBodies) return;
@@ -1300,15 +1360,14 @@ __global__ void k_proj_body(State S, const int *slotToLive)
     for (int e = S.adjStart[i]; e < S.adjStart[i + 1]; e++) {
         const int ref = S.adj[e];
         if (ref < 0) continue;
-        const int t = slotToLive[ref];
-        if (!S.projOn[t]) continue;
+        if (!S.projOn[ref]) continue;
         cnt++;
-        if (S.mA[ref] == i) {
-            sl += ld3(S.projLA[t]);
-            sa += ld3(S.projAA[t]);
+        if (S.mHdr[ref].x == i) {
+            sl += ld3(S.projLA[ref]);
+            sa += ld3(S.projAA[ref]);
         } else {
-            sl -= ld3(S.projLB[t]);
-            sa -= ld3(S.projAB[t]);
+            sl -= ld3(S.projLB[ref]);
+            sa -= ld3(S.projAB[ref]);
         }
     }
     const float inv = 1.0f / static_cast<float>(cnt > 1 ? cnt : 1);
@@ -1339,7 +1398,7 @@ __global__ void k_proj_flags(State S)
 // solver_step.cpp:164-170
 __global__ void k_rebase(State S)
 {
-    FOR_LIVE(k) manifoldRebase(S, S.live[S.gLive[k]]);
+    FOR_LIVE(k) manifoldRebase(S, k);
 }
 
 } // namespace avbd_dev

@@ -65,7 +65,21 @@ struct State {
     float4 *mBasis;
     float *mFric;
     int *mNum;
+    // Contacts, dense in live order (all worlds, world after world): contact i of the
+    // manifold in `slot` is at cOff[slot] + i.  Fields: rA | beta scale, rB | feature key,
+    // C0 | stick, penalty, lambda.
     float4 *cRA, *cRB, *cC0, *cPen, *cLam;
+    uint32_t *cOff;
+    // Staging with a fixed stride (slot * MAXC + i): the narrow phase writes the new
+    // contacts here and k_contact_pack lays them out densely once the live list is final.
+    float4 *sRA, *sRB, *sC0, *sPen, *sLam;
+    uint32_t *cCount, *cScan;   // per gLive entry: contact count, its exclusive scan
+    // Dense view of the live manifolds for one step, index k = gLive position (world after
+    // world, live order inside a world).  Adjacency entries, stamps, brake links and
+    // projection records use k; slots are only the persistent storage across steps.
+    int4 *mHdr;                 // {body a, body b, first contact, contact count}
+    float4 *mBasisK;            // 3 rows per k
+    float *mFricK;
     int *slotWorld;     // static: world owning slot-range position t
     int *gLive;         // all worlds' live positions t, world after world (compact)
     uint32_t *wLiveOff; // per world: offset of its live entries in gLive; [numWorlds] = total
