@@ -424,17 +424,22 @@ AD void addStampSide(Block &blk, const float4 *p)
     blk.rhsAng += v3{p6.x, p6.y, p6.z};
 }
 
-// Both sides' stamps of one manifold at the current poses.
+// Both sides' stamps of one manifold at the current poses (nothing for a frozen one).
 AD void manifoldStamps(const State &S, int k)
 {
     const int4 hd = S.mHdr[k];
     const int a = hd.x, b = hd.y, n = hd.w;
     const uint32_t cb = static_cast<uint32_t>(hd.z);
+    const float ma = bMass(S, a), mb = bMass(S, b);
+    const qt qa = bRot(S, a), qb = bRot(S, b);
+    const v3 pA = bPos(S, a), pB = bPos(S, b);
+    const v3 iLA = ld3(S.initLin[a]), iLB = ld3(S.initLin[b]);
+    const qt iAA = ldq(S.initAng[a]), iAB = ldq(S.initAng[b]);
     const m3 B = loadBasisK(S, k);
     const float mu = S.mFricK[k];
-    const qt qa = bRot(S, a), qb = bRot(S, b);
-    const v3 dqAL = bPos(S, a) - ld3(S.initLin[a]), dqAA = qsub(qa, ldq(S.initAng[a]));
-    const v3 dqBL = bPos(S, b) - ld3(S.initLin[b]), dqBA = qsub(qb, ldq(S.initAng[b]));
+    if (!(ma > 0.0f) && !(mb > 0.0f)) return;
+    const v3 dqAL = pA - iLA, dqAA = qsub(qa, iAA);
+    const v3 dqBL = pB - iLB, dqBA = qsub(qb, iAB);
     const m3 rotA = qmat(qa), rotB = qmat(qb);
     StampAcc acc;
     stampInit(acc);
@@ -460,34 +465,45 @@ AD void manifoldStamps(const State &S, int k)
 template <bool STAMP>
 AD void manifoldDual(const State &S, int k, float decay)
 {
+    // every load that does not depend on another is issued before the first wait; the
+    // frozen test (no movable body) comes after them
     const int4 hd = S.mHdr[k];
     const int a = hd.x, b = hd.y, n = hd.w;
     const uint32_t cb = static_cast<uint32_t>(hd.z);
+    const float ma = bMass(S, a), mb = bMass(S, b);
+    const qt qa = bRot(S, a), qb = bRot(S, b);
+    const v3 pA = bPos(S, a), pB = bPos(S, b);
+    const v3 iLA = ld3(S.initLin[a]), iLB = ld3(S.initLin[b]);
+    const qt iAA = ldq(S.initAng[a]), iAB = ldq(S.initAng[b]);
     const m3 B = loadBasisK(S, k);
     const float mu = S.mFricK[k];
-    const qt qa = bRot(S, a), qb = bRot(S, b);
-    const v3 dqAL = bPos(S, a) - ld3(S.initLin[a]), dqAA = qsub(qa, ldq(S.initAng[a]));
-    const v3 dqBL = bPos(S, b) - ld3(S.initLin[b]), dqBA = qsub(qb, ldq(S.initAng[b]));
+    // contact i + 1 is read before contact i is written (distinct rows)
+    float4 nPen{}, nRA{}, nRB{}, nC0{}, nLam{};
+    if (n > 0) { nPen = S.cPen[cb]; nRA = S.cRA[cb]; nRB = S.cRB[cb]; nC0 = S.cC0[cb]; nLam = S.cLam[cb]; }
+    if (!(ma > 0.0f) && !(mb > 0.0f)) return;
+    const v3 dqAL = pA - iLA, dqAA = qsub(qa, iAA);
+    const v3 dqBL = pB - iLB, dqBA = qsub(qb, iAB);
     const m3 rotA = qmat(qa), rotB = qmat(qb);
     const float pmax = S.P.penaltyMax;
     StampAcc acc;
     if (STAMP) stampInit(acc);
     for (int i = 0; i < n; i++) {
         const uint32_t c = cb + i;
-        v3 pen = ld3(S.cPen[c]);
+        const float4 pen4 = nPen, ra4 = nRA, rb4 = nRB, c0in = nC0, lam4 = nLam;
+        if (i + 1 < n) { nPen = S.cPen[c + 1]; nRA = S.cRA[c + 1]; nRB = S.cRB[c + 1]; nC0 = S.cC0[c + 1]; nLam = S.cLam[c + 1]; }
+        v3 pen = ld3(pen4);
         if (decay < 1.0f) pen = lclamp3(pen * decay, PENALTY_MIN, pmax);
-        const float4 ra4 = S.cRA[c];
         const v3 rA = mul(rotA, ld3(ra4));
-        const v3 rB = mul(rotB, ld3(S.cRB[c]));
+        const v3 rB = mul(rotB, ld3(rb4));
         const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
-        const v3 C = constraintError(S, ld3(S.cC0[c]), br, B);
-        v3 F = cmul(pen, C) + ld3(S.cLam[c]);
+        const v3 C = constraintError(S, ld3(c0in), br, B);
+        v3 F = cmul(pen, C) + ld3(lam4);
         float bounds;
         const float scale = clampForce(F, mu, bounds);
         S.cLam[c] = st3(F);
         const float beta = S.P.betaLin * ra4.w;
         if (F.x < 0) pen.x = smin(pen.x + beta * fabsf(C.x), pmax);
-        float4 c0 = S.cC0[c];
+        float4 c0 = c0in;
         bool writeStick = false;
         if (scale <= bounds) {
             pen.y = smin(pen.y + beta * fabsf(C.y), pmax);
@@ -513,16 +529,19 @@ AD void manifoldRebase(const State &S, int k)
     const int4 hd = S.mHdr[k];
     const int a = hd.x, b = hd.y, n = hd.w;
     const uint32_t cb = static_cast<uint32_t>(hd.z);
-    const m3 B = loadBasisK(S, k);
     const v3 pa = bPos(S, a), pb = bPos(S, b);
     const qt qa = bRot(S, a), qb = bRot(S, b);
+    const m3 B = loadBasisK(S, k);
+    float4 nRA{}, nRB{}, nC0{};
+    if (n > 0) { nRA = S.cRA[cb]; nRB = S.cRB[cb]; nC0 = S.cC0[cb]; }
     for (int i = 0; i < n; i++) {
         const uint32_t c = cb + i;
-        const v3 xA = pa + qrot(qa, ld3(S.cRA[c]));
-        const v3 xB = pb + qrot(qb, ld3(S.cRB[c]));
+        const float4 ra4 = nRA, rb4 = nRB, old = nC0;
+        if (i + 1 < n) { nRA = S.cRA[c + 1]; nRB = S.cRB[c + 1]; nC0 = S.cC0[c + 1]; }
+        const v3 xA = pa + qrot(qa, ld3(ra4));
+        const v3 xB = pb + qrot(qb, ld3(rb4));
         v3 c0 = mul(B, xA - xB) + v3{COLLISION_MARGIN, 0, 0};
         if (c0.x < -S.P.maxPen) c0.x = -S.P.maxPen;
-        const float4 old = S.cC0[c];
         S.cC0[c] = st3(c0, old.w);
     }
 }

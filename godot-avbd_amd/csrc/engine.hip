@@ -283,7 +283,7 @@ struct Engine {
         AL(S.pairValid, P + 1); AL(S.pairValidScan, P + 1);
         AL(S.pairScratch, N * PAIR_BUF); AL(S.wRoot, W);
         AL(S.wBoundsLo, W); AL(S.wBoundsHi, W);
-        AL(S.mHdr, M); AL(S.mBasisK, 3 * M); AL(S.mFricK, M);
+        AL(S.mHdr, M); AL(S.mBasisK, 3 * M); AL(S.mFricK, M); AL(S.brakeOn, M); AL(S.wOrder, W);
         radixBlocks = (Ncap + RADIX_TILE - 1) / RADIX_TILE;
         AL(radixHist, static_cast<size_t>(256) * radixBlocks + 1);
         const int biggest = std::max({Ncap + 1, Pcap + 1, Scap + 1, 256 * radixBlocks + 1, 2 * (Scap + Jcap)});
@@ -300,7 +300,7 @@ struct Engine {
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream));
         setParams(S.P, prm, prm.dt);
-        return AVBD_OK;
+        return refreshWorldOrder();
     }
 
     // ---- primitives -------------------------------------------------------------
@@ -370,6 +370,31 @@ struct Engine {
         for (const auto &w : worlds)
             if (w.active && w.hasJoints) return true;
         return false;
+    }
+
+    // One velocity brake at the current poses: flags, dependency links, level-by-level
+    // brake per world (solver_projection.cpp:96-167).
+    void velocityBrake(float window, int countActive)
+    {
+        k_brake_flags<<<liveGrid(), TPB, 0, stream>>>(S, countActive);
+        k_brake_links<<<liveGrid(), TPB, 0, stream>>>(S);
+        k_brake_levels<<<Wcap, BRAKE_TPB, 0, stream>>>(S, window, 1);
+    }
+
+    // Per-world kernels take worlds in this order, most bodies first, so the longest
+    // workgroups start first.
+    avbd_status refreshWorldOrder()
+    {
+        std::vector<int> order(Wcap);
+        for (int w = 0; w < Wcap; w++) order[w] = w;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            const int na = worlds[a].active ? worlds[a].bodyCount : -1;
+            const int nb = worlds[b].active ? worlds[b].bodyCount : -1;
+            return na > nb;
+        });
+        HIPCHK(hipMemcpyAsync(S.wOrder, order.data(), sizeof(int) * Wcap, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return AVBD_OK;
     }
 
     // ---- the step -------------------------------------------------------------------
@@ -443,10 +468,14 @@ struct Engine {
         k_colour_override<<<nb, TPB, 0, stream>>>(S);
         // ---- brake schedule and first brake (solver_projection.cpp:96-167)
         k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wUpdateCount, 0, Wcap);
-        k_brake_links<<<liveGrid(), TPB, 0, stream>>>(S);
         const bool brake = prm.speculative_margin > 0.0f;
         pbeg(AVBD_K_BRAKE_LEVELS);
-        k_brake_levels<<<Wcap, BRAKE_TPB, 0, stream>>>(S, collisionDt, brake ? 1 : 0);
+        if (brake) {
+            velocityBrake(collisionDt, 1);
+        } else {
+            k_brake_flags<<<liveGrid(), TPB, 0, stream>>>(S, 1);
+            k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.brakeLevels, 0, Wcap);
+        }
         pend();
         if (timed) (void)hipEventRecord(ev[2], stream);
         // ---- substeps (solver_step.cpp:79-105, 142-150)
@@ -461,7 +490,7 @@ struct Engine {
                 k_rebase<<<liveGrid(), TPB, 0, stream>>>(S);
                 if (brake) {
                     pbeg(AVBD_K_BRAKE_REPLAY);
-                    k_brake_replay<<<Wcap, BRAKE_TPB, 0, stream>>>(S, prm.dt * static_cast<float>(subs - sub));
+                    velocityBrake(prm.dt * static_cast<float>(subs - sub), 0);
                     pend();
                 }
             }
@@ -510,7 +539,7 @@ struct Engine {
         // ---- final brake, position projection, rebase (solver_step.cpp:69-70)
         if (brake) {
             pbeg(AVBD_K_BRAKE_REPLAY);
-            k_brake_replay<<<Wcap, BRAKE_TPB, 0, stream>>>(S, collisionDt);
+            velocityBrake(collisionDt, 0);
             pend();
         }
         if (prm.position_projection && prm.projection_iterations > 0) {
@@ -736,7 +765,7 @@ struct Engine {
         wi.active = true;
         csrValid = false;
         treeAge = 1 << 20;
-        return AVBD_OK;
+        return refreshWorldOrder();
     }
 
     avbd_status writeBodies(int w, const avbd_bodies *b)
@@ -783,7 +812,7 @@ struct Engine {
         wi = WorldInfo{};
         csrValid = false;
         treeAge = 1 << 20;
-        return AVBD_OK;
+        return refreshWorldOrder();
     }
 
     avbd_status downloadWorld(int w, avbd_bodies *b, avbd_joints *j)

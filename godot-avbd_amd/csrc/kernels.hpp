@@ -309,7 +309,7 @@ __global__ void k_karras(State S, int n)
 constexpr int REFIT_TPB = 1024;
 __global__ void __launch_bounds__(REFIT_TPB) k_refit_world(State S, int n)
 {
-    const int w = blockIdx.x;
+    const int w = S.wOrder[blockIdx.x];
     const int lo = S.wBodyBase[w], cnt = S.wBodyCount[w];
     if (cnt <= 0) return;
     const int root = S.wRoot[w];
@@ -989,6 +989,39 @@ __device__ __forceinline__ bool manifoldActive(const State &S, int k)
     return h.w > 0 && (bodyMoves(S, h.x) || bodyMoves(S, h.y));
 }
 
+// Separation of one contact along the normal at the current poses
+// (solver_projection.cpp:120-123).
+__device__ __forceinline__ float contactGap(const State &S, v3 n, v3 pa, qt qa, v3 pb, qt qb, uint32_t c, v3 &rA, v3 &rB)
+{
+    rA = qrot(qa, ld3(S.cRA[c]));
+    rB = qrot(qb, ld3(S.cRB[c]));
+    return dot(n, (pa + rA) - (pb + rB));
+}
+
+// A manifold can change a velocity in the brake only if it is active and one of its
+// contacts has a positive gap; gaps depend on poses alone, which the brake leaves alone,
+// so the others are left out of the dependency links altogether.  `count` also counts the
+// world's active manifolds (used by the projection).
+__global__ void k_brake_flags(State S, int count)
+{
+    FOR_LIVE(k) {
+        const int4 h = S.mHdr[k];
+        const bool active = h.w > 0 && (bodyMoves(S, h.x) || bodyMoves(S, h.y));
+        bool on = false;
+        if (active) {
+            if (count) atomicAdd(&S.wUpdateCount[S.bodyWorld[h.x]], 1);
+            const v3 n = ld3(S.mBasisK[3 * k]);
+            const v3 pa = bPos(S, h.x), pb = bPos(S, h.y);
+            const qt qa = bRot(S, h.x), qb = bRot(S, h.y);
+            for (int i = 0; i < h.w && !on; i++) {
+                v3 rA, rB;
+                on = contactGap(S, n, pa, qa, pb, qb, static_cast<uint32_t>(h.z) + i, rA, rB) > 0.0f;
+            }
+        }
+        S.brakeOn[k] = on ? 1 : 0;
+    }
+}
+
 __device__ void brakeLinks(const State &S, int k);
 __global__ void k_brake_links(State S)
 {
@@ -998,8 +1031,7 @@ __device__ void brakeLinks(const State &S, int k)
 {
     S.succA[k] = -1;
     S.succB[k] = -1;
-    S.indeg[k] = 0;
-    if (!manifoldActive(S, k)) { S.indeg[k] = -1; return; }
+    if (!S.brakeOn[k]) { S.indeg[k] = -1; return; }
     const int4 h = S.mHdr[k];
     int deg = 0;
     for (int side = 0; side < 2; side++) {
@@ -1012,11 +1044,11 @@ __device__ void brakeLinks(const State &S, int k)
         for (int e = me - 1; e >= e0; e--) {
             const int r = S.adj[e];
             if (r < 0) break;
-            if (manifoldActive(S, r)) { deg++; break; }
+            if (S.brakeOn[r]) { deg++; break; }
         }
         for (int e = me + 1; e < e1; e++) {
             const int r = S.adj[e];
-            if (r >= 0 && manifoldActive(S, r)) {
+            if (r >= 0 && S.brakeOn[r]) {
                 if (side == 0) S.succA[k] = r;
                 else S.succB[k] = r;
                 break;
@@ -1024,7 +1056,6 @@ __device__ void brakeLinks(const State &S, int k)
         }
     }
     S.indeg[k] = deg;
-    atomicAdd(&S.wUpdateCount[S.bodyWorld[h.x]], 1);  // active manifolds of the world
 }
 
 __device__ void brakeManifold(const State &S, int k, float window)
@@ -1043,10 +1074,8 @@ __device__ void brakeManifold(const State &S, int k, float window)
     int braked = 0;
     const int nc = hd.w;
     for (int i = 0; i < nc; i++) {
-        const uint32_t c = static_cast<uint32_t>(hd.z) + i;
-        const v3 rA = qrot(qa, ld3(S.cRA[c]));
-        const v3 rB = qrot(qb, ld3(S.cRB[c]));
-        const float gap = dot(n, (pa + rA) - (pb + rB));
+        v3 rA, rB;
+        const float gap = contactGap(S, n, pa, qa, pb, qb, static_cast<uint32_t>(hd.z) + i, rA, rB);
         if (!(gap > 0.0f)) continue;
         const v3 vAp = va + cross(wa, rA), vBp = vb + cross(wb, rB);
         const float vn = dot(vAp - vBp, n);
@@ -1084,10 +1113,10 @@ __device__ void brakeManifold(const State &S, int k, float window)
 // One workgroup per world: Kahn's algorithm by levels over the dependency links,
 // braking each level's manifolds in parallel; the level order is kept for the replay.
 // levelStart[base + l] is the first entry of level l; wLevelEnd[w] ends the last level.
-constexpr int BRAKE_TPB = 1024;
+constexpr int BRAKE_TPB = 256;
 __global__ void __launch_bounds__(BRAKE_TPB) k_brake_levels(State S, float window, int doBrake)
 {
-    const int w = blockIdx.x;
+    const int w = S.wOrder[blockIdx.x];
     const int base = static_cast<int>(S.wLiveOff[w]);
     const int nLive = S.wLive[w];
     __shared__ int tail, head, levels;
@@ -1122,20 +1151,6 @@ __global__ void __launch_bounds__(BRAKE_TPB) k_brake_levels(State S, float windo
     if (threadIdx.x == 0) {
         S.brakeLevels[w] = levels;
         S.wLevelEnd[w] = tail;
-    }
-}
-
-__global__ void __launch_bounds__(BRAKE_TPB) k_brake_replay(State S, float window)
-{
-    const int w = blockIdx.x;
-    const int base = static_cast<int>(S.wLiveOff[w]);
-    const int L = S.brakeLevels[w];
-    for (int l = 0; l < L; l++) {
-        const int h = S.levelStart[base + l];
-        const int e = l + 1 < L ? S.levelStart[base + l + 1] : S.wLevelEnd[w];
-        for (int k = h + threadIdx.x; k < e; k += BRAKE_TPB) brakeManifold(S, S.levelOrder[base + k], window);
-        __threadfence_block();
-        __syncthreads();
     }
 }
 
@@ -1261,10 +1276,7 @@ __global__ void k_overflow_apply(State S)
 // Stamps of every live manifold that touches a movable body, at the current poses.
 __global__ void __launch_bounds__(TPB) k_stamp(State S)
 {
-    FOR_LIVE(k) {
-        const int4 h = S.mHdr[k];
-        if (bMass(S, h.x) > 0.0f || bMass(S, h.y) > 0.0f) manifoldStamps(S, k);
-    }
+    FOR_LIVE(k) manifoldStamps(S, k);
 }
 
 // solver_step.cpp:277-296: manifolds (not frozen), then joints.  STAMP: also build the next
@@ -1272,10 +1284,7 @@ __global__ void __launch_bounds__(TPB) k_stamp(State S)
 template <bool STAMP>
 __global__ void __launch_bounds__(TPB) k_dual(State S, float decay)
 {
-    FOR_LIVE(k) {
-        const int4 h = S.mHdr[k];
-        if (bMass(S, h.x) > 0.0f || bMass(S, h.y) > 0.0f) manifoldDual<STAMP>(S, k, decay);
-    }
+    FOR_LIVE(k) manifoldDual<STAMP>(S, k, decay);
     for (int t = GID; t < S.jointCap; t += gridDim.x * blockDim.x)
         if (S.jWorld[t] >= 0 && !S.jDead[t]) jointDual(S, t);
 }
@@ -1301,14 +1310,16 @@ __device__ void projManifold(const State &S, int k)
     const int4 hd = S.mHdr[k];
     const int a = hd.x, b = hd.y;
     const int w = S.bodyWorld[a];
-    if (S.wProjDone[w]) return;
-    S.projOn[k] = 0;
-    if (!(hd.w > 0 && (bodyMoves(S, a) || bodyMoves(S, b)))) return;
-    const bool moveA = bodyMoves(S, a), moveB = bodyMoves(S, b);
-    const float imA = moveA ? 1.0f / bMass(S, a) : 0.0f, imB = moveB ? 1.0f / bMass(S, b) : 0.0f;
+    const float ma = bMass(S, a), mb = bMass(S, b);
+    const int ua = S.updateIndex[a], ub = S.updateIndex[b];
     const v3 n = ld3(S.mBasisK[3 * k]);
     const v3 pa = bPos(S, a), pb = bPos(S, b);
     const qt qa = bRot(S, a), qb = bRot(S, b);
+    if (S.wProjDone[w]) return;
+    S.projOn[k] = 0;
+    const bool moveA = ma > 0.0f && ua >= 0, moveB = mb > 0.0f && ub >= 0;
+    if (!(hd.w > 0 && (moveA || moveB))) return;
+    const float imA = moveA ? 1.0f / ma : 0.0f, imB = moveB ? 1.0f / mb : 0.0f;
     const m3 rotA = qmat(qa), rotB = qmat(qb);
     v3 pRA{0, 0, 0}, pRB{0, 0, 0};
     float depthSum = 0.0f;

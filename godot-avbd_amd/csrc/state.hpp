@@ -80,6 +80,8 @@ struct State {
     int4 *mHdr;                 // {body a, body b, first contact, contact count}
     float4 *mBasisK;            // 3 rows per k
     float *mFricK;
+    uint8_t *brakeOn;           // per k: active and some contact has a positive gap now
+    int *wOrder;                // world ids, most bodies first (per-world launches)
     int *slotWorld;     // static: world owning slot-range position t
     int *gLive;         // all worlds' live positions t, world after world (compact)
     uint32_t *wLiveOff; // per world: offset of its live entries in gLive; [numWorlds] = total
