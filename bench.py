@@ -60,7 +60,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=150)
+    ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--min-bodies", type=int, default=8)
     ap.add_argument("--max-bodies", type=int, default=2048)
@@ -163,13 +163,16 @@ def main():
     log(f"plain: sample {len(pick)} worlds, model {coef[0] * 1e3:.3f} ms + {coef[1] * 1e6:.3f} us/body; "
         f"{plain_value / 1e6:.3f} M body-steps/s")
 
-    # ---- CPU baseline: the oracle (serial restatement of the reference step) on this host
+    # ---- CPU baseline: the oracle (serial restatement of the reference step) on this host,
+    # one core, then the sampled worlds statically partitioned over all cores (one world
+    # per thread at a time; ctypes releases the GIL inside the oracle's step)
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        done_bodies, spent, rounds = 0, 0.0, 0
+        from concurrent.futures import ThreadPoolExecutor
         order = sorted(snaps, key=lambda w: -sizes[w])
+        movable_of = {w: int((snaps[w][0]["mass"] > 0).sum()) for w in order}
         oworlds = {}
         for w in order:
             start, m = snaps[w]
@@ -177,20 +180,40 @@ def main():
             o.upload(start, worlds[w][1])
             o.set_manifolds(m)
             oworlds[w] = o
-        # step the sampled worlds round-robin, one step each, until the time budget is spent
-        while spent < args.cpu_seconds and rounds < 200:
-            for w in order:
-                a = time.perf_counter()
-                oworlds[w].step()
-                spent += time.perf_counter() - a
-                done_bodies += int((snaps[w][0]["mass"] > 0).sum())
-            rounds += 1
-        used = order
-        cpu = {"value": done_bodies / spent, "unit": "body-steps/s", "cores": 1, "kind": "port",
-               "sample": f"the {len(used)} sampled worlds (states copied from the GPU after warmup), "
-                         f"{rounds} steps each: {done_bodies} body-steps in {spent:.1f}s, "
-                         f"{iters} iterations, serial oracle on 1 host core"}
-        log(f"cpu baseline: {cpu['value'] / 1e3:.1f} k body-steps/s")
+
+        def run(ws, seconds):
+            done, steps = 0, 0
+            end_t = time.perf_counter() + seconds
+            while time.perf_counter() < end_t and steps < 200 * len(ws):
+                for w in ws:
+                    oworlds[w].step()
+                    done += movable_of[w]
+                    steps += 1
+            return done
+
+        one_s = args.cpu_seconds / 3.0
+        a = time.perf_counter()
+        one_done = run(order, one_s)
+        one_value = one_done / (time.perf_counter() - a)
+        try:
+            threads = len(os.sched_getaffinity(0))
+        except AttributeError:
+            threads = os.cpu_count() or 1
+        threads = max(1, min(threads, 16, len(order)))
+        parts = [order[t::threads] for t in range(threads)]
+        a = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            multi_done = sum(ex.map(lambda ws: run(ws, args.cpu_seconds - one_s), parts))
+        multi_value = multi_done / (time.perf_counter() - a)
+        cpu = {"value": multi_value, "unit": "body-steps/s", "cores": threads, "kind": "port",
+               "single_core_value": one_value,
+               "sample": f"the {len(order)} stratified sample worlds (states copied from the GPU after warmup, "
+                         f"{sum(movable_of.values())} movable bodies), stepped by the oracle (serial restatement "
+                         f"of the reference step, {iters} iterations): {threads} threads with the worlds "
+                         f"statically partitioned for {args.cpu_seconds - one_s:.0f}s ({multi_done} body-steps), "
+                         f"and 1 thread for {one_s:.0f}s ({one_done} body-steps)"}
+        log(f"cpu baseline: {multi_value / 1e3:.1f} k body-steps/s on {threads} threads, "
+            f"{one_value / 1e3:.1f} k on 1")
 
     line = {
         "metric": "body-steps/sec @10 iters (batched vs plain, same GPU); p50/p99 step latency",

@@ -329,7 +329,12 @@ struct Engine {
 
     // Grid for kernels that walk the compact live list: its length is only known on the
     // device, so they grid-stride over a fixed grid sized to keep every CU busy.
-    int liveGrid() const { return std::max(1, std::min(nblk(std::max(Scap, Jcap)), LIVE_BLOCKS)); }
+    // (a multiple of NUM_XCD: FOR_LIVE splits the list into one part per XCD)
+    int liveGrid() const
+    {
+        const int g = std::min(nblk(std::max(Scap, Jcap)), LIVE_BLOCKS);
+        return std::max(NUM_XCD, (g + NUM_XCD - 1) / NUM_XCD * NUM_XCD);
+    }
 
     // gLive = every world's live positions, world after world (wLiveOff = exclusive scan of wLive).
     void buildLiveList()

@@ -16,6 +16,20 @@ constexpr int LIVE_BLOCKS = 2048;
 
 #define GID (static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x))
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8), and each
+// XCD has its own L2.  xcdBlock() renumbers workgroups so that every XCD gets one contiguous
+// run of logical blocks: consecutive bodies / manifolds, which share bodies, then meet in one
+// L2.  It is a bijection on [0, gridDim.x) for any grid size.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcdBlock()
+{
+    const int nb = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
+    const int x = b % NUM_XCD, j = b / NUM_XCD;
+    const int q = nb / NUM_XCD, r = nb % NUM_XCD;
+    return x * q + (x < r ? x : r) + j;
+}
+#define XGID (xcdBlock() * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x))
+
 // ------------------------------------------------------------------ utilities ----
 __global__ void k_fill_i32(int *p, int v, int n)
 {
@@ -435,7 +449,7 @@ __device__ __forceinline__ bool traverseWorld(const State &S, int n, int i, int 
 __global__ void __launch_bounds__(TPB) k_traverse(State S, int n)
 {
     // queries in Morton order: neighbouring lanes walk neighbouring paths
-    const int p = GID;
+    const int p = XGID;
     if (p >= S.numBodies) return;
     const int i = S.sortVal[p];
     const int w = S.bodyWorld[i];
@@ -498,8 +512,15 @@ __device__ __forceinline__ int otherOf(const State &S, int ref, int body)
 }
 
 // Grid-stride walk over every world's live manifolds (the compact list gLive).
-#define FOR_LIVE(k) for (int k = GID, k##_n = static_cast<int>(S.wLiveOff[S.numWorlds]); k < k##_n; \
-                         k += static_cast<int>(gridDim.x * blockDim.x))
+// Each XCD takes one contiguous eighth of the list (its workgroups grid-stride inside it).
+#define FOR_LIVE(k)                                                                                         \
+    for (int k##_t = static_cast<int>(S.wLiveOff[S.numWorlds]), k##_x = static_cast<int>(blockIdx.x) % NUM_XCD, \
+             k##_nb = static_cast<int>(gridDim.x) / NUM_XCD,                                                \
+             k##_hi = static_cast<int>((static_cast<long long>(k##_t) * (k##_x + 1)) / NUM_XCD),            \
+             k = static_cast<int>((static_cast<long long>(k##_t) * k##_x) / NUM_XCD) +                      \
+                 static_cast<int>(blockIdx.x) / NUM_XCD * static_cast<int>(blockDim.x) +                    \
+                 static_cast<int>(threadIdx.x);                                                             \
+         k < k##_hi; k += k##_nb * static_cast<int>(blockDim.x))
 
 __global__ void k_glive_fill(State S)
 {
@@ -1158,7 +1179,7 @@ __global__ void __launch_bounds__(BRAKE_TPB) k_brake_levels(State S, float windo
 // solver_solve.cpp:109-142
 __global__ void k_warmstart(State S)
 {
-    const int i = GID;
+    const int i = XGID;
     if (i >= S.numBodies || S.bodyWorld[i] < 0) return;
     const float dt = S.P.dt;
     v3 vel = ld3(S.vel[i]), ang = ld3(S.angVel[i]);
@@ -1190,7 +1211,7 @@ __global__ void k_warmstart(State S)
 // solver_solve.cpp:146-176 (sleep off)
 __global__ void k_finish(State S)
 {
-    const int i = GID;
+    const int i = XGID;
     if (i >= S.numBodies || S.bodyWorld[i] < 0) return;
     const float dt = S.P.dt;
     v3 vel = ld3(S.vel[i]), ang = ld3(S.angVel[i]);
@@ -1215,7 +1236,7 @@ __device__ __forceinline__ void queueFallback(const State &S, int body, const Bl
 // solver_step.cpp:209-229: one colour of the Gauss-Seidel sweep.
 __global__ void __launch_bounds__(TPB) k_colour_pass(State S, int colour)
 {
-    const int i = GID;
+    const int i = XGID;
     if (i >= S.numBodies || S.colour[i] != colour || S.bodyWorld[i] < 0) return;
     if (S.updateIndex[i] < 0) return;
     Block blk;
@@ -1229,7 +1250,7 @@ __global__ void __launch_bounds__(TPB) k_colour_pass(State S, int colour)
 // solver_step.cpp:248-264: overflow bodies solve against one pose snapshot.
 __global__ void __launch_bounds__(TPB) k_overflow_solve(State S)
 {
-    const int i = GID;
+    const int i = XGID;
     if (i >= S.numBodies || S.colour[i] != 8 || S.bodyWorld[i] < 0) return;
     if (S.updateIndex[i] < 0) return;
     Block blk;
@@ -1265,7 +1286,7 @@ __global__ void k_solve_fallback(State S, int apply)
 // solver_step.cpp:268-273
 __global__ void k_overflow_apply(State S)
 {
-    const int i = GID;
+    const int i = XGID;
     if (i >= S.numBodies || S.colour[i] != 8 || S.bodyWorld[i] < 0) return;
     if (S.updateIndex[i] < 0) return;
     const float r = S.P.jacobiRelax;
@@ -1361,7 +1382,7 @@ __device__ void projManifold(const State &S, int k)
 // live order, and moves; every body of a world with active manifolds is renormalised.
 __global__ void k_proj_body(State S)
 {
-    const int i = GID;
+    const int i = XGID;
     if (i >= S.numBodies) return;
     const int w = S.bodyWorld[i];
     if (w < 0 || S.updateIndex[i] < 0) return;
