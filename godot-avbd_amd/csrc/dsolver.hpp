@@ -27,6 +27,15 @@ AD qt bRot(const State &S, int b) { return ldq(S.rot[b]); }
 AD float bMass(const State &S, int b) { return S.massI[b].x; }
 AD v3 bMoment(const State &S, int b) { const float4 m = S.massI[b]; return {m.y, m.z, m.w}; }
 
+// A body's packed pose record (see State::poseRec): the step-relative displacement dq of the
+// primal/dual bracket, the rotation and the mass.
+struct PoseRec { v3 dqL, dqA; qt q; float mass; };
+AD PoseRec loadPose(const State &S, int b)
+{
+    const float4 r0 = S.poseRec[4 * b], r1 = S.poseRec[4 * b + 1], r2 = S.poseRec[4 * b + 2];
+    return {ld3(r0), ld3(r2), ldq(r1), r0.w};
+}
+
 // types.hpp:157-172: M/dt^2 block with world inertia R diag(I) R^T / dt^2.
 AD void makeBlock(const State &S, int b, Block &k)
 {
@@ -430,17 +439,12 @@ AD void manifoldStamps(const State &S, int k)
     const int4 hd = S.mHdr[k];
     const int a = hd.x, b = hd.y, n = hd.w;
     const uint32_t cb = static_cast<uint32_t>(hd.z);
-    const float ma = bMass(S, a), mb = bMass(S, b);
-    const qt qa = bRot(S, a), qb = bRot(S, b);
-    const v3 pA = bPos(S, a), pB = bPos(S, b);
-    const v3 iLA = ld3(S.initLin[a]), iLB = ld3(S.initLin[b]);
-    const qt iAA = ldq(S.initAng[a]), iAB = ldq(S.initAng[b]);
+    const PoseRec A = loadPose(S, a), Bd = loadPose(S, b);
     const m3 B = loadBasisK(S, k);
     const float mu = S.mFricK[k];
-    if (!(ma > 0.0f) && !(mb > 0.0f)) return;
-    const v3 dqAL = pA - iLA, dqAA = qsub(qa, iAA);
-    const v3 dqBL = pB - iLB, dqBA = qsub(qb, iAB);
-    const m3 rotA = qmat(qa), rotB = qmat(qb);
+    if (!(A.mass > 0.0f) && !(Bd.mass > 0.0f)) return;
+    const v3 dqAL = A.dqL, dqAA = A.dqA, dqBL = Bd.dqL, dqBA = Bd.dqA;
+    const m3 rotA = qmat(A.q), rotB = qmat(Bd.q);
     StampAcc acc;
     stampInit(acc);
     for (int i = 0; i < n; i++) {
@@ -470,20 +474,15 @@ AD void manifoldDual(const State &S, int k, float decay)
     const int4 hd = S.mHdr[k];
     const int a = hd.x, b = hd.y, n = hd.w;
     const uint32_t cb = static_cast<uint32_t>(hd.z);
-    const float ma = bMass(S, a), mb = bMass(S, b);
-    const qt qa = bRot(S, a), qb = bRot(S, b);
-    const v3 pA = bPos(S, a), pB = bPos(S, b);
-    const v3 iLA = ld3(S.initLin[a]), iLB = ld3(S.initLin[b]);
-    const qt iAA = ldq(S.initAng[a]), iAB = ldq(S.initAng[b]);
+    const PoseRec A = loadPose(S, a), Bd = loadPose(S, b);
     const m3 B = loadBasisK(S, k);
     const float mu = S.mFricK[k];
     // contact i + 1 is read before contact i is written (distinct rows)
     float4 nPen{}, nRA{}, nRB{}, nC0{}, nLam{};
     if (n > 0) { nPen = S.cPen[cb]; nRA = S.cRA[cb]; nRB = S.cRB[cb]; nC0 = S.cC0[cb]; nLam = S.cLam[cb]; }
-    if (!(ma > 0.0f) && !(mb > 0.0f)) return;
-    const v3 dqAL = pA - iLA, dqAA = qsub(qa, iAA);
-    const v3 dqBL = pB - iLB, dqBA = qsub(qb, iAB);
-    const m3 rotA = qmat(qa), rotB = qmat(qb);
+    if (!(A.mass > 0.0f) && !(Bd.mass > 0.0f)) return;
+    const v3 dqAL = A.dqL, dqAA = A.dqA, dqBL = Bd.dqL, dqBA = Bd.dqA;
+    const m3 rotA = qmat(A.q), rotB = qmat(Bd.q);
     const float pmax = S.P.penaltyMax;
     StampAcc acc;
     if (STAMP) stampInit(acc);

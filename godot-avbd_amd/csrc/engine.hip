@@ -254,7 +254,7 @@ struct Engine {
         const size_t N = Ncap, J = Jcap, M = Scap, C = static_cast<size_t>(Scap) * MAXC, W = Wcap, P = Pcap;
         S.numWorlds = Wcap; S.numBodies = Ncap; S.slotCap = Scap; S.jointCap = Jcap; S.pairCap = Pcap;
         AL(S.pos, N); AL(S.rot, N); AL(S.initLin, N); AL(S.initAng, N); AL(S.inertLin, N); AL(S.inertAng, N);
-        AL(S.vel, N); AL(S.angVel, N); AL(S.prevVel, N); AL(S.sizeR, N); AL(S.massI, N);
+        AL(S.vel, N); AL(S.angVel, N); AL(S.prevVel, N); AL(S.sizeR, N); AL(S.massI, N); AL(S.poseRec, 4 * N);
         AL(S.fric, N); AL(S.grav, N); AL(S.layer, N); AL(S.mask, N); AL(S.lockLin, N); AL(S.lockAng, N);
         AL(S.bodyWorld, N); AL(S.updateIndex, N); AL(S.colour, N); AL(S.contactOnly, N);
         AL(S.dLin, N); AL(S.dAng, N); AL(S.aabbC, N); AL(S.aabbH, N);
@@ -283,7 +283,7 @@ struct Engine {
         AL(S.pairValid, P + 1); AL(S.pairValidScan, P + 1);
         AL(S.pairScratch, N * PAIR_BUF); AL(S.wRoot, W);
         AL(S.wBoundsLo, W); AL(S.wBoundsHi, W);
-        AL(S.mHdr, M); AL(S.mBasisK, 3 * M); AL(S.mFricK, M); AL(S.brakeOn, M); AL(S.wOrder, W);
+        AL(S.gOfK, M); AL(S.mHdr, M); AL(S.mBasisK, 3 * M); AL(S.mFricK, M); AL(S.brakeOn, M); AL(S.wOrder, W);
         radixBlocks = (Ncap + RADIX_TILE - 1) / RADIX_TILE;
         AL(radixHist, static_cast<size_t>(256) * radixBlocks + 1);
         const int biggest = std::max({Ncap + 1, Pcap + 1, Scap + 1, 256 * radixBlocks + 1, 2 * (Scap + Jcap)});
@@ -341,6 +341,7 @@ struct Engine {
     {
         scan(reinterpret_cast<const uint32_t *>(S.wLive), S.wLiveOff, Wcap);
         k_glive_fill<<<Wcap, TPB, 0, stream>>>(S);
+        k_manifold_order<<<Wcap, ORDER_TPB, 0, stream>>>(S);
     }
 
     // Dense contact layout for the current live list (staging -> cRA...).
@@ -517,6 +518,7 @@ struct Engine {
                             k_solve_fallback<<<1, TPB, 0, stream>>>(S, 1);
                         }
                     if (stampHere) {
+                        k_pose_pack<<<nb, TPB, 0, stream>>>(S);
                         pbeg(AVBD_K_STAMP);
                         k_stamp<<<liveGrid(), TPB, 0, stream>>>(S);
                         pend();
@@ -529,6 +531,7 @@ struct Engine {
                     k_overflow_apply<<<nb, TPB, 0, stream>>>(S);
                     pend();
                 }
+                k_pose_pack<<<nb, TPB, 0, stream>>>(S);
                 pbeg(AVBD_K_DUAL);
                 if (stampInDual)
                     k_dual<true><<<liveGrid(), TPB, 0, stream>>>(S, newton ? 1.0f : prm.stiffness_decay);

@@ -522,6 +522,56 @@ __device__ __forceinline__ int otherOf(const State &S, int ref, int body)
                  static_cast<int>(threadIdx.x);                                                             \
          k < k##_hi; k += k##_nb * static_cast<int>(blockDim.x))
 
+// One workgroup per world: the view order, a stable counting sort of the world's live
+// manifolds by contact count (ballot ranks inside a wavefront, LDS counts across them).
+constexpr int ORDER_TPB = 256;
+__global__ void __launch_bounds__(ORDER_TPB) k_manifold_order(State S)
+{
+    const int w = S.wOrder[blockIdx.x];
+    const int base = static_cast<int>(S.wLiveOff[w]), n = S.wLive[w];
+    if (n <= 0) return;
+    constexpr int NB = 16, NW = ORDER_TPB / 64;
+    __shared__ uint32_t start[NB], wcnt[NW][NB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (tid < NB) start[tid] = 0;
+    if (tid < NW * NB) wcnt[tid / NB][tid % NB] = 0;
+    __syncthreads();
+    for (int i = tid; i < n; i += ORDER_TPB) atomicAdd(&start[S.mNum[S.live[S.gLive[base + i]]]], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (int v = 0; v < NB; v++) { const uint32_t c = start[v]; start[v] = run; run += c; }
+    }
+    __syncthreads();
+    for (int c0 = 0; c0 < n; c0 += ORDER_TPB) {
+        const int i = c0 + tid;
+        const bool valid = i < n;
+        const uint32_t v = valid ? static_cast<uint32_t>(S.mNum[S.live[S.gLive[base + i]]]) : 0u;
+        unsigned long long m = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 4; bit++) {
+            const unsigned long long b = __ballot((v >> bit) & 1u);
+            m &= ((v >> bit) & 1u) ? b : ~b;
+        }
+        const uint32_t rank = __popcll(m & below);
+        if (valid && rank == 0) wcnt[wave][v] = __popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = start[v];
+            for (int q = 0; q < wave; q++) off += wcnt[q][v];
+            S.gOfK[base + static_cast<int>(off + rank)] = base + i;
+        }
+        __syncthreads();
+        if (tid < NB) {
+            uint32_t add = 0;
+            for (int q = 0; q < NW; q++) { add += wcnt[q][tid]; wcnt[q][tid] = 0; }
+            start[tid] += add;
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void k_glive_fill(State S)
 {
     const int w = blockIdx.x;
@@ -608,19 +658,27 @@ __global__ void k_alloc_counts(State S)
 }
 
 // ------------------------------------------------------- contact storage ----
+// Copy n contact rows of the five contact fields (source and destination never overlap, so
+// every load of a row is issued before its stores).
+struct ContactRows { float4 *ra, *rb, *c0, *pen, *lam; };
+__device__ __forceinline__ void copyContacts(ContactRows dst, size_t d, ContactRows src, size_t o, int n)
+{
+    for (int i = 0; i < n; i++) {
+        const float4 a = src.ra[o + i], b = src.rb[o + i], c = src.c0[o + i], p = src.pen[o + i], l = src.lam[o + i];
+        dst.ra[d + i] = a;
+        dst.rb[d + i] = b;
+        dst.c0[d + i] = c;
+        dst.pen[d + i] = p;
+        dst.lam[d + i] = l;
+    }
+}
+__device__ __forceinline__ ContactRows denseRows(const State &S) { return {S.cRA, S.cRB, S.cC0, S.cPen, S.cLam}; }
+__device__ __forceinline__ ContactRows stageRows(const State &S) { return {S.sRA, S.sRB, S.sC0, S.sPen, S.sLam}; }
+
 // Copy a live manifold's dense contacts to its staging rows.
 __device__ __forceinline__ void stageContacts(const State &S, int slot)
 {
-    const int n = S.mNum[slot];
-    const uint32_t o = S.cOff[slot];
-    const size_t d = static_cast<size_t>(slot) * MAXC;
-    for (int i = 0; i < n; i++) {
-        S.sRA[d + i] = S.cRA[o + i];
-        S.sRB[d + i] = S.cRB[o + i];
-        S.sC0[d + i] = S.cC0[o + i];
-        S.sPen[d + i] = S.cPen[o + i];
-        S.sLam[d + i] = S.cLam[o + i];
-    }
+    copyContacts(stageRows(S), static_cast<size_t>(slot) * MAXC, denseRows(S), S.cOff[slot], S.mNum[slot]);
 }
 
 // Dense per-step view of live manifold k (stored in `slot`, contacts from `o`).
@@ -637,7 +695,7 @@ __device__ __forceinline__ void manifoldView(const State &S, int k, int slot, ui
 __global__ void k_manifold_view(State S)
 {
     FOR_LIVE(k) {
-        const int slot = S.live[S.gLive[k]];
+        const int slot = S.live[S.gLive[S.gOfK[k]]];
         manifoldView(S, k, slot, S.cOff[slot], S.mNum[slot]);
     }
 }
@@ -648,26 +706,19 @@ __global__ void k_contact_count(State S)
 {
     const int total = static_cast<int>(S.wLiveOff[S.numWorlds]);
     for (int k = GID; k < S.slotCap; k += gridDim.x * blockDim.x)
-        S.cCount[k] = k < total ? static_cast<uint32_t>(S.mNum[S.live[S.gLive[k]]]) : 0u;
+        S.cCount[k] = k < total ? static_cast<uint32_t>(S.mNum[S.live[S.gLive[S.gOfK[k]]]]) : 0u;
 }
 
 // Staging -> dense arrays at the scanned offsets.
 __global__ void k_contact_pack(State S)
 {
     FOR_LIVE(k) {
-        const int slot = S.live[S.gLive[k]];
+        const int slot = S.live[S.gLive[S.gOfK[k]]];
         const uint32_t o = S.cScan[k];
-        S.cOff[slot] = o;
         const int n = S.mNum[slot];
+        copyContacts(denseRows(S), o, stageRows(S), static_cast<size_t>(slot) * MAXC, n);
+        S.cOff[slot] = o;
         manifoldView(S, k, slot, o, n);
-        const size_t d = static_cast<size_t>(slot) * MAXC;
-        for (int i = 0; i < n; i++) {
-            S.cRA[o + i] = S.sRA[d + i];
-            S.cRB[o + i] = S.sRB[d + i];
-            S.cC0[o + i] = S.sC0[d + i];
-            S.cPen[o + i] = S.sPen[d + i];
-            S.cLam[o + i] = S.sLam[d + i];
-        }
     }
 }
 
@@ -891,8 +942,8 @@ __global__ void k_adj_fill(State S)
         }
     }
     FOR_LIVE(k) {
-        // k grows with live order inside a world, and a body's entries are all of one world
-        const unsigned long long seq = 0x40000000ull + static_cast<unsigned long long>(k);
+        // the live position orders a body's manifolds (a body's entries are all of one world)
+        const unsigned long long seq = 0x40000000ull + static_cast<unsigned long long>(S.gOfK[k]);
         const unsigned long long e = (seq << 32) | static_cast<uint32_t>(k);
         const int4 h = S.mHdr[k];
         const int a = h.x, b = h.y;
@@ -1027,13 +1078,15 @@ __global__ void k_brake_flags(State S, int count)
 {
     FOR_LIVE(k) {
         const int4 h = S.mHdr[k];
-        const bool active = h.w > 0 && (bodyMoves(S, h.x) || bodyMoves(S, h.y));
+        const float ma = bMass(S, h.x), mb = bMass(S, h.y);
+        const int ua = S.updateIndex[h.x], ub = S.updateIndex[h.y];
+        const v3 n = ld3(S.mBasisK[3 * k]);
+        const v3 pa = bPos(S, h.x), pb = bPos(S, h.y);
+        const qt qa = bRot(S, h.x), qb = bRot(S, h.y);
+        const bool active = h.w > 0 && ((ma > 0.0f && ua >= 0) || (mb > 0.0f && ub >= 0));
         bool on = false;
         if (active) {
-            if (count) atomicAdd(&S.wUpdateCount[S.bodyWorld[h.x]], 1);
-            const v3 n = ld3(S.mBasisK[3 * k]);
-            const v3 pa = bPos(S, h.x), pb = bPos(S, h.y);
-            const qt qa = bRot(S, h.x), qb = bRot(S, h.y);
+            if (count) S.wUpdateCount[S.bodyWorld[h.x]] = 1;   // the world has an active manifold
             for (int i = 0; i < h.w && !on; i++) {
                 v3 rA, rB;
                 on = contactGap(S, n, pa, qa, pb, qb, static_cast<uint32_t>(h.z) + i, rA, rB) > 0.0f;
@@ -1206,6 +1259,20 @@ __global__ void k_warmstart(State S)
         S.pos[i] = st3((p + vel * dt) + v3{0, 0, g} * (wgt * dt * dt));
         S.rot[i] = stq(qadd(q, ang * dt));
     }
+}
+
+// Pose records for the per-manifold passes (the displacement and rotation difference are the
+// primal/dual bracket terms, manifold.cpp:465-501, computed once per body instead of once per
+// manifold side; same floats).
+__global__ void k_pose_pack(State S)
+{
+    const int i = XGID;
+    if (i >= S.numBodies || S.bodyWorld[i] < 0) return;
+    const v3 p = bPos(S, i);
+    const qt q = bRot(S, i);
+    S.poseRec[4 * i] = st3(p - ld3(S.initLin[i]), bMass(S, i));
+    S.poseRec[4 * i + 1] = stq(q);
+    S.poseRec[4 * i + 2] = st3(qsub(q, ldq(S.initAng[i])));
 }
 
 // solver_solve.cpp:146-176 (sleep off)

@@ -37,6 +37,10 @@ struct State {
     // ---- bodies
     float4 *pos, *rot, *initLin, *initAng, *inertLin, *inertAng, *vel, *angVel, *prevVel;
     float4 *sizeR, *massI;
+    // Per body, 64 B (written by k_pose_pack before each pass that gathers body poses per
+    // manifold): {pos - initial, mass}, rot, {rot (-) initial rotation, 0}, unused.  One aligned
+    // half line per gather instead of five separate rows.
+    float4 *poseRec;
     float *fric, *grav;
     uint32_t *layer, *mask;
     uint8_t *lockLin, *lockAng;
@@ -77,6 +81,11 @@ struct State {
     // Dense view of the live manifolds for one step, index k = gLive position (world after
     // world, live order inside a world).  Adjacency entries, stamps, brake links and
     // projection records use k; slots are only the persistent storage across steps.
+    // Inside each world's range the view is ordered by contact count (stable), so that a
+    // wavefront's per-manifold contact loops run about as often as its manifolds have
+    // contacts; gOfK[k] is the manifold's live position (its place in the reference's live
+    // order, which orders adjacency and therefore every sum).
+    int *gOfK;
     int4 *mHdr;                 // {body a, body b, first contact, contact count}
     float4 *mBasisK;            // 3 rows per k
     float *mFricK;
