@@ -402,7 +402,7 @@ AD void stampContact(StampAcc &s, const m3 &B, v3 pen, v3 f, v3 rA, v3 rB)
     s.lin = madd(s.lin, Sm);
     s.rl += fw;
 }
-AD void storeStampSide(float4 *p, const m3 &L, v3 rl, const m3 &A, const m3 &X, v3 ra)
+AD void stampRows(float4 (&p)[7], const m3 &L, v3 rl, const m3 &A, const m3 &X, v3 ra)
 {
     p[0] = make_float4(L.c[0][0], L.c[0][1], L.c[0][2], L.c[1][1]);
     p[1] = make_float4(L.c[1][2], L.c[2][2], rl.x, rl.y);
@@ -412,11 +412,39 @@ AD void storeStampSide(float4 *p, const m3 &L, v3 rl, const m3 &A, const m3 &X, 
     p[5] = make_float4(X.c[1][2], X.c[2][0], X.c[2][1], X.c[2][2]);
     p[6] = make_float4(ra.x, ra.y, ra.z, 0.0f);
 }
-AD void storeStamps(const State &S, int k, const StampAcc &s)
+
+// Wavefront-wide store of the active lanes' rows (7 float4 each, lane l's at dst + 7 l):
+// the rows go through LDS so that every store instruction writes one contiguous run instead
+// of 64 scattered 16-byte pieces.  The active lanes must be a prefix of the wavefront (the
+// live-list walks guarantee it); `lds` holds 7 * 64 float4 for this wavefront.
+AD void waveStoreRows(float4 *dst, const float4 (&rows)[7], float4 *lds, int lane)
 {
-    float4 *p = S.stamp + static_cast<size_t>(STAMP_F4) * k;
-    storeStampSide(p, s.lin, s.rl, s.angA, s.crossA, s.raA);
-    storeStampSide(p + 7, s.lin, s.rl * -1.0f, s.angB, s.crossB, s.raB * -1.0f);
+    const int r = __popcll(__ballot(1));
+#pragma unroll
+    for (int q = 0; q < 7; q++) lds[lane * 7 + q] = rows[q];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+        const int idx = q * r + lane;
+        dst[idx] = lds[idx];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Both sides' stamps of manifold k, lane = k's lane in a wavefront whose active lanes hold
+// consecutive k.  Side B's force terms carry the reference's sign flip (side = -1).
+AD void storeStampsWave(const State &S, int k, const StampAcc &s, float4 *lds, int lane)
+{
+    const size_t k0 = static_cast<size_t>(k - lane);
+    float4 rows[7];
+    stampRows(rows, s.lin, s.rl, s.angA, s.crossA, s.raA);
+    waveStoreRows(S.stampA + 7 * k0, rows, lds, lane);
+    stampRows(rows, s.lin, s.rl * -1.0f, s.angB, s.crossB, s.raB * -1.0f);
+    waveStoreRows(S.stampB + 7 * k0, rows, lds, lane);
 }
 // Symmetric 3x3 from its upper triangle (the sums are symmetric bit for bit).
 AD m3 sym3(float a00, float a01, float a02, float a11, float a12, float a22)
@@ -433,8 +461,9 @@ AD void addStampSide(Block &blk, const float4 *p)
     blk.rhsAng += v3{p6.x, p6.y, p6.z};
 }
 
-// Both sides' stamps of one manifold at the current poses (nothing for a frozen one).
-AD void manifoldStamps(const State &S, int k)
+// Both sides' stamps of one manifold at the current poses (zero for a frozen one, which no
+// body gathers).
+AD void manifoldStamps(const State &S, int k, StampAcc &acc)
 {
     const int4 hd = S.mHdr[k];
     const int a = hd.x, b = hd.y, n = hd.w;
@@ -442,11 +471,10 @@ AD void manifoldStamps(const State &S, int k)
     const PoseRec A = loadPose(S, a), Bd = loadPose(S, b);
     const m3 B = loadBasisK(S, k);
     const float mu = S.mFricK[k];
+    stampInit(acc);
     if (!(A.mass > 0.0f) && !(Bd.mass > 0.0f)) return;
     const v3 dqAL = A.dqL, dqAA = A.dqA, dqBL = Bd.dqL, dqBA = Bd.dqA;
     const m3 rotA = qmat(A.q), rotB = qmat(Bd.q);
-    StampAcc acc;
-    stampInit(acc);
     for (int i = 0; i < n; i++) {
         const uint32_t c = cb + i;
         const v3 rA = mul(rotA, ld3(S.cRA[c]));
@@ -459,7 +487,6 @@ AD void manifoldStamps(const State &S, int k)
         clampForce(f, mu, bounds);
         stampContact(acc, B, pen, f, rA, rB);
     }
-    storeStamps(S, k, acc);
 }
 
 // manifold.cpp:768-856 (arm caching off): lambda <- clamped force, penalty ramps.  With
@@ -467,7 +494,7 @@ AD void manifoldStamps(const State &S, int k)
 // penalty: the poses do not move between this dual and that primal, so its constraint
 // error is the one computed here.
 template <bool STAMP>
-AD void manifoldDual(const State &S, int k, float decay)
+AD void manifoldDual(const State &S, int k, float decay, StampAcc &acc)
 {
     // every load that does not depend on another is issued before the first wait; the
     // frozen test (no movable body) comes after them
@@ -480,12 +507,11 @@ AD void manifoldDual(const State &S, int k, float decay)
     // contact i + 1 is read before contact i is written (distinct rows)
     float4 nPen{}, nRA{}, nRB{}, nC0{}, nLam{};
     if (n > 0) { nPen = S.cPen[cb]; nRA = S.cRA[cb]; nRB = S.cRB[cb]; nC0 = S.cC0[cb]; nLam = S.cLam[cb]; }
+    if (STAMP) stampInit(acc);
     if (!(A.mass > 0.0f) && !(Bd.mass > 0.0f)) return;
     const v3 dqAL = A.dqL, dqAA = A.dqA, dqBL = Bd.dqL, dqBA = Bd.dqA;
     const m3 rotA = qmat(A.q), rotB = qmat(Bd.q);
     const float pmax = S.P.penaltyMax;
-    StampAcc acc;
-    if (STAMP) stampInit(acc);
     for (int i = 0; i < n; i++) {
         const uint32_t c = cb + i;
         const float4 pen4 = nPen, ra4 = nRA, rb4 = nRB, c0in = nC0, lam4 = nLam;
@@ -519,7 +545,6 @@ AD void manifoldDual(const State &S, int k, float decay)
             stampContact(acc, B, pen, f, rA, rB);
         }
     }
-    if (STAMP) storeStamps(S, k, acc);
 }
 
 // manifold.cpp:633-646
@@ -667,7 +692,7 @@ AD void assembleStamped(const State &S, int body, Block &blk)
     const int e0 = S.adjStart[body], e1 = S.adjStart[body + 1];
     for (int e = e0; e < e1; e++) {
         const int ref = S.adj[e];
-        if (ref >= 0) addStampSide(blk, S.stamp + static_cast<size_t>(STAMP_F4) * ref + (S.mHdr[ref].x == body ? 0 : 7));
+        if (ref >= 0) addStampSide(blk, (S.mHdr[ref].x == body ? S.stampA : S.stampB) + 7 * static_cast<size_t>(ref));
         else jointPrimal(S, -ref - 1, body, blk);
     }
 }

@@ -267,7 +267,7 @@ struct Engine {
         AL(S.cRA, C); AL(S.cRB, C); AL(S.cC0, C); AL(S.cPen, C); AL(S.cLam, C);
         AL(S.sRA, C); AL(S.sRB, C); AL(S.sC0, C); AL(S.sPen, C); AL(S.sLam, C);
         AL(S.cOff, M); AL(S.cCount, M + 1); AL(S.cScan, M + 1);
-        AL(S.stamp, STAMP_F4 * M); AL(S.slotWorld, M); AL(S.gLive, M); AL(S.wLiveOff, W + 1); AL(S.live, M); AL(S.liveTmp, M); AL(S.freeList, M);
+        AL(S.stampA, 7 * M); AL(S.stampB, 7 * M); AL(S.slotWorld, M); AL(S.gLive, M); AL(S.wLiveOff, W + 1); AL(S.live, M); AL(S.liveTmp, M); AL(S.freeList, M);
         AL(S.keep, M + 1); AL(S.keepScan, M + 1); AL(S.relScan, M + 1);
         AL(S.succA, M); AL(S.succB, M); AL(S.indeg, M); AL(S.levelOrder, M);
         AL(S.brakeLevels, W); AL(S.levelStart, M); AL(S.wLevelEnd, W);

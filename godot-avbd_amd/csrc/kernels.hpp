@@ -1364,7 +1364,13 @@ __global__ void k_overflow_apply(State S)
 // Stamps of every live manifold that touches a movable body, at the current poses.
 __global__ void __launch_bounds__(TPB) k_stamp(State S)
 {
-    FOR_LIVE(k) manifoldStamps(S, k);
+    __shared__ float4 lds[TPB / 64][7 * 64];
+    const int lane = threadIdx.x & 63;
+    FOR_LIVE(k) {
+        StampAcc acc;
+        manifoldStamps(S, k, acc);
+        storeStampsWave(S, k, acc, lds[threadIdx.x >> 6], lane);
+    }
 }
 
 // solver_step.cpp:277-296: manifolds (not frozen), then joints.  STAMP: also build the next
@@ -1372,7 +1378,13 @@ __global__ void __launch_bounds__(TPB) k_stamp(State S)
 template <bool STAMP>
 __global__ void __launch_bounds__(TPB) k_dual(State S, float decay)
 {
-    FOR_LIVE(k) manifoldDual<STAMP>(S, k, decay);
+    __shared__ float4 lds[STAMP ? TPB / 64 : 1][STAMP ? 7 * 64 : 1];
+    const int lane = threadIdx.x & 63;
+    FOR_LIVE(k) {
+        StampAcc acc;
+        manifoldDual<STAMP>(S, k, decay, acc);
+        if (STAMP) storeStampsWave(S, k, acc, lds[threadIdx.x >> 6], lane);
+    }
     for (int t = GID; t < S.jointCap; t += gridDim.x * blockDim.x)
         if (S.jWorld[t] >= 0 && !S.jDead[t]) jointDual(S, t);
 }
