@@ -20,7 +20,8 @@ namespace avbd_dev {
 struct v3 {
     float x, y, z;
     __device__ __host__ float &operator[](int i) { return (&x)[i]; }
-    __device__ __host__ float operator[](int i) const { return (&x)[i]; }
+    // select chain: a run-time index then stays in registers
+    __device__ __host__ float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
 };
 struct qt {
     float x, y, z, w;

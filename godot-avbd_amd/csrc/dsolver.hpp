@@ -672,6 +672,12 @@ AD void jointDual(const State &S, int j)
     S.jLamA[j] = st3(lamA);
 }
 
+// Adjacency entries: e < 0 is joint -(e + 1); otherwise manifold view index k, with
+// ADJ_SIDE_B set in the entry of the manifold's body b.
+constexpr int ADJ_SIDE_B = 1 << 30;
+AD int adjManifold(int e) { return e & (ADJ_SIDE_B - 1); }
+AD bool adjSideB(int e) { return (e & ADJ_SIDE_B) != 0; }
+
 // Block of one body: inertia plus every incident constraint, in adjacency order (joints in
 // list order, then manifolds in live order; manifold.cpp:123-166, solver_step.cpp:215-226).
 AD void assemble(const State &S, int body, Block &blk)
@@ -680,7 +686,7 @@ AD void assemble(const State &S, int body, Block &blk)
     const int e0 = S.adjStart[body], e1 = S.adjStart[body + 1];
     for (int e = e0; e < e1; e++) {
         const int ref = S.adj[e];
-        if (ref >= 0) stampManifold(S, ref, body, blk);
+        if (ref >= 0) stampManifold(S, adjManifold(ref), body, blk);
         else jointPrimal(S, -ref - 1, body, blk);
     }
 }
@@ -692,7 +698,7 @@ AD void assembleStamped(const State &S, int body, Block &blk)
     const int e0 = S.adjStart[body], e1 = S.adjStart[body + 1];
     for (int e = e0; e < e1; e++) {
         const int ref = S.adj[e];
-        if (ref >= 0) addStampSide(blk, (S.mHdr[ref].x == body ? S.stampA : S.stampB) + 7 * static_cast<size_t>(ref));
+        if (ref >= 0) addStampSide(blk, (adjSideB(ref) ? S.stampB : S.stampA) + 7 * static_cast<size_t>(adjManifold(ref)));
         else jointPrimal(S, -ref - 1, body, blk);
     }
 }
@@ -711,6 +717,8 @@ AD OBB makeOBB(v3 c, qt q, v3 h)
     o.ax[2] = qrot(q, v3{0, 0, 1});
     return o;
 }
+// Axis i of a box by selection (a run-time index into b.ax would put the box in scratch).
+AD v3 axisOf(const OBB &b, int i) { return i == 0 ? b.ax[0] : (i == 1 ? b.ax[1] : b.ax[2]); }
 AD v3 supportPoint(const OBB &b, v3 dir)
 {
     const float sx = dot(dir, b.ax[0]) >= 0.0f ? 1.0f : -1.0f;
@@ -773,8 +781,8 @@ AD void faceManifold(const OBB &bA, const OBB &bB, bool refIsA, int refAxis, v3 
     const OBB &ref = refIsA ? bA : bB;
     const OBB &inc = refIsA ? bB : bA;
     const v3 refOut = refIsA ? nAB : -nAB;
-    const float sgn = dot(refOut, ref.ax[refAxis]) >= 0.0f ? 1.0f : -1.0f;
-    const v3 fn = ref.ax[refAxis] * sgn;
+    const float sgn = dot(refOut, axisOf(ref, refAxis)) >= 0.0f ? 1.0f : -1.0f;
+    const v3 fn = axisOf(ref, refAxis) * sgn;
     const v3 fc = ref.c + fn * ref.h[refAxis];
     v3 fu, fv;
     float feu, fev;
@@ -783,14 +791,14 @@ AD void faceManifold(const OBB &bA, const OBB &bB, bool refIsA, int refAxis, v3 
     {
         float best = -FLT_MAXV;
         for (int i = 0; i < 3; i++) {
-            const float d = fabsf(dot(inc.ax[i], fn));
+            const float d = fabsf(dot(axisOf(inc, i), fn));
             if (d > best) { best = d; incAxis = i; }
         }
     }
     v3 c0[MAX_POLY], c1[MAX_POLY];
     {
-        const float s2 = dot(inc.ax[incAxis], fn) > 0.0f ? -1.0f : 1.0f;
-        const v3 in = inc.ax[incAxis] * s2;
+        const float s2 = dot(axisOf(inc, incAxis), fn) > 0.0f ? -1.0f : 1.0f;
+        const v3 in = axisOf(inc, incAxis) * s2;
         const v3 ic = inc.c + in * inc.h[incAxis];
         v3 u, v;
         float eu, ev;
@@ -828,11 +836,11 @@ AD void faceManifold(const OBB &bA, const OBB &bB, bool refIsA, int refAxis, v3 
 AD void supportEdge(const OBB &b, int ai, v3 dir, v3 &ea, v3 &eb)
 {
     const int a1 = (ai + 1) % 3, a2 = (ai + 2) % 3;
-    const float s1 = dot(dir, b.ax[a1]) >= 0.0f ? 1.0f : -1.0f;
-    const float s2 = dot(dir, b.ax[a2]) >= 0.0f ? 1.0f : -1.0f;
-    const v3 ec = (b.c + b.ax[a1] * (b.h[a1] * s1)) + b.ax[a2] * (b.h[a2] * s2);
-    ea = ec - b.ax[ai] * b.h[ai];
-    eb = ec + b.ax[ai] * b.h[ai];
+    const float s1 = dot(dir, axisOf(b, a1)) >= 0.0f ? 1.0f : -1.0f;
+    const float s2 = dot(dir, axisOf(b, a2)) >= 0.0f ? 1.0f : -1.0f;
+    const v3 ec = (b.c + axisOf(b, a1) * (b.h[a1] * s1)) + axisOf(b, a2) * (b.h[a2] * s2);
+    ea = ec - axisOf(b, ai) * b.h[ai];
+    eb = ec + axisOf(b, ai) * b.h[ai];
 }
 
 AD void closestSeg(v3 p0, v3 p1, v3 q0, v3 q1, v3 &c0, v3 &c1)

@@ -504,7 +504,7 @@ __global__ void __launch_bounds__(TPB) k_pair_emit(State S, int n)
 __device__ __forceinline__ int otherOf(const State &S, int ref, int body)
 {
     int a, b;
-    if (ref >= 0) { const int4 h = S.mHdr[ref]; a = h.x; b = h.y; }
+    if (ref >= 0) { const int4 h = S.mHdr[adjManifold(ref)]; a = h.x; b = h.y; }
     else { const int j = -ref - 1; a = S.jA[j]; b = S.jB[j]; }
     if (a == body) return b;
     if (b == body) return a;
@@ -731,6 +731,10 @@ __global__ void k_contact_unpack(State S)
 // ------------------------------------------------------------ narrow phase ----
 // manifold.cpp:237-435 with collide.cpp:1100-1120: collide at the step-start pose,
 // carry hot state across matching feature keys, C0, release, warm-start decay.
+// One thread per slot-range position.  The live positions are spread over the world
+// ranges' capacity, so each wavefront holds only part of them: this keeps the per-lane
+// scratch (clip polygons, contact list) of the wavefronts in flight small enough to stay in
+// cache, which measured faster than packing the live positions densely.
 __global__ void __launch_bounds__(TPB) k_manifold_init(State S)
 {
     const int t = GID;
@@ -948,7 +952,7 @@ __global__ void k_adj_fill(State S)
         const int4 h = S.mHdr[k];
         const int a = h.x, b = h.y;
         S.adjTmp[S.adjStart[a] + atomicAdd(&S.adjCount[a], 1)] = e;
-        S.adjTmp[S.adjStart[b] + atomicAdd(&S.adjCount[b], 1)] = e;
+        S.adjTmp[S.adjStart[b] + atomicAdd(&S.adjCount[b], 1)] = e | static_cast<uint32_t>(ADJ_SIDE_B);
     }
 }
 
@@ -1112,19 +1116,20 @@ __device__ void brakeLinks(const State &S, int k)
         const int body = side == 0 ? h.x : h.y;
         if (!bodyMoves(S, body)) continue;
         const int e0 = S.adjStart[body], e1 = S.adjStart[body + 1];
+        const int mine = side == 0 ? k : (k | ADJ_SIDE_B);
         int me = e0;
-        while (me < e1 && S.adj[me] != k) me++;
+        while (me < e1 && S.adj[me] != mine) me++;
         if (me == e1) continue;
         for (int e = me - 1; e >= e0; e--) {
             const int r = S.adj[e];
             if (r < 0) break;
-            if (S.brakeOn[r]) { deg++; break; }
+            if (S.brakeOn[adjManifold(r)]) { deg++; break; }
         }
         for (int e = me + 1; e < e1; e++) {
             const int r = S.adj[e];
-            if (r >= 0 && S.brakeOn[r]) {
-                if (side == 0) S.succA[k] = r;
-                else S.succB[k] = r;
+            if (r >= 0 && S.brakeOn[adjManifold(r)]) {
+                if (side == 0) S.succA[k] = adjManifold(r);
+                else S.succB[k] = adjManifold(r);
                 break;
             }
         }
@@ -1471,14 +1476,15 @@ __global__ void k_proj_body(State S)
     for (int e = S.adjStart[i]; e < S.adjStart[i + 1]; e++) {
         const int ref = S.adj[e];
         if (ref < 0) continue;
-        if (!S.projOn[ref]) continue;
+        const int m = adjManifold(ref);
+        if (!S.projOn[m]) continue;
         cnt++;
-        if (S.mHdr[ref].x == i) {
-            sl += ld3(S.projLA[ref]);
-            sa += ld3(S.projAA[ref]);
+        if (!adjSideB(ref)) {
+            sl += ld3(S.projLA[m]);
+            sa += ld3(S.projAA[m]);
         } else {
-            sl -= ld3(S.projLB[ref]);
-            sa -= ld3(S.projAB[ref]);
+            sl -= ld3(S.projLB[m]);
+            sa -= ld3(S.projAB[m]);
         }
     }
     const float inv = 1.0f / static_cast<float>(cnt > 1 ? cnt : 1);
