@@ -45,6 +45,63 @@ def dual_bytes(nm, nc, nj):
     return 160 * nm + 84 * nc + 408 * nj
 
 
+def rocprof_window(rows, steps, big=100000):
+    """Rows (dicts in dispatch order) of the last `steps` batch steps of a bench run: from the
+    dispatch after the batch rebase that precedes the steps' first narrow phase to the last
+    batch-sized dispatch (the plain single-world part that follows has small grids)."""
+    grid = [int(r["Grid_Size"]) if "Grid_Size" in r else
+            int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"]) for r in rows]
+    name = [r["Kernel_Name"] for r in rows]
+    narrow = [i for i, n in enumerate(name) if n.startswith("k_manifold_init") and grid[i] >= big]
+    if len(narrow) < steps:
+        raise SystemExit(f"only {len(narrow)} batch steps in the trace")
+    first = narrow[-steps]
+    rebases = [i for i in range(first) if name[i].startswith("k_rebase") and grid[i] >= big]
+    start = rebases[-1] + 1 if rebases else 0
+    end = max(i for i in range(len(rows)) if grid[i] >= big)
+    return rows[start:end + 1]
+
+
+def summarize_rocprof(path, steps):
+    """Markdown summary of a rocprofv3 CSV (kernel trace or counter collection) over the last
+    `steps` batch steps, per kernel: launches per step, average duration, time per step, and the
+    average of each counter per launch."""
+    import csv
+    import collections
+    with open(path) as f:
+        rows = list(csv.DictReader(f))
+    for r in rows:   # "avbd_dev::k_dual<true>(avbd_dev::State, float)" -> "k_dual<true>"
+        r["Kernel_Name"] = r["Kernel_Name"].split("(")[0].replace("avbd_dev::", "")
+    counters = "Counter_Name" in rows[0]
+    if counters:   # one row per (dispatch, counter): order by dispatch
+        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+        disp = collections.OrderedDict()
+        for r in rows:
+            d = disp.setdefault(r["Dispatch_Id"], dict(r, counters={}))
+            d["counters"][r["Counter_Name"]] = float(r["Counter_Value"])
+        rows = list(disp.values())
+    win = rocprof_window(rows, steps)
+    agg = collections.OrderedDict()
+    for r in win:
+        a = agg.setdefault(r["Kernel_Name"], {"n": 0, "ns": 0.0, "c": collections.Counter()})
+        a["n"] += 1
+        a["ns"] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        for k, v in r.get("counters", {}).items():
+            a["c"][k] += v
+    total = sum(a["ns"] for a in agg.values())
+    names = sorted(agg, key=lambda k: -agg[k]["ns"])
+    cnames = sorted({c for a in agg.values() for c in a["c"]})
+    out = [f"window: last {steps} batch steps, {len(win)} dispatches, "
+           f"{total / steps / 1e6:.3f} ms of kernel time per step", "",
+           "| kernel | launches/step | avg us | ms/step | share |" + "".join(f" {c}/launch |" for c in cnames),
+           "|---|---|---|---|---|" + "---|" * len(cnames)]
+    for k in names:
+        a = agg[k]
+        out.append(f"| {k} | {a['n'] / steps:g} | {a['ns'] / a['n'] / 1e3:.1f} | {a['ns'] / steps / 1e6:.3f} | "
+                   f"{100 * a['ns'] / total:.1f}% |" + "".join(f" {a['c'][c] / a['n']:.1f} |" for c in cnames))
+    print("\n".join(out))
+
+
 def percentile(xs, q):
     return float(np.percentile(np.asarray(xs), q))
 
@@ -71,7 +128,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one (for tracing tools)")
+    ap.add_argument("--summarize-rocprof", metavar="CSV",
+                    help="print a per-kernel summary of a rocprofv3 CSV of a bench run and exit")
+    ap.add_argument("--summary-steps", type=int, default=5)
     args = ap.parse_args()
+    if args.summarize_rocprof:
+        summarize_rocprof(args.summarize_rocprof, args.summary_steps)
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
