@@ -72,9 +72,9 @@ def summarize_rocprof(path, steps):
         rows = list(csv.DictReader(f))
     for r in rows:   # "avbd_dev::k_dual<true>(avbd_dev::State, float)" -> "k_dual<true>"
         r["Kernel_Name"] = r["Kernel_Name"].split("(")[0].replace("avbd_dev::", "")
+    rows.sort(key=lambda r: int(r["Dispatch_Id"]))   # the CSVs are not written in dispatch order
     counters = "Counter_Name" in rows[0]
-    if counters:   # one row per (dispatch, counter): order by dispatch
-        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    if counters:   # one row per (dispatch, counter)
         disp = collections.OrderedDict()
         for r in rows:
             d = disp.setdefault(r["Dispatch_Id"], dict(r, counters={}))
