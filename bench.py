@@ -45,20 +45,26 @@ def dual_bytes(nm, nc, nj):
     return 160 * nm + 84 * nc + 408 * nj
 
 
-def rocprof_window(rows, steps, big=100000):
-    """Rows (dicts in dispatch order) of the last `steps` batch steps of a bench run: from the
-    dispatch after the batch rebase that precedes the steps' first narrow phase to the last
-    batch-sized dispatch (the plain single-world part that follows has small grids)."""
+def rocprof_window(rows, steps):
+    """Rows (dicts in dispatch order) of the last `steps` batch steps of a bench run.  Batch
+    launches are told apart from the plain single-world ones that follow by their grid: the
+    batch's narrow phase and rebase have the largest grids of those kernels in the run.  The
+    window runs from the dispatch after the batch rebase that precedes the first of those
+    steps' narrow phases to the rebase that ends the last one."""
     grid = [int(r["Grid_Size"]) if "Grid_Size" in r else
             int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"]) for r in rows]
     name = [r["Kernel_Name"] for r in rows]
-    narrow = [i for i, n in enumerate(name) if n.startswith("k_manifold_init") and grid[i] >= big]
+    gn = max(g for g, n in zip(grid, name) if n.startswith("k_manifold_init"))
+    gr = max(g for g, n in zip(grid, name) if n.startswith("k_rebase"))
+    narrow = [i for i, n in enumerate(name) if n.startswith("k_manifold_init") and grid[i] == gn]
+    rebase = [i for i, n in enumerate(name) if n.startswith("k_rebase") and grid[i] == gr]
     if len(narrow) < steps:
         raise SystemExit(f"only {len(narrow)} batch steps in the trace")
-    first = narrow[-steps]
-    rebases = [i for i in range(first) if name[i].startswith("k_rebase") and grid[i] >= big]
-    start = rebases[-1] + 1 if rebases else 0
-    end = max(i for i in range(len(rows)) if grid[i] >= big)
+    first, last = narrow[-steps], narrow[-1]
+    before = [i for i in rebase if i < first]
+    after = [i for i in rebase if i > last]
+    start = before[-1] + 1 if before else 0
+    end = after[0] if after else len(rows) - 1
     return rows[start:end + 1]
 
 
