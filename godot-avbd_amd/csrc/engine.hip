@@ -524,14 +524,21 @@ struct Engine {
                         pend();
                     }
                     pbeg(AVBD_K_PRIMAL);
-                    k_overflow_solve<<<nb, TPB, 0, stream>>>(S);
-                    pend();
-                    k_solve_fallback<<<1, TPB, 0, stream>>>(S, 0);
-                    pbeg(AVBD_K_BODY);
-                    k_overflow_apply<<<nb, TPB, 0, stream>>>(S);
-                    pend();
+                    if (withColours) {
+                        k_overflow_solve<false><<<nb, TPB, 0, stream>>>(S);
+                        pend();
+                        k_solve_fallback<<<1, TPB, 0, stream>>>(S, 0);
+                        pbeg(AVBD_K_BODY);
+                        k_overflow_apply<<<nb, TPB, 0, stream>>>(S);
+                        pend();
+                        k_pose_pack<<<nb, TPB, 0, stream>>>(S);
+                    } else {
+                        // no joints: bodies move inside the solve and refresh their pose records
+                        k_overflow_solve<true><<<nb, TPB, 0, stream>>>(S);
+                        pend();
+                        k_solve_fallback<<<1, TPB, 0, stream>>>(S, 2);
+                    }
                 }
-                k_pose_pack<<<nb, TPB, 0, stream>>>(S);
                 pbeg(AVBD_K_DUAL);
                 if (stampInDual)
                     k_dual<true><<<liveGrid(), TPB, 0, stream>>>(S, newton ? 1.0f : prm.stiffness_decay);

@@ -512,9 +512,10 @@ __device__ __forceinline__ int otherOf(const State &S, int ref, int body)
 }
 
 // Grid-stride walk over every world's live manifolds (the compact list gLive).
-// Each XCD takes one contiguous eighth of the list (its workgroups grid-stride inside it).
-#define FOR_LIVE(k)                                                                                         \
-    for (int k##_t = static_cast<int>(S.wLiveOff[S.numWorlds]), k##_x = static_cast<int>(blockIdx.x) % NUM_XCD, \
+// Each XCD takes one contiguous eighth of [0, total) (its workgroups grid-stride inside it).
+#define FOR_LIVE(k) FOR_RANGE(k, static_cast<int>(S.wLiveOff[S.numWorlds]))
+#define FOR_RANGE(k, total)                                                                                 \
+    for (int k##_t = (total), k##_x = static_cast<int>(blockIdx.x) % NUM_XCD,                              \
              k##_nb = static_cast<int>(gridDim.x) / NUM_XCD,                                                \
              k##_hi = static_cast<int>((static_cast<long long>(k##_t) * (k##_x + 1)) / NUM_XCD),            \
              k = static_cast<int>((static_cast<long long>(k##_t) * k##_x) / NUM_XCD) +                      \
@@ -709,16 +710,28 @@ __global__ void k_contact_count(State S)
         S.cCount[k] = k < total ? static_cast<uint32_t>(S.mNum[S.live[S.gLive[S.gOfK[k]]]]) : 0u;
 }
 
-// Staging -> dense arrays at the scanned offsets.
+// Staging -> dense arrays at the scanned offsets: eight lanes per manifold, one per contact
+// row, so each field's rows of a manifold move as one contiguous piece.
 __global__ void k_contact_pack(State S)
 {
-    FOR_LIVE(k) {
+    FOR_RANGE(r, MAXC * static_cast<int>(S.wLiveOff[S.numWorlds])) {
+        const int k = r / MAXC, i = r % MAXC;
         const int slot = S.live[S.gLive[S.gOfK[k]]];
-        const uint32_t o = S.cScan[k];
         const int n = S.mNum[slot];
-        copyContacts(denseRows(S), o, stageRows(S), static_cast<size_t>(slot) * MAXC, n);
-        S.cOff[slot] = o;
-        manifoldView(S, k, slot, o, n);
+        const uint32_t o = S.cScan[k];
+        if (i < n) {
+            const size_t d = static_cast<size_t>(slot) * MAXC + i;
+            const float4 a = S.sRA[d], b = S.sRB[d], c = S.sC0[d], p = S.sPen[d], l = S.sLam[d];
+            S.cRA[o + i] = a;
+            S.cRB[o + i] = b;
+            S.cC0[o + i] = c;
+            S.cPen[o + i] = p;
+            S.cLam[o + i] = l;
+        }
+        if (i == 0) {
+            S.cOff[slot] = o;
+            manifoldView(S, k, slot, o, n);
+        }
     }
 }
 
@@ -1269,15 +1282,28 @@ __global__ void k_warmstart(State S)
 // Pose records for the per-manifold passes (the displacement and rotation difference are the
 // primal/dual bracket terms, manifold.cpp:465-501, computed once per body instead of once per
 // manifold side; same floats).
+__device__ __forceinline__ void writePose(const State &S, int i, v3 p, qt q)
+{
+    S.poseRec[4 * i] = st3(p - ld3(S.initLin[i]), bMass(S, i));
+    S.poseRec[4 * i + 1] = stq(q);
+    S.poseRec[4 * i + 2] = st3(qsub(q, ldq(S.initAng[i])));
+}
 __global__ void k_pose_pack(State S)
 {
     const int i = XGID;
     if (i >= S.numBodies || S.bodyWorld[i] < 0) return;
-    const v3 p = bPos(S, i);
-    const qt q = bRot(S, i);
-    S.poseRec[4 * i] = st3(p - ld3(S.initLin[i]), bMass(S, i));
-    S.poseRec[4 * i + 1] = stq(q);
-    S.poseRec[4 * i + 2] = st3(qsub(q, ldq(S.initAng[i])));
+    writePose(S, i, bPos(S, i), bRot(S, i));
+}
+
+// solver_step.cpp:268-273 for one body: move by the relaxed Jacobi delta.
+__device__ __forceinline__ void jacobiApply(const State &S, int i, v3 dl, v3 da, bool pose)
+{
+    const float r = S.P.jacobiRelax;
+    const v3 p = bPos(S, i) + dl * r;
+    const qt q = qadd(bRot(S, i), da * r);
+    S.pos[i] = st3(p);
+    S.rot[i] = stq(q);
+    if (pose) writePose(S, i, p, q);
 }
 
 // solver_solve.cpp:146-176 (sleep off)
@@ -1319,7 +1345,11 @@ __global__ void __launch_bounds__(TPB) k_colour_pass(State S, int colour)
     S.rot[i] = stq(qadd(bRot(S, i), da));
 }
 
-// solver_step.cpp:248-264: overflow bodies solve against one pose snapshot.
+// solver_step.cpp:248-264: overflow bodies solve against one pose snapshot.  APPLY (worlds
+// without joints): a body's block reads no other body's pose, so each body moves at once
+// (solver_step.cpp:268-273) and refreshes its pose record; otherwise the delta is kept for
+// k_overflow_apply.
+template <bool APPLY>
 __global__ void __launch_bounds__(TPB) k_overflow_solve(State S)
 {
     const int i = XGID;
@@ -1329,12 +1359,17 @@ __global__ void __launch_bounds__(TPB) k_overflow_solve(State S)
     assembleStamped(S, i, blk);
     v3 dl, da;
     if (!solveDeltaFast(blk, dl, da)) { queueFallback(S, i, blk); return; }
-    S.dLin[i] = st3(dl);
-    S.dAng[i] = st3(da);
+    if (APPLY) {
+        jacobiApply(S, i, dl, da, true);
+    } else {
+        S.dLin[i] = st3(dl);
+        S.dAng[i] = st3(da);
+    }
 }
 
 // Pivoted solves of the queued blocks (maths.h:202-343), then the queue is emptied.  One
-// workgroup; `apply` = colour pass (move the body now) or Jacobi (store the delta).
+// workgroup; `apply`: 1 = colour pass (move the body now), 0 = Jacobi (store the delta),
+// 2 = Jacobi moving the body now (with its pose record).
 __global__ void k_solve_fallback(State S, int apply)
 {
     const int n = *S.fbCount;
@@ -1343,7 +1378,9 @@ __global__ void k_solve_fallback(State S, int apply)
         const Block blk = loadBlock(S.fbBlock + 33 * k);
         v3 dl, da;
         solveDeltaSlow(blk, dl, da);
-        if (apply) {
+        if (apply == 2) {
+            jacobiApply(S, i, dl, da, true);
+        } else if (apply) {
             S.pos[i] = st3(bPos(S, i) + dl);
             S.rot[i] = stq(qadd(bRot(S, i), da));
         } else {
