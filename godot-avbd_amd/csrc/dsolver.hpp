@@ -733,50 +733,61 @@ AD void faceAxes(const OBB &b, int i, v3 &u, v3 &v, float &eu, float &ev)
     else { u = b.ax[0]; v = b.ax[1]; eu = b.h.x; ev = b.h.y; }
 }
 constexpr int MAX_POLY = 16;
-AD int clipPoly(const v3 *in, int n, v3 pn, float off, v3 *out)
+// A clip polygon in LDS, one lane's column: vertex v at x[v * NARROW_TPB] (the caller's
+// pointers are offset by the lane), so a wavefront's accesses are consecutive words.
+constexpr int NARROW_TPB = 64;
+struct Poly {
+    float *x, *y, *z;
+    AD v3 get(int v) const { const int i = v * NARROW_TPB; return {x[i], y[i], z[i]}; }
+    AD void put(int v, v3 a) const { const int i = v * NARROW_TPB; x[i] = a.x; y[i] = a.y; z[i] = a.z; }
+};
+// collide.cpp:170-203 (Sutherland-Hodgman against one plane, capacity MAX_POLY)
+AD int clipPoly(const Poly &in, int n, v3 pn, float off, const Poly &out)
 {
     if (n <= 0) return 0;
     int m = 0;
-    v3 a = in[n - 1];
+    v3 a = in.get(n - 1);
     float da = dot(pn, a) - off;
     for (int i = 0; i < n; i++) {
-        const v3 b = in[i];
+        const v3 b = in.get(i);
         const float db = dot(pn, b) - off;
         const bool ai = da <= 1.0e-5f, bi = db <= 1.0e-5f;
         if (ai != bi) {
             float t = 0.0f;
             const float den = da - db;
             if (fabsf(den) > 1.0e-6f) t = sclamp(da / den, 0.0f, 1.0f);
-            if (m < MAX_POLY) out[m++] = a + (b - a) * t;
+            if (m < MAX_POLY) out.put(m++, a + (b - a) * t);
         }
-        if (bi && m < MAX_POLY) out[m++] = b;
+        if (bi && m < MAX_POLY) out.put(m++, b);
         a = b;
         da = db;
     }
     return m;
 }
 
+// The contact list under construction lives in the manifold's staging rows: arms in ra/rb
+// (feature key in rb.w), and the world midpoint, used only for the merge test, in the C0 row
+// until the final C0 overwrites it.
 struct Contacts {
     int n;
-    int key[MAXC];
-    v3 rA[MAXC], rB[MAXC];
-    v3 mid[MAXC];
+    float4 *ra, *rb, *mid;
 };
 
+// collide.cpp:208-233
 AD void addContact(qt qA, v3 cA, qt qB, v3 cB, Contacts &cs, v3 xA, v3 xB, int key)
 {
     const v3 mid = (xA + xB) * 0.5f;
     for (int i = 0; i < cs.n; i++)
-        if (length2(mid - cs.mid[i]) < 1.0e-6f) return;
+        if (length2(mid - ld3(cs.mid[i])) < 1.0e-6f) return;
     if (cs.n >= MAXC) return;
-    cs.key[cs.n] = key;
-    cs.rA[cs.n] = qrot(qconj(qA), xA - cA);
-    cs.rB[cs.n] = qrot(qconj(qB), xB - cB);
-    cs.mid[cs.n] = mid;
+    cs.ra[cs.n] = st3(qrot(qconj(qA), xA - cA));
+    cs.rb[cs.n] = st3(qrot(qconj(qB), xB - cB), __int_as_float(key));
+    cs.mid[cs.n] = st3(mid);
     cs.n++;
 }
 
-AD void faceManifold(const OBB &bA, const OBB &bB, bool refIsA, int refAxis, v3 nAB, float margin, Contacts &cs)
+AD void faceManifold(const OBB &bA, const OBB &bB, bool refIsA, int refAxis, v3 nAB, float margin, Contacts &cs,
+                     const Poly &c0, const Poly &c1)
 {
     const OBB &ref = refIsA ? bA : bB;
     const OBB &inc = refIsA ? bB : bA;
@@ -795,7 +806,6 @@ AD void faceManifold(const OBB &bA, const OBB &bB, bool refIsA, int refAxis, v3 
             if (d > best) { best = d; incAxis = i; }
         }
     }
-    v3 c0[MAX_POLY], c1[MAX_POLY];
     {
         const float s2 = dot(axisOf(inc, incAxis), fn) > 0.0f ? -1.0f : 1.0f;
         const v3 in = axisOf(inc, incAxis) * s2;
@@ -803,10 +813,10 @@ AD void faceManifold(const OBB &bA, const OBB &bB, bool refIsA, int refAxis, v3 
         v3 u, v;
         float eu, ev;
         faceAxes(inc, incAxis, u, v, eu, ev);
-        c0[0] = (ic + u * eu) + v * ev;
-        c0[1] = (ic - u * eu) + v * ev;
-        c0[2] = (ic - u * eu) - v * ev;
-        c0[3] = (ic + u * eu) - v * ev;
+        c0.put(0, (ic + u * eu) + v * ev);
+        c0.put(1, (ic - u * eu) + v * ev);
+        c0.put(2, (ic - u * eu) - v * ev);
+        c0.put(3, (ic + u * eu) - v * ev);
     }
     int n = 4;
     v3 p;
@@ -819,7 +829,7 @@ AD void faceManifold(const OBB &bA, const OBB &bB, bool refIsA, int refAxis, v3 
     prefix |= (incAxis & 0xFF) << 8;
     const float accept = smax(margin, 1.0e-5f);
     for (int i = 0; i < n && cs.n < MAXC; i++) {
-        const v3 pi = c0[i];
+        const v3 pi = c0.get(i);
         const float dist = dot(pi - fc, fn);
         if (dist > accept) continue;
         const v3 pr = pi - fn * dist;
@@ -871,7 +881,7 @@ AD void closestSeg(v3 p0, v3 p1, v3 q0, v3 q1, v3 &c0, v3 &c1)
 // Returns the contact count (0 when separated beyond the margin); `basisOut` is written
 // exactly when the reference writes it.
 AD int collideBoxes(v3 pA, qt qA, v3 hA, v3 pB, qt qB, v3 hB, float margin, m3 &basisOut, bool &wroteBasis,
-                    Contacts &cs)
+                    Contacts &cs, const Poly &c0, const Poly &c1)
 {
     cs.n = 0;
     wroteBasis = false;
@@ -953,7 +963,7 @@ AD int collideBoxes(v3 pA, qt qA, v3 hA, v3 pB, qt qB, v3 hB, float margin, m3 &
         }
         return cs.n;
     }
-    faceManifold(bA, bB, fType == 0, fIdx, nAB, margin, cs);
+    faceManifold(bA, bB, fType == 0, fIdx, nAB, margin, cs, c0, c1);
     return cs.n;
 }
 

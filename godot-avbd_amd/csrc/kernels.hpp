@@ -748,8 +748,9 @@ __global__ void k_contact_unpack(State S)
 // ranges' capacity, so each wavefront holds only part of them: this keeps the per-lane
 // scratch (clip polygons, contact list) of the wavefronts in flight small enough to stay in
 // cache, which measured faster than packing the live positions densely.
-__global__ void __launch_bounds__(TPB) k_manifold_init(State S)
+__global__ void __launch_bounds__(NARROW_TPB) k_manifold_init(State S)
 {
+    __shared__ float poly[2][3][MAX_POLY * NARROW_TPB];   // two clip polygons per lane, 24 KiB
     const int t = GID;
     if (t >= S.slotCap) return;
     const int w = S.slotWorld[t];
@@ -781,9 +782,12 @@ __global__ void __launch_bounds__(TPB) k_manifold_init(State S)
     }
     m3 B;
     bool wrote;
-    Contacts cs;
+    const size_t d = static_cast<size_t>(slot) * MAXC;
+    Contacts cs{0, S.sRA + d, S.sRB + d, S.sC0 + d};
+    const int l = threadIdx.x;
+    const Poly c0{&poly[0][0][l], &poly[0][1][l], &poly[0][2][l]}, c1{&poly[1][0][l], &poly[1][1][l], &poly[1][2][l]};
     const int nn = collideBoxes(pa, qa, v3{sa.x, sa.y, sa.z} * 0.5f, pb, qb, v3{sb.x, sb.y, sb.z} * 0.5f, margin,
-                                B, wrote, cs);
+                                B, wrote, cs, c0, c1);
     if (wrote) {
         S.mBasis[3 * slot] = st3(row(B, 0));
         S.mBasis[3 * slot + 1] = st3(row(B, 1));
@@ -796,13 +800,15 @@ __global__ void __launch_bounds__(TPB) k_manifold_init(State S)
     const uint32_t ob = oldN > 0 ? S.cOff[slot] : 0u;
     const float pmax = S.P.penaltyMax;
     for (int i = 0; i < nn; i++) {
-        v3 rA = cs.rA[i], rB = cs.rB[i];
+        const float4 nra = cs.ra[i], nrb = cs.rb[i];
+        const int key = __float_as_int(nrb.w);
+        v3 rA = ld3(nra), rB = ld3(nrb);
         v3 pen{0, 0, 0}, lam{0, 0, 0};
         uint8_t stick = 0;
         float beta = 1.0f, prevErr = -1.0f;
         for (int j = 0; j < oldN; j++) {
             const float4 orb = S.cRB[ob + j];
-            if (cs.key[i] != __float_as_int(orb.w)) continue;
+            if (key != __float_as_int(orb.w)) continue;
             const float4 ora = S.cRA[ob + j], oc0 = S.cC0[ob + j];
             prevErr = smax(0.0f, COLLISION_MARGIN - oc0.x);
             pen = ld3(S.cPen[ob + j]);
@@ -846,7 +852,7 @@ __global__ void __launch_bounds__(TPB) k_manifold_init(State S)
         pen = lclamp3(pen * S.P.gamma, PENALTY_MIN, pmax);
         const size_t c = static_cast<size_t>(slot) * MAXC + i;
         S.sRA[c] = st3(rA, scale);
-        S.sRB[c] = st3(rB, __int_as_float(cs.key[i]));
+        S.sRB[c] = st3(rB, __int_as_float(key));
         S.sC0[c] = st3(c0, stick ? 1.0f : 0.0f);
         S.sPen[c] = st3(pen);
         S.sLam[c] = st3(lam);
