@@ -283,7 +283,7 @@ struct Engine {
         AL(S.pairValid, P + 1); AL(S.pairValidScan, P + 1);
         AL(S.pairScratch, N * PAIR_BUF); AL(S.wRoot, W);
         AL(S.wBoundsLo, W); AL(S.wBoundsHi, W);
-        AL(S.gOfK, M); AL(S.mHdr, M); AL(S.mBasisK, 3 * M); AL(S.mFricK, M); AL(S.brakeOn, M); AL(S.wOrder, W);
+        AL(S.gOfK, M); AL(S.kSlot, M); AL(S.mHdr, M); AL(S.mBasisK, 3 * M); AL(S.mFricK, M); AL(S.brakeOn, M); AL(S.wOrder, W);
         radixBlocks = (Ncap + RADIX_TILE - 1) / RADIX_TILE;
         AL(radixHist, static_cast<size_t>(256) * radixBlocks + 1);
         const int biggest = std::max({Ncap + 1, Pcap + 1, Scap + 1, 256 * radixBlocks + 1, 2 * (Scap + Jcap)});
@@ -304,12 +304,13 @@ struct Engine {
     }
 
     // ---- primitives -------------------------------------------------------------
-    void scan(const uint32_t *in, uint32_t *out, int n)
+    // `dlen`: a device-side length below n (only that prefix is scanned).
+    void scan(const uint32_t *in, uint32_t *out, int n, const uint32_t *dlen = nullptr)
     {
         const int nt = (n + SCAN_TILE - 1) / SCAN_TILE;
-        k_scan_tiles<<<std::max(nt, 1), TPB, 0, stream>>>(in, out, tileSums, n);
-        k_scan_sums<<<1, TPB, 0, stream>>>(tileSums, std::max(nt, 1), out, n);
-        k_scan_add<<<std::max(nt, 1), TPB, 0, stream>>>(out, tileSums, n);
+        k_scan_tiles<<<std::max(nt, 1), TPB, 0, stream>>>(in, out, tileSums, n, dlen);
+        k_scan_sums<<<1, SCAN_SUMS_TPB, 0, stream>>>(tileSums, std::max(nt, 1), out, n, dlen);
+        k_scan_add<<<std::max(nt, 1), TPB, 0, stream>>>(out, tileSums, n, dlen);
     }
 
     void radixSort()
@@ -348,7 +349,7 @@ struct Engine {
     void packContacts()
     {
         k_contact_count<<<liveGrid(), TPB, 0, stream>>>(S);
-        scan(S.cCount, S.cScan, Scap);
+        scan(S.cCount, S.cScan, Scap, S.wLiveOff + Wcap);
         k_contact_pack<<<liveGrid(), TPB, 0, stream>>>(S);
     }
 
@@ -440,9 +441,9 @@ struct Engine {
         pend();
         scan(S.pairCount, S.pairOff, Ncap);
         k_pair_emit<<<nb, TPB, 0, stream>>>(S, Ncap);
-        k_pair_filter<<<nblk(Pcap), TPB, 0, stream>>>(S);
-        scan(S.pairValid, S.pairValidScan, Pcap);
-        k_alloc_new<<<nblk(Pcap), TPB, 0, stream>>>(S);
+        k_pair_filter<<<liveGrid(), TPB, 0, stream>>>(S);
+        scan(S.pairValid, S.pairValidScan, Pcap, S.pairOff + Ncap);
+        k_alloc_new<<<liveGrid(), TPB, 0, stream>>>(S);
         k_alloc_counts<<<nblk(Wcap), TPB, 0, stream>>>(S);
         if (timed) (void)hipEventRecord(ev[1], stream);
         // ---- warm start forces / narrow phase (solver_solve.cpp:31-105)

@@ -42,30 +42,59 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, int n)
     if (i < n) p[i] = v;
 }
 
-// Exclusive scan of n u32 values, 1024 per block; out has n+1 entries (out[n] = total).
-constexpr int SCAN_ITEMS = 4;
+// Exclusive scan of n u32 values, SCAN_TILE per block; out has n+1 entries (out[n] = total).
+// With `dlen`, only the first min(n, *dlen) values are scanned (a length known on the device
+// only): later tiles do nothing and out[min(n, *dlen)] = out[n] = total.
+constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = TPB * SCAN_ITEMS;
+constexpr int SCAN_SUMS_TPB = 1024;
 
+// Block-wide exclusive scan: wavefront shuffles, then one wavefront scans the wavefront sums.
+// `sm` holds NT / 64 entries.
+template <int NT>
 __device__ __forceinline__ uint32_t blockExclusiveScan(uint32_t v, uint32_t *sm, uint32_t &total)
 {
-    const int tid = threadIdx.x;
-    sm[tid] = v;
-    __syncthreads();
-    for (int off = 1; off < TPB; off <<= 1) {
-        const uint32_t x = tid >= off ? sm[tid - off] : 0u;
-        __syncthreads();
-        sm[tid] += x;
-        __syncthreads();
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
     }
-    total = sm[TPB - 1];
-    const uint32_t incl = sm[tid];
+    if (lane == 63) sm[wave] = x;
     __syncthreads();
-    return incl - v;
+    if (wave == 0) {
+        uint32_t w = lane < NT / 64 ? sm[lane] : 0u;
+#pragma unroll
+        for (int off = 1; off < NT / 64; off <<= 1) {
+            const uint32_t y = __shfl_up(w, off, 64);
+            if (lane >= off) w += y;
+        }
+        if (lane < NT / 64) sm[lane] = w;
+    }
+    __syncthreads();
+    const uint32_t below = wave > 0 ? sm[wave - 1] : 0u;
+    total = sm[NT / 64 - 1];
+    __syncthreads();
+    return below + x - v;
 }
 
-__global__ void k_scan_tiles(const uint32_t *in, uint32_t *out, uint32_t *tileSums, int n)
+__device__ __forceinline__ int scanLength(int n, const uint32_t *dlen)
 {
-    __shared__ uint32_t sm[TPB];
+    if (!dlen) return n;
+    const uint32_t d = *dlen;
+    return d < static_cast<uint32_t>(n) ? static_cast<int>(d) : n;
+}
+
+__global__ void __launch_bounds__(TPB) k_scan_tiles(const uint32_t *in, uint32_t *out, uint32_t *tileSums, int n,
+                                                    const uint32_t *dlen)
+{
+    __shared__ uint32_t sm[TPB / 64];
+    n = scanLength(n, dlen);
+    if (static_cast<int>(blockIdx.x) * SCAN_TILE >= n) {
+        if (threadIdx.x == 0) tileSums[blockIdx.x] = 0;
+        return;
+    }
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     uint32_t s = 0;
@@ -75,7 +104,7 @@ __global__ void k_scan_tiles(const uint32_t *in, uint32_t *out, uint32_t *tileSu
         s += v[k];
     }
     uint32_t total;
-    uint32_t run = blockExclusiveScan(s, sm, total);
+    uint32_t run = blockExclusiveScan<TPB>(s, sm, total);
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; k++) {
         if (base + k < n) out[base + k] = run;
@@ -84,24 +113,30 @@ __global__ void k_scan_tiles(const uint32_t *in, uint32_t *out, uint32_t *tileSu
     if (threadIdx.x == 0) tileSums[blockIdx.x] = total;
 }
 
-__global__ void k_scan_sums(uint32_t *tileSums, int nt, uint32_t *out, int n)
+__global__ void __launch_bounds__(SCAN_SUMS_TPB) k_scan_sums(uint32_t *tileSums, int nt, uint32_t *out, int n,
+                                                            const uint32_t *dlen)
 {
-    __shared__ uint32_t sm[TPB];
+    __shared__ uint32_t sm[SCAN_SUMS_TPB / 64];
     uint32_t carry = 0;
-    for (int start = 0; start < nt; start += TPB) {
+    for (int start = 0; start < nt; start += SCAN_SUMS_TPB) {
         const int i = start + threadIdx.x;
         const uint32_t v = i < nt ? tileSums[i] : 0u;
         uint32_t total;
-        const uint32_t ex = blockExclusiveScan(v, sm, total);
+        const uint32_t ex = blockExclusiveScan<SCAN_SUMS_TPB>(v, sm, total);
         if (i < nt) tileSums[i] = ex + carry;
         carry += total;
     }
-    if (threadIdx.x == 0) out[n] = carry;
+    if (threadIdx.x == 0) {
+        out[scanLength(n, dlen)] = carry;
+        out[n] = carry;
+    }
 }
 
-__global__ void k_scan_add(uint32_t *out, const uint32_t *tileSums, int n)
+__global__ void __launch_bounds__(TPB) k_scan_add(uint32_t *out, const uint32_t *tileSums, int n, const uint32_t *dlen)
 {
+    n = scanLength(n, dlen);
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    if (static_cast<int>(blockIdx.x) * SCAN_TILE >= n) return;
     const uint32_t add = tileSums[blockIdx.x];
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; k++)
@@ -548,7 +583,8 @@ __global__ void __launch_bounds__(ORDER_TPB) k_manifold_order(State S)
     for (int c0 = 0; c0 < n; c0 += ORDER_TPB) {
         const int i = c0 + tid;
         const bool valid = i < n;
-        const uint32_t v = valid ? static_cast<uint32_t>(S.mNum[S.live[S.gLive[base + i]]]) : 0u;
+        const int slot = valid ? S.live[S.gLive[base + i]] : 0;
+        const uint32_t v = valid ? static_cast<uint32_t>(S.mNum[slot]) : 0u;
         unsigned long long m = __ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < 4; bit++) {
@@ -562,6 +598,7 @@ __global__ void __launch_bounds__(ORDER_TPB) k_manifold_order(State S)
             uint32_t off = start[v];
             for (int q = 0; q < wave; q++) off += wcnt[q][v];
             S.gOfK[base + static_cast<int>(off + rank)] = base + i;
+            S.kSlot[base + static_cast<int>(off + rank)] = slot;
         }
         __syncthreads();
         if (tid < NB) {
@@ -582,12 +619,20 @@ __global__ void k_glive_fill(State S)
 
 // solver_broadphase.cpp:30-65: bounds, layers, no force or manifold between the pair,
 // not both static.  `constrained` scans the shorter adjacency list (same answer).
+// Number of candidate pairs of this step (pairs past the capacity are reported, not kept).
+__device__ __forceinline__ int pairTotal(const State &S)
+{
+    const uint32_t t = S.pairOff[S.numBodies];
+    return t < static_cast<uint32_t>(S.pairCap) ? static_cast<int>(t) : S.pairCap;
+}
+
+__device__ void pairFilter(const State &S, int p);
 __global__ void k_pair_filter(State S)
 {
-    const int p = GID;
-    if (p >= S.pairCap) return;
-    const uint32_t total = S.pairOff[S.numBodies];
-    if (static_cast<uint32_t>(p) >= total) { S.pairValid[p] = 0; return; }
+    FOR_RANGE(p, pairTotal(S)) pairFilter(S, p);
+}
+__device__ void pairFilter(const State &S, int p)
+{
     const int i = S.pairI[p], j = S.pairJ[p];
     const v3 ci = ld3(S.aabbC[i]), hi = ld3(S.aabbH[i]), cj = ld3(S.aabbC[j]), hj = ld3(S.aabbH[j]);
     const bool ov = aabbOverlap(ci - hi, ci + hi, cj - hj, cj + hj);
@@ -606,10 +651,13 @@ __global__ void k_pair_filter(State S)
 // solver_broadphase.cpp:153-184 + manifold.cpp:205-268: the r-th new pair of a world
 // takes the r-th slot popped from its free list (LIFO), else a fresh slot, and is
 // appended to the live list in sorted pair order.
+__device__ void allocNew(const State &S, int p);
 __global__ void k_alloc_new(State S)
 {
-    const int p = GID;
-    if (p >= S.pairCap) return;
+    FOR_RANGE(p, pairTotal(S)) allocNew(S, p);
+}
+__device__ void allocNew(const State &S, int p)
+{
     if (!S.pairValid[p]) return;
     const int i = S.pairI[p], j = S.pairJ[p];
     const int w = S.bodyWorld[i];
@@ -696,7 +744,7 @@ __device__ __forceinline__ void manifoldView(const State &S, int k, int slot, ui
 __global__ void k_manifold_view(State S)
 {
     FOR_LIVE(k) {
-        const int slot = S.live[S.gLive[S.gOfK[k]]];
+        const int slot = S.kSlot[k];
         manifoldView(S, k, slot, S.cOff[slot], S.mNum[slot]);
     }
 }
@@ -707,7 +755,7 @@ __global__ void k_contact_count(State S)
 {
     const int total = static_cast<int>(S.wLiveOff[S.numWorlds]);
     for (int k = GID; k < S.slotCap; k += gridDim.x * blockDim.x)
-        S.cCount[k] = k < total ? static_cast<uint32_t>(S.mNum[S.live[S.gLive[S.gOfK[k]]]]) : 0u;
+        S.cCount[k] = k < total ? static_cast<uint32_t>(S.mNum[S.kSlot[k]]) : 0u;
 }
 
 // Staging -> dense arrays at the scanned offsets: eight lanes per manifold, one per contact
@@ -716,7 +764,7 @@ __global__ void k_contact_pack(State S)
 {
     FOR_RANGE(r, MAXC * static_cast<int>(S.wLiveOff[S.numWorlds])) {
         const int k = r / MAXC, i = r % MAXC;
-        const int slot = S.live[S.gLive[S.gOfK[k]]];
+        const int slot = S.kSlot[k];
         const int n = S.mNum[slot];
         const uint32_t o = S.cScan[k];
         if (i < n) {

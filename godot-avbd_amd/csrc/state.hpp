@@ -86,6 +86,7 @@ struct State {
     // contacts; gOfK[k] is the manifold's live position (its place in the reference's live
     // order, which orders adjacency and therefore every sum).
     int *gOfK;
+    int *kSlot;                 // storage slot of view index k
     int4 *mHdr;                 // {body a, body b, first contact, contact count}
     float4 *mBasisK;            // 3 rows per k
     float *mFricK;
