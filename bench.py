@@ -119,6 +119,98 @@ def sample_worlds(sizes, k, seed=0):
     return sorted(set(int(order[i]) for i in idx))
 
 
+def bench_chains(args, log):
+    """C3: one plain world of 4096 chains x 32 unit boxes (ball joints, anchors, ground),
+    beta_lin 1e5, k cap 1e9, 10 iterations: colour Gauss-Seidel for the jointed bodies."""
+    params = pkg.default_params(iterations=args.iterations)
+    params.beta_lin = 1.0e5
+    params.penalty_max = 1.0e9
+    t0 = time.time()
+    bodies, joints = pkg.scenes.chains(n_chains=4096, links=32)
+    w = pkg.World(params, len(bodies["mass"]), len(joints["body_b"]))
+    w.upload(bodies, joints)
+    for _ in range(args.c3_warmup):
+        w.step()
+    a = time.perf_counter()
+    for _ in range(args.c3_steps):
+        w.step()
+    dt = (time.perf_counter() - a) / args.c3_steps
+    movable = int((bodies["mass"] > 0).sum())
+    st = w.stats()
+    w.close()
+    log(f"C3 chains: {dt * 1e3:.3f} ms/step, {movable / dt / 1e6:.2f} M body-steps/s ({time.time() - t0:.1f}s)")
+    return {"workload": "C3 plain world: 4096 chains x 32 unit boxes, ball joints, static anchors, ground; "
+                        "beta_lin 1e5, k cap 1e9", "bodies": len(bodies["mass"]), "movable_bodies": movable,
+            "joints": len(joints["body_b"]), "manifolds": st["manifolds"], "contacts": st["contacts"],
+            "warmup": args.c3_warmup, "steps": args.c3_steps, "ms_per_step": dt * 1e3,
+            "value": movable / dt, "unit": "body-steps/s"}
+
+
+def bench_serving(args, log):
+    """C5: Poisson arrivals of world requests (bodies log-uniform in [16, 16384], horizons
+    U[1, 600] steps, 70 % piles / 30 % jointed chains), admitted by HBM budget into one
+    device-resident batch, continuous batching; body-steps/s and p50/p99 step latency per
+    offered load."""
+    from importlib import import_module
+    serve = import_module("godot-avbd_amd.serve")
+    params = pkg.default_params(iterations=args.iterations)
+    rng = np.random.default_rng(5)
+    chain_cache = {}
+
+    def make(rid):
+        n = int(round(math.exp(rng.uniform(math.log(16), math.log(16384)))))
+        horizon = int(rng.integers(1, 601))
+        if rng.uniform() < 0.7:
+            b, j = pkg.scenes.pile(n, seed=50000 + rid, pitch=4.0, ground=max(40.0, 8.0 * math.sqrt(n)))
+        else:
+            nc = max(1, int(round(n / 33.0)))
+            nc = max(1, int(round(2 ** (round(2 * math.log2(nc)) / 2))))   # a few cached sizes
+            if nc not in chain_cache:
+                chain_cache[nc] = pkg.scenes.chains(n_chains=nc, links=32, seed=51 + nc)
+            b, j = chain_cache[nc]
+        return serve.Request(rid, b, j, horizon, 0.0)
+
+    out = []
+    rid = 0
+    for load in args.serve_loads:
+        srv = serve.Server(params, args.serve_budget_gb * 1e9, max_worlds=args.serve_max_worlds,
+                           joints_per_body=0.5)
+        # start at the steady-state population of this load (mean horizon 300 ticks)
+        for _ in range(int(load * 300)):
+            r = make(rid); rid += 1
+            r.horizon = int(rng.integers(1, r.horizon + 1))
+            srv.submit(r)
+        for _ in range(args.serve_warmup):
+            srv.tick()
+        srv.latencies.clear(); srv.admit_waits.clear(); srv.body_steps = 0
+        resident = []
+        a = time.perf_counter()
+        for _ in range(args.serve_ticks):
+            for _ in range(rng.poisson(load)):
+                r = make(rid); rid += 1
+                srv.submit(r)
+            srv.tick()
+            resident.append(len(srv.resident))
+        el = time.perf_counter() - a
+        lat = np.asarray(srv.latencies) * 1e3
+        waits = np.asarray(srv.admit_waits) * 1e3 if srv.admit_waits else np.zeros(1)
+        rec = {"offered_load_per_tick": load, "ticks": args.serve_ticks, "seconds": el,
+               "value": srv.body_steps / el, "unit": "body-steps/s",
+               "resident_worlds_mean": float(np.mean(resident)), "waiting_at_end": len(srv.waiting),
+               "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
+               "p50_admission_wait_ms": float(np.percentile(waits, 50)),
+               "p99_admission_wait_ms": float(np.percentile(waits, 99)),
+               "hbm_budget_bytes": int(args.serve_budget_gb * 1e9), "device_bytes": srv.device_bytes,
+               "body_capacity": srv.max_bodies}
+        log(f"serving load {load}/tick: {rec['value'] / 1e6:.1f} M body-steps/s, p50 {rec['p50_step_ms']:.2f} ms, "
+            f"p99 {rec['p99_step_ms']:.2f} ms, {rec['resident_worlds_mean']:.0f} resident worlds")
+        srv.close()
+        out.append(rec)
+    return {"workload": "C5 serving: Poisson arrivals, bodies log-uniform [16, 16384], horizons U[1, 600], "
+                        "70% piles / 30% chains, admission by HBM budget, continuous batching",
+            "loads": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +226,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one (for tracing tools)")
+    ap.add_argument("--no-c3", action="store_true", help="skip the C3 chains world")
+    ap.add_argument("--c3-warmup", type=int, default=60)
+    ap.add_argument("--c3-steps", type=int, default=30)
+    ap.add_argument("--no-serve", action="store_true", help="skip the C5 serving loop")
+    ap.add_argument("--serve-loads", type=float, nargs="+", default=[0.5, 1.0, 2.0])
+    ap.add_argument("--serve-ticks", type=int, default=120)
+    ap.add_argument("--serve-warmup", type=int, default=20)
+    ap.add_argument("--serve-budget-gb", type=float, default=32.0)
+    ap.add_argument("--serve-max-worlds", type=int, default=2048)
     ap.add_argument("--summarize-rocprof", metavar="CSV",
                     help="print a per-kernel summary of a rocprofv3 CSV of a bench run and exit")
     ap.add_argument("--summary-steps", type=int, default=5)
@@ -284,6 +385,9 @@ def main():
         log(f"cpu baseline: {multi_value / 1e3:.1f} k body-steps/s on {threads} threads, "
             f"{one_value / 1e3:.1f} k on 1")
 
+    c3 = None if args.no_c3 else bench_chains(args, log)
+    serving = None if args.no_serve else bench_serving(args, log)
+
     line = {
         "metric": "body-steps/sec @10 iters (batched vs plain, same GPU); p50/p99 step latency",
         "value": value,
@@ -318,6 +422,8 @@ def main():
         "kernel_ms": prof["avg_ms"],
         "kernel_launches_per_step": {k: v / max(1, prof["steps"]) for k, v in prof["launches"].items()},
         "cpu_baseline": cpu,
+        "c3": c3,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

@@ -203,12 +203,16 @@ def load_library(path=LIB_PATH):
         "avbd_world_get_manifolds": (C.c_int32, [vp, C.POINTER(Manifolds)]),
         "avbd_world_set_manifolds": (C.c_int32, [vp, C.POINTER(Manifolds)]),
         "avbd_world_stats": (C.c_int32, [vp, C.POINTER(StepStats)]),
+        "avbd_world_colours": (C.c_int32, [vp, u8p]),
         "avbd_world_probe_broadphase": (C.c_int32, [vp, u64p, i32p, u64p, i32p]),
         "avbd_world_profile": (C.c_int32, [vp, C.c_int32, C.POINTER(KernelProfile)]),
         "avbd_batch_profile": (C.c_int32, [vp, C.c_int32, C.POINTER(KernelProfile)]),
         "avbd_batch_create": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                           C.POINTER(vp)]),
         "avbd_batch_destroy": (C.c_int32, [vp]),
+        "avbd_batch_footprint": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                             C.POINTER(C.c_int64)]),
+        "avbd_batch_device_bytes": (C.c_int32, [vp, C.POINTER(C.c_int64)]),
         "avbd_batch_add_world": (C.c_int32, [vp, C.POINTER(Bodies), C.POINTER(Joints), i32p]),
         "avbd_batch_remove_world": (C.c_int32, [vp, C.c_int32]),
         "avbd_batch_step": (C.c_int32, [vp]),

@@ -210,8 +210,12 @@ struct Engine {
     }
 
     std::vector<void *> allocs;
+    bool sizing = false;        // create() only adds up the device bytes it would allocate
+    size_t deviceBytes = 0;
     template <typename T> avbd_status A(T *&p, size_t n)
     {
+        deviceBytes += std::max<size_t>(n, 1) * sizeof(T);
+        if (sizing) return AVBD_OK;
         hipError_t e = dalloc(p, n);
         if (e != hipSuccess) return fail(AVBD_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
         allocs.push_back(p);
@@ -242,13 +246,15 @@ struct Engine {
         bodyAlloc.init(Ncap);
         jointAlloc.init(Jcap);
         slotAlloc.init(Scap);
-        int ndev = 0;
-        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-            return fail(AVBD_ERR_NO_DEVICE, "no HIP device visible");
-        HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
-        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&errHost), sizeof(int) * Wcap, hipHostMallocDefault));
-        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&countsHost), sizeof(int) * 64, hipHostMallocDefault));
+        if (!sizing) {
+            int ndev = 0;
+            if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+                return fail(AVBD_ERR_NO_DEVICE, "no HIP device visible");
+            HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+            HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&errHost), sizeof(int) * Wcap, hipHostMallocDefault));
+            HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&countsHost), sizeof(int) * 64, hipHostMallocDefault));
+        }
         avbd_status st;
 #define AL(ptr, n) if ((st = A(ptr, n)) != AVBD_OK) return st
         const size_t N = Ncap, J = Jcap, M = Scap, C = static_cast<size_t>(Scap) * MAXC, W = Wcap, P = Pcap;
@@ -290,6 +296,7 @@ struct Engine {
         tileSumsCap = (biggest + SCAN_TILE - 1) / SCAN_TILE + 1;
         AL(tileSums, tileSumsCap);
 #undef AL
+        if (sizing) return AVBD_OK;
         // all bodies / slots / joints start outside any world
         k_fill_i32<<<nblk(Ncap), TPB, 0, stream>>>(S.bodyWorld, -1, Ncap);
         k_fill_i32<<<nblk(Ncap), TPB, 0, stream>>>(S.updateIndex, -1, Ncap);
@@ -1358,6 +1365,21 @@ avbd_status avbd_world_stats(avbd_world *world, avbd_step_stats *out)
     return world->eng.fillStats(0, out);
 }
 
+avbd_status avbd_world_colours(avbd_world *world, uint8_t *out)
+{
+    if (!world || !out) return fail(AVBD_ERR_INVALID, "null argument");
+    Engine &e = world->eng;
+    if (!e.worlds[0].active) return fail(AVBD_ERR_INVALID, "nothing uploaded");
+    const WorldInfo &wi = e.worlds[0];
+    std::vector<uint8_t> col(wi.bodyCount);
+    std::vector<int> ui(wi.bodyCount);
+    HIPCHK(hipStreamSynchronize(e.stream));
+    HIPCHK(hipMemcpy(col.data(), e.S.colour + wi.bodyBase, wi.bodyCount, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ui.data(), e.S.updateIndex + wi.bodyBase, sizeof(int) * wi.bodyCount, hipMemcpyDeviceToHost));
+    for (int i = 0; i < wi.bodyCount; i++) out[i] = ui[i] < 0 ? 255 : col[i];
+    return AVBD_OK;
+}
+
 avbd_status avbd_world_profile(avbd_world *world, int32_t steps, avbd_kernel_profile *out)
 {
     if (!world || !out) return fail(AVBD_ERR_INVALID, "null argument");
@@ -1399,6 +1421,29 @@ avbd_status avbd_batch_create(const avbd_params *params, int32_t max_worlds, int
         return st;
     }
     *out = b;
+    return AVBD_OK;
+}
+
+avbd_status avbd_batch_footprint(const avbd_params *params, int32_t max_worlds, int32_t max_bodies,
+                                 int32_t max_joints, int32_t max_manifolds, int64_t *bytes)
+{
+    if (!bytes) return fail(AVBD_ERR_INVALID, "bytes is null");
+    avbd_status st = checkParams(params);
+    if (st != AVBD_OK) return st;
+    if (max_worlds < 1 || max_bodies < 0 || max_joints < 0) return fail(AVBD_ERR_INVALID, "bad capacity");
+    Engine e;
+    e.sizing = true;
+    const int slots = max_manifolds > 0 ? max_manifolds : 8 * std::max(max_bodies, 1) + 64 * max_worlds;
+    st = e.create(*params, max_worlds, max_bodies, max_joints, slots, true);
+    if (st != AVBD_OK) return st;
+    *bytes = static_cast<int64_t>(e.deviceBytes);
+    return AVBD_OK;
+}
+
+avbd_status avbd_batch_device_bytes(avbd_batch *batch, int64_t *bytes)
+{
+    if (!batch || !bytes) return fail(AVBD_ERR_INVALID, "null argument");
+    *bytes = static_cast<int64_t>(batch->eng.deviceBytes);
     return AVBD_OK;
 }
 

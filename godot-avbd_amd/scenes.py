@@ -230,3 +230,75 @@ def batch_piles(worlds=4096, lo=8, hi=2048, seed=4):
     """C4: independent C2-style piles with log-uniform body counts; world w uses seed 4000+w."""
     sizes = batch_sizes(worlds, lo, hi, seed)
     return [pile(int(n), seed=4000 + w, pitch=4.0, ground=max(40.0, 8.0 * math.sqrt(n))) for w, n in enumerate(sizes)]
+
+
+def greedy_colours(bodies, joints, manifolds, contact_only_overflow=True, max_colours=8):
+    """Colouring of solver_solve.cpp:365-424 restated from its definition: movable bodies in list
+    order each take the smallest colour in [0, 8) not used by a lower-ordered neighbour (a body
+    sharing a joint or a manifold), else overflow; bodies touching no joint are moved to the
+    Jacobi bucket (8) when contact_only_overflow; an overflow body that is not contact-only is
+    left unsolved (9).  Bodies outside the update set get 255."""
+    n = len(bodies["mass"])
+    movable = np.asarray(bodies["mass"]) > 0
+    order = np.nonzero(movable)[0]
+    rank = np.full(n, -1)
+    rank[order] = np.arange(len(order))
+    nbr = [set() for _ in range(n)]
+    jointed = np.zeros(n, bool)
+    pairs = []
+    if joints is not None:
+        for a, b in zip(joints["body_a"], joints["body_b"]):
+            pairs.append((int(a), int(b)))
+            if a >= 0:
+                jointed[a] = True
+            jointed[b] = True
+    for a, b in zip(manifolds["body_a"], manifolds["body_b"]):
+        pairs.append((int(a), int(b)))
+    for a, b in pairs:
+        if a >= 0 and b >= 0:
+            nbr[a].add(b)
+            nbr[b].add(a)
+    out = np.full(n, 255, np.uint8)
+    colour = {}
+    for i in order:
+        used = {colour[o] for o in nbr[i] if o in colour and rank[o] >= 0 and rank[o] < rank[i]}
+        c = next((k for k in range(max_colours) if k not in used), max_colours)
+        colour[i] = c
+    for i in order:
+        c = colour[i]
+        if contact_only_overflow and not jointed[i]:
+            out[i] = 8
+        else:
+            out[i] = c if c < max_colours else 9
+    return out
+
+
+def world_statistics(state, manifolds):
+    """The long-run statistics compared between implementations: per-contact penetration depth
+    max(0, 1 mm - C0.x) after the rebase (avbd_physics_server3d.cpp:1547-1560) as p50/p95/p99/max,
+    |C_n| = |C0.x| mean and max over contacts, and the mechanical energy
+    E = sum 1/2 m |v|^2 + 1/2 w^T (R diag(I) R^T) w - m g z."""
+    num = np.asarray(manifolds["num_contacts"])
+    c0 = np.asarray(manifolds["c0"]).reshape(len(num), 8, 3)
+    used = np.arange(8)[None, :] < num[:, None]
+    c0n = c0[..., 0][used]
+    depth = np.maximum(0.0, 1.0e-3 - c0n.astype(np.float64))
+    q = lambda a, k: float(np.percentile(a, k)) if len(a) else 0.0
+    m = np.asarray(state["mass"], np.float64)
+    mov = m > 0
+    v = np.asarray(state["velocity"], np.float64).reshape(-1, 3)
+    w = np.asarray(state["angular_velocity"], np.float64).reshape(-1, 3)
+    inertia = np.asarray(state["moment"], np.float64).reshape(-1, 3)
+    rot = np.asarray(state["rotation"], np.float64).reshape(-1, 4)
+    x, y, z, s = rot[:, 0], rot[:, 1], rot[:, 2], rot[:, 3]
+    R = np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - s * z), 2 * (x * z + s * y)], -1),
+                  np.stack([2 * (x * y + s * z), 1 - 2 * (x * x + z * z), 2 * (y * z - s * x)], -1),
+                  np.stack([2 * (x * z - s * y), 2 * (y * z + s * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+    wl = np.einsum("nji,nj->ni", R, w)   # body-frame angular velocity
+    ke = 0.5 * m * (v * v).sum(1) + 0.5 * (inertia * wl * wl).sum(1)
+    pe = -m * np.asarray(state["gravity"], np.float64) * np.asarray(state["position"], np.float64).reshape(-1, 3)[:, 2]
+    return {"depth_p50": q(depth, 50), "depth_p95": q(depth, 95), "depth_p99": q(depth, 99),
+            "depth_max": float(depth.max()) if len(depth) else 0.0,
+            "residual_mean": float(np.abs(c0n).mean()) if len(c0n) else 0.0,
+            "residual_max": float(np.abs(c0n).max()) if len(c0n) else 0.0,
+            "energy": float((ke + pe)[mov].sum()), "contacts": int(num.sum())}

@@ -1,0 +1,124 @@
+"""Serving loop: continuous batching of world requests on one device-resident batch.
+
+A request is a world (bodies, optional joints) and a horizon in steps.  Requests arrive at any
+step; the server keeps a FIFO of waiting requests and admits each one as soon as the batch has
+room for it (its bodies, joints and manifold slots fit the ranges still free in a batch sized
+once from an HBM byte budget, `avbd_batch_footprint`).  Every admitted world is stepped with all
+other resident worlds by one batch step per tick (one captured HIP graph), stays in HBM between
+ticks, and is retired (its ranges freed for the next admission) when its horizon is done.
+
+Latency is recorded per world per step: from the moment the step was admitted (the request's
+arrival for its first step, the start of the tick otherwise) to the moment its result is ready
+(`avbd_batch_step` returns after the step has completed on the device).
+"""
+import ctypes as C
+import time
+from collections import deque
+
+import numpy as np
+
+from . import abi
+from .world import Batch
+
+
+def footprint(params, max_worlds, max_bodies, max_joints=0, max_manifolds=0):
+    """Device bytes of a batch with these capacities."""
+    lib = abi.load_library()
+    out = C.c_int64()
+    abi.check(lib.avbd_batch_footprint(C.byref(params), int(max_worlds), int(max_bodies), int(max_joints),
+                                       int(max_manifolds), C.byref(out)))
+    return out.value
+
+
+def capacity_for_budget(params, hbm_bytes, max_worlds, joints_per_body=1.0):
+    """Largest body capacity (with joints_per_body joints per body) whose batch fits the budget."""
+    lo, hi = 1, 1 << 27
+    while lo < hi:
+        mid = (lo + hi + 1) // 2
+        if footprint(params, max_worlds, mid, int(mid * joints_per_body)) <= hbm_bytes:
+            lo = mid
+        else:
+            hi = mid - 1
+    return lo, int(lo * joints_per_body)
+
+
+class Request:
+    __slots__ = ("rid", "bodies", "joints", "horizon", "arrival", "wid", "done", "movable", "first_step")
+
+    def __init__(self, rid, bodies, joints, horizon, arrival):
+        self.rid, self.bodies, self.joints, self.horizon, self.arrival = rid, bodies, joints, horizon, arrival
+        self.wid, self.done, self.first_step = -1, 0, True
+        self.movable = int((np.asarray(bodies["mass"]) > 0).sum())
+
+
+class Server:
+    """Continuous batching over one `Batch` sized from `hbm_bytes`."""
+
+    def __init__(self, params, hbm_bytes, max_worlds=1024, joints_per_body=1.0, use_graph=True):
+        self.max_bodies, self.max_joints = capacity_for_budget(params, hbm_bytes, max_worlds, joints_per_body)
+        self.batch = Batch(params, max_worlds, self.max_bodies, self.max_joints)
+        self.batch.use_graph(use_graph)
+        b = C.c_int64()
+        abi.check(self.batch.lib.avbd_batch_device_bytes(self.batch.h, C.byref(b)))
+        self.device_bytes = b.value
+        self.waiting = deque()
+        self.resident = {}
+        self.latencies = []          # seconds, one per world-step
+        self.admit_waits = []        # seconds from arrival to admission
+        self.body_steps = 0
+        self.finished = []           # (request, final state)
+        self.keep_final = False
+
+    def submit(self, req):
+        req.arrival = time.perf_counter()
+        self.waiting.append(req)
+
+    def _admit(self):
+        now = time.perf_counter()
+        while self.waiting:
+            req = self.waiting[0]
+            try:
+                req.wid = self.batch.add(req.bodies, req.joints)
+            except abi.AvbdError as e:
+                if e.code == abi.AVBD_ERR_CAPACITY:
+                    break            # no room until a resident world retires (FIFO: no overtaking)
+                raise
+            self.waiting.popleft()
+            self.admit_waits.append(now - req.arrival)
+            self.resident[req.wid] = req
+
+    def tick(self):
+        """Admit what fits, step every resident world once, retire finished worlds."""
+        self._admit()
+        if not self.resident:
+            return 0
+        t0 = time.perf_counter()
+        self.batch.step()
+        t1 = time.perf_counter()
+        stepped = 0
+        for wid, req in list(self.resident.items()):
+            self.latencies.append(t1 - (req.arrival if req.first_step else t0))
+            req.first_step = False
+            req.done += 1
+            stepped += req.movable
+            if req.done >= req.horizon:
+                if self.keep_final:
+                    self.finished.append((req, self.batch.download(wid)))
+                self.batch.remove(wid)
+                del self.resident[wid]
+        self.body_steps += stepped
+        return stepped
+
+    def close(self):
+        self.batch.close()
+
+
+def run_alone(params, bodies, joints, horizon):
+    """The reference result for `test: same alone and in the mix`: one world, one batch."""
+    b = Batch(params, 1, len(bodies["mass"]), 0 if joints is None else len(joints["body_b"]))
+    wid = b.add(bodies, joints)
+    for _ in range(horizon):
+        b.step()
+    out = b.download(wid)
+    b.close()
+    return out

@@ -91,6 +91,12 @@ class World:
         abi.check(self.lib.avbd_world_stats(self.h, C.byref(s)))
         return {name: getattr(s, name) for name, _ in abi.StepStats._fields_}
 
+    def colours(self):
+        """Per body (list order): 0-7 colour, 8 Jacobi bucket, 9 unsolved overflow, 255 not updated."""
+        out = np.zeros(self.n, np.uint8)
+        abi.check(self.lib.avbd_world_colours(self.h, out.ctypes.data_as(abi.u8p)))
+        return out
+
     def probe_broadphase(self, pair_capacity=None):
         n = self.n
         codes = np.zeros(n, np.uint64)

@@ -234,6 +234,10 @@ avbd_status avbd_world_download(avbd_world *world, avbd_bodies *bodies, avbd_joi
 avbd_status avbd_world_get_manifolds(avbd_world *world, avbd_manifolds *out);
 avbd_status avbd_world_set_manifolds(avbd_world *world, const avbd_manifolds *in);
 avbd_status avbd_world_stats(avbd_world *world, avbd_step_stats *out);
+/* Colour of every body after the last step, in list order (solver_solve.cpp:365-424): 0-7 a
+ * Gauss-Seidel colour, 8 the Jacobi (overflow) bucket, 9 an overflow body the step leaves
+ * unsolved, 255 not in the update set.  `out` holds the world's body count. */
+avbd_status avbd_world_colours(avbd_world *world, uint8_t *out);
 /* Run `steps` steps with every launch bracketed by HIP events; fills per-class averages. */
 avbd_status avbd_world_profile(avbd_world *world, int32_t steps, avbd_kernel_profile *out);
 
@@ -254,6 +258,12 @@ avbd_status avbd_batch_create(const avbd_params *params, int32_t max_worlds,
                               int32_t max_bodies, int32_t max_joints, int32_t max_manifolds,
                               avbd_batch **out);
 avbd_status avbd_batch_destroy(avbd_batch *batch);
+/* Device bytes a batch of these capacities allocates (computed on the host; no device needed),
+ * and the bytes an existing batch holds.  A serving loop sizes its batch from an HBM budget
+ * with the first and admits worlds while their bodies, joints and manifold slots fit. */
+avbd_status avbd_batch_footprint(const avbd_params *params, int32_t max_worlds, int32_t max_bodies,
+                                 int32_t max_joints, int32_t max_manifolds, int64_t *bytes);
+avbd_status avbd_batch_device_bytes(avbd_batch *batch, int64_t *bytes);
 /* Add a world; its id is written to *world_id.  Takes effect at the next step. */
 avbd_status avbd_batch_add_world(avbd_batch *batch, const avbd_bodies *bodies,
                                  const avbd_joints *joints, int32_t *world_id);

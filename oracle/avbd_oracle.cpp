@@ -1592,6 +1592,19 @@ void oracle_stats(oracle_world *o, int32_t out[6])
     out[5] = static_cast<int32_t>(w.overflowOrder.size());
 }
 
+void oracle_colours(oracle_world *o, uint8_t *out)
+{
+    const World &w = o->w;
+    std::vector<uint8_t> jacobi(w.colour.size(), 0);
+    for (int i : w.overflowOrder) jacobi[i] = 1;
+    for (size_t b = 0; b < w.bodies.size(); b++) {
+        const int u = w.bodies[b].updateIndex;
+        if (u < 0) { out[b] = 255; continue; }
+        const uint8_t c = w.colour[u];
+        out[b] = c < kMaxColours ? c : (jacobi[u] ? 8 : 9);
+    }
+}
+
 // Swept AABBs of the current state (solver_broadphase.cpp:101-122).
 void oracle_swept_aabbs(oracle_world *o, float *centers, float *halves)
 {

@@ -23,6 +23,8 @@ int oracle_set_manifolds(oracle_world *o, const avbd_manifolds *m);
 uint64_t oracle_digest(oracle_world *o);
 /* out: manifolds, contacts, candidate pairs, new manifolds, colours, overflow bodies */
 void oracle_stats(oracle_world *o, int32_t out[6]);
+/* per body in list order: 0-7 colour, 8 Jacobi bucket, 9 overflow left unsolved, 255 not updated */
+void oracle_colours(oracle_world *o, uint8_t *out);
 void oracle_swept_aabbs(oracle_world *o, float *centers, float *halves);
 int oracle_candidate_pairs(oracle_world *o, uint64_t *pairs, int capacity);
 void oracle_morton(const float *centers, int n, uint64_t *codes, int32_t *order);

@@ -57,6 +57,7 @@ def lib():
         "oracle_set_manifolds": (C.c_int, [vp, C.POINTER(a.Manifolds)]),
         "oracle_digest": (C.c_uint64, [vp]),
         "oracle_stats": (None, [vp, a.i32p]),
+        "oracle_colours": (None, [vp, a.u8p]),
         "oracle_swept_aabbs": (None, [vp, a.f32p, a.f32p]),
         "oracle_candidate_pairs": (C.c_int, [vp, a.u64p, C.c_int]),
         "oracle_morton": (None, [a.f32p, C.c_int, a.u64p, a.i32p]),
@@ -164,6 +165,11 @@ class OracleWorld:
         self.L.oracle_stats(self.h, out)
         keys = ("manifolds", "contacts", "candidate_pairs", "new_manifolds", "colours", "overflow_bodies")
         return dict(zip(keys, list(out)))
+
+    def colours(self):
+        out = np.zeros(self.n, np.uint8)
+        self.L.oracle_colours(self.h, out.ctypes.data_as(_abi().u8p))
+        return out
 
     def swept_aabbs(self):
         c = np.zeros((self.n, 3), np.float32)

@@ -251,3 +251,51 @@ def test_statistics_match():
         return np.maximum(0.0, 0.001 - c0[mask])
     dg, do = depths(gm), depths(om)
     assert np.array_equal(np.percentile(dg, [50, 99]), np.percentile(do, [50, 99]))
+
+
+def test_colours_valid_deterministic_and_equal_to_oracle():
+    """Colourings: the Jones-Plassmann result equals the oracle's sequential greedy colouring
+    and the greedy restatement, no joint or manifold joins two bodies of one colour, and a
+    second run gives the same colours."""
+    bodies, joints = scenes.chains(n_chains=32, links=16)
+    params = P(iterations=10)
+    params.beta_lin = 1.0e5
+    params.penalty_max = 1.0e9
+    runs = []
+    for rep in range(2):
+        w = gpu_world(bodies, joints, params)
+        o = oracle.OracleWorld(params) if rep == 0 else None
+        if o is not None:
+            o.upload(bodies, joints)
+        seq = []
+        for s in range(120):
+            w.step()
+            if o is not None:
+                o.step()
+            if (s + 1) % 20 == 0:
+                col = w.colours()
+                seq.append(col)
+                if o is not None:
+                    assert np.array_equal(col, o.colours()), f"step {s}"
+                    m = w.manifolds()
+                    assert np.array_equal(col, scenes.greedy_colours({**bodies, **w.download()}, joints, m))
+                    pairs = list(zip(joints["body_a"], joints["body_b"])) + list(zip(m["body_a"], m["body_b"]))
+                    for a, b in pairs:
+                        if a >= 0 and col[a] < 8 and col[b] < 8:
+                            assert col[a] != col[b]
+        runs.append(seq)
+    assert all(np.array_equal(x, y) for x, y in zip(*runs))
+
+
+def test_long_run_statistics_match_oracle():
+    """Long run (C1 pyramid, 300 steps): penetration-depth distribution, contact residual and
+    energy from the GPU state equal the oracle's (tolerance stated: 1e-6 relative; the
+    states are bit-identical, so they are equal)."""
+    bodies, joints = scenes.pyramid()
+    params = P(iterations=10)
+    w, o = lockstep(bodies, joints, params, 300, every=50)
+    gs = scenes.world_statistics({**bodies, **w.download()}, w.manifolds())
+    os_ = scenes.world_statistics({**bodies, **o.download()}, o.manifolds())
+    assert gs["contacts"] == os_["contacts"] and gs["contacts"] > 0
+    for k, v in os_.items():
+        assert abs(gs[k] - v) <= 1e-6 * max(1.0, abs(v)), (k, gs[k], v)

@@ -224,3 +224,21 @@ def test_static_pairs_make_no_manifold():
 def test_unsupported_parameters_rejected():
     with pytest.raises(ValueError):
         oracle.OracleWorld(P(sleep_frames=30))
+
+
+@pytest.mark.parametrize("scene,overflow", [("chains", True), ("pile", True), ("pile", False)])
+def test_colouring_matches_sequential_greedy(scene, overflow):
+    """The oracle's colouring equals greedy colouring restated from its definition (smallest
+    colour unused by lower-ordered neighbours, contact-only bodies to the Jacobi bucket)."""
+    if scene == "chains":
+        bodies, joints = scenes.chains(n_chains=8, links=12)
+        steps = 90
+    else:
+        bodies, joints = scenes.pile(n=300, seed=5)
+        steps = 100
+    params = P(iterations=10)
+    params.contact_only_overflow = 1 if overflow else 0
+    o = run(bodies, joints, params, steps)
+    state = {**bodies, **o.download()}
+    ref = scenes.greedy_colours(state, joints, o.manifolds(), contact_only_overflow=overflow)
+    assert np.array_equal(o.colours(), ref)
