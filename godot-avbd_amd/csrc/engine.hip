@@ -284,7 +284,7 @@ struct Engine {
         AL(S.wColourLeft, W);
         AL(S.mortonCode, N); AL(S.sortKey, N); AL(S.sortKeyAlt, N); AL(S.sortVal, N); AL(S.sortValAlt, N);
         AL(S.nodeLeft, N); AL(S.nodeRight, N); AL(S.nodeParent, 2 * N); AL(S.nodeFirst, N); AL(S.nodeLast, N);
-        AL(S.nodeFlag, N); AL(S.nodeBox, 4 * N); AL(S.nodeRope, 2 * N); AL(S.leafBody, N); AL(S.bodyLeaf, N);
+        AL(S.nodeFlag, N); AL(S.nodeBox, 4 * N); AL(S.nodeRope, 2 * N); AL(S.taskFlag, 2 * N + 1); AL(S.taskScan, 2 * N + 1); AL(S.taskNode, 2 * N); AL(S.leafBody, N); AL(S.bodyLeaf, N);
         AL(S.pairCount, N + 1); AL(S.pairOff, N + 1); AL(S.pairI, P); AL(S.pairJ, P);
         AL(S.pairValid, P + 1); AL(S.pairValidScan, P + 1);
         AL(S.pairScratch, N * PAIR_BUF); AL(S.wRoot, W);
@@ -386,6 +386,15 @@ struct Engine {
         return false;
     }
 
+    // Refit task roots of the tree just built (subtrees of at most REFIT_LEAVES leaves).
+    void buildRefitTasks()
+    {
+        k_task_flags<<<nblk(2 * Ncap - 1), TPB, 0, stream>>>(S, Ncap);
+        scan(S.taskFlag, S.taskScan, 2 * Ncap - 1);
+        k_task_list<<<nblk(2 * Ncap - 1), TPB, 0, stream>>>(S, Ncap);
+    }
+    int refitGrid() const { return std::max(1, std::min(Wcap + 2 * Ncap / REFIT_LEAVES + 8, 8192)); }
+
     // One velocity brake at the current poses: flags, dependency links, level-by-level
     // brake per world (solver_projection.cpp:96-167).
     void velocityBrake(float window, int countActive)
@@ -437,11 +446,12 @@ struct Engine {
             k_world_roots<<<nblk(Ncap - 1), TPB, 0, stream>>>(S, Ncap);
             k_world_roots_check<<<nblk(Wcap), TPB, 0, stream>>>(S);
             k_ropes<<<nblk(2 * Ncap - 1), TPB, 0, stream>>>(S, Ncap);
+            buildRefitTasks();
         } else {
             k_refit_clear<<<nblk(Ncap), TPB, 0, stream>>>(S, Ncap);
         }
         pbeg(AVBD_K_REFIT);
-        k_refit_world<<<Wcap, REFIT_TPB, 0, stream>>>(S, Ncap);
+        k_refit_tasks<<<refitGrid(), REFIT_TPB, 0, stream>>>(S, Ncap);
         pend();
         pbeg(AVBD_K_TRAVERSE);
         k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap);
@@ -474,9 +484,9 @@ struct Engine {
         k_colour_joints<<<nblk(Jcap), TPB, 0, stream>>>(S);
         k_colour_init<<<nb, TPB, 0, stream>>>(S);
         if (withColours) {
-            for (int r = 0; r + 1 < colourRounds; r++) k_colour_round<<<nb, TPB, 0, stream>>>(S);
+            for (int r = 0; r + 1 < colourRounds; r++) k_colour_round<<<nb, TPB, 0, stream>>>(S, 0);
             k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wColourLeft, 0, Wcap);
-            k_colour_round<<<nb, TPB, 0, stream>>>(S);
+            k_colour_round<<<nb, TPB, 0, stream>>>(S, 1);
             k_colour_finish<<<nblk(Wcap, 64), 64, 0, stream>>>(S);
         }
         k_colour_override<<<nb, TPB, 0, stream>>>(S);
@@ -1047,7 +1057,8 @@ struct Engine {
         k_world_roots<<<nblk(Ncap - 1), TPB, 0, stream>>>(S, Ncap);
         k_world_roots_check<<<nblk(Wcap), TPB, 0, stream>>>(S);
         k_ropes<<<nblk(2 * Ncap - 1), TPB, 0, stream>>>(S, Ncap);
-        k_refit_world<<<Wcap, REFIT_TPB, 0, stream>>>(S, Ncap);
+        buildRefitTasks();
+        k_refit_tasks<<<refitGrid(), REFIT_TPB, 0, stream>>>(S, Ncap);
         k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap);
         scan(S.pairCount, S.pairOff, Ncap);
         k_pair_emit<<<nb, TPB, 0, stream>>>(S, Ncap);

@@ -350,43 +350,98 @@ __global__ void k_karras(State S, int n)
     S.nodeFlag[i] = 0;
 }
 
-// Leaf boxes (centre -/+ half, the reference's bounds) and bottom-up refit of each world's
-// subtree, one workgroup per world.  The second child to arrive at a parent writes the
-// union and climbs on; arrivals are counted with workgroup-scope atomics and ordered with
-// workgroup-scope fences (a device-scope fence per thread would write back the L2 of every
-// XCD).  Queries start at world roots, so the levels above them are never read.
+// Refit tasks: a world's tree is cut into subtrees of at most REFIT_LEAVES leaves (the task
+// roots: nodes of that size whose parent is larger, or the world's root), chosen at each tree
+// build.  One workgroup refits a task's subtree bottom up: leaf boxes (centre -/+ half, the
+// reference's bounds), and the second child to arrive at a parent writes the union and climbs
+// on; arrivals are counted with workgroup-scope atomics and ordered with workgroup-scope
+// fences (a device-scope fence per thread would write back the L2 of every XCD).  Above the
+// task roots (worlds larger than REFIT_LEAVES only) one thread per task climbs on with
+// device-scope arrivals, so those few nodes pay the device-scope ordering.  Queries start at
+// world roots, so the levels above them are never read.
 constexpr int REFIT_TPB = 1024;
-__global__ void __launch_bounds__(REFIT_TPB) k_refit_world(State S, int n)
+constexpr int REFIT_LEAVES = 2048;
+
+__device__ __forceinline__ int nodeLeaves(const State &S, int n, int x)
 {
-    const int w = S.wOrder[blockIdx.x];
-    const int lo = S.wBodyBase[w], cnt = S.wBodyCount[w];
-    if (cnt <= 0) return;
-    const int root = S.wRoot[w];
-    for (int p = lo + threadIdx.x; p < lo + cnt; p += REFIT_TPB) {
-        const int body = S.sortVal[p];
-        S.leafBody[p] = body;
-        S.bodyLeaf[body] = p;
-        const v3 c = ld3(S.aabbC[body]), h = ld3(S.aabbH[body]);
-        int node = n - 1 + p;
-        S.nodeBox[2 * node] = st3(c - h, __int_as_float(body));
-        S.nodeBox[2 * node + 1] = st3(c + h, __int_as_float(S.nodeRope[node]));
-        if (node == root) continue;
-        __threadfence_block();
-        int parent = S.nodeParent[node];
-        while (true) {
-            if (__hip_atomic_fetch_add(&S.nodeFlag[parent], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
-                break;
+    return x >= n - 1 ? 1 : S.nodeLast[x] - S.nodeFirst[x] + 1;
+}
+
+__global__ void k_task_flags(State S, int n)
+{
+    const int x = GID;
+    if (x >= 2 * n - 1) return;
+    const int first = x >= n - 1 ? x - (n - 1) : S.nodeFirst[x];
+    const int last = x >= n - 1 ? first : S.nodeLast[x];
+    const int w = S.bodyWorld[S.sortVal[first]];
+    uint32_t flag = 0;
+    if (w >= 0 && S.wRoot[w] >= 0 && first >= S.wBodyBase[w] && last <= S.wBodyBase[w] + S.wBodyCount[w] - 1 &&
+        last - first + 1 <= REFIT_LEAVES)
+        flag = (x == S.wRoot[w] || nodeLeaves(S, n, S.nodeParent[x]) > REFIT_LEAVES) ? 1u : 0u;
+    S.taskFlag[x] = flag;
+}
+
+__global__ void k_task_list(State S, int n)
+{
+    const int x = GID;
+    if (x >= 2 * n - 1) return;
+    if (S.taskFlag[x]) S.taskNode[S.taskScan[x]] = x;
+}
+
+__device__ __forceinline__ void unionInto(const State &S, int parent)
+{
+    const int l = S.nodeLeft[parent], r = S.nodeRight[parent];
+    const float4 a0 = S.nodeBox[2 * l], a1 = S.nodeBox[2 * l + 1];
+    const float4 b0 = S.nodeBox[2 * r], b1 = S.nodeBox[2 * r + 1];
+    S.nodeBox[2 * parent] = make_float4(smin(a0.x, b0.x), smin(a0.y, b0.y), smin(a0.z, b0.z), __int_as_float(l));
+    S.nodeBox[2 * parent + 1] = make_float4(smax(a1.x, b1.x), smax(a1.y, b1.y), smax(a1.z, b1.z),
+                                            __int_as_float(S.nodeRope[parent]));
+}
+
+__global__ void __launch_bounds__(REFIT_TPB) k_refit_tasks(State S, int n)
+{
+    const int nTasks = static_cast<int>(S.taskScan[2 * n - 1]);
+    for (int t = blockIdx.x; t < nTasks; t += gridDim.x) {
+        const int root = S.taskNode[t];
+        const int lo = root >= n - 1 ? root - (n - 1) : S.nodeFirst[root];
+        const int cnt = nodeLeaves(S, n, root);
+        for (int p = lo + threadIdx.x; p < lo + cnt; p += REFIT_TPB) {
+            const int body = S.sortVal[p];
+            S.leafBody[p] = body;
+            S.bodyLeaf[body] = p;
+            const v3 c = ld3(S.aabbC[body]), h = ld3(S.aabbH[body]);
+            const int node = n - 1 + p;
+            S.nodeBox[2 * node] = st3(c - h, __int_as_float(body));
+            S.nodeBox[2 * node + 1] = st3(c + h, __int_as_float(S.nodeRope[node]));
+            if (node == root) continue;
             __threadfence_block();
-            const int l = S.nodeLeft[parent], r = S.nodeRight[parent];
-            const float4 a0 = S.nodeBox[2 * l], a1 = S.nodeBox[2 * l + 1];
-            const float4 b0 = S.nodeBox[2 * r], b1 = S.nodeBox[2 * r + 1];
-            S.nodeBox[2 * parent] = make_float4(smin(a0.x, b0.x), smin(a0.y, b0.y), smin(a0.z, b0.z), __int_as_float(l));
-            S.nodeBox[2 * parent + 1] = make_float4(smax(a1.x, b1.x), smax(a1.y, b1.y), smax(a1.z, b1.z),
-                                                    __int_as_float(S.nodeRope[parent]));
-            __threadfence_block();
-            if (parent == root) break;
-            parent = S.nodeParent[parent];
+            int parent = S.nodeParent[node];
+            while (true) {
+                if (__hip_atomic_fetch_add(&S.nodeFlag[parent], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+                    break;
+                __threadfence_block();
+                unionInto(S, parent);
+                __threadfence_block();
+                if (parent == root) break;
+                parent = S.nodeParent[parent];
+            }
         }
+        __syncthreads();
+        const int w = S.bodyWorld[S.sortVal[lo]];
+        if (threadIdx.x == 0 && root != S.wRoot[w]) {
+            int node = root;
+            __threadfence();
+            while (node != S.wRoot[w]) {
+                const int parent = S.nodeParent[node];
+                if (__hip_atomic_fetch_add(&S.nodeFlag[parent], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                    break;
+                __threadfence();
+                unionInto(S, parent);
+                __threadfence();
+                node = parent;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1089,19 +1144,22 @@ __device__ __forceinline__ bool tryColour(const State &S, int i)
     __hip_atomic_store(&S.colour[i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return true;
 }
-__global__ void k_colour_round(State S)
+// `count`: the last round also flags the worlds that still have uncoloured bodies (a plain
+// store; many bodies of one world may be left).
+__global__ void k_colour_round(State S, int count)
 {
     const int i = GID;
     if (i >= S.numBodies) return;
     if (S.colour[i] != 0xFF) return;
-    if (!tryColour(S, i)) atomicAdd(&S.wColourLeft[S.bodyWorld[i]], 1);
+    if (!tryColour(S, i) && count) S.wColourLeft[S.bodyWorld[i]] = 1;
 }
-// Bodies left after the bounded rounds are finished in list order by one thread per world.
+// Bodies left after the bounded rounds (counted by the last round) are finished in list
+// order by one thread per world.
 __global__ void k_colour_finish(State S)
 {
     const int w = GID;
     if (w >= S.numWorlds) return;
-    if (!S.wHasJoints[w]) return;
+    if (!S.wHasJoints[w] || S.wColourLeft[w] == 0) return;
     const int base = S.wBodyBase[w], cnt = S.wBodyCount[w];
     for (int k = 0; k < cnt; k++)
         if (S.colour[base + k] == 0xFF) tryColour(S, base + k);

@@ -126,6 +126,8 @@ struct State {
     // {min, left child id | leaf: body}, {max, rope = next node after this subtree, -1 = end}
     float4 *nodeBox;
     int *nodeRope;
+    uint32_t *taskFlag, *taskScan;   // refit task roots (see k_task_flags), per node
+    int *taskNode;
     int *leafBody;               // sorted position -> body
     int *bodyLeaf;               // body -> sorted position
     uint32_t *pairCount, *pairOff;
