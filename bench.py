@@ -227,7 +227,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one (for tracing tools)")
     ap.add_argument("--no-c3", action="store_true", help="skip the C3 chains world")
-    ap.add_argument("--c3-warmup", type=int, default=60)
+    ap.add_argument("--c3-warmup", type=int, default=240)
     ap.add_argument("--c3-steps", type=int, default=30)
     ap.add_argument("--no-serve", action="store_true", help="skip the C5 serving loop")
     ap.add_argument("--serve-loads", type=float, nargs="+", default=[0.5, 1.0, 2.0])
