@@ -131,6 +131,7 @@ def bench_chains(args, log):
     w.upload(bodies, joints)
     for _ in range(args.c3_warmup):
         w.step()
+    tuning = w.autotune(args.tune_steps) if args.tune_steps > 0 else None
     a = time.perf_counter()
     for _ in range(args.c3_steps):
         w.step()
@@ -142,7 +143,7 @@ def bench_chains(args, log):
     return {"workload": "C3 plain world: 4096 chains x 32 unit boxes, ball joints, static anchors, ground; "
                         "beta_lin 1e5, k cap 1e9", "bodies": len(bodies["mass"]), "movable_bodies": movable,
             "joints": len(joints["body_b"]), "manifolds": st["manifolds"], "contacts": st["contacts"],
-            "warmup": args.c3_warmup, "steps": args.c3_steps, "ms_per_step": dt * 1e3,
+            "warmup": args.c3_warmup, "steps": args.c3_steps, "ms_per_step": dt * 1e3, "tuning": tuning,
             "value": movable / dt, "unit": "body-steps/s"}
 
 
@@ -226,6 +227,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one (for tracing tools)")
+    ap.add_argument("--tune-steps", type=int, default=2, help="autotuner steps per trial (0: keep defaults)")
     ap.add_argument("--no-c3", action="store_true", help="skip the C3 chains world")
     ap.add_argument("--c3-warmup", type=int, default=240)
     ap.add_argument("--c3-steps", type=int, default=30)
@@ -268,6 +270,9 @@ def main():
         batch.step()
         if (s + 1) % 30 == 0:
             log(f"warmup step {s + 1}/{args.warmup}")
+    tuning = batch.autotune(args.tune_steps) if args.tune_steps > 0 else None
+    if tuning:
+        log(f"autotuned launch shapes ({6 * args.tune_steps} trial steps): {tuning}")
     tot = batch.totals()
     movable = tot["movable_bodies"]
 
@@ -422,6 +427,7 @@ def main():
         "kernel_ms": prof["avg_ms"],
         "kernel_launches_per_step": {k: v / max(1, prof["steps"]) for k, v in prof["launches"].items()},
         "cpu_baseline": cpu,
+        "tuning": tuning,
         "c3": c3,
         "serving": serving,
     }

@@ -26,6 +26,14 @@ u8p = C.POINTER(C.c_uint8)
 u64p = C.POINTER(C.c_uint64)
 
 
+class Tuning(C.Structure):
+    _fields_ = [("block_dual", C.c_int32), ("block_primal", C.c_int32), ("block_stamp", C.c_int32),
+                ("block_projection", C.c_int32), ("block_traverse", C.c_int32), ("live_blocks", C.c_int32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class Params(C.Structure):
     _fields_ = [
         ("dt", C.c_float), ("gravity", C.c_float), ("iterations", C.c_int32), ("alpha", C.c_float),
@@ -210,6 +218,8 @@ def load_library(path=LIB_PATH):
         "avbd_batch_create": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                           C.POINTER(vp)]),
         "avbd_batch_destroy": (C.c_int32, [vp]),
+        "avbd_batch_autotune": (C.c_int32, [vp, C.c_int32, C.POINTER(Tuning)]),
+        "avbd_world_autotune": (C.c_int32, [vp, C.c_int32, C.POINTER(Tuning)]),
         "avbd_batch_footprint": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                              C.POINTER(C.c_int64)]),
         "avbd_batch_device_bytes": (C.c_int32, [vp, C.POINTER(C.c_int64)]),

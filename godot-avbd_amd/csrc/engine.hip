@@ -338,11 +338,17 @@ struct Engine {
     // Grid for kernels that walk the compact live list: its length is only known on the
     // device, so they grid-stride over a fixed grid sized to keep every CU busy.
     // (a multiple of NUM_XCD: FOR_LIVE splits the list into one part per XCD)
-    int liveGrid() const
+    int liveGrid(int tpb = TPB) const
     {
-        const int g = std::min(nblk(std::max(Scap, Jcap)), LIVE_BLOCKS);
+        const int g = std::min(nblk(std::max(Scap, Jcap), tpb), std::max(1, tune.liveBlocks * TPB / tpb));
         return std::max(NUM_XCD, (g + NUM_XCD - 1) / NUM_XCD * NUM_XCD);
     }
+
+    // Block sizes of the hot kernels and the live-list grid, chosen by autotune() (results do
+    // not depend on them: every kernel's work per element is independent of its launch shape).
+    struct Tuning {
+        int dual = TPB, primal = TPB, stamp = TPB, proj = TPB, traverse = TPB, liveBlocks = LIVE_BLOCKS;
+    } tune;
 
     // gLive = every world's live positions, world after world (wLiveOff = exclusive scan of wLive).
     void buildLiveList()
@@ -355,8 +361,7 @@ struct Engine {
     // Dense contact layout for the current live list (staging -> cRA...).
     void packContacts()
     {
-        k_contact_count<<<liveGrid(), TPB, 0, stream>>>(S);
-        scan(S.cCount, S.cScan, Scap, S.wLiveOff + Wcap);
+        scan(S.cCount, S.cScan, Scap, S.wLiveOff + Wcap);   // counts written by k_manifold_order
         k_contact_pack<<<liveGrid(), TPB, 0, stream>>>(S);
     }
 
@@ -454,7 +459,7 @@ struct Engine {
         k_refit_tasks<<<refitGrid(), REFIT_TPB, 0, stream>>>(S, Ncap);
         pend();
         pbeg(AVBD_K_TRAVERSE);
-        k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap);
+        k_traverse<<<nblk(Ncap, tune.traverse), tune.traverse, 0, stream>>>(S, Ncap);
         pend();
         scan(S.pairCount, S.pairOff, Ncap);
         k_pair_emit<<<nb, TPB, 0, stream>>>(S, Ncap);
@@ -538,12 +543,12 @@ struct Engine {
                     if (stampHere) {
                         k_pose_pack<<<nb, TPB, 0, stream>>>(S);
                         pbeg(AVBD_K_STAMP);
-                        k_stamp<<<liveGrid(), TPB, 0, stream>>>(S);
+                        k_stamp<<<liveGrid(tune.stamp), tune.stamp, 0, stream>>>(S);
                         pend();
                     }
                     pbeg(AVBD_K_PRIMAL);
                     if (withColours) {
-                        k_overflow_solve<false><<<nb, TPB, 0, stream>>>(S);
+                        k_overflow_solve<false><<<nblk(Ncap, tune.primal), tune.primal, 0, stream>>>(S);
                         pend();
                         k_solve_fallback<<<1, TPB, 0, stream>>>(S, 0);
                         pbeg(AVBD_K_BODY);
@@ -552,16 +557,16 @@ struct Engine {
                         k_pose_pack<<<nb, TPB, 0, stream>>>(S);
                     } else {
                         // no joints: bodies move inside the solve and refresh their pose records
-                        k_overflow_solve<true><<<nb, TPB, 0, stream>>>(S);
+                        k_overflow_solve<true><<<nblk(Ncap, tune.primal), tune.primal, 0, stream>>>(S);
                         pend();
                         k_solve_fallback<<<1, TPB, 0, stream>>>(S, 2);
                     }
                 }
                 pbeg(AVBD_K_DUAL);
                 if (stampInDual)
-                    k_dual<true><<<liveGrid(), TPB, 0, stream>>>(S, newton ? 1.0f : prm.stiffness_decay);
+                    k_dual<true><<<liveGrid(tune.dual), tune.dual, 0, stream>>>(S, newton ? 1.0f : prm.stiffness_decay);
                 else
-                    k_dual<false><<<liveGrid(), TPB, 0, stream>>>(S, newton ? 1.0f : prm.stiffness_decay);
+                    k_dual<false><<<liveGrid(tune.dual), tune.dual, 0, stream>>>(S, newton ? 1.0f : prm.stiffness_decay);
                 pend();
             }
             pbeg(AVBD_K_BODY);
@@ -579,7 +584,7 @@ struct Engine {
             k_proj_reset<<<nblk(Wcap), TPB, 0, stream>>>(S);
             for (int pass = 0; pass < prm.projection_iterations; pass++) {
                 pbeg(AVBD_K_PROJ_MANIFOLD);
-                k_proj_manifold<<<liveGrid(), TPB, 0, stream>>>(S);
+                k_proj_manifold<<<liveGrid(tune.proj), tune.proj, 0, stream>>>(S);
                 pend();
                 pbeg(AVBD_K_PROJ_BODY);
                 k_proj_body<<<nb, TPB, 0, stream>>>(S);
@@ -662,6 +667,56 @@ struct Engine {
         for (int k = 0; k < AVBD_K_CLASSES; k++) {
             out->launches[k] = static_cast<int32_t>(profLaunches[k]);
             out->avg_ms[k] = profLaunches[k] ? static_cast<float>(profMs[k] / profLaunches[k]) : 0.0f;
+        }
+        return AVBD_OK;
+    }
+
+    // Measure candidate launch shapes on real steps (the steps are part of the simulation:
+    // results do not depend on the shapes) and keep the fastest per kernel class.
+    avbd_status autotune(int stepsPerTrial, avbd_tuning *out)
+    {
+        if (stepsPerTrial < 1) return fail(AVBD_ERR_INVALID, "steps_per_trial must be >= 1");
+        const int sizes[3] = {64, 128, 256};
+        float t[3][AVBD_K_CLASSES];
+        avbd_kernel_profile prof;
+        for (int c = 0; c < 3; c++) {
+            tune.dual = tune.primal = tune.stamp = tune.proj = tune.traverse = sizes[c];
+            avbd_status st = profile(stepsPerTrial, 1, 0, &prof);
+            if (st != AVBD_OK) return st;
+            for (int k = 0; k < AVBD_K_CLASSES; k++) t[c][k] = prof.avg_ms[k];
+        }
+        auto best = [&](int cls) {
+            int bi = 2;
+            for (int c = 0; c < 3; c++)
+                if (t[c][cls] > 0.0f && t[c][cls] < t[bi][cls]) bi = c;
+            return sizes[bi];
+        };
+        tune.dual = best(AVBD_K_DUAL);
+        tune.primal = best(AVBD_K_PRIMAL);
+        tune.stamp = best(AVBD_K_STAMP);
+        tune.proj = best(AVBD_K_PROJ_MANIFOLD);
+        tune.traverse = best(AVBD_K_TRAVERSE);
+        const int grids[3] = {1024, 2048, 4096};
+        float bestT = 0.0f;
+        int bestG = LIVE_BLOCKS;
+        for (int g : grids) {
+            tune.liveBlocks = g;
+            avbd_status st = profile(stepsPerTrial, 1, 0, &prof);
+            if (st != AVBD_OK) return st;
+            const float tt = prof.avg_ms[AVBD_K_DUAL] * prof.launches[AVBD_K_DUAL] +
+                             prof.avg_ms[AVBD_K_STAMP] * prof.launches[AVBD_K_STAMP] +
+                             prof.avg_ms[AVBD_K_PROJ_MANIFOLD] * prof.launches[AVBD_K_PROJ_MANIFOLD];
+            if (bestT == 0.0f || tt < bestT) { bestT = tt; bestG = g; }
+        }
+        tune.liveBlocks = bestG;
+        graphsValid = false;
+        if (out) {
+            out->block_dual = tune.dual;
+            out->block_primal = tune.primal;
+            out->block_stamp = tune.stamp;
+            out->block_projection = tune.proj;
+            out->block_traverse = tune.traverse;
+            out->live_blocks = tune.liveBlocks;
         }
         return AVBD_OK;
     }
@@ -1433,6 +1488,19 @@ avbd_status avbd_batch_create(const avbd_params *params, int32_t max_worlds, int
     }
     *out = b;
     return AVBD_OK;
+}
+
+avbd_status avbd_batch_autotune(avbd_batch *batch, int32_t steps_per_trial, avbd_tuning *out)
+{
+    if (!batch) return fail(AVBD_ERR_INVALID, "null argument");
+    return batch->eng.autotune(steps_per_trial, out);
+}
+
+avbd_status avbd_world_autotune(avbd_world *world, int32_t steps_per_trial, avbd_tuning *out)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "null argument");
+    if (!world->eng.worlds[0].active) return fail(AVBD_ERR_INVALID, "nothing uploaded");
+    return world->eng.autotune(steps_per_trial, out);
 }
 
 avbd_status avbd_batch_footprint(const avbd_params *params, int32_t max_worlds, int32_t max_bodies,

@@ -652,8 +652,10 @@ __global__ void __launch_bounds__(ORDER_TPB) k_manifold_order(State S)
         if (valid) {
             uint32_t off = start[v];
             for (int q = 0; q < wave; q++) off += wcnt[q][v];
-            S.gOfK[base + static_cast<int>(off + rank)] = base + i;
-            S.kSlot[base + static_cast<int>(off + rank)] = slot;
+            const int k = base + static_cast<int>(off + rank);
+            S.gOfK[k] = base + i;
+            S.kSlot[k] = slot;
+            S.cCount[k] = v;   // contact count in view order (scanned into dense offsets)
         }
         __syncthreads();
         if (tid < NB) {
@@ -804,25 +806,16 @@ __global__ void k_manifold_view(State S)
     }
 }
 
-// Contact counts in gLive order (zero past the end, so a scan over the whole capacity
-// gives each manifold's dense offset).
-__global__ void k_contact_count(State S)
-{
-    const int total = static_cast<int>(S.wLiveOff[S.numWorlds]);
-    for (int k = GID; k < S.slotCap; k += gridDim.x * blockDim.x)
-        S.cCount[k] = k < total ? static_cast<uint32_t>(S.mNum[S.kSlot[k]]) : 0u;
-}
-
-// Staging -> dense arrays at the scanned offsets: eight lanes per manifold, one per contact
-// row, so each field's rows of a manifold move as one contiguous piece.
+// Staging -> dense arrays at the scanned offsets: four lanes per manifold, lane q moving
+// contact rows q and q + 4, so each field's rows of a manifold move as contiguous pieces.
 __global__ void k_contact_pack(State S)
 {
-    FOR_RANGE(r, MAXC * static_cast<int>(S.wLiveOff[S.numWorlds])) {
-        const int k = r / MAXC, i = r % MAXC;
+    FOR_RANGE(r, 4 * static_cast<int>(S.wLiveOff[S.numWorlds])) {
+        const int k = r >> 2, q = r & 3;
         const int slot = S.kSlot[k];
-        const int n = S.mNum[slot];
+        const int n = static_cast<int>(S.cCount[k]);
         const uint32_t o = S.cScan[k];
-        if (i < n) {
+        for (int i = q; i < n; i += 4) {
             const size_t d = static_cast<size_t>(slot) * MAXC + i;
             const float4 a = S.sRA[d], b = S.sRB[d], c = S.sC0[d], p = S.sPen[d], l = S.sLam[d];
             S.cRA[o + i] = a;
@@ -831,7 +824,7 @@ __global__ void k_contact_pack(State S)
             S.cPen[o + i] = p;
             S.cLam[o + i] = l;
         }
-        if (i == 0) {
+        if (q == 0) {
             S.cOff[slot] = o;
             manifoldView(S, k, slot, o, n);
         }

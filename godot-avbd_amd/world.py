@@ -118,6 +118,11 @@ class World:
         abi.check(self.lib.avbd_world_profile(self.h, int(steps), C.byref(out)))
         return out.as_dict()
 
+    def autotune(self, steps_per_trial=2):
+        out = abi.Tuning()
+        abi.check(self.lib.avbd_world_autotune(self.h, int(steps_per_trial), C.byref(out)))
+        return out.as_dict()
+
 
 class Batch:
     """Many independent worlds stepped together (avbd_batch_*)."""
@@ -183,6 +188,11 @@ class Batch:
     def set_manifolds(self, wid, arrays):
         ms = abi.manifolds_struct(arrays)
         abi.check(self.lib.avbd_batch_set_manifolds(self.h, int(wid), C.byref(ms)))
+
+    def autotune(self, steps_per_trial=2):
+        out = abi.Tuning()
+        abi.check(self.lib.avbd_batch_autotune(self.h, int(steps_per_trial), C.byref(out)))
+        return out.as_dict()
 
     def profile(self, steps):
         out = abi.KernelProfile()

@@ -258,6 +258,16 @@ avbd_status avbd_batch_create(const avbd_params *params, int32_t max_worlds,
                               int32_t max_bodies, int32_t max_joints, int32_t max_manifolds,
                               avbd_batch **out);
 avbd_status avbd_batch_destroy(avbd_batch *batch);
+/* Launch shapes of the hot kernels (threads per workgroup; workgroups of the live-manifold
+ * walks).  Results do not depend on them. */
+typedef struct avbd_tuning {
+    int32_t block_dual, block_primal, block_stamp, block_projection, block_traverse, live_blocks;
+} avbd_tuning;
+/* Autotuner: steps the batch (or world) 6 x steps_per_trial times with candidate launch shapes,
+ * timing each kernel class with HIP events, and keeps the fastest per class.  The steps are
+ * real steps of the simulation.  No reference counterpart (the reference runs on the CPU). */
+avbd_status avbd_batch_autotune(avbd_batch *batch, int32_t steps_per_trial, avbd_tuning *out);
+avbd_status avbd_world_autotune(avbd_world *world, int32_t steps_per_trial, avbd_tuning *out);
 /* Device bytes a batch of these capacities allocates (computed on the host; no device needed),
  * and the bytes an existing batch holds.  A serving loop sizes its batch from an HBM budget
  * with the first and admits worlds while their bodies, joints and manifold slots fit. */

@@ -299,3 +299,25 @@ def test_long_run_statistics_match_oracle():
     assert gs["contacts"] == os_["contacts"] and gs["contacts"] > 0
     for k, v in os_.items():
         assert abs(gs[k] - v) <= 1e-6 * max(1.0, abs(v)), (k, gs[k], v)
+
+
+def test_autotune_keeps_results():
+    """The autotuner's trial steps are ordinary steps under other launch shapes: a world tuned
+    mid-run ends bit-identical to one that is not, and to the oracle."""
+    bodies, joints = scenes.pile(n=400, seed=21, ground=80.0)
+    params = P(iterations=10)
+    a = gpu_world(bodies, joints, params)
+    b = gpu_world(bodies, joints, params)
+    o = oracle.OracleWorld(params)
+    o.upload(bodies, joints)
+    for _ in range(10):
+        a.step()
+    tuning = a.autotune(1)           # 6 trial steps
+    assert all(v > 0 for v in tuning.values())
+    for _ in range(10):
+        a.step()
+    for _ in range(26):
+        b.step()
+        o.step()
+    assert not state_mismatch(a.download(), b.download())
+    assert not state_mismatch(a.download(), o.download())
