@@ -284,7 +284,7 @@ struct Engine {
         AL(S.wColourLeft, W);
         AL(S.mortonCode, N); AL(S.sortKey, N); AL(S.sortKeyAlt, N); AL(S.sortVal, N); AL(S.sortValAlt, N);
         AL(S.nodeLeft, N); AL(S.nodeRight, N); AL(S.nodeParent, 2 * N); AL(S.nodeFirst, N); AL(S.nodeLast, N);
-        AL(S.nodeFlag, N); AL(S.nodeBox, 4 * N); AL(S.nodeRope, 2 * N); AL(S.taskFlag, 2 * N + 1); AL(S.taskScan, 2 * N + 1); AL(S.taskNode, 2 * N); AL(S.leafBody, N); AL(S.bodyLeaf, N);
+        AL(S.nodeFlag, N); AL(S.nodeBox, 4 * N); AL(S.nodeRope, 2 * N); AL(S.taskFlag, 2 * N + 1); AL(S.taskScan, 2 * N + 1); AL(S.taskNode, 2 * N); AL(S.colourCnt, N); AL(S.colourQueue, N); AL(S.leafBody, N); AL(S.bodyLeaf, N);
         AL(S.pairCount, N + 1); AL(S.pairOff, N + 1); AL(S.pairI, P); AL(S.pairJ, P);
         AL(S.pairValid, P + 1); AL(S.pairValidScan, P + 1);
         AL(S.pairScratch, N * PAIR_BUF); AL(S.wRoot, W);
@@ -492,7 +492,8 @@ struct Engine {
             for (int r = 0; r + 1 < colourRounds; r++) k_colour_round<<<nb, TPB, 0, stream>>>(S, 0);
             k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wColourLeft, 0, Wcap);
             k_colour_round<<<nb, TPB, 0, stream>>>(S, 1);
-            k_colour_finish<<<nblk(Wcap, 64), 64, 0, stream>>>(S);
+            k_colour_counts<<<nb, TPB, 0, stream>>>(S);
+            k_colour_finish<<<Wcap, COLOUR_TPB, 0, stream>>>(S);
         }
         k_colour_override<<<nb, TPB, 0, stream>>>(S);
         // ---- brake schedule and first brake (solver_projection.cpp:96-167)

@@ -1146,16 +1146,63 @@ __global__ void k_colour_round(State S, int count)
     if (S.colour[i] != 0xFF) return;
     if (!tryColour(S, i) && count) S.wColourLeft[S.bodyWorld[i]] = 1;
 }
-// Bodies left after the bounded rounds (counted by the last round) are finished in list
-// order by one thread per world.
-__global__ void k_colour_finish(State S)
+// Bodies left after the bounded rounds (worlds flagged by the last round) are finished by
+// dependency counting: a body is ready once every lower-ordered neighbour has its colour, and
+// ready bodies are coloured level by level (Kahn) in one workgroup per world.  Every body then
+// takes the colour the sequential greedy pass gives it.  The counts are per adjacency entry
+// (entries are symmetric between the two bodies), so each entry of a newly coloured body
+// releases one count of its higher-ordered neighbour.
+__global__ void k_colour_counts(State S)
 {
-    const int w = GID;
-    if (w >= S.numWorlds) return;
-    if (!S.wHasJoints[w] || S.wColourLeft[w] == 0) return;
+    const int i = GID;
+    if (i >= S.numBodies) return;
+    const int w = S.bodyWorld[i];
+    if (w < 0 || !S.wColourLeft[w] || S.colour[i] != 0xFF) return;
+    const int ui = S.updateIndex[i];
+    int cnt = 0;
+    for (int e = S.adjStart[i]; e < S.adjStart[i + 1]; e++) {
+        const int o = otherOf(S, S.adj[e], i);
+        if (o < 0) continue;
+        const int uo = S.updateIndex[o];
+        if (uo < 0 || uo >= ui) continue;
+        if (S.colour[o] == 0xFF) cnt++;
+    }
+    S.colourCnt[i] = cnt;
+}
+
+constexpr int COLOUR_TPB = 256;
+__global__ void __launch_bounds__(COLOUR_TPB) k_colour_finish(State S)
+{
+    const int w = S.wOrder[blockIdx.x];
+    if (!S.wHasJoints[w] || !S.wColourLeft[w]) return;
     const int base = S.wBodyBase[w], cnt = S.wBodyCount[w];
-    for (int k = 0; k < cnt; k++)
-        if (S.colour[base + k] == 0xFF) tryColour(S, base + k);
+    int *queue = S.colourQueue + base;
+    __shared__ int tail, head;
+    if (threadIdx.x == 0) { tail = 0; head = 0; }
+    __syncthreads();
+    for (int k = threadIdx.x; k < cnt; k += COLOUR_TPB)
+        if (S.colour[base + k] == 0xFF && S.colourCnt[base + k] == 0) queue[atomicAdd(&tail, 1)] = base + k;
+    __syncthreads();
+    while (true) {
+        const int h = head, e = tail;
+        __syncthreads();   // every thread holds this level's bounds before the next is appended
+        if (h == e) break;
+        for (int k = h + threadIdx.x; k < e; k += COLOUR_TPB) {
+            const int i = queue[k];
+            tryColour(S, i);
+            const int ui = S.updateIndex[i];
+            for (int a = S.adjStart[i]; a < S.adjStart[i + 1]; a++) {
+                const int o = otherOf(S, S.adj[a], i);
+                if (o < 0 || S.updateIndex[o] <= ui) continue;
+                if (__hip_atomic_load(&S.colour[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0xFF) continue;
+                if (atomicSub(&S.colourCnt[o], 1) == 1) queue[atomicAdd(&tail, 1)] = o;
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        if (threadIdx.x == 0) head = e;
+        __syncthreads();
+    }
 }
 // solver_solve.cpp:399-418: contact-only bodies form the Jacobi bucket (colour 8).  A
 // body whose greedy colour overflowed but touches a joint lands in neither the colour runs

@@ -128,6 +128,7 @@ struct State {
     int *nodeRope;
     uint32_t *taskFlag, *taskScan;   // refit task roots (see k_task_flags), per node
     int *taskNode;
+    int *colourCnt, *colourQueue;    // colour finisher: uncoloured lower neighbours, ready queue
     int *leafBody;               // sorted position -> body
     int *bodyLeaf;               // body -> sorted position
     uint32_t *pairCount, *pairOff;
