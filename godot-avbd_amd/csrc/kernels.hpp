@@ -694,11 +694,16 @@ __device__ void pairFilter(const State &S, int p)
     const v3 ci = ld3(S.aabbC[i]), hi = ld3(S.aabbH[i]), cj = ld3(S.aabbC[j]), hj = ld3(S.aabbH[j]);
     const bool ov = aabbOverlap(ci - hi, ci + hi, cj - hj, cj + hj);
     const bool layers = ((S.layer[i] & S.mask[j]) != 0) || ((S.layer[j] & S.mask[i]) != 0);
-    const bool bothStatic = bMass(S, i) <= 0.0f && bMass(S, j) <= 0.0f;
+    const bool staticI = bMass(S, i) <= 0.0f, staticJ = bMass(S, j) <= 0.0f;
+    const bool bothStatic = staticI && staticJ;
     bool constrained = false;
     if (ov && layers && !bothStatic) {
+        // A static body's list holds its joints only (manifold entries are kept for movable
+        // bodies), so a pair with a static body scans the movable body's list: every force
+        // between the two is in it.
         const int ni = S.adjStart[i + 1] - S.adjStart[i], nj = S.adjStart[j + 1] - S.adjStart[j];
-        const int a = ni <= nj ? i : j, b = ni <= nj ? j : i;
+        const bool useI = staticJ || (!staticI && ni <= nj);
+        const int a = useI ? i : j, b = useI ? j : i;
         for (int e = S.adjStart[a]; e < S.adjStart[a + 1]; e++)
             if (otherOf(S, S.adj[e], a) == b) { constrained = true; break; }
     }
@@ -1030,6 +1035,10 @@ __global__ void k_copy_live(State S)
 // ------------------------------------------------------------ adjacency ----
 // manifold.cpp:123-166: CSR per body, joints in list order then manifolds in live order.
 // Counts and fills use atomics; each segment is then sorted by its sequence number.
+// Manifold entries are made for movable bodies only: no pass reads a static body's manifold
+// entries (the solves, colouring, brake and projection gather movable bodies; the pair filter
+// scans the movable side), and a ground body touched by hundreds of manifolds would otherwise
+// serialise their atomics on one counter.
 __global__ void k_adj_count(State S)
 {
     for (int t = GID; t < S.jointCap; t += gridDim.x * blockDim.x) {
@@ -1040,8 +1049,8 @@ __global__ void k_adj_count(State S)
     }
     FOR_LIVE(k) {
         const int4 h = S.mHdr[k];
-        atomicAdd(&S.adjCount[h.x], 1);
-        atomicAdd(&S.adjCount[h.y], 1);
+        if (bMass(S, h.x) > 0.0f) atomicAdd(&S.adjCount[h.x], 1);
+        if (bMass(S, h.y) > 0.0f) atomicAdd(&S.adjCount[h.y], 1);
     }
 }
 
@@ -1066,8 +1075,9 @@ __global__ void k_adj_fill(State S)
         const unsigned long long e = (seq << 32) | static_cast<uint32_t>(k);
         const int4 h = S.mHdr[k];
         const int a = h.x, b = h.y;
-        S.adjTmp[S.adjStart[a] + atomicAdd(&S.adjCount[a], 1)] = e;
-        S.adjTmp[S.adjStart[b] + atomicAdd(&S.adjCount[b], 1)] = e | static_cast<uint32_t>(ADJ_SIDE_B);
+        if (bMass(S, a) > 0.0f) S.adjTmp[S.adjStart[a] + atomicAdd(&S.adjCount[a], 1)] = e;
+        if (bMass(S, b) > 0.0f)
+            S.adjTmp[S.adjStart[b] + atomicAdd(&S.adjCount[b], 1)] = e | static_cast<uint32_t>(ADJ_SIDE_B);
     }
 }
 
@@ -1593,26 +1603,43 @@ __global__ void k_proj_reset(State S)
 }
 
 // solver_projection.cpp:642-709: per active manifold, patch depth beyond the slop and the
-// per-side corrections (applied by the body gather below).
-__device__ void projManifold(const State &S, int k);
+// per-side corrections (applied by the body gather below).  The world's largest average
+// depth is a max over its manifolds: each wavefront reduces its lanes per world (the view is
+// world after world, so a world's lanes are one run) and one lane per run takes the atomic,
+// instead of every manifold contending for the world's counter.
+__device__ float projManifold(const State &S, int k, int &w);
 __global__ void __launch_bounds__(TPB) k_proj_manifold(State S)
 {
-    FOR_LIVE(k) projManifold(S, k);
+    FOR_LIVE(k) {
+        int w;
+        float d = projManifold(S, k, w);
+        // FOR_RANGE's active lanes are a prefix of the wavefront, so every lane read below is
+        // an active one
+        const int lane = static_cast<int>(threadIdx.x) & 63;
+        const int active = __popcll(__ballot(1));
+        for (int off = 1; off < 64; off <<= 1) {
+            const float od = __shfl_up(d, off);
+            const int ow = __shfl_up(w, off);
+            if (lane >= off && ow == w) d = fmaxf(d, od);
+        }
+        const int nw = __shfl_down(w, 1);
+        if ((lane == active - 1 || nw != w) && d > 0.0f) atomicMax(&S.wMaxDepth[w], __float_as_uint(d));
+    }
 }
-__device__ void projManifold(const State &S, int k)
+__device__ float projManifold(const State &S, int k, int &w)
 {
     const int4 hd = S.mHdr[k];
     const int a = hd.x, b = hd.y;
-    const int w = S.bodyWorld[a];
+    w = S.bodyWorld[a];
     const float ma = bMass(S, a), mb = bMass(S, b);
     const int ua = S.updateIndex[a], ub = S.updateIndex[b];
     const v3 n = ld3(S.mBasisK[3 * k]);
     const v3 pa = bPos(S, a), pb = bPos(S, b);
     const qt qa = bRot(S, a), qb = bRot(S, b);
-    if (S.wProjDone[w]) return;
+    if (S.wProjDone[w]) return 0.0f;
     S.projOn[k] = 0;
     const bool moveA = ma > 0.0f && ua >= 0, moveB = mb > 0.0f && ub >= 0;
-    if (!(hd.w > 0 && (moveA || moveB))) return;
+    if (!(hd.w > 0 && (moveA || moveB))) return 0.0f;
     const float imA = moveA ? 1.0f / ma : 0.0f, imB = moveB ? 1.0f / mb : 0.0f;
     const m3 rotA = qmat(qa), rotB = qmat(qb);
     v3 pRA{0, 0, 0}, pRB{0, 0, 0};
@@ -1631,10 +1658,9 @@ __device__ void projManifold(const State &S, int k)
             corr++;
         }
     }
-    if (corr == 0) return;
+    if (corr == 0) return 0.0f;
     const float invN = 1.0f / static_cast<float>(corr);
     const float avg = depthSum * invN;
-    if (avg > 0.0f) atomicMax(&S.wMaxDepth[w], __float_as_uint(avg));
     const v3 IA = bMoment(S, a), IB = bMoment(S, b);
     const v3 tA = moveA ? qrot(qconj(qa), cross(pRA * invN, n)) : v3{0, 0, 0};
     const v3 tB = moveB ? qrot(qconj(qb), cross(pRB * invN, n)) : v3{0, 0, 0};
@@ -1649,6 +1675,7 @@ __device__ void projManifold(const State &S, int k)
         S.projAB[k] = st3(angResponse(IB, qb, tB) * dp);
     }
     S.projOn[k] = 1;
+    return avg > 0.0f ? avg : 0.0f;
 }
 
 // solver_projection.cpp:711-730: each movable body averages its corrections, summed in
