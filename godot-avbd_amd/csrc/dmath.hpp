@@ -20,8 +20,13 @@ namespace avbd_dev {
 struct v3 {
     float x, y, z;
     __device__ __host__ float &operator[](int i) { return (&x)[i]; }
-    // select chain: a run-time index then stays in registers
-    __device__ __host__ float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    // select chain over values: a run-time index then stays in registers (a conditional over
+    // the members themselves selects their addresses, which keeps the vector in memory)
+    __device__ __host__ float operator[](int i) const
+    {
+        const float a = x, b = y, c = z;
+        return i == 0 ? a : (i == 1 ? b : c);
+    }
 };
 struct qt {
     float x, y, z, w;

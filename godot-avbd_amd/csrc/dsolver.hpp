@@ -717,8 +717,27 @@ AD OBB makeOBB(v3 c, qt q, v3 h)
     o.ax[2] = qrot(q, v3{0, 0, 1});
     return o;
 }
-// Axis i of a box by selection (a run-time index into b.ax would put the box in scratch).
-AD v3 axisOf(const OBB &b, int i) { return i == 0 ? b.ax[0] : (i == 1 ? b.ax[1] : b.ax[2]); }
+// Selections are made per float: a conditional between two aggregates selects their
+// addresses, which keeps both boxes in scratch memory instead of registers.
+AD v3 sel3(bool f, v3 a, v3 b)
+{
+    const float ax = a.x, ay = a.y, az = a.z, bx = b.x, by = b.y, bz = b.z;
+    return v3{f ? ax : bx, f ? ay : by, f ? az : bz};
+}
+AD OBB pickOBB(bool first, const OBB &x, const OBB &y)
+{
+    OBB o;
+    o.c = sel3(first, x.c, y.c);
+    const qt xq = x.q, yq = y.q;
+    o.q = qt{first ? xq.x : yq.x, first ? xq.y : yq.y, first ? xq.z : yq.z, first ? xq.w : yq.w};
+    o.h = sel3(first, x.h, y.h);
+    o.ax[0] = sel3(first, x.ax[0], y.ax[0]);
+    o.ax[1] = sel3(first, x.ax[1], y.ax[1]);
+    o.ax[2] = sel3(first, x.ax[2], y.ax[2]);
+    return o;
+}
+// Axis i of a box (a run-time index into b.ax would put the box in scratch).
+AD v3 axisOf(const OBB &b, int i) { return sel3(i == 0, b.ax[0], sel3(i == 1, b.ax[1], b.ax[2])); }
 AD v3 supportPoint(const OBB &b, v3 dir)
 {
     const float sx = dot(dir, b.ax[0]) >= 0.0f ? 1.0f : -1.0f;
@@ -726,21 +745,41 @@ AD v3 supportPoint(const OBB &b, v3 dir)
     const float sz = dot(dir, b.ax[2]) >= 0.0f ? 1.0f : -1.0f;
     return ((b.c + b.ax[0] * (b.h.x * sx)) + b.ax[1] * (b.h.y * sy)) + b.ax[2] * (b.h.z * sz);
 }
+// The two in-face axes of face i (i = 0: y, z; 1: x, z; 2: x, y) with their half extents.
 AD void faceAxes(const OBB &b, int i, v3 &u, v3 &v, float &eu, float &ev)
 {
-    if (i == 0) { u = b.ax[1]; v = b.ax[2]; eu = b.h.y; ev = b.h.z; }
-    else if (i == 1) { u = b.ax[0]; v = b.ax[2]; eu = b.h.x; ev = b.h.z; }
-    else { u = b.ax[0]; v = b.ax[1]; eu = b.h.x; ev = b.h.y; }
+    const float hx = b.h.x, hy = b.h.y, hz = b.h.z;
+    u = sel3(i == 0, b.ax[1], b.ax[0]);
+    v = sel3(i == 2, b.ax[1], b.ax[2]);
+    eu = i == 0 ? hy : hx;
+    ev = i == 2 ? hy : hz;
 }
 constexpr int MAX_POLY = 16;
-// A clip polygon in LDS, one lane's column: vertex v at x[v * NARROW_TPB] (the caller's
-// pointers are offset by the lane), so a wavefront's accesses are consecutive words.
+// A clip polygon of capacity MAX_POLY.  Clipping a quad by the four side planes adds at most
+// one vertex per plane, so the first POLY_LDS vertices are the ones used in practice: they
+// sit in LDS, one lane's column (vertex v at x[v * NARROW_TPB], the caller's pointers offset
+// by the lane, so a wavefront's accesses are consecutive words).  Vertices past POLY_LDS, which
+// only rounding could produce, go to `spill` (3 floats each), global memory the caller owns.
+// Half the LDS of a full-capacity column doubles the wavefronts a CU can hold.
 constexpr int NARROW_TPB = 64;
+constexpr int POLY_LDS = 8;
 struct Poly {
     float *x, *y, *z;
-    AD v3 get(int v) const { const int i = v * NARROW_TPB; return {x[i], y[i], z[i]}; }
-    AD void put(int v, v3 a) const { const int i = v * NARROW_TPB; x[i] = a.x; y[i] = a.y; z[i] = a.z; }
+    float *spill;
+    AD v3 get(int v) const
+    {
+        if (v < POLY_LDS) { const int i = v * NARROW_TPB; return {x[i], y[i], z[i]}; }
+        const float *s = spill + 3 * (v - POLY_LDS);
+        return {s[0], s[1], s[2]};
+    }
+    AD void put(int v, v3 a) const
+    {
+        if (v < POLY_LDS) { const int i = v * NARROW_TPB; x[i] = a.x; y[i] = a.y; z[i] = a.z; return; }
+        float *s = spill + 3 * (v - POLY_LDS);
+        s[0] = a.x; s[1] = a.y; s[2] = a.z;
+    }
 };
+static_assert(3 * (MAX_POLY - POLY_LDS) <= 4 * MAXC, "polygon overflow must fit one staging row set");
 // collide.cpp:170-203 (Sutherland-Hodgman against one plane, capacity MAX_POLY)
 AD int clipPoly(const Poly &in, int n, v3 pn, float off, const Poly &out)
 {
@@ -789,8 +828,10 @@ AD void addContact(qt qA, v3 cA, qt qB, v3 cB, Contacts &cs, v3 xA, v3 xB, int k
 AD void faceManifold(const OBB &bA, const OBB &bB, bool refIsA, int refAxis, v3 nAB, float margin, Contacts &cs,
                      const Poly &c0, const Poly &c1)
 {
-    const OBB &ref = refIsA ? bA : bB;
-    const OBB &inc = refIsA ? bB : bA;
+    // field-wise selects keep both boxes in registers (a selected reference would put them in
+    // scratch memory)
+    const OBB ref = pickOBB(refIsA, bA, bB);
+    const OBB inc = pickOBB(refIsA, bB, bA);
     const v3 refOut = refIsA ? nAB : -nAB;
     const float sgn = dot(refOut, axisOf(ref, refAxis)) >= 0.0f ? 1.0f : -1.0f;
     const v3 fn = axisOf(ref, refAxis) * sgn;

@@ -471,7 +471,10 @@ struct Engine {
         // ---- warm start forces / narrow phase (solver_solve.cpp:31-105)
         k_joint_init<<<nblk(Jcap), TPB, 0, stream>>>(S);
         pbeg(AVBD_K_NARROW);
-        k_manifold_init<<<nblk(Scap, NARROW_TPB), NARROW_TPB, 0, stream>>>(S);
+        scan(reinterpret_cast<const uint32_t *>(S.wLive), S.wLiveOff, Wcap);
+        k_glive_fill<<<Wcap, TPB, 0, stream>>>(S);
+        k_fill_u32<<<nblk(Scap), TPB, 0, stream>>>(S.keep, 0u, Scap);
+        k_manifold_init<<<liveGrid(NARROW_TPB), NARROW_TPB, 0, stream>>>(S);
         pend();
         k_release_flags<<<nblk(Scap), TPB, 0, stream>>>(S);
         scan(S.keep, S.keepScan, Scap);

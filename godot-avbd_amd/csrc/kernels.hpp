@@ -845,19 +845,18 @@ __global__ void k_contact_unpack(State S)
 // ------------------------------------------------------------ narrow phase ----
 // manifold.cpp:237-435 with collide.cpp:1100-1120: collide at the step-start pose,
 // carry hot state across matching feature keys, C0, release, warm-start decay.
-// One thread per slot-range position.  The live positions are spread over the world
-// ranges' capacity, so each wavefront holds only part of them: this keeps the per-lane
-// scratch (clip polygons, contact list) of the wavefronts in flight small enough to stay in
-// cache, which measured faster than packing the live positions densely.
+// One thread per live manifold (gLive holds every world's live slot-range positions, world
+// after world); the clip polygons live in LDS, two per lane.  `keep` of positions that are
+// not live is cleared beforehand.
+typedef float NarrowPolys[2][3][POLY_LDS * NARROW_TPB];
+AD void manifoldInit(const State &S, int t, NarrowPolys &poly);
 __global__ void __launch_bounds__(NARROW_TPB) k_manifold_init(State S)
 {
-    __shared__ float poly[2][3][MAX_POLY * NARROW_TPB];   // two clip polygons per lane, 24 KiB
-    const int t = GID;
-    if (t >= S.slotCap) return;
-    const int w = S.slotWorld[t];
-    if (w < 0) { S.keep[t] = 0; return; }
-    const int base = S.wSlotBase[w];
-    if (t - base >= S.wLive[w]) { S.keep[t] = 0; return; }
+    __shared__ NarrowPolys poly;   // 12 KiB
+    FOR_LIVE(k) manifoldInit(S, S.gLive[k], poly);
+}
+AD void manifoldInit(const State &S, int t, NarrowPolys &poly)
+{
     const int slot = S.live[t];
     const int a = S.mA[slot], b = S.mB[slot];
     const float ma = bMass(S, a), mb = bMass(S, b);
@@ -886,7 +885,10 @@ __global__ void __launch_bounds__(NARROW_TPB) k_manifold_init(State S)
     const size_t d = static_cast<size_t>(slot) * MAXC;
     Contacts cs{0, S.sRA + d, S.sRB + d, S.sC0 + d};
     const int l = threadIdx.x;
-    const Poly c0{&poly[0][0][l], &poly[0][1][l], &poly[0][2][l]}, c1{&poly[1][0][l], &poly[1][1][l], &poly[1][2][l]};
+    // polygon vertices past the LDS part use this manifold's penalty and lambda staging rows
+    // (32 floats each), which are written only after the contacts are built
+    const Poly c0{&poly[0][0][l], &poly[0][1][l], &poly[0][2][l], reinterpret_cast<float *>(S.sPen + d)};
+    const Poly c1{&poly[1][0][l], &poly[1][1][l], &poly[1][2][l], reinterpret_cast<float *>(S.sLam + d)};
     const int nn = collideBoxes(pa, qa, v3{sa.x, sa.y, sa.z} * 0.5f, pb, qb, v3{sb.x, sb.y, sb.z} * 0.5f, margin,
                                 B, wrote, cs, c0, c1);
     if (wrote) {
