@@ -220,6 +220,7 @@ def load_library(path=LIB_PATH):
         "avbd_batch_destroy": (C.c_int32, [vp]),
         "avbd_batch_autotune": (C.c_int32, [vp, C.c_int32, C.POINTER(Tuning)]),
         "avbd_world_autotune": (C.c_int32, [vp, C.c_int32, C.POINTER(Tuning)]),
+        "avbd_batch_set_tuning": (C.c_int32, [vp, C.POINTER(Tuning)]),
         "avbd_batch_footprint": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                              C.POINTER(C.c_int64)]),
         "avbd_batch_device_bytes": (C.c_int32, [vp, C.POINTER(C.c_int64)]),

@@ -14,6 +14,11 @@ constexpr float STICK_THRESH = 0.00001f;
 constexpr float CONTACT_RELEASE_GAP = 2.0e-3f;
 constexpr float ADAPTIVE_BETA_MAX = 64.0f;
 constexpr int MAXC = 8;
+// Contact storage is sliced: the live manifolds of a step, in view order, form slices of
+// CSLICE (one wavefront); contact i of the manifold in lane l of slice s is at
+// sliceStart[s] + i * CSLICE + l, a slice being as wide as its largest contact count.  A
+// wavefront's load of contact i then reads one contiguous run.
+constexpr int CSLICE = 64;
 constexpr float FLT_MAXV = 3.402823466e+38f;
 
 struct Block {
@@ -353,7 +358,7 @@ AD void stampManifold(const State &S, int k, int body, Block &blk)
     m3 accLin = zero3(), accAng = zero3(), accCross = zero3();
     v3 accRL{0, 0, 0}, accRA{0, 0, 0};
     for (int i = 0; i < n; i++) {
-        const uint32_t c = cb + i;
+        const uint32_t c = cb + i * CSLICE;
         const v3 rA = mul(rotA, ld3(S.cRA[c]));
         const v3 rB = mul(rotB, ld3(S.cRB[c]));
         const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
@@ -476,7 +481,7 @@ AD void manifoldStamps(const State &S, int k, StampAcc &acc)
     const v3 dqAL = A.dqL, dqAA = A.dqA, dqBL = Bd.dqL, dqBA = Bd.dqA;
     const m3 rotA = qmat(A.q), rotB = qmat(Bd.q);
     for (int i = 0; i < n; i++) {
-        const uint32_t c = cb + i;
+        const uint32_t c = cb + i * CSLICE;
         const v3 rA = mul(rotA, ld3(S.cRA[c]));
         const v3 rB = mul(rotB, ld3(S.cRB[c]));
         const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
@@ -513,9 +518,9 @@ AD void manifoldDual(const State &S, int k, float decay, StampAcc &acc)
     const m3 rotA = qmat(A.q), rotB = qmat(Bd.q);
     const float pmax = S.P.penaltyMax;
     for (int i = 0; i < n; i++) {
-        const uint32_t c = cb + i;
+        const uint32_t c = cb + i * CSLICE;
         const float4 pen4 = nPen, ra4 = nRA, rb4 = nRB, c0in = nC0, lam4 = nLam;
-        if (i + 1 < n) { nPen = S.cPen[c + 1]; nRA = S.cRA[c + 1]; nRB = S.cRB[c + 1]; nC0 = S.cC0[c + 1]; nLam = S.cLam[c + 1]; }
+        if (i + 1 < n) { const uint32_t d = c + CSLICE; nPen = S.cPen[d]; nRA = S.cRA[d]; nRB = S.cRB[d]; nC0 = S.cC0[d]; nLam = S.cLam[d]; }
         v3 pen = ld3(pen4);
         if (decay < 1.0f) pen = lclamp3(pen * decay, PENALTY_MIN, pmax);
         const v3 rA = mul(rotA, ld3(ra4));
@@ -559,9 +564,9 @@ AD void manifoldRebase(const State &S, int k)
     float4 nRA{}, nRB{}, nC0{};
     if (n > 0) { nRA = S.cRA[cb]; nRB = S.cRB[cb]; nC0 = S.cC0[cb]; }
     for (int i = 0; i < n; i++) {
-        const uint32_t c = cb + i;
+        const uint32_t c = cb + i * CSLICE;
         const float4 ra4 = nRA, rb4 = nRB, old = nC0;
-        if (i + 1 < n) { nRA = S.cRA[c + 1]; nRB = S.cRB[c + 1]; nC0 = S.cC0[c + 1]; }
+        if (i + 1 < n) { const uint32_t d = c + CSLICE; nRA = S.cRA[d]; nRB = S.cRB[d]; nC0 = S.cC0[d]; }
         const v3 xA = pa + qrot(qa, ld3(ra4));
         const v3 xB = pb + qrot(qb, ld3(rb4));
         v3 c0 = mul(B, xA - xB) + v3{COLLISION_MARGIN, 0, 0};

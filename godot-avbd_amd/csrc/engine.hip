@@ -270,9 +270,11 @@ struct Engine {
         AL(S.jPenL, J); AL(S.jPenA, J); AL(S.jLamL, J); AL(S.jLamA, J); AL(S.jStiffL, J); AL(S.jStiffA, J);
         AL(S.jFrac, J); AL(S.jArm, J); AL(S.jBroken, J); AL(S.jDead, J);
         AL(S.mA, M); AL(S.mB, M); AL(S.mBasis, 3 * M); AL(S.mFric, M); AL(S.mNum, M);
-        AL(S.cRA, C); AL(S.cRB, C); AL(S.cC0, C); AL(S.cPen, C); AL(S.cLam, C);
+        // sliced contacts: at most MAXC rows per lane of every slice, the last slice padded
+        const size_t CD = (static_cast<size_t>(Scap) + CSLICE) * MAXC;
+        AL(S.cRA, CD); AL(S.cRB, CD); AL(S.cC0, CD); AL(S.cPen, CD); AL(S.cLam, CD);
         AL(S.sRA, C); AL(S.sRB, C); AL(S.sC0, C); AL(S.sPen, C); AL(S.sLam, C);
-        AL(S.cOff, M); AL(S.cCount, M + 1); AL(S.cScan, M + 1);
+        AL(S.cOff, M); AL(S.cCount, M + 1); AL(S.cScan, M + 1); AL(S.sliceCount, 1);
         AL(S.stampA, 7 * M); AL(S.stampB, 7 * M); AL(S.slotWorld, M); AL(S.gLive, M); AL(S.wLiveOff, W + 1); AL(S.live, M); AL(S.liveTmp, M); AL(S.freeList, M);
         AL(S.keep, M + 1); AL(S.keepScan, M + 1); AL(S.relScan, M + 1);
         AL(S.succA, M); AL(S.succB, M); AL(S.indeg, M); AL(S.levelOrder, M);
@@ -361,8 +363,10 @@ struct Engine {
     // Dense contact layout for the current live list (staging -> cRA...).
     void packContacts()
     {
-        scan(S.cCount, S.cScan, Scap, S.wLiveOff + Wcap);   // counts written by k_manifold_order
-        k_contact_pack<<<liveGrid(), TPB, 0, stream>>>(S);
+        // slice widths from the counts k_manifold_order wrote, their starts, then the copy
+        k_slice_widths<<<nblk(Scap), TPB, 0, stream>>>(S);
+        scan(S.cScan, S.cScan, Scap / CSLICE + 1, S.sliceCount);
+        k_contact_pack<<<liveGrid(PACK_TPB), PACK_TPB, 0, stream>>>(S);
     }
 
     // Live list and dense manifold view after host edits (contacts stay where they are).
@@ -677,6 +681,24 @@ struct Engine {
 
     // Measure candidate launch shapes on real steps (the steps are part of the simulation:
     // results do not depend on the shapes) and keep the fastest per kernel class.
+    // Launch shapes given by the caller (for instance ones an earlier autotune() chose).
+    avbd_status setTuning(const avbd_tuning &t)
+    {
+        const int v[5] = {t.block_dual, t.block_primal, t.block_stamp, t.block_projection, t.block_traverse};
+        for (int x : v)
+            if (x != 64 && x != 128 && x != 256) return fail(AVBD_ERR_INVALID, "block sizes must be 64, 128 or 256");
+        if (t.live_blocks < NUM_XCD || t.live_blocks > (1 << 20))
+            return fail(AVBD_ERR_INVALID, "live_blocks must be in [8, 2^20]");
+        tune.dual = t.block_dual;
+        tune.primal = t.block_primal;
+        tune.stamp = t.block_stamp;
+        tune.proj = t.block_projection;
+        tune.traverse = t.block_traverse;
+        tune.liveBlocks = t.live_blocks;
+        graphsValid = false;
+        return AVBD_OK;
+    }
+
     avbd_status autotune(int stepsPerTrial, avbd_tuning *out)
     {
         if (stepsPerTrial < 1) return fail(AVBD_ERR_INVALID, "steps_per_trial must be >= 1");
@@ -1498,6 +1520,12 @@ avbd_status avbd_batch_autotune(avbd_batch *batch, int32_t steps_per_trial, avbd
 {
     if (!batch) return fail(AVBD_ERR_INVALID, "null argument");
     return batch->eng.autotune(steps_per_trial, out);
+}
+
+avbd_status avbd_batch_set_tuning(avbd_batch *batch, const avbd_tuning *tuning)
+{
+    if (!batch || !tuning) return fail(AVBD_ERR_INVALID, "null argument");
+    return batch->eng.setTuning(*tuning);
 }
 
 avbd_status avbd_world_autotune(avbd_world *world, int32_t steps_per_trial, avbd_tuning *out)

@@ -601,15 +601,23 @@ __device__ __forceinline__ int otherOf(const State &S, int ref, int body)
     return -1;
 }
 
+// Start of XCD x's part of [0, total): about x / NUM_XCD of it, rounded up to a multiple of
+// 64 so that (with blocks of whole wavefronts) each wavefront covers one contact slice.
+__device__ __forceinline__ int xcdPartStart(int total, int x)
+{
+    if (x >= NUM_XCD) return total;
+    const int b = static_cast<int>((static_cast<long long>(total) * x) / NUM_XCD);
+    const int r = (b + 63) & ~63;
+    return r < total ? r : total;
+}
+
 // Grid-stride walk over every world's live manifolds (the compact list gLive).
 // Each XCD takes one contiguous eighth of [0, total) (its workgroups grid-stride inside it).
 #define FOR_LIVE(k) FOR_RANGE(k, static_cast<int>(S.wLiveOff[S.numWorlds]))
 #define FOR_RANGE(k, total)                                                                                 \
     for (int k##_t = (total), k##_x = static_cast<int>(blockIdx.x) % NUM_XCD,                              \
-             k##_nb = static_cast<int>(gridDim.x) / NUM_XCD,                                                \
-             k##_hi = static_cast<int>((static_cast<long long>(k##_t) * (k##_x + 1)) / NUM_XCD),            \
-             k = static_cast<int>((static_cast<long long>(k##_t) * k##_x) / NUM_XCD) +                      \
-                 static_cast<int>(blockIdx.x) / NUM_XCD * static_cast<int>(blockDim.x) +                    \
+             k##_nb = static_cast<int>(gridDim.x) / NUM_XCD, k##_hi = xcdPartStart(k##_t, k##_x + 1),      \
+             k = xcdPartStart(k##_t, k##_x) + static_cast<int>(blockIdx.x) / NUM_XCD * static_cast<int>(blockDim.x) + \
                  static_cast<int>(threadIdx.x);                                                             \
          k < k##_hi; k += k##_nb * static_cast<int>(blockDim.x))
 
@@ -772,15 +780,18 @@ __global__ void k_alloc_counts(State S)
 // Copy n contact rows of the five contact fields (source and destination never overlap, so
 // every load of a row is issued before its stores).
 struct ContactRows { float4 *ra, *rb, *c0, *pen, *lam; };
-__device__ __forceinline__ void copyContacts(ContactRows dst, size_t d, ContactRows src, size_t o, int n)
+// n contact rows from src (row i at o + i * so) to dst (row i at d + i * sd).
+__device__ __forceinline__ void copyContacts(ContactRows dst, size_t d, size_t sd, ContactRows src, size_t o, size_t so,
+                                             int n)
 {
     for (int i = 0; i < n; i++) {
-        const float4 a = src.ra[o + i], b = src.rb[o + i], c = src.c0[o + i], p = src.pen[o + i], l = src.lam[o + i];
-        dst.ra[d + i] = a;
-        dst.rb[d + i] = b;
-        dst.c0[d + i] = c;
-        dst.pen[d + i] = p;
-        dst.lam[d + i] = l;
+        const size_t x = o + i * so, y = d + i * sd;
+        const float4 a = src.ra[x], b = src.rb[x], c = src.c0[x], p = src.pen[x], l = src.lam[x];
+        dst.ra[y] = a;
+        dst.rb[y] = b;
+        dst.c0[y] = c;
+        dst.pen[y] = p;
+        dst.lam[y] = l;
     }
 }
 __device__ __forceinline__ ContactRows denseRows(const State &S) { return {S.cRA, S.cRB, S.cC0, S.cPen, S.cLam}; }
@@ -789,7 +800,7 @@ __device__ __forceinline__ ContactRows stageRows(const State &S) { return {S.sRA
 // Copy a live manifold's dense contacts to its staging rows.
 __device__ __forceinline__ void stageContacts(const State &S, int slot)
 {
-    copyContacts(stageRows(S), static_cast<size_t>(slot) * MAXC, denseRows(S), S.cOff[slot], S.mNum[slot]);
+    copyContacts(stageRows(S), static_cast<size_t>(slot) * MAXC, 1, denseRows(S), S.cOff[slot], CSLICE, S.mNum[slot]);
 }
 
 // Dense per-step view of live manifold k (stored in `slot`, contacts from `o`).
@@ -811,28 +822,76 @@ __global__ void k_manifold_view(State S)
     }
 }
 
-// Staging -> dense arrays at the scanned offsets: four lanes per manifold, lane q moving
-// contact rows q and q + 4, so each field's rows of a manifold move as contiguous pieces.
-__global__ void k_contact_pack(State S)
+// Width of each contact slice (CSLICE consecutive view indices, one wavefront of this
+// launch): CSLICE times its largest contact count.  sliceCount = the number of slices.
+__global__ void __launch_bounds__(TPB) k_slice_widths(State S)
 {
-    FOR_RANGE(r, 4 * static_cast<int>(S.wLiveOff[S.numWorlds])) {
-        const int k = r >> 2, q = r & 3;
+    const int total = static_cast<int>(S.wLiveOff[S.numWorlds]);
+    const int k = GID;
+    uint32_t w = k < total ? S.cCount[k] : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) w = max(w, static_cast<uint32_t>(__shfl_xor(static_cast<int>(w), off)));
+    // cScan has slotCap + 1 entries; slices past slotCap / CSLICE hold no live manifold
+    if ((threadIdx.x & 63) == 0 && k / CSLICE <= S.slotCap / CSLICE) S.cScan[k / CSLICE] = w * CSLICE;
+    if (k == 0) *S.sliceCount = static_cast<uint32_t>((total + CSLICE - 1) / CSLICE);
+}
+
+// Staging -> sliced dense arrays (cScan holds each slice's start).  A wavefront packs one
+// slice (FOR_LIVE's parts start at multiples of 64, so lane l holds view index 64 s + l):
+// lane j reads the j-th staging row of the slice in manifold order, so each manifold's rows
+// are read as one run, and the rows go through LDS so that the writes are runs too.
+constexpr int PACK_TPB = 256;
+AD void waveSync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__global__ void __launch_bounds__(PACK_TPB) k_contact_pack(State S)
+{
+    __shared__ float4 rowBuf[PACK_TPB / 64][CSLICE * MAXC];   // one field of the slice
+    __shared__ unsigned char owner[PACK_TPB / 64][CSLICE * MAXC];   // slice row -> lane
+    __shared__ int laneSlot[PACK_TPB / 64][CSLICE], lanePre[PACK_TPB / 64][CSLICE], laneN[PACK_TPB / 64][CSLICE];
+    const int lane = static_cast<int>(threadIdx.x) & 63, wv = static_cast<int>(threadIdx.x) >> 6;
+    const ContactRows src = stageRows(S), dst = denseRows(S);
+    FOR_LIVE(k) {
+        const int active = __popcll(__ballot(1));   // a prefix of the wavefront
         const int slot = S.kSlot[k];
         const int n = static_cast<int>(S.cCount[k]);
-        const uint32_t o = S.cScan[k];
-        for (int i = q; i < n; i += 4) {
-            const size_t d = static_cast<size_t>(slot) * MAXC + i;
-            const float4 a = S.sRA[d], b = S.sRB[d], c = S.sC0[d], p = S.sPen[d], l = S.sLam[d];
-            S.cRA[o + i] = a;
-            S.cRB[o + i] = b;
-            S.cC0[o + i] = c;
-            S.cPen[o + i] = p;
-            S.cLam[o + i] = l;
+        const uint32_t s0 = S.cScan[k / CSLICE];
+        const int width = static_cast<int>(S.cScan[k / CSLICE + 1] - s0) / CSLICE;
+        int pre = n;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(pre, off);
+            if (lane >= off) pre += v;
         }
-        if (q == 0) {
-            S.cOff[slot] = o;
-            manifoldView(S, k, slot, o, n);
+        const int rows = __shfl(pre, active - 1);
+        pre -= n;
+        laneSlot[wv][lane] = slot;
+        lanePre[wv][lane] = pre;
+        laneN[wv][lane] = n;
+        for (int i = 0; i < n; i++) owner[wv][pre + i] = static_cast<unsigned char>(lane);
+        waveSync();
+#pragma unroll 1
+        for (int f = 0; f < 5; f++) {
+            const float4 *from = f == 0 ? src.ra : f == 1 ? src.rb : f == 2 ? src.c0 : f == 3 ? src.pen : src.lam;
+            float4 *to = f == 0 ? dst.ra : f == 1 ? dst.rb : f == 2 ? dst.c0 : f == 3 ? dst.pen : dst.lam;
+            for (int r = lane; r < rows; r += active) {
+                const int m = owner[wv][r];
+                const int i = r - lanePre[wv][m];
+                rowBuf[wv][i * CSLICE + m] = from[static_cast<size_t>(laneSlot[wv][m]) * MAXC + i];
+            }
+            waveSync();
+            for (int j = lane; j < width * CSLICE; j += active) {
+                const int m = j % CSLICE;
+                if (m < active && j / CSLICE < laneN[wv][m]) to[s0 + j] = rowBuf[wv][j];
+            }
+            waveSync();
         }
+        const uint32_t o = s0 + static_cast<uint32_t>(lane);
+        S.cOff[slot] = o;
+        manifoldView(S, k, slot, o, n);
     }
 }
 
@@ -910,12 +969,13 @@ AD void manifoldInit(const State &S, int t, NarrowPolys &poly)
         uint8_t stick = 0;
         float beta = 1.0f, prevErr = -1.0f;
         for (int j = 0; j < oldN; j++) {
-            const float4 orb = S.cRB[ob + j];
+            const uint32_t oc = ob + j * CSLICE;
+            const float4 orb = S.cRB[oc];
             if (key != __float_as_int(orb.w)) continue;
-            const float4 ora = S.cRA[ob + j], oc0 = S.cC0[ob + j];
+            const float4 ora = S.cRA[oc], oc0 = S.cC0[oc];
             prevErr = smax(0.0f, COLLISION_MARGIN - oc0.x);
-            pen = ld3(S.cPen[ob + j]);
-            lam = ld3(S.cLam[ob + j]);
+            pen = ld3(S.cPen[oc]);
+            lam = ld3(S.cLam[oc]);
             stick = oc0.w != 0.0f ? 1 : 0;
             beta = ora.w;
             if (stick) {
@@ -1270,7 +1330,7 @@ __global__ void k_brake_flags(State S, int count)
             if (count) S.wUpdateCount[S.bodyWorld[h.x]] = 1;   // the world has an active manifold
             for (int i = 0; i < h.w && !on; i++) {
                 v3 rA, rB;
-                on = contactGap(S, n, pa, qa, pb, qb, static_cast<uint32_t>(h.z) + i, rA, rB) > 0.0f;
+                on = contactGap(S, n, pa, qa, pb, qb, static_cast<uint32_t>(h.z) + i * CSLICE, rA, rB) > 0.0f;
             }
         }
         S.brakeOn[k] = on ? 1 : 0;
@@ -1331,7 +1391,7 @@ __device__ void brakeManifold(const State &S, int k, float window)
     const int nc = hd.w;
     for (int i = 0; i < nc; i++) {
         v3 rA, rB;
-        const float gap = contactGap(S, n, pa, qa, pb, qb, static_cast<uint32_t>(hd.z) + i, rA, rB);
+        const float gap = contactGap(S, n, pa, qa, pb, qb, static_cast<uint32_t>(hd.z) + i * CSLICE, rA, rB);
         if (!(gap > 0.0f)) continue;
         const v3 vAp = va + cross(wa, rA), vBp = vb + cross(wb, rB);
         const float vn = dot(vAp - vBp, n);
@@ -1649,7 +1709,7 @@ __device__ float projManifold(const State &S, int k, int &w)
     int corr = 0;
     const int nc = hd.w;
     for (int i = 0; i < nc; i++) {
-        const uint32_t c = static_cast<uint32_t>(hd.z) + i;
+        const uint32_t c = static_cast<uint32_t>(hd.z) + i * CSLICE;
         const v3 rA = mul(rotA, ld3(S.cRA[c])), rB = mul(rotB, ld3(S.cRB[c]));
         const float raw = -dot(n, (pa + rA) - (pb + rB));
         const float depth = raw - S.P.projSlop;

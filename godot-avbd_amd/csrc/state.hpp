@@ -7,8 +7,11 @@
 //   joints    float4 anchors, C0, penalty, lambda (lin and ang); index = world joint base
 //             + reference force-list position
 //   manifolds stable slots in per-world ranges; per slot: bodies, 3 float4 basis rows,
-//             friction, count; per contact (slot*8 + k): float4 rA|betaScale,
-//             rB|featureKey, C0|stick, penalty, lambda
+//             friction, count; staging rows per contact (slot*8 + i) written by the
+//             narrow phase
+//   contacts  float4 rA|betaScale, rB|featureKey, C0|stick, penalty, lambda per contact,
+//             in view order, sliced by wavefront (contact i of a manifold at its first
+//             row + 64 i)
 //   live      per-world slot lists in creation order (the walk order of every
 //             order-dependent pass)
 //   adjacency CSR per body: joints in list order, then manifolds in live order
@@ -69,15 +72,17 @@ struct State {
     float4 *mBasis;
     float *mFric;
     int *mNum;
-    // Contacts, dense in live order (all worlds, world after world): contact i of the
-    // manifold in `slot` is at cOff[slot] + i.  Fields: rA | beta scale, rB | feature key,
-    // C0 | stick, penalty, lambda.
+    // Contacts in view order (all worlds, world after world), sliced by wavefront: contact i
+    // of the manifold in `slot` is at cOff[slot] + i * CSLICE.  Fields: rA | beta scale,
+    // rB | feature key, C0 | stick, penalty, lambda.
     float4 *cRA, *cRB, *cC0, *cPen, *cLam;
     uint32_t *cOff;
     // Staging with a fixed stride (slot * MAXC + i): the narrow phase writes the new
     // contacts here and k_contact_pack lays them out densely once the live list is final.
     float4 *sRA, *sRB, *sC0, *sPen, *sLam;
-    uint32_t *cCount, *cScan;   // per gLive entry: contact count, its exclusive scan
+    uint32_t *cCount;           // per view index k: contact count
+    uint32_t *cScan;            // per contact slice: its width, then (scanned) its start
+    uint32_t *sliceCount;       // number of contact slices this step
     // Dense view of the live manifolds for one step, index k = gLive position (world after
     // world, live order inside a world).  Adjacency entries, stamps, brake links and
     // projection records use k; slots are only the persistent storage across steps.

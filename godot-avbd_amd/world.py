@@ -194,6 +194,11 @@ class Batch:
         abi.check(self.lib.avbd_batch_autotune(self.h, int(steps_per_trial), C.byref(out)))
         return out.as_dict()
 
+    def set_tuning(self, tuning):
+        """Pin launch shapes (a dict as returned by autotune())."""
+        t = abi.Tuning(**{k: int(v) for k, v in tuning.items()})
+        abi.check(self.lib.avbd_batch_set_tuning(self.h, C.byref(t)))
+
     def profile(self, steps):
         out = abi.KernelProfile()
         abi.check(self.lib.avbd_batch_profile(self.h, int(steps), C.byref(out)))

@@ -268,6 +268,9 @@ typedef struct avbd_tuning {
  * real steps of the simulation.  No reference counterpart (the reference runs on the CPU). */
 avbd_status avbd_batch_autotune(avbd_batch *batch, int32_t steps_per_trial, avbd_tuning *out);
 avbd_status avbd_world_autotune(avbd_world *world, int32_t steps_per_trial, avbd_tuning *out);
+/* Use these launch shapes (block sizes 64, 128 or 256; live_blocks in [8, 2^20]), e.g. ones an
+ * earlier autotune chose, without timing anything.  Results do not depend on them. */
+avbd_status avbd_batch_set_tuning(avbd_batch *batch, const avbd_tuning *tuning);
 /* Device bytes a batch of these capacities allocates (computed on the host; no device needed),
  * and the bytes an existing batch holds.  A serving loop sizes its batch from an HBM budget
  * with the first and admits worlds while their bodies, joints and manifold slots fit. */

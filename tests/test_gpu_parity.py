@@ -226,6 +226,26 @@ def test_batch_alone_equals_in_mix():
                 assert not state_mismatch(batch.download(wid), o.download()), (s, wid)
 
 
+def test_pinned_launch_shapes_keep_results():
+    """Launch shapes set by the caller (here small blocks and a tiny grid, so every live-list
+    kernel strides many times) give the same bits as the defaults."""
+    worlds = [scenes.pile(500, seed=61), scenes.pyramid(rows=10, seed=62), scenes.pile(90, seed=63)]
+    p = P(iterations=10)
+    a, b = pkg.Batch(p, 4, 2000), pkg.Batch(p, 4, 2000)
+    ia = [a.add(bo, j) for bo, j in worlds]
+    ib = [b.add(bo, j) for bo, j in worlds]
+    b.set_tuning({"block_dual": 128, "block_primal": 256, "block_stamp": 128, "block_projection": 64,
+                  "block_traverse": 128, "live_blocks": 8})
+    for _ in range(40):
+        a.step()
+        b.step()
+    for x, y in zip(ia, ib):
+        assert not state_mismatch(a.download(x), b.download(y))
+    with pytest.raises(pkg.abi.AvbdError):
+        b.set_tuning({"block_dual": 96, "block_primal": 64, "block_stamp": 64, "block_projection": 64,
+                      "block_traverse": 64, "live_blocks": 8})
+
+
 def test_full_size_pile_digest_matches_oracle():
     """C2 at its full size (8192 boxes), 40 steps."""
     b, j = scenes.pile(8192, seed=2)
