@@ -225,10 +225,19 @@ __global__ void k_swept_aabb(State S)
     const float cdt = S.P.collisionDt;
     v3 disp = ld3(S.vel[i]) * cdt;
     if (bMass(S, i) > 0) disp += v3{0, 0, S.grav[i] * cdt * cdt};
-    const float rotSweep = sr.w * length(ld3(S.angVel[i])) * cdt;
+    const float4 w4 = S.angVel[i];
+    const float rotSweep = sr.w * length(ld3(w4)) * cdt;
     const float grow = rotSweep + S.P.specMargin;
-    S.aabbC[i] = st3(bPos(S, i) + disp * 0.5f);
+    const float4 p4 = S.pos[i], v4 = S.vel[i];
+    S.aabbC[i] = st3(ld3(p4) + disp * 0.5f);
     S.aabbH[i] = st3((tight + fabs3(disp) * 0.5f) + v3{grow, grow, grow});
+    // the narrow phase's view of the body, one aligned line instead of eight arrays
+    float4 *r = S.narrowRec + NARROW_REC * static_cast<size_t>(i);
+    r[0] = make_float4(p4.x, p4.y, p4.z, bMass(S, i));
+    r[1] = S.rot[i];
+    r[2] = sr;
+    r[3] = make_float4(v4.x, v4.y, v4.z, S.grav[i]);
+    r[4] = make_float4(w4.x, w4.y, w4.z, S.fric[i]);
 }
 
 // Scene bounds of each world's AABB centres (bvh_broadphase.cpp:332-342); one block per
@@ -918,26 +927,30 @@ AD void manifoldInit(const State &S, int t, NarrowPolys &poly)
 {
     const int slot = S.live[t];
     const int a = S.mA[slot], b = S.mB[slot];
-    const float ma = bMass(S, a), mb = bMass(S, b);
+    const float4 *ra = S.narrowRec + NARROW_REC * static_cast<size_t>(a);
+    const float4 *rb = S.narrowRec + NARROW_REC * static_cast<size_t>(b);
+    const float4 a0 = ra[0], a1 = ra[1], a2 = ra[2], a3 = ra[3], a4 = ra[4];
+    const float4 b0 = rb[0], b1 = rb[1], b2 = rb[2], b3 = rb[3], b4 = rb[4];
+    const float ma = a0.w, mb = b0.w;
     if (!(ma > 0.0f) && !(mb > 0.0f)) {   // frozen: parked verbatim
         stageContacts(S, slot);
         S.keep[t] = 1;
         return;
     }
-    const float fa = S.fric[a], fb = S.fric[b];
+    const float fa = a4.w, fb = b4.w;
     const float mu = sqrtf(fa * fb);
     S.mFric[slot] = mu;
-    const v3 pa = bPos(S, a), pb = bPos(S, b);
-    const qt qa = bRot(S, a), qb = bRot(S, b);
-    const float4 sa = S.sizeR[a], sb = S.sizeR[b];
+    const v3 pa = ld3(a0), pb = ld3(b0);
+    const qt qa = ldq(a1), qb = ldq(b1);
+    const float4 sa = a2, sb = b2;
     // speculative margin grown by the pair's predicted relative motion
     float margin = S.P.specMargin;
     if (margin > 0.0f) {
         const float hz = S.P.collisionDt;
-        const float gA = ma > 0.0f ? S.grav[a] : 0.0f;
-        const float gB = mb > 0.0f ? S.grav[b] : 0.0f;
-        const v3 disp = (ld3(S.vel[a]) - ld3(S.vel[b])) * hz + v3{0, 0, (gA - gB) * hz * hz};
-        margin += length(disp) + (sa.w * length(ld3(S.angVel[a])) + sb.w * length(ld3(S.angVel[b]))) * hz;
+        const float gA = ma > 0.0f ? a3.w : 0.0f;
+        const float gB = mb > 0.0f ? b3.w : 0.0f;
+        const v3 disp = (ld3(a3) - ld3(b3)) * hz + v3{0, 0, (gA - gB) * hz * hz};
+        margin += length(disp) + (sa.w * length(ld3(a4)) + sb.w * length(ld3(b4))) * hz;
     }
     m3 B;
     bool wrote;

@@ -29,6 +29,8 @@ struct Params {
     int adaptiveBeta, contactOnlyOverflow, projIters;
 };
 
+constexpr int NARROW_REC = 8;
+
 struct State {
     Params P;
     int numWorlds;
@@ -44,6 +46,9 @@ struct State {
     // manifold): {pos - initial, mass}, rot, {rot (-) initial rotation, 0}, unused.  One aligned
     // half line per gather instead of five separate rows.
     float4 *poseRec;
+    // Per body, NARROW_REC float4 (one 128-byte line) written with the swept AABB for the
+    // narrow phase: {pos, mass}, rot, {size, radius}, {vel, gravity}, {angVel, friction}.
+    float4 *narrowRec;
     float *fric, *grav;
     uint32_t *layer, *mask;
     uint8_t *lockLin, *lockAng;
