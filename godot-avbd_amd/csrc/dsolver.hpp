@@ -770,21 +770,24 @@ constexpr int NARROW_TPB = 64;
 constexpr int POLY_LDS = 8;
 struct Poly {
     float *x, *y, *z;
-    float *spill;
+    float4 *spill;   // float e of the overflow is component e % 4 of row spill[(e / 4) * CSLICE]
+    AD float &spilled(int e) const { return reinterpret_cast<float *>(spill + (e >> 2) * CSLICE)[e & 3]; }
     AD v3 get(int v) const
     {
         if (v < POLY_LDS) { const int i = v * NARROW_TPB; return {x[i], y[i], z[i]}; }
-        const float *s = spill + 3 * (v - POLY_LDS);
-        return {s[0], s[1], s[2]};
+        const int e = 3 * (v - POLY_LDS);
+        return {spilled(e), spilled(e + 1), spilled(e + 2)};
     }
     AD void put(int v, v3 a) const
     {
         if (v < POLY_LDS) { const int i = v * NARROW_TPB; x[i] = a.x; y[i] = a.y; z[i] = a.z; return; }
-        float *s = spill + 3 * (v - POLY_LDS);
-        s[0] = a.x; s[1] = a.y; s[2] = a.z;
+        const int e = 3 * (v - POLY_LDS);
+        spilled(e) = a.x;
+        spilled(e + 1) = a.y;
+        spilled(e + 2) = a.z;
     }
 };
-static_assert(3 * (MAX_POLY - POLY_LDS) <= 4 * MAXC, "polygon overflow must fit one staging row set");
+static_assert(3 * (MAX_POLY - POLY_LDS) <= 4 * MAXC, "polygon overflow must fit one manifold's staging rows");
 // collide.cpp:170-203 (Sutherland-Hodgman against one plane, capacity MAX_POLY)
 AD int clipPoly(const Poly &in, int n, v3 pn, float off, const Poly &out)
 {
@@ -809,24 +812,25 @@ AD int clipPoly(const Poly &in, int n, v3 pn, float off, const Poly &out)
     return m;
 }
 
-// The contact list under construction lives in the manifold's staging rows: arms in ra/rb
-// (feature key in rb.w), and the world midpoint, used only for the merge test, in the C0 row
-// until the final C0 overwrites it.
+// The contact list under construction lives in the manifold's staging rows (row i at
+// i * CSLICE from the pointers): arms in ra/rb (feature key in rb.w), and the world
+// midpoint, used only for the merge test, in the C0 row until the final C0 overwrites it.
 struct Contacts {
     int n;
     float4 *ra, *rb, *mid;
 };
+AD size_t crow(int i) { return static_cast<size_t>(i) * CSLICE; }
 
 // collide.cpp:208-233
 AD void addContact(qt qA, v3 cA, qt qB, v3 cB, Contacts &cs, v3 xA, v3 xB, int key)
 {
     const v3 mid = (xA + xB) * 0.5f;
     for (int i = 0; i < cs.n; i++)
-        if (length2(mid - ld3(cs.mid[i])) < 1.0e-6f) return;
+        if (length2(mid - ld3(cs.mid[crow(i)])) < 1.0e-6f) return;
     if (cs.n >= MAXC) return;
-    cs.ra[cs.n] = st3(qrot(qconj(qA), xA - cA));
-    cs.rb[cs.n] = st3(qrot(qconj(qB), xB - cB), __int_as_float(key));
-    cs.mid[cs.n] = st3(mid);
+    cs.ra[crow(cs.n)] = st3(qrot(qconj(qA), xA - cA));
+    cs.rb[crow(cs.n)] = st3(qrot(qconj(qB), xB - cB), __int_as_float(key));
+    cs.mid[crow(cs.n)] = st3(mid);
     cs.n++;
 }
 

@@ -273,7 +273,8 @@ struct Engine {
         // sliced contacts: at most MAXC rows per lane of every slice, the last slice padded
         const size_t CD = (static_cast<size_t>(Scap) + CSLICE) * MAXC;
         AL(S.cRA, CD); AL(S.cRB, CD); AL(S.cC0, CD); AL(S.cPen, CD); AL(S.cLam, CD);
-        AL(S.sRA, C); AL(S.sRB, C); AL(S.sC0, C); AL(S.sPen, C); AL(S.sLam, C);
+        // staging: sliced by narrow-phase work item (at most MAXC rows per lane of every slice)
+        AL(S.sRA, CD); AL(S.sRB, CD); AL(S.sC0, CD); AL(S.sPen, CD); AL(S.sLam, CD);
         AL(S.cOff, M); AL(S.cCount, M + 1); AL(S.cScan, M + 1); AL(S.sliceCount, 1);
         AL(S.stampA, 7 * M); AL(S.stampB, 7 * M); AL(S.slotWorld, M); AL(S.gLive, M); AL(S.wLiveOff, W + 1); AL(S.live, M); AL(S.liveTmp, M); AL(S.freeList, M);
         AL(S.keep, M + 1); AL(S.keepScan, M + 1); AL(S.relScan, M + 1);
@@ -291,7 +292,7 @@ struct Engine {
         AL(S.pairValid, P + 1); AL(S.pairValidScan, P + 1);
         AL(S.pairScratch, N * PAIR_BUF); AL(S.wRoot, W);
         AL(S.wBoundsLo, W); AL(S.wBoundsHi, W);
-        AL(S.gOfK, M); AL(S.kSlot, M); AL(S.mHdr, M); AL(S.mBasisK, 3 * M); AL(S.mFricK, M); AL(S.brakeOn, M); AL(S.wOrder, W);
+        AL(S.gOfK, M); AL(S.kSlot, M); AL(S.kT, M); AL(S.sPos, M); AL(S.newT, P); AL(S.mHdr, M); AL(S.mBasisK, 3 * M); AL(S.mFricK, M); AL(S.brakeOn, M); AL(S.wOrder, W);
         radixBlocks = (Ncap + RADIX_TILE - 1) / RADIX_TILE;
         AL(radixHist, static_cast<size_t>(256) * radixBlocks + 1);
         const int biggest = std::max({Ncap + 1, Pcap + 1, Scap + 1, 256 * radixBlocks + 1, 2 * (Scap + Jcap)});
@@ -361,12 +362,12 @@ struct Engine {
     }
 
     // Dense contact layout for the current live list (staging -> cRA...).
-    void packContacts()
+    void packContacts(int hostLayout)
     {
         // slice widths from the counts k_manifold_order wrote, their starts, then the copy
         k_slice_widths<<<nblk(Scap), TPB, 0, stream>>>(S);
         scan(S.cScan, S.cScan, Scap / CSLICE + 1, S.sliceCount);
-        k_contact_pack<<<liveGrid(PACK_TPB), PACK_TPB, 0, stream>>>(S);
+        k_contact_pack<<<liveGrid(), TPB, 0, stream>>>(S, hostLayout);
     }
 
     // Live list and dense manifold view after host edits (contacts stay where they are).
@@ -475,8 +476,8 @@ struct Engine {
         // ---- warm start forces / narrow phase (solver_solve.cpp:31-105)
         k_joint_init<<<nblk(Jcap), TPB, 0, stream>>>(S);
         pbeg(AVBD_K_NARROW);
-        scan(reinterpret_cast<const uint32_t *>(S.wLive), S.wLiveOff, Wcap);
-        k_glive_fill<<<Wcap, TPB, 0, stream>>>(S);
+        // work items: the previous step's view (wLiveOff, kSlot, kT, mHdr still hold it), then
+        // this step's new manifolds
         k_fill_u32<<<nblk(Scap), TPB, 0, stream>>>(S.keep, 0u, Scap);
         k_manifold_init<<<liveGrid(NARROW_TPB), NARROW_TPB, 0, stream>>>(S);
         pend();
@@ -487,7 +488,7 @@ struct Engine {
         k_compact_counts<<<nblk(Wcap), TPB, 0, stream>>>(S);
         k_copy_live<<<nblk(Scap), TPB, 0, stream>>>(S);
         buildLiveList();
-        packContacts();
+        packContacts(0);
         pbeg(AVBD_K_ADJACENCY);
         buildAdjacency();
         pend();
@@ -1112,7 +1113,7 @@ struct Engine {
         int *dst[3] = {S.wLive, S.wFree, S.wTop};
         for (int k = 0; k < 3; k++) HIPCHK(hipMemcpy(dst[k] + w, &vals[k], sizeof(int), hipMemcpyHostToDevice));
         buildLiveList();
-        packContacts();
+        packContacts(1);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream));
         csrValid = false;

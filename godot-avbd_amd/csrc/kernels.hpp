@@ -672,6 +672,7 @@ __global__ void __launch_bounds__(ORDER_TPB) k_manifold_order(State S)
             const int k = base + static_cast<int>(off + rank);
             S.gOfK[k] = base + i;
             S.kSlot[k] = slot;
+            S.kT[k] = S.gLive[base + i];
             S.cCount[k] = v;   // contact count in view order (scanned into dense offsets)
         }
         __syncthreads();
@@ -744,15 +745,18 @@ __device__ void allocNew(const State &S, int p)
     const int r = static_cast<int>(S.pairValidScan[p] - S.pairValidScan[start]);
     const int base = S.wSlotBase[w], cap = S.wSlotCap[w];
     const int fr = S.wFree[w], lv = S.wLive[w], top = S.wTop[w];
+    // the narrow phase takes new manifolds in pair order, after the previous step's view
+    int *newT = S.newT + S.pairValidScan[p];
     int slot;
     if (r < fr) {
         slot = S.freeList[base + fr - 1 - r];
     } else {
         const int fresh = top + (r - fr);
-        if (fresh >= cap) { atomicOr(S.wErr + w, 1); return; }
+        if (fresh >= cap) { atomicOr(S.wErr + w, 1); *newT = -1; return; }
         slot = base + fresh;
     }
-    if (lv + r >= cap) { atomicOr(S.wErr + w, 1); return; }
+    if (lv + r >= cap) { atomicOr(S.wErr + w, 1); *newT = -1; return; }
+    *newT = base + lv + r;
     S.live[base + lv + r] = slot;
     S.mA[slot] = i;
     S.mB[slot] = j;
@@ -806,7 +810,14 @@ __device__ __forceinline__ void copyContacts(ContactRows dst, size_t d, size_t s
 __device__ __forceinline__ ContactRows denseRows(const State &S) { return {S.cRA, S.cRB, S.cC0, S.cPen, S.cLam}; }
 __device__ __forceinline__ ContactRows stageRows(const State &S) { return {S.sRA, S.sRB, S.sC0, S.sPen, S.sLam}; }
 
-// Copy a live manifold's dense contacts to its staging rows.
+// Staging row of contact i of narrow-phase work item q: sliced like the dense contacts, so
+// the narrow phase (one lane per work item) writes contact i of a wavefront as one run.
+__device__ __forceinline__ size_t stageRow(int q, int i)
+{
+    return static_cast<size_t>(q / CSLICE) * (CSLICE * MAXC) + static_cast<size_t>(i) * CSLICE + q % CSLICE;
+}
+
+// Copy a live manifold's dense contacts to its host-layout staging rows (slot * MAXC + i).
 __device__ __forceinline__ void stageContacts(const State &S, int slot)
 {
     copyContacts(stageRows(S), static_cast<size_t>(slot) * MAXC, 1, denseRows(S), S.cOff[slot], CSLICE, S.mNum[slot]);
@@ -845,60 +856,19 @@ __global__ void __launch_bounds__(TPB) k_slice_widths(State S)
     if (k == 0) *S.sliceCount = static_cast<uint32_t>((total + CSLICE - 1) / CSLICE);
 }
 
-// Staging -> sliced dense arrays (cScan holds each slice's start).  A wavefront packs one
-// slice (FOR_LIVE's parts start at multiples of 64, so lane l holds view index 64 s + l):
-// lane j reads the j-th staging row of the slice in manifold order, so each manifold's rows
-// are read as one run, and the rows go through LDS so that the writes are runs too.
-constexpr int PACK_TPB = 256;
-AD void waveSync()
+// Staging -> sliced dense arrays (cScan holds each slice's start), one lane per manifold.
+// After the narrow phase the staging rows are those of the manifold's work item (sPos), which
+// mostly runs in the same order as the view, so reads and writes are both runs; after host
+// edits they are the host layout (slot * MAXC + i).
+__global__ void k_contact_pack(State S, int hostLayout)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-__global__ void __launch_bounds__(PACK_TPB) k_contact_pack(State S)
-{
-    __shared__ float4 rowBuf[PACK_TPB / 64][CSLICE * MAXC];   // one field of the slice
-    __shared__ unsigned char owner[PACK_TPB / 64][CSLICE * MAXC];   // slice row -> lane
-    __shared__ int laneSlot[PACK_TPB / 64][CSLICE], lanePre[PACK_TPB / 64][CSLICE], laneN[PACK_TPB / 64][CSLICE];
-    const int lane = static_cast<int>(threadIdx.x) & 63, wv = static_cast<int>(threadIdx.x) >> 6;
     const ContactRows src = stageRows(S), dst = denseRows(S);
     FOR_LIVE(k) {
-        const int active = __popcll(__ballot(1));   // a prefix of the wavefront
         const int slot = S.kSlot[k];
         const int n = static_cast<int>(S.cCount[k]);
-        const uint32_t s0 = S.cScan[k / CSLICE];
-        const int width = static_cast<int>(S.cScan[k / CSLICE + 1] - s0) / CSLICE;
-        int pre = n;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int v = __shfl_up(pre, off);
-            if (lane >= off) pre += v;
-        }
-        const int rows = __shfl(pre, active - 1);
-        pre -= n;
-        laneSlot[wv][lane] = slot;
-        lanePre[wv][lane] = pre;
-        laneN[wv][lane] = n;
-        for (int i = 0; i < n; i++) owner[wv][pre + i] = static_cast<unsigned char>(lane);
-        waveSync();
-#pragma unroll 1
-        for (int f = 0; f < 5; f++) {
-            const float4 *from = f == 0 ? src.ra : f == 1 ? src.rb : f == 2 ? src.c0 : f == 3 ? src.pen : src.lam;
-            float4 *to = f == 0 ? dst.ra : f == 1 ? dst.rb : f == 2 ? dst.c0 : f == 3 ? dst.pen : dst.lam;
-            for (int r = lane; r < rows; r += active) {
-                const int m = owner[wv][r];
-                const int i = r - lanePre[wv][m];
-                rowBuf[wv][i * CSLICE + m] = from[static_cast<size_t>(laneSlot[wv][m]) * MAXC + i];
-            }
-            waveSync();
-            for (int j = lane; j < width * CSLICE; j += active) {
-                const int m = j % CSLICE;
-                if (m < active && j / CSLICE < laneN[wv][m]) to[s0 + j] = rowBuf[wv][j];
-            }
-            waveSync();
-        }
-        const uint32_t o = s0 + static_cast<uint32_t>(lane);
+        const uint32_t o = S.cScan[k / CSLICE] + static_cast<uint32_t>(k % CSLICE);
+        if (hostLayout) copyContacts(dst, o, CSLICE, src, static_cast<size_t>(slot) * MAXC, 1, n);
+        else copyContacts(dst, o, CSLICE, src, stageRow(S.sPos[slot], 0), CSLICE, n);
         S.cOff[slot] = o;
         manifoldView(S, k, slot, o, n);
     }
@@ -913,27 +883,47 @@ __global__ void k_contact_unpack(State S)
 // ------------------------------------------------------------ narrow phase ----
 // manifold.cpp:237-435 with collide.cpp:1100-1120: collide at the step-start pose,
 // carry hot state across matching feature keys, C0, release, warm-start decay.
-// One thread per live manifold (gLive holds every world's live slot-range positions, world
-// after world); the clip polygons live in LDS, two per lane.  `keep` of positions that are
-// not live is cleared beforehand.
+// One thread per live manifold.  Work item q < P is view index q of the previous step (the
+// manifolds that were live then, in that order, so that their old contacts, headers and the
+// staging rows are read and written as runs); items past P are this step's new manifolds in
+// pair order.  The clip polygons live in LDS, two per lane.  `keep` of positions that are not
+// live is cleared beforehand.
 typedef float NarrowPolys[2][3][POLY_LDS * NARROW_TPB];
-AD void manifoldInit(const State &S, int t, NarrowPolys &poly);
-__global__ void __launch_bounds__(NARROW_TPB) k_manifold_init(State S)
+AD void manifoldInit(const State &S, int q, int P, NarrowPolys &poly);
+// At most 128 registers (four wavefronts per SIMD; it fits without spilling vector registers).
+__global__ void __launch_bounds__(NARROW_TPB) __attribute__((amdgpu_waves_per_eu(4))) k_manifold_init(State S)
 {
     __shared__ NarrowPolys poly;   // 12 KiB
-    FOR_LIVE(k) manifoldInit(S, S.gLive[k], poly);
+    const int P = static_cast<int>(S.wLiveOff[S.numWorlds]);
+    const int fresh = static_cast<int>(S.pairValidScan[pairTotal(S)]);
+    FOR_RANGE(q, P + fresh) manifoldInit(S, q, P, poly);
 }
-AD void manifoldInit(const State &S, int t, NarrowPolys &poly)
+AD void manifoldInit(const State &S, int q, int P, NarrowPolys &poly)
 {
-    const int slot = S.live[t];
-    const int a = S.mA[slot], b = S.mB[slot];
+    int t, slot, a, b, oldN;
+    uint32_t ob;
+    if (q < P) {
+        slot = S.kSlot[q];
+        t = S.kT[q];
+        const int4 hd = S.mHdr[q];
+        a = hd.x; b = hd.y; ob = static_cast<uint32_t>(hd.z); oldN = hd.w;
+    } else {
+        t = S.newT[q - P];
+        if (t < 0) return;   // not allocated (capacity error reported)
+        slot = S.live[t];
+        a = S.mA[slot]; b = S.mB[slot]; ob = 0u; oldN = 0;
+        // past the staging capacity only when allocations failed this step (items of failed
+        // pairs hold places in the list): report it for this manifold's world too
+        if (q >= S.slotCap) { atomicOr(S.wErr + S.bodyWorld[a], 1); return; }
+    }
+    S.sPos[slot] = q;
     const float4 *ra = S.narrowRec + NARROW_REC * static_cast<size_t>(a);
     const float4 *rb = S.narrowRec + NARROW_REC * static_cast<size_t>(b);
     const float4 a0 = ra[0], a1 = ra[1], a2 = ra[2], a3 = ra[3], a4 = ra[4];
     const float4 b0 = rb[0], b1 = rb[1], b2 = rb[2], b3 = rb[3], b4 = rb[4];
     const float ma = a0.w, mb = b0.w;
     if (!(ma > 0.0f) && !(mb > 0.0f)) {   // frozen: parked verbatim
-        stageContacts(S, slot);
+        copyContacts(stageRows(S), stageRow(q, 0), CSLICE, denseRows(S), ob, CSLICE, oldN);
         S.keep[t] = 1;
         return;
     }
@@ -954,13 +944,13 @@ AD void manifoldInit(const State &S, int t, NarrowPolys &poly)
     }
     m3 B;
     bool wrote;
-    const size_t d = static_cast<size_t>(slot) * MAXC;
+    const size_t d = stageRow(q, 0);
     Contacts cs{0, S.sRA + d, S.sRB + d, S.sC0 + d};
     const int l = threadIdx.x;
     // polygon vertices past the LDS part use this manifold's penalty and lambda staging rows
     // (32 floats each), which are written only after the contacts are built
-    const Poly c0{&poly[0][0][l], &poly[0][1][l], &poly[0][2][l], reinterpret_cast<float *>(S.sPen + d)};
-    const Poly c1{&poly[1][0][l], &poly[1][1][l], &poly[1][2][l], reinterpret_cast<float *>(S.sLam + d)};
+    const Poly c0{&poly[0][0][l], &poly[0][1][l], &poly[0][2][l], S.sPen + d};
+    const Poly c1{&poly[1][0][l], &poly[1][1][l], &poly[1][2][l], S.sLam + d};
     const int nn = collideBoxes(pa, qa, v3{sa.x, sa.y, sa.z} * 0.5f, pb, qb, v3{sb.x, sb.y, sb.z} * 0.5f, margin,
                                 B, wrote, cs, c0, c1);
     if (wrote) {
@@ -971,11 +961,9 @@ AD void manifoldInit(const State &S, int t, NarrowPolys &poly)
         B = loadBasis(S, slot);
     }
     // old contacts stay in the dense arrays; the new ones go to staging
-    const int oldN = S.mNum[slot];
-    const uint32_t ob = oldN > 0 ? S.cOff[slot] : 0u;
     const float pmax = S.P.penaltyMax;
     for (int i = 0; i < nn; i++) {
-        const float4 nra = cs.ra[i], nrb = cs.rb[i];
+        const float4 nra = cs.ra[crow(i)], nrb = cs.rb[crow(i)];
         const int key = __float_as_int(nrb.w);
         v3 rA = ld3(nra), rB = ld3(nrb);
         v3 pen{0, 0, 0}, lam{0, 0, 0};
@@ -1026,7 +1014,7 @@ AD void manifoldInit(const State &S, int t, NarrowPolys &poly)
         }
         lam = lam * S.P.alpha * S.P.gamma;
         pen = lclamp3(pen * S.P.gamma, PENALTY_MIN, pmax);
-        const size_t c = static_cast<size_t>(slot) * MAXC + i;
+        const size_t c = d + crow(i);
         S.sRA[c] = st3(rA, scale);
         S.sRB[c] = st3(rB, __int_as_float(key));
         S.sC0[c] = st3(c0, stick ? 1.0f : 0.0f);

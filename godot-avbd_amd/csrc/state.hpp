@@ -82,8 +82,9 @@ struct State {
     // rB | feature key, C0 | stick, penalty, lambda.
     float4 *cRA, *cRB, *cC0, *cPen, *cLam;
     uint32_t *cOff;
-    // Staging with a fixed stride (slot * MAXC + i): the narrow phase writes the new
-    // contacts here and k_contact_pack lays them out densely once the live list is final.
+    // Staging: the narrow phase writes the new contacts here (sliced by its work items) and
+    // k_contact_pack lays them out in view order once the live list is final; host reads and
+    // edits use it with a fixed stride (slot * MAXC + i).
     float4 *sRA, *sRB, *sC0, *sPen, *sLam;
     uint32_t *cCount;           // per view index k: contact count
     uint32_t *cScan;            // per contact slice: its width, then (scanned) its start
@@ -97,6 +98,9 @@ struct State {
     // order, which orders adjacency and therefore every sum).
     int *gOfK;
     int *kSlot;                 // storage slot of view index k
+    int *kT;                    // slot-range position (live list entry) of view index k
+    int *newT;                  // per new manifold of this step (pair order): its live entry
+    int *sPos;                  // per slot: its narrow-phase work item this step (staging rows)
     int4 *mHdr;                 // {body a, body b, first contact, contact count}
     float4 *mBasisK;            // 3 rows per k
     float *mFricK;
