@@ -19,6 +19,21 @@ constexpr int MAXC = 8;
 // sliceStart[s] + i * CSLICE + l, a slice being as wide as its largest contact count.  A
 // wavefront's load of contact i then reads one contiguous run.
 constexpr int CSLICE = 64;
+
+// Penalty and lambda of dense contact c: {pen.xyz, lam.x} and {lam.y, lam.z} (24 bytes instead
+// of two 16-byte rows); the stick flag is a byte of its own, so the dual writes no C0 row.
+AD void loadPenLam(const State &S, uint32_t c, v3 &pen, v3 &lam)
+{
+    const float4 a = S.cPL[c];
+    const float2 b = S.cLyz[c];
+    pen = v3{a.x, a.y, a.z};
+    lam = v3{a.w, b.x, b.y};
+}
+AD void storePenLam(const State &S, uint32_t c, v3 pen, v3 lam)
+{
+    S.cPL[c] = make_float4(pen.x, pen.y, pen.z, lam.x);
+    S.cLyz[c] = make_float2(lam.y, lam.z);
+}
 constexpr float FLT_MAXV = 3.402823466e+38f;
 
 struct Block {
@@ -363,8 +378,9 @@ AD void stampManifold(const State &S, int k, int body, Block &blk)
         const v3 rB = mul(rotB, ld3(S.cRB[c]));
         const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
         const v3 C = constraintError(S, ld3(S.cC0[c]), br, B);
-        const v3 pen = ld3(S.cPen[c]);
-        v3 f = cmul(pen, C) + ld3(S.cLam[c]);
+        v3 pen, lam;
+        loadPenLam(S, c, pen, lam);
+        v3 f = cmul(pen, C) + lam;
         float bounds;
         clampForce(f, mu, bounds);
         const Hess h = basisHessian(B, pen);
@@ -391,11 +407,14 @@ constexpr int STAMP_F4 = 14;
 struct StampAcc {
     m3 lin, angA, crossA, angB, crossB;
     v3 rl, raA, raB;
+    bool movA, movB;   // a side whose body is static is never gathered: its stamp is not stored
 };
-AD void stampInit(StampAcc &s)
+AD void stampInit(StampAcc &s, bool movA, bool movB)
 {
     s.lin = s.angA = s.crossA = s.angB = s.crossB = zero3();
     s.rl = s.raA = s.raB = v3{0, 0, 0};
+    s.movA = movA;
+    s.movB = movB;
 }
 AD void stampContact(StampAcc &s, const m3 &B, v3 pen, v3 f, v3 rA, v3 rB)
 {
@@ -421,10 +440,12 @@ AD void stampRows(float4 (&p)[7], const m3 &L, v3 rl, const m3 &A, const m3 &X, 
 // Wavefront-wide store of the active lanes' rows (7 float4 each, lane l's at dst + 7 l):
 // the rows go through LDS so that every store instruction writes one contiguous run instead
 // of 64 scattered 16-byte pieces.  The active lanes must be a prefix of the wavefront (the
-// live-list walks guarantee it); `lds` holds 7 * 64 float4 for this wavefront.
-AD void waveStoreRows(float4 *dst, const float4 (&rows)[7], float4 *lds, int lane)
+// live-list walks guarantee it); `lds` holds 7 * 64 float4 for this wavefront.  Lanes with
+// `keep` false leave their rows unwritten.
+AD void waveStoreRows(float4 *dst, const float4 (&rows)[7], float4 *lds, int lane, bool keep)
 {
     const int r = __popcll(__ballot(1));
+    const unsigned long long kept = __ballot(keep);
 #pragma unroll
     for (int q = 0; q < 7; q++) lds[lane * 7 + q] = rows[q];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -433,7 +454,7 @@ AD void waveStoreRows(float4 *dst, const float4 (&rows)[7], float4 *lds, int lan
 #pragma unroll
     for (int q = 0; q < 7; q++) {
         const int idx = q * r + lane;
-        dst[idx] = lds[idx];
+        if ((kept >> (idx / 7)) & 1ull) dst[idx] = lds[idx];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -447,9 +468,9 @@ AD void storeStampsWave(const State &S, int k, const StampAcc &s, float4 *lds, i
     const size_t k0 = static_cast<size_t>(k - lane);
     float4 rows[7];
     stampRows(rows, s.lin, s.rl, s.angA, s.crossA, s.raA);
-    waveStoreRows(S.stampA + 7 * k0, rows, lds, lane);
+    waveStoreRows(S.stampA + 7 * k0, rows, lds, lane, s.movA);
     stampRows(rows, s.lin, s.rl * -1.0f, s.angB, s.crossB, s.raB * -1.0f);
-    waveStoreRows(S.stampB + 7 * k0, rows, lds, lane);
+    waveStoreRows(S.stampB + 7 * k0, rows, lds, lane, s.movB);
 }
 // Symmetric 3x3 from its upper triangle (the sums are symmetric bit for bit).
 AD m3 sym3(float a00, float a01, float a02, float a11, float a12, float a22)
@@ -476,7 +497,7 @@ AD void manifoldStamps(const State &S, int k, StampAcc &acc)
     const PoseRec A = loadPose(S, a), Bd = loadPose(S, b);
     const m3 B = loadBasisK(S, k);
     const float mu = S.mFricK[k];
-    stampInit(acc);
+    stampInit(acc, A.mass > 0.0f, Bd.mass > 0.0f);
     if (!(A.mass > 0.0f) && !(Bd.mass > 0.0f)) return;
     const v3 dqAL = A.dqL, dqAA = A.dqA, dqBL = Bd.dqL, dqBA = Bd.dqA;
     const m3 rotA = qmat(A.q), rotB = qmat(Bd.q);
@@ -486,8 +507,9 @@ AD void manifoldStamps(const State &S, int k, StampAcc &acc)
         const v3 rB = mul(rotB, ld3(S.cRB[c]));
         const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
         const v3 C = constraintError(S, ld3(S.cC0[c]), br, B);
-        const v3 pen = ld3(S.cPen[c]);
-        v3 f = cmul(pen, C) + ld3(S.cLam[c]);
+        v3 pen, lam;
+        loadPenLam(S, c, pen, lam);
+        v3 f = cmul(pen, C) + lam;
         float bounds;
         clampForce(f, mu, bounds);
         stampContact(acc, B, pen, f, rA, rB);
@@ -510,39 +532,36 @@ AD void manifoldDual(const State &S, int k, float decay, StampAcc &acc)
     const m3 B = loadBasisK(S, k);
     const float mu = S.mFricK[k];
     // contact i + 1 is read before contact i is written (distinct rows)
-    float4 nPen{}, nRA{}, nRB{}, nC0{}, nLam{};
-    if (n > 0) { nPen = S.cPen[cb]; nRA = S.cRA[cb]; nRB = S.cRB[cb]; nC0 = S.cC0[cb]; nLam = S.cLam[cb]; }
-    if (STAMP) stampInit(acc);
+    float4 nPL{}, nRA{}, nRB{}, nC0{};
+    float2 nLyz{};
+    if (n > 0) { nPL = S.cPL[cb]; nRA = S.cRA[cb]; nRB = S.cRB[cb]; nC0 = S.cC0[cb]; nLyz = S.cLyz[cb]; }
+    if (STAMP) stampInit(acc, A.mass > 0.0f, Bd.mass > 0.0f);
     if (!(A.mass > 0.0f) && !(Bd.mass > 0.0f)) return;
     const v3 dqAL = A.dqL, dqAA = A.dqA, dqBL = Bd.dqL, dqBA = Bd.dqA;
     const m3 rotA = qmat(A.q), rotB = qmat(Bd.q);
     const float pmax = S.P.penaltyMax;
     for (int i = 0; i < n; i++) {
         const uint32_t c = cb + i * CSLICE;
-        const float4 pen4 = nPen, ra4 = nRA, rb4 = nRB, c0in = nC0, lam4 = nLam;
-        if (i + 1 < n) { const uint32_t d = c + CSLICE; nPen = S.cPen[d]; nRA = S.cRA[d]; nRB = S.cRB[d]; nC0 = S.cC0[d]; nLam = S.cLam[d]; }
-        v3 pen = ld3(pen4);
+        const float4 pl4 = nPL, ra4 = nRA, rb4 = nRB, c0in = nC0;
+        const float2 lyz = nLyz;
+        if (i + 1 < n) { const uint32_t d = c + CSLICE; nPL = S.cPL[d]; nRA = S.cRA[d]; nRB = S.cRB[d]; nC0 = S.cC0[d]; nLyz = S.cLyz[d]; }
+        v3 pen = ld3(pl4);
         if (decay < 1.0f) pen = lclamp3(pen * decay, PENALTY_MIN, pmax);
         const v3 rA = mul(rotA, ld3(ra4));
         const v3 rB = mul(rotB, ld3(rb4));
         const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
         const v3 C = constraintError(S, ld3(c0in), br, B);
-        v3 F = cmul(pen, C) + ld3(lam4);
+        v3 F = cmul(pen, C) + v3{pl4.w, lyz.x, lyz.y};
         float bounds;
         const float scale = clampForce(F, mu, bounds);
-        S.cLam[c] = st3(F);
         const float beta = S.P.betaLin * ra4.w;
         if (F.x < 0) pen.x = smin(pen.x + beta * fabsf(C.x), pmax);
-        float4 c0 = c0in;
-        bool writeStick = false;
         if (scale <= bounds) {
             pen.y = smin(pen.y + beta * fabsf(C.y), pmax);
             pen.z = smin(pen.z + beta * fabsf(C.z), pmax);
-            c0.w = length2d(C.y, C.z) < STICK_THRESH ? 1.0f : 0.0f;
-            writeStick = true;
+            S.cStick[c] = length2d(C.y, C.z) < STICK_THRESH ? 1 : 0;
         }
-        S.cPen[c] = st3(pen);
-        if (writeStick) S.cC0[c] = c0;
+        storePenLam(S, c, pen, F);
         if (STAMP) {
             v3 f = cmul(pen, C) + F;
             float bd;

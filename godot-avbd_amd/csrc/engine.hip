@@ -257,7 +257,7 @@ struct Engine {
         }
         avbd_status st;
 #define AL(ptr, n) if ((st = A(ptr, n)) != AVBD_OK) return st
-        const size_t N = Ncap, J = Jcap, M = Scap, C = static_cast<size_t>(Scap) * MAXC, W = Wcap, P = Pcap;
+        const size_t N = Ncap, J = Jcap, M = Scap, W = Wcap, P = Pcap;
         S.numWorlds = Wcap; S.numBodies = Ncap; S.slotCap = Scap; S.jointCap = Jcap; S.pairCap = Pcap;
         AL(S.pos, N); AL(S.rot, N); AL(S.initLin, N); AL(S.initAng, N); AL(S.inertLin, N); AL(S.inertAng, N);
         AL(S.vel, N); AL(S.angVel, N); AL(S.prevVel, N); AL(S.sizeR, N); AL(S.massI, N); AL(S.poseRec, 4 * N); AL(S.narrowRec, NARROW_REC * N);
@@ -272,7 +272,7 @@ struct Engine {
         AL(S.mA, M); AL(S.mB, M); AL(S.mBasis, 3 * M); AL(S.mFric, M); AL(S.mNum, M);
         // sliced contacts: at most MAXC rows per lane of every slice, the last slice padded
         const size_t CD = (static_cast<size_t>(Scap) + CSLICE) * MAXC;
-        AL(S.cRA, CD); AL(S.cRB, CD); AL(S.cC0, CD); AL(S.cPen, CD); AL(S.cLam, CD);
+        AL(S.cRA, CD); AL(S.cRB, CD); AL(S.cC0, CD); AL(S.cPL, CD); AL(S.cLyz, CD); AL(S.cStick, CD);
         // staging: sliced by narrow-phase work item (at most MAXC rows per lane of every slice)
         AL(S.sRA, CD); AL(S.sRB, CD); AL(S.sC0, CD); AL(S.sPen, CD); AL(S.sLam, CD);
         AL(S.cOff, M); AL(S.cCount, M + 1); AL(S.cScan, M + 1); AL(S.sliceCount, 1);

@@ -792,24 +792,6 @@ __global__ void k_alloc_counts(State S)
 // ------------------------------------------------------- contact storage ----
 // Copy n contact rows of the five contact fields (source and destination never overlap, so
 // every load of a row is issued before its stores).
-struct ContactRows { float4 *ra, *rb, *c0, *pen, *lam; };
-// n contact rows from src (row i at o + i * so) to dst (row i at d + i * sd).
-__device__ __forceinline__ void copyContacts(ContactRows dst, size_t d, size_t sd, ContactRows src, size_t o, size_t so,
-                                             int n)
-{
-    for (int i = 0; i < n; i++) {
-        const size_t x = o + i * so, y = d + i * sd;
-        const float4 a = src.ra[x], b = src.rb[x], c = src.c0[x], p = src.pen[x], l = src.lam[x];
-        dst.ra[y] = a;
-        dst.rb[y] = b;
-        dst.c0[y] = c;
-        dst.pen[y] = p;
-        dst.lam[y] = l;
-    }
-}
-__device__ __forceinline__ ContactRows denseRows(const State &S) { return {S.cRA, S.cRB, S.cC0, S.cPen, S.cLam}; }
-__device__ __forceinline__ ContactRows stageRows(const State &S) { return {S.sRA, S.sRB, S.sC0, S.sPen, S.sLam}; }
-
 // Staging row of contact i of narrow-phase work item q: sliced like the dense contacts, so
 // the narrow phase (one lane per work item) writes contact i of a wavefront as one run.
 __device__ __forceinline__ size_t stageRow(int q, int i)
@@ -817,10 +799,41 @@ __device__ __forceinline__ size_t stageRow(int q, int i)
     return static_cast<size_t>(q / CSLICE) * (CSLICE * MAXC) + static_cast<size_t>(i) * CSLICE + q % CSLICE;
 }
 
+// Staging rows (float4 rA|beta, rB|key, C0|stick, penalty, lambda; row i of a manifold at
+// x + i * sx) to dense contacts (row i at y + i * sy, penalty and lambda packed, stick a byte),
+// and back.
+__device__ __forceinline__ void stageToDense(const State &S, size_t y, size_t sy, size_t x, size_t sx, int n)
+{
+    for (int i = 0; i < n; i++) {
+        const size_t xi = x + i * sx, yi = y + i * sy;
+        const float4 a = S.sRA[xi], b = S.sRB[xi], c = S.sC0[xi], p = S.sPen[xi], l = S.sLam[xi];
+        S.cRA[yi] = a;
+        S.cRB[yi] = b;
+        S.cC0[yi] = c;
+        S.cPL[yi] = make_float4(p.x, p.y, p.z, l.x);
+        S.cLyz[yi] = make_float2(l.y, l.z);
+        S.cStick[yi] = c.w != 0.0f ? 1 : 0;
+    }
+}
+__device__ __forceinline__ void denseToStage(const State &S, size_t x, size_t sx, size_t y, size_t sy, int n)
+{
+    for (int i = 0; i < n; i++) {
+        const size_t xi = x + i * sx, yi = y + i * sy;
+        const float4 a = S.cRA[yi], b = S.cRB[yi], c = S.cC0[yi], pl = S.cPL[yi];
+        const float2 lyz = S.cLyz[yi];
+        const uint8_t st = S.cStick[yi];
+        S.sRA[xi] = a;
+        S.sRB[xi] = b;
+        S.sC0[xi] = make_float4(c.x, c.y, c.z, st ? 1.0f : 0.0f);
+        S.sPen[xi] = make_float4(pl.x, pl.y, pl.z, 0.0f);
+        S.sLam[xi] = make_float4(pl.w, lyz.x, lyz.y, 0.0f);
+    }
+}
+
 // Copy a live manifold's dense contacts to its host-layout staging rows (slot * MAXC + i).
 __device__ __forceinline__ void stageContacts(const State &S, int slot)
 {
-    copyContacts(stageRows(S), static_cast<size_t>(slot) * MAXC, 1, denseRows(S), S.cOff[slot], CSLICE, S.mNum[slot]);
+    denseToStage(S, static_cast<size_t>(slot) * MAXC, 1, S.cOff[slot], CSLICE, S.mNum[slot]);
 }
 
 // Dense per-step view of live manifold k (stored in `slot`, contacts from `o`).
@@ -862,13 +875,12 @@ __global__ void __launch_bounds__(TPB) k_slice_widths(State S)
 // edits they are the host layout (slot * MAXC + i).
 __global__ void k_contact_pack(State S, int hostLayout)
 {
-    const ContactRows src = stageRows(S), dst = denseRows(S);
     FOR_LIVE(k) {
         const int slot = S.kSlot[k];
         const int n = static_cast<int>(S.cCount[k]);
         const uint32_t o = S.cScan[k / CSLICE] + static_cast<uint32_t>(k % CSLICE);
-        if (hostLayout) copyContacts(dst, o, CSLICE, src, static_cast<size_t>(slot) * MAXC, 1, n);
-        else copyContacts(dst, o, CSLICE, src, stageRow(S.sPos[slot], 0), CSLICE, n);
+        if (hostLayout) stageToDense(S, o, CSLICE, static_cast<size_t>(slot) * MAXC, 1, n);
+        else stageToDense(S, o, CSLICE, stageRow(S.sPos[slot], 0), CSLICE, n);
         S.cOff[slot] = o;
         manifoldView(S, k, slot, o, n);
     }
@@ -923,7 +935,7 @@ AD void manifoldInit(const State &S, int q, int P, NarrowPolys &poly)
     const float4 b0 = rb[0], b1 = rb[1], b2 = rb[2], b3 = rb[3], b4 = rb[4];
     const float ma = a0.w, mb = b0.w;
     if (!(ma > 0.0f) && !(mb > 0.0f)) {   // frozen: parked verbatim
-        copyContacts(stageRows(S), stageRow(q, 0), CSLICE, denseRows(S), ob, CSLICE, oldN);
+        denseToStage(S, stageRow(q, 0), CSLICE, ob, CSLICE, oldN);
         S.keep[t] = 1;
         return;
     }
@@ -975,9 +987,8 @@ AD void manifoldInit(const State &S, int q, int P, NarrowPolys &poly)
             if (key != __float_as_int(orb.w)) continue;
             const float4 ora = S.cRA[oc], oc0 = S.cC0[oc];
             prevErr = smax(0.0f, COLLISION_MARGIN - oc0.x);
-            pen = ld3(S.cPen[oc]);
-            lam = ld3(S.cLam[oc]);
-            stick = oc0.w != 0.0f ? 1 : 0;
+            loadPenLam(S, oc, pen, lam);
+            stick = S.cStick[oc];
             beta = ora.w;
             if (stick) {
                 const v3 oldA = pa + qrot(qa, ld3(ora));

@@ -79,8 +79,11 @@ struct State {
     int *mNum;
     // Contacts in view order (all worlds, world after world), sliced by wavefront: contact i
     // of the manifold in `slot` is at cOff[slot] + i * CSLICE.  Fields: rA | beta scale,
-    // rB | feature key, C0 | stick, penalty, lambda.
-    float4 *cRA, *cRB, *cC0, *cPen, *cLam;
+    // rB | feature key, C0, penalty and lambda (packed in 24 bytes), stick flag.
+    float4 *cRA, *cRB, *cC0;
+    float4 *cPL;                // {penalty.xyz, lambda.x}
+    float2 *cLyz;               // {lambda.y, lambda.z}
+    uint8_t *cStick;            // stick flag
     uint32_t *cOff;
     // Staging: the narrow phase writes the new contacts here (sliced by its work items) and
     // k_contact_pack lays them out in view order once the live list is final; host reads and
