@@ -83,7 +83,7 @@ class Joints(C.Structure):
         ("count", C.c_int32), ("body_a", i32p), ("body_b", i32p), ("r_a", f32p), ("r_b", f32p),
         ("stiffness_lin", f32p), ("stiffness_ang", f32p), ("fracture", f32p), ("torque_arm", f32p),
         ("c0_lin", f32p), ("c0_ang", f32p), ("penalty_lin", f32p), ("penalty_ang", f32p),
-        ("lambda_lin", f32p), ("lambda_ang", f32p), ("broken", u8p),
+        ("lambda_lin", f32p), ("lambda_ang", f32p), ("broken", u8p), ("kind", u8p), ("rest", f32p),
     ]
 
 
@@ -134,8 +134,9 @@ JOINT_FIELDS = {
     "stiffness_lin": (np.float32, 1), "stiffness_ang": (np.float32, 1), "fracture": (np.float32, 1),
     "torque_arm": (np.float32, 1), "c0_lin": (np.float32, 3), "c0_ang": (np.float32, 3),
     "penalty_lin": (np.float32, 3), "penalty_ang": (np.float32, 3), "lambda_lin": (np.float32, 3),
-    "lambda_ang": (np.float32, 3), "broken": (np.uint8, 1),
+    "lambda_ang": (np.float32, 3), "broken": (np.uint8, 1), "kind": (np.uint8, 1), "rest": (np.float32, 1),
 }
+FORCE_JOINT, FORCE_SPRING, FORCE_IGNORE_COLLISION = 0, 1, 2
 MANIFOLD_FIELDS = {
     "body_a": (np.int32, 1), "body_b": (np.int32, 1), "basis": (np.float32, 9), "friction": (np.float32, 1),
     "num_contacts": (np.int32, 1), "feature": (np.int32, 8), "r_a": (np.float32, 24), "r_b": (np.float32, 24),

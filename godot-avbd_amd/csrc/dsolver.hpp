@@ -617,6 +617,50 @@ AD m3 gsBall(int k, v3 v)
     return m;
 }
 // joint.cpp:80-160
+// spring.cpp:31-68: a spring has no constraint stabilisation and no dual state.
+AD void springPrimal(const State &S, int j, int body, Block &blk)
+{
+    const int a = S.jA[j], b = S.jB[j];
+    const v3 rA = ld3(S.jRA[j]), rB = ld3(S.jRB[j]);
+    const qt qa = bRot(S, a), qb = bRot(S, b);
+    const v3 pA = bPos(S, a) + qrot(qa, rA);
+    const v3 pB = bPos(S, b) + qrot(qb, rB);
+    const v3 d = pA - pB;
+    const float dLen = length(d);
+    if (dLen <= 1.0e-6f) return;
+    const v3 n = d / dLen;
+    const float k = S.jStiffL[j];
+    const float C = dLen - S.jRest[j];
+    const float f = k * C;
+    v3 jLin, jAng;
+    if (body == a) {
+        jLin = n;
+        jAng = cross(qrot(qa, rA), n);
+    } else {
+        jLin = -n;
+        jAng = -cross(qrot(qb, rB), n);
+    }
+    const v3 F = jLin * f, Tau = jAng * f;
+    // outer products a b^T (column j = a * b[j]) scaled by the stiffness
+    const m3 Kll = mscale(m3{{jLin * jLin.x, jLin * jLin.y, jLin * jLin.z}}, k);
+    const m3 Kla = mscale(m3{{jAng * jLin.x, jAng * jLin.y, jAng * jLin.z}}, k);
+    const m3 Kaa = mscale(m3{{jAng * jAng.x, jAng * jAng.y, jAng * jAng.z}}, k);
+    blk.lhsLin = madd(blk.lhsLin, Kll);
+    blk.lhsAng = madd(blk.lhsAng, Kaa);
+    blk.lhsCross = madd(blk.lhsCross, Kla);
+    blk.rhsLin += F;
+    blk.rhsAng += Tau;
+}
+
+// A force of the body's list: joint, spring, or a collision filter (which stamps nothing).
+AD void jointPrimal(const State &S, int j, int body, Block &blk);
+AD void forcePrimal(const State &S, int j, int body, Block &blk)
+{
+    const uint8_t kind = S.jKind[j];
+    if (kind == FORCE_JOINT) jointPrimal(S, j, body, blk);
+    else if (kind == FORCE_SPRING) springPrimal(S, j, body, blk);
+}
+
 AD void jointPrimal(const State &S, int j, int body, Block &blk)
 {
     const bool isA = body == S.jA[j];
@@ -662,6 +706,7 @@ AD void jointPrimal(const State &S, int j, int body, Block &blk)
 // joint.cpp:171-230
 AD void jointDual(const State &S, int j)
 {
+    if (S.jKind[j] != FORCE_JOINT) return;   // springs and collision filters keep no dual state
     const float pmax = S.P.penaltyMax;
     v3 penL = ld3(S.jPenL[j]), penA = ld3(S.jPenA[j]);
     v3 lamL = ld3(S.jLamL[j]), lamA = ld3(S.jLamA[j]);
@@ -711,7 +756,7 @@ AD void assemble(const State &S, int body, Block &blk)
     for (int e = e0; e < e1; e++) {
         const int ref = S.adj[e];
         if (ref >= 0) stampManifold(S, adjManifold(ref), body, blk);
-        else jointPrimal(S, -ref - 1, body, blk);
+        else forcePrimal(S, -ref - 1, body, blk);
     }
 }
 
@@ -723,7 +768,7 @@ AD void assembleStamped(const State &S, int body, Block &blk)
     for (int e = e0; e < e1; e++) {
         const int ref = S.adj[e];
         if (ref >= 0) addStampSide(blk, (adjSideB(ref) ? S.stampB : S.stampA) + 7 * static_cast<size_t>(adjManifold(ref)));
-        else jointPrimal(S, -ref - 1, body, blk);
+        else forcePrimal(S, -ref - 1, body, blk);
     }
 }
 

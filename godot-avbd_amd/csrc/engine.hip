@@ -268,7 +268,7 @@ struct Engine {
         AL(S.adjCount, N + 1); AL(S.adjStart, N + 1); AL(S.adj, 2 * (M + J)); AL(S.adjTmp, 2 * (M + J));
         AL(S.jA, J); AL(S.jB, J); AL(S.jWorld, J); AL(S.jRA, J); AL(S.jRB, J); AL(S.jC0L, J); AL(S.jC0A, J);
         AL(S.jPenL, J); AL(S.jPenA, J); AL(S.jLamL, J); AL(S.jLamA, J); AL(S.jStiffL, J); AL(S.jStiffA, J);
-        AL(S.jFrac, J); AL(S.jArm, J); AL(S.jBroken, J); AL(S.jDead, J);
+        AL(S.jFrac, J); AL(S.jArm, J); AL(S.jBroken, J); AL(S.jDead, J); AL(S.jKind, J); AL(S.jRest, J);
         AL(S.mA, M); AL(S.mB, M); AL(S.mBasis, 3 * M); AL(S.mFric, M); AL(S.mNum, M);
         // sliced contacts: at most MAXC rows per lane of every slice, the last slice padded
         const size_t CD = (static_cast<size_t>(Scap) + CSLICE) * MAXC;
@@ -841,8 +841,8 @@ struct Engine {
         if (nj > 0) {
             std::vector<int> ja(nj), jb(nj), jw(nj, w);
             std::vector<float4> ra(nj), rb(nj), c0l(nj), c0a(nj), pl(nj), pa(nj), ll2(nj), la2(nj);
-            std::vector<float> sl(nj), sa(nj), fr(nj), arm(nj);
-            std::vector<uint8_t> br(nj), dead(nj, 0);
+            std::vector<float> sl(nj), sa(nj), fr(nj), arm(nj), rest(nj, 0.0f);
+            std::vector<uint8_t> br(nj), dead(nj, 0), kind(nj, AVBD_FORCE_JOINT);
             auto f4 = [](const float *p, int k) { return p ? make_float4(p[3 * k], p[3 * k + 1], p[3 * k + 2], 0) : make_float4(0, 0, 0, 0); };
             for (int k = 0; k < nj; k++) {
                 if (j->body_b[k] < 0 || j->body_b[k] >= n || j->body_a[k] >= n)
@@ -856,6 +856,15 @@ struct Engine {
                 sl[k] = j->stiffness_lin[k]; sa[k] = j->stiffness_ang[k];
                 fr[k] = j->fracture[k]; arm[k] = j->torque_arm[k];
                 br[k] = j->broken ? j->broken[k] : 0;
+                if (j->kind) kind[k] = j->kind[k];
+                if (kind[k] > AVBD_FORCE_IGNORE_COLLISION) return fail(AVBD_ERR_INVALID, "unknown force kind");
+                if (kind[k] == AVBD_FORCE_SPRING) {
+                    // spring.cpp:19-28 reads both bodies' poses
+                    if (j->body_a[k] < 0) return fail(AVBD_ERR_INVALID, "a spring needs two bodies");
+                    if (!j->rest) return fail(AVBD_ERR_INVALID, "spring rest lengths are null");
+                    rest[k] = j->rest[k];
+                }
+                if (kind[k] != AVBD_FORCE_JOINT) br[k] = 0;   // only joints break
             }
             const int jo = wi.jointBase;
             auto putj = [&](auto *dst, const auto &src) {
@@ -865,7 +874,8 @@ struct Engine {
             HIPCHK(putj(S.jRB, rb)); HIPCHK(putj(S.jC0L, c0l)); HIPCHK(putj(S.jC0A, c0a)); HIPCHK(putj(S.jPenL, pl));
             HIPCHK(putj(S.jPenA, pa)); HIPCHK(putj(S.jLamL, ll2)); HIPCHK(putj(S.jLamA, la2));
             HIPCHK(putj(S.jStiffL, sl)); HIPCHK(putj(S.jStiffA, sa)); HIPCHK(putj(S.jFrac, fr)); HIPCHK(putj(S.jArm, arm));
-            HIPCHK(putj(S.jBroken, br)); HIPCHK(putj(S.jDead, dead));
+            HIPCHK(putj(S.jBroken, br)); HIPCHK(putj(S.jDead, dead)); HIPCHK(putj(S.jKind, kind));
+            HIPCHK(putj(S.jRest, rest));
         }
         // slot range ownership and empty manifold store
         std::vector<int> sw(slotCap, w);

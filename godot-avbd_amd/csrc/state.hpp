@@ -30,6 +30,7 @@ struct Params {
 };
 
 constexpr int NARROW_REC = 8;
+constexpr uint8_t FORCE_JOINT = 0, FORCE_SPRING = 1, FORCE_IGNORE = 2;
 
 struct State {
     Params P;
@@ -70,6 +71,8 @@ struct State {
     int *jA, *jB, *jWorld;
     float4 *jRA, *jRB, *jC0L, *jC0A, *jPenL, *jPenA, *jLamL, *jLamA;
     float *jStiffL, *jStiffA, *jFrac, *jArm;
+    uint8_t *jKind;    // FORCE_JOINT, FORCE_SPRING, FORCE_IGNORE
+    float *jRest;      // spring rest length
     uint8_t *jBroken, *jDead;
 
     // ---- manifold slots

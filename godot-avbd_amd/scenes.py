@@ -51,6 +51,25 @@ class SceneBuilder:
                            k_lin=F(k_lin), k_ang=F(k_ang), fracture=F(fracture), arm=F(arm)))
         return len(self.j) - 1
 
+    def spring(self, a, b, r_a, r_b, stiffness, rest=-1.0):
+        """Reference `Spring` (spring.cpp:16-28) between bodies a and b; a negative rest length
+        takes the anchors' current distance (computed here in float64, then rounded)."""
+        r_a, r_b = np.asarray(r_a, F), np.asarray(r_b, F)
+        if rest < 0:
+            pa = self.b[a]["position"].astype(np.float64) + _qrot64(self.b[a]["rotation"], r_a)
+            pb = self.b[b]["position"].astype(np.float64) + _qrot64(self.b[b]["rotation"], r_b)
+            rest = float(np.linalg.norm(pa - pb))
+        self.j.append(dict(kind=1, a=a, b=b, r_a=r_a, r_b=r_b, k_lin=F(stiffness), k_ang=F(0.0),
+                           fracture=F(math.inf), arm=F(0.0), rest=F(rest)))
+        return len(self.j) - 1
+
+    def ignore_collision(self, a, b):
+        """Reference `IgnoreCollision` (forces.hpp:104-113): the pair skips the broad phase."""
+        z = np.zeros(3, F)
+        self.j.append(dict(kind=2, a=a, b=b, r_a=z, r_b=z, k_lin=F(0.0), k_ang=F(0.0),
+                           fracture=F(math.inf), arm=F(0.0), rest=F(0.0)))
+        return len(self.j) - 1
+
     def build(self):
         """(bodies, joints) arrays in list order."""
         n = len(self.b)
@@ -82,11 +101,48 @@ class SceneBuilder:
                 "stiffness_ang": np.array([self.j[c]["k_ang"] for c in jorder], F),
                 "fracture": np.array([self.j[c]["fracture"] for c in jorder], F),
                 "torque_arm": np.array([self.j[c]["arm"] for c in jorder], F),
+                "kind": np.array([self.j[c].get("kind", 0) for c in jorder], np.uint8),
+                "rest": np.array([self.j[c].get("rest", 0.0) for c in jorder], F),
             }
             for k in ("c0_lin", "c0_ang", "penalty_lin", "penalty_ang", "lambda_lin", "lambda_ang"):
                 joints[k] = np.zeros((nj, 3), F)
             joints["broken"] = np.zeros(nj, np.uint8)
         return bodies, joints
+
+
+def _qrot64(q, v):
+    x, y, z, w = (float(t) for t in q)
+    u = np.array([x, y, z])
+    v = np.asarray(v, np.float64)
+    return 2.0 * np.dot(u, v) * u + (w * w - np.dot(u, u)) * v + 2.0 * w * np.cross(u, v)
+
+
+def springs_and_filters(seed=5):
+    """Forces of every kind in one world: a pile whose boxes are linked in pairs by springs
+    (some with off-centre anchors), boxes hanging from static anchors by springs, jointed
+    pairs, and collision filters between boxes that start interpenetrating (they pass through
+    each other) and between a box and the ground (it falls through)."""
+    rng = np.random.default_rng(seed)
+    s = SceneBuilder()
+    ground = s.box((40, 40, 1), 0.0, 0.5, (0, 0, 0))
+    ids = []
+    for i in range(24):
+        p = (F(rng.uniform(-4, 4)), F(rng.uniform(-4, 4)), F(1.5 + 1.1 * i / 3))
+        ids.append(s.box((1, 1, 1), 1.0, 0.5, p))
+    for k in range(0, 20, 2):
+        off = (0.3, 0.0, 0.2) if k % 4 == 0 else (0.0, 0.0, 0.0)
+        s.spring(ids[k], ids[k + 1], off, (0, 0, 0), 200.0 + 50.0 * k)
+    s.joint(ids[20], ids[21], (0.6, 0, 0), (-0.6, 0, 0))
+    for h in range(3):
+        anchor = s.box((0.5, 0.5, 0.5), 0.0, 0.5, (F(8 + 2 * h), 0, F(8)))
+        bob = s.box((1, 1, 1), 1.0, 0.5, (F(8 + 2 * h), 0, F(6)))
+        s.spring(anchor, bob, (0, 0, 0), (0.2 * h, 0, 0.5), 300.0, rest=1.0)
+    a = s.box((1, 1, 1), 1.0, 0.5, (F(-8), 0, F(2)))
+    b = s.box((1, 1, 1), 1.0, 0.5, (F(-7.6), 0, F(2.3)))
+    s.ignore_collision(a, b)
+    faller = s.box((1, 1, 1), 1.0, 0.5, (F(-12), 0, F(1.2)))
+    s.ignore_collision(ground, faller)
+    return s.build()
 
 
 def digest_stack():

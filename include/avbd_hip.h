@@ -113,7 +113,18 @@ typedef struct avbd_joints {
     float *lambda_lin;    /* 3n */
     float *lambda_ang;    /* 3n */
     uint8_t *broken;      /* n */
+    /* Force kind per entry (NULL: every entry is a joint).  The forces share one list, in the
+     * reference's list order (forces.hpp:19-113): AVBD_FORCE_JOINT is the joint above;
+     * AVBD_FORCE_SPRING is a spring between two bodies (spring.cpp:16-75) with anchors r_a, r_b,
+     * stiffness in stiffness_lin and rest length in rest; AVBD_FORCE_IGNORE_COLLISION keeps the
+     * pair of bodies out of the broad phase and acts on nothing else (forces.hpp:104-113). */
+    uint8_t *kind;        /* n, or NULL */
+    float *rest;          /* n spring rest length (read for springs; NULL when there are none) */
 } avbd_joints;
+
+#define AVBD_FORCE_JOINT 0
+#define AVBD_FORCE_SPRING 1
+#define AVBD_FORCE_IGNORE_COLLISION 2
 
 #define AVBD_MAX_CONTACTS 8
 

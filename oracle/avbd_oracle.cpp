@@ -53,9 +53,12 @@ struct Body {
     bool inUpdate = false; int updateIndex = -1; bool contactOnly = true;
 };
 
+// One entry of the force list (forces.hpp:19-113): a joint, a spring (spring.cpp) or a
+// collision filter (IgnoreCollision: in the list, acts on nothing).
 struct Joint {
     int a, b; f3 rA, rB, C0Lin, C0Ang, penLin, penAng, lamLin, lamAng;
     float stiffLin, stiffAng, fracture, torqueArm; bool broken;
+    uint8_t kind = AVBD_FORCE_JOINT; float rest = 0.0f;
 };
 
 struct Contact { int32_t key; f3 rA, rB; };
@@ -929,6 +932,7 @@ struct World {
     }
     bool jointInit(Joint &j)
     {
+        if (j.kind != AVBD_FORCE_JOINT) return true;   // Spring/IgnoreCollision::initialize
         j.C0Lin = jointLin(j);
         j.C0Ang = jointAng(j);
         j.lamLin = j.lamLin * P.alpha * P.gamma;
@@ -942,6 +946,44 @@ struct World {
         m3 m = diagonal(-v[k], -v[k], -v[k]);
         m[k][0] += v[0]; m[k][1] += v[1]; m[k][2] += v[2];
         return m;
+    }
+    // spring.cpp:31-68
+    void springPrimal(const Joint &j, int body, Block &blk) const
+    {
+        const Body &A = bodies[j.a], &B = bodies[j.b];
+        const f3 pA = A.pos + qrot(A.q, j.rA);
+        const f3 pB = B.pos + qrot(B.q, j.rB);
+        const f3 d = pA - pB;
+        const float dLen = length(d);
+        if (dLen <= 1.0e-6f) return;
+        const f3 n = d / dLen;
+        const float C = dLen - j.rest;
+        const float f = j.stiffLin * C;
+        f3 jLin, jAng;
+        if (body == j.a) {
+            const f3 rWorld = qrot(A.q, j.rA);
+            jLin = n;
+            jAng = cross(rWorld, n);
+        } else {
+            const f3 rWorld = qrot(B.q, j.rB);
+            jLin = -n;
+            jAng = -cross(rWorld, n);
+        }
+        const f3 F = jLin * f, Tau = jAng * f;
+        // linalg outerprod(a, b): column j is a * b[j]
+        const m3 Kll = m3{{jLin * jLin.x, jLin * jLin.y, jLin * jLin.z}} * j.stiffLin;
+        const m3 Kla = m3{{jAng * jLin.x, jAng * jLin.y, jAng * jLin.z}} * j.stiffLin;
+        const m3 Kaa = m3{{jAng * jAng.x, jAng * jAng.y, jAng * jAng.z}} * j.stiffLin;
+        blk.lhsLin += Kll;
+        blk.lhsAng += Kaa;
+        blk.lhsCross += Kla;
+        blk.rhsLin += F;
+        blk.rhsAng += Tau;
+    }
+    void forcePrimal(const Joint &j, int body, Block &blk) const
+    {
+        if (j.kind == AVBD_FORCE_JOINT) jointPrimal(j, body, blk);
+        else if (j.kind == AVBD_FORCE_SPRING) springPrimal(j, body, blk);
     }
     // joint.cpp:80-160
     void jointPrimal(const Joint &j, int body, Block &blk) const
@@ -985,6 +1027,7 @@ struct World {
     // joint.cpp:171-230
     void jointDual(Joint &j) const
     {
+        if (j.kind != AVBD_FORCE_JOINT) return;   // Spring/IgnoreCollision::updateDual do nothing
         if (length2(j.penLin) > 0) {
             const m3 K = diagonal(j.penLin.x, j.penLin.y, j.penLin.z);
             f3 C = jointLin(j);
@@ -1139,7 +1182,7 @@ struct World {
         for (uint32_t e = adjStart[body]; e < adjStart[body + 1]; e++) {
             const int ref = adj[e];
             if (ref >= 0) stampManifold(static_cast<uint32_t>(ref), body, blk);
-            else jointPrimal(joints[-ref - 1], body, blk);
+            else forcePrimal(joints[-ref - 1], body, blk);
         }
         return blk;
     }
@@ -1438,6 +1481,13 @@ int oracle_upload(oracle_world *o, const avbd_bodies *b, const avbd_joints *j)
             x.lamLin = j->lambda_lin ? load3(j->lambda_lin, k) : f3{0, 0, 0};
             x.lamAng = j->lambda_ang ? load3(j->lambda_ang, k) : f3{0, 0, 0};
             x.broken = j->broken ? j->broken[k] != 0 : false;
+            x.kind = j->kind ? j->kind[k] : AVBD_FORCE_JOINT;
+            if (x.kind > AVBD_FORCE_IGNORE_COLLISION) return AVBD_ERR_INVALID;
+            if (x.kind == AVBD_FORCE_SPRING) {
+                if (x.a < 0 || !j->rest) return AVBD_ERR_INVALID;
+                x.rest = j->rest[k];
+            }
+            if (x.kind != AVBD_FORCE_JOINT) x.broken = false;
             if (x.b < 0 || x.b >= b->count || x.a >= b->count) return AVBD_ERR_INVALID;
         }
     }

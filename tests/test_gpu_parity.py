@@ -81,6 +81,23 @@ def test_pile_lockstep_and_manifolds():
     manifolds_equal(w, o)
 
 
+def test_springs_and_collision_filters_lockstep():
+    """Springs (centred and off-centre anchors, between bodies and from static anchors), a
+    joint and collision filters in one force list, bit for bit against the oracle; the
+    filtered pairs make no manifold."""
+    b, j = scenes.springs_and_filters()
+    w, o = lockstep(b, j, P(iterations=10), 120, every=10)
+    manifolds_equal(w, o)
+    gs, os_ = w.stats(), o.stats()
+    for k in ("manifolds", "contacts", "candidate_pairs", "colours", "overflow_bodies"):
+        assert gs[k] == os_[k], k
+    kinds = j["kind"]
+    pairs = {(min(a, c), max(a, c)) for a, c, kk in zip(j["body_a"], j["body_b"], kinds)
+             if kk == pkg.abi.FORCE_IGNORE_COLLISION}
+    m = w.manifolds()
+    assert not pairs & {(min(a, c), max(a, c)) for a, c in zip(m["body_a"], m["body_b"])}
+
+
 def test_chains_colour_sweep_lockstep():
     p = P(iterations=10, beta_lin=1e5, penalty_max=1e9)
     w, o = lockstep(*scenes.chains(16, links=32, grid=(4, 4), pitch_x=4.0), p, 150, every=15)

@@ -242,3 +242,52 @@ def test_colouring_matches_sequential_greedy(scene, overflow):
     state = {**bodies, **o.download()}
     ref = scenes.greedy_colours(state, joints, o.manifolds(), contact_only_overflow=overflow)
     assert np.array_equal(o.colours(), ref)
+
+
+# ---------------------------------------------------------------- springs, filters ----
+# spring.cpp:31-68 and forces.hpp:104-113 have no recorded digest or fixture in the reference;
+# the oracle's restatement is pinned by closed forms: a spring's static equilibrium, where its
+# pull k (|d| - rest) equals the body's weight and the anchor point hangs straight above the
+# centre of mass; a collision filter lets a box fall through the ground it would rest on.
+
+def _hanging(r_b, steps):
+    s = scenes.SceneBuilder()
+    anchor = s.box((0.5, 0.5, 0.5), 0.0, 0.5, (0, 0, 10))
+    bob = s.box((1, 1, 1), 1.0, 0.5, (0, 0, 8.5))
+    s.spring(anchor, bob, (0, 0, 0), r_b, 100.0, rest=1.0)
+    o = run(*s.build(), P(iterations=10), steps)
+    st = o.download()
+    return st["position"][0].astype(np.float64), st["rotation"][0].astype(np.float64)
+
+
+def test_spring_equilibrium_centred():
+    """m g = k (L - rest): a 1 kg box on a k = 100 spring of rest length 1 hangs 1.1 below."""
+    p, q = _hanging((0, 0, 0), 600)
+    assert abs(p[2] - (10.0 - 1.1)) < 1e-3 and abs(p[0]) < 1e-6 and abs(p[1]) < 1e-6
+    assert np.allclose(q, [0, 0, 0, 1], atol=1e-6)
+
+
+def test_spring_equilibrium_off_centre():
+    """Anchored off the centre, the box turns until the anchor point is straight above the
+    centre of mass, 1.1 below the static anchor (wrong arm or torque signs never settle there)."""
+    r_b = np.array([0.3, 0.0, 0.5])
+    p, q = _hanging(r_b, 2400)
+    attach = p + scenes._qrot64(q, r_b)
+    assert abs(np.linalg.norm(attach - np.array([0, 0, 10.0])) - 1.1) < 2e-3
+    assert abs(attach[0] - p[0]) < 2e-2 and abs(attach[1] - p[1]) < 1e-6
+    assert abs((attach[2] - p[2]) - np.linalg.norm(r_b)) < 2e-3
+
+
+@pytest.mark.parametrize("filtered", [False, True])
+def test_ignore_collision_drops_the_pair(filtered):
+    s = scenes.SceneBuilder()
+    ground = s.box((40, 40, 1), 0.0, 0.5, (0, 0, 0))
+    box = s.box((1, 1, 1), 1.0, 0.5, (0, 0, 1.0))
+    if filtered:
+        s.ignore_collision(ground, box)
+    o = run(*s.build(), P(iterations=10), 30)
+    z = o.download()["position"][0][2]
+    if filtered:
+        assert len(o.manifolds()["body_a"]) == 0 and z < 0.0   # fell through the ground
+    else:
+        assert len(o.manifolds()["body_a"]) == 1 and abs(z - 1.0) < 5e-3
