@@ -1,11 +1,13 @@
 // avbd_solver.hpp - C++ facade with the reference solver's interface over the C ABI.
 //
 // A drop-in for the reference's in-process API (src/avbd/solver.h:50-374,
-// src/avbd/types.hpp:75-139, src/avbd/forces.hpp:19-87): bodies and joints are created with
-// `new Rigid(&solver, size, density, friction, position[, velocity])` and
-// `new Joint(&solver, a, b, rA, rB[, kLin, kAng, fracture])`, which prepend themselves to the
-// solver's intrusive lists exactly as the reference's do; the caller sets the same public
-// fields, calls step() / stepInterleaved(), and reads the same `Rigid` / `Joint` fields.
+// src/avbd/types.hpp:75-139, src/avbd/forces.hpp:19-113): bodies and forces are created with
+// `new Rigid(&solver, size, density, friction, position[, velocity])`,
+// `new Joint(&solver, a, b, rA, rB[, kLin, kAng, fracture])`,
+// `new Spring(&solver, a, b, rA, rB, stiffness[, rest])` and `new IgnoreCollision(&solver, a, b)`,
+// which prepend themselves to the solver's intrusive lists exactly as the reference's do; the
+// caller sets the same public fields, calls step() / stepInterleaved(), and reads the same
+// `Rigid` / `Joint` fields.
 // The state lives on the GPU: each step writes the bodies' dynamic fields to the device,
 // steps there (avbd_world_step), and reads them back.  A topology change (a body or joint
 // created or deleted) re-uploads the lists and carries every surviving manifold over, so
@@ -72,6 +74,7 @@ struct Force {
     Rigid *bodyB;
     Force *next = nullptr;
     void *owner = nullptr;
+    uint8_t kind = AVBD_FORCE_JOINT;   // which of the force types below this is
     inline Force(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bodyB);
     inline virtual ~Force();
     virtual bool isBroken() const { return false; }
@@ -102,6 +105,36 @@ struct Joint : Force {
     bool isBroken() const override { return broken; }
     bool dirty = false;   // host-side edit to push before the next step
 };
+
+// reference forces.hpp:89-101 (spring.cpp): a spring between two bodies; a negative rest
+// length takes the anchors' distance at creation.
+struct Spring : Force {
+    float3 rA, rB;
+    float rest;
+    float stiffness;
+    inline Spring(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bodyB, float3 p_rA, float3 p_rB, float p_stiffness,
+                  float p_rest = -1);
+};
+
+// reference forces.hpp:103-113: no physical effect; the two bodies skip each other in the
+// broad phase.
+struct IgnoreCollision : Force {
+    inline IgnoreCollision(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bodyB);
+};
+
+// reference maths.h:65-71 (Eigen's order) and linalg's folds, for the host-side rest length
+inline float3 hostCross(float3 a, float3 b)
+{
+    return float3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline float3 hostQrot(quat q, float3 v)
+{
+    const float3 z{q.x, q.y, q.z};
+    float3 uv = hostCross(z, v);
+    uv = float3{uv.x + uv.x, uv.y + uv.y, uv.z + uv.z};
+    const float3 c = hostCross(z, uv);
+    return float3{(v.x + uv.x * q.w) + c.x, (v.y + uv.y * q.w) + c.y, (v.z + uv.z * q.w) + c.z};
+}
 
 // reference solver.h:50-144
 struct Solver {
@@ -191,7 +224,7 @@ struct Solver {
     avbd_world *world = nullptr;
     int capBodies = 0, capJoints = 0;
     std::vector<Rigid *> uploaded;
-    std::vector<Joint *> uploadedJoints;
+    std::vector<Force *> uploadedJoints;
 
     avbd_params params() const
     {
@@ -270,7 +303,7 @@ struct Solver {
     }
 
     // Re-upload the lists, carrying the manifolds of surviving body pairs over.
-    bool rebuild(const std::vector<Rigid *> &list, const std::vector<Joint *> &jl)
+    bool rebuild(const std::vector<Rigid *> &list, const std::vector<Force *> &jl)
     {
         const int n = static_cast<int>(list.size()), nj = static_cast<int>(jl.size());
         std::vector<int32_t> ma, mb, num, feat;
@@ -319,24 +352,36 @@ struct Solver {
         std::vector<int32_t> ja(nj), jb(nj);
         std::vector<float> jra(3 * nj), jrb(3 * nj), ksl(nj), ksa(nj), jfr(nj), arm(nj), c0l(3 * nj), c0a(3 * nj),
                 pl(3 * nj), pa(3 * nj), ll(3 * nj), la(3 * nj);
-        std::vector<uint8_t> br(nj);
+        std::vector<uint8_t> br(nj), kind(nj);
+        std::vector<float> rest(nj, 0.0f);
         std::unordered_map<const Rigid *, int> idx;
         for (int i = 0; i < n; i++) idx[list[i]] = i;
         for (int k = 0; k < nj; k++) {
-            const Joint *j = jl[k];
-            ja[k] = j->bodyA ? idx[j->bodyA] : -1;
-            jb[k] = idx[j->bodyB];
-            for (int c = 0; c < 3; c++) {
-                jra[3 * k + c] = j->rA[c]; jrb[3 * k + c] = j->rB[c];
-                c0l[3 * k + c] = j->C0Lin[c]; c0a[3 * k + c] = j->C0Ang[c];
-                pl[3 * k + c] = j->penaltyLin[c]; pa[3 * k + c] = j->penaltyAng[c];
-                ll[3 * k + c] = j->lambdaLin[c]; la[3 * k + c] = j->lambdaAng[c];
+            const Force *f = jl[k];
+            ja[k] = f->bodyA ? idx[f->bodyA] : -1;
+            jb[k] = idx[f->bodyB];
+            kind[k] = f->kind;
+            jfr[k] = std::numeric_limits<float>::infinity();
+            if (f->kind == AVBD_FORCE_JOINT) {
+                const Joint *j = static_cast<const Joint *>(f);
+                for (int c = 0; c < 3; c++) {
+                    jra[3 * k + c] = j->rA[c]; jrb[3 * k + c] = j->rB[c];
+                    c0l[3 * k + c] = j->C0Lin[c]; c0a[3 * k + c] = j->C0Ang[c];
+                    pl[3 * k + c] = j->penaltyLin[c]; pa[3 * k + c] = j->penaltyAng[c];
+                    ll[3 * k + c] = j->lambdaLin[c]; la[3 * k + c] = j->lambdaAng[c];
+                }
+                ksl[k] = j->stiffnessLin; ksa[k] = j->stiffnessAng; jfr[k] = j->fracture; arm[k] = j->torqueArm;
+                br[k] = j->broken ? 1 : 0;
+            } else if (f->kind == AVBD_FORCE_SPRING) {
+                const Spring *sp = static_cast<const Spring *>(f);
+                for (int c = 0; c < 3; c++) { jra[3 * k + c] = sp->rA[c]; jrb[3 * k + c] = sp->rB[c]; }
+                ksl[k] = sp->stiffness;
+                rest[k] = sp->rest;
             }
-            ksl[k] = j->stiffnessLin; ksa[k] = j->stiffnessAng; jfr[k] = j->fracture; arm[k] = j->torqueArm;
-            br[k] = j->broken ? 1 : 0;
         }
         avbd_joints jv{nj, ja.data(), jb.data(), jra.data(), jrb.data(), ksl.data(), ksa.data(), jfr.data(),
-                       arm.data(), c0l.data(), c0a.data(), pl.data(), pa.data(), ll.data(), la.data(), br.data()};
+                       arm.data(), c0l.data(), c0a.data(), pl.data(), pa.data(), ll.data(), la.data(), br.data(),
+                       kind.data(), rest.data()};
         lastStatus = avbd_world_upload(world, &bv, &jv);
         if (lastStatus != AVBD_OK) return false;
         if (keepCount > 0) {
@@ -354,11 +399,13 @@ struct Solver {
     {
         std::vector<Rigid *> list;
         for (Rigid *r = bodies; r; r = r->next) list.push_back(r);
-        std::vector<Joint *> jl;
+        std::vector<Force *> jl;
         for (Force *f = forces; f; f = f->next) {
-            Joint *j = static_cast<Joint *>(f);
-            if (j->dirty) { topologyChanged = true; j->dirty = false; }
-            jl.push_back(j);
+            if (f->kind == AVBD_FORCE_JOINT) {
+                Joint *j = static_cast<Joint *>(f);
+                if (j->dirty) { topologyChanged = true; j->dirty = false; }
+            }
+            jl.push_back(f);
         }
         const avbd_params p = params();
         if (world) {
@@ -376,16 +423,16 @@ struct Solver {
             if (lastStatus != AVBD_OK) return;
         }
         // joints that were broken before this step are reaped by it (solver_solve.cpp:39-51)
-        std::vector<Joint *> reaped;
-        for (Joint *j : jl)
-            if (j->broken) reaped.push_back(j);
+        std::vector<Force *> reaped;   // only joints break
+        for (Force *f : jl)
+            if (f->kind == AVBD_FORCE_JOINT && static_cast<Joint *>(f)->broken) reaped.push_back(f);
         collisionDt = substeps > 1 ? dt * static_cast<float>(substeps) : dt;
         lastStatus = substeps > 1 ? avbd_world_step_interleaved(world, substeps, rounds) : avbd_world_step(world);
         if (lastStatus != AVBD_OK) return;
         pull(list, jl, reaped);
     }
 
-    void pull(const std::vector<Rigid *> &list, std::vector<Joint *> jl, const std::vector<Joint *> &reaped)
+    void pull(const std::vector<Rigid *> &list, std::vector<Force *> jl, const std::vector<Force *> &reaped)
     {
         const int n = static_cast<int>(list.size());
         std::vector<float> pos(3 * n), rot(4 * n), vel(3 * n), ang(3 * n), prev(3 * n);
@@ -412,14 +459,15 @@ struct Solver {
             r->positionAng = quat{rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]};
         }
         // the device compacted the reaped joints away; drop them from the host list too
-        for (Joint *dead : reaped) {
+        for (Force *dead : reaped) {
             for (size_t k = 0; k < jl.size(); k++)
                 if (jl[k] == dead) { jl.erase(jl.begin() + static_cast<long>(k)); break; }
-            dead->dirty = false;
+            static_cast<Joint *>(dead)->dirty = false;
             delete dead;
         }
         for (int k = 0; k < j.count && k < static_cast<int>(jl.size()); k++) {
-            Joint *x = jl[k];
+            if (jl[k]->kind != AVBD_FORCE_JOINT) continue;   // springs and filters keep no dual state
+            Joint *x = static_cast<Joint *>(jl[k]);
             for (int c = 0; c < 3; c++) {
                 x->C0Lin[c] = c0l[3 * k + c]; x->C0Ang[c] = c0a[3 * k + c];
                 x->penaltyLin[c] = pl[3 * k + c]; x->penaltyAng[c] = pa[3 * k + c];
@@ -481,6 +529,28 @@ inline Joint::Joint(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bodyB, float3 p_r
     const float3 sa = p_bodyA ? p_bodyA->size : float3{0, 0, 0};
     const float sx = sa.x + p_bodyB->size.x, sy = sa.y + p_bodyB->size.y, sz = sa.z + p_bodyB->size.z;
     torqueArm = ((0.0f + sx * sx) + sy * sy) + sz * sz;
+    p_solver->topologyChanged = true;
+}
+
+inline Spring::Spring(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bodyB, float3 p_rA, float3 p_rB, float p_stiffness,
+                      float p_rest)
+    : Force(p_solver, p_bodyA, p_bodyB), rA(p_rA), rB(p_rB), rest(p_rest), stiffness(p_stiffness)
+{
+    kind = AVBD_FORCE_SPRING;
+    if (rest < 0.0f) {   // spring.cpp:21-27
+        const float3 qa = hostQrot(p_bodyA->positionAng, rA), qb = hostQrot(p_bodyB->positionAng, rB);
+        const float dx = (p_bodyA->positionLin.x + qa.x) - (p_bodyB->positionLin.x + qb.x);
+        const float dy = (p_bodyA->positionLin.y + qa.y) - (p_bodyB->positionLin.y + qb.y);
+        const float dz = (p_bodyA->positionLin.z + qa.z) - (p_bodyB->positionLin.z + qb.z);
+        rest = std::sqrt(((0.0f + dx * dx) + dy * dy) + dz * dz);
+    }
+    p_solver->topologyChanged = true;
+}
+
+inline IgnoreCollision::IgnoreCollision(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bodyB)
+    : Force(p_solver, p_bodyA, p_bodyB)
+{
+    kind = AVBD_FORCE_IGNORE_COLLISION;
     p_solver->topologyChanged = true;
 }
 

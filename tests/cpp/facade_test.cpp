@@ -134,6 +134,28 @@ static void test_break_now_reaps()
     report(joints == 0 && link->positionLin.z < zHeld - 0.5f && s.lastStatus == AVBD_OK, "breakNow reaps joint", d);
 }
 
+// Springs and collision filters through the facade (forces.hpp:89-113): a 1 kg box on a
+// k = 100 spring of rest length 1 hangs m g / k = 0.1 below its rest length; a box filtered
+// against the ground falls through it while an unfiltered one rests on it.
+static void test_spring_and_filter()
+{
+    Solver s;
+    Rigid *ground = new Rigid(&s, {40, 40, 1}, 0.0f, 0.5f, {0, 0, 0});
+    Rigid *anchor = new Rigid(&s, {0.5f, 0.5f, 0.5f}, 0.0f, 0.5f, {10, 0, 10});
+    Rigid *bob = new Rigid(&s, {1, 1, 1}, 1.0f, 0.5f, {10, 0, 8.5f});
+    new Spring(&s, anchor, bob, {0, 0, 0}, {0, 0, 0}, 100.0f, 1.0f);
+    Rigid *resting = new Rigid(&s, {1, 1, 1}, 1.0f, 0.5f, {-5, 0, 1.0f});
+    Rigid *faller = new Rigid(&s, {1, 1, 1}, 1.0f, 0.5f, {5, 5, 1.0f});
+    new IgnoreCollision(&s, ground, faller);
+    for (int i = 0; i < 600; i++) s.step();
+    char d[160];
+    std::snprintf(d, sizeof(d), "spring z %.5f (closed form 8.9), resting z %.4f, filtered z %.2f", bob->positionLin.z,
+                  resting->positionLin.z, faller->positionLin.z);
+    report(std::fabs(bob->positionLin.z - 8.9f) < 1e-3f && std::fabs(resting->positionLin.z - 1.0f) < 5e-3f &&
+               faller->positionLin.z < -10.0f && s.lastStatus == AVBD_OK,
+           "spring equilibrium and collision filter", d);
+}
+
 int main()
 {
     const uint64_t want[3] = {0xfbac4f08cb1c977cull, 0xb03adeb37742850aull, 0xfb3ca7a3162035c2ull};
@@ -146,6 +168,7 @@ int main()
     }
     test_topology_change_keeps_warm_start();
     test_break_now_reaps();
+    test_spring_and_filter();
     std::printf("%d checks, %d failures\n", g_checks, g_failures);
     return g_failures == 0 ? 0 : 1;
 }
