@@ -84,6 +84,9 @@ class Joints(C.Structure):
         ("stiffness_lin", f32p), ("stiffness_ang", f32p), ("fracture", f32p), ("torque_arm", f32p),
         ("c0_lin", f32p), ("c0_ang", f32p), ("penalty_lin", f32p), ("penalty_ang", f32p),
         ("lambda_lin", f32p), ("lambda_ang", f32p), ("broken", u8p), ("kind", u8p), ("rest", f32p),
+        ("rest_rotation", f32p), ("axis_mode", u8p), ("axis_lower", f32p), ("axis_upper", f32p),
+        ("axis_spring", f32p), ("axis_equilibrium", f32p), ("axis_damping", f32p), ("axis_lambda", f32p),
+        ("axis_penalty", f32p), ("axis_initial", f32p),
     ]
 
 
@@ -135,8 +138,13 @@ JOINT_FIELDS = {
     "torque_arm": (np.float32, 1), "c0_lin": (np.float32, 3), "c0_ang": (np.float32, 3),
     "penalty_lin": (np.float32, 3), "penalty_ang": (np.float32, 3), "lambda_lin": (np.float32, 3),
     "lambda_ang": (np.float32, 3), "broken": (np.uint8, 1), "kind": (np.uint8, 1), "rest": (np.float32, 1),
+    "rest_rotation": (np.float32, 4), "axis_mode": (np.uint8, 6), "axis_lower": (np.float32, 6),
+    "axis_upper": (np.float32, 6), "axis_spring": (np.float32, 6), "axis_equilibrium": (np.float32, 6),
+    "axis_damping": (np.float32, 6), "axis_lambda": (np.float32, 6), "axis_penalty": (np.float32, 6),
+    "axis_initial": (np.float32, 6),
 }
-FORCE_JOINT, FORCE_SPRING, FORCE_IGNORE_COLLISION = 0, 1, 2
+FORCE_JOINT, FORCE_SPRING, FORCE_IGNORE_COLLISION, FORCE_GENERIC = 0, 1, 2, 3
+AXIS_FREE, AXIS_LOCKED, AXIS_LIMITED = 0, 1, 2
 MANIFOLD_FIELDS = {
     "body_a": (np.int32, 1), "body_b": (np.int32, 1), "basis": (np.float32, 9), "friction": (np.float32, 1),
     "num_contacts": (np.int32, 1), "feature": (np.int32, 8), "r_a": (np.float32, 24), "r_b": (np.float32, 24),
@@ -165,6 +173,14 @@ def _struct(cls, fields, arrays, count):
 def bodies_struct(arrays):
     n = len(arrays["mass"])
     return _struct(Bodies, BODY_FIELDS, arrays, n)
+
+
+DYNAMIC_FIELDS = ("position", "rotation", "velocity", "angular_velocity", "prev_velocity")
+
+
+def dynamic_bodies_struct(arrays, count):
+    """A Bodies view over only the dynamic fields in `arrays` (others null), for write_bodies."""
+    return _struct(Bodies, BODY_FIELDS, {k: v for k, v in arrays.items() if k in DYNAMIC_FIELDS}, count)
 
 
 def joints_struct(arrays):

@@ -96,6 +96,35 @@ AD qt qmul(qt a, qt b)
             a.w * b.z + a.z * b.w + a.x * b.y - a.y * b.x,
             a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
 }
+// atan2 for the generic joint's twist angles: the odd polynomial atan(a) = a P(a^2) on
+// [0, 1] (degree 19, least-squares fit to arctan with relative weights, <= 1.3 ulp there) after
+// the usual octant reduction; IEEE signs of zero as atan2.  The oracle evaluates the same
+// expression, so both sides agree bit for bit.
+AD float atan2Poly(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y);
+    const bool swap = ay > ax;
+    const float mx = swap ? ay : ax, mn = swap ? ax : ay;
+    float r = 0.0f;
+    if (mx > 0.0f) {
+        const float a = mn / mx, s = a * a;
+        float p = -0.0017904319101944566f;
+        p = p * s + 0.010887323878705502f;
+        p = p * s + -0.0310975369066f;
+        p = p * s + 0.05783987045288086f;
+        p = p * s + -0.0839373916387558f;
+        p = p * s + 0.10947515815496445f;
+        p = p * s + -0.14262957870960236f;
+        p = p * s + 0.199983611702919f;
+        p = p * s + -0.3333328664302826f;
+        p = p * s + 1.0f;
+        r = p * a;
+    }
+    if (swap) r = 1.5707963705062866f - r;
+    if (__builtin_signbit(x)) r = 3.1415927410125732f - r;
+    return __builtin_copysignf(r, y);
+}
+
 AD v3 qrot(qt q, v3 v)
 {
     const v3 z{q.x, q.y, q.z};

@@ -652,13 +652,206 @@ AD void springPrimal(const State &S, int j, int body, Block &blk)
     blk.rhsAng += Tau;
 }
 
-// A force of the body's list: joint, spring, or a collision filter (which stamps nothing).
+// ------------------------------------------------------ generic 6-DOF joint ----
+// generic_joint.cpp.  Axis a of joint j (0-2 linear x, y, z; 3-5 angular) is at 6 j + a:
+// mode (0 free, 1 locked, 2 limited), bounds, spring, and the dual state (lambda, penalty,
+// value at the step's start).
+constexpr uint8_t AXIS_FREE = 0, AXIS_LOCKED = 1, AXIS_LIMITED = 2;
+
+AD v3 frameAxis(int i) { return v3{i == 0 ? 1.0f : 0.0f, i == 1 ? 1.0f : 0.0f, i == 2 ? 1.0f : 0.0f}; }
+AD qt genericFrame(const State &S, int j) { return S.jA[j] >= 0 ? bRot(S, S.jA[j]) : qt{0.0f, 0.0f, 0.0f, 1.0f}; }
+
+// generic_joint.cpp:51-67: take the twist about frame axis i out of dev; r_angle its angle.
+AD void removeTwist(qt &dev, int i, float &angle)
+{
+    const v3 n = frameAxis(i);
+    const float proj = dot(v3{dev.x, dev.y, dev.z}, n);
+    const qt twist{dev.x * proj, dev.y * proj, dev.z * proj, dev.w};
+    if (sqrtf(qlength2(twist)) <= 1.0e-12f) { angle = 0.0f; return; }
+    angle = 2.0f * atan2Poly(proj, dev.w);
+    const float l2 = qlength2(twist);
+    const qt inv{-twist.x / l2, -twist.y / l2, -twist.z / l2, twist.w / l2};
+    dev = qmul(qnormalize(dev), inv);
+}
+
+// generic_joint.cpp:69-90: the three angular measurements (twist of the non-locked axes,
+// then swing components of the locked ones).
+AD void genericAngular(const State &S, int j, float (&out)[3])
+{
+    const qt qA = genericFrame(S, j);
+    const qt relNow = qmul(qconj(qA), qnormalize(bRot(S, S.jB[j])));
+    qt dev = qmul(qnormalize(ldq(S.jGRest[j])), qconj(relNow));
+    uint8_t mode[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) mode[i] = S.jGMode[6 * j + 3 + i];
+    // unrolled so that dev and out stay in registers
+    if (mode[0] != AXIS_LOCKED) removeTwist(dev, 0, out[0]);
+    if (mode[1] != AXIS_LOCKED) removeTwist(dev, 1, out[1]);
+    if (mode[2] != AXIS_LOCKED) removeTwist(dev, 2, out[2]);
+    if (mode[0] == AXIS_LOCKED) out[0] = dev.x * 2.0f;
+    if (mode[1] == AXIS_LOCKED) out[1] = dev.y * 2.0f;
+    if (mode[2] == AXIS_LOCKED) out[2] = dev.z * 2.0f;
+}
+
+// generic_joint.cpp:92-96: anchor A minus anchor B in the joint frame.
+AD v3 genericOffset(const State &S, int j, qt qA)
+{
+    const int a = S.jA[j], b = S.jB[j];
+    const v3 rA = ld3(S.jRA[j]);
+    const v3 anchorA = a >= 0 ? bPos(S, a) + qrot(qA, rA) : rA;
+    const v3 anchorB = bPos(S, b) + qrot(bRot(S, b), ld3(S.jRB[j]));
+    return qrot(qconj(qA), anchorA - anchorB);
+}
+
+// generic_joint.cpp:104-155: constraint value and force of axis x, false when it acts not.
+AD bool axisForce(const State &S, int x, float value, float alpha, float &C, float &F)
+{
+    const uint8_t mode = S.jGMode[x];
+    if (mode == AXIS_LOCKED) {
+        C = value;
+    } else if (mode == AXIS_LIMITED) {
+        if (value > S.jGHi[x]) C = value - S.jGHi[x];
+        else if (value < S.jGLo[x]) C = value - S.jGLo[x];
+        else return false;
+    } else {
+        return false;
+    }
+    if (mode == AXIS_LOCKED) C -= S.jGInit[x] * alpha;
+    float f = S.jGPen[x] * C + S.jGLam[x];
+    if (mode == AXIS_LIMITED) f = C > 0.0f ? smax(f, 0.0f) : smin(f, 0.0f);
+    F = f;
+    return true;
+}
+
+AD m3 outerScaled(v3 a, v3 b, float s) { return mscale(m3{{a * b.x, a * b.y, a * b.z}}, s); }
+
+// generic_joint.cpp:157-187
+AD void stampAxis(const State &S, int x, Block &blk, float value, float alpha, v3 jLin, v3 jAng, float rate)
+{
+    float C, F;
+    if (axisForce(S, x, value, alpha, C, F)) {
+        const float k = S.jGPen[x];
+        blk.lhsLin = madd(blk.lhsLin, outerScaled(jLin, jLin, k));
+        blk.lhsAng = madd(blk.lhsAng, outerScaled(jAng, jAng, k));
+        blk.lhsCross = madd(blk.lhsCross, outerScaled(jAng, jLin, k));
+        blk.rhsLin += jLin * F;
+        blk.rhsAng += jAng * F;
+    }
+    const float ks = S.jGK[x];
+    if (ks > 0.0f) {
+        const float Cs = value - S.jGEq[x];
+        const float Fs = ks * Cs - S.jGDamp[x] * rate;
+        blk.lhsLin = madd(blk.lhsLin, outerScaled(jLin, jLin, ks));
+        blk.lhsAng = madd(blk.lhsAng, outerScaled(jAng, jAng, ks));
+        blk.lhsCross = madd(blk.lhsCross, outerScaled(jAng, jLin, ks));
+        blk.rhsLin += jLin * Fs;
+        blk.rhsAng += jAng * Fs;
+    }
+}
+
+// generic_joint.cpp:234-277
+AD void genericPrimal(const State &S, int j, int body, Block &blk)
+{
+    const int a = S.jA[j], b = S.jB[j];
+    const qt qA = genericFrame(S, j);
+    const v3 rAWorld = qrot(qA, ld3(S.jRA[j]));
+    const v3 rBWorld = qrot(bRot(S, b), ld3(S.jRB[j]));
+    const v3 offset = genericOffset(S, j, qA);
+    float ang[3];
+    genericAngular(S, j, ang);
+    const bool isA = body == a;
+    const float sign = isA ? 1.0f : -1.0f;
+    const v3 arm = isA ? rAWorld : rBWorld;
+    const v3 vA = a >= 0 ? ld3(S.vel[a]) + cross(ld3(S.angVel[a]), rAWorld) : v3{0, 0, 0};
+    const v3 vB = ld3(S.vel[b]) + cross(ld3(S.angVel[b]), rBWorld);
+    const v3 relVel = vA - vB;
+    const v3 wA = a >= 0 ? ld3(S.angVel[a]) : v3{0, 0, 0};
+    const v3 wB = ld3(S.angVel[b]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const v3 axis = qrot(qA, frameAxis(i));
+        const v3 jLin = axis * sign;
+        const v3 jAng = cross(arm, jLin);
+        stampAxis(S, 6 * j + i, blk, offset[i], S.P.alpha, jLin, jAng, dot(relVel, axis));
+        stampAxis(S, 6 * j + 3 + i, blk, ang[i], S.P.alpha, v3{0, 0, 0}, axis * sign, dot(wA - wB, axis));
+    }
+}
+
+// generic_joint.cpp:191-210: the largest penalty this body resolves along the axis.
+AD float penaltyLimit(const State &S, int body, v3 axis, bool angular)
+{
+    float effective;
+    if (angular) {
+        const v3 local = qrot(qconj(bRot(S, body)), axis);
+        const v3 m = bMoment(S, body);
+        effective = (local.x * local.x * m.x + local.y * local.y * m.y) + local.z * local.z * m.z;
+    } else {
+        effective = bMass(S, body);
+    }
+    const float pmax = S.P.penaltyMax;
+    if (effective <= 0.0f) return pmax;
+    const float limit = effective * 0.25f * 0.25f / (S.P.dt * S.P.dt);
+    return smin(limit, pmax);
+}
+
+// generic_joint.cpp:279-319
+AD void genericDual(const State &S, int j)
+{
+    const qt qA = genericFrame(S, j);
+    float ang[3];
+    genericAngular(S, j, ang);
+    const v3 offset = genericOffset(S, j, qA);
+    const int a = S.jA[j], b = S.jB[j];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int x = 6 * j + 3 * k + i;
+            const float value = k == 0 ? offset[i] : ang[i];
+            const float beta = k == 0 ? S.P.betaLin : S.P.betaAng;
+            float C, F;
+            if (!axisForce(S, x, value, S.P.alpha, C, F)) {
+                if (S.jGMode[x] == AXIS_LIMITED) S.jGLam[x] = 0.0f;
+                continue;
+            }
+            S.jGLam[x] = F;
+            const v3 dir = qrot(qA, frameAxis(i));
+            float cap = penaltyLimit(S, b, dir, k == 1);
+            if (a >= 0) cap = smin(cap, penaltyLimit(S, a, dir, k == 1));
+            S.jGPen[x] = sclamp(S.jGPen[x] + fabsf(C) * beta, 0.0f, cap);
+        }
+    }
+}
+
+// generic_joint.cpp:212-232: values at the step's start, warm-started dual state.
+AD void genericInit(const State &S, int j)
+{
+    float ang[3];
+    genericAngular(S, j, ang);
+    const v3 offset = genericOffset(S, j, genericFrame(S, j));
+    const float ag = S.P.alpha * S.P.gamma, pmax = S.P.penaltyMax;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        S.jGInit[6 * j + i] = offset[i];
+        S.jGInit[6 * j + 3 + i] = ang[i];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int x = 6 * j + 3 * k + i;
+            S.jGLam[x] = S.jGLam[x] * ag;
+            S.jGPen[x] = sclamp(S.jGPen[x] * S.P.gamma, PENALTY_MIN, pmax);
+        }
+    }
+}
+
+// A force of the body's list: joint, spring, generic joint, or a collision filter (which
+// stamps nothing).
 AD void jointPrimal(const State &S, int j, int body, Block &blk);
 AD void forcePrimal(const State &S, int j, int body, Block &blk)
 {
     const uint8_t kind = S.jKind[j];
     if (kind == FORCE_JOINT) jointPrimal(S, j, body, blk);
     else if (kind == FORCE_SPRING) springPrimal(S, j, body, blk);
+    else if (kind == FORCE_GENERIC) genericPrimal(S, j, body, blk);
 }
 
 AD void jointPrimal(const State &S, int j, int body, Block &blk)
@@ -706,6 +899,7 @@ AD void jointPrimal(const State &S, int j, int body, Block &blk)
 // joint.cpp:171-230
 AD void jointDual(const State &S, int j)
 {
+    if (S.jKind[j] == FORCE_GENERIC) { genericDual(S, j); return; }
     if (S.jKind[j] != FORCE_JOINT) return;   // springs and collision filters keep no dual state
     const float pmax = S.P.penaltyMax;
     v3 penL = ld3(S.jPenL[j]), penA = ld3(S.jPenA[j]);

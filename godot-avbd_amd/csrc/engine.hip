@@ -269,6 +269,8 @@ struct Engine {
         AL(S.jA, J); AL(S.jB, J); AL(S.jWorld, J); AL(S.jRA, J); AL(S.jRB, J); AL(S.jC0L, J); AL(S.jC0A, J);
         AL(S.jPenL, J); AL(S.jPenA, J); AL(S.jLamL, J); AL(S.jLamA, J); AL(S.jStiffL, J); AL(S.jStiffA, J);
         AL(S.jFrac, J); AL(S.jArm, J); AL(S.jBroken, J); AL(S.jDead, J); AL(S.jKind, J); AL(S.jRest, J);
+        AL(S.jGRest, J); AL(S.jGMode, 6 * J); AL(S.jGLo, 6 * J); AL(S.jGHi, 6 * J); AL(S.jGK, 6 * J);
+        AL(S.jGEq, 6 * J); AL(S.jGDamp, 6 * J); AL(S.jGLam, 6 * J); AL(S.jGPen, 6 * J); AL(S.jGInit, 6 * J);
         AL(S.mA, M); AL(S.mB, M); AL(S.mBasis, 3 * M); AL(S.mFric, M); AL(S.mNum, M);
         // sliced contacts: at most MAXC rows per lane of every slice, the last slice padded
         const size_t CD = (static_cast<size_t>(Scap) + CSLICE) * MAXC;
@@ -842,7 +844,11 @@ struct Engine {
             std::vector<int> ja(nj), jb(nj), jw(nj, w);
             std::vector<float4> ra(nj), rb(nj), c0l(nj), c0a(nj), pl(nj), pa(nj), ll2(nj), la2(nj);
             std::vector<float> sl(nj), sa(nj), fr(nj), arm(nj), rest(nj, 0.0f);
-            std::vector<uint8_t> br(nj), dead(nj, 0), kind(nj, AVBD_FORCE_JOINT);
+            std::vector<uint8_t> br(nj), dead(nj, 0), kind(nj, AVBD_FORCE_JOINT), gmode(6 * nj, 0);
+            std::vector<float4> grest(nj, make_float4(0, 0, 0, 1));
+            std::vector<float> glo(6 * nj, 0.0f), ghi(6 * nj, 0.0f), gk(6 * nj, 0.0f), geq(6 * nj, 0.0f),
+                    gd(6 * nj, 0.0f), glam(6 * nj, 0.0f), gpen(6 * nj, 0.0f), ginit(6 * nj, 0.0f);
+            bool anyGeneric = false;
             auto f4 = [](const float *p, int k) { return p ? make_float4(p[3 * k], p[3 * k + 1], p[3 * k + 2], 0) : make_float4(0, 0, 0, 0); };
             for (int k = 0; k < nj; k++) {
                 if (j->body_b[k] < 0 || j->body_b[k] >= n || j->body_a[k] >= n)
@@ -857,7 +863,25 @@ struct Engine {
                 fr[k] = j->fracture[k]; arm[k] = j->torque_arm[k];
                 br[k] = j->broken ? j->broken[k] : 0;
                 if (j->kind) kind[k] = j->kind[k];
-                if (kind[k] > AVBD_FORCE_IGNORE_COLLISION) return fail(AVBD_ERR_INVALID, "unknown force kind");
+                if (kind[k] > AVBD_FORCE_GENERIC) return fail(AVBD_ERR_INVALID, "unknown force kind");
+                if (kind[k] == AVBD_FORCE_GENERIC) {
+                    if (!j->rest_rotation || !j->axis_mode || !j->axis_lower || !j->axis_upper || !j->axis_spring ||
+                        !j->axis_equilibrium || !j->axis_damping)
+                        return fail(AVBD_ERR_INVALID, "generic joint arrays are null");
+                    anyGeneric = true;
+                    grest[k] = make_float4(j->rest_rotation[4 * k], j->rest_rotation[4 * k + 1],
+                                           j->rest_rotation[4 * k + 2], j->rest_rotation[4 * k + 3]);
+                    for (int a = 0; a < 6; a++) {
+                        const int x = 6 * k + a;
+                        if (j->axis_mode[x] > AVBD_AXIS_LIMITED) return fail(AVBD_ERR_INVALID, "unknown axis mode");
+                        gmode[x] = j->axis_mode[x];
+                        glo[x] = j->axis_lower[x]; ghi[x] = j->axis_upper[x]; gk[x] = j->axis_spring[x];
+                        geq[x] = j->axis_equilibrium[x]; gd[x] = j->axis_damping[x];
+                        glam[x] = j->axis_lambda ? j->axis_lambda[x] : 0.0f;
+                        gpen[x] = j->axis_penalty ? j->axis_penalty[x] : 0.0f;
+                        ginit[x] = j->axis_initial ? j->axis_initial[x] : 0.0f;
+                    }
+                }
                 if (kind[k] == AVBD_FORCE_SPRING) {
                     // spring.cpp:19-28 reads both bodies' poses
                     if (j->body_a[k] < 0) return fail(AVBD_ERR_INVALID, "a spring needs two bodies");
@@ -876,6 +900,16 @@ struct Engine {
             HIPCHK(putj(S.jStiffL, sl)); HIPCHK(putj(S.jStiffA, sa)); HIPCHK(putj(S.jFrac, fr)); HIPCHK(putj(S.jArm, arm));
             HIPCHK(putj(S.jBroken, br)); HIPCHK(putj(S.jDead, dead)); HIPCHK(putj(S.jKind, kind));
             HIPCHK(putj(S.jRest, rest));
+            if (anyGeneric) {
+                HIPCHK(putj(S.jGRest, grest));
+                auto put6 = [&](auto *dst, const auto &src) {
+                    return hipMemcpyAsync(dst + 6 * jo, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice,
+                                          stream);
+                };
+                HIPCHK(put6(S.jGMode, gmode)); HIPCHK(put6(S.jGLo, glo)); HIPCHK(put6(S.jGHi, ghi));
+                HIPCHK(put6(S.jGK, gk)); HIPCHK(put6(S.jGEq, geq)); HIPCHK(put6(S.jGDamp, gd));
+                HIPCHK(put6(S.jGLam, glam)); HIPCHK(put6(S.jGPen, gpen)); HIPCHK(put6(S.jGInit, ginit));
+            }
         }
         // slot range ownership and empty manifold store
         std::vector<int> sw(slotCap, w);
@@ -971,6 +1005,13 @@ struct Engine {
             std::vector<uint8_t> dead(std::max(nj, 1)), broken(std::max(nj, 1));
             std::vector<int> ja(std::max(nj, 1)), jb(std::max(nj, 1));
             std::vector<float4> c0l(nj), c0a(nj), pl(nj), pa(nj), ll(nj), la(nj);
+            std::vector<float> glam(6 * nj), gpen(6 * nj), ginit(6 * nj);
+            const bool wantGeneric = j->axis_lambda || j->axis_penalty || j->axis_initial;
+            if (nj > 0 && wantGeneric) {
+                HIPCHK(hipMemcpy(glam.data(), S.jGLam + 6 * jo, sizeof(float) * 6 * nj, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(gpen.data(), S.jGPen + 6 * jo, sizeof(float) * 6 * nj, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(ginit.data(), S.jGInit + 6 * jo, sizeof(float) * 6 * nj, hipMemcpyDeviceToHost));
+            }
             if (nj > 0) {
                 HIPCHK(hipMemcpy(dead.data(), S.jDead + jo, nj, hipMemcpyDeviceToHost));
                 HIPCHK(hipMemcpy(broken.data(), S.jBroken + jo, nj, hipMemcpyDeviceToHost));
@@ -995,6 +1036,11 @@ struct Engine {
                 put3(j->penalty_lin, k, pl[q]); put3(j->penalty_ang, k, pa[q]);
                 put3(j->lambda_lin, k, ll[q]); put3(j->lambda_ang, k, la[q]);
                 if (j->broken) j->broken[k] = broken[q];
+                for (int a = 0; a < 6; a++) {
+                    if (j->axis_lambda) j->axis_lambda[6 * k + a] = glam[6 * q + a];
+                    if (j->axis_penalty) j->axis_penalty[6 * k + a] = gpen[6 * q + a];
+                    if (j->axis_initial) j->axis_initial[6 * k + a] = ginit[6 * q + a];
+                }
                 k++;
             }
             j->count = k;

@@ -1043,6 +1043,7 @@ __global__ void k_joint_init(State S)
     const int j = GID;
     if (j >= S.jointCap) return;
     if (S.jWorld[j] < 0 || S.jDead[j]) return;
+    if (S.jKind[j] == FORCE_GENERIC) { genericInit(S, j); return; }   // never breaks
     if (S.jKind[j] != FORCE_JOINT) return;   // springs and collision filters: nothing to warm
     if (S.jBroken[j]) { S.jDead[j] = 1; return; }
     S.jC0L[j] = st3(jointLin(S, j));

@@ -30,7 +30,7 @@ struct Params {
 };
 
 constexpr int NARROW_REC = 8;
-constexpr uint8_t FORCE_JOINT = 0, FORCE_SPRING = 1, FORCE_IGNORE = 2;
+constexpr uint8_t FORCE_JOINT = 0, FORCE_SPRING = 1, FORCE_IGNORE = 2, FORCE_GENERIC = 3;
 
 struct State {
     Params P;
@@ -71,8 +71,14 @@ struct State {
     int *jA, *jB, *jWorld;
     float4 *jRA, *jRB, *jC0L, *jC0A, *jPenL, *jPenA, *jLamL, *jLamA;
     float *jStiffL, *jStiffA, *jFrac, *jArm;
-    uint8_t *jKind;    // FORCE_JOINT, FORCE_SPRING, FORCE_IGNORE
+    uint8_t *jKind;    // FORCE_JOINT, FORCE_SPRING, FORCE_IGNORE, FORCE_GENERIC
     float *jRest;      // spring rest length
+    // generic joints: rest orientation per joint; per axis (6 j + a, a = 0-2 linear, 3-5
+    // angular) mode, bounds, spring stiffness / equilibrium / damping, lambda, penalty, and
+    // the value at the step's start
+    float4 *jGRest;
+    uint8_t *jGMode;
+    float *jGLo, *jGHi, *jGK, *jGEq, *jGDamp, *jGLam, *jGPen, *jGInit;
     uint8_t *jBroken, *jDead;
 
     // ---- manifold slots

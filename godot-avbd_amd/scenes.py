@@ -63,6 +63,20 @@ class SceneBuilder:
                            fracture=F(math.inf), arm=F(0.0), rest=F(rest)))
         return len(self.j) - 1
 
+    def generic(self, a, b, r_a, r_b, linear=None, angular=None):
+        """Reference `GenericJoint` (generic_joint.cpp:24-30): rest = conj(frame) * rot_b with
+        frame = rot_a (identity when a is None), in float32 with the reference's product order.
+        linear / angular: three (mode, lower, upper, spring, equilibrium, damping) tuples
+        (default: every axis free)."""
+        qa = self.b[a]["rotation"] if a is not None and a >= 0 else np.array([0, 0, 0, 1], F)
+        rest = _qmul32(np.array([-qa[0], -qa[1], -qa[2], qa[3]], F), self.b[b]["rotation"])
+        axes = list(linear or [(0, 0, 0, 0, 0, 0)] * 3) + list(angular or [(0, 0, 0, 0, 0, 0)] * 3)
+        z = np.zeros(3, F)
+        self.j.append(dict(kind=3, a=-1 if a is None else a, b=b, r_a=np.asarray(r_a, F), r_b=np.asarray(r_b, F),
+                           k_lin=F(0.0), k_ang=F(0.0), fracture=F(math.inf), arm=F(0.0), rest=F(0.0),
+                           rest_rotation=rest, axes=[tuple(x) for x in axes]))
+        return len(self.j) - 1
+
     def ignore_collision(self, a, b):
         """Reference `IgnoreCollision` (forces.hpp:104-113): the pair skips the broad phase."""
         z = np.zeros(3, F)
@@ -85,7 +99,8 @@ class SceneBuilder:
             "position": np.stack([self.b[c]["position"] for c in order]) if n else np.zeros((0, 3), F),
             "rotation": np.stack([self.b[c]["rotation"] for c in order]) if n else np.zeros((0, 4), F),
             "velocity": np.stack([self.b[c]["velocity"] for c in order]) if n else np.zeros((0, 3), F),
-            "angular_velocity": np.zeros((n, 3), F),
+            "angular_velocity": (np.stack([np.asarray(self.b[c].get("ang", (0.0, 0.0, 0.0)), F) for c in order])
+                                 if n else np.zeros((0, 3), F)),
         }
         bodies["prev_velocity"] = bodies["velocity"].copy()
         nj = len(self.j)
@@ -104,10 +119,31 @@ class SceneBuilder:
                 "kind": np.array([self.j[c].get("kind", 0) for c in jorder], np.uint8),
                 "rest": np.array([self.j[c].get("rest", 0.0) for c in jorder], F),
             }
+            if any(self.j[c].get("kind", 0) == 3 for c in jorder):
+                none = [(0, 0, 0, 0, 0, 0)] * 6
+                joints["rest_rotation"] = np.stack([self.j[c].get("rest_rotation", np.array([0, 0, 0, 1], F))
+                                                    for c in jorder]).astype(F)
+                ax = np.array([self.j[c].get("axes", none) for c in jorder], np.float64)   # (nj, 6, 6)
+                joints["axis_mode"] = ax[:, :, 0].astype(np.uint8)
+                for q, name in enumerate(("axis_lower", "axis_upper", "axis_spring", "axis_equilibrium",
+                                          "axis_damping"), start=1):
+                    joints[name] = ax[:, :, q].astype(F)
+                for name in ("axis_lambda", "axis_penalty", "axis_initial"):
+                    joints[name] = np.zeros((nj, 6), F)
             for k in ("c0_lin", "c0_ang", "penalty_lin", "penalty_ang", "lambda_lin", "lambda_ang"):
                 joints[k] = np.zeros((nj, 3), F)
             joints["broken"] = np.zeros(nj, np.uint8)
         return bodies, joints
+
+
+def _qmul32(a, b):
+    """Hamilton product in float32 with the reference's term order (maths.h:87-95)."""
+    a = np.asarray(a, F)
+    b = np.asarray(b, F)
+    ax, ay, az, aw = a
+    bx, by, bz, bw = b
+    return np.array([((aw * bx + ax * bw) + ay * bz) - az * by, ((aw * by + ay * bw) + az * bx) - ax * bz,
+                     ((aw * bz + az * bw) + ax * by) - ay * bx, ((aw * bw - ax * bx) - ay * by) - az * bz], F)
 
 
 def _qrot64(q, v):
@@ -142,6 +178,63 @@ def springs_and_filters(seed=5):
     s.ignore_collision(a, b)
     faller = s.box((1, 1, 1), 1.0, 0.5, (F(-12), 0, F(1.2)))
     s.ignore_collision(ground, faller)
+    return s.build()
+
+
+def generic_joints():
+    """The reference's GenericJoint rigs (core_test.cpp:703-1000) side by side in one world: a
+    hinge spinning about its free axle, a prismatic strut with a spring, an axle with a ±0.4 rad
+    limit, a ball socket anchored off the body's centre and swinging, one joint to a world point
+    (no body A), and one built from a rotated body; then a hinged chain between dynamic bodies
+    dropped onto a ground box."""
+    L, K, M = 1, 2, 0   # locked, limited, free
+    lock = (L, 0, 0, 0, 0, 0)
+    free = (M, 0, 0, 0, 0, 0)
+    s = SceneBuilder()
+    rigs = []
+    for r in range(6):
+        x = F(6.0 * r)
+        anchor = s.box((1, 1, 1), 0.0, 0.5, (x, 0, 10))
+        rigs.append((anchor, x))
+    # hinge: axle Y free
+    a, x = rigs[0]
+    body = s.box((1, 1, 1), 1.0, 0.5, (x, 0, 10), vel=(0, 0, 0))
+    s.b[body]["ang"] = (0.0, 3.0, 0.0)
+    s.generic(a, body, (0, 0, 0), (0, 0, 0), [lock] * 3, [lock, free, lock])
+    # prismatic strut: linear Z limited with a spring
+    a, x = rigs[1]
+    body = s.box((1, 1, 1), 1.0, 0.5, (x, 0, 10))
+    s.generic(a, body, (0, 0, 0), (0, 0, 0), [lock, lock, (K, -2.0, 2.0, 500.0, 0.0, 5.0)], [lock] * 3)
+    # limited axle
+    a, x = rigs[2]
+    body = s.box((1, 1, 1), 1.0, 0.5, (x, 0, 10))
+    s.b[body]["ang"] = (0.0, 6.0, 0.0)
+    s.generic(a, body, (0, 0, 0), (0, 0, 0), [lock] * 3, [lock, (K, -0.4, 0.4, 0, 0, 0), lock])
+    # ball socket off the centre, swinging
+    a, x = rigs[3]
+    body = s.box((1, 1, 1), 1.0, 0.5, (x, 0, 9), vel=(1.5, 0, 0))
+    s.generic(a, body, (0, 0, 0), (0, 0, 1.0), [lock] * 3, [free] * 3)
+    # to a world point, with an angular spring about Z
+    _, x = rigs[4]
+    body = s.box((1, 1, 1), 1.0, 0.5, (x, 0, 8))
+    s.b[body]["ang"] = (0.0, 0.0, 2.0)
+    s.generic(None, body, (x, 0, 9), (0, 0, 1.0), [lock] * 3, [free, free, (M, 0, 0, 40.0, 0.0, 1.0)])
+    # a body created rotated (rest orientation not the identity), axle X free
+    a, x = rigs[5]
+    body = s.box((1.0, 0.6, 0.6), 1.0, 0.5, (x, 0, 9), rot=(0.0, -0.38268343, 0.0, 0.9238795))
+    s.b[body]["ang"] = (4.0, 0.0, 0.0)
+    s.generic(a, body, (0, 0, -1.0), (0, 0, 0), [lock] * 3, [free, lock, lock])
+    # a chain of hinged links between dynamic bodies, dropped onto a ground box: joints and
+    # contacts on the same bodies, the middle hinge limited
+    s.box((40, 10, 1), 0.0, 0.5, (8.0, -8.0, -0.5))
+    prev = None
+    for i in range(6):
+        link = s.box((1.0, 0.4, 0.4), 1.0, 0.5, (F(i) * F(1.1), -8.0, F(2.0) + F(0.3) * F(i)),
+                     rot=(0.0, 0.0, 0.0998334, 0.9950042))
+        if prev is not None:
+            hinge = (K, -0.6, 0.6, 0, 0, 0) if i == 3 else free
+            s.generic(prev, link, (0.55, 0, 0), (-0.55, 0, 0), [lock] * 3, [lock, hinge, lock])
+        prev = link
     return s.build()
 
 

@@ -51,6 +51,11 @@ class World:
     def step_interleaved(self, substeps, rounds):
         abi.check(self.lib.avbd_world_step_interleaved(self.h, int(substeps), int(rounds)))
 
+    def write_bodies(self, arrays):
+        """Overwrite the dynamic state of the uploaded bodies, keeping manifolds and joints."""
+        bs = abi.dynamic_bodies_struct(arrays, self.n)
+        abi.check(self.lib.avbd_world_write_bodies(self.h, C.byref(bs)))
+
     def download(self, joints=False):
         out = abi.empty_arrays({k: abi.BODY_FIELDS[k] for k in
                                 ("position", "rotation", "velocity", "angular_velocity", "prev_velocity")}, self.n)
@@ -62,7 +67,8 @@ class World:
         if joints and self.nj:
             jout = abi.empty_arrays({k: abi.JOINT_FIELDS[k] for k in
                                      ("body_a", "body_b", "c0_lin", "c0_ang", "penalty_lin", "penalty_ang",
-                                      "lambda_lin", "lambda_ang", "broken")}, self.nj)
+                                      "lambda_lin", "lambda_ang", "broken", "axis_lambda", "axis_penalty",
+                                      "axis_initial")}, self.nj)
             js = abi.joints_struct({**jout, "body_b": jout["body_b"]})
             js.count = self.nj
         abi.check(self.lib.avbd_world_download(self.h, C.byref(bs), C.byref(js) if js is not None else None))

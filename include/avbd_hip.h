@@ -120,11 +120,28 @@ typedef struct avbd_joints {
      * pair of bodies out of the broad phase and acts on nothing else (forces.hpp:104-113). */
     uint8_t *kind;        /* n, or NULL */
     float *rest;          /* n spring rest length (read for springs; NULL when there are none) */
+    /* AVBD_FORCE_GENERIC: the 6-DOF joint (generic_joint.cpp) with anchors r_a, r_b.  Axis a of
+     * entry k is at 6 k + a: 0-2 linear x, y, z and 3-5 angular x, y, z of A's frame (the
+     * world's when body_a < 0).  Read only for generic entries; NULL when there are none. */
+    float *rest_rotation;    /* 4n B's orientation relative to A at creation, xyzw          */
+    uint8_t *axis_mode;      /* 6n AVBD_AXIS_FREE / _LOCKED / _LIMITED                      */
+    float *axis_lower;       /* 6n limit bounds                                            */
+    float *axis_upper;       /* 6n                                                          */
+    float *axis_spring;      /* 6n spring stiffness (0: none)                               */
+    float *axis_equilibrium; /* 6n spring equilibrium                                       */
+    float *axis_damping;     /* 6n spring damping                                           */
+    float *axis_lambda;      /* 6n dual state: read on upload, written on download         */
+    float *axis_penalty;     /* 6n                                                          */
+    float *axis_initial;     /* 6n value at the start of the last step                     */
 } avbd_joints;
 
 #define AVBD_FORCE_JOINT 0
 #define AVBD_FORCE_SPRING 1
 #define AVBD_FORCE_IGNORE_COLLISION 2
+#define AVBD_FORCE_GENERIC 3
+#define AVBD_AXIS_FREE 0
+#define AVBD_AXIS_LOCKED 1
+#define AVBD_AXIS_LIMITED 2
 
 #define AVBD_MAX_CONTACTS 8
 

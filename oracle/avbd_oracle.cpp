@@ -55,10 +55,16 @@ struct Body {
 
 // One entry of the force list (forces.hpp:19-113): a joint, a spring (spring.cpp) or a
 // collision filter (IgnoreCollision: in the list, acts on nothing).
+// One degree of freedom of a generic joint (forces.hpp:123-139).
+struct JointAxis {
+    uint8_t mode = AVBD_AXIS_FREE; float lower = 0, upper = 0, spring = 0, equilibrium = 0, damping = 0;
+    float lambda = 0, penalty = 0, initial = 0;
+};
 struct Joint {
     int a, b; f3 rA, rB, C0Lin, C0Ang, penLin, penAng, lamLin, lamAng;
     float stiffLin, stiffAng, fracture, torqueArm; bool broken;
     uint8_t kind = AVBD_FORCE_JOINT; float rest = 0.0f;
+    quat restQ{0, 0, 0, 1}; JointAxis axis[6];   // generic joints: 0-2 linear, 3-5 angular
 };
 
 struct Contact { int32_t key; f3 rA, rB; };
@@ -930,8 +936,168 @@ struct World {
                 smin(sclamp(p.y * gamma, PENALTY_MIN, pmaxv), stiff),
                 smin(sclamp(p.z * gamma, PENALTY_MIN, pmaxv), stiff)};
     }
+    // ---------------------------------------------------------------- generic joint ----
+    // generic_joint.cpp
+    static f3 frameAxis(int i) { return f3{i == 0 ? 1.0f : 0.0f, i == 1 ? 1.0f : 0.0f, i == 2 ? 1.0f : 0.0f}; }
+    quat frameOf(const Joint &j) const { return j.a >= 0 ? bodies[j.a].q : quat{0, 0, 0, 1}; }
+    // :51-67
+    static void removeTwist(quat &dev, int i, float &angle)
+    {
+        const f3 n = frameAxis(i);
+        const float proj = dot(f3{dev.x, dev.y, dev.z}, n);
+        const quat twist{dev.x * proj, dev.y * proj, dev.z * proj, dev.w};
+        if (std::sqrt(qlength2(twist)) <= 1.0e-12f) { angle = 0.0f; return; }
+        angle = 2.0f * atan2Poly(proj, dev.w);
+        const float l2 = qlength2(twist);
+        const quat inv{-twist.x / l2, -twist.y / l2, -twist.z / l2, twist.w / l2};
+        dev = qmul(qnormalize(dev), inv);
+    }
+    // :69-90
+    void angularValues(const Joint &j, float out[3]) const
+    {
+        const quat qA = frameOf(j);
+        const quat relNow = qmul(qconj(qA), qnormalize(bodies[j.b].q));
+        quat dev = qmul(qnormalize(j.restQ), qconj(relNow));
+        for (int i = 0; i < 3; i++)
+            if (j.axis[3 + i].mode != AVBD_AXIS_LOCKED) removeTwist(dev, i, out[i]);
+        const float comp[3] = {dev.x, dev.y, dev.z};
+        for (int i = 0; i < 3; i++)
+            if (j.axis[3 + i].mode == AVBD_AXIS_LOCKED) out[i] = comp[i] * 2.0f;
+    }
+    // :92-96
+    f3 linearValues(const Joint &j) const
+    {
+        const quat qA = frameOf(j);
+        const f3 anchorA = j.a >= 0 ? bodies[j.a].pos + qrot(qA, j.rA) : j.rA;
+        const f3 anchorB = bodies[j.b].pos + qrot(bodies[j.b].q, j.rB);
+        return qrot(qconj(qA), anchorA - anchorB);
+    }
+    // :104-155
+    static bool axisForce(const JointAxis &ax, float value, float alpha, float &C, float &F)
+    {
+        if (ax.mode == AVBD_AXIS_LOCKED) {
+            C = value;
+        } else if (ax.mode == AVBD_AXIS_LIMITED) {
+            if (value > ax.upper) C = value - ax.upper;
+            else if (value < ax.lower) C = value - ax.lower;
+            else return false;
+        } else {
+            return false;
+        }
+        if (ax.mode == AVBD_AXIS_LOCKED) C -= ax.initial * alpha;
+        float f = ax.penalty * C + ax.lambda;
+        if (ax.mode == AVBD_AXIS_LIMITED) f = C > 0.0f ? smax(f, 0.0f) : smin(f, 0.0f);
+        F = f;
+        return true;
+    }
+    static m3 outerScaled(f3 a, f3 b, float s) { return m3{{a * b.x, a * b.y, a * b.z}} * s; }
+    // :157-187
+    static void stampAxis(const JointAxis &ax, Block &blk, float value, float alpha, f3 jLin, f3 jAng, float rate)
+    {
+        float C, F;
+        if (axisForce(ax, value, alpha, C, F)) {
+            blk.lhsLin += outerScaled(jLin, jLin, ax.penalty);
+            blk.lhsAng += outerScaled(jAng, jAng, ax.penalty);
+            blk.lhsCross += outerScaled(jAng, jLin, ax.penalty);
+            blk.rhsLin += jLin * F;
+            blk.rhsAng += jAng * F;
+        }
+        if (ax.spring > 0.0f) {
+            const float Cs = value - ax.equilibrium;
+            const float Fs = ax.spring * Cs - ax.damping * rate;
+            blk.lhsLin += outerScaled(jLin, jLin, ax.spring);
+            blk.lhsAng += outerScaled(jAng, jAng, ax.spring);
+            blk.lhsCross += outerScaled(jAng, jLin, ax.spring);
+            blk.rhsLin += jLin * Fs;
+            blk.rhsAng += jAng * Fs;
+        }
+    }
+    // :234-277
+    void genericPrimal(const Joint &j, int body, Block &blk) const
+    {
+        const quat qA = frameOf(j);
+        const f3 rAWorld = qrot(qA, j.rA);
+        const Body &B = bodies[j.b];
+        const f3 rBWorld = qrot(B.q, j.rB);
+        const f3 offset = linearValues(j);
+        float ang[3];
+        angularValues(j, ang);
+        const bool isA = body == j.a;
+        const float sign = isA ? 1.0f : -1.0f;
+        const f3 arm = isA ? rAWorld : rBWorld;
+        const f3 vA = j.a >= 0 ? bodies[j.a].vel + cross(bodies[j.a].angVel, rAWorld) : f3{0, 0, 0};
+        const f3 vB = B.vel + cross(B.angVel, rBWorld);
+        const f3 relVel = vA - vB;
+        const f3 wA = j.a >= 0 ? bodies[j.a].angVel : f3{0, 0, 0};
+        const f3 wB = B.angVel;
+        for (int i = 0; i < 3; i++) {
+            const f3 axis = qrot(qA, frameAxis(i));
+            const f3 jLin = axis * sign;
+            const f3 jAng = cross(arm, jLin);
+            stampAxis(j.axis[i], blk, offset[i], P.alpha, jLin, jAng, dot(relVel, axis));
+            stampAxis(j.axis[3 + i], blk, ang[i], P.alpha, f3{0, 0, 0}, axis * sign, dot(wA - wB, axis));
+        }
+    }
+    // :191-210
+    float penaltyLimit(int body, f3 axis, bool angular) const
+    {
+        const Body &b = bodies[body];
+        float effective;
+        if (angular) {
+            const f3 local = qrot(qconj(b.q), axis);
+            effective = (local.x * local.x * b.moment.x + local.y * local.y * b.moment.y) + local.z * local.z * b.moment.z;
+        } else {
+            effective = b.mass;
+        }
+        if (effective <= 0.0f) return pmax();
+        const float limit = effective * 0.25f * 0.25f / (P.dt * P.dt);
+        return smin(limit, pmax());
+    }
+    // :279-319
+    void genericDual(Joint &j) const
+    {
+        float ang[3];
+        angularValues(j, ang);
+        const f3 offset = linearValues(j);
+        const quat qA = frameOf(j);
+        for (int i = 0; i < 3; i++) {
+            for (int k = 0; k < 2; k++) {
+                JointAxis &ax = j.axis[3 * k + i];
+                const float value = k == 0 ? offset[i] : ang[i];
+                const float beta = k == 0 ? P.beta_lin : P.beta_ang;
+                float C, F;
+                if (!axisForce(ax, value, P.alpha, C, F)) {
+                    if (ax.mode == AVBD_AXIS_LIMITED) ax.lambda = 0.0f;
+                    continue;
+                }
+                ax.lambda = F;
+                const f3 dir = qrot(qA, frameAxis(i));
+                float cap = penaltyLimit(j.b, dir, k == 1);
+                if (j.a >= 0) cap = smin(cap, penaltyLimit(j.a, dir, k == 1));
+                ax.penalty = sclamp(ax.penalty + std::fabs(C) * beta, 0.0f, cap);
+            }
+        }
+    }
+    // :212-232
+    void genericInit(Joint &j) const
+    {
+        float ang[3];
+        angularValues(j, ang);
+        const f3 offset = linearValues(j);
+        for (int i = 0; i < 3; i++) {
+            j.axis[i].initial = offset[i];
+            j.axis[3 + i].initial = ang[i];
+            for (int k = 0; k < 2; k++) {
+                JointAxis &ax = j.axis[3 * k + i];
+                ax.lambda *= P.alpha * P.gamma;
+                ax.penalty = sclamp(ax.penalty * P.gamma, PENALTY_MIN, pmax());
+            }
+        }
+    }
+
     bool jointInit(Joint &j)
     {
+        if (j.kind == AVBD_FORCE_GENERIC) { genericInit(j); return true; }   // never breaks
         if (j.kind != AVBD_FORCE_JOINT) return true;   // Spring/IgnoreCollision::initialize
         j.C0Lin = jointLin(j);
         j.C0Ang = jointAng(j);
@@ -984,6 +1150,7 @@ struct World {
     {
         if (j.kind == AVBD_FORCE_JOINT) jointPrimal(j, body, blk);
         else if (j.kind == AVBD_FORCE_SPRING) springPrimal(j, body, blk);
+        else if (j.kind == AVBD_FORCE_GENERIC) genericPrimal(j, body, blk);
     }
     // joint.cpp:80-160
     void jointPrimal(const Joint &j, int body, Block &blk) const
@@ -1027,6 +1194,7 @@ struct World {
     // joint.cpp:171-230
     void jointDual(Joint &j) const
     {
+        if (j.kind == AVBD_FORCE_GENERIC) { genericDual(j); return; }
         if (j.kind != AVBD_FORCE_JOINT) return;   // Spring/IgnoreCollision::updateDual do nothing
         if (length2(j.penLin) > 0) {
             const m3 K = diagonal(j.penLin.x, j.penLin.y, j.penLin.z);
@@ -1482,7 +1650,25 @@ int oracle_upload(oracle_world *o, const avbd_bodies *b, const avbd_joints *j)
             x.lamAng = j->lambda_ang ? load3(j->lambda_ang, k) : f3{0, 0, 0};
             x.broken = j->broken ? j->broken[k] != 0 : false;
             x.kind = j->kind ? j->kind[k] : AVBD_FORCE_JOINT;
-            if (x.kind > AVBD_FORCE_IGNORE_COLLISION) return AVBD_ERR_INVALID;
+            if (x.kind > AVBD_FORCE_GENERIC) return AVBD_ERR_INVALID;
+            if (x.kind == AVBD_FORCE_GENERIC) {
+                if (!j->rest_rotation || !j->axis_mode || !j->axis_lower || !j->axis_upper || !j->axis_spring ||
+                    !j->axis_equilibrium || !j->axis_damping)
+                    return AVBD_ERR_INVALID;
+                x.restQ = quat{j->rest_rotation[4 * k], j->rest_rotation[4 * k + 1], j->rest_rotation[4 * k + 2],
+                               j->rest_rotation[4 * k + 3]};
+                for (int a = 0; a < 6; a++) {
+                    JointAxis &ax = x.axis[a];
+                    const int i = 6 * k + a;
+                    ax.mode = j->axis_mode[i];
+                    if (ax.mode > AVBD_AXIS_LIMITED) return AVBD_ERR_INVALID;
+                    ax.lower = j->axis_lower[i]; ax.upper = j->axis_upper[i]; ax.spring = j->axis_spring[i];
+                    ax.equilibrium = j->axis_equilibrium[i]; ax.damping = j->axis_damping[i];
+                    ax.lambda = j->axis_lambda ? j->axis_lambda[i] : 0.0f;
+                    ax.penalty = j->axis_penalty ? j->axis_penalty[i] : 0.0f;
+                    ax.initial = j->axis_initial ? j->axis_initial[i] : 0.0f;
+                }
+            }
             if (x.kind == AVBD_FORCE_SPRING) {
                 if (x.a < 0 || !j->rest) return AVBD_ERR_INVALID;
                 x.rest = j->rest[k];
@@ -1494,6 +1680,21 @@ int oracle_upload(oracle_world *o, const avbd_bodies *b, const avbd_joints *j)
     w.mA.clear(); w.mB.clear(); w.basis.clear(); w.numC.clear(); w.mFric.clear(); w.cont.clear();
     w.C0.clear(); w.pen.clear(); w.lam.clear(); w.stick.clear(); w.betaScale.clear();
     w.live.clear(); w.freeList.clear();
+    return AVBD_OK;
+}
+
+int oracle_write_bodies(oracle_world *o, const avbd_bodies *b)
+{
+    if (!o || !b || b->count != static_cast<int32_t>(o->w.bodies.size())) return AVBD_ERR_INVALID;
+    for (int i = 0; i < b->count; i++) {
+        Body &x = o->w.bodies[i];
+        if (b->position) x.pos = load3(b->position, i);
+        if (b->rotation)
+            x.q = quat{b->rotation[4 * i], b->rotation[4 * i + 1], b->rotation[4 * i + 2], b->rotation[4 * i + 3]};
+        if (b->velocity) x.vel = load3(b->velocity, i);
+        if (b->angular_velocity) x.angVel = load3(b->angular_velocity, i);
+        if (b->prev_velocity) x.prevVel = load3(b->prev_velocity, i);
+    }
     return AVBD_OK;
 }
 
@@ -1541,12 +1742,19 @@ int oracle_download(oracle_world *o, avbd_bodies *b, avbd_joints *j)
             if (j->lambda_lin) store3(j->lambda_lin, i, x.lamLin);
             if (j->lambda_ang) store3(j->lambda_ang, i, x.lamAng);
             if (j->broken) j->broken[i] = x.broken ? 1 : 0;
+            for (int a = 0; a < 6; a++) {
+                if (j->axis_lambda) j->axis_lambda[6 * i + a] = x.axis[a].lambda;
+                if (j->axis_penalty) j->axis_penalty[6 * i + a] = x.axis[a].penalty;
+                if (j->axis_initial) j->axis_initial[6 * i + a] = x.axis[a].initial;
+            }
         }
     }
     return AVBD_OK;
 }
 
 int oracle_manifold_count(oracle_world *o) { return static_cast<int>(o->w.live.size()); }
+
+float oracle_atan2(float y, float x) { return atan2Poly(y, x); }
 
 int oracle_get_manifolds(oracle_world *o, avbd_manifolds *m)
 {
@@ -1710,6 +1918,36 @@ void oracle_morton(const float *centers, int n, uint64_t *codes, int32_t *order)
     for (int i = 0; i < n; i++) ord[i] = i;
     std::sort(ord.begin(), ord.end(), [&](int a, int b) { return codes[a] != codes[b] ? codes[a] < codes[b] : a < b; });
     for (int i = 0; i < n; i++) order[i] = ord[i];
+}
+
+// One force's contribution to one body's primal block at the current state (the per-force
+// stamp of solver_solve.cpp's primal loop): lhsLin, lhsAng, lhsCross row-major, then rhsLin, rhsAng.
+int oracle_force_stamp(oracle_world *o, int joint, int body, float *out33)
+{
+    if (!o || joint < 0 || joint >= static_cast<int>(o->w.joints.size())) return AVBD_ERR_INVALID;
+    const Joint &j = o->w.joints[joint];
+    if (body != j.a && body != j.b) return AVBD_ERR_INVALID;
+    Block blk{};
+    o->w.forcePrimal(j, body, blk);
+    const m3 *ms[3] = {&blk.lhsLin, &blk.lhsAng, &blk.lhsCross};
+    for (int m = 0; m < 3; m++)
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) out33[9 * m + 3 * r + c] = (*ms[m])[c][r];
+    for (int r = 0; r < 3; r++) { out33[27 + r] = blk.rhsLin[r]; out33[30 + r] = blk.rhsAng[r]; }
+    return AVBD_OK;
+}
+
+// One force's warm start (what = 0: Force::initialize, solver_solve.cpp:39-51) or dual
+// update (what = 1: Force::updateDual, solver_solve.cpp's dual loop) applied alone at the
+// current state.
+int oracle_force_update(oracle_world *o, int joint, int what)
+{
+    if (!o || joint < 0 || joint >= static_cast<int>(o->w.joints.size()) || what < 0 || what > 1)
+        return AVBD_ERR_INVALID;
+    Joint &j = o->w.joints[joint];
+    if (what == 0) o->w.jointInit(j);
+    else o->w.jointDual(j);
+    return AVBD_OK;
 }
 
 // ldlt6_solve / pivoted / solve chain on one row-major 6x6 system (maths.h:154-343).

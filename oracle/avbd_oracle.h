@@ -14,10 +14,13 @@ oracle_world *oracle_create(const avbd_params *params); /* null on unsupported p
 void oracle_destroy(oracle_world *o);
 int oracle_set_params(oracle_world *o, const avbd_params *params);
 int oracle_upload(oracle_world *o, const avbd_bodies *b, const avbd_joints *j);
+int oracle_write_bodies(oracle_world *o, const avbd_bodies *b);
 int oracle_step(oracle_world *o);
 int oracle_step_interleaved(oracle_world *o, int substeps, int rounds);
 int oracle_download(oracle_world *o, avbd_bodies *b, avbd_joints *j);
 int oracle_manifold_count(oracle_world *o);
+/* The generic joint's atan2 (the same expression as the device's). */
+float oracle_atan2(float y, float x);
 int oracle_get_manifolds(oracle_world *o, avbd_manifolds *m);
 int oracle_set_manifolds(oracle_world *o, const avbd_manifolds *m);
 uint64_t oracle_digest(oracle_world *o);
@@ -28,6 +31,8 @@ void oracle_colours(oracle_world *o, uint8_t *out);
 void oracle_swept_aabbs(oracle_world *o, float *centers, float *halves);
 int oracle_candidate_pairs(oracle_world *o, uint64_t *pairs, int capacity);
 void oracle_morton(const float *centers, int n, uint64_t *codes, int32_t *order);
+int oracle_force_stamp(oracle_world *o, int joint, int body, float *out33);
+int oracle_force_update(oracle_world *o, int joint, int what);
 int oracle_solve6(const float *a36, const float *b6, float *x6);
 int oracle_collide_boxes(const float posA[3], const float rotA[4], const float sizeA[3],
                          const float posB[3], const float rotB[4], const float sizeB[3],

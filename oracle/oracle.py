@@ -49,6 +49,9 @@ def lib():
         "oracle_destroy": (None, [vp]),
         "oracle_set_params": (C.c_int, [vp, C.POINTER(a.Params)]),
         "oracle_upload": (C.c_int, [vp, C.POINTER(a.Bodies), C.POINTER(a.Joints)]),
+        "oracle_write_bodies": (C.c_int, [vp, C.POINTER(a.Bodies)]),
+        "oracle_force_stamp": (C.c_int, [vp, C.c_int, C.c_int, a.f32p]),
+        "oracle_force_update": (C.c_int, [vp, C.c_int, C.c_int]),
         "oracle_step": (C.c_int, [vp]),
         "oracle_step_interleaved": (C.c_int, [vp, C.c_int, C.c_int]),
         "oracle_download": (C.c_int, [vp, C.POINTER(a.Bodies), C.POINTER(a.Joints)]),
@@ -64,6 +67,7 @@ def lib():
         "oracle_solve6": (C.c_int, [a.f32p, a.f32p, a.f32p]),
         "oracle_collide_boxes": (C.c_int, [a.f32p, a.f32p, a.f32p, a.f32p, a.f32p, a.f32p, C.c_float,
                                            a.f32p, a.i32p, a.f32p, a.f32p]),
+        "oracle_atan2": (C.c_float, [C.c_float, C.c_float]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -120,6 +124,23 @@ class OracleWorld:
         self.n = bs.count
         self.nj = js.count
 
+    def write_bodies(self, arrays):
+        """Overwrite the dynamic state of the uploaded bodies (what a host does between steps)."""
+        bs = self.a.dynamic_bodies_struct(arrays, self.n)
+        assert self.L.oracle_write_bodies(self.h, C.byref(bs)) == 0
+
+    def force_stamp(self, joint, body):
+        """(lhs 6x6 over [lin, ang], rhs 6) that force `joint` adds to `body`'s primal block now."""
+        out = np.zeros(33, np.float32)
+        assert self.L.oracle_force_stamp(self.h, joint, body, out.ctypes.data_as(self.a.f32p)) == 0
+        ll, aa, al = out[0:9].reshape(3, 3), out[9:18].reshape(3, 3), out[18:27].reshape(3, 3)
+        lhs = np.block([[ll, al.T], [al, aa]]).astype(np.float64)
+        return lhs, out[27:33].astype(np.float64)
+
+    def force_update(self, joint, what):
+        """Apply one force's warm start ("init") or dual update ("dual") alone."""
+        assert self.L.oracle_force_update(self.h, joint, {"init": 0, "dual": 1}[what]) == 0
+
     def step(self, n=1):
         for _ in range(n):
             self.L.oracle_step(self.h)
@@ -138,7 +159,8 @@ class OracleWorld:
         if joints and self.nj:
             jout = a.empty_arrays({k: a.JOINT_FIELDS[k] for k in
                                    ("body_a", "body_b", "c0_lin", "c0_ang", "penalty_lin", "penalty_ang",
-                                    "lambda_lin", "lambda_ang", "broken")}, self.nj)
+                                    "lambda_lin", "lambda_ang", "broken", "axis_lambda", "axis_penalty",
+                                    "axis_initial")}, self.nj)
             js = a.joints_struct(jout)
         self.L.oracle_download(self.h, C.byref(bs), C.byref(js) if js is not None else None)
         if jout is not None:
@@ -249,3 +271,8 @@ def solve6(A, b):
 def ref_solve6(A, b):
     R = ref_lib()
     return None if R is None else _solve6(R.ref_solve6, A, b)
+
+
+def atan2(y, x):
+    """The generic joint's atan2, evaluated by the oracle library (one float32 pair)."""
+    return float(lib().oracle_atan2(float(y), float(x)))

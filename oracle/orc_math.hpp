@@ -96,6 +96,35 @@ inline quat qmul(quat a, quat b)
             a.w * b.z + a.z * b.w + a.x * b.y - a.y * b.x,
             a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
 }
+// The generic joint's atan2 (a polynomial the device evaluates with the same operations, so
+// both agree bit for bit): octant reduction, atan(a) = a P(a^2) on [0, 1] with a degree-19 odd
+// polynomial fitted to arctan by relative least squares (<= 1.3 ulp there), IEEE zero signs.
+// It stands in for std::atan2 in generic_joint.cpp:64; test_atan2_poly bounds the difference.
+inline float atan2Poly(float y, float x)
+{
+    const float ax = std::fabs(x), ay = std::fabs(y);
+    const bool swap = ay > ax;
+    const float mx = swap ? ay : ax, mn = swap ? ax : ay;
+    float r = 0.0f;
+    if (mx > 0.0f) {
+        const float a = mn / mx, s = a * a;
+        float p = -0.0017904319101944566f;
+        p = p * s + 0.010887323878705502f;
+        p = p * s + -0.0310975369066f;
+        p = p * s + 0.05783987045288086f;
+        p = p * s + -0.0839373916387558f;
+        p = p * s + 0.10947515815496445f;
+        p = p * s + -0.14262957870960236f;
+        p = p * s + 0.199983611702919f;
+        p = p * s + -0.3333328664302826f;
+        p = p * s + 1.0f;
+        r = p * a;
+    }
+    if (swap) r = 1.5707963705062866f - r;
+    if (std::signbit(x)) r = 3.1415927410125732f - r;
+    return std::copysign(r, y);
+}
+
 inline f3 qrot(quat q, f3 v)
 {
     const f3 z{q.x, q.y, q.z};

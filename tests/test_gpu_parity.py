@@ -98,6 +98,23 @@ def test_springs_and_collision_filters_lockstep():
     assert not pairs & {(min(a, c), max(a, c)) for a, c in zip(m["body_a"], m["body_b"])}
 
 
+def test_generic_joints_lockstep():
+    """Generic 6-DOF joints (locked, limited, free and sprung axes; to a world point, from a
+    rotated body, between dynamic bodies resting on a ground) bit for bit against the oracle,
+    the axes' dual state included."""
+    b, j = scenes.generic_joints()
+    w, o = lockstep(b, j, P(iterations=10), 150, every=10)
+    manifolds_equal(w, o)
+    gs, os_ = w.stats(), o.stats()
+    assert gs["manifolds"] > 0
+    for k in ("manifolds", "contacts", "candidate_pairs", "colours", "overflow_bodies"):
+        assert gs[k] == os_[k], k
+    _, gj = w.download(joints=True)
+    _, oj = o.download(joints=True)
+    for k in ("axis_lambda", "axis_penalty", "axis_initial"):
+        assert bits_equal(gj[k], oj[k]), k
+
+
 def test_chains_colour_sweep_lockstep():
     p = P(iterations=10, beta_lin=1e5, penalty_max=1e9)
     w, o = lockstep(*scenes.chains(16, links=32, grid=(4, 4), pitch_x=4.0), p, 150, every=15)
@@ -219,7 +236,8 @@ def test_narrow_phase_against_reference_fixture():
 
 def test_batch_alone_equals_in_mix():
     worlds = [scenes.pyramid(rows=12, seed=3), scenes.pile(300, seed=51), scenes.digest_mixed(),
-              scenes.chains(4, links=16, grid=(2, 2), pitch_x=3.0), scenes.pile(40, seed=52), scenes.digest_stack()]
+              scenes.chains(4, links=16, grid=(2, 2), pitch_x=3.0), scenes.pile(40, seed=52), scenes.digest_stack(),
+              scenes.generic_joints(), scenes.springs_and_filters()]
     p = P(iterations=10)
     batch = pkg.Batch(p, 16, 4000, 2000)
     ids = [batch.add(b, j) for b, j in worlds]

@@ -11,6 +11,7 @@ Pins, each chosen so that a plausible restatement mistake fails one of them:
   - Morton codes against the reference's own encoder, candidate pairs against brute force;
   - the reference's closed-form scenario checks (test/core_test.cpp:155-300, 1624-1638).
 """
+import math
 import os
 
 import numpy as np
@@ -291,3 +292,284 @@ def test_ignore_collision_drops_the_pair(filtered):
         assert len(o.manifolds()["body_a"]) == 0 and z < 0.0   # fell through the ground
     else:
         assert len(o.manifolds()["body_a"]) == 1 and abs(z - 1.0) < 5e-3
+
+
+# ---------------------------------------------------------------- generic joints ----
+# generic_joint.cpp has no recorded digest; the oracle's restatement is pinned by the
+# reference's own GenericJoint scenarios (test/core_test.cpp:703-1000), measured here from the
+# bodies' poses (not from the joint's formulas), and its atan2 against numpy's.
+LOCK, FREE = (1, 0, 0, 0, 0, 0), (0, 0, 0, 0, 0, 0)
+
+
+def _rig(body_pos, r_b, linear, angular, ang_vel=(0, 0, 0), lin_vel=(0, 0, 0), params=None):
+    s = scenes.SceneBuilder()
+    anchor = s.box((1, 1, 1), 0.0, 0.5, (0, 0, 0))
+    body = s.box((1, 1, 1), 1.0, 0.5, body_pos, vel=lin_vel)
+    s.b[body]["ang"] = ang_vel
+    s.generic(anchor, body, (0, 0, 0), r_b, linear, angular)
+    o = oracle.OracleWorld(params or P(iterations=10))
+    o.upload(*s.build())
+    return o
+
+
+def test_atan2_poly_against_numpy():
+    rng = np.random.default_rng(3)
+    y = rng.standard_normal(4000).astype(np.float32) * np.float32(3)
+    x = rng.standard_normal(4000).astype(np.float32) * np.float32(3)
+    worst = 0.0
+    for a, b in zip(y, x):
+        got = oracle.atan2(a, b)
+        ref = np.arctan2(np.float64(a), np.float64(b))
+        worst = max(worst, abs(got - ref) / float(np.spacing(np.float32(abs(ref)))))
+    assert worst <= 4.0, worst
+    for a, b, want in [(0.0, 1.0, 0.0), (-0.0, 1.0, -0.0), (0.0, -1.0, math.pi), (-0.0, -1.0, -math.pi),
+                       (1.0, 0.0, math.pi / 2), (-1.0, -0.0, -math.pi / 2), (1.0, 1.0, math.pi / 4)]:
+        got = oracle.atan2(a, b)
+        assert abs(got - want) <= 4e-7 * max(1.0, abs(want)) and math.copysign(1, got) == math.copysign(1, want)
+
+
+def test_generic_hinge_holds_its_locked_axes():
+    """core_test.cpp:742-790: a wheel spinning about its free axle; the two locked tilt axes
+    and the hub hold (0.01 rad / 0.01 m), and a slow spin is kept."""
+    o = _rig((0, 0, 0), (0, 0, 0), [LOCK] * 3, [LOCK, FREE, LOCK], ang_vel=(0, 30, 0))
+    worst_tilt = worst_hub = 0.0
+    for _ in range(300):
+        o.step()
+        st = o.download()
+        q, p = st["rotation"][0], st["position"][0]
+        worst_tilt = max(worst_tilt, abs(float(q[0])), abs(float(q[2])))
+        worst_hub = max(worst_hub, float(np.linalg.norm(p)))
+    assert worst_tilt <= 0.005 and worst_hub <= 0.01, (worst_tilt, worst_hub)
+    o = _rig((0, 0, 0), (0, 0, 0), [LOCK] * 3, [LOCK, FREE, LOCK], ang_vel=(0, 1, 0))
+    o.step(120)
+    assert o.download()["angular_velocity"][0][1] >= 0.98
+
+
+def test_generic_prismatic_spring_settles_at_mg_over_k():
+    """core_test.cpp:880-908: a strut free along Z (limited to ±2) with a k = 500 spring hangs
+    m g / k below the anchor."""
+    strut = (2, -2.0, 2.0, 500.0, 0.0, 0.0)
+    o = _rig((0, 0, 0), (0, 0, 0), [LOCK, LOCK, strut], [LOCK] * 3)
+    o.step(600)
+    travel = -float(o.download()["position"][0][2])   # anchor A minus anchor B along Z
+    expected = 1.0 * 10.0 / 500.0
+    assert abs(travel - expected) <= 0.2 * expected + 0.002, travel
+
+
+@pytest.mark.parametrize("beta_ang", [100.0, 1000.0])
+def test_generic_axis_limit_holds(beta_ang):
+    """core_test.cpp:800-878: kicked repeatedly, an axle limited to ±0.4 rad reaches the bound
+    and stays within 0.1 rad past it; the locked axes stay put."""
+    o = _rig((0, 0, 0), (0, 0, 0), [LOCK] * 3, [LOCK, (2, -0.4, 0.4, 0, 0, 0), LOCK],
+             params=P(iterations=10, beta_ang=beta_ang))
+    worst = reached = 0.0
+    for i in range(240):
+        if i % 10 == 0 and i < 60:
+            v = o.download()["angular_velocity"]
+            v[0, 1] += np.float32(6.0)
+            o.write_bodies({"angular_velocity": v})
+        o.step()
+        q = o.download()["rotation"][0].astype(np.float64)
+        angle = abs(2.0 * math.atan2(q[1], q[3]))
+        angle = min(angle, 2 * math.pi - angle)
+        reached = max(reached, angle)
+        worst = max(worst, angle - 0.4)
+        assert abs(q[0]) <= 0.01 and abs(q[2]) <= 0.01
+    assert reached >= 0.35 and worst <= 0.1, (reached, worst)
+
+
+def test_generic_ball_socket_holds_off_centre():
+    """core_test.cpp:915-947: a socket anchored at the body's top holds while it swings."""
+    o = _rig((0, 0, -1.0), (0, 0, 1.0), [LOCK] * 3, [FREE] * 3, lin_vel=(1.5, 0, 0))
+    worst = 0.0
+    for _ in range(300):
+        o.step()
+        st = o.download()
+        p, q = st["position"][0].astype(np.float64), st["rotation"][0].astype(np.float64)
+        worst = max(worst, float(np.linalg.norm(p + scenes._qrot64(q, (0, 0, 1.0)))))
+    assert worst <= 0.01 and o.download()["position"][0][2] < -0.5, worst
+
+
+def _q64mul(a, b):
+    ax, ay, az, aw = a
+    bx, by, bz, bw = b
+    return np.array([aw * bx + ax * bw + ay * bz - az * by, aw * by - ax * bz + ay * bw + az * bx,
+                     aw * bz + ax * by - ay * bx + az * bw, aw * bw - ax * bx - ay * by - az * bz])
+
+
+def _qexp(v):
+    th = float(np.linalg.norm(v))
+    if th == 0.0:
+        return np.array([0.0, 0.0, 0.0, 1.0])
+    return np.concatenate([np.asarray(v) / th * math.sin(th / 2), [math.cos(th / 2)]])
+
+
+def _rotvec(q):
+    q = q if q[3] >= 0 else -q
+    s = float(np.linalg.norm(q[:3]))
+    return np.zeros(3) if s == 0.0 else q[:3] / s * 2.0 * math.atan2(s, q[3])
+
+
+def _conj(q):
+    return np.array([-q[0], -q[1], -q[2], q[3]])
+
+
+def _generic_values(pa, qa, ra, pb, qb, rb, rest):
+    """A generic joint's six measurements, from their definition (generic_joint.cpp:69-96):
+    A's anchor minus B's in A's frame, then the orientation of A relative to B (its rest
+    orientation taken out) in A's frame, as a rotation vector."""
+    anchor_a = pa + scenes._qrot64(qa, ra)
+    anchor_b = pb + scenes._qrot64(qb, rb)
+    lin = scenes._qrot64(_conj(qa), anchor_a - anchor_b)
+    ang = _rotvec(_q64mul(rest, _conj(_q64mul(_conj(qa), qb))))
+    return np.concatenate([lin, ang])
+
+
+def _expected_stamp(st, jt, k, side, h=1e-6):
+    """The penalty-method stamp of every active axis, J^T K J and J^T F, with J and the axis
+    rates taken by finite differences of _generic_values (float64)."""
+    a, b = jt["body_a"][k], jt["body_b"][k]
+    pose = {i: (st["position"][i].astype(np.float64), st["rotation"][i].astype(np.float64)) for i in (a, b)}
+
+    def values(pa, qa, pb, qb):
+        return _generic_values(pa, qa, jt["r_a"][k].astype(np.float64), pb, qb,
+                               jt["r_b"][k].astype(np.float64), jt["rest_rotation"][k].astype(np.float64))
+
+    base = values(*pose[a], *pose[b])
+    J = np.zeros((6, 6))
+    for d in range(6):
+        p, q = pose[side]
+        e = np.zeros(3)
+        e[d % 3] = h
+        moved = (p + e, q) if d < 3 else (p, _q64mul(_qexp(e), q))
+        args = (*moved, *pose[b]) if side == a else (*pose[a], *moved)
+        J[:, d] = (values(*args) - base) / h
+    step = {i: (pose[i][0] + h * st["velocity"][i], _q64mul(_qexp(h * st["angular_velocity"][i]), pose[i][1]))
+            for i in (a, b)}
+    rate = (values(*step[a], *step[b]) - base) / h
+    lhs, rhs = np.zeros((6, 6)), np.zeros(6)
+    alpha = 0.99
+    for i in range(6):
+        mode, lo, hi = jt["axis_mode"][k][i], jt["axis_lower"][k][i], jt["axis_upper"][k][i]
+        pen, lam = float(jt["axis_penalty"][k][i]), float(jt["axis_lambda"][k][i])
+        v = base[i]
+        if mode == 1 or (mode == 2 and not lo <= v <= hi):
+            C = v - jt["axis_initial"][k][i] * alpha if mode == 1 else (v - hi if v > hi else v - lo)
+            F = pen * C + lam
+            if mode == 2:
+                F = max(F, 0.0) if C > 0 else min(F, 0.0)
+            lhs += pen * np.outer(J[i], J[i])
+            rhs += J[i] * F
+        ks = float(jt["axis_spring"][k][i])
+        if ks > 0:
+            lhs += ks * np.outer(J[i], J[i])
+            rhs += J[i] * (ks * (v - jt["axis_equilibrium"][k][i]) - jt["axis_damping"][k][i] * rate[i])
+    return lhs, rhs
+
+
+def _stamp_rig(a_dynamic):
+    s = scenes.SceneBuilder()
+    qa = _qexp(np.array([0.3, -0.5, 0.8]) * 0.9).astype(np.float32)
+    qb = _qexp(np.array([-0.7, 0.2, 0.4]) * 1.3).astype(np.float32)
+    pa = np.array([0.4, -0.2, 1.0], np.float32)
+    ra, rb = np.array([0.3, 0.5, -0.5], np.float32), np.array([-0.5, 0.25, 0.4], np.float32)
+    if a_dynamic:    # anchors coincide: the joint's Jacobian is the exact gradient there on both sides
+        pb = (pa + scenes._qrot64(qa, ra) - scenes._qrot64(qb, rb)).astype(np.float32)
+    else:
+        pb = np.array([0.9, 0.35, -0.6], np.float32)
+    A = s.box((1.0, 0.8, 1.2), 1.0 if a_dynamic else 0.0, 0.5, pa, rot=qa, vel=(0.2, -0.1, 0.3))
+    B = s.box((0.6, 1.0, 0.7), 2.0, 0.5, pb, rot=qb, vel=(-0.4, 0.3, 0.5))
+    s.b[A]["ang"] = (0.3, 0.1, -0.2) if a_dynamic else (0.0, 0.0, 0.0)
+    s.b[B]["ang"] = (0.5, -0.8, 0.6)
+    s.generic(A, B, ra, rb)
+    bodies, joints = s.build()
+    st = {k: bodies[k] for k in ("position", "rotation", "velocity", "angular_velocity")}
+    vals = _generic_values(st["position"][1].astype(np.float64), st["rotation"][1].astype(np.float64),
+                           ra.astype(np.float64), st["position"][0].astype(np.float64),
+                           st["rotation"][0].astype(np.float64), rb.astype(np.float64),
+                           joints["rest_rotation"][0].astype(np.float64))
+    # linear X locked, Y limited past its upper bound, Z a damped spring; angular X locked,
+    # Y limited (inside its range: no force), Z a damped spring
+    joints["axis_mode"][0] = [1, 2, 0, 1, 2, 0]
+    joints["axis_lower"][0] = [0, -1.0, 0, 0, -0.3, 0]
+    joints["axis_upper"][0] = [0, vals[1] - 0.05, 0, 0, 0.3, 0]
+    joints["axis_spring"][0] = [0, 0, 50.0, 0, 0, 7.0]
+    joints["axis_equilibrium"][0] = [0, 0, 0.2, 0, 0, -0.4]
+    joints["axis_damping"][0] = [0, 0, 3.0, 0, 0, 0.6]
+    joints["axis_penalty"] = np.array([[300.0, 200.0, 0, 40.0, 60.0, 0]], np.float32)
+    joints["axis_lambda"] = np.array([[-2.0, 1.5, 0, 0.5, 0, 0]], np.float32)
+    joints["axis_initial"] = np.array([[0.05, 0, 0, 0.02, 0, 0]], np.float32)
+    o = oracle.OracleWorld(P())
+    o.upload(bodies, joints)
+    return o, st, joints
+
+
+@pytest.mark.parametrize("a_dynamic", [False, True])
+def test_generic_stamp_is_the_constraint_gradient(a_dynamic):
+    """Each active axis of a generic joint must add penalty * J J^T and J F to the primal
+    block (generic_joint.cpp:157-277), where J is the gradient of the axis' measurement: here
+    J and the spring rates come from finite differences of the measurements' definition,
+    on a rotated frame with both anchors off-centre."""
+    o, st, jt = _stamp_rig(a_dynamic)
+    sides = (0, 1) if a_dynamic else (0,)
+    for side in sides:
+        lhs, rhs = o.force_stamp(0, side)
+        el, er = _expected_stamp(st, jt, 0, side)
+        assert np.abs(lhs - el).max() <= 1e-4 * np.abs(el).max(), (side, lhs - el)
+        assert np.abs(rhs - er).max() <= 1e-4 * np.abs(er).max() + 1e-6, (side, rhs, er)
+
+
+def _quat_matrix(q):
+    x, y, z, w = (float(c) for c in q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def test_generic_dual_and_warm_start_follow_the_penalty_method():
+    """The augmented-Lagrangian updates of a generic joint's axes (generic_joint.cpp:191-232,
+    279-319): the dual sets lambda to the clamped force and ramps the penalty by beta |C| up to
+    what one body can resolve in a step, (0.25 / dt)^2 times its mass or its inertia about the
+    axis (world inertia R I R^T); the warm start decays lambda by alpha gamma and the penalty
+    by gamma (clamped to [1, penalty_max]) and records the current measurement."""
+    o, st, jt = _stamp_rig(False)
+    a, b = jt["body_a"][0], jt["body_b"][0]
+    vals = _generic_values(st["position"][a].astype(np.float64), st["rotation"][a].astype(np.float64),
+                           jt["r_a"][0].astype(np.float64), st["position"][b].astype(np.float64),
+                           st["rotation"][b].astype(np.float64), jt["r_b"][0].astype(np.float64),
+                           jt["rest_rotation"][0].astype(np.float64))
+    p = P()
+    dt, alpha, gamma = float(np.float32(p.dt)), 0.99, 0.999
+    mass = 0.6 * 1.0 * 0.7 * 2.0
+    moment = np.diag([(1.0 + 0.49) / 12 * mass, (0.36 + 0.49) / 12 * mass, (0.36 + 1.0) / 12 * mass])
+    R_a, R_b = _quat_matrix(st["rotation"][a]), _quat_matrix(st["rotation"][b])
+    capped = []
+    o.force_update(0, "dual")
+    _, j1 = o.download(joints=True)
+    for i in range(6):
+        mode, lo, hi = jt["axis_mode"][0][i], jt["axis_lower"][0][i], jt["axis_upper"][0][i]
+        pen, lam, v = float(jt["axis_penalty"][0][i]), float(jt["axis_lambda"][0][i]), vals[i]
+        axis = R_a[:, i % 3]
+        if i < 3:
+            cap, beta = mass * (0.25 / dt) ** 2, 1.0e4
+        else:
+            cap, beta = float(axis @ (R_b @ moment @ R_b.T) @ axis) * (0.25 / dt) ** 2, 100.0
+        if mode == 1 or (mode == 2 and not lo <= v <= hi):
+            C = v - jt["axis_initial"][0][i] * alpha if mode == 1 else (v - hi if v > hi else v - lo)
+            F = pen * C + lam
+            if mode == 2:
+                F = max(F, 0.0) if C > 0 else min(F, 0.0)
+            want_lam, want_pen = F, min(pen + abs(C) * beta, cap)
+            capped.append(i) if cap < pen + abs(C) * beta else None
+        else:
+            want_lam, want_pen = (0.0 if mode == 2 else lam), pen
+        assert abs(j1["axis_lambda"][0][i] - want_lam) <= 1e-4 * max(1.0, abs(want_lam)), (i, j1["axis_lambda"][0])
+        assert abs(j1["axis_penalty"][0][i] - want_pen) <= 1e-4 * max(1.0, want_pen), (i, j1["axis_penalty"][0], want_pen)
+    assert 0 in capped and 3 in capped   # a linear and an angular cap bind
+    o.force_update(0, "init")
+    _, j2 = o.download(joints=True)
+    want_lam = j1["axis_lambda"][0].astype(np.float64) * alpha * gamma
+    want_pen = np.clip(j1["axis_penalty"][0].astype(np.float64) * gamma, 1.0, float(p.penalty_max))
+    assert np.allclose(j2["axis_lambda"][0], want_lam, rtol=1e-5, atol=1e-7)
+    assert np.allclose(j2["axis_penalty"][0], want_pen, rtol=1e-5)
+    assert np.allclose(j2["axis_initial"][0], vals, rtol=1e-4, atol=1e-5)
