@@ -4,7 +4,8 @@
 // src/avbd/types.hpp:75-139, src/avbd/forces.hpp:19-113): bodies and forces are created with
 // `new Rigid(&solver, size, density, friction, position[, velocity])`,
 // `new Joint(&solver, a, b, rA, rB[, kLin, kAng, fracture])`,
-// `new Spring(&solver, a, b, rA, rB, stiffness[, rest])` and `new IgnoreCollision(&solver, a, b)`,
+// `new Spring(&solver, a, b, rA, rB, stiffness[, rest])`, `new IgnoreCollision(&solver, a, b)`
+// and `new GenericJoint(&solver, a, b, rA, rB)` (then its `linear[i]` / `angular[i]` axes set),
 // which prepend themselves to the solver's intrusive lists exactly as the reference's do; the
 // caller sets the same public fields, calls step() / stepInterleaved(), and reads the same
 // `Rigid` / `Joint` fields.
@@ -122,6 +123,33 @@ struct IgnoreCollision : Force {
     inline IgnoreCollision(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bodyB);
 };
 
+// reference forces.hpp:115-121
+enum class AxisMode : int { Free = AVBD_AXIS_FREE, Locked = AVBD_AXIS_LOCKED, Limited = AVBD_AXIS_LIMITED };
+
+// reference forces.hpp:123-139: one degree of freedom of a GenericJoint and its dual state
+struct JointAxis {
+    AxisMode mode = AxisMode::Free;
+    float lower = 0.0f;
+    float upper = 0.0f;
+    float springStiffness = 0.0f;
+    float springEquilibrium = 0.0f;
+    float springDamping = 0.0f;
+    float lambda = 0.0f;
+    float penalty = 0.0f;
+    float initial = 0.0f;
+};
+
+// reference forces.hpp:141-190 (generic_joint.cpp): six degrees of freedom in A's frame (the
+// world's if A is null), each free, locked or limited, with an optional spring; the zero is
+// the arrangement at creation.
+struct GenericJoint : Force {
+    float3 rA, rB;
+    quat rest;
+    JointAxis linear[3];
+    JointAxis angular[3];
+    inline GenericJoint(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bodyB, float3 p_rA, float3 p_rB);
+};
+
 // reference maths.h:65-71 (Eigen's order) and linalg's folds, for the host-side rest length
 inline float3 hostCross(float3 a, float3 b)
 {
@@ -134,6 +162,13 @@ inline float3 hostQrot(quat q, float3 v)
     uv = float3{uv.x + uv.x, uv.y + uv.y, uv.z + uv.z};
     const float3 c = hostCross(z, uv);
     return float3{(v.x + uv.x * q.w) + c.x, (v.y + uv.y * q.w) + c.y, (v.z + uv.z * q.w) + c.z};
+}
+
+// reference maths.h:87-95: Hamilton product, the reference's term order
+inline quat hostQmul(quat a, quat b)
+{
+    return quat{((a.w * b.x + a.x * b.w) + a.y * b.z) - a.z * b.y, ((a.w * b.y + a.y * b.w) + a.z * b.x) - a.x * b.z,
+                ((a.w * b.z + a.z * b.w) + a.x * b.y) - a.y * b.x, ((a.w * b.w - a.x * b.x) - a.y * b.y) - a.z * b.z};
 }
 
 // reference solver.h:50-144
@@ -353,7 +388,10 @@ struct Solver {
         std::vector<float> jra(3 * nj), jrb(3 * nj), ksl(nj), ksa(nj), jfr(nj), arm(nj), c0l(3 * nj), c0a(3 * nj),
                 pl(3 * nj), pa(3 * nj), ll(3 * nj), la(3 * nj);
         std::vector<uint8_t> br(nj), kind(nj);
-        std::vector<float> rest(nj, 0.0f);
+        std::vector<float> rest(nj, 0.0f), restQ(4 * nj, 0.0f), axLo(6 * nj, 0.0f), axHi(6 * nj, 0.0f),
+                axK(6 * nj, 0.0f), axEq(6 * nj, 0.0f), axDamp(6 * nj, 0.0f), axLam(6 * nj, 0.0f), axPen(6 * nj, 0.0f),
+                axInit(6 * nj, 0.0f);
+        std::vector<uint8_t> axMode(6 * nj, 0);
         std::unordered_map<const Rigid *, int> idx;
         for (int i = 0; i < n; i++) idx[list[i]] = i;
         for (int k = 0; k < nj; k++) {
@@ -377,11 +415,25 @@ struct Solver {
                 for (int c = 0; c < 3; c++) { jra[3 * k + c] = sp->rA[c]; jrb[3 * k + c] = sp->rB[c]; }
                 ksl[k] = sp->stiffness;
                 rest[k] = sp->rest;
+            } else if (f->kind == AVBD_FORCE_GENERIC) {
+                const GenericJoint *g = static_cast<const GenericJoint *>(f);
+                for (int c = 0; c < 3; c++) { jra[3 * k + c] = g->rA[c]; jrb[3 * k + c] = g->rB[c]; }
+                const float q[4] = {g->rest.x, g->rest.y, g->rest.z, g->rest.w};
+                for (int c = 0; c < 4; c++) restQ[4 * k + c] = q[c];
+                for (int a = 0; a < 6; a++) {
+                    const JointAxis &x = a < 3 ? g->linear[a] : g->angular[a - 3];
+                    const int i = 6 * k + a;
+                    axMode[i] = static_cast<uint8_t>(x.mode);
+                    axLo[i] = x.lower; axHi[i] = x.upper;
+                    axK[i] = x.springStiffness; axEq[i] = x.springEquilibrium; axDamp[i] = x.springDamping;
+                    axLam[i] = x.lambda; axPen[i] = x.penalty; axInit[i] = x.initial;
+                }
             }
         }
         avbd_joints jv{nj, ja.data(), jb.data(), jra.data(), jrb.data(), ksl.data(), ksa.data(), jfr.data(),
                        arm.data(), c0l.data(), c0a.data(), pl.data(), pa.data(), ll.data(), la.data(), br.data(),
-                       kind.data(), rest.data()};
+                       kind.data(), rest.data(), restQ.data(), axMode.data(), axLo.data(), axHi.data(), axK.data(),
+                       axEq.data(), axDamp.data(), axLam.data(), axPen.data(), axInit.data()};
         lastStatus = avbd_world_upload(world, &bv, &jv);
         if (lastStatus != AVBD_OK) return false;
         if (keepCount > 0) {
@@ -444,8 +496,10 @@ struct Solver {
         std::vector<int32_t> ja(nj), jb(nj);
         std::vector<float> c0l(3 * nj), c0a(3 * nj), pl(3 * nj), pa(3 * nj), ll(3 * nj), la(3 * nj);
         std::vector<uint8_t> br(nj);
+        std::vector<float> axLam(6 * nj), axPen(6 * nj), axInit(6 * nj);
         avbd_joints j{nj, ja.data(), jb.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c0l.data(),
                       c0a.data(), pl.data(), pa.data(), ll.data(), la.data(), br.data()};
+        j.axis_lambda = axLam.data(); j.axis_penalty = axPen.data(); j.axis_initial = axInit.data();
         lastStatus = avbd_world_download(world, &b, nj ? &j : nullptr);
         if (lastStatus != AVBD_OK) return;
         for (int i = 0; i < n; i++) {
@@ -466,6 +520,14 @@ struct Solver {
             delete dead;
         }
         for (int k = 0; k < j.count && k < static_cast<int>(jl.size()); k++) {
+            if (jl[k]->kind == AVBD_FORCE_GENERIC) {
+                GenericJoint *g = static_cast<GenericJoint *>(jl[k]);
+                for (int a = 0; a < 6; a++) {
+                    JointAxis &x = a < 3 ? g->linear[a] : g->angular[a - 3];
+                    x.lambda = axLam[6 * k + a]; x.penalty = axPen[6 * k + a]; x.initial = axInit[6 * k + a];
+                }
+                continue;
+            }
             if (jl[k]->kind != AVBD_FORCE_JOINT) continue;   // springs and filters keep no dual state
             Joint *x = static_cast<Joint *>(jl[k]);
             for (int c = 0; c < 3; c++) {
@@ -544,6 +606,16 @@ inline Spring::Spring(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bodyB, float3 p
         const float dz = (p_bodyA->positionLin.z + qa.z) - (p_bodyB->positionLin.z + qb.z);
         rest = std::sqrt(((0.0f + dx * dx) + dy * dy) + dz * dz);
     }
+    p_solver->topologyChanged = true;
+}
+
+inline GenericJoint::GenericJoint(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bodyB, float3 p_rA, float3 p_rB)
+    : Force(p_solver, p_bodyA, p_bodyB), rA(p_rA), rB(p_rB)
+{
+    kind = AVBD_FORCE_GENERIC;
+    // generic_joint.cpp:24-30: rest = conj(frame) * rotation of B, frame = A's rotation or identity
+    const quat f = p_bodyA ? p_bodyA->positionAng : quat{};
+    rest = hostQmul(quat{-f.x, -f.y, -f.z, f.w}, p_bodyB->positionAng);
     p_solver->topologyChanged = true;
 }
 

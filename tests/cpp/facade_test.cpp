@@ -156,6 +156,50 @@ static void test_spring_and_filter()
            "spring equilibrium and collision filter", d);
 }
 
+// The reference's generic-joint rigs (test/core_test.cpp:742-790, 880-908) built with its own
+// constructor and axis fields: a spinning hinge holds its locked axes and hub, a sprung strut
+// hangs m g / k below its anchor, and the axes' dual state comes back to the host.
+static void test_generic_joint()
+{
+    Solver s;
+    Rigid *anchor = new Rigid(&s, {1, 1, 1}, 0.0f, 0.5f, {0, 0, 0});
+    Rigid *wheel = new Rigid(&s, {1, 1, 1}, 1.0f, 0.5f, {0, 0, 0});
+    GenericJoint *hinge = new GenericJoint(&s, anchor, wheel, {0, 0, 0}, {0, 0, 0});
+    for (int i = 0; i < 3; i++) hinge->linear[i].mode = AxisMode::Locked;
+    hinge->angular[0].mode = AxisMode::Locked;
+    hinge->angular[2].mode = AxisMode::Locked;
+    wheel->velocityAng = {0, 30.0f, 0};
+    Rigid *anchor2 = new Rigid(&s, {1, 1, 1}, 0.0f, 0.5f, {10, 0, 0});
+    Rigid *strut = new Rigid(&s, {1, 1, 1}, 1.0f, 0.5f, {10, 0, 0});
+    GenericJoint *slide = new GenericJoint(&s, anchor2, strut, {0, 0, 0}, {0, 0, 0});
+    for (int i = 0; i < 3; i++) slide->angular[i].mode = AxisMode::Locked;
+    slide->linear[0].mode = slide->linear[1].mode = AxisMode::Locked;
+    slide->linear[2] = JointAxis{AxisMode::Limited, -2.0f, 2.0f, 500.0f, 0.0f, 0.0f};
+    float worstTilt = 0.0f, worstHub = 0.0f;
+    for (int i = 0; i < 600; i++) {
+        s.step();
+        if (i < 300) {
+            worstTilt = std::fmax(worstTilt, std::fmax(std::fabs(wheel->positionAng.x), std::fabs(wheel->positionAng.z)));
+            const float3 p = wheel->positionLin;
+            worstHub = std::fmax(worstHub, std::sqrt(p.x * p.x + p.y * p.y + p.z * p.z));
+        }
+    }
+    const float travel = -strut->positionLin.z, expected = 1.0f * 10.0f / 500.0f;
+    char d[200];
+    std::snprintf(d, sizeof(d), "hinge |q.x|,|q.z| <= %.5f, hub %.5f m; strut travel %.5f m (m g / k = %.3f)",
+                  worstTilt, worstHub, travel, expected);
+    report(worstTilt <= 0.005f && worstHub <= 0.01f && std::fabs(travel - expected) <= 0.2f * expected + 0.002f &&
+               s.lastStatus == AVBD_OK,
+           "generic joint hinge and sprung strut", d);
+    // at rest the locked vertical axis carries the wheel's weight in its multiplier, the
+    // strut's limited axis (inside its range) holds none, and `initial` is the strut's travel
+    std::snprintf(d, sizeof(d), "hinge Z lambda %.4f (m g = 10), strut Z lambda %.3g, strut Z initial %.5f",
+                  hinge->linear[2].lambda, slide->linear[2].lambda, slide->linear[2].initial);
+    report(std::fabs(std::fabs(hinge->linear[2].lambda) - 10.0f) <= 0.2f && slide->linear[2].lambda == 0.0f &&
+               std::fabs(slide->linear[2].initial - expected) <= 0.2f * expected + 0.002f,
+           "generic joint dual state comes back", d);
+}
+
 int main()
 {
     const uint64_t want[3] = {0xfbac4f08cb1c977cull, 0xb03adeb37742850aull, 0xfb3ca7a3162035c2ull};
@@ -169,6 +213,7 @@ int main()
     test_topology_change_keeps_warm_start();
     test_break_now_reaps();
     test_spring_and_filter();
+    test_generic_joint();
     std::printf("%d checks, %d failures\n", g_checks, g_failures);
     return g_failures == 0 ? 0 : 1;
 }
