@@ -74,7 +74,7 @@ class Bodies(C.Structure):
         ("count", C.c_int32), ("size", f32p), ("mass", f32p), ("moment", f32p), ("friction", f32p),
         ("radius", f32p), ("gravity", f32p), ("layer", u32p), ("mask", u32p), ("lock_linear", u8p),
         ("lock_angular", u8p), ("position", f32p), ("rotation", f32p), ("velocity", f32p),
-        ("angular_velocity", f32p), ("prev_velocity", f32p),
+        ("angular_velocity", f32p), ("prev_velocity", f32p), ("shape", u8p),
     ]
 
 
@@ -131,7 +131,9 @@ BODY_FIELDS = {
     "layer": (np.uint32, 1), "mask": (np.uint32, 1), "lock_linear": (np.uint8, 1),
     "lock_angular": (np.uint8, 1), "position": (np.float32, 3), "rotation": (np.float32, 4),
     "velocity": (np.float32, 3), "angular_velocity": (np.float32, 3), "prev_velocity": (np.float32, 3),
+    "shape": (np.uint8, 1),
 }
+SHAPE_BOX, SHAPE_SPHERE, SHAPE_CYLINDER = 0, 1, 2
 JOINT_FIELDS = {
     "body_a": (np.int32, 1), "body_b": (np.int32, 1), "r_a": (np.float32, 3), "r_b": (np.float32, 3),
     "stiffness_lin": (np.float32, 1), "stiffness_ang": (np.float32, 1), "fracture": (np.float32, 1),
@@ -214,6 +216,7 @@ def load_library(path=LIB_PATH):
     sig = {
         "avbd_params_default": (None, [C.POINTER(Params)]),
         "avbd_box_properties": (None, [f32p, C.c_float, f32p, f32p, f32p]),
+        "avbd_shape_properties": (C.c_int, [C.c_int, f32p, C.c_float, f32p, f32p, f32p]),
         "avbd_state_digest": (C.c_uint64, [C.POINTER(Bodies)]),
         "avbd_last_error": (C.c_char_p, []),
         "avbd_device_count": (C.c_int32, []),

@@ -1364,6 +1364,34 @@ void avbd_box_properties(const float size[3], float density, float *mass, float 
     }
 }
 
+int avbd_shape_properties(int shape, const float size[3], float density, float *mass, float moment[3], float *radius)
+{
+    float m, mo[3], rad;
+    const float r = size[0];
+    if (shape == AVBD_SHAPE_SPHERE) {
+        m = (4.0f / 3.0f) * 3.14159265358979f * r * r * r * density;
+        const float i = 0.4f * m * r * r;
+        mo[0] = mo[1] = mo[2] = i;
+        rad = r;
+    } else if (shape == AVBD_SHAPE_CYLINDER) {
+        const float h = size[2];
+        m = 3.14159265358979f * r * r * h * density;
+        const float axial = 0.5f * m * r * r;
+        const float radial = m * (3.0f * r * r + h * h) / 12.0f;
+        mo[0] = mo[1] = radial; mo[2] = axial;
+        const float hh = h * 0.5f;
+        rad = std::sqrt(((0.0f + r * r) + hh * hh) + r * r);
+    } else if (shape == AVBD_SHAPE_BOX) {
+        avbd_box_properties(size, density, &m, mo, &rad);
+    } else {
+        return fail(AVBD_ERR_INVALID, "unknown shape");
+    }
+    if (mass) *mass = m;
+    if (moment) { moment[0] = mo[0]; moment[1] = mo[1]; moment[2] = mo[2]; }
+    if (radius) *radius = rad;
+    return AVBD_OK;
+}
+
 uint64_t avbd_state_digest(const avbd_bodies *b)
 {
     uint64_t h = 1469598103934665603ull;

@@ -43,6 +43,31 @@ class SceneBuilder:
                            rotation=np.asarray(rot, dtype=F), velocity=np.asarray(vel, dtype=F)))
         return len(self.b) - 1
 
+    def sphere(self, radius, density, friction, pos, rot=(0.0, 0.0, 0.0, 1.0), vel=(0.0, 0.0, 0.0)):
+        """Reference `Rigid(..., ShapeType::Sphere, ...)` (rigid.cpp:36-44): size = (r, 0, 0)."""
+        r, d = F(radius), F(density)
+        m = F(4.0 / 3.0) * F(3.14159265358979) * r * r * r * d
+        i = F(0.4) * m * r * r
+        return self._add(np.array([r, 0, 0], F), m, np.array([i, i, i], F), friction, r, pos, rot, vel, 1)
+
+    def cylinder(self, radius, height, density, friction, pos, rot=(0.0, 0.0, 0.0, 1.0), vel=(0.0, 0.0, 0.0)):
+        """Reference `Rigid(..., ShapeType::Cylinder, ...)` (rigid.cpp:46-57): size = (r, r, h),
+        axis local +Z."""
+        r, h, d = F(radius), F(height), F(density)
+        m = F(3.14159265358979) * r * r * h * d
+        axial = F(0.5) * m * r * r
+        radial = m * (F(3.0) * r * r + h * h) / F(12.0)
+        hh = h * F(0.5)
+        rad = np.sqrt(((F(0.0) + r * r) + hh * hh) + r * r)
+        return self._add(np.array([r, r, h], F), m, np.array([radial, radial, axial], F), friction, rad, pos,
+                         rot, vel, 2)
+
+    def _add(self, size, m, moment, friction, radius, pos, rot, vel, shape):
+        self.b.append(dict(size=size, mass=F(m), moment=moment, friction=F(friction), radius=F(radius),
+                           gravity=self.gravity, position=np.asarray(pos, dtype=F),
+                           rotation=np.asarray(rot, dtype=F), velocity=np.asarray(vel, dtype=F), shape=shape))
+        return len(self.b) - 1
+
     def joint(self, a, b, r_a, r_b, k_lin=math.inf, k_ang=0.0, fracture=math.inf):
         sa = self.b[a]["size"] if a is not None and a >= 0 else np.zeros(3, F)
         s = sa + self.b[b]["size"]
@@ -103,6 +128,8 @@ class SceneBuilder:
                                  if n else np.zeros((0, 3), F)),
         }
         bodies["prev_velocity"] = bodies["velocity"].copy()
+        if any(self.b[c].get("shape", 0) for c in order):
+            bodies["shape"] = np.array([self.b[c].get("shape", 0) for c in order], np.uint8)
         nj = len(self.j)
         jorder = list(range(nj - 1, -1, -1))
         joints = None

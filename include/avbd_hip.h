@@ -91,7 +91,15 @@ typedef struct avbd_bodies {
     float *velocity;         /* 3n                                                   */
     float *angular_velocity; /* 3n                                                   */
     float *prev_velocity;    /* 3n previous step's linear velocity (null: velocity)  */
+    uint8_t *shape;          /* n AVBD_SHAPE_* (null: every body a box)              */
 } avbd_bodies;
+
+/* Body shapes (reference `ShapeType`, src/avbd/types.hpp:52-59).  `size` per shape: box =
+ * full extents; sphere = (radius, unused, unused); cylinder = (radius, unused, height), its
+ * axis the body's local +Z. */
+#define AVBD_SHAPE_BOX 0
+#define AVBD_SHAPE_SPHERE 1
+#define AVBD_SHAPE_CYLINDER 2
 
 /* Joint arrays (reference `Joint`, src/avbd/forces.hpp:58-87), list order.  body_a = -1
  * anchors body_b to the world point r_a.  A ball joint is stiffness_lin = +inf,
@@ -218,6 +226,12 @@ void avbd_params_default(avbd_params *p);
  * radius = |size/2|. */
 void avbd_box_properties(const float size[3], float density, float *mass, float moment[3],
                          float *radius);
+/* Mass properties of any shape exactly as the reference body constructor computes them
+ * (src/avbd/rigid.cpp:34-70): sphere m = 4/3 pi r^3 density, I = 2/5 m r^2, radius r;
+ * cylinder m = pi r^2 h density, I = (m (3 r^2 + h^2) / 12, same, m r^2 / 2), radius
+ * |(r, h/2, r)|; box as avbd_box_properties.  Returns AVBD_ERR_INVALID for an unknown shape. */
+int avbd_shape_properties(int shape, const float size[3], float density, float *mass, float moment[3],
+                          float *radius);
 
 /* FNV-1a digest over the raw bytes of position(12), rotation(16), velocity(12) and
  * angular velocity(12) of every body in list order (reference test/core_test.cpp:133-149). */

@@ -49,7 +49,7 @@ struct Body {
     f3 inertLin; quat inertAng;
     f3 vel, angVel, prevVel;
     f3 size; float mass; f3 moment; float friction, radius, gravity;
-    uint32_t layer, mask; uint8_t lockLin, lockAng;
+    uint32_t layer, mask; uint8_t lockLin, lockAng; uint8_t shape = AVBD_SHAPE_BOX;
     bool inUpdate = false; int updateIndex = -1; bool contactOnly = true;
 };
 
@@ -404,6 +404,325 @@ int edgeContact(const ShapeIn &sA, const ShapeIn &sB, const OBB &bA, const OBB &
     return cnt;
 }
 
+// ------------------------------------------------------------ round shapes ----
+// collide.cpp:505-998: spheres and capped cylinders.  A shape carries what makeShape derives
+// from a body (collide.cpp:505-516): half extents, radius = size.x, half height = size.z / 2,
+// the cylinder axis = local +Z in world space.
+struct RShape { int type; f3 c; quat q; f3 half; float r, hh; f3 axis; };
+
+RShape makeRShape(int type, f3 p, quat q, f3 size)
+{
+    return RShape{type, p, q, size * 0.5f, size.x, size.z * 0.5f, qrot(q, f3{0, 0, 1})};
+}
+
+enum { AXIS_CYL_SAMPLE = 3, AXIS_CYL_CORNER = 4 };
+constexpr int RIM = 8, RINGS = 3;
+constexpr int CYL_SAMPLES = 2 * (RIM + 1) + RIM * (RINGS - 1);   // 34
+constexpr int MAX_CAND = 2 * CYL_SAMPLES + 8;
+
+// collide.cpp:733-753: rim directions at angle steps i*5 mod 8 (coprime stride), from the
+// C library's cosf / sinf evaluated at run time.
+struct Rim { float c[RIM], s[RIM]; };
+const Rim &rim()
+{
+    static const Rim t = [] {
+        Rim r{};
+        for (int i = 0; i < RIM; i++) {
+            volatile float a = 6.28318530718f * static_cast<float>(i * 5 % RIM) / static_cast<float>(RIM);
+            r.c[i] = std::cos(static_cast<float>(a));
+            r.s[i] = std::sin(static_cast<float>(a));
+        }
+        return r;
+    }();
+    return t;
+}
+
+// collide.cpp:520-546
+bool pointInBox(const RShape &b, f3 p, f3 &push, float &depth)
+{
+    const f3 l = qrot(qconj(b.q), p - b.c);
+    float best = FLT_MAX; int axis = 0; float sign = 1.0f;
+    for (int i = 0; i < 3; i++) {
+        const float d = b.half[i] - std::fabs(l[i]);
+        if (d <= 0.0f) return false;
+        if (d < best) { best = d; axis = i; sign = l[i] >= 0.0f ? 1.0f : -1.0f; }
+    }
+    f3 lp{0, 0, 0};
+    lp[axis] = sign;
+    push = qrot(b.q, lp);
+    depth = best;
+    return true;
+}
+
+// collide.cpp:549-577
+bool pointNearBox(const RShape &b, f3 p, float margin, f3 &push, float &depth)
+{
+    const f3 l = qrot(qconj(b.q), p - b.c);
+    bool outside = false;
+    for (int i = 0; i < 3; i++)
+        if (std::fabs(l[i]) >= b.half[i]) { outside = true; break; }
+    if (!outside) return false;
+    const f3 cl{sclamp(l.x, -b.half.x, b.half.x), sclamp(l.y, -b.half.y, b.half.y), sclamp(l.z, -b.half.z, b.half.z)};
+    const f3 d = l - cl;
+    const float gap = length(d);
+    if (gap > margin || gap <= 1.0e-9f) return false;
+    push = qrot(b.q, d / gap);
+    depth = -gap;
+    return true;
+}
+
+// collide.cpp:580-614
+f3 closestOnCylinder(const RShape &cy, f3 p, f3 &outward, float &distance)
+{
+    const f3 l = qrot(qconj(cy.q), p - cy.c);
+    const float t = sclamp(l.z, -cy.hh, cy.hh);
+    const float lenSq = (0.0f + l.x * l.x) + l.y * l.y;
+    f3 cl{0, 0, t};
+    if (lenSq > cy.r * cy.r) {
+        const float scale = cy.r / std::sqrt(lenSq);
+        cl.x = l.x * scale; cl.y = l.y * scale;
+    } else if (lenSq > 1.0e-12f) {
+        cl.x = l.x; cl.y = l.y;
+    } else {
+        cl.x = cy.r;
+    }
+    const f3 w = qrot(cy.q, cl) + cy.c;
+    const f3 d = p - w;
+    distance = length(d);
+    outward = distance > 1.0e-9f ? d / distance : qrot(cy.q, f3{1, 0, 0});
+    return w;
+}
+
+// collide.cpp:617-651
+bool pointInCylinder(const RShape &cy, f3 p, f3 &push, float &surface)
+{
+    const f3 l = qrot(qconj(cy.q), p - cy.c);
+    const float rs = l.x * l.x + l.y * l.y;
+    if (rs >= cy.r * cy.r || std::fabs(l.z) >= cy.hh) return false;
+    const float radialDepth = cy.r - std::sqrt(rs);
+    const float capDepth = cy.hh - std::fabs(l.z);
+    f3 pl{0, 0, 0};
+    if (capDepth < radialDepth) {
+        pl.z = l.z >= 0.0f ? 1.0f : -1.0f;
+        surface = capDepth;
+    } else {
+        const float len = std::sqrt(rs);
+        if (len > 1.0e-6f) { pl.x = l.x / len; pl.y = l.y / len; }
+        else pl.x = 1.0f;
+        surface = radialDepth;
+    }
+    push = qrot(cy.q, pl);
+    return true;
+}
+
+// collide.cpp:654-671
+bool pointNearCylinder(const RShape &cy, f3 p, float margin, f3 &push, float &depth)
+{
+    const f3 l = qrot(qconj(cy.q), p - cy.c);
+    const float rs = l.x * l.x + l.y * l.y;
+    if (rs < cy.r * cy.r && std::fabs(l.z) < cy.hh) return false;
+    f3 out; float dist;
+    closestOnCylinder(cy, p, out, dist);
+    if (dist > margin) return false;
+    push = out;
+    depth = -dist;
+    return true;
+}
+
+// collide.cpp:674-684: the exact inside test first, the speculative one only when it fails
+bool insideBox(const RShape &b, f3 p, float margin, f3 &push, float &depth)
+{
+    return pointInBox(b, p, push, depth) || (margin > 0.0f && pointNearBox(b, p, margin, push, depth));
+}
+bool insideCylinder(const RShape &cy, f3 p, float margin, f3 &push, float &depth)
+{
+    return pointInCylinder(cy, p, push, depth) || (margin > 0.0f && pointNearCylinder(cy, p, margin, push, depth));
+}
+
+// collide.cpp:755-784: both caps (centre, then the rim), then the inner rings
+int cylinderSamples(const RShape &cy, f3 *out)
+{
+    int n = 0;
+    const f3 x = qrot(cy.q, f3{1, 0, 0}), y = qrot(cy.q, f3{0, 1, 0});
+    const Rim &t = rim();
+    for (int end = -1; end <= 1; end += 2) {
+        const f3 cc = cy.c + cy.axis * (cy.hh * static_cast<float>(end));
+        out[n++] = cc;
+        for (int i = 0; i < RIM; i++) out[n++] = cc + (x * t.c[i] + y * t.s[i]) * cy.r;
+    }
+    for (int ring = 1; ring < RINGS; ring++) {
+        const float tt = -cy.hh + 2.0f * cy.hh * static_cast<float>(ring) / static_cast<float>(RINGS);
+        const f3 rc = cy.c + cy.axis * tt;
+        for (int i = 0; i < RIM; i++) out[n++] = rc + (x * t.c[i] + y * t.s[i]) * cy.r;
+    }
+    return n;
+}
+
+struct Cand { f3 xA, xB, n; float depth; int32_t key; };
+
+// collide.cpp:702-730: stable by depth (deepest first), the first MAXC offered in turn,
+// stopping at the first one the manifold refuses
+int emitRanked(const ShapeIn &A, const ShapeIn &B, const Cand *cand, int nc, Contact *cs)
+{
+    Cand ranked[MAX_CAND];
+    for (int i = 0; i < nc; i++) {
+        int j = i;
+        while (j > 0 && ranked[j - 1].depth < cand[i].depth) { ranked[j] = ranked[j - 1]; --j; }
+        ranked[j] = cand[i];
+    }
+    int cnt = 0;
+    f3 mids[MAXC];
+    const int emit = nc < MAXC ? nc : MAXC;
+    for (int i = 0; i < emit; i++)
+        if (!addContact(A, B, cs, cnt, mids, ranked[i].xA, ranked[i].xB, ranked[i].key)) break;
+    return cnt;
+}
+
+// collide.cpp:786-806
+int sphereSphere(const RShape &a, const RShape &b, Contact *cs, m3 &basis, float margin)
+{
+    const f3 d = b.c - a.c;
+    const float distSq = length2(d);
+    const float sumPlus = a.r + b.r + margin;
+    if (distSq >= sumPlus * sumPlus) return 0;
+    const float dist = std::sqrt(distSq);
+    const f3 n = dist > 1.0e-6f ? d / dist : f3{0, 0, 1};
+    basis = orthonormal(-n);
+    int cnt = 0; f3 mids[MAXC];
+    addContact(ShapeIn{a.c, a.q}, ShapeIn{b.c, b.q}, cs, cnt, mids, a.c + n * a.r, b.c - n * b.r, 1);
+    return cnt;
+}
+
+// collide.cpp:809-862
+int sphereBox(const RShape &a, const RShape &b, bool sphereIsA, Contact *cs, m3 &basis, float margin)
+{
+    const RShape &sp = sphereIsA ? a : b;
+    const RShape &bx = sphereIsA ? b : a;
+    const f3 l = qrot(qconj(bx.q), sp.c - bx.c);
+    const f3 cl{sclamp(l.x, -bx.half.x, bx.half.x), sclamp(l.y, -bx.half.y, bx.half.y), sclamp(l.z, -bx.half.z, bx.half.z)};
+    const f3 onBox = qrot(bx.q, cl) + bx.c;
+    const f3 d = sp.c - onBox;
+    const float distSq = length2(d);
+    const float radPlus = sp.r + margin;
+    if (distSq >= radPlus * radPlus) return 0;
+    const float dist = std::sqrt(distSq);
+    f3 n;
+    if (dist > 1.0e-6f) {
+        n = d / dist;
+    } else {
+        float depth;
+        if (!pointInBox(bx, sp.c, n, depth)) return 0;
+    }
+    basis = orthonormal(-(sphereIsA ? -n : n));
+    const f3 onSphere = sp.c - n * sp.r;
+    int cnt = 0; f3 mids[MAXC];
+    addContact(ShapeIn{a.c, a.q}, ShapeIn{b.c, b.q}, cs, cnt, mids, sphereIsA ? onSphere : onBox,
+               sphereIsA ? onBox : onSphere, 1);
+    return cnt;
+}
+
+// collide.cpp:865-891
+int sphereCylinder(const RShape &a, const RShape &b, bool sphereIsA, Contact *cs, m3 &basis, float margin)
+{
+    const RShape &sp = sphereIsA ? a : b;
+    const RShape &cy = sphereIsA ? b : a;
+    f3 out; float dist;
+    const f3 onCyl = closestOnCylinder(cy, sp.c, out, dist);
+    if (dist >= sp.r + margin) return 0;
+    basis = orthonormal(-(sphereIsA ? -out : out));
+    const f3 onSphere = sp.c - out * sp.r;
+    int cnt = 0; f3 mids[MAXC];
+    addContact(ShapeIn{a.c, a.q}, ShapeIn{b.c, b.q}, cs, cnt, mids, sphereIsA ? onSphere : onCyl,
+               sphereIsA ? onCyl : onSphere, 1);
+    return cnt;
+}
+
+// collide.cpp:894-954
+int cylinderBox(const RShape &a, const RShape &b, bool cylIsA, Contact *cs, m3 &basis, float margin)
+{
+    const RShape &cy = cylIsA ? a : b;
+    const RShape &bx = cylIsA ? b : a;
+    Cand cand[MAX_CAND];
+    int nc = 0;
+    f3 nAB{0, 0, 0};
+    f3 smp[CYL_SAMPLES];
+    const int ns = cylinderSamples(cy, smp);
+    for (int i = 0; i < ns && nc < MAX_CAND; i++) {
+        f3 push; float depth;
+        if (!insideBox(bx, smp[i], margin, push, depth)) continue;
+        Cand &c = cand[nc++];
+        c.xA = cylIsA ? smp[i] : smp[i] + push * depth;
+        c.xB = cylIsA ? smp[i] + push * depth : smp[i];
+        c.n = cylIsA ? -push : push;
+        c.depth = depth;
+        c.key = static_cast<int32_t>((uint32_t(AXIS_CYL_SAMPLE) << 24) | (uint32_t(cylIsA ? 1u : 0u) << 20) |
+                                     static_cast<uint32_t>(i));
+        nAB += c.n;
+    }
+    for (int k = 0; k < 8 && nc < MAX_CAND; k++) {
+        const f3 cl{(k & 1) ? bx.half.x : -bx.half.x, (k & 2) ? bx.half.y : -bx.half.y,
+                    (k & 4) ? bx.half.z : -bx.half.z};
+        const f3 cw = qrot(bx.q, cl) + bx.c;
+        f3 push; float sd;
+        if (!insideCylinder(cy, cw, margin, push, sd)) continue;
+        Cand &c = cand[nc++];
+        c.xA = cylIsA ? cw + push * sd : cw;
+        c.xB = cylIsA ? cw : cw + push * sd;
+        c.n = cylIsA ? push : -push;
+        c.depth = sd;
+        c.key = static_cast<int32_t>((uint32_t(AXIS_CYL_CORNER) << 24) | (uint32_t(cylIsA ? 0u : 1u) << 20) |
+                                     static_cast<uint32_t>(k));
+        nAB += c.n;
+    }
+    if (!nc) return 0;
+    basis = orthonormal(-(length2(nAB) > 1.0e-12f ? normalize(nAB) : cy.axis));
+    return emitRanked(ShapeIn{a.c, a.q}, ShapeIn{b.c, b.q}, cand, nc, cs);
+}
+
+// collide.cpp:957-998
+int cylinderCylinder(const RShape &a, const RShape &b, Contact *cs, m3 &basis, float margin)
+{
+    Cand cand[MAX_CAND];
+    int nc = 0;
+    f3 nAB{0, 0, 0};
+    f3 smp[CYL_SAMPLES];
+    for (int pass = 0; pass < 2; pass++) {
+        const RShape &from = pass == 0 ? a : b;
+        const RShape &into = pass == 0 ? b : a;
+        const bool fromIsA = pass == 0;
+        const int ns = cylinderSamples(from, smp);
+        for (int i = 0; i < ns && nc < MAX_CAND; i++) {
+            f3 push; float sd;
+            if (!insideCylinder(into, smp[i], margin, push, sd)) continue;
+            Cand &c = cand[nc++];
+            c.xA = fromIsA ? smp[i] : smp[i] + push * sd;
+            c.xB = fromIsA ? smp[i] + push * sd : smp[i];
+            c.n = fromIsA ? -push : push;
+            c.depth = sd;
+            c.key = static_cast<int32_t>((uint32_t(AXIS_CYL_SAMPLE) << 24) | (uint32_t(fromIsA ? 1u : 0u) << 20) |
+                                         static_cast<uint32_t>(i));
+            nAB += c.n;
+        }
+    }
+    if (!nc) return 0;
+    basis = orthonormal(-(length2(nAB) > 1.0e-12f ? normalize(nAB) : a.axis));
+    return emitRanked(ShapeIn{a.c, a.q}, ShapeIn{b.c, b.q}, cand, nc, cs);
+}
+
+// collide.cpp:1001-1032: the dispatch for pairs with a round shape
+int collideRound(const RShape &a, const RShape &b, Contact *cs, m3 &basis, float margin)
+{
+    const int S = AVBD_SHAPE_SPHERE, C = AVBD_SHAPE_CYLINDER;
+    if (a.type == S && b.type == S) return sphereSphere(a, b, cs, basis, margin);
+    if (a.type == S || b.type == S) {
+        const bool sphereIsA = a.type == S;
+        if (a.type == C || b.type == C) return sphereCylinder(a, b, sphereIsA, cs, basis, margin);
+        return sphereBox(a, b, sphereIsA, cs, basis, margin);
+    }
+    if (a.type == C && b.type == C) return cylinderCylinder(a, b, cs, basis, margin);
+    return cylinderBox(a, b, a.type == C, cs, basis, margin);
+}
+
 } // namespace
 
 // collide.cpp:1034-1097 (box-box SAT + clipping).  Returns the contact count; `basis` is
@@ -475,6 +794,31 @@ extern "C" int oracle_collide_boxes(const float posA[3], const float rotA[4], co
     if (best.type == AXIS_EDGE) n = edgeContact(sA, sB, bA, bB, best.ia, best.ib, best.n, cs);
     else if (best.type == AXIS_FACE_A) n = faceManifold(sA, sB, bA, bB, true, best.ia, best.n, margin, cs);
     else n = faceManifold(sA, sB, bA, bB, false, best.ib, best.n, margin, cs);
+    for (int k = 0; k < n; k++) {
+        keys[k] = cs[k].key;
+        rA[3 * k] = cs[k].rA.x; rA[3 * k + 1] = cs[k].rA.y; rA[3 * k + 2] = cs[k].rA.z;
+        rB[3 * k] = cs[k].rB.x; rB[3 * k + 1] = cs[k].rB.y; rB[3 * k + 2] = cs[k].rB.z;
+    }
+    return n;
+}
+
+// collide.cpp:1001-1097: any pair of shapes (AVBD_SHAPE_*); box-box takes the SAT path above.
+extern "C" int oracle_collide_shapes(int typeA, const float posA[3], const float rotA[4], const float sizeA[3],
+                                     int typeB, const float posB[3], const float rotB[4], const float sizeB[3],
+                                     float margin, float basis[9], int32_t keys[8], float rA[24], float rB[24])
+{
+    if (typeA == AVBD_SHAPE_BOX && typeB == AVBD_SHAPE_BOX)
+        return oracle_collide_boxes(posA, rotA, sizeA, posB, rotB, sizeB, margin, basis, keys, rA, rB);
+    const RShape a = makeRShape(typeA, f3{posA[0], posA[1], posA[2]}, quat{rotA[0], rotA[1], rotA[2], rotA[3]},
+                                f3{sizeA[0], sizeA[1], sizeA[2]});
+    const RShape b = makeRShape(typeB, f3{posB[0], posB[1], posB[2]}, quat{rotB[0], rotB[1], rotB[2], rotB[3]},
+                                f3{sizeB[0], sizeB[1], sizeB[2]});
+    Contact cs[MAXC] = {};
+    m3 B{};
+    const int n = collideRound(a, b, cs, B, margin);
+    if (n > 0)
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) basis[r * 3 + c] = B[c][r];
     for (int k = 0; k < n; k++) {
         keys[k] = cs[k].key;
         rA[3 * k] = cs[k].rA.x; rA[3 * k + 1] = cs[k].rA.y; rA[3 * k + 2] = cs[k].rA.z;
@@ -577,7 +921,10 @@ struct World {
         cen.resize(n); half.resize(n);
         for (int i = 0; i < n; i++) {
             const Body &b = bodies[i];
-            const f3 h = b.size * 0.5f;
+            // rigid.cpp:83-99: the shape's local half extents
+            const f3 h = b.shape == AVBD_SHAPE_SPHERE     ? f3{b.size.x, b.size.x, b.size.x}
+                         : b.shape == AVBD_SHAPE_CYLINDER ? f3{b.size.x, b.size.x, b.size.z * 0.5f}
+                                                          : b.size * 0.5f;
             const m3 R = qmat(b.q);
             const f3 tight{
                 (std::fabs(R[0][0]) * h.x + std::fabs(R[1][0]) * h.y) + std::fabs(R[2][0]) * h.z,
@@ -728,7 +1075,7 @@ struct World {
                     sa[3] = {a.size.x, a.size.y, a.size.z};
         const float pb[3] = {b.pos.x, b.pos.y, b.pos.z}, qb[4] = {b.q.x, b.q.y, b.q.z, b.q.w},
                     sb[3] = {b.size.x, b.size.y, b.size.z};
-        const int nn = oracle_collide_boxes(pa, qa, sa, pb, qb, sb, marginFor(a, b), bas, keys, ra, rb);
+        const int nn = oracle_collide_shapes(a.shape, pa, qa, sa, b.shape, pb, qb, sb, marginFor(a, b), bas, keys, ra, rb);
         for (int k = 0; k < nn; k++)
             nc[k] = Contact{keys[k], {ra[3 * k], ra[3 * k + 1], ra[3 * k + 2]}, {rb[3 * k], rb[3 * k + 1], rb[3 * k + 2]}};
         // The reference writes the basis only when the pair is within the margin on every
@@ -1625,6 +1972,8 @@ int oracle_upload(oracle_world *o, const avbd_bodies *b, const avbd_joints *j)
         x.mask = b->mask ? b->mask[i] : 1u;
         x.lockLin = b->lock_linear ? b->lock_linear[i] : 0;
         x.lockAng = b->lock_angular ? b->lock_angular[i] : 0;
+        x.shape = b->shape ? b->shape[i] : static_cast<uint8_t>(AVBD_SHAPE_BOX);
+        if (x.shape > AVBD_SHAPE_CYLINDER) return AVBD_ERR_INVALID;
         x.pos = load3(b->position, i);
         x.q = quat{b->rotation[4 * i], b->rotation[4 * i + 1], b->rotation[4 * i + 2], b->rotation[4 * i + 3]};
         x.vel = load3(b->velocity, i);

@@ -37,6 +37,9 @@ int oracle_solve6(const float *a36, const float *b6, float *x6);
 int oracle_collide_boxes(const float posA[3], const float rotA[4], const float sizeA[3],
                          const float posB[3], const float rotB[4], const float sizeB[3],
                          float margin, float basis[9], int32_t keys[8], float rA[24], float rB[24]);
+int oracle_collide_shapes(int typeA, const float posA[3], const float rotA[4], const float sizeA[3],
+                          int typeB, const float posB[3], const float rotB[4], const float sizeB[3],
+                          float margin, float basis[9], int32_t keys[8], float rA[24], float rB[24]);
 
 #ifdef __cplusplus
 }

@@ -67,6 +67,8 @@ def lib():
         "oracle_solve6": (C.c_int, [a.f32p, a.f32p, a.f32p]),
         "oracle_collide_boxes": (C.c_int, [a.f32p, a.f32p, a.f32p, a.f32p, a.f32p, a.f32p, C.c_float,
                                            a.f32p, a.i32p, a.f32p, a.f32p]),
+        "oracle_collide_shapes": (C.c_int, [C.c_int, a.f32p, a.f32p, a.f32p, C.c_int, a.f32p, a.f32p, a.f32p,
+                                            C.c_float, a.f32p, a.i32p, a.f32p, a.f32p]),
         "oracle_atan2": (C.c_float, [C.c_float, C.c_float]),
     }
     for name, (res, args) in sig.items():
@@ -89,6 +91,9 @@ def ref_lib():
     R.ref_collide_boxes.restype = C.c_int
     R.ref_collide_boxes.argtypes = [a.f32p, a.f32p, a.f32p, a.f32p, a.f32p, a.f32p, C.c_float,
                                     a.f32p, a.i32p, a.f32p, a.f32p]
+    R.ref_collide_shapes.restype = C.c_int
+    R.ref_collide_shapes.argtypes = [C.c_int, a.f32p, a.f32p, a.f32p, C.c_int, a.f32p, a.f32p, a.f32p, C.c_float,
+                                     a.f32p, a.i32p, a.f32p, a.f32p]
     R.ref_solve6.restype = C.c_int
     R.ref_solve6.argtypes = [a.f32p, a.f32p, a.f32p]
     R.ref_morton3D.restype = C.c_uint64
@@ -253,6 +258,22 @@ def collide_boxes(pa, qa, sa, pb, qb, sb, margin):
 def ref_collide_boxes(pa, qa, sa, pb, qb, sb, margin):
     R = ref_lib()
     return None if R is None else _collide(R.ref_collide_boxes, pa, qa, sa, pb, qb, sb, margin)
+
+
+def _collide_typed(fn, ta, pa, qa, sa, tb, pb, qb, sb, margin):
+    def call(*args):
+        return fn(int(ta), args[0], args[1], args[2], int(tb), *args[3:])
+    return _collide(call, pa, qa, sa, pb, qb, sb, margin)
+
+
+def collide_shapes(ta, pa, qa, sa, tb, pb, qb, sb, margin):
+    """(count, basis row-major, keys, rA, rB) for any pair of shapes (abi.SHAPE_*)."""
+    return _collide_typed(lib().oracle_collide_shapes, ta, pa, qa, sa, tb, pb, qb, sb, margin)
+
+
+def ref_collide_shapes(ta, pa, qa, sa, tb, pb, qb, sb, margin):
+    R = ref_lib()
+    return None if R is None else _collide_typed(R.ref_collide_shapes, ta, pa, qa, sa, tb, pb, qb, sb, margin)
 
 
 def _solve6(fn, A, b):

@@ -1,18 +1,19 @@
 // ORACLE - test infrastructure only.  C entry points over the reference's own sources,
-// compiled from where they lie in the reference tree (see oracle/Makefile): the box-box
-// narrow phase of src/avbd/collide.cpp and the header-only 6x6 solves of
+// compiled from where they lie in the reference tree (see oracle/Makefile): the narrow phase
+// of src/avbd/collide.cpp (every shape pair) and the header-only 6x6 solves of
 // src/avbd/maths.h.  They pin the restatement in avbd_oracle.cpp directly.
 #include "avbd/solver.h"
 
-extern "C" int ref_collide_boxes(const float posA[3], const float rotA[4], const float sizeA[3],
-                                 const float posB[3], const float rotB[4], const float sizeB[3],
-                                 float margin, float basis[9], int32_t keys[8], float rA[24],
-                                 float rB[24])
+// Any pair of shapes (0 box, 1 sphere, 2 cylinder), each built from a body's size as the
+// solver's makeShape does.
+extern "C" int ref_collide_shapes(int typeA, const float posA[3], const float rotA[4], const float sizeA[3],
+                                  int typeB, const float posB[3], const float rotB[4], const float sizeB[3],
+                                  float margin, float basis[9], int32_t keys[8], float rA[24], float rB[24])
 {
     using namespace avbd;
-    auto shape = [](const float *p, const float *q, const float *s) {
+    auto shape = [](int type, const float *p, const float *q, const float *s) {
         Shape sh;
-        sh.type = ShapeType::Box;
+        sh.type = static_cast<ShapeType>(type);
         sh.center = float3{p[0], p[1], p[2]};
         sh.rotation = quat{q[0], q[1], q[2], q[3]};
         sh.half = float3{s[0], s[1], s[2]} * 0.5f;
@@ -23,7 +24,7 @@ extern "C" int ref_collide_boxes(const float posA[3], const float rotA[4], const
     };
     std::array<Manifold::Contact, 8> cs{};
     float3x3 b{};
-    const int n = collideShapes(shape(posA, rotA, sizeA), shape(posB, rotB, sizeB),
+    const int n = collideShapes(shape(typeA, posA, rotA, sizeA), shape(typeB, posB, rotB, sizeB),
                                 std::span<Manifold::Contact>(cs), b, margin);
     for (int r = 0; r < 3; r++)
         for (int c = 0; c < 3; c++) basis[3 * r + c] = b[c][r];
@@ -35,6 +36,14 @@ extern "C" int ref_collide_boxes(const float posA[3], const float rotA[4], const
         }
     }
     return n;
+}
+
+extern "C" int ref_collide_boxes(const float posA[3], const float rotA[4], const float sizeA[3],
+                                 const float posB[3], const float rotB[4], const float sizeB[3],
+                                 float margin, float basis[9], int32_t keys[8], float rA[24],
+                                 float rB[24])
+{
+    return ref_collide_shapes(0, posA, rotA, sizeA, 0, posB, rotB, sizeB, margin, basis, keys, rA, rB);
 }
 
 extern "C" int ref_solve6(const float *a36, const float *b6, float *x6)

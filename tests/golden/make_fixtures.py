@@ -8,6 +8,12 @@ only in the build container, so its outputs travel as this fixture.
 Cases: resting face contacts (stacked cubes, cube on a slab), tilted and edge-on pairs,
 random orientations and sizes near touching, separated pairs inside and outside the
 margin, all with the solver's speculative margin.
+
+tests/golden/collide_shapes.npz holds the same outputs for pairs with a sphere or a
+cylinder (collide.cpp:505-1032): the reference's own scenario poses (resting on a slab, on
+the cap and on the side, the 5 mm speculative gaps, a sphere centred inside a box) and
+random near-touching pairs of every shape combination, at margins 0, 0.01 and 0.05.
+`python make_fixtures.py shapes` writes only that file.
 """
 import os
 import sys
@@ -50,8 +56,77 @@ def cases(rng, n_random=400):
     return out
 
 
+def shape_cases(rng, n_random=600):
+    I = np.array([0, 0, 0, 1], np.float32)
+    side = quat_axis((1, 0, 0), np.pi / 2)
+    slab = (200, 200, 1)
+    S, C, B = 1, 2, 0
+    out = [   # (typeA, pa, qa, sa, typeB, pb, qb, sb, margin)
+        (B, (0, 0, -0.5), I, slab, S, (0, 0, 0.492), I, (0.5, 0, 0), 0.01),
+        (S, (0, 0, 0.49), I, (0.5, 0, 0), B, (0, 0, -0.5), I, slab, 0.01),
+        (B, (0, 0, -0.5), I, slab, C, (0, 0, 0.4921), I, (0.5, 0.5, 1.0), 0.01),
+        (B, (0, 0, -0.5), I, slab, C, (0, 0, 0.4591), side, (0.5, 0.5, 1.0), 0.01),
+        (S, (0, 0, 1.0), I, (0.5, 0, 0), S, (0, 0, 1.9898), I, (0.5, 0, 0), 0.01),
+        (B, (0, 0, 0), I, (1, 1, 1), S, (0.1, 0, 0.55), I, (0.5, 0, 0), 0.0),
+        (S, (0.1, 0, 0.55), I, (0.5, 0, 0), B, (0, 0, 0), I, (1, 1, 1), 0.0),
+        (B, (0, 0, 0), I, (1, 1, 1), S, (0.1, 0.05, 0.0), I, (0.2, 0, 0), 0.0),   # centre inside
+        (B, (0, 0, 0), I, (2, 2, 1), C, (0, 0, 1.005), I, (0.5, 0.5, 1.0), 0.0),
+        (B, (0, 0, 0), I, (2, 2, 1), C, (0, 0, 1.005), I, (0.5, 0.5, 1.0), 0.01),
+        (C, (0, 0, 0), I, (0.5, 0.5, 1.0), C, (0, 0, 1.005), I, (0.5, 0.5, 1.0), 0.0),
+        (C, (0, 0, 0), I, (0.5, 0.5, 1.0), C, (0, 0, 1.005), I, (0.5, 0.5, 1.0), 0.01),
+        (C, (0, 0, 0), I, (0.5, 0.5, 2.0), S, (0.9, 0, 0.3), I, (0.45, 0, 0), 0.01),    # sphere on the side
+        (C, (0, 0, 0), I, (0.5, 0.5, 2.0), S, (0.0, 0, 1.4), I, (0.45, 0, 0), 0.01),    # on the cap
+        (S, (0.6, 0, 1.3), I, (0.3, 0, 0), C, (0, 0, 0), I, (0.5, 0.5, 2.0), 0.01),     # at the rim
+        (C, (0, 0, 0), side, (0.3, 0.3, 0.16), C, (0.0, 0.1, 0.55), I, (0.3, 0.3, 0.16), 0.05),
+    ]
+    for _ in range(n_random):
+        ta, tb = 0, 0
+        while ta == 0 and tb == 0:
+            ta, tb = (int(x) for x in rng.integers(0, 3, 2))
+        sa = rng.uniform(0.2, 1.6, 3).astype(np.float32)
+        sb = rng.uniform(0.2, 1.6, 3).astype(np.float32)
+        qa = rng.normal(size=4); qa /= np.linalg.norm(qa)
+        qb = rng.normal(size=4); qb /= np.linalg.norm(qb)
+        d = rng.normal(size=3); d /= np.linalg.norm(d)
+        dist = rng.uniform(0.3, 1.1) * (np.linalg.norm(sa) + np.linalg.norm(sb)) / 2
+        pa = rng.uniform(-5, 5, 3)
+        out.append((ta, pa, qa.astype(np.float32), sa, tb, pa + d * dist, qb.astype(np.float32), sb,
+                    float(rng.choice([0.0, 0.01, 0.05]))))
+    return out
+
+
+def write_shapes():
+    rng = np.random.default_rng(2468)
+    cs = shape_cases(rng)
+    n = len(cs)
+    rec = {k: [] for k in ("ta", "pa", "qa", "sa", "tb", "pb", "qb", "sb", "margin")}
+    count = np.zeros(n, np.int32)
+    basis = np.zeros((n, 9), np.float32)
+    keys = np.zeros((n, 8), np.int32)
+    ra = np.zeros((n, 8, 3), np.float32)
+    rb = np.zeros((n, 8, 3), np.float32)
+    for i, case in enumerate(cs):
+        for k, v in zip(rec, case):
+            rec[k].append(np.asarray(v, np.int32 if k in ("ta", "tb") else np.float32))
+        ta, pa, qa, sa, tb, pb, qb, sb, m = case
+        c, b, kk, a, bb = oracle.ref_collide_shapes(ta, pa, qa, sa, tb, pb, qb, sb, m)
+        count[i] = c
+        basis[i] = b
+        keys[i, :c] = kk
+        ra[i, :c] = a
+        rb[i, :c] = bb
+    np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "collide_shapes.npz"),
+                        count=count, basis=basis, keys=keys, ra=ra, rb=rb, **{k: np.stack(v) for k, v in rec.items()})
+    print(f"shapes: {n} cases, {int((count > 0).sum())} with contacts, {int(count.sum())} contacts")
+
+
 def main():
     oracle.build(ref=True)
+    if oracle.ref_lib() is None:
+        raise SystemExit("oracle/_ref/libavbd_ref.so is not built (needs the reference tree)")
+    if sys.argv[1:] == ["shapes"]:
+        write_shapes()
+        return
     if oracle.ref_lib() is None:
         raise SystemExit("oracle/_ref/libavbd_ref.so is not built (needs the reference tree)")
     rng = np.random.default_rng(12345)
@@ -86,6 +161,7 @@ def main():
         sets[f"codes{k}"] = oracle.ref_morton_codes(c)
     np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "morton.npz"),
                         sets=np.int32(4), **sets)
+    write_shapes()
 
 
 if __name__ == "__main__":
