@@ -1080,16 +1080,17 @@ struct Contacts {
 AD size_t crow(int i) { return static_cast<size_t>(i) * CSLICE; }
 
 // collide.cpp:208-233
-AD void addContact(qt qA, v3 cA, qt qB, v3 cB, Contacts &cs, v3 xA, v3 xB, int key)
+AD bool addContact(qt qA, v3 cA, qt qB, v3 cB, Contacts &cs, v3 xA, v3 xB, int key)
 {
     const v3 mid = (xA + xB) * 0.5f;
     for (int i = 0; i < cs.n; i++)
-        if (length2(mid - ld3(cs.mid[crow(i)])) < 1.0e-6f) return;
-    if (cs.n >= MAXC) return;
+        if (length2(mid - ld3(cs.mid[crow(i)])) < 1.0e-6f) return false;
+    if (cs.n >= MAXC) return false;
     cs.ra[crow(cs.n)] = st3(qrot(qconj(qA), xA - cA));
     cs.rb[crow(cs.n)] = st3(qrot(qconj(qB), xB - cB), __int_as_float(key));
     cs.mid[crow(cs.n)] = st3(mid);
     cs.n++;
+    return true;
 }
 
 AD void faceManifold(const OBB &bA, const OBB &bB, bool refIsA, int refAxis, v3 nAB, float margin, Contacts &cs,
@@ -1273,6 +1274,289 @@ AD int collideBoxes(v3 pA, qt qA, v3 hA, v3 pB, qt qB, v3 hB, float margin, m3 &
     }
     faceManifold(bA, bB, fType == 0, fIdx, nAB, margin, cs, c0, c1);
     return cs.n;
+}
+
+// ------------------------------------------------------------ round shapes ----
+// collide.cpp:505-1032: spheres and capped cylinders.  A shape holds what makeShape derives from
+// a body (collide.cpp:505-516): half extents, radius = size.x, half height = size.z / 2 and the
+// cylinder axis (local +Z) in world space.
+struct RShp {
+    v3 c; qt q; v3 half; float r, hh; v3 axis;
+};
+AD RShp makeRShp(v3 p, qt q, float4 size)
+{
+    return RShp{p, q, v3{size.x, size.y, size.z} * 0.5f, size.x, size.z * 0.5f, qrot(q, v3{0, 0, 1})};
+}
+AD v3 sel3v(bool c, v3 a, v3 b) { return v3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
+AD v3 clampBox(v3 l, v3 h)
+{
+    return v3{sclamp(l.x, -h.x, h.x), sclamp(l.y, -h.y, h.y), sclamp(l.z, -h.z, h.z)};
+}
+
+// collide.cpp:520-546: inside a box, the way out along the axis of least penetration
+AD bool pointInBox(const RShp &b, v3 p, v3 &push, float &depth)
+{
+    const v3 l = qrot(qconj(b.q), p - b.c);
+    float best = FLT_MAXV, sign = 1.0f;
+    int axis = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float d = b.half[i] - fabsf(l[i]);
+        if (d <= 0.0f) return false;
+        if (d < best) { best = d; axis = i; sign = l[i] >= 0.0f ? 1.0f : -1.0f; }
+    }
+    push = qrot(b.q, v3{axis == 0 ? sign : 0.0f, axis == 1 ? sign : 0.0f, axis == 2 ? sign : 0.0f});
+    depth = best;
+    return true;
+}
+// collide.cpp:549-577: outside but within the margin: negative depth, outward push
+AD bool pointNearBox(const RShp &b, v3 p, float margin, v3 &push, float &depth)
+{
+    const v3 l = qrot(qconj(b.q), p - b.c);
+    if (!(fabsf(l.x) >= b.half.x || fabsf(l.y) >= b.half.y || fabsf(l.z) >= b.half.z)) return false;
+    const v3 d = l - clampBox(l, b.half);
+    const float gap = length(d);
+    if (gap > margin || gap <= 1.0e-9f) return false;
+    push = qrot(b.q, d / gap);
+    depth = -gap;
+    return true;
+}
+// collide.cpp:580-614
+AD v3 closestOnCylinder(const RShp &cy, v3 p, v3 &outward, float &distance)
+{
+    const v3 l = qrot(qconj(cy.q), p - cy.c);
+    const float t = sclamp(l.z, -cy.hh, cy.hh);
+    const float lenSq = (0.0f + l.x * l.x) + l.y * l.y;
+    float cx, cy2;
+    if (lenSq > cy.r * cy.r) {
+        const float scale = cy.r / sqrtf(lenSq);
+        cx = l.x * scale; cy2 = l.y * scale;
+    } else if (lenSq > 1.0e-12f) {
+        cx = l.x; cy2 = l.y;
+    } else {
+        cx = cy.r; cy2 = 0.0f;
+    }
+    const v3 w = qrot(cy.q, v3{cx, cy2, t}) + cy.c;
+    const v3 d = p - w;
+    distance = length(d);
+    outward = distance > 1.0e-9f ? d / distance : qrot(cy.q, v3{1, 0, 0});
+    return w;
+}
+// collide.cpp:617-651
+AD bool pointInCylinder(const RShp &cy, v3 p, v3 &push, float &surface)
+{
+    const v3 l = qrot(qconj(cy.q), p - cy.c);
+    const float rs = l.x * l.x + l.y * l.y;
+    if (rs >= cy.r * cy.r || fabsf(l.z) >= cy.hh) return false;
+    const float radialDepth = cy.r - sqrtf(rs);
+    const float capDepth = cy.hh - fabsf(l.z);
+    v3 pl{0, 0, 0};
+    if (capDepth < radialDepth) {
+        pl.z = l.z >= 0.0f ? 1.0f : -1.0f;
+        surface = capDepth;
+    } else {
+        const float len = sqrtf(rs);
+        if (len > 1.0e-6f) { pl.x = l.x / len; pl.y = l.y / len; }
+        else pl.x = 1.0f;
+        surface = radialDepth;
+    }
+    push = qrot(cy.q, pl);
+    return true;
+}
+// collide.cpp:654-671
+AD bool pointNearCylinder(const RShp &cy, v3 p, float margin, v3 &push, float &depth)
+{
+    const v3 l = qrot(qconj(cy.q), p - cy.c);
+    const float rs = l.x * l.x + l.y * l.y;
+    if (rs < cy.r * cy.r && fabsf(l.z) < cy.hh) return false;
+    v3 out;
+    float dist;
+    closestOnCylinder(cy, p, out, dist);
+    if (dist > margin) return false;
+    push = out;
+    depth = -dist;
+    return true;
+}
+// collide.cpp:674-684: the exact inside test first, the speculative one only when it fails
+AD bool insideBox(const RShp &b, v3 p, float margin, v3 &push, float &depth)
+{
+    return pointInBox(b, p, push, depth) || (margin > 0.0f && pointNearBox(b, p, margin, push, depth));
+}
+AD bool insideCylinder(const RShp &cy, v3 p, float margin, v3 &push, float &depth)
+{
+    return pointInCylinder(cy, p, push, depth) || (margin > 0.0f && pointNearCylinder(cy, p, margin, push, depth));
+}
+
+// Surface sample k (0..33) of a cylinder (collide.cpp:755-784): caps (centre, then 8 rim
+// points) at -h/2 and +h/2, then two inner rings.  x, y: the body's local X and Y in world.
+AD v3 cylSample(const RShp &cy, v3 x, v3 y, const float *rim, int k)
+{
+    v3 base;
+    int i;
+    if (k < 18) {
+        const int end = k < 9 ? -1 : 1;
+        const int j = k < 9 ? k : k - 9;
+        base = cy.c + cy.axis * (cy.hh * static_cast<float>(end));
+        if (j == 0) return base;
+        i = j - 1;
+    } else {
+        const int ring = k < 26 ? 1 : 2;
+        i = (k - 18) & 7;
+        const float t = -cy.hh + 2.0f * cy.hh * static_cast<float>(ring) / static_cast<float>(3);
+        base = cy.c + cy.axis * t;
+    }
+    return base + (x * rim[i] + y * rim[8 + i]) * cy.r;
+}
+
+constexpr int CYL_SAMPLES = 34;
+// Candidate k of a sampled pair, in the order the reference generates them (collide.cpp:894-998).
+// cylBox: samples of the cylinder inside the box (k < 34), then box corners inside the cylinder;
+// otherwise (two cylinders) samples of A inside B, then samples of B inside A.  Returns false
+// when the point is no candidate.
+AD bool roundCandidate(bool cylBox, bool cylIsA, const RShp &a, const RShp &b, const float *rim, float margin, int k,
+                       v3 &xA, v3 &xB, v3 &n, float &depth, int &key)
+{
+    const RShp &cy = cylBox ? (cylIsA ? a : b) : (k < CYL_SAMPLES ? a : b);
+    const RShp &into = cylBox ? (cylIsA ? b : a) : (k < CYL_SAMPLES ? b : a);
+    const int i = k < CYL_SAMPLES ? k : k - CYL_SAMPLES;
+    v3 push;
+    float d;
+    if (cylBox && k >= CYL_SAMPLES) {
+        // box corner i inside the cylinder
+        const v3 cl{(i & 1) ? into.half.x : -into.half.x, (i & 2) ? into.half.y : -into.half.y,
+                    (i & 4) ? into.half.z : -into.half.z};
+        const v3 cw = qrot(into.q, cl) + into.c;
+        if (!insideCylinder(cy, cw, margin, push, d)) return false;
+        xA = cylIsA ? cw + push * d : cw;
+        xB = cylIsA ? cw : cw + push * d;
+        n = cylIsA ? push : -push;
+        depth = d;
+        key = static_cast<int>((4u << 24) | ((cylIsA ? 0u : 1u) << 20) | static_cast<uint32_t>(i));
+        return true;
+    }
+    const v3 x = qrot(cy.q, v3{1, 0, 0}), y = qrot(cy.q, v3{0, 1, 0});
+    const v3 smp = cylSample(cy, x, y, rim, i);
+    const bool fromIsA = cylBox ? cylIsA : k < CYL_SAMPLES;
+    if (!(cylBox ? insideBox(into, smp, margin, push, d) : insideCylinder(into, smp, margin, push, d))) return false;
+    xA = fromIsA ? smp : smp + push * d;
+    xB = fromIsA ? smp + push * d : smp;
+    n = fromIsA ? -push : push;
+    depth = d;
+    key = static_cast<int>((3u << 24) | ((fromIsA ? 1u : 0u) << 20) | static_cast<uint32_t>(i));
+    return true;
+}
+
+// Sampled pairs (cylinder-box, cylinder-cylinder): every candidate is offered in generation
+// order; the deepest MAXC are kept in a register list, stable (an equal depth stays behind the
+// earlier candidate), which is the head of the reference's stable insertion ranking
+// (collide.cpp:702-730).  The kept ones are then rebuilt and offered to the manifold in rank
+// order, stopping at the first it refuses.
+AD int collideSampled(bool cylBox, bool cylIsA, const RShp &a, const RShp &b, const float *rim, float margin,
+                      m3 &basisOut, Contacts &cs)
+{
+    const int total = cylBox ? CYL_SAMPLES + 8 : 2 * CYL_SAMPLES;
+    float td[MAXC] = {};
+    int tk[MAXC] = {};
+    int kept = 0, nc = 0;
+    v3 nAB{0, 0, 0};
+    for (int k = 0; k < total; k++) {
+        v3 xA, xB, n;
+        float d;
+        int key;
+        if (!roundCandidate(cylBox, cylIsA, a, b, rim, margin, k, xA, xB, n, d, key)) continue;
+        nc++;
+        nAB += n;
+        // insert before the first strictly shallower entry; everything after it moves down one
+        float cd = d;
+        int ck = k;
+        bool carry = true, shifting = false;
+#pragma unroll
+        for (int j = 0; j < MAXC; j++) {
+            const bool empty = j >= kept;
+            if (carry && (shifting || empty || td[j] < cd)) {
+                const float od = td[j];
+                const int ok = tk[j];
+                td[j] = cd; tk[j] = ck;
+                cd = od; ck = ok;
+                shifting = true;
+                if (empty) carry = false;
+            }
+        }
+        kept = kept < MAXC ? kept + 1 : MAXC;
+    }
+    if (!nc) return 0;
+    const RShp &cy = cylBox ? (cylIsA ? a : b) : a;
+    basisOut = orthonormal(-(length2(nAB) > 1.0e-12f ? normalize(nAB) : cy.axis));
+#pragma unroll
+    for (int j = 0; j < MAXC; j++) {
+        if (j >= kept) break;
+        v3 xA, xB, n;
+        float d;
+        int key;
+        roundCandidate(cylBox, cylIsA, a, b, rim, margin, tk[j], xA, xB, n, d, key);
+        if (!addContact(a.q, a.c, b.q, b.c, cs, xA, xB, key)) break;
+    }
+    return cs.n;
+}
+
+// collide.cpp:1001-1032 for pairs with a sphere or a cylinder: contact count, basis written
+// exactly when the reference writes it (when there is a contact).
+AD int collideRound(int typeA, int typeB, const RShp &a, const RShp &b, const float *rim, float margin,
+                    m3 &basisOut, bool &wroteBasis, Contacts &cs)
+{
+    cs.n = 0;
+    wroteBasis = false;
+    const int S = AVBD_SHAPE_SPHERE, C = AVBD_SHAPE_CYLINDER;
+    if (typeA == S || typeB == S) {
+        const bool sphereIsA = typeA == S;
+        const RShp &sp = sphereIsA ? a : b;
+        const RShp &other = sphereIsA ? b : a;
+        const int otherType = sphereIsA ? typeB : typeA;
+        v3 n, onOther;
+        if (otherType == S) {
+            // collide.cpp:786-806 (both spheres; n runs from A to B)
+            const v3 d = b.c - a.c;
+            const float distSq = length2(d);
+            const float sumPlus = a.r + b.r + margin;
+            if (distSq >= sumPlus * sumPlus) return 0;
+            const float dist = sqrtf(distSq);
+            const v3 nAB = dist > 1.0e-6f ? d / dist : v3{0, 0, 1};
+            basisOut = orthonormal(-nAB);
+            wroteBasis = true;
+            addContact(a.q, a.c, b.q, b.c, cs, a.c + nAB * a.r, b.c - nAB * b.r, 1);
+            return cs.n;
+        }
+        if (otherType == C) {
+            // collide.cpp:865-891; n leaves the cylinder towards the sphere centre
+            float dist;
+            onOther = closestOnCylinder(other, sp.c, n, dist);
+            if (dist >= sp.r + margin) return 0;
+        } else {
+            // collide.cpp:809-862; n leaves the box towards the sphere centre
+            const v3 l = qrot(qconj(other.q), sp.c - other.c);
+            onOther = qrot(other.q, clampBox(l, other.half)) + other.c;
+            const v3 d = sp.c - onOther;
+            const float distSq = length2(d);
+            const float radPlus = sp.r + margin;
+            if (distSq >= radPlus * radPlus) return 0;
+            const float dist = sqrtf(distSq);
+            if (dist > 1.0e-6f) {
+                n = d / dist;
+            } else {
+                float depth;
+                if (!pointInBox(other, sp.c, n, depth)) return 0;
+            }
+        }
+        basisOut = orthonormal(-(sphereIsA ? -n : n));
+        wroteBasis = true;
+        const v3 onSphere = sp.c - n * sp.r;
+        addContact(a.q, a.c, b.q, b.c, cs, sphereIsA ? onSphere : onOther, sphereIsA ? onOther : onSphere, 1);
+        return cs.n;
+    }
+    const bool cylBox = !(typeA == C && typeB == C);
+    const int n = collideSampled(cylBox, typeA == C, a, b, rim, margin, basisOut, cs);
+    wroteBasis = n > 0;
+    return n;
 }
 
 // ------------------------------------------------------ projection bits ----

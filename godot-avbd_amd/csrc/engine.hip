@@ -122,6 +122,7 @@ struct WorldInfo {
     bool active = false;
     int bodyBase = 0, bodyCount = 0, jointBase = 0, jointCount = 0, slotBase = 0, slotCap = 0;
     bool hasJoints = false;
+    bool hasRound = false;   // a sphere or cylinder among its bodies
 };
 
 } // namespace
@@ -130,6 +131,7 @@ struct Engine {
     avbd_params prm{};
     int Wcap = 0, Ncap = 0, Jcap = 0, Scap = 0, Pcap = 0;
     State S{};
+    float *rimDev = nullptr;   // S.rim
     hipStream_t stream = nullptr;
     std::vector<WorldInfo> worlds;
     RangeAlloc bodyAlloc, jointAlloc, slotAlloc;
@@ -147,6 +149,7 @@ struct Engine {
     int colourRounds = 48;
     hipGraphExec_t graphs[2] = {nullptr, nullptr};  // [rebuild, refit]
     bool graphJoints = false;
+    bool graphRound = false;
     bool graphsValid = false;
     hipEvent_t ev[5] = {};
     avbd_step_stats stats{};
@@ -261,7 +264,7 @@ struct Engine {
         S.numWorlds = Wcap; S.numBodies = Ncap; S.slotCap = Scap; S.jointCap = Jcap; S.pairCap = Pcap;
         AL(S.pos, N); AL(S.rot, N); AL(S.initLin, N); AL(S.initAng, N); AL(S.inertLin, N); AL(S.inertAng, N);
         AL(S.vel, N); AL(S.angVel, N); AL(S.prevVel, N); AL(S.sizeR, N); AL(S.massI, N); AL(S.poseRec, 4 * N); AL(S.narrowRec, NARROW_REC * N);
-        AL(S.fric, N); AL(S.grav, N); AL(S.layer, N); AL(S.mask, N); AL(S.lockLin, N); AL(S.lockAng, N);
+        AL(S.fric, N); AL(S.grav, N); AL(S.layer, N); AL(S.mask, N); AL(S.lockLin, N); AL(S.lockAng, N); AL(S.shape, N); AL(rimDev, 16); S.rim = rimDev;
         AL(S.bodyWorld, N); AL(S.updateIndex, N); AL(S.colour, N); AL(S.contactOnly, N);
         AL(S.dLin, N); AL(S.dAng, N); AL(S.aabbC, N); AL(S.aabbH, N);
         AL(S.fbCount, 1); AL(S.fbBody, N); AL(S.fbBlock, 33 * N);
@@ -390,6 +393,13 @@ struct Engine {
         k_adj_sort<<<nblk(n), TPB, 0, stream>>>(S);
     }
 
+    bool anyRound() const
+    {
+        for (const auto &w : worlds)
+            if (w.active && w.hasRound) return true;
+        return false;
+    }
+
     bool anyJoints() const
     {
         if (!prm.contact_only_overflow) return true;
@@ -482,6 +492,9 @@ struct Engine {
         // this step's new manifolds
         k_fill_u32<<<nblk(Scap), TPB, 0, stream>>>(S.keep, 0u, Scap);
         k_manifold_init<<<liveGrid(NARROW_TPB), NARROW_TPB, 0, stream>>>(S);
+        // pairs with a sphere or a cylinder take their own pass (only launched when such bodies
+        // exist, so box scenes keep the box kernel's register budget and time)
+        if (anyRound()) k_manifold_init_round<<<liveGrid(NARROW_TPB), NARROW_TPB, 0, stream>>>(S);
         pend();
         k_release_flags<<<nblk(Scap), TPB, 0, stream>>>(S);
         scan(S.keep, S.keepScan, Scap);
@@ -767,6 +780,7 @@ struct Engine {
             (void)hipGraphDestroy(graph);
         }
         graphJoints = joints;
+        graphRound = anyRound();
         graphsValid = true;
         return AVBD_OK;
     }
@@ -787,7 +801,7 @@ struct Engine {
             HIPCHK(hipGetLastError());
             return checkErrors();
         }
-        if (!graphsValid || graphJoints != anyJoints()) {
+        if (!graphsValid || graphJoints != anyJoints() || graphRound != anyRound()) {
             avbd_status st = captureGraphs();
             if (st != AVBD_OK) return st;
         }
@@ -807,13 +821,19 @@ struct Engine {
         wi.jointCount = nj;
         wi.slotCap = slotCap;
         wi.hasJoints = nj > 0;
+        wi.hasRound = false;
         std::vector<float4> pos(n), rot(n), vel(n), ang(n), prev(n), sizeR(n), massI(n);
         std::vector<float> fric(n), grav(n);
         std::vector<uint32_t> layer(n), mask(n);
-        std::vector<uint8_t> ll(n), la(n);
+        std::vector<uint8_t> ll(n), la(n), shp(n, AVBD_SHAPE_BOX);
         std::vector<int> bw(n, w), ui(n, -1);
         int movable = 0;
         for (int i = 0; i < n; i++) {
+            if (b->shape) {
+                if (b->shape[i] > AVBD_SHAPE_CYLINDER) return fail(AVBD_ERR_INVALID, "unknown body shape");
+                shp[i] = b->shape[i];
+                if (shp[i] != AVBD_SHAPE_BOX) wi.hasRound = true;
+            }
             pos[i] = make_float4(b->position[3 * i], b->position[3 * i + 1], b->position[3 * i + 2], 0);
             rot[i] = make_float4(b->rotation[4 * i], b->rotation[4 * i + 1], b->rotation[4 * i + 2], b->rotation[4 * i + 3]);
             vel[i] = make_float4(b->velocity[3 * i], b->velocity[3 * i + 1], b->velocity[3 * i + 2], 0);
@@ -839,6 +859,19 @@ struct Engine {
         HIPCHK(put(S.inertLin, pos)); HIPCHK(put(S.inertAng, rot));
         HIPCHK(put(S.sizeR, sizeR)); HIPCHK(put(S.massI, massI)); HIPCHK(put(S.fric, fric)); HIPCHK(put(S.grav, grav));
         HIPCHK(put(S.layer, layer)); HIPCHK(put(S.mask, mask)); HIPCHK(put(S.lockLin, ll)); HIPCHK(put(S.lockAng, la));
+        HIPCHK(put(S.shape, shp));
+        {
+            // collide.cpp:733-753: rim directions at angle steps i*5 mod 8, from the C library's
+            // cosf / sinf at run time (the values the reference's table holds)
+            float rim[16];
+            for (int i = 0; i < 8; i++) {
+                volatile float a = 6.28318530718f * static_cast<float>(i * 5 % 8) / static_cast<float>(8);
+                rim[i] = std::cos(static_cast<float>(a));
+                rim[8 + i] = std::sin(static_cast<float>(a));
+            }
+            HIPCHK(hipMemcpyAsync(rimDev, rim, sizeof(rim), hipMemcpyHostToDevice, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+        }
         HIPCHK(put(S.bodyWorld, bw)); HIPCHK(put(S.updateIndex, ui));
         if (nj > 0) {
             std::vector<int> ja(nj), jb(nj), jw(nj, w);

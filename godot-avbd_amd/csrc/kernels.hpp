@@ -217,7 +217,11 @@ __global__ void k_swept_aabb(State S)
         return;
     }
     const float4 sr = S.sizeR[i];
-    const v3 h = v3{sr.x, sr.y, sr.z} * 0.5f;
+    const int shape = S.shape[i];
+    // rigid.cpp:83-99: the shape's local half extents
+    const v3 h = shape == AVBD_SHAPE_SPHERE     ? v3{sr.x, sr.x, sr.x}
+                 : shape == AVBD_SHAPE_CYLINDER ? v3{sr.x, sr.x, sr.z * 0.5f}
+                                                : v3{sr.x, sr.y, sr.z} * 0.5f;
     const m3 R = qmat(bRot(S, i));
     const v3 tight{(fabsf(R.c[0][0]) * h.x + fabsf(R.c[1][0]) * h.y) + fabsf(R.c[2][0]) * h.z,
                    (fabsf(R.c[0][1]) * h.x + fabsf(R.c[1][1]) * h.y) + fabsf(R.c[2][1]) * h.z,
@@ -238,6 +242,7 @@ __global__ void k_swept_aabb(State S)
     r[2] = sr;
     r[3] = make_float4(v4.x, v4.y, v4.z, S.grav[i]);
     r[4] = make_float4(w4.x, w4.y, w4.z, S.fric[i]);
+    r[5] = make_float4(__int_as_float(shape), 0.0f, 0.0f, 0.0f);
 }
 
 // Scene bounds of each world's AABB centres (bvh_broadphase.cpp:332-342); one block per
@@ -900,17 +905,26 @@ __global__ void k_contact_unpack(State S)
 // staging rows are read and written as runs); items past P are this step's new manifolds in
 // pair order.  The clip polygons live in LDS, two per lane.  `keep` of positions that are not
 // live is cleared beforehand.
+// Box pairs run in k_manifold_init; pairs with a sphere or a cylinder in k_manifold_init_round
+// (launched only when such bodies exist).  Each work item is handled by exactly one of the two:
+// the box pass also parks frozen pairs and reports capacity errors.
 typedef float NarrowPolys[2][3][POLY_LDS * NARROW_TPB];
-AD void manifoldInit(const State &S, int q, int P, NarrowPolys &poly);
+template <bool ROUND> AD void manifoldInit(const State &S, int q, int P, NarrowPolys *poly);
 // At most 128 registers (four wavefronts per SIMD; it fits without spilling vector registers).
 __global__ void __launch_bounds__(NARROW_TPB) __attribute__((amdgpu_waves_per_eu(4))) k_manifold_init(State S)
 {
     __shared__ NarrowPolys poly;   // 12 KiB
     const int P = static_cast<int>(S.wLiveOff[S.numWorlds]);
     const int fresh = static_cast<int>(S.pairValidScan[pairTotal(S)]);
-    FOR_RANGE(q, P + fresh) manifoldInit(S, q, P, poly);
+    FOR_RANGE(q, P + fresh) manifoldInit<false>(S, q, P, &poly);
 }
-AD void manifoldInit(const State &S, int q, int P, NarrowPolys &poly)
+__global__ void __launch_bounds__(NARROW_TPB) k_manifold_init_round(State S)
+{
+    const int P = static_cast<int>(S.wLiveOff[S.numWorlds]);
+    const int fresh = static_cast<int>(S.pairValidScan[pairTotal(S)]);
+    FOR_RANGE(q, P + fresh) manifoldInit<true>(S, q, P, nullptr);
+}
+template <bool ROUND> AD void manifoldInit(const State &S, int q, int P, NarrowPolys *poly)
 {
     int t, slot, a, b, oldN;
     uint32_t ob;
@@ -926,18 +940,30 @@ AD void manifoldInit(const State &S, int q, int P, NarrowPolys &poly)
         a = S.mA[slot]; b = S.mB[slot]; ob = 0u; oldN = 0;
         // past the staging capacity only when allocations failed this step (items of failed
         // pairs hold places in the list): report it for this manifold's world too
-        if (q >= S.slotCap) { atomicOr(S.wErr + S.bodyWorld[a], 1); return; }
+        if (q >= S.slotCap) {
+            if (!ROUND) atomicOr(S.wErr + S.bodyWorld[a], 1);
+            return;
+        }
     }
-    S.sPos[slot] = q;
     const float4 *ra = S.narrowRec + NARROW_REC * static_cast<size_t>(a);
     const float4 *rb = S.narrowRec + NARROW_REC * static_cast<size_t>(b);
     const float4 a0 = ra[0], a1 = ra[1], a2 = ra[2], a3 = ra[3], a4 = ra[4];
     const float4 b0 = rb[0], b1 = rb[1], b2 = rb[2], b3 = rb[3], b4 = rb[4];
     const float ma = a0.w, mb = b0.w;
-    if (!(ma > 0.0f) && !(mb > 0.0f)) {   // frozen: parked verbatim
-        denseToStage(S, stageRow(q, 0), CSLICE, ob, CSLICE, oldN);
-        S.keep[t] = 1;
-        return;
+    const bool frozen = !(ma > 0.0f) && !(mb > 0.0f);
+    const int shA = __float_as_int(ra[5].x), shB = __float_as_int(rb[5].x);
+    const bool round = (shA | shB) != AVBD_SHAPE_BOX;
+    if (ROUND && (frozen || !round)) return;
+    if (!ROUND) {
+        S.sPos[slot] = q;
+        if (frozen) {   // parked verbatim
+            denseToStage(S, stageRow(q, 0), CSLICE, ob, CSLICE, oldN);
+            S.keep[t] = 1;
+            return;
+        }
+        if (round) return;
+    } else {
+        S.sPos[slot] = q;
     }
     const float fa = a4.w, fb = b4.w;
     const float mu = sqrtf(fa * fb);
@@ -958,13 +984,18 @@ AD void manifoldInit(const State &S, int q, int P, NarrowPolys &poly)
     bool wrote;
     const size_t d = stageRow(q, 0);
     Contacts cs{0, S.sRA + d, S.sRB + d, S.sC0 + d};
-    const int l = threadIdx.x;
-    // polygon vertices past the LDS part use this manifold's penalty and lambda staging rows
-    // (32 floats each), which are written only after the contacts are built
-    const Poly c0{&poly[0][0][l], &poly[0][1][l], &poly[0][2][l], S.sPen + d};
-    const Poly c1{&poly[1][0][l], &poly[1][1][l], &poly[1][2][l], S.sLam + d};
-    const int nn = collideBoxes(pa, qa, v3{sa.x, sa.y, sa.z} * 0.5f, pb, qb, v3{sb.x, sb.y, sb.z} * 0.5f, margin,
-                                B, wrote, cs, c0, c1);
+    int nn;
+    if constexpr (ROUND) {
+        nn = collideRound(shA, shB, makeRShp(pa, qa, sa), makeRShp(pb, qb, sb), S.rim, margin, B, wrote, cs);
+    } else {
+        const int l = threadIdx.x;
+        // polygon vertices past the LDS part use this manifold's penalty and lambda staging rows
+        // (32 floats each), which are written only after the contacts are built
+        const Poly c0{&(*poly)[0][0][l], &(*poly)[0][1][l], &(*poly)[0][2][l], S.sPen + d};
+        const Poly c1{&(*poly)[1][0][l], &(*poly)[1][1][l], &(*poly)[1][2][l], S.sLam + d};
+        nn = collideBoxes(pa, qa, v3{sa.x, sa.y, sa.z} * 0.5f, pb, qb, v3{sb.x, sb.y, sb.z} * 0.5f, margin, B, wrote,
+                          cs, c0, c1);
+    }
     if (wrote) {
         S.mBasis[3 * slot] = st3(row(B, 0));
         S.mBasis[3 * slot + 1] = st3(row(B, 1));

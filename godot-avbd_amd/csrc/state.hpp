@@ -53,6 +53,8 @@ struct State {
     float *fric, *grav;
     uint32_t *layer, *mask;
     uint8_t *lockLin, *lockAng;
+    uint8_t *shape;       // AVBD_SHAPE_* per body
+    const float *rim;     // cylinder rim directions: 8 cosines then 8 sines (host libm values)
     int *bodyWorld;
     int *updateIndex;  // rank among movable bodies of the world (list order), -1 static
     uint8_t *colour;   // 0..7, 8 = Jacobi overflow, 0xFF = not yet coloured

@@ -368,6 +368,36 @@ def pile(n=8192, seed=2, grid=None, pitch=4.0, ground=400.0):
     return bodies, None
 
 
+def round_pile(n=300, seed=4, pitch=2.2):
+    """Boxes, spheres and capped cylinders (a third each, random sizes, densities and
+    orientations) dropped on a jittered grid onto a static ground box, plus a sphere resting on
+    a sphere, a cylinder on its side given spin and a static cylinder and sphere as obstacles."""
+    rng = np.random.default_rng(seed)
+    s = SceneBuilder()
+    s.box((200, 200, 1), 0.0, 0.5, (0, 0, -0.5))
+    s.cylinder(1.0, 3.0, 0.0, 0.6, (14.0, 0.0, 1.5))
+    s.sphere(1.5, 0.0, 0.6, (-14.0, 0.0, 1.0))
+    s.sphere(0.5, 1000.0, 0.5, (20.0, 20.0, 1.0))
+    s.sphere(0.5, 1000.0, 0.5, (20.0, 20.0, 2.5))
+    roll = s.cylinder(0.5, 1.0, 500.0, 0.9, (-20.0, 20.0, 0.52), rot=(0.70710677, 0.0, 0.0, 0.70710677))
+    s.b[roll]["ang"] = (8.0, 0.0, 0.0)
+    nx = max(1, int(round(n ** (1.0 / 3.0))))
+    q = _shoemake(rng, n)
+    for k in range(n):
+        x, y, z = k % nx, (k // nx) % nx, k // (nx * nx)
+        pos = ((x - (nx - 1) / 2.0) * pitch + rng.uniform(-0.2, 0.2), (y - (nx - 1) / 2.0) * pitch + rng.uniform(-0.2, 0.2),
+               2.0 + z * pitch + rng.uniform(-0.2, 0.2))
+        dens = float(np.exp(np.clip(rng.normal(0.0, 0.5), -1.0, 1.0)))
+        kind = k % 3
+        if kind == 0:
+            s.box(rng.uniform(0.4, 1.4, 3), dens, 0.5, pos, rot=q[k])
+        elif kind == 1:
+            s.sphere(rng.uniform(0.25, 0.7), dens, 0.5, pos, rot=q[k])
+        else:
+            s.cylinder(rng.uniform(0.25, 0.6), rng.uniform(0.4, 1.4), dens, 0.5, pos, rot=q[k])
+    return s.build()
+
+
 def chains(n_chains=4096, links=32, seed=3, grid=None, pitch_y=1.5, pitch_x=34.0, anchor_z=30.0):
     """C3: chains of unit boxes along +x linked by ball joints (k_lin = inf, k_ang = 0),
     each hanging from a static anchor, with a ground box below."""

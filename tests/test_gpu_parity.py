@@ -115,6 +115,30 @@ def test_generic_joints_lockstep():
         assert bits_equal(gj[k], oj[k]), k
 
 
+def test_round_shapes_lockstep():
+    """Spheres and capped cylinders among boxes (every shape pair: sphere-sphere, -box,
+    -cylinder, cylinder-box, -cylinder), with rolling, stacking and static round obstacles,
+    bit for bit against the oracle, manifolds included."""
+    b, j = scenes.round_pile(300)
+    w, o = lockstep(b, j, P(iterations=10), 150, every=10)
+    manifolds_equal(w, o)
+    gs, os_ = w.stats(), o.stats()
+    assert gs["manifolds"] > 300
+    for k in ("manifolds", "contacts", "candidate_pairs", "new_manifolds"):
+        assert gs[k] == os_[k], k
+    m = w.manifolds()
+    shape = b["shape"]
+    kinds = {(min(shape[a], shape[c]), max(shape[a], shape[c])) for a, c in zip(m["body_a"], m["body_b"])}
+    assert kinds >= {(0, 1), (0, 2), (1, 1), (1, 2), (2, 2)}, kinds
+
+
+def test_shape_validation():
+    b, j = scenes.round_pile(30)
+    b["shape"][3] = 7
+    with pytest.raises(pkg.abi.AvbdError):
+        gpu_world(b, j, P())
+
+
 def test_chains_colour_sweep_lockstep():
     p = P(iterations=10, beta_lin=1e5, penalty_max=1e9)
     w, o = lockstep(*scenes.chains(16, links=32, grid=(4, 4), pitch_x=4.0), p, 150, every=15)
@@ -237,7 +261,7 @@ def test_narrow_phase_against_reference_fixture():
 def test_batch_alone_equals_in_mix():
     worlds = [scenes.pyramid(rows=12, seed=3), scenes.pile(300, seed=51), scenes.digest_mixed(),
               scenes.chains(4, links=16, grid=(2, 2), pitch_x=3.0), scenes.pile(40, seed=52), scenes.digest_stack(),
-              scenes.generic_joints(), scenes.springs_and_filters()]
+              scenes.generic_joints(), scenes.springs_and_filters(), scenes.round_pile(120, seed=9)]
     p = P(iterations=10)
     batch = pkg.Batch(p, 16, 4000, 2000)
     ids = [batch.add(b, j) for b, j in worlds]
