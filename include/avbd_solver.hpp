@@ -2,7 +2,7 @@
 //
 // A drop-in for the reference's in-process API (src/avbd/solver.h:50-374,
 // src/avbd/types.hpp:75-139, src/avbd/forces.hpp:19-113): bodies and forces are created with
-// `new Rigid(&solver, size, density, friction, position[, velocity])`,
+// `new Rigid(&solver, size[, shape], density, friction, position[, velocity])`,
 // `new Joint(&solver, a, b, rA, rB[, kLin, kAng, fracture])`,
 // `new Spring(&solver, a, b, rA, rB, stiffness[, rest])`, `new IgnoreCollision(&solver, a, b)`
 // and `new GenericJoint(&solver, a, b, rA, rB)` (then its `linear[i]` / `angular[i]` axes set),
@@ -41,7 +41,11 @@ struct quat {
 
 struct Solver;
 
-// reference types.hpp:75-139 (box bodies)
+// reference types.hpp:52-59: `size` is the full extents of a box, (radius, -, -) of a sphere,
+// (radius, -, height) of a cylinder whose axis is the body's local +Z
+enum class ShapeType : int { Box = AVBD_SHAPE_BOX, Sphere = AVBD_SHAPE_SPHERE, Cylinder = AVBD_SHAPE_CYLINDER };
+
+// reference types.hpp:75-139
 struct Rigid {
     Solver *solver;
     Rigid *next = nullptr;
@@ -51,6 +55,7 @@ struct Rigid {
     float3 velocityAng;
     float3 prevVelocityLin;
     float3 size;
+    ShapeType shape = ShapeType::Box;
     float mass = 0;
     float3 moment;
     float friction = 0;
@@ -63,6 +68,8 @@ struct Rigid {
 
     inline Rigid(Solver *p_solver, float3 p_size, float p_density, float p_friction, float3 p_position,
                  float3 p_velocity = float3{0, 0, 0});
+    inline Rigid(Solver *p_solver, float3 p_size, ShapeType p_shape, float p_density, float p_friction,
+                 float3 p_position, float3 p_velocity = float3{0, 0, 0});
     inline ~Rigid();
     Rigid(const Rigid &) = delete;
     Rigid &operator=(const Rigid &) = delete;
@@ -300,7 +307,7 @@ struct Solver {
     struct Arrays {
         std::vector<float> size, mass, moment, friction, radius, gravity, pos, rot, vel, ang, prev;
         std::vector<uint32_t> layer, mask;
-        std::vector<uint8_t> ll, la;
+        std::vector<uint8_t> ll, la, shape;
         avbd_bodies view(int n)
         {
             avbd_bodies b{};
@@ -309,6 +316,7 @@ struct Solver {
             b.radius = radius.data(); b.gravity = gravity.data(); b.layer = layer.data(); b.mask = mask.data();
             b.lock_linear = ll.data(); b.lock_angular = la.data(); b.position = pos.data(); b.rotation = rot.data();
             b.velocity = vel.data(); b.angular_velocity = ang.data(); b.prev_velocity = prev.data();
+            b.shape = shape.data();
             return b;
         }
     };
@@ -319,8 +327,10 @@ struct Solver {
         a.size.resize(3 * n); a.mass.resize(n); a.moment.resize(3 * n); a.friction.resize(n); a.radius.resize(n);
         a.gravity.resize(n); a.pos.resize(3 * n); a.rot.resize(4 * n); a.vel.resize(3 * n); a.ang.resize(3 * n);
         a.prev.resize(3 * n); a.layer.resize(n); a.mask.resize(n); a.ll.resize(n); a.la.resize(n);
+        a.shape.resize(n);
         for (size_t i = 0; i < n; i++) {
             const Rigid *r = list[i];
+            a.shape[i] = static_cast<uint8_t>(r->shape);
             for (int c = 0; c < 3; c++) {
                 a.size[3 * i + c] = r->size[c];
                 a.moment[3 * i + c] = r->moment[c];
@@ -547,14 +557,20 @@ struct Solver {
 
 inline Rigid::Rigid(Solver *p_solver, float3 p_size, float p_density, float p_friction, float3 p_position,
                     float3 p_velocity)
+    : Rigid(p_solver, p_size, ShapeType::Box, p_density, p_friction, p_position, p_velocity)
+{
+}
+
+inline Rigid::Rigid(Solver *p_solver, float3 p_size, ShapeType p_shape, float p_density, float p_friction,
+                    float3 p_position, float3 p_velocity)
     : solver(p_solver), positionLin(p_position), velocityLin(p_velocity), prevVelocityLin(p_velocity),
-      size(p_size), friction(p_friction), gravity(p_solver->gravity)
+      size(p_size), shape(p_shape), friction(p_friction), gravity(p_solver->gravity)
 {
     next = p_solver->bodies;
     p_solver->bodies = this;
     const float s[3] = {p_size.x, p_size.y, p_size.z};
     float mom[3];
-    avbd_box_properties(s, p_density, &mass, mom, &radius);
+    avbd_shape_properties(static_cast<int>(p_shape), s, p_density, &mass, mom, &radius);   // rigid.cpp:34-70
     moment = float3{mom[0], mom[1], mom[2]};
     p_solver->topologyChanged = true;
 }

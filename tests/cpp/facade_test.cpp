@@ -200,6 +200,39 @@ static void test_generic_joint()
            "generic joint dual state comes back", d);
 }
 
+// The reference's recorded round-shape runs (test/core_test.cpp:1061-1090 and 1409-1430;
+// tools/baseline/core_test.txt:85-87 and 124): a spun sphere rolls 3.9950 m keeping 12.546
+// rad/s, and a cylinder thrown at a slab at 15 m/s ends at z = 1.4665.
+static void test_round_shapes()
+{
+    Solver s;
+    new Rigid(&s, {200, 200, 1}, 0.0f, 0.9f, {0, 0, -0.5f});
+    Rigid *wheel = new Rigid(&s, {0.25f, 0, 0}, ShapeType::Sphere, 1000.0f, 1.0f, {0, 0, 0.25f});
+    for (int i = 0; i < 120; i++) s.step();
+    wheel->velocityAng = {20.0f, 0, 0};
+    const float startY = wheel->positionLin.y;
+    for (int i = 0; i < 120; i++) s.step();
+    const float travelled = wheel->positionLin.y - startY, spin = wheel->velocityAng.x;
+
+    Solver t;
+    t.speculativeMargin = 0.01f;
+    new Rigid(&t, {100, 100, 1}, 0.0f, 0.5f, {0, 0, 0});
+    Rigid *cyl = new Rigid(&t, {0.5f, 0.5f, 1.0f}, ShapeType::Cylinder, 1.0f, 0.5f, {0, 0, 1.008f}, {0, 0, -15.0f});
+    cyl->gravity = 0.0f;
+    float lowest = 1e9f;
+    for (int i = 0; i < 60; i++) {
+        t.step();
+        lowest = std::fmin(lowest, cyl->positionLin.z);
+    }
+    char d[200];
+    std::snprintf(d, sizeof(d), "rolled %.4f m at %.3f rad/s (recorded -3.9950, 12.546); cylinder z %.4f, lowest %.4f "
+                  "(recorded 1.4665)", travelled, spin, cyl->positionLin.z, lowest);
+    report(std::fabs(travelled + 3.9950f) <= 5.01e-5f && std::fabs(spin - 12.546f) <= 5.01e-4f &&
+               std::fabs(cyl->positionLin.z - 1.4665f) <= 5.01e-5f && lowest - 0.5f >= 0.49f &&
+               s.lastStatus == AVBD_OK && t.lastStatus == AVBD_OK,
+           "sphere rolls and cylinder lands as recorded", d);
+}
+
 int main()
 {
     const uint64_t want[3] = {0xfbac4f08cb1c977cull, 0xb03adeb37742850aull, 0xfb3ca7a3162035c2ull};
@@ -214,6 +247,7 @@ int main()
     test_break_now_reaps();
     test_spring_and_filter();
     test_generic_joint();
+    test_round_shapes();
     std::printf("%d checks, %d failures\n", g_checks, g_failures);
     return g_failures == 0 ? 0 : 1;
 }

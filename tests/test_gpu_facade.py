@@ -1,7 +1,8 @@
 """GPU: the C++ facade (include/avbd_solver.hpp) run by tests/cpp/facade_test.cpp, which
 builds the reference's digest scenes with `new Rigid` / `new Joint` / `step()` exactly as the
 reference's core tests do and checks the reference's recorded digests, then springs and
-collision filters against closed forms, and generic joints against the reference's rig checks."""
+collision filters against closed forms, generic joints against the reference's rig checks and
+round shapes against the reference's recorded runs."""
 import os
 import subprocess
 
@@ -18,4 +19,4 @@ def test_facade_program():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "8 checks, 0 failures" in r.stdout
+    assert "9 checks, 0 failures" in r.stdout

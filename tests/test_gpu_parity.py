@@ -258,6 +258,49 @@ def test_narrow_phase_against_reference_fixture():
     assert checked > 100
 
 
+def test_round_narrow_phase_against_reference_fixture():
+    """The same for the sphere and cylinder pairs of tests/golden/collide_shapes.npz (every
+    pair of shapes, margins 0 / 0.01 / 0.05: one batch per margin, the solver's speculative
+    margin set to it; the bodies rest, so the margin is not grown)."""
+    import os
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", "collide_shapes.npz"))
+    checked = 0
+
+    def add(s, t, size, pos, rot):
+        if t == pkg.abi.SHAPE_SPHERE:
+            s.sphere(size[0], 1.0, 0.5, pos, rot=rot)
+        elif t == pkg.abi.SHAPE_CYLINDER:
+            s.cylinder(size[0], size[2], 1.0, 0.5, pos, rot=rot)
+        else:
+            s.box(size, 1.0, 0.5, pos, rot=rot)
+
+    for margin in sorted(set(float(x) for x in f["margin"])):
+        cases = [i for i in range(len(f["count"])) if float(f["margin"][i]) == margin]
+        batch = pkg.Batch(P(speculative_margin=margin), len(cases), 2 * len(cases))
+        ids = []
+        for i in cases:
+            s = scenes.SceneBuilder()
+            add(s, f["tb"][i], f["sb"][i], f["pb"][i], f["qb"][i])
+            add(s, f["ta"][i], f["sa"][i], f["pa"][i], f["qa"][i])   # list index 0 = A
+            ids.append(batch.add(*s.build()))
+        batch.step()
+        for i, wid in zip(cases, ids):
+            m = batch.manifolds(wid)
+            c = int(f["count"][i])
+            if c == 0:
+                assert len(m["body_a"]) == 0, i
+                continue
+            if len(m["body_a"]) == 0:
+                continue   # dropped by the swept-AABB broad phase before the narrow phase
+            assert m["num_contacts"][0] == c, i
+            assert np.array_equal(m["feature"][0, :c], f["keys"][i, :c]), i
+            assert bits_equal(m["r_a"][0].reshape(8, 3)[:c], f["ra"][i, :c]), i
+            assert bits_equal(m["r_b"][0].reshape(8, 3)[:c], f["rb"][i, :c]), i
+            assert bits_equal(m["basis"][0], f["basis"][i]), i
+            checked += 1
+    assert checked > 300
+
+
 def test_batch_alone_equals_in_mix():
     worlds = [scenes.pyramid(rows=12, seed=3), scenes.pile(300, seed=51), scenes.digest_mixed(),
               scenes.chains(4, links=16, grid=(2, 2), pitch_x=3.0), scenes.pile(40, seed=52), scenes.digest_stack(),
