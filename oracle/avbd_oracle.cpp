@@ -723,6 +723,128 @@ int collideRound(const RShape &a, const RShape &b, Contact *cs, m3 &basis, float
     return cylinderBox(a, b, a.type == C, cs, basis, margin);
 }
 
+// ------------------------------------------------------------ space queries ----
+// query/query_shape.cpp:8-270: exact ray against one shape.
+constexpr float QEPS = 1.0e-6f;
+struct RayHit { float t; f3 point, normal; bool inside; };
+
+bool insideLocal(const RShape &s, f3 p, float m)
+{
+    if (s.type == AVBD_SHAPE_SPHERE) return length2(p) <= (s.r + m) * (s.r + m);
+    if (s.type == AVBD_SHAPE_CYLINDER) return (p.x * p.x + p.y * p.y) <= (s.r + m) * (s.r + m) && std::fabs(p.z) <= s.hh + m;
+    return std::fabs(p.x) <= s.half.x + m && std::fabs(p.y) <= s.half.y + m && std::fabs(p.z) <= s.half.z + m;
+}
+f3 boxNormal(f3 p, f3 h)
+{
+    const f3 q{std::fabs(p.x) / (h.x > 0.0f ? h.x : 1.0f), std::fabs(p.y) / (h.y > 0.0f ? h.y : 1.0f),
+               std::fabs(p.z) / (h.z > 0.0f ? h.z : 1.0f)};
+    if (q.x >= q.y && q.x >= q.z) return f3{p.x < 0.0f ? -1.0f : 1.0f, 0, 0};
+    if (q.y >= q.z) return f3{0, p.y < 0.0f ? -1.0f : 1.0f, 0};
+    return f3{0, 0, p.z < 0.0f ? -1.0f : 1.0f};
+}
+f3 cylinderNormal(f3 p, float hh)
+{
+    if (std::fabs(std::fabs(p.z) - hh) <= 1.0e-4f) return f3{0, 0, p.z < 0.0f ? -1.0f : 1.0f};
+    const float radial = std::sqrt(p.x * p.x + p.y * p.y);
+    if (radial <= QEPS) return f3{1, 0, 0};
+    return f3{p.x / radial, p.y / radial, 0};
+}
+bool raySphere(const RShape &s, f3 o, f3 d, float maxT, RayHit &h)
+{
+    const float a = dot(d, d);
+    if (a <= QEPS) return false;
+    const float b = dot(o, d);
+    const float c = dot(o, o) - s.r * s.r;
+    const float disc = b * b - a * c;
+    if (disc < 0.0f) return false;
+    const float root = std::sqrt(disc);
+    float t0 = (-b - root) / a, t1 = (-b + root) / a;
+    if (t0 > t1) std::swap(t0, t1);
+    const bool inside = c < 0.0f;
+    float t = inside ? t1 : t0;
+    if (t < 0.0f) t = t1;
+    if (t < 0.0f || t > maxT) return false;
+    const f3 n = o + d * t;
+    h = RayHit{t, s.c + qrot(s.q, n), qrot(s.q, normalize(n)), inside};
+    return true;
+}
+bool rayBox(const RShape &s, f3 o, f3 d, float maxT, RayHit &h)
+{
+    float tEnter = 0.0f, tExit = std::numeric_limits<float>::infinity();
+    for (int i = 0; i < 3; i++) {
+        if (std::fabs(d[i]) <= QEPS) {
+            if (o[i] < -s.half[i] || o[i] > s.half[i]) return false;
+            continue;
+        }
+        const float inv = 1.0f / d[i];
+        float t0 = (-s.half[i] - o[i]) * inv, t1 = (s.half[i] - o[i]) * inv;
+        if (t0 > t1) std::swap(t0, t1);
+        tEnter = smax(tEnter, t0);
+        tExit = smin(tExit, t1);
+        if (tEnter > tExit) return false;
+    }
+    const bool inside = insideLocal(s, o, 0.0f);
+    const float t = tEnter > 0.0f ? tEnter : tExit;
+    if (t < 0.0f || t > maxT) return false;
+    const f3 p = o + d * t;
+    h = RayHit{t, s.c + qrot(s.q, p), qrot(s.q, boxNormal(p, s.half)), inside};
+    return true;
+}
+bool rayCylinder(const RShape &s, f3 o, f3 d, float maxT, RayHit &h)
+{
+    const float r = s.r, hh = s.hh;
+    float hits[4] = {};
+    int count = 0;
+    auto add = [&](float t) {
+        if (t >= -QEPS && t <= maxT + QEPS && count < 4) hits[count++] = smax(0.0f, t);
+    };
+    const float a = d.x * d.x + d.y * d.y;
+    if (a > QEPS) {
+        const float b = o.x * d.x + o.y * d.y;
+        const float c = o.x * o.x + o.y * o.y - r * r;
+        const float disc = b * b - a * c;
+        if (disc >= 0.0f) {
+            const float root = std::sqrt(disc);
+            const float ts[2] = {(-b - root) / a, (-b + root) / a};
+            for (float t : ts)
+                if (std::fabs(o.z + d.z * t) <= hh + QEPS) add(t);
+        }
+    } else if (o.x * o.x + o.y * o.y > r * r + QEPS) {
+        return false;
+    }
+    if (std::fabs(d.z) > QEPS) {
+        const float zs[2] = {hh, -hh};
+        for (float z : zs) {
+            const float t = (z - o.z) / d.z;
+            const float px = o.x + d.x * t, py = o.y + d.y * t;
+            if (px * px + py * py <= r * r + QEPS) add(t);
+        }
+    }
+    if (count == 0) return false;
+    for (int i = 1; i < count; i++) {
+        const float v = hits[i];
+        int j = i - 1;
+        while (j >= 0 && hits[j] > v) { hits[j + 1] = hits[j]; j--; }
+        hits[j + 1] = v;
+    }
+    const bool inside = insideLocal(s, o, 0.0f);
+    const float t = inside ? hits[count - 1] : hits[0];
+    if (t < 0.0f || t > maxT) return false;
+    const f3 p = o + d * t;
+    h = RayHit{t, s.c + qrot(s.q, p), qrot(s.q, cylinderNormal(p, hh)), inside};
+    return true;
+}
+// query_shape.cpp:258-274
+bool intersectRay(const RShape &s, f3 origin, f3 dir, float rayMaxT, RayHit &h)
+{
+    const float maxT = rayMaxT > 0.0f ? rayMaxT : std::numeric_limits<float>::infinity();
+    const f3 o = qrot(qconj(s.q), origin - s.c);
+    const f3 d = qrot(qconj(s.q), dir);
+    if (s.type == AVBD_SHAPE_SPHERE) return raySphere(s, o, d, maxT, h);
+    if (s.type == AVBD_SHAPE_CYLINDER) return rayCylinder(s, o, d, maxT, h);
+    return rayBox(s, o, d, maxT, h);
+}
+
 } // namespace
 
 // collide.cpp:1034-1097 (box-box SAT + clipping).  Returns the contact count; `basis` is
@@ -915,21 +1037,24 @@ struct World {
     }
 
     // ------------------------------------------------------------ broad phase ----
+    // rigid.cpp:83-99: world-space half extents of the body's shape at its current pose
+    static f3 tightHalf(const Body &b)
+    {
+        const f3 h = b.shape == AVBD_SHAPE_SPHERE     ? f3{b.size.x, b.size.x, b.size.x}
+                     : b.shape == AVBD_SHAPE_CYLINDER ? f3{b.size.x, b.size.x, b.size.z * 0.5f}
+                                                      : b.size * 0.5f;
+        const m3 R = qmat(b.q);
+        return f3{(std::fabs(R[0][0]) * h.x + std::fabs(R[1][0]) * h.y) + std::fabs(R[2][0]) * h.z,
+                  (std::fabs(R[0][1]) * h.x + std::fabs(R[1][1]) * h.y) + std::fabs(R[2][1]) * h.z,
+                  (std::fabs(R[0][2]) * h.x + std::fabs(R[1][2]) * h.y) + std::fabs(R[2][2]) * h.z};
+    }
     void sweptAabbs(std::vector<f3> &cen, std::vector<f3> &half) const
     {
         const int n = static_cast<int>(bodies.size());
         cen.resize(n); half.resize(n);
         for (int i = 0; i < n; i++) {
             const Body &b = bodies[i];
-            // rigid.cpp:83-99: the shape's local half extents
-            const f3 h = b.shape == AVBD_SHAPE_SPHERE     ? f3{b.size.x, b.size.x, b.size.x}
-                         : b.shape == AVBD_SHAPE_CYLINDER ? f3{b.size.x, b.size.x, b.size.z * 0.5f}
-                                                          : b.size * 0.5f;
-            const m3 R = qmat(b.q);
-            const f3 tight{
-                (std::fabs(R[0][0]) * h.x + std::fabs(R[1][0]) * h.y) + std::fabs(R[2][0]) * h.z,
-                (std::fabs(R[0][1]) * h.x + std::fabs(R[1][1]) * h.y) + std::fabs(R[2][1]) * h.z,
-                (std::fabs(R[0][2]) * h.x + std::fabs(R[1][2]) * h.y) + std::fabs(R[2][2]) * h.z};
+            const f3 tight = tightHalf(b);
             f3 disp = b.vel * collisionDt;
             if (b.mass > 0) disp += f3{0, 0, b.gravity * collisionDt * collisionDt};
             const float rotSweep = b.radius * length(b.angVel) * collisionDt;
@@ -2102,6 +2227,142 @@ int oracle_download(oracle_world *o, avbd_bodies *b, avbd_joints *j)
 }
 
 int oracle_manifold_count(oracle_world *o) { return static_cast<int>(o->w.live.size()); }
+
+// query_shape.cpp:258-274 on one shape built from a body's size (makeShape's fields).
+int oracle_intersect_ray(int type, const float pos[3], const float rot[4], const float size[3], const float origin[3],
+                         const float dir[3], float maxT, float *t, float point[3], float normal[3], uint8_t *inside)
+{
+    const RShape s = makeRShape(type, f3{pos[0], pos[1], pos[2]}, quat{rot[0], rot[1], rot[2], rot[3]},
+                                f3{size[0], size[1], size[2]});
+    RayHit h{};
+    if (!intersectRay(s, f3{origin[0], origin[1], origin[2]}, f3{dir[0], dir[1], dir[2]}, maxT, h)) return 0;
+    *t = h.t;
+    store3(point, 0, h.point);
+    store3(normal, 0, h.normal);
+    *inside = h.inside ? 1 : 0;
+    return 1;
+}
+
+// Solver::collectAABB (solver_query.cpp:158-171): list indices, in list order, of the bodies
+// whose tight bounds (rigid.cpp:83-99) overlap [min, max] and whose layer passes the mask.
+int oracle_collect_aabb(oracle_world *o, const float mn[3], const float mx[3], uint32_t mask, int32_t *out, int cap)
+{
+    const World &w = o->w;
+    int n = 0;
+    for (size_t i = 0; i < w.bodies.size(); i++) {
+        const Body &b = w.bodies[i];
+        if ((b.layer & mask) == 0) continue;
+        const f3 wx = w.tightHalf(b);
+        const f3 lo = b.pos - wx, hi = b.pos + wx;
+        if (!(lo.x > mx[0] || lo.y > mx[1] || lo.z > mx[2] || mn[0] > hi.x || mn[1] > hi.y || mn[2] > hi.z)) {
+            if (n < cap) out[n] = static_cast<int32_t>(i);
+            n++;
+        }
+    }
+    return n;
+}
+
+// The bodies part of AVBDDirectSpaceState3D::_intersect_ray (avbd_direct_space_state3d.cpp:
+// 103-195) for n segments from -> to: candidates from collectAABB over the segment's box padded
+// by 1e-4, in list order; the nearest hit (the earlier candidate on a tie); a hit from inside
+// (only when hit_from_inside) reports t = 0 at the origin with normal -dir and ends the search.
+int oracle_raycast(oracle_world *o, int n, const float *from, const float *to, const uint32_t *masks,
+                   int hitFromInside, int32_t *body, float *tOut, float *point, float *normal, uint8_t *inside)
+{
+    World &w = o->w;
+    std::vector<int32_t> cand(w.bodies.size());
+    for (int r = 0; r < n; r++) {
+        body[r] = -1; tOut[r] = 0.0f; inside[r] = 0;
+        store3(point, r, f3{0, 0, 0}); store3(normal, r, f3{0, 0, 0});
+        const f3 origin = load3(from, r), end = load3(to, r);
+        f3 dir = end - origin;
+        const float maxD = length(dir);
+        if (maxD <= 1.0e-6f) continue;
+        dir = dir * (1.0f / maxD);
+        const float pad = 1.0e-4f;
+        const float mn[3] = {smin(origin.x, end.x) - pad, smin(origin.y, end.y) - pad, smin(origin.z, end.z) - pad};
+        const float mx[3] = {smax(origin.x, end.x) + pad, smax(origin.y, end.y) + pad, smax(origin.z, end.z) + pad};
+        const int nc = oracle_collect_aabb(o, mn, mx, masks ? masks[r] : 0xFFFFFFFFu, cand.data(),
+                                           static_cast<int>(cand.size()));
+        bool valid = false;
+        for (int k = 0; k < nc; k++) {
+            const Body &b = w.bodies[cand[k]];
+            const RShape s = makeRShape(b.shape, b.pos, b.q, b.size);
+            RayHit h{};
+            if (!intersectRay(s, origin, dir, maxD, h)) continue;
+            if (h.inside && !hitFromInside) continue;
+            if (h.inside) {
+                body[r] = cand[k]; tOut[r] = 0.0f; store3(point, r, origin); store3(normal, r, dir * -1.0f);
+                inside[r] = 1;
+                valid = true;
+                break;
+            }
+            if (h.t > maxD || (valid && h.t >= tOut[r])) continue;
+            body[r] = cand[k]; tOut[r] = h.t; store3(point, r, h.point); store3(normal, r, h.normal);
+            inside[r] = 0;
+            valid = true;
+        }
+    }
+    return AVBD_OK;
+}
+
+// Solver::pick (solver_query.cpp:29-155) for n rays: every body whose layer passes the mask, an
+// exact sphere or the box bounding the shape (cylinder: its local box), nearest t in units of
+// dir (the earlier body on a tie); body -1 when nothing is hit, else the hit point in the
+// body's local frame.
+int oracle_pick(oracle_world *o, int n, const float *origin, const float *dir, const uint32_t *masks, int32_t *body,
+                float *local)
+{
+    const World &w = o->w;
+    for (int r = 0; r < n; r++) {
+        const f3 org = load3(origin, r), dr = load3(dir, r);
+        const uint32_t mask = masks ? masks[r] : 0xFFFFFFFFu;
+        float bestT = std::numeric_limits<float>::infinity();
+        int best = -1;
+        f3 bestLocal{0, 0, 0};
+        for (size_t i = 0; i < w.bodies.size(); i++) {
+            const Body &b = w.bodies[i];
+            if ((b.layer & mask) == 0) continue;
+            const quat inv = qconj(b.q);
+            const f3 ol = qrot(inv, org - b.pos), d = qrot(inv, dr);
+            if (b.shape == AVBD_SHAPE_SPHERE) {
+                const float rr = b.size.x;
+                const float bb = dot(ol, d);
+                const float c = length2(ol) - rr * rr;
+                const float disc = bb * bb - c;
+                if (disc < 0.0f) continue;
+                const float sq = std::sqrt(disc);
+                float t = -bb - sq;
+                if (t < 0.0f) t = -bb + sq;
+                if (t < 0.0f || t >= bestT) continue;
+                bestT = t; best = static_cast<int>(i); bestLocal = ol + d * t;
+                continue;
+            }
+            const f3 half = b.shape == AVBD_SHAPE_CYLINDER ? f3{b.size.x, b.size.x, b.size.z * 0.5f} : b.size * 0.5f;
+            float tEnter = 0.0f, tExit = std::numeric_limits<float>::infinity();
+            bool hit = true;
+            for (int k = 0; k < 3; k++) {
+                if (std::fabs(d[k]) < 1.0e-6f) {
+                    if (ol[k] < -half[k] || ol[k] > half[k]) { hit = false; break; }
+                    continue;
+                }
+                const float invD = 1.0f / d[k];
+                float t0 = (-half[k] - ol[k]) * invD, t1 = (half[k] - ol[k]) * invD;
+                if (t0 > t1) std::swap(t0, t1);
+                tEnter = smax(tEnter, t0);
+                tExit = smin(tExit, t1);
+                if (tEnter > tExit) { hit = false; break; }
+            }
+            if (!hit) continue;
+            const float tHit = tEnter >= 0.0f ? tEnter : tExit;
+            if (tHit < 0.0f) continue;
+            if (tHit < bestT) { bestT = tHit; best = static_cast<int>(i); bestLocal = ol + d * tHit; }
+        }
+        body[r] = best;
+        store3(local, r, best >= 0 ? bestLocal : f3{0, 0, 0});
+    }
+    return AVBD_OK;
+}
 
 float oracle_atan2(float y, float x) { return atan2Poly(y, x); }
 

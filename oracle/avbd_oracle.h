@@ -33,6 +33,13 @@ int oracle_candidate_pairs(oracle_world *o, uint64_t *pairs, int capacity);
 void oracle_morton(const float *centers, int n, uint64_t *codes, int32_t *order);
 int oracle_force_stamp(oracle_world *o, int joint, int body, float *out33);
 int oracle_force_update(oracle_world *o, int joint, int what);
+int oracle_intersect_ray(int type, const float pos[3], const float rot[4], const float size[3], const float origin[3],
+                         const float dir[3], float maxT, float *t, float point[3], float normal[3], uint8_t *inside);
+int oracle_collect_aabb(oracle_world *o, const float mn[3], const float mx[3], uint32_t mask, int32_t *out, int cap);
+int oracle_raycast(oracle_world *o, int n, const float *from, const float *to, const uint32_t *masks,
+                   int hitFromInside, int32_t *body, float *t, float *point, float *normal, uint8_t *inside);
+int oracle_pick(oracle_world *o, int n, const float *origin, const float *dir, const uint32_t *masks, int32_t *body,
+                float *local);
 int oracle_solve6(const float *a36, const float *b6, float *x6);
 int oracle_collide_boxes(const float posA[3], const float rotA[4], const float sizeA[3],
                          const float posB[3], const float rotB[4], const float sizeB[3],

@@ -70,6 +70,12 @@ def lib():
         "oracle_collide_shapes": (C.c_int, [C.c_int, a.f32p, a.f32p, a.f32p, C.c_int, a.f32p, a.f32p, a.f32p,
                                             C.c_float, a.f32p, a.i32p, a.f32p, a.f32p]),
         "oracle_atan2": (C.c_float, [C.c_float, C.c_float]),
+        "oracle_intersect_ray": (C.c_int, [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, a.f32p, C.c_float, a.f32p,
+                                           a.f32p, a.f32p, a.u8p]),
+        "oracle_collect_aabb": (C.c_int, [vp, a.f32p, a.f32p, C.c_uint32, a.i32p, C.c_int]),
+        "oracle_raycast": (C.c_int, [vp, C.c_int, a.f32p, a.f32p, a.u32p, C.c_int, a.i32p, a.f32p, a.f32p, a.f32p,
+                                     a.u8p]),
+        "oracle_pick": (C.c_int, [vp, C.c_int, a.f32p, a.f32p, a.u32p, a.i32p, a.f32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -94,6 +100,9 @@ def ref_lib():
     R.ref_collide_shapes.restype = C.c_int
     R.ref_collide_shapes.argtypes = [C.c_int, a.f32p, a.f32p, a.f32p, C.c_int, a.f32p, a.f32p, a.f32p, C.c_float,
                                      a.f32p, a.i32p, a.f32p, a.f32p]
+    R.ref_intersect_ray.restype = C.c_int
+    R.ref_intersect_ray.argtypes = [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, a.f32p, C.c_float, a.f32p, a.f32p,
+                                    a.f32p, a.u8p]
     R.ref_solve6.restype = C.c_int
     R.ref_solve6.argtypes = [a.f32p, a.f32p, a.f32p]
     R.ref_morton3D.restype = C.c_uint64
@@ -145,6 +154,42 @@ class OracleWorld:
     def force_update(self, joint, what):
         """Apply one force's warm start ("init") or dual update ("dual") alone."""
         assert self.L.oracle_force_update(self.h, joint, {"init": 0, "dual": 1}[what]) == 0
+
+    def collect_aabb(self, mn, mx, mask=0xFFFFFFFF):
+        """Solver::collectAABB: list indices of the bodies whose bounds overlap [mn, mx]."""
+        a = self.a
+        out = np.zeros(max(self.n, 1), np.int32)
+        lo, hi = np.ascontiguousarray(mn, np.float32), np.ascontiguousarray(mx, np.float32)
+        k = self.L.oracle_collect_aabb(self.h, lo.ctypes.data_as(a.f32p), hi.ctypes.data_as(a.f32p), int(mask),
+                                       out.ctypes.data_as(a.i32p), len(out))
+        return out[:k]
+
+    def raycast(self, frm, to, masks=None, hit_from_inside=False):
+        """Segments frm -> to (n x 3): dict of body (-1 none), t, point, normal, inside."""
+        a = self.a
+        frm, to = np.ascontiguousarray(frm, np.float32), np.ascontiguousarray(to, np.float32)
+        n = len(frm)
+        out = {"body": np.zeros(n, np.int32), "t": np.zeros(n, np.float32), "point": np.zeros((n, 3), np.float32),
+               "normal": np.zeros((n, 3), np.float32), "inside": np.zeros(n, np.uint8)}
+        m = None if masks is None else np.ascontiguousarray(masks, np.uint32)
+        self.L.oracle_raycast(self.h, n, frm.ctypes.data_as(a.f32p), to.ctypes.data_as(a.f32p),
+                              None if m is None else m.ctypes.data_as(a.u32p), int(bool(hit_from_inside)),
+                              out["body"].ctypes.data_as(a.i32p), out["t"].ctypes.data_as(a.f32p),
+                              out["point"].ctypes.data_as(a.f32p), out["normal"].ctypes.data_as(a.f32p),
+                              out["inside"].ctypes.data_as(a.u8p))
+        return out
+
+    def pick(self, origin, direction, masks=None):
+        """Solver::pick for n rays: (body list index or -1, local hit point)."""
+        a = self.a
+        o, d = np.ascontiguousarray(origin, np.float32), np.ascontiguousarray(direction, np.float32)
+        n = len(o)
+        body, local = np.zeros(n, np.int32), np.zeros((n, 3), np.float32)
+        m = None if masks is None else np.ascontiguousarray(masks, np.uint32)
+        self.L.oracle_pick(self.h, n, o.ctypes.data_as(a.f32p), d.ctypes.data_as(a.f32p),
+                           None if m is None else m.ctypes.data_as(a.u32p), body.ctypes.data_as(a.i32p),
+                           local.ctypes.data_as(a.f32p))
+        return body, local
 
     def step(self, n=1):
         for _ in range(n):
@@ -274,6 +319,25 @@ def collide_shapes(ta, pa, qa, sa, tb, pb, qb, sb, margin):
 def ref_collide_shapes(ta, pa, qa, sa, tb, pb, qb, sb, margin):
     R = ref_lib()
     return None if R is None else _collide_typed(R.ref_collide_shapes, ta, pa, qa, sa, tb, pb, qb, sb, margin)
+
+
+def _ray(fn, t, pos, rot, size, origin, direction, max_t):
+    a = _abi()
+    args = [np.ascontiguousarray(x, np.float32) for x in (pos, rot, size, origin, direction)]
+    tt, point, normal, inside = np.zeros(1, np.float32), np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(1, np.uint8)
+    hit = fn(int(t), *[x.ctypes.data_as(a.f32p) for x in args], float(max_t), tt.ctypes.data_as(a.f32p),
+             point.ctypes.data_as(a.f32p), normal.ctypes.data_as(a.f32p), inside.ctypes.data_as(a.u8p))
+    return bool(hit), tt[0], point, normal, bool(inside[0])
+
+
+def intersect_ray(t, pos, rot, size, origin, direction, max_t):
+    """query::intersectRay on one shape: (hit, t, point, normal, inside)."""
+    return _ray(lib().oracle_intersect_ray, t, pos, rot, size, origin, direction, max_t)
+
+
+def ref_intersect_ray(t, pos, rot, size, origin, direction, max_t):
+    R = ref_lib()
+    return None if R is None else _ray(R.ref_intersect_ray, t, pos, rot, size, origin, direction, max_t)
 
 
 def _solve6(fn, A, b):

@@ -1,8 +1,10 @@
 // ORACLE - test infrastructure only.  C entry points over the reference's own sources,
 // compiled from where they lie in the reference tree (see oracle/Makefile): the narrow phase
-// of src/avbd/collide.cpp (every shape pair) and the header-only 6x6 solves of
+// of src/avbd/collide.cpp (every shape pair), the ray test of src/avbd/query/query_shape.cpp
+// and the header-only 6x6 solves of
 // src/avbd/maths.h.  They pin the restatement in avbd_oracle.cpp directly.
 #include "avbd/solver.h"
+#include "avbd/query/query_shape.hpp"
 
 // Any pair of shapes (0 box, 1 sphere, 2 cylinder), each built from a body's size as the
 // solver's makeShape does.
@@ -64,4 +66,27 @@ extern "C" int ref_solve6(const float *a36, const float *b6, float *x6)
     }
     for (int i = 0; i < 6; i++) x6[i] = x[i];
     return path;
+}
+
+// query::intersectRay on a shape built from a body's size as the solver's makeShape does.
+extern "C" int ref_intersect_ray(int type, const float pos[3], const float rot[4], const float size[3],
+                                 const float origin[3], const float dir[3], float maxT, float *t, float point[3],
+                                 float normal[3], uint8_t *inside)
+{
+    using namespace avbd;
+    Shape sh;
+    sh.type = static_cast<ShapeType>(type);
+    sh.center = float3{pos[0], pos[1], pos[2]};
+    sh.rotation = quat{rot[0], rot[1], rot[2], rot[3]};
+    sh.half = float3{size[0], size[1], size[2]} * 0.5f;
+    sh.radius = size[0];
+    sh.halfHeight = size[2] * 0.5f;
+    sh.axis = qrot(sh.rotation, float3{0, 0, 1});
+    query::Ray ray{float3{origin[0], origin[1], origin[2]}, float3{dir[0], dir[1], dir[2]}, maxT};
+    query::RayHit hit{};
+    if (!query::intersectRay(sh, ray, hit)) return 0;
+    *t = hit.t;
+    for (int c = 0; c < 3; c++) { point[c] = hit.point[c]; normal[c] = hit.normal[c]; }
+    *inside = hit.inside ? 1 : 0;
+    return 1;
 }

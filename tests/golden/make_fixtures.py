@@ -14,6 +14,11 @@ cylinder (collide.cpp:505-1032): the reference's own scenario poses (resting on 
 the cap and on the side, the 5 mm speculative gaps, a sphere centred inside a box) and
 random near-touching pairs of every shape combination, at margins 0, 0.01 and 0.05.
 `python make_fixtures.py shapes` writes only that file.
+
+tests/golden/ray_shapes.npz holds query::intersectRay outputs (hit, t, point, normal, inside)
+of the reference's own src/avbd/query/query_shape.cpp for rays against boxes, spheres and
+cylinders: axis-aligned and oblique, from outside and inside, unnormalised directions, a zero
+direction component, bounded and unbounded.  `python make_fixtures.py rays` writes only that file.
 """
 import os
 import sys
@@ -120,12 +125,49 @@ def write_shapes():
     print(f"shapes: {n} cases, {int((count > 0).sum())} with contacts, {int(count.sum())} contacts")
 
 
+def write_rays(n=1500):
+    rng = np.random.default_rng(1357)
+    rec = {k: [] for k in ("type", "pos", "rot", "size", "origin", "dir", "max_t")}
+    out = {k: [] for k in ("hit", "t", "point", "normal", "inside")}
+    I = np.array([0, 0, 0, 1], np.float32)
+    fixed = [(1, (0, 0, 0), I, (1, 0, 0), (-5, 0, 0), (1, 0, 0), 10.0), (1, (0, 0, 0), I, (1, 0, 0), (-5, 0, 0), (1, 0, 0), 3.0),
+             (1, (0, 0, 0), I, (1, 0, 0), (0, 0, 0), (1, 0, 0), 10.0), (0, (0, 0, 0), I, (2, 2, 2), (-5, 0, 0), (1, 0, 0), 10.0),
+             (2, (0, 0, 0), I, (1, 1, 4), (-5, 0, 0), (1, 0, 0), 10.0), (2, (0, 0, 0), I, (1, 1, 4), (0, 0, 5), (0, 0, -1), 10.0),
+             (1, (0, 0, 0), I, (1, 0, 0), (0, 2, 0), (0, 0, 1), 10.0)]
+    cases = list(fixed)
+    for _ in range(n):
+        t = int(rng.integers(0, 3))
+        size = (rng.random(3) * 1.5 + 0.2).astype(np.float32)
+        q = rng.standard_normal(4)
+        q = (q / np.linalg.norm(q)).astype(np.float32)
+        pos = rng.standard_normal(3).astype(np.float32)
+        org = (rng.standard_normal(3) * (3 if rng.random() < 0.8 else 0.3)).astype(np.float32)
+        d = (pos + rng.standard_normal(3).astype(np.float32) * 0.5 - org).astype(np.float32)
+        if rng.random() < 0.5:
+            d = (d / np.linalg.norm(d)).astype(np.float32)
+        if rng.random() < 0.1:
+            d[int(rng.integers(0, 3))] = 0
+        cases.append((t, pos, q, size, org, d, float(rng.choice([0.0, 2.0, 10.0]))))
+    for c in cases:
+        for k, v in zip(rec, c):
+            rec[k].append(np.asarray(v, np.int32 if k == "type" else np.float32))
+        hit, tt, point, normal, inside = oracle.ref_intersect_ray(*c)
+        out["hit"].append(np.uint8(hit)); out["t"].append(np.float32(tt)); out["point"].append(point)
+        out["normal"].append(normal); out["inside"].append(np.uint8(inside))
+    np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "ray_shapes.npz"),
+                        **{k: np.stack(v) for k, v in rec.items()}, **{k: np.stack(v) for k, v in out.items()})
+    print(f"rays: {len(cases)} cases, {int(np.sum(out['hit']))} hits, {int(np.sum(out['inside']))} from inside")
+
+
 def main():
     oracle.build(ref=True)
     if oracle.ref_lib() is None:
         raise SystemExit("oracle/_ref/libavbd_ref.so is not built (needs the reference tree)")
     if sys.argv[1:] == ["shapes"]:
         write_shapes()
+        return
+    if sys.argv[1:] == ["rays"]:
+        write_rays()
         return
     if oracle.ref_lib() is None:
         raise SystemExit("oracle/_ref/libavbd_ref.so is not built (needs the reference tree)")
@@ -162,6 +204,7 @@ def main():
     np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "morton.npz"),
                         sets=np.int32(4), **sets)
     write_shapes()
+    write_rays()
 
 
 if __name__ == "__main__":

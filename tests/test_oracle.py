@@ -706,3 +706,132 @@ def test_cylinder_speculative_contacts_match_recorded_runs():
         lowest = min(lowest, float(o.download()["position"][0][2]))
     assert lowest - 0.5 >= 0.5 - 0.01
     assert abs(float(o.download()["position"][0][2]) - 1.4665) <= 5.01e-5
+
+
+# ---------------------------------------------------------------- space queries ----
+def test_intersect_ray_matches_reference_fixture():
+    """query::intersectRay (query_shape.cpp:258-274) against tests/golden/ray_shapes.npz, made
+    by the reference's own query_shape.cpp: hit flag, t, point, normal and the inside flag."""
+    f = np.load(os.path.join(ROOT, "tests", "golden", "ray_shapes.npz"))
+    for i in range(len(f["hit"])):
+        hit, t, point, normal, inside = oracle.intersect_ray(f["type"][i], f["pos"][i], f["rot"][i], f["size"][i],
+                                                             f["origin"][i], f["dir"][i], f["max_t"][i])
+        assert hit == bool(f["hit"][i]), i
+        if hit:
+            assert np.float32(t).view(np.uint32) == f["t"][i].view(np.uint32), i
+            assert np.array_equal(point.view(np.uint32), f["point"][i].view(np.uint32)), i
+            assert np.array_equal(normal.view(np.uint32), f["normal"][i].view(np.uint32)), i
+            assert inside == bool(f["inside"][i]), i
+    assert f["hit"].sum() > 500 and f["inside"].sum() > 20
+
+
+def test_intersect_ray_matches_reference_live():
+    if oracle.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(91)
+    for _ in range(4000):
+        t = int(rng.integers(0, 3))
+        size = (rng.random(3) * 1.5 + 0.2).astype(np.float32)
+        q = rng.standard_normal(4)
+        q = (q / np.linalg.norm(q)).astype(np.float32)
+        pos = rng.standard_normal(3).astype(np.float32)
+        org = (rng.standard_normal(3) * 3).astype(np.float32)
+        d = (pos - org + rng.standard_normal(3).astype(np.float32) * 0.5).astype(np.float32)
+        max_t = float(rng.choice([0.0, 3.0]))
+        a = oracle.intersect_ray(t, pos, q, size, org, d, max_t)
+        r = oracle.ref_intersect_ray(t, pos, q, size, org, d, max_t)
+        assert a[0] == r[0]
+        if a[0]:
+            assert np.float32(a[1]) == np.float32(r[1]) and np.array_equal(a[2], r[2]) and np.array_equal(a[3], r[3])
+            assert a[4] == r[4]
+
+
+@pytest.mark.parametrize("shape,size,origin,direction,max_t,t,normal,inside", [
+    (1, (1, 0, 0), (-5, 0, 0), (1, 0, 0), 10.0, 4.0, (-1, 0, 0), False),    # query_test.cpp:79-85
+    (1, (1, 0, 0), (-5, 0, 0), (1, 0, 0), 3.0, None, None, None),           # :87-90 maxT rejects
+    (1, (1, 0, 0), (0, 0, 0), (1, 0, 0), 10.0, 1.0, (1, 0, 0), True),       # :92-98 from inside
+    (0, (2, 2, 2), (-5, 0, 0), (1, 0, 0), 10.0, 4.0, (-1, 0, 0), False),    # :100-103 box entry
+    (2, (1, 1, 4), (-5, 0, 0), (1, 0, 0), 10.0, 4.0, (-1, 0, 0), False),    # :105-109 cylinder side
+    (2, (1, 1, 4), (0, 0, 5), (0, 0, -1), 10.0, 3.0, (0, 0, 1), False),     # :111-118 cylinder cap
+    (1, (1, 0, 0), (0, 2, 0), (0, 0, 1), 10.0, None, None, None),           # :120-124 miss
+])
+def test_intersect_ray_known_answers(shape, size, origin, direction, max_t, t, normal, inside):
+    """The reference's own ray checks (test/query_test.cpp:69-125) on unit shapes at the origin."""
+    hit, tt, point, n, ins = oracle.intersect_ray(shape, (0, 0, 0), (0, 0, 0, 1), size, origin, direction, max_t)
+    assert hit == (t is not None)
+    if hit:
+        assert abs(tt - t) <= 1e-5 and np.allclose(n, normal, atol=1e-5) and ins == inside
+        assert np.allclose(point, np.asarray(origin) + np.asarray(direction) * t, atol=1e-5)
+
+
+def _query_scene():
+    s = scenes.SceneBuilder()
+    s.box((20, 20, 1), 0.0, 0.5, (0, 0, -0.5))
+    s.box((1, 1, 1), 1.0, 0.5, (0, 0, 0.5))
+    s.box((1, 1, 1), 1.0, 0.5, (0, 0, 1.5))
+    s.sphere(0.5, 1.0, 0.5, (3, 0, 0.5))
+    s.cylinder(0.5, 2.0, 1.0, 0.5, (-3, 0, 1.0))
+    b, j = s.build()
+    b["layer"] = np.array([1, 1, 2, 1, 1], np.uint32)   # list order: cylinder, sphere, upper box, lower box, ground
+    o = oracle.OracleWorld(P())
+    o.upload(b, j)
+    return o, b
+
+
+def test_raycast_and_pick_closed_forms():
+    """Rays straight down onto the stack, the sphere, the cylinder and the ground hit the top
+    surfaces (z = 2, 1, 2, 0) with upward normals; a mask that excludes the upper box's layer
+    hits the lower box at z = 1; a ray starting inside a body hits at t = 0 only when asked."""
+    o, b = _query_scene()
+    frm = np.array([[0, 0, 5], [3, 0, 5], [-3, 0, 5], [6, 6, 5], [0, 0, 5], [0, 0, 1.5]], np.float32)
+    to = frm - np.array([0, 0, 10], np.float32)
+    masks = np.array([0xFFFFFFFF] * 4 + [1, 0xFFFFFFFF], np.uint32)
+    r = o.raycast(frm, to, masks)
+    assert list(r["body"]) == [2, 1, 0, 4, 3, 3]
+    assert np.allclose(r["point"][:, 2], [2, 1, 2, 0, 1, 1], atol=1e-5)
+    assert np.allclose(r["t"][:5], [3, 4, 3, 5, 4], atol=1e-5) and np.allclose(r["normal"][:5], [0, 0, 1], atol=1e-5)
+    assert r["inside"][5] == 0 and abs(r["t"][5] - 0.5) <= 1e-5   # from inside, not asked: the next body below
+    ri = o.raycast(frm[5:], to[5:], None, hit_from_inside=True)
+    assert ri["body"][0] == 2 and ri["inside"][0] == 1 and ri["t"][0] == 0.0
+    assert np.allclose(ri["normal"][0], [0, 0, 1]) and np.allclose(ri["point"][0], frm[5])
+    assert o.raycast(np.array([[0, 0, 5]], np.float32), np.array([[0, 0, 4]], np.float32))["body"][0] == -1
+    down = np.tile(np.array([0, 0, -1], np.float32), (4, 1))   # pick's sphere test assumes a unit direction
+    body, local = o.pick(frm[:4], down)
+    assert list(body) == [2, 1, 0, 4]
+    assert np.allclose(local[:, 2], [0.5, 0.5, 1.0, 0.5], atol=1e-5)
+    body, _ = o.pick(frm[:1], down[:1], np.array([1], np.uint32))
+    assert body[0] == 3
+
+
+def test_collect_aabb_against_brute_force():
+    """Solver::collectAABB: list order, mask, and each body's tight world bound computed here
+    from its rotation matrix."""
+    b, j = scenes.round_pile(60, seed=12)
+    o = oracle.OracleWorld(P())
+    o.upload(b, j)
+    o.step(30)
+    st = o.download()
+    rng = np.random.default_rng(4)
+    shape = b["shape"]
+    for _ in range(50):
+        c = rng.uniform(-6, 6, 3)
+        h = rng.uniform(0.2, 3.0, 3)
+        mask = int(rng.choice([1, 2, 3, 0xFFFFFFFF]))
+        got = o.collect_aabb(c - h, c + h, mask)
+        strict, loose = [], []
+        for i in range(len(b["mass"])):
+            if (int(b.get("layer", np.ones(len(b["mass"]), np.uint32))[i]) & mask) == 0:
+                continue
+            x, y, z, w = st["rotation"][i].astype(np.float64)
+            R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                          [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                          [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+            s3 = b["size"][i].astype(np.float64)
+            half = {0: s3 / 2, 1: np.array([s3[0]] * 3), 2: np.array([s3[0], s3[0], s3[2] / 2])}[int(shape[i])]
+            wx = np.abs(R) @ half
+            p = st["position"][i].astype(np.float64)
+            if np.all(p - wx <= c + h - 1e-4) and np.all(c - h <= p + wx - 1e-4):
+                strict.append(i)
+            if np.all(p - wx <= c + h + 1e-4) and np.all(c - h <= p + wx + 1e-4):
+                loose.append(i)
+        assert list(got) == sorted(got) and set(strict) <= set(got) <= set(loose)
