@@ -98,6 +98,10 @@ class Manifolds(C.Structure):
     ]
 
 
+class RayHits(C.Structure):
+    _fields_ = [("body", i32p), ("t", f32p), ("point", f32p), ("normal", f32p), ("inside", u8p)]
+
+
 class StepStats(C.Structure):
     _fields_ = [
         ("bodies", C.c_int32), ("joints", C.c_int32), ("manifolds", C.c_int32), ("contacts", C.c_int32),
@@ -172,6 +176,15 @@ def _struct(cls, fields, arrays, count):
     return s
 
 
+def ray_hits(n):
+    """Output arrays for n rays and the RayHits view over them."""
+    out = {"body": np.zeros(n, np.int32), "t": np.zeros(n, np.float32), "point": np.zeros((n, 3), np.float32),
+           "normal": np.zeros((n, 3), np.float32), "inside": np.zeros(n, np.uint8)}
+    h = RayHits(_ptr(out["body"], i32p), _ptr(out["t"], f32p), _ptr(out["point"], f32p), _ptr(out["normal"], f32p),
+                _ptr(out["inside"], u8p))
+    return out, h
+
+
 def bodies_struct(arrays):
     n = len(arrays["mass"])
     return _struct(Bodies, BODY_FIELDS, arrays, n)
@@ -229,6 +242,11 @@ def load_library(path=LIB_PATH):
         "avbd_world_step_interleaved": (C.c_int32, [vp, C.c_int32, C.c_int32]),
         "avbd_world_download": (C.c_int32, [vp, C.POINTER(Bodies), C.POINTER(Joints)]),
         "avbd_world_get_manifolds": (C.c_int32, [vp, C.POINTER(Manifolds)]),
+        "avbd_world_raycast": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, C.c_int32, C.POINTER(RayHits)]),
+        "avbd_batch_raycast": (C.c_int32, [vp, C.c_int32, C.c_int32, f32p, f32p, u32p, C.c_int32,
+                                           C.POINTER(RayHits)]),
+        "avbd_world_pick": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, i32p, f32p]),
+        "avbd_world_collect_aabb": (C.c_int32, [vp, f32p, f32p, C.c_uint32, i32p, C.c_int32, i32p]),
         "avbd_world_set_manifolds": (C.c_int32, [vp, C.POINTER(Manifolds)]),
         "avbd_world_stats": (C.c_int32, [vp, C.POINTER(StepStats)]),
         "avbd_world_colours": (C.c_int32, [vp, u8p]),

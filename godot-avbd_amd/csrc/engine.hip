@@ -12,6 +12,7 @@
 
 #include "../../include/avbd_hip.h"
 #include "kernels.hpp"
+#include "queries.hpp"
 
 using namespace avbd_dev;
 
@@ -234,6 +235,129 @@ struct Engine {
     {
         for (void *p : allocs) (void)hipFree(p);
         allocs.clear();
+        qCap = 0;
+    }
+
+    // ---- space queries (solver_query.cpp, query/query_shape.cpp, the server's intersect_ray) ----
+    int qCap = 0;   // rays the query buffers hold
+    float4 *qRays = nullptr, *qA = nullptr, *qB = nullptr;
+    int *qBody = nullptr;
+    uint8_t *qFlag = nullptr;
+
+    avbd_status ensureQuery(int n)
+    {
+        if (n <= qCap) return AVBD_OK;
+        const int cap = std::max(n, std::max(1024, 2 * qCap));
+        const int bodies = static_cast<int>(Ncap);
+        auto grow = [&](auto *&p, size_t count) -> avbd_status {
+            if (p) {
+                allocs.erase(std::remove(allocs.begin(), allocs.end(), static_cast<void *>(p)), allocs.end());
+                (void)hipFree(p);
+                p = nullptr;
+            }
+            hipError_t e = dalloc(p, count);
+            if (e != hipSuccess) return fail(AVBD_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+            allocs.push_back(p);
+            return AVBD_OK;
+        };
+        avbd_status st;
+        if ((st = grow(qRays, 2 * static_cast<size_t>(cap))) != AVBD_OK) return st;
+        if ((st = grow(qA, cap)) != AVBD_OK) return st;
+        if ((st = grow(qB, cap)) != AVBD_OK) return st;
+        if ((st = grow(qBody, cap)) != AVBD_OK) return st;
+        if ((st = grow(qFlag, std::max(cap, bodies))) != AVBD_OK) return st;
+        qCap = cap;
+        return AVBD_OK;
+    }
+
+    avbd_status raycast(int w, int n, const float *from, const float *to, const uint32_t *masks, bool fromInside,
+                        int32_t *body, float *t, float *point, float *normal, uint8_t *inside)
+    {
+        const WorldInfo &wi = worlds[w];
+        if (!wi.active) return fail(AVBD_ERR_INVALID, "world is not active");
+        if (n < 0 || (n > 0 && (!from || !to || !body))) return fail(AVBD_ERR_INVALID, "bad ray arrays");
+        if (n == 0) return AVBD_OK;
+        avbd_status st = ensureQuery(n);
+        if (st != AVBD_OK) return st;
+        std::vector<float4> rays(2 * static_cast<size_t>(n));
+        for (int r = 0; r < n; r++) {
+            const uint32_t m = masks ? masks[r] : 0xFFFFFFFFu;
+            float mf;
+            std::memcpy(&mf, &m, sizeof(mf));
+            rays[2 * r] = make_float4(from[3 * r], from[3 * r + 1], from[3 * r + 2], mf);
+            rays[2 * r + 1] = make_float4(to[3 * r], to[3 * r + 1], to[3 * r + 2], 0.0f);
+        }
+        HIPCHK(hipMemcpyAsync(qRays, rays.data(), sizeof(float4) * rays.size(), hipMemcpyHostToDevice, stream));
+        k_raycast<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qRays, fromInside ? 1 : 0, qBody, qA, qB);
+        HIPCHK(hipGetLastError());
+        std::vector<float4> a(n), b2(n);
+        HIPCHK(hipMemcpyAsync(body, qBody, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(a.data(), qA, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(b2.data(), qB, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int r = 0; r < n; r++) {
+            if (t) t[r] = a[r].w;
+            if (point) { point[3 * r] = a[r].x; point[3 * r + 1] = a[r].y; point[3 * r + 2] = a[r].z; }
+            if (normal) { normal[3 * r] = b2[r].x; normal[3 * r + 1] = b2[r].y; normal[3 * r + 2] = b2[r].z; }
+            if (inside) inside[r] = b2[r].w != 0.0f ? 1 : 0;
+        }
+        return AVBD_OK;
+    }
+
+    avbd_status pick(int w, int n, const float *origin, const float *dir, const uint32_t *masks, int32_t *body,
+                     float *local)
+    {
+        const WorldInfo &wi = worlds[w];
+        if (!wi.active) return fail(AVBD_ERR_INVALID, "world is not active");
+        if (n < 0 || (n > 0 && (!origin || !dir || !body))) return fail(AVBD_ERR_INVALID, "bad ray arrays");
+        if (n == 0) return AVBD_OK;
+        avbd_status st = ensureQuery(n);
+        if (st != AVBD_OK) return st;
+        std::vector<float4> rays(2 * static_cast<size_t>(n));
+        for (int r = 0; r < n; r++) {
+            const uint32_t m = masks ? masks[r] : 0xFFFFFFFFu;
+            float mf;
+            std::memcpy(&mf, &m, sizeof(mf));
+            rays[2 * r] = make_float4(origin[3 * r], origin[3 * r + 1], origin[3 * r + 2], mf);
+            rays[2 * r + 1] = make_float4(dir[3 * r], dir[3 * r + 1], dir[3 * r + 2], 0.0f);
+        }
+        HIPCHK(hipMemcpyAsync(qRays, rays.data(), sizeof(float4) * rays.size(), hipMemcpyHostToDevice, stream));
+        k_pick<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qRays, qBody, qA);
+        HIPCHK(hipGetLastError());
+        std::vector<float4> a(n);
+        HIPCHK(hipMemcpyAsync(body, qBody, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(a.data(), qA, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (local)
+            for (int r = 0; r < n; r++) { local[3 * r] = a[r].x; local[3 * r + 1] = a[r].y; local[3 * r + 2] = a[r].z; }
+        return AVBD_OK;
+    }
+
+    avbd_status collectAabb(int w, const float mn[3], const float mx[3], uint32_t mask, int32_t *out, int32_t cap,
+                            int32_t *count)
+    {
+        const WorldInfo &wi = worlds[w];
+        if (!wi.active) return fail(AVBD_ERR_INVALID, "world is not active");
+        if (!mn || !mx || !count || cap < 0 || (cap > 0 && !out)) return fail(AVBD_ERR_INVALID, "bad arguments");
+        avbd_status st = ensureQuery(1);
+        if (st != AVBD_OK) return st;
+        const int n = wi.bodyCount;
+        *count = 0;
+        if (n == 0) return AVBD_OK;
+        k_collect_aabb<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, n, make_float4(mn[0], mn[1], mn[2], 0),
+                                                   make_float4(mx[0], mx[1], mx[2], 0), mask, qFlag);
+        HIPCHK(hipGetLastError());
+        std::vector<uint8_t> flag(n);
+        HIPCHK(hipMemcpyAsync(flag.data(), qFlag, n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        int k = 0;
+        for (int j = 0; j < n; j++)
+            if (flag[j]) {
+                if (k < cap) out[k] = j;
+                k++;
+            }
+        *count = k;
+        return k > cap ? fail(AVBD_ERR_CAPACITY, "more bodies than the output holds (count has the total)") : AVBD_OK;
     }
 
     avbd_status create(const avbd_params &p, int worldsCap, int bodies, int joints, int slots, bool batch)
@@ -1734,6 +1858,37 @@ avbd_status avbd_batch_step(avbd_batch *batch)
 {
     if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
     return batch->eng.stepBatch();
+}
+
+avbd_status avbd_world_raycast(avbd_world *world, int32_t n, const float *from, const float *to,
+                               const uint32_t *masks, int32_t hit_from_inside, avbd_ray_hits *out)
+{
+    if (!world || !out) return fail(AVBD_ERR_INVALID, "null argument");
+    return world->eng.raycast(0, n, from, to, masks, hit_from_inside != 0, out->body, out->t, out->point, out->normal,
+                              out->inside);
+}
+
+avbd_status avbd_batch_raycast(avbd_batch *batch, int32_t world_id, int32_t n, const float *from, const float *to,
+                               const uint32_t *masks, int32_t hit_from_inside, avbd_ray_hits *out)
+{
+    if (!batch || !out) return fail(AVBD_ERR_INVALID, "null argument");
+    if (world_id < 0 || world_id >= batch->eng.Wcap) return fail(AVBD_ERR_INVALID, "bad world id");
+    return batch->eng.raycast(world_id, n, from, to, masks, hit_from_inside != 0, out->body, out->t, out->point,
+                              out->normal, out->inside);
+}
+
+avbd_status avbd_world_pick(avbd_world *world, int32_t n, const float *origin, const float *dir,
+                            const uint32_t *masks, int32_t *body, float *local)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    return world->eng.pick(0, n, origin, dir, masks, body, local);
+}
+
+avbd_status avbd_world_collect_aabb(avbd_world *world, const float min[3], const float max[3], uint32_t mask,
+                                    int32_t *out, int32_t capacity, int32_t *count)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    return world->eng.collectAabb(0, min, max, mask, out, capacity, count);
 }
 
 avbd_status avbd_batch_download_world(avbd_batch *batch, int32_t world_id, avbd_bodies *bodies, avbd_joints *joints)

@@ -1,0 +1,298 @@
+// queries.hpp - space queries on the device state: ray casts (the server's intersect_ray and
+// Solver::pick) and AABB collection, evaluated against each body's current pose.
+//
+// One thread per ray walks the world's bodies in list order, so the nearest hit and its
+// tie-break (the earlier body) come out as in the reference's serial loops; the body index is
+// uniform across a wavefront, so body records are read once per wavefront.
+#pragma once
+
+#include "dsolver.hpp"
+
+namespace avbd_dev {
+
+constexpr float QEPS = 1.0e-6f;
+AD float qinf() { return __int_as_float(0x7f800000); }
+
+struct QHit {
+    float t; v3 point, normal; bool inside;
+};
+
+// query_shape.cpp:27-44
+AD bool insideLocal(int type, const RShp &s, v3 p, float m)
+{
+    if (type == AVBD_SHAPE_SPHERE) return length2(p) <= (s.r + m) * (s.r + m);
+    if (type == AVBD_SHAPE_CYLINDER)
+        return (p.x * p.x + p.y * p.y) <= (s.r + m) * (s.r + m) && fabsf(p.z) <= s.hh + m;
+    return fabsf(p.x) <= s.half.x + m && fabsf(p.y) <= s.half.y + m && fabsf(p.z) <= s.half.z + m;
+}
+// query_shape.cpp:46-56
+AD v3 boxNormal(v3 p, v3 h)
+{
+    const float qx = fabsf(p.x) / (h.x > 0.0f ? h.x : 1.0f), qy = fabsf(p.y) / (h.y > 0.0f ? h.y : 1.0f),
+                qz = fabsf(p.z) / (h.z > 0.0f ? h.z : 1.0f);
+    if (qx >= qy && qx >= qz) return v3{p.x < 0.0f ? -1.0f : 1.0f, 0, 0};
+    if (qy >= qz) return v3{0, p.y < 0.0f ? -1.0f : 1.0f, 0};
+    return v3{0, 0, p.z < 0.0f ? -1.0f : 1.0f};
+}
+// query_shape.cpp:58-67
+AD v3 cylinderNormal(v3 p, float hh)
+{
+    if (fabsf(fabsf(p.z) - hh) <= 1.0e-4f) return v3{0, 0, p.z < 0.0f ? -1.0f : 1.0f};
+    const float radial = sqrtf(p.x * p.x + p.y * p.y);
+    if (radial <= QEPS) return v3{1, 0, 0};
+    return v3{p.x / radial, p.y / radial, 0};
+}
+// query_shape.cpp:69-96
+AD bool raySphere(const RShp &s, v3 o, v3 d, float maxT, QHit &h)
+{
+    const float a = dot(d, d);
+    if (a <= QEPS) return false;
+    const float b = dot(o, d);
+    const float c = dot(o, o) - s.r * s.r;
+    const float disc = b * b - a * c;
+    if (disc < 0.0f) return false;
+    const float root = sqrtf(disc);
+    float t0 = (-b - root) / a, t1 = (-b + root) / a;
+    if (t0 > t1) { const float x = t0; t0 = t1; t1 = x; }
+    const bool inside = c < 0.0f;
+    float t = inside ? t1 : t0;
+    if (t < 0.0f) t = t1;
+    if (t < 0.0f || t > maxT) return false;
+    const v3 n = o + d * t;
+    h = QHit{t, s.c + qrot(s.q, n), qrot(s.q, normalize(n)), inside};
+    return true;
+}
+// query_shape.cpp:98-130
+AD bool rayBox(const RShp &s, v3 o, v3 d, float maxT, QHit &h)
+{
+    float tEnter = 0.0f, tExit = qinf();
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float di = d[i], oi = o[i], hi = s.half[i];
+        if (fabsf(di) <= QEPS) {
+            if (oi < -hi || oi > hi) return false;
+            continue;
+        }
+        const float inv = 1.0f / di;
+        float t0 = (-hi - oi) * inv, t1 = (hi - oi) * inv;
+        if (t0 > t1) { const float x = t0; t0 = t1; t1 = x; }
+        tEnter = smax(tEnter, t0);
+        tExit = smin(tExit, t1);
+        if (tEnter > tExit) return false;
+    }
+    const bool inside = insideLocal(AVBD_SHAPE_BOX, s, o, 0.0f);
+    const float t = tEnter > 0.0f ? tEnter : tExit;
+    if (t < 0.0f || t > maxT) return false;
+    const v3 p = o + d * t;
+    h = QHit{t, s.c + qrot(s.q, p), qrot(s.q, boxNormal(p, s.half)), inside};
+    return true;
+}
+// query_shape.cpp:132-210: side hits clipped to the caps, cap hits clipped to the radius, the
+// (at most four) hits sorted; the first, or the last when the origin is inside
+AD bool rayCylinder(const RShp &s, v3 o, v3 d, float maxT, QHit &h)
+{
+    const float r = s.r, hh = s.hh;
+    float hits[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int count = 0;
+    auto add = [&](float t) {
+        if (t >= -QEPS && t <= maxT + QEPS && count < 4) {
+            const float v = smax(0.0f, t);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (k == count) hits[k] = v;
+            count++;
+        }
+    };
+    const float a = d.x * d.x + d.y * d.y;
+    if (a > QEPS) {
+        const float b = o.x * d.x + o.y * d.y;
+        const float c = o.x * o.x + o.y * o.y - r * r;
+        const float disc = b * b - a * c;
+        if (disc >= 0.0f) {
+            const float root = sqrtf(disc);
+            const float t0 = (-b - root) / a, t1 = (-b + root) / a;
+            if (fabsf(o.z + d.z * t0) <= hh + QEPS) add(t0);
+            if (fabsf(o.z + d.z * t1) <= hh + QEPS) add(t1);
+        }
+    } else if (o.x * o.x + o.y * o.y > r * r + QEPS) {
+        return false;
+    }
+    if (fabsf(d.z) > QEPS) {
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const float z = e == 0 ? hh : -hh;
+            const float t = (z - o.z) / d.z;
+            const float px = o.x + d.x * t, py = o.y + d.y * t;
+            if (px * px + py * py <= r * r + QEPS) add(t);
+        }
+    }
+    if (count == 0) return false;
+    // ascending insertion sort (stable; a sorting network would give the same values)
+#pragma unroll
+    for (int i = 1; i < 4; i++) {
+        if (i >= count) break;
+#pragma unroll
+        for (int j = i; j > 0; j--)
+            if (hits[j - 1] > hits[j]) { const float x = hits[j]; hits[j] = hits[j - 1]; hits[j - 1] = x; }
+    }
+    const bool inside = insideLocal(AVBD_SHAPE_CYLINDER, s, o, 0.0f);
+    float t = hits[0];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (inside && k == count - 1) t = hits[k];
+    if (t < 0.0f || t > maxT) return false;
+    const v3 p = o + d * t;
+    h = QHit{t, s.c + qrot(s.q, p), qrot(s.q, cylinderNormal(p, hh)), inside};
+    return true;
+}
+// query_shape.cpp:258-274
+AD bool intersectRay(int type, const RShp &s, v3 origin, v3 dir, float rayMaxT, QHit &h)
+{
+    const float maxT = rayMaxT > 0.0f ? rayMaxT : qinf();
+    const v3 o = qrot(qconj(s.q), origin - s.c);
+    const v3 d = qrot(qconj(s.q), dir);
+    if (type == AVBD_SHAPE_SPHERE) return raySphere(s, o, d, maxT, h);
+    if (type == AVBD_SHAPE_CYLINDER) return rayCylinder(s, o, d, maxT, h);
+    return rayBox(s, o, d, maxT, h);
+}
+
+// rigid.cpp:83-99: world half extents of body i at its current pose
+AD v3 tightHalf(const State &S, int i, int type, float4 sr)
+{
+    const v3 h = type == AVBD_SHAPE_SPHERE     ? v3{sr.x, sr.x, sr.x}
+                 : type == AVBD_SHAPE_CYLINDER ? v3{sr.x, sr.x, sr.z * 0.5f}
+                                               : v3{sr.x, sr.y, sr.z} * 0.5f;
+    const m3 R = qmat(ldq(S.rot[i]));
+    return v3{(fabsf(R.c[0][0]) * h.x + fabsf(R.c[1][0]) * h.y) + fabsf(R.c[2][0]) * h.z,
+              (fabsf(R.c[0][1]) * h.x + fabsf(R.c[1][1]) * h.y) + fabsf(R.c[2][1]) * h.z,
+              (fabsf(R.c[0][2]) * h.x + fabsf(R.c[1][2]) * h.y) + fabsf(R.c[2][2]) * h.z};
+}
+AD bool boundsOverlap(v3 p, v3 wx, v3 mn, v3 mx)
+{
+    const v3 lo = p - wx, hi = p + wx;
+    return !(lo.x > mx.x || lo.y > mx.y || lo.z > mx.z || mn.x > hi.x || mn.y > hi.y || mn.z > hi.z);
+}
+
+// The bodies part of the server's intersect_ray (avbd_direct_space_state3d.cpp:103-195): ray r
+// runs from rays[2r] to rays[2r+1] (w of the first: the mask's bits); candidates are the bodies
+// whose tight bounds overlap the segment's box padded by 1e-4 (Solver::collectAABB), in list
+// order.  Output: body list index or -1; hitA = (point, t); hitB = (normal, inside).
+__global__ void k_raycast(State S, int base, int count, int n, const float4 *rays, int hitFromInside, int *body,
+                          float4 *hitA, float4 *hitB)
+{
+    const int r = GID;
+    if (r >= n) return;
+    const float4 f4 = rays[2 * r], t4 = rays[2 * r + 1];
+    const v3 origin = ld3(f4), end = ld3(t4);
+    const uint32_t mask = __float_as_uint(f4.w);
+    int best = -1;
+    QHit bh{0.0f, v3{0, 0, 0}, v3{0, 0, 0}, false};
+    v3 dir = end - origin;
+    const float maxD = length(dir);
+    if (maxD > 1.0e-6f) {
+        dir = dir * (1.0f / maxD);
+        const float pad = 1.0e-4f;
+        const v3 mn{smin(origin.x, end.x) - pad, smin(origin.y, end.y) - pad, smin(origin.z, end.z) - pad};
+        const v3 mx{smax(origin.x, end.x) + pad, smax(origin.y, end.y) + pad, smax(origin.z, end.z) + pad};
+        for (int j = 0; j < count; j++) {
+            const int i = base + j;
+            if ((S.layer[i] & mask) == 0) continue;
+            const int type = S.shape[i];
+            const float4 sr = S.sizeR[i];
+            const v3 p = ld3(S.pos[i]);
+            if (!boundsOverlap(p, tightHalf(S, i, type, sr), mn, mx)) continue;
+            QHit h;
+            if (!intersectRay(type, makeRShp(p, ldq(S.rot[i]), sr), origin, dir, maxD, h)) continue;
+            if (h.inside && !hitFromInside) continue;
+            if (h.inside) {
+                best = j;
+                bh = QHit{0.0f, origin, dir * -1.0f, true};
+                break;
+            }
+            if (h.t > maxD || (best >= 0 && h.t >= bh.t)) continue;
+            best = j;
+            bh = QHit{h.t, h.point, h.normal, false};
+        }
+    }
+    body[r] = best;
+    hitA[r] = st3(bh.point, bh.t);
+    hitB[r] = st3(bh.normal, bh.inside ? 1.0f : 0.0f);
+}
+
+// Solver::pick (solver_query.cpp:29-155): rays[2r] = (origin, mask bits), rays[2r+1] = dir.
+// Every body that passes the mask: an exact sphere test, else the box bounding the shape; the
+// nearest t (the earlier body on a tie).  local[r] = hit point in the body's frame.
+__global__ void k_pick(State S, int base, int count, int n, const float4 *rays, int *body, float4 *local)
+{
+    const int r = GID;
+    if (r >= n) return;
+    const float4 o4 = rays[2 * r], d4 = rays[2 * r + 1];
+    const v3 org = ld3(o4), dr = ld3(d4);
+    const uint32_t mask = __float_as_uint(o4.w);
+    float bestT = qinf();
+    int best = -1;
+    v3 bestLocal{0, 0, 0};
+    for (int j = 0; j < count; j++) {
+        const int i = base + j;
+        if ((S.layer[i] & mask) == 0) continue;
+        const int type = S.shape[i];
+        const float4 sr = S.sizeR[i];
+        const qt inv = qconj(ldq(S.rot[i]));
+        const v3 ol = qrot(inv, org - ld3(S.pos[i])), d = qrot(inv, dr);
+        if (type == AVBD_SHAPE_SPHERE) {
+            const float rr = sr.x;
+            const float bb = dot(ol, d);
+            const float c = length2(ol) - rr * rr;
+            const float disc = bb * bb - c;
+            if (disc < 0.0f) continue;
+            const float sq = sqrtf(disc);
+            float t = -bb - sq;
+            if (t < 0.0f) t = -bb + sq;
+            if (t < 0.0f || t >= bestT) continue;
+            bestT = t; best = j; bestLocal = ol + d * t;
+            continue;
+        }
+        const v3 half = type == AVBD_SHAPE_CYLINDER ? v3{sr.x, sr.x, sr.z * 0.5f} : v3{sr.x, sr.y, sr.z} * 0.5f;
+        float tEnter = 0.0f, tExit = qinf();
+        bool hit = true;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float dk = d[k], ok = ol[k], hk = half[k];
+            if (hit) {
+                if (fabsf(dk) < 1.0e-6f) {
+                    if (ok < -hk || ok > hk) hit = false;
+                } else {
+                    const float invD = 1.0f / dk;
+                    float t0 = (-hk - ok) * invD, t1 = (hk - ok) * invD;
+                    if (t0 > t1) { const float x = t0; t0 = t1; t1 = x; }
+                    tEnter = smax(tEnter, t0);
+                    tExit = smin(tExit, t1);
+                    if (tEnter > tExit) hit = false;
+                }
+            }
+        }
+        if (!hit) continue;
+        const float tHit = tEnter >= 0.0f ? tEnter : tExit;
+        if (tHit < 0.0f) continue;
+        if (tHit < bestT) { bestT = tHit; best = j; bestLocal = ol + d * tHit; }
+    }
+    body[r] = best;
+    local[r] = st3(best >= 0 ? bestLocal : v3{0, 0, 0});
+}
+
+// Solver::collectAABB (solver_query.cpp:158-171): flag[j] = body j of the world passes the mask
+// and its tight bounds overlap [mn, mx].
+__global__ void k_collect_aabb(State S, int base, int count, float4 mn4, float4 mx4, uint32_t mask, uint8_t *flag)
+{
+    const int j = GID;
+    if (j >= count) return;
+    const int i = base + j;
+    uint8_t f = 0;
+    if ((S.layer[i] & mask) != 0) {
+        const int type = S.shape[i];
+        f = boundsOverlap(ld3(S.pos[i]), tightHalf(S, i, type, S.sizeR[i]), ld3(mn4), ld3(mx4)) ? 1 : 0;
+    }
+    flag[j] = f;
+}
+
+} // namespace avbd_dev

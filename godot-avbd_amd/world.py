@@ -77,6 +77,36 @@ class World:
             jout = {key: v[:k] for key, v in jout.items()}
         return (out, jout) if joints else out
 
+    def raycast(self, frm, to, masks=None, hit_from_inside=False):
+        """Segments frm -> to (n x 3) against the bodies: dict of body (-1 none), t, point,
+        normal, inside (the server's intersect_ray rules)."""
+        frm, to = np.ascontiguousarray(frm, np.float32), np.ascontiguousarray(to, np.float32)
+        out, h = abi.ray_hits(len(frm))
+        m = None if masks is None else np.ascontiguousarray(masks, np.uint32)
+        abi.check(self.lib.avbd_world_raycast(self.h, len(frm), frm.ctypes.data_as(abi.f32p), to.ctypes.data_as(abi.f32p),
+                                              None if m is None else m.ctypes.data_as(abi.u32p), int(bool(hit_from_inside)),
+                                              C.byref(h)))
+        return out
+
+    def pick(self, origin, direction, masks=None):
+        """Solver::pick for n rays: (body list index or -1, local hit point)."""
+        o, d = np.ascontiguousarray(origin, np.float32), np.ascontiguousarray(direction, np.float32)
+        body, local = np.zeros(len(o), np.int32), np.zeros((len(o), 3), np.float32)
+        m = None if masks is None else np.ascontiguousarray(masks, np.uint32)
+        abi.check(self.lib.avbd_world_pick(self.h, len(o), o.ctypes.data_as(abi.f32p), d.ctypes.data_as(abi.f32p),
+                                           None if m is None else m.ctypes.data_as(abi.u32p),
+                                           body.ctypes.data_as(abi.i32p), local.ctypes.data_as(abi.f32p)))
+        return body, local
+
+    def collect_aabb(self, mn, mx, mask=0xFFFFFFFF):
+        """Solver::collectAABB: list indices of the bodies whose bounds overlap [mn, mx]."""
+        out = np.zeros(max(self.n, 1), np.int32)
+        count = C.c_int32()
+        lo, hi = np.ascontiguousarray(mn, np.float32), np.ascontiguousarray(mx, np.float32)
+        abi.check(self.lib.avbd_world_collect_aabb(self.h, lo.ctypes.data_as(abi.f32p), hi.ctypes.data_as(abi.f32p),
+                                                   int(mask), out.ctypes.data_as(abi.i32p), len(out), C.byref(count)))
+        return out[:count.value]
+
     def manifolds(self, capacity=None):
         cap = capacity or max(64, 8 * self.n)
         arr = abi.empty_arrays(abi.MANIFOLD_FIELDS, cap)
@@ -180,6 +210,15 @@ class Batch:
         abi.check(self.lib.avbd_batch_download_world(self.h, int(wid), C.byref(bs), None))
         return out
 
+
+    def raycast(self, wid, frm, to, masks=None, hit_from_inside=False):
+        frm, to = np.ascontiguousarray(frm, np.float32), np.ascontiguousarray(to, np.float32)
+        out, h = abi.ray_hits(len(frm))
+        m = None if masks is None else np.ascontiguousarray(masks, np.uint32)
+        abi.check(self.lib.avbd_batch_raycast(self.h, int(wid), len(frm), frm.ctypes.data_as(abi.f32p),
+                                              to.ctypes.data_as(abi.f32p), None if m is None else m.ctypes.data_as(abi.u32p),
+                                              int(bool(hit_from_inside)), C.byref(h)))
+        return out
 
     def manifolds(self, wid, capacity=None):
         cap = capacity or max(64, 8 * self.sizes[wid])

@@ -273,6 +273,36 @@ avbd_status avbd_world_step_interleaved(avbd_world *world, int32_t substeps, int
 avbd_status avbd_world_download(avbd_world *world, avbd_bodies *bodies, avbd_joints *joints);
 /* Manifold accessors: replace Solver::manifoldCount/manifoldContact/... (solver.h:170-231).
  * get: `out` arrays must hold `out->count` manifolds on entry; count is set on return. */
+/* Per-ray results of avbd_world_raycast (arrays the caller owns; null arrays are skipped,
+ * except body). */
+typedef struct avbd_ray_hits {
+    int32_t *body;     /* n list index of the hit body, -1 for none                   */
+    float *t;          /* n distance along the segment (0 for a hit from inside)      */
+    float *point;      /* 3n hit point (the origin for a hit from inside)             */
+    float *normal;     /* 3n outward surface normal (-direction for a hit from inside) */
+    uint8_t *inside;   /* n 1 when the segment starts inside the body                 */
+} avbd_ray_hits;
+
+/* n segments from[3r..] -> to[3r..] against the bodies at their current pose: replaces the
+ * bodies part of the Godot server's intersect_ray (src/server/avbd_direct_space_state3d.cpp:
+ * 103-195 with query::intersectRay, src/avbd/query/query_shape.cpp:258-274).  Candidates are
+ * the bodies whose layer passes masks[r] (null: all) and whose tight bounds overlap the
+ * segment's box padded by 1e-4, in list order; the nearest hit wins, the earlier body on a tie;
+ * with hit_from_inside, a segment starting inside a body reports it at t = 0 and stops there.
+ * Exclusion lists, areas and ray_pickable are the server's and stay there. */
+avbd_status avbd_world_raycast(avbd_world *world, int32_t n, const float *from, const float *to,
+                               const uint32_t *masks, int32_t hit_from_inside, avbd_ray_hits *out);
+/* Replaces `Solver::pick(origin, dir, local, mask)` (src/avbd/solver.h:161-164,
+ * solver_query.cpp:29-155) for n rays: body[r] = list index of the nearest body (-1 none),
+ * local[3r..] = the hit point in its frame (zeros for none); masks null = all layers. */
+avbd_status avbd_world_pick(avbd_world *world, int32_t n, const float *origin, const float *dir,
+                            const uint32_t *masks, int32_t *body, float *local);
+/* Replaces `Solver::collectAABB(min, max, mask, out)` (solver.h:165-167, solver_query.cpp:
+ * 158-171): list indices, in list order, of the bodies whose layer passes the mask and whose
+ * tight bounds overlap [min, max].  *count is the total; AVBD_ERR_CAPACITY when it exceeds
+ * capacity (the first capacity indices are written). */
+avbd_status avbd_world_collect_aabb(avbd_world *world, const float min[3], const float max[3], uint32_t mask,
+                                    int32_t *out, int32_t capacity, int32_t *count);
 avbd_status avbd_world_get_manifolds(avbd_world *world, avbd_manifolds *out);
 avbd_status avbd_world_set_manifolds(avbd_world *world, const avbd_manifolds *in);
 avbd_status avbd_world_stats(avbd_world *world, avbd_step_stats *out);
@@ -328,6 +358,9 @@ avbd_status avbd_batch_remove_world(avbd_batch *batch, int32_t world_id);
 avbd_status avbd_batch_use_graph(avbd_batch *batch, int32_t enable);
 /* One step of every world in the batch. */
 avbd_status avbd_batch_step(avbd_batch *batch);
+/* avbd_world_raycast for one world of a batch. */
+avbd_status avbd_batch_raycast(avbd_batch *batch, int32_t world_id, int32_t n, const float *from, const float *to,
+                               const uint32_t *masks, int32_t hit_from_inside, avbd_ray_hits *out);
 avbd_status avbd_batch_download_world(avbd_batch *batch, int32_t world_id,
                                       avbd_bodies *bodies, avbd_joints *joints);
 avbd_status avbd_batch_world_count(avbd_batch *batch, int32_t *count);

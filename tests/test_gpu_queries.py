@@ -1,0 +1,88 @@
+"""GPU: space queries (ray casts, picks, AABB collection) on the device state against the
+oracle, which is pinned to the reference's own query_shape.cpp.  Bar: bit-exact."""
+import numpy as np
+import pytest
+
+from helpers import bits_equal, oracle, pkg, scenes
+
+pytestmark = pytest.mark.gpu
+P = pkg.default_params
+
+
+def _worlds(steps=60):
+    b, j = scenes.round_pile(200, seed=17)
+    n = len(b["mass"])
+    b["layer"] = np.where(np.arange(n) % 5 == 0, 2, 1).astype(np.uint32)
+    b["mask"] = np.full(n, 3, np.uint32)
+    w = pkg.World(P(iterations=10), n, 0)
+    w.upload(b, j)
+    o = oracle.OracleWorld(P(iterations=10))
+    o.upload(b, j)
+    w.step(steps)
+    o.step(steps)
+    return w, o, b
+
+
+def _rays(rng, n, inside_from=None):
+    frm = rng.uniform(-12, 12, (n, 3)).astype(np.float32)
+    frm[:, 2] = rng.uniform(0.2, 14, n)
+    to = (frm + rng.normal(0, 8, (n, 3))).astype(np.float32)
+    to[: n // 3, 2] = -2.0                       # a third straight-ish down onto the pile
+    if inside_from is not None:
+        frm[-len(inside_from):] = inside_from    # some start at body centres
+    return frm, to
+
+
+def test_raycast_matches_oracle():
+    w, o, b = _worlds()
+    rng = np.random.default_rng(5)
+    centres = w.download()["position"][:40]
+    frm, to = _rays(rng, 3000, centres)
+    masks = rng.choice(np.array([1, 2, 3, 0xFFFFFFFF], np.uint32), 3000)
+    for inside in (False, True):
+        g = w.raycast(frm, to, masks, hit_from_inside=inside)
+        r = o.raycast(frm, to, masks, hit_from_inside=inside)
+        assert np.array_equal(g["body"], r["body"])
+        for k in ("t", "point", "normal"):
+            assert bits_equal(g[k], r[k]), k
+        assert np.array_equal(g["inside"], r["inside"])
+        assert (g["body"] >= 0).sum() > 1000
+    assert r["inside"].sum() >= 30
+
+
+def test_pick_and_collect_match_oracle():
+    w, o, b = _worlds()
+    rng = np.random.default_rng(6)
+    org = rng.uniform(-10, 10, (2000, 3)).astype(np.float32)
+    org[:, 2] = rng.uniform(1, 12, 2000)
+    d = rng.normal(0, 1, (2000, 3)).astype(np.float32)
+    d[:1000] /= np.linalg.norm(d[:1000], axis=1, keepdims=True)
+    d[:500] = (0, 0, -1)
+    masks = rng.choice(np.array([1, 2, 0xFFFFFFFF], np.uint32), 2000)
+    gb, gl = w.pick(org, d, masks)
+    ob, ol = o.pick(org, d, masks)
+    assert np.array_equal(gb, ob) and bits_equal(gl, ol)
+    assert (gb >= 0).sum() > 500
+    for _ in range(40):
+        c = rng.uniform(-8, 8, 3).astype(np.float32)
+        h = rng.uniform(0.3, 4, 3).astype(np.float32)
+        mask = int(rng.choice([1, 2, 0xFFFFFFFF]))
+        assert np.array_equal(w.collect_aabb(c - h, c + h, mask), o.collect_aabb(c - h, c + h, mask))
+
+
+def test_batch_raycast_equals_plain():
+    b, j = scenes.round_pile(120, seed=23)
+    p = P(iterations=10)
+    batch = pkg.Batch(p, 4, 2000)
+    other = batch.add(*scenes.pile(200, seed=3))
+    wid = batch.add(b, j)
+    w = pkg.World(p, len(b["mass"]), 0)
+    w.upload(b, j)
+    for _ in range(40):
+        batch.step()
+        w.step()
+    rng = np.random.default_rng(7)
+    frm, to = _rays(rng, 1000)
+    g, r = batch.raycast(wid, frm, to), w.raycast(frm, to)
+    assert np.array_equal(g["body"], r["body"]) and bits_equal(g["t"], r["t"]) and bits_equal(g["normal"], r["normal"])
+    assert other != wid
