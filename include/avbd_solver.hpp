@@ -252,6 +252,37 @@ struct Solver {
         avbd_step_stats s{};
         return world && avbd_world_stats(world, &s) == AVBD_OK ? s.contacts : 0;
     }
+    // Solver::pick (solver.h:161-164, solver_query.cpp:29-155): the nearest body along the ray
+    // (null if none) and the hit point in its frame; every body is tested, static ones too.
+    Rigid *pick(float3 origin, float3 dir, float3 &local) { return pick(origin, dir, local, 0xFFFFFFFFu); }
+    Rigid *pick(float3 origin, float3 dir, float3 &local, uint32_t mask)
+    {
+        std::vector<Rigid *> list;
+        std::vector<Force *> jl;
+        if (!bodies || !pushState(list, jl)) return nullptr;
+        const float o[3] = {origin.x, origin.y, origin.z}, d[3] = {dir.x, dir.y, dir.z};
+        int32_t body = -1;
+        float loc[3];
+        lastStatus = avbd_world_pick(world, 1, o, d, &mask, &body, loc);
+        if (lastStatus != AVBD_OK || body < 0) return nullptr;
+        local = float3{loc[0], loc[1], loc[2]};
+        return list[static_cast<size_t>(body)];
+    }
+    // Solver::collectAABB (solver.h:165-167, solver_query.cpp:158-171): appends, in list order,
+    // the bodies whose layer passes the mask and whose bounds overlap [min, max].
+    void collectAABB(float3 min, float3 max, uint32_t mask, std::vector<Rigid *> &out)
+    {
+        std::vector<Rigid *> list;
+        std::vector<Force *> jl;
+        if (!bodies || !pushState(list, jl)) return;
+        std::vector<int32_t> idx(list.size());
+        int32_t count = 0;
+        const float lo[3] = {min.x, min.y, min.z}, hi[3] = {max.x, max.y, max.z};
+        lastStatus = avbd_world_collect_aabb(world, lo, hi, mask, idx.data(), static_cast<int32_t>(idx.size()), &count);
+        if (lastStatus != AVBD_OK) return;
+        for (int32_t k = 0; k < count; k++) out.push_back(list[static_cast<size_t>(idx[k])]);
+    }
+
     avbd_step_stats lastStepStats()
     {
         avbd_step_stats s{};
@@ -457,11 +488,11 @@ struct Solver {
         return true;
     }
 
-    void advance(int substeps, int rounds)
+    // Bring the device up to the host: parameters, then a re-upload after a topology change or
+    // the bodies' dynamic fields otherwise.
+    bool pushState(std::vector<Rigid *> &list, std::vector<Force *> &jl)
     {
-        std::vector<Rigid *> list;
         for (Rigid *r = bodies; r; r = r->next) list.push_back(r);
-        std::vector<Force *> jl;
         for (Force *f = forces; f; f = f->next) {
             if (f->kind == AVBD_FORCE_JOINT) {
                 Joint *j = static_cast<Joint *>(f);
@@ -472,18 +503,26 @@ struct Solver {
         const avbd_params p = params();
         if (world) {
             lastStatus = avbd_world_set_params(world, &p);
-            if (lastStatus != AVBD_OK) return;
+            if (lastStatus != AVBD_OK) return false;
         }
         if (topologyChanged || !world) {
-            if (!rebuild(list, jl)) return;
+            if (!rebuild(list, jl)) return false;
             topologyChanged = false;
         } else {
             Arrays a;
             gather(list, a);
             avbd_bodies bv = a.view(static_cast<int>(list.size()));
             lastStatus = avbd_world_write_bodies(world, &bv);
-            if (lastStatus != AVBD_OK) return;
+            if (lastStatus != AVBD_OK) return false;
         }
+        return true;
+    }
+
+    void advance(int substeps, int rounds)
+    {
+        std::vector<Rigid *> list;
+        std::vector<Force *> jl;
+        if (!pushState(list, jl)) return;
         // joints that were broken before this step are reaped by it (solver_solve.cpp:39-51)
         std::vector<Force *> reaped;   // only joints break
         for (Force *f : jl)

@@ -233,6 +233,36 @@ static void test_round_shapes()
            "sphere rolls and cylinder lands as recorded", d);
 }
 
+// Solver::pick and Solver::collectAABB through the facade on a small scene: a ray straight down
+// finds the top box of a stack (hit 0.5 above its centre), a mask skips its layer, a body moved
+// on the host is found where it was moved to, and the AABB query returns list order.
+static void test_queries()
+{
+    Solver s;
+    Rigid *ground = new Rigid(&s, {20, 20, 1}, 0.0f, 0.5f, {0, 0, -0.5f});
+    Rigid *lower = new Rigid(&s, {1, 1, 1}, 1.0f, 0.5f, {0, 0, 0.5f});
+    Rigid *upper = new Rigid(&s, {1, 1, 1}, 1.0f, 0.5f, {0, 0, 1.5f});
+    Rigid *ball = new Rigid(&s, {0.5f, 0, 0}, ShapeType::Sphere, 1.0f, 0.5f, {3, 0, 0.5f});
+    upper->collisionLayer = 2;
+    s.step();
+    float3 local{};
+    Rigid *hitTop = s.pick({0, 0, 5}, {0, 0, -1}, local);
+    const float topLocal = local.z;
+    Rigid *hitMasked = s.pick({0, 0, 5}, {0, 0, -1}, local, 1u);
+    ball->positionLin = float3{-4, 2, 0.5f};   // moved on the host: queries see it there
+    Rigid *hitBall = s.pick({-4, 2, 5}, {0, 0, -1}, local);
+    std::vector<Rigid *> found;
+    s.collectAABB({-0.2f, -0.2f, -0.2f}, {0.2f, 0.2f, 1.2f}, 0xFFFFFFFFu, found);
+    const bool listOrder = found.size() == 3 && found[0] == upper && found[1] == lower && found[2] == ground;
+    char d[200];
+    std::snprintf(d, sizeof(d), "top %s (local z %.4f), masked %s, moved ball %s, aabb %zu bodies in list order %d",
+                  hitTop == upper ? "upper" : "?", topLocal, hitMasked == lower ? "lower" : "?",
+                  hitBall == ball ? "ball" : "?", found.size(), listOrder ? 1 : 0);
+    report(hitTop == upper && std::fabs(topLocal - 0.5f) < 1e-3f && hitMasked == lower && hitBall == ball && listOrder &&
+               s.lastStatus == AVBD_OK,
+           "pick and collectAABB", d);
+}
+
 int main()
 {
     const uint64_t want[3] = {0xfbac4f08cb1c977cull, 0xb03adeb37742850aull, 0xfb3ca7a3162035c2ull};
@@ -248,6 +278,7 @@ int main()
     test_spring_and_filter();
     test_generic_joint();
     test_round_shapes();
+    test_queries();
     std::printf("%d checks, %d failures\n", g_checks, g_failures);
     return g_failures == 0 ? 0 : 1;
 }
