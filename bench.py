@@ -212,6 +212,67 @@ def bench_serving(args, log):
             "loads": out}
 
 
+def bench_queries(args, log):
+    """Ray casts per second on one world of `query_bodies` mixed boxes, spheres and cylinders
+    after 120 steps: `query_rays` segments per call (a third straight down onto the pile), timed
+    around avbd_world_raycast (host arrays in and out) and, with HIP events, around the kernel
+    alone.  Each ray tests every body's bounds in list order and the exact shape where they
+    overlap, so the work is rays x bodies."""
+    import ctypes as C
+    abi = pkg.abi
+    b, j = pkg.scenes.round_pile(args.query_bodies, seed=31, pitch=2.4)
+    n = len(b["mass"])
+    w = pkg.World(pkg.default_params(iterations=args.iterations), n, 0)
+    w.upload(b, j)
+    w.step(120)
+    rng = np.random.default_rng(8)
+    r = args.query_rays
+    pos = w.download()["position"]
+    lo, hi = pos.min(axis=0), pos.max(axis=0)
+    frm = rng.uniform(lo, hi, (r, 3)).astype(np.float32)
+    frm[:, 2] = hi[2] + 2.0
+    to = (frm + rng.normal(0, 6, (r, 3))).astype(np.float32)
+    to[: r // 3, 2] = lo[2] - 2.0
+    for _ in range(2):
+        out = w.raycast(frm, to)
+    times = []
+    for _ in range(5):
+        t = time.perf_counter()
+        out = w.raycast(frm, to)
+        times.append(time.perf_counter() - t)
+    host_s = float(np.median(times))
+    hits = int((out["body"] >= 0).sum())
+    # kernel alone, HIP events on the library's stream through the profiling entry point
+    prof = abi.QueryProfile()
+    abi.check(w.lib.avbd_world_profile_raycast(w.h, r, frm.ctypes.data_as(abi.f32p), to.ctypes.data_as(abi.f32p), 5,
+                                              C.byref(prof)))
+    kern_s = prof.kernel_ms / 1e3
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the oracle's serial restatement of the same query (the reference's server scans the
+        # body list per query the same way), on a bounded sample of the rays, one core
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        o = orc.OracleWorld(pkg.default_params(iterations=args.iterations))
+        st = w.download()
+        o.upload({**b, **{k: st[k] for k in ("position", "rotation", "velocity", "angular_velocity",
+                                                 "prev_velocity")}}, j)
+        k = 4096
+        t = time.perf_counter()
+        ref = o.raycast(frm[:k], to[:k])
+        cpu_s = time.perf_counter() - t
+        assert np.array_equal(ref["body"], out["body"][:k])
+        cpu = {"value": k / cpu_s, "unit": "rays/s", "cores": 1, "kind": "port",
+               "sample": f"the first {k} rays of the same call, same world state"}
+    line = {"metric": "rays/sec (segment casts, every body's bounds then its exact shape)",
+            "value": r / kern_s, "unit": "rays/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic", "config": {"workload": f"round_pile({args.query_bodies}) settled 120 steps",
+                                            "bodies": n, "rays": r, "hits": hits},
+            "kernel_ms": prof.kernel_ms, "bounds_ms": prof.bounds_ms, "host_call_ms": host_s * 1e3,
+            "rays_per_s_host": r / host_s, "pair_tests_per_s": r * n / kern_s, "cpu_baseline": cpu}
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,6 +301,10 @@ def main():
     ap.add_argument("--summarize-rocprof", metavar="CSV",
                     help="print a per-kernel summary of a rocprofv3 CSV of a bench run and exit")
     ap.add_argument("--summary-steps", type=int, default=5)
+    ap.add_argument("--queries", action="store_true",
+                    help="measure batched ray casts (the server's intersect_ray) on a settled mixed-shape pile and exit")
+    ap.add_argument("--query-bodies", type=int, default=8192)
+    ap.add_argument("--query-rays", type=int, default=262144)
     args = ap.parse_args()
     if args.summarize_rocprof:
         summarize_rocprof(args.summarize_rocprof, args.summary_steps)
@@ -252,6 +317,10 @@ def main():
 
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    if args.queries:
+        bench_queries(args, log)
+        return
 
     params = pkg.default_params(iterations=args.iterations)
     t0 = time.time()

@@ -102,6 +102,10 @@ class RayHits(C.Structure):
     _fields_ = [("body", i32p), ("t", f32p), ("point", f32p), ("normal", f32p), ("inside", u8p)]
 
 
+class QueryProfile(C.Structure):
+    _fields_ = [("bounds_ms", C.c_float), ("kernel_ms", C.c_float)]
+
+
 class StepStats(C.Structure):
     _fields_ = [
         ("bodies", C.c_int32), ("joints", C.c_int32), ("manifolds", C.c_int32), ("contacts", C.c_int32),
@@ -246,6 +250,7 @@ def load_library(path=LIB_PATH):
         "avbd_batch_raycast": (C.c_int32, [vp, C.c_int32, C.c_int32, f32p, f32p, u32p, C.c_int32,
                                            C.POINTER(RayHits)]),
         "avbd_world_pick": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, i32p, f32p]),
+        "avbd_world_profile_raycast": (C.c_int32, [vp, C.c_int32, f32p, f32p, C.c_int32, C.POINTER(QueryProfile)]),
         "avbd_world_collect_aabb": (C.c_int32, [vp, f32p, f32p, C.c_uint32, i32p, C.c_int32, i32p]),
         "avbd_world_set_manifolds": (C.c_int32, [vp, C.POINTER(Manifolds)]),
         "avbd_world_stats": (C.c_int32, [vp, C.POINTER(StepStats)]),

@@ -240,7 +240,7 @@ struct Engine {
 
     // ---- space queries (solver_query.cpp, query/query_shape.cpp, the server's intersect_ray) ----
     int qCap = 0;   // rays the query buffers hold
-    float4 *qRays = nullptr, *qA = nullptr, *qB = nullptr;
+    float4 *qRays = nullptr, *qA = nullptr, *qB = nullptr, *qBounds = nullptr;
     int *qBody = nullptr;
     uint8_t *qFlag = nullptr;
 
@@ -266,6 +266,7 @@ struct Engine {
         if ((st = grow(qB, cap)) != AVBD_OK) return st;
         if ((st = grow(qBody, cap)) != AVBD_OK) return st;
         if ((st = grow(qFlag, std::max(cap, bodies))) != AVBD_OK) return st;
+        if ((st = grow(qBounds, 2 * static_cast<size_t>(bodies))) != AVBD_OK) return st;
         qCap = cap;
         return AVBD_OK;
     }
@@ -288,7 +289,9 @@ struct Engine {
             rays[2 * r + 1] = make_float4(to[3 * r], to[3 * r + 1], to[3 * r + 2], 0.0f);
         }
         HIPCHK(hipMemcpyAsync(qRays, rays.data(), sizeof(float4) * rays.size(), hipMemcpyHostToDevice, stream));
-        k_raycast<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qRays, fromInside ? 1 : 0, qBody, qA, qB);
+        if (wi.bodyCount > 0) k_query_bounds<<<nblk(wi.bodyCount), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, qBounds);
+        k_raycast<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qRays, qBounds, fromInside ? 1 : 0, qBody,
+                                               qA, qB);
         HIPCHK(hipGetLastError());
         std::vector<float4> a(n), b2(n);
         HIPCHK(hipMemcpyAsync(body, qBody, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
@@ -301,6 +304,46 @@ struct Engine {
             if (normal) { normal[3 * r] = b2[r].x; normal[3 * r + 1] = b2[r].y; normal[3 * r + 2] = b2[r].z; }
             if (inside) inside[r] = b2[r].w != 0.0f ? 1 : 0;
         }
+        return AVBD_OK;
+    }
+
+    // Average device time of the bounds and ray kernels over `reps` launches of n segments
+    // (every mask bit set, no hit from inside), HIP events on the library's stream.
+    avbd_status profileRaycast(int w, int n, const float *from, const float *to, int reps, avbd_query_profile *out)
+    {
+        const WorldInfo &wi = worlds[w];
+        if (!wi.active) return fail(AVBD_ERR_INVALID, "world is not active");
+        if (n <= 0 || reps <= 0 || !from || !to || !out) return fail(AVBD_ERR_INVALID, "bad arguments");
+        avbd_status st = ensureQuery(n);
+        if (st != AVBD_OK) return st;
+        std::vector<float4> rays(2 * static_cast<size_t>(n));
+        const uint32_t all = 0xFFFFFFFFu;
+        float mf;
+        std::memcpy(&mf, &all, sizeof(mf));
+        for (int r = 0; r < n; r++) {
+            rays[2 * r] = make_float4(from[3 * r], from[3 * r + 1], from[3 * r + 2], mf);
+            rays[2 * r + 1] = make_float4(to[3 * r], to[3 * r + 1], to[3 * r + 2], 0.0f);
+        }
+        HIPCHK(hipMemcpyAsync(qRays, rays.data(), sizeof(float4) * rays.size(), hipMemcpyHostToDevice, stream));
+        hipEvent_t e[3];
+        for (auto &x : e) HIPCHK(hipEventCreate(&x));
+        float bounds = 0.0f, kern = 0.0f;
+        for (int k = 0; k < reps; k++) {
+            HIPCHK(hipEventRecord(e[0], stream));
+            k_query_bounds<<<nblk(std::max(wi.bodyCount, 1)), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, qBounds);
+            HIPCHK(hipEventRecord(e[1], stream));
+            k_raycast<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qRays, qBounds, 0, qBody, qA, qB);
+            HIPCHK(hipEventRecord(e[2], stream));
+            HIPCHK(hipEventSynchronize(e[2]));
+            float a = 0.0f, b = 0.0f;
+            HIPCHK(hipEventElapsedTime(&a, e[0], e[1]));
+            HIPCHK(hipEventElapsedTime(&b, e[1], e[2]));
+            bounds += a;
+            kern += b;
+        }
+        for (auto &x : e) (void)hipEventDestroy(x);
+        out->bounds_ms = bounds / reps;
+        out->kernel_ms = kern / reps;
         return AVBD_OK;
     }
 
@@ -344,7 +387,8 @@ struct Engine {
         const int n = wi.bodyCount;
         *count = 0;
         if (n == 0) return AVBD_OK;
-        k_collect_aabb<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, n, make_float4(mn[0], mn[1], mn[2], 0),
+        k_query_bounds<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, n, qBounds);
+        k_collect_aabb<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, n, qBounds, make_float4(mn[0], mn[1], mn[2], 0),
                                                    make_float4(mx[0], mx[1], mx[2], 0), mask, qFlag);
         HIPCHK(hipGetLastError());
         std::vector<uint8_t> flag(n);
@@ -1875,6 +1919,13 @@ avbd_status avbd_batch_raycast(avbd_batch *batch, int32_t world_id, int32_t n, c
     if (world_id < 0 || world_id >= batch->eng.Wcap) return fail(AVBD_ERR_INVALID, "bad world id");
     return batch->eng.raycast(world_id, n, from, to, masks, hit_from_inside != 0, out->body, out->t, out->point,
                               out->normal, out->inside);
+}
+
+avbd_status avbd_world_profile_raycast(avbd_world *world, int32_t n, const float *from, const float *to, int32_t reps,
+                                       avbd_query_profile *out)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    return world->eng.profileRaycast(0, n, from, to, reps, out);
 }
 
 avbd_status avbd_world_pick(avbd_world *world, int32_t n, const float *origin, const float *dir,

@@ -2,11 +2,13 @@
 // Solver::pick) and AABB collection, evaluated against each body's current pose.
 //
 // One thread per ray walks the world's bodies in list order, so the nearest hit and its
-// tie-break (the earlier body) come out as in the reference's serial loops; the body index is
-// uniform across a wavefront, so body records are read once per wavefront.
+// tie-break (the earlier body) come out as in the reference's serial loops.  Each body's bounds
+// (with its layer bits) are computed once per launch; the body index is uniform across a
+// wavefront, so they are read once per wavefront, and only bodies whose bounds meet a segment's
+// box are read for the exact test.
 #pragma once
 
-#include "dsolver.hpp"
+#include "kernels.hpp"
 
 namespace avbd_dev {
 
@@ -167,18 +169,30 @@ AD v3 tightHalf(const State &S, int i, int type, float4 sr)
               (fabsf(R.c[0][1]) * h.x + fabsf(R.c[1][1]) * h.y) + fabsf(R.c[2][1]) * h.z,
               (fabsf(R.c[0][2]) * h.x + fabsf(R.c[1][2]) * h.y) + fabsf(R.c[2][2]) * h.z};
 }
-AD bool boundsOverlap(v3 p, v3 wx, v3 mn, v3 mx)
+AD bool boxesOverlap(v3 lo, v3 hi, v3 mn, v3 mx)
 {
-    const v3 lo = p - wx, hi = p + wx;
     return !(lo.x > mx.x || lo.y > mx.y || lo.z > mx.z || mn.x > hi.x || mn.y > hi.y || mn.z > hi.z);
+}
+
+// Each body's current world bounds (position -/+ tight half extents), computed once per query
+// launch: bounds[2j] = lower corner, bounds[2j+1] = upper corner of body j of the world.
+__global__ void k_query_bounds(State S, int base, int count, float4 *bounds)
+{
+    const int j = GID;
+    if (j >= count) return;
+    const int i = base + j;
+    const v3 p = ld3(S.pos[i]);
+    const v3 wx = tightHalf(S, i, S.shape[i], S.sizeR[i]);
+    bounds[2 * j] = st3(p - wx, __uint_as_float(S.layer[i]));   // w: the body's layer bits
+    bounds[2 * j + 1] = st3(p + wx);
 }
 
 // The bodies part of the server's intersect_ray (avbd_direct_space_state3d.cpp:103-195): ray r
 // runs from rays[2r] to rays[2r+1] (w of the first: the mask's bits); candidates are the bodies
 // whose tight bounds overlap the segment's box padded by 1e-4 (Solver::collectAABB), in list
 // order.  Output: body list index or -1; hitA = (point, t); hitB = (normal, inside).
-__global__ void k_raycast(State S, int base, int count, int n, const float4 *rays, int hitFromInside, int *body,
-                          float4 *hitA, float4 *hitB)
+__global__ void k_raycast(State S, int base, int count, int n, const float4 *rays, const float4 *bounds,
+                          int hitFromInside, int *body, float4 *hitA, float4 *hitB)
 {
     const int r = GID;
     if (r >= n) return;
@@ -195,14 +209,13 @@ __global__ void k_raycast(State S, int base, int count, int n, const float4 *ray
         const v3 mn{smin(origin.x, end.x) - pad, smin(origin.y, end.y) - pad, smin(origin.z, end.z) - pad};
         const v3 mx{smax(origin.x, end.x) + pad, smax(origin.y, end.y) + pad, smax(origin.z, end.z) + pad};
         for (int j = 0; j < count; j++) {
+            const float4 lo = bounds[2 * j], hi = bounds[2 * j + 1];   // uniform across the wavefront
+            if ((__float_as_uint(lo.w) & mask) == 0) continue;
+            if (!boxesOverlap(ld3(lo), ld3(hi), mn, mx)) continue;
             const int i = base + j;
-            if ((S.layer[i] & mask) == 0) continue;
-            const int type = S.shape[i];
-            const float4 sr = S.sizeR[i];
-            const v3 p = ld3(S.pos[i]);
-            if (!boundsOverlap(p, tightHalf(S, i, type, sr), mn, mx)) continue;
             QHit h;
-            if (!intersectRay(type, makeRShp(p, ldq(S.rot[i]), sr), origin, dir, maxD, h)) continue;
+            if (!intersectRay(S.shape[i], makeRShp(ld3(S.pos[i]), ldq(S.rot[i]), S.sizeR[i]), origin, dir, maxD, h))
+                continue;
             if (h.inside && !hitFromInside) continue;
             if (h.inside) {
                 best = j;
@@ -282,17 +295,14 @@ __global__ void k_pick(State S, int base, int count, int n, const float4 *rays, 
 
 // Solver::collectAABB (solver_query.cpp:158-171): flag[j] = body j of the world passes the mask
 // and its tight bounds overlap [mn, mx].
-__global__ void k_collect_aabb(State S, int base, int count, float4 mn4, float4 mx4, uint32_t mask, uint8_t *flag)
+__global__ void k_collect_aabb(State S, int base, int count, const float4 *bounds, float4 mn4, float4 mx4, uint32_t mask,
+                               uint8_t *flag)
 {
     const int j = GID;
     if (j >= count) return;
     const int i = base + j;
-    uint8_t f = 0;
-    if ((S.layer[i] & mask) != 0) {
-        const int type = S.shape[i];
-        f = boundsOverlap(ld3(S.pos[i]), tightHalf(S, i, type, S.sizeR[i]), ld3(mn4), ld3(mx4)) ? 1 : 0;
-    }
-    flag[j] = f;
+    flag[j] = (S.layer[i] & mask) != 0 && boxesOverlap(ld3(bounds[2 * j]), ld3(bounds[2 * j + 1]), ld3(mn4), ld3(mx4))
+                  ? 1 : 0;   // bounds from k_query_bounds
 }
 
 } // namespace avbd_dev

@@ -292,6 +292,15 @@ typedef struct avbd_ray_hits {
  * Exclusion lists, areas and ray_pickable are the server's and stay there. */
 avbd_status avbd_world_raycast(avbd_world *world, int32_t n, const float *from, const float *to,
                                const uint32_t *masks, int32_t hit_from_inside, avbd_ray_hits *out);
+/* Measurement (no reference counterpart): average device time of the bounds kernel and of
+ * the ray kernel of avbd_world_raycast over `reps` launches of n segments (all layers, no
+ * hit from inside), timed with HIP events on the library's stream. */
+typedef struct avbd_query_profile {
+    float bounds_ms;
+    float kernel_ms;
+} avbd_query_profile;
+avbd_status avbd_world_profile_raycast(avbd_world *world, int32_t n, const float *from, const float *to,
+                                       int32_t reps, avbd_query_profile *out);
 /* Replaces `Solver::pick(origin, dir, local, mask)` (src/avbd/solver.h:161-164,
  * solver_query.cpp:29-155) for n rays: body[r] = list index of the nearest body (-1 none),
  * local[3r..] = the hit point in its frame (zeros for none); masks null = all layers. */
