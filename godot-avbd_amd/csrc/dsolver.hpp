@@ -1288,6 +1288,14 @@ AD RShp makeRShp(v3 p, qt q, float4 size)
     return RShp{p, q, v3{size.x, size.y, size.z} * 0.5f, size.x, size.z * 0.5f, qrot(q, v3{0, 0, 1})};
 }
 AD v3 sel3v(bool c, v3 a, v3 b) { return v3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
+// field-wise select: both shapes stay in registers (a selected reference would put them in
+// scratch memory)
+AD RShp selR(bool c, const RShp &a, const RShp &b)
+{
+    return RShp{sel3v(c, a.c, b.c),
+                qt{c ? a.q.x : b.q.x, c ? a.q.y : b.q.y, c ? a.q.z : b.q.z, c ? a.q.w : b.q.w},
+                sel3v(c, a.half, b.half), c ? a.r : b.r, c ? a.hh : b.hh, sel3v(c, a.axis, b.axis)};
+}
 AD v3 clampBox(v3 l, v3 h)
 {
     return v3{sclamp(l.x, -h.x, h.x), sclamp(l.y, -h.y, h.y), sclamp(l.z, -h.z, h.z)};
@@ -1416,8 +1424,9 @@ constexpr int CYL_SAMPLES = 34;
 AD bool roundCandidate(bool cylBox, bool cylIsA, const RShp &a, const RShp &b, const float *rim, float margin, int k,
                        v3 &xA, v3 &xB, v3 &n, float &depth, int &key)
 {
-    const RShp &cy = cylBox ? (cylIsA ? a : b) : (k < CYL_SAMPLES ? a : b);
-    const RShp &into = cylBox ? (cylIsA ? b : a) : (k < CYL_SAMPLES ? b : a);
+    const bool cyIsA = cylBox ? cylIsA : k < CYL_SAMPLES;
+    const RShp cy = selR(cyIsA, a, b);
+    const RShp into = selR(!cyIsA, a, b);
     const int i = k < CYL_SAMPLES ? k : k - CYL_SAMPLES;
     v3 push;
     float d;
@@ -1485,8 +1494,8 @@ AD int collideSampled(bool cylBox, bool cylIsA, const RShp &a, const RShp &b, co
         kept = kept < MAXC ? kept + 1 : MAXC;
     }
     if (!nc) return 0;
-    const RShp &cy = cylBox ? (cylIsA ? a : b) : a;
-    basisOut = orthonormal(-(length2(nAB) > 1.0e-12f ? normalize(nAB) : cy.axis));
+    const v3 axis = sel3v(!cylBox || cylIsA, a.axis, b.axis);
+    basisOut = orthonormal(-(length2(nAB) > 1.0e-12f ? normalize(nAB) : axis));
 #pragma unroll
     for (int j = 0; j < MAXC; j++) {
         if (j >= kept) break;
@@ -1509,8 +1518,8 @@ AD int collideRound(int typeA, int typeB, const RShp &a, const RShp &b, const fl
     const int S = AVBD_SHAPE_SPHERE, C = AVBD_SHAPE_CYLINDER;
     if (typeA == S || typeB == S) {
         const bool sphereIsA = typeA == S;
-        const RShp &sp = sphereIsA ? a : b;
-        const RShp &other = sphereIsA ? b : a;
+        const RShp sp = selR(sphereIsA, a, b);
+        const RShp other = selR(!sphereIsA, a, b);
         const int otherType = sphereIsA ? typeB : typeA;
         v3 n, onOther;
         if (otherType == S) {
