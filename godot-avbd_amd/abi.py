@@ -189,6 +189,12 @@ def ray_hits(n):
     return out, h
 
 
+def report_arrays(n, stride):
+    """Output arrays of a contact report for n bodies."""
+    return (np.zeros(n, np.int32), np.zeros((n, stride, 3), np.float32), np.zeros((n, stride, 3), np.float32),
+            np.full((n, stride), -1, np.int32))
+
+
 def bodies_struct(arrays):
     n = len(arrays["mass"])
     return _struct(Bodies, BODY_FIELDS, arrays, n)
@@ -250,6 +256,8 @@ def load_library(path=LIB_PATH):
         "avbd_batch_raycast": (C.c_int32, [vp, C.c_int32, C.c_int32, f32p, f32p, u32p, C.c_int32,
                                            C.POINTER(RayHits)]),
         "avbd_world_pick": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, i32p, f32p]),
+        "avbd_world_contact_report": (C.c_int32, [vp, i32p, C.c_int32, i32p, f32p, f32p, i32p]),
+        "avbd_batch_contact_report": (C.c_int32, [vp, C.c_int32, i32p, C.c_int32, i32p, f32p, f32p, i32p]),
         "avbd_world_profile_raycast": (C.c_int32, [vp, C.c_int32, f32p, f32p, C.c_int32, C.POINTER(QueryProfile)]),
         "avbd_world_collect_aabb": (C.c_int32, [vp, f32p, f32p, C.c_uint32, i32p, C.c_int32, i32p]),
         "avbd_world_set_manifolds": (C.c_int32, [vp, C.POINTER(Manifolds)]),

@@ -376,6 +376,88 @@ struct Engine {
         return AVBD_OK;
     }
 
+    // per-body contact reports (scratch sized to the world's bodies and live manifolds)
+    int *rMax = nullptr, *rEntries = nullptr, *rCount = nullptr;
+    uint32_t *rCnt = nullptr, *rOff = nullptr, *rCursor = nullptr;
+    float4 *rPos = nullptr, *rNormal = nullptr;
+    size_t rEntryCap = 0, rOutCap = 0;
+
+    template <typename T> avbd_status regrow(T *&p, size_t count)
+    {
+        if (p) {
+            allocs.erase(std::remove(allocs.begin(), allocs.end(), static_cast<void *>(p)), allocs.end());
+            (void)hipFree(p);
+            p = nullptr;
+        }
+        hipError_t e = dalloc(p, count);
+        if (e != hipSuccess) return fail(AVBD_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(p);
+        return AVBD_OK;
+    }
+
+    avbd_status contactReport(int w, const int32_t *maxRep, int32_t stride, int32_t *count, float *position, float *normal,
+                              int32_t *other)
+    {
+        const WorldInfo &wi = worlds[w];
+        if (!wi.active) return fail(AVBD_ERR_INVALID, "world is not active");
+        const int n = wi.bodyCount;
+        if (!maxRep || !count || stride < 1) return fail(AVBD_ERR_INVALID, "bad arguments");
+        int need = 0;
+        for (int j = 0; j < n; j++) need = std::max(need, static_cast<int>(maxRep[j]));
+        if (need > stride) return fail(AVBD_ERR_INVALID, "stride is smaller than the largest max_reported");
+        if (n == 0) return AVBD_OK;
+        avbd_status st;
+        HIPCHK(hipStreamSynchronize(stream));
+        int nLive = 0;
+        HIPCHK(hipMemcpy(&nLive, S.wLive + w, sizeof(int), hipMemcpyDeviceToHost));
+        if (!rMax) {
+            if ((st = regrow(rMax, Ncap)) != AVBD_OK || (st = regrow(rCnt, Ncap + 1)) != AVBD_OK ||
+                (st = regrow(rOff, Ncap + 1)) != AVBD_OK || (st = regrow(rCursor, Ncap)) != AVBD_OK ||
+                (st = regrow(rCount, Ncap)) != AVBD_OK)
+                return st;
+        }
+        if (static_cast<size_t>(2 * nLive + 1) > rEntryCap) {
+            rEntryCap = std::max<size_t>(2 * nLive + 1, 2 * rEntryCap);
+            if ((st = regrow(rEntries, rEntryCap)) != AVBD_OK) return st;
+        }
+        const size_t outN = static_cast<size_t>(n) * stride;
+        if (outN > rOutCap) {
+            rOutCap = std::max(outN, 2 * rOutCap);
+            if ((st = regrow(rPos, rOutCap)) != AVBD_OK || (st = regrow(rNormal, rOutCap)) != AVBD_OK) return st;
+        }
+        HIPCHK(hipMemcpyAsync(rMax, maxRep, sizeof(int) * n, hipMemcpyHostToDevice, stream));
+        k_fill_u32<<<nblk(n + 1), TPB, 0, stream>>>(rCnt, 0u, n + 1);
+        k_fill_u32<<<nblk(n), TPB, 0, stream>>>(rCursor, 0u, n);
+        if (nLive > 0) {
+            buildLiveList();
+            k_contact_unpack<<<liveGrid(), TPB, 0, stream>>>(S);
+            k_report_count<<<nblk(nLive), TPB, 0, stream>>>(S, w, rMax, rCnt);
+            scan(rCnt, rOff, n);   // exclusive: rOff[0..n], rOff[n] = total
+            k_report_fill<<<nblk(nLive), TPB, 0, stream>>>(S, w, rMax, rOff, rCursor, rEntries);
+        } else {
+            k_fill_u32<<<nblk(n + 1), TPB, 0, stream>>>(rOff, 0u, n + 1);
+        }
+        k_report_emit<<<nblk(n), TPB, 0, stream>>>(S, w, n, rMax, stride, rOff, rEntries, rCount, rPos, rNormal);
+        HIPCHK(hipGetLastError());
+        std::vector<float4> p4(outN), n4(outN);
+        HIPCHK(hipMemcpyAsync(count, rCount, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(p4.data(), rPos, sizeof(float4) * outN, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(n4.data(), rNormal, sizeof(float4) * outN, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int j = 0; j < n; j++)
+            for (int c = 0; c < count[j]; c++) {
+                const size_t o = static_cast<size_t>(j) * stride + c;
+                if (position) { position[3 * o] = p4[o].x; position[3 * o + 1] = p4[o].y; position[3 * o + 2] = p4[o].z; }
+                if (normal) { normal[3 * o] = n4[o].x; normal[3 * o + 1] = n4[o].y; normal[3 * o + 2] = n4[o].z; }
+                if (other) {
+                    int32_t ob;
+                    std::memcpy(&ob, &p4[o].w, sizeof(ob));
+                    other[o] = ob;
+                }
+            }
+        return AVBD_OK;
+    }
+
     avbd_status collectAabb(int w, const float mn[3], const float mx[3], uint32_t mask, int32_t *out, int32_t cap,
                             int32_t *count)
     {
@@ -1926,6 +2008,21 @@ avbd_status avbd_world_profile_raycast(avbd_world *world, int32_t n, const float
 {
     if (!world) return fail(AVBD_ERR_INVALID, "world is null");
     return world->eng.profileRaycast(0, n, from, to, reps, out);
+}
+
+avbd_status avbd_world_contact_report(avbd_world *world, const int32_t *max_reported, int32_t stride, int32_t *count,
+                                      float *position, float *normal, int32_t *other)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    return world->eng.contactReport(0, max_reported, stride, count, position, normal, other);
+}
+
+avbd_status avbd_batch_contact_report(avbd_batch *batch, int32_t world_id, const int32_t *max_reported, int32_t stride,
+                                      int32_t *count, float *position, float *normal, int32_t *other)
+{
+    if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
+    if (world_id < 0 || world_id >= batch->eng.Wcap) return fail(AVBD_ERR_INVALID, "bad world id");
+    return batch->eng.contactReport(world_id, max_reported, stride, count, position, normal, other);
 }
 
 avbd_status avbd_world_pick(avbd_world *world, int32_t n, const float *origin, const float *dir,

@@ -305,4 +305,71 @@ __global__ void k_collect_aabb(State S, int base, int count, const float4 *bound
                   ? 1 : 0;   // bounds from k_query_bounds
 }
 
+// ------------------------------------------------------------ contact reports ----
+// The server's per-body contact report (avbd_physics_server3d.cpp:1593-1652), from the live
+// manifolds of world w (contacts unpacked to the slot layout first).  maxRep[j] > 0 asks for
+// body j's first maxRep[j] contacts in live order.
+// 1) entries per asking body, 2) their positions (any order: each entry carries its live rank),
+// 3) per body, the entries taken in live-rank order until it holds its contacts.
+__global__ void k_report_count(State S, int w, const int *maxRep, uint32_t *cnt)
+{
+    const int k = GID;
+    if (k >= S.wLive[w]) return;
+    const int sb = S.wSlotBase[w], base = S.wBodyBase[w];
+    const int slot = S.live[sb + k];
+    if (S.mNum[slot] <= 0) return;
+    const int a = S.mA[slot] - base, b = S.mB[slot] - base;
+    if (maxRep[a] > 0) atomicAdd(cnt + a, 1u);
+    if (maxRep[b] > 0) atomicAdd(cnt + b, 1u);
+}
+__global__ void k_report_fill(State S, int w, const int *maxRep, const uint32_t *off, uint32_t *cursor, int *entries)
+{
+    const int k = GID;
+    if (k >= S.wLive[w]) return;
+    const int sb = S.wSlotBase[w], base = S.wBodyBase[w];
+    const int slot = S.live[sb + k];
+    if (S.mNum[slot] <= 0) return;
+    const int a = S.mA[slot] - base, b = S.mB[slot] - base;
+    if (maxRep[a] > 0) entries[off[a] + atomicAdd(cursor + a, 1u)] = 2 * k;
+    if (maxRep[b] > 0) entries[off[b] + atomicAdd(cursor + b, 1u)] = 2 * k + 1;
+}
+__global__ void k_report_emit(State S, int w, int n, const int *maxRep, int stride, const uint32_t *off,
+                              const int *entries, int *count, float4 *pos, float4 *normal)
+{
+    const int j = GID;
+    if (j >= n) return;
+    const int lim = maxRep[j] < stride ? maxRep[j] : stride;
+    int emitted = 0;
+    if (lim > 0) {
+        const int sb = S.wSlotBase[w], base = S.wBodyBase[w], i = base + j;
+        const v3 p = ld3(S.pos[i]);
+        const qt q = ldq(S.rot[i]);
+        const int e0 = static_cast<int>(off[j]), e1 = static_cast<int>(off[j + 1]);
+        int last = -1;
+        while (emitted < lim) {
+            int key = 0x7fffffff;
+            for (int e = e0; e < e1; e++) {
+                const int x = entries[e];
+                if (x > last && x < key) key = x;
+            }
+            if (key == 0x7fffffff) break;
+            last = key;
+            const int slot = S.live[sb + (key >> 1)], side = key & 1;
+            const int nc = S.mNum[slot];
+            const float4 b0 = S.mBasis[3 * slot];
+            const float sgn = side == 0 ? 1.0f : -1.0f;
+            const int otherBody = (side == 0 ? S.mB[slot] : S.mA[slot]) - base;
+            for (int c = 0; c < nc && emitted < lim; c++) {
+                const size_t row = static_cast<size_t>(slot) * MAXC + c;
+                const v3 arm = ld3(side == 0 ? S.sRA[row] : S.sRB[row]);
+                const size_t o = static_cast<size_t>(j) * stride + emitted;
+                pos[o] = st3(p + qrot(q, arm), __int_as_float(otherBody));
+                normal[o] = st3(ld3(b0) * sgn);
+                emitted++;
+            }
+        }
+    }
+    count[j] = emitted;
+}
+
 } // namespace avbd_dev

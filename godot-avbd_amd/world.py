@@ -98,6 +98,16 @@ class World:
                                            body.ctypes.data_as(abi.i32p), local.ctypes.data_as(abi.f32p)))
         return body, local
 
+    def contact_report(self, max_reported):
+        """Per-body contact reports: (count, position, normal, other), stride max(max_reported)."""
+        mr = np.ascontiguousarray(max_reported, np.int32)
+        stride = max(1, int(mr.max()) if len(mr) else 1)
+        count, pos, nrm, other = abi.report_arrays(self.n, stride)
+        abi.check(self.lib.avbd_world_contact_report(self.h, mr.ctypes.data_as(abi.i32p), stride,
+                                                     count.ctypes.data_as(abi.i32p), pos.ctypes.data_as(abi.f32p),
+                                                     nrm.ctypes.data_as(abi.f32p), other.ctypes.data_as(abi.i32p)))
+        return count, pos, nrm, other
+
     def collect_aabb(self, mn, mx, mask=0xFFFFFFFF):
         """Solver::collectAABB: list indices of the bodies whose bounds overlap [mn, mx]."""
         out = np.zeros(max(self.n, 1), np.int32)

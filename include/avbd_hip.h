@@ -292,6 +292,16 @@ typedef struct avbd_ray_hits {
  * Exclusion lists, areas and ray_pickable are the server's and stay there. */
 avbd_status avbd_world_raycast(avbd_world *world, int32_t n, const float *from, const float *to,
                                const uint32_t *masks, int32_t hit_from_inside, avbd_ray_hits *out);
+/* Per-body contact reports: replaces the server's contact collection after a step
+ * (src/server/avbd_physics_server3d.cpp:1593-1652, read through Solver::manifoldBodyA/B,
+ * manifoldContact and manifoldNormal, solver.h:193-220).  Body j asks for max_reported[j]
+ * contacts (0: none).  Live manifolds are walked in order; each side whose body asks takes the
+ * manifold's contacts in order until it holds its count: position[3(j*stride+c)..] = the
+ * body's own arm in world space at its current pose, normal = the manifold normal (first basis
+ * row) signed towards the body (+ for A, - for B), other = the other body's list index.
+ * count[j] = contacts written.  stride >= every max_reported[j]. */
+avbd_status avbd_world_contact_report(avbd_world *world, const int32_t *max_reported, int32_t stride, int32_t *count,
+                                      float *position, float *normal, int32_t *other);
 /* Measurement (no reference counterpart): average device time of the bounds kernel and of
  * the ray kernel of avbd_world_raycast over `reps` launches of n segments (all layers, no
  * hit from inside), timed with HIP events on the library's stream. */
@@ -367,6 +377,9 @@ avbd_status avbd_batch_remove_world(avbd_batch *batch, int32_t world_id);
 avbd_status avbd_batch_use_graph(avbd_batch *batch, int32_t enable);
 /* One step of every world in the batch. */
 avbd_status avbd_batch_step(avbd_batch *batch);
+/* avbd_world_contact_report for one world of a batch. */
+avbd_status avbd_batch_contact_report(avbd_batch *batch, int32_t world_id, const int32_t *max_reported, int32_t stride,
+                                      int32_t *count, float *position, float *normal, int32_t *other);
 /* avbd_world_raycast for one world of a batch. */
 avbd_status avbd_batch_raycast(avbd_batch *batch, int32_t world_id, int32_t n, const float *from, const float *to,
                                const uint32_t *masks, int32_t hit_from_inside, avbd_ray_hits *out);

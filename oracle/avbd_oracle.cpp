@@ -2366,6 +2366,39 @@ int oracle_pick(oracle_world *o, int n, const float *origin, const float *dir, c
 
 float oracle_atan2(float y, float x) { return atan2Poly(y, x); }
 
+// The server's per-body contact report (avbd_physics_server3d.cpp:1593-1652): live manifolds
+// in order, each side whose body asks for reports (maxRep > 0) takes the manifold's contacts in
+// order until it holds maxRep: the world point of its own arm at its current pose, the
+// manifold normal (first basis row) signed towards the body (+ for A, - for B), the other body.
+int oracle_contact_report(oracle_world *o, const int32_t *maxRep, int stride, int32_t *count, float *pos, float *normal,
+                          int32_t *other)
+{
+    const World &w = o->w;
+    const int n = static_cast<int>(w.bodies.size());
+    for (int i = 0; i < n; i++) count[i] = 0;
+    for (uint32_t s : w.live) {
+        const int nc = w.numC[s];
+        if (nc <= 0) continue;
+        const f3 nrm{w.basis[s][0][0], w.basis[s][1][0], w.basis[s][2][0]};
+        for (int side = 0; side < 2; side++) {
+            const int b = side == 0 ? w.mA[s] : w.mB[s];
+            const int lim = maxRep[b] < stride ? maxRep[b] : stride;
+            if (lim <= 0) continue;
+            const Body &me = w.bodies[b];
+            for (int c = 0; c < nc && count[b] < lim; c++) {
+                const Contact &ct = w.cont[static_cast<size_t>(s) * MAXC + c];
+                const f3 world = me.pos + qrot(me.q, side == 0 ? ct.rA : ct.rB);
+                const int k = b * stride + count[b];
+                store3(pos, k, world);
+                store3(normal, k, nrm * (side == 0 ? 1.0f : -1.0f));
+                other[k] = side == 0 ? w.mB[s] : w.mA[s];
+                count[b]++;
+            }
+        }
+    }
+    return AVBD_OK;
+}
+
 int oracle_get_manifolds(oracle_world *o, avbd_manifolds *m)
 {
     const World &w = o->w;

@@ -40,6 +40,8 @@ int oracle_raycast(oracle_world *o, int n, const float *from, const float *to, c
                    int hitFromInside, int32_t *body, float *t, float *point, float *normal, uint8_t *inside);
 int oracle_pick(oracle_world *o, int n, const float *origin, const float *dir, const uint32_t *masks, int32_t *body,
                 float *local);
+int oracle_contact_report(oracle_world *o, const int32_t *maxRep, int stride, int32_t *count, float *pos, float *normal,
+                          int32_t *other);
 int oracle_solve6(const float *a36, const float *b6, float *x6);
 int oracle_collide_boxes(const float posA[3], const float rotA[4], const float sizeA[3],
                          const float posB[3], const float rotB[4], const float sizeB[3],

@@ -76,6 +76,7 @@ def lib():
         "oracle_raycast": (C.c_int, [vp, C.c_int, a.f32p, a.f32p, a.u32p, C.c_int, a.i32p, a.f32p, a.f32p, a.f32p,
                                      a.u8p]),
         "oracle_pick": (C.c_int, [vp, C.c_int, a.f32p, a.f32p, a.u32p, a.i32p, a.f32p]),
+        "oracle_contact_report": (C.c_int, [vp, a.i32p, C.c_int, a.i32p, a.f32p, a.f32p, a.i32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -190,6 +191,20 @@ class OracleWorld:
                            None if m is None else m.ctypes.data_as(a.u32p), body.ctypes.data_as(a.i32p),
                            local.ctypes.data_as(a.f32p))
         return body, local
+
+    def contact_report(self, max_reported):
+        """Per-body contact reports: (count, position, normal, other) with a stride of
+        max(max_reported)."""
+        a = self.a
+        mr = np.ascontiguousarray(max_reported, np.int32)
+        stride = max(1, int(mr.max()) if len(mr) else 1)
+        count = np.zeros(self.n, np.int32)
+        pos = np.zeros((self.n, stride, 3), np.float32)
+        nrm = np.zeros((self.n, stride, 3), np.float32)
+        other = np.full((self.n, stride), -1, np.int32)
+        self.L.oracle_contact_report(self.h, mr.ctypes.data_as(a.i32p), stride, count.ctypes.data_as(a.i32p),
+                                     pos.ctypes.data_as(a.f32p), nrm.ctypes.data_as(a.f32p), other.ctypes.data_as(a.i32p))
+        return count, pos, nrm, other
 
     def step(self, n=1):
         for _ in range(n):

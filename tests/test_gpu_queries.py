@@ -86,3 +86,19 @@ def test_batch_raycast_equals_plain():
     g, r = batch.raycast(wid, frm, to), w.raycast(frm, to)
     assert np.array_equal(g["body"], r["body"]) and bits_equal(g["t"], r["t"]) and bits_equal(g["normal"], r["normal"])
     assert other != wid
+
+
+def test_contact_report_matches_oracle():
+    w, o, b = _worlds(90)
+    n = len(b["mass"])
+    rng = np.random.default_rng(9)
+    maxrep = rng.choice(np.array([0, 0, 1, 3, 8]), n).astype(np.int32)
+    maxrep[-1] = 16                      # the ground, touched by many manifolds
+    g = w.contact_report(maxrep)
+    r = o.contact_report(maxrep)
+    assert np.array_equal(g[0], r[0]) and g[0].sum() > 100
+    assert bits_equal(g[1], r[1]) and bits_equal(g[2], r[2]) and np.array_equal(g[3], r[3])
+    w.step(3)                            # reports leave the simulation alone
+    o.step(3)
+    from helpers import state_mismatch
+    assert not state_mismatch(w.download(), o.download())

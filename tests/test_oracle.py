@@ -835,3 +835,24 @@ def test_collect_aabb_against_brute_force():
             if np.all(p - wx <= c + h + 1e-4) and np.all(c - h <= p + wx + 1e-4):
                 loose.append(i)
         assert list(got) == sorted(got) and set(strict) <= set(got) <= set(loose)
+
+
+def test_contact_report_closed_form():
+    """The server's contact report (avbd_physics_server3d.cpp:1593-1652) on a box resting on a
+    slab: the box gets its four bottom corners with the normal pointing up out of the slab, the
+    slab (asking for two) its first two with the normal pointing down; a body asking for none
+    gets none; the other body is reported."""
+    s = scenes.SceneBuilder()
+    s.box((10, 10, 1), 0.0, 0.5, (0, 0, -0.5))
+    s.box((1, 1, 1), 1.0, 0.5, (0, 0, 0.5))
+    s.box((1, 1, 1), 1.0, 0.5, (3, 0, 0.5))
+    o = run(*s.build(), P(), 60)               # list order: right box, centre box, slab
+    count, pos, nrm, other = o.contact_report([0, 8, 2])
+    assert list(count) == [0, 4, 2]
+    assert np.allclose(pos[1, :4, 2], 0.0, atol=2e-3) and np.allclose(np.abs(pos[1, :4, :2]), 0.5, atol=1e-3)
+    assert np.allclose(nrm[1, :4], [0, 0, 1], atol=1e-5) and list(other[1, :4]) == [2] * 4
+    assert np.allclose(nrm[2, :2], [0, 0, -1], atol=1e-5) and set(other[2, :2]) <= {0, 1}
+    assert np.allclose(pos[2, :2, 2], 0.0, atol=2e-3)
+    m = o.manifolds()
+    first = [a if a != 2 else b for a, b in zip(m["body_a"], m["body_b"])][0]
+    assert other[2, 0] == first               # the slab's first contacts come from the first live manifold
