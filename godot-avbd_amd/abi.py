@@ -74,7 +74,8 @@ class Bodies(C.Structure):
         ("count", C.c_int32), ("size", f32p), ("mass", f32p), ("moment", f32p), ("friction", f32p),
         ("radius", f32p), ("gravity", f32p), ("layer", u32p), ("mask", u32p), ("lock_linear", u8p),
         ("lock_angular", u8p), ("position", f32p), ("rotation", f32p), ("velocity", f32p),
-        ("angular_velocity", f32p), ("prev_velocity", f32p), ("shape", u8p),
+        ("angular_velocity", f32p), ("prev_velocity", f32p), ("shape", u8p), ("sleep_mode", u8p),
+        ("sleeping", u8p), ("still_frames", i32p),
     ]
 
 
@@ -139,8 +140,9 @@ BODY_FIELDS = {
     "layer": (np.uint32, 1), "mask": (np.uint32, 1), "lock_linear": (np.uint8, 1),
     "lock_angular": (np.uint8, 1), "position": (np.float32, 3), "rotation": (np.float32, 4),
     "velocity": (np.float32, 3), "angular_velocity": (np.float32, 3), "prev_velocity": (np.float32, 3),
-    "shape": (np.uint8, 1),
+    "shape": (np.uint8, 1), "sleep_mode": (np.uint8, 1), "sleeping": (np.uint8, 1), "still_frames": (np.int32, 1),
 }
+SLEEP_NEVER, SLEEP_IDLE, SLEEP_START_ASLEEP = 0, 1, 2
 SHAPE_BOX, SHAPE_SPHERE, SHAPE_CYLINDER = 0, 1, 2
 JOINT_FIELDS = {
     "body_a": (np.int32, 1), "body_b": (np.int32, 1), "r_a": (np.float32, 3), "r_b": (np.float32, 3),
@@ -200,7 +202,7 @@ def bodies_struct(arrays):
     return _struct(Bodies, BODY_FIELDS, arrays, n)
 
 
-DYNAMIC_FIELDS = ("position", "rotation", "velocity", "angular_velocity", "prev_velocity")
+DYNAMIC_FIELDS = ("position", "rotation", "velocity", "angular_velocity", "prev_velocity", "sleeping", "still_frames")
 
 
 def dynamic_bodies_struct(arrays, count):

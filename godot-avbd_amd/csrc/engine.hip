@@ -103,6 +103,9 @@ void setParams(Params &P, const avbd_params &p, float collisionDt)
     P.adaptiveBeta = p.adaptive_beta;
     P.contactOnlyOverflow = p.contact_only_overflow;
     P.projIters = p.projection_iterations;
+    P.sleepFrames = p.sleep_frames;
+    P.sleepLin = p.sleep_linear_threshold;
+    P.sleepAng = p.sleep_angular_threshold;
 }
 
 avbd_status checkParams(const avbd_params *p)
@@ -110,7 +113,6 @@ avbd_status checkParams(const avbd_params *p)
     if (!p) return fail(AVBD_ERR_INVALID, "params is null");
     if (!(p->dt > 0.0f)) return fail(AVBD_ERR_INVALID, "dt must be positive");
     if (p->iterations < 1) return fail(AVBD_ERR_INVALID, "iterations must be >= 1");
-    if (p->sleep_frames > 0) return fail(AVBD_ERR_UNSUPPORTED, "sleeping (sleep_frames > 0) is outside this path");
     if (p->contact_grace_ticks > 0) return fail(AVBD_ERR_UNSUPPORTED, "contact grace ticks are outside this path");
     if (p->jacobian_rebuild_distance > 0.0f)
         return fail(AVBD_ERR_UNSUPPORTED, "jacobian arm caching is outside this path");
@@ -124,6 +126,7 @@ struct WorldInfo {
     int bodyBase = 0, bodyCount = 0, jointBase = 0, jointCount = 0, slotBase = 0, slotCap = 0;
     bool hasJoints = false;
     bool hasRound = false;   // a sphere or cylinder among its bodies
+    bool hasSleep = false;   // a body that can sleep (sleep mode) or sleeps
 };
 
 } // namespace
@@ -151,6 +154,7 @@ struct Engine {
     hipGraphExec_t graphs[2] = {nullptr, nullptr};  // [rebuild, refit]
     bool graphJoints = false;
     bool graphRound = false;
+    bool graphSleep = false;
     bool graphsValid = false;
     hipEvent_t ev[5] = {};
     avbd_step_stats stats{};
@@ -514,7 +518,8 @@ struct Engine {
         S.numWorlds = Wcap; S.numBodies = Ncap; S.slotCap = Scap; S.jointCap = Jcap; S.pairCap = Pcap;
         AL(S.pos, N); AL(S.rot, N); AL(S.initLin, N); AL(S.initAng, N); AL(S.inertLin, N); AL(S.inertAng, N);
         AL(S.vel, N); AL(S.angVel, N); AL(S.prevVel, N); AL(S.sizeR, N); AL(S.massI, N); AL(S.poseRec, 4 * N); AL(S.narrowRec, NARROW_REC * N);
-        AL(S.fric, N); AL(S.grav, N); AL(S.layer, N); AL(S.mask, N); AL(S.lockLin, N); AL(S.lockAng, N); AL(S.shape, N); AL(rimDev, 16); S.rim = rimDev;
+        AL(S.fric, N); AL(S.grav, N); AL(S.layer, N); AL(S.mask, N); AL(S.lockLin, N); AL(S.lockAng, N); AL(S.shape, N);
+        AL(S.sleepMode, N); AL(S.sleeping, N); AL(S.stillFrames, N); AL(S.updFlag, N + 1); AL(S.updScan, N + 1); AL(rimDev, 16); S.rim = rimDev;
         AL(S.bodyWorld, N); AL(S.updateIndex, N); AL(S.colour, N); AL(S.contactOnly, N);
         AL(S.dLin, N); AL(S.dAng, N); AL(S.aabbC, N); AL(S.aabbH, N);
         AL(S.fbCount, 1); AL(S.fbBody, N); AL(S.fbBlock, 33 * N);
@@ -643,6 +648,13 @@ struct Engine {
         k_adj_sort<<<nblk(n), TPB, 0, stream>>>(S);
     }
 
+    bool anySleep() const
+    {
+        for (const auto &w : worlds)
+            if (w.active && w.hasSleep) return true;
+        return false;
+    }
+
     bool anyRound() const
     {
         for (const auto &w : worlds)
@@ -699,6 +711,8 @@ struct Engine {
         const bool inter = substeps > 1;
         const float collisionDt = inter ? prm.dt * static_cast<float>(substeps) : prm.dt;
         setParams(S.P, prm, collisionDt);
+        const bool sleepOn = anySleep();
+        S.P.sleepOn = sleepOn ? 1 : 0;
         const int nb = nblk(Ncap);
         if (timed) (void)hipEventRecord(ev[0], stream);
         if (!csrValid) {
@@ -731,6 +745,7 @@ struct Engine {
         scan(S.pairCount, S.pairOff, Ncap);
         k_pair_emit<<<nb, TPB, 0, stream>>>(S, Ncap);
         k_pair_filter<<<liveGrid(), TPB, 0, stream>>>(S);
+        if (sleepOn) k_wake_pairs<<<nblk(Wcap), TPB, 0, stream>>>(S);
         scan(S.pairValid, S.pairValidScan, Pcap, S.pairOff + Ncap);
         k_alloc_new<<<liveGrid(), TPB, 0, stream>>>(S);
         k_alloc_counts<<<nblk(Wcap), TPB, 0, stream>>>(S);
@@ -757,6 +772,13 @@ struct Engine {
         pbeg(AVBD_K_ADJACENCY);
         buildAdjacency();
         pend();
+        if (sleepOn) {
+            // solver_solve.cpp:264-339: wake pass, then this step's update set and its ranks
+            k_wake_neighbours<<<nblk(Wcap), TPB, 0, stream>>>(S);
+            k_update_flags<<<nb, TPB, 0, stream>>>(S);
+            scan(S.updFlag, S.updScan, Ncap);
+            k_update_index<<<nb, TPB, 0, stream>>>(S);
+        }
         // ---- update set and colouring (solver_solve.cpp:263-424)
         k_colour_prepare<<<nblk(std::max(Ncap, Wcap)), TPB, 0, stream>>>(S);
         k_colour_joints<<<nblk(Jcap), TPB, 0, stream>>>(S);
@@ -844,7 +866,7 @@ struct Engine {
                 pend();
             }
             pbeg(AVBD_K_BODY);
-            k_finish<<<nb, TPB, 0, stream>>>(S);
+            k_finish<<<nb, TPB, 0, stream>>>(S, sub + 1 == subs ? 1 : 0);
             pend();
         }
         if (timed) (void)hipEventRecord(ev[3], stream);
@@ -1031,6 +1053,7 @@ struct Engine {
         }
         graphJoints = joints;
         graphRound = anyRound();
+        graphSleep = anySleep();
         graphsValid = true;
         return AVBD_OK;
     }
@@ -1051,7 +1074,7 @@ struct Engine {
             HIPCHK(hipGetLastError());
             return checkErrors();
         }
-        if (!graphsValid || graphJoints != anyJoints() || graphRound != anyRound()) {
+        if (!graphsValid || graphJoints != anyJoints() || graphRound != anyRound() || graphSleep != anySleep()) {
             avbd_status st = captureGraphs();
             if (st != AVBD_OK) return st;
         }
@@ -1072,10 +1095,12 @@ struct Engine {
         wi.slotCap = slotCap;
         wi.hasJoints = nj > 0;
         wi.hasRound = false;
+        wi.hasSleep = false;
         std::vector<float4> pos(n), rot(n), vel(n), ang(n), prev(n), sizeR(n), massI(n);
         std::vector<float> fric(n), grav(n);
         std::vector<uint32_t> layer(n), mask(n);
-        std::vector<uint8_t> ll(n), la(n), shp(n, AVBD_SHAPE_BOX);
+        std::vector<uint8_t> ll(n), la(n), shp(n, AVBD_SHAPE_BOX), smode(n, AVBD_SLEEP_NEVER), slp(n, 0);
+        std::vector<int> still(n, 0);
         std::vector<int> bw(n, w), ui(n, -1);
         int movable = 0;
         for (int i = 0; i < n; i++) {
@@ -1084,6 +1109,13 @@ struct Engine {
                 shp[i] = b->shape[i];
                 if (shp[i] != AVBD_SHAPE_BOX) wi.hasRound = true;
             }
+            if (b->sleep_mode) {
+                if (b->sleep_mode[i] > AVBD_SLEEP_START_ASLEEP) return fail(AVBD_ERR_INVALID, "unknown sleep mode");
+                smode[i] = b->sleep_mode[i];
+            }
+            if (b->sleeping) slp[i] = b->sleeping[i] ? 1 : 0;
+            if (b->still_frames) still[i] = b->still_frames[i];
+            if (smode[i] != AVBD_SLEEP_NEVER || slp[i]) wi.hasSleep = true;
             pos[i] = make_float4(b->position[3 * i], b->position[3 * i + 1], b->position[3 * i + 2], 0);
             rot[i] = make_float4(b->rotation[4 * i], b->rotation[4 * i + 1], b->rotation[4 * i + 2], b->rotation[4 * i + 3]);
             vel[i] = make_float4(b->velocity[3 * i], b->velocity[3 * i + 1], b->velocity[3 * i + 2], 0);
@@ -1110,6 +1142,7 @@ struct Engine {
         HIPCHK(put(S.sizeR, sizeR)); HIPCHK(put(S.massI, massI)); HIPCHK(put(S.fric, fric)); HIPCHK(put(S.grav, grav));
         HIPCHK(put(S.layer, layer)); HIPCHK(put(S.mask, mask)); HIPCHK(put(S.lockLin, ll)); HIPCHK(put(S.lockAng, la));
         HIPCHK(put(S.shape, shp));
+        HIPCHK(put(S.sleepMode, smode)); HIPCHK(put(S.sleeping, slp)); HIPCHK(put(S.stillFrames, still));
         {
             // collide.cpp:733-753: rim directions at angle steps i*5 mod 8, from the C library's
             // cosf / sinf at run time (the values the reference's table holds)
@@ -1233,6 +1266,15 @@ struct Engine {
         HIPCHK(put(S.vel, b->velocity, 3));
         HIPCHK(put(S.angVel, b->angular_velocity, 3));
         HIPCHK(put(S.prevVel, b->prev_velocity, 3));
+        if (b->sleeping) {
+            std::vector<uint8_t> slp(n);
+            for (int i = 0; i < n; i++) {
+                slp[i] = b->sleeping[i] ? 1 : 0;
+                if (slp[i]) worlds[w].hasSleep = true;
+            }
+            HIPCHK(hipMemcpy(S.sleeping + o, slp.data(), n, hipMemcpyHostToDevice));
+        }
+        if (b->still_frames) HIPCHK(hipMemcpy(S.stillFrames + o, b->still_frames, sizeof(int) * n, hipMemcpyHostToDevice));
         return AVBD_OK;
     }
 
@@ -1282,6 +1324,8 @@ struct Engine {
             HIPCHK(get(S.vel, b->velocity, 3));
             HIPCHK(get(S.angVel, b->angular_velocity, 3));
             HIPCHK(get(S.prevVel, b->prev_velocity, 3));
+            if (b->sleeping) HIPCHK(hipMemcpy(b->sleeping, S.sleeping + o, n, hipMemcpyDeviceToHost));
+            if (b->still_frames) HIPCHK(hipMemcpy(b->still_frames, S.stillFrames + o, sizeof(int) * n, hipMemcpyDeviceToHost));
         }
         if (j) {
             const int nj = wi.jointCount, jo = wi.jointBase;

@@ -950,7 +950,9 @@ template <bool ROUND> AD void manifoldInit(const State &S, int q, int P, NarrowP
     const float4 a0 = ra[0], a1 = ra[1], a2 = ra[2], a3 = ra[3], a4 = ra[4];
     const float4 b0 = rb[0], b1 = rb[1], b2 = rb[2], b3 = rb[3], b4 = rb[4];
     const float ma = a0.w, mb = b0.w;
-    const bool frozen = !(ma > 0.0f) && !(mb > 0.0f);
+    // manifold.cpp:168-172: parked when neither side is an awake movable body
+    const bool slA = S.P.sleepOn && S.sleeping[a], slB = S.P.sleepOn && S.sleeping[b];
+    const bool frozen = !(ma > 0.0f && !slA) && !(mb > 0.0f && !slB);
     const int shA = __float_as_int(ra[5].x), shB = __float_as_int(rb[5].x);
     const bool round = (shA | shB) != AVBD_SHAPE_BOX;
     if (ROUND && (frozen || !round)) return;
@@ -1551,9 +1553,11 @@ __global__ void k_warmstart(State S)
 // Pose records for the per-manifold passes (the displacement and rotation difference are the
 // primal/dual bracket terms, manifold.cpp:465-501, computed once per body instead of once per
 // manifold side; same floats).
+// mass slot: the body's mass while it is awake, 0 while it sleeps (the manifold passes treat a
+// sleeper as they treat a static body: manifold.cpp:168-172)
 __device__ __forceinline__ void writePose(const State &S, int i, v3 p, qt q)
 {
-    S.poseRec[4 * i] = st3(p - ld3(S.initLin[i]), bMass(S, i));
+    S.poseRec[4 * i] = st3(p - ld3(S.initLin[i]), S.P.sleepOn && S.sleeping[i] ? 0.0f : bMass(S, i));
     S.poseRec[4 * i + 1] = stq(q);
     S.poseRec[4 * i + 2] = st3(qsub(q, ldq(S.initAng[i])));
 }
@@ -1576,21 +1580,108 @@ __device__ __forceinline__ void jacobiApply(const State &S, int i, v3 dl, v3 da,
 }
 
 // solver_solve.cpp:146-176 (sleep off)
-__global__ void k_finish(State S)
+// solver_solve.cpp:143-176: BDF1 velocity (a sleeper keeps its own), locks, then on the tick's
+// last substep (`idle`) the idle count and falling asleep.
+__global__ void k_finish(State S, int idle)
 {
     const int i = XGID;
     if (i >= S.numBodies || S.bodyWorld[i] < 0) return;
     const float dt = S.P.dt;
     v3 vel = ld3(S.vel[i]), ang = ld3(S.angVel[i]);
     S.prevVel[i] = st3(vel);
-    if (bMass(S, i) > 0) {
+    const bool asleep = S.P.sleepOn && S.sleeping[i];
+    const bool movable = bMass(S, i) > 0;
+    if (movable && !asleep) {
         vel = (bPos(S, i) - ld3(S.initLin[i])) / dt;
         ang = qsub(bRot(S, i), ldq(S.initAng[i])) / dt;
     }
     maskLocked(S.lockLin[i], vel);
     maskLocked(S.lockAng[i], ang);
+    if (idle && S.P.sleepOn && S.P.sleepFrames > 0 && movable && !asleep && S.sleepMode[i] == AVBD_SLEEP_IDLE) {
+        const float lin2 = S.P.sleepLin * S.P.sleepLin, ang2 = S.P.sleepAng * S.P.sleepAng;
+        int still = S.stillFrames[i];
+        if (length2(vel) < lin2 && length2(ang) < ang2) ++still;
+        else still = 0;
+        if (still >= S.P.sleepFrames) {
+            S.sleeping[i] = 1;
+            vel = v3{0, 0, 0};
+            ang = v3{0, 0, 0};
+        }
+        S.stillFrames[i] = still;
+    }
     S.vel[i] = st3(vel);
     S.angVel[i] = st3(ang);
+}
+
+// ------------------------------------------------------------ sleeping ----
+// solver_broadphase.cpp:67-80 with :153-161: for each pair about to get its first manifold, in
+// pair order, a sleeper whose bounds touch an awake movable body wakes.  Serial per world (one
+// thread each): a body woken by one pair can wake the next.
+__device__ __forceinline__ void wakePair(const State &S, int a, int b)
+{
+    if (S.sleeping[a] && !S.sleeping[b] && bMass(S, b) > 0.0f) { S.sleeping[a] = 0; S.stillFrames[a] = 0; }
+    else if (S.sleeping[b] && !S.sleeping[a] && bMass(S, a) > 0.0f) { S.sleeping[b] = 0; S.stillFrames[b] = 0; }
+}
+__global__ void k_wake_pairs(State S)
+{
+    const int w = GID;
+    if (w >= S.numWorlds || S.wBodyCount[w] <= 0) return;
+    const int base = S.wBodyBase[w];
+    const uint32_t p0 = S.pairOff[base], p1 = S.pairOff[base + S.wBodyCount[w]];
+    for (uint32_t p = p0; p < p1; p++)
+        if (S.pairValid[p]) wakePair(S, S.pairI[p], S.pairJ[p]);
+}
+// solver_solve.cpp:264-289: with idle detection on, a sleeper (mode IDLE) sharing a force with
+// an awake movable body that moves faster than the thresholds wakes; bodies in list order,
+// serially per world.  (The island pass that follows in the reference only ever re-sleeps
+// islands whose members all sleep already, so it changes nothing and is not run.)
+__global__ void k_wake_neighbours(State S)
+{
+    const int w = GID;
+    if (w >= S.numWorlds || S.P.sleepFrames <= 0) return;
+    const int base = S.wBodyBase[w], cnt = S.wBodyCount[w];
+    const float lin2 = S.P.sleepLin * S.P.sleepLin, ang2 = S.P.sleepAng * S.P.sleepAng;
+    for (int i = base; i < base + cnt; i++) {
+        if (!S.sleeping[i] || S.sleepMode[i] != AVBD_SLEEP_IDLE) continue;
+        for (int e = S.adjStart[i]; e < S.adjStart[i + 1]; e++) {
+            const int o = otherOf(S, S.adj[e], i);
+            if (o < 0) continue;
+            if (bMass(S, o) > 0.0f && !S.sleeping[o] &&
+                (length2(ld3(S.vel[o])) > lin2 || length2(ld3(S.angVel[o])) > ang2)) {
+                S.sleeping[i] = 0;
+                S.stillFrames[i] = 0;
+                break;
+            }
+        }
+    }
+}
+// solver_solve.cpp:296-339: the update set (movable bodies that are awake; a START_ASLEEP body
+// at rest with no force stays out, else it wakes) ...
+__global__ void k_update_flags(State S)
+{
+    const int i = GID;
+    if (i >= S.numBodies) return;
+    uint32_t f = 0;
+    if (S.bodyWorld[i] >= 0 && bMass(S, i) > 0.0f) {
+        f = 1;
+        const uint8_t mode = S.sleepMode[i];
+        if (mode == AVBD_SLEEP_IDLE) {
+            if (S.sleeping[i]) f = 0;
+        } else if (mode == AVBD_SLEEP_START_ASLEEP) {
+            if (!S.sleeping[i] && length2(ld3(S.vel[i])) == 0.0f && S.adjStart[i + 1] == S.adjStart[i]) f = 0;
+            else S.sleeping[i] = 0;
+        }
+    }
+    S.updFlag[i] = f;
+}
+// ... ranked in list order within each world
+__global__ void k_update_index(State S)
+{
+    const int i = GID;
+    if (i >= S.numBodies) return;
+    const int w = S.bodyWorld[i];
+    if (w < 0) return;
+    S.updateIndex[i] = S.updFlag[i] ? static_cast<int>(S.updScan[i] - S.updScan[S.wBodyBase[w]]) : -1;
 }
 
 __device__ __forceinline__ void queueFallback(const State &S, int body, const Block &blk)

@@ -27,6 +27,10 @@ struct Params {
     float jacobiRelax, stickDriftHeal, maxPen, specMargin;
     float projBeta, projSlop, projMaxCorr, penaltyMax, adaptiveFloor, stiffnessDecay;
     int adaptiveBeta, contactOnlyOverflow, projIters;
+    // sleeping (solver_solve.cpp:143-339): sleepOn = some body can sleep or sleeps; sleepFrames
+    // = idle steps before sleeping (0: no idle detection); thresholds in m/s and rad/s
+    int sleepOn, sleepFrames;
+    float sleepLin, sleepAng;
 };
 
 constexpr int NARROW_REC = 8;
@@ -54,6 +58,9 @@ struct State {
     uint32_t *layer, *mask;
     uint8_t *lockLin, *lockAng;
     uint8_t *shape;       // AVBD_SHAPE_* per body
+    uint8_t *sleepMode, *sleeping;   // AVBD_SLEEP_* per body; 1 while asleep
+    int *stillFrames;                // steps below the sleep thresholds
+    uint32_t *updFlag, *updScan;     // per step: in the update set; its per-world ranks
     const float *rim;     // cylinder rim directions: 8 cosines then 8 sines (host libm values)
     int *bodyWorld;
     int *updateIndex;  // rank among movable bodies of the world (list order), -1 static

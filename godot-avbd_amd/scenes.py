@@ -130,6 +130,8 @@ class SceneBuilder:
         bodies["prev_velocity"] = bodies["velocity"].copy()
         if any(self.b[c].get("shape", 0) for c in order):
             bodies["shape"] = np.array([self.b[c].get("shape", 0) for c in order], np.uint8)
+        if any(self.b[c].get("sleep_mode", 0) for c in order):
+            bodies["sleep_mode"] = np.array([self.b[c].get("sleep_mode", 0) for c in order], np.uint8)
         nj = len(self.j)
         jorder = list(range(nj - 1, -1, -1))
         joints = None
@@ -366,6 +368,21 @@ def pile(n=8192, seed=2, grid=None, pitch=4.0, ground=400.0):
     bodies = {key: np.concatenate([gnd[key], boxes[key]])[::-1].copy() for key in gnd}
     bodies["prev_velocity"] = bodies["velocity"].copy()
     return bodies, None
+
+
+def sleepy_pile(n=240, seed=6, modes=(1, 1, 1, 0, 2)):
+    """A settling pile with sleep enabled: sleep modes cycle through `modes` (IDLE, NEVER,
+    START_ASLEEP), plus a late box held high above the pile (it falls when the pile sleeps and
+    wakes the sleepers it lands on) and two boxes created asleep."""
+    b, j = pile(n, seed=seed, pitch=2.2)
+    m = len(b["mass"])
+    b["sleep_mode"] = np.array([modes[k % len(modes)] if b["mass"][k] > 0 else 0 for k in range(m)], np.uint8)
+    b["sleeping"] = np.zeros(m, np.uint8)
+    b["sleeping"][[1, 4]] = 1
+    b["position"][0] = (0.3, 0.2, 40.0)     # the last-created box: high above the pile
+    b["velocity"][0] = (0.0, 0.0, 0.0)
+    b["prev_velocity"][0] = (0.0, 0.0, 0.0)
+    return b, j
 
 
 def round_pile(n=300, seed=4, pitch=2.2):

@@ -58,7 +58,8 @@ class World:
 
     def download(self, joints=False):
         out = abi.empty_arrays({k: abi.BODY_FIELDS[k] for k in
-                                ("position", "rotation", "velocity", "angular_velocity", "prev_velocity")}, self.n)
+                                ("position", "rotation", "velocity", "angular_velocity", "prev_velocity", "sleeping",
+                                 "still_frames")}, self.n)
         bs = abi.bodies_struct({**out, "mass": np.zeros(self.n, np.float32)})
         bs.mass = None
         bs.count = self.n
@@ -213,7 +214,8 @@ class Batch:
     def download(self, wid):
         n = self.sizes[wid]
         out = abi.empty_arrays({k: abi.BODY_FIELDS[k] for k in
-                                ("position", "rotation", "velocity", "angular_velocity", "prev_velocity")}, n)
+                                ("position", "rotation", "velocity", "angular_velocity", "prev_velocity", "sleeping",
+                                 "still_frames")}, n)
         bs = abi.bodies_struct({**out, "mass": np.zeros(n, np.float32)})
         bs.mass = None
         bs.count = n

@@ -92,7 +92,17 @@ typedef struct avbd_bodies {
     float *angular_velocity; /* 3n                                                   */
     float *prev_velocity;    /* 3n previous step's linear velocity (null: velocity)  */
     uint8_t *shape;          /* n AVBD_SHAPE_* (null: every body a box)              */
+    uint8_t *sleep_mode;     /* n AVBD_SLEEP_* (null: never)                         */
+    uint8_t *sleeping;       /* n in/out: 1 asleep (null: awake; download fills it)  */
+    int32_t *still_frames;   /* n in/out: steps below the thresholds (null: 0)       */
 } avbd_bodies;
+
+/* Sleep modes (reference `Rigid::sleep_mode`, src/avbd/types.hpp:138): never; sleep after
+ * params.sleep_frames idle steps and wake on contact or a moving neighbour; start asleep and
+ * wake on motion or forces. */
+#define AVBD_SLEEP_NEVER 0
+#define AVBD_SLEEP_IDLE 1
+#define AVBD_SLEEP_START_ASLEEP 2
 
 /* Body shapes (reference `ShapeType`, src/avbd/types.hpp:52-59).  `size` per shape: box =
  * full extents; sphere = (radius, unused, unused); cylinder = (radius, unused, height), its

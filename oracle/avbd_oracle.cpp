@@ -16,9 +16,8 @@
 // in the reference's floating-point evaluation order, serially (the reference's results do
 // not depend on its thread count: test/core_test.cpp:612-681).
 //
-// Out of scope (rejected with AVBD_ERR_UNSUPPORTED): sleeping (sleep_frames > 0), contact
-// grace ticks, jacobian arm caching, the convergence early exit, non-box shapes, springs
-// and 6-DOF joints.
+// Out of scope (rejected with AVBD_ERR_UNSUPPORTED): contact grace ticks, jacobian arm
+// caching, the convergence early exit.
 #include "avbd_oracle.h"
 
 #include <algorithm>
@@ -50,6 +49,7 @@ struct Body {
     f3 vel, angVel, prevVel;
     f3 size; float mass; f3 moment; float friction, radius, gravity;
     uint32_t layer, mask; uint8_t lockLin, lockAng; uint8_t shape = AVBD_SHAPE_BOX;
+    uint8_t sleepMode = AVBD_SLEEP_NEVER; bool sleeping = false; int stillFrames = 0;
     bool inUpdate = false; int updateIndex = -1; bool contactOnly = true;
 };
 
@@ -1106,6 +1106,14 @@ struct World {
         return false;
     }
 
+    // solver_broadphase.cpp:67-80: a sleeper whose bounds touch an awake movable body's
+    // (a pair about to get its first manifold) wakes, in pair order
+    static void wakePair(Body &a, Body &b)
+    {
+        if (a.sleeping && !b.sleeping && b.mass > 0.0f) { a.sleeping = false; a.stillFrames = 0; }
+        else if (b.sleeping && !a.sleeping && a.mass > 0.0f) { b.sleeping = false; b.stillFrames = 0; }
+    }
+
     void broadPhase()
     {
         buildAdjacency();
@@ -1129,6 +1137,7 @@ struct World {
         for (size_t k = 0; k < pairs.size(); k++) {
             if (!valid[k]) continue;
             const int i = static_cast<int>(pairs[k] >> 32), j = static_cast<int>(pairs[k] & 0xFFFFFFFFu);
+            wakePair(bodies[i], bodies[j]);
             bool exists = false;
             for (uint32_t e = adjStart[i]; e < adjStart[i + 1]; e++)
                 if (adj[e] >= 0 && otherOf(adj[e], i) == j) { exists = true; break; }
@@ -1275,7 +1284,7 @@ struct World {
 
     bool frozen(uint32_t s) const
     {
-        auto awake = [](const Body &b) { return b.mass > 0.0f; };
+        auto awake = [](const Body &b) { return b.mass > 0.0f && !b.sleeping; };
         return !awake(bodies[mA[s]]) && !awake(bodies[mB[s]]);
     }
 
@@ -1753,16 +1762,47 @@ struct World {
             if (colourBegins[c + 1] > colourBegins[c]) colourCount = c + 1;
     }
 
+    // solver_solve.cpp:264-339
     void selectUpdateBodies()
     {
         updateOrder.clear();
+        if (P.sleep_frames > 0) {
+            // a sleeper sharing a force with a moving, awake movable neighbour wakes (list order);
+            // the island pass that follows in the reference (solver_solve.cpp:177-258) only ever
+            // re-sleeps islands whose members all sleep already, so it changes nothing
+            const float lin2 = P.sleep_linear_threshold * P.sleep_linear_threshold;
+            const float ang2 = P.sleep_angular_threshold * P.sleep_angular_threshold;
+            for (size_t i = 0; i < bodies.size(); i++) {
+                Body &b = bodies[i];
+                if (!b.sleeping || b.sleepMode != AVBD_SLEEP_IDLE) continue;
+                for (uint32_t e = adjStart[i]; e < adjStart[i + 1]; e++) {
+                    const int o = otherOf(adj[e], static_cast<int>(i));
+                    if (o < 0) continue;
+                    const Body &x = bodies[o];
+                    if (x.mass > 0 && !x.sleeping && (length2(x.vel) > lin2 || length2(x.angVel) > ang2)) {
+                        b.sleeping = false;
+                        b.stillFrames = 0;
+                        break;
+                    }
+                }
+            }
+        }
         for (size_t i = 0; i < bodies.size(); i++) {
             Body &b = bodies[i];
             b.inUpdate = false; b.updateIndex = -1; b.contactOnly = true;
             if (b.mass > 0) {
-                b.updateIndex = static_cast<int>(updateOrder.size());
-                updateOrder.push_back(static_cast<int>(i));
-                b.inUpdate = true;
+                bool update = true;
+                if (b.sleepMode == AVBD_SLEEP_IDLE) {
+                    if (b.sleeping) update = false;
+                } else if (b.sleepMode == AVBD_SLEEP_START_ASLEEP) {
+                    if (!b.sleeping && length2(b.vel) == 0.0f && adjStart[i + 1] == adjStart[i]) update = false;
+                    else b.sleeping = false;
+                }
+                if (update) {
+                    b.updateIndex = static_cast<int>(updateOrder.size());
+                    updateOrder.push_back(static_cast<int>(i));
+                    b.inUpdate = true;
+                }
             }
         }
         dLin.assign(updateOrder.size(), f3{0, 0, 0});
@@ -1802,16 +1842,29 @@ struct World {
         }
     }
 
-    void finishBody(Body &b)
+    // solver_solve.cpp:143-176: BDF1 velocity (not for sleepers), then idle detection on
+    // the tick's last substep
+    void finishBody(Body &b, bool idle)
     {
         const float dt = P.dt;
         b.prevVel = b.vel;
-        if (b.mass > 0) {
+        if (b.mass > 0 && !b.sleeping) {
             b.vel = (b.pos - b.initLin) / dt;
             b.angVel = qsub(b.q, b.initAng) / dt;
         }
         maskLocked(b.lockLin, b.vel);
         maskLocked(b.lockAng, b.angVel);
+        if (idle && P.sleep_frames > 0 && b.mass > 0 && !b.sleeping && b.sleepMode == AVBD_SLEEP_IDLE) {
+            const float lin2 = P.sleep_linear_threshold * P.sleep_linear_threshold;
+            const float ang2 = P.sleep_angular_threshold * P.sleep_angular_threshold;
+            if (length2(b.vel) < lin2 && length2(b.angVel) < ang2) ++b.stillFrames;
+            else b.stillFrames = 0;
+            if (b.stillFrames >= P.sleep_frames) {
+                b.sleeping = true;
+                b.vel = f3{0, 0, 0};
+                b.angVel = f3{0, 0, 0};
+            }
+        }
     }
 
     Block assemble(int body, bool fromCache) const
@@ -2000,11 +2053,11 @@ struct World {
         for (uint32_t s : live) rebase(s);
     }
 
-    void runSubstep(int target)
+    void runSubstep(int target, bool idle)
     {
         for (Body &b : bodies) warmstartBody(b);
         iterate(target);
-        for (Body &b : bodies) finishBody(b);
+        for (Body &b : bodies) finishBody(b, idle);
     }
 
     void step()
@@ -2016,7 +2069,7 @@ struct World {
         projectContacts(false, 0.0f);
         const int target = P.iterations;
         roundsThisStep = std::max(1, target);
-        runSubstep(target);
+        runSubstep(target, true);
         projectContacts(P.position_projection != 0, 0.0f);
         rebaseAll();
     }
@@ -2034,7 +2087,7 @@ struct World {
         for (int sub = 0; sub < substeps; ++sub) {
             if (sub > 0) rebaseAll();
             projectContacts(false, P.dt * (substeps - sub));
-            runSubstep(rounds);
+            runSubstep(rounds, sub == substeps - 1);
         }
         projectContacts(P.position_projection != 0, collisionDt);
         rebaseAll();
@@ -2044,7 +2097,7 @@ struct World {
 int checkParams(const avbd_params *p)
 {
     if (!p) return AVBD_ERR_INVALID;
-    if (p->sleep_frames > 0 || p->contact_grace_ticks > 0 || p->jacobian_rebuild_distance > 0.0f ||
+    if (p->contact_grace_ticks > 0 || p->jacobian_rebuild_distance > 0.0f ||
         p->convergence_threshold > 0.0f)
         return AVBD_ERR_UNSUPPORTED;
     return AVBD_OK;
@@ -2099,6 +2152,10 @@ int oracle_upload(oracle_world *o, const avbd_bodies *b, const avbd_joints *j)
         x.lockAng = b->lock_angular ? b->lock_angular[i] : 0;
         x.shape = b->shape ? b->shape[i] : static_cast<uint8_t>(AVBD_SHAPE_BOX);
         if (x.shape > AVBD_SHAPE_CYLINDER) return AVBD_ERR_INVALID;
+        x.sleepMode = b->sleep_mode ? b->sleep_mode[i] : static_cast<uint8_t>(AVBD_SLEEP_NEVER);
+        if (x.sleepMode > AVBD_SLEEP_START_ASLEEP) return AVBD_ERR_INVALID;
+        x.sleeping = b->sleeping ? b->sleeping[i] != 0 : false;
+        x.stillFrames = b->still_frames ? b->still_frames[i] : 0;
         x.pos = load3(b->position, i);
         x.q = quat{b->rotation[4 * i], b->rotation[4 * i + 1], b->rotation[4 * i + 2], b->rotation[4 * i + 3]};
         x.vel = load3(b->velocity, i);
@@ -2168,6 +2225,8 @@ int oracle_write_bodies(oracle_world *o, const avbd_bodies *b)
         if (b->velocity) x.vel = load3(b->velocity, i);
         if (b->angular_velocity) x.angVel = load3(b->angular_velocity, i);
         if (b->prev_velocity) x.prevVel = load3(b->prev_velocity, i);
+        if (b->sleeping) x.sleeping = b->sleeping[i] != 0;
+        if (b->still_frames) x.stillFrames = b->still_frames[i];
     }
     return AVBD_OK;
 }
@@ -2200,6 +2259,8 @@ int oracle_download(oracle_world *o, avbd_bodies *b, avbd_joints *j)
             if (b->velocity) store3(b->velocity, k, x.vel);
             if (b->angular_velocity) store3(b->angular_velocity, k, x.angVel);
             if (b->prev_velocity) store3(b->prev_velocity, k, x.prevVel);
+            if (b->sleeping) b->sleeping[k] = x.sleeping ? 1 : 0;
+            if (b->still_frames) b->still_frames[k] = x.stillFrames;
         }
     }
     if (j) {

@@ -216,7 +216,8 @@ class OracleWorld:
     def download(self, joints=False):
         a = self.a
         out = a.empty_arrays({k: a.BODY_FIELDS[k] for k in
-                              ("position", "rotation", "velocity", "angular_velocity", "prev_velocity")}, self.n)
+                              ("position", "rotation", "velocity", "angular_velocity", "prev_velocity", "sleeping",
+                                 "still_frames")}, self.n)
         bs = a.bodies_struct({**out, "mass": np.zeros(self.n, np.float32)})
         bs.mass = None
         js = None

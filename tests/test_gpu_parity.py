@@ -443,3 +443,57 @@ def test_autotune_keeps_results():
         o.step()
     assert not state_mismatch(a.download(), b.download())
     assert not state_mismatch(a.download(), o.download())
+
+
+def _sleep_equal(g, o):
+    return np.array_equal(g["sleeping"], o["sleeping"]) and np.array_equal(g["still_frames"], o["still_frames"])
+
+
+def test_sleeping_lockstep():
+    """Island sleep and wake (idle detection, wake on first bounds contact, wake by a moving
+    neighbour, START_ASLEEP bodies, bodies uploaded asleep), bit for bit with the oracle,
+    the sleep state included."""
+    b, j = scenes.sleepy_pile()
+    p = P(iterations=10, sleep_frames=10)
+    w = gpu_world(b, j, p)
+    o = oracle.OracleWorld(p)
+    o.upload(b, j)
+    slept = 0
+    for s in range(300):
+        w.step()
+        o.step()
+        if (s + 1) % 20 == 0:
+            g, r = w.download(), o.download()
+            assert not state_mismatch(g, r), s
+            assert _sleep_equal(g, r), s
+            slept = max(slept, int(r["sleeping"].sum()))
+    manifolds_equal(w, o)
+    assert slept >= 10
+
+
+def test_sleeping_interleaved_and_batch():
+    p = P(iterations=10, sleep_frames=8)
+    p.dt = dt120()
+    worlds = [scenes.sleepy_pile(120, seed=7), scenes.pile(60, seed=8), scenes.sleepy_pile(90, seed=9, modes=(1,))]
+    batch = pkg.Batch(p, 4, 2000)
+    ids = [batch.add(b, j) for b, j in worlds]
+    alone = []
+    for b, j in worlds:
+        o = oracle.OracleWorld(p)
+        o.upload(b, j)
+        alone.append(o)
+    for s in range(200):
+        batch.step()
+        for o in alone:
+            o.step()
+        if (s + 1) % 25 == 0:
+            for wid, o in zip(ids, alone):
+                assert not state_mismatch(batch.download(wid), o.download()), (s, wid)
+    w = gpu_world(*worlds[0], p)
+    o = oracle.OracleWorld(p)
+    o.upload(*worlds[0])
+    for s in range(100):
+        w.step_interleaved(2, 0)
+        o.step_interleaved(2, 0)
+    g, r = w.download(), o.download()
+    assert not state_mismatch(g, r) and _sleep_equal(g, r)
