@@ -65,6 +65,9 @@ struct Rigid {
     uint32_t collisionMask = 1;
     uint8_t axisLockLinear = 0;
     uint8_t axisLockAngular = 0;
+    bool sleeping = false;   // types.hpp:97-100, 138: asleep; steps below the thresholds; 0 NEVER,
+    int stillFrames = 0;     // 1 SLEEP, 2 START_IN_SLEEP (set before the first step)
+    int sleep_mode = 0;
 
     inline Rigid(Solver *p_solver, float3 p_size, float p_density, float p_friction, float3 p_position,
                  float3 p_velocity = float3{0, 0, 0});
@@ -338,7 +341,8 @@ struct Solver {
     struct Arrays {
         std::vector<float> size, mass, moment, friction, radius, gravity, pos, rot, vel, ang, prev;
         std::vector<uint32_t> layer, mask;
-        std::vector<uint8_t> ll, la, shape;
+        std::vector<uint8_t> ll, la, shape, smode, asleep;
+        std::vector<int32_t> still;
         avbd_bodies view(int n)
         {
             avbd_bodies b{};
@@ -348,6 +352,7 @@ struct Solver {
             b.lock_linear = ll.data(); b.lock_angular = la.data(); b.position = pos.data(); b.rotation = rot.data();
             b.velocity = vel.data(); b.angular_velocity = ang.data(); b.prev_velocity = prev.data();
             b.shape = shape.data();
+            b.sleep_mode = smode.data(); b.sleeping = asleep.data(); b.still_frames = still.data();
             return b;
         }
     };
@@ -358,10 +363,13 @@ struct Solver {
         a.size.resize(3 * n); a.mass.resize(n); a.moment.resize(3 * n); a.friction.resize(n); a.radius.resize(n);
         a.gravity.resize(n); a.pos.resize(3 * n); a.rot.resize(4 * n); a.vel.resize(3 * n); a.ang.resize(3 * n);
         a.prev.resize(3 * n); a.layer.resize(n); a.mask.resize(n); a.ll.resize(n); a.la.resize(n);
-        a.shape.resize(n);
+        a.shape.resize(n); a.smode.resize(n); a.asleep.resize(n); a.still.resize(n);
         for (size_t i = 0; i < n; i++) {
             const Rigid *r = list[i];
             a.shape[i] = static_cast<uint8_t>(r->shape);
+            a.smode[i] = static_cast<uint8_t>(r->sleep_mode);
+            a.asleep[i] = r->sleeping ? 1 : 0;
+            a.still[i] = r->stillFrames;
             for (int c = 0; c < 3; c++) {
                 a.size[3 * i + c] = r->size[c];
                 a.moment[3 * i + c] = r->moment[c];
@@ -541,6 +549,9 @@ struct Solver {
         b.count = n;
         b.position = pos.data(); b.rotation = rot.data(); b.velocity = vel.data();
         b.angular_velocity = ang.data(); b.prev_velocity = prev.data();
+        std::vector<uint8_t> asleep(n);
+        std::vector<int32_t> still(n);
+        b.sleeping = asleep.data(); b.still_frames = still.data();
         const int nj = static_cast<int>(jl.size());
         std::vector<int32_t> ja(nj), jb(nj);
         std::vector<float> c0l(3 * nj), c0a(3 * nj), pl(3 * nj), pa(3 * nj), ll(3 * nj), la(3 * nj);
@@ -560,6 +571,8 @@ struct Solver {
                 r->prevVelocityLin[c] = prev[3 * i + c];
             }
             r->positionAng = quat{rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]};
+            r->sleeping = asleep[i] != 0;
+            r->stillFrames = still[i];
         }
         // the device compacted the reaped joints away; drop them from the host list too
         for (Force *dead : reaped) {

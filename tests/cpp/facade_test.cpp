@@ -263,6 +263,26 @@ static void test_queries()
            "pick and collectAABB", d);
 }
 
+// core_test.cpp:1645-1659: a two-box stack with sleep_mode SLEEP and sleepFrames 5 is asleep
+// after 120 steps and stays finite; the sleepers keep zero velocity.
+static void test_idle_sleep()
+{
+    Solver s;
+    s.sleepFrames = 5;
+    new Rigid(&s, {100, 100, 1}, 0.0f, 0.5f, {0, 0, -0.5f});
+    Rigid *a = new Rigid(&s, {1, 1, 1}, 1.0f, 0.5f, {0, 0, 0.5f});
+    Rigid *b = new Rigid(&s, {1, 1, 1}, 1.0f, 0.5f, {0, 0, 1.5f});
+    a->sleep_mode = 1;
+    b->sleep_mode = 1;
+    for (int i = 0; i < 120; i++) s.step();
+    char d[160];
+    std::snprintf(d, sizeof(d), "a=%d b=%d still %d %d, b z %.4f vz %.1f", a->sleeping, b->sleeping, a->stillFrames,
+                  b->stillFrames, b->positionLin.z, b->velocityLin.z);
+    report(a->sleeping && b->sleeping && std::isfinite(b->positionLin.z) && b->velocityLin.z == 0.0f &&
+               s.lastStatus == AVBD_OK,
+           "idle bodies fall asleep", d);
+}
+
 int main()
 {
     const uint64_t want[3] = {0xfbac4f08cb1c977cull, 0xb03adeb37742850aull, 0xfb3ca7a3162035c2ull};
@@ -279,6 +299,7 @@ int main()
     test_generic_joint();
     test_round_shapes();
     test_queries();
+    test_idle_sleep();
     std::printf("%d checks, %d failures\n", g_checks, g_failures);
     return g_failures == 0 ? 0 : 1;
 }

@@ -19,4 +19,4 @@ def test_facade_program():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "10 checks, 0 failures" in r.stdout
+    assert "11 checks, 0 failures" in r.stdout
