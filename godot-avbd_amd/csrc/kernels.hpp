@@ -242,7 +242,8 @@ __global__ void k_swept_aabb(State S)
     r[2] = sr;
     r[3] = make_float4(v4.x, v4.y, v4.z, S.grav[i]);
     r[4] = make_float4(w4.x, w4.y, w4.z, S.fric[i]);
-    r[5] = make_float4(__int_as_float(shape), 0.0f, 0.0f, 0.0f);
+    // w: the body's awake mass (0 while it sleeps; k_wake_pairs refreshes it for bodies it wakes)
+    r[5] = make_float4(__int_as_float(shape), 0.0f, 0.0f, S.P.sleepOn && S.sleeping[i] ? 0.0f : bMass(S, i));
 }
 
 // Scene bounds of each world's AABB centres (bvh_broadphase.cpp:332-342); one block per
@@ -947,13 +948,12 @@ template <bool ROUND> AD void manifoldInit(const State &S, int q, int P, NarrowP
     }
     const float4 *ra = S.narrowRec + NARROW_REC * static_cast<size_t>(a);
     const float4 *rb = S.narrowRec + NARROW_REC * static_cast<size_t>(b);
-    const float4 a0 = ra[0], a1 = ra[1], a2 = ra[2], a3 = ra[3], a4 = ra[4];
-    const float4 b0 = rb[0], b1 = rb[1], b2 = rb[2], b3 = rb[3], b4 = rb[4];
+    const float4 a0 = ra[0], a1 = ra[1], a2 = ra[2], a3 = ra[3], a4 = ra[4], a5 = ra[5];
+    const float4 b0 = rb[0], b1 = rb[1], b2 = rb[2], b3 = rb[3], b4 = rb[4], b5 = rb[5];
     const float ma = a0.w, mb = b0.w;
     // manifold.cpp:168-172: parked when neither side is an awake movable body
-    const bool slA = S.P.sleepOn && S.sleeping[a], slB = S.P.sleepOn && S.sleeping[b];
-    const bool frozen = !(ma > 0.0f && !slA) && !(mb > 0.0f && !slB);
-    const int shA = __float_as_int(ra[5].x), shB = __float_as_int(rb[5].x);
+    const bool frozen = !(a5.w > 0.0f) && !(b5.w > 0.0f);
+    const int shA = __float_as_int(a5.x), shB = __float_as_int(b5.x);
     const bool round = (shA | shB) != AVBD_SHAPE_BOX;
     if (ROUND && (frozen || !round)) return;
     if (!ROUND) {
@@ -1617,10 +1617,16 @@ __global__ void k_finish(State S, int idle)
 // solver_broadphase.cpp:67-80 with :153-161: for each pair about to get its first manifold, in
 // pair order, a sleeper whose bounds touch an awake movable body wakes.  Serial per world (one
 // thread each): a body woken by one pair can wake the next.
+__device__ __forceinline__ void wakeBody(const State &S, int a)
+{
+    S.sleeping[a] = 0;
+    S.stillFrames[a] = 0;
+    S.narrowRec[NARROW_REC * static_cast<size_t>(a) + 5].w = bMass(S, a);   // awake mass for the narrow phase
+}
 __device__ __forceinline__ void wakePair(const State &S, int a, int b)
 {
-    if (S.sleeping[a] && !S.sleeping[b] && bMass(S, b) > 0.0f) { S.sleeping[a] = 0; S.stillFrames[a] = 0; }
-    else if (S.sleeping[b] && !S.sleeping[a] && bMass(S, a) > 0.0f) { S.sleeping[b] = 0; S.stillFrames[b] = 0; }
+    if (S.sleeping[a] && !S.sleeping[b] && bMass(S, b) > 0.0f) wakeBody(S, a);
+    else if (S.sleeping[b] && !S.sleeping[a] && bMass(S, a) > 0.0f) wakeBody(S, b);
 }
 __global__ void k_wake_pairs(State S)
 {
