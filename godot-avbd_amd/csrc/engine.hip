@@ -351,6 +351,43 @@ struct Engine {
         return AVBD_OK;
     }
 
+    int *qOut = nullptr;
+    size_t qOutCap = 0;
+
+    avbd_status intersectPoint(int w, int n, const float *points, const uint32_t *masks, int maxRes, int32_t *count,
+                               int32_t *out)
+    {
+        const WorldInfo &wi = worlds[w];
+        if (!wi.active) return fail(AVBD_ERR_INVALID, "world is not active");
+        if (n < 0 || maxRes < 1 || (n > 0 && (!points || !count || !out))) return fail(AVBD_ERR_INVALID, "bad arguments");
+        if (n == 0) return AVBD_OK;
+        avbd_status st = ensureQuery(n);
+        if (st != AVBD_OK) return st;
+        const size_t outN = static_cast<size_t>(n) * maxRes;
+        if (outN > qOutCap) {
+            qOutCap = std::max(outN, 2 * qOutCap);
+            if ((st = regrow(qOut, qOutCap)) != AVBD_OK) return st;
+        }
+        std::vector<float4> pts(n);
+        for (int r = 0; r < n; r++) {
+            const uint32_t m = masks ? masks[r] : 0xFFFFFFFFu;
+            float mf;
+            std::memcpy(&mf, &m, sizeof(mf));
+            pts[r] = make_float4(points[3 * r], points[3 * r + 1], points[3 * r + 2], mf);
+        }
+        HIPCHK(hipMemcpyAsync(qRays, pts.data(), sizeof(float4) * n, hipMemcpyHostToDevice, stream));
+        if (wi.bodyCount > 0) k_query_bounds<<<nblk(wi.bodyCount), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, qBounds);
+        k_intersect_point<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qRays, qBounds, maxRes, qBody, qOut);
+        HIPCHK(hipGetLastError());
+        std::vector<int> o(outN);
+        HIPCHK(hipMemcpyAsync(count, qBody, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(o.data(), qOut, sizeof(int) * outN, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int r = 0; r < n; r++)
+            for (int c = 0; c < maxRes; c++) out[static_cast<size_t>(r) * maxRes + c] = c < count[r] ? o[static_cast<size_t>(r) * maxRes + c] : -1;
+        return AVBD_OK;
+    }
+
     avbd_status pick(int w, int n, const float *origin, const float *dir, const uint32_t *masks, int32_t *body,
                      float *local)
     {
@@ -2052,6 +2089,13 @@ avbd_status avbd_world_profile_raycast(avbd_world *world, int32_t n, const float
 {
     if (!world) return fail(AVBD_ERR_INVALID, "world is null");
     return world->eng.profileRaycast(0, n, from, to, reps, out);
+}
+
+avbd_status avbd_world_intersect_point(avbd_world *world, int32_t n, const float *points, const uint32_t *masks,
+                                       int32_t max_results, int32_t *count, int32_t *out)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    return world->eng.intersectPoint(0, n, points, masks, max_results, count, out);
 }
 
 avbd_status avbd_world_contact_report(avbd_world *world, const int32_t *max_reported, int32_t stride, int32_t *count,

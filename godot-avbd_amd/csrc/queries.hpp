@@ -232,6 +232,34 @@ __global__ void k_raycast(State S, int base, int count, int n, const float4 *ray
     hitB[r] = st3(bh.normal, bh.inside ? 1.0f : 0.0f);
 }
 
+// The bodies part of the server's intersect_point (avbd_direct_space_state3d.cpp:225-259):
+// pts[r] = (point, mask bits); candidates are the bodies whose bounds overlap the point's box
+// padded by 1e-4, in list order; out[r * maxRes + c] = the c-th body containing the point
+// (query::containsPoint, query_shape.cpp:334-337), up to maxRes.
+__global__ void k_intersect_point(State S, int base, int count, int n, const float4 *pts, const float4 *bounds,
+                                  int maxRes, int *cnt, int *out)
+{
+    const int r = GID;
+    if (r >= n) return;
+    const float4 p4 = pts[r];
+    const v3 p = ld3(p4);
+    const uint32_t mask = __float_as_uint(p4.w);
+    const float pad = 1.0e-4f;
+    const v3 mn{p.x - pad, p.y - pad, p.z - pad}, mx{p.x + pad, p.y + pad, p.z + pad};
+    int c = 0;
+    for (int j = 0; j < count && c < maxRes; j++) {
+        const float4 lo = bounds[2 * j], hi = bounds[2 * j + 1];
+        if ((__float_as_uint(lo.w) & mask) == 0) continue;
+        if (!boxesOverlap(ld3(lo), ld3(hi), mn, mx)) continue;
+        const int i = base + j;
+        const RShp s = makeRShp(ld3(S.pos[i]), ldq(S.rot[i]), S.sizeR[i]);
+        if (!insideLocal(S.shape[i], s, qrot(qconj(s.q), p - s.c), 0.0f)) continue;
+        out[static_cast<size_t>(r) * maxRes + c] = j;
+        c++;
+    }
+    cnt[r] = c;
+}
+
 // Solver::pick (solver_query.cpp:29-155): rays[2r] = (origin, mask bits), rays[2r+1] = dir.
 // Every body that passes the mask: an exact sphere test, else the box bounding the shape; the
 // nearest t (the earlier body on a tie).  local[r] = hit point in the body's frame.

@@ -99,6 +99,17 @@ class World:
                                            body.ctypes.data_as(abi.i32p), local.ctypes.data_as(abi.f32p)))
         return body, local
 
+    def intersect_point(self, points, masks=None, max_results=8):
+        """The server's intersect_point for n points: (count, body indices n x max_results)."""
+        pts = np.ascontiguousarray(points, np.float32)
+        n = len(pts)
+        count, out = np.zeros(n, np.int32), np.full((n, max_results), -1, np.int32)
+        m = None if masks is None else np.ascontiguousarray(masks, np.uint32)
+        abi.check(self.lib.avbd_world_intersect_point(self.h, n, pts.ctypes.data_as(abi.f32p),
+                                                      None if m is None else m.ctypes.data_as(abi.u32p), max_results,
+                                                      count.ctypes.data_as(abi.i32p), out.ctypes.data_as(abi.i32p)))
+        return count, out
+
     def contact_report(self, max_reported):
         """Per-body contact reports: (count, position, normal, other), stride max(max_reported)."""
         mr = np.ascontiguousarray(max_reported, np.int32)

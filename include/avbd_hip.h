@@ -321,6 +321,13 @@ typedef struct avbd_query_profile {
 } avbd_query_profile;
 avbd_status avbd_world_profile_raycast(avbd_world *world, int32_t n, const float *from, const float *to,
                                        int32_t reps, avbd_query_profile *out);
+/* n points against the bodies at their current pose: replaces the bodies part of the server's
+ * intersect_point (src/server/avbd_direct_space_state3d.cpp:225-259 with query::containsPoint,
+ * query_shape.cpp:334-337).  Candidates are the bodies whose layer passes masks[r] (null: all)
+ * and whose bounds overlap the point's box padded by 1e-4, in list order; out[r*max_results + c]
+ * = the list index of the c-th body containing point r (-1 past count[r]). */
+avbd_status avbd_world_intersect_point(avbd_world *world, int32_t n, const float *points, const uint32_t *masks,
+                                       int32_t max_results, int32_t *count, int32_t *out);
 /* Replaces `Solver::pick(origin, dir, local, mask)` (src/avbd/solver.h:161-164,
  * solver_query.cpp:29-155) for n rays: body[r] = list index of the nearest body (-1 none),
  * local[3r..] = the hit point in its frame (zeros for none); masks null = all layers. */

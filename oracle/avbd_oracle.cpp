@@ -834,6 +834,11 @@ bool rayCylinder(const RShape &s, f3 o, f3 d, float maxT, RayHit &h)
     h = RayHit{t, s.c + qrot(s.q, p), qrot(s.q, cylinderNormal(p, hh)), inside};
     return true;
 }
+// query_shape.cpp:334-337
+bool containsPoint(const RShape &s, f3 p, float margin)
+{
+    return insideLocal(s, qrot(qconj(s.q), p - s.c), margin);
+}
 // query_shape.cpp:258-274
 bool intersectRay(const RShape &s, f3 origin, f3 dir, float rayMaxT, RayHit &h)
 {
@@ -2362,6 +2367,39 @@ int oracle_raycast(oracle_world *o, int n, const float *from, const float *to, c
             body[r] = cand[k]; tOut[r] = h.t; store3(point, r, h.point); store3(normal, r, h.normal);
             inside[r] = 0;
             valid = true;
+        }
+    }
+    return AVBD_OK;
+}
+
+// query::containsPoint on one shape built from a body's size (makeShape's fields).
+int oracle_contains_point(int type, const float pos[3], const float rot[4], const float size[3], const float point[3],
+                          float margin)
+{
+    const RShape s = makeRShape(type, f3{pos[0], pos[1], pos[2]}, quat{rot[0], rot[1], rot[2], rot[3]},
+                                f3{size[0], size[1], size[2]});
+    return containsPoint(s, f3{point[0], point[1], point[2]}, margin) ? 1 : 0;
+}
+
+// The bodies part of the server's intersect_point (avbd_direct_space_state3d.cpp:225-259) for n
+// points: candidates from collectAABB over the point's box padded by 1e-4, in list order; the
+// bodies whose shape contains the point, up to maxResults per point.
+int oracle_intersect_point(oracle_world *o, int n, const float *points, const uint32_t *masks, int maxResults,
+                           int32_t *count, int32_t *out)
+{
+    World &w = o->w;
+    std::vector<int32_t> cand(w.bodies.size() + 1);
+    for (int r = 0; r < n; r++) {
+        count[r] = 0;
+        const f3 p = load3(points, r);
+        const float pad = 1.0e-4f;
+        const float mn[3] = {p.x - pad, p.y - pad, p.z - pad}, mx[3] = {p.x + pad, p.y + pad, p.z + pad};
+        const int nc = oracle_collect_aabb(o, mn, mx, masks ? masks[r] : 0xFFFFFFFFu, cand.data(),
+                                           static_cast<int>(cand.size()));
+        for (int k = 0; k < nc && count[r] < maxResults; k++) {
+            const Body &b = w.bodies[cand[k]];
+            if (!containsPoint(makeRShape(b.shape, b.pos, b.q, b.size), p, 0.0f)) continue;
+            out[static_cast<size_t>(r) * maxResults + count[r]++] = cand[k];
         }
     }
     return AVBD_OK;

@@ -42,6 +42,10 @@ int oracle_pick(oracle_world *o, int n, const float *origin, const float *dir, c
                 float *local);
 int oracle_contact_report(oracle_world *o, const int32_t *maxRep, int stride, int32_t *count, float *pos, float *normal,
                           int32_t *other);
+int oracle_contains_point(int type, const float pos[3], const float rot[4], const float size[3], const float point[3],
+                          float margin);
+int oracle_intersect_point(oracle_world *o, int n, const float *points, const uint32_t *masks, int maxResults,
+                           int32_t *count, int32_t *out);
 int oracle_solve6(const float *a36, const float *b6, float *x6);
 int oracle_collide_boxes(const float posA[3], const float rotA[4], const float sizeA[3],
                          const float posB[3], const float rotB[4], const float sizeB[3],

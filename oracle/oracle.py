@@ -77,6 +77,8 @@ def lib():
                                      a.u8p]),
         "oracle_pick": (C.c_int, [vp, C.c_int, a.f32p, a.f32p, a.u32p, a.i32p, a.f32p]),
         "oracle_contact_report": (C.c_int, [vp, a.i32p, C.c_int, a.i32p, a.f32p, a.f32p, a.i32p]),
+        "oracle_contains_point": (C.c_int, [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, C.c_float]),
+        "oracle_intersect_point": (C.c_int, [vp, C.c_int, a.f32p, a.u32p, C.c_int, a.i32p, a.i32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -104,6 +106,8 @@ def ref_lib():
     R.ref_intersect_ray.restype = C.c_int
     R.ref_intersect_ray.argtypes = [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, a.f32p, C.c_float, a.f32p, a.f32p,
                                     a.f32p, a.u8p]
+    R.ref_contains_point.restype = C.c_int
+    R.ref_contains_point.argtypes = [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, C.c_float]
     R.ref_solve6.restype = C.c_int
     R.ref_solve6.argtypes = [a.f32p, a.f32p, a.f32p]
     R.ref_morton3D.restype = C.c_uint64
@@ -191,6 +195,17 @@ class OracleWorld:
                            None if m is None else m.ctypes.data_as(a.u32p), body.ctypes.data_as(a.i32p),
                            local.ctypes.data_as(a.f32p))
         return body, local
+
+    def intersect_point(self, points, masks=None, max_results=8):
+        """The server's intersect_point for n points: (count, body indices n x max_results)."""
+        a = self.a
+        pts = np.ascontiguousarray(points, np.float32)
+        n = len(pts)
+        count, out = np.zeros(n, np.int32), np.full((n, max_results), -1, np.int32)
+        m = None if masks is None else np.ascontiguousarray(masks, np.uint32)
+        self.L.oracle_intersect_point(self.h, n, pts.ctypes.data_as(a.f32p), None if m is None else m.ctypes.data_as(a.u32p),
+                                      max_results, count.ctypes.data_as(a.i32p), out.ctypes.data_as(a.i32p))
+        return count, out
 
     def contact_report(self, max_reported):
         """Per-body contact reports: (count, position, normal, other) with a stride of
@@ -349,6 +364,22 @@ def _ray(fn, t, pos, rot, size, origin, direction, max_t):
 def intersect_ray(t, pos, rot, size, origin, direction, max_t):
     """query::intersectRay on one shape: (hit, t, point, normal, inside)."""
     return _ray(lib().oracle_intersect_ray, t, pos, rot, size, origin, direction, max_t)
+
+
+def _contains(fn, t, pos, rot, size, point, margin):
+    a = _abi()
+    args = [np.ascontiguousarray(x, np.float32) for x in (pos, rot, size, point)]
+    return bool(fn(int(t), *[x.ctypes.data_as(a.f32p) for x in args], float(margin)))
+
+
+def contains_point(t, pos, rot, size, point, margin=0.0):
+    """query::containsPoint on one shape."""
+    return _contains(lib().oracle_contains_point, t, pos, rot, size, point, margin)
+
+
+def ref_contains_point(t, pos, rot, size, point, margin=0.0):
+    R = ref_lib()
+    return None if R is None else _contains(R.ref_contains_point, t, pos, rot, size, point, margin)
 
 
 def ref_intersect_ray(t, pos, rot, size, origin, direction, max_t):

@@ -68,6 +68,22 @@ extern "C" int ref_solve6(const float *a36, const float *b6, float *x6)
     return path;
 }
 
+// query::containsPoint on a shape built from a body's size as the solver's makeShape does.
+extern "C" int ref_contains_point(int type, const float pos[3], const float rot[4], const float size[3],
+                                  const float point[3], float margin)
+{
+    using namespace avbd;
+    Shape sh;
+    sh.type = static_cast<ShapeType>(type);
+    sh.center = float3{pos[0], pos[1], pos[2]};
+    sh.rotation = quat{rot[0], rot[1], rot[2], rot[3]};
+    sh.half = float3{size[0], size[1], size[2]} * 0.5f;
+    sh.radius = size[0];
+    sh.halfHeight = size[2] * 0.5f;
+    sh.axis = qrot(sh.rotation, float3{0, 0, 1});
+    return query::containsPoint(sh, float3{point[0], point[1], point[2]}, margin) ? 1 : 0;
+}
+
 // query::intersectRay on a shape built from a body's size as the solver's makeShape does.
 extern "C" int ref_intersect_ray(int type, const float pos[3], const float rot[4], const float size[3],
                                  const float origin[3], const float dir[3], float maxT, float *t, float point[3],

@@ -102,3 +102,16 @@ def test_contact_report_matches_oracle():
     o.step(3)
     from helpers import state_mismatch
     assert not state_mismatch(w.download(), o.download())
+
+
+def test_intersect_point_matches_oracle():
+    w, o, b = _worlds()
+    rng = np.random.default_rng(10)
+    centres = w.download()["position"]
+    pts = np.concatenate([centres + rng.normal(0, 0.3, centres.shape), rng.uniform(-10, 10, (700, 3))]).astype(np.float32)
+    masks = rng.choice(np.array([1, 2, 0xFFFFFFFF], np.uint32), len(pts))
+    for k in (1, 4):
+        gc, go = w.intersect_point(pts, masks, max_results=k)
+        oc, oo = o.intersect_point(pts, masks, max_results=k)
+        assert np.array_equal(gc, oc) and np.array_equal(go, oo)
+    assert gc.sum() > 80

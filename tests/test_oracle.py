@@ -961,3 +961,37 @@ def test_moving_neighbour_wakes_sleeper_and_start_asleep():
     o.write_bodies({"velocity": v})
     o.step(10)
     assert o.download()["position"][0][2] < 5.9
+
+
+def test_contains_point_matches_reference_live():
+    if oracle.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(33)
+    inside = 0
+    for _ in range(4000):
+        t = int(rng.integers(0, 3))
+        size = (rng.random(3) * 1.5 + 0.2).astype(np.float32)
+        q = rng.standard_normal(4)
+        q = (q / np.linalg.norm(q)).astype(np.float32)
+        pos = rng.standard_normal(3).astype(np.float32)
+        pt = (pos + rng.standard_normal(3) * 0.8).astype(np.float32)
+        m = float(rng.choice([0.0, 0.05]))
+        a = oracle.contains_point(t, pos, q, size, pt, m)
+        assert a == oracle.ref_contains_point(t, pos, q, size, pt, m)
+        inside += a
+    assert inside > 800
+
+
+def test_intersect_point_closed_forms():
+    """The server's intersect_point: a point inside the lower of two stacked boxes finds it, a
+    point on their shared face finds both (list order: upper first), a mask skips a layer, a
+    point in the air finds nothing; shapes are tested exactly (a point in a sphere's bounding
+    box corner is outside the sphere)."""
+    o, b = _query_scene()     # list order: cylinder, sphere, upper box, lower box, ground
+    pts = np.array([[0, 0, 0.5], [0, 0, 1.0], [0, 0, 1.0], [0, 0, 9.0], [3.45, 0.45, 0.95], [3, 0, 0.5]], np.float32)
+    masks = np.array([0xFFFFFFFF, 0xFFFFFFFF, 1, 0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFF], np.uint32)
+    count, out = o.intersect_point(pts, masks, max_results=4)
+    assert list(count) == [1, 2, 1, 0, 0, 1]
+    assert out[0, 0] == 3 and list(out[1, :2]) == [2, 3] and out[2, 0] == 3 and out[5, 0] == 1
+    count, out = o.intersect_point(pts[1:2], None, max_results=1)
+    assert count[0] == 1 and out[0, 0] == 2
