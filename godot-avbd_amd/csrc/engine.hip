@@ -106,6 +106,7 @@ void setParams(Params &P, const avbd_params &p, float collisionDt)
     P.sleepFrames = p.sleep_frames;
     P.sleepLin = p.sleep_linear_threshold;
     P.sleepAng = p.sleep_angular_threshold;
+    P.grace = p.contact_grace_ticks > 0 ? p.contact_grace_ticks : 0;
 }
 
 avbd_status checkParams(const avbd_params *p)
@@ -113,7 +114,6 @@ avbd_status checkParams(const avbd_params *p)
     if (!p) return fail(AVBD_ERR_INVALID, "params is null");
     if (!(p->dt > 0.0f)) return fail(AVBD_ERR_INVALID, "dt must be positive");
     if (p->iterations < 1) return fail(AVBD_ERR_INVALID, "iterations must be >= 1");
-    if (p->contact_grace_ticks > 0) return fail(AVBD_ERR_UNSUPPORTED, "contact grace ticks are outside this path");
     if (p->jacobian_rebuild_distance > 0.0f)
         return fail(AVBD_ERR_UNSUPPORTED, "jacobian arm caching is outside this path");
     if (p->convergence_threshold > 0.0f)
@@ -556,7 +556,9 @@ struct Engine {
         AL(S.pos, N); AL(S.rot, N); AL(S.initLin, N); AL(S.initAng, N); AL(S.inertLin, N); AL(S.inertAng, N);
         AL(S.vel, N); AL(S.angVel, N); AL(S.prevVel, N); AL(S.sizeR, N); AL(S.massI, N); AL(S.poseRec, 4 * N); AL(S.narrowRec, NARROW_REC * N);
         AL(S.fric, N); AL(S.grav, N); AL(S.layer, N); AL(S.mask, N); AL(S.lockLin, N); AL(S.lockAng, N); AL(S.shape, N);
-        AL(S.sleepMode, N); AL(S.sleeping, N); AL(S.stillFrames, N); AL(S.updFlag, N + 1); AL(S.updScan, N + 1); AL(rimDev, 16); S.rim = rimDev;
+        AL(S.sleepMode, N); AL(S.sleeping, N); AL(S.stillFrames, N); AL(S.updFlag, N + 1); AL(S.updScan, N + 1);
+        AL(S.mDormant, M); AL(S.mSaved, M); AL(S.dRA, M * MAXC); AL(S.dRB, M * MAXC); AL(S.dC0, M * MAXC);
+        AL(S.dPen, M * MAXC); AL(S.pairParked, P); AL(rimDev, 16); S.rim = rimDev;
         AL(S.bodyWorld, N); AL(S.updateIndex, N); AL(S.colour, N); AL(S.contactOnly, N);
         AL(S.dLin, N); AL(S.dAng, N); AL(S.aabbC, N); AL(S.aabbH, N);
         AL(S.fbCount, 1); AL(S.fbBody, N); AL(S.fbBlock, 33 * N);
@@ -793,10 +795,15 @@ struct Engine {
         // work items: the previous step's view (wLiveOff, kSlot, kT, mHdr still hold it), then
         // this step's new manifolds
         k_fill_u32<<<nblk(Scap), TPB, 0, stream>>>(S.keep, 0u, Scap);
-        k_manifold_init<<<liveGrid(NARROW_TPB), NARROW_TPB, 0, stream>>>(S);
-        // pairs with a sphere or a cylinder take their own pass (only launched when such bodies
-        // exist, so box scenes keep the box kernel's register budget and time)
-        if (anyRound()) k_manifold_init_round<<<liveGrid(NARROW_TPB), NARROW_TPB, 0, stream>>>(S);
+        if (prm.contact_grace_ticks > 0) {
+            // contact grace: one pass for every pair with the parking rules
+            k_manifold_init_grace<<<liveGrid(NARROW_TPB), NARROW_TPB, 0, stream>>>(S);
+        } else {
+            k_manifold_init<<<liveGrid(NARROW_TPB), NARROW_TPB, 0, stream>>>(S);
+            // pairs with a sphere or a cylinder take their own pass (only launched when such
+            // bodies exist, so box scenes keep the box kernel's register budget and time)
+            if (anyRound()) k_manifold_init_round<<<liveGrid(NARROW_TPB), NARROW_TPB, 0, stream>>>(S);
+        }
         pend();
         k_release_flags<<<nblk(Scap), TPB, 0, stream>>>(S);
         scan(S.keep, S.keepScan, Scap);
@@ -1483,6 +1490,8 @@ struct Engine {
         const int n = m->count;
         if (n > wi.slotCap) return fail(AVBD_ERR_CAPACITY, "more manifolds than the world's slot capacity");
         const int sb = wi.slotBase, o = wi.bodyBase;
+        fill(S.mDormant + sb, 0, wi.slotCap);   // restored manifolds are not parked
+        fill(S.mSaved + sb, 0, wi.slotCap);
         std::vector<int> live(n), mA(n), mB(n), num(n);
         std::vector<float> fr(n);
         std::vector<float4> basis(3 * n), cra(n * MAXC), crb(n * MAXC), cc0(n * MAXC), cp(n * MAXC), cl(n * MAXC);

@@ -720,7 +720,7 @@ __device__ void pairFilter(const State &S, int p)
     const bool layers = ((S.layer[i] & S.mask[j]) != 0) || ((S.layer[j] & S.mask[i]) != 0);
     const bool staticI = bMass(S, i) <= 0.0f, staticJ = bMass(S, j) <= 0.0f;
     const bool bothStatic = staticI && staticJ;
-    bool constrained = false;
+    bool constrained = false, parked = false;
     if (ov && layers && !bothStatic) {
         // A static body's list holds its joints only (manifold entries are kept for movable
         // bodies), so a pair with a static body scans the movable body's list: every force
@@ -728,10 +728,21 @@ __device__ void pairFilter(const State &S, int p)
         const int ni = S.adjStart[i + 1] - S.adjStart[i], nj = S.adjStart[j + 1] - S.adjStart[j];
         const bool useI = staticJ || (!staticI && ni <= nj);
         const int a = useI ? i : j, b = useI ? j : i;
-        for (int e = S.adjStart[a]; e < S.adjStart[a + 1]; e++)
-            if (otherOf(S, S.adj[e], a) == b) { constrained = true; break; }
+        for (int e = S.adjStart[a]; e < S.adjStart[a + 1]; e++) {
+            if (otherOf(S, S.adj[e], a) != b) continue;
+            // solver_internal.hpp:67-84: a manifold parked by contact grace does not join the
+            // pair (it can still wake sleepers), but it keeps a second manifold from being made
+            if (S.P.grace > 0 && S.adj[e] >= 0) {
+                const int k = adjManifold(S.adj[e]);
+                if (S.mHdr[k].w == 0 && S.mDormant[S.kSlot[k]] > 0) { parked = true; continue; }
+            }
+            constrained = true;
+            break;
+        }
     }
-    S.pairValid[p] = (ov && layers && !bothStatic && !constrained) ? 1u : 0u;
+    const bool pass = ov && layers && !bothStatic && !constrained;
+    S.pairValid[p] = (pass && !parked) ? 1u : 0u;
+    if (S.P.grace > 0) S.pairParked[p] = pass && parked ? 1 : 0;
 }
 
 // solver_broadphase.cpp:153-184 + manifold.cpp:205-268: the r-th new pair of a world
@@ -768,6 +779,8 @@ __device__ void allocNew(const State &S, int p)
     S.mB[slot] = j;
     S.mNum[slot] = 0;
     S.mFric[slot] = 0.0f;
+    S.mDormant[slot] = 0;
+    S.mSaved[slot] = 0;
     S.mBasis[3 * slot] = make_float4(0, 0, 0, 0);
     S.mBasis[3 * slot + 1] = make_float4(0, 0, 0, 0);
     S.mBasis[3 * slot + 2] = make_float4(0, 0, 0, 0);
@@ -910,23 +923,33 @@ __global__ void k_contact_unpack(State S)
 // (launched only when such bodies exist).  Each work item is handled by exactly one of the two:
 // the box pass also parks frozen pairs and reports capacity errors.
 typedef float NarrowPolys[2][3][POLY_LDS * NARROW_TPB];
-template <bool ROUND> AD void manifoldInit(const State &S, int q, int P, NarrowPolys *poly);
+// MODE 0: box pairs, MODE 1: pairs with a round shape, MODE 2 (contact grace on): every pair,
+// with the grace rules (manifold.cpp:249-378).
+template <int MODE> AD void manifoldInit(const State &S, int q, int P, NarrowPolys *poly);
 // At most 128 registers (four wavefronts per SIMD; it fits without spilling vector registers).
 __global__ void __launch_bounds__(NARROW_TPB) __attribute__((amdgpu_waves_per_eu(4))) k_manifold_init(State S)
 {
     __shared__ NarrowPolys poly;   // 12 KiB
     const int P = static_cast<int>(S.wLiveOff[S.numWorlds]);
     const int fresh = static_cast<int>(S.pairValidScan[pairTotal(S)]);
-    FOR_RANGE(q, P + fresh) manifoldInit<false>(S, q, P, &poly);
+    FOR_RANGE(q, P + fresh) manifoldInit<0>(S, q, P, &poly);
+}
+__global__ void __launch_bounds__(NARROW_TPB) k_manifold_init_grace(State S)
+{
+    __shared__ NarrowPolys poly;
+    const int P = static_cast<int>(S.wLiveOff[S.numWorlds]);
+    const int fresh = static_cast<int>(S.pairValidScan[pairTotal(S)]);
+    FOR_RANGE(q, P + fresh) manifoldInit<2>(S, q, P, &poly);
 }
 __global__ void __launch_bounds__(NARROW_TPB) k_manifold_init_round(State S)
 {
     const int P = static_cast<int>(S.wLiveOff[S.numWorlds]);
     const int fresh = static_cast<int>(S.pairValidScan[pairTotal(S)]);
-    FOR_RANGE(q, P + fresh) manifoldInit<true>(S, q, P, nullptr);
+    FOR_RANGE(q, P + fresh) manifoldInit<1>(S, q, P, nullptr);
 }
-template <bool ROUND> AD void manifoldInit(const State &S, int q, int P, NarrowPolys *poly)
+template <int MODE> AD void manifoldInit(const State &S, int q, int P, NarrowPolys *poly)
 {
+    constexpr bool ROUND = MODE == 1;
     int t, slot, a, b, oldN;
     uint32_t ob;
     if (q < P) {
@@ -956,7 +979,14 @@ template <bool ROUND> AD void manifoldInit(const State &S, int q, int P, NarrowP
     const int shA = __float_as_int(a5.x), shB = __float_as_int(b5.x);
     const bool round = (shA | shB) != AVBD_SHAPE_BOX;
     if (ROUND && (frozen || !round)) return;
-    if (!ROUND) {
+    if (MODE == 2) {
+        S.sPos[slot] = q;
+        if (frozen) {   // parked verbatim (a frozen parked manifold keeps its kept contacts too)
+            denseToStage(S, stageRow(q, 0), CSLICE, ob, CSLICE, oldN);
+            S.keep[t] = 1;
+            return;
+        }
+    } else if (!ROUND) {
         S.sPos[slot] = q;
         if (frozen) {   // parked verbatim
             denseToStage(S, stageRow(q, 0), CSLICE, ob, CSLICE, oldN);
@@ -987,7 +1017,7 @@ template <bool ROUND> AD void manifoldInit(const State &S, int q, int P, NarrowP
     const size_t d = stageRow(q, 0);
     Contacts cs{0, S.sRA + d, S.sRB + d, S.sC0 + d};
     int nn;
-    if constexpr (ROUND) {
+    if (ROUND || (MODE == 2 && round)) {
         nn = collideRound(shA, shB, makeRShp(pa, qa, sa), makeRShp(pb, qb, sb), S.rim, margin, B, wrote, cs);
     } else {
         const int l = threadIdx.x;
@@ -1007,6 +1037,14 @@ template <bool ROUND> AD void manifoldInit(const State &S, int q, int P, NarrowP
     }
     // old contacts stay in the dense arrays; the new ones go to staging
     const float pmax = S.P.penaltyMax;
+    // contact grace (manifold.cpp:249-253): a parked manifold (no contacts in the view) matches
+    // against the contacts it kept, from the slot's keep rows
+    bool revive = false;
+    int liveCount = oldN;
+    const size_t ds = static_cast<size_t>(slot) * MAXC;
+    if constexpr (MODE == 2) {
+        if (oldN == 0 && S.mSaved[slot] > 0) { revive = true; liveCount = S.mSaved[slot]; }
+    }
     for (int i = 0; i < nn; i++) {
         const float4 nra = cs.ra[crow(i)], nrb = cs.rb[crow(i)];
         const int key = __float_as_int(nrb.w);
@@ -1014,14 +1052,26 @@ template <bool ROUND> AD void manifoldInit(const State &S, int q, int P, NarrowP
         v3 pen{0, 0, 0}, lam{0, 0, 0};
         uint8_t stick = 0;
         float beta = 1.0f, prevErr = -1.0f;
-        for (int j = 0; j < oldN; j++) {
+        for (int j = 0; j < liveCount; j++) {
             const uint32_t oc = ob + j * CSLICE;
-            const float4 orb = S.cRB[oc];
-            if (key != __float_as_int(orb.w)) continue;
-            const float4 ora = S.cRA[oc], oc0 = S.cC0[oc];
+            float4 orb, ora, oc0;
+            if (MODE == 2 && revive) {
+                orb = S.dRB[ds + j];
+                if (key != __float_as_int(orb.w)) continue;
+                ora = S.dRA[ds + j];
+                oc0 = S.dC0[ds + j];
+                pen = ld3(S.dPen[ds + j]);   // kept with no force: lambda 0, not stuck
+                lam = v3{0, 0, 0};
+                stick = 0;
+            } else {
+                orb = S.cRB[oc];
+                if (key != __float_as_int(orb.w)) continue;
+                ora = S.cRA[oc];
+                oc0 = S.cC0[oc];
+                loadPenLam(S, oc, pen, lam);
+                stick = S.cStick[oc];
+            }
             prevErr = smax(0.0f, COLLISION_MARGIN - oc0.x);
-            loadPenLam(S, oc, pen, lam);
-            stick = S.cStick[oc];
             beta = ora.w;
             if (stick) {
                 const v3 oldA = pa + qrot(qa, ld3(ora));
@@ -1056,6 +1106,13 @@ template <bool ROUND> AD void manifoldInit(const State &S, int q, int P, NarrowP
             lam = v3{0, 0, 0};
             stick = 0;
         }
+        if constexpr (MODE == 2) {
+            // manifold.cpp:338-347: an unmatched contact starts at 5 % of beta_lin
+            if (prevErr < 0.0f) {
+                const float seed = smax(PENALTY_MIN, 0.05f * S.P.betaLin);
+                pen = v3{seed, seed, seed};
+            }
+        }
         lam = lam * S.P.alpha * S.P.gamma;
         pen = lclamp3(pen * S.P.gamma, PENALTY_MIN, pmax);
         const size_t c = d + crow(i);
@@ -1064,6 +1121,31 @@ template <bool ROUND> AD void manifoldInit(const State &S, int q, int P, NarrowP
         S.sC0[c] = st3(c0, stick ? 1.0f : 0.0f);
         S.sPen[c] = st3(pen);
         S.sLam[c] = st3(lam);
+    }
+    if constexpr (MODE == 2) {
+        // manifold.cpp:359-378: a separated pair is parked for up to `grace` ticks, keeping its
+        // contacts (in the slot's keep rows) with no force
+        if (nn == 0) {
+            if (S.mDormant[slot] < S.P.grace) {
+                if (!revive)
+                    for (int j = 0; j < oldN; j++) {
+                        const uint32_t oc = ob + j * CSLICE;
+                        v3 kp, kl;
+                        loadPenLam(S, oc, kp, kl);
+                        S.dRA[ds + j] = S.cRA[oc];
+                        S.dRB[ds + j] = S.cRB[oc];
+                        S.dC0[ds + j] = S.cC0[oc];
+                        S.dPen[ds + j] = st3(kp);
+                    }
+                S.mDormant[slot] += 1;
+                S.mSaved[slot] = liveCount;
+                S.mNum[slot] = 0;
+                S.keep[t] = 1u;
+                return;
+            }
+            S.mSaved[slot] = 0;
+        }
+        S.mDormant[slot] = 0;
     }
     S.mNum[slot] = nn;
     S.keep[t] = nn > 0 ? 1u : 0u;
@@ -1635,7 +1717,7 @@ __global__ void k_wake_pairs(State S)
     const int base = S.wBodyBase[w];
     const uint32_t p0 = S.pairOff[base], p1 = S.pairOff[base + S.wBodyCount[w]];
     for (uint32_t p = p0; p < p1; p++)
-        if (S.pairValid[p]) wakePair(S, S.pairI[p], S.pairJ[p]);
+        if (S.pairValid[p] || (S.P.grace > 0 && S.pairParked[p])) wakePair(S, S.pairI[p], S.pairJ[p]);
 }
 // solver_solve.cpp:264-289: with idle detection on, a sleeper (mode IDLE) sharing a force with
 // an awake movable body that moves faster than the thresholds wakes; bodies in list order,

@@ -30,6 +30,7 @@ struct Params {
     // sleeping (solver_solve.cpp:143-339): sleepOn = some body can sleep or sleeps; sleepFrames
     // = idle steps before sleeping (0: no idle detection); thresholds in m/s and rad/s
     int sleepOn, sleepFrames;
+    int grace;   // contact grace ticks (0: a separated pair's manifold is released at once)
     float sleepLin, sleepAng;
 };
 
@@ -61,6 +62,11 @@ struct State {
     uint8_t *sleepMode, *sleeping;   // AVBD_SLEEP_* per body; 1 while asleep
     int *stillFrames;                // steps below the sleep thresholds
     uint32_t *updFlag, *updScan;     // per step: in the update set; its per-world ranks
+    // contact grace: per slot, ticks parked and contacts kept; the kept contacts (slot * MAXC
+    // rows: arm A + beta scale, arm B + key, C0, penalty); per pair, "has a parked manifold"
+    int *mDormant, *mSaved;
+    float4 *dRA, *dRB, *dC0, *dPen;
+    uint8_t *pairParked;
     const float *rim;     // cylinder rim directions: 8 cosines then 8 sines (host libm values)
     int *bodyWorld;
     int *updateIndex;  // rank among movable bodies of the world (list order), -1 static

@@ -16,8 +16,8 @@
 // in the reference's floating-point evaluation order, serially (the reference's results do
 // not depend on its thread count: test/core_test.cpp:612-681).
 //
-// Out of scope (rejected with AVBD_ERR_UNSUPPORTED): contact grace ticks, jacobian arm
-// caching, the convergence early exit.
+// Out of scope (rejected with AVBD_ERR_UNSUPPORTED): jacobian arm caching, the convergence
+// early exit.
 #include "avbd_oracle.h"
 
 #include <algorithm>
@@ -968,6 +968,7 @@ struct World {
     std::vector<m3> basis;
     std::vector<int> numC;
     std::vector<float> mFric;
+    std::vector<int> dormant, saved;   // contact grace (manifold.hpp): ticks parked, contacts kept
     std::vector<Contact> cont;
     std::vector<f3> C0, pen, lam;
     std::vector<uint8_t> stick;
@@ -994,10 +995,12 @@ struct World {
         else {
             s = static_cast<uint32_t>(mA.size());
             mA.push_back(0); mB.push_back(0); basis.push_back(zero3()); numC.push_back(0); mFric.push_back(0);
+            dormant.push_back(0); saved.push_back(0);
             cont.resize(cont.size() + MAXC); C0.resize(C0.size() + MAXC); pen.resize(pen.size() + MAXC);
             lam.resize(lam.size() + MAXC); stick.resize(stick.size() + MAXC); betaScale.resize(betaScale.size() + MAXC);
         }
         mA[s] = a; mB[s] = b; basis[s] = zero3(); numC[s] = 0; mFric[s] = 0.0f;
+        dormant[s] = 0; saved[s] = 0;
         for (int k = 0; k < MAXC; k++) {
             const int i = s * MAXC + k;
             cont[i] = Contact{0, {0, 0, 0}, {0, 0, 0}};
@@ -1104,10 +1107,15 @@ struct World {
         out.erase(std::unique(out.begin(), out.end()), out.end());
     }
 
+    // solver_internal.hpp:67-84: a force or a live manifold joins the pair; a manifold parked by
+    // contact grace does not (its pair can still wake sleepers on bounds overlap)
     bool constrained(int a, int b) const
     {
-        for (uint32_t k = adjStart[a]; k < adjStart[a + 1]; k++)
-            if (otherOf(adj[k], a) == b) return true;
+        for (uint32_t k = adjStart[a]; k < adjStart[a + 1]; k++) {
+            if (otherOf(adj[k], a) != b) continue;
+            if (adj[k] >= 0 && numC[adj[k]] == 0 && dormant[adj[k]] > 0) continue;
+            return true;
+        }
         return false;
     }
 
@@ -1207,6 +1215,9 @@ struct World {
         const Body &a = bodies[mA[s]], &b = bodies[mB[s]];
         const int first = static_cast<int>(s) * MAXC;
         mFric[s] = std::sqrt(a.friction * b.friction);
+        // manifold.cpp:249-253: a parked manifold re-enters with the contacts it kept
+        if (numC[s] == 0 && saved[s] > 0) numC[s] = saved[s];
+        const int liveCount = numC[s];
         Contact nc[MAXC] = {};
         float bas[9];
         int32_t keys[8]; float ra[24], rb[24];
@@ -1254,11 +1265,29 @@ struct World {
                 break;
             }
         }
+        // manifold.cpp:338-347: with grace, unmatched contacts start at 5 % of beta_lin
+        if (P.contact_grace_ticks > 0) {
+            const float seed = smax(PENALTY_MIN, 0.05f * P.beta_lin);
+            for (int i = 0; i < nn; i++)
+                if (prevErr[i] < 0.0f) nPen[i] = f3{seed, seed, seed};
+        }
         numC[s] = nn;
         for (int i = 0; i < nn; i++) {
             cont[first + i] = nc[i]; pen[first + i] = nPen[i]; lam[first + i] = nLam[i];
             stick[first + i] = nStick[i]; betaScale[first + i] = nBeta[i];
         }
+        // manifold.cpp:359-378: a separated pair is parked (not released) for up to grace ticks,
+        // its contacts kept with no force
+        if (nn == 0) {
+            if (P.contact_grace_ticks > 0 && dormant[s] < P.contact_grace_ticks) {
+                for (int i = 0; i < liveCount; i++) { lam[first + i] = f3{0, 0, 0}; stick[first + i] = 0; }
+                dormant[s]++;
+                saved[s] = liveCount;
+                return false;
+            }
+            saved[s] = 0;
+        }
+        dormant[s] = 0;
         for (int i = 0; i < numC[s]; i++) {
             const f3 xA = a.pos + qrot(a.q, cont[first + i].rA);
             const f3 xB = b.pos + qrot(b.q, cont[first + i].rB);
@@ -1721,7 +1750,8 @@ struct World {
         for (size_t i = 0; i < live.size(); i++) mact[i] = frozen(live[i]) || initialize(live[i]);
         size_t lw = 0;
         for (size_t i = 0; i < live.size(); i++) {
-            if (!mact[i]) { freeList.push_back(live[i]); continue; }
+            // solver_solve.cpp:66-86: a parked manifold keeps its place in the live list
+            if (!mact[i] && !(P.contact_grace_ticks > 0 && dormant[live[i]] > 0)) { freeList.push_back(live[i]); continue; }
             live[lw++] = live[i];
         }
         live.resize(lw);
@@ -2102,7 +2132,7 @@ struct World {
 int checkParams(const avbd_params *p)
 {
     if (!p) return AVBD_ERR_INVALID;
-    if (p->contact_grace_ticks > 0 || p->jacobian_rebuild_distance > 0.0f ||
+    if (p->jacobian_rebuild_distance > 0.0f ||
         p->convergence_threshold > 0.0f)
         return AVBD_ERR_UNSUPPORTED;
     return AVBD_OK;
@@ -2214,6 +2244,7 @@ int oracle_upload(oracle_world *o, const avbd_bodies *b, const avbd_joints *j)
         }
     }
     w.mA.clear(); w.mB.clear(); w.basis.clear(); w.numC.clear(); w.mFric.clear(); w.cont.clear();
+    w.dormant.clear(); w.saved.clear();
     w.C0.clear(); w.pen.clear(); w.lam.clear(); w.stick.clear(); w.betaScale.clear();
     w.live.clear(); w.freeList.clear();
     return AVBD_OK;
@@ -2545,6 +2576,7 @@ int oracle_set_manifolds(oracle_world *o, const avbd_manifolds *m)
 {
     World &w = o->w;
     w.mA.clear(); w.mB.clear(); w.basis.clear(); w.numC.clear(); w.mFric.clear(); w.cont.clear();
+    w.dormant.clear(); w.saved.clear();
     w.C0.clear(); w.pen.clear(); w.lam.clear(); w.stick.clear(); w.betaScale.clear();
     w.live.clear(); w.freeList.clear();
     for (int i = 0; i < m->count; i++) {

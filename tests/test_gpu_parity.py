@@ -497,3 +497,30 @@ def test_sleeping_interleaved_and_batch():
         o.step_interleaved(2, 0)
     g, r = w.download(), o.download()
     assert not state_mismatch(g, r) and _sleep_equal(g, r)
+
+
+def test_contact_grace_lockstep():
+    """Contact grace (manifolds parked for up to contact_grace_ticks after their pair separates,
+    revived by feature key, fresh contacts seeded at 5 % of beta_lin) on a bouncing mixed pile,
+    bit for bit with the oracle, plain and in a batch with sleeping on."""
+    b, j = scenes.round_pile(150, seed=19)
+    b["velocity"][:, 2] += np.where(np.arange(len(b["mass"])) % 4 == 0, 3.0, 0.0).astype(np.float32)
+    p = P(iterations=10, contact_grace_ticks=3)
+    w, o = lockstep(b, j, p, 150, every=10)
+    manifolds_equal(w, o)
+    assert w.stats()["manifolds"] == o.stats()["manifolds"]
+    p2 = P(iterations=8, contact_grace_ticks=2, sleep_frames=10)
+    worlds = [scenes.sleepy_pile(100, seed=21), scenes.pile(80, seed=22)]
+    batch = pkg.Batch(p2, 4, 1000)
+    ids = [batch.add(bb, jj) for bb, jj in worlds]
+    alone = []
+    for bb, jj in worlds:
+        ox = oracle.OracleWorld(p2)
+        ox.upload(bb, jj)
+        alone.append(ox)
+    for s in range(120):
+        batch.step()
+        for ox in alone:
+            ox.step()
+    for wid, ox in zip(ids, alone):
+        assert not state_mismatch(batch.download(wid), ox.download()), wid

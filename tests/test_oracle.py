@@ -224,7 +224,7 @@ def test_static_pairs_make_no_manifold():
 
 def test_unsupported_parameters_rejected():
     with pytest.raises(ValueError):
-        oracle.OracleWorld(P(contact_grace_ticks=2))
+        oracle.OracleWorld(P(jacobian_rebuild_distance=0.02))
 
 
 @pytest.mark.parametrize("scene,overflow", [("chains", True), ("pile", True), ("pile", False)])
@@ -995,3 +995,51 @@ def test_intersect_point_closed_forms():
     assert out[0, 0] == 3 and list(out[1, :2]) == [2, 3] and out[2, 0] == 3 and out[5, 0] == 1
     count, out = o.intersect_point(pts[1:2], None, max_results=1)
     assert count[0] == 1 and out[0, 0] == 2
+
+
+# ---------------------------------------------------------------- contact grace ----
+def test_contact_grace_parks_and_revives():
+    """manifold.cpp:249-378 with solver_solve.cpp:66-86: with contact_grace_ticks = 3, a box that
+    hops off the ground keeps its manifold parked (no contacts, no force) for up to 3 ticks; when
+    it lands within them the contacts re-match by feature key and keep their penalty; a box that
+    stays away longer has its manifold released; fresh contacts start at 5 % of beta_lin."""
+    def world(vz, grace):
+        s = scenes.SceneBuilder()
+        s.box((20, 20, 1), 0.0, 0.5, (0, 0, -0.5))
+        s.box((1, 1, 1), 1.0, 0.5, (0, 0, 0.5))
+        o = oracle.OracleWorld(P(contact_grace_ticks=grace))
+        o.upload(*s.build())
+        o.step(30)
+        st = o.download()
+        v = st["velocity"].copy()
+        v[0] = (0, 0, vz)
+        o.write_bodies({"velocity": v})
+        return o
+    o = world(0.75, 3)                  # a short hop: about 9 steps in the air at 60 Hz
+    counts = []
+    for _ in range(30):
+        o.step()
+        m = o.manifolds()
+        counts.append((len(m["body_a"]), int(m["num_contacts"][0]) if len(m["body_a"]) else -1))
+    parked = [c for c in counts if c == (1, 0)]
+    assert 1 <= len(parked) <= 3 and counts[-1][0] == 1 and counts[-1][1] == 4, counts
+    o = world(3.0, 3)                   # a long flight: released after 3 parked ticks
+    seen = []
+    for _ in range(12):
+        o.step()
+        seen.append(len(o.manifolds()["body_a"]))
+    assert seen[:3] == [1, 1, 1] and 0 in seen, seen
+    # a fresh contact's penalty starts at 5 % of beta_lin (then the warm-start decay and one
+    # dual ramp); without grace it starts at the floor
+    s = scenes.SceneBuilder()
+    s.box((20, 20, 1), 0.0, 0.5, (0, 0, -0.5))
+    s.box((1, 1, 1), 1.0, 0.5, (0, 0, 0.505))
+    first = {}
+    for grace in (3, 0):
+        o = oracle.OracleWorld(P(contact_grace_ticks=grace, iterations=1))
+        o.upload(*s.build())
+        o.step()
+        m = o.manifolds()
+        first[grace] = float(m["penalty"][0][0])
+    seed = float(np.float32(np.float32(0.05) * np.float32(1e4)) * np.float32(0.999))
+    assert first[3] >= seed and first[0] < seed, first
