@@ -260,6 +260,67 @@ __global__ void k_intersect_point(State S, int base, int count, int n, const flo
     cnt[r] = c;
 }
 
+// The bodies part of the server's intersect_shape (avbd_direct_space_state3d.cpp:261-306):
+// query r is qin[4r] = (centre, type bits), qin[4r+1] = rotation, qin[4r+2] = (size, margin),
+// qin[4r+3].x = mask bits.  Candidates: bodies whose bounds overlap the query's bounds
+// (query_shape.cpp:371-394) grown by the margin, in list order; a hit is a body the narrow
+// phase (query first) finds within the margin.  Contact rows and polygon spill rows per query
+// are staging-shaped scratch (row stageRow(r, i)).
+__global__ void __launch_bounds__(NARROW_TPB) k_intersect_shape(State S, int base, int count, int n, const float4 *qin,
+                                                               const float4 *bounds, int maxRes, int *cnt, int *out,
+                                                               float4 *cra, float4 *crb, float4 *cmid, float4 *spA,
+                                                               float4 *spB)
+{
+    __shared__ NarrowPolys poly;
+    const int r = GID;
+    if (r >= n) return;
+    const float4 c4 = qin[4 * r], r4 = qin[4 * r + 1], s4 = qin[4 * r + 2], m4 = qin[4 * r + 3];
+    const int type = __float_as_int(c4.w);
+    const v3 qc = ld3(c4);
+    const qt qq = ldq(r4);
+    const float m = smax(0.0f, s4.w);
+    const uint32_t mask = __float_as_uint(m4.x);
+    const v3 h = type == AVBD_SHAPE_SPHERE     ? v3{s4.x, s4.x, s4.x}
+                 : type == AVBD_SHAPE_CYLINDER ? v3{s4.x, s4.x, s4.z * 0.5f}
+                                               : v3{s4.x, s4.y, s4.z} * 0.5f;
+    const m3 R = qmat(qq);
+    const v3 wx{(fabsf(R.c[0][0]) * h.x + fabsf(R.c[1][0]) * h.y) + fabsf(R.c[2][0]) * h.z,
+                (fabsf(R.c[0][1]) * h.x + fabsf(R.c[1][1]) * h.y) + fabsf(R.c[2][1]) * h.z,
+                (fabsf(R.c[0][2]) * h.x + fabsf(R.c[1][2]) * h.y) + fabsf(R.c[2][2]) * h.z};
+    v3 mn = qc - wx, mx = qc + wx;
+    if (m > 0.0f) { mn = mn - v3{m, m, m}; mx = mx + v3{m, m, m}; }
+    const size_t d = stageRow(r, 0);
+    const int l = threadIdx.x;
+    int c = 0;
+    for (int j = 0; j < count && c < maxRes; j++) {
+        const float4 lo = bounds[2 * j], hi = bounds[2 * j + 1];
+        if ((__float_as_uint(lo.w) & mask) == 0) continue;
+        if (!boxesOverlap(ld3(lo), ld3(hi), mn, mx)) continue;
+        const int i = base + j;
+        const int bt = S.shape[i];
+        const v3 bp = ld3(S.pos[i]);
+        const qt bq = ldq(S.rot[i]);
+        const float4 bs = S.sizeR[i];
+        Contacts cs{0, cra + d, crb + d, cmid + d};
+        m3 B;
+        bool wrote;
+        int hits;
+        if (type != AVBD_SHAPE_BOX || bt != AVBD_SHAPE_BOX) {
+            hits = collideRound(type, bt, makeRShp(qc, qq, make_float4(s4.x, s4.y, s4.z, 0.0f)), makeRShp(bp, bq, bs),
+                                S.rim, m, B, wrote, cs);
+        } else {
+            const Poly c0{&poly[0][0][l], &poly[0][1][l], &poly[0][2][l], spA + d};
+            const Poly c1{&poly[1][0][l], &poly[1][1][l], &poly[1][2][l], spB + d};
+            hits = collideBoxes(qc, qq, v3{s4.x, s4.y, s4.z} * 0.5f, bp, bq, v3{bs.x, bs.y, bs.z} * 0.5f, m, B, wrote, cs,
+                                c0, c1);
+        }
+        if (hits <= 0) continue;
+        out[static_cast<size_t>(r) * maxRes + c] = j;
+        c++;
+    }
+    cnt[r] = c;
+}
+
 // Solver::pick (solver_query.cpp:29-155): rays[2r] = (origin, mask bits), rays[2r+1] = dir.
 // Every body that passes the mask: an exact sphere test, else the box bounding the shape; the
 // nearest t (the earlier body on a tie).  local[r] = hit point in the body's frame.

@@ -110,6 +110,20 @@ class World:
                                                       count.ctypes.data_as(abi.i32p), out.ctypes.data_as(abi.i32p)))
         return count, out
 
+    def intersect_shape(self, types, centers, rotations, sizes, margins, masks=None, max_results=8):
+        """The server's intersect_shape for n query shapes: (count, body indices n x max_results)."""
+        t = np.ascontiguousarray(types, np.int32)
+        c, q, s3, m = (np.ascontiguousarray(x, np.float32) for x in (centers, rotations, sizes, margins))
+        n = len(t)
+        count, out = np.zeros(n, np.int32), np.full((n, max_results), -1, np.int32)
+        mk = None if masks is None else np.ascontiguousarray(masks, np.uint32)
+        abi.check(self.lib.avbd_world_intersect_shape(self.h, n, t.ctypes.data_as(abi.i32p), c.ctypes.data_as(abi.f32p),
+                                                      q.ctypes.data_as(abi.f32p), s3.ctypes.data_as(abi.f32p),
+                                                      m.ctypes.data_as(abi.f32p),
+                                                      None if mk is None else mk.ctypes.data_as(abi.u32p), max_results,
+                                                      count.ctypes.data_as(abi.i32p), out.ctypes.data_as(abi.i32p)))
+        return count, out
+
     def contact_report(self, max_reported):
         """Per-body contact reports: (count, position, normal, other), stride max(max_reported)."""
         mr = np.ascontiguousarray(max_reported, np.int32)

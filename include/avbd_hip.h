@@ -328,6 +328,16 @@ avbd_status avbd_world_profile_raycast(avbd_world *world, int32_t n, const float
  * = the list index of the c-th body containing point r (-1 past count[r]). */
 avbd_status avbd_world_intersect_point(avbd_world *world, int32_t n, const float *points, const uint32_t *masks,
                                        int32_t max_results, int32_t *count, int32_t *out);
+/* n query shapes against the bodies at their current pose: replaces the bodies part of the
+ * server's intersect_shape (src/server/avbd_direct_space_state3d.cpp:261-306 with
+ * query::shapeAabb and query::shapeContacts = collideShapes).  Query r: type[r] (AVBD_SHAPE_*),
+ * center[3r..] (the pose already moved by any motion), rotation[4r..], size[3r..] (as a body's),
+ * margin[r] >= 0.  Candidates are the bodies whose layer passes masks[r] (null: all) and whose
+ * bounds overlap the query's bounds grown by the margin, in list order; a hit is a body the narrow
+ * phase (query first) finds in contact within the margin.  out as avbd_world_intersect_point. */
+avbd_status avbd_world_intersect_shape(avbd_world *world, int32_t n, const int32_t *type, const float *center,
+                                       const float *rotation, const float *size, const float *margin,
+                                       const uint32_t *masks, int32_t max_results, int32_t *count, int32_t *out);
 /* Replaces `Solver::pick(origin, dir, local, mask)` (src/avbd/solver.h:161-164,
  * solver_query.cpp:29-155) for n rays: body[r] = list index of the nearest body (-1 none),
  * local[3r..] = the hit point in its frame (zeros for none); masks null = all layers. */

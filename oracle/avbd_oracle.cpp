@@ -2436,6 +2436,45 @@ int oracle_intersect_point(oracle_world *o, int n, const float *points, const ui
     return AVBD_OK;
 }
 
+// The bodies part of the server's intersect_shape (avbd_direct_space_state3d.cpp:261-306) for
+// n query shapes (type, centre, rotation, size as a body's, margin >= 0): candidates from
+// collectAABB over the shape's bounds (query::shapeAabb, query_shape.cpp:371-394) grown by the
+// margin, in list order; the bodies the narrow phase (collideShapes, query first) finds in
+// contact within the margin, up to maxResults per query.
+int oracle_intersect_shape(oracle_world *o, int n, const int32_t *type, const float *center, const float *rot,
+                           const float *size, const float *margin, const uint32_t *masks, int maxResults,
+                           int32_t *count, int32_t *out)
+{
+    World &w = o->w;
+    std::vector<int32_t> cand(w.bodies.size() + 1);
+    for (int r = 0; r < n; r++) {
+        count[r] = 0;
+        Body q{};
+        q.shape = static_cast<uint8_t>(type[r]);
+        q.pos = load3(center, r);
+        q.q = quat{rot[4 * r], rot[4 * r + 1], rot[4 * r + 2], rot[4 * r + 3]};
+        q.size = load3(size, r);
+        const float m = smax(0.0f, margin[r]);
+        const f3 wx = World::tightHalf(q);
+        f3 lo = q.pos - wx, hi = q.pos + wx;
+        if (m > 0.0f) { lo = lo - f3{m, m, m}; hi = hi + f3{m, m, m}; }
+        const float mn[3] = {lo.x, lo.y, lo.z}, mx[3] = {hi.x, hi.y, hi.z};
+        const int nc = oracle_collect_aabb(o, mn, mx, masks ? masks[r] : 0xFFFFFFFFu, cand.data(),
+                                           static_cast<int>(cand.size()));
+        const float qp[3] = {q.pos.x, q.pos.y, q.pos.z}, qq[4] = {q.q.x, q.q.y, q.q.z, q.q.w},
+                    qs[3] = {q.size.x, q.size.y, q.size.z};
+        for (int k = 0; k < nc && count[r] < maxResults; k++) {
+            const Body &b = w.bodies[cand[k]];
+            const float bp[3] = {b.pos.x, b.pos.y, b.pos.z}, bq[4] = {b.q.x, b.q.y, b.q.z, b.q.w},
+                        bs[3] = {b.size.x, b.size.y, b.size.z};
+            float bas[9]; int32_t keys[8]; float ra[24], rb[24];
+            if (oracle_collide_shapes(q.shape, qp, qq, qs, b.shape, bp, bq, bs, m, bas, keys, ra, rb) <= 0) continue;
+            out[static_cast<size_t>(r) * maxResults + count[r]++] = cand[k];
+        }
+    }
+    return AVBD_OK;
+}
+
 // Solver::pick (solver_query.cpp:29-155) for n rays: every body whose layer passes the mask, an
 // exact sphere or the box bounding the shape (cylinder: its local box), nearest t in units of
 // dir (the earlier body on a tie); body -1 when nothing is hit, else the hit point in the

@@ -78,6 +78,8 @@ def lib():
         "oracle_pick": (C.c_int, [vp, C.c_int, a.f32p, a.f32p, a.u32p, a.i32p, a.f32p]),
         "oracle_contact_report": (C.c_int, [vp, a.i32p, C.c_int, a.i32p, a.f32p, a.f32p, a.i32p]),
         "oracle_contains_point": (C.c_int, [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, C.c_float]),
+        "oracle_intersect_shape": (C.c_int, [vp, C.c_int, a.i32p, a.f32p, a.f32p, a.f32p, a.f32p, a.u32p, C.c_int,
+                                             a.i32p, a.i32p]),
         "oracle_intersect_point": (C.c_int, [vp, C.c_int, a.f32p, a.u32p, C.c_int, a.i32p, a.i32p]),
     }
     for name, (res, args) in sig.items():
@@ -205,6 +207,20 @@ class OracleWorld:
         m = None if masks is None else np.ascontiguousarray(masks, np.uint32)
         self.L.oracle_intersect_point(self.h, n, pts.ctypes.data_as(a.f32p), None if m is None else m.ctypes.data_as(a.u32p),
                                       max_results, count.ctypes.data_as(a.i32p), out.ctypes.data_as(a.i32p))
+        return count, out
+
+    def intersect_shape(self, types, centers, rotations, sizes, margins, masks=None, max_results=8):
+        """The server's intersect_shape for n query shapes: (count, body indices n x max_results)."""
+        a = self.a
+        t = np.ascontiguousarray(types, np.int32)
+        c, q, s3, m = (np.ascontiguousarray(x, np.float32) for x in (centers, rotations, sizes, margins))
+        n = len(t)
+        count, out = np.zeros(n, np.int32), np.full((n, max_results), -1, np.int32)
+        mk = None if masks is None else np.ascontiguousarray(masks, np.uint32)
+        self.L.oracle_intersect_shape(self.h, n, t.ctypes.data_as(a.i32p), c.ctypes.data_as(a.f32p),
+                                      q.ctypes.data_as(a.f32p), s3.ctypes.data_as(a.f32p), m.ctypes.data_as(a.f32p),
+                                      None if mk is None else mk.ctypes.data_as(a.u32p), max_results,
+                                      count.ctypes.data_as(a.i32p), out.ctypes.data_as(a.i32p))
         return count, out
 
     def contact_report(self, max_reported):

@@ -115,3 +115,31 @@ def test_intersect_point_matches_oracle():
         oc, oo = o.intersect_point(pts, masks, max_results=k)
         assert np.array_equal(gc, oc) and np.array_equal(go, oo)
     assert gc.sum() > 80
+
+
+def test_intersect_shape_matches_oracle():
+    """The server's intersect_shape on the device: random boxes, spheres and cylinders (random
+    rotations, margins 0 or 0.05, layer masks) over the settled round pile, two result caps;
+    a second call with fewer queries reuses the grown buffers."""
+    w, o, b = _worlds()
+    rng = np.random.default_rng(12)
+    centres = w.download()["position"]
+    n = 900
+    t = rng.integers(0, 3, n).astype(np.int32)
+    c = np.concatenate([centres[rng.integers(0, len(centres), n - 200)] + rng.normal(0, 0.5, (n - 200, 3)),
+                        rng.uniform(-10, 10, (200, 3))]).astype(np.float32)
+    qr = rng.normal(size=(n, 4)).astype(np.float32)
+    qr /= np.linalg.norm(qr, axis=1, keepdims=True)
+    s = rng.uniform(0.2, 1.5, (n, 3)).astype(np.float32)
+    m = np.where(rng.random(n) < 0.5, 0.0, 0.05).astype(np.float32)
+    masks = rng.choice(np.array([1, 2, 3, 0xFFFFFFFF], np.uint32), n)
+    for k in (2, 16):
+        gc, go = w.intersect_shape(t, c, qr, s, m, masks, max_results=k)
+        oc, oo = o.intersect_shape(t, c, qr, s, m, masks, max_results=k)
+        assert np.array_equal(gc, oc) and np.array_equal(go, oo)
+    assert gc.sum() > 500 and (gc >= 3).sum() > 20
+    gc, go = w.intersect_shape(t[:50], c[:50], qr[:50], s[:50], m[:50], None, max_results=4)
+    oc, oo = o.intersect_shape(t[:50], c[:50], qr[:50], s[:50], m[:50], None, max_results=4)
+    assert np.array_equal(gc, oc) and np.array_equal(go, oo)
+    with pytest.raises(Exception):
+        w.intersect_shape([7], c[:1], qr[:1], s[:1], m[:1])

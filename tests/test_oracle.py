@@ -997,6 +997,64 @@ def test_intersect_point_closed_forms():
     assert count[0] == 1 and out[0, 0] == 2
 
 
+def test_intersect_shape_closed_forms():
+    """The server's intersect_shape (avbd_direct_space_state3d.cpp:261-306): a sphere sunk 0.05
+    into the upper box of the stack reaches the lower one too (list order: upper first); a mask
+    skips the upper box's layer; a sphere 0.05 above the stack touches nothing until the margin
+    is 0.1; a thin slab at the ground's top face finds every body resting on it and the ground;
+    max_results truncates in list order; a query in the air finds nothing."""
+    o, b = _query_scene()     # list order: cylinder, sphere, upper box, lower box, ground
+    I = [0, 0, 0, 1]
+    q = [(1, [0, 0, 1.05], [0.25] * 3, 0.0, 0xFFFFFFFF), (1, [0, 0, 1.05], [0.25] * 3, 0.0, 1),
+         (1, [0, 0, 2.3], [0.25] * 3, 0.0, 0xFFFFFFFF), (1, [0, 0, 2.3], [0.25] * 3, 0.1, 0xFFFFFFFF),
+         (0, [3, 0, 0.5], [0.5] * 3, 0.0, 0xFFFFFFFF), (2, [-3, 0, 2.1], [0.3, 0.3, 0.4], 0.0, 0xFFFFFFFF),
+         (0, [0, 0, 0.05], [10, 10, 0.2], 0.0, 0xFFFFFFFF), (0, [6, 6, 5], [1, 1, 1], 0.0, 0xFFFFFFFF)]
+    t, c, s, m, mk = (np.array([x[k] for x in q]) for k in range(5))
+    count, out = o.intersect_shape(t, c, [I] * len(q), s, m, mk.astype(np.uint32), max_results=6)
+    expect = [[2, 3], [3], [], [2], [1], [0], [0, 1, 3, 4], []]
+    assert [list(out[i, :count[i]]) for i in range(len(q))] == expect
+    count, out = o.intersect_shape(t[6:7], c[6:7], [I], s[6:7], m[6:7], None, max_results=2)
+    assert count[0] == 2 and list(out[0]) == [0, 1]
+
+
+def test_intersect_shape_is_the_narrow_phase_over_collected_bodies():
+    """Random query shapes over a settled round pile: each result is exactly the bodies that
+    collect_aabb returns for the query's bounds grown by the margin, filtered by the narrow
+    phase (collide_shapes, query first) finding a contact, in list order."""
+    b, j = scenes.round_pile(60, seed=3)
+    o = oracle.OracleWorld(P())
+    o.upload(b, j)
+    o.step(20)
+    st = o.download()
+    rng = np.random.default_rng(8)
+    n = 60
+    t = rng.integers(0, 3, n).astype(np.int32)
+    c = (st["position"][rng.integers(0, len(st["position"]), n)] + rng.normal(0, 0.6, (n, 3))).astype(np.float32)
+    qr = rng.normal(size=(n, 4)).astype(np.float32)
+    qr /= np.linalg.norm(qr, axis=1, keepdims=True)
+    s = rng.uniform(0.2, 1.2, (n, 3)).astype(np.float32)
+    m = np.where(rng.random(n) < 0.5, 0.0, 0.05).astype(np.float32)
+    count, out = o.intersect_shape(t, c, qr, s, m, max_results=64)
+    hits = 0
+    for r in range(n):
+        x, y, z, w = qr[r].astype(np.float64)
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+        half = {0: s[r] * 0.5, 1: np.full(3, s[r][0]), 2: np.array([s[r][0], s[r][0], s[r][2] * 0.5])}[int(t[r])]
+        wx = np.abs(R) @ half + m[r] + 1e-4
+        want = [k for k in range(len(b["mass"]))
+                if oracle.collide_shapes(t[r], c[r], qr[r], s[r], b["shape"][k], st["position"][k], st["rotation"][k],
+                                         b["size"][k], m[r])[0] > 0]
+        # a contact within the margin implies the grown bounds overlap, so the bounds pre-filter
+        # drops no hit; every hit is also among the bodies whose bounds the query's reach
+        near = set(o.collect_aabb(c[r] - wx, c[r] + wx))
+        assert set(want) <= near
+        assert list(out[r, :count[r]]) == want, r
+        hits += count[r]
+    assert hits > 20
+
+
 # ---------------------------------------------------------------- contact grace ----
 def test_contact_grace_parks_and_revives():
     """manifold.cpp:249-378 with solver_solve.cpp:66-86: with contact_grace_ticks = 3, a box that
