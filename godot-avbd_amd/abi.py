@@ -260,6 +260,8 @@ def load_library(path=LIB_PATH):
         "avbd_world_pick": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, i32p, f32p]),
         "avbd_world_contact_report": (C.c_int32, [vp, i32p, C.c_int32, i32p, f32p, f32p, i32p]),
         "avbd_world_intersect_point": (C.c_int32, [vp, C.c_int32, f32p, u32p, C.c_int32, i32p, i32p]),
+        "avbd_world_cast_motion": (C.c_int32, [vp, C.c_int32, i32p, f32p, f32p, f32p, f32p, f32p, u32p, i32p, f32p,
+                                               f32p, f32p]),
         "avbd_world_intersect_shape": (C.c_int32, [vp, C.c_int32, i32p, f32p, f32p, f32p, f32p, u32p, C.c_int32, i32p,
                                                    i32p]),
         "avbd_batch_contact_report": (C.c_int32, [vp, C.c_int32, i32p, C.c_int32, i32p, f32p, f32p, i32p]),

@@ -409,14 +409,7 @@ struct Engine {
             qOutCap = std::max(outN, 2 * qOutCap);
             if ((st = regrow(qOut, qOutCap)) != AVBD_OK) return st;
         }
-        if (static_cast<size_t>(n) > qShapeCap) {
-            qShapeCap = std::max<size_t>(n, 2 * qShapeCap);
-            const size_t rows = (qShapeCap + CSLICE - 1) / CSLICE * CSLICE * MAXC;
-            if ((st = regrow(qShapes, 4 * qShapeCap)) != AVBD_OK || (st = regrow(qCRA, rows)) != AVBD_OK ||
-                (st = regrow(qCRB, rows)) != AVBD_OK || (st = regrow(qCMid, rows)) != AVBD_OK ||
-                (st = regrow(qSpA, rows)) != AVBD_OK || (st = regrow(qSpB, rows)) != AVBD_OK)
-                return st;
-        }
+        if ((st = ensureShapeQuery(n)) != AVBD_OK) return st;
         std::vector<float4> in(4 * static_cast<size_t>(n));
         for (int r = 0; r < n; r++) {
             float tf, mf;
@@ -440,6 +433,63 @@ struct Engine {
         HIPCHK(hipStreamSynchronize(stream));
         for (int r = 0; r < n; r++)
             for (int c = 0; c < maxRes; c++) out[static_cast<size_t>(r) * maxRes + c] = c < count[r] ? o[static_cast<size_t>(r) * maxRes + c] : -1;
+        return AVBD_OK;
+    }
+
+    avbd_status ensureShapeQuery(int n)
+    {
+        if (static_cast<size_t>(n) <= qShapeCap) return AVBD_OK;
+        qShapeCap = std::max<size_t>(n, 2 * qShapeCap);
+        const size_t rows = (qShapeCap + CSLICE - 1) / CSLICE * CSLICE * MAXC;
+        avbd_status st;
+        if ((st = regrow(qShapes, 4 * qShapeCap)) != AVBD_OK || (st = regrow(qCRA, rows)) != AVBD_OK ||
+            (st = regrow(qCRB, rows)) != AVBD_OK || (st = regrow(qCMid, rows)) != AVBD_OK ||
+            (st = regrow(qSpA, rows)) != AVBD_OK || (st = regrow(qSpB, rows)) != AVBD_OK)
+            return st;
+        return AVBD_OK;
+    }
+
+    avbd_status castMotion(int w, int n, const int32_t *type, const float *center, const float *rot, const float *size,
+                           const float *motion, const float *margin, const uint32_t *masks, int32_t *body, float *frac,
+                           float *point, float *normal)
+    {
+        const WorldInfo &wi = worlds[w];
+        if (!wi.active) return fail(AVBD_ERR_INVALID, "world is not active");
+        if (n < 0 || (n > 0 && (!type || !center || !rot || !size || !motion || !margin || !body || !frac)))
+            return fail(AVBD_ERR_INVALID, "bad arguments");
+        for (int r = 0; r < n; r++)
+            if (type[r] < AVBD_SHAPE_BOX || type[r] > AVBD_SHAPE_CYLINDER) return fail(AVBD_ERR_INVALID, "unknown shape");
+        if (n == 0) return AVBD_OK;
+        avbd_status st = ensureQuery(n);
+        if (st != AVBD_OK || (st = ensureShapeQuery(n)) != AVBD_OK) return st;
+        std::vector<float4> in(4 * static_cast<size_t>(n));
+        for (int r = 0; r < n; r++) {
+            float tf, mf;
+            const int32_t t = type[r];
+            const uint32_t m = masks ? masks[r] : 0xFFFFFFFFu;
+            std::memcpy(&tf, &t, sizeof(tf));
+            std::memcpy(&mf, &m, sizeof(mf));
+            in[4 * r] = make_float4(center[3 * r], center[3 * r + 1], center[3 * r + 2], tf);
+            in[4 * r + 1] = make_float4(rot[4 * r], rot[4 * r + 1], rot[4 * r + 2], rot[4 * r + 3]);
+            in[4 * r + 2] = make_float4(size[3 * r], size[3 * r + 1], size[3 * r + 2], margin[r]);
+            in[4 * r + 3] = make_float4(motion[3 * r], motion[3 * r + 1], motion[3 * r + 2], mf);
+        }
+        HIPCHK(hipMemcpyAsync(qShapes, in.data(), sizeof(float4) * in.size(), hipMemcpyHostToDevice, stream));
+        if (wi.bodyCount > 0) k_query_bounds<<<nblk(wi.bodyCount), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, qBounds);
+        k_cast_motion<<<nblk(n, NARROW_TPB), NARROW_TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qShapes, qBounds, qBody,
+                                                                      qA, qB, qCRA, qCRB, qCMid, qSpA, qSpB);
+        HIPCHK(hipGetLastError());
+        std::vector<float4> a(n), b(n);
+        HIPCHK(hipMemcpyAsync(body, qBody, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(a.data(), qA, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(b.data(), qB, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int r = 0; r < n; r++) {
+            frac[2 * r] = a[r].w;
+            frac[2 * r + 1] = b[r].w;
+            if (point) { point[3 * r] = a[r].x; point[3 * r + 1] = a[r].y; point[3 * r + 2] = a[r].z; }
+            if (normal) { normal[3 * r] = b[r].x; normal[3 * r + 1] = b[r].y; normal[3 * r + 2] = b[r].z; }
+        }
         return AVBD_OK;
     }
 
@@ -2168,6 +2218,14 @@ avbd_status avbd_world_intersect_shape(avbd_world *world, int32_t n, const int32
 {
     if (!world) return fail(AVBD_ERR_INVALID, "world is null");
     return world->eng.intersectShape(0, n, type, center, rotation, size, margin, masks, max_results, count, out);
+}
+
+avbd_status avbd_world_cast_motion(avbd_world *world, int32_t n, const int32_t *type, const float *center,
+                                   const float *rotation, const float *size, const float *motion, const float *margin,
+                                   const uint32_t *masks, int32_t *body, float *fraction, float *point, float *normal)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    return world->eng.castMotion(0, n, type, center, rotation, size, motion, margin, masks, body, fraction, point, normal);
 }
 
 avbd_status avbd_world_contact_report(avbd_world *world, const int32_t *max_reported, int32_t stride, int32_t *count,

@@ -321,6 +321,179 @@ __global__ void __launch_bounds__(NARROW_TPB) k_intersect_shape(State S, int bas
     cnt[r] = c;
 }
 
+// ---- shape sweeps (query_shape.cpp:213-258, 402-498; the server's cast_motion) ----
+struct SweepShape {
+    int type;
+    v3 c;
+    qt q;
+    float4 size;   // w = 0
+};
+
+// query::shapeAabb's half extent (query_shape.cpp:371-394)
+AD v3 shapeHalf(int type, qt q, float4 s)
+{
+    const v3 h = type == AVBD_SHAPE_SPHERE     ? v3{s.x, s.x, s.x}
+                 : type == AVBD_SHAPE_CYLINDER ? v3{s.x, s.x, s.z * 0.5f}
+                                               : v3{s.x, s.y, s.z} * 0.5f;
+    const m3 R = qmat(q);
+    return v3{(fabsf(R.c[0][0]) * h.x + fabsf(R.c[1][0]) * h.y) + fabsf(R.c[2][0]) * h.z,
+              (fabsf(R.c[0][1]) * h.x + fabsf(R.c[1][1]) * h.y) + fabsf(R.c[2][1]) * h.z,
+              (fabsf(R.c[0][2]) * h.x + fabsf(R.c[1][2]) * h.y) + fabsf(R.c[2][2]) * h.z};
+}
+
+// sweptAabbInterval (query_shape.cpp:213-241)
+AD bool sweptInterval(v3 aMin, v3 aMax, v3 bMin, v3 bMax, v3 mv, float &rEnter, float &rExit)
+{
+    float enter = 0.0f, exit = 1.0f;
+    for (int i = 0; i < 3; i++) {
+        const float m = mv[i];
+        if (fabsf(m) <= 1.0e-6f) {
+            if (aMax[i] < bMin[i] || aMin[i] > bMax[i]) return false;
+            continue;
+        }
+        float t0 = (bMin[i] - aMax[i]) / m, t1 = (bMax[i] - aMin[i]) / m;
+        if (t0 > t1) { const float x = t0; t0 = t1; t1 = x; }
+        enter = smax(enter, t0);
+        exit = smin(exit, t1);
+        if (enter > exit) return false;
+    }
+    if (exit < 0.0f || enter > 1.0f) return false;
+    rEnter = smax(0.0f, enter);
+    rExit = smin(1.0f, exit);
+    return rEnter <= rExit;
+}
+
+// contactAt (query_shape.cpp:243-258): the narrow phase with the query moved to c + mv * t; the
+// contact with the largest |rA - rB| (the first of equals) and the basis normal.
+AD bool sweepContact(const State &S, const SweepShape &a, v3 mv, float t, int bt, v3 bp, qt bq, float4 bs, float margin,
+                     Contacts &cs, const Poly &c0, const Poly &c1, v3 &normal, v3 &rA)
+{
+    const v3 c = a.c + mv * t;
+    m3 B;
+    bool wrote;
+    const int n = a.type != AVBD_SHAPE_BOX || bt != AVBD_SHAPE_BOX
+                      ? collideRound(a.type, bt, makeRShp(c, a.q, a.size), makeRShp(bp, bq, bs), S.rim, margin, B, wrote, cs)
+                      : collideBoxes(c, a.q, v3{a.size.x, a.size.y, a.size.z} * 0.5f, bp, bq, v3{bs.x, bs.y, bs.z} * 0.5f,
+                                     margin, B, wrote, cs, c0, c1);
+    if (n <= 0) return false;
+    int best = 0;
+    float bestL = -1.0f;
+    for (int k = 0; k < n; k++) {
+        const float l = length2(ld3(cs.ra[crow(k)]) - ld3(cs.rb[crow(k)]));
+        if (k == 0 || bestL < l) { best = k; bestL = l; }
+    }
+    normal = v3{B.c[0][0], B.c[1][0], B.c[2][0]};
+    rA = ld3(cs.ra[crow(best)]);
+    return true;
+}
+
+// sweepShape (query_shape.cpp:402-498) without the depth: the AABB interval, 32 samples over
+// it, 12 bisections of the first bracket, the contact re-read at the first colliding pose.
+AD bool sweepShape(const State &S, const SweepShape &a, v3 mv, int bt, v3 bp, qt bq, float4 bs, float margin,
+                   Contacts &cs, const Poly &c0, const Poly &c1, float &safe, float &unsafe, v3 &point, v3 &normal)
+{
+    v3 rA;
+    if (length2(mv) <= 1.0e-6f * 1.0e-6f) {
+        if (!sweepContact(S, a, v3{0.0f, 0.0f, 0.0f}, 0.0f, bt, bp, bq, bs, margin, cs, c0, c1, normal, rA)) return false;
+        safe = unsafe = 0.0f;
+        point = a.c + qrot(a.q, rA);
+        return true;
+    }
+    const v3 ha = shapeHalf(a.type, a.q, a.size), hb = shapeHalf(bt, bq, bs);
+    v3 aMin = a.c - ha, aMax = a.c + ha;
+    if (margin > 0.0f) { aMin = aMin - v3{margin, margin, margin}; aMax = aMax + v3{margin, margin, margin}; }
+    float enter = 0.0f, exit = 1.0f;
+    if (!sweptInterval(aMin, aMax, bp - hb, bp + hb, mv, enter, exit)) return false;
+    float low = enter, high = exit;
+    if (sweepContact(S, a, mv, enter, bt, bp, bq, bs, margin, cs, c0, c1, normal, rA)) {
+        safe = unsafe = enter;
+        point = (a.c + mv * enter) + qrot(a.q, rA);
+        return true;
+    }
+    bool found = false;
+    float previous = enter;
+    for (int i = 1; i <= 32; i++) {
+        const float t = enter + (exit - enter) * (static_cast<float>(i) / 32.0f);
+        if (sweepContact(S, a, mv, t, bt, bp, bq, bs, margin, cs, c0, c1, normal, rA)) {
+            low = previous;
+            high = t;
+            found = true;
+            break;
+        }
+        previous = t;
+    }
+    if (!found && sweepContact(S, a, mv, exit, bt, bp, bq, bs, margin, cs, c0, c1, normal, rA)) {
+        low = previous;
+        high = exit;
+        found = true;
+    }
+    if (!found) return false;
+    for (int i = 0; i < 12; i++) {
+        const float mid = (low + high) * 0.5f;
+        if (sweepContact(S, a, mv, mid, bt, bp, bq, bs, margin, cs, c0, c1, normal, rA)) high = mid;
+        else low = mid;
+    }
+    if (!sweepContact(S, a, mv, high, bt, bp, bq, bs, margin, cs, c0, c1, normal, rA)) return false;
+    safe = low;
+    unsafe = high;
+    point = (a.c + mv * high) + qrot(a.q, rA);
+    return true;
+}
+
+// The bodies part of the server's cast_motion (avbd_direct_space_state3d.cpp:310-385): query r
+// is qin[4r] = (centre, type bits), qin[4r+1] = rotation, qin[4r+2] = (size, margin),
+// qin[4r+3] = (motion, mask bits).  Candidates: bodies whose bounds overlap the query's bounds
+// swept by the motion and grown by the margin, in list order; the sweep with the smallest safe
+// fraction (the first of equals) wins: body[r] (-1: none), hitA[r] = (point, safe),
+// hitB[r] = (normal, unsafe), safe = unsafe = 1 when none.
+__global__ void __launch_bounds__(NARROW_TPB) k_cast_motion(State S, int base, int count, int n, const float4 *qin,
+                                                           const float4 *bounds, int *body, float4 *hitA, float4 *hitB,
+                                                           float4 *cra, float4 *crb, float4 *cmid, float4 *spA, float4 *spB)
+{
+    __shared__ NarrowPolys poly;
+    const int r = GID;
+    if (r >= n) return;
+    const float4 c4 = qin[4 * r], r4 = qin[4 * r + 1], s4 = qin[4 * r + 2], m4 = qin[4 * r + 3];
+    const SweepShape a{__float_as_int(c4.w), ld3(c4), ldq(r4), make_float4(s4.x, s4.y, s4.z, 0.0f)};
+    const v3 mv = ld3(m4);
+    const float m = smax(0.0f, s4.w);
+    const uint32_t mask = __float_as_uint(m4.w);
+    const v3 wx = shapeHalf(a.type, a.q, a.size);
+    v3 mn = a.c - wx, mx = a.c + wx;
+    mn = v3{smin(mn.x, mn.x + mv.x), smin(mn.y, mn.y + mv.y), smin(mn.z, mn.z + mv.z)};
+    mx = v3{smax(mx.x, mx.x + mv.x), smax(mx.y, mx.y + mv.y), smax(mx.z, mx.z + mv.z)};
+    if (m > 0.0f) { mn = mn - v3{m, m, m}; mx = mx + v3{m, m, m}; }
+    const size_t d = stageRow(r, 0);
+    const int l = threadIdx.x;
+    const Poly c0{&poly[0][0][l], &poly[0][1][l], &poly[0][2][l], spA + d};
+    const Poly c1{&poly[1][0][l], &poly[1][1][l], &poly[1][2][l], spB + d};
+    int bestBody = -1;
+    float bestSafe = 1.0f, bestUnsafe = 1.0f;
+    v3 bestPoint{0.0f, 0.0f, 0.0f}, bestNormal{0.0f, 0.0f, 0.0f};
+    for (int j = 0; j < count; j++) {
+        const float4 lo = bounds[2 * j], hi = bounds[2 * j + 1];
+        if ((__float_as_uint(lo.w) & mask) == 0) continue;
+        if (!boxesOverlap(ld3(lo), ld3(hi), mn, mx)) continue;
+        const int i = base + j;
+        Contacts cs{0, cra + d, crb + d, cmid + d};
+        float safe, unsafe;
+        v3 point, normal;
+        if (!sweepShape(S, a, mv, S.shape[i], ld3(S.pos[i]), ldq(S.rot[i]), S.sizeR[i], m, cs, c0, c1, safe, unsafe, point,
+                        normal))
+            continue;
+        if (bestBody < 0 || safe < bestSafe) {
+            bestBody = j;
+            bestSafe = safe;
+            bestUnsafe = unsafe;
+            bestPoint = point;
+            bestNormal = normal;
+        }
+    }
+    body[r] = bestBody;
+    hitA[r] = st3(bestPoint, bestSafe);
+    hitB[r] = st3(bestNormal, bestUnsafe);
+}
+
 // Solver::pick (solver_query.cpp:29-155): rays[2r] = (origin, mask bits), rays[2r+1] = dir.
 // Every body that passes the mask: an exact sphere test, else the box bounding the shape; the
 // nearest t (the earlier body on a tie).  local[r] = hit point in the body's frame.

@@ -124,6 +124,23 @@ class World:
                                                       count.ctypes.data_as(abi.i32p), out.ctypes.data_as(abi.i32p)))
         return count, out
 
+    def cast_motion(self, types, centers, rotations, sizes, motions, margins, masks=None):
+        """The server's cast_motion for n query shapes: dict of body (-1: none), safe, unsafe,
+        point, normal."""
+        t = np.ascontiguousarray(types, np.int32)
+        c, q, s3, mv, m = (np.ascontiguousarray(x, np.float32) for x in (centers, rotations, sizes, motions, margins))
+        n = len(t)
+        body, frac = np.zeros(n, np.int32), np.zeros((n, 2), np.float32)
+        point, normal = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+        mk = None if masks is None else np.ascontiguousarray(masks, np.uint32)
+        abi.check(self.lib.avbd_world_cast_motion(self.h, n, t.ctypes.data_as(abi.i32p), c.ctypes.data_as(abi.f32p),
+                                                  q.ctypes.data_as(abi.f32p), s3.ctypes.data_as(abi.f32p),
+                                                  mv.ctypes.data_as(abi.f32p), m.ctypes.data_as(abi.f32p),
+                                                  None if mk is None else mk.ctypes.data_as(abi.u32p),
+                                                  body.ctypes.data_as(abi.i32p), frac.ctypes.data_as(abi.f32p),
+                                                  point.ctypes.data_as(abi.f32p), normal.ctypes.data_as(abi.f32p)))
+        return {"body": body, "safe": frac[:, 0], "unsafe": frac[:, 1], "point": point, "normal": normal}
+
     def contact_report(self, max_reported):
         """Per-body contact reports: (count, position, normal, other), stride max(max_reported)."""
         mr = np.ascontiguousarray(max_reported, np.int32)

@@ -338,6 +338,17 @@ avbd_status avbd_world_intersect_point(avbd_world *world, int32_t n, const float
 avbd_status avbd_world_intersect_shape(avbd_world *world, int32_t n, const int32_t *type, const float *center,
                                        const float *rotation, const float *size, const float *margin,
                                        const uint32_t *masks, int32_t max_results, int32_t *count, int32_t *out);
+/* n query shapes swept by a motion against the bodies at their current pose: replaces the
+ * bodies part of the server's cast_motion (src/server/avbd_direct_space_state3d.cpp:310-385 with
+ * query::sweepShape, src/avbd/query/query_shape.cpp:402-498).  Query r as in
+ * avbd_world_intersect_shape plus motion[3r..].  Candidates are the bodies whose layer passes
+ * masks[r] and whose bounds overlap the query's bounds swept by the motion and grown by the
+ * margin, in list order; the sweep with the smallest safe fraction (the first of equals) wins:
+ * body[r] (-1: none), fraction[2r..] = (safe, unsafe) ((1, 1) when none), point[3r..] and
+ * normal[3r..] (optional) at the first colliding pose. */
+avbd_status avbd_world_cast_motion(avbd_world *world, int32_t n, const int32_t *type, const float *center,
+                                   const float *rotation, const float *size, const float *motion, const float *margin,
+                                   const uint32_t *masks, int32_t *body, float *fraction, float *point, float *normal);
 /* Replaces `Solver::pick(origin, dir, local, mask)` (src/avbd/solver.h:161-164,
  * solver_query.cpp:29-155) for n rays: body[r] = list index of the nearest body (-1 none),
  * local[3r..] = the hit point in its frame (zeros for none); masks null = all layers. */

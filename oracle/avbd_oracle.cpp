@@ -2475,6 +2475,195 @@ int oracle_intersect_shape(oracle_world *o, int n, const int32_t *type, const fl
     return AVBD_OK;
 }
 
+// ---- shape sweeps (query_shape.cpp:213-258, 340-369, 402-498; the server's cast_motion) ----
+namespace {
+constexpr float kQueryEps = 1.0e-6f;
+
+struct QShape { int type; f3 c; quat q; f3 size; };
+
+// sweptAabbInterval (query_shape.cpp:213-241): the t in [0, 1] where the moving box overlaps.
+bool sweptInterval(f3 aMin, f3 aMax, f3 bMin, f3 bMax, f3 motion, float &rEnter, float &rExit)
+{
+    float enter = 0.0f, exit = 1.0f;
+    const float amn[3] = {aMin.x, aMin.y, aMin.z}, amx[3] = {aMax.x, aMax.y, aMax.z};
+    const float bmn[3] = {bMin.x, bMin.y, bMin.z}, bmx[3] = {bMax.x, bMax.y, bMax.z};
+    const float mv[3] = {motion.x, motion.y, motion.z};
+    for (int i = 0; i < 3; i++) {
+        if (std::fabs(mv[i]) <= kQueryEps) {
+            if (amx[i] < bmn[i] || amn[i] > bmx[i]) return false;
+            continue;
+        }
+        float t0 = (bmn[i] - amx[i]) / mv[i], t1 = (bmx[i] - amn[i]) / mv[i];
+        if (t0 > t1) std::swap(t0, t1);
+        enter = smax(enter, t0);
+        exit = smin(exit, t1);
+        if (enter > exit) return false;
+    }
+    if (exit < 0.0f || enter > 1.0f) return false;
+    rEnter = smax(0.0f, enter);
+    rExit = smin(1.0f, exit);
+    return rEnter <= rExit;
+}
+
+// contactAt (query_shape.cpp:243-258): the narrow phase at base + motion * t; the contact with
+// the largest |rA - rB| (the first of equals) and the basis normal.
+bool contactAt(const QShape &a, f3 motion, float t, const QShape &b, float margin, f3 &normal, f3 &rA)
+{
+    const f3 c = a.c + motion * t;
+    const float pa[3] = {c.x, c.y, c.z}, qa[4] = {a.q.x, a.q.y, a.q.z, a.q.w}, sa[3] = {a.size.x, a.size.y, a.size.z};
+    const float pb[3] = {b.c.x, b.c.y, b.c.z}, qb[4] = {b.q.x, b.q.y, b.q.z, b.q.w}, sb[3] = {b.size.x, b.size.y, b.size.z};
+    float basis[9], ra[24], rb[24];
+    int32_t keys[8];
+    const int n = oracle_collide_shapes(a.type, pa, qa, sa, b.type, pb, qb, sb, margin, basis, keys, ra, rb);
+    if (n <= 0) return false;
+    int best = 0;
+    float bestL = -1.0f;
+    for (int k = 0; k < n; k++) {
+        const float l = length2(f3{ra[3 * k], ra[3 * k + 1], ra[3 * k + 2]} - f3{rb[3 * k], rb[3 * k + 1], rb[3 * k + 2]});
+        if (k == 0 || bestL < l) { best = k; bestL = l; }
+    }
+    normal = f3{basis[0], basis[1], basis[2]};
+    rA = f3{ra[3 * best], ra[3 * best + 1], ra[3 * best + 2]};
+    return true;
+}
+
+// supportRadius (query_shape.cpp:340-360)
+float supportRadius(const QShape &s, f3 axis)
+{
+    const f3 n = normalize(axis);
+    if (s.type == AVBD_SHAPE_SPHERE) return s.size.x;
+    if (s.type == AVBD_SHAPE_CYLINDER) {
+        const float axial = std::fabs(dot(qrot(s.q, f3{0, 0, 1}), n));
+        const float radial = std::sqrt(smax(0.0f, 1.0f - axial * axial));
+        return s.size.x * radial + (s.size.z * 0.5f) * axial;
+    }
+    const m3 R = qmat(s.q);
+    const f3 h = s.size * 0.5f;
+    return (h.x * std::fabs(dot(f3{R[0][0], R[1][0], R[2][0]}, n)) + h.y * std::fabs(dot(f3{R[0][1], R[1][1], R[2][1]}, n))) +
+           h.z * std::fabs(dot(f3{R[0][2], R[1][2], R[2][2]}, n));
+}
+
+// penetrationDepth (query_shape.cpp:362-369)
+float penetrationDepth(const QShape &a, const QShape &b, f3 normal)
+{
+    const float len = length(normal);
+    if (len <= kQueryEps) return 0.0f;
+    const f3 n = normal * (1.0f / len);
+    return (supportRadius(a, n) + supportRadius(b, n)) - std::fabs(dot(b.c - a.c, n));
+}
+
+Body asBody(const QShape &s)
+{
+    Body b{};
+    b.shape = static_cast<uint8_t>(s.type);
+    b.pos = s.c;
+    b.q = s.q;
+    b.size = s.size;
+    return b;
+}
+
+// sweepShape (query_shape.cpp:402-498): out = safe, unsafe, depth, point, normal.
+bool sweepShape(const QShape &a, f3 motion, const QShape &b, float margin, float out[9])
+{
+    f3 normal{}, rA{};
+    auto emit = [&](float safe, float unsafe, float t) {
+        QShape posed = a;
+        posed.c = a.c + motion * t;
+        const f3 p = posed.c + qrot(posed.q, rA);
+        const float v[9] = {safe, unsafe, penetrationDepth(posed, b, normal), p.x, p.y, p.z, normal.x, normal.y, normal.z};
+        for (int k = 0; k < 9; k++) out[k] = v[k];
+        return true;
+    };
+    if (length2(motion) <= kQueryEps * kQueryEps) {
+        if (!contactAt(a, f3{0, 0, 0}, 0.0f, b, margin, normal, rA)) return false;
+        QShape posed = a;
+        const f3 p = posed.c + qrot(posed.q, rA);
+        const float v[9] = {0.0f, 0.0f, penetrationDepth(a, b, normal), p.x, p.y, p.z, normal.x, normal.y, normal.z};
+        for (int k = 0; k < 9; k++) out[k] = v[k];
+        return true;
+    }
+    const f3 ha = World::tightHalf(asBody(a)), hb = World::tightHalf(asBody(b));
+    f3 aMin = a.c - ha, aMax = a.c + ha;
+    if (margin > 0.0f) { aMin = aMin - f3{margin, margin, margin}; aMax = aMax + f3{margin, margin, margin}; }
+    float enter = 0.0f, exit = 1.0f;
+    if (!sweptInterval(aMin, aMax, b.c - hb, b.c + hb, motion, enter, exit)) return false;
+    if (contactAt(a, motion, enter, b, margin, normal, rA)) return emit(enter, enter, enter);
+    constexpr int kSamples = 32;
+    float low = enter, high = exit, previous = enter;
+    bool found = false;
+    for (int i = 1; i <= kSamples; i++) {
+        const float t = enter + (exit - enter) * (static_cast<float>(i) / static_cast<float>(kSamples));
+        if (contactAt(a, motion, t, b, margin, normal, rA)) { low = previous; high = t; found = true; break; }
+        previous = t;
+    }
+    if (!found && contactAt(a, motion, exit, b, margin, normal, rA)) { low = previous; high = exit; found = true; }
+    if (!found) return false;
+    for (int i = 0; i < 12; i++) {
+        const float mid = (low + high) * 0.5f;
+        if (contactAt(a, motion, mid, b, margin, normal, rA)) high = mid;
+        else low = mid;
+    }
+    if (!contactAt(a, motion, high, b, margin, normal, rA)) return false;
+    return emit(low, high, high);
+}
+} // namespace
+
+extern "C" int oracle_sweep_shape(int typeA, const float posA[3], const float rotA[4], const float sizeA[3],
+                                  const float motion[3], int typeB, const float posB[3], const float rotB[4],
+                                  const float sizeB[3], float margin, float out[9])
+{
+    const QShape a{typeA, load3(posA, 0), quat{rotA[0], rotA[1], rotA[2], rotA[3]}, load3(sizeA, 0)};
+    const QShape b{typeB, load3(posB, 0), quat{rotB[0], rotB[1], rotB[2], rotB[3]}, load3(sizeB, 0)};
+    return sweepShape(a, load3(motion, 0), b, margin, out) ? 1 : 0;
+}
+
+// The bodies part of the server's cast_motion (avbd_direct_space_state3d.cpp:310-385) for n
+// query shapes: candidates from collectAABB over the shape's bounds swept by the motion and
+// grown by the margin, in list order; the sweep with the smallest safe fraction (the first of
+// equals) wins.  Per query: body[r] (-1: none), frac[2r..] = (safe, unsafe) (1, 1 when none),
+// point[3r..], normal[3r..].
+int oracle_cast_motion(oracle_world *o, int n, const int32_t *type, const float *center, const float *rot,
+                       const float *size, const float *motion, const float *margin, const uint32_t *masks,
+                       int32_t *body, float *frac, float *point, float *normal)
+{
+    World &w = o->w;
+    std::vector<int32_t> cand(w.bodies.size() + 1);
+    for (int r = 0; r < n; r++) {
+        body[r] = -1;
+        frac[2 * r] = frac[2 * r + 1] = 1.0f;
+        const QShape q{type[r], load3(center, r), quat{rot[4 * r], rot[4 * r + 1], rot[4 * r + 2], rot[4 * r + 3]},
+                       load3(size, r)};
+        const f3 mv = load3(motion, r);
+        const float m = smax(0.0f, margin[r]);
+        const f3 wx = World::tightHalf(asBody(q));
+        f3 lo = q.c - wx, hi = q.c + wx;
+        lo = f3{smin(lo.x, lo.x + mv.x), smin(lo.y, lo.y + mv.y), smin(lo.z, lo.z + mv.z)};
+        hi = f3{smax(hi.x, hi.x + mv.x), smax(hi.y, hi.y + mv.y), smax(hi.z, hi.z + mv.z)};
+        if (m > 0.0f) { lo = lo - f3{m, m, m}; hi = hi + f3{m, m, m}; }
+        const float mn[3] = {lo.x, lo.y, lo.z}, mx[3] = {hi.x, hi.y, hi.z};
+        const int nc = oracle_collect_aabb(o, mn, mx, masks ? masks[r] : 0xFFFFFFFFu, cand.data(),
+                                           static_cast<int>(cand.size()));
+        float best[9];
+        bool found = false;
+        for (int k = 0; k < nc; k++) {
+            const Body &b = w.bodies[cand[k]];
+            const QShape t{b.shape, b.pos, b.q, b.size};
+            float hit[9];
+            if (!sweepShape(q, mv, t, m, hit)) continue;
+            if (!found || hit[0] < best[0]) {
+                found = true;
+                body[r] = cand[k];
+                for (int e = 0; e < 9; e++) best[e] = hit[e];
+            }
+        }
+        if (!found) continue;
+        frac[2 * r] = best[0];
+        frac[2 * r + 1] = best[1];
+        for (int e = 0; e < 3; e++) { point[3 * r + e] = best[3 + e]; normal[3 * r + e] = best[6 + e]; }
+    }
+    return AVBD_OK;
+}
+
 // Solver::pick (solver_query.cpp:29-155) for n rays: every body whose layer passes the mask, an
 // exact sphere or the box bounding the shape (cylinder: its local box), nearest t in units of
 // dir (the earlier body on a tie); body -1 when nothing is hit, else the hit point in the

@@ -49,6 +49,12 @@ int oracle_intersect_point(oracle_world *o, int n, const float *points, const ui
 int oracle_intersect_shape(oracle_world *o, int n, const int32_t *type, const float *center, const float *rot,
                            const float *size, const float *margin, const uint32_t *masks, int maxResults,
                            int32_t *count, int32_t *out);
+int oracle_sweep_shape(int typeA, const float posA[3], const float rotA[4], const float sizeA[3],
+                       const float motion[3], int typeB, const float posB[3], const float rotB[4],
+                       const float sizeB[3], float margin, float out[9]);
+int oracle_cast_motion(oracle_world *o, int n, const int32_t *type, const float *center, const float *rot,
+                       const float *size, const float *motion, const float *margin, const uint32_t *masks,
+                       int32_t *body, float *frac, float *point, float *normal);
 int oracle_solve6(const float *a36, const float *b6, float *x6);
 int oracle_collide_boxes(const float posA[3], const float rotA[4], const float sizeA[3],
                          const float posB[3], const float rotB[4], const float sizeB[3],

@@ -78,6 +78,10 @@ def lib():
         "oracle_pick": (C.c_int, [vp, C.c_int, a.f32p, a.f32p, a.u32p, a.i32p, a.f32p]),
         "oracle_contact_report": (C.c_int, [vp, a.i32p, C.c_int, a.i32p, a.f32p, a.f32p, a.i32p]),
         "oracle_contains_point": (C.c_int, [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, C.c_float]),
+        "oracle_sweep_shape": (C.c_int, [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, C.c_int, a.f32p, a.f32p, a.f32p,
+                                         C.c_float, a.f32p]),
+        "oracle_cast_motion": (C.c_int, [vp, C.c_int, a.i32p, a.f32p, a.f32p, a.f32p, a.f32p, a.f32p, a.u32p, a.i32p,
+                                         a.f32p, a.f32p, a.f32p]),
         "oracle_intersect_shape": (C.c_int, [vp, C.c_int, a.i32p, a.f32p, a.f32p, a.f32p, a.f32p, a.u32p, C.c_int,
                                              a.i32p, a.i32p]),
         "oracle_intersect_point": (C.c_int, [vp, C.c_int, a.f32p, a.u32p, C.c_int, a.i32p, a.i32p]),
@@ -108,6 +112,9 @@ def ref_lib():
     R.ref_intersect_ray.restype = C.c_int
     R.ref_intersect_ray.argtypes = [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, a.f32p, C.c_float, a.f32p, a.f32p,
                                     a.f32p, a.u8p]
+    R.ref_sweep_shape.restype = C.c_int
+    R.ref_sweep_shape.argtypes = [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, C.c_int, a.f32p, a.f32p, a.f32p, C.c_float,
+                                  a.f32p]
     R.ref_contains_point.restype = C.c_int
     R.ref_contains_point.argtypes = [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, C.c_float]
     R.ref_solve6.restype = C.c_int
@@ -222,6 +229,22 @@ class OracleWorld:
                                       None if mk is None else mk.ctypes.data_as(a.u32p), max_results,
                                       count.ctypes.data_as(a.i32p), out.ctypes.data_as(a.i32p))
         return count, out
+
+    def cast_motion(self, types, centers, rotations, sizes, motions, margins, masks=None):
+        """The server's cast_motion for n query shapes: dict of body (-1: none), safe, unsafe,
+        point, normal."""
+        a = self.a
+        t = np.ascontiguousarray(types, np.int32)
+        c, q, s3, mv, m = (np.ascontiguousarray(x, np.float32) for x in (centers, rotations, sizes, motions, margins))
+        n = len(t)
+        body, frac = np.zeros(n, np.int32), np.zeros((n, 2), np.float32)
+        point, normal = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+        mk = None if masks is None else np.ascontiguousarray(masks, np.uint32)
+        self.L.oracle_cast_motion(self.h, n, t.ctypes.data_as(a.i32p), c.ctypes.data_as(a.f32p), q.ctypes.data_as(a.f32p),
+                                  s3.ctypes.data_as(a.f32p), mv.ctypes.data_as(a.f32p), m.ctypes.data_as(a.f32p),
+                                  None if mk is None else mk.ctypes.data_as(a.u32p), body.ctypes.data_as(a.i32p),
+                                  frac.ctypes.data_as(a.f32p), point.ctypes.data_as(a.f32p), normal.ctypes.data_as(a.f32p))
+        return {"body": body, "safe": frac[:, 0], "unsafe": frac[:, 1], "point": point, "normal": normal}
 
     def contact_report(self, max_reported):
         """Per-body contact reports: (count, position, normal, other) with a stride of
@@ -396,6 +419,25 @@ def contains_point(t, pos, rot, size, point, margin=0.0):
 def ref_contains_point(t, pos, rot, size, point, margin=0.0):
     R = ref_lib()
     return None if R is None else _contains(R.ref_contains_point, t, pos, rot, size, point, margin)
+
+
+def _sweep(fn, ta, pa, qa, sa, motion, tb, pb, qb, sb, margin):
+    a = _abi()
+    f = [np.ascontiguousarray(x, np.float32) for x in (pa, qa, sa, motion, pb, qb, sb)]
+    p = [x.ctypes.data_as(a.f32p) for x in f]
+    out = np.zeros(9, np.float32)
+    hit = fn(int(ta), p[0], p[1], p[2], p[3], int(tb), p[4], p[5], p[6], float(margin), out.ctypes.data_as(a.f32p))
+    return bool(hit), out
+
+
+def sweep_shape(ta, pa, qa, sa, motion, tb, pb, qb, sb, margin=0.0):
+    """query::sweepShape: (hit, [safe, unsafe, depth, point xyz, normal xyz])."""
+    return _sweep(lib().oracle_sweep_shape, ta, pa, qa, sa, motion, tb, pb, qb, sb, margin)
+
+
+def ref_sweep_shape(ta, pa, qa, sa, motion, tb, pb, qb, sb, margin=0.0):
+    R = ref_lib()
+    return None if R is None else _sweep(R.ref_sweep_shape, ta, pa, qa, sa, motion, tb, pb, qb, sb, margin)
 
 
 def ref_intersect_ray(t, pos, rot, size, origin, direction, max_t):

@@ -84,6 +84,32 @@ extern "C" int ref_contains_point(int type, const float pos[3], const float rot[
     return query::containsPoint(sh, float3{point[0], point[1], point[2]}, margin) ? 1 : 0;
 }
 
+// query::sweepShape of shape a moved by motion against shape b; out = safe, unsafe, depth,
+// point, normal.
+extern "C" int ref_sweep_shape(int typeA, const float posA[3], const float rotA[4], const float sizeA[3],
+                               const float motion[3], int typeB, const float posB[3], const float rotB[4],
+                               const float sizeB[3], float margin, float out[9])
+{
+    using namespace avbd;
+    auto shape = [](int type, const float *p, const float *q, const float *s) {
+        Shape sh;
+        sh.type = static_cast<ShapeType>(type);
+        sh.center = float3{p[0], p[1], p[2]};
+        sh.rotation = quat{q[0], q[1], q[2], q[3]};
+        sh.half = float3{s[0], s[1], s[2]} * 0.5f;
+        sh.radius = s[0];
+        sh.halfHeight = s[2] * 0.5f;
+        sh.axis = qrot(sh.rotation, float3{0, 0, 1});
+        return sh;
+    };
+    const query::SweepHit h = query::sweepShape(shape(typeA, posA, rotA, sizeA), float3{motion[0], motion[1], motion[2]},
+                                                shape(typeB, posB, rotB, sizeB), margin);
+    if (!h.hit) return 0;
+    const float v[9] = {h.safe, h.unsafe, h.depth, h.point.x, h.point.y, h.point.z, h.normal.x, h.normal.y, h.normal.z};
+    for (int k = 0; k < 9; k++) out[k] = v[k];
+    return 1;
+}
+
 // query::intersectRay on a shape built from a body's size as the solver's makeShape does.
 extern "C" int ref_intersect_ray(int type, const float pos[3], const float rot[4], const float size[3],
                                  const float origin[3], const float dir[3], float maxT, float *t, float point[3],

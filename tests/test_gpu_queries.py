@@ -143,3 +143,30 @@ def test_intersect_shape_matches_oracle():
     assert np.array_equal(gc, oc) and np.array_equal(go, oo)
     with pytest.raises(Exception):
         w.intersect_shape([7], c[:1], qr[:1], s[:1], m[:1])
+
+
+def test_cast_motion_matches_oracle():
+    """The server's cast_motion on the device: random boxes, spheres and cylinders swept by
+    random motions (some zero, some straight down onto the pile) with margins 0 or 0.05 and
+    layer masks over the settled round pile: body, safe, unsafe, point and normal bit-exact."""
+    w, o, b = _worlds()
+    rng = np.random.default_rng(21)
+    centres = w.download()["position"]
+    n = 400
+    t = rng.integers(0, 3, n).astype(np.int32)
+    c = (centres[rng.integers(0, len(centres), n)] + rng.normal(0, 1.5, (n, 3))).astype(np.float32)
+    c[: n // 2, 2] += 4.0
+    qr = rng.normal(size=(n, 4)).astype(np.float32)
+    qr /= np.linalg.norm(qr, axis=1, keepdims=True)
+    s = rng.uniform(0.2, 1.0, (n, 3)).astype(np.float32)
+    mv = rng.normal(0, 2.0, (n, 3)).astype(np.float32)
+    mv[: n // 2] = (0, 0, -8)
+    mv[::10] = 0
+    m = np.where(rng.random(n) < 0.5, 0.0, 0.05).astype(np.float32)
+    masks = rng.choice(np.array([1, 2, 3, 0xFFFFFFFF], np.uint32), n)
+    g = w.cast_motion(t, c, qr, s, mv, m, masks)
+    r = o.cast_motion(t, c, qr, s, mv, m, masks)
+    assert np.array_equal(g["body"], r["body"])
+    for k in ("safe", "unsafe", "point", "normal"):
+        assert bits_equal(g[k], r[k]), k
+    assert (r["body"] >= 0).sum() > 150 and ((r["safe"] > 0) & (r["safe"] < 1)).sum() > 50

@@ -1055,6 +1055,44 @@ def test_intersect_shape_is_the_narrow_phase_over_collected_bodies():
     assert hits > 20
 
 
+def test_sweep_shape_matches_reference_live():
+    """query::sweepShape restated: safe/unsafe fractions, depth, point and normal equal the
+    reference's own (built from its sources) bit for bit over random shape pairs, motions
+    (a tenth of them zero) and margins."""
+    if oracle.ref_lib() is None:
+        pytest.skip("reference library not built")
+    rng = np.random.default_rng(1)
+    hits = 0
+    for i in range(300):
+        ta, tb = rng.integers(0, 3, 2)
+        qa, qb = rng.normal(size=4), rng.normal(size=4)
+        qa, qb = qa / np.linalg.norm(qa), qb / np.linalg.norm(qb)
+        pa, pb = rng.normal(0, 1.5, 3), rng.normal(0, 0.5, 3)
+        mv = np.zeros(3) if i % 10 == 0 else (pb - pa) * rng.uniform(0.5, 2.5) + rng.normal(0, 0.5, 3)
+        sa, sb = rng.uniform(0.3, 1.2, 3), rng.uniform(0.3, 1.2, 3)
+        m = [0.0, 0.05][i % 2]
+        o = oracle.sweep_shape(ta, pa, qa, sa, mv, tb, pb, qb, sb, m)
+        r = oracle.ref_sweep_shape(ta, pa, qa, sa, mv, tb, pb, qb, sb, m)
+        assert o[0] == r[0] and (not o[0] or np.array_equal(o[1].view(np.uint32), r[1].view(np.uint32))), i
+        hits += r[0]
+    assert hits > 200
+
+
+def test_cast_motion_closed_forms():
+    """The server's cast_motion: a unit box dropped 3 from 1.5 above the stack (gap 1) touches the
+    upper box at safe = unsafe = 1/3 (the swept bounds' entry) with a vertical normal; a mask that skips the
+    upper box's layer passes through it to the lower box at 2/3; a sideways motion that misses everything returns (1, 1) and no body."""
+    o, b = _query_scene()     # list order: cylinder, sphere, upper box, lower box, ground
+    I = [0, 0, 0, 1]
+    n = 3
+    r = o.cast_motion([0] * n, [[0, 0, 3.5]] * n, [I] * n, [[1, 1, 1]] * n, [[0, 0, -3], [0, 0, -3], [0, 0, 0.5]],
+                      [0.0] * n, np.array([0xFFFFFFFF, 1, 0xFFFFFFFF], np.uint32))
+    assert r["body"][0] == 2 and abs(r["safe"][0] - 1 / 3) < 2e-3 and 0 <= r["unsafe"][0] - r["safe"][0] < 1e-3
+    assert np.allclose(np.abs(r["normal"][0]), [0, 0, 1], atol=1e-5) and abs(r["point"][0][2] - 2.0) < 5e-3
+    assert r["body"][1] == 3 and abs(r["safe"][1] - 2 / 3) < 2e-3
+    assert r["body"][2] == -1 and r["safe"][2] == 1 and r["unsafe"][2] == 1
+
+
 # ---------------------------------------------------------------- contact grace ----
 def test_contact_grace_parks_and_revives():
     """manifold.cpp:249-378 with solver_solve.cpp:66-86: with contact_grace_ticks = 3, a box that
