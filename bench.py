@@ -277,7 +277,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=5,
+                    help="untimed steps after the settle phase and the autotuner")
+    ap.add_argument("--settle", type=int, default=300,
+                    help="steps that let the piles come to rest before anything is tuned or timed")
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--min-bodies", type=int, default=8)
     ap.add_argument("--max-bodies", type=int, default=2048)
@@ -286,6 +289,10 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-sample", type=int, default=160,
+                    help="worlds (stratified by size) stepped by the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="threads of the CPU baseline (the host share of one GPU)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one (for tracing tools)")
     ap.add_argument("--tune-steps", type=int, default=2, help="autotuner steps per trial (0: keep defaults)")
@@ -335,13 +342,15 @@ def main():
         batch.use_graph(False)
     log(f"uploaded in {time.time() - t0:.1f}s")
 
-    for s in range(args.warmup):
+    for s in range(args.settle):
         batch.step()
         if (s + 1) % 30 == 0:
-            log(f"warmup step {s + 1}/{args.warmup}")
+            log(f"settle step {s + 1}/{args.settle}")
     tuning = batch.autotune(args.tune_steps) if args.tune_steps > 0 else None
     if tuning:
         log(f"autotuned launch shapes ({6 * args.tune_steps} trial steps): {tuning}")
+    for _ in range(args.warmup):
+        batch.step()
     tot = batch.totals()
     movable = tot["movable_bodies"]
 
@@ -408,31 +417,36 @@ def main():
         f"{plain_value / 1e6:.3f} M body-steps/s")
 
     # ---- CPU baseline: the oracle (serial restatement of the reference step) on this host,
-    # one core, then the sampled worlds statically partitioned over all cores (one world
-    # per thread at a time; ctypes releases the GIL inside the oracle's step)
+    # one core, then a larger stratified sample of worlds partitioned over the threads by
+    # body count (longest first onto the least-loaded thread), each thread stepping its worlds
+    # round-robin for the same wall time (ctypes releases the GIL inside the oracle's step)
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         from concurrent.futures import ThreadPoolExecutor
-        order = sorted(snaps, key=lambda w: -sizes[w])
-        movable_of = {w: int((snaps[w][0]["mass"] > 0).sum()) for w in order}
+        cpu_pick = sample_worlds(sizes, min(args.cpu_sample, len(worlds)))
+        order = sorted(cpu_pick, key=lambda w: -sizes[w])
+        movable_of = {w: int((worlds[w][0]["mass"] > 0).sum()) for w in order}
         oworlds = {}
         for w in order:
-            start, m = snaps[w]
+            b, j = worlds[w]
+            if w in snaps:
+                start, m = snaps[w]
+            else:
+                start, m = {**b, **batch.download(ids[w])}, batch.manifolds(ids[w])
             o = oracle.OracleWorld(params)
-            o.upload(start, worlds[w][1])
+            o.upload(start, j)
             o.set_manifolds(m)
             oworlds[w] = o
 
         def run(ws, seconds):
-            done, steps = 0, 0
+            done = 0
             end_t = time.perf_counter() + seconds
-            while time.perf_counter() < end_t and steps < 200 * len(ws):
+            while time.perf_counter() < end_t:
                 for w in ws:
                     oworlds[w].step()
                     done += movable_of[w]
-                    steps += 1
             return done
 
         one_s = args.cpu_seconds / 3.0
@@ -440,24 +454,36 @@ def main():
         one_done = run(order, one_s)
         one_value = one_done / (time.perf_counter() - a)
         try:
-            threads = len(os.sched_getaffinity(0))
+            affinity = len(os.sched_getaffinity(0))
         except AttributeError:
-            threads = os.cpu_count() or 1
-        threads = max(1, min(threads, 16, len(order)))
-        parts = [order[t::threads] for t in range(threads)]
+            affinity = os.cpu_count() or 1
+        threads = max(1, min(affinity, args.cpu_threads, len(order)))
+        parts, load = [[] for _ in range(threads)], [0] * threads
+        for w in order:
+            t = load.index(min(load))
+            parts[t].append(w)
+            load[t] += movable_of[w]
         a = time.perf_counter()
         with ThreadPoolExecutor(threads) as ex:
             multi_done = sum(ex.map(lambda ws: run(ws, args.cpu_seconds - one_s), parts))
         multi_value = multi_done / (time.perf_counter() - a)
+        cpu_model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                cpu_model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+        except OSError:
+            pass
         cpu = {"value": multi_value, "unit": "body-steps/s", "cores": threads, "kind": "port",
-               "single_core_value": one_value,
-               "sample": f"the {len(order)} stratified sample worlds (states copied from the GPU after warmup, "
-                         f"{sum(movable_of.values())} movable bodies), stepped by the oracle (serial restatement "
-                         f"of the reference step, {iters} iterations): {threads} threads with the worlds "
-                         f"statically partitioned for {args.cpu_seconds - one_s:.0f}s ({multi_done} body-steps), "
-                         f"and 1 thread for {one_s:.0f}s ({one_done} body-steps)"}
+               "single_core_value": one_value, "host_cpus": os.cpu_count(), "host_cpus_usable": affinity,
+               "cpu_model": cpu_model,
+               "sample": f"{len(order)} worlds stratified by size (states copied from the GPU after the settle "
+                         f"phase, {sum(movable_of.values())} movable bodies), stepped by the oracle (serial "
+                         f"restatement of the reference step, {iters} iterations): {threads} threads (the host "
+                         f"share of one GPU; the host has {os.cpu_count()} CPUs) with the worlds partitioned by "
+                         f"body count for {args.cpu_seconds - one_s:.0f}s ({multi_done} body-steps), and 1 thread "
+                         f"for {one_s:.0f}s ({one_done} body-steps)"}
         log(f"cpu baseline: {multi_value / 1e3:.1f} k body-steps/s on {threads} threads, "
-            f"{one_value / 1e3:.1f} k on 1")
+            f"{one_value / 1e3:.1f} k on 1 ({cpu_model}, {os.cpu_count()} CPUs)")
 
     c3 = None if args.no_c3 else bench_chains(args, log)
     serving = None if args.no_serve else bench_serving(args, log)
@@ -480,7 +506,7 @@ def main():
                         f"[{args.min_bodies}, {args.max_bodies}], seeded random box piles, one captured graph per step",
             "worlds": len(worlds), "bodies": total_bodies, "movable_bodies": movable,
             "manifolds": tot["manifolds"], "contacts": tot["contacts"], "iterations": iters,
-            "dt": "1/60", "parallelism": "single GPU",
+            "dt": "1/60", "parallelism": "single GPU", "settle_steps": args.settle,
         },
         "p50_step_ms": percentile(lat, 50),
         "p99_step_ms": percentile(lat, 99),
