@@ -33,6 +33,22 @@ using namespace orc;
 
 namespace {
 
+// Planted mistakes, for the oracle's own tests only: a nonzero id swaps one expression of the
+// restatement for a plausible wrong one, so that tests/test_oracle_pins.py can show that some
+// pin fails for each.  Zero (the default, and the only value anything but those tests uses)
+// is the restatement itself.  The ids and what they change:
+//    1 layer filter needs both directions      2 layer filter tests one direction only
+//    3 a linear lock on z keeps gravity        4 velocity recovery ignores the locks
+//    5 angular locks ignored                   6 adaptive beta doubles when converging
+//    7 adaptive beta tests the old error       8 adaptive beta scale left out of the dual
+//    9 Newton rounds rounded down             10 stiffness decay after the dual
+//   11 stiffness decay ignored                12 hard joint keeps its lambda
+//   13 hard joint stabilisation sign flipped  14 joint penalty cap ignores penalty_max
+//   15 fracture tests the linear lambda       16 fracture threshold not squared
+//   17 projection cap ignored                 18 projection sums not averaged
+//   19 contact-only bodies always to Jacobi   20 speculative margin not extended by motion
+int g_plant = 0;
+
 constexpr float PENALTY_MIN = 1.0f;
 constexpr float COLLISION_MARGIN = 0.001f;
 constexpr float STICK_THRESH = 0.00001f;
@@ -1143,7 +1159,9 @@ struct World {
             const int i = static_cast<int>(pairs[k] >> 32), j = static_cast<int>(pairs[k] & 0xFFFFFFFFu);
             const Body &A = bodies[i], &B = bodies[j];
             const bool ov = overlap(cen[i], half[i], cen[j], half[j]);
-            const bool layers = ((A.layer & B.mask) != 0) || ((B.layer & A.mask) != 0);
+            const bool layers = g_plant == 1   ? ((A.layer & B.mask) != 0) && ((B.layer & A.mask) != 0)
+                                : g_plant == 2 ? ((A.layer & B.mask) != 0)
+                                               : ((A.layer & B.mask) != 0) || ((B.layer & A.mask) != 0);
             const bool bothStatic = A.mass <= 0.0f && B.mass <= 0.0f;
             valid[k] = ov && layers && !constrained(i, j) && !bothStatic;
         }
@@ -1199,7 +1217,7 @@ struct World {
     float marginFor(const Body &a, const Body &b) const
     {
         float m = P.speculative_margin;
-        if (m > 0.0f) {
+        if (m > 0.0f && g_plant != 20) {
             const float h = collisionDt;
             const float gA = a.mass > 0.0f ? a.gravity : 0.0f;
             const float gB = b.mass > 0.0f ? b.gravity : 0.0f;
@@ -1297,10 +1315,10 @@ struct World {
                 const float err = smax(0.0f, COLLISION_MARGIN - C0[first + i].x);
                 const float floor = P.adaptive_beta_error_floor;
                 float sc = nBeta[i];
-                if (err <= floor) sc = smax(sc * 0.5f, 1.0f);
-                else if (prevErr[i] <= floor) sc = smin(sc * 2.0f, ADAPTIVE_BETA_MAX);
+                if ((g_plant == 7 ? prevErr[i] : err) <= floor) sc = smax(sc * 0.5f, 1.0f);
+                else if ((g_plant == 7 ? err : prevErr[i]) <= floor) sc = smin(sc * 2.0f, ADAPTIVE_BETA_MAX);
                 else if (err > 0.5f * prevErr[i]) sc = smin(sc * 2.0f, ADAPTIVE_BETA_MAX);
-                else if (err < 0.1f * prevErr[i]) sc = smax(sc * 0.5f, 1.0f);
+                else if (err < 0.1f * prevErr[i]) sc = g_plant == 6 ? smin(sc * 2.0f, ADAPTIVE_BETA_MAX) : smax(sc * 0.5f, 1.0f);
                 betaScale[first + i] = sc;
             } else {
                 betaScale[first + i] = 1.0f;
@@ -1403,7 +1421,7 @@ struct World {
             f3 F = cmul(pen[first + i], C) + lam[first + i];
             const Clamp cone = clampForce(F, mu);
             lam[first + i] = F;
-            const float beta = P.beta_lin * betaScale[first + i];
+            const float beta = g_plant == 8 ? P.beta_lin : P.beta_lin * betaScale[first + i];
             if (F[0] < 0) pen[first + i][0] = smin(pen[first + i][0] + beta * std::fabs(C[0]), pmax());
             if (cone.scale <= cone.bounds) {
                 pen[first + i][1] = smin(pen[first + i][1] + beta * std::fabs(C[1]), pmax());
@@ -1674,7 +1692,7 @@ struct World {
         if (length2(j.penLin) > 0) {
             const m3 K = diagonal(j.penLin.x, j.penLin.y, j.penLin.z);
             f3 C = jointLin(j);
-            if (std::isinf(j.stiffLin)) C -= j.C0Lin * P.alpha;
+            if (std::isinf(j.stiffLin)) C -= j.C0Lin * (g_plant == 13 ? -P.alpha : P.alpha);
             const f3 F = mul(K, C) + j.lamLin;
             const m3 jLin = isA ? identity3() : -identity3();
             m3 jAng;
@@ -1715,11 +1733,12 @@ struct World {
             const m3 K = diagonal(j.penLin.x, j.penLin.y, j.penLin.z);
             f3 C = jointLin(j);
             if (std::isinf(j.stiffLin)) {
-                C -= j.C0Lin * P.alpha;
-                j.lamLin = mul(K, C) + j.lamLin;
+                C -= j.C0Lin * (g_plant == 13 ? -P.alpha : P.alpha);
+                if (g_plant != 12) j.lamLin = mul(K, C) + j.lamLin;
             }
+            const float cap = g_plant == 14 ? 1e10f : pmax();
             j.penLin = lmin3(j.penLin + fabs3(C) * P.beta_lin,
-                             lmin3(f3{j.stiffLin, j.stiffLin, j.stiffLin}, f3{pmax(), pmax(), pmax()}));
+                             lmin3(f3{j.stiffLin, j.stiffLin, j.stiffLin}, f3{cap, cap, cap}));
         }
         if (length2(j.penAng) > 0) {
             const m3 K = diagonal(j.penAng.x, j.penAng.y, j.penAng.z);
@@ -1728,10 +1747,11 @@ struct World {
                 C -= j.C0Ang * P.alpha;
                 j.lamAng = mul(K, C) + j.lamAng;
             }
+            const float cap = g_plant == 14 ? 1e10f : pmax();
             j.penAng = lmin3(j.penAng + fabs3(C) * P.beta_ang,
-                             lmin3(f3{j.stiffAng, j.stiffAng, j.stiffAng}, f3{pmax(), pmax(), pmax()}));
+                             lmin3(f3{j.stiffAng, j.stiffAng, j.stiffAng}, f3{cap, cap, cap}));
         }
-        if (length2(j.lamAng) > j.fracture * j.fracture) {
+        if (length2(g_plant == 15 ? j.lamLin : j.lamAng) > (g_plant == 16 ? j.fracture : j.fracture * j.fracture)) {
             j.penLin = j.penAng = j.lamLin = j.lamAng = f3{0, 0, 0};
             j.broken = true;
         }
@@ -1780,7 +1800,7 @@ struct World {
         }
         colourBegins.assign(kMaxColours + 1, 0);
         for (int i = 0; i < n; i++) {
-            if (P.contact_only_overflow && bodies[updateOrder[i]].contactOnly) {
+            if ((P.contact_only_overflow || g_plant == 19) && bodies[updateOrder[i]].contactOnly) {
                 colour[i] = kOverflow;
                 overflowOrder.push_back(i);
             }
@@ -1860,8 +1880,8 @@ struct World {
     {
         const float dt = P.dt;
         maskLocked(b.lockLin, b.vel);
-        maskLocked(b.lockAng, b.angVel);
-        const float g = (b.lockLin & 0x4) ? 0.0f : b.gravity;
+        if (g_plant != 5) maskLocked(b.lockAng, b.angVel);
+        const float g = (b.lockLin & 0x4) && g_plant != 3 ? 0.0f : b.gravity;
         b.inertLin = b.pos + b.vel * dt;
         if (b.mass > 0) b.inertLin += f3{0, 0, g} * (dt * dt);
         b.inertAng = qadd(b.q, b.angVel * dt);
@@ -1887,8 +1907,8 @@ struct World {
             b.vel = (b.pos - b.initLin) / dt;
             b.angVel = qsub(b.q, b.initAng) / dt;
         }
-        maskLocked(b.lockLin, b.vel);
-        maskLocked(b.lockAng, b.angVel);
+        if (g_plant != 4) maskLocked(b.lockLin, b.vel);
+        if (g_plant != 4 && g_plant != 5) maskLocked(b.lockAng, b.angVel);
         if (idle && P.sleep_frames > 0 && b.mass > 0 && !b.sleeping && b.sleepMode == AVBD_SLEEP_IDLE) {
             const float lin2 = P.sleep_linear_threshold * P.sleep_linear_threshold;
             const float ang2 = P.sleep_angular_threshold * P.sleep_angular_threshold;
@@ -1917,8 +1937,9 @@ struct World {
 
     void iterate(int target)
     {
+        const float rounds = P.newton_ratio * static_cast<float>(target);
         const int newtonRounds = std::clamp(
-                static_cast<int>(std::ceil(P.newton_ratio * static_cast<float>(target))), 1, target);
+                static_cast<int>(g_plant == 9 ? std::floor(rounds) : std::ceil(rounds)), 1, target);
         for (int it = 0; it < target; it++) {
             const bool newton = it < newtonRounds;
             for (int c = 0; newton && c < colourCount; c++) {
@@ -1948,8 +1969,9 @@ struct World {
             const float decay = newton ? 1.0f : P.stiffness_decay;
             for (uint32_t s : live) {
                 if (frozen(s)) continue;
-                if (decay < 1.0f) decayPenalty(s, decay);
+                if (decay < 1.0f && g_plant != 10 && g_plant != 11) decayPenalty(s, decay);
                 updateDual(s);
+                if (decay < 1.0f && g_plant == 10) decayPenalty(s, decay);
             }
             for (Joint &j : joints) jointDual(j);
         }
@@ -2068,9 +2090,10 @@ struct World {
                 Body &b = bodies[updateOrder[i]];
                 const float inv = 1.0f / std::max(cnt[i], 1);
                 f3 dl = pl[i] * inv, da = pa[i] * inv;
+                if (g_plant == 18) { dl = pl[i]; da = pa[i]; }
                 maskLocked(b.lockLin, dl); maskLocked(b.lockAng, da);
                 const float cap = P.projection_max_correction;
-                if (cap > 0.0f) {
+                if (cap > 0.0f && g_plant != 17) {
                     const float moved = length(dl);
                     if (moved > cap) dl = dl * (cap / moved);
                     const float turned = length(da);
@@ -2825,6 +2848,8 @@ int oracle_set_manifolds(oracle_world *o, const avbd_manifolds *m)
     }
     return AVBD_OK;
 }
+
+void oracle_plant(int id) { g_plant = id; }
 
 uint64_t oracle_digest(oracle_world *o)
 {

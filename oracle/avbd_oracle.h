@@ -23,6 +23,8 @@ int oracle_manifold_count(oracle_world *o);
 float oracle_atan2(float y, float x);
 int oracle_get_manifolds(oracle_world *o, avbd_manifolds *m);
 int oracle_set_manifolds(oracle_world *o, const avbd_manifolds *m);
+// Planted mistakes for the oracle's own tests (0 = none; the ids are listed in avbd_oracle.cpp).
+void oracle_plant(int id);
 uint64_t oracle_digest(oracle_world *o);
 /* out: manifolds, contacts, candidate pairs, new manifolds, colours, overflow bodies */
 void oracle_stats(oracle_world *o, int32_t out[6]);

@@ -59,6 +59,7 @@ def lib():
         "oracle_get_manifolds": (C.c_int, [vp, C.POINTER(a.Manifolds)]),
         "oracle_set_manifolds": (C.c_int, [vp, C.POINTER(a.Manifolds)]),
         "oracle_digest": (C.c_uint64, [vp]),
+        "oracle_plant": (None, [C.c_int]),
         "oracle_stats": (None, [vp, a.i32p]),
         "oracle_colours": (None, [vp, a.u8p]),
         "oracle_swept_aabbs": (None, [vp, a.f32p, a.f32p]),
@@ -466,3 +467,19 @@ def ref_solve6(A, b):
 def atan2(y, x):
     """The generic joint's atan2, evaluated by the oracle library (one float32 pair)."""
     return float(lib().oracle_atan2(float(y), float(x)))
+
+
+class planted:
+    """Context manager for the oracle's own tests: plants mistake `pid` (see avbd_oracle.cpp)
+    in the restatement for the duration of the block."""
+
+    def __init__(self, pid):
+        self.pid = pid
+
+    def __enter__(self):
+        lib().oracle_plant(int(self.pid))
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_plant(0)
+        return False

@@ -1,0 +1,138 @@
+"""The HIP path against the oracle on the scenes that pin the oracle's less common features
+(tests/test_oracle_pins.py): collision layers, axis locks under a constraint, the projection and
+its cap, the colour sweep for contact bodies, adaptive beta with its per-contact multipliers,
+Newton rounds with stiffness decay, hard joints at the penalty cap, and joint fracture.  Bar:
+bodies, every manifold field and every joint field bit-equal, every step.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import bits_equal, oracle, pkg, scenes, state_mismatch
+
+pytestmark = pytest.mark.gpu
+P = pkg.default_params
+F = np.float32
+MANIFOLD_KEYS = ("body_a", "body_b", "basis", "friction", "num_contacts", "feature", "r_a", "r_b", "c0", "penalty",
+                 "lambda_", "stick", "beta_scale")
+JOINT_KEYS = ("body_a", "body_b", "c0_lin", "c0_ang", "penalty_lin", "penalty_ang", "lambda_lin", "lambda_ang", "broken")
+
+
+def _lockstep(bodies, joints, params, steps, before_step=None, inter=None):
+    nj = 0 if joints is None else len(joints["body_b"])
+    w = pkg.World(params, len(bodies["mass"]), nj)
+    w.upload(bodies, joints)
+    o = oracle.OracleWorld(params)
+    o.upload(bodies, joints)
+    for s in range(steps):
+        if before_step:
+            before_step(s, w, o)
+        if inter:
+            w.step_interleaved(*inter)
+            o.step_interleaved(*inter)
+        else:
+            w.step()
+            o.step()
+        bad = state_mismatch(w.download(), o.download())
+        assert not bad, f"step {s}: {bad}"
+        gm, om = w.manifolds(), o.manifolds()
+        assert len(gm["body_a"]) == len(om["body_a"]), f"step {s}"
+        for k in MANIFOLD_KEYS:
+            assert bits_equal(gm[k], om[k]), f"step {s}: manifold {k}"
+        if nj:
+            gj, oj = w.download(joints=True)[1], o.download(joints=True)[1]
+            for k in JOINT_KEYS:
+                assert bits_equal(gj[k], oj[k]), f"step {s}: joint {k}"
+    return w, o
+
+
+def _pos(created, n):
+    return n - 1 - created
+
+
+def test_layers_and_locks_under_a_spring():
+    s = scenes.SceneBuilder()
+    s.box((100, 100, 1), 0.0, 0.5, (0, 0, 0))
+    a = s.box((1, 1, 1), 1.0, 0.5, (0, 0, 4))
+    f = s.box((1, 1, 1), 1.0, 0.5, (5, 0, 4))
+    anchor = s.box((0.5, 0.5, 0.5), 0.0, 0.5, (3, 3, 5))
+    pulled = s.box((1, 1, 1), 1.0, 0.5, (0, 3, 5))
+    spin = s.box((1, 1, 1), 1.0, 0.5, (-3, 0, 5.0))
+    s.b[spin]["ang"] = (5.0, 1.0, 0.0)
+    s.spring(anchor, pulled, (0, 0, 0), (0.2, 0, 0), 50.0, rest=1.0)
+    b, j = s.build()
+    n = len(s.b)
+    b["layer"], b["mask"] = np.ones(n, np.uint32), np.ones(n, np.uint32)
+    b["layer"][_pos(a, n)], b["mask"][_pos(a, n)] = 4, 0xFFFFFFFF
+    b["layer"][_pos(f, n)], b["mask"][_pos(f, n)] = 2, 2
+    b["lock_linear"], b["lock_angular"] = np.zeros(n, np.uint8), np.zeros(n, np.uint8)
+    b["lock_linear"][_pos(pulled, n)] = 0x5
+    b["lock_angular"][_pos(spin, n)] = 0x1
+    _lockstep(b, j, P(iterations=10), 120)
+
+
+@pytest.mark.parametrize("cap", [0.0, 0.01])
+def test_projection_cap_on_shared_planes(cap):
+    s = scenes.SceneBuilder(gravity=0.0)
+    for _ in range(3):
+        s.box((100, 100, 1), 0.0, 0.5, (0, 0, 0))
+    s.sphere(0.5, 10.0, 0.5, (0, 0, 0.9))
+    s.box((1, 1, 1), 2.0, 0.5, (3, 0, 0.85))
+    p = P(alpha=1.0, iterations=1, projection_iterations=2, projection_beta=1.0, projection_slop=0.0,
+          projection_max_correction=cap)
+    _lockstep(*s.build(), p, 5)
+
+
+def test_pile_through_the_colour_sweep_with_projection_cap():
+    b, j = scenes.pile(200, seed=12, pitch=2.0)
+    _lockstep(b, j, P(iterations=10, contact_only_overflow=0, projection_max_correction=0.004), 90)
+
+
+def test_adaptive_beta_multipliers_under_teleports():
+    s = scenes.SceneBuilder()
+    s.box((100, 100, 1), 0.0, 0.5, (0, 0, 0))
+    s.box((1, 1, 1), 1.0, 0.5, (0, 0, 1.0005))
+    b, j = s.build()
+    b["gravity"][0] = 0.0
+    rng = np.random.default_rng(1)
+    depths = rng.choice([0.0, 0.0005, 0.0015, 0.003, 0.008, 0.012, 0.016, 0.019], 60)
+    zero = np.zeros((2, 3), F)
+
+    def teleport(k, w, o):
+        z = F(1.0 - float(depths[k]))
+        arrays = {"position": np.array([[0, 0, z], [0, 0, 0]], F), "rotation": np.array([[0, 0, 0, 1]] * 2, F),
+                  "velocity": zero, "angular_velocity": zero, "prev_velocity": zero}
+        w.write_bodies(arrays)
+        o.write_bodies(arrays)
+    _lockstep(b, j, P(iterations=1, adaptive_beta=1, beta_lin=1.0, position_projection=0), 60, teleport)
+
+
+def test_newton_ratio_and_stiffness_decay():
+    b, j = scenes.pile(150, seed=3, pitch=2.0)
+    _lockstep(b, j, P(iterations=7, newton_ratio=0.5, stiffness_decay=0.5), 60)
+
+
+def test_hard_joints_at_the_penalty_cap():
+    s = scenes.SceneBuilder()
+    a1 = s.box((1, 1, 1), 0.0, 0.5, (-1, 0, 10))
+    a2 = s.box((1, 1, 1), 0.0, 0.5, (1, 0, 10))
+    box = s.box((1, 1, 1), 1.0, 0.5, (0, 0, 10))
+    s.joint(a1, box, (0, 0, 0), (0, 0, 0))
+    s.joint(a2, box, (0, 0, 0), (0, 0, 0))
+    hang = s.box((1, 1, 1), 2.0, 0.5, (0, 0, 8.9))
+    s.joint(box, hang, (0, 0, -0.5), (0, 0, 0.5), k_lin=500.0)
+    w, o = _lockstep(*s.build(), P(iterations=10, beta_lin=1e12, penalty_max=1e8), 20)
+    assert o.download(joints=True)[1]["penalty_lin"][1:, 0].tolist() == [1e8, 1e8]
+
+
+@pytest.mark.parametrize("fracture", [math.inf, 0.7, 0.5])
+def test_weld_fracture(fracture):
+    s = scenes.SceneBuilder()
+    anchor = s.box((1, 1, 1), 0.0, 0.5, (0, 0, 10))
+    box = s.box((1, 1, 1), 1.0, 0.5, (1, 0, 10))
+    s.joint(anchor, box, (0.5, 0, 0), (-0.5, 0, 0), k_lin=math.inf, k_ang=math.inf, fracture=fracture)
+    s.box((40, 40, 1), 0.0, 0.5, (0, 0, 0))
+    w, o = _lockstep(*s.build(), P(iterations=10), 240)
+    broken = len(o.download(joints=True)[1]["broken"]) == 0
+    assert broken == (fracture == 0.5)
