@@ -17,6 +17,8 @@ AVBD_ERR_UNSUPPORTED = -2
 AVBD_ERR_CAPACITY = -3
 AVBD_ERR_DEVICE = -4
 AVBD_ERR_NO_DEVICE = -5
+AVBD_ERR_WORLD = -6
+AVBD_WORLD_SLOTS, AVBD_WORLD_TREE, AVBD_WORLD_NONFINITE = 1, 12, 16
 MAX_CONTACTS = 8
 
 f32p = C.POINTER(C.c_float)
@@ -285,6 +287,8 @@ def load_library(path=LIB_PATH):
         "avbd_batch_add_world": (C.c_int32, [vp, C.POINTER(Bodies), C.POINTER(Joints), i32p]),
         "avbd_batch_remove_world": (C.c_int32, [vp, C.c_int32]),
         "avbd_batch_step": (C.c_int32, [vp]),
+        "avbd_batch_failed_worlds": (C.c_int32, [vp, C.c_int32, i32p, i32p, i32p]),
+        "avbd_batch_fits": (C.c_int32, [vp, C.c_int32, C.c_int32, i32p, i32p]),
         "avbd_batch_use_graph": (C.c_int32, [vp, C.c_int32]),
         "avbd_batch_download_world": (C.c_int32, [vp, C.c_int32, C.POINTER(Bodies), C.POINTER(Joints)]),
         "avbd_batch_world_count": (C.c_int32, [vp, i32p]),

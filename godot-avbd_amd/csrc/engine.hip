@@ -56,6 +56,13 @@ struct RangeAlloc {
         }
         return -1;
     }
+    bool fits(int n) const
+    {
+        if (n == 0) return true;
+        for (const auto &r : freeRanges)
+            if (r.second >= n) return true;
+        return false;
+    }
     void give(int s, int n)
     {
         if (n <= 0) return;
@@ -1042,26 +1049,44 @@ struct Engine {
         csrValid = true;
     }
 
+    // Worlds whose last step failed, with their reasons (AVBD_WORLD_* bits).
+    std::vector<int32_t> failedIds, failedWhy;
+
+    // Per-world error words of the step just enqueued.  The kernels never write outside a
+    // world's ranges when its capacity runs out (they skip the work and set a bit), so one
+    // world's failure leaves the others' results intact: a batch reports the failed worlds
+    // (AVBD_ERR_WORLD, listed by avbd_batch_failed_worlds) and its caller evicts them.  A plain
+    // world keeps the reference's behaviour for non-finite state (no check) and fails its step
+    // on the other bits.  Overflowing the shared candidate-pair capacity fails the whole step.
     avbd_status checkErrors()
     {
         HIPCHK(hipMemcpyAsync(errHost, S.wErr, sizeof(int) * Wcap, hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
-        for (int w = 0; w < Wcap; w++) {
-            if (errHost[w]) {
-                const int e = errHost[w];
-                (void)hipMemsetAsync(S.wErr, 0, sizeof(int) * Wcap, stream);
-                (void)hipStreamSynchronize(stream);
-                if (e & 1) return fail(AVBD_ERR_CAPACITY, "manifold slot capacity exceeded in world " + std::to_string(w));
-                if (e & 4) return fail(AVBD_ERR_DEVICE, "broad-phase tree walk did not terminate in world " + std::to_string(w));
-                if (e & 8) return fail(AVBD_ERR_DEVICE, "broad-phase tree has no subtree for world " + std::to_string(w));
-                return fail(AVBD_ERR_CAPACITY, "capacity exceeded in world " + std::to_string(w));
-            }
-        }
-        // pair capacity
         uint32_t total = 0;
-        HIPCHK(hipMemcpy(&total, S.pairOff + Ncap, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(countsHost, S.pairOff + Ncap, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        std::memcpy(&total, countsHost, sizeof(uint32_t));
+        failedIds.clear();
+        failedWhy.clear();
+        for (int w = 0; w < Wcap; w++) {
+            const int e = batchMode ? errHost[w] : (errHost[w] & ~AVBD_WORLD_NONFINITE);
+            if (e) { failedIds.push_back(w); failedWhy.push_back(e); }
+        }
+        bool any = false;
+        for (int w = 0; w < Wcap; w++) any = any || errHost[w] != 0;
+        if (any) {
+            HIPCHK(hipMemsetAsync(S.wErr, 0, sizeof(int) * Wcap, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+        }
         if (total > static_cast<uint32_t>(Pcap)) return fail(AVBD_ERR_CAPACITY, "candidate pair capacity exceeded");
-        return AVBD_OK;
+        if (failedIds.empty()) return AVBD_OK;
+        const int w = failedIds[0], e = failedWhy[0];
+        const std::string what = (e & AVBD_WORLD_SLOTS)      ? "manifold slot capacity exceeded"
+                                 : (e & AVBD_WORLD_TREE)     ? "broad-phase tree walk failed"
+                                                             : "non-finite body state";
+        if (batchMode)
+            return fail(AVBD_ERR_WORLD, std::to_string(failedIds.size()) + " world(s) failed this step (first: world " +
+                                            std::to_string(w) + ", " + what + "); the others stepped");
+        return fail((e & AVBD_WORLD_SLOTS) ? AVBD_ERR_CAPACITY : AVBD_ERR_DEVICE, what);
     }
 
     avbd_status stepPlain(int substeps, int rounds)
@@ -2127,6 +2152,23 @@ avbd_status avbd_batch_destroy(avbd_batch *batch)
     return AVBD_OK;
 }
 
+// Manifold slots a world of nb bodies is given in a batch.
+static int worldSlots(int nb) { return std::max(8 * nb + 16, 32); }
+
+avbd_status avbd_batch_fits(avbd_batch *batch, int32_t n_bodies, int32_t n_joints, int32_t *fits_now,
+                            int32_t *fits_ever)
+{
+    if (!batch || n_bodies < 0 || n_joints < 0) return fail(AVBD_ERR_INVALID, "bad argument");
+    const Engine &e = batch->eng;
+    const int slots = worldSlots(n_bodies);
+    bool freeWorld = false;
+    for (int k = 0; k < e.Wcap; k++) freeWorld = freeWorld || !e.worlds[k].active;
+    if (fits_ever) *fits_ever = n_bodies <= e.Ncap && n_joints <= e.Jcap && slots <= e.Scap;
+    if (fits_now)
+        *fits_now = freeWorld && e.bodyAlloc.fits(n_bodies) && e.jointAlloc.fits(n_joints) && e.slotAlloc.fits(slots);
+    return AVBD_OK;
+}
+
 avbd_status avbd_batch_add_world(avbd_batch *batch, const avbd_bodies *bodies, const avbd_joints *joints,
                                  int32_t *world_id)
 {
@@ -2140,7 +2182,7 @@ avbd_status avbd_batch_add_world(avbd_batch *batch, const avbd_bodies *bodies, c
         if (!e.worlds[k].active) { w = k; break; }
     if (w < 0) return fail(AVBD_ERR_CAPACITY, "no free world slot");
     const int nb = bodies->count, nj = joints ? joints->count : 0;
-    const int slots = std::max(8 * nb + 16, 32);
+    const int slots = worldSlots(nb);
     const int bb = e.bodyAlloc.take(nb);
     if (bb < 0) return fail(AVBD_ERR_CAPACITY, "not enough body capacity");
     const int jb = e.jointAlloc.take(nj);
@@ -2156,7 +2198,12 @@ avbd_status avbd_batch_add_world(avbd_batch *batch, const avbd_bodies *bodies, c
     wi.jointBase = jb;
     wi.slotBase = sb;
     st = e.uploadWorld(w, bodies, joints, slots);
-    if (st != AVBD_OK) return st;
+    if (st != AVBD_OK) {   // rejected input: the world stays inactive and its ranges go back
+        e.bodyAlloc.give(bb, nb);
+        e.jointAlloc.give(jb, nj);
+        e.slotAlloc.give(sb, slots);
+        return st;
+    }
     *world_id = w;
     return AVBD_OK;
 }
@@ -2179,6 +2226,20 @@ avbd_status avbd_batch_step(avbd_batch *batch)
 {
     if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
     return batch->eng.stepBatch();
+}
+
+avbd_status avbd_batch_failed_worlds(avbd_batch *batch, int32_t capacity, int32_t *world_ids, int32_t *reasons,
+                                     int32_t *count)
+{
+    if (!batch || !count || capacity < 0 || (capacity > 0 && !world_ids)) return fail(AVBD_ERR_INVALID, "null argument");
+    const Engine &e = batch->eng;
+    const int n = static_cast<int>(e.failedIds.size());
+    for (int k = 0; k < n && k < capacity; k++) {
+        world_ids[k] = e.failedIds[k];
+        if (reasons) reasons[k] = e.failedWhy[k];
+    }
+    *count = n;
+    return AVBD_OK;
 }
 
 avbd_status avbd_world_raycast(avbd_world *world, int32_t n, const float *from, const float *to,

@@ -1693,6 +1693,14 @@ __global__ void k_finish(State S, int idle)
     }
     S.vel[i] = st3(vel);
     S.angVel[i] = st3(ang);
+    if (movable) {
+        const v3 p = bPos(S, i);
+        const float4 q = S.rot[i];
+        const float v[13] = {p.x, p.y, p.z, q.x, q.y, q.z, q.w, vel.x, vel.y, vel.z, ang.x, ang.y, ang.z};
+        bool finite = true;
+        for (int c = 0; c < 13; c++) finite = finite && isfinite(v[c]);
+        if (!finite) atomicOr(S.wErr + S.bodyWorld[i], AVBD_WORLD_NONFINITE);
+    }
 }
 
 // ------------------------------------------------------------ sleeping ----

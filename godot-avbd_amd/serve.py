@@ -7,6 +7,11 @@ once from an HBM byte budget, `avbd_batch_footprint`).  Every admitted world is 
 other resident worlds by one batch step per tick (one captured HIP graph), stays in HBM between
 ticks, and is retired (its ranges freed for the next admission) when its horizon is done.
 
+A request that could not fit even the empty batch is rejected at submission (it would otherwise
+block the FIFO for ever).  A world whose step fails on the device (its manifold slots overflow,
+its broad-phase walk fails, or its state stops being finite) is evicted after that step and
+reported in `evicted`; the other worlds of the batch stepped normally and go on.
+
 Latency is recorded per world per step: from the moment the step was admitted (the request's
 arrival for its first step, the start of the tick otherwise) to the moment its result is ready
 (`avbd_batch_step` returns after the step has completed on the device).
@@ -68,10 +73,18 @@ class Server:
         self.body_steps = 0
         self.finished = []           # (request, final state)
         self.keep_final = False
+        self.rejected = []           # requests larger than the whole batch
+        self.evicted = []            # (request, AVBD_WORLD_* reason bits, steps done)
 
     def submit(self, req):
+        """Queue a request; False (and the request in `rejected`) when it can never fit."""
         req.arrival = time.perf_counter()
+        nj = 0 if req.joints is None else len(req.joints["body_b"])
+        if not self.batch.fits(len(req.bodies["mass"]), nj)[1]:
+            self.rejected.append(req)
+            return False
         self.waiting.append(req)
+        return True
 
     def _admit(self):
         now = time.perf_counter()
@@ -93,7 +106,16 @@ class Server:
         if not self.resident:
             return 0
         t0 = time.perf_counter()
-        self.batch.step()
+        try:
+            self.batch.step()
+        except abi.AvbdError as e:
+            if e.code != abi.AVBD_ERR_WORLD:
+                raise
+            for wid, why in self.batch.failed_worlds():
+                req = self.resident.pop(wid, None)
+                if req is not None:
+                    self.evicted.append((req, why, req.done))
+                    self.batch.remove(wid)
         t1 = time.perf_counter()
         stepped = 0
         for wid, req in list(self.resident.items()):

@@ -248,7 +248,24 @@ class Batch:
         self.sizes.pop(wid, None)
 
     def step(self):
+        """One step of every world.  Raises AvbdError with code AVBD_ERR_WORLD when some worlds
+        failed (the others stepped): `failed_worlds()` lists them."""
         abi.check(self.lib.avbd_batch_step(self.h))
+
+    def failed_worlds(self):
+        """[(world id, AVBD_WORLD_* reason bits)] of the worlds that failed the last step."""
+        n = C.c_int32()
+        abi.check(self.lib.avbd_batch_failed_worlds(self.h, 0, None, None, C.byref(n)))
+        ids, why = np.zeros(max(n.value, 1), np.int32), np.zeros(max(n.value, 1), np.int32)
+        abi.check(self.lib.avbd_batch_failed_worlds(self.h, n.value, ids.ctypes.data_as(abi.i32p),
+                                                    why.ctypes.data_as(abi.i32p), C.byref(n)))
+        return [(int(ids[k]), int(why[k])) for k in range(n.value)]
+
+    def fits(self, n_bodies, n_joints=0):
+        """(fits now, fits an empty batch) for a world of this size."""
+        now, ever = C.c_int32(), C.c_int32()
+        abi.check(self.lib.avbd_batch_fits(self.h, int(n_bodies), int(n_joints), C.byref(now), C.byref(ever)))
+        return bool(now.value), bool(ever.value)
 
     def use_graph(self, enable):
         abi.check(self.lib.avbd_batch_use_graph(self.h, 1 if enable else 0))

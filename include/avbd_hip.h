@@ -34,6 +34,16 @@ typedef int32_t avbd_status;
 #define AVBD_ERR_CAPACITY (-3)     /* a device buffer would overflow its capacity        */
 #define AVBD_ERR_DEVICE (-4)       /* a HIP runtime call failed                          */
 #define AVBD_ERR_NO_DEVICE (-5)    /* no usable gfx950 device                            */
+#define AVBD_ERR_WORLD (-6)        /* some worlds of a batch failed their step; the others
+                                      stepped (avbd_batch_failed_worlds lists them)      */
+
+/* Why a world of a batch failed its step (bits of avbd_batch_failed_worlds' reasons).  The
+ * reference has no such check (it never detects non-finite state, reference
+ * test/core_test.cpp:105-120 checks it after the fact); a batch needs it so that one diverged
+ * or overfull world can be evicted without stopping the others. */
+#define AVBD_WORLD_SLOTS 1         /* more live manifolds than the world's slot range     */
+#define AVBD_WORLD_TREE 12         /* its broad-phase tree walk failed (4) or was missing (8) */
+#define AVBD_WORLD_NONFINITE 16    /* a movable body's pose or velocity is not finite     */
 
 /* Solver parameters.  Field meaning and defaults follow the reference's public Solver
  * fields (reference src/avbd/solver.h:54-144) and constants (src/avbd/types.hpp:23-45).
@@ -410,11 +420,23 @@ avbd_status avbd_batch_device_bytes(avbd_batch *batch, int64_t *bytes);
 avbd_status avbd_batch_add_world(avbd_batch *batch, const avbd_bodies *bodies,
                                  const avbd_joints *joints, int32_t *world_id);
 avbd_status avbd_batch_remove_world(avbd_batch *batch, int32_t world_id);
+/* Whether a world of n_bodies bodies and n_joints joints can be added now (*fits_now: a free
+ * world slot and free body, joint and manifold-slot ranges) and whether it could be added to
+ * this batch when empty (*fits_ever); either pointer may be null.  A serving loop rejects a
+ * request that never fits instead of waiting for it. */
+avbd_status avbd_batch_fits(avbd_batch *batch, int32_t n_bodies, int32_t n_joints, int32_t *fits_now,
+                            int32_t *fits_ever);
 /* Launch batch steps as one captured HIP graph (enable = 1, the default) or as the same
  * kernels launched one by one on the batch's stream (enable = 0; for tracing tools). */
 avbd_status avbd_batch_use_graph(avbd_batch *batch, int32_t enable);
-/* One step of every world in the batch. */
+/* One step of every world in the batch.  AVBD_ERR_WORLD: every world stepped except the ones
+ * avbd_batch_failed_worlds lists, whose state this step is not valid (remove them; until then
+ * they keep failing). */
 avbd_status avbd_batch_step(avbd_batch *batch);
+/* The worlds that failed the last step (ids, and AVBD_WORLD_* reason bits when reasons is not
+ * null): at most `capacity` are written, *count is the total. */
+avbd_status avbd_batch_failed_worlds(avbd_batch *batch, int32_t capacity, int32_t *world_ids, int32_t *reasons,
+                                     int32_t *count);
 /* avbd_world_contact_report for one world of a batch. */
 avbd_status avbd_batch_contact_report(avbd_batch *batch, int32_t world_id, const int32_t *max_reported, int32_t stride,
                                       int32_t *count, float *position, float *normal, int32_t *other);
