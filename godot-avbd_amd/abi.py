@@ -257,6 +257,7 @@ def load_library(path=LIB_PATH):
         "avbd_world_download": (C.c_int32, [vp, C.POINTER(Bodies), C.POINTER(Joints)]),
         "avbd_world_get_manifolds": (C.c_int32, [vp, C.POINTER(Manifolds)]),
         "avbd_world_raycast": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, C.c_int32, C.POINTER(RayHits)]),
+        "avbd_world_raycast_noalloc": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, C.c_int32, C.POINTER(RayHits)]),
         "avbd_batch_raycast": (C.c_int32, [vp, C.c_int32, C.c_int32, f32p, f32p, u32p, C.c_int32,
                                            C.POINTER(RayHits)]),
         "avbd_world_pick": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, i32p, f32p]),

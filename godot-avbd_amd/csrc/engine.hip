@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -144,6 +145,7 @@ struct Engine {
     State S{};
     float *rimDev = nullptr;   // S.rim
     hipStream_t stream = nullptr;
+    int device = 0;             // the device current when the engine was created
     std::vector<WorldInfo> worlds;
     RangeAlloc bodyAlloc, jointAlloc, slotAlloc;
     // scratch
@@ -656,6 +658,7 @@ struct Engine {
             int ndev = 0;
             if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
                 return fail(AVBD_ERR_NO_DEVICE, "no HIP device visible");
+            HIPCHK(hipGetDevice(&device));
             HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
             for (auto &e : ev) HIPCHK(hipEventCreate(&e));
             HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&errHost), sizeof(int) * Wcap, hipHostMallocDefault));
@@ -2248,6 +2251,128 @@ avbd_status avbd_world_raycast(avbd_world *world, int32_t n, const float *from, 
     if (!world || !out) return fail(AVBD_ERR_INVALID, "null argument");
     return world->eng.raycast(0, n, from, to, masks, hit_from_inside != 0, out->body, out->t, out->point, out->normal,
                               out->inside);
+}
+
+// ---- avbd_world_raycast without allocation ------------------------------------------------
+// All device memory the call needs beyond the world's own state lives in `scratch`, one
+// fixed array per device (this library's code object is loaded once per device, each with its
+// own copy at its own address).  Layout, in 256-byte aligned parts: the caller's segments and
+// masks as copied (from, to, masks), the packed segments, the bodies' bounds, the kernel's
+// hits, and the hits unpacked into the caller's layout (t, point, normal, inside).
+namespace {
+constexpr size_t NA_RAYS = AVBD_NOALLOC_MAX_RAYS, NA_BODIES = AVBD_NOALLOC_MAX_BODIES;
+constexpr size_t naAlign(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+constexpr size_t NA_FROM = 0;
+constexpr size_t NA_TO = NA_FROM + naAlign(12 * NA_RAYS);
+constexpr size_t NA_MASK = NA_TO + naAlign(12 * NA_RAYS);
+constexpr size_t NA_RAYS4 = NA_MASK + naAlign(4 * NA_RAYS);
+constexpr size_t NA_BOUNDS = NA_RAYS4 + naAlign(32 * NA_RAYS);
+constexpr size_t NA_BODY = NA_BOUNDS + naAlign(32 * NA_BODIES);
+constexpr size_t NA_HITA = NA_BODY + naAlign(4 * NA_RAYS);
+constexpr size_t NA_HITB = NA_HITA + naAlign(16 * NA_RAYS);
+constexpr size_t NA_T = NA_HITB + naAlign(16 * NA_RAYS);
+constexpr size_t NA_POINT = NA_T + naAlign(4 * NA_RAYS);
+constexpr size_t NA_NORMAL = NA_POINT + naAlign(12 * NA_RAYS);
+constexpr size_t NA_INSIDE = NA_NORMAL + naAlign(12 * NA_RAYS);
+constexpr size_t NA_BYTES = NA_INSIDE + naAlign(NA_RAYS);
+static_assert(NA_BYTES == AVBD_NOALLOC_SCRATCH_BYTES, "scratch size in avbd_hip.h is out of date");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+static unsigned char *g_scratchPtr[NA_MAX_DEVICES];
+static std::once_flag g_scratchOnce[NA_MAX_DEVICES];
+static hipError_t g_scratchErr[NA_MAX_DEVICES];
+
+// The device address of `scratch` on `dev`, resolved once per device with that device current
+// (host code never uses the array's own name: that is a host-side stand-in).
+static avbd_status scratchOn(int dev, unsigned char **out)
+{
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return fail(AVBD_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    std::call_once(g_scratchOnce[dev], [dev] {
+        int cur = 0;
+        g_scratchErr[dev] = hipGetDevice(&cur);
+        if (g_scratchErr[dev] != hipSuccess) return;
+        if (cur != dev && (g_scratchErr[dev] = hipSetDevice(dev)) != hipSuccess) return;
+        void *p = nullptr;
+        g_scratchErr[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratchPtr[dev] = static_cast<unsigned char *>(p);
+        if (cur != dev) (void)hipSetDevice(cur);
+    });
+    if (g_scratchErr[dev] != hipSuccess || !g_scratchPtr[dev])
+        return fail(AVBD_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(g_scratchErr[dev]));
+    *out = g_scratchPtr[dev];
+    return AVBD_OK;
+}
+
+__global__ void k_na_pack_rays(int n, const float *from, const float *to, const uint32_t *masks, int haveMasks,
+                               float4 *rays)
+{
+    const int r = GID;
+    if (r >= n) return;
+    const uint32_t m = haveMasks ? masks[r] : 0xFFFFFFFFu;
+    rays[2 * r] = make_float4(from[3 * r], from[3 * r + 1], from[3 * r + 2], __uint_as_float(m));
+    rays[2 * r + 1] = make_float4(to[3 * r], to[3 * r + 1], to[3 * r + 2], 0.0f);
+}
+
+__global__ void k_na_unpack_hits(int n, const float4 *a, const float4 *b, float *t, float *point, float *normal,
+                                 uint8_t *inside)
+{
+    const int r = GID;
+    if (r >= n) return;
+    const float4 x = a[r], y = b[r];
+    t[r] = x.w;
+    point[3 * r] = x.x; point[3 * r + 1] = x.y; point[3 * r + 2] = x.z;
+    normal[3 * r] = y.x; normal[3 * r + 1] = y.y; normal[3 * r + 2] = y.z;
+    inside[r] = y.w != 0.0f ? 1 : 0;
+}
+
+avbd_status avbd_world_raycast_noalloc(avbd_world *world, int32_t n, const float *from, const float *to,
+                                       const uint32_t *masks, int32_t hit_from_inside, avbd_ray_hits *out)
+{
+    if (!world || !out) return fail(AVBD_ERR_INVALID, "null argument");
+    Engine &e = world->eng;
+    const WorldInfo &wi = e.worlds[0];
+    if (!wi.active) return fail(AVBD_ERR_INVALID, "world is not active");
+    if (n < 0 || (n > 0 && (!from || !to || !out->body))) return fail(AVBD_ERR_INVALID, "bad ray arrays");
+    if (n > AVBD_NOALLOC_MAX_RAYS)
+        return fail(AVBD_ERR_CAPACITY, "more than AVBD_NOALLOC_MAX_RAYS segments in one no-allocation call");
+    if (wi.bodyCount > AVBD_NOALLOC_MAX_BODIES)
+        return fail(AVBD_ERR_CAPACITY, "world has more than AVBD_NOALLOC_MAX_BODIES bodies for a no-allocation call");
+    if (n == 0) return AVBD_OK;
+    unsigned char *base = nullptr;
+    avbd_status st = scratchOn(e.device, &base);
+    if (st != AVBD_OK) return st;
+    auto *dFrom = reinterpret_cast<float *>(base + NA_FROM);
+    auto *dTo = reinterpret_cast<float *>(base + NA_TO);
+    auto *dMask = reinterpret_cast<uint32_t *>(base + NA_MASK);
+    auto *rays = reinterpret_cast<float4 *>(base + NA_RAYS4);
+    auto *bounds = reinterpret_cast<float4 *>(base + NA_BOUNDS);
+    auto *body = reinterpret_cast<int *>(base + NA_BODY);
+    auto *hitA = reinterpret_cast<float4 *>(base + NA_HITA);
+    auto *hitB = reinterpret_cast<float4 *>(base + NA_HITB);
+    auto *t = reinterpret_cast<float *>(base + NA_T);
+    auto *point = reinterpret_cast<float *>(base + NA_POINT);
+    auto *normal = reinterpret_cast<float *>(base + NA_NORMAL);
+    auto *inside = reinterpret_cast<uint8_t *>(base + NA_INSIDE);
+    hipStream_t s = e.stream;
+    HIPCHK(hipMemcpyAsync(dFrom, from, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dTo, to, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+    if (masks) HIPCHK(hipMemcpyAsync(dMask, masks, sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+    k_na_pack_rays<<<nblk(n), TPB, 0, s>>>(n, dFrom, dTo, dMask, masks ? 1 : 0, rays);
+    if (wi.bodyCount > 0) k_query_bounds<<<nblk(wi.bodyCount), TPB, 0, s>>>(e.S, wi.bodyBase, wi.bodyCount, bounds);
+    k_raycast<<<nblk(n), TPB, 0, s>>>(e.S, wi.bodyBase, wi.bodyCount, n, rays, bounds, hit_from_inside ? 1 : 0, body,
+                                      hitA, hitB);
+    k_na_unpack_hits<<<nblk(n), TPB, 0, s>>>(n, hitA, hitB, t, point, normal, inside);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out->body, body, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    if (out->t) HIPCHK(hipMemcpyAsync(out->t, t, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    if (out->point) HIPCHK(hipMemcpyAsync(out->point, point, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+    if (out->normal) HIPCHK(hipMemcpyAsync(out->normal, normal, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+    if (out->inside) HIPCHK(hipMemcpyAsync(out->inside, inside, n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return AVBD_OK;
 }
 
 avbd_status avbd_batch_raycast(avbd_batch *batch, int32_t world_id, int32_t n, const float *from, const float *to,

@@ -9,6 +9,20 @@ import numpy as np
 from . import abi
 
 
+def _rows(a, width, name, dtype=np.float32, n=None):
+    """`a` as a contiguous (n, width) array ((n,) for width 1); ValueError when its shape is
+    not that (the C side reads n rows of width values through a raw pointer)."""
+    a = np.ascontiguousarray(a, dtype)
+    if width == 1:
+        ok = a.ndim == 1
+    else:
+        ok = a.ndim == 2 and a.shape[1] == width
+    if not ok or (n is not None and len(a) != n):
+        want = f"({n if n is not None else 'n'}{'' if width == 1 else f', {width}'})"
+        raise ValueError(f"{name} has shape {a.shape}, expected {want}")
+    return a
+
+
 class World:
     """One world, stepped by the plain single-world path (avbd_world_*)."""
 
@@ -78,15 +92,19 @@ class World:
             jout = {key: v[:k] for key, v in jout.items()}
         return (out, jout) if joints else out
 
-    def raycast(self, frm, to, masks=None, hit_from_inside=False):
+    def raycast(self, frm, to, masks=None, hit_from_inside=False, no_alloc=False):
         """Segments frm -> to (n x 3) against the bodies: dict of body (-1 none), t, point,
-        normal, inside (the server's intersect_ray rules)."""
-        frm, to = np.ascontiguousarray(frm, np.float32), np.ascontiguousarray(to, np.float32)
-        out, h = abi.ray_hits(len(frm))
-        m = None if masks is None else np.ascontiguousarray(masks, np.uint32)
-        abi.check(self.lib.avbd_world_raycast(self.h, len(frm), frm.ctypes.data_as(abi.f32p), to.ctypes.data_as(abi.f32p),
-                                              None if m is None else m.ctypes.data_as(abi.u32p), int(bool(hit_from_inside)),
-                                              C.byref(h)))
+        normal, inside (the server's intersect_ray rules).  no_alloc: through
+        avbd_world_raycast_noalloc (fixed device scratch, no device allocation)."""
+        frm, to = _rows(frm, 3, "frm"), _rows(to, 3, "to")
+        n = len(frm)
+        if len(to) != n:
+            raise ValueError(f"to has {len(to)} rows, frm {n}")
+        out, h = abi.ray_hits(n)
+        m = None if masks is None else _rows(masks, 1, "masks", np.uint32, n)
+        fn = self.lib.avbd_world_raycast_noalloc if no_alloc else self.lib.avbd_world_raycast
+        abi.check(fn(self.h, n, frm.ctypes.data_as(abi.f32p), to.ctypes.data_as(abi.f32p),
+                     None if m is None else m.ctypes.data_as(abi.u32p), int(bool(hit_from_inside)), C.byref(h)))
         return out
 
     def pick(self, origin, direction, masks=None):

@@ -322,6 +322,19 @@ avbd_status avbd_world_raycast(avbd_world *world, int32_t n, const float *from, 
  * count[j] = contacts written.  stride >= every max_reported[j]. */
 avbd_status avbd_world_contact_report(avbd_world *world, const int32_t *max_reported, int32_t stride, int32_t *count,
                                       float *position, float *normal, int32_t *other);
+/* avbd_world_raycast for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No reference counterpart.
+ * Same arguments and results, bit for bit; it calls no hipMalloc / hipFree: its staging (the
+ * segments, the bodies' bounds and the hits) lives in one fixed device array per GPU,
+ * AVBD_NOALLOC_SCRATCH_BYTES long, held until the process ends.  Limits: at most
+ * AVBD_NOALLOC_MAX_RAYS segments per call and AVBD_NOALLOC_MAX_BODIES bodies in the world
+ * (AVBD_ERR_CAPACITY beyond).  Calls on one GPU share that array: calls that overlap on one GPU
+ * (from several host threads) race and must be serialised by the caller. */
+#define AVBD_NOALLOC_MAX_RAYS 16384
+#define AVBD_NOALLOC_MAX_BODIES 65536
+#define AVBD_NOALLOC_SCRATCH_BYTES 4145152
+avbd_status avbd_world_raycast_noalloc(avbd_world *world, int32_t n, const float *from, const float *to,
+                                       const uint32_t *masks, int32_t hit_from_inside, avbd_ray_hits *out);
 /* Measurement (no reference counterpart): average device time of the bounds kernel and of
  * the ray kernel of avbd_world_raycast over `reps` launches of n segments (all layers, no
  * hit from inside), timed with HIP events on the library's stream. */
