@@ -115,6 +115,7 @@ class StepStats(C.Structure):
         ("candidate_pairs", C.c_int32), ("new_manifolds", C.c_int32), ("colours", C.c_int32),
         ("overflow_bodies", C.c_int32), ("brake_levels", C.c_int32), ("ms_broad", C.c_float),
         ("ms_narrow", C.c_float), ("ms_solve", C.c_float), ("ms_finish", C.c_float), ("ms_total", C.c_float),
+        ("ms_broad_tree", C.c_float), ("ms_broad_emit", C.c_float),
     ]
 
 
@@ -258,6 +259,7 @@ def load_library(path=LIB_PATH):
         "avbd_world_get_manifolds": (C.c_int32, [vp, C.POINTER(Manifolds)]),
         "avbd_world_raycast": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, C.c_int32, C.POINTER(RayHits)]),
         "avbd_world_raycast_noalloc": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, C.c_int32, C.POINTER(RayHits)]),
+        "avbd_world_collide_pair": (C.c_int32, [vp, C.c_int32, C.c_int32, C.c_float, i32p, i32p, f32p, f32p, f32p]),
         "avbd_batch_raycast": (C.c_int32, [vp, C.c_int32, C.c_int32, f32p, f32p, u32p, C.c_int32,
                                            C.POINTER(RayHits)]),
         "avbd_world_pick": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, i32p, f32p]),

@@ -165,7 +165,7 @@ struct Engine {
     bool graphRound = false;
     bool graphSleep = false;
     bool graphsValid = false;
-    hipEvent_t ev[5] = {};
+    hipEvent_t ev[8] = {};   // step phases: 0 start, 1 broad, 2 narrow, 3 solve, 4 finish; 5-7 inside broad
     avbd_step_stats stats{};
     // per-launch profiling (avbd_*_profile): event pairs around launches of each class
     bool profiling = false;
@@ -498,6 +498,38 @@ struct Engine {
             frac[2 * r + 1] = b[r].w;
             if (point) { point[3 * r] = a[r].x; point[3 * r + 1] = a[r].y; point[3 * r + 2] = a[r].z; }
             if (normal) { normal[3 * r] = b[r].x; normal[3 * r + 1] = b[r].y; normal[3 * r + 2] = b[r].z; }
+        }
+        return AVBD_OK;
+    }
+
+    // avbd::Manifold::collide for two bodies of world 0 at their current pose (one narrow-phase
+    // work item; staging rows of query 0).
+    avbd_status collidePair(int a, int b, float hz, int32_t *count, int32_t *features, float *rA, float *rB,
+                            float *basis)
+    {
+        const WorldInfo &wi = worlds[0];
+        if (!wi.active) return fail(AVBD_ERR_INVALID, "world is not active");
+        if (a < 0 || b < 0 || a >= wi.bodyCount || b >= wi.bodyCount || a == b)
+            return fail(AVBD_ERR_INVALID, "bad body index");
+        avbd_status st = ensureQuery(1);
+        if (st != AVBD_OK || (st = ensureShapeQuery(1)) != AVBD_OK) return st;
+        k_collide_pair<<<1, NARROW_TPB, 0, stream>>>(S, wi.bodyBase + a, wi.bodyBase + b, hz, qA, qCRA, qCRB, qCMid,
+                                                     qSpA, qSpB);
+        HIPCHK(hipGetLastError());
+        float4 out[4 + 2 * 8];
+        HIPCHK(hipMemcpyAsync(out, qA, sizeof(out), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        const int n = static_cast<int>(out[0].x);
+        *count = n;
+        if (basis && out[0].y != 0.0f)
+            for (int r = 0; r < 3; r++) {
+                basis[3 * r] = out[1 + r].x; basis[3 * r + 1] = out[1 + r].y; basis[3 * r + 2] = out[1 + r].z;
+            }
+        for (int k = 0; k < n; k++) {
+            const float4 x = out[4 + 2 * k], y = out[5 + 2 * k];
+            if (features) std::memcpy(&features[k], &x.w, sizeof(int32_t));
+            if (rA) { rA[3 * k] = x.x; rA[3 * k + 1] = x.y; rA[3 * k + 2] = x.z; }
+            if (rB) { rB[3 * k] = y.x; rB[3 * k + 1] = y.y; rB[3 * k + 2] = y.z; }
         }
         return AVBD_OK;
     }
@@ -875,6 +907,7 @@ struct Engine {
         }
         // ---- broad phase (solver_broadphase.cpp:85-186)
         k_swept_aabb<<<nb, TPB, 0, stream>>>(S);
+        if (timed) (void)hipEventRecord(ev[5], stream);
         if (rebuildTree) {
             k_world_bounds<<<Wcap, TPB, 0, stream>>>(S);
             k_morton<<<nb, TPB, 0, stream>>>(S, batchMode ? 0 : 1);
@@ -893,11 +926,13 @@ struct Engine {
         pbeg(AVBD_K_REFIT);
         k_refit_tasks<<<refitGrid(), REFIT_TPB, 0, stream>>>(S, Ncap);
         pend();
+        if (timed) (void)hipEventRecord(ev[6], stream);
         pbeg(AVBD_K_TRAVERSE);
         k_traverse<<<nblk(Ncap, tune.traverse), tune.traverse, 0, stream>>>(S, Ncap);
         pend();
         scan(S.pairCount, S.pairOff, Ncap);
         k_pair_emit<<<nb, TPB, 0, stream>>>(S, Ncap);
+        if (timed) (void)hipEventRecord(ev[7], stream);
         k_pair_filter<<<liveGrid(), TPB, 0, stream>>>(S);
         if (sleepOn) k_wake_pairs<<<nblk(Wcap), TPB, 0, stream>>>(S);
         scan(S.pairValid, S.pairValidScan, Pcap, S.pairOff + Ncap);
@@ -1100,7 +1135,11 @@ struct Engine {
         HIPCHK(hipGetLastError());
         avbd_status st = checkErrors();
         if (st != AVBD_OK) return st;
-        float t01 = 0, t12 = 0, t23 = 0, t34 = 0;
+        float t01 = 0, t12 = 0, t23 = 0, t34 = 0, t56 = 0, t67 = 0;
+        (void)hipEventElapsedTime(&t56, ev[5], ev[6]);
+        (void)hipEventElapsedTime(&t67, ev[6], ev[7]);
+        stats.ms_broad_tree = t56;
+        stats.ms_broad_emit = t67;
         (void)hipEventElapsedTime(&t01, ev[0], ev[1]);
         (void)hipEventElapsedTime(&t12, ev[1], ev[2]);
         (void)hipEventElapsedTime(&t23, ev[2], ev[3]);
@@ -2251,6 +2290,13 @@ avbd_status avbd_world_raycast(avbd_world *world, int32_t n, const float *from, 
     if (!world || !out) return fail(AVBD_ERR_INVALID, "null argument");
     return world->eng.raycast(0, n, from, to, masks, hit_from_inside != 0, out->body, out->t, out->point, out->normal,
                               out->inside);
+}
+
+avbd_status avbd_world_collide_pair(avbd_world *world, int32_t body_a, int32_t body_b, float collision_dt,
+                                    int32_t *count, int32_t *features, float *r_a, float *r_b, float *basis)
+{
+    if (!world || !count) return fail(AVBD_ERR_INVALID, "null argument");
+    return world->eng.collidePair(body_a, body_b, collision_dt, count, features, r_a, r_b, basis);
 }
 
 // ---- avbd_world_raycast without allocation ------------------------------------------------

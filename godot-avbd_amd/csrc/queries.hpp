@@ -321,6 +321,52 @@ __global__ void __launch_bounds__(NARROW_TPB) k_intersect_shape(State S, int bas
     cnt[r] = c;
 }
 
+// avbd::Manifold::collide (collide.cpp:1100-1120) for bodies a (as A) and b at their current
+// pose: the speculative margin grown by the pair's predicted relative motion over hz (the same
+// expression as the step's narrow phase), then the step's narrow phase.  One work item, run by
+// lane 0 of one workgroup in the staging rows of query 0.  out[0] = (count, basis written),
+// out[1..3] = basis rows, out[4 + 2k] = (rA, feature key bits), out[5 + 2k] = (rB, 0).
+__global__ void __launch_bounds__(NARROW_TPB) k_collide_pair(State S, int a, int b, float hz, float4 *out, float4 *cra,
+                                                            float4 *crb, float4 *cmid, float4 *spA, float4 *spB)
+{
+    __shared__ NarrowPolys poly;
+    if (threadIdx.x != 0) return;
+    const float4 ma4 = S.massI[a], mb4 = S.massI[b];
+    const float4 sa = S.sizeR[a], sb = S.sizeR[b];
+    const v3 pa = ld3(S.pos[a]), pb = ld3(S.pos[b]);
+    const qt qa = ldq(S.rot[a]), qb = ldq(S.rot[b]);
+    float margin = S.P.specMargin;
+    if (margin > 0.0f) {
+        const float gA = ma4.x > 0.0f ? S.grav[a] : 0.0f;
+        const float gB = mb4.x > 0.0f ? S.grav[b] : 0.0f;
+        const v3 disp = (ld3(S.vel[a]) - ld3(S.vel[b])) * hz + v3{0, 0, (gA - gB) * hz * hz};
+        margin += length(disp) + (sa.w * length(ld3(S.angVel[a])) + sb.w * length(ld3(S.angVel[b]))) * hz;
+    }
+    const int shA = S.shape[a], shB = S.shape[b];
+    Contacts cs{0, cra, crb, cmid};
+    m3 B;
+    bool wrote = false;
+    int n;
+    if (shA != AVBD_SHAPE_BOX || shB != AVBD_SHAPE_BOX) {
+        n = collideRound(shA, shB, makeRShp(pa, qa, sa), makeRShp(pb, qb, sb), S.rim, margin, B, wrote, cs);
+    } else {
+        const Poly c0{&poly[0][0][0], &poly[0][1][0], &poly[0][2][0], spA};
+        const Poly c1{&poly[1][0][0], &poly[1][1][0], &poly[1][2][0], spB};
+        n = collideBoxes(pa, qa, v3{sa.x, sa.y, sa.z} * 0.5f, pb, qb, v3{sb.x, sb.y, sb.z} * 0.5f, margin, B, wrote, cs,
+                         c0, c1);
+    }
+    n = n < 0 ? 0 : n;
+    out[0] = make_float4(static_cast<float>(n), wrote ? 1.0f : 0.0f, 0.0f, 0.0f);
+    out[1] = st3(row(B, 0));
+    out[2] = st3(row(B, 1));
+    out[3] = st3(row(B, 2));
+    for (int k = 0; k < n; k++) {
+        const float4 x = cra[crow(k)], y = crb[crow(k)];
+        out[4 + 2 * k] = make_float4(x.x, x.y, x.z, y.w);
+        out[5 + 2 * k] = make_float4(y.x, y.y, y.z, 0.0f);
+    }
+}
+
 // ---- shape sweeps (query_shape.cpp:213-258, 402-498; the server's cast_motion) ----
 struct SweepShape {
     int type;

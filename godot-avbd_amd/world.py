@@ -159,6 +159,18 @@ class World:
                                                   point.ctypes.data_as(abi.f32p), normal.ctypes.data_as(abi.f32p)))
         return {"body": body, "safe": frac[:, 0], "unsafe": frac[:, 1], "point": point, "normal": normal}
 
+    def collide_pair(self, a, b, collision_dt):
+        """avbd::Manifold::collide for bodies a and b at their current pose on the device:
+        (count, keys, rA, rB, basis 3x3 rows; zeros when the pair is beyond the margin)."""
+        n = C.c_int32()
+        keys, ra, rb = np.zeros(8, np.int32), np.zeros((8, 3), np.float32), np.zeros((8, 3), np.float32)
+        bas = np.zeros(9, np.float32)
+        abi.check(self.lib.avbd_world_collide_pair(self.h, int(a), int(b), float(collision_dt), C.byref(n),
+                                                   keys.ctypes.data_as(abi.i32p), ra.ctypes.data_as(abi.f32p),
+                                                   rb.ctypes.data_as(abi.f32p), bas.ctypes.data_as(abi.f32p)))
+        k = n.value
+        return k, keys[:k], ra[:k], rb[:k], bas.reshape(3, 3)
+
     def contact_report(self, max_reported):
         """Per-body contact reports: (count, position, normal, other), stride max(max_reported)."""
         mr = np.ascontiguousarray(max_reported, np.int32)

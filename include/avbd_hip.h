@@ -209,6 +209,8 @@ typedef struct avbd_step_stats {
     float ms_solve;
     float ms_finish;
     float ms_total;
+    float ms_broad_tree;             /* tree build or refit (part of ms_broad)              */
+    float ms_broad_emit;             /* candidate pair walk and emission (part of ms_broad) */
 } avbd_step_stats;
 
 /* Kernel classes timed by avbd_*_profile (average duration per launch, HIP events on the
@@ -322,6 +324,15 @@ avbd_status avbd_world_raycast(avbd_world *world, int32_t n, const float *from, 
  * count[j] = contacts written.  stride >= every max_reported[j]. */
 avbd_status avbd_world_contact_report(avbd_world *world, const int32_t *max_reported, int32_t stride, int32_t *count,
                                       float *position, float *normal, int32_t *other);
+/* Replaces `avbd::Manifold::collide(bodyA, bodyB, contacts, basis)` (reference
+ * src/avbd/manifold.hpp:61-63, collide.cpp:1100-1120): the solver's narrow phase for bodies
+ * body_a (as A) and body_b (list indices) of the world at their current pose, with the
+ * speculative margin grown by their predicted relative motion over collision_dt (the solver's
+ * collisionDt).  Writes *count contacts (at most 8): features[k] (the feature key), r_a[3k..] /
+ * r_b[3k..] (contact offsets in A's / B's local frame), and basis[9] (row-major; row 0 is the
+ * normal from B to A; left as given when the pair is beyond the margin).  Null outputs skip. */
+avbd_status avbd_world_collide_pair(avbd_world *world, int32_t body_a, int32_t body_b, float collision_dt,
+                                    int32_t *count, int32_t *features, float *r_a, float *r_b, float *basis);
 /* avbd_world_raycast for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  No reference counterpart.
  * Same arguments and results, bit for bit; it calls no hipMalloc / hipFree: its staging (the

@@ -19,6 +19,7 @@
 // unchanged and sets lastStatus / lastError().  Header-only; link libavbd_hip.so.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -38,6 +39,28 @@ struct float3 {
 struct quat {
     float x = 0, y = 0, z = 0, w = 1;
 };
+
+// reference types.hpp:31-33
+constexpr float COLLISION_MARGIN = 0.001f;
+
+// The few vector and matrix operations host code written against the reference uses on
+// solver output (the server's contact loops): linalg's component-wise operators and folds.
+inline float3 operator+(float3 a, float3 b) { return float3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline float3 operator-(float3 a, float3 b) { return float3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline float3 operator-(float3 a) { return float3{-a.x, -a.y, -a.z}; }
+inline float3 operator*(float3 a, float s) { return float3{a.x * s, a.y * s, a.z * s}; }
+inline float3 operator*(float s, float3 a) { return float3{s * a.x, s * a.y, s * a.z}; }
+inline float dot(float3 a, float3 b) { return ((0.0f + a.x * b.x) + a.y * b.y) + a.z * b.z; }
+inline float length(float3 a) { return std::sqrt(dot(a, a)); }
+
+// 3x3 in the reference's layout (maths.h:12: m[col][row])
+struct float3x3 {
+    float3 x, y, z;   // columns
+    float3 &operator[](int c) { return (&x)[c]; }
+    const float3 &operator[](int c) const { return (&x)[c]; }
+};
+inline float3 row(const float3x3 &m, int r) { return float3{m.x[r], m.y[r], m.z[r]}; }
+inline float3 mul(const float3x3 &m, float3 v) { return (m.x * v.x + m.y * v.y) + m.z * v.z; }
 
 struct Solver;
 
@@ -181,6 +204,32 @@ inline quat hostQmul(quat a, quat b)
                 ((a.w * b.z + a.z * b.w) + a.x * b.y) - a.y * b.x, ((a.w * b.w - a.x * b.x) - a.y * b.y) - a.z * b.z};
 }
 
+inline float3 qrot(quat q, float3 v) { return hostQrot(q, v); }
+
+// reference manifold.hpp:28-63: a contact as the manifold accessors hand it out, and the
+// narrow phase for two bodies at their current pose.
+struct Manifold {
+    union FeaturePair {
+        struct {
+            char inR;
+            char outR;
+            char inI;
+            char outI;
+        };
+        int key;
+    };
+    struct Contact {
+        FeaturePair feature;
+        float3 rA;   // contact offset in A's local space
+        float3 rB;   // contact offset in B's local space
+    };
+    // collide.cpp:1100-1120: the solver's narrow phase (speculative margin grown by the pair's
+    // predicted relative motion over the solver's collisionDt), run on the device through
+    // avbd_world_collide_pair.  `contacts` is any contiguous range of Contact (std::span,
+    // std::array, ...); returns the count written.
+    template <class Range> static int collide(Rigid *bodyA, Rigid *bodyB, Range &&contacts, float3x3 &basis);
+};
+
 // reference solver.h:50-144
 struct Solver {
     float dt = 1.0f / 60.0f;
@@ -244,17 +293,64 @@ struct Solver {
     void step() { advance(1, 0); }
     void stepInterleaved(int substeps, int roundsPerSubstep) { advance(substeps, roundsPerSubstep); }
 
-    // Manifold accessors (solver.h:170-231), read from the device.
-    int manifoldCount()
+    // Manifold accessors (solver.h:168-231).  `index` is a position in the live list; the live
+    // manifolds are read from the device once per step, on the first accessor call.
+    int manifoldCount() const { return loadManifolds() ? static_cast<int>(mfA.size()) : 0; }
+    int manifoldContactPoints() const
     {
-        avbd_step_stats s{};
-        return world && avbd_world_stats(world, &s) == AVBD_OK ? s.manifolds : 0;
+        if (!loadManifolds()) return 0;
+        int total = 0;
+        for (int32_t c : mfNum) total += c;
+        return total;
     }
-    int manifoldContactPoints()
+    Rigid *manifoldBodyA(int index) const { return loadManifolds() ? uploaded[static_cast<size_t>(mfA[index])] : nullptr; }
+    Rigid *manifoldBodyB(int index) const { return loadManifolds() ? uploaded[static_cast<size_t>(mfB[index])] : nullptr; }
+    int manifoldContactCount(int index) const { return loadManifolds() ? mfNum[static_cast<size_t>(index)] : 0; }
+    const Manifold::Contact &manifoldContact(int index, int k) const
     {
-        avbd_step_stats s{};
-        return world && avbd_world_stats(world, &s) == AVBD_OK ? s.contacts : 0;
+        loadManifolds();
+        return mfContacts[8 * static_cast<size_t>(index) + static_cast<size_t>(k)];
     }
+    // Constraint value at the step-start pose: x is the normal error (positive = overlap).
+    float3 manifoldContactError(int index, int k) const
+    {
+        loadManifolds();
+        const float *c = &mfC0[24 * static_cast<size_t>(index) + 3 * static_cast<size_t>(k)];
+        return float3{c[0], c[1], c[2]};
+    }
+    // First basis row: the contact normal, pointing from B to A.
+    float3 manifoldNormal(int index) const
+    {
+        loadManifolds();
+        const float *b = &mfBasis[9 * static_cast<size_t>(index)];
+        return float3{b[0], b[1], b[2]};
+    }
+    // Diagnostics (solver.h:171-174, 219-220): colour groups in use, bodies in the Jacobi
+    // bucket, candidate pairs of the last broad phase.
+    int colourGroups() const { return stat().colours; }
+    int overflowBodies() const { return stat().overflow_bodies; }
+    int broadPhasePairCount() const { return stat().candidate_pairs; }
+    // solver.h:242-243: host threads driving the step (one; the step itself runs on the GPU).
+    int threadCount() const { return 1; }
+
+    // solver.h:245-275: per-phase device time of the last step (ms, HIP events).  "graph" is the
+    // warm start of the forces, the narrow phase and the constraint graph with its colouring.
+    struct PhaseTimes {
+        double broadPhase = 0.0;
+        double broadIndex = 0.0;
+        double broadTree = 0.0;
+        double broadEmit = 0.0;
+        double graph = 0.0;
+        double solve = 0.0;
+        double finish = 0.0;
+    };
+    PhaseTimes lastStepTimes;
+    double get_broadPhase_time() const { return lastStepTimes.broadPhase; }
+    double get_broadTree_time() const { return lastStepTimes.broadTree; }
+    double get_broadEmit_time() const { return lastStepTimes.broadEmit; }
+    double get_colourGraph_time() const { return lastStepTimes.graph; }
+    double get_solve_time() const { return lastStepTimes.solve; }
+    double get_finish_time() const { return lastStepTimes.finish; }
     // Solver::pick (solver.h:161-164, solver_query.cpp:29-155): the nearest body along the ray
     // (null if none) and the hit point in its frame; every body is tested, static ones too.
     Rigid *pick(float3 origin, float3 dir, float3 &local) { return pick(origin, dir, local, 0xFFFFFFFFu); }
@@ -296,8 +392,74 @@ struct Solver {
     // --- internal ---------------------------------------------------------------
     bool topologyChanged = true;
 
+    // Manifold::collide for two of this solver's bodies (the device's narrow phase).
+    int collidePair(Rigid *a, Rigid *b, Manifold::Contact *out, size_t capacity, float3x3 &basis)
+    {
+        std::vector<Rigid *> list;
+        std::vector<Force *> jl;
+        if (!pushState(list, jl)) return 0;
+        int ia = -1, ib = -1;
+        for (size_t i = 0; i < list.size(); i++) {
+            if (list[i] == a) ia = static_cast<int>(i);
+            if (list[i] == b) ib = static_cast<int>(i);
+        }
+        if (ia < 0 || ib < 0) return 0;
+        int32_t count = 0, feat[8];
+        float ra[24], rb[24], bas[9];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) bas[3 * r + c] = basis[c][r];
+        lastStatus = avbd_world_collide_pair(world, ia, ib, collisionDt, &count, feat, ra, rb, bas);
+        if (lastStatus != AVBD_OK) return 0;
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) basis[c][r] = bas[3 * r + c];
+        const int n = static_cast<int>(std::min<size_t>(static_cast<size_t>(count), capacity));
+        for (int k = 0; k < n; k++) {
+            out[k].feature.key = feat[k];
+            out[k].rA = float3{ra[3 * k], ra[3 * k + 1], ra[3 * k + 2]};
+            out[k].rB = float3{rb[3 * k], rb[3 * k + 1], rb[3 * k + 2]};
+        }
+        return n;
+    }
+
   private:
     avbd_world *world = nullptr;
+    // live manifolds of the last step, read on the first accessor call after it
+    mutable bool mfValid = false;
+    mutable std::vector<int32_t> mfA, mfB, mfNum;
+    mutable std::vector<float> mfBasis, mfC0;
+    mutable std::vector<Manifold::Contact> mfContacts;
+
+    avbd_step_stats stat() const
+    {
+        avbd_step_stats s{};
+        if (world) avbd_world_stats(world, &s);
+        return s;
+    }
+
+    bool loadManifolds() const
+    {
+        if (mfValid) return true;
+        if (!world) return false;
+        const int m = stat().manifolds;
+        mfA.assign(m, 0); mfB.assign(m, 0); mfNum.assign(m, 0);
+        mfBasis.assign(9 * static_cast<size_t>(m), 0.0f);
+        mfC0.assign(24 * static_cast<size_t>(m), 0.0f);
+        mfContacts.assign(8 * static_cast<size_t>(m), Manifold::Contact{});
+        std::vector<float> fr(m), ra(24 * static_cast<size_t>(m)), rb(24 * static_cast<size_t>(m)),
+                pen(24 * static_cast<size_t>(m)), lam(24 * static_cast<size_t>(m)), beta(8 * static_cast<size_t>(m));
+        std::vector<int32_t> feat(8 * static_cast<size_t>(m));
+        std::vector<uint8_t> stick(8 * static_cast<size_t>(m));
+        avbd_manifolds mf{m, mfA.data(), mfB.data(), mfBasis.data(), fr.data(), mfNum.data(), feat.data(), ra.data(),
+                          rb.data(), mfC0.data(), pen.data(), lam.data(), stick.data(), beta.data()};
+        if (m > 0 && avbd_world_get_manifolds(world, &mf) != AVBD_OK) return false;
+        for (size_t k = 0; k < 8 * static_cast<size_t>(m); k++) {
+            mfContacts[k].feature.key = feat[k];
+            mfContacts[k].rA = float3{ra[3 * k], ra[3 * k + 1], ra[3 * k + 2]};
+            mfContacts[k].rB = float3{rb[3 * k], rb[3 * k + 1], rb[3 * k + 2]};
+        }
+        mfValid = true;
+        return true;
+    }
     int capBodies = 0, capJoints = 0;
     std::vector<Rigid *> uploaded;
     std::vector<Force *> uploadedJoints;
@@ -493,6 +655,7 @@ struct Solver {
         }
         uploaded = list;
         uploadedJoints = jl;
+        mfValid = false;
         return true;
     }
 
@@ -536,8 +699,17 @@ struct Solver {
         for (Force *f : jl)
             if (f->kind == AVBD_FORCE_JOINT && static_cast<Joint *>(f)->broken) reaped.push_back(f);
         collisionDt = substeps > 1 ? dt * static_cast<float>(substeps) : dt;
+        mfValid = false;
         lastStatus = substeps > 1 ? avbd_world_step_interleaved(world, substeps, rounds) : avbd_world_step(world);
         if (lastStatus != AVBD_OK) return;
+        const avbd_step_stats st = stat();
+        lastStepTimes.broadPhase = st.ms_broad;
+        lastStepTimes.broadTree = st.ms_broad_tree;
+        lastStepTimes.broadEmit = st.ms_broad_emit;
+        lastStepTimes.broadIndex = static_cast<double>(st.ms_broad_tree) + st.ms_broad_emit;
+        lastStepTimes.graph = st.ms_narrow;
+        lastStepTimes.solve = st.ms_solve;
+        lastStepTimes.finish = st.ms_finish;
         pull(list, jl, reaped);
     }
 
@@ -685,6 +857,12 @@ inline GenericJoint::GenericJoint(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bod
     const quat f = p_bodyA ? p_bodyA->positionAng : quat{};
     rest = hostQmul(quat{-f.x, -f.y, -f.z, f.w}, p_bodyB->positionAng);
     p_solver->topologyChanged = true;
+}
+
+template <class Range>
+int Manifold::collide(Rigid *bodyA, Rigid *bodyB, Range &&contacts, float3x3 &basis)
+{
+    return bodyA->solver->collidePair(bodyA, bodyB, contacts.data(), contacts.size(), basis);
 }
 
 inline IgnoreCollision::IgnoreCollision(Solver *p_solver, Rigid *p_bodyA, Rigid *p_bodyB)

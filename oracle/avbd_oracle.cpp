@@ -2851,6 +2851,25 @@ int oracle_set_manifolds(oracle_world *o, const avbd_manifolds *m)
 
 void oracle_plant(int id) { g_plant = id; }
 
+// avbd::Manifold::collide (collide.cpp:1100-1120) for bodies a (as A) and b at their current
+// pose: the step's narrow phase with the speculative margin grown over hz.
+int oracle_collide_pair(oracle_world *o, int a, int b, float hz, int32_t *keys, float *rA, float *rB, float *basis)
+{
+    World &w = o->w;
+    const int n = static_cast<int>(w.bodies.size());
+    if (a < 0 || b < 0 || a >= n || b >= n || a == b) return -1;
+    const float keep = w.collisionDt;
+    w.collisionDt = hz;
+    const float margin = w.marginFor(w.bodies[a], w.bodies[b]);
+    w.collisionDt = keep;
+    const Body &A = w.bodies[a], &B = w.bodies[b];
+    const float pa[3] = {A.pos.x, A.pos.y, A.pos.z}, qa[4] = {A.q.x, A.q.y, A.q.z, A.q.w},
+                sa[3] = {A.size.x, A.size.y, A.size.z};
+    const float pb[3] = {B.pos.x, B.pos.y, B.pos.z}, qb[4] = {B.q.x, B.q.y, B.q.z, B.q.w},
+                sb[3] = {B.size.x, B.size.y, B.size.z};
+    return oracle_collide_shapes(A.shape, pa, qa, sa, B.shape, pb, qb, sb, margin, basis, keys, rA, rB);
+}
+
 uint64_t oracle_digest(oracle_world *o)
 {
     uint64_t h = 1469598103934665603ull;

@@ -25,6 +25,9 @@ int oracle_get_manifolds(oracle_world *o, avbd_manifolds *m);
 int oracle_set_manifolds(oracle_world *o, const avbd_manifolds *m);
 // Planted mistakes for the oracle's own tests (0 = none; the ids are listed in avbd_oracle.cpp).
 void oracle_plant(int id);
+// avbd::Manifold::collide for bodies a and b at their current pose (margin grown over hz);
+// returns the contact count (-1 for bad indices).
+int oracle_collide_pair(oracle_world *o, int a, int b, float hz, int32_t *keys, float *rA, float *rB, float *basis);
 uint64_t oracle_digest(oracle_world *o);
 /* out: manifolds, contacts, candidate pairs, new manifolds, colours, overflow bodies */
 void oracle_stats(oracle_world *o, int32_t out[6]);

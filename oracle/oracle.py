@@ -60,6 +60,7 @@ def lib():
         "oracle_set_manifolds": (C.c_int, [vp, C.POINTER(a.Manifolds)]),
         "oracle_digest": (C.c_uint64, [vp]),
         "oracle_plant": (None, [C.c_int]),
+        "oracle_collide_pair": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, a.i32p, a.f32p, a.f32p, a.f32p]),
         "oracle_stats": (None, [vp, a.i32p]),
         "oracle_colours": (None, [vp, a.u8p]),
         "oracle_swept_aabbs": (None, [vp, a.f32p, a.f32p]),
@@ -302,6 +303,16 @@ class OracleWorld:
 
     def digest(self):
         return int(self.L.oracle_digest(self.h))
+
+    def collide_pair(self, a, b, hz):
+        """avbd::Manifold::collide for bodies a and b now: (count, keys, rA, rB, basis 3x3 rows)."""
+        keys, ra, rb = np.zeros(8, np.int32), np.zeros((8, 3), np.float32), np.zeros((8, 3), np.float32)
+        bas = np.zeros(9, np.float32)
+        n = self.L.oracle_collide_pair(self.h, int(a), int(b), float(hz), keys.ctypes.data_as(self.a.i32p),
+                                       ra.ctypes.data_as(self.a.f32p), rb.ctypes.data_as(self.a.f32p),
+                                       bas.ctypes.data_as(self.a.f32p))
+        assert n >= 0
+        return n, keys[:n], ra[:n], rb[:n], bas.reshape(3, 3)
 
     def stats(self):
         out = (C.c_int32 * 6)()

@@ -2,9 +2,11 @@
 // the way the reference's own core tests build their scenes (test/core_test.cpp:391-414,
 // 612-631, 271-300): `new Rigid(&s, ...)`, `new Joint(&s, ...)`, `s.step()`.  The expected
 // digests are the reference's recorded ones (tools/baseline/digests.txt).
+#include <array>
 #include <cinttypes>
 #include <cmath>
 #include <cstdio>
+#include <string>
 #include <vector>
 
 #include "avbd_solver.hpp"
@@ -283,8 +285,153 @@ static void test_idle_sleep()
            "idle bodies fall asleep", d);
 }
 
-int main()
+// The Godot server's two per-tick loops over the solver's output, as it writes them
+// (avbd_physics_server3d.cpp:1547-1585: the deepest stored contact's pair re-measured by
+// Manifold::collide at the current pose; :1620-1652: each reporting body's contacts, in live
+// order, up to its max_contacts_reported), run on a scene read from `scene` after `steps`
+// steps.  Everything the loops read is written to `out` (floats as C99 hex) for the caller to
+// compare with the oracle.  Scene format: n steps, then per body in creation order: shape,
+// size xyz, mass, moment xyz, friction, radius, position xyz, rotation xyzw, velocity xyz,
+// angular velocity xyz, max_contacts_reported.
+namespace avbd = avbd_hip;   // what a host written against the reference switches by
+
+static int server_loops(const char *scenePath, const char *outPath)
 {
+    FILE *in = std::fopen(scenePath, "r");
+    if (!in) return 2;
+    int n = 0, steps = 0;
+    if (std::fscanf(in, "%d %d", &n, &steps) != 2) return 2;
+    avbd::Solver solver;
+    std::vector<avbd::Rigid *> created;
+    std::vector<int> maxReported;
+    for (int i = 0; i < n; i++) {
+        int shape = 0, rep = 0;
+        float v[22];
+        if (std::fscanf(in, "%d", &shape) != 1) return 2;
+        for (float &x : v)
+            if (std::fscanf(in, "%f", &x) != 1) return 2;
+        if (std::fscanf(in, "%d", &rep) != 1) return 2;
+        auto *r = new avbd::Rigid(&solver, {v[0], v[1], v[2]}, static_cast<avbd::ShapeType>(shape), 1.0f, v[7],
+                                  {v[9], v[10], v[11]}, {v[16], v[17], v[18]});
+        r->mass = v[3];
+        r->moment = avbd::float3{v[4], v[5], v[6]};
+        r->radius = v[8];
+        r->positionAng = avbd::quat{v[12], v[13], v[14], v[15]};
+        r->velocityAng = avbd::float3{v[19], v[20], v[21]};
+        created.push_back(r);
+        maxReported.push_back(rep);
+    }
+    std::fclose(in);
+    for (int k = 0; k < steps; k++) solver.step();
+    if (solver.lastStatus != AVBD_OK) return 3;
+    std::vector<avbd::Rigid *> list;
+    for (avbd::Rigid *b = solver.bodies; b; b = b->next) list.push_back(b);
+    auto indexOf = [&](const avbd::Rigid *r) {
+        for (size_t i = 0; i < list.size(); i++)
+            if (list[i] == r) return static_cast<int>(i);
+        return -1;
+    };
+    FILE *out = std::fopen(outPath, "w");
+    if (!out) return 2;
+    auto f3 = [&](avbd::float3 v) { std::fprintf(out, " %a %a %a", v.x, v.y, v.z); };
+    // every accessor, as the loops call them
+    std::fprintf(out, "M %d\n", solver.manifoldCount());
+    for (int m = 0; m < solver.manifoldCount(); m++) {
+        const int count = solver.manifoldContactCount(m);
+        std::fprintf(out, "%d %d %d", indexOf(solver.manifoldBodyA(m)), indexOf(solver.manifoldBodyB(m)), count);
+        f3(solver.manifoldNormal(m));
+        std::fprintf(out, "\n");
+        for (int i = 0; i < count; i++) {
+            const avbd::Manifold::Contact &c = solver.manifoldContact(m, i);
+            std::fprintf(out, "%d", c.feature.key);
+            f3(solver.manifoldContactError(m, i));
+            f3(c.rA);
+            f3(c.rB);
+            std::fprintf(out, "\n");
+        }
+    }
+    // avbd_physics_server3d.cpp:1547-1585
+    avbd::Rigid *deepest_a = nullptr;
+    avbd::Rigid *deepest_b = nullptr;
+    double deepest_stored = 0.0;
+    for (int m = 0; m < solver.manifoldCount(); m++) {
+        const int contacts = solver.manifoldContactCount(m);
+        for (int i = 0; i < contacts; i++) {
+            const double depth = static_cast<double>(avbd::COLLISION_MARGIN - solver.manifoldContactError(m, i).x);
+            if (depth > deepest_stored) {
+                deepest_stored = depth;
+                deepest_a = solver.manifoldBodyA(m);
+                deepest_b = solver.manifoldBodyB(m);
+            }
+        }
+    }
+    double diag_max_penetration = 0.0;
+    if (deepest_a != nullptr) {
+        std::array<avbd::Manifold::Contact, 8> fresh{};
+        avbd::float3x3 freshBasis{};
+        const int freshCount = avbd::Manifold::collide(deepest_a, deepest_b, fresh, freshBasis);
+        std::fprintf(out, "D %d %d %d", indexOf(deepest_a), indexOf(deepest_b), freshCount);
+        f3(avbd::row(freshBasis, 0));
+        f3(avbd::row(freshBasis, 1));
+        f3(avbd::row(freshBasis, 2));
+        std::fprintf(out, "\n");
+        for (int i = 0; i < freshCount; i++) {
+            const avbd::float3 xA = deepest_a->positionLin + avbd::qrot(deepest_a->positionAng, fresh[i].rA);
+            const avbd::float3 xB = deepest_b->positionLin + avbd::qrot(deepest_b->positionAng, fresh[i].rB);
+            const avbd::float3 c0 = avbd::mul(freshBasis, xA - xB) + avbd::float3{avbd::COLLISION_MARGIN, 0, 0};
+            const double depth = static_cast<double>(avbd::COLLISION_MARGIN - c0.x);
+            if (depth > diag_max_penetration) diag_max_penetration = depth;
+            std::fprintf(out, "%d", fresh[i].feature.key);
+            f3(fresh[i].rA);
+            f3(fresh[i].rB);
+            std::fprintf(out, "\n");
+        }
+    } else {
+        std::fprintf(out, "D -1 -1 0\n");
+    }
+    std::fprintf(out, "P %a\n", diag_max_penetration);
+    // avbd_physics_server3d.cpp:1620-1652, with frame_contacts a vector per body
+    struct FrameContact { avbd::float3 position, normal; int other; };
+    std::vector<std::vector<FrameContact>> frame(list.size());
+    for (int m = 0; m < solver.manifoldCount(); m++) {
+        const int count = solver.manifoldContactCount(m);
+        if (count <= 0) continue;
+        avbd::Rigid *body_a = solver.manifoldBodyA(m);
+        avbd::Rigid *body_b = solver.manifoldBodyB(m);
+        const int data[2] = {indexOf(body_a), indexOf(body_b)};
+        const avbd::float3 normal = solver.manifoldNormal(m);
+        for (int side = 0; side < 2; side++) {
+            const int fill = data[side];
+            const int maxc = maxReported[static_cast<size_t>(n - 1 - fill)];   // list index -> creation index
+            if (fill < 0 || maxc <= 0) continue;
+            const int other = data[side == 0 ? 1 : 0];
+            const avbd::Rigid *self_rigid = side == 0 ? body_a : body_b;
+            for (int c = 0; c < count && static_cast<int32_t>(frame[fill].size()) < maxc; c++) {
+                const avbd::Manifold::Contact &contact = solver.manifoldContact(m, c);
+                const avbd::float3 world = self_rigid->positionLin +
+                                           avbd::qrot(self_rigid->positionAng, side == 0 ? contact.rA : contact.rB);
+                frame[fill].push_back(FrameContact{world, normal * (side == 0 ? 1.0f : -1.0f), other});
+            }
+        }
+    }
+    for (size_t i = 0; i < frame.size(); i++) {
+        std::fprintf(out, "R %zu %zu\n", i, frame[i].size());
+        for (const FrameContact &fc : frame[i]) {
+            f3(fc.position);
+            f3(fc.normal);
+            std::fprintf(out, " %d\n", fc.other);
+        }
+    }
+    const avbd::Solver::PhaseTimes &t = solver.lastStepTimes;
+    std::fprintf(out, "T %d %.6f %.6f %.6f %.6f %.6f %.6f %.6f\n", solver.threadCount(), t.broadPhase, t.broadIndex,
+                 t.broadTree, t.broadEmit, t.graph, t.solve, t.finish);
+    std::fclose(out);
+    return 0;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc == 4 && std::string(argv[1]) == "--server-loops") return server_loops(argv[2], argv[3]);
     const uint64_t want[3] = {0xfbac4f08cb1c977cull, 0xb03adeb37742850aull, 0xfb3ca7a3162035c2ull};
     const uint64_t got[3] = {stack_digest(), interleaved_digest(), mixed_digest()};
     const char *names[3] = {"digest stack_300", "digest interleaved_300", "digest mixed_240"};
