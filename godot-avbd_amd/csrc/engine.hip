@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -124,8 +125,6 @@ avbd_status checkParams(const avbd_params *p)
     if (p->iterations < 1) return fail(AVBD_ERR_INVALID, "iterations must be >= 1");
     if (p->jacobian_rebuild_distance > 0.0f)
         return fail(AVBD_ERR_UNSUPPORTED, "jacobian arm caching is outside this path");
-    if (p->convergence_threshold > 0.0f)
-        return fail(AVBD_ERR_UNSUPPORTED, "the convergence early exit is outside this path");
     return AVBD_OK;
 }
 
@@ -215,10 +214,13 @@ struct Engine {
     {
         for (auto &g : graphs)
             if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+        dropPlainGraphs();
         freeAll();
         if (errHost) (void)hipHostFree(errHost);
         if (countsHost) (void)hipHostFree(countsHost);
+        if (phaseTicksHost) (void)hipHostFree(phaseTicksHost);
         errHost = countsHost = nullptr;
+        phaseTicksHost = nullptr;
         for (auto &e : ev)
             if (e) { (void)hipEventDestroy(e); e = nullptr; }
         for (auto &e : profEvents) (void)hipEventDestroy(e);
@@ -695,6 +697,9 @@ struct Engine {
             for (auto &e : ev) HIPCHK(hipEventCreate(&e));
             HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&errHost), sizeof(int) * Wcap, hipHostMallocDefault));
             HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&countsHost), sizeof(int) * 64, hipHostMallocDefault));
+            HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&phaseTicksHost), 8 * sizeof(uint64_t), hipHostMallocDefault));
+            int khz = 0;
+            if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess) wallClockKHz = khz;
         }
         avbd_status st;
 #define AL(ptr, n) if ((st = A(ptr, n)) != AVBD_OK) return st
@@ -732,6 +737,8 @@ struct Engine {
         AL(S.wHasJoints, W); AL(S.wUpdateCount, W); AL(S.wErr, W); AL(S.wProjDone, W); AL(S.wMaxDepth, W);
         AL(S.wColourLeft, W);
         AL(S.mortonCode, N); AL(S.sortKey, N); AL(S.sortKeyAlt, N); AL(S.sortVal, N); AL(S.sortValAlt, N);
+        AL(S.colKey, N); AL(S.colKeyAlt, N); AL(S.colVal, N); AL(S.colValAlt, N); AL(S.colStart, 16);
+        AL(phaseTicks, 8);
         AL(S.nodeLeft, N); AL(S.nodeRight, N); AL(S.nodeParent, 2 * N); AL(S.nodeFirst, N); AL(S.nodeLast, N);
         AL(S.nodeFlag, N); AL(S.nodeBox, 4 * N); AL(S.nodeRope, 2 * N); AL(S.taskFlag, 2 * N + 1); AL(S.taskScan, 2 * N + 1); AL(S.taskNode, 2 * N); AL(S.colourCnt, N); AL(S.colourQueue, N); AL(S.leafBody, N); AL(S.bodyLeaf, N);
         AL(S.pairCount, N + 1); AL(S.pairOff, N + 1); AL(S.pairI, P); AL(S.pairJ, P);
@@ -768,6 +775,22 @@ struct Engine {
         k_scan_sums<<<1, SCAN_SUMS_TPB, 0, stream>>>(tileSums, std::max(nt, 1), out, n, dlen);
         k_scan_add<<<std::max(nt, 1), TPB, 0, stream>>>(out, tileSums, n, dlen);
     }
+
+    // Bodies sorted by colour, one stable 8-bit radix pass (the colour passes then walk their
+    // own run instead of every body).
+    void colourRuns()
+    {
+        const int n = Ncap, nb = radixBlocks;
+        k_colour_keys<<<nblk(n), TPB, 0, stream>>>(S);
+        k_radix_hist<<<nb, TPB, 0, stream>>>(S.colKey, n, 0, radixHist, nb);
+        scan(radixHist, radixHist, 256 * nb);
+        k_radix_scatter<<<nb, TPB, 0, stream>>>(S.colKey, S.colVal, S.colKeyAlt, S.colValAlt, n, 0, radixHist, nb);
+        k_colour_starts<<<1, 64, 0, stream>>>(S, radixHist, nb);
+    }
+    // one lane per body of the largest possible run, in one-wavefront workgroups so that a run
+    // spreads over every CU (the passes are bound by the latency of their dependent loads)
+    static constexpr int COLOUR_PASS_TPB = 64;
+    int colourGrid() const { return nblk(Ncap, COLOUR_PASS_TPB); }
 
     void radixSort()
     {
@@ -900,14 +923,14 @@ struct Engine {
         const bool sleepOn = anySleep();
         S.P.sleepOn = sleepOn ? 1 : 0;
         const int nb = nblk(Ncap);
-        if (timed) (void)hipEventRecord(ev[0], stream);
+        if (timed) phaseMark(0);
         if (!csrValid) {
             refreshView();
             buildAdjacency();
         }
         // ---- broad phase (solver_broadphase.cpp:85-186)
         k_swept_aabb<<<nb, TPB, 0, stream>>>(S);
-        if (timed) (void)hipEventRecord(ev[5], stream);
+        if (timed) phaseMark(5);
         if (rebuildTree) {
             k_world_bounds<<<Wcap, TPB, 0, stream>>>(S);
             k_morton<<<nb, TPB, 0, stream>>>(S, batchMode ? 0 : 1);
@@ -926,19 +949,19 @@ struct Engine {
         pbeg(AVBD_K_REFIT);
         k_refit_tasks<<<refitGrid(), REFIT_TPB, 0, stream>>>(S, Ncap);
         pend();
-        if (timed) (void)hipEventRecord(ev[6], stream);
+        if (timed) phaseMark(6);
         pbeg(AVBD_K_TRAVERSE);
         k_traverse<<<nblk(Ncap, tune.traverse), tune.traverse, 0, stream>>>(S, Ncap);
         pend();
         scan(S.pairCount, S.pairOff, Ncap);
         k_pair_emit<<<nb, TPB, 0, stream>>>(S, Ncap);
-        if (timed) (void)hipEventRecord(ev[7], stream);
+        if (timed) phaseMark(7);
         k_pair_filter<<<liveGrid(), TPB, 0, stream>>>(S);
         if (sleepOn) k_wake_pairs<<<nblk(Wcap), TPB, 0, stream>>>(S);
         scan(S.pairValid, S.pairValidScan, Pcap, S.pairOff + Ncap);
         k_alloc_new<<<liveGrid(), TPB, 0, stream>>>(S);
         k_alloc_counts<<<nblk(Wcap), TPB, 0, stream>>>(S);
-        if (timed) (void)hipEventRecord(ev[1], stream);
+        if (timed) phaseMark(1);
         // ---- warm start forces / narrow phase (solver_solve.cpp:31-105)
         k_joint_init<<<nblk(Jcap), TPB, 0, stream>>>(S);
         pbeg(AVBD_K_NARROW);
@@ -985,6 +1008,7 @@ struct Engine {
             k_colour_finish<<<Wcap, COLOUR_TPB, 0, stream>>>(S);
         }
         k_colour_override<<<nb, TPB, 0, stream>>>(S);
+        if (withColours) colourRuns();
         // ---- brake schedule and first brake (solver_projection.cpp:96-167)
         k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wUpdateCount, 0, Wcap);
         const bool brake = prm.speculative_margin > 0.0f;
@@ -996,7 +1020,7 @@ struct Engine {
             k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.brakeLevels, 0, Wcap);
         }
         pend();
-        if (timed) (void)hipEventRecord(ev[2], stream);
+        if (timed) phaseMark(2);
         // ---- substeps (solver_step.cpp:79-105, 142-150)
         const int rounds = !inter ? std::max(1, prm.iterations)
                                   : (roundsPer > 0 ? std::max(1, roundsPer)
@@ -1004,6 +1028,11 @@ struct Engine {
         const int newtonRounds =
                 std::clamp(static_cast<int>(std::ceil(prm.newton_ratio * static_cast<float>(rounds))), 1, rounds);
         const int subs = inter ? substeps : 1;
+        // solver_step.cpp:299-314: the convergence probe compares a round's mean penetration with
+        // the previous round's, starting from 0 each substep (:88-89).  It reads the step-start C0,
+        // which no round changes, so residual >= 0 and 0 - residual < threshold always holds: with
+        // a positive threshold the reference stops after the first round of every substep.
+        const int runRounds = prm.convergence_threshold > 0.0f && rounds > 1 ? 1 : rounds;
         for (int sub = 0; sub < subs; sub++) {
             if (sub > 0) {
                 k_rebase<<<liveGrid(), TPB, 0, stream>>>(S);
@@ -1016,19 +1045,18 @@ struct Engine {
             pbeg(AVBD_K_BODY);
             k_warmstart<<<nb, TPB, 0, stream>>>(S);
             pend();
-            for (int it = 0; it < rounds; it++) {
+            for (int it = 0; it < runRounds; it++) {
                 const bool newton = it < newtonRounds;
                 // Jacobi stamps come from the previous dual unless colour passes moved bodies
                 // since (or this is the first iteration of the substep).
                 const bool stampHere = withColours || it == 0;
-                const bool stampInDual = !withColours && it + 1 < newtonRounds;
+                const bool stampInDual = !withColours && it + 1 < std::min(newtonRounds, runRounds);
                 if (newton) {
                     if (withColours)
                         for (int c = 0; c < 8; c++) {
                             pbeg(AVBD_K_COLOUR_PASS);
-                            k_colour_pass<<<nb, TPB, 0, stream>>>(S, c);
+                            k_colour_pass<<<colourGrid(), COLOUR_PASS_TPB, 0, stream>>>(S, c);
                             pend();
-                            k_solve_fallback<<<1, TPB, 0, stream>>>(S, 1);
                         }
                     if (stampHere) {
                         k_pose_pack<<<nb, TPB, 0, stream>>>(S);
@@ -1063,7 +1091,7 @@ struct Engine {
             k_finish<<<nb, TPB, 0, stream>>>(S, sub + 1 == subs ? 1 : 0);
             pend();
         }
-        if (timed) (void)hipEventRecord(ev[3], stream);
+        if (timed) phaseMark(3);
         // ---- final brake, position projection, rebase (solver_step.cpp:69-70)
         if (brake) {
             pbeg(AVBD_K_BRAKE_REPLAY);
@@ -1083,7 +1111,7 @@ struct Engine {
             }
         }
         k_rebase<<<liveGrid(), TPB, 0, stream>>>(S);
-        if (timed) (void)hipEventRecord(ev[4], stream);
+        if (timed) phaseMark(4);
         csrValid = true;
     }
 
@@ -1127,23 +1155,80 @@ struct Engine {
         return fail((e & AVBD_WORLD_SLOTS) ? AVBD_ERR_CAPACITY : AVBD_ERR_DEVICE, what);
     }
 
+    // Phase boundaries of a timed step: a one-thread kernel stores the device's constant-rate
+    // clock (wall_clock64) at each mark, so the marks time the step inside a captured graph too.
+    uint64_t *phaseTicks = nullptr;    // device, 8 marks
+    uint64_t *phaseTicksHost = nullptr;
+    double wallClockKHz = 0.0;
+    void phaseMark(int k) { k_phase_mark<<<1, 64, 0, stream>>>(phaseTicks, k); }
+    float phaseMs(int a, int b) const
+    {
+        const uint64_t t0 = phaseTicksHost[a], t1 = phaseTicksHost[b];
+        return wallClockKHz > 0.0 && t1 >= t0 ? static_cast<float>(static_cast<double>(t1 - t0) / wallClockKHz) : 0.0f;
+    }
+
+    // Captured plain steps, one per (substeps, rounds, tree rebuilt, joints, round shapes,
+    // sleeping): a plain world's step is one graph launch, like a batch's.  The phase events are
+    // captured with it (event record nodes); a step whose launch shapes or parameters changed
+    // is captured again.
+    std::map<std::array<int, 6>, hipGraphExec_t> plainGraphs;
+    avbd_params plainGraphPrm{};
+    bool plainGraphUnusable = false;   // capture failed once: step kernel by kernel
+    void dropPlainGraphs()
+    {
+        for (auto &kv : plainGraphs)
+            if (kv.second) (void)hipGraphExecDestroy(kv.second);
+        plainGraphs.clear();
+    }
+    hipGraphExec_t plainGraph(int substeps, int rounds, bool rebuild)
+    {
+        if (std::memcmp(&plainGraphPrm, &prm, sizeof(prm)) != 0 || !graphsValid) {
+            dropPlainGraphs();
+            plainGraphPrm = prm;
+            graphsValid = true;   // the batch graphs share the validity flag; a plain world has none
+        }
+        const std::array<int, 6> key{substeps, rounds, rebuild ? 1 : 0, anyJoints() ? 1 : 0, anyRound() ? 1 : 0,
+                                     anySleep() ? 1 : 0};
+        auto it = plainGraphs.find(key);
+        if (it != plainGraphs.end()) return it->second;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+        enqueueStep(substeps, rounds, rebuild, anyJoints(), true);
+        if (hipStreamEndCapture(stream, &graph) != hipSuccess || !graph) return nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return nullptr;
+        plainGraphs[key] = exec;
+        return exec;
+    }
+
     avbd_status stepPlain(int substeps, int rounds)
     {
         const bool rebuild = treeAge >= refitBudget;
         treeAge = rebuild ? 0 : treeAge + 1;
-        enqueueStep(substeps, rounds, rebuild, anyJoints(), true);
+        hipGraphExec_t g = nullptr;
+        if (useGraph && !plainGraphUnusable) {
+            if (!csrValid) {
+                refreshView();
+                buildAdjacency();
+                csrValid = true;
+            }
+            g = plainGraph(substeps, rounds, rebuild);
+            if (!g) {
+                (void)hipGetLastError();
+                plainGraphUnusable = true;
+            }
+        }
+        if (g) HIPCHK(hipGraphLaunch(g, stream));
+        else enqueueStep(substeps, rounds, rebuild, anyJoints(), true);
         HIPCHK(hipGetLastError());
         avbd_status st = checkErrors();
         if (st != AVBD_OK) return st;
-        float t01 = 0, t12 = 0, t23 = 0, t34 = 0, t56 = 0, t67 = 0;
-        (void)hipEventElapsedTime(&t56, ev[5], ev[6]);
-        (void)hipEventElapsedTime(&t67, ev[6], ev[7]);
-        stats.ms_broad_tree = t56;
-        stats.ms_broad_emit = t67;
-        (void)hipEventElapsedTime(&t01, ev[0], ev[1]);
-        (void)hipEventElapsedTime(&t12, ev[1], ev[2]);
-        (void)hipEventElapsedTime(&t23, ev[2], ev[3]);
-        (void)hipEventElapsedTime(&t34, ev[3], ev[4]);
+        HIPCHK(hipMemcpy(phaseTicksHost, phaseTicks, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        const float t01 = phaseMs(0, 1), t12 = phaseMs(1, 2), t23 = phaseMs(2, 3), t34 = phaseMs(3, 4);
+        stats.ms_broad_tree = phaseMs(5, 6);
+        stats.ms_broad_emit = phaseMs(6, 7);
         stats.ms_broad = t01;
         stats.ms_narrow = t12;
         stats.ms_solve = t23;
@@ -2255,6 +2340,13 @@ avbd_status avbd_batch_remove_world(avbd_batch *batch, int32_t world_id)
     if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
     if (world_id < 0 || world_id >= batch->eng.Wcap) return fail(AVBD_ERR_INVALID, "bad world id");
     return batch->eng.clearWorld(world_id);
+}
+
+avbd_status avbd_world_use_graph(avbd_world *world, int32_t enable)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    world->eng.useGraph = enable != 0;
+    return AVBD_OK;
 }
 
 avbd_status avbd_batch_use_graph(avbd_batch *batch, int32_t enable)

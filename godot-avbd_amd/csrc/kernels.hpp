@@ -36,6 +36,12 @@ __global__ void k_fill_i32(int *p, int v, int n)
     const int i = GID;
     if (i < n) p[i] = v;
 }
+// A phase boundary of a timed step: the device's constant-rate clock at this point of the stream.
+__global__ void k_phase_mark(uint64_t *ticks, int k)
+{
+    if (threadIdx.x == 0) ticks[k] = wall_clock64();
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, int n)
 {
     const int i = GID;
@@ -1787,18 +1793,36 @@ __device__ __forceinline__ void queueFallback(const State &S, int body, const Bl
     storeBlock(S.fbBlock + 33 * k, blk);
 }
 
-// solver_step.cpp:209-229: one colour of the Gauss-Seidel sweep.
+// Sort keys of the colour runs: a body's colour (0..9), 255 outside the update set.
+__global__ void k_colour_keys(State S)
+{
+    const int i = GID;
+    if (i >= S.numBodies) return;
+    const bool in = S.bodyWorld[i] >= 0 && S.updateIndex[i] >= 0;
+    S.colKey[i] = in ? static_cast<unsigned long long>(S.colour[i]) : 255ull;
+    S.colVal[i] = i;
+}
+// Run starts from the pass's exclusive digit offsets (digit d of tile 0 = keys below d).
+__global__ void k_colour_starts(State S, const uint32_t *offs, int nb)
+{
+    const int d = threadIdx.x;
+    if (d < 16) S.colStart[d] = offs[d * nb];
+}
+
+// solver_step.cpp:209-229: one colour of the Gauss-Seidel sweep, over that colour's run.
+// The pivoted solve (rare) runs in place: the next colour reads this one's poses.
 __global__ void __launch_bounds__(TPB) k_colour_pass(State S, int colour)
 {
-    const int i = XGID;
-    if (i >= S.numBodies || S.colour[i] != colour || S.bodyWorld[i] < 0) return;
-    if (S.updateIndex[i] < 0) return;
-    Block blk;
-    assemble(S, i, blk);
-    v3 dl, da;
-    if (!solveDeltaFast(blk, dl, da)) { queueFallback(S, i, blk); return; }
-    S.pos[i] = st3(bPos(S, i) + dl);
-    S.rot[i] = stq(qadd(bRot(S, i), da));
+    const uint32_t begin = S.colStart[colour], end = S.colStart[colour + 1];
+    for (uint32_t r = begin + GID; r < end; r += gridDim.x * blockDim.x) {
+        const int i = S.colValAlt[r];
+        Block blk;
+        assemble(S, i, blk);
+        v3 dl, da;
+        if (!solveDeltaFast(blk, dl, da)) solveDeltaSlow(blk, dl, da);
+        S.pos[i] = st3(bPos(S, i) + dl);
+        S.rot[i] = stq(qadd(bRot(S, i), da));
+    }
 }
 
 // solver_step.cpp:248-264: overflow bodies solve against one pose snapshot.  APPLY (worlds

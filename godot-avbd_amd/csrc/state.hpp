@@ -170,6 +170,11 @@ struct State {
     uint32_t *taskFlag, *taskScan;   // refit task roots (see k_task_flags), per node
     int *taskNode;
     int *colourCnt, *colourQueue;    // colour finisher: uncoloured lower neighbours, ready queue
+    // bodies sorted by colour (stable in body order; key 255 = outside the update set) and the
+    // start of each colour's run: run c is colVal[colStart[c] .. colStart[c + 1])
+    unsigned long long *colKey, *colKeyAlt;
+    int *colVal, *colValAlt;
+    uint32_t *colStart;
     int *leafBody;               // sorted position -> body
     int *bodyLeaf;               // body -> sorted position
     uint32_t *pairCount, *pairOff;

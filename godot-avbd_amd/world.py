@@ -62,6 +62,10 @@ class World:
         for _ in range(n):
             abi.check(self.lib.avbd_world_step(self.h))
 
+    def use_graph(self, enable):
+        """Step as one captured HIP graph (default) or kernel by kernel (for tracing tools)."""
+        abi.check(self.lib.avbd_world_use_graph(self.h, 1 if enable else 0))
+
     def step_interleaved(self, substeps, rounds):
         abi.check(self.lib.avbd_world_step_interleaved(self.h, int(substeps), int(rounds)))
 

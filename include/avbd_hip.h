@@ -61,7 +61,8 @@ typedef struct avbd_params {
     float stiffness_decay;           /* 1 */
     int32_t adaptive_beta;           /* 0 */
     float adaptive_beta_error_floor; /* 1e-3 */
-    float convergence_threshold;     /* 0 (only 0 is supported) */
+    float convergence_threshold;     /* 0; > 0 ends each substep's rounds as the reference's
+                                        probe does (solver_step.cpp:299-314) */
     float jacobi_relaxation;         /* 0.7 */
     int32_t contact_only_overflow;   /* 1 */
     float stick_drift_heal;          /* 0.002 */
@@ -453,6 +454,9 @@ avbd_status avbd_batch_fits(avbd_batch *batch, int32_t n_bodies, int32_t n_joint
 /* Launch batch steps as one captured HIP graph (enable = 1, the default) or as the same
  * kernels launched one by one on the batch's stream (enable = 0; for tracing tools). */
 avbd_status avbd_batch_use_graph(avbd_batch *batch, int32_t enable);
+/* The same choice for a plain world: its step is one captured HIP graph per (substeps, rounds,
+ * tree rebuild) by default; enable = 0 launches the kernels one by one. */
+avbd_status avbd_world_use_graph(avbd_world *world, int32_t enable);
 /* One step of every world in the batch.  AVBD_ERR_WORLD: every world stepped except the ones
  * avbd_batch_failed_worlds lists, whose state this step is not valid (remove them; until then
  * they keep failing). */

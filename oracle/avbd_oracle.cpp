@@ -47,6 +47,7 @@ namespace {
 //   15 fracture tests the linear lambda       16 fracture threshold not squared
 //   17 projection cap ignored                 18 projection sums not averaged
 //   19 contact-only bodies always to Jacobi   20 speculative margin not extended by motion
+//   21 convergence probe compares the wrong way
 int g_plant = 0;
 
 constexpr float PENALTY_MIN = 1.0f;
@@ -1935,11 +1936,26 @@ struct World {
         return blk;
     }
 
+    // manifold.cpp:672-692: the convergence probe's sum over one manifold's contacts (normal
+    // rows with penetration; speculative gaps skipped)
+    void accumulatePenetration(uint32_t s, float &total, int &count) const
+    {
+        const bool speculative = P.speculative_margin > 0.0f;
+        const int first = static_cast<int>(s) * MAXC;
+        for (int i = 0; i < numC[s]; i++) {
+            if (speculative && C0[first + i].x > 0.0f) continue;
+            const float penetration = -C0[first + i].x;
+            if (penetration > 0.0f) { total += penetration; count++; }
+        }
+    }
+
     void iterate(int target)
     {
         const float rounds = P.newton_ratio * static_cast<float>(target);
         const int newtonRounds = std::clamp(
                 static_cast<int>(g_plant == 9 ? std::floor(rounds) : std::ceil(rounds)), 1, target);
+        float prevResidual = 0.0f;   // solver_step.cpp:88-89, per substep
+        bool converged = false;
         for (int it = 0; it < target; it++) {
             const bool newton = it < newtonRounds;
             for (int c = 0; newton && c < colourCount; c++) {
@@ -1974,6 +1990,18 @@ struct World {
                 if (decay < 1.0f && g_plant == 10) decayPenalty(s, decay);
             }
             for (Joint &j : joints) jointDual(j);
+            // solver_step.cpp:299-314: after a round (not the last), the mean penetration over
+            // the live manifolds against the previous round's; a change below the threshold
+            // (or a worsening) ends the iterations
+            if (P.convergence_threshold > 0.0f && it + 1 < target) {
+                float total = 0.0f;
+                int count = 0;
+                for (uint32_t s : live) accumulatePenetration(s, total, count);
+                const float residual = count > 0 ? total / static_cast<float>(count) : 0.0f;
+                converged = (g_plant == 21 ? residual - prevResidual : prevResidual - residual) < P.convergence_threshold;
+                prevResidual = residual;
+            }
+            if (converged) break;
         }
     }
 
@@ -2155,8 +2183,7 @@ struct World {
 int checkParams(const avbd_params *p)
 {
     if (!p) return AVBD_ERR_INVALID;
-    if (p->jacobian_rebuild_distance > 0.0f ||
-        p->convergence_threshold > 0.0f)
+    if (p->jacobian_rebuild_distance > 0.0f)
         return AVBD_ERR_UNSUPPORTED;
     return AVBD_OK;
 }

@@ -65,8 +65,7 @@ def test_state_digest_matches_python():
 def test_unsupported_and_invalid_arguments():
     lib = abi.load_library()
     h = C.c_void_p()
-    for bad in (dict(jacobian_rebuild_distance=1e-4),
-                dict(convergence_threshold=1e-5)):
+    for bad in (dict(jacobian_rebuild_distance=1e-4),):
         p = pkg.default_params(**bad)
         assert lib.avbd_world_create(C.byref(p), 10, 0, 0, C.byref(h)) == abi.AVBD_ERR_UNSUPPORTED
         assert lib.avbd_last_error()

@@ -136,3 +136,31 @@ def test_weld_fracture(fracture):
     w, o = _lockstep(*s.build(), P(iterations=10), 240)
     broken = len(o.download(joints=True)[1]["broken"]) == 0
     assert broken == (fracture == 0.5)
+
+
+@pytest.mark.parametrize("inter", [None, (2, 3)])
+def test_convergence_threshold(inter):
+    b, j = scenes.pile(150, seed=9, pitch=2.0)
+    p = P(iterations=10, convergence_threshold=1e-5)
+    if inter:
+        p.dt = float(F(1.0) / F(120.0))
+    _lockstep(b, j, p, 40, inter=inter)
+
+
+def test_convergence_threshold_in_a_batch():
+    p = P(iterations=10, convergence_threshold=1e-5)
+    worlds = [scenes.pile(60 + 40 * k, seed=30 + k, ground=60.0) for k in range(4)]
+    batch = pkg.Batch(p, 4, sum(len(w[0]["mass"]) for w in worlds), 0)
+    ids = [batch.add(bb, jj) for bb, jj in worlds]
+    orc = []
+    for bb, jj in worlds:
+        o = oracle.OracleWorld(p)
+        o.upload(bb, jj)
+        orc.append(o)
+    for _ in range(30):
+        batch.step()
+        for o in orc:
+            o.step()
+    for wid, o in zip(ids, orc):
+        assert not state_mismatch(batch.download(wid), o.download())
+    batch.close()

@@ -463,6 +463,23 @@ def pin_stiffness_decay():
     assert checked > 100
 
 
+def pin_convergence_probe():
+    """solver_step.cpp:88-89, 299-314: the probe starts each substep from a previous residual of
+    0 and reads the step-start C0 (manifold.cpp:672-692), so the mean penetration it sees is the
+    same every round and never negative: with any positive threshold the first round ends the
+    iterations.  A penetrating pile stepped with 10 rounds and threshold 1e-6 (and 1e3) equals, bit
+    for bit, the same step with 1 round; with threshold 0 it does not (the other rounds matter)."""
+    state = _pile_state()
+    one, m1 = _one_step(state, iterations=1)
+    assert float(np.max(-m1["c0"].reshape(-1, 8, 3)[:, :, 0])) > 1e-4      # it does penetrate
+    for thr in (1e-6, 1e3):
+        got, mg = _one_step(state, iterations=10, convergence_threshold=thr)
+        assert all(bits_equal(got[k], one[k]) for k in ("position", "rotation", "velocity", "angular_velocity"))
+        assert bits_equal(mg["lambda_"], m1["lambda_"]) and bits_equal(mg["penalty"], m1["penalty"])
+    ten, _ = _one_step(state, iterations=10)
+    assert not bits_equal(ten["position"], one["position"])
+
+
 # ------------------------------------------------------------------- hard joints ----
 
 def _anchor_error(st, r_anchor, body, r_b):
@@ -566,7 +583,7 @@ PLANTS = {
     12: ["pin_hard_joint_force_is_the_weight"], 13: ["pin_hard_joint_error_decays_by_alpha"],
     14: ["pin_joint_penalty_cap"], 15: ["pin_fracture"], 16: ["pin_fracture"],
     17: ["pin_projection_shared_plane"], 18: ["pin_projection_shared_plane"],
-    19: ["pin_contact_only_overflow_off"], 20: ["pin_speculative_horizon"],
+    19: ["pin_contact_only_overflow_off"], 20: ["pin_speculative_horizon"], 21: ["pin_convergence_probe"],
 }
 
 
