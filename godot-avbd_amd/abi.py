@@ -260,6 +260,11 @@ def load_library(path=LIB_PATH):
         "avbd_world_raycast": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, C.c_int32, C.POINTER(RayHits)]),
         "avbd_world_raycast_noalloc": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, C.c_int32, C.POINTER(RayHits)]),
         "avbd_world_collide_pair": (C.c_int32, [vp, C.c_int32, C.c_int32, C.c_float, i32p, i32p, f32p, f32p, f32p]),
+        "avbd_world_collide_shape": (C.c_int32, [vp, C.c_int32, i32p, f32p, f32p, f32p, f32p, u32p, C.c_int32, i32p,
+                                                 u8p, f32p]),
+        "avbd_world_rest_info": (C.c_int32, [vp, C.c_int32, i32p, f32p, f32p, f32p, f32p, u32p, i32p, f32p, f32p,
+                                             f32p]),
+        "avbd_world_closest_point": (C.c_int32, [vp, C.c_int32, i32p, f32p, f32p]),
         "avbd_batch_raycast": (C.c_int32, [vp, C.c_int32, C.c_int32, f32p, f32p, u32p, C.c_int32,
                                            C.POINTER(RayHits)]),
         "avbd_world_pick": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, i32p, f32p]),

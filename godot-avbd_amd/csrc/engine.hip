@@ -399,8 +399,122 @@ struct Engine {
         return AVBD_OK;
     }
 
+    float4 *qPts = nullptr;   // collide_shape point pairs
+    size_t qPtsCap = 0;
     float4 *qShapes = nullptr, *qCRA = nullptr, *qCRB = nullptr, *qCMid = nullptr, *qSpA = nullptr, *qSpB = nullptr;
     size_t qShapeCap = 0;
+
+    // Query shapes to the device (layout of k_intersect_shape), bounds of world w's bodies too.
+    avbd_status uploadShapes(int w, int n, const int32_t *type, const float *center, const float *rot,
+                             const float *size, const float *margin, const uint32_t *masks)
+    {
+        for (int r = 0; r < n; r++)
+            if (type[r] < AVBD_SHAPE_BOX || type[r] > AVBD_SHAPE_CYLINDER) return fail(AVBD_ERR_INVALID, "unknown shape");
+        avbd_status st = ensureQuery(n);
+        if (st != AVBD_OK || (st = ensureShapeQuery(n)) != AVBD_OK) return st;
+        std::vector<float4> in(4 * static_cast<size_t>(n));
+        for (int r = 0; r < n; r++) {
+            float tf, mf;
+            const int32_t t = type[r];
+            const uint32_t m = masks ? masks[r] : 0xFFFFFFFFu;
+            std::memcpy(&tf, &t, sizeof(tf));
+            std::memcpy(&mf, &m, sizeof(mf));
+            in[4 * r] = make_float4(center[3 * r], center[3 * r + 1], center[3 * r + 2], tf);
+            in[4 * r + 1] = make_float4(rot[4 * r], rot[4 * r + 1], rot[4 * r + 2], rot[4 * r + 3]);
+            in[4 * r + 2] = make_float4(size[3 * r], size[3 * r + 1], size[3 * r + 2], margin[r]);
+            in[4 * r + 3] = make_float4(mf, 0.0f, 0.0f, 0.0f);
+        }
+        HIPCHK(hipMemcpyAsync(qShapes, in.data(), sizeof(float4) * in.size(), hipMemcpyHostToDevice, stream));
+        const WorldInfo &wi = worlds[w];
+        if (wi.bodyCount > 0) k_query_bounds<<<nblk(wi.bodyCount), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, qBounds);
+        HIPCHK(hipStreamSynchronize(stream));   // `in` is host memory the copy reads
+        return AVBD_OK;
+    }
+
+    avbd_status collideShape(int w, int n, const int32_t *type, const float *center, const float *rot,
+                             const float *size, const float *margin, const uint32_t *masks, int maxPairs,
+                             int32_t *count, uint8_t *any, float *points)
+    {
+        const WorldInfo &wi = worlds[w];
+        if (!wi.active) return fail(AVBD_ERR_INVALID, "world is not active");
+        if (n < 0 || maxPairs < 0 || (n > 0 && (!type || !center || !rot || !size || !margin || !count || !any ||
+                                                (maxPairs > 0 && !points))))
+            return fail(AVBD_ERR_INVALID, "bad arguments");
+        if (n == 0) return AVBD_OK;
+        avbd_status st = uploadShapes(w, n, type, center, rot, size, margin, masks);
+        if (st != AVBD_OK) return st;
+        const size_t outN = 2 * static_cast<size_t>(n) * static_cast<size_t>(std::max(maxPairs, 1));
+        if (outN > qPtsCap) {
+            qPtsCap = std::max(outN, 2 * qPtsCap);
+            if ((st = regrow(qPts, qPtsCap)) != AVBD_OK) return st;
+        }
+        k_collide_shape<<<nblk(n, NARROW_TPB), NARROW_TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qShapes, qBounds,
+                                                                        maxPairs, qBody, qFlag, qPts, qCRA, qCRB, qCMid,
+                                                                        qSpA, qSpB);
+        HIPCHK(hipGetLastError());
+        std::vector<float4> pts(static_cast<size_t>(n) * 2 * maxPairs);
+        HIPCHK(hipMemcpyAsync(count, qBody, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(any, qFlag, n, hipMemcpyDeviceToHost, stream));
+        if (maxPairs > 0)
+            HIPCHK(hipMemcpyAsync(pts.data(), qPts, sizeof(float4) * pts.size(), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (size_t k = 0; k < pts.size(); k++) {
+            const size_t r = k / (2 * static_cast<size_t>(maxPairs)), p = (k % (2 * static_cast<size_t>(maxPairs))) / 2;
+            if (static_cast<int>(p) >= count[r]) continue;
+            points[3 * k] = pts[k].x; points[3 * k + 1] = pts[k].y; points[3 * k + 2] = pts[k].z;
+        }
+        return AVBD_OK;
+    }
+
+    avbd_status restInfo(int w, int n, const int32_t *type, const float *center, const float *rot, const float *size,
+                         const float *margin, const uint32_t *masks, int32_t *body, float *point, float *normal,
+                         float *depth)
+    {
+        const WorldInfo &wi = worlds[w];
+        if (!wi.active) return fail(AVBD_ERR_INVALID, "world is not active");
+        if (n < 0 || (n > 0 && (!type || !center || !rot || !size || !margin || !body)))
+            return fail(AVBD_ERR_INVALID, "bad arguments");
+        if (n == 0) return AVBD_OK;
+        avbd_status st = uploadShapes(w, n, type, center, rot, size, margin, masks);
+        if (st != AVBD_OK) return st;
+        k_rest_info<<<nblk(n, NARROW_TPB), NARROW_TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qShapes, qBounds, qBody,
+                                                                    qA, qB, qCRA, qCRB, qCMid, qSpA, qSpB);
+        HIPCHK(hipGetLastError());
+        std::vector<float4> a(n), b(n);
+        HIPCHK(hipMemcpyAsync(body, qBody, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(a.data(), qA, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(b.data(), qB, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int r = 0; r < n; r++) {
+            if (point) { point[3 * r] = a[r].x; point[3 * r + 1] = a[r].y; point[3 * r + 2] = a[r].z; }
+            if (normal) { normal[3 * r] = b[r].x; normal[3 * r + 1] = b[r].y; normal[3 * r + 2] = b[r].z; }
+            if (depth) depth[r] = a[r].w;
+        }
+        return AVBD_OK;
+    }
+
+    avbd_status closestPointQuery(int w, int n, const int32_t *idx, const float *points, float *out)
+    {
+        const WorldInfo &wi = worlds[w];
+        if (!wi.active) return fail(AVBD_ERR_INVALID, "world is not active");
+        if (n < 0 || (n > 0 && (!idx || !points || !out))) return fail(AVBD_ERR_INVALID, "bad arguments");
+        for (int r = 0; r < n; r++)
+            if (idx[r] < 0 || idx[r] >= wi.bodyCount) return fail(AVBD_ERR_INVALID, "body index out of range");
+        if (n == 0) return AVBD_OK;
+        avbd_status st = ensureQuery(n);
+        if (st != AVBD_OK) return st;
+        std::vector<float4> in(n);
+        for (int r = 0; r < n; r++) in[r] = make_float4(points[3 * r], points[3 * r + 1], points[3 * r + 2], 0.0f);
+        HIPCHK(hipMemcpyAsync(qBody, idx, sizeof(int) * n, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(qA, in.data(), sizeof(float4) * n, hipMemcpyHostToDevice, stream));
+        k_closest_point<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, n, qBody, qA, qB);
+        HIPCHK(hipGetLastError());
+        std::vector<float4> res(n);
+        HIPCHK(hipMemcpyAsync(res.data(), qB, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int r = 0; r < n; r++) { out[3 * r] = res[r].x; out[3 * r + 1] = res[r].y; out[3 * r + 2] = res[r].z; }
+        return AVBD_OK;
+    }
 
     avbd_status intersectShape(int w, int n, const int32_t *type, const float *center, const float *rot,
                                const float *size, const float *margin, const uint32_t *masks, int maxRes,
@@ -2382,6 +2496,30 @@ avbd_status avbd_world_raycast(avbd_world *world, int32_t n, const float *from, 
     if (!world || !out) return fail(AVBD_ERR_INVALID, "null argument");
     return world->eng.raycast(0, n, from, to, masks, hit_from_inside != 0, out->body, out->t, out->point, out->normal,
                               out->inside);
+}
+
+avbd_status avbd_world_collide_shape(avbd_world *world, int32_t n, const int32_t *type, const float *center,
+                                     const float *rotation, const float *size, const float *margin,
+                                     const uint32_t *masks, int32_t max_pairs, int32_t *count, uint8_t *any,
+                                     float *points)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    return world->eng.collideShape(0, n, type, center, rotation, size, margin, masks, max_pairs, count, any, points);
+}
+
+avbd_status avbd_world_rest_info(avbd_world *world, int32_t n, const int32_t *type, const float *center,
+                                 const float *rotation, const float *size, const float *margin, const uint32_t *masks,
+                                 int32_t *body, float *point, float *normal, float *depth)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    return world->eng.restInfo(0, n, type, center, rotation, size, margin, masks, body, point, normal, depth);
+}
+
+avbd_status avbd_world_closest_point(avbd_world *world, int32_t n, const int32_t *body, const float *points,
+                                     float *out)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    return world->eng.closestPointQuery(0, n, body, points, out);
 }
 
 avbd_status avbd_world_collide_pair(avbd_world *world, int32_t body_a, int32_t body_b, float collision_dt,

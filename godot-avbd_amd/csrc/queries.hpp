@@ -367,6 +367,205 @@ __global__ void __launch_bounds__(NARROW_TPB) k_collide_pair(State S, int a, int
     }
 }
 
+// query_shape.cpp:340-369: a shape's extent along an axis, and the overlap of two shapes along
+// a contact normal (type, rotation, size as a body's: box extents, sphere (r, -, -), cylinder
+// (r, -, height) about local +Z)
+AD float supportRadius(int type, qt q, v3 size, v3 axis)
+{
+    const v3 n = normalize(axis);
+    if (type == AVBD_SHAPE_SPHERE) return size.x;
+    if (type == AVBD_SHAPE_CYLINDER) {
+        const float axial = fabsf(dot(qrot(q, v3{0, 0, 1}), n));
+        const float radial = sqrtf(smax(0.0f, 1.0f - axial * axial));
+        return size.x * radial + (size.z * 0.5f) * axial;
+    }
+    const m3 R = qmat(q);
+    const v3 h = size * 0.5f;
+    return (h.x * fabsf(dot(row(R, 0), n)) + h.y * fabsf(dot(row(R, 1), n))) + h.z * fabsf(dot(row(R, 2), n));
+}
+AD float penetrationDepth(int ta, v3 ca, qt qa, v3 sa, int tb, v3 cb, qt qb, v3 sb, v3 normal)
+{
+    const float len = length(normal);
+    if (len <= 1.0e-6f) return 0.0f;
+    const v3 n = normal * (1.0f / len);
+    return (supportRadius(ta, qa, sa, n) + supportRadius(tb, qb, sb, n)) - fabsf(dot(cb - ca, n));
+}
+
+// One query shape against one candidate body: the narrow phase, query first (its contacts in
+// the query's staging rows of lane l), returning the contact count and the basis.
+AD int queryContacts(const State &S, int type, v3 qc, qt qq, float4 s4, float m, int i, NarrowPolys &poly, int l,
+                     size_t d, float4 *cra, float4 *crb, float4 *cmid, float4 *spA, float4 *spB, m3 &B)
+{
+    const int bt = S.shape[i];
+    const v3 bp = ld3(S.pos[i]);
+    const qt bq = ldq(S.rot[i]);
+    const float4 bs = S.sizeR[i];
+    Contacts cs{0, cra + d, crb + d, cmid + d};
+    bool wrote;
+    if (type != AVBD_SHAPE_BOX || bt != AVBD_SHAPE_BOX)
+        return collideRound(type, bt, makeRShp(qc, qq, make_float4(s4.x, s4.y, s4.z, 0.0f)), makeRShp(bp, bq, bs),
+                            S.rim, m, B, wrote, cs);
+    const Poly c0{&poly[0][0][l], &poly[0][1][l], &poly[0][2][l], spA + d};
+    const Poly c1{&poly[1][0][l], &poly[1][1][l], &poly[1][2][l], spB + d};
+    return collideBoxes(qc, qq, v3{s4.x, s4.y, s4.z} * 0.5f, bp, bq, v3{bs.x, bs.y, bs.z} * 0.5f, m, B, wrote, cs, c0, c1);
+}
+
+// The query's candidate bounds: its tight bounds grown by the margin (query::shapeAabb).
+AD void queryBox(int type, qt qq, v3 qc, float4 s4, float m, v3 &mn, v3 &mx)
+{
+    const v3 h = type == AVBD_SHAPE_SPHERE     ? v3{s4.x, s4.x, s4.x}
+                 : type == AVBD_SHAPE_CYLINDER ? v3{s4.x, s4.x, s4.z * 0.5f}
+                                               : v3{s4.x, s4.y, s4.z} * 0.5f;
+    const m3 R = qmat(qq);
+    const v3 wx{(fabsf(R.c[0][0]) * h.x + fabsf(R.c[1][0]) * h.y) + fabsf(R.c[2][0]) * h.z,
+                (fabsf(R.c[0][1]) * h.x + fabsf(R.c[1][1]) * h.y) + fabsf(R.c[2][1]) * h.z,
+                (fabsf(R.c[0][2]) * h.x + fabsf(R.c[1][2]) * h.y) + fabsf(R.c[2][2]) * h.z};
+    mn = qc - wx;
+    mx = qc + wx;
+    if (m > 0.0f) { mn = mn - v3{m, m, m}; mx = mx + v3{m, m, m}; }
+}
+
+// The bodies part of the server's collide_shape (avbd_direct_space_state3d.cpp:386-447): query
+// r (laid out as for k_intersect_shape) against its candidates in list order; every contact as
+// a pair of world points (on the query, on the body) until maxPairs pairs are out:
+// pts[2 (r maxPairs + p)] and [.. + 1]; cnt[r] pairs, anyHit[r] = some candidate touched.
+__global__ void __launch_bounds__(NARROW_TPB) k_collide_shape(State S, int base, int count, int n, const float4 *qin,
+                                                             const float4 *bounds, int maxPairs, int *cnt,
+                                                             uint8_t *anyHit, float4 *pts, float4 *cra, float4 *crb,
+                                                             float4 *cmid, float4 *spA, float4 *spB)
+{
+    __shared__ NarrowPolys poly;
+    const int r = GID;
+    if (r >= n) return;
+    const float4 c4 = qin[4 * r], r4 = qin[4 * r + 1], s4 = qin[4 * r + 2], m4 = qin[4 * r + 3];
+    const int type = __float_as_int(c4.w);
+    const v3 qc = ld3(c4);
+    const qt qq = ldq(r4);
+    const float m = smax(0.0f, s4.w);
+    const uint32_t mask = __float_as_uint(m4.x);
+    v3 mn, mx;
+    queryBox(type, qq, qc, s4, m, mn, mx);
+    const size_t d = stageRow(r, 0);
+    int pairs = 0;
+    bool any = false;
+    for (int j = 0; j < count; j++) {
+        const float4 lo = bounds[2 * j], hi = bounds[2 * j + 1];
+        if ((__float_as_uint(lo.w) & mask) == 0) continue;
+        if (!boxesOverlap(ld3(lo), ld3(hi), mn, mx)) continue;
+        const int i = base + j;
+        m3 B;
+        const int hits = queryContacts(S, type, qc, qq, s4, m, i, poly, threadIdx.x, d, cra, crb, cmid, spA, spB, B);
+        if (hits <= 0) continue;
+        any = true;
+        const v3 bp = ld3(S.pos[i]);
+        const qt bq = ldq(S.rot[i]);
+        for (int k = 0; k < hits && pairs < maxPairs; k++) {
+            const v3 wa = qc + qrot(qq, ld3(cra[d + crow(k)]));
+            const v3 wb = bp + qrot(bq, ld3(crb[d + crow(k)]));
+            const size_t o = 2 * (static_cast<size_t>(r) * maxPairs + pairs);
+            pts[o] = st3(wa);
+            pts[o + 1] = st3(wb);
+            pairs++;
+        }
+        if (pairs >= maxPairs) break;
+    }
+    cnt[r] = pairs;
+    anyHit[r] = any ? 1 : 0;
+}
+
+// The bodies part of the server's rest_info (avbd_direct_space_state3d.cpp:449-513): over the
+// candidates in list order and their contacts, the deepest by penetrationDepth along the
+// contact normal (a later one must be strictly deeper).  body[r] (-1: none); hitA[r] = (point
+// on the query, depth); hitB[r] = (normal = the basis' first row, 0).
+__global__ void __launch_bounds__(NARROW_TPB) k_rest_info(State S, int base, int count, int n, const float4 *qin,
+                                                         const float4 *bounds, int *body, float4 *hitA, float4 *hitB,
+                                                         float4 *cra, float4 *crb, float4 *cmid, float4 *spA,
+                                                         float4 *spB)
+{
+    __shared__ NarrowPolys poly;
+    const int r = GID;
+    if (r >= n) return;
+    const float4 c4 = qin[4 * r], r4 = qin[4 * r + 1], s4 = qin[4 * r + 2], m4 = qin[4 * r + 3];
+    const int type = __float_as_int(c4.w);
+    const v3 qc = ld3(c4);
+    const qt qq = ldq(r4);
+    const float m = smax(0.0f, s4.w);
+    const uint32_t mask = __float_as_uint(m4.x);
+    v3 mn, mx;
+    queryBox(type, qq, qc, s4, m, mn, mx);
+    const size_t d = stageRow(r, 0);
+    float best = -1.0f;
+    v3 bestPoint{0, 0, 0}, bestNormal{0, 0, 0};
+    int bestBody = -1;
+    for (int j = 0; j < count; j++) {
+        const float4 lo = bounds[2 * j], hi = bounds[2 * j + 1];
+        if ((__float_as_uint(lo.w) & mask) == 0) continue;
+        if (!boxesOverlap(ld3(lo), ld3(hi), mn, mx)) continue;
+        const int i = base + j;
+        m3 B;
+        const int hits = queryContacts(S, type, qc, qq, s4, m, i, poly, threadIdx.x, d, cra, crb, cmid, spA, spB, B);
+        if (hits <= 0) continue;
+        const v3 nrm = row(B, 0);
+        const float dep = penetrationDepth(type, qc, qq, v3{s4.x, s4.y, s4.z}, S.shape[i], ld3(S.pos[i]), ldq(S.rot[i]),
+                                          ld3(S.sizeR[i]), nrm);
+        for (int k = 0; k < hits; k++) {
+            if (dep <= best) continue;
+            best = dep;
+            bestNormal = nrm;
+            bestPoint = qc + qrot(qq, ld3(cra[d + crow(k)]));
+            bestBody = j;
+        }
+    }
+    body[r] = bestBody;
+    hitA[r] = st3(bestPoint, best);
+    hitB[r] = st3(bestNormal, 0.0f);
+}
+
+// query_shape.cpp:280-323: the closest point of body i's shape at its current pose to p.
+AD v3 closestPoint(int type, v3 c, qt q, float4 size, v3 point)
+{
+    const v3 p = qrot(qconj(q), point - c);
+    v3 local = p;
+    if (type == AVBD_SHAPE_SPHERE) {
+        const float len = length(p);
+        local = len > 1.0e-6f ? p * (size.x / len) : v3{size.x, 0, 0};
+    } else if (type == AVBD_SHAPE_CYLINDER) {
+        const float r = size.x, hh = size.z * 0.5f;
+        const float radial = sqrtf(p.x * p.x + p.y * p.y);
+        const float clampedZ = sclamp(p.z, -hh, hh);
+        if (radial > r) {
+            const float scale = r / radial;
+            local = v3{p.x * scale, p.y * scale, clampedZ};
+        } else if (radial <= 1.0e-6f) {
+            const float capDistance = hh - fabsf(p.z);
+            local = capDistance < r ? v3{0, 0, p.z < 0.0f ? -hh : hh} : v3{r, 0, p.z};
+        } else {
+            const float sideDistance = r - radial;
+            const float capDistance = hh - fabsf(p.z);
+            if (capDistance < sideDistance) {
+                local = v3{p.x, p.y, p.z < 0.0f ? -hh : hh};
+            } else {
+                const float scale = r / radial;
+                local = v3{p.x * scale, p.y * scale, p.z};
+            }
+        }
+    } else {
+        const v3 h = v3{size.x, size.y, size.z} * 0.5f;
+        local = v3{sclamp(p.x, -h.x, h.x), sclamp(p.y, -h.y, h.y), sclamp(p.z, -h.z, h.z)};
+    }
+    return c + qrot(q, local);
+}
+
+// The server's get_closest_point_to_object_volume for bodies: out[r] = the closest point of
+// body (base + idx[r])'s volume to pts[r] (w of pts unused).
+__global__ void k_closest_point(State S, int base, int n, const int *idx, const float4 *pts, float4 *out)
+{
+    const int r = GID;
+    if (r >= n) return;
+    const int i = base + idx[r];
+    out[r] = st3(closestPoint(S.shape[i], ld3(S.pos[i]), ldq(S.rot[i]), S.sizeR[i], ld3(pts[r])));
+}
+
 // ---- shape sweeps (query_shape.cpp:213-258, 402-498; the server's cast_motion) ----
 struct SweepShape {
     int type;

@@ -163,6 +163,47 @@ class World:
                                                   point.ctypes.data_as(abi.f32p), normal.ctypes.data_as(abi.f32p)))
         return {"body": body, "safe": frac[:, 0], "unsafe": frac[:, 1], "point": point, "normal": normal}
 
+    def _shapes(self, types, centers, rotations, sizes, margins, masks):
+        t = _rows(types, 1, "types", np.int32)
+        n = len(t)
+        c, q, s3 = _rows(centers, 3, "centers", n=n), _rows(rotations, 4, "rotations", n=n), _rows(sizes, 3, "sizes", n=n)
+        m = _rows(margins, 1, "margins", n=n)
+        mk = None if masks is None else _rows(masks, 1, "masks", np.uint32, n)
+        keep = (t, c, q, s3, m, mk)
+        return keep, [n, t.ctypes.data_as(abi.i32p), c.ctypes.data_as(abi.f32p), q.ctypes.data_as(abi.f32p),
+                      s3.ctypes.data_as(abi.f32p), m.ctypes.data_as(abi.f32p),
+                      None if mk is None else mk.ctypes.data_as(abi.u32p)]
+
+    def collide_shape(self, types, centers, rotations, sizes, margins, masks=None, max_pairs=8):
+        """The server's collide_shape for n query shapes: (pair count, any, points n x max_pairs x 2 x 3)."""
+        keep, args = self._shapes(types, centers, rotations, sizes, margins, masks)
+        n = args[0]
+        count, anyhit = np.zeros(n, np.int32), np.zeros(n, np.uint8)
+        pts = np.zeros((n, max_pairs, 2, 3), np.float32)
+        abi.check(self.lib.avbd_world_collide_shape(self.h, *args, int(max_pairs), count.ctypes.data_as(abi.i32p),
+                                                    anyhit.ctypes.data_as(abi.u8p), pts.ctypes.data_as(abi.f32p)))
+        return count, anyhit, pts
+
+    def rest_info(self, types, centers, rotations, sizes, margins, masks=None):
+        """The server's rest_info for n query shapes: (body or -1, point, normal, depth)."""
+        keep, args = self._shapes(types, centers, rotations, sizes, margins, masks)
+        n = args[0]
+        body, point, normal, depth = (np.zeros(n, np.int32), np.zeros((n, 3), np.float32),
+                                      np.zeros((n, 3), np.float32), np.zeros(n, np.float32))
+        abi.check(self.lib.avbd_world_rest_info(self.h, *args, body.ctypes.data_as(abi.i32p),
+                                                point.ctypes.data_as(abi.f32p), normal.ctypes.data_as(abi.f32p),
+                                                depth.ctypes.data_as(abi.f32p)))
+        return body, point, normal, depth
+
+    def closest_point(self, bodies, points):
+        """The server's closest point to each body's volume (body list indices, points n x 3)."""
+        b = _rows(bodies, 1, "bodies", np.int32)
+        p = _rows(points, 3, "points", n=len(b))
+        out = np.zeros((len(b), 3), np.float32)
+        abi.check(self.lib.avbd_world_closest_point(self.h, len(b), b.ctypes.data_as(abi.i32p),
+                                                    p.ctypes.data_as(abi.f32p), out.ctypes.data_as(abi.f32p)))
+        return out
+
     def collide_pair(self, a, b, collision_dt):
         """avbd::Manifold::collide for bodies a and b at their current pose on the device:
         (count, keys, rA, rB, basis 3x3 rows; zeros when the pair is beyond the margin)."""

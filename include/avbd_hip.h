@@ -325,6 +325,31 @@ avbd_status avbd_world_raycast(avbd_world *world, int32_t n, const float *from, 
  * count[j] = contacts written.  stride >= every max_reported[j]. */
 avbd_status avbd_world_contact_report(avbd_world *world, const int32_t *max_reported, int32_t stride, int32_t *count,
                                       float *position, float *normal, int32_t *other);
+/* The bodies part of the Godot server's collide_shape (src/server/avbd_direct_space_state3d.cpp:
+ * 386-447): for each of n query shapes (laid out as for avbd_world_intersect_shape; the caller
+ * applies any motion to the centre), the candidates in list order, the narrow phase (query first)
+ * within the margin, and every contact as a pair of world points (on the query shape, on the
+ * body) until max_pairs pairs are out: points[6 (r max_pairs + p) ..] = (query point, body
+ * point).  count[r] = pairs written; any[r] = 1 when some candidate touched.  Areas stay in the
+ * server. */
+avbd_status avbd_world_collide_shape(avbd_world *world, int32_t n, const int32_t *type, const float *center,
+                                     const float *rotation, const float *size, const float *margin,
+                                     const uint32_t *masks, int32_t max_pairs, int32_t *count, uint8_t *any,
+                                     float *points);
+/* The bodies part of the server's rest_info (avbd_direct_space_state3d.cpp:449-513): over the
+ * candidates in list order and their contacts, the deepest by query::penetrationDepth along the
+ * contact normal (src/avbd/query/query_shape.cpp:340-369; a later contact must be strictly
+ * deeper).  body[r] = its list index (-1: none); point = the contact on the query shape, world
+ * space; normal = the basis' first row (from the body towards the query); depth.  Null point,
+ * normal or depth skip. */
+avbd_status avbd_world_rest_info(avbd_world *world, int32_t n, const int32_t *type, const float *center,
+                                 const float *rotation, const float *size, const float *margin, const uint32_t *masks,
+                                 int32_t *body, float *point, float *normal, float *depth);
+/* The server's get_closest_point_to_object_volume for bodies (avbd_direct_space_state3d.cpp:
+ * 515-545 with query::closestPoint, query_shape.cpp:280-323): out[3r..] = the point of body
+ * body[r]'s volume closest to points[3r..] at the body's current pose. */
+avbd_status avbd_world_closest_point(avbd_world *world, int32_t n, const int32_t *body, const float *points,
+                                     float *out);
 /* Replaces `avbd::Manifold::collide(bodyA, bodyB, contacts, basis)` (reference
  * src/avbd/manifold.hpp:61-63, collide.cpp:1100-1120): the solver's narrow phase for bodies
  * body_a (as A) and body_b (list indices) of the world at their current pose, with the

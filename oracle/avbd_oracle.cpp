@@ -2462,6 +2462,66 @@ int oracle_contains_point(int type, const float pos[3], const float rot[4], cons
     return containsPoint(s, f3{point[0], point[1], point[2]}, margin) ? 1 : 0;
 }
 
+// query_shape.cpp:280-323: the closest point of a shape's volume to a point (the point itself
+// inside a box or a cylinder side's slab; the surface otherwise)
+f3 closestPoint(const RShape &s, f3 point)
+{
+    const f3 p = qrot(qconj(s.q), point - s.c);
+    f3 local = p;
+    if (s.type == AVBD_SHAPE_SPHERE) {
+        const float len = length(p);
+        local = len > 1.0e-6f ? p * (s.r / len) : f3{s.r, 0, 0};
+    } else if (s.type == AVBD_SHAPE_CYLINDER) {
+        const float hh = s.hh;
+        const float radial = std::sqrt(p.x * p.x + p.y * p.y);
+        const float clampedZ = sclamp(p.z, -hh, hh);
+        if (radial > s.r) {
+            const float scale = s.r / radial;
+            local = f3{p.x * scale, p.y * scale, clampedZ};
+        } else if (radial <= 1.0e-6f) {
+            const float capDistance = hh - std::fabs(p.z);
+            if (capDistance < s.r) local = f3{0, 0, p.z < 0.0f ? -hh : hh};
+            else local = f3{s.r, 0, p.z};
+        } else {
+            const float sideDistance = s.r - radial;
+            const float capDistance = hh - std::fabs(p.z);
+            if (capDistance < sideDistance) {
+                local = f3{p.x, p.y, p.z < 0.0f ? -hh : hh};
+            } else {
+                const float scale = s.r / radial;
+                local = f3{p.x * scale, p.y * scale, p.z};
+            }
+        }
+    } else {
+        const f3 h = s.half;
+        local = f3{sclamp(p.x, -h.x, h.x), sclamp(p.y, -h.y, h.y), sclamp(p.z, -h.z, h.z)};
+    }
+    return s.c + qrot(s.q, local);
+}
+
+void oracle_closest_point_shape(int type, const float pos[3], const float rot[4], const float size[3],
+                                const float point[3], float out[3])
+{
+    const RShape s = makeRShape(type, f3{pos[0], pos[1], pos[2]}, quat{rot[0], rot[1], rot[2], rot[3]},
+                                f3{size[0], size[1], size[2]});
+    const f3 c = closestPoint(s, f3{point[0], point[1], point[2]});
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+}
+
+// The server's get_closest_point_to_object_volume (avbd_direct_space_state3d.cpp:515-545) for
+// bodies: the closest point of body[r]'s shape at its current pose to points[r].
+int oracle_closest_point(oracle_world *o, int n, const int32_t *body, const float *points, float *out)
+{
+    const World &w = o->w;
+    for (int r = 0; r < n; r++) {
+        if (body[r] < 0 || body[r] >= static_cast<int>(w.bodies.size())) return AVBD_ERR_INVALID;
+        const Body &b = w.bodies[body[r]];
+        const f3 c = closestPoint(makeRShape(b.shape, b.pos, b.q, b.size), load3(points, r));
+        store3(out, r, c);
+    }
+    return AVBD_OK;
+}
+
 // The bodies part of the server's intersect_point (avbd_direct_space_state3d.cpp:225-259) for n
 // points: candidates from collectAABB over the point's box padded by 1e-4, in list order; the
 // bodies whose shape contains the point, up to maxResults per point.
@@ -2521,6 +2581,112 @@ int oracle_intersect_shape(oracle_world *o, int n, const int32_t *type, const fl
             if (oracle_collide_shapes(q.shape, qp, qq, qs, b.shape, bp, bq, bs, m, bas, keys, ra, rb) <= 0) continue;
             out[static_cast<size_t>(r) * maxResults + count[r]++] = cand[k];
         }
+    }
+    return AVBD_OK;
+}
+
+namespace {
+f3 shapeCentre(const float *center, int r) { return f3{center[3 * r], center[3 * r + 1], center[3 * r + 2]}; }
+}  // namespace
+
+// Candidates of query shape r exactly as intersect_shape takes them.
+static int queryCandidates(oracle_world *o, int r, const int32_t *type, const float *center, const float *rot,
+                           const float *size, float m, const uint32_t *masks, std::vector<int32_t> &cand, Body &q)
+{
+    q = Body{};
+    q.shape = static_cast<uint8_t>(type[r]);
+    q.pos = shapeCentre(center, r);
+    q.q = quat{rot[4 * r], rot[4 * r + 1], rot[4 * r + 2], rot[4 * r + 3]};
+    q.size = load3(size, r);
+    const f3 wx = World::tightHalf(q);
+    f3 lo = q.pos - wx, hi = q.pos + wx;
+    if (m > 0.0f) { lo = lo - f3{m, m, m}; hi = hi + f3{m, m, m}; }
+    const float mn[3] = {lo.x, lo.y, lo.z}, mx[3] = {hi.x, hi.y, hi.z};
+    return oracle_collect_aabb(o, mn, mx, masks ? masks[r] : 0xFFFFFFFFu, cand.data(), static_cast<int>(cand.size()));
+}
+
+// The bodies part of the server's collide_shape (avbd_direct_space_state3d.cpp:386-447): for
+// each query shape, its candidates in list order (as intersect_shape), the narrow phase query
+// first within the margin, and every contact as a pair of world points (on the query, on the
+// body) until maxPairs pairs are out.  any[r]: some candidate gave contacts.
+int oracle_collide_shape(oracle_world *o, int n, const int32_t *type, const float *center, const float *rot,
+                         const float *size, const float *margin, const uint32_t *masks, int maxPairs, int32_t *count,
+                         uint8_t *any, float *points)
+{
+    World &w = o->w;
+    std::vector<int32_t> cand(w.bodies.size() + 1);
+    for (int r = 0; r < n; r++) {
+        count[r] = 0;
+        any[r] = 0;
+        const float m = smax(0.0f, margin[r]);
+        Body q;
+        const int nc = queryCandidates(o, r, type, center, rot, size, m, masks, cand, q);
+        const float qp[3] = {q.pos.x, q.pos.y, q.pos.z}, qq[4] = {q.q.x, q.q.y, q.q.z, q.q.w},
+                    qs[3] = {q.size.x, q.size.y, q.size.z};
+        int pairs = 0;
+        for (int k = 0; k < nc; k++) {
+            const Body &b = w.bodies[cand[k]];
+            const float bp[3] = {b.pos.x, b.pos.y, b.pos.z}, bq[4] = {b.q.x, b.q.y, b.q.z, b.q.w},
+                        bs[3] = {b.size.x, b.size.y, b.size.z};
+            float bas[9]; int32_t keys[8]; float ra[24], rb[24];
+            const int cnt = oracle_collide_shapes(q.shape, qp, qq, qs, b.shape, bp, bq, bs, m, bas, keys, ra, rb);
+            if (cnt <= 0) continue;
+            any[r] = 1;
+            for (int i = 0; i < cnt && pairs < maxPairs; i++) {
+                const f3 wa = q.pos + qrot(q.q, load3(ra, i));
+                const f3 wb = b.pos + qrot(b.q, load3(rb, i));
+                float *pt = points + (static_cast<size_t>(r) * maxPairs + pairs) * 6;
+                pt[0] = wa.x; pt[1] = wa.y; pt[2] = wa.z; pt[3] = wb.x; pt[4] = wb.y; pt[5] = wb.z;
+                pairs++;
+            }
+            if (pairs >= maxPairs) break;
+        }
+        count[r] = pairs;
+    }
+    return AVBD_OK;
+}
+
+static float queryPenetration(const Body &a, const Body &b, f3 normal);
+
+// The bodies part of the server's rest_info (avbd_direct_space_state3d.cpp:449-513): over the
+// candidates in list order and their contacts, the deepest by query::penetrationDepth along the
+// contact normal (a later contact must be strictly deeper); body[r] = -1 when none.  point =
+// the contact on the query shape in world space, normal = the basis' first row.
+int oracle_rest_info(oracle_world *o, int n, const int32_t *type, const float *center, const float *rot,
+                     const float *size, const float *margin, const uint32_t *masks, int32_t *body, float *point,
+                     float *normal, float *depth)
+{
+    World &w = o->w;
+    std::vector<int32_t> cand(w.bodies.size() + 1);
+    for (int r = 0; r < n; r++) {
+        const float m = smax(0.0f, margin[r]);
+        Body q;
+        const int nc = queryCandidates(o, r, type, center, rot, size, m, masks, cand, q);
+        const float qp[3] = {q.pos.x, q.pos.y, q.pos.z}, qq[4] = {q.q.x, q.q.y, q.q.z, q.q.w},
+                    qs[3] = {q.size.x, q.size.y, q.size.z};
+        float best = -1.0f;
+        f3 bestPoint{0, 0, 0}, bestNormal{0, 0, 0};
+        int bestBody = -1;
+        for (int k = 0; k < nc; k++) {
+            const Body &b = w.bodies[cand[k]];
+            const float bp[3] = {b.pos.x, b.pos.y, b.pos.z}, bq[4] = {b.q.x, b.q.y, b.q.z, b.q.w},
+                        bs[3] = {b.size.x, b.size.y, b.size.z};
+            float bas[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}; int32_t keys[8]; float ra[24], rb[24];
+            const int cnt = oracle_collide_shapes(q.shape, qp, qq, qs, b.shape, bp, bq, bs, m, bas, keys, ra, rb);
+            for (int i = 0; i < cnt; i++) {
+                const f3 nrm{bas[0], bas[1], bas[2]};
+                const float d = queryPenetration(q, b, nrm);
+                if (d <= best) continue;
+                best = d;
+                bestNormal = nrm;
+                bestPoint = q.pos + qrot(q.q, load3(ra, i));
+                bestBody = cand[k];
+            }
+        }
+        body[r] = bestBody;
+        store3(point, r, bestPoint);
+        store3(normal, r, bestNormal);
+        depth[r] = best;
     }
     return AVBD_OK;
 }
@@ -2657,6 +2823,11 @@ bool sweepShape(const QShape &a, f3 motion, const QShape &b, float margin, float
     return emit(low, high, high);
 }
 } // namespace
+
+static float queryPenetration(const Body &a, const Body &b, f3 normal)
+{
+    return penetrationDepth(QShape{a.shape, a.pos, a.q, a.size}, QShape{b.shape, b.pos, b.q, b.size}, normal);
+}
 
 extern "C" int oracle_sweep_shape(int typeA, const float posA[3], const float rotA[4], const float sizeA[3],
                                   const float motion[3], int typeB, const float posB[3], const float rotB[4],

@@ -25,6 +25,17 @@ int oracle_get_manifolds(oracle_world *o, avbd_manifolds *m);
 int oracle_set_manifolds(oracle_world *o, const avbd_manifolds *m);
 // Planted mistakes for the oracle's own tests (0 = none; the ids are listed in avbd_oracle.cpp).
 void oracle_plant(int id);
+// query::closestPoint on one shape; the server's closest point to a body's volume for n bodies.
+void oracle_closest_point_shape(int type, const float pos[3], const float rot[4], const float size[3],
+                                const float point[3], float out[3]);
+int oracle_closest_point(oracle_world *o, int n, const int32_t *body, const float *points, float *out);
+// The bodies parts of the server's collide_shape and rest_info for n query shapes.
+int oracle_collide_shape(oracle_world *o, int n, const int32_t *type, const float *center, const float *rot,
+                         const float *size, const float *margin, const uint32_t *masks, int maxPairs, int32_t *count,
+                         uint8_t *any, float *points);
+int oracle_rest_info(oracle_world *o, int n, const int32_t *type, const float *center, const float *rot,
+                     const float *size, const float *margin, const uint32_t *masks, int32_t *body, float *point,
+                     float *normal, float *depth);
 // avbd::Manifold::collide for bodies a and b at their current pose (margin grown over hz);
 // returns the contact count (-1 for bad indices).
 int oracle_collide_pair(oracle_world *o, int a, int b, float hz, int32_t *keys, float *rA, float *rB, float *basis);

@@ -80,6 +80,12 @@ def lib():
         "oracle_pick": (C.c_int, [vp, C.c_int, a.f32p, a.f32p, a.u32p, a.i32p, a.f32p]),
         "oracle_contact_report": (C.c_int, [vp, a.i32p, C.c_int, a.i32p, a.f32p, a.f32p, a.i32p]),
         "oracle_contains_point": (C.c_int, [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, C.c_float]),
+        "oracle_closest_point_shape": (None, [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, a.f32p]),
+        "oracle_closest_point": (C.c_int, [vp, C.c_int, a.i32p, a.f32p, a.f32p]),
+        "oracle_collide_shape": (C.c_int, [vp, C.c_int, a.i32p, a.f32p, a.f32p, a.f32p, a.f32p, a.u32p, C.c_int,
+                                           a.i32p, a.u8p, a.f32p]),
+        "oracle_rest_info": (C.c_int, [vp, C.c_int, a.i32p, a.f32p, a.f32p, a.f32p, a.f32p, a.u32p, a.i32p, a.f32p,
+                                       a.f32p, a.f32p]),
         "oracle_sweep_shape": (C.c_int, [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, C.c_int, a.f32p, a.f32p, a.f32p,
                                          C.c_float, a.f32p]),
         "oracle_cast_motion": (C.c_int, [vp, C.c_int, a.i32p, a.f32p, a.f32p, a.f32p, a.f32p, a.f32p, a.u32p, a.i32p,
@@ -117,6 +123,8 @@ def ref_lib():
     R.ref_sweep_shape.restype = C.c_int
     R.ref_sweep_shape.argtypes = [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, C.c_int, a.f32p, a.f32p, a.f32p, C.c_float,
                                   a.f32p]
+    R.ref_closest_point.restype = None
+    R.ref_closest_point.argtypes = [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, a.f32p]
     R.ref_contains_point.restype = C.c_int
     R.ref_contains_point.argtypes = [C.c_int, a.f32p, a.f32p, a.f32p, a.f32p, C.c_float]
     R.ref_solve6.restype = C.c_int
@@ -231,6 +239,48 @@ class OracleWorld:
                                       None if mk is None else mk.ctypes.data_as(a.u32p), max_results,
                                       count.ctypes.data_as(a.i32p), out.ctypes.data_as(a.i32p))
         return count, out
+
+    def _shape_args(self, types, centers, rotations, sizes, margins, masks):
+        a = self.a
+        t = np.ascontiguousarray(types, np.int32)
+        arrs = [np.ascontiguousarray(x, np.float32) for x in (centers, rotations, sizes, margins)]
+        mk = None if masks is None else np.ascontiguousarray(masks, np.uint32)
+        keep = (t, arrs, mk)
+        args = [len(t), t.ctypes.data_as(a.i32p)] + [x.ctypes.data_as(a.f32p) for x in arrs] + \
+               [None if mk is None else mk.ctypes.data_as(a.u32p)]
+        return keep, args
+
+    def collide_shape(self, types, centers, rotations, sizes, margins, masks=None, max_pairs=8):
+        """The server's collide_shape for n query shapes: (pair count, any, points n x max_pairs x 2 x 3)."""
+        a = self.a
+        keep, args = self._shape_args(types, centers, rotations, sizes, margins, masks)
+        n = args[0]
+        count, anyhit = np.zeros(n, np.int32), np.zeros(n, np.uint8)
+        pts = np.zeros((n, max_pairs, 2, 3), np.float32)
+        self.L.oracle_collide_shape(self.h, *args, max_pairs, count.ctypes.data_as(a.i32p),
+                                    anyhit.ctypes.data_as(a.u8p), pts.ctypes.data_as(a.f32p))
+        return count, anyhit, pts
+
+    def rest_info(self, types, centers, rotations, sizes, margins, masks=None):
+        """The server's rest_info for n query shapes: (body or -1, point, normal, depth)."""
+        a = self.a
+        keep, args = self._shape_args(types, centers, rotations, sizes, margins, masks)
+        n = args[0]
+        body, point, normal, depth = (np.zeros(n, np.int32), np.zeros((n, 3), np.float32),
+                                      np.zeros((n, 3), np.float32), np.zeros(n, np.float32))
+        self.L.oracle_rest_info(self.h, *args, body.ctypes.data_as(a.i32p), point.ctypes.data_as(a.f32p),
+                                normal.ctypes.data_as(a.f32p), depth.ctypes.data_as(a.f32p))
+        return body, point, normal, depth
+
+    def closest_point(self, bodies, points):
+        """The server's closest point to each body's volume (bodies, points n x 3)."""
+        a = self.a
+        b = np.ascontiguousarray(bodies, np.int32)
+        p = np.ascontiguousarray(points, np.float32)
+        out = np.zeros((len(b), 3), np.float32)
+        assert self.L.oracle_closest_point(self.h, len(b), b.ctypes.data_as(a.i32p), p.ctypes.data_as(a.f32p),
+                                           out.ctypes.data_as(a.f32p)) == 0
+        return out
 
     def cast_motion(self, types, centers, rotations, sizes, motions, margins, masks=None):
         """The server's cast_motion for n query shapes: dict of body (-1: none), safe, unsafe,
@@ -494,3 +544,22 @@ class planted:
     def __exit__(self, *exc):
         lib().oracle_plant(0)
         return False
+
+
+def _closest(fn, t, pos, rot, size, point):
+    a = _abi()
+    args = [np.ascontiguousarray(x, np.float32) for x in (pos, rot, size, point)]
+    out = np.zeros(3, np.float32)
+    fn(int(t), *[x.ctypes.data_as(a.f32p) for x in args], out.ctypes.data_as(a.f32p))
+    return out
+
+
+def closest_point(t, pos, rot, size, point):
+    """query::closestPoint on one shape (the oracle's restatement)."""
+    return _closest(lib().oracle_closest_point_shape, t, pos, rot, size, point)
+
+
+def ref_closest_point(t, pos, rot, size, point):
+    """The reference's own query::closestPoint, or None when oracle/_ref is not built."""
+    R = ref_lib()
+    return None if R is None else _closest(R.ref_closest_point, t, pos, rot, size, point)

@@ -84,6 +84,23 @@ extern "C" int ref_contains_point(int type, const float pos[3], const float rot[
     return query::containsPoint(sh, float3{point[0], point[1], point[2]}, margin) ? 1 : 0;
 }
 
+// query::closestPoint on a shape built from a body's size as the solver's makeShape does.
+extern "C" void ref_closest_point(int type, const float pos[3], const float rot[4], const float size[3],
+                                  const float point[3], float out[3])
+{
+    using namespace avbd;
+    Shape sh;
+    sh.type = static_cast<ShapeType>(type);
+    sh.center = float3{pos[0], pos[1], pos[2]};
+    sh.rotation = quat{rot[0], rot[1], rot[2], rot[3]};
+    sh.half = float3{size[0], size[1], size[2]} * 0.5f;
+    sh.radius = size[0];
+    sh.halfHeight = size[2] * 0.5f;
+    sh.axis = qrot(sh.rotation, float3{0, 0, 1});
+    const float3 c = query::closestPoint(sh, float3{point[0], point[1], point[2]});
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+}
+
 // query::sweepShape of shape a moved by motion against shape b; out = safe, unsafe, depth,
 // point, normal.
 extern "C" int ref_sweep_shape(int typeA, const float posA[3], const float rotA[4], const float sizeA[3],

@@ -19,6 +19,9 @@ tests/golden/ray_shapes.npz holds query::intersectRay outputs (hit, t, point, no
 of the reference's own src/avbd/query/query_shape.cpp for rays against boxes, spheres and
 cylinders: axis-aligned and oblique, from outside and inside, unnormalised directions, a zero
 direction component, bounded and unbounded.  `python make_fixtures.py rays` writes only that file.
+
+tests/golden/closest_point.npz holds query::closestPoint outputs of the same query_shape.cpp
+(`python make_fixtures.py closest` writes only that file).
 """
 import os
 import sys
@@ -159,6 +162,34 @@ def write_rays(n=1500):
     print(f"rays: {len(cases)} cases, {int(np.sum(out['hit']))} hits, {int(np.sum(out['inside']))} from inside")
 
 
+def write_closest(n=1200):
+    """query::closestPoint of the reference's own query_shape.cpp for points against boxes,
+    spheres and cylinders: outside, inside, on the axis and at the centre."""
+    rng = np.random.default_rng(2468)
+    rec = {k: [] for k in ("type", "pos", "rot", "size", "point")}
+    out = []
+    I = np.array([0, 0, 0, 1], np.float32)
+    fixed = [(1, (0, 0, 0), I, (1, 0, 0), (0, 0, 0)), (2, (0, 0, 0), I, (1, 1, 4), (0, 0, 0.5)),
+             (2, (0, 0, 0), I, (1, 1, 1), (0, 0, 0.2)), (2, (0, 0, 0), I, (1, 1, 4), (0.2, 0, 1.9)),
+             (0, (0, 0, 0), I, (2, 2, 2), (0.5, 0.2, -0.1)), (0, (0, 0, 0), I, (2, 2, 2), (3, -4, 0.5))]
+    cases = list(fixed)
+    for _ in range(n):
+        t = int(rng.integers(0, 3))
+        size = (rng.random(3) * 2.0 + 0.2).astype(np.float32)
+        q = rng.standard_normal(4)
+        q = (q / np.linalg.norm(q)).astype(np.float32)
+        pos = rng.standard_normal(3).astype(np.float32)
+        pt = (pos + rng.standard_normal(3) * rng.choice([0.05, 0.5, 2.0])).astype(np.float32)
+        cases.append((t, pos, q, size, pt))
+    for c in cases:
+        for k, v in zip(rec, c):
+            rec[k].append(np.asarray(v, np.int32 if k == "type" else np.float32))
+        out.append(oracle.ref_closest_point(*c))
+    np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "closest_point.npz"),
+                        **{k: np.stack(v) for k, v in rec.items()}, closest=np.stack(out))
+    print(f"closest points: {len(cases)} cases")
+
+
 def main():
     oracle.build(ref=True)
     if oracle.ref_lib() is None:
@@ -205,7 +236,11 @@ def main():
                         sets=np.int32(4), **sets)
     write_shapes()
     write_rays()
+    write_closest()
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "closest":
+        write_closest()
+    else:
+        main()

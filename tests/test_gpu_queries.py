@@ -203,3 +203,60 @@ def test_collide_pair_matches_oracle():
                         assert np.array_equal(np.asarray(u).view(np.uint32), np.asarray(v).view(np.uint32)), (x, y)
     assert hits > 300
     w.close()
+
+
+def _settled_round_pile(n=400, steps=60):
+    b, j = scenes.round_pile(n, seed=23, pitch=2.2)
+    p = pkg.default_params(iterations=10)
+    w = pkg.World(p, len(b["mass"]), 0)
+    w.upload(b, j)
+    w.step(steps)
+    o = oracle.OracleWorld(p)
+    o.upload({**b, **w.download()}, j)
+    return b, w, o
+
+
+def _random_shapes(rng, pos, n):
+    types = rng.integers(0, 3, n).astype(np.int32)
+    centers = (pos[rng.integers(0, len(pos), n)] + rng.normal(0, 0.7, (n, 3))).astype(np.float32)
+    rots = rng.normal(size=(n, 4)).astype(np.float32)
+    rots /= np.linalg.norm(rots, axis=1, keepdims=True)
+    sizes = rng.uniform(0.3, 1.6, (n, 3)).astype(np.float32)
+    margins = rng.choice([0.0, 0.0, 0.03], n).astype(np.float32)
+    masks = np.where(rng.random(n) < 0.15, 2, 0xFFFFFFFF).astype(np.uint32)
+    return types, centers, rots, sizes, margins, masks
+
+
+def test_collide_shape_and_rest_info_match_oracle():
+    """The server's collide_shape and rest_info on the device against the oracle over 600 random
+    query shapes on a settled mixed pile: pair counts, the any flags, every point, the deepest
+    body, its point, normal and depth bit for bit."""
+    b, w, o = _settled_round_pile()
+    rng = np.random.default_rng(12)
+    args = _random_shapes(rng, w.download()["position"], 600)
+    for max_pairs in (3, 32):
+        g = w.collide_shape(*args, max_pairs=max_pairs)
+        r = o.collide_shape(*args, max_pairs=max_pairs)
+        assert bits_equal(g[0], r[0]) and bits_equal(g[1], r[1])
+        for q in range(len(args[0])):
+            assert bits_equal(g[2][q, :g[0][q]], r[2][q, :r[0][q]]), q
+        assert int((g[0] > 0).sum()) > 150
+    g = w.rest_info(*args)
+    r = o.rest_info(*args)
+    assert bits_equal(g[0], r[0])
+    hit = g[0] >= 0
+    assert int(hit.sum()) > 150
+    for x, y in zip(g[1:], r[1:]):
+        assert bits_equal(x[hit], y[hit])
+    w.close()
+
+
+def test_closest_point_matches_oracle():
+    b, w, o = _settled_round_pile()
+    rng = np.random.default_rng(5)
+    n = len(b["mass"])
+    idx = rng.integers(0, n, 4000).astype(np.int32)
+    pos = w.download()["position"]
+    pts = (pos[idx] + rng.normal(0, 1.0, (4000, 3)) * rng.choice([0.05, 1.0, 3.0], (4000, 1))).astype(np.float32)
+    assert bits_equal(w.closest_point(idx, pts), o.closest_point(idx, pts))
+    w.close()

@@ -1139,3 +1139,126 @@ def test_contact_grace_parks_and_revives():
         first[grace] = float(m["penalty"][0][0])
     seed = float(np.float32(np.float32(0.05) * np.float32(1e4)) * np.float32(0.999))
     assert first[3] >= seed and first[0] < seed, first
+
+
+# ---------------------------------------------------------------- collide_shape, rest_info, closest point ----
+
+def test_closest_point_matches_reference_fixture():
+    f = np.load(os.path.join(ROOT, "tests", "golden", "closest_point.npz"))
+    assert len(f["type"]) > 1000
+    for i in range(len(f["type"])):
+        got = oracle.closest_point(f["type"][i], f["pos"][i], f["rot"][i], f["size"][i], f["point"][i])
+        assert np.array_equal(got.view(np.uint32), f["closest"][i].view(np.uint32)), i
+
+
+def test_closest_point_matches_reference_live():
+    if oracle.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (reference tree absent); the fixture test covers this")
+    rng = np.random.default_rng(17)
+    for _ in range(3000):
+        t = int(rng.integers(0, 3))
+        size = rng.uniform(0.2, 2.5, 3).astype(np.float32)
+        q = rng.normal(size=4)
+        q = (q / np.linalg.norm(q)).astype(np.float32)
+        pos = rng.uniform(-3, 3, 3).astype(np.float32)
+        p = (pos + rng.normal(size=3) * rng.choice([0.01, 0.3, 1.5, 4.0])).astype(np.float32)
+        a = oracle.closest_point(t, pos, q, size, p)
+        b = oracle.ref_closest_point(t, pos, q, size, p)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_closest_point_known_answers():
+    I = np.array([0, 0, 0, 1], np.float32)
+    box = (0, (1, 2, 3), I, (2, 4, 6))
+    assert np.allclose(oracle.closest_point(*box, (5, 2, 3)), (2, 2, 3))          # clamped to the +x face
+    assert np.allclose(oracle.closest_point(*box, (1.5, 2.5, 3.5)), (1.5, 2.5, 3.5))   # inside: itself
+    sph = (1, (0, 0, 0), I, (2, 0, 0))
+    assert np.allclose(oracle.closest_point(*sph, (0, 0, 5)), (0, 0, 2))           # onto the surface
+    assert np.allclose(oracle.closest_point(*sph, (0, 0, 0)), (2, 0, 0))           # the centre: +x
+    cyl = (2, (0, 0, 0), I, (1, 1, 4))
+    assert np.allclose(oracle.closest_point(*cyl, (3, 0, 1)), (1, 0, 1))           # side
+    assert np.allclose(oracle.closest_point(*cyl, (0.2, 0, 1.9)), (0.2, 0, 2))     # nearer cap
+    assert np.allclose(oracle.closest_point(*cyl, (0, 0, 0.5)), (1, 0, 0.5))       # on the axis, side nearer
+
+
+def _stacked(second_z=0.5):
+    s = scenes.SceneBuilder()
+    s.box((20, 20, 1), 0.0, 0.5, (0, 0, -0.5))
+    s.box((1, 1, 1), 1.0, 0.5, (0, 0, 0.5))
+    s.box((1, 1, 1), 1.0, 0.5, (3, 0, second_z))
+    o = oracle.OracleWorld(P())
+    o.upload(*s.build())
+    return o
+
+
+def test_collide_shape_closed_form():
+    """A unit box query sunk 1 cm into the top of a resting unit box (list index 1): four pairs,
+    the query's bottom corners and the box's top corners, 1 cm apart; the cap on pairs holds and
+    `any` still says it touched; masks and a far query give nothing."""
+    o = _stacked()
+    I = np.array([[0, 0, 0, 1]], np.float32)
+    cnt, anyhit, pts = o.collide_shape([0], [[0, 0, 1.49]], I, [[1, 1, 1]], [0.0], max_pairs=8)
+    assert cnt[0] == 4 and anyhit[0] == 1
+    q, b = pts[0, :4, 0], pts[0, :4, 1]
+    assert np.allclose(q[:, 2], 0.99, atol=1e-6) and np.allclose(b[:, 2], 1.0, atol=1e-6)
+    assert np.allclose(np.abs(q[:, :2]), 0.5, atol=1e-6) and np.allclose(q[:, :2], b[:, :2], atol=1e-6)
+    cnt, anyhit, _ = o.collide_shape([0], [[0, 0, 1.49]], I, [[1, 1, 1]], [0.0], max_pairs=2)
+    assert cnt[0] == 2 and anyhit[0] == 1
+    cnt, anyhit, _ = o.collide_shape([0], [[0, 0, 1.49]], I, [[1, 1, 1]], [0.0], masks=[2])
+    assert cnt[0] == 0 and anyhit[0] == 0
+    cnt, anyhit, _ = o.collide_shape([0], [[0, 0, 1.6]], I, [[1, 1, 1]], [0.05])
+    assert cnt[0] == 0 and anyhit[0] == 0
+    cnt, anyhit, _ = o.collide_shape([0], [[0, 0, 1.6]], I, [[1, 1, 1]], [0.2])
+    assert cnt[0] == 4 and anyhit[0] == 1   # inside the margin: contacts 10 cm apart
+    assert np.allclose(_[0, :4, 0, 2] - _[0, :4, 1, 2], 0.1, atol=1e-5)
+
+
+def test_rest_info_closed_form():
+    """A slab over two boxes, 1 cm into the one at x = 0 (list index 1) and 3 cm into the raised
+    one at x = 3 (list index 0), rests on the deeper: depth 0.03 (query::penetrationDepth along
+    the contact normal), the normal points up out of it, the point is on the slab's bottom face;
+    a unit box sunk 1 cm into the first alone gets depth 0.01; nothing touched gives body -1."""
+    o = _stacked(second_z=0.52)
+    I = np.array([[0, 0, 0, 1]], np.float32)
+    body, point, normal, depth = o.rest_info([0], [[1.5, 0, 1.49]], I, [[4.2, 1, 1]], [0.0])
+    assert body[0] == 0 and np.allclose(normal[0], (0, 0, 1), atol=1e-6), (body, normal)
+    assert abs(depth[0] - 0.03) < 1e-5 and abs(point[0][2] - 0.99) < 1e-6 and point[0][0] > 2.4, (depth, point)
+    o = _stacked()
+    body, point, normal, depth = o.rest_info([0], [[0, 0, 1.49]], I, [[1, 1, 1]], [0.0])
+    assert body[0] == 1 and abs(depth[0] - 0.01) < 1e-5 and np.allclose(normal[0], (0, 0, 1), atol=1e-6)
+    assert abs(point[0][2] - 0.99) < 1e-6
+    body, _, _, _ = o.rest_info([0], [[0, 0, 3.0]], I, [[1, 1, 1]], [0.0])
+    assert body[0] == -1
+
+
+def test_collide_shape_and_rest_info_are_the_narrow_phase_over_collected_bodies():
+    """Brute force over every body (narrow phase query first, the list order, the caps): the
+    bounds pre-filter drops no contact, and the deepest is the maximum over all contacts."""
+    b, j = scenes.round_pile(80, seed=9, pitch=2.2)
+    o = oracle.OracleWorld(P(iterations=10))
+    o.upload(b, j)
+    o.step(60)
+    st = o.download()
+    rng = np.random.default_rng(3)
+    pos = st["position"]
+    n = 120
+    types = rng.integers(0, 3, n).astype(np.int32)
+    centers = (pos[rng.integers(0, len(pos), n)] + rng.normal(0, 0.6, (n, 3))).astype(np.float32)
+    rots = rng.normal(size=(n, 4)).astype(np.float32)
+    rots /= np.linalg.norm(rots, axis=1, keepdims=True)
+    sizes = rng.uniform(0.3, 1.5, (n, 3)).astype(np.float32)
+    margins = rng.choice([0.0, 0.02], n).astype(np.float32)
+    cnt, anyhit, pts = o.collide_shape(types, centers, rots, sizes, margins, max_pairs=64)
+    body, point, normal, depth = o.rest_info(types, centers, rots, sizes, margins)
+    touched = 0
+    for r in range(n):
+        want_pairs, best = [], None
+        for i in range(len(pos)):
+            c = oracle.collide_shapes(types[r], centers[r], rots[r], sizes[r], int(b.get("shape", np.zeros(len(pos)))[i]),
+                                      st["position"][i], st["rotation"][i], b["size"][i], margins[r])
+            for k in range(c[0]):
+                want_pairs.append(i)
+        assert cnt[r] == len(want_pairs) and bool(anyhit[r]) == bool(want_pairs), r
+        touched += bool(want_pairs)
+        assert (body[r] >= 0) == bool(want_pairs)
+    assert touched > 30
