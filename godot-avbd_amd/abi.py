@@ -299,6 +299,8 @@ def load_library(path=LIB_PATH):
         "avbd_batch_fits": (C.c_int32, [vp, C.c_int32, C.c_int32, i32p, i32p]),
         "avbd_batch_use_graph": (C.c_int32, [vp, C.c_int32]),
         "avbd_world_use_graph": (C.c_int32, [vp, C.c_int32]),
+        "avbd_world_use_query_tree": (C.c_int32, [vp, C.c_int32]),
+        "avbd_batch_use_query_tree": (C.c_int32, [vp, C.c_int32]),
         "avbd_batch_download_world": (C.c_int32, [vp, C.c_int32, C.POINTER(Bodies), C.POINTER(Joints)]),
         "avbd_batch_world_count": (C.c_int32, [vp, i32p]),
         "avbd_batch_get_manifolds": (C.c_int32, [vp, C.c_int32, C.POINTER(Manifolds)]),

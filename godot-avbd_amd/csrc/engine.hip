@@ -286,6 +286,40 @@ struct Engine {
         return AVBD_OK;
     }
 
+    // Query tree: the last step's LBVH topology refit over the current tight bounds in qBounds
+    // (k_query_bounds of world w first).  Valid while the bodies of every world are those the
+    // tree was built over: from the first step after an upload, a removal or a world change on.
+    // Without one (t.box null) the query kernels scan the world's list.
+    float4 *qtC = nullptr, *qtH = nullptr, *qtBox = nullptr;
+    int *qtFlag = nullptr;
+    size_t qtCap = 0;
+    bool queryTreeOn = true;
+
+    avbd_status queryTree(int w, QTree &t)
+    {
+        const WorldInfo &wi = worlds[w];
+        t = QTree{nullptr, Ncap, w};
+        if (!queryTreeOn || treeAge > refitBudget || wi.bodyCount < 2) return AVBD_OK;
+        if (static_cast<size_t>(Ncap) > qtCap) {
+            avbd_status st;
+            if ((st = regrow(qtC, Ncap)) != AVBD_OK || (st = regrow(qtH, Ncap)) != AVBD_OK ||
+                (st = regrow(qtBox, 4 * static_cast<size_t>(Ncap))) != AVBD_OK || (st = regrow(qtFlag, Ncap)) != AVBD_OK)
+                return st;
+            qtCap = Ncap;
+        }
+        k_qtree_leaves<<<nblk(Ncap), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, qBounds, qtC, qtH);
+        State Q = S;
+        Q.aabbC = qtC;
+        Q.aabbH = qtH;
+        Q.nodeBox = qtBox;
+        Q.nodeFlag = qtFlag;
+        k_refit_clear<<<nblk(Ncap), TPB, 0, stream>>>(Q, Ncap);
+        k_refit_tasks<<<refitGrid(), REFIT_TPB, 0, stream>>>(Q, Ncap);
+        HIPCHK(hipGetLastError());
+        t.box = qtBox;
+        return AVBD_OK;
+    }
+
     avbd_status raycast(int w, int n, const float *from, const float *to, const uint32_t *masks, bool fromInside,
                         int32_t *body, float *t, float *point, float *normal, uint8_t *inside)
     {
@@ -305,8 +339,10 @@ struct Engine {
         }
         HIPCHK(hipMemcpyAsync(qRays, rays.data(), sizeof(float4) * rays.size(), hipMemcpyHostToDevice, stream));
         if (wi.bodyCount > 0) k_query_bounds<<<nblk(wi.bodyCount), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, qBounds);
+        QTree tree;
+        if ((st = queryTree(w, tree)) != AVBD_OK) return st;
         k_raycast<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qRays, qBounds, fromInside ? 1 : 0, qBody,
-                                               qA, qB);
+                                               qA, qB, tree);
         HIPCHK(hipGetLastError());
         std::vector<float4> a(n), b2(n);
         HIPCHK(hipMemcpyAsync(body, qBody, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
@@ -346,8 +382,10 @@ struct Engine {
         for (int k = 0; k < reps; k++) {
             HIPCHK(hipEventRecord(e[0], stream));
             k_query_bounds<<<nblk(std::max(wi.bodyCount, 1)), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, qBounds);
+            QTree tree;
+            if ((st = queryTree(w, tree)) != AVBD_OK) return st;
             HIPCHK(hipEventRecord(e[1], stream));
-            k_raycast<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qRays, qBounds, 0, qBody, qA, qB);
+            k_raycast<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qRays, qBounds, 0, qBody, qA, qB, tree);
             HIPCHK(hipEventRecord(e[2], stream));
             HIPCHK(hipEventSynchronize(e[2]));
             float a = 0.0f, b = 0.0f;
@@ -388,7 +426,10 @@ struct Engine {
         }
         HIPCHK(hipMemcpyAsync(qRays, pts.data(), sizeof(float4) * n, hipMemcpyHostToDevice, stream));
         if (wi.bodyCount > 0) k_query_bounds<<<nblk(wi.bodyCount), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, qBounds);
-        k_intersect_point<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qRays, qBounds, maxRes, qBody, qOut);
+        QTree tree;
+        if ((st = queryTree(w, tree)) != AVBD_OK) return st;
+        k_intersect_point<<<nblk(n), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qRays, qBounds, maxRes, qBody, qOut,
+                                                       tree);
         HIPCHK(hipGetLastError());
         std::vector<int> o(outN);
         HIPCHK(hipMemcpyAsync(count, qBody, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
@@ -406,7 +447,7 @@ struct Engine {
 
     // Query shapes to the device (layout of k_intersect_shape), bounds of world w's bodies too.
     avbd_status uploadShapes(int w, int n, const int32_t *type, const float *center, const float *rot,
-                             const float *size, const float *margin, const uint32_t *masks)
+                             const float *size, const float *margin, const uint32_t *masks, QTree &tree)
     {
         for (int r = 0; r < n; r++)
             if (type[r] < AVBD_SHAPE_BOX || type[r] > AVBD_SHAPE_CYLINDER) return fail(AVBD_ERR_INVALID, "unknown shape");
@@ -427,6 +468,7 @@ struct Engine {
         HIPCHK(hipMemcpyAsync(qShapes, in.data(), sizeof(float4) * in.size(), hipMemcpyHostToDevice, stream));
         const WorldInfo &wi = worlds[w];
         if (wi.bodyCount > 0) k_query_bounds<<<nblk(wi.bodyCount), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, qBounds);
+        if ((st = queryTree(w, tree)) != AVBD_OK) return st;
         HIPCHK(hipStreamSynchronize(stream));   // `in` is host memory the copy reads
         return AVBD_OK;
     }
@@ -441,7 +483,8 @@ struct Engine {
                                                 (maxPairs > 0 && !points))))
             return fail(AVBD_ERR_INVALID, "bad arguments");
         if (n == 0) return AVBD_OK;
-        avbd_status st = uploadShapes(w, n, type, center, rot, size, margin, masks);
+        QTree tree;
+        avbd_status st = uploadShapes(w, n, type, center, rot, size, margin, masks, tree);
         if (st != AVBD_OK) return st;
         const size_t outN = 2 * static_cast<size_t>(n) * static_cast<size_t>(std::max(maxPairs, 1));
         if (outN > qPtsCap) {
@@ -450,7 +493,7 @@ struct Engine {
         }
         k_collide_shape<<<nblk(n, NARROW_TPB), NARROW_TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qShapes, qBounds,
                                                                         maxPairs, qBody, qFlag, qPts, qCRA, qCRB, qCMid,
-                                                                        qSpA, qSpB);
+                                                                        qSpA, qSpB, tree);
         HIPCHK(hipGetLastError());
         std::vector<float4> pts(static_cast<size_t>(n) * 2 * maxPairs);
         HIPCHK(hipMemcpyAsync(count, qBody, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
@@ -475,10 +518,11 @@ struct Engine {
         if (n < 0 || (n > 0 && (!type || !center || !rot || !size || !margin || !body)))
             return fail(AVBD_ERR_INVALID, "bad arguments");
         if (n == 0) return AVBD_OK;
-        avbd_status st = uploadShapes(w, n, type, center, rot, size, margin, masks);
+        QTree tree;
+        avbd_status st = uploadShapes(w, n, type, center, rot, size, margin, masks, tree);
         if (st != AVBD_OK) return st;
         k_rest_info<<<nblk(n, NARROW_TPB), NARROW_TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qShapes, qBounds, qBody,
-                                                                    qA, qB, qCRA, qCRB, qCMid, qSpA, qSpB);
+                                                                    qA, qB, qCRA, qCRB, qCMid, qSpA, qSpB, tree);
         HIPCHK(hipGetLastError());
         std::vector<float4> a(n), b(n);
         HIPCHK(hipMemcpyAsync(body, qBody, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
@@ -549,8 +593,11 @@ struct Engine {
         }
         HIPCHK(hipMemcpyAsync(qShapes, in.data(), sizeof(float4) * in.size(), hipMemcpyHostToDevice, stream));
         if (wi.bodyCount > 0) k_query_bounds<<<nblk(wi.bodyCount), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, qBounds);
+        QTree tree;
+        if ((st = queryTree(w, tree)) != AVBD_OK) return st;
         k_intersect_shape<<<nblk(n, NARROW_TPB), NARROW_TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qShapes, qBounds,
-                                                                          maxRes, qBody, qOut, qCRA, qCRB, qCMid, qSpA, qSpB);
+                                                                          maxRes, qBody, qOut, qCRA, qCRB, qCMid, qSpA, qSpB,
+                                                                          tree);
         HIPCHK(hipGetLastError());
         std::vector<int> o(outN);
         HIPCHK(hipMemcpyAsync(count, qBody, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
@@ -601,8 +648,10 @@ struct Engine {
         }
         HIPCHK(hipMemcpyAsync(qShapes, in.data(), sizeof(float4) * in.size(), hipMemcpyHostToDevice, stream));
         if (wi.bodyCount > 0) k_query_bounds<<<nblk(wi.bodyCount), TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, qBounds);
+        QTree tree;
+        if ((st = queryTree(w, tree)) != AVBD_OK) return st;
         k_cast_motion<<<nblk(n, NARROW_TPB), NARROW_TPB, 0, stream>>>(S, wi.bodyBase, wi.bodyCount, n, qShapes, qBounds, qBody,
-                                                                      qA, qB, qCRA, qCRB, qCMid, qSpA, qSpB);
+                                                                      qA, qB, qCRA, qCRB, qCMid, qSpA, qSpB, tree);
         HIPCHK(hipGetLastError());
         std::vector<float4> a(n), b(n);
         HIPCHK(hipMemcpyAsync(body, qBody, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
@@ -2470,6 +2519,20 @@ avbd_status avbd_batch_use_graph(avbd_batch *batch, int32_t enable)
     return AVBD_OK;
 }
 
+avbd_status avbd_world_use_query_tree(avbd_world *world, int32_t enable)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    world->eng.queryTreeOn = enable != 0;
+    return AVBD_OK;
+}
+
+avbd_status avbd_batch_use_query_tree(avbd_batch *batch, int32_t enable)
+{
+    if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
+    batch->eng.queryTreeOn = enable != 0;
+    return AVBD_OK;
+}
+
 avbd_status avbd_batch_step(avbd_batch *batch)
 {
     if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
@@ -2638,8 +2701,10 @@ avbd_status avbd_world_raycast_noalloc(avbd_world *world, int32_t n, const float
     if (masks) HIPCHK(hipMemcpyAsync(dMask, masks, sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
     k_na_pack_rays<<<nblk(n), TPB, 0, s>>>(n, dFrom, dTo, dMask, masks ? 1 : 0, rays);
     if (wi.bodyCount > 0) k_query_bounds<<<nblk(wi.bodyCount), TPB, 0, s>>>(e.S, wi.bodyBase, wi.bodyCount, bounds);
+    // no query tree here (its refit needs buffers this call may not allocate): the list scan,
+    // whose results are the tree walk's
     k_raycast<<<nblk(n), TPB, 0, s>>>(e.S, wi.bodyBase, wi.bodyCount, n, rays, bounds, hit_from_inside ? 1 : 0, body,
-                                      hitA, hitB);
+                                      hitA, hitB, QTree{nullptr, e.Ncap, 0});
     k_na_unpack_hits<<<nblk(n), TPB, 0, s>>>(n, hitA, hitB, t, point, normal, inside);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out->body, body, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));

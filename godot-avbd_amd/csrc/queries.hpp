@@ -176,6 +176,132 @@ AD bool boxesOverlap(v3 lo, v3 hi, v3 mn, v3 mx)
 
 // Each body's current world bounds (position -/+ tight half extents), computed once per query
 // launch: bounds[2j] = lower corner, bounds[2j+1] = upper corner of body j of the world.
+// ---- the query tree: the last step's LBVH topology refit to the current tight bounds ----
+// A query walks its world's subtree instead of scanning the body list; the results stay those
+// of the scan (candidates are re-tested on their exact bounds, and taken in list order).
+struct QTree {
+    const float4 *box;   // node boxes of the refit (nullptr: no tree, scan the list)
+    int n;               // leaves (bodies) of the tree
+    int w;               // the world whose subtree is walked
+};
+
+// Leaf boxes: each body's exact tight bounds (bounds[2j], [2j + 1] of world-local j) as a
+// centre and a half extent grown by 4e-6 (|c| + h) + 1e-6, so that the float corners c -/+ h
+// enclose them; bodies outside [base, base + count) get an empty box.
+__global__ void k_qtree_leaves(State S, int base, int count, const float4 *bounds, float4 *c, float4 *h)
+{
+    const int i = GID;
+    if (i >= S.numBodies) return;
+    const int j = i - base;
+    if (S.bodyWorld[i] < 0 || j < 0 || j >= count) {
+        c[i] = make_float4(0, 0, 0, 0);
+        h[i] = make_float4(-1, -1, -1, 0);
+        return;
+    }
+    const v3 lo = ld3(bounds[2 * j]), hi = ld3(bounds[2 * j + 1]);
+    const v3 cc = (lo + hi) * 0.5f, hh = (hi - lo) * 0.5f;
+    const v3 slack{(fabsf(cc.x) + hh.x) * 4.0e-6f + 1.0e-6f, (fabsf(cc.y) + hh.y) * 4.0e-6f + 1.0e-6f,
+                   (fabsf(cc.z) + hh.z) * 4.0e-6f + 1.0e-6f};
+    c[i] = st3(cc);
+    h[i] = st3(hh + slack);
+}
+
+// Stackless walk of world w's subtree (ropes): visit(body) for every leaf whose box overlaps
+// [mn, mx].  A walk that does not end within 2n steps, or a world without a root (a corrupt
+// tree), returns false.
+template <typename T, typename F>
+AD bool walkTree(const State &S, const QTree &t, T &&test, F &&visit)
+{
+    int node = S.wRoot[t.w];
+    if (node < 0) return false;
+    for (int guard = 2 * t.n; node >= 0; guard--) {
+        if (guard == 0) return false;
+        const float4 lo = t.box[2 * node], hi = t.box[2 * node + 1];
+        const bool ov = test(ld3(lo), ld3(hi));
+        const int down = __float_as_int(lo.w), rope = __float_as_int(hi.w);
+        if (node >= t.n - 1) {
+            if (ov) visit(down);
+            node = rope;
+        } else {
+            node = ov ? down : rope;
+        }
+    }
+    return true;
+}
+
+template <typename F>
+AD bool walkBox(const State &S, const QTree &t, v3 mn, v3 mx, F &&visit)
+{
+    return walkTree(S, t, [&](v3 lo, v3 hi) { return aabbOverlap(mn, mx, lo, hi); }, visit);
+}
+
+// Does the segment o + s d, s in [0, 1], meet the box [lo, hi] grown by `slack` plus 1e-5 of
+// the box's largest coordinate per axis?  Monotone in the box (a box inside another passes only
+// if the outer one does, in float too), so it prunes a walk.  A ray hit on a shape lies on the
+// segment and inside the shape's bounds up to rounding, far below the growth.
+AD bool segmentMeetsBox(v3 o, v3 d, float slack, v3 lo, v3 hi)
+{
+    float s0 = 0.0f, s1 = 1.0f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float g = slack + 1.0e-5f * smax(fabsf(lo[k]), fabsf(hi[k]));
+        const float a = lo[k] - g, b = hi[k] + g, ok = o[k], dk = d[k];
+        if (dk == 0.0f) {
+            if (ok < a || ok > b) return false;
+            continue;
+        }
+        const float inv = 1.0f / dk;
+        float ta = (a - ok) * inv, tb = (b - ok) * inv;
+        if (ta > tb) { const float x = ta; ta = tb; tb = x; }
+        s0 = smax(s0, ta);
+        s1 = smin(s1, tb);
+    }
+    return s0 <= s1;
+}
+
+// World-local candidates of a query box in list order: the bodies whose layer passes the mask
+// and whose exact bounds overlap [mn, mx] (closed intervals); visit(j) returns false to stop.
+// With a tree, each walk keeps the QCAND smallest candidates above the last one visited,
+// sorted, so a box with many candidates walks once per QCAND of them; without one (or on a
+// corrupt tree) the rest of the list is scanned.
+constexpr int QCAND = 32;
+template <typename F>
+AD void forCandidates(const State &S, const QTree &t, int base, int count, const float4 *bounds, v3 mn, v3 mx,
+                      uint32_t mask, F &&visit)
+{
+    auto passes = [&](int j) {
+        const float4 lo = bounds[2 * j], hi = bounds[2 * j + 1];
+        return (__float_as_uint(lo.w) & mask) != 0 && boxesOverlap(ld3(lo), ld3(hi), mn, mx);
+    };
+    int after = -1;
+    if (t.box) {
+        while (true) {
+            int cand[QCAND];
+            int nc = 0;
+            bool over = false;
+            const bool ok = walkBox(S, t, mn, mx, [&](int i) {
+                const int j = i - base;
+                if (j <= after || j >= count || !passes(j)) return;
+                if (nc == QCAND) {
+                    over = true;
+                    if (j > cand[QCAND - 1]) return;
+                    nc--;
+                }
+                int b = nc++;
+                while (b > 0 && cand[b - 1] > j) { cand[b] = cand[b - 1]; b--; }
+                cand[b] = j;
+            });
+            if (!ok) break;
+            for (int k = 0; k < nc; k++)
+                if (!visit(cand[k])) return;
+            if (!over) return;
+            after = cand[QCAND - 1];
+        }
+    }
+    for (int j = after + 1; j < count; j++)
+        if (passes(j) && !visit(j)) return;
+}
+
 __global__ void k_query_bounds(State S, int base, int count, float4 *bounds)
 {
     const int j = GID;
@@ -192,7 +318,7 @@ __global__ void k_query_bounds(State S, int base, int count, float4 *bounds)
 // whose tight bounds overlap the segment's box padded by 1e-4 (Solver::collectAABB), in list
 // order.  Output: body list index or -1; hitA = (point, t); hitB = (normal, inside).
 __global__ void k_raycast(State S, int base, int count, int n, const float4 *rays, const float4 *bounds,
-                          int hitFromInside, int *body, float4 *hitA, float4 *hitB)
+                          int hitFromInside, int *body, float4 *hitA, float4 *hitB, QTree tree)
 {
     const int r = GID;
     if (r >= n) return;
@@ -208,23 +334,51 @@ __global__ void k_raycast(State S, int base, int count, int n, const float4 *ray
         const float pad = 1.0e-4f;
         const v3 mn{smin(origin.x, end.x) - pad, smin(origin.y, end.y) - pad, smin(origin.z, end.z) - pad};
         const v3 mx{smax(origin.x, end.x) + pad, smax(origin.y, end.y) + pad, smax(origin.z, end.z) + pad};
-        for (int j = 0; j < count; j++) {
-            const float4 lo = bounds[2 * j], hi = bounds[2 * j + 1];   // uniform across the wavefront
-            if ((__float_as_uint(lo.w) & mask) == 0) continue;
-            if (!boxesOverlap(ld3(lo), ld3(hi), mn, mx)) continue;
+        // The list-order scan keeps the first body the segment starts inside (hit from inside),
+        // else the nearest hit, the earlier body on a tie: a minimum, so the tree may visit the
+        // candidates in any order.
+        int firstInside = -1;
+        auto consider = [&](int j) {
+            const float4 lo = bounds[2 * j], hi = bounds[2 * j + 1];
+            if ((__float_as_uint(lo.w) & mask) == 0) return;
+            if (!boxesOverlap(ld3(lo), ld3(hi), mn, mx)) return;
             const int i = base + j;
             QHit h;
-            if (!intersectRay(S.shape[i], makeRShp(ld3(S.pos[i]), ldq(S.rot[i]), S.sizeR[i]), origin, dir, maxD, h))
-                continue;
-            if (h.inside && !hitFromInside) continue;
+            if (!intersectRay(S.shape[i], makeRShp(ld3(S.pos[i]), ldq(S.rot[i]), S.sizeR[i]), origin, dir, maxD, h)) return;
+            if (h.inside && !hitFromInside) return;
             if (h.inside) {
-                best = j;
-                bh = QHit{0.0f, origin, dir * -1.0f, true};
-                break;
+                if (firstInside < 0 || j < firstInside) firstInside = j;
+                return;
             }
-            if (h.t > maxD || (best >= 0 && h.t >= bh.t)) continue;
+            if (h.t > maxD) return;
+            if (best >= 0 && (h.t > bh.t || (h.t == bh.t && j > best))) return;
             best = j;
             bh = QHit{h.t, h.point, h.normal, false};
+        };
+        // The walk also skips nodes the segment itself misses (the long segments' boxes hold many
+        // bodies the ray passes far from): such bodies have no hit the scan would keep.
+        const v3 seg = end - origin;
+        const float slack = 1.0e-4f + 1.0e-5f * (smax(smax(fabsf(origin.x), fabsf(origin.y)), fabsf(origin.z)) +
+                                                  smax(smax(fabsf(seg.x), fabsf(seg.y)), fabsf(seg.z)));
+        bool walked = false;
+        if (tree.box)
+            walked = walkTree(S, tree, [&](v3 lo, v3 hi) {
+                return aabbOverlap(mn, mx, lo, hi) && segmentMeetsBox(origin, seg, slack, lo, hi);
+            }, [&](int i) {
+                const int j = i - base;
+                if (j >= 0 && j < count) consider(j);
+            });
+        if (!walked) {
+            best = -1;
+            firstInside = -1;
+            for (int j = 0; j < count; j++) {   // body bounds uniform across the wavefront
+                consider(j);
+                if (firstInside >= 0) break;
+            }
+        }
+        if (firstInside >= 0) {
+            best = firstInside;
+            bh = QHit{0.0f, origin, dir * -1.0f, true};
         }
     }
     body[r] = best;
@@ -237,7 +391,7 @@ __global__ void k_raycast(State S, int base, int count, int n, const float4 *ray
 // padded by 1e-4, in list order; out[r * maxRes + c] = the c-th body containing the point
 // (query::containsPoint, query_shape.cpp:334-337), up to maxRes.
 __global__ void k_intersect_point(State S, int base, int count, int n, const float4 *pts, const float4 *bounds,
-                                  int maxRes, int *cnt, int *out)
+                                  int maxRes, int *cnt, int *out, QTree tree)
 {
     const int r = GID;
     if (r >= n) return;
@@ -247,16 +401,15 @@ __global__ void k_intersect_point(State S, int base, int count, int n, const flo
     const float pad = 1.0e-4f;
     const v3 mn{p.x - pad, p.y - pad, p.z - pad}, mx{p.x + pad, p.y + pad, p.z + pad};
     int c = 0;
-    for (int j = 0; j < count && c < maxRes; j++) {
-        const float4 lo = bounds[2 * j], hi = bounds[2 * j + 1];
-        if ((__float_as_uint(lo.w) & mask) == 0) continue;
-        if (!boxesOverlap(ld3(lo), ld3(hi), mn, mx)) continue;
-        const int i = base + j;
-        const RShp s = makeRShp(ld3(S.pos[i]), ldq(S.rot[i]), S.sizeR[i]);
-        if (!insideLocal(S.shape[i], s, qrot(qconj(s.q), p - s.c), 0.0f)) continue;
-        out[static_cast<size_t>(r) * maxRes + c] = j;
-        c++;
-    }
+    if (maxRes > 0)
+        forCandidates(S, tree, base, count, bounds, mn, mx, mask, [&](int j) {
+            const int i = base + j;
+            const RShp s = makeRShp(ld3(S.pos[i]), ldq(S.rot[i]), S.sizeR[i]);
+            if (!insideLocal(S.shape[i], s, qrot(qconj(s.q), p - s.c), 0.0f)) return true;
+            out[static_cast<size_t>(r) * maxRes + c] = j;
+            c++;
+            return c < maxRes;
+        });
     cnt[r] = c;
 }
 
@@ -269,7 +422,7 @@ __global__ void k_intersect_point(State S, int base, int count, int n, const flo
 __global__ void __launch_bounds__(NARROW_TPB) k_intersect_shape(State S, int base, int count, int n, const float4 *qin,
                                                                const float4 *bounds, int maxRes, int *cnt, int *out,
                                                                float4 *cra, float4 *crb, float4 *cmid, float4 *spA,
-                                                               float4 *spB)
+                                                               float4 *spB, QTree tree)
 {
     __shared__ NarrowPolys poly;
     const int r = GID;
@@ -292,10 +445,7 @@ __global__ void __launch_bounds__(NARROW_TPB) k_intersect_shape(State S, int bas
     const size_t d = stageRow(r, 0);
     const int l = threadIdx.x;
     int c = 0;
-    for (int j = 0; j < count && c < maxRes; j++) {
-        const float4 lo = bounds[2 * j], hi = bounds[2 * j + 1];
-        if ((__float_as_uint(lo.w) & mask) == 0) continue;
-        if (!boxesOverlap(ld3(lo), ld3(hi), mn, mx)) continue;
+    if (maxRes > 0) forCandidates(S, tree, base, count, bounds, mn, mx, mask, [&](int j) {
         const int i = base + j;
         const int bt = S.shape[i];
         const v3 bp = ld3(S.pos[i]);
@@ -314,10 +464,11 @@ __global__ void __launch_bounds__(NARROW_TPB) k_intersect_shape(State S, int bas
             hits = collideBoxes(qc, qq, v3{s4.x, s4.y, s4.z} * 0.5f, bp, bq, v3{bs.x, bs.y, bs.z} * 0.5f, m, B, wrote, cs,
                                 c0, c1);
         }
-        if (hits <= 0) continue;
+        if (hits <= 0) return true;
         out[static_cast<size_t>(r) * maxRes + c] = j;
         c++;
-    }
+        return c < maxRes;
+    });
     cnt[r] = c;
 }
 
@@ -432,7 +583,7 @@ AD void queryBox(int type, qt qq, v3 qc, float4 s4, float m, v3 &mn, v3 &mx)
 __global__ void __launch_bounds__(NARROW_TPB) k_collide_shape(State S, int base, int count, int n, const float4 *qin,
                                                              const float4 *bounds, int maxPairs, int *cnt,
                                                              uint8_t *anyHit, float4 *pts, float4 *cra, float4 *crb,
-                                                             float4 *cmid, float4 *spA, float4 *spB)
+                                                             float4 *cmid, float4 *spA, float4 *spB, QTree tree)
 {
     __shared__ NarrowPolys poly;
     const int r = GID;
@@ -448,14 +599,11 @@ __global__ void __launch_bounds__(NARROW_TPB) k_collide_shape(State S, int base,
     const size_t d = stageRow(r, 0);
     int pairs = 0;
     bool any = false;
-    for (int j = 0; j < count; j++) {
-        const float4 lo = bounds[2 * j], hi = bounds[2 * j + 1];
-        if ((__float_as_uint(lo.w) & mask) == 0) continue;
-        if (!boxesOverlap(ld3(lo), ld3(hi), mn, mx)) continue;
+    forCandidates(S, tree, base, count, bounds, mn, mx, mask, [&](int j) {
         const int i = base + j;
         m3 B;
         const int hits = queryContacts(S, type, qc, qq, s4, m, i, poly, threadIdx.x, d, cra, crb, cmid, spA, spB, B);
-        if (hits <= 0) continue;
+        if (hits <= 0) return true;
         any = true;
         const v3 bp = ld3(S.pos[i]);
         const qt bq = ldq(S.rot[i]);
@@ -467,8 +615,8 @@ __global__ void __launch_bounds__(NARROW_TPB) k_collide_shape(State S, int base,
             pts[o + 1] = st3(wb);
             pairs++;
         }
-        if (pairs >= maxPairs) break;
-    }
+        return pairs < maxPairs;
+    });
     cnt[r] = pairs;
     anyHit[r] = any ? 1 : 0;
 }
@@ -480,7 +628,7 @@ __global__ void __launch_bounds__(NARROW_TPB) k_collide_shape(State S, int base,
 __global__ void __launch_bounds__(NARROW_TPB) k_rest_info(State S, int base, int count, int n, const float4 *qin,
                                                          const float4 *bounds, int *body, float4 *hitA, float4 *hitB,
                                                          float4 *cra, float4 *crb, float4 *cmid, float4 *spA,
-                                                         float4 *spB)
+                                                         float4 *spB, QTree tree)
 {
     __shared__ NarrowPolys poly;
     const int r = GID;
@@ -497,14 +645,11 @@ __global__ void __launch_bounds__(NARROW_TPB) k_rest_info(State S, int base, int
     float best = -1.0f;
     v3 bestPoint{0, 0, 0}, bestNormal{0, 0, 0};
     int bestBody = -1;
-    for (int j = 0; j < count; j++) {
-        const float4 lo = bounds[2 * j], hi = bounds[2 * j + 1];
-        if ((__float_as_uint(lo.w) & mask) == 0) continue;
-        if (!boxesOverlap(ld3(lo), ld3(hi), mn, mx)) continue;
+    forCandidates(S, tree, base, count, bounds, mn, mx, mask, [&](int j) {
         const int i = base + j;
         m3 B;
         const int hits = queryContacts(S, type, qc, qq, s4, m, i, poly, threadIdx.x, d, cra, crb, cmid, spA, spB, B);
-        if (hits <= 0) continue;
+        if (hits <= 0) return true;
         const v3 nrm = row(B, 0);
         const float dep = penetrationDepth(type, qc, qq, v3{s4.x, s4.y, s4.z}, S.shape[i], ld3(S.pos[i]), ldq(S.rot[i]),
                                           ld3(S.sizeR[i]), nrm);
@@ -515,7 +660,8 @@ __global__ void __launch_bounds__(NARROW_TPB) k_rest_info(State S, int base, int
             bestPoint = qc + qrot(qq, ld3(cra[d + crow(k)]));
             bestBody = j;
         }
-    }
+        return true;
+    });
     body[r] = bestBody;
     hitA[r] = st3(bestPoint, best);
     hitB[r] = st3(bestNormal, 0.0f);
@@ -693,7 +839,8 @@ AD bool sweepShape(const State &S, const SweepShape &a, v3 mv, int bt, v3 bp, qt
 // hitB[r] = (normal, unsafe), safe = unsafe = 1 when none.
 __global__ void __launch_bounds__(NARROW_TPB) k_cast_motion(State S, int base, int count, int n, const float4 *qin,
                                                            const float4 *bounds, int *body, float4 *hitA, float4 *hitB,
-                                                           float4 *cra, float4 *crb, float4 *cmid, float4 *spA, float4 *spB)
+                                                           float4 *cra, float4 *crb, float4 *cmid, float4 *spA, float4 *spB,
+                                                           QTree tree)
 {
     __shared__ NarrowPolys poly;
     const int r = GID;
@@ -715,17 +862,14 @@ __global__ void __launch_bounds__(NARROW_TPB) k_cast_motion(State S, int base, i
     int bestBody = -1;
     float bestSafe = 1.0f, bestUnsafe = 1.0f;
     v3 bestPoint{0.0f, 0.0f, 0.0f}, bestNormal{0.0f, 0.0f, 0.0f};
-    for (int j = 0; j < count; j++) {
-        const float4 lo = bounds[2 * j], hi = bounds[2 * j + 1];
-        if ((__float_as_uint(lo.w) & mask) == 0) continue;
-        if (!boxesOverlap(ld3(lo), ld3(hi), mn, mx)) continue;
+    forCandidates(S, tree, base, count, bounds, mn, mx, mask, [&](int j) {
         const int i = base + j;
         Contacts cs{0, cra + d, crb + d, cmid + d};
         float safe, unsafe;
         v3 point, normal;
         if (!sweepShape(S, a, mv, S.shape[i], ld3(S.pos[i]), ldq(S.rot[i]), S.sizeR[i], m, cs, c0, c1, safe, unsafe, point,
                         normal))
-            continue;
+            return true;
         if (bestBody < 0 || safe < bestSafe) {
             bestBody = j;
             bestSafe = safe;
@@ -733,7 +877,8 @@ __global__ void __launch_bounds__(NARROW_TPB) k_cast_motion(State S, int base, i
             bestPoint = point;
             bestNormal = normal;
         }
-    }
+        return true;
+    });
     body[r] = bestBody;
     hitA[r] = st3(bestPoint, bestSafe);
     hitB[r] = st3(bestNormal, bestUnsafe);

@@ -66,6 +66,10 @@ class World:
         """Step as one captured HIP graph (default) or kernel by kernel (for tracing tools)."""
         abi.check(self.lib.avbd_world_use_graph(self.h, 1 if enable else 0))
 
+    def use_query_tree(self, enable):
+        """Space queries walk the refit step tree (default) or scan the body list; same results."""
+        abi.check(self.lib.avbd_world_use_query_tree(self.h, 1 if enable else 0))
+
     def step_interleaved(self, substeps, rounds):
         abi.check(self.lib.avbd_world_step_interleaved(self.h, int(substeps), int(rounds)))
 
@@ -344,6 +348,9 @@ class Batch:
 
     def use_graph(self, enable):
         abi.check(self.lib.avbd_batch_use_graph(self.h, 1 if enable else 0))
+
+    def use_query_tree(self, enable):
+        abi.check(self.lib.avbd_batch_use_query_tree(self.h, 1 if enable else 0))
 
     def download(self, wid):
         n = self.sizes[wid]

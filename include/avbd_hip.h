@@ -372,9 +372,10 @@ avbd_status avbd_world_collide_pair(avbd_world *world, int32_t body_a, int32_t b
 #define AVBD_NOALLOC_SCRATCH_BYTES 4145152
 avbd_status avbd_world_raycast_noalloc(avbd_world *world, int32_t n, const float *from, const float *to,
                                        const uint32_t *masks, int32_t hit_from_inside, avbd_ray_hits *out);
-/* Measurement (no reference counterpart): average device time of the bounds kernel and of
- * the ray kernel of avbd_world_raycast over `reps` launches of n segments (all layers, no
- * hit from inside), timed with HIP events on the library's stream. */
+/* Measurement (no reference counterpart): average device time of the bounds kernel with the
+ * query tree's refit (bounds_ms) and of the ray kernel (kernel_ms) of avbd_world_raycast over
+ * `reps` launches of n segments (all layers, no hit from inside), timed with HIP events on the
+ * library's stream. */
 typedef struct avbd_query_profile {
     float bounds_ms;
     float kernel_ms;
@@ -482,6 +483,13 @@ avbd_status avbd_batch_use_graph(avbd_batch *batch, int32_t enable);
 /* The same choice for a plain world: its step is one captured HIP graph per (substeps, rounds,
  * tree rebuild) by default; enable = 0 launches the kernels one by one. */
 avbd_status avbd_world_use_graph(avbd_world *world, int32_t enable);
+/* Space queries (ray, point, shape, rest info, motion cast) walk the last step's LBVH refit to
+ * the bodies' current tight bounds, from the first step after the world set last changed on;
+ * enable = 0 makes them scan the body list instead.  The results are the same either way: the
+ * tree only skips bodies whose bounds cannot overlap the query's, and candidates are still taken
+ * in list order.  No reference counterpart (the reference's server scans the list). */
+avbd_status avbd_world_use_query_tree(avbd_world *world, int32_t enable);
+avbd_status avbd_batch_use_query_tree(avbd_batch *batch, int32_t enable);
 /* One step of every world in the batch.  AVBD_ERR_WORLD: every world stepped except the ones
  * avbd_batch_failed_worlds lists, whose state this step is not valid (remove them; until then
  * they keep failing). */

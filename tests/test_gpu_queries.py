@@ -260,3 +260,109 @@ def test_closest_point_matches_oracle():
     pts = (pos[idx] + rng.normal(0, 1.0, (4000, 3)) * rng.choice([0.05, 1.0, 3.0], (4000, 1))).astype(np.float32)
     assert bits_equal(w.closest_point(idx, pts), o.closest_point(idx, pts))
     w.close()
+
+
+def _all_queries(q, rng_seed, pos, wide):
+    """Every space query of q (a World or the oracle) on one seeded set of inputs; `wide` adds
+    query boxes that span most of the pile (hundreds of candidates each)."""
+    rng = np.random.default_rng(rng_seed)
+    frm, to = _rays(rng, 1500, pos[:30])
+    out = {"ray": q.raycast(frm, to, hit_from_inside=True), "ray_out": q.raycast(frm, to)}
+    pts = (pos[rng.integers(0, len(pos), 600)] + rng.normal(0, 0.4, (600, 3))).astype(np.float32)
+    out["point"] = q.intersect_point(pts, max_results=3)
+    t, c, r, s, m, masks = _random_shapes(rng, pos, 300)
+    if wide:
+        s[:40] = rng.uniform(8.0, 20.0, (40, 3))
+        c[:40] = pos.mean(axis=0)
+    out["shape"] = q.intersect_shape(t, c, r, s, m, masks, max_results=80 if wide else 6)
+    out["collide"] = q.collide_shape(t, c, r, s, m, masks, max_pairs=120 if wide else 8)
+    out["rest"] = q.rest_info(t, c, r, s, m, masks)
+    mv = rng.normal(0, 3.0, (300, 3)).astype(np.float32)
+    out["cast"] = q.cast_motion(t, c, r, s, mv, m, masks)
+    return out
+
+
+def _same(key, a, b):
+    """Bit equality of one query's results; collide_shape points past each count and rest_info
+    fields of queries without a hit are unspecified."""
+    if key == "collide":
+        return (bits_equal(a[0], b[0]) and bits_equal(a[1], b[1])
+                and all(bits_equal(a[2][q, :a[0][q]], b[2][q, :b[0][q]]) for q in range(len(a[0]))))
+    if key == "rest":
+        hit = a[0] >= 0
+        return bits_equal(a[0], b[0]) and all(bits_equal(x[hit], y[hit]) for x, y in zip(a[1:], b[1:]))
+    if isinstance(a, dict):
+        return all(bits_equal(a[k], b[k]) for k in a)
+    return all(bits_equal(x, y) for x, y in zip(a, b))
+
+
+def test_query_tree_equals_list_scan_and_oracle():
+    """Queries walk the step's LBVH refit to the current bounds: the same bits as the list scan
+    and as the oracle, also after teleports that leave the tree's topology far from the bodies,
+    for query boxes with hundreds of candidates (several walks of 32), and on a fresh upload of
+    the same state (no tree yet: the scan)."""
+    b, w, o = _settled_round_pile(300, 45)
+    n = len(b["mass"])
+    st = w.download()
+    moved = {k: st[k].copy() for k in ("position", "rotation", "velocity", "angular_velocity", "prev_velocity")}
+    moved["position"][::7] += np.float32(6.0)          # every 7th body jumps: bounds move, topology stays
+    moved["position"][3::11, 2] += np.float32(9.0)
+    w.write_bodies(moved)
+    o.write_bodies(moved)
+    pos = moved["position"]
+    for wide in (False, True):
+        tree = _all_queries(w, 40 + wide, pos, wide)
+        w.use_query_tree(False)
+        scan = _all_queries(w, 40 + wide, pos, wide)
+        w.use_query_tree(True)
+        ref = _all_queries(o, 40 + wide, pos, wide)
+        for k in tree:
+            assert _same(k, tree[k], scan[k]), k
+            assert _same(k, tree[k], ref[k]), k
+    assert int((tree["shape"][0] >= 32).sum()) >= 20      # boxes that took several walks
+    fresh = pkg.World(pkg.default_params(iterations=10), n, 0)
+    fresh.upload({**b, **moved}, None)
+    again = _all_queries(fresh, 41, pos, True)
+    for k in again:
+        assert _same(k, again[k], tree[k]), k
+    fresh.close()
+    w.close()
+
+
+def test_batch_queries_through_the_tree_after_world_changes():
+    """A batch world's queries walk its own subtree (the other worlds' leaves are empty); after
+    another world is removed the next query scans, the step after walks again: all equal to
+    the plain world's."""
+    b, j = scenes.round_pile(150, seed=29)
+    p = P(iterations=10)
+    batch = pkg.Batch(p, 4, 3000)
+    first = batch.add(*scenes.pile(300, seed=4))
+    wid = batch.add(b, j)
+    last = batch.add(*scenes.round_pile(90, seed=8))
+    w = pkg.World(p, len(b["mass"]), 0)
+    w.upload(b, j)
+    for _ in range(30):
+        batch.step()
+        w.step()
+    rng = np.random.default_rng(31)
+    frm, to = _rays(rng, 1200)
+    ref = w.raycast(frm, to)
+
+    def check():
+        g = batch.raycast(wid, frm, to)
+        assert np.array_equal(g["body"], ref["body"]) and bits_equal(g["t"], ref["t"])
+        assert bits_equal(g["normal"], ref["normal"])
+
+    check()
+    batch.remove(first)
+    check()
+    batch.use_query_tree(False)
+    check()
+    batch.use_query_tree(True)
+    batch.step()
+    w.step()
+    ref = w.raycast(frm, to)
+    check()
+    assert (ref["body"] >= 0).sum() > 300 and last != wid
+    batch.close()
+    w.close()
