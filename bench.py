@@ -129,6 +129,8 @@ def bench_chains(args, log):
     bodies, joints = pkg.scenes.chains(n_chains=4096, links=32)
     w = pkg.World(params, len(bodies["mass"]), len(joints["body_b"]))
     w.upload(bodies, joints)
+    if args.no_graph:
+        w.use_graph(False)
     for _ in range(args.c3_warmup):
         w.step()
     tuning = w.autotune(args.tune_steps) if args.tune_steps > 0 else None
@@ -138,12 +140,14 @@ def bench_chains(args, log):
     dt = (time.perf_counter() - a) / args.c3_steps
     movable = int((bodies["mass"] > 0).sum())
     st = w.stats()
+    lists = w.list_stats()
     w.close()
     log(f"C3 chains: {dt * 1e3:.3f} ms/step, {movable / dt / 1e6:.2f} M body-steps/s ({time.time() - t0:.1f}s)")
     return {"workload": "C3 plain world: 4096 chains x 32 unit boxes, ball joints, static anchors, ground; "
                         "beta_lin 1e5, k cap 1e9", "bodies": len(bodies["mass"]), "movable_bodies": movable,
             "joints": len(joints["body_b"]), "manifolds": st["manifolds"], "contacts": st["contacts"],
             "warmup": args.c3_warmup, "steps": args.c3_steps, "ms_per_step": dt * 1e3, "tuning": tuning,
+            "broadphase_lists": lists,
             "value": movable / dt, "unit": "body-steps/s"}
 
 
@@ -294,7 +298,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16,
                     help="threads of the CPU baseline (the host share of one GPU)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one (for tracing tools)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch kernels one by one, batch and plain worlds (for tracing tools)")
     ap.add_argument("--tune-steps", type=int, default=2, help="autotuner steps per trial (0: keep defaults)")
     ap.add_argument("--no-c3", action="store_true", help="skip the C3 chains world")
     ap.add_argument("--c3-warmup", type=int, default=240)
@@ -364,9 +369,10 @@ def main():
         lat.append((time.perf_counter() - a) * 1e3)
     elapsed = time.perf_counter() - t_start
     tot = batch.totals()
+    lists = batch.list_stats()
     value = movable * args.steps / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
-    log(f"batched: {ms_per_step:.3f} ms/step, {value / 1e6:.1f} M body-steps/s, totals {tot}")
+    log(f"batched: {ms_per_step:.3f} ms/step, {value / 1e6:.1f} M body-steps/s, totals {tot}, lists {lists}")
 
     # ---- per-kernel durations (HIP events on the batch's stream) for the roofline
     prof = batch.profile(args.profile_steps)
@@ -401,6 +407,8 @@ def main():
         pw = pkg.World(params, len(b["mass"]), 0)
         pw.upload(start, j)
         pw.set_manifolds(m)
+        if args.no_graph:
+            pw.use_graph(False)
         pw.step()                   # first call pays one-time setup
         a = time.perf_counter()
         pw.step(args.plain_steps)
@@ -523,6 +531,7 @@ def main():
         "kernel_launches_per_step": {k: v / max(1, prof["steps"]) for k, v in prof["launches"].items()},
         "cpu_baseline": cpu,
         "tuning": tuning,
+        "broadphase_lists": lists,
         "c3": c3,
         "serving": serving,
     }

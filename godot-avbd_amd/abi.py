@@ -305,6 +305,8 @@ def load_library(path=LIB_PATH):
         "avbd_batch_world_count": (C.c_int32, [vp, i32p]),
         "avbd_batch_get_manifolds": (C.c_int32, [vp, C.c_int32, C.POINTER(Manifolds)]),
         "avbd_batch_set_manifolds": (C.c_int32, [vp, C.c_int32, C.POINTER(Manifolds)]),
+        "avbd_batch_list_stats": (C.c_int32, [vp, i32p]),
+        "avbd_world_list_stats": (C.c_int32, [vp, i32p]),
         "avbd_batch_totals": (C.c_int32, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                           C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     }

@@ -314,7 +314,7 @@ struct Engine {
         Q.nodeBox = qtBox;
         Q.nodeFlag = qtFlag;
         k_refit_clear<<<nblk(Ncap), TPB, 0, stream>>>(Q, Ncap);
-        k_refit_tasks<<<refitGrid(), REFIT_TPB, 0, stream>>>(Q, Ncap);
+        k_refit_tasks<<<refitGrid(), REFIT_TPB, 0, stream>>>(Q, Ncap, 0);
         HIPCHK(hipGetLastError());
         t.box = qtBox;
         return AVBD_OK;
@@ -907,6 +907,8 @@ struct Engine {
         AL(S.pairCount, N + 1); AL(S.pairOff, N + 1); AL(S.pairI, P); AL(S.pairJ, P);
         AL(S.pairValid, P + 1); AL(S.pairValidScan, P + 1);
         AL(S.pairScratch, N * PAIR_BUF); AL(S.wRoot, W);
+        AL(S.listC, N); AL(S.listH, N); AL(S.listJ, N * LIST_K); AL(S.listCnt, N); AL(S.listAdd, N); AL(S.listFlag, N);
+        AL(S.wRewalk, W); AL(S.wRefit, W); AL(S.wMoved, W); AL(S.wStale, W); AL(S.listStats, 4);
         AL(S.wBoundsLo, W); AL(S.wBoundsHi, W);
         AL(S.gOfK, M); AL(S.kSlot, M); AL(S.kT, M); AL(S.sPos, M); AL(S.newT, P); AL(S.mHdr, M); AL(S.mBasisK, 3 * M); AL(S.mFricK, M); AL(S.brakeOn, M); AL(S.wOrder, W);
         radixBlocks = (Ncap + RADIX_TILE - 1) / RADIX_TILE;
@@ -1050,6 +1052,21 @@ struct Engine {
         k_task_list<<<nblk(2 * Ncap - 1), TPB, 0, stream>>>(S, Ncap);
     }
     int refitGrid() const { return std::max(1, std::min(Wcap + 2 * Ncap / REFIT_LEAVES + 8, 8192)); }
+    avbd_status listStats(int32_t *out)
+    {
+        HIPCHK(hipMemcpyAsync(out, S.listStats, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return AVBD_OK;
+    }
+
+    // The state as the step's refit sees it: leaves are the candidate lists' grown boxes.
+    State listTree() const
+    {
+        State T = S;
+        T.aabbC = S.listC;
+        T.aabbH = S.listH;
+        return T;
+    }
 
     // One velocity brake at the current poses: flags, dependency links, level-by-level
     // brake per world (solver_projection.cpp:96-167).
@@ -1092,6 +1109,8 @@ struct Engine {
             buildAdjacency();
         }
         // ---- broad phase (solver_broadphase.cpp:85-186)
+        // every world lists again at a tree rebuild, dropping entries that no longer meet
+        if (rebuildTree) k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wRewalk, 1, Wcap);
         k_swept_aabb<<<nb, TPB, 0, stream>>>(S);
         if (timed) phaseMark(5);
         if (rebuildTree) {
@@ -1110,12 +1129,15 @@ struct Engine {
             k_refit_clear<<<nblk(Ncap), TPB, 0, stream>>>(S, Ncap);
         }
         pbeg(AVBD_K_REFIT);
-        k_refit_tasks<<<refitGrid(), REFIT_TPB, 0, stream>>>(S, Ncap);
+        k_refit_tasks<<<refitGrid(), REFIT_TPB, 0, stream>>>(listTree(), Ncap, rebuildTree ? 0 : 1);
         pend();
         if (timed) phaseMark(6);
         pbeg(AVBD_K_TRAVERSE);
+        k_list_walk<<<nblk(Ncap, tune.traverse), tune.traverse, 0, stream>>>(S, Ncap);
         k_traverse<<<nblk(Ncap, tune.traverse), tune.traverse, 0, stream>>>(S, Ncap);
         pend();
+        k_fill_i32<<<1, 64, 0, stream>>>(S.listStats, 0, 4);
+        k_list_world_end<<<nblk(Wcap), TPB, 0, stream>>>(S);
         scan(S.pairCount, S.pairOff, Ncap);
         k_pair_emit<<<nb, TPB, 0, stream>>>(S, Ncap);
         if (timed) phaseMark(7);
@@ -1697,9 +1719,11 @@ struct Engine {
         const int tab[6] = {wi.bodyBase, n, wi.jointBase, nj, wi.slotBase, slotCap};
         int *dst[6] = {S.wBodyBase, S.wBodyCount, S.wJointBase, S.wJointCount, S.wSlotBase, S.wSlotCap};
         for (int k = 0; k < 6; k++) HIPCHK(hipMemcpyAsync(dst[k] + w, &tab[k], sizeof(int), hipMemcpyHostToDevice, stream));
-        const int zero = 0;
+        const int zero = 0, one = 1;
         int *zs[5] = {S.wLive, S.wFree, S.wTop, S.wNew, S.wErr};
         for (int k = 0; k < 5; k++) HIPCHK(hipMemcpyAsync(zs[k] + w, &zero, sizeof(int), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(S.wRewalk + w, &one, sizeof(int), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(S.wStale + w, &zero, sizeof(int), hipMemcpyHostToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));
         wi.active = true;
         csrValid = false;
@@ -1979,6 +2003,8 @@ struct Engine {
         }
         csrValid = true;
         const int nb = nblk(Ncap);
+        k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wRewalk, 1, Wcap);
+        k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wStale, 0, Wcap);
         k_swept_aabb<<<nb, TPB, 0, stream>>>(S);
         k_world_bounds<<<Wcap, TPB, 0, stream>>>(S);
         k_morton<<<nb, TPB, 0, stream>>>(S, 1);
@@ -1989,8 +2015,11 @@ struct Engine {
         k_world_roots_check<<<nblk(Wcap), TPB, 0, stream>>>(S);
         k_ropes<<<nblk(2 * Ncap - 1), TPB, 0, stream>>>(S, Ncap);
         buildRefitTasks();
-        k_refit_tasks<<<refitGrid(), REFIT_TPB, 0, stream>>>(S, Ncap);
+        k_refit_tasks<<<refitGrid(), REFIT_TPB, 0, stream>>>(listTree(), Ncap, 0);
+        k_list_walk<<<nb, TPB, 0, stream>>>(S, Ncap);
         k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap);
+        k_fill_i32<<<1, 64, 0, stream>>>(S.listStats, 0, 4);
+        k_list_world_end<<<nblk(Wcap), TPB, 0, stream>>>(S);
         scan(S.pairCount, S.pairOff, Ncap);
         k_pair_emit<<<nb, TPB, 0, stream>>>(S, Ncap);
         HIPCHK(hipGetLastError());
@@ -2824,6 +2853,18 @@ avbd_status avbd_batch_totals(avbd_batch *batch, int64_t *movable_bodies, int64_
 {
     if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
     return batch->eng.totals(movable_bodies, manifolds, contacts, joints);
+}
+
+avbd_status avbd_batch_list_stats(avbd_batch *batch, int32_t out[4])
+{
+    if (!batch || !out) return fail(AVBD_ERR_INVALID, "null argument");
+    return batch->eng.listStats(out);
+}
+
+avbd_status avbd_world_list_stats(avbd_world *world, int32_t out[4])
+{
+    if (!world || !out) return fail(AVBD_ERR_INVALID, "null argument");
+    return world->eng.listStats(out);
 }
 
 } // extern "C"

@@ -213,13 +213,18 @@ __global__ void k_radix_scatter(const unsigned long long *kin, const int *vin, u
 // ------------------------------------------------------------------ broad phase ----
 // solver_broadphase.cpp:101-122 + rigid.cpp:83-106: swept AABB of every body.  Bodies
 // outside any world get an empty box so they never overlap.
+constexpr float LIST_GROW = 0.1f;
 __global__ void k_swept_aabb(State S)
 {
     const int i = GID;
     if (i >= S.numBodies) return;
-    if (S.bodyWorld[i] < 0) {
+    const int w = S.bodyWorld[i];
+    if (w < 0) {
         S.aabbC[i] = make_float4(0, 0, 0, 0);
         S.aabbH[i] = make_float4(-1, -1, -1, 0);
+        S.listC[i] = make_float4(0, 0, 0, 0);
+        S.listH[i] = make_float4(-1, -1, -1, 0);
+        S.listFlag[i] = 0;
         return;
     }
     const float4 sr = S.sizeR[i];
@@ -239,8 +244,44 @@ __global__ void k_swept_aabb(State S)
     const float rotSweep = sr.w * length(ld3(w4)) * cdt;
     const float grow = rotSweep + S.P.specMargin;
     const float4 p4 = S.pos[i], v4 = S.vel[i];
-    S.aabbC[i] = st3(ld3(p4) + disp * 0.5f);
-    S.aabbH[i] = st3((tight + fabs3(disp) * 0.5f) + v3{grow, grow, grow});
+    const v3 ac = ld3(p4) + disp * 0.5f, ah = (tight + fabs3(disp) * 0.5f) + v3{grow, grow, grow};
+    S.aabbC[i] = st3(ac);
+    S.aabbH[i] = st3(ah);
+    // A box that left the grown box of its candidate list is grown again (by LIST_GROW of its
+    // smallest half extent plus twice this step's displacement) and walks.  While every box
+    // stays inside its grown box, two boxes can only overlap if their grown boxes do (the same
+    // float comparisons), and those pairs are what the lists hold.  A stale world's boxes are
+    // not grown (its tree holds the exact boxes).  Where the last boxes were not grown ones
+    // (a stale world, or one listing again), a body counts as moved when this step's
+    // displacement by its velocity exceeds half the base growth, LIST_GROW of its smallest half
+    // extent.
+    const float g0 = LIST_GROW * smin(smin(ah.x, ah.y), ah.z);
+    const v3 vdisp = ld3(S.vel[i]) * cdt;   // the velocity's part of the displacement
+    const float step = smax(smax(fabsf(vdisp.x), fabsf(vdisp.y)), fabsf(vdisp.z));
+    const float reach = smax(smax(fabsf(disp.x), fabsf(disp.y)), fabsf(disp.z));
+    const bool stale = S.wStale[w] != 0, listing = stale || S.wRewalk[w];
+    const v3 ec = ld3(S.listC[i]), eh = ld3(S.listH[i]);
+    const v3 alo = ac - ah, ahi = ac + ah, elo = ec - eh, ehi = ec + eh;
+    const bool inside = alo.x >= elo.x && alo.y >= elo.y && alo.z >= elo.z && ahi.x <= ehi.x && ahi.y <= ehi.y &&
+                        ahi.z <= ehi.z;
+    const bool moved = listing ? step > 0.5f * g0 : !inside;
+    const bool regrow = listing || !inside;
+    if (regrow) {
+        const float g = stale ? 0.0f : g0 + 2.0f * reach;
+        S.listC[i] = st3(ac);
+        S.listH[i] = st3(ah + v3{g, g, g});
+        S.wRefit[w] = 1;
+    }
+    S.listFlag[i] = regrow ? 2 : 0;
+    // moved bodies per world: one atomic per world present among the wavefront's moved lanes
+    unsigned long long pending = __ballot(moved);
+    while (pending) {
+        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+        const int wl = __shfl(w, leader);
+        const unsigned long long same = __ballot(moved && w == wl);
+        if (static_cast<int>(__lane_id()) == leader) atomicAdd(S.wMoved + wl, static_cast<int>(__popcll(same)));
+        pending &= ~same;
+    }
     // the narrow phase's view of the body, one aligned line instead of eight arrays
     float4 *r = S.narrowRec + NARROW_REC * static_cast<size_t>(i);
     r[0] = make_float4(p4.x, p4.y, p4.z, bMass(S, i));
@@ -373,7 +414,7 @@ __global__ void k_karras(State S, int n)
 
 // Refit tasks: a world's tree is cut into subtrees of at most REFIT_LEAVES leaves (the task
 // roots: nodes of that size whose parent is larger, or the world's root), chosen at each tree
-// build.  One workgroup refits a task's subtree bottom up: leaf boxes (centre -/+ half, the
+// build.  The step's tree holds the grown boxes of the candidate lists (listC/listH).  One workgroup refits a task's subtree bottom up: leaf boxes (centre -/+ half, the
 // reference's bounds), and the second child to arrive at a parent writes the union and climbs
 // on; arrivals are counted with workgroup-scope atomics and ordered with workgroup-scope
 // fences (a device-scope fence per thread would write back the L2 of every XCD).  Above the
@@ -419,12 +460,14 @@ __device__ __forceinline__ void unionInto(const State &S, int parent)
                                             __int_as_float(S.nodeRope[parent]));
 }
 
-__global__ void __launch_bounds__(REFIT_TPB) k_refit_tasks(State S, int n)
+__global__ void __launch_bounds__(REFIT_TPB) k_refit_tasks(State S, int n, int onlyRewalked)
 {
     const int nTasks = static_cast<int>(S.taskScan[2 * n - 1]);
     for (int t = blockIdx.x; t < nTasks; t += gridDim.x) {
         const int root = S.taskNode[t];
         const int lo = root >= n - 1 ? root - (n - 1) : S.nodeFirst[root];
+        // a world whose grown boxes did not change keeps its node boxes (same topology)
+        if (onlyRewalked && !S.wRefit[S.bodyWorld[S.sortVal[lo]]]) continue;
         const int cnt = nodeLeaves(S, n, root);
         for (int p = lo + threadIdx.x; p < lo + cnt; p += REFIT_TPB) {
             const int body = S.sortVal[p];
@@ -527,17 +570,15 @@ __global__ void k_ropes(State S, int n)
     S.nodeRope[node] = rope;
 }
 
-// Walk body i's world subtree without a stack (each node carries its rope); `visit(j)`
-// for every j > i whose swept AABB overlaps i's, in left-first depth-first order.
-// A walk visits each node at most once; one that does not end within 2n steps (a corrupt
-// tree) stops and returns false.
-template <typename F>
-__device__ __forceinline__ bool traverseWorld(const State &S, int n, int i, int w, F &&visit)
+// Walk body i's world subtree without a stack (each node carries its rope); `visit(j)` for
+// every j > i (Dir 1), j < i (Dir -1) or j != i (Dir 0) whose leaf box overlaps [qMin, qMax],
+// in left-first depth-first order.  A walk visits each node at most once; one that does not
+// end within 2n steps (a corrupt tree) stops and returns false.
+template <int Dir = 1, typename F>
+__device__ __forceinline__ bool walkWorld(const State &S, int n, int i, int w, v3 qMin, v3 qMax, F &&visit)
 {
     const int root = S.wRoot[w];
     if (root < 0) return true;   // flagged by k_world_roots_check
-    const v3 c = ld3(S.aabbC[i]), h = ld3(S.aabbH[i]);
-    const v3 qMin = c - h, qMax = c + h;
     int node = root;
     for (int guard = 2 * n; node >= 0; guard--) {
         if (guard == 0) return false;
@@ -545,7 +586,7 @@ __device__ __forceinline__ bool traverseWorld(const State &S, int n, int i, int 
         const bool ov = aabbOverlap(qMin, qMax, ld3(lo), ld3(hi));
         const int down = __float_as_int(lo.w), rope = __float_as_int(hi.w);
         if (node >= n - 1) {
-            if (ov && down > i) visit(down);
+            if (ov && (Dir > 0 ? down > i : Dir < 0 ? down < i : down != i)) visit(down);
             node = rope;
         } else {
             node = ov ? down : rope;
@@ -554,25 +595,156 @@ __device__ __forceinline__ bool traverseWorld(const State &S, int n, int i, int 
     return true;
 }
 
-// Candidate pairs of body i: every j > i of the same world whose swept AABB overlaps.
-// One traversal counts them and keeps the first PAIR_BUF; the emit pass copies those (or,
-// for a body with more, walks again) into pairJ at pairOff[i], sorted by j.
-__global__ void __launch_bounds__(TPB) k_traverse(State S, int n)
+// The reference's pair test on the swept boxes (closed intervals, solver_broadphase.cpp:35-40).
+__device__ __forceinline__ bool sweptOverlap(const State &S, v3 qMin, v3 qMax, int j)
 {
-    // queries in Morton order: neighbouring lanes walk neighbouring paths
-    const int p = XGID;
-    if (p >= S.numBodies) return;
-    const int i = S.sortVal[p];
-    const int w = S.bodyWorld[i];
-    if (w < 0 || n < 2) { S.pairCount[i] = 0; return; }
+    const v3 c = ld3(S.aabbC[j]), h = ld3(S.aabbH[j]);
+    return aabbOverlap(qMin, qMax, c - h, c + h);
+}
+
+constexpr int LIST_K = PAIR_BUF;
+
+// Over half of a world's bodies moved out of their grown boxes: it is busy.  This step every
+// body of it lists again (appends would cost more); from the next step on, until it calms
+// down, it is stale.
+__device__ __forceinline__ bool worldBusy(const State &S, int w)
+{
+    return S.wMoved[w] * 2 > S.wBodyCount[w];
+}
+
+// Body i's swept-box pairs (j > i) by walking the tree; the count and the first PAIR_BUF
+// to pairScratch.  `exactLeaves`: the leaves are the swept boxes themselves.
+__device__ __forceinline__ void walkPairs(const State &S, int n, int i, int w, v3 qMin, v3 qMax, bool exactLeaves)
+{
     uint32_t count = 0;
     int *buf = S.pairScratch + static_cast<size_t>(i) * PAIR_BUF;
-    const bool ok = traverseWorld(S, n, i, w, [&](int j) {
+    const bool ok = walkWorld(S, n, i, w, qMin, qMax, [&](int j) {
+        if (!exactLeaves && !sweptOverlap(S, qMin, qMax, j)) return;
         if (count < PAIR_BUF) buf[count] = j;
         count++;
     });
     if (!ok) atomicOr(S.wErr + w, 4);
     S.pairCount[i] = count;
+}
+
+// The walks of a step, one kernel so that they wait on each other's latency only once:
+//  - a stale world's bodies walk the exact boxes for their pairs;
+//  - in a world that lists again (new, calmed down, busy, or at a tree rebuild) every body lists
+//    the j > i whose grown boxes meet its own;
+//  - elsewhere a body whose grown box changed lists again, and adds itself to the list of each
+//    lower body whose grown box meets its new one and that keeps its list (unless already there:
+//    the lists' first listCnt entries do not change in this kernel, appends go after them);
+//  - a body with more than LIST_K candidates walks for its pairs.
+// Bodies whose pairs are done here get listFlag 3.
+__global__ void __launch_bounds__(TPB) k_list_walk(State S, int n)
+{
+    const int p = XGID;
+    if (p >= S.numBodies || n < 2) return;
+    const int i = S.sortVal[p];
+    const int w = S.bodyWorld[i];
+    if (w < 0) return;
+    const v3 c = ld3(S.aabbC[i]), h = ld3(S.aabbH[i]);
+    if (S.wStale[w]) {
+        walkPairs(S, n, i, w, c - h, c + h, true);
+        S.listFlag[i] = 3;
+        return;
+    }
+    const bool all = S.wRewalk[w] || worldBusy(S, w);
+    int lc;
+    if (all || S.listFlag[i] == 2) {
+        const v3 ec = ld3(S.listC[i]), eh = ld3(S.listH[i]);
+        int v[LIST_K];
+        lc = 0;
+        auto own = [&](int j) {
+            if (lc < LIST_K) {
+                int b = lc;
+                while (b > 0 && v[b - 1] > j) { v[b] = v[b - 1]; b--; }
+                v[b] = j;
+            }
+            lc++;
+        };
+        bool ok;
+        if (all) {
+            ok = walkWorld(S, n, i, w, ec - eh, ec + eh, own);
+        } else {
+            ok = walkWorld<0>(S, n, i, w, ec - eh, ec + eh, [&](int j) {
+                if (j > i) { own(j); return; }
+                if (S.listFlag[j] != 0) return;   // lists again itself
+                const int base = S.listCnt[j];
+                if (base > LIST_K) return;          // walks for its pairs
+                const int *lj = S.listJ + static_cast<size_t>(j) * LIST_K;
+                for (int k = 0; k < base; k++)
+                    if (lj[k] == i) return;
+                const int slot = base + atomicAdd(S.listAdd + j, 1);
+                if (slot < LIST_K) S.listJ[static_cast<size_t>(j) * LIST_K + slot] = i;
+            });
+        }
+        if (!ok) atomicOr(S.wErr + w, 4);
+        int *list = S.listJ + static_cast<size_t>(i) * LIST_K;
+        if (lc <= LIST_K)
+            for (int k = 0; k < lc; k++) list[k] = v[k];
+        S.listCnt[i] = lc;
+    } else {
+        lc = S.listCnt[i];
+    }
+    if (lc > LIST_K) {
+        walkPairs(S, n, i, w, c - h, c + h, false);
+        S.listFlag[i] = 3;
+    }
+}
+
+// Candidate pairs of body i: every j > i of the same world whose swept box overlaps i's, from
+// its list (with this step's appends; a list that overflowed with them walks), unless the walk
+// kernel made them.  The count and the first PAIR_BUF pairs go to pairScratch; the emit pass
+// copies those (or, for a body with more, walks again) into pairJ at pairOff[i], sorted by j.
+__global__ void __launch_bounds__(TPB) k_traverse(State S, int n)
+{
+    const int p = XGID;
+    if (p >= S.numBodies) return;
+    const int i = S.sortVal[p];
+    const int w = S.bodyWorld[i];
+    if (w < 0 || n < 2) { S.pairCount[i] = 0; return; }
+    if (S.listFlag[i] == 3) return;
+    int lc = S.listCnt[i];
+    const int add = S.listAdd[i];
+    if (add) {
+        lc += add;
+        S.listAdd[i] = 0;
+        S.listCnt[i] = lc;
+    }
+    const v3 c = ld3(S.aabbC[i]), h = ld3(S.aabbH[i]);
+    const v3 qMin = c - h, qMax = c + h;
+    if (lc > LIST_K) {
+        walkPairs(S, n, i, w, qMin, qMax, false);
+        return;
+    }
+    uint32_t count = 0;
+    int *buf = S.pairScratch + static_cast<size_t>(i) * PAIR_BUF;
+    const int *list = S.listJ + static_cast<size_t>(i) * LIST_K;
+    for (int k = 0; k < lc; k++) {
+        const int j = list[k];
+        if (sweptOverlap(S, qMin, qMax, j)) buf[count++] = j;
+    }
+    S.pairCount[i] = count;
+}
+
+// Per world after the walks: busy now = stale next step; a stale world that calmed down lists
+// again; counters for the next step cleared.  listStats (cleared by the host's fill) gathers
+// the step's figures.
+__global__ void k_list_world_end(State S)
+{
+    const int w = GID;
+    if (w >= S.numWorlds) return;
+    const bool busy = worldBusy(S, w), wasStale = S.wStale[w] != 0;
+    const int moved = S.wMoved[w];
+    if (moved) atomicAdd(S.listStats, moved);
+    if (busy) atomicAdd(S.listStats + 1, 1);
+    if (wasStale) atomicAdd(S.listStats + 2, 1);
+    if (S.wRewalk[w] && !wasStale) atomicAdd(S.listStats + 3, 1);
+    S.wRewalk[w] = wasStale && !busy ? 1 : 0;
+    S.wStale[w] = busy ? 1 : 0;
+    S.wMoved[w] = 0;
+    S.wRefit[w] = 0;
 }
 
 __global__ void __launch_bounds__(TPB) k_pair_emit(State S, int n)
@@ -598,9 +770,11 @@ __global__ void __launch_bounds__(TPB) k_pair_emit(State S, int n)
         }
         return;
     }
+    const v3 c = ld3(S.aabbC[i]), h = ld3(S.aabbH[i]);
+    const v3 qMin = c - h, qMax = c + h;
     uint32_t out = begin;
-    traverseWorld(S, n, i, S.bodyWorld[i], [&](int j) {
-        if (out < end) S.pairJ[out++] = j;
+    walkWorld(S, n, i, S.bodyWorld[i], qMin, qMax, [&](int j) {
+        if (out < end && sweptOverlap(S, qMin, qMax, j)) S.pairJ[out++] = j;
     });
     for (uint32_t a = begin + 1; a < out; a++) {
         const int v = S.pairJ[a];

@@ -180,6 +180,21 @@ struct State {
     uint32_t *pairCount, *pairOff;
     int *pairI, *pairJ;
     int *pairScratch;            // PAIR_BUF candidates per body from the single traversal
+    // Candidate lists: each body's box grown by a margin (listC/listH, centre and half) is what
+    // the tree's leaves hold; listJ keeps the LIST_K lowest j > i of the same world whose grown
+    // box meets i's (sorted), listCnt their number (more than LIST_K: walk the tree each step).
+    // listFlag per body this step: 2 = its box left its grown box (grown again, walks its list
+    // and marks), 1 = marked by such a body (walks its list), 0 = keeps its list.
+    // Per world: wRewalk = every body walks and lists again (a new world, or one that calmed
+    // down); wStale = it was busy last step, so its boxes are not grown and its bodies walk the
+    // exact boxes without lists; wMoved = bodies that moved out of their grown boxes this step
+    // (busy: over half); wRefit = some grown box changed.  listStats (last step): bodies moved,
+    // busy worlds, stale worlds, worlds that listed again.
+    // listAdd: entries appended to a list this step (after its first listCnt).
+    float4 *listC, *listH;
+    int *listJ, *listCnt, *listAdd, *listFlag;
+    int *wRewalk, *wRefit, *wMoved, *wStale;
+    int *listStats;
     int *wRoot;                  // per world: node whose leaf range is exactly the world
     uint32_t *pairValid, *pairValidScan;
     float4 *wBoundsLo, *wBoundsHi;

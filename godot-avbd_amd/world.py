@@ -66,6 +66,12 @@ class World:
         """Step as one captured HIP graph (default) or kernel by kernel (for tracing tools)."""
         abi.check(self.lib.avbd_world_use_graph(self.h, 1 if enable else 0))
 
+    def list_stats(self):
+        """The broad phase's candidate lists in the last step."""
+        out = np.zeros(4, np.int32)
+        abi.check(self.lib.avbd_world_list_stats(self.h, out.ctypes.data_as(abi.i32p)))
+        return dict(zip(("moved_bodies", "busy_worlds", "exact_walk_worlds", "relisted_worlds"), out.tolist()))
+
     def use_query_tree(self, enable):
         """Space queries walk the refit step tree (default) or scan the body list; same results."""
         abi.check(self.lib.avbd_world_use_query_tree(self.h, 1 if enable else 0))
@@ -401,6 +407,12 @@ class Batch:
         out = abi.KernelProfile()
         abi.check(self.lib.avbd_batch_profile(self.h, int(steps), C.byref(out)))
         return out.as_dict()
+
+    def list_stats(self):
+        """The broad phase's candidate lists in the last step."""
+        out = np.zeros(4, np.int32)
+        abi.check(self.lib.avbd_batch_list_stats(self.h, out.ctypes.data_as(abi.i32p)))
+        return dict(zip(("moved_bodies", "busy_worlds", "exact_walk_worlds", "relisted_worlds"), out.tolist()))
 
     def totals(self):
         v = [C.c_int64() for _ in range(4)]

@@ -517,6 +517,14 @@ avbd_status avbd_batch_profile(avbd_batch *batch, int32_t steps, avbd_kernel_pro
 /* Totals over the batch after the last step: bodies (movable), manifolds, contacts. */
 avbd_status avbd_batch_totals(avbd_batch *batch, int64_t *movable_bodies, int64_t *manifolds,
                               int64_t *contacts, int64_t *joints);
+/* Measurement (no reference counterpart): the broad phase's candidate lists in the last step.
+ * Each body lists the bodies whose boxes, grown by a margin, meet its own, and walks the tree
+ * again only when its box leaves its grown box (or a moved body's new grown box meets it);
+ * a world where over half the bodies moved walks the exact boxes until it calms down.
+ * out[0] = bodies that moved out of their grown boxes, out[1] = busy worlds, out[2] = worlds
+ * that walked the exact boxes, out[3] = worlds that listed all their bodies again. */
+avbd_status avbd_batch_list_stats(avbd_batch *batch, int32_t out[4]);
+avbd_status avbd_world_list_stats(avbd_world *world, int32_t out[4]);
 
 #ifdef __cplusplus
 }
