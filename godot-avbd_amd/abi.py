@@ -293,6 +293,9 @@ def load_library(path=LIB_PATH):
                                              C.POINTER(C.c_int64)]),
         "avbd_batch_device_bytes": (C.c_int32, [vp, C.POINTER(C.c_int64)]),
         "avbd_batch_add_world": (C.c_int32, [vp, C.POINTER(Bodies), C.POINTER(Joints), i32p]),
+        "avbd_batch_add_world_async": (C.c_int32, [vp, C.POINTER(Bodies), C.POINTER(Joints), i32p]),
+        "avbd_batch_step_launch": (C.c_int32, [vp]),
+        "avbd_batch_step_wait": (C.c_int32, [vp]),
         "avbd_batch_remove_world": (C.c_int32, [vp, C.c_int32]),
         "avbd_batch_step": (C.c_int32, [vp]),
         "avbd_batch_failed_worlds": (C.c_int32, [vp, C.c_int32, i32p, i32p, i32p]),

@@ -130,6 +130,7 @@ avbd_status checkParams(const avbd_params *p)
 
 struct WorldInfo {
     bool active = false;
+    bool pending = false;    // uploaded by avbd_batch_add_world_async, active from the next step
     int bodyBase = 0, bodyCount = 0, jointBase = 0, jointCount = 0, slotBase = 0, slotCap = 0;
     bool hasJoints = false;
     bool hasRound = false;   // a sphere or cylinder among its bodies
@@ -226,6 +227,11 @@ struct Engine {
         for (auto &e : profEvents) (void)hipEventDestroy(e);
         profEvents.clear();
         if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
+        if (upStream) { (void)hipStreamSynchronize(upStream); (void)hipStreamDestroy(upStream); upStream = nullptr; }
+        if (upEvent) { (void)hipEventDestroy(upEvent); upEvent = nullptr; }
+        if (arena) { (void)hipHostFree(arena); arena = nullptr; }
+        for (auto *p : arenaOld) (void)hipHostFree(p);
+        arenaOld.clear();
     }
 
     std::vector<void *> allocs;
@@ -857,6 +863,8 @@ struct Engine {
                 return fail(AVBD_ERR_NO_DEVICE, "no HIP device visible");
             HIPCHK(hipGetDevice(&device));
             HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            HIPCHK(hipStreamCreateWithFlags(&upStream, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&upEvent, hipEventDisableTiming));
             for (auto &e : ev) HIPCHK(hipEventCreate(&e));
             HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&errHost), sizeof(int) * Wcap, hipHostMallocDefault));
             HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&countsHost), sizeof(int) * 64, hipHostMallocDefault));
@@ -874,6 +882,17 @@ struct Engine {
         AL(S.sleepMode, N); AL(S.sleeping, N); AL(S.stillFrames, N); AL(S.updFlag, N + 1); AL(S.updScan, N + 1);
         AL(S.mDormant, M); AL(S.mSaved, M); AL(S.dRA, M * MAXC); AL(S.dRB, M * MAXC); AL(S.dC0, M * MAXC);
         AL(S.dPen, M * MAXC); AL(S.pairParked, P); AL(rimDev, 16); S.rim = rimDev;
+        if (!sizing) {
+            // collide.cpp:733-753: rim directions at angle steps i*5 mod 8, from the C library's
+            // cosf / sinf at run time (the values the reference's table holds)
+            float rim[16];
+            for (int i = 0; i < 8; i++) {
+                volatile float a = 6.28318530718f * static_cast<float>(i * 5 % 8) / static_cast<float>(8);
+                rim[i] = std::cos(static_cast<float>(a));
+                rim[8 + i] = std::sin(static_cast<float>(a));
+            }
+            HIPCHK(hipMemcpy(rimDev, rim, sizeof(rim), hipMemcpyHostToDevice));
+        }
         AL(S.bodyWorld, N); AL(S.updateIndex, N); AL(S.colour, N); AL(S.contactOnly, N);
         AL(S.dLin, N); AL(S.dAng, N); AL(S.aabbC, N); AL(S.aabbH, N);
         AL(S.fbCount, 1); AL(S.fbBody, N); AL(S.fbBlock, 33 * N);
@@ -1311,9 +1330,20 @@ struct Engine {
     // on the other bits.  Overflowing the shared candidate-pair capacity fails the whole step.
     avbd_status checkErrors()
     {
+        avbd_status st = queueErrorCopies();
+        if (st != AVBD_OK) return st;
+        return readErrors();
+    }
+    // The step's error words and pair total to pinned host memory (after the step's kernels).
+    avbd_status queueErrorCopies()
+    {
         HIPCHK(hipMemcpyAsync(errHost, S.wErr, sizeof(int) * Wcap, hipMemcpyDeviceToHost, stream));
-        uint32_t total = 0;
         HIPCHK(hipMemcpyAsync(countsHost, S.pairOff + Ncap, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        return AVBD_OK;
+    }
+    avbd_status readErrors()
+    {
+        uint32_t total = 0;
         HIPCHK(hipStreamSynchronize(stream));
         std::memcpy(&total, countsHost, sizeof(uint32_t));
         failedIds.clear();
@@ -1548,6 +1578,17 @@ struct Engine {
 
     avbd_status stepBatch()
     {
+        avbd_status st = stepBatchLaunch();
+        if (st != AVBD_OK) return st;
+        return stepBatchWait();
+    }
+    // The step's work on the stream, returning at once: worlds uploaded since the last step are
+    // activated first; until stepBatchWait only asynchronous admissions may be made.
+    avbd_status stepBatchLaunch()
+    {
+        if (stepInFlight) return fail(AVBD_ERR_INVALID, "a step is in flight");
+        avbd_status st = activatePending();
+        if (st != AVBD_OK) return st;
         if (!csrValid) {
             refreshView();
             buildAdjacency();
@@ -1558,23 +1599,122 @@ struct Engine {
             treeAge = rebuild ? 0 : treeAge + 1;
             enqueueStep(1, 0, rebuild, anyJoints(), false);
             HIPCHK(hipGetLastError());
-            return checkErrors();
+        } else {
+            if (!graphsValid || graphJoints != anyJoints() || graphRound != anyRound() || graphSleep != anySleep()) {
+                st = captureGraphs();
+                if (st != AVBD_OK) return st;
+            }
+            const bool rebuild = treeAge >= refitBudget;
+            treeAge = rebuild ? 0 : treeAge + 1;
+            HIPCHK(hipGraphLaunch(graphs[rebuild ? 0 : 1], stream));
         }
-        if (!graphsValid || graphJoints != anyJoints() || graphRound != anyRound() || graphSleep != anySleep()) {
-            avbd_status st = captureGraphs();
-            if (st != AVBD_OK) return st;
-        }
-        const bool rebuild = treeAge >= refitBudget;
-        treeAge = rebuild ? 0 : treeAge + 1;
-        HIPCHK(hipGraphLaunch(graphs[rebuild ? 0 : 1], stream));
-        return checkErrors();
+        if ((st = queueErrorCopies()) != AVBD_OK) return st;
+        stepInFlight = true;
+        return AVBD_OK;
+    }
+    avbd_status stepBatchWait()
+    {
+        if (!stepInFlight) return fail(AVBD_ERR_INVALID, "no step in flight");
+        stepInFlight = false;
+        avbd_status st = readErrors();
+        arenaQuiesce();
+        return st;
+    }
+    // Entry points other than the step and asynchronous admission: no step in flight, and
+    // worlds admitted asynchronously made active first.
+    avbd_status idle()
+    {
+        if (stepInFlight) return fail(AVBD_ERR_INVALID, "a step is in flight: call avbd_batch_step_wait first");
+        avbd_status st = activatePending();
+        if (st != AVBD_OK) return st;
+        arenaQuiesce();
+        return AVBD_OK;
     }
 
     // ---- upload / download --------------------------------------------------------
-    avbd_status uploadWorld(int w, const avbd_bodies *b, const avbd_joints *j, int slotCap)
+    // A world's arrays are packed into pinned host memory (the arena) and copied from there:
+    // its data arrays at once (on the main stream, or on the upload stream for an admission
+    // that overlaps a running step), its identity arrays (bodyWorld, updateIndex, jWorld,
+    // slotWorld, the world table), which every kernel checks first, on the main stream when
+    // the world becomes active.  The arena is reused once the streams have drained.
+    struct Copy { void *dst; const void *src; size_t bytes; };
+    struct Pending { int w; std::vector<Copy> activate; };
+    unsigned char *arena = nullptr;
+    size_t arenaCap = 0, arenaUsed = 0;
+    std::vector<unsigned char *> arenaOld;
+    hipStream_t upStream = nullptr;
+    hipEvent_t upEvent = nullptr;
+    bool upRecorded = false;
+    std::vector<Pending> pending;
+    bool stepInFlight = false;
+
+    avbd_status ensureArena(size_t bytes)
+    {
+        if (arenaUsed + bytes <= arenaCap) return AVBD_OK;
+        const size_t cap = std::max<size_t>({bytes, 2 * arenaCap, 1 << 20});
+        unsigned char *p = nullptr;
+        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&p), cap, hipHostMallocDefault));
+        if (arena) {
+            if (arenaUsed == 0 && pending.empty() && !stepInFlight) (void)hipHostFree(arena);
+            else arenaOld.push_back(arena);   // copies from it may still be queued
+        }
+        arena = p;
+        arenaCap = cap;
+        arenaUsed = 0;
+        return AVBD_OK;
+    }
+    const void *arenaPut(const void *src, size_t bytes)
+    {
+        unsigned char *p = arena + arenaUsed;
+        std::memcpy(p, src, bytes);
+        arenaUsed += (bytes + 255) & ~static_cast<size_t>(255);
+        return p;
+    }
+    // Both streams drained: the arena's copies are done.
+    void arenaQuiesce()
+    {
+        if (!pending.empty() || stepInFlight) return;
+        arenaUsed = 0;
+        for (auto *p : arenaOld) (void)hipHostFree(p);
+        arenaOld.clear();
+    }
+    // Worlds uploaded since the last step become active (main stream, after their copies).
+    avbd_status activatePending()
+    {
+        if (pending.empty()) return AVBD_OK;
+        if (upRecorded) HIPCHK(hipStreamWaitEvent(stream, upEvent, 0));
+        for (auto &p : pending) {
+            for (auto &c : p.activate) HIPCHK(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, stream));
+            worlds[p.w].pending = false;
+            worlds[p.w].active = true;
+        }
+        pending.clear();
+        csrValid = false;
+        treeAge = 1 << 20;
+        return refreshWorldOrder();   // waits for the copies
+    }
+
+    // async: the data copies go on the upload stream and the world is activated by the next
+    // step (avbd_batch_add_world_async); otherwise it is active on return.
+    avbd_status uploadWorld(int w, const avbd_bodies *b, const avbd_joints *j, int slotCap, bool async = false)
+    {
+        const int nj = j ? j->count : 0;
+        avbd_status st = ensureArena(209 * static_cast<size_t>(b->count) + 377 * static_cast<size_t>(nj) +
+                                     4 * static_cast<size_t>(slotCap) + 64 * 256 + 4096);
+        if (st != AVBD_OK) return st;
+        const size_t mark = arenaUsed;
+        st = stageWorld(w, b, j, slotCap, async);
+        if (st != AVBD_OK && !worlds[w].pending) arenaUsed = mark;   // rejected input: nothing was queued
+        return st;
+    }
+
+    avbd_status stageWorld(int w, const avbd_bodies *b, const avbd_joints *j, int slotCap, bool async)
     {
         const int n = b->count;
         const int nj = j ? j->count : 0;
+        hipStream_t cs = async ? upStream : stream;
+        std::vector<Copy> data;
+        Pending act{w, {}};
         WorldInfo &wi = worlds[w];
         wi.bodyCount = n;
         wi.jointCount = nj;
@@ -1620,7 +1760,13 @@ struct Engine {
         }
         const int o = wi.bodyBase;
         auto put = [&](auto *dst, const auto &src) {
-            return hipMemcpyAsync(dst + o, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice, stream);
+            const size_t bytes = src.size() * sizeof(src[0]);
+            data.push_back({dst + o, arenaPut(src.data(), bytes), bytes});
+            return hipSuccess;
+        };
+        auto idn = [&](auto *dst, const auto &src) {   // identity arrays: copied on activation
+            const size_t bytes = src.size() * sizeof(src[0]);
+            act.activate.push_back({dst, arenaPut(src.data(), bytes), bytes});
         };
         HIPCHK(put(S.pos, pos)); HIPCHK(put(S.rot, rot)); HIPCHK(put(S.vel, vel)); HIPCHK(put(S.angVel, ang));
         HIPCHK(put(S.prevVel, prev)); HIPCHK(put(S.initLin, pos)); HIPCHK(put(S.initAng, rot));
@@ -1629,19 +1775,8 @@ struct Engine {
         HIPCHK(put(S.layer, layer)); HIPCHK(put(S.mask, mask)); HIPCHK(put(S.lockLin, ll)); HIPCHK(put(S.lockAng, la));
         HIPCHK(put(S.shape, shp));
         HIPCHK(put(S.sleepMode, smode)); HIPCHK(put(S.sleeping, slp)); HIPCHK(put(S.stillFrames, still));
-        {
-            // collide.cpp:733-753: rim directions at angle steps i*5 mod 8, from the C library's
-            // cosf / sinf at run time (the values the reference's table holds)
-            float rim[16];
-            for (int i = 0; i < 8; i++) {
-                volatile float a = 6.28318530718f * static_cast<float>(i * 5 % 8) / static_cast<float>(8);
-                rim[i] = std::cos(static_cast<float>(a));
-                rim[8 + i] = std::sin(static_cast<float>(a));
-            }
-            HIPCHK(hipMemcpyAsync(rimDev, rim, sizeof(rim), hipMemcpyHostToDevice, stream));
-            HIPCHK(hipStreamSynchronize(stream));
-        }
-        HIPCHK(put(S.bodyWorld, bw)); HIPCHK(put(S.updateIndex, ui));
+        idn(S.bodyWorld + o, bw);
+        idn(S.updateIndex + o, ui);
         if (nj > 0) {
             std::vector<int> ja(nj), jb(nj), jw(nj, w);
             std::vector<float4> ra(nj), rb(nj), c0l(nj), c0a(nj), pl(nj), pa(nj), ll2(nj), la2(nj);
@@ -1694,9 +1829,12 @@ struct Engine {
             }
             const int jo = wi.jointBase;
             auto putj = [&](auto *dst, const auto &src) {
-                return hipMemcpyAsync(dst + jo, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice, stream);
+                const size_t bytes = src.size() * sizeof(src[0]);
+                data.push_back({dst + jo, arenaPut(src.data(), bytes), bytes});
+                return hipSuccess;
             };
-            HIPCHK(putj(S.jA, ja)); HIPCHK(putj(S.jB, jb)); HIPCHK(putj(S.jWorld, jw)); HIPCHK(putj(S.jRA, ra));
+            idn(S.jWorld + jo, jw);
+            HIPCHK(putj(S.jA, ja)); HIPCHK(putj(S.jB, jb)); HIPCHK(putj(S.jRA, ra));
             HIPCHK(putj(S.jRB, rb)); HIPCHK(putj(S.jC0L, c0l)); HIPCHK(putj(S.jC0A, c0a)); HIPCHK(putj(S.jPenL, pl));
             HIPCHK(putj(S.jPenA, pa)); HIPCHK(putj(S.jLamL, ll2)); HIPCHK(putj(S.jLamA, la2));
             HIPCHK(putj(S.jStiffL, sl)); HIPCHK(putj(S.jStiffA, sa)); HIPCHK(putj(S.jFrac, fr)); HIPCHK(putj(S.jArm, arm));
@@ -1705,30 +1843,38 @@ struct Engine {
             if (anyGeneric) {
                 HIPCHK(putj(S.jGRest, grest));
                 auto put6 = [&](auto *dst, const auto &src) {
-                    return hipMemcpyAsync(dst + 6 * jo, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice,
-                                          stream);
+                    const size_t bytes = src.size() * sizeof(src[0]);
+                    data.push_back({dst + 6 * jo, arenaPut(src.data(), bytes), bytes});
+                    return hipSuccess;
                 };
                 HIPCHK(put6(S.jGMode, gmode)); HIPCHK(put6(S.jGLo, glo)); HIPCHK(put6(S.jGHi, ghi));
                 HIPCHK(put6(S.jGK, gk)); HIPCHK(put6(S.jGEq, geq)); HIPCHK(put6(S.jGDamp, gd));
                 HIPCHK(put6(S.jGLam, glam)); HIPCHK(put6(S.jGPen, gpen)); HIPCHK(put6(S.jGInit, ginit));
             }
         }
-        // slot range ownership and empty manifold store
-        std::vector<int> sw(slotCap, w);
-        HIPCHK(hipMemcpyAsync(S.slotWorld + wi.slotBase, sw.data(), sizeof(int) * slotCap, hipMemcpyHostToDevice, stream));
-        const int tab[6] = {wi.bodyBase, n, wi.jointBase, nj, wi.slotBase, slotCap};
+        // slot range ownership and empty manifold store, the world table
+        idn(S.slotWorld + wi.slotBase, std::vector<int>(slotCap, w));
+        const std::vector<int> tab{wi.bodyBase, n, wi.jointBase, nj, wi.slotBase, slotCap, 0, 1};
+        const int *tp = static_cast<const int *>(arenaPut(tab.data(), sizeof(int) * tab.size()));
         int *dst[6] = {S.wBodyBase, S.wBodyCount, S.wJointBase, S.wJointCount, S.wSlotBase, S.wSlotCap};
-        for (int k = 0; k < 6; k++) HIPCHK(hipMemcpyAsync(dst[k] + w, &tab[k], sizeof(int), hipMemcpyHostToDevice, stream));
-        const int zero = 0, one = 1;
-        int *zs[5] = {S.wLive, S.wFree, S.wTop, S.wNew, S.wErr};
-        for (int k = 0; k < 5; k++) HIPCHK(hipMemcpyAsync(zs[k] + w, &zero, sizeof(int), hipMemcpyHostToDevice, stream));
-        HIPCHK(hipMemcpyAsync(S.wRewalk + w, &one, sizeof(int), hipMemcpyHostToDevice, stream));
-        HIPCHK(hipMemcpyAsync(S.wStale + w, &zero, sizeof(int), hipMemcpyHostToDevice, stream));
+        for (int k = 0; k < 6; k++) act.activate.push_back({dst[k] + w, tp + k, sizeof(int)});
+        int *zs[6] = {S.wLive, S.wFree, S.wTop, S.wNew, S.wErr, S.wStale};
+        for (int k = 0; k < 6; k++) act.activate.push_back({zs[k] + w, tp + 6, sizeof(int)});
+        act.activate.push_back({S.wRewalk + w, tp + 7, sizeof(int)});
+        if (arenaUsed > arenaCap) return fail(AVBD_ERR_DEVICE, "upload staging overflow");
+        for (const auto &c : data) HIPCHK(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, cs));
+        pending.push_back(std::move(act));
+        wi.pending = true;
+        if (async) {
+            HIPCHK(hipEventRecord(upEvent, upStream));
+            upRecorded = true;
+            return AVBD_OK;
+        }
+        avbd_status st = activatePending();
+        if (st != AVBD_OK) return st;
         HIPCHK(hipStreamSynchronize(stream));
-        wi.active = true;
-        csrValid = false;
-        treeAge = 1 << 20;
-        return refreshWorldOrder();
+        arenaQuiesce();
+        return AVBD_OK;
     }
 
     avbd_status writeBodies(int w, const avbd_bodies *b)
@@ -2389,6 +2535,7 @@ avbd_status avbd_world_profile(avbd_world *world, int32_t steps, avbd_kernel_pro
 avbd_status avbd_batch_profile(avbd_batch *batch, int32_t steps, avbd_kernel_profile *out)
 {
     if (!batch || !out) return fail(AVBD_ERR_INVALID, "null argument");
+    if (avbd_status busy = batch->eng.idle(); busy != AVBD_OK) return busy;
     if (!batch->eng.csrValid) {
         batch->eng.buildAdjacency();
         batch->eng.csrValid = true;
@@ -2426,12 +2573,14 @@ avbd_status avbd_batch_create(const avbd_params *params, int32_t max_worlds, int
 avbd_status avbd_batch_autotune(avbd_batch *batch, int32_t steps_per_trial, avbd_tuning *out)
 {
     if (!batch) return fail(AVBD_ERR_INVALID, "null argument");
+    if (avbd_status busy = batch->eng.idle(); busy != AVBD_OK) return busy;
     return batch->eng.autotune(steps_per_trial, out);
 }
 
 avbd_status avbd_batch_set_tuning(avbd_batch *batch, const avbd_tuning *tuning)
 {
     if (!batch || !tuning) return fail(AVBD_ERR_INVALID, "null argument");
+    if (avbd_status busy = batch->eng.idle(); busy != AVBD_OK) return busy;
     return batch->eng.setTuning(*tuning);
 }
 
@@ -2467,6 +2616,7 @@ avbd_status avbd_batch_device_bytes(avbd_batch *batch, int64_t *bytes)
 
 avbd_status avbd_batch_destroy(avbd_batch *batch)
 {
+    if (batch && batch->eng.stepInFlight) (void)batch->eng.stepBatchWait();
     delete batch;
     return AVBD_OK;
 }
@@ -2481,24 +2631,23 @@ avbd_status avbd_batch_fits(avbd_batch *batch, int32_t n_bodies, int32_t n_joint
     const Engine &e = batch->eng;
     const int slots = worldSlots(n_bodies);
     bool freeWorld = false;
-    for (int k = 0; k < e.Wcap; k++) freeWorld = freeWorld || !e.worlds[k].active;
+    for (int k = 0; k < e.Wcap; k++) freeWorld = freeWorld || (!e.worlds[k].active && !e.worlds[k].pending);
     if (fits_ever) *fits_ever = n_bodies <= e.Ncap && n_joints <= e.Jcap && slots <= e.Scap;
     if (fits_now)
         *fits_now = freeWorld && e.bodyAlloc.fits(n_bodies) && e.jointAlloc.fits(n_joints) && e.slotAlloc.fits(slots);
     return AVBD_OK;
 }
 
-avbd_status avbd_batch_add_world(avbd_batch *batch, const avbd_bodies *bodies, const avbd_joints *joints,
-                                 int32_t *world_id)
+static avbd_status addWorld(avbd_batch *batch, const avbd_bodies *bodies, const avbd_joints *joints,
+                            int32_t *world_id, bool async)
 {
-    if (!batch || !world_id) return fail(AVBD_ERR_INVALID, "null argument");
     avbd_status st = validBodies(bodies);
     if (st != AVBD_OK) return st;
     if ((st = validJoints(joints)) != AVBD_OK) return st;
     Engine &e = batch->eng;
     int w = -1;
     for (int k = 0; k < e.Wcap; k++)
-        if (!e.worlds[k].active) { w = k; break; }
+        if (!e.worlds[k].active && !e.worlds[k].pending) { w = k; break; }
     if (w < 0) return fail(AVBD_ERR_CAPACITY, "no free world slot");
     const int nb = bodies->count, nj = joints ? joints->count : 0;
     const int slots = worldSlots(nb);
@@ -2516,7 +2665,7 @@ avbd_status avbd_batch_add_world(avbd_batch *batch, const avbd_bodies *bodies, c
     wi.bodyBase = bb;
     wi.jointBase = jb;
     wi.slotBase = sb;
-    st = e.uploadWorld(w, bodies, joints, slots);
+    st = e.uploadWorld(w, bodies, joints, slots, async);
     if (st != AVBD_OK) {   // rejected input: the world stays inactive and its ranges go back
         e.bodyAlloc.give(bb, nb);
         e.jointAlloc.give(jb, nj);
@@ -2527,9 +2676,37 @@ avbd_status avbd_batch_add_world(avbd_batch *batch, const avbd_bodies *bodies, c
     return AVBD_OK;
 }
 
+avbd_status avbd_batch_add_world(avbd_batch *batch, const avbd_bodies *bodies, const avbd_joints *joints,
+                                 int32_t *world_id)
+{
+    if (!batch || !world_id) return fail(AVBD_ERR_INVALID, "null argument");
+    if (avbd_status busy = batch->eng.idle(); busy != AVBD_OK) return busy;
+    return addWorld(batch, bodies, joints, world_id, false);
+}
+
+avbd_status avbd_batch_add_world_async(avbd_batch *batch, const avbd_bodies *bodies, const avbd_joints *joints,
+                                       int32_t *world_id)
+{
+    if (!batch || !world_id) return fail(AVBD_ERR_INVALID, "null argument");
+    return addWorld(batch, bodies, joints, world_id, true);
+}
+
+avbd_status avbd_batch_step_launch(avbd_batch *batch)
+{
+    if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
+    return batch->eng.stepBatchLaunch();
+}
+
+avbd_status avbd_batch_step_wait(avbd_batch *batch)
+{
+    if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
+    return batch->eng.stepBatchWait();
+}
+
 avbd_status avbd_batch_remove_world(avbd_batch *batch, int32_t world_id)
 {
     if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
+    if (avbd_status busy = batch->eng.idle(); busy != AVBD_OK) return busy;
     if (world_id < 0 || world_id >= batch->eng.Wcap) return fail(AVBD_ERR_INVALID, "bad world id");
     return batch->eng.clearWorld(world_id);
 }
@@ -2749,6 +2926,7 @@ avbd_status avbd_batch_raycast(avbd_batch *batch, int32_t world_id, int32_t n, c
                                const uint32_t *masks, int32_t hit_from_inside, avbd_ray_hits *out)
 {
     if (!batch || !out) return fail(AVBD_ERR_INVALID, "null argument");
+    if (avbd_status busy = batch->eng.idle(); busy != AVBD_OK) return busy;
     if (world_id < 0 || world_id >= batch->eng.Wcap) return fail(AVBD_ERR_INVALID, "bad world id");
     return batch->eng.raycast(world_id, n, from, to, masks, hit_from_inside != 0, out->body, out->t, out->point,
                               out->normal, out->inside);
@@ -2795,6 +2973,7 @@ avbd_status avbd_batch_contact_report(avbd_batch *batch, int32_t world_id, const
                                       int32_t *count, float *position, float *normal, int32_t *other)
 {
     if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
+    if (avbd_status busy = batch->eng.idle(); busy != AVBD_OK) return busy;
     if (world_id < 0 || world_id >= batch->eng.Wcap) return fail(AVBD_ERR_INVALID, "bad world id");
     return batch->eng.contactReport(world_id, max_reported, stride, count, position, normal, other);
 }
@@ -2816,6 +2995,7 @@ avbd_status avbd_world_collect_aabb(avbd_world *world, const float min[3], const
 avbd_status avbd_batch_download_world(avbd_batch *batch, int32_t world_id, avbd_bodies *bodies, avbd_joints *joints)
 {
     if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
+    if (avbd_status busy = batch->eng.idle(); busy != AVBD_OK) return busy;
     if (world_id < 0 || world_id >= batch->eng.Wcap) return fail(AVBD_ERR_INVALID, "bad world id");
     return batch->eng.downloadWorld(world_id, bodies, joints);
 }
@@ -2823,6 +3003,7 @@ avbd_status avbd_batch_download_world(avbd_batch *batch, int32_t world_id, avbd_
 avbd_status avbd_batch_world_count(avbd_batch *batch, int32_t *count)
 {
     if (!batch || !count) return fail(AVBD_ERR_INVALID, "null argument");
+    if (avbd_status busy = batch->eng.idle(); busy != AVBD_OK) return busy;
     int c = 0;
     for (const auto &w : batch->eng.worlds) c += w.active ? 1 : 0;
     *count = c;
@@ -2832,6 +3013,7 @@ avbd_status avbd_batch_world_count(avbd_batch *batch, int32_t *count)
 avbd_status avbd_batch_get_manifolds(avbd_batch *batch, int32_t world_id, avbd_manifolds *out)
 {
     if (!batch || !out) return fail(AVBD_ERR_INVALID, "null argument");
+    if (avbd_status busy = batch->eng.idle(); busy != AVBD_OK) return busy;
     if (world_id < 0 || world_id >= batch->eng.Wcap || !batch->eng.worlds[world_id].active)
         return fail(AVBD_ERR_INVALID, "bad world id");
     return batch->eng.getManifolds(world_id, out);
@@ -2840,6 +3022,7 @@ avbd_status avbd_batch_get_manifolds(avbd_batch *batch, int32_t world_id, avbd_m
 avbd_status avbd_batch_set_manifolds(avbd_batch *batch, int32_t world_id, const avbd_manifolds *in)
 {
     if (!batch || !in) return fail(AVBD_ERR_INVALID, "null argument");
+    if (avbd_status busy = batch->eng.idle(); busy != AVBD_OK) return busy;
     if (world_id < 0 || world_id >= batch->eng.Wcap || !batch->eng.worlds[world_id].active)
         return fail(AVBD_ERR_INVALID, "bad world id");
     if (in->count > 0 && (!in->body_a || !in->body_b || !in->basis || !in->friction || !in->num_contacts ||
@@ -2852,12 +3035,14 @@ avbd_status avbd_batch_totals(avbd_batch *batch, int64_t *movable_bodies, int64_
                               int64_t *joints)
 {
     if (!batch) return fail(AVBD_ERR_INVALID, "batch is null");
+    if (avbd_status busy = batch->eng.idle(); busy != AVBD_OK) return busy;
     return batch->eng.totals(movable_bodies, manifolds, contacts, joints);
 }
 
 avbd_status avbd_batch_list_stats(avbd_batch *batch, int32_t out[4])
 {
     if (!batch || !out) return fail(AVBD_ERR_INVALID, "null argument");
+    if (avbd_status busy = batch->eng.idle(); busy != AVBD_OK) return busy;
     return batch->eng.listStats(out);
 }
 

@@ -12,9 +12,14 @@ block the FIFO for ever).  A world whose step fails on the device (its manifold 
 its broad-phase walk fails, or its state stops being finite) is evicted after that step and
 reported in `evicted`; the other worlds of the batch stepped normally and go on.
 
+Admission overlaps the device work: a tick launches the step of the resident worlds, admits
+waiting requests while it runs (their arrays go to the device on a second stream from pinned
+host memory, `avbd_batch_add_world_async`), then waits for the step; the worlds admitted this
+way are stepped from the next tick on.  (overlap=False admits before the step, synchronously.)
+
 Latency is recorded per world per step: from the moment the step was admitted (the request's
 arrival for its first step, the start of the tick otherwise) to the moment its result is ready
-(`avbd_batch_step` returns after the step has completed on the device).
+(`avbd_batch_step_wait` returns after the step has completed on the device).
 """
 import ctypes as C
 import time
@@ -59,7 +64,8 @@ class Request:
 class Server:
     """Continuous batching over one `Batch` sized from `hbm_bytes`."""
 
-    def __init__(self, params, hbm_bytes, max_worlds=1024, joints_per_body=1.0, use_graph=True):
+    def __init__(self, params, hbm_bytes, max_worlds=1024, joints_per_body=1.0, use_graph=True, overlap=True):
+        self.overlap = overlap
         self.max_bodies, self.max_joints = capacity_for_budget(params, hbm_bytes, max_worlds, joints_per_body)
         self.batch = Batch(params, max_worlds, self.max_bodies, self.max_joints)
         self.batch.use_graph(use_graph)
@@ -91,7 +97,7 @@ class Server:
         while self.waiting:
             req = self.waiting[0]
             try:
-                req.wid = self.batch.add(req.bodies, req.joints)
+                req.wid = self.batch.add(req.bodies, req.joints, overlap=self.overlap)
             except abi.AvbdError as e:
                 if e.code == abi.AVBD_ERR_CAPACITY:
                     break            # no room until a resident world retires (FIFO: no overtaking)
@@ -101,13 +107,23 @@ class Server:
             self.resident[req.wid] = req
 
     def tick(self):
-        """Admit what fits, step every resident world once, retire finished worlds."""
-        self._admit()
+        """Step every resident world once, admitting what fits meanwhile (or before, without
+        overlap), then retire finished worlds."""
+        if not self.overlap:
+            self._admit()
         if not self.resident:
+            if self.overlap:
+                self._admit()   # nothing to overlap with: these worlds step from the next tick
             return 0
         t0 = time.perf_counter()
+        stepping = list(self.resident.items())
         try:
-            self.batch.step()
+            if self.overlap:
+                self.batch.step_launch()
+                self._admit()
+                self.batch.step_wait()
+            else:
+                self.batch.step()
         except abi.AvbdError as e:
             if e.code != abi.AVBD_ERR_WORLD:
                 raise
@@ -118,7 +134,9 @@ class Server:
                     self.batch.remove(wid)
         t1 = time.perf_counter()
         stepped = 0
-        for wid, req in list(self.resident.items()):
+        for wid, req in stepping:
+            if wid not in self.resident:   # evicted
+                continue
             self.latencies.append(t1 - (req.arrival if req.first_step else t0))
             req.first_step = False
             req.done += 1

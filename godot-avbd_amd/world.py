@@ -320,11 +320,15 @@ class Batch:
         except Exception:
             pass
 
-    def add(self, bodies, joints=None):
+    def add(self, bodies, joints=None, overlap=False):
+        """Add a world (active on return).  overlap: upload it on the second stream, returning
+        before the copies land (it may be called while a step is in flight); the world is
+        stepped from the next step launch."""
         bs = abi.bodies_struct(bodies)
         js = abi.joints_struct(joints)
         wid = C.c_int32(-1)
-        abi.check(self.lib.avbd_batch_add_world(self.h, C.byref(bs), C.byref(js), C.byref(wid)))
+        fn = self.lib.avbd_batch_add_world_async if overlap else self.lib.avbd_batch_add_world
+        abi.check(fn(self.h, C.byref(bs), C.byref(js), C.byref(wid)))
         self.sizes[wid.value] = bs.count
         return wid.value
 
@@ -336,6 +340,15 @@ class Batch:
         """One step of every world.  Raises AvbdError with code AVBD_ERR_WORLD when some worlds
         failed (the others stepped): `failed_worlds()` lists them."""
         abi.check(self.lib.avbd_batch_step(self.h))
+
+    def step_launch(self):
+        """Queue one step and return; only add(..., overlap=True), fits and failed_worlds may be
+        called before step_wait."""
+        abi.check(self.lib.avbd_batch_step_launch(self.h))
+
+    def step_wait(self):
+        """Wait for the launched step; raises as step() does."""
+        abi.check(self.lib.avbd_batch_step_wait(self.h))
 
     def failed_worlds(self):
         """[(world id, AVBD_WORLD_* reason bits)] of the worlds that failed the last step."""

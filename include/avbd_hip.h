@@ -470,6 +470,21 @@ avbd_status avbd_batch_device_bytes(avbd_batch *batch, int64_t *bytes);
 /* Add a world; its id is written to *world_id.  Takes effect at the next step. */
 avbd_status avbd_batch_add_world(avbd_batch *batch, const avbd_bodies *bodies,
                                  const avbd_joints *joints, int32_t *world_id);
+/* Serving without a reference counterpart: the same, but the world's arrays are copied to the
+ * device on a second stream from pinned host memory and the call returns before they land, so
+ * an admission made between avbd_batch_step_launch and avbd_batch_step_wait overlaps the
+ * running step.  The world becomes active (and is stepped) from the next step launch; every
+ * other call except the step, avbd_batch_fits and more asynchronous additions first makes
+ * such worlds active.  The caller's arrays may be reused as soon as the call returns. */
+avbd_status avbd_batch_add_world_async(avbd_batch *batch, const avbd_bodies *bodies,
+                                       const avbd_joints *joints, int32_t *world_id);
+/* avbd_batch_step in two halves: launch queues the step (after activating worlds added
+ * asynchronously) and returns; wait blocks until it is done and returns what avbd_batch_step
+ * would (AVBD_ERR_WORLD with avbd_batch_failed_worlds).  Between the two only
+ * avbd_batch_add_world_async, avbd_batch_fits and avbd_batch_failed_worlds may be called;
+ * anything else returns AVBD_ERR_INVALID. */
+avbd_status avbd_batch_step_launch(avbd_batch *batch);
+avbd_status avbd_batch_step_wait(avbd_batch *batch);
 avbd_status avbd_batch_remove_world(avbd_batch *batch, int32_t world_id);
 /* Whether a world of n_bodies bodies and n_joints joints can be added now (*fits_now: a free
  * world slot and free body, joint and manifold-slot ranges) and whether it could be added to

@@ -18,10 +18,11 @@ def _serve():
     return import_module("godot-avbd_amd.serve")
 
 
-def test_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("overlap", [True, False])
+def test_same_alone_and_in_the_mix(overlap):
     serve = _serve()
     params = pkg.default_params(iterations=10)
-    srv = serve.Server(params, 1.5e9, max_worlds=64, joints_per_body=0.5)
+    srv = serve.Server(params, 1.5e9, max_worlds=64, joints_per_body=0.5, overlap=overlap)
     srv.keep_final = True
     rng = np.random.default_rng(11)
     background = []
@@ -173,22 +174,61 @@ def test_rejected_world_gives_its_ranges_back():
     batch.close()
 
 
-def test_admission_waits_for_budget():
-    """A budget that holds about one large world: the second waits until the first retires."""
+@pytest.mark.parametrize("overlap", [True, False])
+def test_admission_waits_for_budget(overlap):
+    """A budget that holds about one large world: the second waits until the first retires,
+    then is admitted at the next tick and steps its horizon."""
     serve = _serve()
     params = pkg.default_params(iterations=10)
     b, j = scenes.pile(2000, seed=3, ground=200.0)
     need = serve.footprint(params, 4, 2000, 0)
-    srv = serve.Server(params, need * 1.3, max_worlds=4, joints_per_body=0.0)
+    srv = serve.Server(params, need * 1.3, max_worlds=4, joints_per_body=0.0, overlap=overlap)
     first, second = serve.Request(0, b, j, 10, 0.0), serve.Request(1, b, j, 5, 0.0)
     srv.submit(first)
     srv.submit(second)
     srv.tick()
     assert len(srv.resident) == 1 and len(srv.waiting) == 1
-    for _ in range(10):
+    while first.done < first.horizon:
+        assert second.wid < 0 or second.done == 0
         srv.tick()
-    assert len(srv.waiting) == 0 and len(srv.resident) == 1   # admitted once the first retired
-    for _ in range(5):
+    assert first.wid not in srv.resident and len(srv.waiting) == 1
+    srv.tick()                                               # room again: admitted
+    assert len(srv.waiting) == 0 and len(srv.resident) == 1
+    while srv.resident:
         srv.tick()
-    assert not srv.resident
+    assert second.done == second.horizon
     srv.close()
+
+
+def test_async_admission_during_a_step():
+    """Worlds added while a step is in flight (second stream, pinned copies) are stepped from the
+    next launch and end equal to the oracle; the caller's arrays can change as soon as the add
+    returns; other calls are refused until the step is waited for."""
+    params = pkg.default_params(iterations=10)
+    resident = [scenes.pile(600, seed=41, ground=80.0), scenes.chains(n_chains=6, links=16, seed=42)]
+    batch = pkg.Batch(params, 8, 4000, 400)
+    for bb, jj in resident:
+        batch.add(bb, jj)
+    for _ in range(3):
+        batch.step()
+    late = [scenes.pile(400, seed=43, ground=60.0), scenes.chains(n_chains=4, links=16, seed=44)]
+    kept = [({k: v.copy() for k, v in bb.items()}, None if jj is None else {k: v.copy() for k, v in jj.items()})
+            for bb, jj in late]
+    batch.step_launch()
+    ids = [batch.add(bb, jj, overlap=True) for bb, jj in late]
+    for bb, _ in late:                      # the library copied them during the call
+        bb["position"][:] = 1e6
+        bb["velocity"][:] = np.nan
+    with pytest.raises(pkg.abi.AvbdError):
+        batch.download(ids[0])
+    with pytest.raises(pkg.abi.AvbdError):
+        batch.add(*kept[0])                 # only asynchronous additions while in flight
+    batch.step_wait()
+    for _ in range(12):
+        batch.step()
+    for wid, (bb, jj) in zip(ids, kept):
+        o = oracle.OracleWorld(params)
+        o.upload(bb, jj)
+        o.step(12)
+        assert not state_mismatch(batch.download(wid), o.download()), wid
+    batch.close()
