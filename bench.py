@@ -141,6 +141,15 @@ def bench_chains(args, log):
     movable = int((bodies["mass"] > 0).sum())
     st = w.stats()
     lists = w.list_stats()
+    # roofline of the colour sweep (the dominant kernel here): per iteration it reads each
+    # movable body once and each manifold, contact and joint as the primal does
+    # (100 N_b + 160 N_m + 60 N_c + 256 N_j); a launch is one of the 8 colours
+    prof = w.profile(args.profile_steps)
+    cp_ms = prof["avg_ms"]["colour_pass"]
+    nj = len(joints["body_b"])
+    sweep = 100 * movable + 160 * st["manifolds"] + 60 * st["contacts"] + 256 * nj
+    per_launch = sweep / 8
+    achieved = per_launch / (cp_ms / 1e3) / 1e9 if cp_ms > 0 else None
     w.close()
     log(f"C3 chains: {dt * 1e3:.3f} ms/step, {movable / dt / 1e6:.2f} M body-steps/s ({time.time() - t0:.1f}s)")
     return {"workload": "C3 plain world: 4096 chains x 32 unit boxes, ball joints, static anchors, ground; "
@@ -148,6 +157,12 @@ def bench_chains(args, log):
             "joints": len(joints["body_b"]), "manifolds": st["manifolds"], "contacts": st["contacts"],
             "warmup": args.c3_warmup, "steps": args.c3_steps, "ms_per_step": dt * 1e3, "tuning": tuning,
             "broadphase_lists": lists,
+            "roofline": {"bound": "hbm", "kernel": "colour_pass", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": None if achieved is None else achieved / HBM_PEAK_GBS,
+                         "traffic": None, "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": cp_ms,
+                         "launches_per_step": prof["launches"]["colour_pass"] / max(prof["steps"], 1)},
+            "kernel_ms": prof["avg_ms"], "kernel_launches_per_step":
+                {k: v / max(prof["steps"], 1) for k, v in prof["launches"].items()},
             "value": movable / dt, "unit": "body-steps/s"}
 
 

@@ -158,6 +158,10 @@ struct Engine {
     bool csrValid = false;
     int treeAge = 1 << 20;
     int refitBudget = 16;
+    int movingWorlds = 0;   // busy or exact-walking worlds in the last checked step
+    // Steps between tree rebuilds: the reference's 16, or 4 while some world moves fast (its
+    // refit boxes bloat and walks lengthen; the topology never changes the pairs).
+    int rebuildInterval() const { return movingWorlds > 0 ? 4 : refitBudget; }
     bool batchMode = false;
     int colourRounds = 48;
     hipGraphExec_t graphs[2] = {nullptr, nullptr};  // [rebuild, refit]
@@ -1339,6 +1343,7 @@ struct Engine {
     {
         HIPCHK(hipMemcpyAsync(errHost, S.wErr, sizeof(int) * Wcap, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipMemcpyAsync(countsHost, S.pairOff + Ncap, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(countsHost + 4, S.listStats, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
         return AVBD_OK;
     }
     avbd_status readErrors()
@@ -1346,6 +1351,7 @@ struct Engine {
         uint32_t total = 0;
         HIPCHK(hipStreamSynchronize(stream));
         std::memcpy(&total, countsHost, sizeof(uint32_t));
+        movingWorlds = countsHost[4 + 1] + countsHost[4 + 2];
         failedIds.clear();
         failedWhy.clear();
         for (int w = 0; w < Wcap; w++) {
@@ -1420,7 +1426,7 @@ struct Engine {
 
     avbd_status stepPlain(int substeps, int rounds)
     {
-        const bool rebuild = treeAge >= refitBudget;
+        const bool rebuild = treeAge >= rebuildInterval();
         treeAge = rebuild ? 0 : treeAge + 1;
         hipGraphExec_t g = nullptr;
         if (useGraph && !plainGraphUnusable) {
@@ -1459,7 +1465,7 @@ struct Engine {
         profiling = true;
         double stepMs = 0.0;
         for (int s2 = 0; s2 < steps; s2++) {
-            const bool rebuild = treeAge >= refitBudget;
+            const bool rebuild = treeAge >= rebuildInterval();
             treeAge = rebuild ? 0 : treeAge + 1;
             (void)hipEventRecord(ev[0], stream);
             enqueueStep(substeps, rounds, rebuild, anyJoints(), false);
@@ -1595,7 +1601,7 @@ struct Engine {
             csrValid = true;
         }
         if (!useGraph) {
-            const bool rebuild = treeAge >= refitBudget;
+            const bool rebuild = treeAge >= rebuildInterval();
             treeAge = rebuild ? 0 : treeAge + 1;
             enqueueStep(1, 0, rebuild, anyJoints(), false);
             HIPCHK(hipGetLastError());
@@ -1604,7 +1610,7 @@ struct Engine {
                 st = captureGraphs();
                 if (st != AVBD_OK) return st;
             }
-            const bool rebuild = treeAge >= refitBudget;
+            const bool rebuild = treeAge >= rebuildInterval();
             treeAge = rebuild ? 0 : treeAge + 1;
             HIPCHK(hipGraphLaunch(graphs[rebuild ? 0 : 1], stream));
         }
