@@ -372,14 +372,20 @@ AD void stampManifold(const State &S, int k, int body, Block &blk)
     const v3 b0 = row(B, 0), b1 = row(B, 1), b2 = row(B, 2);
     m3 accLin = zero3(), accAng = zero3(), accCross = zero3();
     v3 accRL{0, 0, 0}, accRA{0, 0, 0};
+    // contact i + 1 is loaded while contact i is computed
+    float4 nPL{}, nRA{}, nRB{}, nC0{};
+    float2 nLyz{};
+    if (n > 0) { nPL = S.cPL[cb]; nRA = S.cRA[cb]; nRB = S.cRB[cb]; nC0 = S.cC0[cb]; nLyz = S.cLyz[cb]; }
     for (int i = 0; i < n; i++) {
         const uint32_t c = cb + i * CSLICE;
-        const v3 rA = mul(rotA, ld3(S.cRA[c]));
-        const v3 rB = mul(rotB, ld3(S.cRB[c]));
+        const float4 pl4 = nPL, ra4 = nRA, rb4 = nRB, c04 = nC0;
+        const float2 lyz = nLyz;
+        if (i + 1 < n) { const uint32_t d = c + CSLICE; nPL = S.cPL[d]; nRA = S.cRA[d]; nRB = S.cRB[d]; nC0 = S.cC0[d]; nLyz = S.cLyz[d]; }
+        const v3 rA = mul(rotA, ld3(ra4));
+        const v3 rB = mul(rotB, ld3(rb4));
         const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
-        const v3 C = constraintError(S, ld3(S.cC0[c]), br, B);
-        v3 pen, lam;
-        loadPenLam(S, c, pen, lam);
+        const v3 C = constraintError(S, ld3(c04), br, B);
+        const v3 pen = ld3(pl4), lam{pl4.w, lyz.x, lyz.y};
         v3 f = cmul(pen, C) + lam;
         float bounds;
         clampForce(f, mu, bounds);
@@ -856,20 +862,31 @@ AD void forcePrimal(const State &S, int j, int body, Block &blk)
 
 AD void jointPrimal(const State &S, int j, int body, Block &blk)
 {
-    const bool isA = body == S.jA[j];
-    const v3 penL = ld3(S.jPenL[j]);
+    // Every load is issued before the first branch (one round of waits for the joint's fields,
+    // one for the two poses); the arithmetic is jointLin / jointAng's, in their order.
+    const int ja = S.jA[j], jb = S.jB[j];
+    const v3 penL = ld3(S.jPenL[j]), penA = ld3(S.jPenA[j]);
+    const v3 lamL = ld3(S.jLamL[j]), lamA = ld3(S.jLamA[j]);
+    const v3 c0L = ld3(S.jC0L[j]), c0A = ld3(S.jC0A[j]);
+    const float stL = S.jStiffL[j], stA = S.jStiffA[j], armA = S.jArm[j];
+    const v3 rA = ld3(S.jRA[j]), rB = ld3(S.jRB[j]);
+    const int aa = ja >= 0 ? ja : jb;   // a world anchor (ja < 0) reads no body
+    const v3 posA = bPos(S, aa), posB = bPos(S, jb);
+    const qt rotA = bRot(S, aa), rotB = bRot(S, jb);
+    const bool isA = body == ja;
     if (length2(penL) > 0) {
         const m3 K = diag(penL.x, penL.y, penL.z);
-        v3 C = jointLin(S, j);
-        if (__builtin_isinf(S.jStiffL[j])) C -= ld3(S.jC0L[j]) * S.P.alpha;
-        const v3 F = mul(K, C) + ld3(S.jLamL[j]);
+        const v3 pa = ja >= 0 ? posA + qrot(rotA, rA) : rA;
+        v3 C = pa - (posB + qrot(rotB, rB));
+        if (__builtin_isinf(stL)) C -= c0L * S.P.alpha;
+        const v3 F = mul(K, C) + lamL;
         const m3 jLin = isA ? ident() : mneg(ident());
         m3 jAng;
         if (isA) {
-            const v3 r = -qrot(bRot(S, S.jA[j]), ld3(S.jRA[j]));
+            const v3 r = -qrot(rotA, rA);
             jAng = m3{{v3{0, r.z, -r.y}, v3{-r.z, 0, r.x}, v3{r.y, -r.x, 0}}};
         } else {
-            const v3 r = qrot(bRot(S, S.jB[j]), ld3(S.jRB[j]));
+            const v3 r = qrot(rotB, rB);
             jAng = m3{{v3{0, r.z, -r.y}, v3{-r.z, 0, r.x}, v3{r.y, -r.x, 0}}};
         }
         const m3 jLinT = transpose(jLin), jAngT = transpose(jAng);
@@ -877,19 +894,19 @@ AD void jointPrimal(const State &S, int j, int body, Block &blk)
         blk.lhsLin = madd(blk.lhsLin, mul(mul(jLinT, K), jLin));
         blk.lhsAng = madd(blk.lhsAng, mul(jAngTk, jAng));
         blk.lhsCross = madd(blk.lhsCross, mul(jAngTk, jLin));
-        const v3 r = isA ? qrot(bRot(S, S.jA[j]), ld3(S.jRA[j])) : -qrot(bRot(S, S.jB[j]), ld3(S.jRB[j]));
+        const v3 r = isA ? qrot(rotA, rA) : -qrot(rotB, rB);
         const m3 H = madd(madd(mscale(gsBall(0, r), F.x), mscale(gsBall(1, r), F.y)), mscale(gsBall(2, r), F.z));
         blk.lhsAng = madd(blk.lhsAng, diag(H.c[0][0], H.c[1][1], H.c[2][2]));
         blk.rhsLin += mul(jLinT, F);
         blk.rhsAng += mul(jAngT, F);
     }
-    const v3 penA = ld3(S.jPenA[j]);
     if (length2(penA) > 0) {
         const m3 K = diag(penA.x, penA.y, penA.z);
-        v3 C = jointAng(S, j);
-        if (__builtin_isinf(S.jStiffA[j])) C -= ld3(S.jC0A[j]) * S.P.alpha;
-        const v3 F = mul(K, C) + ld3(S.jLamA[j]);
-        const float arm = S.jArm[j];
+        const qt qa = ja >= 0 ? rotA : qt{0, 0, 0, 1};
+        v3 C = qsub(qa, rotB) * armA;
+        if (__builtin_isinf(stA)) C -= c0A * S.P.alpha;
+        const v3 F = mul(K, C) + lamA;
+        const float arm = armA;
         const m3 jA = mscale(isA ? ident() : m3{{v3{0, 0, 0}, v3{0, -1, 0}, v3{0, 0, -1}}}, arm);
         blk.lhsAng = madd(blk.lhsAng, mul(mul(transpose(jA), K), jA));
         blk.rhsAng += mul(transpose(jA), F);
