@@ -1036,13 +1036,14 @@ AD void faceAxes(const OBB &b, int i, v3 &u, v3 &v, float &eu, float &ev)
 }
 constexpr int MAX_POLY = 16;
 // A clip polygon of capacity MAX_POLY.  Clipping a quad by the four side planes adds at most
-// one vertex per plane, so the first POLY_LDS vertices are the ones used in practice: they
-// sit in LDS, one lane's column (vertex v at x[v * NARROW_TPB], the caller's pointers offset
-// by the lane, so a wavefront's accesses are consecutive words).  Vertices past POLY_LDS, which
-// only rounding could produce, go to `spill` (3 floats each), global memory the caller owns.
-// Half the LDS of a full-capacity column doubles the wavefronts a CU can hold.
+// one vertex per plane (eight in all, more only through rounding).  The first POLY_LDS sit in
+// LDS, one lane's column (vertex v at x[v * NARROW_TPB], the caller's pointers offset by the
+// lane, so a wavefront's accesses are consecutive words); the rest (the seventh and eighth of
+// an octagon, and any past them) go to `spill` (3 floats each), global memory the caller owns.
+// Six LDS vertices (9 KiB per 64 lanes) let a CU hold as many wavefronts as the kernel's
+// registers allow; eight held it to 13 (measured 1.48 -> 1.38 ms on the settled C4 batch).
 constexpr int NARROW_TPB = 64;
-constexpr int POLY_LDS = 8;
+constexpr int POLY_LDS = 6;
 struct Poly {
     float *x, *y, *z;
     float4 *spill;   // float e of the overflow is component e % 4 of row spill[(e / 4) * CSLICE]
