@@ -405,10 +405,9 @@ AD void stampManifold(const State &S, int k, int body, Block &blk)
 
 // ---- per-manifold stamps, gathered by the Jacobi solve ----
 // A manifold's summed stamp is the same for every body that gathers it except for the arm
-// terms, so it is built once per iteration for both bodies (7 float4 per side, STAMP_F4 per
-// slot) and each body's solve adds its side in adjacency order: the floats are those
+// terms, so it is built once per iteration for both bodies (7 float4 per side in a record of
+// STAMP_ROWS) and each body's solve adds its side in adjacency order: the floats are those
 // stampManifold produces.
-constexpr int STAMP_F4 = 14;
 
 struct StampAcc {
     m3 lin, angA, crossA, angB, crossB;
@@ -443,24 +442,31 @@ AD void stampRows(float4 (&p)[7], const m3 &L, v3 rl, const m3 &A, const m3 &X, 
     p[6] = make_float4(ra.x, ra.y, ra.z, 0.0f);
 }
 
-// Wavefront-wide store of the active lanes' rows (7 float4 each, lane l's at dst + 7 l):
-// the rows go through LDS so that every store instruction writes one contiguous run instead
-// of 64 scattered 16-byte pieces.  The active lanes must be a prefix of the wavefront (the
-// live-list walks guarantee it); `lds` holds 7 * 64 float4 for this wavefront.  Lanes with
-// `keep` false leave their rows unwritten.
+// A side's stamp record is STAMP_ROWS float4 (the 7 of stampRows and an empty eighth), so each
+// record is one aligned 128-byte line and the primal's gather of it reads exactly two 64-byte
+// sectors (seven rows at 112-byte strides straddle three sectors in half the cases).
+constexpr int STAMP_ROWS = 8;
+constexpr int STAMP_LDS = STAMP_ROWS * 65;   // float4 of LDS per wavefront (row stride 65)
+
+// Wavefront-wide store of the active lanes' records (lane l's at dst + STAMP_ROWS l): the rows
+// go through LDS (row q of lane l at q * 65 + l, conflict free both ways) so that every store
+// instruction writes one contiguous run instead of 64 scattered 16-byte pieces.  The active
+// lanes must be a prefix of the wavefront (the live-list walks guarantee it); `lds` holds
+// STAMP_LDS float4 for this wavefront.  Lanes with `keep` false leave their record unwritten.
 AD void waveStoreRows(float4 *dst, const float4 (&rows)[7], float4 *lds, int lane, bool keep)
 {
     const int r = __popcll(__ballot(1));
     const unsigned long long kept = __ballot(keep);
 #pragma unroll
-    for (int q = 0; q < 7; q++) lds[lane * 7 + q] = rows[q];
+    for (int q = 0; q < 7; q++) lds[q * 65 + lane] = rows[q];
+    lds[7 * 65 + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-    for (int q = 0; q < 7; q++) {
-        const int idx = q * r + lane;
-        if ((kept >> (idx / 7)) & 1ull) dst[idx] = lds[idx];
+    for (int q = 0; q < STAMP_ROWS; q++) {
+        const int idx = q * r + lane, owner = idx / STAMP_ROWS;
+        if ((kept >> owner) & 1ull) dst[idx] = lds[(idx % STAMP_ROWS) * 65 + owner];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -474,9 +480,9 @@ AD void storeStampsWave(const State &S, int k, const StampAcc &s, float4 *lds, i
     const size_t k0 = static_cast<size_t>(k - lane);
     float4 rows[7];
     stampRows(rows, s.lin, s.rl, s.angA, s.crossA, s.raA);
-    waveStoreRows(S.stampA + 7 * k0, rows, lds, lane, s.movA);
+    waveStoreRows(S.stampA + STAMP_ROWS * k0, rows, lds, lane, s.movA);
     stampRows(rows, s.lin, s.rl * -1.0f, s.angB, s.crossB, s.raB * -1.0f);
-    waveStoreRows(S.stampB + 7 * k0, rows, lds, lane, s.movB);
+    waveStoreRows(S.stampB + STAMP_ROWS * k0, rows, lds, lane, s.movB);
 }
 // Symmetric 3x3 from its upper triangle (the sums are symmetric bit for bit).
 AD m3 sym3(float a00, float a01, float a02, float a11, float a12, float a22)
@@ -978,7 +984,8 @@ AD void assembleStamped(const State &S, int body, Block &blk)
     const int e0 = S.adjStart[body], e1 = S.adjStart[body + 1];
     for (int e = e0; e < e1; e++) {
         const int ref = S.adj[e];
-        if (ref >= 0) addStampSide(blk, (adjSideB(ref) ? S.stampB : S.stampA) + 7 * static_cast<size_t>(adjManifold(ref)));
+        if (ref >= 0)
+            addStampSide(blk, (adjSideB(ref) ? S.stampB : S.stampA) + STAMP_ROWS * static_cast<size_t>(adjManifold(ref)));
         else forcePrimal(S, -ref - 1, body, blk);
     }
 }

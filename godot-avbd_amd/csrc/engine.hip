@@ -913,7 +913,7 @@ struct Engine {
         // staging: sliced by narrow-phase work item (at most MAXC rows per lane of every slice)
         AL(S.sRA, CD); AL(S.sRB, CD); AL(S.sC0, CD); AL(S.sPen, CD); AL(S.sLam, CD);
         AL(S.cOff, M); AL(S.cCount, M + 1); AL(S.cScan, M + 1); AL(S.sliceCount, 1);
-        AL(S.stampA, 7 * M); AL(S.stampB, 7 * M); AL(S.slotWorld, M); AL(S.gLive, M); AL(S.wLiveOff, W + 1); AL(S.live, M); AL(S.liveTmp, M); AL(S.freeList, M);
+        AL(S.stampA, STAMP_ROWS * M); AL(S.stampB, STAMP_ROWS * M); AL(S.slotWorld, M); AL(S.gLive, M); AL(S.wLiveOff, W + 1); AL(S.live, M); AL(S.liveTmp, M); AL(S.freeList, M);
         AL(S.keep, M + 1); AL(S.keepScan, M + 1); AL(S.relScan, M + 1);
         AL(S.succA, M); AL(S.succB, M); AL(S.indeg, M); AL(S.levelOrder, M);
         AL(S.brakeLevels, W); AL(S.levelStart, M); AL(S.wLevelEnd, W);

@@ -2060,7 +2060,7 @@ __global__ void k_overflow_apply(State S)
 // Stamps of every live manifold that touches a movable body, at the current poses.
 __global__ void __launch_bounds__(TPB) k_stamp(State S)
 {
-    __shared__ float4 lds[TPB / 64][7 * 64];
+    __shared__ float4 lds[TPB / 64][STAMP_LDS];
     const int lane = threadIdx.x & 63;
     FOR_LIVE(k) {
         StampAcc acc;
@@ -2074,7 +2074,7 @@ __global__ void __launch_bounds__(TPB) k_stamp(State S)
 template <bool STAMP>
 __global__ void __launch_bounds__(TPB) k_dual(State S, float decay)
 {
-    __shared__ float4 lds[STAMP ? TPB / 64 : 1][STAMP ? 7 * 64 : 1];
+    __shared__ float4 lds[STAMP ? TPB / 64 : 1][STAMP ? STAMP_LDS : 1];
     const int lane = threadIdx.x & 63;
     FOR_LIVE(k) {
         StampAcc acc;

@@ -136,7 +136,7 @@ struct State {
     int *slotWorld;     // static: world owning slot-range position t
     int *gLive;         // all worlds' live positions t, world after world (compact)
     uint32_t *wLiveOff; // per world: offset of its live entries in gLive; [numWorlds] = total
-    float4 *stampA, *stampB;    // per view index k, 7 float4: the Jacobi stamp of each side
+    float4 *stampA, *stampB;    // per view index k, STAMP_ROWS float4: the Jacobi stamp of each side
     int *live;          // per world range: slots in live order
     int *liveTmp;
     int *freeList;      // per world range
