@@ -68,10 +68,21 @@ def rocprof_window(rows, steps):
     return rows[start:end + 1]
 
 
-def summarize_rocprof(path, steps):
+def plain_window(rows, steps):
+    """Rows of the last `steps` steps of the last world stepped in the run (a plain world run
+    after the batch, e.g. C3): from the dispatch after the rebase that ends the step before them
+    to the last rebase."""
+    rebase = [i for i, r in enumerate(rows) if r["Kernel_Name"].startswith("k_rebase")]
+    if len(rebase) < steps + 1:
+        raise SystemExit(f"only {len(rebase)} steps in the trace")
+    return rows[rebase[-steps - 1] + 1:rebase[-1] + 1]
+
+
+def summarize_rocprof(path, steps, plain=False):
     """Markdown summary of a rocprofv3 CSV (kernel trace or counter collection) over the last
-    `steps` batch steps, per kernel: launches per step, average duration, time per step, and the
-    average of each counter per launch."""
+    `steps` batch steps (or, with `plain`, the last steps of the run's last plain world), per
+    kernel: launches per step, average duration, time per step, and the average of each counter
+    per launch."""
     import csv
     import collections
     with open(path) as f:
@@ -86,7 +97,7 @@ def summarize_rocprof(path, steps):
             d = disp.setdefault(r["Dispatch_Id"], dict(r, counters={}))
             d["counters"][r["Counter_Name"]] = float(r["Counter_Value"])
         rows = list(disp.values())
-    win = rocprof_window(rows, steps)
+    win = plain_window(rows, steps) if plain else rocprof_window(rows, steps)
     agg = collections.OrderedDict()
     for r in win:
         a = agg.setdefault(r["Kernel_Name"], {"n": 0, "ns": 0.0, "c": collections.Counter()})
@@ -97,7 +108,7 @@ def summarize_rocprof(path, steps):
     total = sum(a["ns"] for a in agg.values())
     names = sorted(agg, key=lambda k: -agg[k]["ns"])
     cnames = sorted({c for a in agg.values() for c in a["c"]})
-    out = [f"window: last {steps} batch steps, {len(win)} dispatches, "
+    out = [f"window: last {steps} {'plain-world' if plain else 'batch'} steps, {len(win)} dispatches, "
            f"{total / steps / 1e6:.3f} ms of kernel time per step", "",
            "| kernel | launches/step | avg us | ms/step | share |" + "".join(f" {c}/launch |" for c in cnames),
            "|---|---|---|---|---|" + "---|" * len(cnames)]
@@ -328,13 +339,15 @@ def main():
     ap.add_argument("--summarize-rocprof", metavar="CSV",
                     help="print a per-kernel summary of a rocprofv3 CSV of a bench run and exit")
     ap.add_argument("--summary-steps", type=int, default=5)
+    ap.add_argument("--summary-plain", action="store_true",
+                    help="summarize the last plain world's steps (C3 runs last) instead of the batch's")
     ap.add_argument("--queries", action="store_true",
                     help="measure batched ray casts (the server's intersect_ray) on a settled mixed-shape pile and exit")
     ap.add_argument("--query-bodies", type=int, default=8192)
     ap.add_argument("--query-rays", type=int, default=262144)
     args = ap.parse_args()
     if args.summarize_rocprof:
-        summarize_rocprof(args.summarize_rocprof, args.summary_steps)
+        summarize_rocprof(args.summarize_rocprof, args.summary_steps, args.summary_plain)
         return
 
     if args.gpus != 1:
