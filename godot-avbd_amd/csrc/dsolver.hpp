@@ -1212,15 +1212,17 @@ AD void closestSeg(v3 p0, v3 p1, v3 q0, v3 q1, v3 &c0, v3 &c1)
     c1 = q0 + d2 * t;
 }
 
-// Returns the contact count (0 when separated beyond the margin); `basisOut` is written
-// exactly when the reference writes it.
-AD int collideBoxes(v3 pA, qt qA, v3 hA, v3 pB, qt qB, v3 hB, float margin, m3 &basisOut, bool &wroteBasis,
-                    Contacts &cs, const Poly &c0, const Poly &c1)
+// The separating-axis part of collideBoxes (collide.cpp:238-312): the best face and edge axes,
+// or false as soon as an axis separates the boxes by more than the margin (collideBoxes then
+// makes no contact).  The pair screen calls it too, so both decide with the same floats.
+struct BoxSat {
+    bool fValid, eValid;
+    int fType, fIdx, eI, eJ;
+    float fSep, eSep;
+    v3 fN, eN;
+};
+AD bool boxSat(const OBB &bA, const OBB &bB, float margin, BoxSat &s)
 {
-    cs.n = 0;
-    wroteBasis = false;
-    const OBB bA = makeOBB(pA, qA, hA);
-    const OBB bB = makeOBB(pB, qB, hB);
     const v3 delta = bB.c - bA.c;
     float M[3][3];
 #pragma unroll
@@ -1230,34 +1232,36 @@ AD int collideBoxes(v3 pA, qt qA, v3 hA, v3 pB, qt qB, v3 hB, float margin, m3 &
     const v3 dA{dot(delta, bA.ax[0]), dot(delta, bA.ax[1]), dot(delta, bA.ax[2])};
     const v3 dB{dot(delta, bB.ax[0]), dot(delta, bB.ax[1]), dot(delta, bB.ax[2])};
     // best face axis: type 0 = face of A, 1 = face of B; best edge separately.
-    bool fValid = false;
-    int fType = 0, fIdx = 0;
-    float fSep = -FLT_MAXV;
-    v3 fN{0, 0, 0};
+    s.fValid = false;
+    s.fType = 0;
+    s.fIdx = 0;
+    s.fSep = -FLT_MAXV;
+    s.fN = v3{0, 0, 0};
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const float sep = fabsf(dA[i]) -
                 (((bA.h[i] + bB.h.x * fabsf(M[i][0])) + bB.h.y * fabsf(M[i][1])) + bB.h.z * fabsf(M[i][2]));
-        if (sep > margin) return 0;
-        if (!fValid || sep > fSep) {
-            fValid = true; fType = 0; fIdx = i; fSep = sep;
-            fN = bA.ax[i] * (dA[i] < 0.0f ? -1.0f : 1.0f);
+        if (sep > margin) return false;
+        if (!s.fValid || sep > s.fSep) {
+            s.fValid = true; s.fType = 0; s.fIdx = i; s.fSep = sep;
+            s.fN = bA.ax[i] * (dA[i] < 0.0f ? -1.0f : 1.0f);
         }
     }
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         const float sep = fabsf(dB[j]) -
                 (((bB.h[j] + bA.h.x * fabsf(M[0][j])) + bA.h.y * fabsf(M[1][j])) + bA.h.z * fabsf(M[2][j]));
-        if (sep > margin) return 0;
-        if (!fValid || sep > fSep) {
-            fValid = true; fType = 1; fIdx = j; fSep = sep;
-            fN = bB.ax[j] * (dB[j] < 0.0f ? -1.0f : 1.0f);
+        if (sep > margin) return false;
+        if (!s.fValid || sep > s.fSep) {
+            s.fValid = true; s.fType = 1; s.fIdx = j; s.fSep = sep;
+            s.fN = bB.ax[j] * (dB[j] < 0.0f ? -1.0f : 1.0f);
         }
     }
-    bool eValid = false;
-    int eI = 0, eJ = 0;
-    float eSep = -FLT_MAXV;
-    v3 eN{0, 0, 0};
+    s.eValid = false;
+    s.eI = 0;
+    s.eJ = 0;
+    s.eSep = -FLT_MAXV;
+    s.eN = v3{0, 0, 0};
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -1271,12 +1275,30 @@ AD int collideBoxes(v3 pA, qt qA, v3 hA, v3 pB, qt qB, v3 hB, float margin, m3 &
             const float rb = (bB.h[j1] * fabsf(M[i][j2]) + bB.h[j2] * fabsf(M[i][j1])) * invLen;
             const float sd = dot(delta, axis) * invLen;
             const float sep = fabsf(sd) - (ra + rb);
-            if (sep > margin) return 0;
-            if (!eValid || sep > eSep) {
-                eValid = true; eI = i; eJ = j; eSep = sep;
-                eN = axis * (sd < 0.0f ? -invLen : invLen);
+            if (sep > margin) return false;
+            if (!s.eValid || sep > s.eSep) {
+                s.eValid = true; s.eI = i; s.eJ = j; s.eSep = sep;
+                s.eN = axis * (sd < 0.0f ? -invLen : invLen);
             }
         }
+    return true;
+}
+
+// Returns the contact count (0 when separated beyond the margin); `basisOut` is written
+// exactly when the reference writes it.
+AD int collideBoxes(v3 pA, qt qA, v3 hA, v3 pB, qt qB, v3 hB, float margin, m3 &basisOut, bool &wroteBasis,
+                    Contacts &cs, const Poly &c0, const Poly &c1)
+{
+    cs.n = 0;
+    wroteBasis = false;
+    const OBB bA = makeOBB(pA, qA, hA);
+    const OBB bB = makeOBB(pB, qB, hB);
+    BoxSat sat;
+    if (!boxSat(bA, bB, margin, sat)) return 0;
+    const bool fValid = sat.fValid, eValid = sat.eValid;
+    const int fType = sat.fType, fIdx = sat.fIdx, eI = sat.eI, eJ = sat.eJ;
+    const float fSep = sat.fSep, eSep = sat.eSep;
+    const v3 fN = sat.fN, eN = sat.eN;
     if (!fValid) return 0;
     const bool useEdge = eValid && 0.95f * eSep > fSep + 0.01f;
     const v3 nAB = useEdge ? eN : fN;

@@ -885,7 +885,7 @@ struct Engine {
         AL(S.fric, N); AL(S.grav, N); AL(S.layer, N); AL(S.mask, N); AL(S.lockLin, N); AL(S.lockAng, N); AL(S.shape, N);
         AL(S.sleepMode, N); AL(S.sleeping, N); AL(S.stillFrames, N); AL(S.updFlag, N + 1); AL(S.updScan, N + 1);
         AL(S.mDormant, M); AL(S.mSaved, M); AL(S.dRA, M * MAXC); AL(S.dRB, M * MAXC); AL(S.dC0, M * MAXC);
-        AL(S.dPen, M * MAXC); AL(S.pairParked, P); AL(rimDev, 16); S.rim = rimDev;
+        AL(S.dPen, M * MAXC); AL(S.pairParked, P); AL(S.pairApart, P); AL(rimDev, 16); S.rim = rimDev;
         if (!sizing) {
             // collide.cpp:733-753: rim directions at angle steps i*5 mod 8, from the C library's
             // cosf / sinf at run time (the values the reference's table holds)
@@ -2234,6 +2234,23 @@ struct Engine {
         return AVBD_OK;
     }
 
+    // Pairs of the world's last step that the pair filter found apart: the reference makes and
+    // releases a manifold for each, so they count as new manifolds.
+    avbd_status apartPairs(const WorldInfo &wi, int &n)
+    {
+        n = 0;
+        uint32_t r[2] = {0, 0};
+        HIPCHK(hipMemcpy(&r[0], S.pairOff + wi.bodyBase, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&r[1], S.pairOff + wi.bodyBase + wi.bodyCount, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        const uint32_t a = std::min<uint32_t>(r[0], static_cast<uint32_t>(Pcap));
+        const uint32_t b = std::min<uint32_t>(r[1], static_cast<uint32_t>(Pcap));
+        if (b <= a) return AVBD_OK;
+        std::vector<uint8_t> f(b - a);
+        HIPCHK(hipMemcpy(f.data(), S.pairApart + a, b - a, hipMemcpyDeviceToHost));
+        for (uint8_t x : f) n += x;
+        return AVBD_OK;
+    }
+
     avbd_status fillStats(int w, avbd_step_stats *out)
     {
         HIPCHK(hipStreamSynchronize(stream));
@@ -2245,8 +2262,10 @@ struct Engine {
         HIPCHK(hipMemcpy(&v[0], S.wLive + w, sizeof(int), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(&v[1], S.wNew + w, sizeof(int), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(&v[2], S.brakeLevels + w, sizeof(int), hipMemcpyDeviceToHost));
+        int apart = 0;
+        if (avbd_status st = apartPairs(wi, apart); st != AVBD_OK) return st;
         s.manifolds = v[0];
-        s.new_manifolds = v[1];
+        s.new_manifolds = v[1] + apart;
         s.brake_levels = v[2];
         if (v[0] > 0) {
             std::vector<int> live(v[0]);

@@ -887,6 +887,45 @@ __device__ __forceinline__ int pairTotal(const State &S)
     return t < static_cast<uint32_t>(S.pairCap) ? static_cast<int>(t) : S.pairCap;
 }
 
+// collide.cpp:1107-1117: the speculative margin grown by the pair's predicted relative motion,
+// from the two bodies' narrow-phase records ({pos, mass}, rot, {size, radius}, {vel, gravity},
+// {angVel, friction}).
+AD float pairMargin(const State &S, float ma, float4 a2, float4 a3, float4 a4, float mb, float4 b2, float4 b3,
+                    float4 b4)
+{
+    float margin = S.P.specMargin;
+    if (margin > 0.0f) {
+        const float hz = S.P.collisionDt;
+        const float gA = ma > 0.0f ? a3.w : 0.0f;
+        const float gB = mb > 0.0f ? b3.w : 0.0f;
+        const v3 disp = (ld3(a3) - ld3(b3)) * hz + v3{0, 0, (gA - gB) * hz * hz};
+        margin += length(disp) + (a2.w * length(ld3(a4)) + b2.w * length(ld3(b4))) * hz;
+    }
+    return margin;
+}
+
+// A new pair of boxes that a separating axis parts by more than the speculative margin: the
+// narrow phase would make its manifold with no contact (collide.cpp:238-312) and release it in
+// the same step (solver_solve.cpp:66-87), so no manifold is made.  Same inputs and floats as
+// the narrow phase's own test.  Pairs with no awake movable side are left to the narrow phase,
+// which keeps their manifolds; a side asleep now may still be woken before it runs, which only
+// turns such a pair from frozen to awake, so an awake pair here is awake there.
+AD bool boxesApart(const State &S, int i, int j)
+{
+    const float4 *ra = S.narrowRec + NARROW_REC * static_cast<size_t>(i);
+    const float4 *rb = S.narrowRec + NARROW_REC * static_cast<size_t>(j);
+    const float4 a5 = ra[5], b5 = rb[5];
+    if (__float_as_int(a5.x) != AVBD_SHAPE_BOX || __float_as_int(b5.x) != AVBD_SHAPE_BOX) return false;
+    if (!(a5.w > 0.0f) && !(b5.w > 0.0f)) return false;
+    const float4 a0 = ra[0], a1 = ra[1], a2 = ra[2], a3 = ra[3], a4 = ra[4];
+    const float4 b0 = rb[0], b1 = rb[1], b2 = rb[2], b3 = rb[3], b4 = rb[4];
+    const float margin = pairMargin(S, a0.w, a2, a3, a4, b0.w, b2, b3, b4);
+    const OBB bA = makeOBB(ld3(a0), ldq(a1), v3{a2.x, a2.y, a2.z} * 0.5f);
+    const OBB bB = makeOBB(ld3(b0), ldq(b1), v3{b2.x, b2.y, b2.z} * 0.5f);
+    BoxSat sat;
+    return !boxSat(bA, bB, margin, sat);
+}
+
 __device__ void pairFilter(const State &S, int p);
 __global__ void k_pair_filter(State S)
 {
@@ -921,7 +960,11 @@ __device__ void pairFilter(const State &S, int p)
         }
     }
     const bool pass = ov && layers && !bothStatic && !constrained;
-    S.pairValid[p] = (pass && !parked) ? 1u : 0u;
+    // contact grace keeps a new manifold with no contact (parked), so only without it can a
+    // separated pair skip the narrow phase
+    const bool apart = pass && !parked && S.P.grace == 0 && boxesApart(S, i, j);
+    S.pairValid[p] = (pass && !parked && !apart) ? 1u : 0u;
+    S.pairApart[p] = apart ? 1 : 0;
     if (S.P.grace > 0) S.pairParked[p] = pass && parked ? 1 : 0;
 }
 
@@ -1184,14 +1227,7 @@ template <int MODE> AD void manifoldInit(const State &S, int q, int P, NarrowPol
     const qt qa = ldq(a1), qb = ldq(b1);
     const float4 sa = a2, sb = b2;
     // speculative margin grown by the pair's predicted relative motion
-    float margin = S.P.specMargin;
-    if (margin > 0.0f) {
-        const float hz = S.P.collisionDt;
-        const float gA = ma > 0.0f ? a3.w : 0.0f;
-        const float gB = mb > 0.0f ? b3.w : 0.0f;
-        const v3 disp = (ld3(a3) - ld3(b3)) * hz + v3{0, 0, (gA - gB) * hz * hz};
-        margin += length(disp) + (sa.w * length(ld3(a4)) + sb.w * length(ld3(b4))) * hz;
-    }
+    const float margin = pairMargin(S, ma, a2, a3, a4, mb, b2, b3, b4);
     m3 B;
     bool wrote;
     const size_t d = stageRow(q, 0);
@@ -1905,7 +1941,8 @@ __global__ void k_wake_pairs(State S)
     const int base = S.wBodyBase[w];
     const uint32_t p0 = S.pairOff[base], p1 = S.pairOff[base + S.wBodyCount[w]];
     for (uint32_t p = p0; p < p1; p++)
-        if (S.pairValid[p] || (S.P.grace > 0 && S.pairParked[p])) wakePair(S, S.pairI[p], S.pairJ[p]);
+        if (S.pairValid[p] || S.pairApart[p] || (S.P.grace > 0 && S.pairParked[p]))
+            wakePair(S, S.pairI[p], S.pairJ[p]);
 }
 // solver_solve.cpp:264-289: with idle detection on, a sleeper (mode IDLE) sharing a force with
 // an awake movable body that moves faster than the thresholds wakes; bodies in list order,

@@ -197,6 +197,9 @@ struct State {
     int *listStats;
     int *wRoot;                  // per world: node whose leaf range is exactly the world
     uint32_t *pairValid, *pairValidScan;
+    // per pair: a new pair of boxes parted beyond the margin (no manifold is made for it, though
+    // the reference makes one and releases it in the same step; it still wakes a sleeper)
+    uint8_t *pairApart;
     float4 *wBoundsLo, *wBoundsHi;
 };
 
