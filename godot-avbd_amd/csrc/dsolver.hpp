@@ -353,10 +353,14 @@ AD void stampSide(const Hess &h, const m3 &Sm, v3 r, v3 fw, m3 &accAng, m3 &accC
     accRA += cross(r, fw);
 }
 
-// One manifold's contribution to one body's block (manifold.cpp:650-686 force + :688-755
-// stamp, fused).  Forces are recomputed at the current poses; the reference's cache holds
-// the same floats because nothing it reads moves between fill and use.
-AD void stampManifold(const State &S, int k, int body, Block &blk)
+// One manifold's contribution to one body's block (manifold.cpp:474-497 force + :503-568
+// stamp, fused).  Forces are recomputed at the current poses; the reference's force cache
+// holds the same floats because nothing it reads moves between fill and use.  ARMS (arm reuse
+// on): a round's first touch after the first round (`cachedRound`, the other body not solved
+// earlier in this round) takes the world arms the last dual left, as the reference's cached
+// stamp does (manifold.cpp:570-581); every other touch computes them and stores them.
+template <bool ARMS>
+AD void stampManifold(const State &S, int k, int body, Block &blk, bool cachedRound = false, int colour = 0)
 {
     const int4 hd = S.mHdr[k];
     const int a = hd.x, b = hd.y, n = hd.w;
@@ -372,6 +376,13 @@ AD void stampManifold(const State &S, int k, int body, Block &blk)
     const v3 b0 = row(B, 0), b1 = row(B, 1), b2 = row(B, 2);
     m3 accLin = zero3(), accAng = zero3(), accCross = zero3();
     v3 accRL{0, 0, 0}, accRA{0, 0, 0};
+    bool cached = false;
+    size_t arm0 = 0;
+    if (ARMS) {
+        const int o = fromA ? b : a;
+        cached = cachedRound && !(S.updateIndex[o] >= 0 && S.colour[o] < colour);
+        arm0 = static_cast<size_t>(S.kSlot[k]) * MAXC;
+    }
     // contact i + 1 is loaded while contact i is computed
     float4 nPL{}, nRA{}, nRB{}, nC0{};
     float2 nLyz{};
@@ -381,8 +392,15 @@ AD void stampManifold(const State &S, int k, int body, Block &blk)
         const float4 pl4 = nPL, ra4 = nRA, rb4 = nRB, c04 = nC0;
         const float2 lyz = nLyz;
         if (i + 1 < n) { const uint32_t d = c + CSLICE; nPL = S.cPL[d]; nRA = S.cRA[d]; nRB = S.cRB[d]; nC0 = S.cC0[d]; nLyz = S.cLyz[d]; }
-        const v3 rA = mul(rotA, ld3(ra4));
-        const v3 rB = mul(rotB, ld3(rb4));
+        v3 rA, rB;
+        if (ARMS && cached) {
+            rA = ld3(S.armA[arm0 + i]);
+            rB = ld3(S.armB[arm0 + i]);
+        } else {
+            rA = mul(rotA, ld3(ra4));
+            rB = mul(rotB, ld3(rb4));
+            if (ARMS) { S.armA[arm0 + i] = st3(rA); S.armB[arm0 + i] = st3(rB); }
+        }
         const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
         const v3 C = constraintError(S, ld3(c04), br, B);
         const v3 pen = ld3(pl4), lam{pl4.w, lyz.x, lyz.y};
@@ -499,8 +517,16 @@ AD void addStampSide(Block &blk, const float4 *p)
     blk.rhsAng += v3{p6.x, p6.y, p6.z};
 }
 
+// A manifold the reference's overflow phase A precomputes (solver_step.cpp:235-246): one of
+// its bodies is in the update set and in the overflow bucket.
+AD bool phaseA(const State &S, int a, int b)
+{
+    return (S.updateIndex[a] >= 0 && S.colour[a] == 8) || (S.updateIndex[b] >= 0 && S.colour[b] == 8);
+}
+
 // Both sides' stamps of one manifold at the current poses (zero for a frozen one, which no
-// body gathers).
+// body gathers).  ARMS: the arms are stored for the manifolds the reference precomputes.
+template <bool ARMS>
 AD void manifoldStamps(const State &S, int k, StampAcc &acc)
 {
     const int4 hd = S.mHdr[k];
@@ -513,10 +539,13 @@ AD void manifoldStamps(const State &S, int k, StampAcc &acc)
     if (!(A.mass > 0.0f) && !(Bd.mass > 0.0f)) return;
     const v3 dqAL = A.dqL, dqAA = A.dqA, dqBL = Bd.dqL, dqBA = Bd.dqA;
     const m3 rotA = qmat(A.q), rotB = qmat(Bd.q);
+    const bool store = ARMS && phaseA(S, a, b);
+    const size_t arm0 = ARMS ? static_cast<size_t>(S.kSlot[k]) * MAXC : 0;
     for (int i = 0; i < n; i++) {
         const uint32_t c = cb + i * CSLICE;
         const v3 rA = mul(rotA, ld3(S.cRA[c]));
         const v3 rB = mul(rotB, ld3(S.cRB[c]));
+        if (store) { S.armA[arm0 + i] = st3(rA); S.armB[arm0 + i] = st3(rB); }
         const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
         const v3 C = constraintError(S, ld3(S.cC0[c]), br, B);
         v3 pen, lam;
@@ -528,11 +557,14 @@ AD void manifoldStamps(const State &S, int k, StampAcc &acc)
     }
 }
 
-// manifold.cpp:768-856 (arm caching off): lambda <- clamped force, penalty ramps.  With
-// STAMP, the stamps of the next primal iteration are built from the updated lambda and
-// penalty: the poses do not move between this dual and that primal, so its constraint
-// error is the one computed here.
-template <bool STAMP>
+// manifold.cpp:583-671: lambda <- clamped force, penalty ramps.  With STAMP, the stamps of
+// the next primal iteration are built from the updated lambda and penalty: the poses do not
+// move between this dual and that primal, so its constraint error is the one computed here.
+// ARMS (arm reuse on): while neither body's first rotation column has moved farther than the
+// rebuild distance since the arms were captured, the world arms stored last are used (the
+// stamps still take arms at the current pose, as the reference's phase A recomputes them);
+// otherwise the arms are computed, stored and captured.
+template <bool STAMP, bool ARMS>
 AD void manifoldDual(const State &S, int k, float decay, StampAcc &acc)
 {
     // every load that does not depend on another is issued before the first wait; the
@@ -552,6 +584,20 @@ AD void manifoldDual(const State &S, int k, float decay, StampAcc &acc)
     const v3 dqAL = A.dqL, dqAA = A.dqA, dqBL = Bd.dqL, dqBA = Bd.dqA;
     const m3 rotA = qmat(A.q), rotB = qmat(Bd.q);
     const float pmax = S.P.penaltyMax;
+    bool usable = false, store = false;
+    size_t arm0 = 0;
+    if (ARMS) {
+        const int slot = S.kSlot[k];
+        arm0 = static_cast<size_t>(slot) * MAXC;
+        const float4 capA = S.armCap[2 * static_cast<size_t>(slot)], capB = S.armCap[2 * static_cast<size_t>(slot) + 1];
+        const v3 dA = rotA.c[0] - ld3(capA), dB = rotB.c[0] - ld3(capB);
+        usable = capA.w != 0.0f && length2(dA) <= S.P.armRebuild2 && length2(dB) <= S.P.armRebuild2;
+        store = STAMP && phaseA(S, a, b);
+        if (!usable) {
+            S.armCap[2 * static_cast<size_t>(slot)] = st3(rotA.c[0], 1.0f);
+            S.armCap[2 * static_cast<size_t>(slot) + 1] = st3(rotB.c[0], 0.0f);
+        }
+    }
     for (int i = 0; i < n; i++) {
         const uint32_t c = cb + i * CSLICE;
         const float4 pl4 = nPL, ra4 = nRA, rb4 = nRB, c0in = nC0;
@@ -559,8 +605,15 @@ AD void manifoldDual(const State &S, int k, float decay, StampAcc &acc)
         if (i + 1 < n) { const uint32_t d = c + CSLICE; nPL = S.cPL[d]; nRA = S.cRA[d]; nRB = S.cRB[d]; nC0 = S.cC0[d]; nLyz = S.cLyz[d]; }
         v3 pen = ld3(pl4);
         if (decay < 1.0f) pen = lclamp3(pen * decay, PENALTY_MIN, pmax);
-        const v3 rA = mul(rotA, ld3(ra4));
-        const v3 rB = mul(rotB, ld3(rb4));
+        v3 rA, rB;
+        if (ARMS && usable) {
+            rA = ld3(S.armA[arm0 + i]);
+            rB = ld3(S.armB[arm0 + i]);
+        } else {
+            rA = mul(rotA, ld3(ra4));
+            rB = mul(rotB, ld3(rb4));
+            if (ARMS) { S.armA[arm0 + i] = st3(rA); S.armB[arm0 + i] = st3(rB); }
+        }
         const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
         const v3 C = constraintError(S, ld3(c0in), br, B);
         v3 F = cmul(pen, C) + v3{pl4.w, lyz.x, lyz.y};
@@ -575,10 +628,21 @@ AD void manifoldDual(const State &S, int k, float decay, StampAcc &acc)
         }
         storePenLam(S, c, pen, F);
         if (STAMP) {
-            v3 f = cmul(pen, C) + F;
-            float bd;
-            clampForce(f, mu, bd);
-            stampContact(acc, B, pen, f, rA, rB);
+            if (ARMS && usable) {
+                // the next round's phase A at the current pose
+                const v3 fA = mul(rotA, ld3(ra4)), fB = mul(rotB, ld3(rb4));
+                const v3 fbr = (dqAL - dqBL) + cross(dqAA, fA) - cross(dqBA, fB);
+                v3 f = cmul(pen, constraintError(S, ld3(c0in), fbr, B)) + F;
+                float bd;
+                clampForce(f, mu, bd);
+                stampContact(acc, B, pen, f, fA, fB);
+                if (store) { S.armA[arm0 + i] = st3(fA); S.armB[arm0 + i] = st3(fB); }
+            } else {
+                v3 f = cmul(pen, C) + F;
+                float bd;
+                clampForce(f, mu, bd);
+                stampContact(acc, B, pen, f, rA, rB);
+            }
         }
     }
 }
@@ -966,13 +1030,14 @@ AD bool adjSideB(int e) { return (e & ADJ_SIDE_B) != 0; }
 
 // Block of one body: inertia plus every incident constraint, in adjacency order (joints in
 // list order, then manifolds in live order; manifold.cpp:123-166, solver_step.cpp:215-226).
-AD void assemble(const State &S, int body, Block &blk)
+template <bool ARMS>
+AD void assemble(const State &S, int body, Block &blk, bool cachedRound = false, int colour = 0)
 {
     makeBlock(S, body, blk);
     const int e0 = S.adjStart[body], e1 = S.adjStart[body + 1];
     for (int e = e0; e < e1; e++) {
         const int ref = S.adj[e];
-        if (ref >= 0) stampManifold(S, adjManifold(ref), body, blk);
+        if (ref >= 0) stampManifold<ARMS>(S, adjManifold(ref), body, blk, cachedRound, colour);
         else forcePrimal(S, -ref - 1, body, blk);
     }
 }

@@ -116,6 +116,8 @@ void setParams(Params &P, const avbd_params &p, float collisionDt)
     P.sleepLin = p.sleep_linear_threshold;
     P.sleepAng = p.sleep_angular_threshold;
     P.grace = p.contact_grace_ticks > 0 ? p.contact_grace_ticks : 0;
+    P.armCache = p.jacobian_rebuild_distance > 0.0f ? 1 : 0;
+    P.armRebuild2 = p.jacobian_rebuild_distance * p.jacobian_rebuild_distance;
 }
 
 avbd_status checkParams(const avbd_params *p)
@@ -123,8 +125,6 @@ avbd_status checkParams(const avbd_params *p)
     if (!p) return fail(AVBD_ERR_INVALID, "params is null");
     if (!(p->dt > 0.0f)) return fail(AVBD_ERR_INVALID, "dt must be positive");
     if (p->iterations < 1) return fail(AVBD_ERR_INVALID, "iterations must be >= 1");
-    if (p->jacobian_rebuild_distance > 0.0f)
-        return fail(AVBD_ERR_UNSUPPORTED, "jacobian arm caching is outside this path");
     return AVBD_OK;
 }
 
@@ -251,6 +251,16 @@ struct Engine {
         e = hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T));
         if (e != hipSuccess) return fail(AVBD_ERR_DEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
         return AVBD_OK;
+    }
+    // Arm-cache arrays, allocated the first time arm reuse is switched on (then kept).
+    avbd_status ensureArmCache(const avbd_params &p)
+    {
+        if (S.armCap || !(p.jacobian_rebuild_distance > 0.0f)) return AVBD_OK;
+        const size_t M = Scap;
+        avbd_status st;
+        if ((st = A(S.armCap, 2 * M)) != AVBD_OK) return st;
+        if ((st = A(S.armA, M * MAXC)) != AVBD_OK) return st;
+        return A(S.armB, M * MAXC);
     }
     void fill(int *p, int v, int n)
     {
@@ -886,6 +896,10 @@ struct Engine {
         AL(S.sleepMode, N); AL(S.sleeping, N); AL(S.stillFrames, N); AL(S.updFlag, N + 1); AL(S.updScan, N + 1);
         AL(S.mDormant, M); AL(S.mSaved, M); AL(S.dRA, M * MAXC); AL(S.dRB, M * MAXC); AL(S.dC0, M * MAXC);
         AL(S.dPen, M * MAXC); AL(S.pairParked, P); AL(S.pairApart, P); AL(rimDev, 16); S.rim = rimDev;
+        S.armCap = S.armA = S.armB = nullptr;
+        if (p.jacobian_rebuild_distance > 0.0f) {
+            AL(S.armCap, 2 * M); AL(S.armA, M * MAXC); AL(S.armB, M * MAXC);
+        }
         if (!sizing) {
             // collide.cpp:733-753: rim directions at angle steps i*5 mod 8, from the C library's
             // cosf / sinf at run time (the values the reference's table holds)
@@ -1241,6 +1255,8 @@ struct Engine {
         // which no round changes, so residual >= 0 and 0 - residual < threshold always holds: with
         // a positive threshold the reference stops after the first round of every substep.
         const int runRounds = prm.convergence_threshold > 0.0f && rounds > 1 ? 1 : rounds;
+        // manifold.cpp:600-607: drift-gated arm reuse
+        const bool arms = prm.jacobian_rebuild_distance > 0.0f;
         for (int sub = 0; sub < subs; sub++) {
             if (sub > 0) {
                 k_rebase<<<liveGrid(), TPB, 0, stream>>>(S);
@@ -1263,13 +1279,19 @@ struct Engine {
                     if (withColours)
                         for (int c = 0; c < 8; c++) {
                             pbeg(AVBD_K_COLOUR_PASS);
-                            k_colour_pass<<<colourGrid(), COLOUR_PASS_TPB, 0, stream>>>(S, c);
+                            if (arms)
+                                k_colour_pass<true><<<colourGrid(), COLOUR_PASS_TPB, 0, stream>>>(S, c, it > 0 ? 1 : 0);
+                            else
+                                k_colour_pass<false><<<colourGrid(), COLOUR_PASS_TPB, 0, stream>>>(S, c, 0);
                             pend();
                         }
                     if (stampHere) {
                         k_pose_pack<<<nb, TPB, 0, stream>>>(S);
                         pbeg(AVBD_K_STAMP);
-                        k_stamp<<<liveGrid(tune.stamp), tune.stamp, 0, stream>>>(S);
+                        if (arms)
+                            k_stamp<true><<<liveGrid(tune.stamp), tune.stamp, 0, stream>>>(S);
+                        else
+                            k_stamp<false><<<liveGrid(tune.stamp), tune.stamp, 0, stream>>>(S);
                         pend();
                     }
                     pbeg(AVBD_K_PRIMAL);
@@ -1289,10 +1311,14 @@ struct Engine {
                     }
                 }
                 pbeg(AVBD_K_DUAL);
-                if (stampInDual)
-                    k_dual<true><<<liveGrid(tune.dual), tune.dual, 0, stream>>>(S, newton ? 1.0f : prm.stiffness_decay);
-                else
-                    k_dual<false><<<liveGrid(tune.dual), tune.dual, 0, stream>>>(S, newton ? 1.0f : prm.stiffness_decay);
+                const float decay = newton ? 1.0f : prm.stiffness_decay;
+                if (arms) {
+                    if (stampInDual) k_dual<true, true><<<liveGrid(tune.dual), tune.dual, 0, stream>>>(S, decay);
+                    else k_dual<false, true><<<liveGrid(tune.dual), tune.dual, 0, stream>>>(S, decay);
+                } else {
+                    if (stampInDual) k_dual<true, false><<<liveGrid(tune.dual), tune.dual, 0, stream>>>(S, decay);
+                    else k_dual<false, false><<<liveGrid(tune.dual), tune.dual, 0, stream>>>(S, decay);
+                }
                 pend();
             }
             pbeg(AVBD_K_BODY);
@@ -2088,6 +2114,7 @@ struct Engine {
         const int sb = wi.slotBase, o = wi.bodyBase;
         fill(S.mDormant + sb, 0, wi.slotCap);   // restored manifolds are not parked
         fill(S.mSaved + sb, 0, wi.slotCap);
+        if (S.armCap) fill(reinterpret_cast<int *>(S.armCap + 2 * static_cast<size_t>(sb)), 0, 8 * wi.slotCap);   // no captured arms
         std::vector<int> live(n), mA(n), mB(n), num(n);
         std::vector<float> fr(n);
         std::vector<float4> basis(3 * n), cra(n * MAXC), crb(n * MAXC), cc0(n * MAXC), cp(n * MAXC), cl(n * MAXC);
@@ -2463,6 +2490,7 @@ avbd_status avbd_world_set_params(avbd_world *world, const avbd_params *params)
     if (!world) return fail(AVBD_ERR_INVALID, "world is null");
     avbd_status st = checkParams(params);
     if (st != AVBD_OK) return st;
+    if ((st = world->eng.ensureArmCache(*params)) != AVBD_OK) return st;
     world->eng.prm = *params;
     return AVBD_OK;
 }

@@ -1209,6 +1209,7 @@ template <int MODE> AD void manifoldInit(const State &S, int q, int P, NarrowPol
             S.keep[t] = 1;
             return;
         }
+        if (S.armCap) S.armCap[2 * static_cast<size_t>(slot)].w = 0.0f;   // manifold.cpp:246-247
     } else if (!ROUND) {
         S.sPos[slot] = q;
         if (frozen) {   // parked verbatim
@@ -1216,6 +1217,8 @@ template <int MODE> AD void manifoldInit(const State &S, int q, int P, NarrowPol
             S.keep[t] = 1;
             return;
         }
+        // manifold.cpp:246-247: a new contact set drops the captured arms
+        if (S.armCap) S.armCap[2 * static_cast<size_t>(slot)].w = 0.0f;
         if (round) return;
     } else {
         S.sPos[slot] = q;
@@ -2021,14 +2024,16 @@ __global__ void k_colour_starts(State S, const uint32_t *offs, int nb)
 }
 
 // solver_step.cpp:209-229: one colour of the Gauss-Seidel sweep, over that colour's run.
-// The pivoted solve (rare) runs in place: the next colour reads this one's poses.
-__global__ void __launch_bounds__(TPB) k_colour_pass(State S, int colour)
+// The pivoted solve (rare) runs in place: the next colour reads this one's poses.  ARMS: arm
+// reuse is on; `cachedRound` = a round after the substep's first.
+template <bool ARMS>
+__global__ void __launch_bounds__(TPB) k_colour_pass(State S, int colour, int cachedRound)
 {
     const uint32_t begin = S.colStart[colour], end = S.colStart[colour + 1];
     for (uint32_t r = begin + GID; r < end; r += gridDim.x * blockDim.x) {
         const int i = S.colValAlt[r];
         Block blk;
-        assemble(S, i, blk);
+        assemble<ARMS>(S, i, blk, cachedRound != 0, colour);
         v3 dl, da;
         if (!solveDeltaFast(blk, dl, da)) solveDeltaSlow(blk, dl, da);
         S.pos[i] = st3(bPos(S, i) + dl);
@@ -2095,27 +2100,28 @@ __global__ void k_overflow_apply(State S)
 }
 
 // Stamps of every live manifold that touches a movable body, at the current poses.
+template <bool ARMS>
 __global__ void __launch_bounds__(TPB) k_stamp(State S)
 {
     __shared__ float4 lds[TPB / 64][STAMP_LDS];
     const int lane = threadIdx.x & 63;
     FOR_LIVE(k) {
         StampAcc acc;
-        manifoldStamps(S, k, acc);
+        manifoldStamps<ARMS>(S, k, acc);
         storeStampsWave(S, k, acc, lds[threadIdx.x >> 6], lane);
     }
 }
 
 // solver_step.cpp:277-296: manifolds (not frozen), then joints.  STAMP: also build the next
 // Jacobi iteration's stamps.
-template <bool STAMP>
+template <bool STAMP, bool ARMS>
 __global__ void __launch_bounds__(TPB) k_dual(State S, float decay)
 {
     __shared__ float4 lds[STAMP ? TPB / 64 : 1][STAMP ? STAMP_LDS : 1];
     const int lane = threadIdx.x & 63;
     FOR_LIVE(k) {
         StampAcc acc;
-        manifoldDual<STAMP>(S, k, decay, acc);
+        manifoldDual<STAMP, ARMS>(S, k, decay, acc);
         if (STAMP) storeStampsWave(S, k, acc, lds[threadIdx.x >> 6], lane);
     }
     for (int t = GID; t < S.jointCap; t += gridDim.x * blockDim.x)

@@ -32,6 +32,10 @@ struct Params {
     int sleepOn, sleepFrames;
     int grace;   // contact grace ticks (0: a separated pair's manifold is released at once)
     float sleepLin, sleepAng;
+    // drift-gated arm reuse (manifold.cpp:600-607): on when the rebuild distance is positive;
+    // armRebuild2 = distance squared
+    int armCache;
+    float armRebuild2;
 };
 
 constexpr int NARROW_REC = 8;
@@ -66,6 +70,11 @@ struct State {
     // rows: arm A + beta scale, arm B + key, C0, penalty); per pair, "has a parked manifold"
     int *mDormant, *mSaved;
     float4 *dRA, *dRB, *dC0, *dPen;
+    // arm cache (allocated once arm reuse is first switched on, else null): per slot the first
+    // rotation columns of both bodies when the arms were captured (armCap[2 slot] = {column of
+    // A, 1 when captured}, armCap[2 slot + 1] = {column of B, 0}); per contact (slot * MAXC + i)
+    // the world arms computed last (armA, armB)
+    float4 *armCap, *armA, *armB;
     uint8_t *pairParked;
     const float *rim;     // cylinder rim directions: 8 cosines then 8 sines (host libm values)
     int *bodyWorld;

@@ -67,7 +67,7 @@ typedef struct avbd_params {
     int32_t contact_only_overflow;   /* 1 */
     float stick_drift_heal;          /* 0.002 */
     float max_penetration;           /* 0.02 */
-    float jacobian_rebuild_distance; /* 0 (only 0 is supported) */
+    float jacobian_rebuild_distance; /* 0 = off; > 0: duals reuse world arms until a rotation drifts this far */
     float speculative_margin;        /* 0.01 */
     int32_t position_projection;     /* 1 */
     int32_t projection_iterations;   /* 4 */

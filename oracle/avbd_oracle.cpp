@@ -16,8 +16,9 @@
 // in the reference's floating-point evaluation order, serially (the reference's results do
 // not depend on its thread count: test/core_test.cpp:612-681).
 //
-// Out of scope (rejected with AVBD_ERR_UNSUPPORTED): jacobian arm caching, the convergence
-// early exit.
+// The manifold primal follows the reference's cache state machine literally (force cache,
+// world arms, the PRIMAL_READY and ARMS_CACHED flags: manifold.cpp:440-671, solver_step.cpp:
+// 209-296), including drift-gated arm reuse (jacobianRebuildDistance > 0).
 #include "avbd_oracle.h"
 
 #include <algorithm>
@@ -990,6 +991,14 @@ struct World {
     std::vector<f3> C0, pen, lam;
     std::vector<uint8_t> stick;
     std::vector<float> betaScale;
+    // Drift-gated arm cache (manifold.hpp:76-97): per slot the flags (ARMS_CACHED = 1,
+    // PRIMAL_READY = 2) and the first columns of both bodies' rotation matrices when the arms
+    // were captured; per contact the world arms last computed and the primal's force cache.
+    std::vector<uint8_t> mFlags;
+    std::vector<f3> capA, capB;
+    std::vector<f3> rAW, rBW, fCache;
+    // test hooks: dual evaluations that reused cached arms, primal touches served by the cache
+    long long armHits = 0, cachedTouches = 0;
     std::vector<uint32_t> live, freeList;
     // Adjacency CSR by body index: joints (list order) then manifolds (live order).
     // An entry >= 0 is a manifold slot, < 0 is joint -(j+1).
@@ -1013,15 +1022,19 @@ struct World {
             s = static_cast<uint32_t>(mA.size());
             mA.push_back(0); mB.push_back(0); basis.push_back(zero3()); numC.push_back(0); mFric.push_back(0);
             dormant.push_back(0); saved.push_back(0);
+            mFlags.push_back(0); capA.push_back(f3{0, 0, 0}); capB.push_back(f3{0, 0, 0});
             cont.resize(cont.size() + MAXC); C0.resize(C0.size() + MAXC); pen.resize(pen.size() + MAXC);
             lam.resize(lam.size() + MAXC); stick.resize(stick.size() + MAXC); betaScale.resize(betaScale.size() + MAXC);
+            rAW.resize(rAW.size() + MAXC); rBW.resize(rBW.size() + MAXC); fCache.resize(fCache.size() + MAXC);
         }
         mA[s] = a; mB[s] = b; basis[s] = zero3(); numC[s] = 0; mFric[s] = 0.0f;
         dormant[s] = 0; saved[s] = 0;
+        mFlags[s] = 0; capA[s] = capB[s] = f3{0, 0, 0};   // manifold.cpp:55-80
         for (int k = 0; k < MAXC; k++) {
             const int i = s * MAXC + k;
             cont[i] = Contact{0, {0, 0, 0}, {0, 0, 0}};
             C0[i] = pen[i] = lam[i] = f3{0, 0, 0};
+            rAW[i] = rBW[i] = fCache[i] = f3{0, 0, 0};
             stick[i] = 0; betaScale[i] = 1.0f;
         }
         live.push_back(s);
@@ -1234,11 +1247,14 @@ struct World {
         const Body &a = bodies[mA[s]], &b = bodies[mB[s]];
         const int first = static_cast<int>(s) * MAXC;
         mFric[s] = std::sqrt(a.friction * b.friction);
+        // manifold.cpp:246-247: a new contact set drops the cached arms and forces
+        mFlags[s] = static_cast<uint8_t>(mFlags[s] & (g_plant == 24 ? ~PRIMAL_READY : ~(ARMS_CACHED | PRIMAL_READY)));
         // manifold.cpp:249-253: a parked manifold re-enters with the contacts it kept
         if (numC[s] == 0 && saved[s] > 0) numC[s] = saved[s];
         const int liveCount = numC[s];
         Contact nc[MAXC] = {};
         float bas[9];
+        bas[0] = std::numeric_limits<float>::quiet_NaN();   // written only by a collision that writes the basis
         int32_t keys[8]; float ra[24], rb[24];
         const float pa[3] = {a.pos.x, a.pos.y, a.pos.z}, qa[4] = {a.q.x, a.q.y, a.q.z, a.q.w},
                     sa[3] = {a.size.x, a.size.y, a.size.z};
@@ -1247,9 +1263,11 @@ struct World {
         const int nn = oracle_collide_shapes(a.shape, pa, qa, sa, b.shape, pb, qb, sb, marginFor(a, b), bas, keys, ra, rb);
         for (int k = 0; k < nn; k++)
             nc[k] = Contact{keys[k], {ra[3 * k], ra[3 * k + 1], ra[3 * k + 2]}, {rb[3 * k], rb[3 * k + 1], rb[3 * k + 2]}};
-        // The reference writes the basis only when the pair is within the margin on every
-        // axis; a pair with no contact is released before its basis is read again.
-        if (nn > 0)
+        // The reference's box pairs write the basis once the pair is within the margin on every
+        // axis, even when clipping leaves no contact (collide.cpp:1089); round pairs write it
+        // with their contact.  A pair left without contacts is released, or parked under
+        // contact grace with this basis.
+        if (!std::isnan(bas[0]))
             for (int r = 0; r < 3; r++)
                 for (int c = 0; c < 3; c++) basis[s][c][r] = bas[r * 3 + c];
 
@@ -1341,9 +1359,8 @@ struct World {
         return !awake(bodies[mA[s]]) && !awake(bodies[mB[s]]);
     }
 
-    // Contact forces of one manifold at the current poses (manifold.cpp:650-686): world
-    // arms and cone-clamped forces.  The reference caches these; recomputing gives the same
-    // floats because nothing they read changes between the cache fill and its use.
+    // Contact forces of one manifold at the current poses (manifold.cpp:474-497): world arms
+    // and cone-clamped forces.
     void contactForces(uint32_t s, f3 *F, f3 *rAw, f3 *rBw) const
     {
         const Body &a = bodies[mA[s]], &b = bodies[mB[s]];
@@ -1363,11 +1380,19 @@ struct World {
         }
     }
 
-    // manifold.cpp:688-755
-    void stampManifold(uint32_t s, int body, Block &blk) const
+    static constexpr uint8_t ARMS_CACHED = 1, PRIMAL_READY = 2;   // manifold.hpp:96-97
+
+    // manifold.cpp:465-501: the shared half of a manifold's primal, cached for the stamps
+    void preCompute(uint32_t s)
     {
-        f3 F[MAXC], rAw[MAXC], rBw[MAXC];
-        contactForces(s, F, rAw, rBw);
+        const size_t first = static_cast<size_t>(s) * MAXC;
+        contactForces(s, &fCache[first], &rAW[first], &rBW[first]);
+        mFlags[s] |= PRIMAL_READY;
+    }
+
+    // manifold.cpp:503-568: one body's stamp from the cached forces and arms
+    void stampCached(uint32_t s, int body, Block &blk) const
+    {
         const bool fromA = body == mA[s];
         const float side = fromA ? 1.0f : -1.0f;
         const int first = static_cast<int>(s) * MAXC;
@@ -1376,8 +1401,8 @@ struct World {
         m3 accLin = zero3(), accAng = zero3(), accCross = zero3();
         f3 accRL{0, 0, 0}, accRA{0, 0, 0};
         for (int i = 0; i < numC[s]; i++) {
-            const f3 f = F[i];
-            const f3 r = fromA ? rAw[i] : rBw[i];
+            const f3 f = fCache[first + i];
+            const f3 r = fromA ? rAW[first + i] : rBW[first + i];
             const Hess h = basisHessian(fr, pen[first + i]);
             const m3 S{{f3{h.s00, h.s01, h.s02}, f3{h.s01, h.s11, h.s12}, f3{h.s02, h.s12, h.s22}}};
             const m3 RS{{cross(r, S[0]), cross(r, S[1]), cross(r, S[2])}};
@@ -1404,7 +1429,20 @@ struct World {
         blk.rhsAng += accRA * side;
     }
 
-    // manifold.cpp:768-856 (arm caching off)
+    // manifold.cpp:570-581: the first touch of a round consumes the previous dual's cache (no
+    // pose moved since); the second recomputes at the poses the first body's move left.
+    void updatePrimal(uint32_t s, int body, Block &blk)
+    {
+        if (!(mFlags[s] & PRIMAL_READY) || g_plant == 26) preCompute(s);
+        else cachedTouches++;
+        stampCached(s, body, blk);
+        mFlags[s] = static_cast<uint8_t>(mFlags[s] & ~PRIMAL_READY);
+    }
+
+    // manifold.cpp:583-671: lambda <- clamped force, penalty ramps; with arm caching
+    // (jacobianRebuildDistance > 0) the world arms computed last are reused while neither body's
+    // first rotation column has moved farther than the distance since they were captured.  The
+    // next primal's force cache is left ready.
     void updateDual(uint32_t s)
     {
         const Body &a = bodies[mA[s]], &b = bodies[mB[s]];
@@ -1414,9 +1452,24 @@ struct World {
         const f3 dqAL = a.pos - a.initLin, dqAA = qsub(a.q, a.initAng);
         const f3 dqBL = b.pos - b.initLin, dqBA = qsub(b.q, b.initAng);
         const m3 rotA = qmat(a.q), rotB = qmat(b.q);
+        const bool cacheable = P.jacobian_rebuild_distance > 0.0f;
+        const float rebuild2 = P.jacobian_rebuild_distance * P.jacobian_rebuild_distance;
+        const f3 dA = rotA[0] - capA[s];
+        const f3 dB = rotB[0] - capB[s];
+        const bool usable = cacheable && (mFlags[s] & ARMS_CACHED) != 0 &&
+                (g_plant == 23 ? length2(dA) >= rebuild2 && length2(dB) >= rebuild2
+                               : length2(dA) <= rebuild2 && length2(dB) <= rebuild2);
         for (int i = 0; i < numC[s]; i++) {
-            const f3 rA = mul(rotA, cont[first + i].rA);
-            const f3 rB = mul(rotB, cont[first + i].rB);
+            f3 rA, rB;
+            if (usable) {
+                rA = g_plant == 22 ? rBW[first + i] : rAW[first + i];
+                rB = g_plant == 22 ? rAW[first + i] : rBW[first + i];
+            } else {
+                rA = mul(rotA, cont[first + i].rA);
+                rB = mul(rotB, cont[first + i].rB);
+                rAW[first + i] = rA;
+                rBW[first + i] = rB;
+            }
             const f3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
             const f3 C = constraintError(C0[first + i], br, fr, 1 - P.alpha);
             f3 F = cmul(pen[first + i], C) + lam[first + i];
@@ -1429,7 +1482,17 @@ struct World {
                 pen[first + i][2] = smin(pen[first + i][2] + beta * std::fabs(C[2]), pmax());
                 stick[first + i] = length2d(C[1], C[2]) < STICK_THRESH;
             }
+            f3 fNext = cmul(pen[first + i], C) + lam[first + i];
+            clampForce(fNext, mu);
+            fCache[first + i] = fNext;
         }
+        if (usable) armHits++;
+        if (!usable && cacheable && !(g_plant == 25 && (mFlags[s] & ARMS_CACHED))) {
+            capA[s] = rotA[0];
+            capB[s] = rotB[0];
+            mFlags[s] |= ARMS_CACHED;
+        }
+        mFlags[s] |= PRIMAL_READY;
     }
 
     void decayPenalty(uint32_t s, float decay)
@@ -1450,6 +1513,7 @@ struct World {
             C0[first + i] = mul(basis[s], xA - xB) + f3{COLLISION_MARGIN, 0, 0};
             if (C0[first + i].x < -P.max_penetration) C0[first + i].x = -P.max_penetration;
         }
+        mFlags[s] = static_cast<uint8_t>(mFlags[s] & ~PRIMAL_READY);   // manifold.cpp:460
     }
 
     // ------------------------------------------------------------ joints ----
@@ -1923,15 +1987,21 @@ struct World {
         }
     }
 
-    Block assemble(int body, bool fromCache) const
+    // A body's block: inertia plus its forces in adjacency order; the colour sweep goes through
+    // updatePrimal, the overflow solve stamps from the phase-A cache (solver_step.cpp:215-226,
+    // 252-262).
+    Block assemble(int body, bool fromCache)
     {
-        (void)fromCache;
         const Body &b = bodies[body];
         Block blk = makeBlock(b, P.dt * P.dt);
         for (uint32_t e = adjStart[body]; e < adjStart[body + 1]; e++) {
             const int ref = adj[e];
-            if (ref >= 0) stampManifold(static_cast<uint32_t>(ref), body, blk);
-            else forcePrimal(joints[-ref - 1], body, blk);
+            if (ref >= 0) {
+                if (fromCache) stampCached(static_cast<uint32_t>(ref), body, blk);
+                else updatePrimal(static_cast<uint32_t>(ref), body, blk);
+            } else {
+                forcePrimal(joints[-ref - 1], body, blk);
+            }
         }
         return blk;
     }
@@ -1970,6 +2040,13 @@ struct World {
             }
             const int ov = newton ? static_cast<int>(overflowOrder.size()) : 0;
             if (ov > 0) {
+                // solver_step.cpp:235-246: phase A caches the manifolds touching an overflow body
+                for (uint32_t s : live) {
+                    const Body &A = bodies[mA[s]], &B = bodies[mB[s]];
+                    const bool aIn = A.inUpdate && colour[A.updateIndex] == kOverflow;
+                    const bool bIn = B.inUpdate && colour[B.updateIndex] == kOverflow;
+                    if (aIn || bIn) preCompute(s);
+                }
                 for (int k = 0; k < ov; k++) {
                     const int i = overflowOrder[k];
                     const Block blk = assemble(updateOrder[i], true);
@@ -2183,8 +2260,6 @@ struct World {
 int checkParams(const avbd_params *p)
 {
     if (!p) return AVBD_ERR_INVALID;
-    if (p->jacobian_rebuild_distance > 0.0f)
-        return AVBD_ERR_UNSUPPORTED;
     return AVBD_OK;
 }
 
@@ -2296,6 +2371,7 @@ int oracle_upload(oracle_world *o, const avbd_bodies *b, const avbd_joints *j)
     w.mA.clear(); w.mB.clear(); w.basis.clear(); w.numC.clear(); w.mFric.clear(); w.cont.clear();
     w.dormant.clear(); w.saved.clear();
     w.C0.clear(); w.pen.clear(); w.lam.clear(); w.stick.clear(); w.betaScale.clear();
+    w.mFlags.clear(); w.capA.clear(); w.capB.clear(); w.rAW.clear(); w.rBW.clear(); w.fCache.clear();
     w.live.clear(); w.freeList.clear();
     return AVBD_OK;
 }
@@ -3048,6 +3124,14 @@ int oracle_set_manifolds(oracle_world *o, const avbd_manifolds *m)
 }
 
 void oracle_plant(int id) { g_plant = id; }
+
+// Test hook: since the world was created, dual evaluations that reused cached arms and primal
+// touches served by the force cache.
+void oracle_cache_counters(oracle_world *o, long long *out)
+{
+    out[0] = o->w.armHits;
+    out[1] = o->w.cachedTouches;
+}
 
 // avbd::Manifold::collide (collide.cpp:1100-1120) for bodies a (as A) and b at their current
 // pose: the step's narrow phase with the speculative margin grown over hz.

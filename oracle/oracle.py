@@ -60,6 +60,7 @@ def lib():
         "oracle_set_manifolds": (C.c_int, [vp, C.POINTER(a.Manifolds)]),
         "oracle_digest": (C.c_uint64, [vp]),
         "oracle_plant": (None, [C.c_int]),
+        "oracle_cache_counters": (None, [vp, C.POINTER(C.c_longlong)]),
         "oracle_collide_pair": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, a.i32p, a.f32p, a.f32p, a.f32p]),
         "oracle_stats": (None, [vp, a.i32p]),
         "oracle_colours": (None, [vp, a.u8p]),
@@ -353,6 +354,13 @@ class OracleWorld:
 
     def digest(self):
         return int(self.L.oracle_digest(self.h))
+
+    def cache_counters(self):
+        """(dual evaluations that reused cached arms, primal touches served by the force cache)
+        since the world was created."""
+        out = (C.c_longlong * 2)()
+        self.L.oracle_cache_counters(self.h, out)
+        return int(out[0]), int(out[1])
 
     def collide_pair(self, a, b, hz):
         """avbd::Manifold::collide for bodies a and b now: (count, keys, rA, rB, basis 3x3 rows)."""

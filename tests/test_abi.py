@@ -1,6 +1,7 @@
 """CPU tests of the C-ABI library: it loads, exports every entry point the public headers
 declare, and its host-side helpers and argument checks behave (no GPU compute here)."""
 import ctypes as C
+import importlib
 import glob
 import os
 import re
@@ -62,13 +63,9 @@ def test_state_digest_matches_python():
     assert lib.avbd_state_digest(C.byref(st)) == pkg.state_digest(b)
 
 
-def test_unsupported_and_invalid_arguments():
+def test_invalid_arguments():
     lib = abi.load_library()
     h = C.c_void_p()
-    for bad in (dict(jacobian_rebuild_distance=1e-4),):
-        p = pkg.default_params(**bad)
-        assert lib.avbd_world_create(C.byref(p), 10, 0, 0, C.byref(h)) == abi.AVBD_ERR_UNSUPPORTED
-        assert lib.avbd_last_error()
     p = pkg.default_params(dt=0.0)
     assert lib.avbd_world_create(C.byref(p), 10, 0, 0, C.byref(h)) == abi.AVBD_ERR_INVALID
     assert lib.avbd_world_create(None, 10, 0, 0, C.byref(h)) == abi.AVBD_ERR_INVALID
@@ -93,3 +90,12 @@ def test_config_scene_sizes():
     assert int((b["mass"] > 0).sum()) == 256 and len(j["body_b"]) == 256
     sizes = scenes.batch_sizes(4096)
     assert sizes.min() >= 8 and sizes.max() <= 2048
+
+
+def test_arm_cache_arrays_only_when_switched_on():
+    """Arm reuse (jacobian_rebuild_distance > 0) adds its per-slot captures and per-contact arms
+    to a batch's device footprint (2 + 2 * 8 float4 per manifold slot); off, nothing."""
+    serve = importlib.import_module("godot-avbd_amd.serve")
+    off = serve.footprint(pkg.default_params(), 4, 1000, 0, 5000)
+    on = serve.footprint(pkg.default_params(jacobian_rebuild_distance=0.02), 4, 1000, 0, 5000)
+    assert on - off == 5000 * (2 + 2 * 8) * 16

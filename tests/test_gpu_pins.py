@@ -1,7 +1,8 @@
 """The HIP path against the oracle on the scenes that pin the oracle's less common features
 (tests/test_oracle_pins.py): collision layers, axis locks under a constraint, the projection and
 its cap, the colour sweep for contact bodies, adaptive beta with its per-contact multipliers,
-Newton rounds with stiffness decay, hard joints at the penalty cap, and joint fracture.  Bar:
+Newton rounds with stiffness decay, hard joints at the penalty cap, joint fracture, and drift-gated
+arm reuse (plain, interleaved, sleeping, round shapes with grace, switched mid-run, batched).  Bar:
 bodies, every manifold field and every joint field bit-equal, every step.
 """
 import math
@@ -158,6 +159,84 @@ def test_convergence_threshold_in_a_batch():
         o.upload(bb, jj)
         orc.append(o)
     for _ in range(30):
+        batch.step()
+        for o in orc:
+            o.step()
+    for wid, o in zip(ids, orc):
+        assert not state_mismatch(batch.download(wid), o.download())
+    batch.close()
+
+
+# ---- arm reuse (jacobianRebuildDistance, manifold.cpp:583-671) ----
+
+@pytest.mark.parametrize("overflow,dist", [(1, 0.02), (1, 2.0), (0, 0.02), (0, 2.0)])
+def test_arm_cache_pile_lockstep(overflow, dist):
+    """Arm reuse through the Jacobi path and through the colour sweep (contact bodies coloured),
+    at the reference's 2 cm and at a distance past any drift."""
+    b, j = scenes.pile(150, seed=9, pitch=2.0)
+    _lockstep(b, j, P(iterations=10, contact_only_overflow=overflow, jacobian_rebuild_distance=dist), 60)
+
+
+@pytest.mark.parametrize("dist", [0.02, 2.0])
+def test_arm_cache_chains_and_mixed_lockstep(dist):
+    """Jointed chains (colour sweep) colliding with each other and the ground, and the reference's
+    mixed digest scene, with arm reuse: first touches served from the stored arms."""
+    b, j = scenes.chains(n_chains=6, links=16, seed=5)
+    _lockstep(b, j, P(iterations=10, jacobian_rebuild_distance=dist), 80)
+    b, j = scenes.digest_mixed()
+    _lockstep(b, j, P(iterations=10, jacobian_rebuild_distance=dist), 80)
+
+
+def test_arm_cache_interleaved_spinner_and_pile():
+    """Interleaved substeps keep the captured arms across substeps of a tick: the spinning
+    frictionless box of the drift-rule pin, and a pile."""
+    s = scenes.SceneBuilder()
+    s.box((100, 100, 1), 0.0, 0.0, (0, 0, 0))
+    s.box((1, 1, 1), 1.0, 0.0, (0, 0, 1.0))
+    b, j = s.build()
+    b["angular_velocity"] = np.array([[0, 0, 12.0], [0, 0, 0]], np.float32)
+    dt = float(F(1.0) / F(240.0))
+    for dist in (0.03, 0.075, 0.12, 0.2):
+        _lockstep(b, j, P(dt=dt, jacobian_rebuild_distance=dist), 20, inter=(4, 2))
+    b, j = scenes.pile(120, seed=4, pitch=2.0)
+    _lockstep(b, j, P(dt=dt, jacobian_rebuild_distance=0.05), 40, inter=(4, 2))
+
+
+def test_arm_cache_with_sleep_round_shapes_and_grace():
+    """Frozen manifolds keep their captured arms until a neighbour wakes their bodies (the
+    sleepy pile); round shapes and contact grace take the narrow passes that drop the capture."""
+    b, j = scenes.sleepy_pile(160, seed=6)
+    _lockstep(b, j, P(iterations=10, sleep_frames=20, jacobian_rebuild_distance=2.0), 160)
+    b, j = scenes.round_pile(120, seed=4)
+    for dist in (0.0, 0.02):    # 0: also the basis of box pairs parked without contacts
+        _lockstep(b, j, P(iterations=10, contact_grace_ticks=3, jacobian_rebuild_distance=dist), 80)
+
+
+def test_arm_cache_switched_on_mid_run():
+    """set_params turns arm reuse on after 20 steps (the arrays are allocated then) and off again."""
+    b, j = scenes.pile(100, seed=12, pitch=2.0)
+    on = P(iterations=10, jacobian_rebuild_distance=0.5)
+
+    def switch(s, w, o):
+        if s in (20, 45):
+            prm = on if s == 20 else P(iterations=10)
+            w.set_params(prm)
+            o.set_params(prm)
+    _lockstep(b, j, P(iterations=10), 60, before_step=switch)
+
+
+def test_arm_cache_in_a_batch():
+    p = P(iterations=10, jacobian_rebuild_distance=0.02)
+    worlds = [scenes.pile(60 + 40 * k, seed=40 + k, ground=60.0) for k in range(3)]
+    worlds.append(scenes.chains(n_chains=4, links=16, seed=44))
+    batch = pkg.Batch(p, 4, sum(len(w[0]["mass"]) for w in worlds), sum(0 if w[1] is None else len(w[1]["body_b"]) for w in worlds))
+    ids = [batch.add(bb, jj) for bb, jj in worlds]
+    orc = []
+    for bb, jj in worlds:
+        o = oracle.OracleWorld(p)
+        o.upload(bb, jj)
+        orc.append(o)
+    for _ in range(40):
         batch.step()
         for o in orc:
             o.step()

@@ -222,11 +222,6 @@ def test_static_pairs_make_no_manifold():
     assert o.stats()["manifolds"] == 0
 
 
-def test_unsupported_parameters_rejected():
-    with pytest.raises(ValueError):
-        oracle.OracleWorld(P(jacobian_rebuild_distance=0.02))
-
-
 @pytest.mark.parametrize("scene,overflow", [("chains", True), ("pile", True), ("pile", False)])
 def test_colouring_matches_sequential_greedy(scene, overflow):
     """The oracle's colouring equals greedy colouring restated from its definition (smallest

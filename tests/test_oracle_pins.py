@@ -1,7 +1,8 @@
 """CPU pins for the oracle's functions that no recorded digest or golden fixture covers:
 collision layers and masks, axis locks, the speculative contact paths, friction, the update set,
 the colour-overflow bucket, the position projection and its cap, adaptive beta, Newton rounds and
-stiffness decay, hard (infinite-stiffness) joints with the penalty cap, and joint fracture.
+stiffness decay, hard (infinite-stiffness) joints with the penalty cap, joint fracture, and the
+manifold caches (the primal's force cache, drift-gated arm reuse).
 
 Each pin is either one of the reference's own scenario checks, restated with its thresholds
 (test/core_test.cpp, cited per function), or a closed form the mechanics fix (a hanging body's
@@ -564,6 +565,201 @@ def pin_fracture():
     assert len(jt["broken"]) == 0 and st["position"][0, 2] < 0.0
 
 
+# ------------------------------------------------------------------- arm caching ----
+# manifold.cpp:583-671 (jacobianRebuildDistance): the dual reuses the world arms computed last
+# while neither body's first rotation column has moved farther than the distance from where it
+# was when the arms were captured; a new contact set (every tick) drops the capture.
+
+def _both(bodies, joints, params_a, params_b, steps, interleaved=None):
+    out = []
+    for prm in (params_a, params_b):
+        o = oracle.OracleWorld(prm)
+        o.upload(bodies, joints)
+        for _ in range(steps):
+            if interleaved:
+                o.step_interleaved(*interleaved)
+            else:
+                o.step()
+        out.append(o)
+    return out
+
+
+def _same_run(x, y):
+    sx, sy = x.download(), y.download()
+    mx, my = x.manifolds(), y.manifolds()
+    return all(bits_equal(sx[k], sy[k]) for k in ("position", "rotation", "velocity", "angular_velocity")) and \
+        all(bits_equal(mx[k], my[k]) for k in ("penalty", "lambda_", "c0"))
+
+
+def pin_arm_cache_tiny_distance_is_no_cache():
+    """A distance so small that only an unchanged first rotation column passes the drift test:
+    the cached arms are then the arms a recomputation gives, so a tumbling pile (Jacobi path) and
+    a pile through the colour sweep (contact-only bodies coloured) end bit-identical to a run
+    without caching."""
+    b, j = scenes.pile(40, seed=9, pitch=1.6, ground=30.0)
+    for overflow in (1, 0):
+        x, y = _both(b, j, P(contact_only_overflow=overflow), P(contact_only_overflow=overflow,
+                     jacobian_rebuild_distance=1e-19), 60)
+        assert _same_run(x, y), overflow
+
+
+def _sliders(friction=0.0):
+    """Boxes alone on the ground (resting, sliding frictionless): their rotations stay within
+    1e-8 of the identity, so their world arms do not move in float32."""
+    s = scenes.SceneBuilder()
+    s.box((100, 100, 1), 0.0, friction, (0, 0, 0))
+    s.box((1, 1, 1), 1.0, friction, (0.0, 0.0, 1.0), vel=(2.0, 0.0, 0))
+    s.box((1, 2, 1), 2.0, friction, (6.0, 0.0, 1.0), vel=(0.0, -1.0, 0))
+    s.box((1, 1, 1), 1.0, friction, (-6.0, 3.0, 1.2), vel=(1.0, 0.5, 0))
+    return s.build()
+
+
+def pin_arm_cache_reuses_the_arms_it_stored():
+    """With a distance past any drift (2: the first columns are unit vectors) every dual after a
+    tick's first reuses stored arms.  For boxes whose rotations stay within 1e-8 of the identity
+    the stored arms are the recomputed ones up to the last bits, so the run stays within 1e-6 of
+    one without caching, through the Jacobi path and the colour sweep, while the cache is used (a
+    swapped side or contact would change the constraint error by centimetres)."""
+    b, j = _sliders()
+    for overflow in (1, 0):
+        x, y = _both(b, j, P(contact_only_overflow=overflow), P(contact_only_overflow=overflow,
+                     jacobian_rebuild_distance=2.0), 40)
+        sx, sy = x.download(), y.download()
+        assert np.abs(sy["rotation"][:, :3].astype(np.float64)).max() < 1e-8, sy["rotation"]
+        hits = y.cache_counters()[0]
+        assert hits >= 40 * 2 * 9, hits            # 9 cached duals of 10 per step and manifold
+        for k in ("position", "rotation", "velocity"):
+            d = np.abs(sx[k].astype(np.float64) - sy[k].astype(np.float64)).max()
+            assert d < 1e-6, (overflow, k, d)
+
+
+def pin_arm_cache_changes_a_tumbling_pile():
+    """The same large distance on a pile whose boxes tumble: the reused arms lag the rotations,
+    so the run departs from the uncached one (the cache is in effect)."""
+    b, j = scenes.pile(40, seed=9, pitch=1.6, ground=30.0)
+    x, y = _both(b, j, P(), P(jacobian_rebuild_distance=2.0), 60)
+    assert y.cache_counters()[0] > 0 and not _same_run(x, y)
+
+
+def _spinner(omega):
+    s = scenes.SceneBuilder()
+    s.box((100, 100, 1), 0.0, 0.0, (0, 0, 0))
+    s.box((1, 1, 1), 1.0, 0.0, (0, 0, 1.0))
+    b, j = s.build()
+    b["angular_velocity"] = np.array([[0, 0, omega], [0, 0, 0]], np.float32)
+    return b, j
+
+
+def _predicted_hits(theta, dist, substeps, rounds):
+    """Drift rule on a body turning theta per substep about the contact normal (its first column
+    turns in the ground plane, |col0(a) - col0(b)| = 2 sin(|a - b| / 2)): per tick the capture is
+    dropped, then each substep's duals hit while the chord to the capture is within the distance
+    and recapture on a miss."""
+    hits, cap = 0, None
+    for sub in range(substeps):
+        ang = sub * theta
+        for _ in range(rounds):
+            if cap is not None and 2.0 * math.sin(abs(ang - cap) / 2.0) <= dist:
+                hits += 1
+            else:
+                cap = ang
+    return hits
+
+
+def pin_arm_cache_hits_follow_the_drift_rule():
+    """A frictionless box spinning about the vertical on the ground, four interleaved substeps
+    of two rounds: the rounds never turn it about another axis, so whether a dual reuses the arms
+    follows from the chord its first column has swept since the capture.  Distances between the
+    one-, two- and three-substep chords give 4, 6 and 7 hits per tick, and a tick's first dual never
+    hits (its contact set is new)."""
+    omega, dt = 12.0, 1.0 / 240.0
+    theta = omega * dt
+    b, j = _spinner(omega)
+    ticks = 20
+    for dist, want in ((0.03, 4), (0.075, 6), (0.12, 6), (0.2, 7)):
+        o = oracle.OracleWorld(P(dt=dt, jacobian_rebuild_distance=dist))
+        o.upload(b, j)
+        for _ in range(ticks):
+            o.step_interleaved(4, 2)
+        st = o.download()
+        spin = float(st["angular_velocity"][0, 2])
+        # the spin slows a little (the normal rows' warm start); the prediction holds over the range
+        assert 0.95 * omega < spin <= omega and np.abs(st["rotation"][0, :2]).max() < 1e-6, st["rotation"][0]
+        assert _predicted_hits(theta, dist, 4, 2) == want == _predicted_hits(spin * dt, dist, 4, 2)
+        assert o.cache_counters()[0] == ticks * want, (dist, o.cache_counters())
+
+
+def pin_primal_cache_serves_the_first_touch():
+    """solver_step.cpp:215-226 with manifold.cpp:570-581: in the colour sweep a manifold's first
+    touch of every round after the first is served from the force cache the dual left; a box on
+    the ground through the colour sweep (three rounds) has two such touches per step, a chain of
+    boxes standing on each other one per manifold and round."""
+    s = scenes.SceneBuilder()
+    s.box((100, 100, 1), 0.0, 0.5, (0, 0, 0))
+    s.box((1, 1, 1), 1.0, 0.5, (0, 0, 1.0))
+    o = _run(*s.build(), P(iterations=3, contact_only_overflow=0), 10)
+    assert o.cache_counters()[1] == 10 * 2, o.cache_counters()
+    s = scenes.SceneBuilder()
+    s.box((100, 100, 1), 0.0, 0.5, (0, 0, 0))
+    for k in range(4):
+        s.box((1, 1, 1), 1.0, 0.5, (0, 0, 1.0 + k))
+    o = _run(*s.build(), P(iterations=3, contact_only_overflow=0), 10)
+    assert o.stats()["manifolds"] == 4 and o.cache_counters()[1] == 10 * 2 * 4, o.cache_counters()
+
+
+def _tight_stack_params(**kw):
+    return P(dt=1.0 / 240.0, gravity=-9.8, iterations=12, beta_lin=1e5, speculative_margin=0.02,
+             jacobian_rebuild_distance=0.02, projection_max_correction=0.1, sleep_frames=30,
+             sleep_linear_threshold=0.05, sleep_angular_threshold=0.05, **kw)
+
+
+def pin_arm_cache_reference_tight_stack():
+    """core_test.cpp:426-456: ten unit cubes stacked on the ground (sleep when idle), arm caching
+    at 2 cm, 600 ticks of stepInterleaved(4, 2): height error <= 5 cm, interface penetration
+    <= 1 cm, lateral drift <= 1 cm."""
+    s = scenes.SceneBuilder(gravity=-9.8)
+    s.box((100, 100, 1), 0.0, 0.8, (0, 0, 0))
+    ids = [s.box((1, 1, 1), 1.0, 0.8, (0, 0, 1.0 + i)) for i in range(10)]
+    b, j = s.build()
+    n = len(s.b)
+    b["sleep_mode"] = np.where(b["mass"] > 0, 1, 0).astype(np.uint8)
+    o = oracle.OracleWorld(_tight_stack_params())
+    o.upload(b, j)
+    for _ in range(600):
+        o.step_interleaved(4, 2)
+    pos = o.download()["position"].astype(np.float64)
+    z = [pos[_pos(k, n)] for k in ids]
+    height = max(abs(z[i][2] - (1.0 + i)) for i in range(10))
+    pen = max((1.0 - z[i][2]) if i == 0 else 1.0 - (z[i][2] - z[i - 1][2]) for i in range(10))
+    drift = max(math.hypot(z[i][0], z[i][1]) for i in range(10))
+    assert height <= 0.05 and pen <= 0.01 and drift <= 0.01, (height, pen, drift)
+    assert o.cache_counters()[0] > 0
+
+
+def pin_arm_cache_reference_impact():
+    """core_test.cpp:536-575: a cube arriving at 18 m/s on a resting cube, arm caching at 2 cm,
+    240 ticks of stepInterleaved(4, 2): the overlap stays <= 2 cm, no body passes 20 m/s, the base
+    stays within 5 cm of z = 1."""
+    s = scenes.SceneBuilder(gravity=-9.8)
+    s.box((100, 100, 1), 0.0, 0.8, (0, 0, 0))
+    target = s.box((1, 1, 1), 1.0, 0.8, (0, 0, 1.0))
+    striker = s.box((1, 1, 1), 1.0, 0.8, (0, 0, 3.0), vel=(0, 0, -18.0))
+    b, j = s.build()
+    n = len(s.b)
+    b["sleep_mode"] = np.where(b["mass"] > 0, 1, 0).astype(np.uint8)
+    o = oracle.OracleWorld(_tight_stack_params())
+    o.upload(b, j)
+    worst = speed = 0.0
+    for _ in range(240):
+        o.step_interleaved(4, 2)
+        st = o.download()
+        zs, zt = float(st["position"][_pos(striker, n), 2]), float(st["position"][_pos(target, n), 2])
+        worst = max(worst, 1.0 - (zs - zt))
+        speed = max(speed, float(np.linalg.norm(st["velocity"].astype(np.float64), axis=1).max()))
+    assert worst <= 0.02 and speed <= 20.0 and abs(zt - 1.0) <= 0.05, (worst, speed, zt)
+    assert o.cache_counters()[0] > 0
+
+
 # ------------------------------------------------------------------- test entry points ----
 
 PINS = {name: fn for name, fn in globals().items() if name.startswith("pin_")}
@@ -584,6 +780,9 @@ PLANTS = {
     14: ["pin_joint_penalty_cap"], 15: ["pin_fracture"], 16: ["pin_fracture"],
     17: ["pin_projection_shared_plane"], 18: ["pin_projection_shared_plane"],
     19: ["pin_contact_only_overflow_off"], 20: ["pin_speculative_horizon"], 21: ["pin_convergence_probe"],
+    22: ["pin_arm_cache_reuses_the_arms_it_stored"], 23: ["pin_arm_cache_tiny_distance_is_no_cache"],
+    24: ["pin_arm_cache_hits_follow_the_drift_rule"], 25: ["pin_arm_cache_hits_follow_the_drift_rule"],
+    26: ["pin_primal_cache_serves_the_first_touch"],
 }
 
 
