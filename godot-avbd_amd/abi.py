@@ -30,7 +30,8 @@ u64p = C.POINTER(C.c_uint64)
 
 class Tuning(C.Structure):
     _fields_ = [("block_dual", C.c_int32), ("block_primal", C.c_int32), ("block_stamp", C.c_int32),
-                ("block_projection", C.c_int32), ("block_traverse", C.c_int32), ("live_blocks", C.c_int32)]
+                ("block_projection", C.c_int32), ("block_traverse", C.c_int32), ("live_blocks", C.c_int32),
+                ("colour_lanes", C.c_int32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -124,30 +124,44 @@ AD bool ldlt6(const float (&a)[6][6], const float (&b)[6], float (&x)[6])
     return __builtin_isfinite(worst) && worst < 1.0e1f;
 }
 
-// maths.h:202-307: Eigen-order pivoted LDL^T, the fallback for indefinite blocks.
-__device__ __noinline__ bool ldlt6Pivoted(const float (&ain)[6][6], const float (&b)[6], float (&x)[6])
+// maths.h:202-307: Eigen-order pivoted LDL^T, the fallback for indefinite blocks.  Fully
+// unrolled so that every index is a constant: a row or column swap with the pivot row is a
+// select over the candidate rows, and the permutation of x swaps through the same selects, so
+// the matrix stays in registers (no scratch memory) and the arithmetic is the serial order.
+AD bool ldlt6Pivoted(const float (&ain)[6][6], const float (&b)[6], float (&x)[6])
 {
     float a[6][6];
+#pragma unroll
     for (int i = 0; i < 6; i++)
+#pragma unroll
         for (int j = 0; j < 6; j++) a[i][j] = ain[i][j];
     float d[6] = {0, 0, 0, 0, 0, 0};
     int t[6] = {0, 1, 2, 3, 4, 5};
     bool zeroPivot = false;
+#pragma unroll
     for (int k = 0; k < 6; k++) {
         int j = k;
         float best = fabsf(a[k][k]);
+#pragma unroll
         for (int i = k + 1; i < 6; i++) {
             const float c = fabsf(a[i][i]);
             if (c > best) { best = c; j = i; }
         }
         t[k] = j;
-        if (j != k) {
-            for (int c = 0; c < 6; c++) { const float tmp = a[k][c]; a[k][c] = a[j][c]; a[j][c] = tmp; }
-            for (int r = 0; r < 6; r++) { const float tmp = a[r][k]; a[r][k] = a[r][j]; a[r][j] = tmp; }
+#pragma unroll
+        for (int r = k + 1; r < 6; r++) {
+            if (j == r) {
+#pragma unroll
+                for (int c = 0; c < 6; c++) { const float tmp = a[k][c]; a[k][c] = a[r][c]; a[r][c] = tmp; }
+#pragma unroll
+                for (int q = 0; q < 6; q++) { const float tmp = a[q][k]; a[q][k] = a[q][r]; a[q][r] = tmp; }
+            }
         }
         if (k > 0) {
             float temp[6], prod[6];
+#pragma unroll
             for (int i = 0; i < k; i++) temp[i] = d[i] * a[k][i];
+#pragma unroll
             for (int i = 0; i < k; i++) prod[i] = a[k][i] * temp[i];
             float dt;
             if (k == 1) dt = prod[0];
@@ -156,14 +170,17 @@ __device__ __noinline__ bool ldlt6Pivoted(const float (&ain)[6][6], const float 
             else if (k == 4) dt = (prod[0] + prod[2]) + (prod[1] + prod[3]);
             else dt = ((prod[0] + prod[2]) + (prod[1] + prod[3])) + prod[4];
             a[k][k] -= dt;
+#pragma unroll
             for (int i = 0; i < k; i++) {
                 const float ti = temp[i];
+#pragma unroll
                 for (int r = k + 1; r < 6; r++) a[r][k] -= a[r][i] * ti;
             }
         }
         const float pivot = a[k][k];
         const bool valid = fabsf(pivot) > 0.0f;
         if (!valid) {
+#pragma unroll
             for (int r = k + 1; r < 6; r++)
                 if (a[r][k] != 0.0f) return false;
             zeroPivot = true;
@@ -171,25 +188,41 @@ __device__ __noinline__ bool ldlt6Pivoted(const float (&ain)[6][6], const float 
             return false;
         }
         if (valid)
+#pragma unroll
             for (int r = k + 1; r < 6; r++) a[r][k] /= pivot;
         d[k] = pivot;
     }
+#pragma unroll
     for (int i = 0; i < 6; i++) x[i] = b[i];
-    for (int k = 0; k < 6; k++) { const float tmp = x[k]; x[k] = x[t[k]]; x[t[k]] = tmp; }
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+#pragma unroll
+        for (int r = k + 1; r < 6; r++)
+            if (t[k] == r) { const float tmp = x[k]; x[k] = x[r]; x[r] = tmp; }
+#pragma unroll
     for (int jj = 0; jj < 5; jj++) {
         const float v = x[jj];
         if (v != 0.0f)
+#pragma unroll
             for (int i = jj + 1; i < 6; i++) x[i] -= a[i][jj] * v;
     }
     const float tol = 1.17549435e-38f;
+#pragma unroll
     for (int i = 0; i < 6; i++) x[i] = fabsf(d[i]) > tol ? x[i] / d[i] : 0.0f;
+#pragma unroll
     for (int jj = 5; jj >= 1; jj--) {
         const float v = x[jj];
         if (v != 0.0f)
+#pragma unroll
             for (int i = jj - 1; i >= 0; i--) x[i] -= a[jj][i] * v;
     }
-    for (int k = 5; k >= 0; k--) { const float tmp = x[k]; x[k] = x[t[k]]; x[t[k]] = tmp; }
+#pragma unroll
+    for (int k = 5; k >= 0; k--)
+#pragma unroll
+        for (int r = k + 1; r < 6; r++)
+            if (t[k] == r) { const float tmp = x[k]; x[k] = x[r]; x[r] = tmp; }
     float worst = 0.0f;
+#pragma unroll
     for (int i = 0; i < 6; i++) worst = smax(worst, fabsf(x[i]));
     return __builtin_isfinite(worst) && worst < 1.0e1f;
 }
@@ -222,7 +255,7 @@ AD bool solveDeltaFast(const Block &k, v3 &dLin, v3 &dAng)
 }
 
 // The rest of the chain for a queued block: pivoted LDL^T, else a zero update.
-__device__ void solveDeltaSlow(const Block &k, v3 &dLin, v3 &dAng)
+AD void solveDeltaSlow(const Block &k, v3 &dLin, v3 &dAng)
 {
     float a[6][6], b[6], x[6];
     packA(k, a, b);
@@ -353,6 +386,33 @@ AD void stampSide(const Hess &h, const m3 &Sm, v3 r, v3 fw, m3 &accAng, m3 &accC
     accRA += cross(r, fw);
 }
 
+// What one adjacency entry adds to its body's block, kept apart per add: the serial assembly
+// adds a joint's terms to lhsAng up to three times and to rhsAng twice, and float sums depend
+// on that order (adding a zero is not a no-op on a zero's sign), so a term is applied exactly
+// when the serial code adds it.  flags: 1 = lin, ang, cross, rl, ra; 2 = the diagonal
+// geometric-stiffness add to lhsAng; 4 = ang2, ra2 (a joint's angular rows); 8 = a force the
+// block's owner stamps itself (spring, generic joint).
+struct EntryTerms {
+    m3 lin, ang, cross, ang2;
+    v3 rl, ra, angDiag, ra2;
+    int flags;
+};
+AD void applyTerms(Block &blk, const EntryTerms &t)
+{
+    if (t.flags & 1) {
+        blk.lhsLin = madd(blk.lhsLin, t.lin);
+        blk.lhsAng = madd(blk.lhsAng, t.ang);
+        blk.lhsCross = madd(blk.lhsCross, t.cross);
+        blk.rhsLin += t.rl;
+        blk.rhsAng += t.ra;
+    }
+    if (t.flags & 2) blk.lhsAng = madd(blk.lhsAng, diag(t.angDiag.x, t.angDiag.y, t.angDiag.z));
+    if (t.flags & 4) {
+        blk.lhsAng = madd(blk.lhsAng, t.ang2);
+        blk.rhsAng += t.ra2;
+    }
+}
+
 // One manifold's contribution to one body's block (manifold.cpp:474-497 force + :503-568
 // stamp, fused).  Forces are recomputed at the current poses; the reference's force cache
 // holds the same floats because nothing it reads moves between fill and use.  ARMS (arm reuse
@@ -360,7 +420,7 @@ AD void stampSide(const Hess &h, const m3 &Sm, v3 r, v3 fw, m3 &accAng, m3 &accC
 // earlier in this round) takes the world arms the last dual left, as the reference's cached
 // stamp does (manifold.cpp:570-581); every other touch computes them and stores them.
 template <bool ARMS>
-AD void stampManifold(const State &S, int k, int body, Block &blk, bool cachedRound = false, int colour = 0)
+AD void manifoldTerms(const State &S, int k, int body, EntryTerms &t, bool cachedRound = false, int colour = 0)
 {
     const int4 hd = S.mHdr[k];
     const int a = hd.x, b = hd.y, n = hd.w;
@@ -414,11 +474,19 @@ AD void stampManifold(const State &S, int k, int body, Block &blk, bool cachedRo
         accLin = madd(accLin, Sm);
         accRL += fw;
     }
-    blk.lhsLin = madd(blk.lhsLin, accLin);
-    blk.lhsAng = madd(blk.lhsAng, accAng);
-    blk.lhsCross = madd(blk.lhsCross, accCross);
-    blk.rhsLin += accRL * side;
-    blk.rhsAng += accRA * side;
+    t.lin = accLin;
+    t.ang = accAng;
+    t.cross = accCross;
+    t.rl = accRL * side;
+    t.ra = accRA * side;
+    t.flags = 1;
+}
+template <bool ARMS>
+AD void stampManifold(const State &S, int k, int body, Block &blk, bool cachedRound = false, int colour = 0)
+{
+    EntryTerms t;
+    manifoldTerms<ARMS>(S, k, body, t, cachedRound, colour);
+    applyTerms(blk, t);
 }
 
 // ---- per-manifold stamps, gathered by the Jacobi solve ----
@@ -930,8 +998,18 @@ AD void forcePrimal(const State &S, int j, int body, Block &blk)
     else if (kind == FORCE_GENERIC) genericPrimal(S, j, body, blk);
 }
 
+AD void jointTerms(const State &S, int j, int body, EntryTerms &t);
 AD void jointPrimal(const State &S, int j, int body, Block &blk)
 {
+    EntryTerms t;
+    jointTerms(S, j, body, t);
+    applyTerms(blk, t);
+}
+// joint.cpp:80-160: a ball / finite joint's linear rows (with the geometric stiffness of the
+// rotating arm) and angular rows, as terms of the block.
+AD void jointTerms(const State &S, int j, int body, EntryTerms &t)
+{
+    t.flags = 0;
     // Every load is issued before the first branch (one round of waits for the joint's fields,
     // one for the two poses); the arithmetic is jointLin / jointAng's, in their order.
     const int ja = S.jA[j], jb = S.jB[j];
@@ -961,14 +1039,15 @@ AD void jointPrimal(const State &S, int j, int body, Block &blk)
         }
         const m3 jLinT = transpose(jLin), jAngT = transpose(jAng);
         const m3 jAngTk = mul(jAngT, K);
-        blk.lhsLin = madd(blk.lhsLin, mul(mul(jLinT, K), jLin));
-        blk.lhsAng = madd(blk.lhsAng, mul(jAngTk, jAng));
-        blk.lhsCross = madd(blk.lhsCross, mul(jAngTk, jLin));
+        t.lin = mul(mul(jLinT, K), jLin);
+        t.ang = mul(jAngTk, jAng);
+        t.cross = mul(jAngTk, jLin);
         const v3 r = isA ? qrot(rotA, rA) : -qrot(rotB, rB);
         const m3 H = madd(madd(mscale(gsBall(0, r), F.x), mscale(gsBall(1, r), F.y)), mscale(gsBall(2, r), F.z));
-        blk.lhsAng = madd(blk.lhsAng, diag(H.c[0][0], H.c[1][1], H.c[2][2]));
-        blk.rhsLin += mul(jLinT, F);
-        blk.rhsAng += mul(jAngT, F);
+        t.angDiag = v3{H.c[0][0], H.c[1][1], H.c[2][2]};
+        t.rl = mul(jLinT, F);
+        t.ra = mul(jAngT, F);
+        t.flags = 1 | 2;
     }
     if (length2(penA) > 0) {
         const m3 K = diag(penA.x, penA.y, penA.z);
@@ -978,8 +1057,9 @@ AD void jointPrimal(const State &S, int j, int body, Block &blk)
         const v3 F = mul(K, C) + lamA;
         const float arm = armA;
         const m3 jA = mscale(isA ? ident() : m3{{v3{0, 0, 0}, v3{0, -1, 0}, v3{0, 0, -1}}}, arm);
-        blk.lhsAng = madd(blk.lhsAng, mul(mul(transpose(jA), K), jA));
-        blk.rhsAng += mul(transpose(jA), F);
+        t.ang2 = mul(mul(transpose(jA), K), jA);
+        t.ra2 = mul(transpose(jA), F);
+        t.flags |= 4;
     }
 }
 

@@ -993,6 +993,8 @@ struct Engine {
     // spreads over every CU (the passes are bound by the latency of their dependent loads)
     static constexpr int COLOUR_PASS_TPB = 64;
     int colourGrid() const { return nblk(Ncap, COLOUR_PASS_TPB); }
+    // four lanes per body; the grid walks longer colour runs in strides
+    int quadGrid() const { return std::min(nblk(4 * Ncap, QUAD_TPB), 2048); }
 
     void radixSort()
     {
@@ -1022,6 +1024,7 @@ struct Engine {
     // not depend on them: every kernel's work per element is independent of its launch shape).
     struct Tuning {
         int dual = TPB, primal = TPB, stamp = TPB, proj = TPB, traverse = TPB, liveBlocks = LIVE_BLOCKS;
+        int colourLanes = 1;   // lanes per body in the colour passes (1 or 4)
     } tune;
 
     // gLive = every world's live positions, world after world (wLiveOff = exclusive scan of wLive).
@@ -1279,10 +1282,16 @@ struct Engine {
                     if (withColours)
                         for (int c = 0; c < 8; c++) {
                             pbeg(AVBD_K_COLOUR_PASS);
-                            if (arms)
+                            if (tune.colourLanes == 4) {
+                                if (arms)
+                                    k_colour_pass_quad<true><<<quadGrid(), QUAD_TPB, 0, stream>>>(S, c, it > 0 ? 1 : 0);
+                                else
+                                    k_colour_pass_quad<false><<<quadGrid(), QUAD_TPB, 0, stream>>>(S, c, 0);
+                            } else if (arms) {
                                 k_colour_pass<true><<<colourGrid(), COLOUR_PASS_TPB, 0, stream>>>(S, c, it > 0 ? 1 : 0);
-                            else
+                            } else {
                                 k_colour_pass<false><<<colourGrid(), COLOUR_PASS_TPB, 0, stream>>>(S, c, 0);
+                            }
                             pend();
                         }
                     if (stampHere) {
@@ -1525,12 +1534,15 @@ struct Engine {
             if (x != 64 && x != 128 && x != 256) return fail(AVBD_ERR_INVALID, "block sizes must be 64, 128 or 256");
         if (t.live_blocks < NUM_XCD || t.live_blocks > (1 << 20))
             return fail(AVBD_ERR_INVALID, "live_blocks must be in [8, 2^20]");
+        if (t.colour_lanes != 0 && t.colour_lanes != 1 && t.colour_lanes != 4)
+            return fail(AVBD_ERR_INVALID, "colour_lanes must be 1 or 4 (0: 1)");
         tune.dual = t.block_dual;
         tune.primal = t.block_primal;
         tune.stamp = t.block_stamp;
         tune.proj = t.block_projection;
         tune.traverse = t.block_traverse;
         tune.liveBlocks = t.live_blocks;
+        tune.colourLanes = t.colour_lanes == 4 ? 4 : 1;
         graphsValid = false;
         return AVBD_OK;
     }
@@ -1571,6 +1583,18 @@ struct Engine {
             if (bestT == 0.0f || tt < bestT) { bestT = tt; bestG = g; }
         }
         tune.liveBlocks = bestG;
+        // colour passes: one lane per body, or four (a body's adjacency entries in parallel)
+        if (prof.launches[AVBD_K_COLOUR_PASS] > 0) {
+            float lanesT[2];
+            const int lanes[2] = {1, 4};
+            for (int c = 0; c < 2; c++) {
+                tune.colourLanes = lanes[c];
+                avbd_status st = profile(stepsPerTrial, 1, 0, &prof);
+                if (st != AVBD_OK) return st;
+                lanesT[c] = prof.avg_ms[AVBD_K_COLOUR_PASS];
+            }
+            tune.colourLanes = lanesT[1] < lanesT[0] ? 4 : 1;
+        }
         graphsValid = false;
         if (out) {
             out->block_dual = tune.dual;
@@ -1579,6 +1603,7 @@ struct Engine {
             out->block_projection = tune.proj;
             out->block_traverse = tune.traverse;
             out->live_blocks = tune.liveBlocks;
+            out->colour_lanes = tune.colourLanes;
         }
         return AVBD_OK;
     }

@@ -2041,6 +2041,73 @@ __global__ void __launch_bounds__(TPB) k_colour_pass(State S, int colour, int ca
     }
 }
 
+// The colour pass with four lanes per body: lane s of a body's quad takes adjacency entries
+// s, s + 4, ... (their loads are in flight together instead of one entry after another), and
+// the quad's first lane adds the entries' terms to the block in adjacency order, so the block
+// holds the floats the one-lane assembly makes.  Springs and generic joints (flag 8) are
+// stamped by the first lane itself at their place in the order.
+AD float shflF(float v, int src) { return __shfl(v, src); }
+AD v3 shfl3(v3 v, int src) { return v3{shflF(v.x, src), shflF(v.y, src), shflF(v.z, src)}; }
+AD m3 shflM(const m3 &m, int src) { return m3{{shfl3(m.c[0], src), shfl3(m.c[1], src), shfl3(m.c[2], src)}}; }
+AD EntryTerms shflTerms(const EntryTerms &t, int src)
+{
+    EntryTerms u;
+    u.flags = __shfl(t.flags, src);
+    u.lin = shflM(t.lin, src);
+    u.ang = shflM(t.ang, src);
+    u.cross = shflM(t.cross, src);
+    u.ang2 = shflM(t.ang2, src);
+    u.rl = shfl3(t.rl, src);
+    u.ra = shfl3(t.ra, src);
+    u.angDiag = shfl3(t.angDiag, src);
+    u.ra2 = shfl3(t.ra2, src);
+    return u;
+}
+constexpr int QUAD_TPB = 256;
+template <bool ARMS>
+__global__ void __launch_bounds__(QUAD_TPB) k_colour_pass_quad(State S, int colour, int cachedRound)
+{
+    const uint32_t begin = S.colStart[colour], end = S.colStart[colour + 1];
+    const int sub = threadIdx.x & 3;
+    const int lead = (threadIdx.x & 63) & ~3;
+    const uint32_t quads = (gridDim.x * blockDim.x) >> 2;
+    for (uint32_t r = begin + (GID >> 2); r < end; r += quads) {
+        const int i = S.colValAlt[r];
+        const int e0 = S.adjStart[i], e1 = S.adjStart[i + 1];
+        Block blk;
+        if (sub == 0) makeBlock(S, i, blk);
+        for (int c = e0; c < e1; c += 4) {
+            EntryTerms t;
+            t.flags = 0;
+            const int e = c + sub;
+            if (e < e1) {
+                const int ref = S.adj[e];
+                if (ref >= 0) {
+                    manifoldTerms<ARMS>(S, adjManifold(ref), i, t, cachedRound != 0, colour);
+                } else {
+                    const uint8_t kind = S.jKind[-ref - 1];
+                    if (kind == FORCE_JOINT) jointTerms(S, -ref - 1, i, t);
+                    else if (kind != FORCE_IGNORE) t.flags = 8;
+                }
+            }
+            const int n = min(4, e1 - c);
+            for (int q = 0; q < n; q++) {
+                const EntryTerms u = shflTerms(t, lead + q);
+                if (sub == 0) {
+                    if (u.flags & 8) forcePrimal(S, -S.adj[c + q] - 1, i, blk);
+                    else applyTerms(blk, u);
+                }
+            }
+        }
+        if (sub == 0) {
+            v3 dl, da;
+            if (!solveDeltaFast(blk, dl, da)) solveDeltaSlow(blk, dl, da);
+            S.pos[i] = st3(bPos(S, i) + dl);
+            S.rot[i] = stq(qadd(bRot(S, i), da));
+        }
+    }
+}
+
 // solver_step.cpp:248-264: overflow bodies solve against one pose snapshot.  APPLY (worlds
 // without joints): a body's block reads no other body's pose, so each body moves at once
 // (solver_step.cpp:268-273) and refreshes its pose record; otherwise the delta is kept for

@@ -348,6 +348,37 @@ def test_pinned_launch_shapes_keep_results():
                       "block_traverse": 64, "live_blocks": 8})
 
 
+def test_colour_lanes_keep_results():
+    """The colour passes with four lanes per body (a body's adjacency entries in parallel, the
+    block summed in adjacency order by its first lane) give the one-lane bits: jointed chains that
+    collide, the mixed digest scene with springs and generic joints, and a pile through the colour
+    sweep; both equal the oracle."""
+    p = P(iterations=10)
+    worlds = [scenes.chains(n_chains=6, links=16, seed=71), scenes.springs_and_filters(), scenes.generic_joints()]
+    pc = P(iterations=10, contact_only_overflow=0)
+    runs = []
+    for lanes in (1, 4):
+        for prm, ws in ((p, worlds), (pc, [scenes.pile(300, seed=72, pitch=2.0)])):
+            nb = sum(len(w[0]["mass"]) for w in ws)
+            nj = sum(0 if w[1] is None else len(w[1]["body_b"]) for w in ws)
+            batch = pkg.Batch(prm, 4, nb, nj)
+            ids = [batch.add(bb, jj) for bb, jj in ws]
+            batch.set_tuning({"block_dual": 64, "block_primal": 64, "block_stamp": 64, "block_projection": 64,
+                              "block_traverse": 64, "live_blocks": 64, "colour_lanes": lanes})
+            for _ in range(60):
+                batch.step()
+            runs.append([batch.download(i) for i in ids])
+            batch.close()
+    for one, four in zip(runs[:2], runs[2:]):
+        for x, y in zip(one, four):
+            assert not state_mismatch(x, y)
+    for (bb, jj), st in zip(worlds, runs[2]):
+        o = oracle.OracleWorld(p)
+        o.upload(bb, jj)
+        o.step(60)
+        assert not state_mismatch(st, o.download())
+
+
 def test_full_size_pile_digest_matches_oracle():
     """C2 at its full size (8192 boxes), 40 steps."""
     b, j = scenes.pile(8192, seed=2)
