@@ -258,6 +258,8 @@ def load_library(path=LIB_PATH):
         "avbd_world_step_interleaved": (C.c_int32, [vp, C.c_int32, C.c_int32]),
         "avbd_world_download": (C.c_int32, [vp, C.POINTER(Bodies), C.POINTER(Joints)]),
         "avbd_world_get_manifolds": (C.c_int32, [vp, C.POINTER(Manifolds)]),
+        "avbd_world_record_residuals": (C.c_int32, [vp, C.c_int32]),
+        "avbd_world_get_residuals": (C.c_int32, [vp, i32p, f32p, i32p, f32p]),
         "avbd_world_raycast": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, C.c_int32, C.POINTER(RayHits)]),
         "avbd_world_raycast_noalloc": (C.c_int32, [vp, C.c_int32, f32p, f32p, u32p, C.c_int32, C.POINTER(RayHits)]),
         "avbd_world_collide_pair": (C.c_int32, [vp, C.c_int32, C.c_int32, C.c_float, i32p, i32p, f32p, f32p, f32p]),

@@ -632,7 +632,7 @@ AD void manifoldStamps(const State &S, int k, StampAcc &acc)
 // rebuild distance since the arms were captured, the world arms stored last are used (the
 // stamps still take arms at the current pose, as the reference's phase A recomputes them);
 // otherwise the arms are computed, stored and captured.
-template <bool STAMP, bool ARMS>
+template <bool STAMP, bool ARMS, bool RESID = false>
 AD void manifoldDual(const State &S, int k, float decay, StampAcc &acc)
 {
     // every load that does not depend on another is issued before the first wait; the
@@ -648,7 +648,11 @@ AD void manifoldDual(const State &S, int k, float decay, StampAcc &acc)
     float2 nLyz{};
     if (n > 0) { nPL = S.cPL[cb]; nRA = S.cRA[cb]; nRB = S.cRB[cb]; nC0 = S.cC0[cb]; nLyz = S.cLyz[cb]; }
     if (STAMP) stampInit(acc, A.mass > 0.0f, Bd.mass > 0.0f);
-    if (!(A.mass > 0.0f) && !(Bd.mass > 0.0f)) return;
+    if (!(A.mass > 0.0f) && !(Bd.mass > 0.0f)) {
+        if (RESID)   // a frozen manifold's dual does not run: no constraint value
+            for (int i = 0; i < n; i++) S.resC[static_cast<size_t>(S.kSlot[k]) * MAXC + i] = 0.0f;
+        return;
+    }
     const v3 dqAL = A.dqL, dqAA = A.dqA, dqBL = Bd.dqL, dqBA = Bd.dqA;
     const m3 rotA = qmat(A.q), rotB = qmat(Bd.q);
     const float pmax = S.P.penaltyMax;
@@ -684,6 +688,7 @@ AD void manifoldDual(const State &S, int k, float decay, StampAcc &acc)
         }
         const v3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
         const v3 C = constraintError(S, ld3(c0in), br, B);
+        if (RESID) S.resC[static_cast<size_t>(S.kSlot[k]) * MAXC + i] = C.x;
         v3 F = cmul(pen, C) + v3{pl4.w, lyz.x, lyz.y};
         float bounds;
         const float scale = clampForce(F, mu, bounds);
@@ -1064,8 +1069,11 @@ AD void jointTerms(const State &S, int j, int body, EntryTerms &t)
 }
 
 // joint.cpp:171-230
-AD void jointDual(const State &S, int j)
+// `rec`: where to keep the linear rows' constraint value C (joint.cpp:178; 0 when the joint
+// has no linear rows or is not a ball / finite joint), or null.
+AD void jointDual(const State &S, int j, float4 *rec = nullptr)
 {
+    if (rec) *rec = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (S.jKind[j] == FORCE_GENERIC) { genericDual(S, j); return; }
     if (S.jKind[j] != FORCE_JOINT) return;   // springs and collision filters keep no dual state
     const float pmax = S.P.penaltyMax;
@@ -1079,6 +1087,7 @@ AD void jointDual(const State &S, int j)
             C -= ld3(S.jC0L[j]) * S.P.alpha;
             lamL = mul(K, C) + lamL;
         }
+        if (rec) *rec = st3(C);
         penL = lmin3(penL + fabs3(C) * S.P.betaLin, lmin3(v3{st, st, st}, v3{pmax, pmax, pmax}));
     }
     if (length2(penA) > 0) {

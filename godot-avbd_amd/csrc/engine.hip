@@ -252,6 +252,19 @@ struct Engine {
         if (e != hipSuccess) return fail(AVBD_ERR_DEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
         return AVBD_OK;
     }
+    // Constraint values of each step's last dual, kept while on (arrays allocated the first time).
+    bool recordRes = false;
+    avbd_status recordResiduals(bool on)
+    {
+        if (on && !S.resC) {
+            avbd_status st;
+            if ((st = A(S.resC, static_cast<size_t>(Scap) * MAXC)) != AVBD_OK) return st;
+            if ((st = A(S.resJ, Jcap)) != AVBD_OK) return st;
+        }
+        if (on != recordRes) graphsValid = false;
+        recordRes = on;
+        return AVBD_OK;
+    }
     // Arm-cache arrays, allocated the first time arm reuse is switched on (then kept).
     avbd_status ensureArmCache(const avbd_params &p)
     {
@@ -897,6 +910,8 @@ struct Engine {
         AL(S.mDormant, M); AL(S.mSaved, M); AL(S.dRA, M * MAXC); AL(S.dRB, M * MAXC); AL(S.dC0, M * MAXC);
         AL(S.dPen, M * MAXC); AL(S.pairParked, P); AL(S.pairApart, P); AL(rimDev, 16); S.rim = rimDev;
         S.armCap = S.armA = S.armB = nullptr;
+        S.resC = nullptr;
+        S.resJ = nullptr;
         if (p.jacobian_rebuild_distance > 0.0f) {
             AL(S.armCap, 2 * M); AL(S.armA, M * MAXC); AL(S.armB, M * MAXC);
         }
@@ -1321,7 +1336,11 @@ struct Engine {
                 }
                 pbeg(AVBD_K_DUAL);
                 const float decay = newton ? 1.0f : prm.stiffness_decay;
-                if (arms) {
+                if (recordRes && it + 1 == runRounds) {
+                    // the step's last round keeps its constraint values (never a stamping dual)
+                    if (arms) k_dual<false, true, true><<<liveGrid(tune.dual), tune.dual, 0, stream>>>(S, decay);
+                    else k_dual<false, false, true><<<liveGrid(tune.dual), tune.dual, 0, stream>>>(S, decay);
+                } else if (arms) {
                     if (stampInDual) k_dual<true, true><<<liveGrid(tune.dual), tune.dual, 0, stream>>>(S, decay);
                     else k_dual<false, true><<<liveGrid(tune.dual), tune.dual, 0, stream>>>(S, decay);
                 } else {
@@ -2131,6 +2150,47 @@ struct Engine {
         return AVBD_OK;
     }
 
+    avbd_status getResiduals(int w, int32_t *numManifolds, float *cn, int32_t *numJoints, float *cl)
+    {
+        if (!S.resC) return fail(AVBD_ERR_INVALID, "residuals are not being kept (avbd_world_record_residuals)");
+        const WorldInfo &wi = worlds[w];
+        HIPCHK(hipStreamSynchronize(stream));
+        int nLive = 0;
+        HIPCHK(hipMemcpy(&nLive, S.wLive + w, sizeof(int), hipMemcpyDeviceToHost));
+        const int nj = wi.jointCount, jo = wi.jointBase;
+        std::vector<uint8_t> dead(std::max(nj, 1));
+        if (nj > 0) HIPCHK(hipMemcpy(dead.data(), S.jDead + jo, nj, hipMemcpyDeviceToHost));
+        int alive = 0;
+        for (int q = 0; q < nj; q++) alive += dead[q] ? 0 : 1;
+        const bool fits = *numManifolds >= nLive && *numJoints >= alive;
+        *numManifolds = nLive;
+        *numJoints = alive;
+        if (!fits) return fail(AVBD_ERR_CAPACITY, "residual arrays are too small");
+        if (nLive > 0 && cn) {
+            std::vector<int> live(nLive), num(wi.slotCap);
+            std::vector<float> res(static_cast<size_t>(wi.slotCap) * MAXC);
+            HIPCHK(hipMemcpy(live.data(), S.live + wi.slotBase, sizeof(int) * nLive, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(num.data(), S.mNum + wi.slotBase, sizeof(int) * wi.slotCap, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(res.data(), S.resC + static_cast<size_t>(wi.slotBase) * MAXC, sizeof(float) * res.size(),
+                             hipMemcpyDeviceToHost));
+            for (int i = 0; i < nLive; i++) {
+                const int s = live[i] - wi.slotBase;
+                for (int k = 0; k < MAXC; k++) cn[MAXC * i + k] = k < num[s] ? res[static_cast<size_t>(s) * MAXC + k] : 0.0f;
+            }
+        }
+        if (nj > 0 && cl) {
+            std::vector<float4> res(nj);
+            HIPCHK(hipMemcpy(res.data(), S.resJ + jo, sizeof(float4) * nj, hipMemcpyDeviceToHost));
+            int k = 0;
+            for (int q = 0; q < nj; q++) {
+                if (dead[q]) continue;
+                cl[3 * k] = res[q].x; cl[3 * k + 1] = res[q].y; cl[3 * k + 2] = res[q].z;
+                k++;
+            }
+        }
+        return AVBD_OK;
+    }
+
     avbd_status setManifolds(int w, const avbd_manifolds *m)
     {
         const WorldInfo &wi = worlds[w];
@@ -2571,6 +2631,19 @@ avbd_status avbd_world_get_manifolds(avbd_world *world, avbd_manifolds *out)
 {
     if (!world || !out) return fail(AVBD_ERR_INVALID, "null argument");
     return world->eng.getManifolds(0, out);
+}
+
+avbd_status avbd_world_record_residuals(avbd_world *world, int32_t on)
+{
+    if (!world) return fail(AVBD_ERR_INVALID, "world is null");
+    return world->eng.recordResiduals(on != 0);
+}
+
+avbd_status avbd_world_get_residuals(avbd_world *world, int32_t *num_manifolds, float *contact_cn,
+                                     int32_t *num_joints, float *joint_clin)
+{
+    if (!world || !num_manifolds || !num_joints) return fail(AVBD_ERR_INVALID, "null argument");
+    return world->eng.getResiduals(0, num_manifolds, contact_cn, num_joints, joint_clin);
 }
 
 avbd_status avbd_world_set_manifolds(avbd_world *world, const avbd_manifolds *in)

@@ -2181,18 +2181,19 @@ __global__ void __launch_bounds__(TPB) k_stamp(State S)
 
 // solver_step.cpp:277-296: manifolds (not frozen), then joints.  STAMP: also build the next
 // Jacobi iteration's stamps.
-template <bool STAMP, bool ARMS>
+// RESID: keep each constraint value the dual computes (the step's last round; host reads).
+template <bool STAMP, bool ARMS, bool RESID = false>
 __global__ void __launch_bounds__(TPB) k_dual(State S, float decay)
 {
     __shared__ float4 lds[STAMP ? TPB / 64 : 1][STAMP ? STAMP_LDS : 1];
     const int lane = threadIdx.x & 63;
     FOR_LIVE(k) {
         StampAcc acc;
-        manifoldDual<STAMP, ARMS>(S, k, decay, acc);
+        manifoldDual<STAMP, ARMS, RESID>(S, k, decay, acc);
         if (STAMP) storeStampsWave(S, k, acc, lds[threadIdx.x >> 6], lane);
     }
     for (int t = GID; t < S.jointCap; t += gridDim.x * blockDim.x)
-        if (S.jWorld[t] >= 0 && !S.jDead[t]) jointDual(S, t);
+        if (S.jWorld[t] >= 0 && !S.jDead[t]) jointDual(S, t, RESID ? S.resJ + t : nullptr);
 }
 
 // ------------------------------------------------------------ projection ----

@@ -75,6 +75,10 @@ struct State {
     // A, 1 when captured}, armCap[2 slot + 1] = {column of B, 0}); per contact (slot * MAXC + i)
     // the world arms computed last (armA, armB)
     float4 *armCap, *armA, *armB;
+    // constraint values of the step's last dual, kept while the host asks for them (else null):
+    // per contact (slot * MAXC + i) the normal row, per joint the linear rows
+    float *resC;
+    float4 *resJ;
     uint8_t *pairParked;
     const float *rim;     // cylinder rim directions: 8 cosines then 8 sines (host libm values)
     int *bodyWorld;

@@ -496,32 +496,55 @@ def greedy_colours(bodies, joints, manifolds, contact_only_overflow=True, max_co
     return out
 
 
-def world_statistics(state, manifolds):
-    """The long-run statistics compared between implementations: per-contact penetration depth
-    max(0, 1 mm - C0.x) after the rebase (avbd_physics_server3d.cpp:1547-1560) as p50/p95/p99/max,
-    |C_n| = |C0.x| mean and max over contacts, and the mechanical energy
-    E = sum 1/2 m |v|^2 + 1/2 w^T (R diag(I) R^T) w - m g z."""
-    num = np.asarray(manifolds["num_contacts"])
-    c0 = np.asarray(manifolds["c0"]).reshape(len(num), 8, 3)
-    used = np.arange(8)[None, :] < num[:, None]
-    c0n = c0[..., 0][used]
-    depth = np.maximum(0.0, 1.0e-3 - c0n.astype(np.float64))
-    q = lambda a, k: float(np.percentile(a, k)) if len(a) else 0.0
+def world_energy(state):
+    """Mechanical energy of a world's movable bodies, E = sum 1/2 m |v|^2 + 1/2 w^T (R diag(I) R^T) w
+    - m g z (in float64; the reference has no energy metric, so both implementations use this)."""
     m = np.asarray(state["mass"], np.float64)
     mov = m > 0
     v = np.asarray(state["velocity"], np.float64).reshape(-1, 3)
     w = np.asarray(state["angular_velocity"], np.float64).reshape(-1, 3)
     inertia = np.asarray(state["moment"], np.float64).reshape(-1, 3)
     rot = np.asarray(state["rotation"], np.float64).reshape(-1, 4)
-    x, y, z, s = rot[:, 0], rot[:, 1], rot[:, 2], rot[:, 3]
-    R = np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - s * z), 2 * (x * z + s * y)], -1),
-                  np.stack([2 * (x * y + s * z), 1 - 2 * (x * x + z * z), 2 * (y * z - s * x)], -1),
-                  np.stack([2 * (x * z - s * y), 2 * (y * z + s * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+    x, y, z, s_ = rot[:, 0], rot[:, 1], rot[:, 2], rot[:, 3]
+    R = np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - s_ * z), 2 * (x * z + s_ * y)], -1),
+                  np.stack([2 * (x * y + s_ * z), 1 - 2 * (x * x + z * z), 2 * (y * z - s_ * x)], -1),
+                  np.stack([2 * (x * z - s_ * y), 2 * (y * z + s_ * x), 1 - 2 * (x * x + y * y)], -1)], 1)
     wl = np.einsum("nji,nj->ni", R, w)   # body-frame angular velocity
     ke = 0.5 * m * (v * v).sum(1) + 0.5 * (inertia * wl * wl).sum(1)
     pe = -m * np.asarray(state["gravity"], np.float64) * np.asarray(state["position"], np.float64).reshape(-1, 3)[:, 2]
-    return {"depth_p50": q(depth, 50), "depth_p95": q(depth, 95), "depth_p99": q(depth, 99),
-            "depth_max": float(depth.max()) if len(depth) else 0.0,
-            "residual_mean": float(np.abs(c0n).mean()) if len(c0n) else 0.0,
-            "residual_max": float(np.abs(c0n).max()) if len(c0n) else 0.0,
-            "energy": float((ke + pe)[mov].sum()), "contacts": int(num.sum())}
+    return float((ke + pe)[mov].sum())
+
+
+def world_statistics(state, manifolds, residuals=None, initial_state=None):
+    """The long-run statistics compared between implementations:
+    * penetration depth per contact, max(0, 1 mm - C0.x) after the rebase
+      (avbd_physics_server3d.cpp:1547-1560), as p50/p95/p99/max;
+    * constraint residual |C| from the step's last dual (`residuals` = World.residuals() with
+      residual recording on): mean and max |C_n| over contacts (manifold.cpp:626-630) and of
+      |C_lin| over joints (joint.cpp:176-182); without `residuals`, |C0.x| after the rebase;
+    * energy E (world_energy) and, given the run's `initial_state`, its drift E - E(initial)."""
+    num = np.asarray(manifolds["num_contacts"])
+    c0 = np.asarray(manifolds["c0"]).reshape(len(num), 8, 3)
+    used = np.arange(8)[None, :] < num[:, None]
+    c0n = c0[..., 0][used]
+    depth = np.maximum(0.0, 1.0e-3 - c0n.astype(np.float64))
+    q = lambda a, k: float(np.percentile(a, k)) if len(a) else 0.0
+    out = {"depth_p50": q(depth, 50), "depth_p95": q(depth, 95), "depth_p99": q(depth, 99),
+           "depth_max": float(depth.max()) if len(depth) else 0.0, "contacts": int(num.sum())}
+    if residuals is not None:
+        cn, cl = residuals
+        rn = np.abs(np.asarray(cn, np.float64).reshape(len(num), 8)[used])
+        rj = np.linalg.norm(np.asarray(cl, np.float64).reshape(-1, 3), axis=1)
+        out["residual_source"] = "dual"
+    else:
+        rn = np.abs(c0n.astype(np.float64))
+        rj = np.zeros(0)
+        out["residual_source"] = "c0"
+    out.update({"residual_mean": float(rn.mean()) if len(rn) else 0.0,
+                "residual_max": float(rn.max()) if len(rn) else 0.0,
+                "joint_residual_mean": float(rj.mean()) if len(rj) else 0.0,
+                "joint_residual_max": float(rj.max()) if len(rj) else 0.0,
+                "energy": world_energy(state)})
+    if initial_state is not None:
+        out["energy_drift"] = out["energy"] - world_energy(initial_state)
+    return out

@@ -256,6 +256,23 @@ class World:
         m = ms.count
         return {k: v[:m] for k, v in arr.items()}
 
+    def record_residuals(self, on=True):
+        """Keep the constraint values of each step's last dual (see residuals())."""
+        abi.check(self.lib.avbd_world_record_residuals(self.h, 1 if on else 0))
+
+    def residuals(self):
+        """The last step's dual constraint values: (manifolds x 8 contact normal rows C.x in the
+        order of manifolds(), joints x 3 linear rows in the order of download(joints=True))."""
+        nm, nj = C.c_int32(0), C.c_int32(0)
+        rc = self.lib.avbd_world_get_residuals(self.h, C.byref(nm), None, C.byref(nj), None)
+        if rc != abi.AVBD_ERR_CAPACITY:
+            abi.check(rc)
+        cn = np.zeros((max(nm.value, 1), 8), np.float32)
+        cl = np.zeros((max(nj.value, 1), 3), np.float32)
+        abi.check(self.lib.avbd_world_get_residuals(self.h, C.byref(nm), cn.ctypes.data_as(abi.f32p), C.byref(nj),
+                                                    cl.ctypes.data_as(abi.f32p)))
+        return cn[:nm.value], cl[:nj.value]
+
     def set_manifolds(self, arrays):
         ms = abi.manifolds_struct(arrays)
         abi.check(self.lib.avbd_world_set_manifolds(self.h, C.byref(ms)))

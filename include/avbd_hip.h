@@ -424,6 +424,19 @@ avbd_status avbd_world_collect_aabb(avbd_world *world, const float min[3], const
 avbd_status avbd_world_get_manifolds(avbd_world *world, avbd_manifolds *out);
 avbd_status avbd_world_set_manifolds(avbd_world *world, const avbd_manifolds *in);
 avbd_status avbd_world_stats(avbd_world *world, avbd_step_stats *out);
+/* Keep (on = 1) or stop keeping the constraint values C the last dual of each step computes:
+ * a contact's normal row (ManifoldStore::updateDual, src/avbd/manifold.cpp:626-630) and a ball
+ * / finite joint's linear rows (Joint::updateDual, src/avbd/joint.cpp:176-182), which the
+ * reference computes and discards.  Off by default; on, the step's last dual writes them.
+ * For long-run residual statistics (no reference counterpart). */
+avbd_status avbd_world_record_residuals(avbd_world *world, int32_t on);
+/* The kept values of the last step: contact_cn[8 i + k] for live manifold i (the order of
+ * avbd_world_get_manifolds) and contact k below its count (0 past it and for a manifold whose
+ * bodies were frozen), joint_clin[3 q..] for joint q in the force-list order of
+ * avbd_world_download (0 for joints without linear rows).  *num_manifolds / *num_joints hold
+ * the arrays' capacities and receive the counts (AVBD_ERR_CAPACITY when too small). */
+avbd_status avbd_world_get_residuals(avbd_world *world, int32_t *num_manifolds, float *contact_cn,
+                                     int32_t *num_joints, float *joint_clin);
 /* Colour of every body after the last step, in list order (solver_solve.cpp:365-424): 0-7 a
  * Gauss-Seidel colour, 8 the Jacobi (overflow) bucket, 9 an overflow body the step leaves
  * unsolved, 255 not in the update set.  `out` holds the world's body count. */

@@ -83,6 +83,7 @@ struct Joint {
     float stiffLin, stiffAng, fracture, torqueArm; bool broken;
     uint8_t kind = AVBD_FORCE_JOINT; float rest = 0.0f;
     quat restQ{0, 0, 0, 1}; JointAxis axis[6];   // generic joints: 0-2 linear, 3-5 angular
+    f3 resLin{0, 0, 0};   // the last dual's linear constraint value C (joint.cpp:178), 0 if none
 };
 
 struct Contact { int32_t key; f3 rA, rB; };
@@ -997,6 +998,8 @@ struct World {
     std::vector<uint8_t> mFlags;
     std::vector<f3> capA, capB;
     std::vector<f3> rAW, rBW, fCache;
+    // the last dual's normal constraint value per contact (manifold.cpp:626-630; 0 while frozen)
+    std::vector<float> resC;
     // test hooks: dual evaluations that reused cached arms, primal touches served by the cache
     long long armHits = 0, cachedTouches = 0;
     std::vector<uint32_t> live, freeList;
@@ -1026,6 +1029,7 @@ struct World {
             cont.resize(cont.size() + MAXC); C0.resize(C0.size() + MAXC); pen.resize(pen.size() + MAXC);
             lam.resize(lam.size() + MAXC); stick.resize(stick.size() + MAXC); betaScale.resize(betaScale.size() + MAXC);
             rAW.resize(rAW.size() + MAXC); rBW.resize(rBW.size() + MAXC); fCache.resize(fCache.size() + MAXC);
+            resC.resize(resC.size() + MAXC);
         }
         mA[s] = a; mB[s] = b; basis[s] = zero3(); numC[s] = 0; mFric[s] = 0.0f;
         dormant[s] = 0; saved[s] = 0;
@@ -1035,6 +1039,7 @@ struct World {
             cont[i] = Contact{0, {0, 0, 0}, {0, 0, 0}};
             C0[i] = pen[i] = lam[i] = f3{0, 0, 0};
             rAW[i] = rBW[i] = fCache[i] = f3{0, 0, 0};
+            resC[i] = 0.0f;
             stick[i] = 0; betaScale[i] = 1.0f;
         }
         live.push_back(s);
@@ -1472,6 +1477,7 @@ struct World {
             }
             const f3 br = (dqAL - dqBL) + cross(dqAA, rA) - cross(dqBA, rB);
             const f3 C = constraintError(C0[first + i], br, fr, 1 - P.alpha);
+            resC[first + i] = g_plant == 28 ? C0[first + i][0] : C[0];
             f3 F = cmul(pen[first + i], C) + lam[first + i];
             const Clamp cone = clampForce(F, mu);
             lam[first + i] = F;
@@ -1794,6 +1800,7 @@ struct World {
     {
         if (j.kind == AVBD_FORCE_GENERIC) { genericDual(j); return; }
         if (j.kind != AVBD_FORCE_JOINT) return;   // Spring/IgnoreCollision::updateDual do nothing
+        j.resLin = f3{0, 0, 0};
         if (length2(j.penLin) > 0) {
             const m3 K = diagonal(j.penLin.x, j.penLin.y, j.penLin.z);
             f3 C = jointLin(j);
@@ -1801,6 +1808,7 @@ struct World {
                 C -= j.C0Lin * (g_plant == 13 ? -P.alpha : P.alpha);
                 if (g_plant != 12) j.lamLin = mul(K, C) + j.lamLin;
             }
+            j.resLin = g_plant == 27 ? jointLin(j) : C;
             const float cap = g_plant == 14 ? 1e10f : pmax();
             j.penLin = lmin3(j.penLin + fabs3(C) * P.beta_lin,
                              lmin3(f3{j.stiffLin, j.stiffLin, j.stiffLin}, f3{cap, cap, cap}));
@@ -2061,7 +2069,10 @@ struct World {
             }
             const float decay = newton ? 1.0f : P.stiffness_decay;
             for (uint32_t s : live) {
-                if (frozen(s)) continue;
+                if (frozen(s)) {
+                    for (int i = 0; i < numC[s]; i++) resC[static_cast<size_t>(s) * MAXC + i] = 0.0f;
+                    continue;
+                }
                 if (decay < 1.0f && g_plant != 10 && g_plant != 11) decayPenalty(s, decay);
                 updateDual(s);
                 if (decay < 1.0f && g_plant == 10) decayPenalty(s, decay);
@@ -2371,7 +2382,7 @@ int oracle_upload(oracle_world *o, const avbd_bodies *b, const avbd_joints *j)
     w.mA.clear(); w.mB.clear(); w.basis.clear(); w.numC.clear(); w.mFric.clear(); w.cont.clear();
     w.dormant.clear(); w.saved.clear();
     w.C0.clear(); w.pen.clear(); w.lam.clear(); w.stick.clear(); w.betaScale.clear();
-    w.mFlags.clear(); w.capA.clear(); w.capB.clear(); w.rAW.clear(); w.rBW.clear(); w.fCache.clear();
+    w.mFlags.clear(); w.capA.clear(); w.capB.clear(); w.rAW.clear(); w.rBW.clear(); w.fCache.clear(); w.resC.clear();
     w.live.clear(); w.freeList.clear();
     return AVBD_OK;
 }
@@ -3124,6 +3135,18 @@ int oracle_set_manifolds(oracle_world *o, const avbd_manifolds *m)
 }
 
 void oracle_plant(int id) { g_plant = id; }
+
+// The last dual's constraint values: per live manifold (live order) MAXC normal values C.x
+// (entries past its contact count unused), per joint (force-list order) the linear C.
+int oracle_get_residuals(oracle_world *o, float *contact_cn, float *joint_clin)
+{
+    const World &w = o->w;
+    for (size_t i = 0; i < w.live.size(); i++)
+        for (int k = 0; k < MAXC; k++)
+            contact_cn[i * MAXC + k] = k < w.numC[w.live[i]] ? w.resC[static_cast<size_t>(w.live[i]) * MAXC + k] : 0.0f;
+    for (size_t q = 0; q < w.joints.size(); q++) store3(joint_clin, static_cast<int>(q), w.joints[q].resLin);
+    return AVBD_OK;
+}
 
 // Test hook: since the world was created, dual evaluations that reused cached arms and primal
 // touches served by the force cache.

@@ -26,6 +26,7 @@ int oracle_set_manifolds(oracle_world *o, const avbd_manifolds *m);
 // Planted mistakes for the oracle's own tests (0 = none; the ids are listed in avbd_oracle.cpp).
 void oracle_plant(int id);
 void oracle_cache_counters(oracle_world *o, long long *out);
+int oracle_get_residuals(oracle_world *o, float *contact_cn, float *joint_clin);
 // query::closestPoint on one shape; the server's closest point to a body's volume for n bodies.
 void oracle_closest_point_shape(int type, const float pos[3], const float rot[4], const float size[3],
                                 const float point[3], float out[3]);

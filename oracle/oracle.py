@@ -61,6 +61,7 @@ def lib():
         "oracle_digest": (C.c_uint64, [vp]),
         "oracle_plant": (None, [C.c_int]),
         "oracle_cache_counters": (None, [vp, C.POINTER(C.c_longlong)]),
+        "oracle_get_residuals": (C.c_int, [vp, a.f32p, a.f32p]),
         "oracle_collide_pair": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, a.i32p, a.f32p, a.f32p, a.f32p]),
         "oracle_stats": (None, [vp, a.i32p]),
         "oracle_colours": (None, [vp, a.u8p]),
@@ -354,6 +355,17 @@ class OracleWorld:
 
     def digest(self):
         return int(self.L.oracle_digest(self.h))
+
+    def residuals(self):
+        """The last dual's constraint values: (live manifolds x 8 normal values C.x, joints x 3
+        linear C), in the order of manifolds() and download(joints=True)."""
+        nm = int(self.L.oracle_manifold_count(self.h))
+        _, jt = self.download(joints=True)
+        nj = 0 if jt is None else len(jt["body_b"])
+        cn = np.zeros((nm, 8), np.float32)
+        cl = np.zeros((max(nj, 1), 3), np.float32)
+        assert self.L.oracle_get_residuals(self.h, cn.ctypes.data_as(self.a.f32p), cl.ctypes.data_as(self.a.f32p)) == 0
+        return cn, cl[:nj]
 
     def cache_counters(self):
         """(dual evaluations that reused cached arms, primal touches served by the force cache)

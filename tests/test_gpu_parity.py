@@ -440,18 +440,36 @@ def test_colours_valid_deterministic_and_equal_to_oracle():
     assert all(np.array_equal(x, y) for x, y in zip(*runs))
 
 
-def test_long_run_statistics_match_oracle():
-    """Long run (C1 pyramid, 300 steps): penetration-depth distribution, contact residual and
-    energy from the GPU state equal the oracle's (tolerance stated: 1e-6 relative; the
-    states are bit-identical, so they are equal)."""
-    bodies, joints = scenes.pyramid()
-    params = P(iterations=10)
-    w, o = lockstep(bodies, joints, params, 300, every=50)
-    gs = scenes.world_statistics({**bodies, **w.download()}, w.manifolds())
-    os_ = scenes.world_statistics({**bodies, **o.download()}, o.manifolds())
-    assert gs["contacts"] == os_["contacts"] and gs["contacts"] > 0
+@pytest.mark.parametrize("scene", ["pyramid", "chains"])
+def test_long_run_statistics_match_oracle(scene):
+    """Long runs (C1 pyramid, 300 steps; reduced C3 chains, 120 steps): penetration-depth
+    distribution, the last dual's residuals |C_n| over contacts and |C_lin| over joints, and the
+    energy drift over the run equal the oracle's (tolerance stated: 1e-6 relative; the states
+    are bit-identical, so they are equal)."""
+    if scene == "pyramid":
+        bodies, joints = scenes.pyramid()
+        params, steps = P(iterations=10), 300
+    else:
+        bodies, joints = scenes.chains(16, links=32, grid=(4, 4), pitch_x=4.0)
+        params, steps = P(iterations=10, beta_lin=1e5, penalty_max=1e9), 120
+    w = gpu_world(bodies, joints, params)
+    w.record_residuals(True)
+    o = oracle.OracleWorld(params)
+    o.upload(bodies, joints)
+    for _ in range(steps):
+        w.step()
+        o.step()
+    gs = scenes.world_statistics({**bodies, **w.download()}, w.manifolds(), w.residuals(), bodies)
+    os_ = scenes.world_statistics({**bodies, **o.download()}, o.manifolds(), o.residuals(), bodies)
+    assert gs["contacts"] == os_["contacts"] and gs["contacts"] > 0 and gs["residual_source"] == "dual"
+    if joints is not None:
+        assert os_["joint_residual_max"] > 0
     for k, v in os_.items():
-        assert abs(gs[k] - v) <= 1e-6 * max(1.0, abs(v)), (k, gs[k], v)
+        if isinstance(v, float):
+            assert abs(gs[k] - v) <= 1e-6 * max(1.0, abs(v)), (k, gs[k], v)
+    gn, gl = w.residuals()
+    on, ol = o.residuals()
+    assert bits_equal(gn, on) and bits_equal(gl, ol)
 
 
 def test_autotune_keeps_results():

@@ -760,6 +760,62 @@ def pin_arm_cache_reference_impact():
     assert o.cache_counters()[0] > 0
 
 
+# ------------------------------------------------------------------- residuals ----
+# The constraint values the step's last dual computes (manifold.cpp:626-630, joint.cpp:176-182),
+# kept for the long-run residual statistics.
+
+def pin_dual_residual_of_a_resting_contact():
+    """A box resting on the ground keeps its stuck contacts, so a step starts from the contact
+    error the previous step's rebase left (C0_prev), and the rebase after the dual measures the
+    pose the dual saw (C0_now; nothing moves the resting box in between): the dual's normal row
+    C = (1 - alpha) C0_prev + n . bracket equals C0_now - alpha C0_prev to rounding."""
+    s = scenes.SceneBuilder()
+    s.box((100, 100, 1), 0.0, 0.5, (0, 0, 0))
+    s.box((1, 1, 1), 1.0, 0.5, (0, 0, 1.0))
+    o = oracle.OracleWorld(P(iterations=10))
+    o.upload(*s.build())
+    o.step(40)
+    prev = o.manifolds()["c0"].reshape(-1, 8, 3)[0, :4, 0].astype(np.float64)
+    for _ in range(10):
+        o.step()
+        m = o.manifolds()
+        assert m["num_contacts"].tolist() == [4]
+        now = m["c0"].reshape(-1, 8, 3)[0, :4, 0].astype(np.float64)
+        cn, _ = o.residuals()
+        assert np.abs(cn[0, :4] - (now - 0.99 * prev)).max() < 1e-9, (cn[0, :4], now - 0.99 * prev)
+        assert np.abs(cn[0, :4]).max() > 1e-8
+        prev = now
+
+
+def pin_dual_residual_of_a_hard_joint():
+    """joint.cpp:176-182: a hard joint's dual value is C(x) - alpha C0.  A box hanging from a static
+    anchor: its final pose is the pose of the last dual (no contact moves it afterwards), so the
+    kept value equals the anchor gap at the final pose minus alpha times the step's C0, to
+    rounding; a spring records nothing."""
+    s = scenes.SceneBuilder()
+    anchor = s.box((1, 1, 1), 0.0, 0.5, (0, 0, 10))
+    box = s.box((1, 1, 1), 2.0, 0.5, (0.3, 0, 9))
+    s.joint(anchor, box, (0, 0, -0.5), (0, 0, 0.5))
+    a2 = s.box((1, 1, 1), 0.0, 0.5, (5, 0, 10))
+    b2 = s.box((1, 1, 1), 1.0, 0.5, (5, 0, 8.5))
+    s.spring(a2, b2, (0, 0, -0.5), (0, 0, 0.5), 50.0, rest=1.0)
+    bodies, joints = s.build()
+    n = len(s.b)
+    o = oracle.OracleWorld(P(iterations=10))
+    o.upload(bodies, joints)
+    for _ in range(25):
+        o.step()
+        st, jt = o.download(joints=True)
+        _, cl = o.residuals()
+        jb = int(np.nonzero(joints["kind"] == pkg.abi.FORCE_JOINT)[0][0])     # the force list is newest first
+        sp = int(np.nonzero(joints["kind"] == pkg.abi.FORCE_SPRING)[0][0])
+        pb, qb = st["position"][_pos(box, n)].astype(np.float64), st["rotation"][_pos(box, n)]
+        gap = np.array([0.0, 0.0, 9.5]) - (pb + scenes._qrot64(qb, (0, 0, 0.5)))
+        want = gap - 0.99 * jt["c0_lin"][jb].astype(np.float64)
+        assert np.abs(cl[jb] - want).max() < 1e-6 + 1e-5 * np.abs(want).max(), (cl[jb], want)
+        assert np.abs(cl[jb]).max() > 0 and not cl[sp].any()
+
+
 # ------------------------------------------------------------------- test entry points ----
 
 PINS = {name: fn for name, fn in globals().items() if name.startswith("pin_")}
@@ -783,6 +839,7 @@ PLANTS = {
     22: ["pin_arm_cache_reuses_the_arms_it_stored"], 23: ["pin_arm_cache_tiny_distance_is_no_cache"],
     24: ["pin_arm_cache_hits_follow_the_drift_rule"], 25: ["pin_arm_cache_hits_follow_the_drift_rule"],
     26: ["pin_primal_cache_serves_the_first_touch"],
+    27: ["pin_dual_residual_of_a_hard_joint"], 28: ["pin_dual_residual_of_a_resting_contact"],
 }
 
 
