@@ -177,6 +177,81 @@ def bench_chains(args, log):
             "value": movable / dt, "unit": "body-steps/s"}
 
 
+def bench_plain_configs(args, log):
+    """C1 (the reference's CPU-runnable case: 406-cube pyramid, 300 steps) and C2 (one world of
+    8192 boxes, seeded pile) on the plain single-world path, each beside the oracle on this
+    host's CPU; C1 also gives the long-run statistics of the GPU and the oracle (penetration
+    depth, the last dual's residuals, energy drift), which must agree within 1e-6 relative."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    params = pkg.default_params(iterations=args.iterations)
+    out = {}
+    # ---- C1: 300 steps, timed whole (the reference's bench shape), statistics at the end
+    b, j = pkg.scenes.pyramid()
+    w = pkg.World(params, len(b["mass"]), 0)
+    w.upload(b, j)
+    w.record_residuals(True)
+    w.step()                      # one-time setup (graph capture)
+    w2 = pkg.World(params, len(b["mass"]), 0)
+    w2.upload(b, j)
+    w2.record_residuals(True)
+    w2.step()
+    a = time.perf_counter()
+    w2.step(299)
+    gpu_s = time.perf_counter() - a
+    o = oracle.OracleWorld(params)
+    o.upload(b, j)
+    a = time.perf_counter()
+    o.step(300)
+    cpu_s = time.perf_counter() - a
+    w.step(299)
+    movable = int((b["mass"] > 0).sum())
+    gst = pkg.scenes.world_statistics({**b, **w.download()}, w.manifolds(), w.residuals(), b)
+    ost = pkg.scenes.world_statistics({**b, **o.download()}, o.manifolds(), o.residuals(), b)
+    agree = all(abs(gst[k] - v) <= 1e-6 * max(1.0, abs(v)) for k, v in ost.items() if isinstance(v, float))
+    same = pkg.state_digest(w.download()) == o.digest()
+    out["c1"] = {"workload": "C1 plain world: pyramid of 406 unit cubes (28 rows) on a static ground box, "
+                             "dt 1/60, 300 steps", "movable_bodies": movable, "steps": 299,
+                 "ms_per_step": gpu_s * 1e3 / 299, "value": movable * 299 / gpu_s, "unit": "body-steps/s",
+                 "cpu_oracle": {"value": movable * 300 / cpu_s, "unit": "body-steps/s", "cores": 1, "kind": "port",
+                                "ms_per_step": cpu_s * 1e3 / 300},
+                 "statistics_gpu": gst, "statistics_oracle": ost, "statistics_agree_1e-6": agree,
+                 "state_bit_identical": same}
+    w.close()
+    w2.close()
+    log(f"C1 pyramid: {gpu_s * 1e3 / 299:.3f} ms/step on the GPU, {cpu_s * 1e3 / 300:.3f} ms/step oracle; "
+        f"statistics agree {agree}, bit-identical {same}")
+    # ---- C2: 8192 boxes, 60 settle steps then 60 timed
+    b, j = pkg.scenes.pile(8192, seed=2)
+    w = pkg.World(params, len(b["mass"]), 0)
+    w.upload(b, j)
+    for _ in range(60):
+        w.step()
+    a = time.perf_counter()
+    w.step(60)
+    gpu_s = time.perf_counter() - a
+    movable = int((b["mass"] > 0).sum())
+    st, m = {**b, **w.download()}, w.manifolds()
+    o = oracle.OracleWorld(params)
+    o.upload(st, j)
+    o.set_manifolds(m)
+    a = time.perf_counter()
+    n_cpu = 0
+    while time.perf_counter() - a < 5.0 or n_cpu < 1:
+        o.step()
+        n_cpu += 1
+    cpu_s = time.perf_counter() - a
+    out["c2"] = {"workload": "C2 plain world: 8192 boxes, half extents U[0.25, 1] m, log-normal density, random "
+                             "orientations, dropped on a ground box; mu 0.5", "movable_bodies": movable,
+                 "settle_steps": 60, "steps": 60, "ms_per_step": gpu_s * 1e3 / 60, "value": movable * 60 / gpu_s,
+                 "unit": "body-steps/s", "manifolds": w.stats()["manifolds"],
+                 "cpu_oracle": {"value": movable * n_cpu / cpu_s, "unit": "body-steps/s", "cores": 1, "kind": "port",
+                                "steps": n_cpu}}
+    w.close()
+    log(f"C2 pile: {out['c2']['ms_per_step']:.3f} ms/step on the GPU, oracle {cpu_s * 1e3 / n_cpu:.1f} ms/step")
+    return out
+
+
 def bench_serving(args, log):
     """C5: Poisson arrivals of world requests (bodies log-uniform in [16, 16384], horizons
     U[1, 600] steps, 70 % piles / 30 % jointed chains), admitted by HBM budget into one
@@ -328,6 +403,8 @@ def main():
                     help="launch kernels one by one, batch and plain worlds (for tracing tools)")
     ap.add_argument("--tune-steps", type=int, default=2, help="autotuner steps per trial (0: keep defaults)")
     ap.add_argument("--no-c3", action="store_true", help="skip the C3 chains world")
+    ap.add_argument("--no-plain-configs", action="store_true",
+                    help="skip C1 (406-cube pyramid with long-run statistics) and C2 (8192-box world)")
     ap.add_argument("--c3-warmup", type=int, default=240)
     ap.add_argument("--c3-steps", type=int, default=30)
     ap.add_argument("--no-serve", action="store_true", help="skip the C5 serving loop")
@@ -521,6 +598,7 @@ def main():
         log(f"cpu baseline: {multi_value / 1e3:.1f} k body-steps/s on {threads} threads, "
             f"{one_value / 1e3:.1f} k on 1 ({cpu_model}, {os.cpu_count()} CPUs)")
 
+    plain_cfg = None if args.no_plain_configs else bench_plain_configs(args, log)
     c3 = None if args.no_c3 else bench_chains(args, log)
     serving = None if args.no_serve else bench_serving(args, log)
 
@@ -560,6 +638,8 @@ def main():
         "cpu_baseline": cpu,
         "tuning": tuning,
         "broadphase_lists": lists,
+        "c1": None if plain_cfg is None else plain_cfg["c1"],
+        "c2": None if plain_cfg is None else plain_cfg["c2"],
         "c3": c3,
         "serving": serving,
     }
