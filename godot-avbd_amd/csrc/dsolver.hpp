@@ -327,9 +327,21 @@ AD m3 loadBasisK(const State &S, int k)
 }
 AD m3 loadBasis(const State &S, int slot)
 {
-    const v3 r0 = ld3(S.mBasis[3 * slot]), r1 = ld3(S.mBasis[3 * slot + 1]), r2 = ld3(S.mBasis[3 * slot + 2]);
+    const float4 *r = S.mRec + 4 * static_cast<size_t>(slot);
+    const v3 r0 = ld3(r[0]), r1 = ld3(r[1]), r2 = ld3(r[2]);
     return transpose(m3{{r0, r1, r2}});
 }
+// The slot's whole record (see State::mRec), one aligned line.
+AD void storeRec(const State &S, int slot, const m3 &B, int a, int b, int item, float mu, int num)
+{
+    float4 *r = S.mRec + 4 * static_cast<size_t>(slot);
+    r[0] = st3(row(B, 0), __int_as_float(a));
+    r[1] = st3(row(B, 1), __int_as_float(b));
+    r[2] = st3(row(B, 2), __int_as_float(item));
+    r[3] = make_float4(mu, __int_as_float(num), 0.0f, 0.0f);
+}
+AD float recFric(const State &S, int slot) { return S.mRec[4 * static_cast<size_t>(slot) + 3].x; }
+AD int recItem(const State &S, int slot) { return __float_as_int(S.mRec[4 * static_cast<size_t>(slot) + 2].w); }
 
 // manifold.cpp:383-388
 AD v3 constraintError(const State &S, v3 c0, v3 bracket, const m3 &B)

@@ -84,6 +84,14 @@ struct RangeAlloc {
     }
 };
 
+// An int's bits as a float (record fields that hold ints).
+float intBits(int v)
+{
+    float f;
+    std::memcpy(&f, &v, sizeof(f));
+    return f;
+}
+
 template <typename T> hipError_t dalloc(T *&p, size_t n)
 {
     return hipMalloc(reinterpret_cast<void **>(&p), std::max<size_t>(n, 1) * sizeof(T));
@@ -935,7 +943,7 @@ struct Engine {
         AL(S.jFrac, J); AL(S.jArm, J); AL(S.jBroken, J); AL(S.jDead, J); AL(S.jKind, J); AL(S.jRest, J);
         AL(S.jGRest, J); AL(S.jGMode, 6 * J); AL(S.jGLo, 6 * J); AL(S.jGHi, 6 * J); AL(S.jGK, 6 * J);
         AL(S.jGEq, 6 * J); AL(S.jGDamp, 6 * J); AL(S.jGLam, 6 * J); AL(S.jGPen, 6 * J); AL(S.jGInit, 6 * J);
-        AL(S.mA, M); AL(S.mB, M); AL(S.mBasis, 3 * M); AL(S.mFric, M); AL(S.mNum, M);
+        AL(S.mA, M); AL(S.mB, M); AL(S.mRec, 4 * M); AL(S.mNum, M);
         // sliced contacts: at most MAXC rows per lane of every slice, the last slice padded
         const size_t CD = (static_cast<size_t>(Scap) + CSLICE) * MAXC;
         AL(S.cRA, CD); AL(S.cRB, CD); AL(S.cC0, CD); AL(S.cPL, CD); AL(S.cLyz, CD); AL(S.cStick, CD);
@@ -962,7 +970,7 @@ struct Engine {
         AL(S.listC, N); AL(S.listH, N); AL(S.listJ, N * LIST_K); AL(S.listCnt, N); AL(S.listAdd, N); AL(S.listFlag, N);
         AL(S.wRewalk, W); AL(S.wRefit, W); AL(S.wMoved, W); AL(S.wStale, W); AL(S.listStats, 4);
         AL(S.wBoundsLo, W); AL(S.wBoundsHi, W);
-        AL(S.gOfK, M); AL(S.kSlot, M); AL(S.kT, M); AL(S.sPos, M); AL(S.newT, P); AL(S.mHdr, M); AL(S.mBasisK, 3 * M); AL(S.mFricK, M); AL(S.brakeOn, M); AL(S.wOrder, W);
+        AL(S.gOfK, M); AL(S.kSlot, M); AL(S.kT, M); AL(S.newT, P); AL(S.mHdr, M); AL(S.mBasisK, 3 * M); AL(S.mFricK, M); AL(S.brakeOn, M); AL(S.wOrder, W);
         radixBlocks = (Ncap + RADIX_TILE - 1) / RADIX_TILE;
         AL(radixHist, static_cast<size_t>(256) * radixBlocks + 1);
         const int biggest = std::max({Ncap + 1, Pcap + 1, Scap + 1, 256 * radixBlocks + 1, 2 * (Scap + Jcap)});
@@ -2105,13 +2113,11 @@ struct Engine {
         HIPCHK(hipMemcpy(live.data(), S.live + wi.slotBase, sizeof(int) * nLive, hipMemcpyDeviceToHost));
         const int cap = wi.slotCap, sb = wi.slotBase;
         std::vector<int> mA(cap), mB(cap), num(cap);
-        std::vector<float> fr(cap);
-        std::vector<float4> basis(3 * cap), cra(cap * MAXC), crb(cap * MAXC), cc0(cap * MAXC), cp(cap * MAXC), cl(cap * MAXC);
+        std::vector<float4> rec(4 * cap), cra(cap * MAXC), crb(cap * MAXC), cc0(cap * MAXC), cp(cap * MAXC), cl(cap * MAXC);
         HIPCHK(hipMemcpy(mA.data(), S.mA + sb, sizeof(int) * cap, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(mB.data(), S.mB + sb, sizeof(int) * cap, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(num.data(), S.mNum + sb, sizeof(int) * cap, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(fr.data(), S.mFric + sb, sizeof(float) * cap, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(basis.data(), S.mBasis + 3 * sb, sizeof(float4) * 3 * cap, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(rec.data(), S.mRec + 4 * static_cast<size_t>(sb), sizeof(float4) * 4 * cap, hipMemcpyDeviceToHost));
         const size_t cb = static_cast<size_t>(sb) * MAXC, cn = static_cast<size_t>(cap) * MAXC;
         HIPCHK(hipMemcpy(cra.data(), S.sRA + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(crb.data(), S.sRB + cb, sizeof(float4) * cn, hipMemcpyDeviceToHost));
@@ -2126,10 +2132,10 @@ struct Engine {
             if (m->body_b) m->body_b[i] = mB[s] - o;
             if (m->basis)
                 for (int r = 0; r < 3; r++) {
-                    const float4 v = basis[3 * s + r];
+                    const float4 v = rec[4 * s + r];
                     m->basis[9 * i + 3 * r] = v.x; m->basis[9 * i + 3 * r + 1] = v.y; m->basis[9 * i + 3 * r + 2] = v.z;
                 }
-            if (m->friction) m->friction[i] = fr[s];
+            if (m->friction) m->friction[i] = rec[4 * s + 3].x;
             if (m->num_contacts) m->num_contacts[i] = num[s];
             for (int k = 0; k < MAXC; k++) {
                 const int src = s * MAXC + k, dst = i * MAXC + k;
@@ -2201,8 +2207,7 @@ struct Engine {
         fill(S.mSaved + sb, 0, wi.slotCap);
         if (S.armCap) fill(reinterpret_cast<int *>(S.armCap + 2 * static_cast<size_t>(sb)), 0, 8 * wi.slotCap);   // no captured arms
         std::vector<int> live(n), mA(n), mB(n), num(n);
-        std::vector<float> fr(n);
-        std::vector<float4> basis(3 * n), cra(n * MAXC), crb(n * MAXC), cc0(n * MAXC), cp(n * MAXC), cl(n * MAXC);
+        std::vector<float4> rec(4 * n), cra(n * MAXC), crb(n * MAXC), cc0(n * MAXC), cp(n * MAXC), cl(n * MAXC);
         for (int i = 0; i < n; i++) {
             live[i] = sb + i;
             if (m->body_a[i] < 0 || m->body_a[i] >= wi.bodyCount || m->body_b[i] < 0 || m->body_b[i] >= wi.bodyCount)
@@ -2211,9 +2216,10 @@ struct Engine {
             mB[i] = m->body_b[i] + o;
             num[i] = m->num_contacts[i];
             if (num[i] < 0 || num[i] > MAXC) return fail(AVBD_ERR_INVALID, "manifold contact count out of range");
-            fr[i] = m->friction[i];
+            rec[4 * i + 3] = make_float4(m->friction[i], intBits(num[i]), 0.0f, 0.0f);
             for (int r = 0; r < 3; r++)
-                basis[3 * i + r] = make_float4(m->basis[9 * i + 3 * r], m->basis[9 * i + 3 * r + 1], m->basis[9 * i + 3 * r + 2], 0);
+                rec[4 * i + r] = make_float4(m->basis[9 * i + 3 * r], m->basis[9 * i + 3 * r + 1], m->basis[9 * i + 3 * r + 2],
+                                             intBits(r == 0 ? mA[i] : r == 1 ? mB[i] : 0));
             for (int k = 0; k < MAXC; k++) {
                 const int q = i * MAXC + k;
                 float keyf;
@@ -2237,8 +2243,7 @@ struct Engine {
             HIPCHK(hipMemcpy(S.mA + sb, mA.data(), sizeof(int) * n, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(S.mB + sb, mB.data(), sizeof(int) * n, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(S.mNum + sb, num.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(S.mFric + sb, fr.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(S.mBasis + 3 * sb, basis.data(), sizeof(float4) * 3 * n, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S.mRec + 4 * static_cast<size_t>(sb), rec.data(), sizeof(float4) * 4 * n, hipMemcpyHostToDevice));
             const size_t cb = static_cast<size_t>(sb) * MAXC, cn = static_cast<size_t>(n) * MAXC;
             HIPCHK(hipMemcpy(S.sRA + cb, cra.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(S.sRB + cb, crb.data(), sizeof(float4) * cn, hipMemcpyHostToDevice));

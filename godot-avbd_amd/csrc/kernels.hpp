@@ -1001,12 +1001,9 @@ __device__ void allocNew(const State &S, int p)
     S.mA[slot] = i;
     S.mB[slot] = j;
     S.mNum[slot] = 0;
-    S.mFric[slot] = 0.0f;
     S.mDormant[slot] = 0;
     S.mSaved[slot] = 0;
-    S.mBasis[3 * slot] = make_float4(0, 0, 0, 0);
-    S.mBasis[3 * slot + 1] = make_float4(0, 0, 0, 0);
-    S.mBasis[3 * slot + 2] = make_float4(0, 0, 0, 0);
+    storeRec(S, slot, zero3(), i, j, 0, 0.0f, 0);   // manifold.cpp:55-66: basis, friction, count reset
 }
 
 __global__ void k_alloc_counts(State S)
@@ -1081,11 +1078,13 @@ __device__ __forceinline__ void stageContacts(const State &S, int slot)
 // Dense per-step view of live manifold k (stored in `slot`, contacts from `o`).
 __device__ __forceinline__ void manifoldView(const State &S, int k, int slot, uint32_t o, int n)
 {
-    S.mHdr[k] = make_int4(S.mA[slot], S.mB[slot], static_cast<int>(o), n);
-    S.mBasisK[3 * k] = S.mBasis[3 * slot];
-    S.mBasisK[3 * k + 1] = S.mBasis[3 * slot + 1];
-    S.mBasisK[3 * k + 2] = S.mBasis[3 * slot + 2];
-    S.mFricK[k] = S.mFric[slot];
+    const float4 *r = S.mRec + 4 * static_cast<size_t>(slot);
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+    S.mHdr[k] = make_int4(__float_as_int(r0.w), __float_as_int(r1.w), static_cast<int>(o), n);
+    S.mBasisK[3 * k] = make_float4(r0.x, r0.y, r0.z, 0.0f);
+    S.mBasisK[3 * k + 1] = make_float4(r1.x, r1.y, r1.z, 0.0f);
+    S.mBasisK[3 * k + 2] = make_float4(r2.x, r2.y, r2.z, 0.0f);
+    S.mFricK[k] = r3.x;
 }
 
 // The view alone, contacts left where they are (after host edits of the live lists).
@@ -1112,7 +1111,8 @@ __global__ void __launch_bounds__(TPB) k_slice_widths(State S)
 }
 
 // Staging -> sliced dense arrays (cScan holds each slice's start), one lane per manifold.
-// After the narrow phase the staging rows are those of the manifold's work item (sPos), which
+// After the narrow phase the staging rows are those of the manifold's work item (its record's
+// row 2 w), which
 // mostly runs in the same order as the view, so reads and writes are both runs; after host
 // edits they are the host layout (slot * MAXC + i).
 __global__ void k_contact_pack(State S, int hostLayout)
@@ -1122,7 +1122,7 @@ __global__ void k_contact_pack(State S, int hostLayout)
         const int n = static_cast<int>(S.cCount[k]);
         const uint32_t o = S.cScan[k / CSLICE] + static_cast<uint32_t>(k % CSLICE);
         if (hostLayout) stageToDense(S, o, CSLICE, static_cast<size_t>(slot) * MAXC, 1, n);
-        else stageToDense(S, o, CSLICE, stageRow(S.sPos[slot], 0), CSLICE, n);
+        else stageToDense(S, o, CSLICE, stageRow(recItem(S, slot), 0), CSLICE, n);
         S.cOff[slot] = o;
         manifoldView(S, k, slot, o, n);
     }
@@ -1202,30 +1202,17 @@ template <int MODE> AD void manifoldInit(const State &S, int q, int P, NarrowPol
     const int shA = __float_as_int(a5.x), shB = __float_as_int(b5.x);
     const bool round = (shA | shB) != AVBD_SHAPE_BOX;
     if (ROUND && (frozen || !round)) return;
-    if (MODE == 2) {
-        S.sPos[slot] = q;
-        if (frozen) {   // parked verbatim (a frozen parked manifold keeps its kept contacts too)
-            denseToStage(S, stageRow(q, 0), CSLICE, ob, CSLICE, oldN);
-            S.keep[t] = 1;
-            return;
-        }
-        if (S.armCap) S.armCap[2 * static_cast<size_t>(slot)].w = 0.0f;   // manifold.cpp:246-247
-    } else if (!ROUND) {
-        S.sPos[slot] = q;
-        if (frozen) {   // parked verbatim
-            denseToStage(S, stageRow(q, 0), CSLICE, ob, CSLICE, oldN);
-            S.keep[t] = 1;
-            return;
-        }
-        // manifold.cpp:246-247: a new contact set drops the captured arms
-        if (S.armCap) S.armCap[2 * static_cast<size_t>(slot)].w = 0.0f;
-        if (round) return;
-    } else {
-        S.sPos[slot] = q;
+    if (!ROUND && frozen) {   // parked verbatim (under grace with its kept contacts too)
+        denseToStage(S, stageRow(q, 0), CSLICE, ob, CSLICE, oldN);
+        S.keep[t] = 1;
+        S.mRec[4 * static_cast<size_t>(slot) + 2].w = __int_as_float(q);
+        return;
     }
+    if (MODE == 0 && round) return;   // the round pass takes it
+    // manifold.cpp:246-247: a new contact set drops the captured arms
+    if (S.armCap) S.armCap[2 * static_cast<size_t>(slot)].w = 0.0f;
     const float fa = a4.w, fb = b4.w;
     const float mu = sqrtf(fa * fb);
-    S.mFric[slot] = mu;
     const v3 pa = ld3(a0), pb = ld3(b0);
     const qt qa = ldq(a1), qb = ldq(b1);
     const float4 sa = a2, sb = b2;
@@ -1247,13 +1234,7 @@ template <int MODE> AD void manifoldInit(const State &S, int q, int P, NarrowPol
         nn = collideBoxes(pa, qa, v3{sa.x, sa.y, sa.z} * 0.5f, pb, qb, v3{sb.x, sb.y, sb.z} * 0.5f, margin, B, wrote,
                           cs, c0, c1);
     }
-    if (wrote) {
-        S.mBasis[3 * slot] = st3(row(B, 0));
-        S.mBasis[3 * slot + 1] = st3(row(B, 1));
-        S.mBasis[3 * slot + 2] = st3(row(B, 2));
-    } else {
-        B = loadBasis(S, slot);
-    }
+    if (!wrote) B = loadBasis(S, slot);
     // old contacts stay in the dense arrays; the new ones go to staging
     const float pmax = S.P.penaltyMax;
     // contact grace (manifold.cpp:249-253): a parked manifold (no contacts in the view) matches
@@ -1359,6 +1340,7 @@ template <int MODE> AD void manifoldInit(const State &S, int q, int P, NarrowPol
                 S.mDormant[slot] += 1;
                 S.mSaved[slot] = liveCount;
                 S.mNum[slot] = 0;
+                storeRec(S, slot, B, a, b, q, mu, 0);
                 S.keep[t] = 1u;
                 return;
             }
@@ -1367,6 +1349,7 @@ template <int MODE> AD void manifoldInit(const State &S, int q, int P, NarrowPol
         S.mDormant[slot] = 0;
     }
     S.mNum[slot] = nn;
+    storeRec(S, slot, B, a, b, q, mu, nn);
     S.keep[t] = nn > 0 ? 1u : 0u;
 }
 

@@ -1008,7 +1008,7 @@ __global__ void k_report_emit(State S, int w, int n, const int *maxRep, int stri
             last = key;
             const int slot = S.live[sb + (key >> 1)], side = key & 1;
             const int nc = S.mNum[slot];
-            const float4 b0 = S.mBasis[3 * slot];
+            const float4 b0 = S.mRec[4 * static_cast<size_t>(slot)];
             const float sgn = side == 0 ? 1.0f : -1.0f;
             const int otherBody = (side == 0 ? S.mB[slot] : S.mA[slot]) - base;
             for (int c = 0; c < nc && emitted < lim; c++) {

@@ -111,9 +111,12 @@ struct State {
 
     // ---- manifold slots
     int *mA, *mB;
-    float4 *mBasis;
-    float *mFric;
-    int *mNum;
+    // Per slot one aligned 64-byte record (mRec[4 slot + r]), written whole by the narrow phase
+    // for every manifold it keeps (one full line instead of scattered partial lines): rows 0-2
+    // the basis rows (xyz) with body a, body b and this step's narrow-phase work item (its
+    // staging rows) in w as int bits; row 3 {friction, contact count (int bits), 0, 0}.
+    float4 *mRec;
+    int *mNum;          // contact count per slot (also in the record)
     // Contacts in view order (all worlds, world after world), sliced by wavefront: contact i
     // of the manifold in `slot` is at cOff[slot] + i * CSLICE.  Fields: rA | beta scale,
     // rB | feature key, C0, penalty and lambda (packed in 24 bytes), stick flag.
@@ -140,7 +143,6 @@ struct State {
     int *kSlot;                 // storage slot of view index k
     int *kT;                    // slot-range position (live list entry) of view index k
     int *newT;                  // per new manifold of this step (pair order): its live entry
-    int *sPos;                  // per slot: its narrow-phase work item this step (staging rows)
     int4 *mHdr;                 // {body a, body b, first contact, contact count}
     float4 *mBasisK;            // 3 rows per k
     float *mFricK;
