@@ -342,7 +342,6 @@ AD void storeRec(const State &S, int slot, const m3 &B, int a, int b, int item, 
 }
 AD float recFric(const State &S, int slot) { return S.mRec[4 * static_cast<size_t>(slot) + 3].x; }
 AD int recItem(const State &S, int slot) { return __float_as_int(S.mRec[4 * static_cast<size_t>(slot) + 2].w); }
-AD int recNum(const State &S, int slot) { return __float_as_int(S.mRec[4 * static_cast<size_t>(slot) + 3].y); }
 
 // manifold.cpp:383-388
 AD v3 constraintError(const State &S, v3 c0, v3 bracket, const m3 &B)

@@ -822,7 +822,6 @@ struct Engine {
         k_fill_u32<<<nblk(n + 1), TPB, 0, stream>>>(rCnt, 0u, n + 1);
         k_fill_u32<<<nblk(n), TPB, 0, stream>>>(rCursor, 0u, n);
         if (nLive > 0) {
-            syncSlots();
             buildLiveList();
             k_contact_unpack<<<liveGrid(), TPB, 0, stream>>>(S);
             k_report_count<<<nblk(nLive), TPB, 0, stream>>>(S, w, rMax, rCnt);
@@ -1068,21 +1067,9 @@ struct Engine {
         k_contact_pack<<<liveGrid(), TPB, 0, stream>>>(S, hostLayout);
     }
 
-    // A step keeps each live manifold's contact offset and count in its view and slot record
-    // only; host-side paths that address manifolds by slot first copy them to cOff / mNum from
-    // the last step's view (before anything rebuilds the view).
-    bool slotSyncNeeded = false;
-    void syncSlots()
-    {
-        if (!slotSyncNeeded) return;
-        k_slot_sync<<<liveGrid(), TPB, 0, stream>>>(S);
-        slotSyncNeeded = false;
-    }
-
     // Live list and dense manifold view after host edits (contacts stay where they are).
     void refreshView()
     {
-        syncSlots();
         buildLiveList();
         k_manifold_view<<<liveGrid(), TPB, 0, stream>>>(S);
     }
@@ -1184,7 +1171,6 @@ struct Engine {
             refreshView();
             buildAdjacency();
         }
-        slotSyncNeeded = true;
         // ---- broad phase (solver_broadphase.cpp:85-186)
         // every world lists again at a tree rebuild, dropping entries that no longer meet
         if (rebuildTree) k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wRewalk, 1, Wcap);
@@ -1517,10 +1503,7 @@ struct Engine {
                 plainGraphUnusable = true;
             }
         }
-        if (g) {
-            HIPCHK(hipGraphLaunch(g, stream));
-            slotSyncNeeded = true;
-        }
+        if (g) HIPCHK(hipGraphLaunch(g, stream));
         else enqueueStep(substeps, rounds, rebuild, anyJoints(), true);
         HIPCHK(hipGetLastError());
         avbd_status st = checkErrors();
@@ -1708,7 +1691,6 @@ struct Engine {
             const bool rebuild = treeAge >= rebuildInterval();
             treeAge = rebuild ? 0 : treeAge + 1;
             HIPCHK(hipGraphLaunch(graphs[rebuild ? 0 : 1], stream));
-            slotSyncNeeded = true;
         }
         if ((st = queueErrorCopies()) != AVBD_OK) return st;
         stepInFlight = true;
@@ -2114,7 +2096,6 @@ struct Engine {
     avbd_status getManifolds(int w, avbd_manifolds *m)
     {
         const WorldInfo &wi = worlds[w];
-        syncSlots();
         HIPCHK(hipStreamSynchronize(stream));
         int nLive = 0;
         HIPCHK(hipMemcpy(&nLive, S.wLive + w, sizeof(int), hipMemcpyDeviceToHost));
@@ -2179,7 +2160,6 @@ struct Engine {
     {
         if (!S.resC) return fail(AVBD_ERR_INVALID, "residuals are not being kept (avbd_world_record_residuals)");
         const WorldInfo &wi = worlds[w];
-        syncSlots();
         HIPCHK(hipStreamSynchronize(stream));
         int nLive = 0;
         HIPCHK(hipMemcpy(&nLive, S.wLive + w, sizeof(int), hipMemcpyDeviceToHost));
@@ -2253,7 +2233,6 @@ struct Engine {
         }
         // every other live manifold's contacts go to staging, this world's are written there,
         // and the dense layout is rebuilt from staging
-        syncSlots();
         HIPCHK(hipStreamSynchronize(stream));
         buildLiveList();
         k_contact_unpack<<<liveGrid(), TPB, 0, stream>>>(S);
@@ -2347,7 +2326,6 @@ struct Engine {
 
     avbd_status totals(int64_t *movable, int64_t *manifolds, int64_t *contacts, int64_t *joints)
     {
-        syncSlots();
         HIPCHK(hipStreamSynchronize(stream));
         std::vector<int> bw(Ncap), ui(Ncap), live(Wcap), slotWorld(Scap), liveList(Scap), num(Scap), jw(Jcap);
         std::vector<uint8_t> dead(Jcap);
@@ -2392,7 +2370,6 @@ struct Engine {
 
     avbd_status fillStats(int w, avbd_step_stats *out)
     {
-        syncSlots();
         HIPCHK(hipStreamSynchronize(stream));
         avbd_step_stats s = stats;
         const WorldInfo &wi = worlds[w];

@@ -831,7 +831,7 @@ __global__ void __launch_bounds__(ORDER_TPB) k_manifold_order(State S)
     if (tid < NB) start[tid] = 0;
     if (tid < NW * NB) wcnt[tid / NB][tid % NB] = 0;
     __syncthreads();
-    for (int i = tid; i < n; i += ORDER_TPB) atomicAdd(&start[recNum(S, S.live[S.gLive[base + i]])], 1u);
+    for (int i = tid; i < n; i += ORDER_TPB) atomicAdd(&start[S.mNum[S.live[S.gLive[base + i]]]], 1u);
     __syncthreads();
     if (tid == 0) {
         uint32_t run = 0;
@@ -842,7 +842,7 @@ __global__ void __launch_bounds__(ORDER_TPB) k_manifold_order(State S)
         const int i = c0 + tid;
         const bool valid = i < n;
         const int slot = valid ? S.live[S.gLive[base + i]] : 0;
-        const uint32_t v = valid ? static_cast<uint32_t>(recNum(S, slot)) : 0u;
+        const uint32_t v = valid ? static_cast<uint32_t>(S.mNum[slot]) : 0u;
         unsigned long long m = __ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < 4; bit++) {
@@ -1123,21 +1123,8 @@ __global__ void k_contact_pack(State S, int hostLayout)
         const uint32_t o = S.cScan[k / CSLICE] + static_cast<uint32_t>(k % CSLICE);
         if (hostLayout) stageToDense(S, o, CSLICE, static_cast<size_t>(slot) * MAXC, 1, n);
         else stageToDense(S, o, CSLICE, stageRow(recItem(S, slot), 0), CSLICE, n);
-        if (hostLayout) S.cOff[slot] = o;
+        S.cOff[slot] = o;
         manifoldView(S, k, slot, o, n);
-    }
-}
-
-// The slot-indexed copies of each live manifold's contact offset and count (cOff, mNum) from
-// the step's view, for the host-side paths that address manifolds by slot (the step itself
-// keeps them in the view and the slot records only).
-__global__ void k_slot_sync(State S)
-{
-    FOR_LIVE(k) {
-        const int slot = S.kSlot[k];
-        const int4 h = S.mHdr[k];
-        S.cOff[slot] = static_cast<uint32_t>(h.z);
-        S.mNum[slot] = h.w;
     }
 }
 
@@ -1352,6 +1339,7 @@ template <int MODE> AD void manifoldInit(const State &S, int q, int P, NarrowPol
                     }
                 S.mDormant[slot] += 1;
                 S.mSaved[slot] = liveCount;
+                S.mNum[slot] = 0;
                 storeRec(S, slot, B, a, b, q, mu, 0);
                 S.keep[t] = 1u;
                 return;
@@ -1360,6 +1348,7 @@ template <int MODE> AD void manifoldInit(const State &S, int q, int P, NarrowPol
         }
         S.mDormant[slot] = 0;
     }
+    S.mNum[slot] = nn;
     storeRec(S, slot, B, a, b, q, mu, nn);
     S.keep[t] = nn > 0 ? 1u : 0u;
 }

@@ -116,8 +116,7 @@ struct State {
     // the basis rows (xyz) with body a, body b and this step's narrow-phase work item (its
     // staging rows) in w as int bits; row 3 {friction, contact count (int bits), 0, 0}.
     float4 *mRec;
-    int *mNum;          // contact count per slot: kept in the record during steps, copied here
-                        // (with cOff) from the view before host-side paths read it
+    int *mNum;          // contact count per slot (also in the record)
     // Contacts in view order (all worlds, world after world), sliced by wavefront: contact i
     // of the manifold in `slot` is at cOff[slot] + i * CSLICE.  Fields: rA | beta scale,
     // rB | feature key, C0, penalty and lambda (packed in 24 bytes), stick flag.
@@ -125,7 +124,7 @@ struct State {
     float4 *cPL;                // {penalty.xyz, lambda.x}
     float2 *cLyz;               // {lambda.y, lambda.z}
     uint8_t *cStick;            // stick flag
-    uint32_t *cOff;             // per slot: its contacts' first row (see mNum)
+    uint32_t *cOff;
     // Staging: the narrow phase writes the new contacts here (sliced by its work items) and
     // k_contact_pack lays them out in view order once the live list is final; host reads and
     // edits use it with a fixed stride (slot * MAXC + i).
