@@ -1318,7 +1318,7 @@ struct Engine {
                             pend();
                         }
                     if (stampHere) {
-                        k_pose_pack<<<nb, TPB, 0, stream>>>(S);
+                        // pose records: written by the warm start and by every move since
                         pbeg(AVBD_K_STAMP);
                         if (arms)
                             k_stamp<true><<<liveGrid(tune.stamp), tune.stamp, 0, stream>>>(S);
@@ -1334,7 +1334,6 @@ struct Engine {
                         pbeg(AVBD_K_BODY);
                         k_overflow_apply<<<nb, TPB, 0, stream>>>(S);
                         pend();
-                        k_pose_pack<<<nb, TPB, 0, stream>>>(S);
                     } else {
                         // no joints: bodies move inside the solve and refresh their pose records
                         k_overflow_solve<true><<<nblk(Ncap, tune.primal), tune.primal, 0, stream>>>(S);

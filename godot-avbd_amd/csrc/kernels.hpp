@@ -1802,6 +1802,15 @@ __global__ void __launch_bounds__(BRAKE_TPB) k_brake_levels(State S, float windo
 }
 
 // ------------------------------------------------------------ body phases ----
+// A body's pose record (see State::poseRec) at pose (p, q): its displacement from the step's
+// initial pose, its awake mass, the rotation and the rotation difference.
+__device__ __forceinline__ void writePose(const State &S, int i, v3 p, qt q)
+{
+    S.poseRec[4 * i] = st3(p - ld3(S.initLin[i]), S.P.sleepOn && S.sleeping[i] ? 0.0f : bMass(S, i));
+    S.poseRec[4 * i + 1] = stq(q);
+    S.poseRec[4 * i + 2] = st3(qsub(q, ldq(S.initAng[i])));
+}
+
 // solver_solve.cpp:109-142
 __global__ void k_warmstart(State S)
 {
@@ -1828,10 +1837,18 @@ __global__ void k_warmstart(State S)
     if (!__builtin_isfinite(wgt)) wgt = 0.0f;
     S.initLin[i] = st3(p);
     S.initAng[i] = stq(q);
+    v3 np = p;
+    qt nq = q;
     if (m > 0) {
-        S.pos[i] = st3((p + vel * dt) + v3{0, 0, g} * (wgt * dt * dt));
-        S.rot[i] = stq(qadd(q, ang * dt));
+        np = (p + vel * dt) + v3{0, 0, g} * (wgt * dt * dt);
+        nq = qadd(q, ang * dt);
+        S.pos[i] = st3(np);
+        S.rot[i] = stq(nq);
     }
+    // the substep's first pose records (moves in the rounds refresh their bodies')
+    S.poseRec[4 * i] = st3(np - p, S.P.sleepOn && S.sleeping[i] ? 0.0f : m);
+    S.poseRec[4 * i + 1] = stq(nq);
+    S.poseRec[4 * i + 2] = st3(qsub(nq, q));
 }
 
 // Pose records for the per-manifold passes (the displacement and rotation difference are the
@@ -1839,18 +1856,6 @@ __global__ void k_warmstart(State S)
 // manifold side; same floats).
 // mass slot: the body's mass while it is awake, 0 while it sleeps (the manifold passes treat a
 // sleeper as they treat a static body: manifold.cpp:168-172)
-__device__ __forceinline__ void writePose(const State &S, int i, v3 p, qt q)
-{
-    S.poseRec[4 * i] = st3(p - ld3(S.initLin[i]), S.P.sleepOn && S.sleeping[i] ? 0.0f : bMass(S, i));
-    S.poseRec[4 * i + 1] = stq(q);
-    S.poseRec[4 * i + 2] = st3(qsub(q, ldq(S.initAng[i])));
-}
-__global__ void k_pose_pack(State S)
-{
-    const int i = XGID;
-    if (i >= S.numBodies || S.bodyWorld[i] < 0) return;
-    writePose(S, i, bPos(S, i), bRot(S, i));
-}
 
 // solver_step.cpp:268-273 for one body: move by the relaxed Jacobi delta.
 __device__ __forceinline__ void jacobiApply(const State &S, int i, v3 dl, v3 da, bool pose)
@@ -2019,8 +2024,11 @@ __global__ void __launch_bounds__(TPB) k_colour_pass(State S, int colour, int ca
         assemble<ARMS>(S, i, blk, cachedRound != 0, colour);
         v3 dl, da;
         if (!solveDeltaFast(blk, dl, da)) solveDeltaSlow(blk, dl, da);
-        S.pos[i] = st3(bPos(S, i) + dl);
-        S.rot[i] = stq(qadd(bRot(S, i), da));
+        const v3 np = bPos(S, i) + dl;
+        const qt nq = qadd(bRot(S, i), da);
+        S.pos[i] = st3(np);
+        S.rot[i] = stq(nq);
+        writePose(S, i, np, nq);
     }
 }
 
@@ -2085,8 +2093,11 @@ __global__ void __launch_bounds__(QUAD_TPB) k_colour_pass_quad(State S, int colo
         if (sub == 0) {
             v3 dl, da;
             if (!solveDeltaFast(blk, dl, da)) solveDeltaSlow(blk, dl, da);
-            S.pos[i] = st3(bPos(S, i) + dl);
-            S.rot[i] = stq(qadd(bRot(S, i), da));
+            const v3 np = bPos(S, i) + dl;
+            const qt nq = qadd(bRot(S, i), da);
+            S.pos[i] = st3(np);
+            S.rot[i] = stq(nq);
+            writePose(S, i, np, nq);
         }
     }
 }
@@ -2144,9 +2155,7 @@ __global__ void k_overflow_apply(State S)
     const int i = XGID;
     if (i >= S.numBodies || S.colour[i] != 8 || S.bodyWorld[i] < 0) return;
     if (S.updateIndex[i] < 0) return;
-    const float r = S.P.jacobiRelax;
-    S.pos[i] = st3(bPos(S, i) + ld3(S.dLin[i]) * r);
-    S.rot[i] = stq(qadd(bRot(S, i), ld3(S.dAng[i]) * r));
+    jacobiApply(S, i, ld3(S.dLin[i]), ld3(S.dAng[i]), true);
 }
 
 // Stamps of every live manifold that touches a movable body, at the current poses.

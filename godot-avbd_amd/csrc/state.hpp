@@ -52,9 +52,10 @@ struct State {
     // ---- bodies
     float4 *pos, *rot, *initLin, *initAng, *inertLin, *inertAng, *vel, *angVel, *prevVel;
     float4 *sizeR, *massI;
-    // Per body, 64 B (written by k_pose_pack before each pass that gathers body poses per
-    // manifold): {pos - initial, mass}, rot, {rot (-) initial rotation, 0}, unused.  One aligned
-    // half line per gather instead of five separate rows.
+    // Per body, 64 B (written by the substep's warm start and by every move in the rounds, so
+    // it is current whenever a pass gathers body poses per manifold): {pos - initial, awake
+    // mass}, rot, {rot (-) initial rotation, 0}, unused.  One aligned half line per gather
+    // instead of five separate rows.
     float4 *poseRec;
     // Per body, NARROW_REC float4 (one 128-byte line) written with the swept AABB for the
     // narrow phase: {pos, mass}, rot, {size, radius}, {vel, gravity}, {angVel, friction}.
