@@ -458,7 +458,7 @@ def main():
             log(f"settle step {s + 1}/{args.settle}")
     tuning = batch.autotune(args.tune_steps) if args.tune_steps > 0 else None
     if tuning:
-        log(f"autotuned launch shapes ({6 * args.tune_steps} trial steps): {tuning}")
+        log(f"autotuned launch shapes ({8 * args.tune_steps} trial steps): {tuning}")
     for _ in range(args.warmup):
         batch.step()
     tot = batch.totals()

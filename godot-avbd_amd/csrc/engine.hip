@@ -1596,7 +1596,7 @@ struct Engine {
         tune.stamp = best(AVBD_K_STAMP);
         tune.proj = best(AVBD_K_PROJ_MANIFOLD);
         tune.traverse = best(AVBD_K_TRAVERSE);
-        const int grids[3] = {1024, 2048, 4096};
+        const int grids[5] = {64, 256, 1024, 2048, 4096};   // few live manifolds want a small grid
         float bestT = 0.0f;
         int bestG = LIVE_BLOCKS;
         for (int g : grids) {

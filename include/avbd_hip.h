@@ -467,9 +467,11 @@ typedef struct avbd_tuning {
     int32_t block_dual, block_primal, block_stamp, block_projection, block_traverse, live_blocks;
     int32_t colour_lanes;   /* lanes per body in the colour passes: 1 or 4 (0 = 1) */
 } avbd_tuning;
-/* Autotuner: steps the batch (or world) 6 x steps_per_trial times with candidate launch shapes,
- * timing each kernel class with HIP events, and keeps the fastest per class.  The steps are
- * real steps of the simulation.  No reference counterpart (the reference runs on the CPU). */
+/* Autotuner: steps the batch (or world) with candidate launch shapes, steps_per_trial steps per
+ * candidate (3 block sizes, 5 live-list grids, and 2 colour-pass lane counts when colour passes
+ * run: 8 or 10 x steps_per_trial steps), timing each kernel class with HIP events, and keeps the
+ * fastest per class.  The steps are real steps of the simulation.  No reference counterpart
+ * (the reference runs on the CPU). */
 avbd_status avbd_batch_autotune(avbd_batch *batch, int32_t steps_per_trial, avbd_tuning *out);
 avbd_status avbd_world_autotune(avbd_world *world, int32_t steps_per_trial, avbd_tuning *out);
 /* Use these launch shapes (block sizes 64, 128 or 256; live_blocks in [8, 2^20]; colour_lanes 1

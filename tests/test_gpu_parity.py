@@ -483,11 +483,11 @@ def test_autotune_keeps_results():
     o.upload(bodies, joints)
     for _ in range(10):
         a.step()
-    tuning = a.autotune(1)           # 6 trial steps
+    tuning = a.autotune(1)           # 8 trial steps: 3 block sizes, 5 live grids (no colour passes here)
     assert all(v > 0 for v in tuning.values())
     for _ in range(10):
         a.step()
-    for _ in range(26):
+    for _ in range(28):
         b.step()
         o.step()
     assert not state_mismatch(a.download(), b.download())
