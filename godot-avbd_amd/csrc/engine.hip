@@ -970,6 +970,8 @@ struct Engine {
         AL(S.listC, N); AL(S.listH, N); AL(S.listJ, N * LIST_K); AL(S.listCnt, N); AL(S.listAdd, N); AL(S.listFlag, N);
         AL(S.wRewalk, W); AL(S.wRefit, W); AL(S.wMoved, W); AL(S.wStale, W); AL(S.listStats, 4);
         AL(S.wBoundsLo, W); AL(S.wBoundsHi, W);
+        wideWords = nblk(Ncap) * (TPB / 64);
+        AL(S.wWide, W); AL(S.wideMask, wideWords + 1); AL(S.wideWord, wideWords + 1); AL(S.wideScan, wideWords + 1);
         AL(S.gOfK, M); AL(S.kSlot, M); AL(S.kT, M); AL(S.newT, P); AL(S.mHdr, M); AL(S.mBasisK, 3 * M); AL(S.mFricK, M); AL(S.brakeOn, M); AL(S.wOrder, W);
         radixBlocks = (Ncap + RADIX_TILE - 1) / RADIX_TILE;
         AL(radixHist, static_cast<size_t>(256) * radixBlocks + 1);
@@ -985,6 +987,7 @@ struct Engine {
         k_fill_i32<<<nblk(Scap), TPB, 0, stream>>>(reinterpret_cast<int *>(S.cOff), 0, Scap);
         k_fill_i32<<<nblk(Scap), TPB, 0, stream>>>(S.mNum, 0, Scap);
         k_fill_i32<<<nblk(Jcap), TPB, 0, stream>>>(S.jWorld, -1, Jcap);
+        k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wWide, NO_WIDE, Wcap);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream));
         setParams(S.P, prm, prm.dt);
@@ -999,6 +1002,20 @@ struct Engine {
         k_scan_tiles<<<std::max(nt, 1), TPB, 0, stream>>>(in, out, tileSums, n, dlen);
         k_scan_sums<<<1, SCAN_SUMS_TPB, 0, stream>>>(tileSums, std::max(nt, 1), out, n, dlen);
         k_scan_add<<<std::max(nt, 1), TPB, 0, stream>>>(out, tileSums, n, dlen);
+    }
+
+    // Wide bodies (State::wWide): chosen at a tree rebuild, after the world bounds ...
+    int wideWords = 0;
+    void widePick()
+    {
+        k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wWide, NO_WIDE, Wcap);
+        k_wide_pick<<<nblk(Ncap), TPB, 0, stream>>>(S);
+    }
+    // ... and their pairs marked every step before the walks (wideMask bits and their ranks).
+    void wideFlags()
+    {
+        k_wide_flags<<<nblk(Ncap), TPB, 0, stream>>>(S);
+        scan(S.wideWord, S.wideScan, wideWords);
     }
 
     // Bodies sorted by colour, one stable 8-bit radix pass (the colour passes then walk their
@@ -1178,6 +1195,7 @@ struct Engine {
         if (timed) phaseMark(5);
         if (rebuildTree) {
             k_world_bounds<<<Wcap, TPB, 0, stream>>>(S);
+            widePick();
             k_morton<<<nb, TPB, 0, stream>>>(S, batchMode ? 0 : 1);
             pbeg(AVBD_K_SORT);
             radixSort();
@@ -1196,6 +1214,7 @@ struct Engine {
         pend();
         if (timed) phaseMark(6);
         pbeg(AVBD_K_TRAVERSE);
+        wideFlags();
         k_list_walk<<<nblk(Ncap, tune.traverse), tune.traverse, 0, stream>>>(S, Ncap);
         k_traverse<<<nblk(Ncap, tune.traverse), tune.traverse, 0, stream>>>(S, Ncap);
         pend();
@@ -2275,6 +2294,7 @@ struct Engine {
         k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wStale, 0, Wcap);
         k_swept_aabb<<<nb, TPB, 0, stream>>>(S);
         k_world_bounds<<<Wcap, TPB, 0, stream>>>(S);
+        widePick();
         k_morton<<<nb, TPB, 0, stream>>>(S, 1);
         radixSort();
         k_karras<<<nblk(Ncap - 1), TPB, 0, stream>>>(S, Ncap);
@@ -2284,6 +2304,7 @@ struct Engine {
         k_ropes<<<nblk(2 * Ncap - 1), TPB, 0, stream>>>(S, Ncap);
         buildRefitTasks();
         k_refit_tasks<<<refitGrid(), REFIT_TPB, 0, stream>>>(listTree(), Ncap, 0);
+        wideFlags();
         k_list_walk<<<nb, TPB, 0, stream>>>(S, Ncap);
         k_traverse<<<nb, TPB, 0, stream>>>(S, Ncap);
         k_fill_i32<<<1, 64, 0, stream>>>(S.listStats, 0, 4);

@@ -612,6 +612,60 @@ __device__ __forceinline__ bool worldBusy(const State &S, int w)
     return S.wMoved[w] * 2 > S.wBodyCount[w];
 }
 
+// ---- wide bodies (State::wWide) ----
+constexpr int NO_WIDE = 0x7fffffff;
+
+// At a tree rebuild, after k_world_bounds (wWide filled with NO_WIDE): a body whose swept box
+// spans at least a quarter of its world's centre bounds on two axes is a candidate; the
+// world's lowest-indexed candidate is its wide body.  Which body is wide changes how its
+// pairs are found, never which pairs are found.
+__global__ void k_wide_pick(State S)
+{
+    const int i = GID;
+    if (i >= S.numBodies) return;
+    const int w = S.bodyWorld[i];
+    if (w < 0 || S.wBodyCount[w] < 2) return;
+    const float4 lo = S.wBoundsLo[w], hi = S.wBoundsHi[w];
+    const v3 h = ld3(S.aabbH[i]);
+    const int axes = (8.0f * h.x >= hi.x - lo.x ? 1 : 0) + (8.0f * h.y >= hi.y - lo.y ? 1 : 0) +
+                     (8.0f * h.z >= hi.z - lo.z ? 1 : 0);
+    if (axes >= 2) atomicMin(S.wWide + w, i);
+}
+
+// Every step, after the swept boxes and before the walks: bit j of wideMask is set when body j
+// lies above its world's wide body and their swept boxes meet (the pair test the wide body's
+// own walk would make); wideWord counts each word's bits for the scan.  The wide body's list
+// is kept closed, so that no walk appends to it.  Launched with a multiple of 64 threads per
+// workgroup over every word of wideMask.
+__global__ void __launch_bounds__(TPB) k_wide_flags(State S)
+{
+    const int j = GID;
+    bool on = false;
+    if (j < S.numBodies) {
+        const int w = S.bodyWorld[j];
+        const int wi = w >= 0 ? S.wWide[w] : NO_WIDE;
+        if (wi == j) {
+            S.listCnt[j] = LIST_K + 1;
+        } else if (wi != NO_WIDE && j > wi) {
+            const v3 c = ld3(S.aabbC[wi]), h = ld3(S.aabbH[wi]);
+            on = sweptOverlap(S, c - h, c + h, j);
+        }
+    }
+    const unsigned long long m = __ballot(on);
+    if ((threadIdx.x & 63) == 0) {
+        S.wideMask[j >> 6] = m;
+        S.wideWord[j >> 6] = static_cast<uint32_t>(__popcll(m));
+    }
+}
+
+// Number of bodies below index x with their wideMask bit set (x <= numBodies).
+__device__ __forceinline__ uint32_t wideRank(const State &S, int x)
+{
+    const int word = x >> 6, bit = x & 63;
+    const unsigned long long below = bit ? (S.wideMask[word] & ((1ull << bit) - 1ull)) : 0ull;
+    return S.wideScan[word] + static_cast<uint32_t>(__popcll(below));
+}
+
 // Body i's swept-box pairs (j > i) by walking the tree; the count and the first PAIR_BUF
 // to pairScratch.  `exactLeaves`: the leaves are the swept boxes themselves.
 __device__ __forceinline__ void walkPairs(const State &S, int n, int i, int w, v3 qMin, v3 qMax, bool exactLeaves)
@@ -643,6 +697,11 @@ __global__ void __launch_bounds__(TPB) k_list_walk(State S, int n)
     const int i = S.sortVal[p];
     const int w = S.bodyWorld[i];
     if (w < 0) return;
+    if (S.wWide[w] == i) {   // its pairs are the bodies above it with their wideMask bit set
+        S.pairCount[i] = wideRank(S, S.wBodyBase[w] + S.wBodyCount[w]) - wideRank(S, i + 1);
+        S.listFlag[i] = 3;
+        return;
+    }
     const v3 c = ld3(S.aabbC[i]), h = ld3(S.aabbH[i]);
     if (S.wStale[w]) {
         walkPairs(S, n, i, w, c - h, c + h, true);
@@ -751,6 +810,18 @@ __global__ void __launch_bounds__(TPB) k_pair_emit(State S, int n)
 {
     const int i = GID;
     if (i >= S.numBodies) return;
+    const int w = S.bodyWorld[i];
+    if (w >= 0) {
+        // the wide body's range is written by the bodies that meet it, each at its rank
+        const int wi = S.wWide[w];
+        if (wi == i) return;
+        if (wi != NO_WIDE && i > wi && ((S.wideMask[i >> 6] >> (i & 63)) & 1ull) &&
+            S.pairOff[wi + 1] <= static_cast<uint32_t>(S.pairCap)) {
+            const uint32_t at = S.pairOff[wi] + (wideRank(S, i) - wideRank(S, wi + 1));
+            S.pairJ[at] = i;
+            S.pairI[at] = wi;
+        }
+    }
     const uint32_t begin = S.pairOff[i], end = S.pairOff[i + 1];
     const uint32_t cnt = end - begin;
     if (cnt == 0) return;

@@ -217,6 +217,16 @@ struct State {
     // the reference makes one and releases it in the same step; it still wakes a sleeper)
     uint8_t *pairApart;
     float4 *wBoundsLo, *wBoundsHi;
+    // A world's wide body (its lowest-indexed body whose box spans at least a quarter of the
+    // world's extent on two axes, chosen at each tree rebuild; NO_WIDE if none) does not walk
+    // the tree: its rope walk would visit every node that reaches into its box (a ground box
+    // under hanging chains: thousands of dependent node loads on one lane).  Instead every
+    // body j above it tests its box against the wide body's (wideMask: one bit per body,
+    // wideScan: exclusive scan of the per-word bit counts), and writes itself into the wide
+    // body's pair range at its rank, so the range comes out sorted by j.
+    int *wWide;
+    unsigned long long *wideMask;
+    uint32_t *wideWord, *wideScan;
 };
 
 } // namespace avbd_dev

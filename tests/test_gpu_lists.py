@@ -3,8 +3,9 @@ bodies whose boxes, grown by a margin, meet its own, and walks the tree again on
 leaves its grown box (a moved body adds itself to the lists of the lower bodies it now meets);
 a world where most bodies move walks the exact boxes until it calms down.  None of this may
 change the pairs: bodies, every manifold field and the pair order stay bit-equal to the oracle's
-every step, through teleports next to lower and higher bodies, lists that overflow, and worlds
-that go busy and calm down again."""
+every step, through teleports next to lower and higher bodies, lists that overflow, worlds
+that go busy and calm down again, and a wide body (a ground spanning its world) whose pairs
+are written by the bodies that meet it instead of by its own walk."""
 import numpy as np
 import pytest
 
@@ -107,6 +108,51 @@ def test_batch_lists_with_busy_and_calm_worlds():
         o.upload(bb, jj)
         orc.append(o)
     for s in range(120):
+        batch.step()
+        for o in orc:
+            o.step()
+        if s % 10 == 9:
+            for wid, o in zip(ids, orc):
+                assert not state_mismatch(batch.download(wid), o.download()), (s, wid)
+    batch.close()
+
+
+def _ground_first(bodies):
+    """The pile's arrays with the ground (last) moved to list index 0."""
+    n = len(bodies["mass"])
+    order = np.r_[n - 1, np.arange(n - 1)]
+    return {k: v[order].copy() for k, v in bodies.items()}
+
+
+def test_wide_ground_with_list_index_zero():
+    """A ground box at list index 0 under a pile: the ground spans its world, so it does not walk
+    the tree; the bodies that meet it write themselves into its pair range, ranked by index.
+    Its pairs (far more than 16) and everything downstream stay bit-equal to the oracle while the
+    pile falls (exact walks) and after it settles (lists), in a plain world and in a batch."""
+    b, _ = scenes.pile(300, seed=23, pitch=2.0, ground=60.0)
+    b = _ground_first(b)
+    p = P(iterations=10)
+    w = pkg.World(p, len(b["mass"]), 0)
+    w.upload(b, None)
+    o = oracle.OracleWorld(p)
+    o.upload(b, None)
+    for s in range(150):
+        w.step()
+        o.step()
+        _check(w, o, s)
+    on_ground = int(np.sum(w.manifolds()["body_a"] == 0))
+    assert on_ground > 16, on_ground
+    w.close()
+
+    worlds = [_ground_first(scenes.pile(n, seed=60 + n, pitch=2.0, ground=40.0)[0]) for n in (40, 150, 90)]
+    batch = pkg.Batch(p, 4, sum(len(x["mass"]) for x in worlds) + 8, 0)
+    ids = [batch.add(bb, None) for bb in worlds]
+    orc = []
+    for bb in worlds:
+        o = oracle.OracleWorld(p)
+        o.upload(bb, None)
+        orc.append(o)
+    for s in range(100):
         batch.step()
         for o in orc:
             o.step()
