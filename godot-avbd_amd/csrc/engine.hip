@@ -1196,7 +1196,7 @@ struct Engine {
         if (rebuildTree) {
             k_world_bounds<<<Wcap, TPB, 0, stream>>>(S);
             widePick();
-            k_morton<<<nb, TPB, 0, stream>>>(S, batchMode ? 0 : 1);
+            k_morton<<<nb, TPB, 0, stream>>>(S, batchMode ? 0 : 1, 0);
             pbeg(AVBD_K_SORT);
             radixSort();
             pend();
@@ -2295,7 +2295,7 @@ struct Engine {
         k_swept_aabb<<<nb, TPB, 0, stream>>>(S);
         k_world_bounds<<<Wcap, TPB, 0, stream>>>(S);
         widePick();
-        k_morton<<<nb, TPB, 0, stream>>>(S, 1);
+        k_morton<<<nb, TPB, 0, stream>>>(S, 1, 1);
         radixSort();
         k_karras<<<nblk(Ncap - 1), TPB, 0, stream>>>(S, Ncap);
         k_world_roots_init<<<nblk(Wcap), TPB, 0, stream>>>(S, Ncap);

@@ -336,16 +336,34 @@ __device__ __forceinline__ unsigned long long expandBits21(uint32_t v)
     return x;
 }
 
+// The standard 3-D bit spreading (two zero bits after each bit of v).
+__device__ __forceinline__ unsigned long long spreadBits21(uint32_t v)
+{
+    unsigned long long x = v & 0x1fffffu;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
 // bvh_broadphase.cpp:19-36, 343-348: 63-bit Morton code of the centre, every axis offset
-// by the scene's lo.x (the reference's quantisation, kept for bit-exact codes).  The sort
-// key is the full code for a single world; for a batch it is (segment << 43 | code >> 20)
-// so each world's bodies sort within their own index range.
-__global__ void k_morton(State S, int fullKey)
+// by the scene's lo.x (the reference's quantisation and bit spreading, kept for bit-exact
+// codes in mortonCode).  The reference's fourth spreading mask keeps bits that belong to the
+// neighbouring axes, so its codes interleave the axes only roughly: sorted by them, leaves that
+// are far apart share subtrees (on the chains scene a walk visits ≈6 times the nodes).  The
+// tree the step walks is therefore sorted on the standard interleave of the same quantised
+// coordinates; the tree only decides how long the walks are, never which pairs they find
+// (refOrder 1 sorts on the reference's codes instead, for the broad-phase probe).  The sort key
+// is the full code for a single world; for a batch it is (segment << 43 | code >> 20) so each
+// world's bodies sort within their own index range.
+__global__ void k_morton(State S, int fullKey, int refOrder)
 {
     const int i = GID;
     if (i >= S.numBodies) return;
     const int w = S.bodyWorld[i];
-    unsigned long long code = 0;
+    unsigned long long code = 0, tree = 0;
     if (w >= 0) {
         const float4 lo = S.wBoundsLo[w], hi = S.wBoundsHi[w];
         const float ex = hi.x - lo.x, ey = hi.y - lo.y, ez = hi.z - lo.z;
@@ -359,12 +377,13 @@ __global__ void k_morton(State S, int fullKey)
         const uint32_t qy = static_cast<uint32_t>(sclamp((c.y - lo.x) * scale, 0.0f, 2097151.0f));
         const uint32_t qz = static_cast<uint32_t>(sclamp((c.z - lo.x) * scale, 0.0f, 2097151.0f));
         code = expandBits21(qx) | (expandBits21(qy) << 1) | (expandBits21(qz) << 2);
+        tree = refOrder ? code : (spreadBits21(qx) | (spreadBits21(qy) << 1) | (spreadBits21(qz) << 2));
     }
     S.mortonCode[i] = code;
     const unsigned long long seg = w >= 0 ? static_cast<unsigned long long>(S.wBodyBase[w]) : static_cast<unsigned long long>(i);
     // bodies outside every world sort after the world (single-world key) or into their
     // own index position (segment key)
-    S.sortKey[i] = fullKey ? (w >= 0 ? code : ~0ull) : ((seg << 43) | (code >> 20));
+    S.sortKey[i] = fullKey ? (w >= 0 ? tree : ~0ull) : ((seg << 43) | (tree >> 20));
     S.sortVal[i] = i;
 }
 
