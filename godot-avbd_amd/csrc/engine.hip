@@ -958,7 +958,7 @@ struct Engine {
         AL(S.wBodyBase, W); AL(S.wBodyCount, W); AL(S.wJointBase, W); AL(S.wJointCount, W);
         AL(S.wSlotBase, W); AL(S.wSlotCap, W); AL(S.wLive, W); AL(S.wFree, W); AL(S.wTop, W); AL(S.wNew, W);
         AL(S.wHasJoints, W); AL(S.wUpdateCount, W); AL(S.wErr, W); AL(S.wProjDone, W); AL(S.wMaxDepth, W);
-        AL(S.wColourLeft, W);
+        AL(S.wColourLeft, W); AL(S.wColourTail, W);
         AL(S.mortonCode, N); AL(S.sortKey, N); AL(S.sortKeyAlt, N); AL(S.sortVal, N); AL(S.sortValAlt, N);
         AL(S.colKey, N); AL(S.colKeyAlt, N); AL(S.colVal, N); AL(S.colValAlt, N); AL(S.colStart, 16);
         AL(phaseTicks, 8);
@@ -1271,6 +1271,7 @@ struct Engine {
             for (int r = 0; r + 1 < colourRounds; r++) k_colour_round<<<nb, TPB, 0, stream>>>(S, 0);
             k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wColourLeft, 0, Wcap);
             k_colour_round<<<nb, TPB, 0, stream>>>(S, 1);
+            k_fill_i32<<<nblk(Wcap), TPB, 0, stream>>>(S.wColourTail, 0, Wcap);
             k_colour_counts<<<nb, TPB, 0, stream>>>(S);
             k_colour_finish<<<Wcap, COLOUR_TPB, 0, stream>>>(S);
         }

@@ -1642,9 +1642,11 @@ __global__ void k_colour_round(State S, int count)
 // Bodies left after the bounded rounds (worlds flagged by the last round) are finished by
 // dependency counting: a body is ready once every lower-ordered neighbour has its colour, and
 // ready bodies are coloured level by level (Kahn) in one workgroup per world.  Every body then
-// takes the colour the sequential greedy pass gives it.  The counts are per adjacency entry
+// takes the colour the sequential greedy pass gives it (bodies of one level are never
+// neighbours, so the order inside a level does not matter).  The counts are per adjacency entry
 // (entries are symmetric between the two bodies), so each entry of a newly coloured body
-// releases one count of its higher-ordered neighbour.
+// releases one count of its higher-ordered neighbour.  The bodies ready from the start are
+// queued here (wColourTail cleared before), so the finisher does not scan its world.
 __global__ void k_colour_counts(State S)
 {
     const int i = GID;
@@ -1661,6 +1663,7 @@ __global__ void k_colour_counts(State S)
         if (S.colour[o] == 0xFF) cnt++;
     }
     S.colourCnt[i] = cnt;
+    if (cnt == 0) S.colourQueue[S.wBodyBase[w] + atomicAdd(S.wColourTail + w, 1)] = i;
 }
 
 constexpr int COLOUR_TPB = 256;
@@ -1668,13 +1671,9 @@ __global__ void __launch_bounds__(COLOUR_TPB) k_colour_finish(State S)
 {
     const int w = S.wOrder[blockIdx.x];
     if (!S.wHasJoints[w] || !S.wColourLeft[w]) return;
-    const int base = S.wBodyBase[w], cnt = S.wBodyCount[w];
-    int *queue = S.colourQueue + base;
+    int *queue = S.colourQueue + S.wBodyBase[w];
     __shared__ int tail, head;
-    if (threadIdx.x == 0) { tail = 0; head = 0; }
-    __syncthreads();
-    for (int k = threadIdx.x; k < cnt; k += COLOUR_TPB)
-        if (S.colour[base + k] == 0xFF && S.colourCnt[base + k] == 0) queue[atomicAdd(&tail, 1)] = base + k;
+    if (threadIdx.x == 0) { tail = S.wColourTail[w]; head = 0; }
     __syncthreads();
     while (true) {
         const int h = head, e = tail;

@@ -174,6 +174,7 @@ struct State {
     int *wProjDone;
     unsigned int *wMaxDepth;   // float bits (non-negative) for atomicMax
     int *wColourLeft;
+    int *wColourTail;            // per world: bodies k_colour_counts queued as ready (colourQueue)
 
     // ---- broad phase
     unsigned long long *mortonCode, *sortKey, *sortKeyAlt;
