@@ -777,9 +777,9 @@ __global__ void __launch_bounds__(TPB) k_list_walk(State S, int n)
 // copies those (or, for a body with more, walks again) into pairJ at pairOff[i], sorted by j.
 __global__ void __launch_bounds__(TPB) k_traverse(State S, int n)
 {
-    const int p = XGID;
-    if (p >= S.numBodies) return;
-    const int i = S.sortVal[p];
+    // body order (not the tree's): a lane's list and outputs sit next to its neighbours'
+    const int i = XGID;
+    if (i >= S.numBodies) return;
     const int w = S.bodyWorld[i];
     if (w < 0 || n < 2) { S.pairCount[i] = 0; return; }
     if (S.listFlag[i] == 3) return;
