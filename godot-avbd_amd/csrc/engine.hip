@@ -1153,7 +1153,7 @@ struct Engine {
     void velocityBrake(float window, int countActive)
     {
         k_brake_flags<<<liveGrid(), TPB, 0, stream>>>(S, countActive);
-        k_brake_links<<<liveGrid(), TPB, 0, stream>>>(S);
+        k_brake_links<<<nblk(Ncap), TPB, 0, stream>>>(S);
         k_brake_levels<<<Wcap, BRAKE_TPB, 0, stream>>>(S, window, 1);
     }
 

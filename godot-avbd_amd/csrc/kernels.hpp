@@ -1754,44 +1754,36 @@ __global__ void k_brake_flags(State S, int count)
             }
         }
         S.brakeOn[k] = on ? 1 : 0;
+        S.succA[k] = -1;
+        S.succB[k] = -1;
+        S.indeg[k] = on ? 0 : -1;
     }
 }
 
-__device__ void brakeLinks(const State &S, int k);
+// Each movable body walks its adjacency once (manifolds in live order after its joints) and
+// links each braking manifold to the next one of that body: succA / succB by the side the body
+// is on, one count of the later manifold's in-degree per link (both bodies of a manifold may
+// add one).  k_brake_flags has set succ* to -1 and indeg to 0 (braking) or -1 (not).
 __global__ void k_brake_links(State S)
 {
-    FOR_LIVE(k) brakeLinks(S, k);
-}
-__device__ void brakeLinks(const State &S, int k)
-{
-    S.succA[k] = -1;
-    S.succB[k] = -1;
-    if (!S.brakeOn[k]) { S.indeg[k] = -1; return; }
-    const int4 h = S.mHdr[k];
-    int deg = 0;
-    for (int side = 0; side < 2; side++) {
-        const int body = side == 0 ? h.x : h.y;
-        if (!bodyMoves(S, body)) continue;
-        const int e0 = S.adjStart[body], e1 = S.adjStart[body + 1];
-        const int mine = side == 0 ? k : (k | ADJ_SIDE_B);
-        int me = e0;
-        while (me < e1 && S.adj[me] != mine) me++;
-        if (me == e1) continue;
-        for (int e = me - 1; e >= e0; e--) {
-            const int r = S.adj[e];
-            if (r < 0) break;
-            if (S.brakeOn[adjManifold(r)]) { deg++; break; }
+    const int i = XGID;
+    if (i >= S.numBodies || S.bodyWorld[i] < 0 || !bodyMoves(S, i)) return;
+    int prev = -1;
+    bool prevB = false;
+    const int e1 = S.adjStart[i + 1];
+    for (int e = S.adjStart[i]; e < e1; e++) {
+        const int r = S.adj[e];
+        if (r < 0) continue;
+        const int k = adjManifold(r);
+        if (!S.brakeOn[k]) continue;
+        if (prev >= 0) {
+            if (prevB) S.succB[prev] = k;
+            else S.succA[prev] = k;
+            atomicAdd(S.indeg + k, 1);
         }
-        for (int e = me + 1; e < e1; e++) {
-            const int r = S.adj[e];
-            if (r >= 0 && S.brakeOn[adjManifold(r)]) {
-                if (side == 0) S.succA[k] = adjManifold(r);
-                else S.succB[k] = adjManifold(r);
-                break;
-            }
-        }
+        prev = k;
+        prevB = adjSideB(r);
     }
-    S.indeg[k] = deg;
 }
 
 __device__ void brakeManifold(const State &S, int k, float window)
