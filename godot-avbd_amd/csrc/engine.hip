@@ -171,7 +171,7 @@ struct Engine {
     // refit boxes bloat and walks lengthen; the topology never changes the pairs).
     int rebuildInterval() const { return movingWorlds > 0 ? 4 : refitBudget; }
     bool batchMode = false;
-    int colourRounds = 48;
+    int colourRounds = 64;   // Jones-Plassmann rounds before the finisher (measured best on the chains world)
     hipGraphExec_t graphs[2] = {nullptr, nullptr};  // [rebuild, refit]
     bool graphJoints = false;
     bool graphRound = false;
