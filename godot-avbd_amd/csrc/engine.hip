@@ -951,7 +951,7 @@ struct Engine {
         AL(S.sRA, CD); AL(S.sRB, CD); AL(S.sC0, CD); AL(S.sPen, CD); AL(S.sLam, CD);
         AL(S.cOff, M); AL(S.cCount, M + 1); AL(S.cScan, M + 1); AL(S.sliceCount, 1);
         AL(S.stampA, STAMP_ROWS * M); AL(S.stampB, STAMP_ROWS * M); AL(S.slotWorld, M); AL(S.gLive, M); AL(S.wLiveOff, W + 1); AL(S.live, M); AL(S.liveTmp, M); AL(S.freeList, M);
-        AL(S.keep, M + 1); AL(S.keepScan, M + 1); AL(S.relScan, M + 1);
+        AL(S.keep, M + 1);
         AL(S.succA, M); AL(S.succB, M); AL(S.indeg, M); AL(S.levelOrder, M);
         AL(S.brakeLevels, W); AL(S.levelStart, M); AL(S.wLevelEnd, W);
         AL(S.projLA, M); AL(S.projAA, M); AL(S.projLB, M); AL(S.projAB, M); AL(S.projOn, M);
@@ -1245,12 +1245,7 @@ struct Engine {
             if (anyRound()) k_manifold_init_round<<<liveGrid(NARROW_TPB), NARROW_TPB, 0, stream>>>(S);
         }
         pend();
-        k_release_flags<<<nblk(Scap), TPB, 0, stream>>>(S);
-        scan(S.keep, S.keepScan, Scap);
-        scan(S.relScan, S.relScan, Scap);
-        k_compact<<<nblk(Scap), TPB, 0, stream>>>(S);
-        k_compact_counts<<<nblk(Wcap), TPB, 0, stream>>>(S);
-        k_copy_live<<<nblk(Scap), TPB, 0, stream>>>(S);
+        k_compact_world<<<Wcap, TPB, 0, stream>>>(S);
         buildLiveList();
         packContacts(0);
         pbeg(AVBD_K_ADJACENCY);

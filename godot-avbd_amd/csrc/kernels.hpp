@@ -1468,51 +1468,40 @@ __global__ void k_joint_init(State S)
 }
 
 // solver_solve.cpp:66-87: stable compaction of each world's live list, releasing the
-// slots of separated pairs to the free list.
-__global__ void k_release_flags(State S)
+// slots of separated pairs to the free list.  One workgroup per world walks its live prefix
+// in chunks (the narrow phase set keep only there): a block scan of the kept and released
+// flags (two 16-bit counts in one word) places each slot, kept ones in order in liveTmp and
+// released ones after the world's free entries, then the kept run is copied back.  The
+// world's slot range is this workgroup's alone.
+__global__ void __launch_bounds__(TPB) k_compact_world(State S)
 {
-    const int t = GID;
-    if (t >= S.slotCap) return;
-    const int w = S.slotWorld[t];
-    S.relScan[t] = 0;
-    if (w < 0) return;
-    // relScan temporarily holds the "released" flag; scanned in place afterwards
-    S.relScan[t] = (t - S.wSlotBase[w] < S.wLive[w] && !S.keep[t]) ? 1u : 0u;
-}
-
-__global__ void k_compact(State S)
-{
-    const int t = GID;
-    if (t >= S.slotCap) return;
-    const int w = S.slotWorld[t];
-    if (w < 0) return;
-    const int base = S.wSlotBase[w];
-    if (t - base >= S.wLive[w]) return;
-    const int slot = S.live[t];
-    if (S.keep[t]) {
-        S.liveTmp[base + static_cast<int>(S.keepScan[t] - S.keepScan[base])] = slot;
-    } else {
-        S.freeList[base + S.wFree[w] + static_cast<int>(S.relScan[t] - S.relScan[base])] = slot;
+    const int w = blockIdx.x;
+    if (S.wSlotCap[w] <= 0) return;
+    const int base = S.wSlotBase[w], live = S.wLive[w], freeBase = S.wFree[w];
+    __shared__ uint32_t sm[TPB / 64];
+    int kept = 0, released = 0;
+    for (int c = 0; c < live; c += TPB) {
+        const int at = c + static_cast<int>(threadIdx.x);
+        const bool in = at < live;
+        const uint32_t k = in && S.keep[base + at] ? 1u : 0u;
+        const uint32_t r = in && !k ? 1u : 0u;
+        uint32_t total;
+        const uint32_t ex = blockExclusiveScan<TPB>(k | (r << 16), sm, total);
+        if (in) {
+            const int slot = S.live[base + at];
+            if (k) S.liveTmp[base + kept + static_cast<int>(ex & 0xFFFFu)] = slot;
+            else S.freeList[base + freeBase + released + static_cast<int>(ex >> 16)] = slot;
+        }
+        kept += static_cast<int>(total & 0xFFFFu);
+        released += static_cast<int>(total >> 16);
     }
-}
-
-__global__ void k_compact_counts(State S)
-{
-    const int w = GID;
-    if (w >= S.numWorlds) return;
-    const int base = S.wSlotBase[w], cap = S.wSlotCap[w];
-    if (cap <= 0) return;
-    S.wLive[w] = static_cast<int>(S.keepScan[base + cap] - S.keepScan[base]);
-    S.wFree[w] += static_cast<int>(S.relScan[base + cap] - S.relScan[base]);
-}
-
-__global__ void k_copy_live(State S)
-{
-    const int t = GID;
-    if (t >= S.slotCap) return;
-    const int w = S.slotWorld[t];
-    if (w < 0) return;
-    if (t - S.wSlotBase[w] < S.wLive[w]) S.live[t] = S.liveTmp[t];
+    __threadfence_block();
+    __syncthreads();
+    for (int c = threadIdx.x; c < kept; c += TPB) S.live[base + c] = S.liveTmp[base + c];
+    if (threadIdx.x == 0) {
+        S.wLive[w] = kept;
+        S.wFree[w] = freeBase + released;
+    }
 }
 
 // ------------------------------------------------------------ adjacency ----

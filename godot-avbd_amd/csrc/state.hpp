@@ -156,9 +156,7 @@ struct State {
     int *live;          // per world range: slots in live order
     int *liveTmp;
     int *freeList;      // per world range
-    uint32_t *keep;     // per slot-range position flag / scan input
-    uint32_t *keepScan;
-    uint32_t *relScan;
+    uint32_t *keep;     // per live-list position: the manifold stays live this step
     // per-manifold brake schedule (indexed by slot-range position of the live entry)
     int *succA, *succB, *indeg, *levelOrder;
     int *brakeLevels;   // per world: number of levels
