@@ -1033,8 +1033,9 @@ struct Engine {
     // spreads over every CU (the passes are bound by the latency of their dependent loads)
     static constexpr int COLOUR_PASS_TPB = 64;
     int colourGrid() const { return nblk(Ncap, COLOUR_PASS_TPB); }
-    // four lanes per body; the grid walks longer colour runs in strides
-    int quadGrid() const { return std::min(nblk(4 * Ncap, QUAD_TPB), 2048); }
+    // four lanes per body; the grid walks longer colour runs in strides (512 workgroups, 64 per
+    // XCD: on the chains world 2.58 ms per step against 2.62 with 1024 or 2048 and 2.67 with 256)
+    int quadGrid() const { return std::min(nblk(4 * Ncap, QUAD_TPB), 512); }
 
     void radixSort()
     {
