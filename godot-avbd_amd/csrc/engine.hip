@@ -937,7 +937,7 @@ struct Engine {
         AL(S.bodyWorld, N); AL(S.updateIndex, N); AL(S.colour, N); AL(S.contactOnly, N);
         AL(S.dLin, N); AL(S.dAng, N); AL(S.aabbC, N); AL(S.aabbH, N);
         AL(S.fbCount, 1); AL(S.fbBody, N); AL(S.fbBlock, 33 * N);
-        AL(S.adjCount, N + 1); AL(S.adjStart, N + 1); AL(S.adj, 2 * (M + J)); AL(S.adjTmp, 2 * (M + J));
+        AL(S.adjCount, N + 1); AL(S.adjStart, N + 1); AL(S.adj, 2 * (M + J)); AL(S.adjTmp, 2 * (M + J)); AL(S.adjPos, 2 * (M + J));
         AL(S.jA, J); AL(S.jB, J); AL(S.jWorld, J); AL(S.jRA, J); AL(S.jRB, J); AL(S.jC0L, J); AL(S.jC0A, J);
         AL(S.jPenL, J); AL(S.jPenA, J); AL(S.jLamL, J); AL(S.jLamA, J); AL(S.jStiffL, J); AL(S.jStiffA, J);
         AL(S.jFrac, J); AL(S.jArm, J); AL(S.jBroken, J); AL(S.jDead, J); AL(S.jKind, J); AL(S.jRest, J);
@@ -1098,7 +1098,6 @@ struct Engine {
         k_fill_i32<<<nblk(n + 1), TPB, 0, stream>>>(S.adjCount, 0, n + 1);
         k_adj_count<<<liveGrid(), TPB, 0, stream>>>(S);
         scan(reinterpret_cast<uint32_t *>(S.adjCount), reinterpret_cast<uint32_t *>(S.adjStart), n);
-        k_fill_i32<<<nblk(n + 1), TPB, 0, stream>>>(S.adjCount, 0, n + 1);
         k_adj_fill<<<liveGrid(), TPB, 0, stream>>>(S);
         k_adj_sort<<<nblk(n), TPB, 0, stream>>>(S);
     }

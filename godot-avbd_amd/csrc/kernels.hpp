@@ -1511,34 +1511,35 @@ __global__ void __launch_bounds__(TPB) k_compact_world(State S)
 // entries (the solves, colouring, brake and projection gather movable bodies; the pair filter
 // scans the movable side), and a ground body touched by hundreds of manifolds would otherwise
 // serialise their atomics on one counter.
+// Each entry's place in its body's list is the count its atomic took (adjPos: two per
+// manifold view index, then two per joint after 2 * slotCap), so the fill writes without
+// atomics; k_adj_sort then orders every list.
 __global__ void k_adj_count(State S)
 {
+    int *jp = S.adjPos + 2 * static_cast<size_t>(S.slotCap);
     for (int t = GID; t < S.jointCap; t += gridDim.x * blockDim.x) {
         if (S.jWorld[t] >= 0 && !S.jDead[t]) {
-            if (S.jA[t] >= 0) atomicAdd(&S.adjCount[S.jA[t]], 1);
-            atomicAdd(&S.adjCount[S.jB[t]], 1);
+            if (S.jA[t] >= 0) jp[2 * t] = atomicAdd(&S.adjCount[S.jA[t]], 1);
+            jp[2 * t + 1] = atomicAdd(&S.adjCount[S.jB[t]], 1);
         }
     }
     FOR_LIVE(k) {
         const int4 h = S.mHdr[k];
-        if (bMass(S, h.x) > 0.0f) atomicAdd(&S.adjCount[h.x], 1);
-        if (bMass(S, h.y) > 0.0f) atomicAdd(&S.adjCount[h.y], 1);
+        if (bMass(S, h.x) > 0.0f) S.adjPos[2 * k] = atomicAdd(&S.adjCount[h.x], 1);
+        if (bMass(S, h.y) > 0.0f) S.adjPos[2 * k + 1] = atomicAdd(&S.adjCount[h.y], 1);
     }
 }
 
 __global__ void k_adj_fill(State S)
 {
+    const int *jp = S.adjPos + 2 * static_cast<size_t>(S.slotCap);
     for (int t = GID; t < S.jointCap; t += gridDim.x * blockDim.x) {
         const int w = S.jWorld[t];
         if (w >= 0 && !S.jDead[t]) {
             const unsigned long long seq = static_cast<unsigned long long>(t - S.wJointBase[w]);
             const unsigned long long e = (seq << 32) | static_cast<uint32_t>(-(t + 1));
-            if (S.jA[t] >= 0) {
-                const int a = S.jA[t];
-                S.adjTmp[S.adjStart[a] + atomicAdd(&S.adjCount[a], 1)] = e;
-            }
-            const int b = S.jB[t];
-            S.adjTmp[S.adjStart[b] + atomicAdd(&S.adjCount[b], 1)] = e;
+            if (S.jA[t] >= 0) S.adjTmp[S.adjStart[S.jA[t]] + jp[2 * t]] = e;
+            S.adjTmp[S.adjStart[S.jB[t]] + jp[2 * t + 1]] = e;
         }
     }
     FOR_LIVE(k) {
@@ -1547,9 +1548,8 @@ __global__ void k_adj_fill(State S)
         const unsigned long long e = (seq << 32) | static_cast<uint32_t>(k);
         const int4 h = S.mHdr[k];
         const int a = h.x, b = h.y;
-        if (bMass(S, a) > 0.0f) S.adjTmp[S.adjStart[a] + atomicAdd(&S.adjCount[a], 1)] = e;
-        if (bMass(S, b) > 0.0f)
-            S.adjTmp[S.adjStart[b] + atomicAdd(&S.adjCount[b], 1)] = e | static_cast<uint32_t>(ADJ_SIDE_B);
+        if (bMass(S, a) > 0.0f) S.adjTmp[S.adjStart[a] + S.adjPos[2 * k]] = e;
+        if (bMass(S, b) > 0.0f) S.adjTmp[S.adjStart[b] + S.adjPos[2 * k + 1]] = e | static_cast<uint32_t>(ADJ_SIDE_B);
     }
 }
 

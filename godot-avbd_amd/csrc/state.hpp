@@ -95,6 +95,7 @@ struct State {
     float *fbBlock;  // 33 floats per entry (lhsLin, lhsAng, lhsCross column-major, rhsLin, rhsAng)
     int *adjCount, *adjStart, *adj;
     unsigned long long *adjTmp;
+    int *adjPos;     // place of each manifold side / joint end in its body's list (count pass)
 
     // ---- joints
     int *jA, *jB, *jWorld;
