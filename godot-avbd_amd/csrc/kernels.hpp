@@ -1553,12 +1553,34 @@ __global__ void k_adj_fill(State S)
     }
 }
 
+// Sorts up to ADJ_REG keys in registers (odd-even transposition; unused places hold the
+// largest key and stay last); the keys of one body are distinct.
+constexpr int ADJ_REG = 8;
 __global__ void k_adj_sort(State S)
 {
     const int b = GID;
     if (b >= S.numBodies) return;
     const int e0 = S.adjStart[b], e1 = S.adjStart[b + 1];
     const bool ordered = bMass(S, b) > 0.0f;  // only movable bodies sum in this order
+    if (ordered && e1 - e0 <= ADJ_REG) {
+        unsigned long long v[ADJ_REG];
+#pragma unroll
+        for (int q = 0; q < ADJ_REG; q++) v[q] = e0 + q < e1 ? S.adjTmp[e0 + q] : ~0ull;
+#pragma unroll
+        for (int phase = 0; phase < ADJ_REG; phase++) {
+#pragma unroll
+            for (int q = phase & 1; q + 1 < ADJ_REG; q += 2) {
+                const unsigned long long lo = v[q] < v[q + 1] ? v[q] : v[q + 1];
+                const unsigned long long hi = v[q] < v[q + 1] ? v[q + 1] : v[q];
+                v[q] = lo;
+                v[q + 1] = hi;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < ADJ_REG; q++)
+            if (e0 + q < e1) S.adj[e0 + q] = static_cast<int>(static_cast<uint32_t>(v[q] & 0xFFFFFFFFull));
+        return;
+    }
     if (ordered) {
         for (int a = e0 + 1; a < e1; a++) {
             const unsigned long long v = S.adjTmp[a];
